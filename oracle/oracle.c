@@ -1,0 +1,822 @@
+/*
+ * CPU oracle for the OversetDevFort hot path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Plain-C restatement of the reference Fortran (paths relative to the
+ * reference's src/).  Only tests, smoke() and bench.py's cpu_baseline leg
+ * use this file, as the checker; the HIP product library never links it.
+ *
+ * Pinning (what checks each function against the real reference or the
+ * mathematics) is listed in the header comment of oracle/oracle.py.
+ *
+ * Build with -ffp-contract=off and without -march so that no fused
+ * multiply-add changes the rounding against the reference Fortran build.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define NSD 3
+
+/* param.f90:469-480 */
+void orc_genalpha(double rhoinf, double out[4]) {
+  double almi, alfi, gami, beti;
+  if (rhoinf < 0.0 || rhoinf > 1.0) {
+    almi = alfi = gami = beti = 1.0;
+  } else {
+    almi = (3.0 - rhoinf) / (1.0 + rhoinf) / 2.0;
+    alfi = 1.0 / (1.0 + rhoinf);
+    gami = 0.5 + almi - alfi;
+    beti = 0.25 * (1.0 + almi - alfi) * (1.0 + almi - alfi);
+  }
+  out[0] = almi;
+  out[1] = alfi;
+  out[2] = gami;
+  out[3] = beti;
+}
+
+/* genGP_GW.f90:317-430 (genGPandGWtetpri); hex rule keeps the 14-digit
+ * abscissa literal of genGP_GW.f90:357 */
+int orc_gauss_volume(int ngp, double *gp, double *gw) {
+  if (ngp == 1) {
+    gp[0] = gp[1] = gp[2] = 0.25;
+    gw[0] = 1.0 / 6.0;
+  } else if (ngp == 4) {
+    const double a = 0.5854101966249685, b = 0.1381966011250105;
+    const double t[4][3] = {{a, b, b}, {b, a, b}, {b, b, a}, {b, b, b}};
+    for (int q = 0; q < 4; ++q) {
+      for (int d = 0; d < 3; ++d) gp[d * 4 + q] = t[q][d];
+      gw[q] = 0.2500000000000000 / 6.0;
+    }
+  } else if (ngp == 8) {
+    const double s = 0.57735026918963;
+    for (int q = 0; q < 8; ++q) {
+      gp[0 * 8 + q] = (q & 1) ? s : -s;
+      gp[1 * 8 + q] = (q & 2) ? s : -s;
+      gp[2 * 8 + q] = (q & 4) ? s : -s;
+      gw[q] = 1.0;
+    }
+  } else if (ngp == 6) {
+    const double a = 0.6666666666666667, b = 0.1666666666666667;
+    const double z = 0.5773502691896257645091488;
+    const double t[6][3] = {{a, b, -z}, {b, a, -z}, {b, b, -z},
+                            {a, b, z},  {b, a, z},  {b, b, z}};
+    for (int q = 0; q < 6; ++q) {
+      for (int d = 0; d < 3; ++d) gp[d * 6 + q] = t[q][d];
+      gw[q] = 0.1666666666666667;
+    }
+  } else {
+    return -1;
+  }
+  return 0;
+}
+
+/* eval_shape_FEM.f90:148-196 (linhexshl), node order of :161-168 */
+void orc_linhexshl(const double gp[3], double *shl, double *sg) {
+  const double zi = gp[0], eta = gp[1], zeta = gp[2], e = 0.125;
+  shl[0] = (zi - 1.0) * (eta + 1.0) * (zeta - 1.0) * e;
+  shl[1] = -(zi + 1.0) * (eta + 1.0) * (zeta - 1.0) * e;
+  shl[2] = (zi + 1.0) * (eta + 1.0) * (zeta + 1.0) * e;
+  shl[3] = -(zi - 1.0) * (eta + 1.0) * (zeta + 1.0) * e;
+  shl[4] = -(zi - 1.0) * (eta - 1.0) * (zeta - 1.0) * e;
+  shl[5] = (zi + 1.0) * (eta - 1.0) * (zeta - 1.0) * e;
+  shl[6] = -(zi + 1.0) * (eta - 1.0) * (zeta + 1.0) * e;
+  shl[7] = (zi - 1.0) * (eta - 1.0) * (zeta + 1.0) * e;
+
+  sg[0] = (eta + 1.0) * (zeta - 1.0) * e;
+  sg[1] = -(eta + 1.0) * (zeta - 1.0) * e;
+  sg[2] = (eta + 1.0) * (zeta + 1.0) * e;
+  sg[3] = -(eta + 1.0) * (zeta + 1.0) * e;
+  sg[4] = -(eta - 1.0) * (zeta - 1.0) * e;
+  sg[5] = (eta - 1.0) * (zeta - 1.0) * e;
+  sg[6] = -(eta - 1.0) * (zeta + 1.0) * e;
+  sg[7] = (eta - 1.0) * (zeta + 1.0) * e;
+
+  sg[8 + 0] = (zi - 1.0) * (zeta - 1.0) * e;
+  sg[8 + 1] = -(zi + 1.0) * (zeta - 1.0) * e;
+  sg[8 + 2] = (zi + 1.0) * (zeta + 1.0) * e;
+  sg[8 + 3] = -(zi - 1.0) * (zeta + 1.0) * e;
+  sg[8 + 4] = -(zi - 1.0) * (zeta - 1.0) * e;
+  sg[8 + 5] = (zi + 1.0) * (zeta - 1.0) * e;
+  sg[8 + 6] = -(zi + 1.0) * (zeta + 1.0) * e;
+  sg[8 + 7] = (zi - 1.0) * (zeta + 1.0) * e;
+
+  sg[16 + 0] = (zi - 1.0) * (eta + 1.0) * e;
+  sg[16 + 1] = -(zi + 1.0) * (eta + 1.0) * e;
+  sg[16 + 2] = (zi + 1.0) * (eta + 1.0) * e;
+  sg[16 + 3] = -(zi - 1.0) * (eta + 1.0) * e;
+  sg[16 + 4] = -(zi - 1.0) * (eta - 1.0) * e;
+  sg[16 + 5] = (zi + 1.0) * (eta - 1.0) * e;
+  sg[16 + 6] = -(zi + 1.0) * (eta - 1.0) * e;
+  sg[16 + 7] = (zi - 1.0) * (eta - 1.0) * e;
+}
+
+/* eval_shape_FEM.f90:201-225 (lintetshl) */
+void orc_lintetshl(const double gp[3], double *shl, double *sg) {
+  shl[0] = gp[0];
+  shl[1] = gp[1];
+  shl[2] = gp[2];
+  shl[3] = 1.0 - gp[0] - gp[1] - gp[2];
+  for (int i = 0; i < 12; ++i) sg[i] = 0.0;
+  sg[0 + 0] = 1.0;
+  sg[4 + 1] = 1.0;
+  sg[8 + 2] = 1.0;
+  sg[0 + 3] = -1.0;
+  sg[4 + 3] = -1.0;
+  sg[8 + 3] = -1.0;
+}
+
+/* eval_shape_FEM.f90:438-477 (linprishl) */
+void orc_linprishl(const double gp[3], double *shl, double *sg) {
+  const double zi = gp[0], eta = gp[1], zeta = gp[2];
+  shl[0] = 0.5 * zi * (1.0 - zeta);
+  shl[1] = 0.5 * eta * (1.0 - zeta);
+  shl[2] = 0.5 * (1.0 - zi - eta) * (1.0 - zeta);
+  shl[3] = 0.5 * zi * (1.0 + zeta);
+  shl[4] = 0.5 * eta * (1.0 + zeta);
+  shl[5] = 0.5 * (1.0 - zi - eta) * (1.0 + zeta);
+  for (int i = 0; i < 18; ++i) sg[i] = 0.0;
+  sg[0] = 0.5 * (1.0 - zeta);
+  sg[2] = -0.5 * (1.0 - zeta);
+  sg[3] = 0.5 * (1.0 + zeta);
+  sg[5] = -0.5 * (1.0 + zeta);
+  sg[6 + 1] = 0.5 * (1.0 - zeta);
+  sg[6 + 2] = -0.5 * (1.0 - zeta);
+  sg[6 + 4] = 0.5 * (1.0 + zeta);
+  sg[6 + 5] = -0.5 * (1.0 + zeta);
+  sg[12 + 0] = -0.5 * zi;
+  sg[12 + 1] = -0.5 * eta;
+  sg[12 + 2] = -0.5 * (1.0 - zi - eta);
+  sg[12 + 3] = 0.5 * zi;
+  sg[12 + 4] = 0.5 * eta;
+  sg[12 + 5] = 0.5 * (1.0 - zi - eta);
+}
+
+/* column-major 3x3 accessor */
+#define M3(A, i, j) (A)[(i) + 3 * (j)]
+
+/* eval_shape.f90:85-112 (get_inverse_3x3) */
+void orc_inverse3x3(const double *A, double *Ai, double *detj) {
+  double tmp;
+  M3(Ai, 0, 0) = M3(A, 1, 1) * M3(A, 2, 2) - M3(A, 2, 1) * M3(A, 1, 2);
+  M3(Ai, 0, 1) = M3(A, 2, 1) * M3(A, 0, 2) - M3(A, 0, 1) * M3(A, 2, 2);
+  M3(Ai, 0, 2) = M3(A, 0, 1) * M3(A, 1, 2) - M3(A, 0, 2) * M3(A, 1, 1);
+  tmp = 1.0 / (M3(Ai, 0, 0) * M3(A, 0, 0) + M3(Ai, 0, 1) * M3(A, 1, 0) +
+               M3(Ai, 0, 2) * M3(A, 2, 0));
+  M3(Ai, 0, 0) = M3(Ai, 0, 0) * tmp;
+  M3(Ai, 0, 1) = M3(Ai, 0, 1) * tmp;
+  M3(Ai, 0, 2) = M3(Ai, 0, 2) * tmp;
+  M3(Ai, 1, 0) = (M3(A, 1, 2) * M3(A, 2, 0) - M3(A, 1, 0) * M3(A, 2, 2)) * tmp;
+  M3(Ai, 1, 1) = (M3(A, 0, 0) * M3(A, 2, 2) - M3(A, 2, 0) * M3(A, 0, 2)) * tmp;
+  M3(Ai, 1, 2) = (M3(A, 1, 0) * M3(A, 0, 2) - M3(A, 0, 0) * M3(A, 1, 2)) * tmp;
+  M3(Ai, 2, 0) = (M3(A, 1, 0) * M3(A, 2, 1) - M3(A, 1, 1) * M3(A, 2, 0)) * tmp;
+  M3(Ai, 2, 1) = (M3(A, 2, 0) * M3(A, 0, 1) - M3(A, 0, 0) * M3(A, 2, 1)) * tmp;
+  M3(Ai, 2, 2) = (M3(A, 0, 0) * M3(A, 1, 1) - M3(A, 0, 1) * M3(A, 1, 0)) * tmp;
+  *detj = 1.0 / tmp;
+}
+
+/* eval_shape_FEM.f90:4-61 (eval_shape_fem); the hex branch stands for the
+ * commented-out linhexshl call at :26 (disabled in the reference, Q2) */
+int orc_eval_shape_fem(int nshl, const double gp[3], const double *xl,
+                       const double *dl, double *shlu, double *shgradgu,
+                       double *dxidx, double *Gij, double *Ginv,
+                       double *detj) {
+  double shgradlu[8 * 3], dxdxi[9];
+  if (nshl == 4)
+    orc_lintetshl(gp, shlu, shgradlu);
+  else if (nshl == 6)
+    orc_linprishl(gp, shlu, shgradlu);
+  else if (nshl == 8)
+    orc_linhexshl(gp, shlu, shgradlu);
+  else
+    return -1;
+  for (int i = 0; i < NSD; ++i)
+    for (int j = 0; j < NSD; ++j) {
+      double s = 0.0;
+      for (int a = 0; a < nshl; ++a)
+        s += (xl[i * nshl + a] + dl[i * nshl + a]) * shgradlu[j * nshl + a];
+      M3(dxdxi, i, j) = s;
+    }
+  orc_inverse3x3(dxdxi, dxidx, detj);
+  for (int a = 0; a < nshl; ++a)
+    for (int j = 0; j < NSD; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < NSD; ++k) s += shgradlu[k * nshl + a] * M3(dxidx, k, j);
+      shgradgu[j * nshl + a] = s;
+    }
+  for (int j = 0; j < NSD; ++j)
+    for (int i = 0; i < NSD; ++i) {
+      double s = 0.0, t = 0.0;
+      for (int k = 0; k < NSD; ++k) {
+        s += M3(dxidx, k, i) * M3(dxidx, k, j);
+        t += M3(dxdxi, i, k) * M3(dxdxi, j, k);
+      }
+      M3(Gij, i, j) = s;
+      M3(Ginv, i, j) = t;
+    }
+  return (*detj < 0.0) ? 1 : 0;
+}
+
+/* e3STAB.f90:4-44 (e3STAB_3D) */
+void orc_stab_ns(const double *G, double dt, const double *uadv, double rho,
+                 double mu, double *tauM, double *tauP, double *tauC) {
+  double taua = 0.0, gg = 0.0;
+  for (int j = 0; j < NSD; ++j)
+    for (int i = 0; i < NSD; ++i) taua = taua + uadv[i] * M3(G, i, j) * uadv[j];
+  double nu = mu / rho;
+  for (int k = 0; k < 9; ++k) gg += G[k] * G[k];
+  double taud = nu * nu * 12.0 * gg;
+  double taut = 4.0 / (dt * dt);
+  *tauM = 1.0 / (rho * sqrt(taut + taua + taud));
+  *tauP = *tauM;
+  *tauC = rho * sqrt(taua + taud) / (M3(G, 0, 0) + M3(G, 1, 1) + M3(G, 2, 2));
+}
+
+/* e3STAB.f90:394-412 (e3CFL) */
+static double orc_cfl(const double *uadv, const double *G, double dt) {
+  double c = 0.0;
+  for (int i = 0; i < NSD; ++i)
+    for (int j = 0; j < NSD; ++j) c = c + uadv[i] * M3(G, i, j) * uadv[j];
+  return dt * sqrt(c);
+}
+
+/* genSparStruc.f90:4-105: node adjacency through shared elements, each row
+ * sorted ascending (diagonal included).  The reference builds the list in
+ * element order with linear de-duplication (Asadj) and selection-sorts it;
+ * the sorted set is what both produce. */
+long orc_sparsity(int nnode, int nelem, int nshl, const int *ien, int *col,
+                  int *row) {
+  const int cap = 27 * 8 > 27 * nshl ? 27 * 8 : 27 * nshl;
+  int *list = (int *)malloc(sizeof(int) * (size_t)nnode * cap);
+  int *cnt = (int *)calloc((size_t)nnode, sizeof(int));
+  long icnt = 0;
+  if (!list || !cnt) {
+    free(list);
+    free(cnt);
+    return -1;
+  }
+  for (int e = 0; e < nelem; ++e) {
+    for (int j = 0; j < nshl; ++j) {
+      int jn = ien[j * nelem + e];
+      int *L = list + (size_t)jn * cap;
+      int len = cnt[jn];
+      for (int k = 0; k < nshl; ++k) {
+        int kn = ien[k * nelem + e];
+        int found = 0;
+        for (int l = 0; l < len; ++l)
+          if (L[l] == kn) {
+            found = 1;
+            break;
+          }
+        if (!found) {
+          if (len >= cap) {
+            free(list);
+            free(cnt);
+            return -2;
+          }
+          L[len++] = kn;
+        }
+      }
+      cnt[jn] = len;
+    }
+  }
+  for (int i = 0; i < nnode; ++i) icnt += cnt[i];
+  if (col && row) {
+    long k = 0;
+    col[0] = 0;
+    for (int i = 0; i < nnode; ++i) {
+      int *L = list + (size_t)i * cap;
+      int n = cnt[i];
+      /* selection sort, as genSparStruc.f90:35-45 */
+      for (int s = 0; s < n; ++s) {
+        int best = s;
+        for (int t = s + 1; t < n; ++t)
+          if (L[t] < L[best]) best = t;
+        int tmp = L[s];
+        L[s] = L[best];
+        L[best] = tmp;
+        row[k++] = L[s];
+      }
+      col[i + 1] = (int)k;
+    }
+  }
+  free(list);
+  free(cnt);
+  return icnt;
+}
+
+/* FillSparseMat.f90:4-28 (SparseMatLoc_3D): index of the last entry <= goal */
+int orc_sparse_loc(const int *row, int lo, int hi, int goal) {
+  int vl = 0, vh = hi - lo; /* 0-based over the row segment, vh = n */
+  while (vh - vl > 1) {
+    int v = (vh + vl) / 2;
+    if (row[lo + v] > goal)
+      vh = v;
+    else
+      vl = v;
+  }
+  return lo + vl;
+}
+
+/*
+ * Element kernel for one NS element, the body of the element loop of
+ * assembleNavSto.f90:195-291 (gather done by the caller), with
+ * e3LHS.f90:4-161 and e3RHS.f90:4-97 inlined.  Terms multiplied by the
+ * identically-zero inputs of the live call (tauBar = 0, uprime = shhessgu = 0,
+ * Hessian = 0; Q5) are exact zeros and are dropped.
+ *
+ * xl,dl,ul,uml,acl,acml: (NSHL,3) col-major; pl(NSHL).
+ * ke(9,NSHL,NSHL) ge(3,..) de(3,..) me(NSHL,NSHL) ru(3,NSHL) rp(NSHL): accumulated.
+ */
+int orc_element_ns(int nshl, int ngauss, const double *xl, const double *dl,
+                   const double *ul, const double *uml, const double *acl,
+                   const double *acml, const double *pl, const double *prm,
+                   int tensor_flag, double *ke, double *ge, double *de,
+                   double *me, double *ru, double *rp, double *cfl) {
+  (void)acml;
+  const double almi = prm[0], alfi = prm[1], gami = prm[2];
+  const double delt = prm[4], rhoi = prm[5], mui = prm[6], kdc = prm[7];
+  const double grav[3] = {prm[8], prm[9], prm[10]};
+  double gp[8 * 3], gw[8];
+  int nneg = 0;
+  if (orc_gauss_volume(ngauss, gp, gw) != 0) return -1;
+  const int do_mat = (tensor_flag & 4) != 0, do_vec = (tensor_flag & 2) != 0;
+
+  for (int q = 0; q < ngauss; ++q) {
+    double pt[3] = {gp[0 * ngauss + q], gp[1 * ngauss + q], gp[2 * ngauss + q]};
+    double shlu[8], shg[8 * 3], dxidx[9], Gij[9], Ginv[9], detj;
+    if (orc_eval_shape_fem(nshl, pt, xl, dl, shlu, shg, dxidx, Gij, Ginv,
+                           &detj) > 0)
+      ++nneg;
+    const double gwt = gw[q];
+    /* e3int_qr / e3int_qr_grad (e3int.f90:18-47) */
+    double ui[3], umi[3], aci[3], pri = 0.0, du[9], dp[3];
+    for (int i = 0; i < 3; ++i) {
+      double su = 0.0, sm = 0.0, sa = 0.0;
+      for (int a = 0; a < nshl; ++a) {
+        su += ul[i * nshl + a] * shlu[a];
+        sm += uml[i * nshl + a] * shlu[a];
+        sa += acl[i * nshl + a] * shlu[a];
+      }
+      ui[i] = su;
+      umi[i] = sm;
+      aci[i] = sa;
+    }
+    for (int a = 0; a < nshl; ++a) pri += pl[a] * shlu[a];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        double s = 0.0;
+        for (int a = 0; a < nshl; ++a) s += ul[i * nshl + a] * shg[j * nshl + a];
+        M3(du, i, j) = s;
+      }
+    for (int j = 0; j < 3; ++j) {
+      double s = 0.0;
+      for (int a = 0; a < nshl; ++a) s += pl[a] * shg[j * nshl + a];
+      dp[j] = s;
+    }
+    /* assembleNavSto.f90:252-266 */
+    double uadv[3], fi[3], rm[3], rLi[3];
+    for (int i = 0; i < 3; ++i) uadv[i] = ui[i] - umi[i];
+    const double divu = M3(du, 0, 0) + M3(du, 1, 1) + M3(du, 2, 2);
+    for (int i = 0; i < 3; ++i) fi[i] = grav[i] * rhoi;
+    for (int i = 0; i < 3; ++i) {
+      double mv = 0.0;
+      for (int j = 0; j < 3; ++j) mv += M3(du, i, j) * uadv[j];
+      rm[i] = rhoi * (aci[i] + mv) - fi[i];
+      rLi[i] = rm[i] + dp[i] - mui * (0.0 + 0.0 + 0.0);
+    }
+    double tauM, tauP, tauC;
+    orc_stab_ns(Gij, delt, uadv, rhoi, mui, &tauM, &tauP, &tauC);
+    double c = orc_cfl(uadv, Gij, delt);
+    if (c > *cfl) *cfl = c;
+    (void)rLi;
+
+    if (do_mat) {
+      /* e3LHS.f90:34-159 */
+      const double fact1 = almi, fact2 = alfi * gami * delt;
+      const double mupkdc = mui + kdc;
+      double tmp1[8];
+      for (int a = 0; a < nshl; ++a)
+        tmp1[a] = rhoi * (uadv[0] * shg[0 * nshl + a] + uadv[1] * shg[1 * nshl + a] +
+                          uadv[2] * shg[2 * nshl + a]);
+      for (int b = 0; b < nshl; ++b)
+        for (int a = 0; a < nshl; ++a) {
+          const double Na = shlu[a], Nb = shlu[b];
+          const double ga[3] = {shg[a], shg[nshl + a], shg[2 * nshl + a]};
+          const double gb[3] = {shg[b], shg[nshl + b], shg[2 * nshl + b]};
+          const double tmp4 =
+              fact1 * (Na * rhoi * Nb + tmp1[a] * tauM * rhoi * Nb) +
+              fact2 * (Na * tmp1[b] + tmp1[a] * tauM * tmp1[b] +
+                       mupkdc * (ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2]));
+          double *K = ke + (size_t)(b * nshl + a) * 9;
+          /* K11 index (r)*3+c : row comp r, column comp c */
+          K[0] += (tmp4 + fact2 * (ga[0] * mupkdc * gb[0] + ga[0] * tauC * gb[0])) * detj * gwt;
+          K[4] += (tmp4 + fact2 * (ga[1] * mupkdc * gb[1] + ga[1] * tauC * gb[1])) * detj * gwt;
+          K[8] += (tmp4 + fact2 * (ga[2] * mupkdc * gb[2] + ga[2] * tauC * gb[2])) * detj * gwt;
+          K[1] += fact2 * (ga[1] * mupkdc * gb[0] + ga[0] * tauC * gb[1]) * detj * gwt;
+          K[3] += fact2 * (ga[0] * mupkdc * gb[1] + ga[1] * tauC * gb[0]) * detj * gwt;
+          K[2] += fact2 * (ga[2] * mupkdc * gb[0] + ga[0] * tauC * gb[2]) * detj * gwt;
+          K[6] += fact2 * (ga[0] * mupkdc * gb[2] + ga[2] * tauC * gb[0]) * detj * gwt;
+          K[5] += fact2 * (ga[2] * mupkdc * gb[1] + ga[1] * tauC * gb[2]) * detj * gwt;
+          K[7] += fact2 * (ga[1] * mupkdc * gb[2] + ga[2] * tauC * gb[1]) * detj * gwt;
+          double *G = ge + (size_t)(b * nshl + a) * 3;
+          double *D = de + (size_t)(b * nshl + a) * 3;
+          for (int i = 0; i < 3; ++i) {
+            G[i] += fact2 * (-ga[i] * Nb + tmp1[a] * tauM * gb[i]) * detj * gwt;
+            D[i] += (fact2 * (Na * gb[i] + ga[i] * tauP * tmp1[b]) +
+                     fact1 * (ga[i] * tauP * rhoi * Nb)) *
+                    detj * gwt;
+          }
+          me[b * nshl + a] += fact2 * tauP * (ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2]) *
+                              detj * gwt;
+        }
+    }
+    if (do_vec) {
+      /* e3RHS.f90:26-95 with uprime = 0 */
+      const double mupkdc = mui + kdc;
+      const double ptot = pri - tauC * divu;
+      double t1[3], t2[9];
+      for (int i = 0; i < 3; ++i)
+        t1[i] = rhoi * (aci[i] + uadv[0] * M3(du, i, 0) + uadv[1] * M3(du, i, 1) +
+                        uadv[2] * M3(du, i, 2)) -
+                fi[i];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          if (i == j)
+            M3(t2, i, j) = -ptot + 2.0 * mupkdc * M3(du, i, i);
+          else
+            M3(t2, i, j) = mupkdc * (M3(du, i, j) + M3(du, j, i));
+        }
+      for (int a = 0; a < nshl; ++a) {
+        for (int i = 0; i < 3; ++i) {
+          double s = 0.0;
+          for (int j = 0; j < 3; ++j) s += shg[j * nshl + a] * M3(t2, i, j);
+          ru[a * 3 + i] = ru[a * 3 + i] - (shlu[a] * t1[i] + s) * detj * gwt;
+        }
+      }
+      for (int a = 0; a < nshl; ++a)
+        rp[a] = rp[a] - (shlu[a] * divu - 0.0) * detj * gwt;
+    }
+  }
+  return nneg;
+}
+
+/* BCLhs_3D.f90:4-122 */
+void orc_bclhs_3d(int nshl, const int *ibcl, double *ke, double *ge, double *de,
+                  double *me, double *ru, double *rp) {
+#define KE(i, a, b) ke[((size_t)(b) * nshl + (a)) * 9 + (i)]
+#define GE(i, a, b) ge[((size_t)(b) * nshl + (a)) * 3 + (i)]
+#define DE(i, a, b) de[((size_t)(b) * nshl + (a)) * 3 + (i)]
+#define ME(a, b) me[(size_t)(b) * nshl + (a)]
+  for (int a = 0; a < nshl; ++a) {
+    for (int c = 0; c < 3; ++c) {
+      if (ibcl[c * nshl + a] != 1) continue;
+      for (int b = 0; b < nshl; ++b) {
+        KE(3 * c + 0, a, b) = 0.0;
+        KE(3 * c + 1, a, b) = 0.0;
+        KE(3 * c + 2, a, b) = 0.0;
+        GE(c, a, b) = 0.0;
+      }
+      for (int b = 0; b < nshl; ++b) {
+        KE(c + 0, b, a) = 0.0;
+        KE(c + 3, b, a) = 0.0;
+        KE(c + 6, b, a) = 0.0;
+        DE(c, b, a) = 0.0;
+      }
+      KE(4 * c, a, a) = 1.0;
+      ru[a * 3 + c] = 0.0;
+    }
+    if (ibcl[3 * nshl + a] == 1) {
+      for (int b = 0; b < nshl; ++b) {
+        ME(a, b) = 0.0;
+        DE(0, a, b) = DE(1, a, b) = DE(2, a, b) = 0.0;
+        GE(0, b, a) = GE(1, b, a) = GE(2, b, a) = 0.0;
+      }
+      ME(a, a) = 1.0;
+      rp[a] = 0.0;
+    }
+  }
+#undef KE
+#undef GE
+#undef DE
+#undef ME
+}
+
+/* assembleNavSto.f90:76-336 (IntElmAss_NS_overset) with BCLhs_3D,
+ * FillSparseMat_3D (FillSparseMat.f90:33-67) and the RHS scatter of :300-305 */
+int orc_assemble_ns(int nnode, int nelem, int nshl, int ngauss,
+                    const int *ien, const int *elm_id, const int *elm_id_set,
+                    int subdomain_flag, int tensor_flag, const double *xg,
+                    const int *ibc, int ibc_ncol, const double *dgA,
+                    const double *ugA, const double *ugmA, const double *acgA,
+                    const double *acgmA, const double *pgA, const double *prm,
+                    const int *col, const int *row, double *k11, double *g,
+                    double *d1, double *m, double *rhsu, double *rhsp,
+                    double *cfl_out) {
+  (void)ibc_ncol;
+  const int n2 = nshl * nshl;
+  double xl[24], dl[24], ul[24], uml[24], acl[24], acml[24], pl[8];
+  double ke[9 * 64], ge[3 * 64], de[3 * 64], me[64], ru[24], rp[8];
+  int ibcl[32];
+  int nneg = 0;
+  double cfl = 0.0;
+  for (int e = 0; e < nelem; ++e) {
+    int tgt = 0;
+    if (subdomain_flag & 1) tgt = tgt || elm_id[e] == elm_id_set[0];
+    if (subdomain_flag & 2) tgt = tgt || elm_id[e] == elm_id_set[1];
+    for (int a = 0; a < nshl; ++a) {
+      const int n = ien[a * nelem + e];
+      for (int i = 0; i < 3; ++i) {
+        xl[i * nshl + a] = xg[i * nnode + n];
+        dl[i * nshl + a] = dgA[i * nnode + n];
+        ul[i * nshl + a] = ugA[i * nnode + n];
+        acl[i * nshl + a] = acgA[i * nnode + n];
+        uml[i * nshl + a] = ugmA[i * nnode + n];
+        acml[i * nshl + a] = acgmA[i * nnode + n];
+      }
+      pl[a] = pgA[n];
+      for (int c = 0; c < 4; ++c) ibcl[c * nshl + a] = ibc[c * nnode + n];
+    }
+    memset(ke, 0, sizeof(double) * 9 * n2);
+    memset(ge, 0, sizeof(double) * 3 * n2);
+    memset(de, 0, sizeof(double) * 3 * n2);
+    memset(me, 0, sizeof(double) * n2);
+    memset(ru, 0, sizeof(double) * 3 * nshl);
+    memset(rp, 0, sizeof(double) * nshl);
+    if (tgt) {
+      int r = orc_element_ns(nshl, ngauss, xl, dl, ul, uml, acl, acml, pl, prm,
+                             tensor_flag, ke, ge, de, me, ru, rp, &cfl);
+      if (r < 0) return r;
+      nneg += r;
+    } else {
+      /* non-target elements still evaluate shape/CFL in the reference but
+       * contribute nothing before the BC projection */
+      double tmpk[9 * 64], tmpg[3 * 64], tmpd[3 * 64], tmpm[64], tmpu[24], tmpp[8];
+      memset(tmpk, 0, sizeof tmpk);
+      memset(tmpg, 0, sizeof tmpg);
+      memset(tmpd, 0, sizeof tmpd);
+      memset(tmpm, 0, sizeof tmpm);
+      memset(tmpu, 0, sizeof tmpu);
+      memset(tmpp, 0, sizeof tmpp);
+      int r = orc_element_ns(nshl, ngauss, xl, dl, ul, uml, acl, acml, pl, prm,
+                             0, tmpk, tmpg, tmpd, tmpm, tmpu, tmpp, &cfl);
+      if (r < 0) return r;
+      nneg += r;
+    }
+    orc_bclhs_3d(nshl, ibcl, ke, ge, de, me, ru, rp);
+    if (tensor_flag & 4) {
+      for (int a = 0; a < nshl; ++a) {
+        const int i = ien[a * nelem + e];
+        const int c0 = col[i], c1 = col[i + 1];
+        for (int b = 0; b < nshl; ++b) {
+          const int k = orc_sparse_loc(row, c0, c1, ien[b * nelem + e]);
+          const size_t ab = (size_t)b * nshl + a;
+          for (int q = 0; q < 9; ++q) k11[(size_t)k * 9 + q] += ke[ab * 9 + q];
+          for (int q = 0; q < 3; ++q) g[(size_t)k * 3 + q] += ge[ab * 3 + q];
+          for (int q = 0; q < 3; ++q) d1[(size_t)k * 3 + q] += de[ab * 3 + q];
+          m[k] += me[ab];
+        }
+      }
+    }
+    if (tensor_flag & 2) {
+      for (int b = 0; b < nshl; ++b) {
+        const int n = ien[b * nelem + e];
+        for (int i = 0; i < 3; ++i) rhsu[i * nnode + n] += ru[b * 3 + i];
+        rhsp[n] += rp[b];
+      }
+    }
+  }
+  if (cfl_out) *cfl_out = cfl;
+  return nneg;
+}
+
+/* SparseProd.f90:4-60 (SparseProdUP_3D) */
+void orc_spmv_up(int nnode, const int *col, const int *row, const double *k11,
+                 const double *g, const double *d1, const double *m,
+                 const double *xu, const double *xp, double *yu, double *yp) {
+  for (int a = 0; a < nnode; ++a) {
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
+    for (int b = col[a]; b < col[a + 1]; ++b) {
+      const int c = row[b];
+      const double *K = k11 + (size_t)b * 9;
+      const double x0 = xu[c], x1 = xu[nnode + c], x2 = xu[2 * nnode + c], xp_ = xp[c];
+      t0 = t0 + K[0] * x0 + K[1] * x1 + K[2] * x2 + g[b * 3 + 0] * xp_;
+      t1 = t1 + K[3] * x0 + K[4] * x1 + K[5] * x2 + g[b * 3 + 1] * xp_;
+      t2 = t2 + K[6] * x0 + K[7] * x1 + K[8] * x2 + g[b * 3 + 2] * xp_;
+      t3 = t3 + d1[b * 3 + 0] * x0 + d1[b * 3 + 1] * x1 + d1[b * 3 + 2] * x2 + m[b] * xp_;
+    }
+    yu[a] = 0.0 + t0;
+    yu[nnode + a] = 0.0 + t1;
+    yu[2 * nnode + a] = 0.0 + t2;
+    yp[a] = 0.0 + t3;
+  }
+}
+
+/* SparseGMRES_NS.f90:4-369 (SparseGMRES_up), single-rank path */
+int orc_gmres_up(int nnode, const int *col, const int *row, const double *k11,
+                 const double *g, const double *d1, const double *m,
+                 const double *rhsu, const double *rhsp, double *solu,
+                 double *solp, double tol, int kspace, int kspace_min,
+                 double *resid, double *rnorm0) {
+  const size_t N = (size_t)nnode;
+  const int K = kspace;
+  double *Bd = (double *)calloc(N * 9, sizeof(double)); /* lhsKBdiagu(i, 1:9) */
+  double *Md = (double *)calloc(N, sizeof(double));
+  double *Vu = (double *)malloc(sizeof(double) * N * 3 * (K + 1));
+  double *Vp = (double *)malloc(sizeof(double) * N * (K + 1));
+  double *tu = (double *)malloc(sizeof(double) * N * 3);
+  double *tp = (double *)malloc(sizeof(double) * N);
+  double *H = (double *)calloc((size_t)(K + 1) * K, sizeof(double));
+  double *e = (double *)calloc((size_t)K + 1, sizeof(double));
+  double *y = (double *)calloc((size_t)K, sizeof(double));
+  double *Rc = (double *)calloc((size_t)K, sizeof(double));
+  double *Rs = (double *)calloc((size_t)K, sizeof(double));
+#define HH(i, j) H[(i) + (size_t)(K + 1) * (j)]
+#define VU(i, c, k) Vu[(i) + N * (c) + N * 3 * (k)]
+#define VP(i, k) Vp[(i) + N * (k)]
+  /* :53-61 diagonal blocks */
+  for (size_t i = 0; i < N; ++i)
+    for (int j = col[i]; j < col[i + 1]; ++j)
+      if ((size_t)row[j] == i) {
+        for (int q = 0; q < 9; ++q) Bd[i * 9 + q] = k11[(size_t)j * 9 + q];
+        Md[i] = m[j];
+      }
+  /* :73-121 invert */
+  for (size_t i = 0; i < N; ++i) {
+    double *B = Bd + i * 9, Bi[9], tmp, tmp2;
+    Bi[0] = B[4] * B[8] - B[7] * B[5];
+    Bi[1] = B[7] * B[2] - B[1] * B[8];
+    Bi[2] = B[1] * B[5] - B[2] * B[4];
+    tmp2 = Bi[0] * B[0] + Bi[1] * B[3] + Bi[2] * B[6];
+    tmp = 1.0 / tmp2;
+    Bi[0] = Bi[0] * tmp;
+    Bi[1] = Bi[1] * tmp;
+    Bi[2] = Bi[2] * tmp;
+    Bi[3] = (B[5] * B[6] - B[3] * B[8]) * tmp;
+    Bi[4] = (B[0] * B[8] - B[6] * B[2]) * tmp;
+    Bi[5] = (B[3] * B[2] - B[0] * B[5]) * tmp;
+    Bi[6] = (B[3] * B[7] - B[4] * B[6]) * tmp;
+    Bi[7] = (B[6] * B[1] - B[0] * B[7]) * tmp;
+    Bi[8] = (B[0] * B[4] - B[1] * B[3]) * tmp;
+    memcpy(B, Bi, sizeof Bi);
+    Md[i] = (Md[i] == 0.0) ? 1.0 : 1.0 / Md[i];
+  }
+  /* :124-139 precondition residual */
+  for (size_t i = 0; i < N; ++i) {
+    const double *B = Bd + i * 9;
+    const double r0 = rhsu[i], r1 = rhsu[N + i], r2 = rhsu[2 * N + i];
+    VU(i, 0, 0) = B[0] * r0 + B[1] * r1 + B[2] * r2;
+    VU(i, 1, 0) = B[3] * r0 + B[4] * r1 + B[5] * r2;
+    VU(i, 2, 0) = B[6] * r0 + B[7] * r1 + B[8] * r2;
+    VP(i, 0) = rhsp[i] * Md[i];
+  }
+  /* :143-155 norm */
+  double rr = 0.0;
+  for (size_t i = 0; i < N; ++i) rr += VP(i, 0) * VP(i, 0);
+  for (int c = 0; c < 3; ++c) {
+    double s = 0.0;
+    for (size_t i = 0; i < N; ++i) s += VU(i, c, 0) * VU(i, c, 0);
+    rr = rr + s;
+  }
+  double unorm = sqrt(rr);
+  if (rnorm0) *rnorm0 = unorm;
+  const double epsnrm = tol * unorm;
+  e[0] = unorm;
+  for (size_t i = 0; i < N; ++i) {
+    for (int c = 0; c < 3; ++c) VU(i, c, 0) = VU(i, c, 0) / unorm;
+    VP(i, 0) = VP(i, 0) / unorm;
+  }
+  int iKs = 0;
+  double ercheck = 0.0, beta = 0.0;
+  for (int iK = 1; iK <= K; ++iK) {
+    iKs = iK;
+    const int kc = iKs - 1; /* 0-based index of current vector */
+    /* :200-203 product into v_{k+1} */
+    orc_spmv_up(nnode, col, row, k11, g, d1, m, &VU(0, 0, kc), &VP(0, kc), tu, tp);
+    /* :231-243 precondition product */
+    for (size_t i = 0; i < N; ++i) {
+      const double *B = Bd + i * 9;
+      const double r0 = tu[i], r1 = tu[N + i], r2 = tu[2 * N + i];
+      VU(i, 0, kc + 1) = B[0] * r0 + B[1] * r1 + B[2] * r2;
+      VU(i, 1, kc + 1) = B[3] * r0 + B[4] * r1 + B[5] * r2;
+      VU(i, 2, kc + 1) = B[6] * r0 + B[7] * r1 + B[8] * r2;
+      VP(i, kc + 1) = tp[i] * Md[i];
+    }
+    /* :246-287 modified Gram-Schmidt in the project-with-previous-beta form */
+    for (int jK = 1; jK <= iKs + 1; ++jK) {
+      const int jc = jK - 1;
+      if (jK > 1) {
+        for (size_t i = 0; i < N; ++i) {
+          for (int c = 0; c < 3; ++c)
+            VU(i, c, kc + 1) = VU(i, c, kc + 1) - beta * VU(i, c, jc - 1);
+          VP(i, kc + 1) = VP(i, kc + 1) - beta * VP(i, jc - 1);
+        }
+      }
+      double r = 0.0;
+      for (size_t i = 0; i < N; ++i) r += VP(i, kc + 1) * VP(i, jc);
+      for (int c = 0; c < 3; ++c) {
+        double s = 0.0;
+        for (size_t i = 0; i < N; ++i) s += VU(i, c, kc + 1) * VU(i, c, jc);
+        r = r + s;
+      }
+      beta = r;
+      HH(jc, kc) = beta;
+    }
+    unorm = sqrt(beta);
+    HH(kc + 1, kc) = unorm;
+    for (size_t i = 0; i < N; ++i) {
+      for (int c = 0; c < 3; ++c) VU(i, c, kc + 1) = VU(i, c, kc + 1) / unorm;
+      VP(i, kc + 1) = VP(i, kc + 1) / unorm;
+    }
+    /* :305-322 Givens */
+    for (int jK = 0; jK < kc; ++jK) {
+      double tmp = Rc[jK] * HH(jK, kc) + Rs[jK] * HH(jK + 1, kc);
+      HH(jK + 1, kc) = -Rs[jK] * HH(jK, kc) + Rc[jK] * HH(jK + 1, kc);
+      HH(jK, kc) = tmp;
+    }
+    double tmp = sqrt(HH(kc, kc) * HH(kc, kc) + HH(kc + 1, kc) * HH(kc + 1, kc));
+    Rc[kc] = HH(kc, kc) / tmp;
+    Rs[kc] = HH(kc + 1, kc) / tmp;
+    HH(kc, kc) = tmp;
+    HH(kc + 1, kc) = 0.0;
+    tmp = Rc[kc] * e[kc] + Rs[kc] * e[kc + 1];
+    e[kc + 1] = -Rs[kc] * e[kc] + Rc[kc] * e[kc + 1];
+    e[kc] = tmp;
+    ercheck = fabs(e[kc + 1]);
+    if (ercheck <= epsnrm && iKs >= kspace_min) break;
+  }
+  /* :341-352 back substitution and update */
+  for (int jK = iKs - 1; jK >= 0; --jK) {
+    y[jK] = e[jK] / HH(jK, jK);
+    for (int l = 0; l < jK; ++l) e[l] = e[l] - y[jK] * HH(l, jK);
+  }
+  for (int jK = 0; jK < iKs; ++jK)
+    for (size_t i = 0; i < N; ++i) {
+      for (int c = 0; c < 3; ++c) solu[c * N + i] = solu[c * N + i] + y[jK] * VU(i, c, jK);
+      solp[i] = solp[i] + y[jK] * VP(i, jK);
+    }
+  if (resid) *resid = ercheck;
+#undef HH
+#undef VU
+#undef VP
+  free(Bd);
+  free(Md);
+  free(Vu);
+  free(Vp);
+  free(tu);
+  free(tp);
+  free(H);
+  free(e);
+  free(y);
+  free(Rc);
+  free(Rs);
+  return iKs;
+}
+
+/* interpolation.f90:5-76 (interpolation_tet), single rank */
+void orc_interp_tet(int nnode, int nelem, int nshl, const int *ien,
+                    const int *elm_id, int src_id, const double *xg,
+                    const double *dg, const double *ug, int nrecv,
+                    const double *xr, const double *dr, int *donor,
+                    double *val) {
+  (void)nshl;
+  for (int j = 0; j < nrecv; ++j) {
+    donor[j] = -1;
+    val[j] = val[nrecv + j] = val[2 * nrecv + j] = 0.0;
+  }
+  for (int e = 0; e < nelem; ++e) {
+    if (elm_id[e] != src_id) continue;
+    if (nshl > 4) continue;
+    double xl[4][3], dl[4][3], A[9], Ai[9], tmp;
+    for (int a = 0; a < 4; ++a)
+      for (int k = 0; k < 3; ++k) {
+        xl[a][k] = xg[k * nnode + ien[a * nelem + e]];
+        dl[a][k] = dg[k * nnode + ien[a * nelem + e]];
+      }
+    /* Amat(j-1,k) = x_j - x_1 (+ d) then transposed: Amat(k, j-1) */
+    for (int k = 0; k < 3; ++k)
+      for (int a = 1; a < 4; ++a)
+        M3(A, k, a - 1) = xl[a][k] - xl[0][k] + dl[a][k] - dl[0][k];
+    orc_inverse3x3(A, Ai, &tmp);
+    for (int j = 0; j < nrecv; ++j) {
+      if (donor[j] >= 0) continue;
+      double bv[3], xv[4];
+      for (int k = 0; k < 3; ++k) bv[k] = xr[k * nrecv + j] - xl[0][k] + dr[k * nrecv + j];
+      for (int i = 0; i < 3; ++i)
+        xv[i] = M3(Ai, i, 0) * bv[0] + M3(Ai, i, 1) * bv[1] + M3(Ai, i, 2) * bv[2];
+      xv[3] = 1.0 - (xv[0] + xv[1] + xv[2]);
+      if (xv[0] >= 0 && xv[1] >= 0 && xv[2] >= 0 && xv[3] >= 0) {
+        for (int k = 0; k < 3; ++k) {
+          double s = val[k * nrecv + j];
+          s = s + xv[0] * ug[k * nnode + ien[0 * nelem + e]];
+          s = s + xv[1] * ug[k * nnode + ien[1 * nelem + e]];
+          s = s + xv[2] * ug[k * nnode + ien[2 * nelem + e]];
+          s = s + xv[3] * ug[k * nnode + ien[3 * nelem + e]];
+          val[k * nrecv + j] = s;
+        }
+        donor[j] = e;
+      }
+    }
+  }
+}

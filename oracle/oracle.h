@@ -1,0 +1,115 @@
+/*
+ * CPU oracle for the OversetDevFort hot path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * This library is a plain-C restatement of the reference Fortran routines on
+ * the data-parallel path (shape functions, quadrature, VMS/SUPG element
+ * assembly, Dirichlet projection, CSR pattern and scatter, block SpMV,
+ * block-Jacobi GMRES, tet fringe interpolation).  It exists so that tests,
+ * smoke() and the CPU-baseline leg of bench.py can check and time the HIP
+ * library against it.  The product library never links or calls it.
+ *
+ * Conventions follow the reference's in-memory layouts, but with 0-based C
+ * indices:
+ *   node fields     u(NNODE,3) column-major  -> u[c*nnode + i]
+ *   connectivity    IEN(NELEM,NSHL)          -> ien[a*nelem + e]  (0-based node ids)
+ *   IBC(NNODE,ncol)                          -> ibc[c*nnode + i]
+ *   CSR pattern     col(NNODE+1), row(icnt)  -> col[i]..col[i+1]-1, row[k] (0-based)
+ *   LHSK11(9,icnt)  -> k11[k*9 + r*3 + c]   (row-major 3x3, index (r-1)*3+c)
+ *   LHSG(3,icnt), LHSD1(3,icnt), LHSM(icnt)
+ *   element locals  xKebe11(9,NSHL,NSHL) -> ke[(b*nshl + a)*9 + i]
+ */
+#ifndef OFD_ORACLE_H
+#define OFD_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* generalized-alpha constants: out = {almi, alfi, gami, beti} */
+void orc_genalpha(double rhoinf, double out[4]);
+
+/* volume quadrature, gp(ngp,3) column-major, gw(ngp); returns 0 or -1 */
+int orc_gauss_volume(int ngp, double *gp, double *gw);
+
+/* reference-element shape functions; shgradl(NSHL,3) column-major */
+void orc_linhexshl(const double gp[3], double *shl, double *shgradl);
+void orc_lintetshl(const double gp[3], double *shl, double *shgradl);
+void orc_linprishl(const double gp[3], double *shl, double *shgradl);
+
+/* 3x3 inverse by cofactors; returns DetJ = 1/(1/det) as the reference does */
+void orc_inverse3x3(const double *A, double *Ainv, double *detj);
+
+/* eval_shape_fem: xl,dl (NSHL,3) col-major; outputs col-major 3x3 */
+int orc_eval_shape_fem(int nshl, const double gp[3], const double *xl,
+                       const double *dl, double *shlu, double *shgradgu,
+                       double *dxidx, double *Gij, double *Ginv, double *detj);
+
+/* stabilisation parameters */
+void orc_stab_ns(const double *Gij, double dt, const double *uadv, double rho,
+                 double mu, double *tauM, double *tauP, double *tauC);
+
+/* genSparStruc: pass col=NULL,row=NULL to get icnt only */
+long orc_sparsity(int nnode, int nelem, int nshl, const int *ien, int *col,
+                  int *row);
+
+/* binary search of SparseMatLoc_3D over row[lo:hi) */
+int orc_sparse_loc(const int *row, int lo, int hi, int goal);
+
+/*
+ * IntElmAss_NS_overset + BCLhs_3D + FillSparseMat_3D + RHS scatter.
+ * prm = {almi, alfi, gami, beti, delt, rho, mu, kdc, gx, gy, gz}
+ * Caller zeroes (or not) the outputs, exactly like assembleNavSto does.
+ * cfl_out receives the maximum Gauss-point CFL number.
+ */
+int orc_assemble_ns(int nnode, int nelem, int nshl, int ngauss,
+                    const int *ien, const int *elm_id, const int *elm_id_set,
+                    int subdomain_flag, int tensor_flag, const double *xg,
+                    const int *ibc, int ibc_ncol, const double *dgA,
+                    const double *ugA, const double *ugmA, const double *acgA,
+                    const double *acgmA, const double *pgA, const double *prm,
+                    const int *col, const int *row, double *k11, double *g,
+                    double *d1, double *m, double *rhsu, double *rhsp,
+                    double *cfl_out);
+
+/* one element only, before BC (for kernel-level tests) */
+int orc_element_ns(int nshl, int ngauss, const double *xl, const double *dl,
+                   const double *ul, const double *uml, const double *acl,
+                   const double *acml, const double *pl, const double *prm,
+                   int tensor_flag, double *ke, double *ge, double *de,
+                   double *me, double *ru, double *rp, double *cfl);
+
+/* BCLhs_3D on element locals; ibcl(NSHL,4) col-major */
+void orc_bclhs_3d(int nshl, const int *ibcl, double *ke, double *ge, double *de,
+                  double *me, double *ru, double *rp);
+
+/* SparseProdUP_3D; x,y as u(N,3) SoA + p(N) */
+void orc_spmv_up(int nnode, const int *col, const int *row, const double *k11,
+                 const double *g, const double *d1, const double *m,
+                 const double *xu, const double *xp, double *yu, double *yp);
+
+/*
+ * SparseGMRES_up, single MGS cycle.  sol is accumulated into (caller zeroes).
+ * Returns the number of Arnoldi iterations; final |e_{k+1}| in *resid,
+ * the preconditioned initial norm in *rnorm0.
+ */
+int orc_gmres_up(int nnode, const int *col, const int *row, const double *k11,
+                 const double *g, const double *d1, const double *m,
+                 const double *rhsu, const double *rhsp, double *solu,
+                 double *solp, double tol, int kspace, int kspace_min,
+                 double *resid, double *rnorm0);
+
+/*
+ * interpolation_tet: donor search on tet source elements with
+ * elm_id == src_id, for nrecv receiver points (xr,dr: (nrecv,3) SoA).
+ * donor[j] = first element index that contains point j (or -1), val(nrecv,3).
+ */
+void orc_interp_tet(int nnode, int nelem, int nshl, const int *ien,
+                    const int *elm_id, int src_id, const double *xg,
+                    const double *dg, const double *ug, int nrecv,
+                    const double *xr, const double *dr, int *donor,
+                    double *val);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

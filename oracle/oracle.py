@@ -1,0 +1,225 @@
+"""ctypes wrapper of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import this module, and only as the checker (or, for the
+baseline, as the thing timed on the host cores).  The HIP product path never
+imports it and fails loudly when its own library is missing.
+
+Pinning of each oracle function (what checks it independently of itself):
+
+=====================  =====================================================
+function               pinned by
+=====================  =====================================================
+genalpha               closed form (param.f90:469-480) + reference harness
+gauss_volume           reference harness genGPandGWtetpri bit-exact;
+                       exact integration of monomials (tests)
+linhexshl/tet/pri      reference harness bit-exact; partition of unity,
+                       Kronecker property, gradient sums (tests)
+inverse3x3             reference harness bit-exact; A @ Ainv = I
+eval_shape_fem         reference harness (tet, prism); affine volume
+                       identity sum DetJ*w = |element| for hex (tests)
+sparsity               reference genSparStruc bit-exact; brute force set
+                       construction; closed form prod(3n-2) (tests)
+assemble_ns            reference IntElmAss_NS_overset on tet meshes
+                       (element kernels, BC, scatter: <=1e-13); hex differs
+                       only by the shape/quadrature pinned above
+spmv_up                reference SparseProdUP_3D; dense product (tests)
+gmres_up               reference SparseGMRES_up; Arnoldi invariants and a
+                       direct solve (tests)
+interp_tet             reference interpolation_tet values + donors parsed
+                       from its own "detected element" print; brute force
+=====================  =====================================================
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle not built: {path} (run make -C oracle)")
+        L = ctypes.CDLL(path)
+        L.orc_genalpha.argtypes = [ctypes.c_double, _dp]
+        L.orc_gauss_volume.argtypes = [ctypes.c_int, _dp, _dp]
+        L.orc_gauss_volume.restype = ctypes.c_int
+        for f in (L.orc_linhexshl, L.orc_lintetshl, L.orc_linprishl):
+            f.argtypes = [_dp, _dp, _dp]
+        L.orc_inverse3x3.argtypes = [_dp, _dp, _dp]
+        L.orc_eval_shape_fem.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp,
+                                         _dp, _dp, _dp, _dp]
+        L.orc_eval_shape_fem.restype = ctypes.c_int
+        L.orc_stab_ns.argtypes = [_dp, ctypes.c_double, _dp, ctypes.c_double,
+                                  ctypes.c_double, _dp, _dp, _dp]
+        L.orc_sparsity.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   _ip, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_sparsity.restype = ctypes.c_long
+        L.orc_assemble_ns.argtypes = (
+            [ctypes.c_int] * 4 + [_ip, _ip, _ip, ctypes.c_int, ctypes.c_int, _dp,
+                                  _ip, ctypes.c_int] + [_dp] * 7 +
+            [_ip, _ip] + [_dp] * 6 + [_dp])
+        L.orc_assemble_ns.restype = ctypes.c_int
+        L.orc_element_ns.argtypes = ([ctypes.c_int, ctypes.c_int] + [_dp] * 8 +
+                                     [ctypes.c_int] + [_dp] * 7)
+        L.orc_element_ns.restype = ctypes.c_int
+        L.orc_bclhs_3d.argtypes = [ctypes.c_int, _ip] + [_dp] * 6
+        L.orc_spmv_up.argtypes = [ctypes.c_int, _ip, _ip] + [_dp] * 8
+        L.orc_gmres_up.argtypes = ([ctypes.c_int, _ip, _ip] + [_dp] * 8 +
+                                   [ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                    _dp, _dp])
+        L.orc_gmres_up.restype = ctypes.c_int
+        L.orc_interp_tet.argtypes = ([ctypes.c_int] * 3 + [_ip, _ip, ctypes.c_int] +
+                                     [_dp] * 3 + [ctypes.c_int, _dp, _dp, _ip, _dp])
+        _LIB = L
+    return _LIB
+
+
+def _c(a, dt=np.float64):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def _soa(a):
+    """(N,3) node field -> column-major flat (3*N) as the reference stores it."""
+    return _c(np.asarray(a, dtype=np.float64).T).ravel()
+
+
+def genalpha(rhoinf):
+    out = np.zeros(4)
+    lib().orc_genalpha(float(rhoinf), out)
+    return out
+
+
+def gauss_volume(ngp):
+    gp = np.zeros(3 * ngp)
+    gw = np.zeros(ngp)
+    if lib().orc_gauss_volume(ngp, gp, gw) != 0:
+        raise ValueError(ngp)
+    return gp.reshape(3, ngp).T.copy(), gw
+
+
+def shape_ref(nshl, pt):
+    f = {8: lib().orc_linhexshl, 4: lib().orc_lintetshl, 6: lib().orc_linprishl}[nshl]
+    shl = np.zeros(nshl)
+    sg = np.zeros(3 * nshl)
+    f(_c(pt), shl, sg)
+    return shl, sg.reshape(3, nshl).T.copy()
+
+
+def inverse3x3(A):
+    Ai = np.zeros(9)
+    dj = np.zeros(1)
+    lib().orc_inverse3x3(_c(np.asarray(A).T).ravel(), Ai, dj)
+    return Ai.reshape(3, 3).T.copy(), float(dj[0])
+
+
+def eval_shape_fem(nshl, pt, xl, dl):
+    shlu = np.zeros(nshl)
+    sg = np.zeros(3 * nshl)
+    dxidx, G, Gi, dj = np.zeros(9), np.zeros(9), np.zeros(9), np.zeros(1)
+    lib().orc_eval_shape_fem(nshl, _c(pt), _soa(xl), _soa(dl), shlu, sg, dxidx,
+                             G, Gi, dj)
+    return (shlu, sg.reshape(3, nshl).T.copy(), dxidx.reshape(3, 3).T.copy(),
+            G.reshape(3, 3).T.copy(), Gi.reshape(3, 3).T.copy(), float(dj[0]))
+
+
+def sparsity(nnode, ien):
+    """genSparStruc -> (col[N+1], row[icnt]) 0-based."""
+    ien = np.asarray(ien, dtype=np.int32)
+    nelem, nshl = ien.shape
+    ienf = _c(ien.T, np.int32).ravel()
+    icnt = lib().orc_sparsity(nnode, nelem, nshl, ienf, None, None)
+    if icnt < 0:
+        raise RuntimeError("sparsity overflow")
+    col = np.zeros(nnode + 1, np.int32)
+    row = np.zeros(icnt, np.int32)
+    lib().orc_sparsity(nnode, nelem, nshl, ienf, col.ctypes.data, row.ctypes.data)
+    return col, row
+
+
+def params_array(p):
+    """p: dict with almi alfi gami beti delt rho mu kdc grav(3)."""
+    g = p.get("grav", (0.0, 0.0, -9.81))
+    return np.array([p["almi"], p["alfi"], p["gami"], p["beti"], p["delt"],
+                     p["rho"], p["mu"], p.get("kdc", 0.0), g[0], g[1], g[2]],
+                    dtype=np.float64)
+
+
+def assemble_ns(mesh, ibc, state, prm, col, row, tensor_flag=6,
+                subdomain_flag=1, elm_id_set=(1, 2), ngauss=None):
+    """IntElmAss_NS_overset on zeroed outputs.
+
+    state: dict of alpha-stage node fields dg, ug, ugm, acg, acgm (N,3), pg (N,).
+    ibc: (N, 4) int.  Returns dict with k11 (icnt,9), g (icnt,3), d1 (icnt,3),
+    m (icnt,), rhsu (N,3), rhsp (N,), cfl, nneg.
+    """
+    N, E, nshl = mesh.nnode, mesh.nelem, mesh.nshl
+    if ngauss is None:
+        ngauss = nshl
+    icnt = row.size
+    k11 = np.zeros(icnt * 9)
+    g = np.zeros(icnt * 3)
+    d1 = np.zeros(icnt * 3)
+    m = np.zeros(icnt)
+    rhsu = np.zeros(3 * N)
+    rhsp = np.zeros(N)
+    cfl = np.zeros(1)
+    ibcf = _c(np.asarray(ibc)[:, :4].T, np.int32).ravel()
+    nneg = lib().orc_assemble_ns(
+        N, E, nshl, ngauss, _c(mesh.ien.T, np.int32).ravel(),
+        _c(mesh.elm_id, np.int32), _c(elm_id_set, np.int32), subdomain_flag,
+        tensor_flag, _soa(mesh.xg), ibcf, 4, _soa(state["dg"]),
+        _soa(state["ug"]), _soa(state["ugm"]), _soa(state["acg"]),
+        _soa(state["acgm"]), _c(state["pg"]), _c(prm), _c(col, np.int32),
+        _c(row, np.int32), k11, g, d1, m, rhsu, rhsp, cfl)
+    if nneg < 0:
+        raise RuntimeError("oracle assembly failed")
+    return dict(k11=k11.reshape(icnt, 9), g=g.reshape(icnt, 3),
+                d1=d1.reshape(icnt, 3), m=m, rhsu=rhsu.reshape(3, N).T.copy(),
+                rhsp=rhsp, cfl=float(cfl[0]), nneg=int(nneg))
+
+
+def spmv_up(col, row, A, xu, xp):
+    N = col.size - 1
+    yu = np.zeros(3 * N)
+    yp = np.zeros(N)
+    lib().orc_spmv_up(N, _c(col, np.int32), _c(row, np.int32),
+                      _c(A["k11"]).ravel(), _c(A["g"]).ravel(), _c(A["d1"]).ravel(),
+                      _c(A["m"]), _soa(xu), _c(xp), yu, yp)
+    return yu.reshape(3, N).T.copy(), yp
+
+
+def gmres_up(col, row, A, rhsu, rhsp, tol, kspace, kspace_min=0):
+    N = col.size - 1
+    solu = np.zeros(3 * N)
+    solp = np.zeros(N)
+    res = np.zeros(1)
+    r0 = np.zeros(1)
+    its = lib().orc_gmres_up(N, _c(col, np.int32), _c(row, np.int32),
+                             _c(A["k11"]).ravel(), _c(A["g"]).ravel(),
+                             _c(A["d1"]).ravel(), _c(A["m"]), _soa(rhsu),
+                             _c(rhsp), solu, solp, float(tol), int(kspace),
+                             int(kspace_min), res, r0)
+    return dict(solu=solu.reshape(3, N).T.copy(), solp=solp, iters=int(its),
+                resid=float(res[0]), rnorm0=float(r0[0]))
+
+
+def interp_tet(mesh, src_id, dg, ug, xr, dr):
+    N, E = mesh.nnode, mesh.nelem
+    nr = xr.shape[0]
+    donor = np.zeros(nr, np.int32)
+    val = np.zeros(3 * nr)
+    lib().orc_interp_tet(N, E, mesh.nshl, _c(mesh.ien.T, np.int32).ravel(),
+                         _c(mesh.elm_id, np.int32), int(src_id), _soa(mesh.xg),
+                         _soa(dg), _soa(ug), nr, _soa(xr), _soa(dr), donor, val)
+    return donor, val.reshape(3, nr).T.copy()

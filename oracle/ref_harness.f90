@@ -1,0 +1,207 @@
+!======================================================================
+! Harness around the reference Fortran routines -- TEST INFRASTRUCTURE.
+!
+! Linked (by oracle/Makefile, target `ref`) against the reference's own
+! object files compiled from their sources where they lie; nothing of the
+! reference is copied.  It replaces driver.f90's main program: it fills the
+! reference modules' state from a binary input file and calls the routines
+! on the hot path directly, writing their outputs to a binary file.  It is
+! used only to write golden vectors (tests/golden/make_golden.py) and, where
+! built, as the CPU baseline.
+!
+! usage: ofd_refharness <in.bin> <out.bin>
+!
+! modes (first int32 of in.bin):
+!   1  shape:  linhexshl / lintetshl / linprishl / genGPandGWtetpri /
+!              get_inverse_3x3 / eval_shape_fem at given points
+!   2  ns:     genSparStruc, IntElmAss_NS_overset + BCLhs_3D + FillSparseMat_3D,
+!              SparseProdUP_3D on a given vector, SparseGMRES_up on the
+!              assembled system
+!   3  interp: interpolation_tet for receiver points
+!======================================================================
+program ofd_refharness
+  use aAdjKeep
+  use commonvars
+  use commonpars
+  use mpi
+  implicit none
+
+  character(len=512) :: fin, fout
+  integer :: mode, ierr
+
+  call get_command_argument(1, fin)
+  call get_command_argument(2, fout)
+
+  call MPI_INIT(ierr)
+  call MPI_COMM_SIZE(MPI_COMM_WORLD, numnodes, ierr)
+  call MPI_COMM_RANK(MPI_COMM_WORLD, myid, ierr)
+  ismaster = (myid == 0)
+  iga = .false.
+  NSD = 3
+  NBOUND = 0
+
+  open (10, file=trim(fin), access='stream', form='unformatted', status='old')
+  open (11, file=trim(fout), access='stream', form='unformatted', &
+        status='replace')
+  read (10) mode
+  select case (mode)
+  case (1)
+    call run_shape()
+  case (2)
+    call run_ns()
+  case (3)
+    call run_interp()
+  case default
+    write (*, *) 'unknown mode', mode
+  end select
+  close (10)
+  close (11)
+  call MPI_FINALIZE(ierr)
+
+contains
+
+  subroutine run_shape()
+    integer :: npt, ip, q
+    real(8), allocatable :: pts(:, :)
+    real(8) :: shl(8), sg(8, 3), gp8(8, 3), gw8(8), gp4(4, 3), gw4(4), &
+               gp6(6, 3), gw6(6), gp1(1, 3), gw1(1)
+    real(8) :: A(3, 3), Ai(3, 3), dj
+    real(8) :: xl(6, 3), dl(6, 3), shlu(6), shgradgu(6, 3), dxidx(3, 3), &
+               Gij(3, 3), Ginv(3, 3)
+    integer :: ntet, npri
+    real(8), allocatable :: xt(:, :, :), xp(:, :, :)
+    real(8) :: xl4(4, 3), dl4(4, 3), shlu4(4), shgradgu4(4, 3)
+
+    read (10) npt
+    allocate (pts(npt, 3))
+    read (10) pts
+    ! reference-element shape functions at each point
+    do ip = 1, npt
+      call linhexshl(pts(ip, :), 8, 3, shl, sg)
+      write (11) shl(1:8), sg(1:8, :)
+    end do
+    do ip = 1, npt
+      call lintetshl(pts(ip, :), 4, 3, shlu4, shgradgu4)
+      write (11) shlu4, shgradgu4
+    end do
+    do ip = 1, npt
+      call linprishl(pts(ip, :), 6, 3, shlu, shgradgu)
+      write (11) shlu, shgradgu
+    end do
+    ! quadrature rules
+    call genGPandGWtetpri(gp1, gw1, 1)
+    call genGPandGWtetpri(gp4, gw4, 4)
+    call genGPandGWtetpri(gp6, gw6, 6)
+    call genGPandGWtetpri(gp8, gw8, 8)
+    write (11) gp1, gw1, gp4, gw4, gp6, gw6, gp8, gw8
+    ! 3x3 inverse of each point-derived matrix
+    do ip = 1, npt
+      A(:, 1) = pts(ip, :) + (/2.0d0, 0.1d0, -0.3d0/)
+      A(:, 2) = (/0.2d0, 1.5d0, 0.4d0/) - pts(ip, :)
+      A(:, 3) = (/-0.1d0, 0.3d0, 1.7d0/) + 0.5d0*pts(ip, (/2, 3, 1/))
+      call get_inverse_3x3(A, Ai, dj)
+      write (11) Ai, dj
+    end do
+    ! eval_shape_fem on given tets, then given prisms, at each quadrature pt
+    read (10) ntet
+    allocate (xt(ntet, 4, 3))
+    read (10) xt
+    do ip = 1, ntet
+      xl4 = xt(ip, :, :)
+      dl4 = 0.01d0*xt(ip, :, (/3, 1, 2/))
+      do q = 1, 4
+        call eval_shape_fem(4, gp4(q, :), xl4, dl4, shlu4, shgradgu4, dxidx, &
+                            Gij, Ginv)
+        write (11) shlu4, shgradgu4, dxidx, Gij, Ginv, DetJ
+      end do
+    end do
+    read (10) npri
+    allocate (xp(npri, 6, 3))
+    read (10) xp
+    do ip = 1, npri
+      xl = xp(ip, :, :)
+      dl = 0.01d0*xp(ip, :, (/3, 1, 2/))
+      do q = 1, 6
+        call eval_shape_fem(6, gp6(q, :), xl, dl, shlu, shgradgu, dxidx, &
+                            Gij, Ginv)
+        write (11) shlu, shgradgu, dxidx, Gij, Ginv, DetJ
+      end do
+    end do
+  end subroutine run_shape
+
+  subroutine run_ns()
+    integer :: nshl, ngauss, tflag, sflag, kspace, kmin, i
+    real(8) :: prm(11), tol
+    real(8), allocatable :: dA(:, :), uA(:, :), umA(:, :), aA(:, :), &
+                            amA(:, :), pA(:), xu(:, :), xp(:), yu(:, :), &
+                            yp(:), solu(:, :), solp(:)
+    integer, allocatable :: iper(:), dflag(:), pflag(:)
+
+    read (10) NNODE, NELEM, nshl, ngauss, tflag, sflag, ELM_ID_SET(1:2)
+    maxNSHL = nshl
+    allocate (IEN(NELEM, nshl), ELM_ID(NELEM), ELMNSHL(NELEM), &
+              ELMNGAUSS(NELEM), IBC(NNODE, 8), xg(NNODE, 3), wg(NNODE))
+    read (10) IEN, ELM_ID, IBC
+    read (10) xg
+    wg = 1.0d0
+    ELMNSHL = nshl
+    ELMNGAUSS = ngauss
+    allocate (dA(NNODE, 3), uA(NNODE, 3), umA(NNODE, 3), aA(NNODE, 3), &
+              amA(NNODE, 3), pA(NNODE))
+    read (10) dA, uA, umA, aA, amA, pA, prm
+    almi = prm(1); alfi = prm(2); gami = prm(3); beti = prm(4)
+    Delt = prm(5); rhoa = prm(6); mua = prm(7); NS_kdc_a = prm(8)
+    gravvec = prm(9:11)
+    read (10) tol, kspace, kmin
+    allocate (xu(NNODE, 3), xp(NNODE), yu(NNODE, 3), yp(NNODE))
+    read (10) xu, xp
+
+    call genSparStruc()
+    allocate (LHSK11(9, icnt), LHSG(3, icnt), LHSD1(3, icnt), LHSM(icnt), &
+              RHSGu(NNODE, 3), RHSGp(NNODE))
+    LHSK11 = 0.0d0; LHSG = 0.0d0; LHSD1 = 0.0d0; LHSM = 0.0d0
+    RHSGu = 0.0d0; RHSGp = 0.0d0
+    call IntElmAss_NS_overset(dA, uA, umA, aA, amA, pA, tflag, sflag)
+
+    write (11) icnt
+    write (11) col, row
+    write (11) LHSK11, LHSG, LHSD1, LHSM, RHSGu, RHSGp
+
+    allocate (solu(NNODE, 3), solp(NNODE), iper(NNODE), dflag(NNODE), &
+              pflag(NNODE))
+    iper = (/(i, i=1, NNODE)/)
+    dflag = 0; pflag = 0
+
+    call SparseProdUP_3D(col, row, LHSK11, LHSG, LHSD1, LHSM, xu, xp, yu, yp, &
+                         dflag, pflag, NNODE, nshl, icnt, NSD)
+    write (11) yu, yp
+
+    solu = 0.0d0; solp = 0.0d0
+    if (kspace > 0) then
+      call SparseGMRES_up(col, row, IBC, iper, dflag, pflag, RHSGu, RHSGp, &
+                          solu, solp, LHSK11, LHSG, LHSD1, LHSM, icnt, tol, &
+                          kspace, kmin, NNODE, nshl, NSD)
+    end if
+    write (11) solu, solp
+  end subroutine run_ns
+
+  subroutine run_interp()
+    integer :: nshl, nrecv, srcid
+    read (10) NNODE, NELEM, nshl, srcid, nrecv
+    maxNSHL = nshl
+    allocate (IEN(NELEM, nshl), ELM_ID(NELEM), ELMNSHL(NELEM), xg(NNODE, 3), &
+              dg(NNODE, 3), ug(NNODE, 3))
+    read (10) IEN, ELM_ID
+    read (10) xg, dg, ug
+    ELMNSHL = nshl
+    SRC_ID_SET(1) = srcid
+    bmesh(1)%NNODE = nrecv
+    bmesh(1)%NSD = 3
+    allocate (bmesh(1)%xg(nrecv, 3), bmesh(1)%dg(nrecv, 3), &
+              bmesh(1)%ug(nrecv, 3))
+    read (10) bmesh(1)%xg, bmesh(1)%dg
+    call interpolation_tet(1)
+    write (11) bmesh(1)%ug
+  end subroutine run_interp
+
+end program ofd_refharness

@@ -1,0 +1,94 @@
+"""The CPU oracle against the golden vectors written by the reference Fortran
+(tests/golden/make_golden.py).  The element kernels, BC projection, scatter,
+pattern, SpMV and GMRES are compared on tet meshes, where the reference's
+own dispatch runs unmodified; hex shape functions and the 8-point rule are
+compared point by point against linhexshl / genGPandGWtetpri."""
+import numpy as np
+import pytest
+
+from conftest import golden
+from oversetdevfort_amd.mesh import Mesh
+
+
+def _cat(*a):
+    return np.concatenate([np.ravel(x, order="F") for x in a])
+
+
+def test_shape_functions_bit_exact(oracle_mod):
+    O = oracle_mod
+    z = golden("shape")
+    for i, pt in enumerate(z["pts"]):
+        for nshl, key in ((8, "hex"), (4, "tet"), (6, "pri")):
+            shl, sg = O.shape_ref(nshl, pt)
+            assert np.array_equal(_cat(shl, sg), z[key][i]), (nshl, i)
+
+
+def test_quadrature_bit_exact(oracle_mod):
+    z = golden("shape")
+    for n in (1, 4, 6, 8):
+        gp, gw = oracle_mod.gauss_volume(n)
+        assert np.array_equal(gp, z[f"gp{n}"]) and np.array_equal(gw, z[f"gw{n}"])
+
+
+def test_inverse3x3_bit_exact(oracle_mod):
+    z = golden("shape")
+    for i, pt in enumerate(z["pts"]):
+        A = np.zeros((3, 3))
+        A[:, 0] = pt + [2.0, 0.1, -0.3]
+        A[:, 1] = np.array([0.2, 1.5, 0.4]) - pt
+        A[:, 2] = np.array([-0.1, 0.3, 1.7]) + 0.5 * pt[[1, 2, 0]]
+        Ai, dj = oracle_mod.inverse3x3(A)
+        assert np.array_equal(_cat(Ai, [dj]), z["inv"][i])
+
+
+@pytest.mark.parametrize("nshl,key,xkey", [(4, "ev4", "xt"), (6, "ev6", "xp")])
+def test_eval_shape_fem_bit_exact(oracle_mod, nshl, key, xkey):
+    O = oracle_mod
+    z = golden("shape")
+    gp, _ = O.gauss_volume(nshl)
+    r = 0
+    for xl in z[xkey]:
+        dl = 0.01 * xl[:, [2, 0, 1]]
+        for q in range(nshl):
+            s = O.eval_shape_fem(nshl, gp[q], xl, dl)
+            assert np.array_equal(_cat(*s[:5], [s[5]]), z[key][r]), (xkey, r)
+            r += 1
+
+
+def _mesh(z):
+    return Mesh(xg=z["xg"], ien=z["ien"], elm_id=z["elm_id"])
+
+
+@pytest.mark.parametrize("case", ["ns_tet_a", "ns_tet_b", "ns_tet_vec"])
+def test_ns_assembly_spmv_gmres_vs_reference(oracle_mod, case):
+    O = oracle_mod
+    z = golden(case)
+    m = _mesh(z)
+    col, row = O.sparsity(m.nnode, m.ien)
+    assert np.array_equal(col, z["col"]) and np.array_equal(row, z["row"])
+    st = {k[3:]: z[k] for k in z.files if k.startswith("st_")}
+    A = O.assemble_ns(m, z["ibc"], st, z["prm"], col, row, int(z["tensor_flag"]),
+                      int(z["subdomain_flag"]))
+    for k in ("k11", "g", "d1", "m", "rhsu", "rhsp"):
+        ref = z[k]
+        scale = max(np.abs(ref).max(), 1e-300)
+        assert np.abs(A[k] - ref).max() <= 1e-13 * scale, k
+    yu, yp = O.spmv_up(col, row, A, z["xu"], z["xp"])
+    sc = max(np.abs(z["yu"]).max(), np.abs(z["yp"]).max())
+    assert np.abs(yu - z["yu"]).max() <= 1e-13 * sc
+    assert np.abs(yp - z["yp"]).max() <= 1e-13 * sc
+    if int(z["kspace"]) > 0:
+        G = O.gmres_up(col, row, A, A["rhsu"], A["rhsp"], float(z["tol"]),
+                       int(z["kspace"]), int(z["kmin"]))
+        sc = max(np.abs(z["solu"]).max(), np.abs(z["solp"]).max())
+        assert np.abs(G["solu"] - z["solu"]).max() <= 1e-10 * sc
+        assert np.abs(G["solp"] - z["solp"]).max() <= 1e-10 * sc
+
+
+def test_interp_tet_donors_and_values(oracle_mod):
+    z = golden("interp_tet")
+    donor, val = oracle_mod.interp_tet(_mesh(z), int(z["src_id"]), z["dg"], z["ug"],
+                                       z["xr"], z["dr"])
+    assert np.array_equal(donor, z["donor"])
+    assert (donor >= 0).sum() > 30
+    assert np.abs(val - z["val"]).max() <= 1e-14 * np.abs(z["val"]).max()
