@@ -1,0 +1,239 @@
+#!/usr/bin/env python3
+"""Benchmark: Newton-step wall time and assembly Melem/s on one MI355X.
+
+Workload: the channel configuration -- [0,4]x[0,1]^2 on 64x32x32 trilinear
+hexes (65,536 elements, 70,785 nodes, 283,140 dof), Re = 1000, dt = 0.005,
+Poiseuille inflow + seeded divergence-free perturbation (synthetic data).
+A step is one time step with one Newton iteration: predictor, Dirichlet
+values, assembly (VEC+MAT), linear solve to a 1e-8 relative FP64 residual
+|b - A x| <= 1e-8 |b|, Newton update, residual re-assembly (VEC) and norm,
+old <- new.  It runs once with the FP64 solver and once with the mixed one
+(FP32-stored matrix and block-Jacobi preconditioner, FP64 vectors,
+Gram-Schmidt, Givens and outer residual).  The headline value is the mixed
+Newton step in elements per second; the FP64 figures ride along.
+
+Prints progress on stderr and ONE JSON line on stdout.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("Newton-step wall time & Melem/s assembly, FP32-mixed vs FP64 at 1e-8 "
+          "residual")
+F_NS = 1.07e5          # flop per hex element, one VEC+MAT NS assembly
+PEAK_HBM = 8.0e12      # B/s
+PEAK_FP64 = 78.6e12    # flop/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def spmv_bytes(Z, N, s_a, s_x=8, s_y=8, b=4, S=16):
+    return Z * (S * s_a + 4) + 4 * (N + 1) + N * b * (s_x + s_y)
+
+
+def run_steps(ctx, S, L, prob, precision, steps, warmup):
+    """Reset to the initial state, run warmup + timed time steps."""
+    ctx.set_state(prob.fields)
+    newton_ms, step_ms, iters, cycles, rel, res = [], [], [], [], [], []
+    for k in range(warmup + steps):
+        t0 = time.perf_counter()
+        S.begin_step(ctx)
+        t1 = time.perf_counter()
+        info = S.newton_step(ctx, rtol=1e-8, precision=precision,
+                             ortho=L.OFD_ORTHO_CGS2, kspace=30, max_cycles=60)
+        t2 = time.perf_counter()
+        ctx.advance()
+        t3 = time.perf_counter()
+        tag = "warmup" if k < warmup else "timed"
+        log(f"  {tag} step {k}: newton {1e3 * (t2 - t1):.2f} ms, "
+            f"iters {info['iters']} cycles {info['cycles']} relres {info['relres']:.2e} "
+            f"|Ru| {info['res'][0]:.3e}")
+        if k >= warmup:
+            newton_ms.append(1e3 * (t2 - t1))
+            step_ms.append(1e3 * (t3 - t0))
+            iters.append(info["iters"])
+            cycles.append(info["cycles"])
+            rel.append(info["relres"])
+            res.append(info["res"])
+    return dict(newton_ms=sum(newton_ms) / len(newton_ms),
+                step_ms=sum(step_ms) / len(step_ms),
+                total_ms=sum(step_ms), gmres_iters=iters, cycles=cycles,
+                relres=max(rel), res_u=[r[0] for r in res])
+
+
+def cpu_baseline(prob, budget_s=25.0):
+    """One FP64 Newton step of the same workload on the host, single thread,
+    through the C oracle (loop-for-loop restatement of the reference):
+    assembly VEC+MAT, restarted MGS GMRES(30) to |b-Ax| <= 1e-8 |b|, update,
+    VEC re-assembly and norm."""
+    import numpy as np
+
+    from oracle import oracle as O
+    m, p = prob.mesh, prob.params
+    N = m.nnode
+    st = {k: v.copy() for k, v in prob.fields.items()}
+    # predictor + Dirichlet values, as on the GPU
+    st["ug"] = st["ugold"].copy()
+    st["acg"] = (p["gami"] - 1.0) / p["gami"] * st["acgold"]
+    st["pg"] = st["pgold"].copy()
+    from oversetdevfort_amd import configs as C
+    st["ug"], st["acg"] = C.apply_dirichlet(st["ug"], st["acg"], st["ugold"],
+                                            st["acgold"], prob)
+    col, row = O.sparsity(N, m.ien)
+    prm = O.params_array(p)
+
+    def alpha(s):
+        return dict(acg=s["acgold"] + p["almi"] * (s["acg"] - s["acgold"]),
+                    acgm=s["acgmold"] + p["almi"] * (s["acgm"] - s["acgmold"]),
+                    ug=s["ugold"] + p["alfi"] * (s["ug"] - s["ugold"]),
+                    ugm=s["ugmold"] + p["alfi"] * (s["ugm"] - s["ugmold"]),
+                    dg=s["dgold"] + p["alfi"] * (s["dg"] - s["dgold"]), pg=s["pg"])
+
+    t0 = time.perf_counter()
+    R = O.assemble_ns(m, prob.ibc, alpha(st), prm, col, row, 6, 1)
+    t1 = time.perf_counter()
+    bu, bp = R["rhsu"], R["rhsp"]
+    bn = np.sqrt((bu ** 2).sum() + (bp ** 2).sum())
+    xu = np.zeros((N, 3))
+    xp = np.zeros(N)
+    ru, rp = bu, bp
+    its = 0
+    rel = 1.0
+    for cyc in range(60):
+        g = O.gmres_up(col, row, R, ru, rp, 1e-2, 30, 0)
+        its += g["iters"]
+        xu += g["solu"]
+        xp += g["solp"]
+        yu, yp = O.spmv_up(col, row, R, xu, xp)
+        ru, rp = bu - yu, bp - yp
+        rel = np.sqrt((ru ** 2).sum() + (rp ** 2).sum()) / bn
+        if rel <= 1e-8 or time.perf_counter() - t0 > budget_s:
+            break
+    t2 = time.perf_counter()
+    st["acg"] = st["acg"] + xu
+    st["ug"] = st["ug"] + p["gami"] * p["delt"] * xu
+    st["pg"] = st["pg"] + p["alfi"] * p["gami"] * p["delt"] * xp
+    R2 = O.assemble_ns(m, prob.ibc, alpha(st), prm, col, row, 2, 1)
+    np.sqrt((R2["rhsu"] ** 2).sum())
+    t3 = time.perf_counter()
+    return dict(newton_s=t3 - t0, assemble_s=t1 - t0, solve_s=t2 - t1,
+                gmres_iters=its, relres=float(rel), converged=bool(rel <= 1e-8))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--dims", type=str, default="64,32,32")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the host baseline")
+    ap.add_argument("--spmv-reps", type=int, default=50)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs one GPU as a single rank; --gpus {args.gpus} "
+              "is not supported (the multi-GPU sharding is design-only)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from oversetdevfort_amd import _lib as L
+    from oversetdevfort_amd import configs as C
+    from oversetdevfort_amd import solver as S
+
+    nx, ny, nz = (int(v) for v in args.dims.split(","))
+    prob = C.channel(nx, ny, nz)
+    E, N = prob.mesh.nelem, prob.mesh.nnode
+    log(f"bench: channel {nx}x{ny}x{nz}: {E} elements, {N} nodes")
+    t0 = time.perf_counter()
+    ctx = S.setup(prob, device=0)
+    Z = ctx.icnt
+    log(f"bench: setup {time.perf_counter() - t0:.2f} s, Z = {Z} blocks, "
+        f"device bytes {ctx.device_bytes() / 2**20:.1f} MiB")
+
+    res64 = run_steps(ctx, S, L, prob, L.OFD_FP64, args.steps, args.warmup)
+    log(f"bench: FP64 newton {res64['newton_ms']:.2f} ms")
+    res32 = run_steps(ctx, S, L, prob, L.OFD_FP32, args.steps, args.warmup)
+    log(f"bench: mixed newton {res32['newton_ms']:.2f} ms")
+
+    # kernel-level figures on the context stream (HIP events)
+    ctx.set_state(prob.fields)
+    S.begin_step(ctx)
+    asm_ms = ctx.time_assemble(6, reps=5)
+    asm_vec_ms = ctx.time_assemble(2, reps=5)
+    ctx.assembleNavSto(6, 1)
+    spmv64_ms = ctx.time_spmv(L.OFD_FP64, reps=args.spmv_reps)
+    spmv32_ms = ctx.time_spmv(L.OFD_FP32, reps=args.spmv_reps)
+    b64 = spmv_bytes(Z, N, 8)
+    b32 = spmv_bytes(Z, N, 4)
+    bw64 = b64 / (spmv64_ms * 1e-3)
+    bw32 = b32 / (spmv32_ms * 1e-3)
+    asm_tf = E * F_NS / (asm_ms * 1e-3)
+    log(f"bench: assembly VEC+MAT {asm_ms:.3f} ms ({E / asm_ms / 1e3:.1f} Melem/s, "
+        f"{asm_tf / 1e12:.2f} TFLOP/s); VEC {asm_vec_ms:.3f} ms")
+    log(f"bench: SpMV FP64 {spmv64_ms * 1e3:.1f} us {bw64 / 1e9:.0f} GB/s; "
+        f"FP32-matrix {spmv32_ms * 1e3:.1f} us {bw32 / 1e9:.0f} GB/s")
+    ctx.close()
+
+    cpu = None
+    if not args.no_cpu:
+        log("bench: host baseline (C oracle, 1 thread) ...")
+        c = cpu_baseline(prob)
+        log(f"bench: host Newton step {c['newton_s']:.2f} s (assembly {c['assemble_s']:.2f} s, "
+            f"solve {c['solve_s']:.2f} s, {c['gmres_iters']} its, relres {c['relres']:.1e})")
+        cpu = {"value": E / c["newton_s"] / 1e6, "unit": "Melem/s (Newton step)",
+               "cores": 1, "kind": "port",
+               "sample": (f"one FP64 Newton step of the same {nx}x{ny}x{nz} channel "
+                          f"workload through the C oracle (restatement of "
+                          f"IntElmAss_NS_overset/SparseProdUP_3D/SparseGMRES_up), "
+                          f"restarted GMRES(30) to 1e-8; newton {c['newton_s']:.2f} s, "
+                          f"assembly {c['assemble_s']:.2f} s, solve {c['solve_s']:.2f} s, "
+                          f"{c['gmres_iters']} iterations, relres {c['relres']:.1e}, "
+                          f"host {platform.processor() or platform.machine()}"),
+               "newton_ms": 1e3 * c["newton_s"], "assembly_ms": 1e3 * c["assemble_s"]}
+
+    out = {
+        "metric": METRIC,
+        "value": E / (res32["newton_ms"] * 1e-3) / 1e6,
+        "unit": "Melem/s (mixed Newton step)",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res32["newton_ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-matrix/f64-vector (mixed); f64 reported alongside",
+        "data": "synthetic",
+        "config": {"workload": (f"channel [0,4]x[0,1]^2, {nx}x{ny}x{nz} hex, Re=1000, "
+                                "dt=0.005, 1 Newton step/time step, rtol 1e-8"),
+                   "elements": E, "nodes": N, "dof": 4 * N, "nnz_blocks": Z},
+        "newton_ms": {"mixed": res32["newton_ms"], "fp64": res64["newton_ms"]},
+        "speedup_mixed_vs_fp64": res64["newton_ms"] / res32["newton_ms"],
+        "gmres_iters": {"mixed": res32["gmres_iters"], "fp64": res64["gmres_iters"]},
+        "relres": {"mixed": res32["relres"], "fp64": res64["relres"]},
+        "assembly": {"vecmat_ms": asm_ms, "vec_ms": asm_vec_ms,
+                     "melem_per_s": E / asm_ms / 1e3,
+                     "fp64_tflops": asm_tf / 1e12, "frac_fp64_peak": asm_tf / PEAK_FP64},
+        "spmv": {"fp64_us": spmv64_ms * 1e3, "fp64_GBps": bw64 / 1e9,
+                 "fp32_us": spmv32_ms * 1e3, "fp32_GBps": bw32 / 1e9,
+                 "fp64_bytes": b64, "fp32_bytes": b32},
+        "roofline": {"bound": "hbm", "kernel": "k_spmv<double> (block-CSR SpMV, FP64)",
+                     "achieved": bw64 / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                     "frac": bw64 / PEAK_HBM, "traffic": None,
+                     "algorithmic_bytes": b64},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,165 @@
+/*
+ * ofd.h -- C ABI of the MI355X (gfx950) implementation of the OversetDevFort
+ * hot path: VMS/SUPG element assembly, CSR scatter, block SpMV, block-Jacobi
+ * GMRES and the overset fringe interpolation.
+ *
+ * Plain pointers and sizes only.  Every entry returns an int status
+ * (OFD_OK = 0); ofd_last_error() gives the message of the last failure on
+ * the calling thread.  A context owns its device buffers and one HIP stream;
+ * calls on a context are serialised on that stream and are synchronous on
+ * return (host outputs are valid when the call returns).  A context is not
+ * thread-safe; use one per thread.
+ *
+ * Array conventions are the reference's (Fortran column-major, 1-based
+ * indices), so a Fortran caller passes its module arrays unchanged:
+ *   IEN(NELEM,NSHL), ELM_ID(NELEM), IBC(NNODE,ncol), xg/ug/...(NNODE,3),
+ *   pg(NNODE), col(NNODE+1), row(icnt), LHSK11(9,icnt), LHSG(3,icnt),
+ *   LHSD1(3,icnt), LHSM(icnt), RHSGu(NNODE,3), RHSGp(NNODE).
+ * Citations name the reference routine each entry point replaces
+ * (paths relative to the reference's src/).
+ */
+#ifndef OFD_H
+#define OFD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ofd_ctx ofd_ctx;
+
+enum {
+  OFD_OK = 0,
+  OFD_ERR_ARG = 1,        /* bad argument / size / call order */
+  OFD_ERR_HIP = 2,        /* HIP runtime error */
+  OFD_ERR_NEGDETJ = 3,    /* assembly met a negative Jacobian (warning in the reference) */
+  OFD_ERR_NONFINITE = 4,  /* non-finite norm */
+  OFD_ERR_NOCONV = 5,     /* solver stopped before the tolerance */
+  OFD_ERR_ALLOC = 6,
+  OFD_ERR_UNSUPPORTED = 7
+};
+
+/* assemble_tensor_flag bits: modules.f90:225-228 */
+enum { OFD_ASSEMBLE_VEC = 2, OFD_ASSEMBLE_MAT = 4 };
+
+/* precision of the stored matrix / preconditioner used by SpMV and GMRES */
+enum { OFD_FP64 = 0, OFD_FP32 = 1 };
+
+/* Gram-Schmidt variant of the Arnoldi loop */
+enum {
+  OFD_ORTHO_MGS = 0,  /* SparseGMRES_NS.f90:245-287 sequence, k+1 sweeps */
+  OFD_ORTHO_CGS2 = 1  /* classical GS with one re-orthogonalisation, 3 fused sweeps */
+};
+
+/* node fields held by a context, all (NNODE,3) except OFD_PG/OFD_PGOLD (NNODE) */
+enum {
+  OFD_XG = 0, OFD_DG, OFD_DGOLD, OFD_UG, OFD_UGOLD, OFD_UGM, OFD_UGMOLD,
+  OFD_ACG, OFD_ACGOLD, OFD_ACGM, OFD_ACGMOLD, OFD_PG, OFD_PGOLD,
+  OFD_SOLU, OFD_SOLP, OFD_NFIELDS
+};
+
+/* parameter vector of ofd_set_params */
+enum {
+  OFD_P_ALMI = 0, OFD_P_ALFI, OFD_P_GAMI, OFD_P_BETI, OFD_P_DELT, OFD_P_RHO,
+  OFD_P_MU, OFD_P_KDC, OFD_P_GX, OFD_P_GY, OFD_P_GZ, OFD_NPARAMS
+};
+
+const char *ofd_last_error(void);
+int ofd_version(void);
+
+/* context lifetime; device = HIP ordinal */
+int ofd_ctx_create(int device, ofd_ctx **out);
+int ofd_ctx_destroy(ofd_ctx *ctx);
+
+/* mesh: replaces input_mesh (input.f90:15-135) for the in-memory case */
+int ofd_upload_mesh(ofd_ctx *ctx, int nnode, int nelem, int nshl,
+                    const int32_t *ien, const int32_t *elm_id, const double *xg);
+
+/* genSparStruc (genSparStruc.f90:4-47): builds col/row, colours the
+ * elements and the (element,a,b) -> nonzero map used by the scatter */
+int ofd_build_pattern(ofd_ctx *ctx, int64_t *icnt);
+/* the pattern in the reference's 1-based form */
+int ofd_get_pattern(ofd_ctx *ctx, int32_t *col, int32_t *row);
+
+/* physics and time-integration parameters (getparam, param.f90:205-490):
+ * prm[OFD_NPARAMS] = almi alfi gami beti Delt rho mu NS_kdc gravvec(3) */
+int ofd_set_params(ofd_ctx *ctx, const double *prm);
+/* ELM_ID_SET(1:2) (param.f90:246-247) */
+int ofd_set_subdomains(ofd_ctx *ctx, int32_t id1, int32_t id2);
+/* Dirichlet flags IBC(NNODE,ncol), columns 1..4 = u,v,w,p (BCLhs_3D.f90) */
+int ofd_set_ibc(ofd_ctx *ctx, const int32_t *ibc, int ncol);
+
+/* Dirichlet velocity values BCugValu per node, (NNODE,3); used by
+ * ofd_apply_dirichlet on the nodes flagged in IBC columns 1..3 */
+int ofd_set_bcvals(ofd_ctx *ctx, const double *vals);
+/* setBCs_CFD value part (setBCs.f90:84-91): ug = value and the consistent
+ * acg = ((ug - ugold)/Delt + (gami-1) acgold)/gami on Dirichlet dofs */
+int ofd_apply_dirichlet(ofd_ctx *ctx);
+
+/* copy a node field between host and the context */
+int ofd_set_field(ofd_ctx *ctx, int field, const double *host);
+int ofd_get_field(ofd_ctx *ctx, int field, double *host);
+
+/* assembleNavSto(assemble_tensor_flag, subdomain_flag)
+ * (assembleNavSto.f90:4-72 -> IntElmAss_NS_overset :76-336, BCLhs_3D,
+ * FillSparseMat_3D): alpha stage from the context fields, zeroing, element
+ * loop, Dirichlet projection, scatter.  LHS/RHS stay on the device.
+ * cfl (may be NULL) receives the maximum Gauss-point CFL number. */
+int ofd_assemble_ns(ofd_ctx *ctx, int tensor_flag, int subdomain_flag,
+                    double *cfl);
+
+/* assembled system in the reference layout */
+int ofd_get_rhs(ofd_ctx *ctx, double *rhsu, double *rhsp);
+int ofd_get_lhs(ofd_ctx *ctx, double *lhsk11, double *lhsg, double *lhsd1,
+                double *lhsm);
+/* load a system given in the reference layout (for the drop-in wrappers of
+ * SparseProdUP_3D / SparseGMRES_up that receive host matrices) */
+int ofd_set_lhs(ofd_ctx *ctx, const double *lhsk11, const double *lhsg,
+                const double *lhsd1, const double *lhsm);
+int ofd_set_rhs(ofd_ctx *ctx, const double *rhsu, const double *rhsp);
+
+/* SparseProdUP_3D (SparseProd.f90:4-60) on the context matrix:
+ * y = A x with x, y in the reference layout (NNODE,3)+(NNODE) */
+int ofd_spmv_up(ofd_ctx *ctx, int precision, const double *xu,
+                const double *xp, double *yu, double *yp);
+
+/* SparseGMRES_up (SparseGMRES_NS.f90:4-369): one cycle of left
+ * block-Jacobi-preconditioned GMRES on the context system, <= kspace
+ * iterations, stop when |e_{k+1}| <= tol*|P^-1 b| and k >= kspace_min.
+ * sol (reference layout, may be NULL) receives the increment; it is also
+ * kept in the context as OFD_SOLU/OFD_SOLP. */
+int ofd_gmres_up(ofd_ctx *ctx, int precision, int ortho, double tol,
+                 int kspace, int kspace_min, double *solu, double *solp,
+                 int *iters, double *resid);
+
+/* Restarted GMRES with an FP64 outer residual (iterative refinement):
+ * repeat GMRES(kspace) cycles in `precision` on r = b - A64 x until
+ * |b - A64 x|_2 <= rtol*|b|_2 or max_cycles.  x stays in the context
+ * (OFD_SOLU/OFD_SOLP).  iters = total Arnoldi iterations. */
+int ofd_solve(ofd_ctx *ctx, int precision, int ortho, double rtol,
+              double inner_tol, int kspace, int max_cycles, int *iters,
+              int *cycles, double *relres);
+
+/* Newton update (solveOverset.f90:76-78): acg += du; ug += gami*Delt*du;
+ * pg += alfi*gami*Delt*dp with du, dp the context solution */
+int ofd_newton_update(ofd_ctx *ctx);
+/* predictor (driver.f90:83-85) and old <- new (driver.f90:98-103) */
+int ofd_predictor(ofd_ctx *ctx);
+int ofd_advance(ofd_ctx *ctx);
+/* |RHSGu|_2, |RHSGp|_2 over all nodes (calculate_residual_overset,
+ * solveOverset.f90:131-173, one subdomain) */
+int ofd_rhs_norms(ofd_ctx *ctx, double *norms2);
+
+/* kernel timing on the context stream (HIP events around `reps`
+ * back-to-back launches); ms = average duration of one launch */
+int ofd_time_spmv(ofd_ctx *ctx, int precision, int reps, double *ms);
+int ofd_time_assemble(ofd_ctx *ctx, int tensor_flag, int reps, double *ms);
+
+/* device memory footprint of the context in bytes */
+int ofd_ctx_bytes(ofd_ctx *ctx, int64_t *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

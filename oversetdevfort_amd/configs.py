@@ -1,0 +1,134 @@
+"""Synthetic problem set-ups: meshes, Dirichlet flags, initial state, parameters.
+
+Host-side set-up that the reference does through param.dat, the mesh files
+and setBCs_CFD; the GPU path consumes the resulting arrays.
+
+* ``cavity``  -- 3D lid-driven cavity, unit cube, n^3 trilinear hexes with
+  interior nodes jittered by U(-5%, 5%) of h; Re = 100 (rho 1, mu 0.01), lid
+  velocity (1,0,0) on z = 1, no-slip elsewhere, at rest; dt = 0.01.
+* ``channel`` -- [0,4]x[0,1]x[0,1] on nx*ny*nz hexes; Poiseuille inflow of
+  peak 1 at x = 0, no-slip walls y,z in {0,1}, natural outflow at x = 4;
+  initial field = Poiseuille profile + a seeded divergence-free sinusoidal
+  perturbation of amplitude 0.01; Re = 1000 (rho 1, mu 1e-3); dt = 0.005.
+
+Both use generalized-alpha with rho_inf = 0.5 (param.f90:469-480) and the
+reference's hard-coded gravity (0, 0, -9.81) (param.f90:253-257).
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+
+from . import mesh as M
+
+
+def genalpha(rhoinf):
+    """param.f90:469-480."""
+    if rhoinf < 0.0 or rhoinf > 1.0:
+        return dict(almi=1.0, alfi=1.0, gami=1.0, beti=1.0)
+    almi = (3.0 - rhoinf) / (1.0 + rhoinf) / 2.0
+    alfi = 1.0 / (1.0 + rhoinf)
+    gami = 0.5 + almi - alfi
+    beti = 0.25 * (1.0 + almi - alfi) * (1.0 + almi - alfi)
+    return dict(almi=almi, alfi=alfi, gami=gami, beti=beti)
+
+
+@dataclasses.dataclass
+class Problem:
+    name: str
+    mesh: M.Mesh
+    ibc: np.ndarray        # (N, 4) int32, columns u, v, w, p
+    bcval: np.ndarray      # (N, 3) Dirichlet velocity values
+    fields: dict           # initial node fields, reference names (lower case)
+    params: dict
+
+    @property
+    def dirichlet_mask(self):
+        return self.ibc[:, :3] == 1
+
+
+def _zero_fields(N):
+    z3 = np.zeros((N, 3))
+    return {k: z3.copy() for k in ("dg", "dgold", "ug", "ugold", "ugm", "ugmold",
+                                   "acg", "acgold", "acgm", "acgmold")} | {
+        "pg": np.zeros(N), "pgold": np.zeros(N)}
+
+
+def cavity(n=16, seed=1, rhoinf=0.5, jitter=0.05) -> Problem:
+    mesh = M.structured_hex(n, n, n, jitter=jitter, seed=seed)
+    N = mesh.nnode
+    faces = M.boundary_masks(mesh, (0, 0, 0), (1, 1, 1))
+    wall = np.zeros(N, bool)
+    for v in faces.values():
+        wall |= v
+    ibc = np.zeros((N, 4), np.int32)
+    ibc[wall, :3] = 1
+    bcval = np.zeros((N, 3))
+    bcval[faces["zhi"], 0] = 1.0      # the lid wins on its edges
+    ibc[0, 3] = 1                     # pressure fixed at node 0 (enclosed flow)
+    f = _zero_fields(N)
+    f["ug"][wall] = bcval[wall]
+    f["ugold"][:] = f["ug"]
+    p = dict(genalpha(rhoinf), delt=0.01, rho=1.0, mu=0.01, kdc=0.0,
+             grav=(0.0, 0.0, -9.81))
+    return Problem("cavity", mesh, ibc, bcval, f, p)
+
+
+def poiseuille(y, z):
+    return 16.0 * y * (1.0 - y) * z * (1.0 - z)
+
+
+def div_free_perturbation(x, lengths, seed, amp=0.01, nmodes=4, raw=False):
+    """u' = curl(psi), psi = sum_m c_m sin(2 pi k_m . x / L + phi_m), scaled so
+    that max |u'| over the given points is ``amp`` (unscaled when ``raw``)."""
+    rng = np.random.default_rng(seed)
+    u = np.zeros_like(x)
+    Lv = np.asarray(lengths, dtype=np.float64)
+    for _ in range(nmodes):
+        k = rng.integers(1, 4, size=3).astype(np.float64)
+        c = rng.normal(size=3)
+        phi = rng.uniform(0, 2 * np.pi)
+        g = 2 * np.pi * k / Lv
+        th = x @ g + phi
+        u += np.cos(th)[:, None] * np.cross(g, c)[None, :]
+    return u if raw else amp * u / np.abs(u).max()
+
+
+def channel(nx=64, ny=32, nz=32, seed=1, rhoinf=0.5) -> Problem:
+    lo, hi = (0.0, 0.0, 0.0), (4.0, 1.0, 1.0)
+    mesh = M.structured_hex(nx, ny, nz, lo=lo, hi=hi, jitter=0.0, seed=seed)
+    N = mesh.nnode
+    x = mesh.xg
+    faces = M.boundary_masks(mesh, lo, hi)
+    wall = faces["ylo"] | faces["yhi"] | faces["zlo"] | faces["zhi"]
+    inflow = faces["xlo"]
+    ibc = np.zeros((N, 4), np.int32)
+    ibc[wall | inflow, :3] = 1
+    bcval = np.zeros((N, 3))
+    bcval[inflow, 0] = poiseuille(x[inflow, 1], x[inflow, 2])
+    bcval[wall] = 0.0
+    f = _zero_fields(N)
+    u0 = np.zeros((N, 3))
+    u0[:, 0] = poiseuille(x[:, 1], x[:, 2])
+    u0 += div_free_perturbation(x, (4.0, 1.0, 1.0), seed)
+    d = ibc[:, :3] == 1
+    u0[d] = bcval[d]
+    f["ug"][:] = u0
+    f["ugold"][:] = u0
+    p = dict(genalpha(rhoinf), delt=0.005, rho=1.0, mu=1e-3, kdc=0.0,
+             grav=(0.0, 0.0, -9.81))
+    return Problem("channel", mesh, ibc, bcval, f, p)
+
+
+def apply_dirichlet(ug, acg, ugold, acgold, prob: Problem):
+    """setBCs_CFD value part (setBCs.f90:84-91): Dirichlet velocity and the
+    consistent acceleration acg = ((u - uold)/dt + (gami-1) acgold)/gami."""
+    p = prob.params
+    m = prob.dirichlet_mask
+    ug = ug.copy()
+    acg = acg.copy()
+    ug[m] = prob.bcval[m]
+    acg[m] = (((ug - ugold) * (1.0 / p["delt"]) + (p["gami"] - 1.0) * acgold) /
+              p["gami"])[m]
+    return ug, acg

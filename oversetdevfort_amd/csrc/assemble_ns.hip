@@ -1,0 +1,439 @@
+// VMS/SUPG Navier-Stokes element assembly on gfx950, hex elements.
+//
+// Restates IntElmAss_NS_overset (assembleNavSto.f90:76-336) with
+// eval_shape_fem/linhexshl (eval_shape_FEM.f90:4-61,148-196), the 8-point
+// rule (genGP_GW.f90:355-389), e3STAB_3D (e3STAB.f90:4-44), e3LHS_3D_fluid_Old
+// (e3LHS.f90:4-161), e3Rhs_3D_fluid (e3RHS.f90:4-97) with the live call's
+// zero fine-scale velocity, BCLhs_3D (BCLhs_3D.f90:4-122) and FillSparseMat_3D
+// (FillSparseMat.f90:33-67).
+//
+// One 64-lane wavefront per element:
+//   phase A  lane = (gauss point q, node a): shape functions, Jacobian,
+//            Gauss-point interpolation (8-lane butterfly reductions), tau's;
+//            shape data staged in LDS; RHS contributions reduced over q.
+//   phase B  lane = (a, b) block pair: the 16 entries of the 4x4 block (a,b)
+//            accumulated over the 8 Gauss points in registers.
+//   phase C  Dirichlet projection as row/column masks, then a deterministic
+//            read-modify-write of the block at emap[e][lane]: elements are
+//            launched one colour at a time and no two elements of a colour
+//            share a node, so no atomics and a fixed summation order.
+#include "ofd_internal.h"
+
+namespace ofd {
+
+namespace {
+
+constexpr int kWavesPerBlock = 4;
+// linhexshl corner signs (sx, sy, sz) in the reference node order
+__constant__ int8_t c_sign[8][3] = {{-1, 1, -1}, {1, 1, -1}, {1, 1, 1},  {-1, 1, 1},
+                                    {-1, -1, -1}, {1, -1, -1}, {1, -1, 1}, {-1, -1, 1}};
+
+__device__ __forceinline__ double red8(double v) {
+  v += __shfl_xor(v, 1);
+  v += __shfl_xor(v, 2);
+  v += __shfl_xor(v, 4);
+  return v;
+}
+
+__device__ __forceinline__ double max8(double v) {
+  v = fmax(v, __shfl_xor(v, 1));
+  v = fmax(v, __shfl_xor(v, 2));
+  v = fmax(v, __shfl_xor(v, 4));
+  return v;
+}
+
+}  // namespace
+
+// alpha stage (assembleNavSto.f90:25-30) packed into one 128-byte record per node
+__global__ void k_alpha_stage(int N, const double *__restrict__ xg,
+                              const double *__restrict__ dg,
+                              const double *__restrict__ dgold,
+                              const double *__restrict__ ug,
+                              const double *__restrict__ ugold,
+                              const double *__restrict__ ugm,
+                              const double *__restrict__ ugmold,
+                              const double *__restrict__ acg,
+                              const double *__restrict__ acgold,
+                              const double *__restrict__ pg, double almi,
+                              double alfi, double *__restrict__ rec) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double r[16];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const size_t o = (size_t)c * N + i;
+    const double d = dgold[o] + alfi * (dg[o] - dgold[o]);
+    r[c] = xg[o] + d;
+    r[3 + c] = ugold[o] + alfi * (ug[o] - ugold[o]);
+    r[6 + c] = acgold[o] + almi * (acg[o] - acgold[o]);
+    r[9 + c] = ugmold[o] + alfi * (ugm[o] - ugmold[o]);
+  }
+  r[12] = pg[i];
+  r[13] = r[14] = r[15] = 0.0;
+  double2 *out = reinterpret_cast<double2 *>(rec + (size_t)i * 16);
+#pragma unroll
+  for (int t = 0; t < 8; ++t) out[t] = make_double2(r[2 * t], r[2 * t + 1]);
+}
+
+struct AsmParams {
+  double almi, alfi, gami, delt, rho, mu, kdc, gx, gy, gz;
+};
+
+__global__ void __launch_bounds__(256)
+k_assemble_ns_hex(int N, int E, const int32_t *__restrict__ ien,
+                  const int32_t *__restrict__ elm_id,
+                  const int32_t *__restrict__ elems, int first, int count,
+                  const int32_t *__restrict__ emap,
+                  const double *__restrict__ rec,
+                  const uint8_t *__restrict__ ibc, AsmParams P, int id1,
+                  int id2, int subdomain_flag, int do_mat, int do_vec,
+                  double *__restrict__ A, double *__restrict__ rhs,
+                  double *__restrict__ cfl_out, int32_t *__restrict__ negdet) {
+  __shared__ double s_sh[kWavesPerBlock][8][8][5];  // [q][a]: N, dN/dx(3), tmp1
+  __shared__ double s_q[kWavesPerBlock][8][4];      // [q]: tauM, tauP, tauC, DetJ*w
+  __shared__ int32_t s_node[kWavesPerBlock][8];
+
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int idx = first + blockIdx.x * kWavesPerBlock + w;
+  const bool valid = idx < first + count;
+  const int e = valid ? elems[idx] : 0;
+
+  bool tgt = false;
+  if (valid) {
+    const int id = elm_id[e];
+    if (subdomain_flag & 1) tgt = tgt || id == id1;
+    if (subdomain_flag & 2) tgt = tgt || id == id2;
+  }
+  const bool mat = do_mat && tgt;
+  const bool vec = do_vec && tgt;
+
+  // ---------------- phase A: lane = (q, a) ----------------
+  const int q = lane >> 3;
+  const int a = lane & 7;
+  const int na = valid ? ien[(size_t)a * E + e] : 0;
+  if (lane < 8) s_node[w][lane] = na;
+
+  const double2 *r2 = reinterpret_cast<const double2 *>(rec + (size_t)na * 16);
+  double xr[13];
+#pragma unroll
+  for (int t = 0; t < 6; ++t) {
+    const double2 v = r2[t];
+    xr[2 * t] = v.x;
+    xr[2 * t + 1] = v.y;
+  }
+  xr[12] = r2[6].x;
+  const double *xd = xr;       // x + d
+  const double *ua = xr + 3;   // u
+  const double *aca = xr + 6;  // ac
+  const double *uma = xr + 9;  // um
+  const double pa = xr[12];
+
+  // reference shape functions (linhexshl) at Gauss point q
+  const double gpt = 0.57735026918963;
+  const double xi = (q & 1) ? gpt : -gpt;
+  const double eta = (q & 2) ? gpt : -gpt;
+  const double zeta = (q & 4) ? gpt : -gpt;
+  const double sx = c_sign[a][0], sy = c_sign[a][1], sz = c_sign[a][2];
+  const double fx = (sx > 0) ? (xi + 1.0) : (1.0 - xi);
+  const double fy = (sy > 0) ? (eta + 1.0) : (1.0 - eta);
+  const double fz = (sz > 0) ? (zeta + 1.0) : (1.0 - zeta);
+  const double Na = fx * fy * fz * 0.125;
+  const double dl0 = sx * (fy * fz * 0.125);
+  const double dl1 = sy * (fx * fz * 0.125);
+  const double dl2 = sz * (fx * fy * 0.125);
+
+  // dx/dxi (i,j) = sum_a (x+d)_a,i dN_a/dxi_j
+  double J[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    J[i + 0] = red8(xd[i] * dl0);
+    J[i + 3] = red8(xd[i] * dl1);
+    J[i + 6] = red8(xd[i] * dl2);
+  }
+  // get_inverse_3x3 (eval_shape.f90:85-112); column-major Ji(i + 3 j)
+  double Ji[9];
+  Ji[0] = J[4] * J[8] - J[5] * J[7];
+  Ji[3] = J[5] * J[6] - J[3] * J[8];
+  Ji[6] = J[3] * J[7] - J[6] * J[4];
+  const double tmp = 1.0 / (Ji[0] * J[0] + Ji[3] * J[1] + Ji[6] * J[2]);
+  Ji[0] *= tmp;
+  Ji[3] *= tmp;
+  Ji[6] *= tmp;
+  Ji[1] = (J[7] * J[2] - J[1] * J[8]) * tmp;
+  Ji[4] = (J[0] * J[8] - J[2] * J[6]) * tmp;
+  Ji[7] = (J[1] * J[6] - J[0] * J[7]) * tmp;
+  Ji[2] = (J[1] * J[5] - J[4] * J[2]) * tmp;
+  Ji[5] = (J[2] * J[3] - J[0] * J[5]) * tmp;
+  Ji[8] = (J[0] * J[4] - J[3] * J[1]) * tmp;
+  const double detj = 1.0 / tmp;
+
+  // global gradients and metric Gij = sum_k dxi_k/dx_i dxi_k/dx_j
+  double ga[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+    ga[j] = dl0 * Ji[0 + 3 * j] + dl1 * Ji[1 + 3 * j] + dl2 * Ji[2 + 3 * j];
+  double G[9];
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      G[i + 3 * j] = Ji[0 + 3 * i] * Ji[0 + 3 * j] + Ji[1 + 3 * i] * Ji[1 + 3 * j] +
+                     Ji[2 + 3 * i] * Ji[2 + 3 * j];
+
+  // Gauss-point interpolation (e3int_qr, e3int_qr_grad)
+  double ui[3], umi[3], aci[3], du[9], dp[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    ui[i] = red8(ua[i] * Na);
+    umi[i] = red8(uma[i] * Na);
+    aci[i] = red8(aca[i] * Na);
+  }
+  const double pri = red8(pa * Na);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) du[i + 3 * j] = red8(ua[i] * ga[j]);
+#pragma unroll
+  for (int j = 0; j < 3; ++j) dp[j] = red8(pa * ga[j]);
+  (void)dp;
+
+  const double rho = P.rho, mu = P.mu;
+  double uadv[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) uadv[i] = ui[i] - umi[i];
+  const double divu = du[0] + du[4] + du[8];
+
+  // e3STAB_3D
+  double taua = 0.0, gg = 0.0;
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) taua += uadv[i] * G[i + 3 * j] * uadv[j];
+#pragma unroll
+  for (int t = 0; t < 9; ++t) gg += G[t] * G[t];
+  const double nu = mu / rho;
+  const double taud = nu * nu * 12.0 * gg;
+  const double taut = 4.0 / (P.delt * P.delt);
+  const double tauM = 1.0 / (rho * sqrt(taut + taua + taud));
+  const double tauP = tauM;
+  const double tauC = rho * sqrt(taua + taud) / (G[0] + G[4] + G[8]);
+  // e3CFL
+  double cflq = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) cflq += uadv[i] * G[i + 3 * j] * uadv[j];
+  cflq = P.delt * sqrt(cflq);
+  const double dw = detj;  // 8-point rule weights are 1
+  if (valid && a == 0 && detj < 0.0) atomicAdd(negdet, 1);
+
+  const double tmp1a = rho * (uadv[0] * ga[0] + uadv[1] * ga[1] + uadv[2] * ga[2]);
+  s_sh[w][q][a][0] = Na;
+  s_sh[w][q][a][1] = ga[0];
+  s_sh[w][q][a][2] = ga[1];
+  s_sh[w][q][a][3] = ga[2];
+  s_sh[w][q][a][4] = tmp1a;
+  if (a == 0) {
+    s_q[w][q][0] = tauM;
+    s_q[w][q][1] = tauP;
+    s_q[w][q][2] = tauC;
+    s_q[w][q][3] = dw;
+  }
+
+  // RHS (e3Rhs_3D_fluid with uprime = 0): contribution of Gauss point q to node a
+  double rcu[3] = {0.0, 0.0, 0.0}, rcp = 0.0;
+  if (vec) {
+    const double mupkdc = mu + P.kdc;
+    const double ptot = pri - tauC * divu;
+    const double fi[3] = {P.gx * rho, P.gy * rho, P.gz * rho};
+    double t1[3], t2[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      t1[i] = rho * (aci[i] + uadv[0] * du[i + 0] + uadv[1] * du[i + 3] +
+                     uadv[2] * du[i + 6]) - fi[i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        t2[i + 3 * j] = (i == j) ? (-ptot + 2.0 * mupkdc * du[i + 3 * i])
+                                 : mupkdc * (du[i + 3 * j] + du[j + 3 * i]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      rcu[i] = -(Na * t1[i] + (ga[0] * t2[i] + ga[1] * t2[i + 3] + ga[2] * t2[i + 6])) * dw;
+    rcp = -(Na * divu) * dw;
+  }
+  // reduce over the 8 Gauss points (lanes a, a+8, ..., a+56)
+#pragma unroll
+  for (int m = 8; m < 64; m <<= 1) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) rcu[i] += __shfl_xor(rcu[i], m);
+    rcp += __shfl_xor(rcp, m);
+    cflq = fmax(cflq, __shfl_xor(cflq, m));
+  }
+  cflq = max8(cflq);
+  if (valid && lane == 0) cfl_out[e] = cflq;
+
+  __syncthreads();
+
+  // ---------------- phase B: lane = (a, b) ----------------
+  const int pa_ = lane & 7;   // row node a
+  const int pb_ = lane >> 3;  // column node b
+  double acc[16];
+#pragma unroll
+  for (int t = 0; t < 16; ++t) acc[t] = 0.0;
+  if (mat) {
+    const double fact1 = P.almi;
+    const double fact2 = P.alfi * P.gami * P.delt;
+    const double mupkdc = mu + P.kdc;
+#pragma unroll 2
+    for (int qq = 0; qq < 8; ++qq) {
+      const double Nq = s_sh[w][qq][pa_][0];
+      const double g0 = s_sh[w][qq][pa_][1], g1 = s_sh[w][qq][pa_][2],
+                   g2 = s_sh[w][qq][pa_][3], t1a = s_sh[w][qq][pa_][4];
+      const double Mb = s_sh[w][qq][pb_][0];
+      const double h0 = s_sh[w][qq][pb_][1], h1 = s_sh[w][qq][pb_][2],
+                   h2 = s_sh[w][qq][pb_][3], t1b = s_sh[w][qq][pb_][4];
+      const double tM = s_q[w][qq][0], tP = s_q[w][qq][1], tC = s_q[w][qq][2],
+                   dj = s_q[w][qq][3];
+      const double gab = g0 * h0 + g1 * h1 + g2 * h2;
+      const double tmp4 = fact1 * (Nq * rho * Mb + t1a * tM * rho * Mb) +
+                          fact2 * (Nq * t1b + t1a * tM * t1b + mupkdc * gab);
+      // K11 (row-major r*3+c in block slots r*4+c)
+      acc[0] += (tmp4 + fact2 * (g0 * mupkdc * h0 + g0 * tC * h0)) * dj;
+      acc[5] += (tmp4 + fact2 * (g1 * mupkdc * h1 + g1 * tC * h1)) * dj;
+      acc[10] += (tmp4 + fact2 * (g2 * mupkdc * h2 + g2 * tC * h2)) * dj;
+      acc[1] += fact2 * (g1 * mupkdc * h0 + g0 * tC * h1) * dj;
+      acc[4] += fact2 * (g0 * mupkdc * h1 + g1 * tC * h0) * dj;
+      acc[2] += fact2 * (g2 * mupkdc * h0 + g0 * tC * h2) * dj;
+      acc[8] += fact2 * (g0 * mupkdc * h2 + g2 * tC * h0) * dj;
+      acc[6] += fact2 * (g2 * mupkdc * h1 + g1 * tC * h2) * dj;
+      acc[9] += fact2 * (g1 * mupkdc * h2 + g2 * tC * h1) * dj;
+      // G (column p)
+      acc[3] += fact2 * (-g0 * Mb + t1a * tM * h0) * dj;
+      acc[7] += fact2 * (-g1 * Mb + t1a * tM * h1) * dj;
+      acc[11] += fact2 * (-g2 * Mb + t1a * tM * h2) * dj;
+      // D1 (row p)
+      acc[12] += (fact2 * (Nq * h0 + g0 * tP * t1b) + fact1 * (g0 * tP * rho * Mb)) * dj;
+      acc[13] += (fact2 * (Nq * h1 + g1 * tP * t1b) + fact1 * (g1 * tP * rho * Mb)) * dj;
+      acc[14] += (fact2 * (Nq * h2 + g2 * tP * t1b) + fact1 * (g2 * tP * rho * Mb)) * dj;
+      // M
+      acc[15] += fact2 * tP * gab * dj;
+    }
+  }
+
+  // ---------------- phase C: Dirichlet projection + scatter ----------------
+  if (valid) {
+    const int nra = s_node[w][pa_];
+    const int nrb = s_node[w][pb_];
+    const unsigned ma = ibc[nra];
+    const unsigned mb = ibc[nrb];
+    if (do_mat) {
+      const int k = emap[(size_t)e * 64 + lane];
+      double2 *blk = reinterpret_cast<double2 *>(A + (size_t)k * 16);
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        double v[2] = {acc[2 * t], acc[2 * t + 1]};
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int r = (2 * t + h) >> 2, cc = (2 * t + h) & 3;
+          if (((ma >> r) & 1u) || ((mb >> cc) & 1u)) v[h] = 0.0;
+          if (pa_ == pb_ && r == cc && ((ma >> r) & 1u)) v[h] = 1.0;
+        }
+        double2 old = blk[t];
+        old.x += v[0];
+        old.y += v[1];
+        blk[t] = old;
+      }
+    }
+    if (do_vec && lane < 8) {
+      // lane < 8 holds the q-reduced RHS of node a = lane
+      const unsigned m0 = ibc[na];
+      double o[4] = {rcu[0], rcu[1], rcu[2], rcp};
+#pragma unroll
+      for (int cc = 0; cc < 4; ++cc)
+        if ((m0 >> cc) & 1u) o[cc] = 0.0;
+      double2 *rr = reinterpret_cast<double2 *>(rhs + (size_t)na * 4);
+      double2 r0 = rr[0], r1 = rr[1];
+      r0.x += o[0];
+      r0.y += o[1];
+      r1.x += o[2];
+      r1.y += o[3];
+      rr[0] = r0;
+      rr[1] = r1;
+    }
+  }
+}
+
+__global__ void k_zero(double *p, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = 0.0;
+}
+
+__global__ void k_max_reduce(const double *v, int n, double *out) {
+  __shared__ double s[256];
+  double m = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, v[i]);
+  s[threadIdx.x] = m;
+  __syncthreads();
+  for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + h]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = s[0];
+}
+
+hipError_t launch_alpha_stage(Ctx &c) {
+  const int N = c.nnode;
+  const double almi = c.prm[OFD_P_ALMI], alfi = c.prm[OFD_P_ALFI];
+  hipLaunchKernelGGL(k_alpha_stage, dim3((N + 255) / 256), dim3(256), 0, c.stream, N,
+                     c.d_field[OFD_XG], c.d_field[OFD_DG], c.d_field[OFD_DGOLD],
+                     c.d_field[OFD_UG], c.d_field[OFD_UGOLD], c.d_field[OFD_UGM],
+                     c.d_field[OFD_UGMOLD], c.d_field[OFD_ACG],
+                     c.d_field[OFD_ACGOLD], c.d_field[OFD_PG], almi, alfi,
+                     c.d_noderec);
+  return hipGetLastError();
+}
+
+hipError_t launch_assemble_ns_hex(Ctx &c, int tensor_flag, int subdomain_flag,
+                                  int color) {
+  const int first = c.color_off[color];
+  const int count = c.color_off[color + 1] - first;
+  if (count <= 0) return hipSuccess;
+  AsmParams P;
+  P.almi = c.prm[OFD_P_ALMI];
+  P.alfi = c.prm[OFD_P_ALFI];
+  P.gami = c.prm[OFD_P_GAMI];
+  P.delt = c.prm[OFD_P_DELT];
+  P.rho = c.prm[OFD_P_RHO];
+  P.mu = c.prm[OFD_P_MU];
+  P.kdc = c.prm[OFD_P_KDC];
+  P.gx = c.prm[OFD_P_GX];
+  P.gy = c.prm[OFD_P_GY];
+  P.gz = c.prm[OFD_P_GZ];
+  const int blocks = (count + kWavesPerBlock - 1) / kWavesPerBlock;
+  hipLaunchKernelGGL(k_assemble_ns_hex, dim3(blocks), dim3(64 * kWavesPerBlock), 0,
+                     c.stream, c.nnode, c.nelem, c.d_ien, c.d_elm_id,
+                     c.d_color_elems, first, count, c.d_emap, c.d_noderec, c.d_ibc,
+                     P, c.elm_id_set[0], c.elm_id_set[1], subdomain_flag,
+                     (tensor_flag & OFD_ASSEMBLE_MAT) ? 1 : 0,
+                     (tensor_flag & OFD_ASSEMBLE_VEC) ? 1 : 0, c.d_A64, c.d_rhs,
+                     c.d_cfl, c.d_negdet);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero(Ctx &c, double *p, size_t n) {
+  if (n == 0) return hipSuccess;
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_zero, dim3((unsigned)blocks), dim3(256), 0, c.stream, p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_max_reduce(Ctx &c, const double *v, int n, double *out_dev) {
+  hipLaunchKernelGGL(k_max_reduce, dim3(1), dim3(256), 0, c.stream, v, n, out_dev);
+  return hipGetLastError();
+}
+
+}  // namespace ofd

@@ -1,0 +1,834 @@
+// C ABI of the gfx950 OversetDevFort hot-path library (declared in
+// include/ofd.h).  Host-side bookkeeping: context and device buffers, the
+// sparsity pattern (genSparStruc.f90:4-105), element colouring, the
+// (element, a, b) -> nonzero scatter map that replaces the binary search of
+// SparseMatLoc_3D (FillSparseMat.f90:4-28), and the GMRES / Newton drivers
+// that sequence the kernels.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "ofd_internal.h"
+
+namespace ofd {
+
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+
+template <typename T>
+static int dalloc(Ctx &c, T **p, size_t n) {
+  if (*p) {
+    (void)hipFree(*p);
+    *p = nullptr;
+  }
+  if (n == 0) return OFD_OK;
+  hipError_t e = hipMalloc(reinterpret_cast<void **>(p), n * sizeof(T));
+  if (e != hipSuccess) {
+    set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+    return OFD_ERR_ALLOC;
+  }
+  c.dev_bytes += (int64_t)(n * sizeof(T));
+  return OFD_OK;
+}
+
+template <typename T>
+static void dfree(T *&p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+static int ensure_pin(Ctx &c, size_t bytes) {
+  if (c.h_pin_bytes >= bytes) return OFD_OK;
+  if (c.h_pin) (void)hipHostFree(c.h_pin);
+  c.h_pin = nullptr;
+  c.h_pin_bytes = 0;
+  OFD_HIP(hipHostMalloc(reinterpret_cast<void **>(&c.h_pin), bytes));
+  c.h_pin_bytes = bytes;
+  return OFD_OK;
+}
+
+// host -> device through the pinned staging buffer
+static int h2d(Ctx &c, void *dst, const void *src, size_t bytes) {
+  if (bytes == 0) return OFD_OK;
+  int rc = ensure_pin(c, bytes);
+  if (rc) return rc;
+  std::memcpy(c.h_pin, src, bytes);
+  OFD_HIP(hipMemcpyAsync(dst, c.h_pin, bytes, hipMemcpyHostToDevice, c.stream));
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  return OFD_OK;
+}
+
+static int d2h(Ctx &c, void *dst, const void *src, size_t bytes) {
+  if (bytes == 0) return OFD_OK;
+  int rc = ensure_pin(c, bytes);
+  if (rc) return rc;
+  OFD_HIP(hipMemcpyAsync(c.h_pin, src, bytes, hipMemcpyDeviceToHost, c.stream));
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  std::memcpy(dst, c.h_pin, bytes);
+  return OFD_OK;
+}
+
+static int field_len(const Ctx &c, int f) {
+  return (f == OFD_PG || f == OFD_PGOLD || f == OFD_SOLP) ? c.nnode : 3 * c.nnode;
+}
+
+static int need_mesh(const Ctx &c) {
+  OFD_CHECK(c.nnode > 0 && c.nelem > 0, OFD_ERR_ARG, "no mesh uploaded");
+  return OFD_OK;
+}
+
+static int need_pattern(const Ctx &c) {
+  OFD_CHECK(c.icnt > 0 && c.d_col, OFD_ERR_ARG, "no sparsity pattern (ofd_build_pattern)");
+  return OFD_OK;
+}
+
+// Krylov workspace for restart length K
+static int ensure_krylov(Ctx &c, int K) {
+  OFD_CHECK(K >= 1 && K <= kMaxKspace, OFD_ERR_ARG, "kspace out of range [1, 64]");
+  if (c.kalloc >= K) return OFD_OK;
+  int rc = dalloc(c, &c.d_W, (size_t)(K + 1) * c.nnode * 4);
+  if (rc) return rc;
+  c.kalloc = K;
+  return OFD_OK;
+}
+
+// preconditioner (and FP32 copies) of the current matrix
+static int ensure_prec(Ctx &c, int precision) {
+  OFD_CHECK(c.mat_valid, OFD_ERR_ARG, "no assembled matrix");
+  if (precision == OFD_FP32 && !c.a32_valid) {
+    if (!c.d_A32) {
+      int rc = dalloc(c, &c.d_A32, (size_t)c.icnt * 16);
+      if (rc) return rc;
+    }
+    if (!c.d_P32) {
+      int rc = dalloc(c, &c.d_P32, (size_t)c.nnode * 16);
+      if (rc) return rc;
+    }
+    OFD_HIP(launch_convert_a32(c));
+    OFD_HIP(launch_build_prec(c, true));
+    c.a32_valid = true;
+    c.prec_valid = true;
+  }
+  if (!c.prec_valid) {
+    OFD_HIP(launch_build_prec(c, c.d_P32 != nullptr));
+    c.prec_valid = true;
+  }
+  return OFD_OK;
+}
+
+// one GMRES cycle on right-hand side b (device [N][4]); x += increment
+static int gmres_cycle(Ctx &c, int precision, int ortho, const double *b, double tol,
+                       int K, int kmin, double *x, int *iters, double *resid,
+                       double *rnorm0) {
+  int rc = ensure_krylov(c, K);
+  if (rc) return rc;
+  rc = ensure_prec(c, precision);
+  if (rc) return rc;
+  OFD_HIP(launch_gmres_init(c, precision, b, tol, K, kmin));
+  for (int k = 0; k < K; ++k) {
+    OFD_HIP(launch_spmv_prec(c, precision, k));
+    if (ortho == OFD_ORTHO_MGS) {
+      for (int j = 0; j <= k + 1; ++j) OFD_HIP(launch_mgs_step(c, k, j));
+    } else {
+      OFD_HIP(launch_cgs_pass(c, k, 1));
+      OFD_HIP(launch_cgs_pass(c, k, 2));
+      OFD_HIP(launch_cgs_pass(c, k, 3));
+    }
+    // poll the device-side convergence flag now and then: the kernels of a
+    // finished cycle return at once, so polling only bounds wasted launches
+    if ((k % 4) == 3 && k + 1 < K) {
+      OFD_HIP(hipMemcpyAsync(c.h_gs, c.d_gs, sizeof(GmresState), hipMemcpyDeviceToHost,
+                             c.stream));
+      OFD_HIP(hipStreamSynchronize(c.stream));
+      if (c.h_gs->done) break;
+    }
+  }
+  OFD_HIP(launch_gmres_finish(c, x));
+  OFD_HIP(hipMemcpyAsync(c.h_gs, c.d_gs, sizeof(GmresState), hipMemcpyDeviceToHost,
+                         c.stream));
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  if (iters) *iters = c.h_gs->iters;
+  if (resid) *resid = c.h_gs->resid;
+  if (rnorm0) *rnorm0 = c.h_gs->rnorm0;
+  if (!std::isfinite(c.h_gs->resid) || !std::isfinite(c.h_gs->rnorm0)) {
+    set_error("GMRES: non-finite residual");
+    return OFD_ERR_NONFINITE;
+  }
+  return OFD_OK;
+}
+
+static int norm2_dev(Ctx &c, const double *v, double *out) {
+  OFD_HIP(launch_norm2(c, v, 0));
+  OFD_HIP(hipMemcpyAsync(c.h_gs, c.d_gs, sizeof(GmresState), hipMemcpyDeviceToHost,
+                         c.stream));
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  *out = c.h_gs->scal[0];
+  return OFD_OK;
+}
+
+}  // namespace ofd
+
+using namespace ofd;
+
+extern "C" {
+
+const char *ofd_last_error(void) { return g_err.c_str(); }
+
+int ofd_version(void) { return 1; }
+
+int ofd_ctx_create(int device, ofd_ctx **out) {
+  OFD_CHECK(out, OFD_ERR_ARG, "null output");
+  int n = 0;
+  OFD_HIP(hipGetDeviceCount(&n));
+  OFD_CHECK(device >= 0 && device < n, OFD_ERR_ARG, "device ordinal out of range");
+  OFD_HIP(hipSetDevice(device));
+  ofd_ctx *x = new (std::nothrow) ofd_ctx();
+  OFD_CHECK(x, OFD_ERR_ALLOC, "out of host memory");
+  Ctx &c = x->c;
+  c.device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete x;
+    set_error(std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    return OFD_ERR_HIP;
+  }
+  int rc = dalloc(c, &c.d_gs, 1);
+  if (!rc) rc = dalloc(c, &c.d_part, (size_t)kMaxRedBlocks * (kMaxKspace + 1));
+  if (!rc) rc = dalloc(c, &c.d_negdet, 1);
+  if (rc) {
+    ofd_ctx_destroy(x);
+    return rc;
+  }
+  e = hipHostMalloc(reinterpret_cast<void **>(&c.h_gs), sizeof(GmresState));
+  if (e != hipSuccess) {
+    ofd_ctx_destroy(x);
+    set_error("hipHostMalloc failed");
+    return OFD_ERR_ALLOC;
+  }
+  std::memset(c.h_gs, 0, sizeof(GmresState));
+  e = hipMemcpy(c.d_gs, c.h_gs, sizeof(GmresState), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    ofd_ctx_destroy(x);
+    set_error("hipMemcpy failed");
+    return OFD_ERR_HIP;
+  }
+  c.prm[OFD_P_ALMI] = c.prm[OFD_P_ALFI] = c.prm[OFD_P_GAMI] = c.prm[OFD_P_BETI] = 1.0;
+  c.prm[OFD_P_DELT] = 1.0;
+  c.prm[OFD_P_RHO] = 1.0;
+  c.prm[OFD_P_GZ] = -9.81;
+  *out = x;
+  return OFD_OK;
+}
+
+int ofd_ctx_destroy(ofd_ctx *x) {
+  if (!x) return OFD_OK;
+  Ctx &c = x->c;
+  (void)hipSetDevice(c.device);
+  if (c.stream) (void)hipStreamSynchronize(c.stream);
+  dfree(c.d_ien);
+  dfree(c.d_elm_id);
+  dfree(c.d_col);
+  dfree(c.d_row);
+  dfree(c.d_emap);
+  dfree(c.d_color_elems);
+  dfree(c.d_diag);
+  for (auto &f : c.d_field) dfree(f);
+  dfree(c.d_ibc);
+  dfree(c.d_bcval);
+  dfree(c.d_noderec);
+  dfree(c.d_A64);
+  dfree(c.d_A32);
+  dfree(c.d_P64);
+  dfree(c.d_P32);
+  dfree(c.d_rhs);
+  dfree(c.d_b);
+  dfree(c.d_x);
+  dfree(c.d_r);
+  dfree(c.d_cfl);
+  dfree(c.d_negdet);
+  dfree(c.d_W);
+  dfree(c.d_part);
+  dfree(c.d_gs);
+  if (c.h_gs) (void)hipHostFree(c.h_gs);
+  if (c.h_pin) (void)hipHostFree(c.h_pin);
+  if (c.stream) (void)hipStreamDestroy(c.stream);
+  delete x;
+  return OFD_OK;
+}
+
+int ofd_upload_mesh(ofd_ctx *x, int nnode, int nelem, int nshl, const int32_t *ien,
+                    const int32_t *elm_id, const double *xg) {
+  OFD_CHECK(x && ien && elm_id && xg, OFD_ERR_ARG, "null argument");
+  OFD_CHECK(nnode > 0 && nelem > 0, OFD_ERR_ARG, "empty mesh");
+  OFD_CHECK(nshl == 8, OFD_ERR_UNSUPPORTED,
+            "only trilinear hex elements (NSHL=8) are implemented on the GPU");
+  Ctx &c = x->c;
+  OFD_HIP(hipSetDevice(c.device));
+  const size_t ne = (size_t)nelem * nshl;
+  c.h_ien.resize(ne);
+  for (size_t t = 0; t < ne; ++t) {
+    const int v = ien[t] - 1;
+    OFD_CHECK(v >= 0 && v < nnode, OFD_ERR_ARG, "IEN entry out of range [1, NNODE]");
+    c.h_ien[t] = v;
+  }
+  c.h_elm_id.assign(elm_id, elm_id + nelem);
+  c.nnode = nnode;
+  c.nelem = nelem;
+  c.nshl = nshl;
+  c.icnt = 0;
+  c.mat_valid = c.prec_valid = c.a32_valid = false;
+  int rc = dalloc(c, &c.d_ien, ne);
+  if (!rc) rc = dalloc(c, &c.d_elm_id, (size_t)nelem);
+  for (int f = 0; f < OFD_NFIELDS && !rc; ++f)
+    rc = dalloc(c, &c.d_field[f], (size_t)field_len(c, f));
+  if (!rc) rc = dalloc(c, &c.d_ibc, (size_t)nnode);
+  if (!rc) rc = dalloc(c, &c.d_bcval, (size_t)nnode * 3);
+  if (!rc) rc = dalloc(c, &c.d_noderec, (size_t)nnode * 16);
+  if (!rc) rc = dalloc(c, &c.d_rhs, (size_t)nnode * 4);
+  if (!rc) rc = dalloc(c, &c.d_b, (size_t)nnode * 4);
+  if (!rc) rc = dalloc(c, &c.d_x, (size_t)nnode * 4);
+  if (!rc) rc = dalloc(c, &c.d_r, (size_t)nnode * 4);
+  if (!rc) rc = dalloc(c, &c.d_cfl, (size_t)nelem);
+  if (!rc) rc = dalloc(c, &c.d_P64, (size_t)nnode * 16);
+  if (rc) return rc;
+  rc = h2d(c, c.d_ien, c.h_ien.data(), ne * sizeof(int32_t));
+  if (!rc) rc = h2d(c, c.d_elm_id, elm_id, (size_t)nelem * sizeof(int32_t));
+  if (!rc) rc = h2d(c, c.d_field[OFD_XG], xg, (size_t)nnode * 3 * sizeof(double));
+  if (rc) return rc;
+  for (int f = 0; f < OFD_NFIELDS; ++f)
+    if (f != OFD_XG)
+      OFD_HIP(hipMemsetAsync(c.d_field[f], 0, (size_t)field_len(c, f) * sizeof(double),
+                             c.stream));
+  OFD_HIP(hipMemsetAsync(c.d_ibc, 0, (size_t)nnode, c.stream));
+  OFD_HIP(hipMemsetAsync(c.d_bcval, 0, (size_t)nnode * 3 * sizeof(double), c.stream));
+  OFD_HIP(hipMemsetAsync(c.d_x, 0, (size_t)nnode * 4 * sizeof(double), c.stream));
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  return OFD_OK;
+}
+
+int ofd_build_pattern(ofd_ctx *x, int64_t *icnt_out) {
+  OFD_CHECK(x, OFD_ERR_ARG, "null context");
+  Ctx &c = x->c;
+  int rc = need_mesh(c);
+  if (rc) return rc;
+  OFD_HIP(hipSetDevice(c.device));
+  const int N = c.nnode, E = c.nelem, S = c.nshl;
+  const int cap = 27 * S;
+  // Asadj (genSparStruc.f90:52-105): per-node neighbour lists in element order
+  std::vector<int32_t> list((size_t)N * cap);
+  std::vector<int32_t> cnt(N, 0);
+  for (int e = 0; e < E; ++e) {
+    for (int j = 0; j < S; ++j) {
+      const int jn = c.h_ien[(size_t)j * E + e];
+      int32_t *L = &list[(size_t)jn * cap];
+      int len = cnt[jn];
+      for (int k = 0; k < S; ++k) {
+        const int kn = c.h_ien[(size_t)k * E + e];
+        bool found = false;
+        for (int l = 0; l < len; ++l)
+          if (L[l] == kn) {
+            found = true;
+            break;
+          }
+        if (!found) {
+          OFD_CHECK(len < cap, OFD_ERR_ARG, "node adjacency exceeds 27*NSHL");
+          L[len++] = kn;
+        }
+      }
+      cnt[jn] = len;
+    }
+  }
+  // col / row with each row sorted (genSparStruc.f90:27-45)
+  c.h_col.assign(N + 1, 0);
+  for (int i = 0; i < N; ++i) c.h_col[i + 1] = c.h_col[i] + cnt[i];
+  c.icnt = c.h_col[N];
+  c.h_row.resize(c.icnt);
+  for (int i = 0; i < N; ++i) {
+    int32_t *L = &list[(size_t)i * cap];
+    std::sort(L, L + cnt[i]);
+    std::copy(L, L + cnt[i], c.h_row.begin() + c.h_col[i]);
+  }
+  // diagonal block of each row (SparseGMRES_NS.f90:53-61)
+  std::vector<int32_t> diag(N, -1);
+  for (int i = 0; i < N; ++i)
+    for (int k = c.h_col[i]; k < c.h_col[i + 1]; ++k)
+      if (c.h_row[k] == i) diag[i] = k;
+  // scatter map: SparseMatLoc_3D's search done once per (e, a, b)
+  std::vector<int32_t> emap((size_t)E * 64, 0);
+  for (int e = 0; e < E; ++e)
+    for (int b = 0; b < S; ++b)
+      for (int a = 0; a < S; ++a) {
+        const int ia = c.h_ien[(size_t)a * E + e], ib = c.h_ien[(size_t)b * E + e];
+        const int32_t *lo = c.h_row.data() + c.h_col[ia];
+        const int32_t *hi = c.h_row.data() + c.h_col[ia + 1];
+        const int32_t *p = std::lower_bound(lo, hi, ib);
+        emap[(size_t)e * 64 + a + 8 * b] = (int32_t)(p - c.h_row.data());
+      }
+  // greedy colouring: elements of one colour share no node
+  std::vector<uint32_t> nodemask(N, 0u);
+  std::vector<int32_t> ecol(E);
+  int ncolor = 0;
+  for (int e = 0; e < E; ++e) {
+    uint32_t used = 0;
+    for (int a = 0; a < S; ++a) used |= nodemask[c.h_ien[(size_t)a * E + e]];
+    OFD_CHECK(used != 0xffffffffu, OFD_ERR_UNSUPPORTED, "element colouring needs > 32 colours");
+    int col = 0;
+    while (used & (1u << col)) ++col;
+    ecol[e] = col;
+    ncolor = std::max(ncolor, col + 1);
+    for (int a = 0; a < S; ++a) nodemask[c.h_ien[(size_t)a * E + e]] |= (1u << col);
+  }
+  c.color_off.assign(ncolor + 1, 0);
+  for (int e = 0; e < E; ++e) c.color_off[ecol[e] + 1]++;
+  for (int k = 0; k < ncolor; ++k) c.color_off[k + 1] += c.color_off[k];
+  std::vector<int32_t> order(E);
+  {
+    std::vector<int32_t> pos(c.color_off.begin(), c.color_off.end() - 1);
+    for (int e = 0; e < E; ++e) order[pos[ecol[e]]++] = e;
+  }
+  rc = dalloc(c, &c.d_col, (size_t)N + 1);
+  if (!rc) rc = dalloc(c, &c.d_row, (size_t)c.icnt);
+  if (!rc) rc = dalloc(c, &c.d_diag, (size_t)N);
+  if (!rc) rc = dalloc(c, &c.d_emap, (size_t)E * 64);
+  if (!rc) rc = dalloc(c, &c.d_color_elems, (size_t)E);
+  if (!rc) rc = dalloc(c, &c.d_A64, (size_t)c.icnt * 16);
+  if (rc) return rc;
+  dfree(c.d_A32);
+  dfree(c.d_P32);
+  rc = h2d(c, c.d_col, c.h_col.data(), (N + 1) * sizeof(int32_t));
+  if (!rc) rc = h2d(c, c.d_row, c.h_row.data(), c.icnt * sizeof(int32_t));
+  if (!rc) rc = h2d(c, c.d_diag, diag.data(), N * sizeof(int32_t));
+  if (!rc) rc = h2d(c, c.d_emap, emap.data(), emap.size() * sizeof(int32_t));
+  if (!rc) rc = h2d(c, c.d_color_elems, order.data(), E * sizeof(int32_t));
+  if (rc) return rc;
+  OFD_HIP(hipMemsetAsync(c.d_A64, 0, (size_t)c.icnt * 16 * sizeof(double), c.stream));
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  c.mat_valid = c.prec_valid = c.a32_valid = false;
+  if (icnt_out) *icnt_out = c.icnt;
+  return OFD_OK;
+}
+
+int ofd_get_pattern(ofd_ctx *x, int32_t *col, int32_t *row) {
+  OFD_CHECK(x && col && row, OFD_ERR_ARG, "null argument");
+  Ctx &c = x->c;
+  int rc = need_pattern(c);
+  if (rc) return rc;
+  for (int i = 0; i <= c.nnode; ++i) col[i] = c.h_col[i] + 1;
+  for (int64_t k = 0; k < c.icnt; ++k) row[k] = c.h_row[k] + 1;
+  return OFD_OK;
+}
+
+int ofd_set_params(ofd_ctx *x, const double *prm) {
+  OFD_CHECK(x && prm, OFD_ERR_ARG, "null argument");
+  OFD_CHECK(prm[OFD_P_RHO] > 0.0 && prm[OFD_P_DELT] > 0.0, OFD_ERR_ARG,
+            "rho and Delt must be positive");
+  std::memcpy(x->c.prm, prm, sizeof(double) * OFD_NPARAMS);
+  return OFD_OK;
+}
+
+int ofd_set_subdomains(ofd_ctx *x, int32_t id1, int32_t id2) {
+  OFD_CHECK(x, OFD_ERR_ARG, "null context");
+  x->c.elm_id_set[0] = id1;
+  x->c.elm_id_set[1] = id2;
+  return OFD_OK;
+}
+
+int ofd_set_ibc(ofd_ctx *x, const int32_t *ibc, int ncol) {
+  OFD_CHECK(x && ibc, OFD_ERR_ARG, "null argument");
+  OFD_CHECK(ncol >= 4, OFD_ERR_ARG, "IBC needs at least 4 columns (u,v,w,p)");
+  Ctx &c = x->c;
+  int rc = need_mesh(c);
+  if (rc) return rc;
+  OFD_HIP(hipSetDevice(c.device));
+  std::vector<uint8_t> m(c.nnode, 0);
+  for (int i = 0; i < c.nnode; ++i)
+    for (int col = 0; col < 4; ++col)
+      if (ibc[(size_t)col * c.nnode + i] == 1) m[i] |= (uint8_t)(1u << col);
+  return h2d(c, c.d_ibc, m.data(), m.size());
+}
+
+int ofd_set_bcvals(ofd_ctx *x, const double *vals) {
+  OFD_CHECK(x && vals, OFD_ERR_ARG, "null argument");
+  Ctx &c = x->c;
+  int rc = need_mesh(c);
+  if (rc) return rc;
+  OFD_HIP(hipSetDevice(c.device));
+  return h2d(c, c.d_bcval, vals, (size_t)c.nnode * 3 * sizeof(double));
+}
+
+int ofd_apply_dirichlet(ofd_ctx *x) {
+  OFD_CHECK(x, OFD_ERR_ARG, "null context");
+  Ctx &c = x->c;
+  int rc = need_mesh(c);
+  if (rc) return rc;
+  OFD_HIP(hipSetDevice(c.device));
+  OFD_HIP(launch_apply_dirichlet(c));
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  return OFD_OK;
+}
+
+int ofd_set_field(ofd_ctx *x, int field, const double *host) {
+  OFD_CHECK(x && host, OFD_ERR_ARG, "null argument");
+  OFD_CHECK(field >= 0 && field < OFD_NFIELDS, OFD_ERR_ARG, "unknown field");
+  Ctx &c = x->c;
+  int rc = need_mesh(c);
+  if (rc) return rc;
+  OFD_HIP(hipSetDevice(c.device));
+  rc = h2d(c, c.d_field[field], host, (size_t)field_len(c, field) * sizeof(double));
+  if (rc) return rc;
+  if (field == OFD_SOLU || field == OFD_SOLP) {
+    OFD_HIP(launch_soa_to_node4(c, c.d_field[OFD_SOLU], c.d_field[OFD_SOLP], c.d_x));
+    OFD_HIP(hipStreamSynchronize(c.stream));
+  }
+  return OFD_OK;
+}
+
+int ofd_get_field(ofd_ctx *x, int field, double *host) {
+  OFD_CHECK(x && host, OFD_ERR_ARG, "null argument");
+  OFD_CHECK(field >= 0 && field < OFD_NFIELDS, OFD_ERR_ARG, "unknown field");
+  Ctx &c = x->c;
+  int rc = need_mesh(c);
+  if (rc) return rc;
+  OFD_HIP(hipSetDevice(c.device));
+  if (field == OFD_SOLU || field == OFD_SOLP)
+    OFD_HIP(launch_node4_to_soa(c, c.d_x, c.d_field[OFD_SOLU], c.d_field[OFD_SOLP]));
+  return d2h(c, host, c.d_field[field], (size_t)field_len(c, field) * sizeof(double));
+}
+
+int ofd_assemble_ns(ofd_ctx *x, int tensor_flag, int subdomain_flag, double *cfl) {
+  OFD_CHECK(x, OFD_ERR_ARG, "null context");
+  Ctx &c = x->c;
+  int rc = need_pattern(c);
+  if (rc) return rc;
+  OFD_CHECK((tensor_flag & ~(OFD_ASSEMBLE_VEC | OFD_ASSEMBLE_MAT)) == 0, OFD_ERR_ARG,
+            "tensor_flag must combine OFD_ASSEMBLE_VEC / OFD_ASSEMBLE_MAT");
+  OFD_HIP(hipSetDevice(c.device));
+  OFD_HIP(launch_alpha_stage(c));
+  if (tensor_flag & OFD_ASSEMBLE_VEC) OFD_HIP(launch_zero(c, c.d_rhs, (size_t)c.nnode * 4));
+  if (tensor_flag & OFD_ASSEMBLE_MAT) {
+    OFD_HIP(launch_zero(c, c.d_A64, (size_t)c.icnt * 16));
+    c.mat_valid = true;
+    c.prec_valid = c.a32_valid = false;
+  }
+  OFD_HIP(hipMemsetAsync(c.d_negdet, 0, sizeof(int32_t), c.stream));
+  for (int col = 0; col + 1 < (int)c.color_off.size(); ++col)
+    OFD_HIP(launch_assemble_ns_hex(c, tensor_flag, subdomain_flag, col));
+  double *d_out = c.d_part;  // scratch for the CFL maximum
+  OFD_HIP(launch_max_reduce(c, c.d_cfl, c.nelem, d_out));
+  double h[1] = {0.0};
+  int32_t neg = 0;
+  OFD_HIP(hipMemcpyAsync(h, d_out, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  OFD_HIP(hipMemcpyAsync(&neg, c.d_negdet, sizeof(int32_t), hipMemcpyDeviceToHost,
+                         c.stream));
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  if (cfl) *cfl = h[0];
+  if (neg > 0) {
+    set_error("negative determinant Jacobian at " + std::to_string(neg) +
+              " Gauss points (eval_shape_fem warning)");
+    return OFD_ERR_NEGDETJ;
+  }
+  return OFD_OK;
+}
+
+int ofd_get_rhs(ofd_ctx *x, double *rhsu, double *rhsp) {
+  OFD_CHECK(x && rhsu && rhsp, OFD_ERR_ARG, "null argument");
+  Ctx &c = x->c;
+  int rc = need_mesh(c);
+  if (rc) return rc;
+  OFD_HIP(hipSetDevice(c.device));
+  OFD_HIP(launch_node4_to_soa(c, c.d_rhs, c.d_field[OFD_SOLU], c.d_field[OFD_SOLP]));
+  rc = d2h(c, rhsu, c.d_field[OFD_SOLU], (size_t)c.nnode * 3 * sizeof(double));
+  if (!rc) rc = d2h(c, rhsp, c.d_field[OFD_SOLP], (size_t)c.nnode * sizeof(double));
+  // restore the solution view of the scratch fields
+  OFD_HIP(launch_node4_to_soa(c, c.d_x, c.d_field[OFD_SOLU], c.d_field[OFD_SOLP]));
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  return rc;
+}
+
+int ofd_set_rhs(ofd_ctx *x, const double *rhsu, const double *rhsp) {
+  OFD_CHECK(x && rhsu && rhsp, OFD_ERR_ARG, "null argument");
+  Ctx &c = x->c;
+  int rc = need_mesh(c);
+  if (rc) return rc;
+  OFD_HIP(hipSetDevice(c.device));
+  rc = h2d(c, c.d_r, rhsu, (size_t)c.nnode * 3 * sizeof(double));
+  if (rc) return rc;
+  std::vector<double> tmp;  // p goes through the b buffer
+  rc = h2d(c, c.d_b, rhsp, (size_t)c.nnode * sizeof(double));
+  if (rc) return rc;
+  OFD_HIP(launch_soa_to_node4(c, c.d_r, c.d_b, c.d_rhs));
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  return OFD_OK;
+}
+
+int ofd_get_lhs(ofd_ctx *x, double *k11, double *g, double *d1, double *m) {
+  OFD_CHECK(x && k11 && g && d1 && m, OFD_ERR_ARG, "null argument");
+  Ctx &c = x->c;
+  int rc = need_pattern(c);
+  if (rc) return rc;
+  OFD_HIP(hipSetDevice(c.device));
+  std::vector<double> A((size_t)c.icnt * 16);
+  rc = d2h(c, A.data(), c.d_A64, A.size() * sizeof(double));
+  if (rc) return rc;
+  for (int64_t k = 0; k < c.icnt; ++k) {
+    const double *b = &A[(size_t)k * 16];
+    for (int r = 0; r < 3; ++r) {
+      for (int cc = 0; cc < 3; ++cc) k11[(size_t)k * 9 + r * 3 + cc] = b[r * 4 + cc];
+      g[(size_t)k * 3 + r] = b[r * 4 + 3];
+      d1[(size_t)k * 3 + r] = b[12 + r];
+    }
+    m[k] = b[15];
+  }
+  return OFD_OK;
+}
+
+int ofd_set_lhs(ofd_ctx *x, const double *k11, const double *g, const double *d1,
+                const double *m) {
+  OFD_CHECK(x && k11 && g && d1 && m, OFD_ERR_ARG, "null argument");
+  Ctx &c = x->c;
+  int rc = need_pattern(c);
+  if (rc) return rc;
+  OFD_HIP(hipSetDevice(c.device));
+  std::vector<double> A((size_t)c.icnt * 16);
+  for (int64_t k = 0; k < c.icnt; ++k) {
+    double *b = &A[(size_t)k * 16];
+    for (int r = 0; r < 3; ++r) {
+      for (int cc = 0; cc < 3; ++cc) b[r * 4 + cc] = k11[(size_t)k * 9 + r * 3 + cc];
+      b[r * 4 + 3] = g[(size_t)k * 3 + r];
+      b[12 + r] = d1[(size_t)k * 3 + r];
+    }
+    b[15] = m[k];
+  }
+  rc = h2d(c, c.d_A64, A.data(), A.size() * sizeof(double));
+  if (rc) return rc;
+  c.mat_valid = true;
+  c.prec_valid = c.a32_valid = false;
+  return OFD_OK;
+}
+
+int ofd_spmv_up(ofd_ctx *x, int precision, const double *xu, const double *xp,
+                double *yu, double *yp) {
+  OFD_CHECK(x && xu && xp && yu && yp, OFD_ERR_ARG, "null argument");
+  OFD_CHECK(precision == OFD_FP64 || precision == OFD_FP32, OFD_ERR_ARG, "bad precision");
+  Ctx &c = x->c;
+  int rc = need_pattern(c);
+  if (rc) return rc;
+  OFD_CHECK(c.mat_valid, OFD_ERR_ARG, "no assembled matrix");
+  OFD_HIP(hipSetDevice(c.device));
+  if (precision == OFD_FP32) {
+    rc = ensure_prec(c, OFD_FP32);
+    if (rc) return rc;
+  }
+  double *su = c.d_field[OFD_SOLU], *sp = c.d_field[OFD_SOLP];
+  rc = h2d(c, su, xu, (size_t)c.nnode * 3 * sizeof(double));
+  if (!rc) rc = h2d(c, sp, xp, (size_t)c.nnode * sizeof(double));
+  if (rc) return rc;
+  OFD_HIP(launch_soa_to_node4(c, su, sp, c.d_b));
+  OFD_HIP(launch_spmv(c, precision, c.d_b, c.d_r, nullptr, 0));
+  OFD_HIP(launch_node4_to_soa(c, c.d_r, su, sp));
+  rc = d2h(c, yu, su, (size_t)c.nnode * 3 * sizeof(double));
+  if (!rc) rc = d2h(c, yp, sp, (size_t)c.nnode * sizeof(double));
+  OFD_HIP(launch_node4_to_soa(c, c.d_x, su, sp));
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  return rc;
+}
+
+int ofd_gmres_up(ofd_ctx *x, int precision, int ortho, double tol, int kspace,
+                 int kspace_min, double *solu, double *solp, int *iters, double *resid) {
+  OFD_CHECK(x, OFD_ERR_ARG, "null context");
+  OFD_CHECK(precision == OFD_FP64 || precision == OFD_FP32, OFD_ERR_ARG, "bad precision");
+  OFD_CHECK(ortho == OFD_ORTHO_MGS || ortho == OFD_ORTHO_CGS2, OFD_ERR_ARG, "bad ortho");
+  Ctx &c = x->c;
+  int rc = need_pattern(c);
+  if (rc) return rc;
+  OFD_HIP(hipSetDevice(c.device));
+  OFD_HIP(hipMemsetAsync(c.d_x, 0, (size_t)c.nnode * 4 * sizeof(double), c.stream));
+  int it = 0;
+  double res = 0.0;
+  rc = gmres_cycle(c, precision, ortho, c.d_rhs, tol, kspace, kspace_min, c.d_x, &it,
+                   &res, nullptr);
+  if (iters) *iters = it;
+  if (resid) *resid = res;
+  if (rc) return rc;
+  OFD_HIP(launch_node4_to_soa(c, c.d_x, c.d_field[OFD_SOLU], c.d_field[OFD_SOLP]));
+  if (solu) rc = d2h(c, solu, c.d_field[OFD_SOLU], (size_t)c.nnode * 3 * sizeof(double));
+  if (!rc && solp) rc = d2h(c, solp, c.d_field[OFD_SOLP], (size_t)c.nnode * sizeof(double));
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  return rc;
+}
+
+int ofd_solve(ofd_ctx *x, int precision, int ortho, double rtol, double inner_tol,
+              int kspace, int max_cycles, int *iters, int *cycles, double *relres) {
+  OFD_CHECK(x, OFD_ERR_ARG, "null context");
+  OFD_CHECK(precision == OFD_FP64 || precision == OFD_FP32, OFD_ERR_ARG, "bad precision");
+  OFD_CHECK(max_cycles >= 1, OFD_ERR_ARG, "max_cycles >= 1");
+  Ctx &c = x->c;
+  int rc = need_pattern(c);
+  if (rc) return rc;
+  OFD_HIP(hipSetDevice(c.device));
+  OFD_HIP(hipMemsetAsync(c.d_x, 0, (size_t)c.nnode * 4 * sizeof(double), c.stream));
+  double bnorm = 0.0;
+  rc = norm2_dev(c, c.d_rhs, &bnorm);
+  if (rc) return rc;
+  int total = 0, cyc = 0;
+  double rel = 0.0;
+  if (!(bnorm > 0.0)) {
+    if (iters) *iters = 0;
+    if (cycles) *cycles = 0;
+    if (relres) *relres = 0.0;
+    return std::isfinite(bnorm) ? OFD_OK : OFD_ERR_NONFINITE;
+  }
+  const double *rhs = c.d_rhs;
+  for (cyc = 1; cyc <= max_cycles; ++cyc) {
+    int it = 0;
+    rc = gmres_cycle(c, precision, ortho, rhs, inner_tol, kspace, 0, c.d_x, &it, nullptr,
+                     nullptr);
+    total += it;
+    if (rc) break;
+    // FP64 outer residual r = b - A x
+    OFD_HIP(launch_spmv(c, OFD_FP64, c.d_x, c.d_r, c.d_rhs, 1));
+    double rn = 0.0;
+    rc = norm2_dev(c, c.d_r, &rn);
+    if (rc) break;
+    rel = rn / bnorm;
+    rhs = c.d_r;
+    if (!std::isfinite(rel)) {
+      rc = OFD_ERR_NONFINITE;
+      set_error("solve: non-finite residual");
+      break;
+    }
+    if (rel <= rtol) break;
+  }
+  if (cyc > max_cycles) cyc = max_cycles;
+  if (iters) *iters = total;
+  if (cycles) *cycles = cyc;
+  if (relres) *relres = rel;
+  if (rc) return rc;
+  OFD_HIP(launch_node4_to_soa(c, c.d_x, c.d_field[OFD_SOLU], c.d_field[OFD_SOLP]));
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  if (rel > rtol) {
+    set_error("solve: tolerance not reached within max_cycles");
+    return OFD_ERR_NOCONV;
+  }
+  return OFD_OK;
+}
+
+int ofd_newton_update(ofd_ctx *x) {
+  OFD_CHECK(x, OFD_ERR_ARG, "null context");
+  Ctx &c = x->c;
+  int rc = need_mesh(c);
+  if (rc) return rc;
+  OFD_HIP(hipSetDevice(c.device));
+  OFD_HIP(launch_axpy_fields(c, 0));
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  return OFD_OK;
+}
+
+int ofd_predictor(ofd_ctx *x) {
+  OFD_CHECK(x, OFD_ERR_ARG, "null context");
+  Ctx &c = x->c;
+  int rc = need_mesh(c);
+  if (rc) return rc;
+  OFD_HIP(hipSetDevice(c.device));
+  OFD_HIP(launch_axpy_fields(c, 1));
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  return OFD_OK;
+}
+
+int ofd_advance(ofd_ctx *x) {
+  OFD_CHECK(x, OFD_ERR_ARG, "null context");
+  Ctx &c = x->c;
+  int rc = need_mesh(c);
+  if (rc) return rc;
+  OFD_HIP(hipSetDevice(c.device));
+  OFD_HIP(launch_axpy_fields(c, 2));
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  return OFD_OK;
+}
+
+int ofd_rhs_norms(ofd_ctx *x, double *norms2) {
+  OFD_CHECK(x && norms2, OFD_ERR_ARG, "null argument");
+  Ctx &c = x->c;
+  int rc = need_mesh(c);
+  if (rc) return rc;
+  OFD_HIP(hipSetDevice(c.device));
+  std::vector<double> r((size_t)c.nnode * 4);
+  rc = d2h(c, r.data(), c.d_rhs, r.size() * sizeof(double));
+  if (rc) return rc;
+  double su = 0.0, sp = 0.0;
+  for (int i = 0; i < c.nnode; ++i) {
+    su += r[(size_t)i * 4] * r[(size_t)i * 4] + r[(size_t)i * 4 + 1] * r[(size_t)i * 4 + 1] +
+          r[(size_t)i * 4 + 2] * r[(size_t)i * 4 + 2];
+    sp += r[(size_t)i * 4 + 3] * r[(size_t)i * 4 + 3];
+  }
+  norms2[0] = std::sqrt(su);
+  norms2[1] = std::sqrt(sp);
+  return OFD_OK;
+}
+
+int ofd_time_spmv(ofd_ctx *x, int precision, int reps, double *ms) {
+  OFD_CHECK(x && ms && reps > 0, OFD_ERR_ARG, "bad argument");
+  Ctx &c = x->c;
+  int rc = need_pattern(c);
+  if (rc) return rc;
+  OFD_CHECK(c.mat_valid, OFD_ERR_ARG, "no assembled matrix");
+  OFD_HIP(hipSetDevice(c.device));
+  if (precision == OFD_FP32) {
+    rc = ensure_prec(c, OFD_FP32);
+    if (rc) return rc;
+  }
+  hipEvent_t e0, e1;
+  OFD_HIP(hipEventCreate(&e0));
+  OFD_HIP(hipEventCreate(&e1));
+  OFD_HIP(launch_spmv(c, precision, c.d_x, c.d_r, nullptr, 0));  // warm
+  OFD_HIP(hipEventRecord(e0, c.stream));
+  for (int r = 0; r < reps; ++r) OFD_HIP(launch_spmv(c, precision, c.d_x, c.d_r, nullptr, 0));
+  OFD_HIP(hipEventRecord(e1, c.stream));
+  OFD_HIP(hipEventSynchronize(e1));
+  float t = 0.f;
+  OFD_HIP(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = (double)t / reps;
+  return OFD_OK;
+}
+
+int ofd_time_assemble(ofd_ctx *x, int tensor_flag, int reps, double *ms) {
+  OFD_CHECK(x && ms && reps > 0, OFD_ERR_ARG, "bad argument");
+  Ctx &c = x->c;
+  int rc = need_pattern(c);
+  if (rc) return rc;
+  OFD_HIP(hipSetDevice(c.device));
+  hipEvent_t e0, e1;
+  OFD_HIP(hipEventCreate(&e0));
+  OFD_HIP(hipEventCreate(&e1));
+  OFD_HIP(hipEventRecord(e0, c.stream));
+  for (int r = 0; r < reps; ++r) {
+    OFD_HIP(launch_alpha_stage(c));
+    if (tensor_flag & OFD_ASSEMBLE_VEC) OFD_HIP(launch_zero(c, c.d_rhs, (size_t)c.nnode * 4));
+    if (tensor_flag & OFD_ASSEMBLE_MAT) OFD_HIP(launch_zero(c, c.d_A64, (size_t)c.icnt * 16));
+    for (int col = 0; col + 1 < (int)c.color_off.size(); ++col)
+      OFD_HIP(launch_assemble_ns_hex(c, tensor_flag, 3, col));
+  }
+  OFD_HIP(hipEventRecord(e1, c.stream));
+  OFD_HIP(hipEventSynchronize(e1));
+  float t = 0.f;
+  OFD_HIP(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (tensor_flag & OFD_ASSEMBLE_MAT) {
+    c.mat_valid = true;
+    c.prec_valid = c.a32_valid = false;
+  }
+  *ms = (double)t / reps;
+  return OFD_OK;
+}
+
+int ofd_ctx_bytes(ofd_ctx *x, int64_t *bytes) {
+  OFD_CHECK(x && bytes, OFD_ERR_ARG, "null argument");
+  *bytes = x->c.dev_bytes;
+  return OFD_OK;
+}
+
+}  // extern "C"

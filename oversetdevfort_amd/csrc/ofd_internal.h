@@ -1,0 +1,151 @@
+// Internal definitions of the gfx950 OversetDevFort hot-path library.
+//
+// Device data layout (HBM):
+//   node fields        SoA [3][N] doubles (the reference's column-major layout)
+//   node record        [N][16] doubles: x+d (3), u (3), ac (3), um (3), p,
+//                      pad (3) -- the alpha-stage gather unit of the element
+//                      kernels, one 128-byte line per node
+//   system matrix      block CSR in the reference's nonzero order k:
+//                      A[k][16] row-major 4x4 = [K11 G; D1 M]
+//                      (double, plus a float copy for the mixed solver)
+//   preconditioner     [N][16] 4x4 block-diagonal inverse [Binv 0; 0 1/M]
+//   vectors            [N][4] interleaved (u, v, w, p) doubles
+//   scatter map        emap[e][64]: nonzero index of pair (a, b), lane = a + 8 b
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/ofd.h"
+
+namespace ofd {
+
+void set_error(const std::string &msg);
+
+#define OFD_HIP(call)                                                    \
+  do {                                                                   \
+    hipError_t _e = (call);                                              \
+    if (_e != hipSuccess) {                                              \
+      ::ofd::set_error(std::string(#call) + ": " + hipGetErrorString(_e)); \
+      return OFD_ERR_HIP;                                                \
+    }                                                                    \
+  } while (0)
+
+#define OFD_CHECK(cond, code, msg) \
+  do {                             \
+    if (!(cond)) {                 \
+      ::ofd::set_error(msg);       \
+      return code;                 \
+    }                              \
+  } while (0)
+
+constexpr int kMaxKspace = 64;     // GMRES restart length upper bound
+constexpr int kRedBlock = 256;     // threads per reduction block
+constexpr int kMaxRedBlocks = 2048;
+
+// per-solve scalars living on the device (one GMRES cycle)
+struct GmresState {
+  double H[(kMaxKspace + 1) * kMaxKspace];  // column-major (K+1) x K
+  double e[kMaxKspace + 1];
+  double rc[kMaxKspace], rs[kMaxKspace];
+  double s[kMaxKspace + 1];   // scale of stored basis vector j (v_j = s_j W_j)
+  double h1[kMaxKspace + 1], h2[kMaxKspace + 1];
+  double y[kMaxKspace];
+  double beta;                // running MGS projection coefficient
+  double eps;                 // tol * |P^-1 b|
+  double rnorm0;
+  double resid;               // |e_{k+1}| at exit
+  double scal[4];             // misc reductions (norms)
+  int32_t iters;
+  int32_t done;
+  int32_t kmin;
+  int32_t kspace;
+  uint32_t counter;           // last-block-done ticket
+  uint32_t pad;
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // mesh
+  int nnode = 0, nelem = 0, nshl = 0;
+  std::vector<int32_t> h_ien;   // [nshl][nelem] 0-based
+  std::vector<int32_t> h_elm_id;
+  int32_t *d_ien = nullptr;
+  int32_t *d_elm_id = nullptr;
+  // pattern
+  int64_t icnt = 0;
+  std::vector<int32_t> h_col, h_row;  // 0-based
+  int32_t *d_col = nullptr, *d_row = nullptr;
+  int32_t *d_emap = nullptr;          // [E][64]
+  int32_t *d_color_elems = nullptr;   // elements grouped by colour
+  std::vector<int32_t> color_off;     // ncolor+1
+  int32_t *d_diag = nullptr;          // nonzero index of the diagonal block per row
+  // parameters
+  double prm[OFD_NPARAMS] = {0};
+  int32_t elm_id_set[2] = {1, 2};
+  // state
+  double *d_field[OFD_NFIELDS] = {nullptr};
+  uint8_t *d_ibc = nullptr;           // bit c set: IBC(node, c+1) == 1
+  double *d_bcval = nullptr;          // [3][N] Dirichlet velocity values
+  double *d_noderec = nullptr;        // [N][16]
+  // system
+  double *d_A64 = nullptr;            // [icnt][16]
+  float *d_A32 = nullptr;
+  double *d_P64 = nullptr;            // [N][16]
+  float *d_P32 = nullptr;
+  bool mat_valid = false, prec_valid = false, a32_valid = false;
+  double *d_rhs = nullptr;            // [N][4]
+  double *d_b = nullptr;              // [N][4] right-hand side of a solve
+  double *d_x = nullptr;              // [N][4] solution accumulator
+  double *d_r = nullptr;              // [N][4] outer residual
+  double *d_cfl = nullptr;            // [E] per-element max CFL
+  int32_t *d_negdet = nullptr;        // count of negative Jacobians
+  // GMRES workspace
+  int kalloc = 0;
+  double *d_W = nullptr;              // [(kalloc+1)][N][4]
+  double *d_part = nullptr;           // [kMaxRedBlocks][kMaxKspace + 1]
+  GmresState *d_gs = nullptr;
+  GmresState *h_gs = nullptr;         // pinned mirror
+  double *h_pin = nullptr;            // pinned scratch (host transfers)
+  size_t h_pin_bytes = 0;
+  int64_t dev_bytes = 0;
+};
+
+// ---- launchers (defined in the .hip files) ----
+// assembly
+hipError_t launch_alpha_stage(Ctx &c);
+hipError_t launch_assemble_ns_hex(Ctx &c, int tensor_flag, int subdomain_flag,
+                                  int color);
+hipError_t launch_zero(Ctx &c, double *p, size_t n);
+hipError_t launch_max_reduce(Ctx &c, const double *v, int n, double *out_dev);
+// matrix
+hipError_t launch_build_prec(Ctx &c, bool want32);
+hipError_t launch_convert_a32(Ctx &c);
+hipError_t launch_spmv(Ctx &c, int precision, const double *x, double *y,
+                       const double *b, int mode);
+hipError_t launch_spmv_prec(Ctx &c, int precision, int k);
+// vectors
+hipError_t launch_soa_to_node4(Ctx &c, const double *u, const double *p,
+                               double *out);
+hipError_t launch_node4_to_soa(Ctx &c, const double *in, double *u, double *p);
+hipError_t launch_axpy_fields(Ctx &c, int which);
+hipError_t launch_apply_dirichlet(Ctx &c);
+// gmres pieces
+hipError_t launch_gmres_init(Ctx &c, int precision, const double *b,
+                             double tol, int kspace, int kmin);
+hipError_t launch_mgs_step(Ctx &c, int k, int j);
+hipError_t launch_cgs_pass(Ctx &c, int k, int pass);
+hipError_t launch_gmres_finish(Ctx &c, double *x);
+hipError_t launch_norm2(Ctx &c, const double *v, int slot);
+
+int nblocks_nodes(const Ctx &c);
+
+}  // namespace ofd
+
+struct ofd_ctx {
+  ofd::Ctx c;
+};

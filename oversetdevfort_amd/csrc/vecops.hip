@@ -1,0 +1,122 @@
+// Node-vector kernels: layout conversion between the reference's SoA node
+// fields and the interleaved [N][4] vectors of the solver, and the
+// generalized-alpha field updates of the time/Newton drivers.
+#include "ofd_internal.h"
+
+namespace ofd {
+
+// (NNODE,3) + (NNODE) column-major -> [N][4]
+__global__ void k_soa_to_node4(int N, const double *__restrict__ u,
+                               const double *__restrict__ p, double *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double2 *o = reinterpret_cast<double2 *>(out + (size_t)i * 4);
+  o[0] = make_double2(u[i], u[(size_t)N + i]);
+  o[1] = make_double2(u[2 * (size_t)N + i], p ? p[i] : 0.0);
+}
+
+__global__ void k_node4_to_soa(int N, const double *__restrict__ in,
+                               double *__restrict__ u, double *__restrict__ p) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double2 *s = reinterpret_cast<const double2 *>(in + (size_t)i * 4);
+  const double2 a = s[0], b = s[1];
+  u[i] = a.x;
+  u[(size_t)N + i] = a.y;
+  u[2 * (size_t)N + i] = b.x;
+  if (p) p[i] = b.y;
+}
+
+// which: 0 Newton update (solveOverset.f90:76-78), 1 predictor
+// (driver.f90:83-85), 2 old <- new (driver.f90:98-103)
+__global__ void k_fields(int N, int which, double gami, double delt, double alfi,
+                         const double *__restrict__ x, double *ug, double *ugold,
+                         double *acg, double *acgold, double *pg, double *pgold,
+                         double *dg, double *dgold, double *ugm, double *ugmold,
+                         double *acgm, double *acgmold) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  if (which == 0) {
+    const double fu = gami * delt;
+    const double fp = alfi * gami * delt;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const size_t o = (size_t)c * N + i;
+      const double d = x[(size_t)i * 4 + c];
+      acg[o] = acg[o] + d;
+      ug[o] = ug[o] + fu * d;
+    }
+    pg[i] = pg[i] + fp * x[(size_t)i * 4 + 3];
+  } else if (which == 1) {
+    const double f = (gami - 1.0) / gami;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const size_t o = (size_t)c * N + i;
+      ug[o] = ugold[o];
+      acg[o] = f * acgold[o];
+    }
+    pg[i] = pgold[i];
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const size_t o = (size_t)c * N + i;
+      ugold[o] = ug[o];
+      acgold[o] = acg[o];
+      dgold[o] = dg[o];
+      ugmold[o] = ugm[o];
+      acgmold[o] = acgm[o];
+    }
+    pgold[i] = pg[i];
+  }
+}
+
+// setBCs_CFD (setBCs.f90:84-91), value part
+__global__ void k_apply_dirichlet(int N, const uint8_t *__restrict__ ibc,
+                                  const double *__restrict__ val, double dtgl,
+                                  double gami, double *ug, const double *ugold,
+                                  double *acg, const double *acgold) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const unsigned m = ibc[i];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if ((m >> c) & 1u) {
+      const size_t o = (size_t)c * N + i;
+      ug[o] = val[o];
+      acg[o] = ((ug[o] - ugold[o]) * dtgl + (gami - 1.0) * acgold[o]) / gami;
+    }
+  }
+}
+
+hipError_t launch_apply_dirichlet(Ctx &c) {
+  double **f = c.d_field;
+  hipLaunchKernelGGL(k_apply_dirichlet, dim3((c.nnode + 255) / 256), dim3(256), 0,
+                     c.stream, c.nnode, c.d_ibc, c.d_bcval, 1.0 / c.prm[OFD_P_DELT],
+                     c.prm[OFD_P_GAMI], f[OFD_UG], f[OFD_UGOLD], f[OFD_ACG],
+                     f[OFD_ACGOLD]);
+  return hipGetLastError();
+}
+
+hipError_t launch_soa_to_node4(Ctx &c, const double *u, const double *p, double *out) {
+  hipLaunchKernelGGL(k_soa_to_node4, dim3((c.nnode + 255) / 256), dim3(256), 0,
+                     c.stream, c.nnode, u, p, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_node4_to_soa(Ctx &c, const double *in, double *u, double *p) {
+  hipLaunchKernelGGL(k_node4_to_soa, dim3((c.nnode + 255) / 256), dim3(256), 0,
+                     c.stream, c.nnode, in, u, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_axpy_fields(Ctx &c, int which) {
+  double **f = c.d_field;
+  hipLaunchKernelGGL(k_fields, dim3((c.nnode + 255) / 256), dim3(256), 0, c.stream,
+                     c.nnode, which, c.prm[OFD_P_GAMI], c.prm[OFD_P_DELT],
+                     c.prm[OFD_P_ALFI], c.d_x, f[OFD_UG], f[OFD_UGOLD], f[OFD_ACG],
+                     f[OFD_ACGOLD], f[OFD_PG], f[OFD_PGOLD], f[OFD_DG], f[OFD_DGOLD],
+                     f[OFD_UGM], f[OFD_UGMOLD], f[OFD_ACGM], f[OFD_ACGMOLD]);
+  return hipGetLastError();
+}
+
+}  // namespace ofd

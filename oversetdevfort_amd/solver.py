@@ -1,0 +1,250 @@
+"""Host-side mirror of the reference's hot-path routines over the C ABI.
+
+Method names follow the reference Fortran routines they replace, with the
+same argument meaning; arrays use the reference layouts ((NNODE,3) node
+fields, 0-based indices on the Python side).  All compute runs in libofd.so
+on the GPU; this module only moves arrays and sequences calls.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _soa(a):
+    """(N,3) -> column-major (the reference's (NNODE,3) memory order)."""
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64).T)
+
+
+class Context:
+    """One device context (module state of aAdjKeep/commonvars, on the GPU)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = L.load()
+        h = ctypes.c_void_p()
+        L.check(self.lib.ofd_ctx_create(device, ctypes.byref(h)))
+        self.h = h
+        self.nnode = self.nelem = self.nshl = 0
+        self.icnt = 0
+
+    def close(self):
+        if self.h:
+            self.lib.ofd_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- setup (input_mesh / genSparStruc / getparam / setBCs) ----
+    def upload_mesh(self, mesh):
+        ien = np.ascontiguousarray((np.asarray(mesh.ien, np.int32) + 1).T)
+        elm = np.ascontiguousarray(mesh.elm_id, dtype=np.int32)
+        xg = _soa(mesh.xg)
+        self.nnode, self.nelem, self.nshl = mesh.nnode, mesh.nelem, mesh.nshl
+        L.check(self.lib.ofd_upload_mesh(self.h, self.nnode, self.nelem, self.nshl,
+                                         _ptr(ien), _ptr(elm), _ptr(xg)))
+
+    def genSparStruc(self):
+        """genSparStruc.f90:4-47 -> (col, row) 0-based."""
+        n = ctypes.c_int64()
+        L.check(self.lib.ofd_build_pattern(self.h, ctypes.byref(n)))
+        self.icnt = int(n.value)
+        col = np.zeros(self.nnode + 1, np.int32)
+        row = np.zeros(self.icnt, np.int32)
+        L.check(self.lib.ofd_get_pattern(self.h, _ptr(col), _ptr(row)))
+        return col - 1, row - 1
+
+    def build_pattern(self):
+        n = ctypes.c_int64()
+        L.check(self.lib.ofd_build_pattern(self.h, ctypes.byref(n)))
+        self.icnt = int(n.value)
+        return self.icnt
+
+    def set_params(self, p):
+        """p: dict with the keys of _lib.PARAMS (grav may be given as 'grav')."""
+        q = dict(p)
+        if "grav" in q:
+            q["gx"], q["gy"], q["gz"] = q.pop("grav")
+        arr = np.array([q.get(k, 0.0) for k in L.PARAMS], dtype=np.float64)
+        L.check(self.lib.ofd_set_params(self.h, _ptr(arr)))
+
+    def set_subdomains(self, id1, id2):
+        L.check(self.lib.ofd_set_subdomains(self.h, int(id1), int(id2)))
+
+    def set_ibc(self, ibc):
+        """IBC (NNODE, >=4) int, columns u,v,w,p."""
+        a = np.ascontiguousarray(np.asarray(ibc, np.int32)[:, :4].T)
+        L.check(self.lib.ofd_set_ibc(self.h, _ptr(a), 4))
+
+    def set_bcvals(self, vals):
+        a = _soa(vals)
+        L.check(self.lib.ofd_set_bcvals(self.h, _ptr(a)))
+
+    def apply_dirichlet(self):
+        """setBCs_CFD value part (setBCs.f90:84-91) on the device."""
+        L.check(self.lib.ofd_apply_dirichlet(self.h))
+
+    def set_field(self, name, arr):
+        a = _f64(arr).ravel() if np.ndim(arr) == 1 else _soa(arr)
+        L.check(self.lib.ofd_set_field(self.h, L.FIELD[name.upper()], _ptr(a)))
+
+    def get_field(self, name):
+        name = name.upper()
+        if name in ("PG", "PGOLD", "SOLP"):
+            out = np.zeros(self.nnode)
+            L.check(self.lib.ofd_get_field(self.h, L.FIELD[name], _ptr(out)))
+            return out
+        out = np.zeros((3, self.nnode))
+        L.check(self.lib.ofd_get_field(self.h, L.FIELD[name], _ptr(out)))
+        return out.T.copy()
+
+    def set_state(self, fields: dict):
+        for k, v in fields.items():
+            self.set_field(k, v)
+
+    # ---- assembly (assembleNavSto.f90:4-72) ----
+    def assembleNavSto(self, assemble_tensor_flag=6, subdomain_flag=1):
+        cfl = ctypes.c_double()
+        L.check(self.lib.ofd_assemble_ns(self.h, int(assemble_tensor_flag),
+                                         int(subdomain_flag), ctypes.byref(cfl)))
+        return cfl.value
+
+    def get_rhs(self):
+        u = np.zeros((3, self.nnode))
+        p = np.zeros(self.nnode)
+        L.check(self.lib.ofd_get_rhs(self.h, _ptr(u), _ptr(p)))
+        return u.T.copy(), p
+
+    def set_rhs(self, rhsu, rhsp):
+        u = _soa(rhsu)
+        p = _f64(rhsp)
+        L.check(self.lib.ofd_set_rhs(self.h, _ptr(u), _ptr(p)))
+
+    def get_lhs(self):
+        k = np.zeros((self.icnt, 9))
+        g = np.zeros((self.icnt, 3))
+        d = np.zeros((self.icnt, 3))
+        m = np.zeros(self.icnt)
+        L.check(self.lib.ofd_get_lhs(self.h, _ptr(k), _ptr(g), _ptr(d), _ptr(m)))
+        return dict(k11=k, g=g, d1=d, m=m)
+
+    def set_lhs(self, A):
+        k, g, d, m = (_f64(A[n]) for n in ("k11", "g", "d1", "m"))
+        L.check(self.lib.ofd_set_lhs(self.h, _ptr(k), _ptr(g), _ptr(d), _ptr(m)))
+
+    # ---- linear algebra (SparseProd.f90, SparseGMRES_NS.f90) ----
+    def SparseProdUP_3D(self, xu, xp, precision=L.OFD_FP64):
+        xu_, xp_ = _soa(xu), _f64(xp)
+        yu = np.zeros((3, self.nnode))
+        yp = np.zeros(self.nnode)
+        L.check(self.lib.ofd_spmv_up(self.h, precision, _ptr(xu_), _ptr(xp_), _ptr(yu),
+                                     _ptr(yp)))
+        return yu.T.copy(), yp
+
+    def SparseGMRES_up(self, tol, kspace, kspace_min=0, precision=L.OFD_FP64,
+                       ortho=L.OFD_ORTHO_MGS):
+        su = np.zeros((3, self.nnode))
+        sp = np.zeros(self.nnode)
+        it = ctypes.c_int()
+        res = ctypes.c_double()
+        L.check(self.lib.ofd_gmres_up(self.h, precision, ortho, float(tol), int(kspace),
+                                      int(kspace_min), _ptr(su), _ptr(sp),
+                                      ctypes.byref(it), ctypes.byref(res)))
+        return dict(solu=su.T.copy(), solp=sp, iters=it.value, resid=res.value)
+
+    def solve(self, rtol=1e-8, precision=L.OFD_FP64, ortho=L.OFD_ORTHO_CGS2,
+              inner_tol=None, kspace=30, max_cycles=50):
+        if inner_tol is None:
+            inner_tol = 1e-4 if precision == L.OFD_FP32 else 1e-2
+        it = ctypes.c_int()
+        cyc = ctypes.c_int()
+        rel = ctypes.c_double()
+        L.check(self.lib.ofd_solve(self.h, precision, ortho, float(rtol), float(inner_tol),
+                                   int(kspace), int(max_cycles), ctypes.byref(it),
+                                   ctypes.byref(cyc), ctypes.byref(rel)))
+        return dict(iters=it.value, cycles=cyc.value, relres=rel.value)
+
+    # ---- time / Newton drivers ----
+    def newton_update(self):
+        L.check(self.lib.ofd_newton_update(self.h))
+
+    def predictor(self):
+        L.check(self.lib.ofd_predictor(self.h))
+
+    def advance(self):
+        L.check(self.lib.ofd_advance(self.h))
+
+    def rhs_norms(self):
+        out = (ctypes.c_double * 2)()
+        L.check(self.lib.ofd_rhs_norms(self.h, out))
+        return float(out[0]), float(out[1])
+
+    # ---- measurement ----
+    def time_spmv(self, precision=L.OFD_FP64, reps=20):
+        ms = ctypes.c_double()
+        L.check(self.lib.ofd_time_spmv(self.h, precision, int(reps), ctypes.byref(ms)))
+        return ms.value
+
+    def time_assemble(self, tensor_flag=6, reps=5):
+        ms = ctypes.c_double()
+        L.check(self.lib.ofd_time_assemble(self.h, int(tensor_flag), int(reps),
+                                           ctypes.byref(ms)))
+        return ms.value
+
+    def device_bytes(self):
+        b = ctypes.c_int64()
+        L.check(self.lib.ofd_ctx_bytes(self.h, ctypes.byref(b)))
+        return b.value
+
+
+def setup(prob, device=0) -> Context:
+    """Context holding a configs.Problem: mesh, pattern, parameters, Dirichlet
+    flags and values, initial fields."""
+    ctx = Context(device)
+    ctx.upload_mesh(prob.mesh)
+    ctx.build_pattern()
+    ctx.set_params(prob.params)
+    ctx.set_ibc(prob.ibc)
+    ctx.set_bcvals(prob.bcval)
+    ctx.set_state(prob.fields)
+    return ctx
+
+
+def begin_step(ctx: Context):
+    """Predictor (driver.f90:83-85) then Dirichlet values (setBCs_CFD)."""
+    ctx.predictor()
+    ctx.apply_dirichlet()
+
+
+def newton_step(ctx: Context, rtol=1e-8, precision=L.OFD_FP64, ortho=L.OFD_ORTHO_CGS2,
+                kspace=30, inner_tol=None, max_cycles=50):
+    """One Newton iteration of solveOverset_stag for one block
+    (solveOverset.f90:55-121): assemble VEC+MAT, solve to rtol, update,
+    re-assemble VEC and return the residual norms."""
+    ctx.assembleNavSto(6, 1)
+    info = ctx.solve(rtol=rtol, precision=precision, ortho=ortho, kspace=kspace,
+                     inner_tol=inner_tol, max_cycles=max_cycles)
+    ctx.newton_update()
+    ctx.assembleNavSto(2, 1)
+    info["res"] = ctx.rhs_norms()
+    return info
