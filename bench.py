@@ -41,7 +41,7 @@ def spmv_bytes(Z, N, s_a, s_x=8, s_y=8, b=4, S=16):
     return Z * (S * s_a + 4) + 4 * (N + 1) + N * b * (s_x + s_y)
 
 
-def run_steps(ctx, S, L, prob, precision, steps, warmup):
+def run_steps(ctx, S, L, prob, precision, steps, warmup, kspace=30):
     """Reset to the initial state, run warmup + timed time steps."""
     ctx.set_state(prob.fields)
     newton_ms, step_ms, iters, cycles, rel, res = [], [], [], [], [], []
@@ -50,7 +50,11 @@ def run_steps(ctx, S, L, prob, precision, steps, warmup):
         S.begin_step(ctx)
         t1 = time.perf_counter()
         info = S.newton_step(ctx, rtol=1e-8, precision=precision,
-                             ortho=L.OFD_ORTHO_CGS2, kspace=30, max_cycles=60)
+                             ortho=L.OFD_ORTHO_CGS2, kspace=kspace, max_cycles=400,
+                             raise_noconv=False)
+        if not info["converged"]:
+            h = ctx.solve_history()
+            log(f"  not converged: relres by cycle {['%.1e' % v for v in h[::10]]}")
         t2 = time.perf_counter()
         ctx.advance()
         t3 = time.perf_counter()
@@ -71,11 +75,13 @@ def run_steps(ctx, S, L, prob, precision, steps, warmup):
                 relres=max(rel), res_u=[r[0] for r in res])
 
 
-def cpu_baseline(prob, budget_s=25.0):
-    """One FP64 Newton step of the same workload on the host, single thread,
+def cpu_baseline(prob, budget_s=15.0):
+    """FP64 Newton step of the same workload on the host, single thread,
     through the C oracle (loop-for-loop restatement of the reference):
-    assembly VEC+MAT, restarted MGS GMRES(30) to |b-Ax| <= 1e-8 |b|, update,
-    VEC re-assembly and norm."""
+    assembly VEC+MAT, restarted MGS GMRES(30) cycles to |b-Ax| <= 1e-8 |b|,
+    update, VEC re-assembly and norm.  The solve is bounded to ``budget_s``;
+    the per-iteration cost of the cycles run is returned so the caller can
+    scale to the iteration count the full solve takes."""
     import numpy as np
 
     from oracle import oracle as O
@@ -127,7 +133,8 @@ def cpu_baseline(prob, budget_s=25.0):
     np.sqrt((R2["rhsu"] ** 2).sum())
     t3 = time.perf_counter()
     return dict(newton_s=t3 - t0, assemble_s=t1 - t0, solve_s=t2 - t1,
-                gmres_iters=its, relres=float(rel), converged=bool(rel <= 1e-8))
+                vec_s=t3 - t2, gmres_iters=its, s_per_iter=(t2 - t1) / max(its, 1),
+                relres=float(rel), converged=bool(rel <= 1e-8))
 
 
 def main():
@@ -187,18 +194,27 @@ def main():
     if not args.no_cpu:
         log("bench: host baseline (C oracle, 1 thread) ...")
         c = cpu_baseline(prob)
-        log(f"bench: host Newton step {c['newton_s']:.2f} s (assembly {c['assemble_s']:.2f} s, "
-            f"solve {c['solve_s']:.2f} s, {c['gmres_iters']} its, relres {c['relres']:.1e})")
-        cpu = {"value": E / c["newton_s"] / 1e6, "unit": "Melem/s (Newton step)",
+        # the full solve takes as many iterations as the same FP64 algorithm
+        # on the GPU (mean over the timed steps); scale the sampled cycles
+        need = sum(res64["gmres_iters"]) / len(res64["gmres_iters"])
+        solve_s = c["solve_s"] if c["converged"] else c["s_per_iter"] * need
+        newton_s = c["assemble_s"] + solve_s + c["vec_s"]
+        log(f"bench: host sample: assembly {c['assemble_s']:.2f} s, {c['gmres_iters']} "
+            f"GMRES its in {c['solve_s']:.2f} s ({1e3 * c['s_per_iter']:.1f} ms/it), "
+            f"VEC {c['vec_s']:.2f} s -> Newton step {newton_s:.2f} s")
+        cpu = {"value": E / newton_s / 1e6, "unit": "Melem/s (FP64 Newton step)",
                "cores": 1, "kind": "port",
-               "sample": (f"one FP64 Newton step of the same {nx}x{ny}x{nz} channel "
-                          f"workload through the C oracle (restatement of "
-                          f"IntElmAss_NS_overset/SparseProdUP_3D/SparseGMRES_up), "
-                          f"restarted GMRES(30) to 1e-8; newton {c['newton_s']:.2f} s, "
-                          f"assembly {c['assemble_s']:.2f} s, solve {c['solve_s']:.2f} s, "
-                          f"{c['gmres_iters']} iterations, relres {c['relres']:.1e}, "
+               "sample": (f"{nx}x{ny}x{nz} channel, host, 1 thread, C oracle "
+                          f"(restatement of IntElmAss_NS_overset + BCLhs_3D + "
+                          f"FillSparseMat_3D, SparseProdUP_3D, SparseGMRES_up): full "
+                          f"VEC+MAT assembly {c['assemble_s']:.2f} s, "
+                          f"{c['gmres_iters']} restarted-GMRES(30) iterations in "
+                          f"{c['solve_s']:.2f} s (relres {c['relres']:.1e}) scaled to "
+                          f"{need:.0f} iterations, VEC re-assembly {c['vec_s']:.2f} s; "
                           f"host {platform.processor() or platform.machine()}"),
-               "newton_ms": 1e3 * c["newton_s"], "assembly_ms": 1e3 * c["assemble_s"]}
+               "newton_ms": 1e3 * newton_s, "assembly_ms": 1e3 * c["assemble_s"],
+               "assembly_melem_per_s": E / c["assemble_s"] / 1e6,
+               "ms_per_gmres_iter": 1e3 * c["s_per_iter"]}
 
     out = {
         "metric": METRIC,

@@ -141,6 +141,10 @@ int ofd_solve(ofd_ctx *ctx, int precision, int ortho, double rtol,
               double inner_tol, int kspace, int max_cycles, int *iters,
               int *cycles, double *relres);
 
+/* relative FP64 residual after each cycle of the last ofd_solve; n gets the
+ * number of cycles (at most maxn values are written) */
+int ofd_solve_history(ofd_ctx *ctx, double *relres, int maxn, int *n);
+
 /* Newton update (solveOverset.f90:76-78): acg += du; ug += gami*Delt*du;
  * pg += alfi*gami*Delt*dp with du, dp the context solution */
 int ofd_newton_update(ofd_ctx *ctx);

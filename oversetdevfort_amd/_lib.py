@@ -60,6 +60,7 @@ SIGNATURES = {
                       _dp], _c_int),
     "ofd_solve": ([_vp, _c_int, _c_int, _c_dbl, _c_dbl, _c_int, _c_int, _ip, _ip,
                    _dp], _c_int),
+    "ofd_solve_history": ([_vp, _dp, _c_int, _ip], _c_int),
     "ofd_newton_update": ([_vp], _c_int),
     "ofd_predictor": ([_vp], _c_int),
     "ofd_advance": ([_vp], _c_int),

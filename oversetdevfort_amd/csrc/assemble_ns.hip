@@ -13,7 +13,9 @@
 //            shape data staged in LDS; RHS contributions reduced over q.
 //   phase B  lane = (a, b) block pair: the 16 entries of the 4x4 block (a,b)
 //            accumulated over the 8 Gauss points in registers.
-//   phase C  Dirichlet projection as row/column masks, then a deterministic
+//   phase C  Dirichlet projection as row/column masks (BCLhs_3D zeroes rows and
+//            columns of constrained dofs and sets their diagonal to 1, in an
+//            order that makes the masks exact), then a deterministic
 //            read-modify-write of the block at emap[e][lane]: elements are
 //            launched one colour at a time and no two elements of a colour
 //            share a node, so no atomics and a fixed summation order.
@@ -337,7 +339,11 @@ k_assemble_ns_hex(int N, int E, const int32_t *__restrict__ ien,
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
           const int r = (2 * t + h) >> 2, cc = (2 * t + h) & 3;
-          if (((ma >> r) & 1u) || ((mb >> cc) & 1u)) v[h] = 0.0;
+          // row r of node a and column cc of node b are projected away, except
+          // that a pressure constraint clears the G column but not the M
+          // column (BCLhs_3D.f90:105-118)
+          const bool colz = ((mb >> cc) & 1u) && !(r == 3 && cc == 3);
+          if (((ma >> r) & 1u) || colz) v[h] = 0.0;
           if (pa_ == pb_ && r == cc && ((ma >> r) & 1u)) v[h] = 1.0;
         }
         double2 old = blk[t];

@@ -680,6 +680,7 @@ int ofd_solve(ofd_ctx *x, int precision, int ortho, double rtol, double inner_to
     return std::isfinite(bnorm) ? OFD_OK : OFD_ERR_NONFINITE;
   }
   const double *rhs = c.d_rhs;
+  c.solve_hist.clear();
   for (cyc = 1; cyc <= max_cycles; ++cyc) {
     int it = 0;
     rc = gmres_cycle(c, precision, ortho, rhs, inner_tol, kspace, 0, c.d_x, &it, nullptr,
@@ -692,6 +693,7 @@ int ofd_solve(ofd_ctx *x, int precision, int ortho, double rtol, double inner_to
     rc = norm2_dev(c, c.d_r, &rn);
     if (rc) break;
     rel = rn / bnorm;
+    c.solve_hist.push_back(rel);
     rhs = c.d_r;
     if (!std::isfinite(rel)) {
       rc = OFD_ERR_NONFINITE;
@@ -711,6 +713,14 @@ int ofd_solve(ofd_ctx *x, int precision, int ortho, double rtol, double inner_to
     set_error("solve: tolerance not reached within max_cycles");
     return OFD_ERR_NOCONV;
   }
+  return OFD_OK;
+}
+
+int ofd_solve_history(ofd_ctx *x, double *relres, int maxn, int *n) {
+  OFD_CHECK(x && n, OFD_ERR_ARG, "null argument");
+  const std::vector<double> &h = x->c.solve_hist;
+  *n = (int)h.size();
+  for (int i = 0; i < maxn && i < (int)h.size(); ++i) relres[i] = h[i];
   return OFD_OK;
 }
 

@@ -113,6 +113,7 @@ struct Ctx {
   double *h_pin = nullptr;            // pinned scratch (host transfers)
   size_t h_pin_bytes = 0;
   int64_t dev_bytes = 0;
+  std::vector<double> solve_hist;     // relres after each cycle of the last solve
 };
 
 // ---- launchers (defined in the .hip files) ----

@@ -1,0 +1,238 @@
+!======================================================================
+! ISO_C_BINDING interfaces to libofd.so (include/ofd.h).
+!
+! Every entry returns an integer status (OFD_OK = 0).  Arrays are passed
+! in the reference's own layouts: IEN(NELEM,NSHL), IBC(NNODE,ncol),
+! node fields (NNODE,3), pg(NNODE), col(NNODE+1), row(icnt),
+! LHSK11(9,icnt), LHSG(3,icnt), LHSD1(3,icnt), LHSM(icnt).
+!======================================================================
+module ofd_iface
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: OFD_OK = 0, OFD_ERR_ARG = 1, OFD_ERR_HIP = 2, &
+                               OFD_ERR_NEGDETJ = 3, OFD_ERR_NONFINITE = 4, &
+                               OFD_ERR_NOCONV = 5
+  integer(c_int), parameter :: OFD_ASSEMBLE_VEC = 2, OFD_ASSEMBLE_MAT = 4
+  integer(c_int), parameter :: OFD_FP64 = 0, OFD_FP32 = 1
+  integer(c_int), parameter :: OFD_ORTHO_MGS = 0, OFD_ORTHO_CGS2 = 1
+  integer(c_int), parameter :: OFD_XG = 0, OFD_DG = 1, OFD_DGOLD = 2, &
+                               OFD_UG = 3, OFD_UGOLD = 4, OFD_UGM = 5, &
+                               OFD_UGMOLD = 6, OFD_ACG = 7, OFD_ACGOLD = 8, &
+                               OFD_ACGM = 9, OFD_ACGMOLD = 10, OFD_PG = 11, &
+                               OFD_PGOLD = 12, OFD_SOLU = 13, OFD_SOLP = 14
+  integer(c_int), parameter :: OFD_NPARAMS = 11
+
+  interface
+    function ofd_last_error() bind(C, name='ofd_last_error') result(p)
+      import :: c_ptr
+      type(c_ptr) :: p
+    end function
+
+    function ofd_ctx_create(device, ctx) bind(C, name='ofd_ctx_create') result(rc)
+      import :: c_int, c_ptr
+      integer(c_int), value :: device
+      type(c_ptr), intent(out) :: ctx
+      integer(c_int) :: rc
+    end function
+
+    function ofd_ctx_destroy(ctx) bind(C, name='ofd_ctx_destroy') result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+
+    function ofd_upload_mesh(ctx, nnode, nelem, nshl, ien, elm_id, xg) &
+        bind(C, name='ofd_upload_mesh') result(rc)
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nnode, nelem, nshl
+      integer(c_int32_t), intent(in) :: ien(*), elm_id(*)
+      real(c_double), intent(in) :: xg(*)
+      integer(c_int) :: rc
+    end function
+
+    function ofd_build_pattern(ctx, icnt) bind(C, name='ofd_build_pattern') result(rc)
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), intent(out) :: icnt
+      integer(c_int) :: rc
+    end function
+
+    function ofd_get_pattern(ctx, col, row) bind(C, name='ofd_get_pattern') result(rc)
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(out) :: col(*), row(*)
+      integer(c_int) :: rc
+    end function
+
+    function ofd_set_params(ctx, prm) bind(C, name='ofd_set_params') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: prm(*)
+      integer(c_int) :: rc
+    end function
+
+    function ofd_set_subdomains(ctx, id1, id2) bind(C, name='ofd_set_subdomains') &
+        result(rc)
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: id1, id2
+      integer(c_int) :: rc
+    end function
+
+    function ofd_set_ibc(ctx, ibc, ncol) bind(C, name='ofd_set_ibc') result(rc)
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(in) :: ibc(*)
+      integer(c_int), value :: ncol
+      integer(c_int) :: rc
+    end function
+
+    function ofd_set_bcvals(ctx, vals) bind(C, name='ofd_set_bcvals') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: vals(*)
+      integer(c_int) :: rc
+    end function
+
+    function ofd_apply_dirichlet(ctx) bind(C, name='ofd_apply_dirichlet') result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+
+    function ofd_set_field(ctx, field, host) bind(C, name='ofd_set_field') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: field
+      real(c_double), intent(in) :: host(*)
+      integer(c_int) :: rc
+    end function
+
+    function ofd_get_field(ctx, field, host) bind(C, name='ofd_get_field') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: field
+      real(c_double), intent(out) :: host(*)
+      integer(c_int) :: rc
+    end function
+
+    function ofd_assemble_ns(ctx, tensor_flag, subdomain_flag, cfl) &
+        bind(C, name='ofd_assemble_ns') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: tensor_flag, subdomain_flag
+      real(c_double), intent(out) :: cfl
+      integer(c_int) :: rc
+    end function
+
+    function ofd_get_rhs(ctx, rhsu, rhsp) bind(C, name='ofd_get_rhs') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: rhsu(*), rhsp(*)
+      integer(c_int) :: rc
+    end function
+
+    function ofd_set_rhs(ctx, rhsu, rhsp) bind(C, name='ofd_set_rhs') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: rhsu(*), rhsp(*)
+      integer(c_int) :: rc
+    end function
+
+    function ofd_get_lhs(ctx, k11, g, d1, m) bind(C, name='ofd_get_lhs') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: k11(*), g(*), d1(*), m(*)
+      integer(c_int) :: rc
+    end function
+
+    function ofd_set_lhs(ctx, k11, g, d1, m) bind(C, name='ofd_set_lhs') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: k11(*), g(*), d1(*), m(*)
+      integer(c_int) :: rc
+    end function
+
+    function ofd_spmv_up(ctx, precision, xu, xp, yu, yp) bind(C, name='ofd_spmv_up') &
+        result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: precision
+      real(c_double), intent(in) :: xu(*), xp(*)
+      real(c_double), intent(out) :: yu(*), yp(*)
+      integer(c_int) :: rc
+    end function
+
+    function ofd_gmres_up(ctx, precision, ortho, tol, kspace, kspace_min, solu, solp, &
+                          iters, resid) bind(C, name='ofd_gmres_up') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: precision, ortho, kspace, kspace_min
+      real(c_double), value :: tol
+      real(c_double), intent(out) :: solu(*), solp(*)
+      integer(c_int), intent(out) :: iters
+      real(c_double), intent(out) :: resid
+      integer(c_int) :: rc
+    end function
+
+    function ofd_solve(ctx, precision, ortho, rtol, inner_tol, kspace, max_cycles, &
+                       iters, cycles, relres) bind(C, name='ofd_solve') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: precision, ortho, kspace, max_cycles
+      real(c_double), value :: rtol, inner_tol
+      integer(c_int), intent(out) :: iters, cycles
+      real(c_double), intent(out) :: relres
+      integer(c_int) :: rc
+    end function
+
+    function ofd_newton_update(ctx) bind(C, name='ofd_newton_update') result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+
+    function ofd_predictor(ctx) bind(C, name='ofd_predictor') result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+
+    function ofd_advance(ctx) bind(C, name='ofd_advance') result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+
+    function ofd_rhs_norms(ctx, norms2) bind(C, name='ofd_rhs_norms') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: norms2(2)
+      integer(c_int) :: rc
+    end function
+  end interface
+
+contains
+
+  ! status check: print the library message and stop, as the reference
+  ! stops on fatal errors (eval_shape.f90:35-36, param.f90:595-598)
+  subroutine ofd_check(rc, where)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: where
+    character(kind=c_char), pointer :: msg(:)
+    character(len=512) :: text
+    integer :: i
+    if (rc == OFD_OK) return
+    text = ''
+    call c_f_pointer(ofd_last_error(), msg, [512])
+    do i = 1, 512
+      if (msg(i) == c_null_char) exit
+      text(i:i) = msg(i)
+    end do
+    write (*, '(a,a,a,i0,a,a)') 'ofd error in ', where, ' (', rc, '): ', trim(text)
+    error stop 911
+  end subroutine ofd_check
+
+end module ofd_iface

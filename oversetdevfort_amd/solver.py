@@ -173,16 +173,25 @@ class Context:
         return dict(solu=su.T.copy(), solp=sp, iters=it.value, resid=res.value)
 
     def solve(self, rtol=1e-8, precision=L.OFD_FP64, ortho=L.OFD_ORTHO_CGS2,
-              inner_tol=None, kspace=30, max_cycles=50):
+              inner_tol=None, kspace=30, max_cycles=50, raise_noconv=True):
         if inner_tol is None:
             inner_tol = 1e-4 if precision == L.OFD_FP32 else 1e-2
         it = ctypes.c_int()
         cyc = ctypes.c_int()
         rel = ctypes.c_double()
-        L.check(self.lib.ofd_solve(self.h, precision, ortho, float(rtol), float(inner_tol),
-                                   int(kspace), int(max_cycles), ctypes.byref(it),
-                                   ctypes.byref(cyc), ctypes.byref(rel)))
-        return dict(iters=it.value, cycles=cyc.value, relres=rel.value)
+        rc = self.lib.ofd_solve(self.h, precision, ortho, float(rtol), float(inner_tol),
+                                int(kspace), int(max_cycles), ctypes.byref(it),
+                                ctypes.byref(cyc), ctypes.byref(rel))
+        if rc != L.OFD_OK and (raise_noconv or rc != 5):
+            L.check(rc)
+        return dict(iters=it.value, cycles=cyc.value, relres=rel.value,
+                    converged=rc == L.OFD_OK)
+
+    def solve_history(self, maxn=4096):
+        buf = (ctypes.c_double * maxn)()
+        n = ctypes.c_int()
+        L.check(self.lib.ofd_solve_history(self.h, buf, maxn, ctypes.byref(n)))
+        return [buf[i] for i in range(min(n.value, maxn))]
 
     # ---- time / Newton drivers ----
     def newton_update(self):
@@ -237,13 +246,14 @@ def begin_step(ctx: Context):
 
 
 def newton_step(ctx: Context, rtol=1e-8, precision=L.OFD_FP64, ortho=L.OFD_ORTHO_CGS2,
-                kspace=30, inner_tol=None, max_cycles=50):
+                kspace=30, inner_tol=None, max_cycles=50, raise_noconv=True):
     """One Newton iteration of solveOverset_stag for one block
     (solveOverset.f90:55-121): assemble VEC+MAT, solve to rtol, update,
     re-assemble VEC and return the residual norms."""
     ctx.assembleNavSto(6, 1)
     info = ctx.solve(rtol=rtol, precision=precision, ortho=ortho, kspace=kspace,
-                     inner_tol=inner_tol, max_cycles=max_cycles)
+                     inner_tol=inner_tol, max_cycles=max_cycles,
+                     raise_noconv=raise_noconv)
     ctx.newton_update()
     ctx.assembleNavSto(2, 1)
     info["res"] = ctx.rhs_norms()
