@@ -145,6 +145,7 @@ def main():
     ap.add_argument("--dims", type=str, default="64,32,32")
     ap.add_argument("--no-cpu", action="store_true", help="skip the host baseline")
     ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--kspace", type=int, default=30, help="GMRES restart length")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs one GPU as a single rank; --gpus {args.gpus} "
@@ -166,9 +167,11 @@ def main():
     log(f"bench: setup {time.perf_counter() - t0:.2f} s, Z = {Z} blocks, "
         f"device bytes {ctx.device_bytes() / 2**20:.1f} MiB")
 
-    res64 = run_steps(ctx, S, L, prob, L.OFD_FP64, args.steps, args.warmup)
+    res64 = run_steps(ctx, S, L, prob, L.OFD_FP64, args.steps, args.warmup,
+                      kspace=args.kspace)
     log(f"bench: FP64 newton {res64['newton_ms']:.2f} ms")
-    res32 = run_steps(ctx, S, L, prob, L.OFD_FP32, args.steps, args.warmup)
+    res32 = run_steps(ctx, S, L, prob, L.OFD_FP32, args.steps, args.warmup,
+                      kspace=args.kspace)
     log(f"bench: mixed newton {res32['newton_ms']:.2f} ms")
 
     # kernel-level figures on the context stream (HIP events)
@@ -230,7 +233,8 @@ def main():
         "dtype": "f32-matrix/f64-vector (mixed); f64 reported alongside",
         "data": "synthetic",
         "config": {"workload": (f"channel [0,4]x[0,1]^2, {nx}x{ny}x{nz} hex, Re=1000, "
-                                "dt=0.005, 1 Newton step/time step, rtol 1e-8"),
+                                "dt=0.005, 1 Newton step/time step, rtol 1e-8, "
+                                f"block-Jacobi GMRES({args.kspace})"),
                    "elements": E, "nodes": N, "dof": 4 * N, "nnz_blocks": Z},
         "newton_ms": {"mixed": res32["newton_ms"], "fp64": res64["newton_ms"]},
         "speedup_mixed_vs_fp64": res64["newton_ms"] / res32["newton_ms"],

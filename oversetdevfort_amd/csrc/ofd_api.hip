@@ -86,6 +86,12 @@ static int need_pattern(const Ctx &c) {
 // Krylov workspace for restart length K
 static int ensure_krylov(Ctx &c, int K) {
   OFD_CHECK(K >= 1 && K <= kMaxKspace, OFD_ERR_ARG, "kspace out of range [1, 64]");
+  if (!c.h_flags) {
+    OFD_HIP(hipHostMalloc(reinterpret_cast<void **>(&c.h_flags),
+                          sizeof(int32_t) * kMaxKspace));
+    c.iter_ev.resize(kMaxKspace);
+    for (auto &ev : c.iter_ev) OFD_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  }
   if (c.kalloc >= K) return OFD_OK;
   int rc = dalloc(c, &c.d_W, (size_t)(K + 1) * c.nnode * 4);
   if (rc) return rc;
@@ -126,7 +132,17 @@ static int gmres_cycle(Ctx &c, int precision, int ortho, const double *b, double
   rc = ensure_prec(c, precision);
   if (rc) return rc;
   OFD_HIP(launch_gmres_init(c, precision, b, tol, K, kmin));
+  // The host enqueues iterations ahead of the device and learns about
+  // convergence through a pinned copy of the device flag made after every
+  // iteration; it waits only on the flag of iteration k - kLag, so the
+  // device queue never drains.  Iterations launched after convergence
+  // return at once (their kernels test the flag).
+  constexpr int kLag = 3;
   for (int k = 0; k < K; ++k) {
+    if (k >= kLag) {
+      OFD_HIP(hipEventSynchronize(c.iter_ev[k - kLag]));
+      if (c.h_flags[k - kLag]) break;
+    }
     OFD_HIP(launch_spmv_prec(c, precision, k));
     if (ortho == OFD_ORTHO_MGS) {
       for (int j = 0; j <= k + 1; ++j) OFD_HIP(launch_mgs_step(c, k, j));
@@ -135,14 +151,9 @@ static int gmres_cycle(Ctx &c, int precision, int ortho, const double *b, double
       OFD_HIP(launch_cgs_pass(c, k, 2));
       OFD_HIP(launch_cgs_pass(c, k, 3));
     }
-    // poll the device-side convergence flag now and then: the kernels of a
-    // finished cycle return at once, so polling only bounds wasted launches
-    if ((k % 4) == 3 && k + 1 < K) {
-      OFD_HIP(hipMemcpyAsync(c.h_gs, c.d_gs, sizeof(GmresState), hipMemcpyDeviceToHost,
-                             c.stream));
-      OFD_HIP(hipStreamSynchronize(c.stream));
-      if (c.h_gs->done) break;
-    }
+    OFD_HIP(hipMemcpyAsync(c.h_flags + k, &c.d_gs->done, sizeof(int32_t),
+                           hipMemcpyDeviceToHost, c.stream));
+    OFD_HIP(hipEventRecord(c.iter_ev[k], c.stream));
   }
   OFD_HIP(launch_gmres_finish(c, x));
   OFD_HIP(hipMemcpyAsync(c.h_gs, c.d_gs, sizeof(GmresState), hipMemcpyDeviceToHost,
@@ -251,6 +262,8 @@ int ofd_ctx_destroy(ofd_ctx *x) {
   dfree(c.d_part);
   dfree(c.d_gs);
   if (c.h_gs) (void)hipHostFree(c.h_gs);
+  if (c.h_flags) (void)hipHostFree(c.h_flags);
+  for (auto &ev : c.iter_ev) (void)hipEventDestroy(ev);
   if (c.h_pin) (void)hipHostFree(c.h_pin);
   if (c.stream) (void)hipStreamDestroy(c.stream);
   delete x;

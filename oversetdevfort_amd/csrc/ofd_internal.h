@@ -110,6 +110,8 @@ struct Ctx {
   double *d_part = nullptr;           // [kMaxRedBlocks][kMaxKspace + 1]
   GmresState *d_gs = nullptr;
   GmresState *h_gs = nullptr;         // pinned mirror
+  int32_t *h_flags = nullptr;         // pinned: per-iteration copy of GmresState::done
+  std::vector<hipEvent_t> iter_ev;    // event after each iteration's flag copy
   double *h_pin = nullptr;            // pinned scratch (host transfers)
   size_t h_pin_bytes = 0;
   int64_t dev_bytes = 0;
