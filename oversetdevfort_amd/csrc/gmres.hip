@@ -13,7 +13,7 @@
 // in GmresState, which saves the normalisation sweep of :293-294.
 //
 // Vectors are node-interleaved [N][4]; one thread per node.
-#include "ofd_internal.h"
+#include "gmres_dev.h"
 
 namespace ofd {
 
@@ -64,63 +64,26 @@ __device__ void block_partials(const double (&acc)[NV], int n, double *part, int
 }
 
 
-// Sum of part[b][j] over the nb producer blocks for j < m <= 65, by one
-// 256-thread block: thread (g, j) adds blocks g, g+G, ... (independent
-// loads that pipeline), then the G group sums are added in a fixed order,
-// so the result is the same on every run.
-__device__ void final_sums(const double *part, int stride, int nb, int m, double *s_out) {
-  __shared__ double s_acc[kRedBlock];
-  const int t = threadIdx.x;
-  const int G = (m <= 32) ? 8 : ((m <= 64) ? 4 : 2);
-  const int jw = kRedBlock / G;
-  const int j = t % jw, g = t / jw;
-  double s = 0.0;
-  if (j < m) {
-#pragma unroll 8
-    for (int b = g; b < nb; b += G) s += part[(size_t)b * stride + j];
-  }
-  s_acc[t] = s;
-  __syncthreads();
-  if (t < m) {
-    double r = 0.0;
-    for (int q = 0; q < G; ++q) r += s_acc[q * jw + t];
-    s_out[t] = r;
-  }
-  __syncthreads();
-}
-
-// Givens update of column k after H(0..k+1, k) is complete
-// (SparseGMRES_NS.f90:305-327); single thread.
-__device__ void givens_and_check(GmresState *g, int k) {
-  const int K1 = g->kspace + 1;
-  double *H = g->H;
-  for (int j = 0; j < k; ++j) {
-    const double t = g->rc[j] * H[j + K1 * k] + g->rs[j] * H[j + 1 + K1 * k];
-    H[j + 1 + K1 * k] = -g->rs[j] * H[j + K1 * k] + g->rc[j] * H[j + 1 + K1 * k];
-    H[j + K1 * k] = t;
-  }
-  const double hk = H[k + K1 * k], hk1 = H[k + 1 + K1 * k];
-  const double t = sqrt(hk * hk + hk1 * hk1);
-  g->rc[k] = hk / t;
-  g->rs[k] = hk1 / t;
-  H[k + K1 * k] = t;
-  H[k + 1 + K1 * k] = 0.0;
-  const double e0 = g->e[k], e1 = g->e[k + 1];
-  g->e[k] = g->rc[k] * e0 + g->rs[k] * e1;
-  g->e[k + 1] = -g->rs[k] * e0 + g->rc[k] * e1;
-  const double er = fabs(g->e[k + 1]);
-  g->resid = er;
-  g->iters = k + 1;
-  if ((er <= g->eps && k + 1 >= g->kmin) || !(er == er) || hk1 == 0.0) g->done = 1;
-}
-
 }  // namespace
 
-// W_0 = P^-1 b and partial |W_0|^2 (SparseGMRES_NS.f90:124-155)
+// W_0 = P^-1 b and partials of |W_0|^2 (SparseGMRES_NS.f90:124-155); block 0
+// also resets the cycle scalars.  The first Arnoldi SpMV closes the norm.
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_gmres_init(int N, const T *__restrict__ P, const double *__restrict__ b,
-             double *__restrict__ W0, double *__restrict__ part) {
+             double *__restrict__ W0, double *__restrict__ part, GmresState *__restrict__ g,
+             double tol, int kspace, int kmin) {
+  if (blockIdx.x == 0) {
+    for (int q = threadIdx.x; q <= kspace; q += blockDim.x) g->e[q] = 0.0;
+    if (threadIdx.x == 0) {
+      g->tol = tol;
+      g->kspace = kspace;
+      g->kmin = kmin;
+      g->done = 0;
+      g->iters = 0;
+      g->beta = 0.0;
+    }
+  }
   double acc[1] = {0.0};
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N;
        i += gridDim.x * blockDim.x) {
@@ -134,20 +97,30 @@ k_gmres_init(int N, const T *__restrict__ P, const double *__restrict__ b,
     st4(W0 + (size_t)i * 4, z);
     acc[0] += z[0] * z[0] + z[1] * z[1] + z[2] * z[2] + z[3] * z[3];
   }
-  block_partials<1>(acc, 1, part, kMaxKspace + 1, 0);
+  block_partials<1>(acc, 1, part, kPartStride, 0);
 }
 
 // One MGS sweep j of Arnoldi step k (SparseGMRES_NS.f90:246-287):
-//   j > 0:  z -= beta * v_{j-1};   partial of <z, v_j> (j <= k) or <z, z> (j = k+1)
+//   j > 0:  beta = <z, v_{j-1}> from the previous sweep's partials (H(j-1,k));
+//           z -= beta v_{j-1}
+//   partials of <z, W_j> (j <= k) or <z, z> (j = k+1) into buffer j & 1
 __global__ void __launch_bounds__(256)
-k_mgs_step(int N, double *__restrict__ W, double *__restrict__ part,
-           const GmresState *__restrict__ g, int k, int j) {
+k_mgs_step(int N, double *__restrict__ W, double *__restrict__ part_base,
+           GmresState *__restrict__ g, int k, int j, int nb) {
   if (g->done) return;
+  __shared__ double s_b[1];
   const size_t vs = (size_t)N * 4;
+  const size_t pb = (size_t)kMaxRedBlocks * kPartStride;
   double *z = W + vs * (k + 1);
+  double cp = 0.0;
+  if (j > 0) {
+    final_sums(part_base + pb * ((j - 1) & 1), nb, 1, s_b);
+    const double beta = s_b[0] * g->s[j - 1];
+    if (blockIdx.x == 0 && threadIdx.x == 0) g->H[(j - 1) + (g->kspace + 1) * k] = beta;
+    cp = beta * g->s[j - 1];
+  }
   const double *vp = W + vs * (j > 0 ? j - 1 : 0);
   const double *vj = W + vs * (j <= k ? j : k + 1);
-  const double cp = (j > 0) ? g->beta * g->s[j - 1] : 0.0;
   double acc[1] = {0.0};
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N;
        i += gridDim.x * blockDim.x) {
@@ -169,26 +142,36 @@ k_mgs_step(int N, double *__restrict__ W, double *__restrict__ part,
     }
     acc[0] += zz[0] * v[0] + zz[1] * v[1] + zz[2] * v[2] + zz[3] * v[3];
   }
-  block_partials<1>(acc, 1, part, kMaxKspace + 1, 0);
+  block_partials<1>(acc, 1, part_base + pb * (j & 1), kPartStride, 0);
 }
 
-// CGS2 sweeps of Arnoldi step k:
-//   pass 1: partials of <z, W_j>                  j = 0..k
-//   pass 2: z -= sum h1_j v_j ; partials of <z, W_j>
-//   pass 3: z -= sum h2_j v_j ; partial of |z|^2
+// CGS2 sweeps of Arnoldi step k (partials: pass 1 -> buffer 0, pass 2 -> 1,
+// pass 3 -> 0):
+//   pass 1: partials of <z, W_j>                        j = 0..k
+//   pass 2: h1 = sums of pass 1; z -= sum h1_j v_j; partials of <z, W_j>
+//   pass 3: h2 = sums of pass 2; z -= sum h2_j v_j; partial of |z|^2
 __global__ void __launch_bounds__(256)
-k_cgs_pass(int N, double *__restrict__ W, double *__restrict__ part,
-           const GmresState *__restrict__ g, int k, int pass) {
+k_cgs_pass(int N, double *__restrict__ W, double *__restrict__ part_base,
+           GmresState *__restrict__ g, int k, int pass, int nb) {
   if (g->done) return;
   const size_t vs = (size_t)N * 4;
+  const size_t pb = (size_t)kMaxRedBlocks * kPartStride;
   double *z = W + vs * (k + 1);
   const int m = k + 1;
   __shared__ double s_coef[kMaxKspace + 1];
-  for (int j = threadIdx.x; j < m; j += blockDim.x)
-    s_coef[j] = (pass == 2) ? g->h1[j] * g->s[j] : (pass == 3 ? g->h2[j] * g->s[j] : 0.0);
-  __syncthreads();
-  const int stride = kMaxKspace + 1;
   if (pass >= 2) {
+    final_sums(part_base + pb * (pass == 2 ? 0 : 1), nb, m, s_coef);
+    for (int j = threadIdx.x; j < m; j += blockDim.x) {
+      const double h = s_coef[j] * g->s[j];
+      if (blockIdx.x == 0) {
+        if (pass == 2)
+          g->h1[j] = h;
+        else
+          g->h2[j] = h;
+      }
+      s_coef[j] = h * g->s[j];
+    }
+    __syncthreads();
     double acc[1] = {0.0};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N;
          i += gridDim.x * blockDim.x) {
@@ -205,11 +188,12 @@ k_cgs_pass(int N, double *__restrict__ W, double *__restrict__ part,
       acc[0] += zz[0] * zz[0] + zz[1] * zz[1] + zz[2] * zz[2] + zz[3] * zz[3];
     }
     if (pass == 3) {
-      block_partials<1>(acc, 1, part, stride, 0);
+      block_partials<1>(acc, 1, part_base, kPartStride, 0);
       return;
     }
   }
   // dots <z, W_j> in chunks of kCh; pass 2 reads the z the same thread wrote
+  double *part = part_base + pb * (pass == 2 ? 1 : 0);
   for (int j0 = 0; j0 < m; j0 += kCh) {
     const int n = (m - j0 < kCh) ? (m - j0) : kCh;
     double acc[kCh];
@@ -228,8 +212,17 @@ k_cgs_pass(int N, double *__restrict__ W, double *__restrict__ part,
         }
       }
     }
-    block_partials<kCh>(acc, n, part, stride, j0);
+    block_partials<kCh>(acc, n, part, kPartStride, j0);
   }
+}
+
+// closes the last Arnoldi step of a cycle that ran to kspace (one block)
+__global__ void __launch_bounds__(256)
+k_gmres_close(const double *__restrict__ part, int nb, GmresState *__restrict__ g, int k,
+              int cgs) {
+  if (g->done) return;
+  double sc;
+  arnoldi_close(g, part, nb, k, cgs, nullptr, &sc);
 }
 
 // |v|^2 partials of a [N][4] vector
@@ -242,72 +235,14 @@ k_norm2(int N, const double *__restrict__ v, double *__restrict__ part) {
     ld4(v + (size_t)i * 4, z);
     acc[0] += z[0] * z[0] + z[1] * z[1] + z[2] * z[2] + z[3] * z[3];
   }
-  block_partials<1>(acc, 1, part, kMaxKspace + 1, 0);
+  block_partials<1>(acc, 1, part, kPartStride, 0);
 }
 
-// One block: finish the reduction of the preceding producer kernel and
-// update the cycle scalars.  op: 0 init (e, s_0, eps), 1 MGS sweep j,
-// 2 CGS pass 1, 3 CGS pass 2, 4 CGS pass 3, 5 norm into scal[slot].
 __global__ void __launch_bounds__(256)
-k_finalize(const double *__restrict__ part, int nb, GmresState *__restrict__ g, int op,
-           int k, int j, double tol, int kspace, int kmin) {
-  __shared__ double s_sum[kMaxKspace + 1];
-  if (op >= 1 && op <= 4 && g->done) return;
-  const int stride = kMaxKspace + 1;
-  const int m = (op == 2 || op == 3) ? k + 1 : 1;
-  final_sums(part, stride, nb, m, s_sum);
-  const int t = threadIdx.x;
-  if (op == 0) {
-    if (t == 0) {
-      const double nrm = sqrt(s_sum[0]);
-      g->kspace = kspace;
-      g->kmin = kmin;
-      g->rnorm0 = nrm;
-      g->eps = tol * nrm;
-      g->e[0] = nrm;
-      g->s[0] = (nrm > 0.0) ? 1.0 / nrm : 0.0;
-      g->iters = 0;
-      g->resid = nrm;
-      g->beta = 0.0;
-      g->done = (nrm > 0.0 && nrm == nrm) ? 0 : 1;
-    }
-    for (int q = t + 1; q <= kspace; q += blockDim.x) g->e[q] = 0.0;
-  } else if (op == 1) {
-    if (t == 0) {
-      const int K1 = g->kspace + 1;
-      if (j <= k) {
-        const double beta = s_sum[0] * g->s[j];
-        g->beta = beta;
-        g->H[j + K1 * k] = beta;
-      } else {
-        const double beta = s_sum[0];
-        g->beta = beta;
-        const double nrm = sqrt(beta);
-        g->H[k + 1 + K1 * k] = nrm;
-        g->s[k + 1] = (nrm > 0.0) ? 1.0 / nrm : 0.0;
-        givens_and_check(g, k);
-      }
-    }
-  } else if (op == 2 || op == 3) {
-    for (int q = t; q < m; q += blockDim.x) {
-      const double h = s_sum[q] * g->s[q];
-      if (op == 2)
-        g->h1[q] = h;
-      else
-        g->h2[q] = h;
-    }
-  } else if (op == 4) {
-    if (t == 0) {
-      const int K1 = g->kspace + 1;
-      for (int q = 0; q <= k; ++q) g->H[q + K1 * k] = g->h1[q] + g->h2[q];
-      const double nrm = sqrt(s_sum[0]);
-      g->H[k + 1 + K1 * k] = nrm;
-      g->s[k + 1] = (nrm > 0.0) ? 1.0 / nrm : 0.0;
-      givens_and_check(g, k);
-    }
-  } else {
-    if (t == 0) g->scal[j] = sqrt(s_sum[0]);
-  }
+k_norm_fin(const double *__restrict__ part, int nb, GmresState *__restrict__ g, int slot) {
+  __shared__ double s_sum[1];
+  final_sums(part, nb, 1, s_sum);
+  if (threadIdx.x == 0) g->scal[slot] = sqrt(s_sum[0]);
 }
 
 // back substitution (SparseGMRES_NS.f90:341-346) and x += sum y_j v_j (:349-352)
@@ -355,41 +290,38 @@ int nblocks_nodes(const Ctx &c) {
   return nb;
 }
 
-static hipError_t fin(Ctx &c, int op, int k, int j, double tol = 0.0, int kspace = 0,
-                      int kmin = 0) {
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kRedBlock), 0, c.stream, c.d_part,
-                     nblocks_nodes(c), c.d_gs, op, k, j, tol, kspace, kmin);
-  return hipGetLastError();
-}
-
 hipError_t launch_gmres_init(Ctx &c, int precision, const double *b, double tol,
                              int kspace, int kmin) {
   const int nb = nblocks_nodes(c);
   if (precision == OFD_FP32)
     hipLaunchKernelGGL(k_gmres_init<float>, dim3(nb), dim3(kRedBlock), 0, c.stream,
-                       c.nnode, c.d_P32, b, c.d_W, c.d_part);
+                       c.nnode, c.d_P32, b, c.d_W, c.d_part, c.d_gs, tol, kspace, kmin);
   else
     hipLaunchKernelGGL(k_gmres_init<double>, dim3(nb), dim3(kRedBlock), 0, c.stream,
-                       c.nnode, c.d_P64, b, c.d_W, c.d_part);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  return fin(c, 0, 0, 0, tol, kspace, kmin);
+                       c.nnode, c.d_P64, b, c.d_W, c.d_part, c.d_gs, tol, kspace, kmin);
+  return hipGetLastError();
 }
 
 hipError_t launch_mgs_step(Ctx &c, int k, int j) {
-  hipLaunchKernelGGL(k_mgs_step, dim3(nblocks_nodes(c)), dim3(kRedBlock), 0, c.stream,
-                     c.nnode, c.d_W, c.d_part, c.d_gs, k, j);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  return fin(c, 1, k, j);
+  const int nb = nblocks_nodes(c);
+  hipLaunchKernelGGL(k_mgs_step, dim3(nb), dim3(kRedBlock), 0, c.stream, c.nnode, c.d_W,
+                     c.d_part, c.d_gs, k, j, nb);
+  return hipGetLastError();
 }
 
 hipError_t launch_cgs_pass(Ctx &c, int k, int pass) {
-  hipLaunchKernelGGL(k_cgs_pass, dim3(nblocks_nodes(c)), dim3(kRedBlock), 0, c.stream,
-                     c.nnode, c.d_W, c.d_part, c.d_gs, k, pass);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  return fin(c, 1 + pass, k, 0);
+  const int nb = nblocks_nodes(c);
+  hipLaunchKernelGGL(k_cgs_pass, dim3(nb), dim3(kRedBlock), 0, c.stream, c.nnode, c.d_W,
+                     c.d_part, c.d_gs, k, pass, nb);
+  return hipGetLastError();
+}
+
+hipError_t launch_gmres_close(Ctx &c, int k, int ortho) {
+  const double *part =
+      c.d_part + (size_t)kMaxRedBlocks * kPartStride * close_buffer(k, ortho);
+  hipLaunchKernelGGL(k_gmres_close, dim3(1), dim3(kRedBlock), 0, c.stream, part,
+                     nblocks_nodes(c), c.d_gs, k, ortho == OFD_ORTHO_MGS ? 0 : 1);
+  return hipGetLastError();
 }
 
 hipError_t launch_gmres_finish(Ctx &c, double *x) {
@@ -399,11 +331,14 @@ hipError_t launch_gmres_finish(Ctx &c, double *x) {
 }
 
 hipError_t launch_norm2(Ctx &c, const double *v, int slot) {
-  hipLaunchKernelGGL(k_norm2, dim3(nblocks_nodes(c)), dim3(kRedBlock), 0, c.stream,
-                     c.nnode, v, c.d_part);
+  const int nb = nblocks_nodes(c);
+  hipLaunchKernelGGL(k_norm2, dim3(nb), dim3(kRedBlock), 0, c.stream, c.nnode, v,
+                     c.d_part);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  return fin(c, 5, 0, slot);
+  hipLaunchKernelGGL(k_norm_fin, dim3(1), dim3(kRedBlock), 0, c.stream, c.d_part, nb,
+                     c.d_gs, slot);
+  return hipGetLastError();
 }
 
 }  // namespace ofd

@@ -138,12 +138,16 @@ static int gmres_cycle(Ctx &c, int precision, int ortho, const double *b, double
   // device queue never drains.  Iterations launched after convergence
   // return at once (their kernels test the flag).
   constexpr int kLag = 3;
+  for (int k = 0; k < K; ++k) c.h_flags[k] = 0;
+  int klast = -1;
   for (int k = 0; k < K; ++k) {
-    if (k >= kLag) {
+    // the SpMV of iteration j closes step j-1 and writes its flag to h_flags[j-1]
+    if (k >= kLag + 1) {
       OFD_HIP(hipEventSynchronize(c.iter_ev[k - kLag]));
-      if (c.h_flags[k - kLag]) break;
+      if (c.h_flags[k - kLag - 1]) break;
     }
-    OFD_HIP(launch_spmv_prec(c, precision, k));
+    OFD_HIP(launch_spmv_prec(c, precision, k, ortho));
+    OFD_HIP(hipEventRecord(c.iter_ev[k], c.stream));
     if (ortho == OFD_ORTHO_MGS) {
       for (int j = 0; j <= k + 1; ++j) OFD_HIP(launch_mgs_step(c, k, j));
     } else {
@@ -151,10 +155,9 @@ static int gmres_cycle(Ctx &c, int precision, int ortho, const double *b, double
       OFD_HIP(launch_cgs_pass(c, k, 2));
       OFD_HIP(launch_cgs_pass(c, k, 3));
     }
-    OFD_HIP(hipMemcpyAsync(c.h_flags + k, &c.d_gs->done, sizeof(int32_t),
-                           hipMemcpyDeviceToHost, c.stream));
-    OFD_HIP(hipEventRecord(c.iter_ev[k], c.stream));
+    klast = k;
   }
+  if (klast >= 0) OFD_HIP(launch_gmres_close(c, klast, ortho));
   OFD_HIP(launch_gmres_finish(c, x));
   OFD_HIP(hipMemcpyAsync(c.h_gs, c.d_gs, sizeof(GmresState), hipMemcpyDeviceToHost,
                          c.stream));
@@ -205,7 +208,7 @@ int ofd_ctx_create(int device, ofd_ctx **out) {
     return OFD_ERR_HIP;
   }
   int rc = dalloc(c, &c.d_gs, 1);
-  if (!rc) rc = dalloc(c, &c.d_part, (size_t)kMaxRedBlocks * (kMaxKspace + 1));
+  if (!rc) rc = dalloc(c, &c.d_part, (size_t)2 * kMaxRedBlocks * (kMaxKspace + 1));
   if (!rc) rc = dalloc(c, &c.d_negdet, 1);
   if (rc) {
     ofd_ctx_destroy(x);

@@ -45,6 +45,8 @@ void set_error(const std::string &msg);
 constexpr int kMaxKspace = 64;     // GMRES restart length upper bound
 constexpr int kRedBlock = 256;     // threads per reduction block
 constexpr int kMaxRedBlocks = 2048;
+// partial buffer read by the step that closes Arnoldi step k
+inline int close_buffer(int k, int ortho) { return ortho == OFD_ORTHO_MGS ? ((k + 1) & 1) : 0; }
 
 // per-solve scalars living on the device (one GMRES cycle)
 struct GmresState {
@@ -56,6 +58,7 @@ struct GmresState {
   double y[kMaxKspace];
   double beta;                // running MGS projection coefficient
   double eps;                 // tol * |P^-1 b|
+  double tol;
   double rnorm0;
   double resid;               // |e_{k+1}| at exit
   double scal[4];             // misc reductions (norms)
@@ -107,7 +110,7 @@ struct Ctx {
   // GMRES workspace
   int kalloc = 0;
   double *d_W = nullptr;              // [(kalloc+1)][N][4]
-  double *d_part = nullptr;           // [kMaxRedBlocks][kMaxKspace + 1]
+  double *d_part = nullptr;           // 2 x [kMaxRedBlocks][kMaxKspace + 1] (double-buffered)
   GmresState *d_gs = nullptr;
   GmresState *h_gs = nullptr;         // pinned mirror
   int32_t *h_flags = nullptr;         // pinned: per-iteration copy of GmresState::done
@@ -130,7 +133,7 @@ hipError_t launch_build_prec(Ctx &c, bool want32);
 hipError_t launch_convert_a32(Ctx &c);
 hipError_t launch_spmv(Ctx &c, int precision, const double *x, double *y,
                        const double *b, int mode);
-hipError_t launch_spmv_prec(Ctx &c, int precision, int k);
+hipError_t launch_spmv_prec(Ctx &c, int precision, int k, int ortho);
 // vectors
 hipError_t launch_soa_to_node4(Ctx &c, const double *u, const double *p,
                                double *out);
@@ -142,6 +145,7 @@ hipError_t launch_gmres_init(Ctx &c, int precision, const double *b,
                              double tol, int kspace, int kmin);
 hipError_t launch_mgs_step(Ctx &c, int k, int j);
 hipError_t launch_cgs_pass(Ctx &c, int k, int pass);
+hipError_t launch_gmres_close(Ctx &c, int k, int ortho);
 hipError_t launch_gmres_finish(Ctx &c, double *x);
 hipError_t launch_norm2(Ctx &c, const double *v, int slot);
 

@@ -15,7 +15,7 @@
 // The block-Jacobi inverse (SparseGMRES_NS.f90:53-121) is kept as a 4x4
 // block-diagonal [Binv 0; 0 1/M] so that applying it is a quad-lane 4x4
 // product fused into the SpMV epilogue (Arnoldi step z = P^-1 A v).
-#include "ofd_internal.h"
+#include "gmres_dev.h"
 
 namespace ofd {
 
@@ -59,14 +59,21 @@ __device__ __forceinline__ void load_x4(const double *x, double &x0, double &x1,
 
 }  // namespace
 
-// mode 0: y = A x; mode 1: y = b - A x; mode 2: y = s * P^-1 (A x)
+// mode 0: y = A x; mode 1: y = b - A x; mode 2: Arnoldi product
+// y = s_k P^-1 (A W_k), whose prologue first closes Arnoldi step k-1 from the
+// partials the previous sweep left in `part` (k = 0: the initial residual)
 template <typename T, int MODE>
 __global__ void __launch_bounds__(256)
 k_spmv(int N, const int32_t *__restrict__ col, const int32_t *__restrict__ row,
        const T *__restrict__ A, const double *__restrict__ x,
        double *__restrict__ y, const double *__restrict__ b,
-       const T *__restrict__ Pinv, const GmresState *__restrict__ gs, int kidx) {
-  if (MODE == 2 && gs->done) return;
+       const T *__restrict__ Pinv, GmresState *__restrict__ gs, int kidx,
+       const double *__restrict__ part, int nb, int cgs, int32_t *hflag) {
+  double s = 1.0;
+  if (MODE == 2) {
+    if (gs->done) return;
+    if (arnoldi_close(gs, part, nb, kidx - 1, cgs, hflag, &s)) return;
+  }
   const int gt = blockIdx.x * blockDim.x + threadIdx.x;
   const int r = gt >> 2;
   const int c = gt & 3;
@@ -93,7 +100,6 @@ k_spmv(int N, const int32_t *__restrict__ col, const int32_t *__restrict__ row,
     const double z3 = __shfl(acc, base + 3);
     double p0, p1, p2, p3;
     Row4<T>::load(Pinv + (size_t)r * 16 + c * 4, p0, p1, p2, p3);
-    const double s = gs->s[kidx];
     y[(size_t)r * 4 + c] = s * (p0 * z0 + p1 * z1 + p2 * z2 + p3 * z3);
   }
 }
@@ -187,35 +193,43 @@ hipError_t launch_spmv(Ctx &c, int precision, const double *x, double *y,
   if (precision == OFD_FP32) {
     if (mode == 0)
       hipLaunchKernelGGL((k_spmv<float, 0>), dim3(nb), dim3(256), 0, c.stream, c.nnode,
-                         c.d_col, c.d_row, c.d_A32, x, y, b, c.d_P32, c.d_gs, 0);
+                         c.d_col, c.d_row, c.d_A32, x, y, b, c.d_P32, c.d_gs, 0,
+                         nullptr, 0, 0, nullptr);
     else
       hipLaunchKernelGGL((k_spmv<float, 1>), dim3(nb), dim3(256), 0, c.stream, c.nnode,
-                         c.d_col, c.d_row, c.d_A32, x, y, b, c.d_P32, c.d_gs, 0);
+                         c.d_col, c.d_row, c.d_A32, x, y, b, c.d_P32, c.d_gs, 0,
+                         nullptr, 0, 0, nullptr);
   } else {
     if (mode == 0)
       hipLaunchKernelGGL((k_spmv<double, 0>), dim3(nb), dim3(256), 0, c.stream,
                          c.nnode, c.d_col, c.d_row, c.d_A64, x, y, b, c.d_P64,
-                         c.d_gs, 0);
+                         c.d_gs, 0, nullptr, 0, 0, nullptr);
     else
       hipLaunchKernelGGL((k_spmv<double, 1>), dim3(nb), dim3(256), 0, c.stream,
                          c.nnode, c.d_col, c.d_row, c.d_A64, x, y, b, c.d_P64,
-                         c.d_gs, 0);
+                         c.d_gs, 0, nullptr, 0, 0, nullptr);
   }
   return hipGetLastError();
 }
 
-// Arnoldi product: W_{k+1} = s_k P^-1 A W_k
-hipError_t launch_spmv_prec(Ctx &c, int precision, int k) {
+// Arnoldi product: W_{k+1} = s_k P^-1 A W_k (closes step k-1 first)
+hipError_t launch_spmv_prec(Ctx &c, int precision, int k, int ortho) {
   const unsigned nb = spmv_blocks(c);
   const size_t vs = (size_t)c.nnode * 4;
   const double *x = c.d_W + vs * k;
   double *y = c.d_W + vs * (k + 1);
+  const double *part =
+      c.d_part + (size_t)kMaxRedBlocks * kPartStride * close_buffer(k - 1, ortho);
+  const int pnb = nblocks_nodes(c);
+  const int cgs = ortho == OFD_ORTHO_MGS ? 0 : 1;
   if (precision == OFD_FP32)
     hipLaunchKernelGGL((k_spmv<float, 2>), dim3(nb), dim3(256), 0, c.stream, c.nnode,
-                       c.d_col, c.d_row, c.d_A32, x, y, nullptr, c.d_P32, c.d_gs, k);
+                       c.d_col, c.d_row, c.d_A32, x, y, nullptr, c.d_P32, c.d_gs, k, part,
+                       pnb, cgs, c.h_flags);
   else
     hipLaunchKernelGGL((k_spmv<double, 2>), dim3(nb), dim3(256), 0, c.stream, c.nnode,
-                       c.d_col, c.d_row, c.d_A64, x, y, nullptr, c.d_P64, c.d_gs, k);
+                       c.d_col, c.d_row, c.d_A64, x, y, nullptr, c.d_P64, c.d_gs, k, part,
+                       pnb, cgs, c.h_flags);
   return hipGetLastError();
 }
 
