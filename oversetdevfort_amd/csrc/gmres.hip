@@ -177,7 +177,25 @@ k_cgs_pass(int N, double *__restrict__ W, double *__restrict__ part_base,
          i += gridDim.x * blockDim.x) {
       double zz[4];
       ld4(z + (size_t)i * 4, zz);
-      for (int j = 0; j < m; ++j) {
+      // sum_j c_j v_j in batches of 4 independent loads, added in j order
+      int j = 0;
+      for (; j + 4 <= m; j += 4) {
+        double v0[4], v1[4], v2[4], v3[4];
+        ld4(W + vs * (j + 0) + (size_t)i * 4, v0);
+        ld4(W + vs * (j + 1) + (size_t)i * 4, v1);
+        ld4(W + vs * (j + 2) + (size_t)i * 4, v2);
+        ld4(W + vs * (j + 3) + (size_t)i * 4, v3);
+        const double c0 = s_coef[j], c1 = s_coef[j + 1], c2 = s_coef[j + 2],
+                     c3 = s_coef[j + 3];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          zz[c] -= c0 * v0[c];
+          zz[c] -= c1 * v1[c];
+          zz[c] -= c2 * v2[c];
+          zz[c] -= c3 * v3[c];
+        }
+      }
+      for (; j < m; ++j) {
         double v[4];
         ld4(W + vs * j + (size_t)i * 4, v);
         const double cj = s_coef[j];
@@ -273,7 +291,22 @@ k_gmres_finish(int N, const double *__restrict__ W, const GmresState *__restrict
        i += gridDim.x * blockDim.x) {
     double xx[4];
     ld4(x + (size_t)i * 4, xx);
-    for (int j = 0; j < it; ++j) {
+    int j = 0;
+    for (; j + 4 <= it; j += 4) {
+      double v0[4], v1[4], v2[4], v3[4];
+      ld4(W + vs * (j + 0) + (size_t)i * 4, v0);
+      ld4(W + vs * (j + 1) + (size_t)i * 4, v1);
+      ld4(W + vs * (j + 2) + (size_t)i * 4, v2);
+      ld4(W + vs * (j + 3) + (size_t)i * 4, v3);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        xx[c] += s_y[j] * v0[c];
+        xx[c] += s_y[j + 1] * v1[c];
+        xx[c] += s_y[j + 2] * v2[c];
+        xx[c] += s_y[j + 3] * v3[c];
+      }
+    }
+    for (; j < it; ++j) {
       double v[4];
       ld4(W + vs * j + (size_t)i * 4, v);
 #pragma unroll

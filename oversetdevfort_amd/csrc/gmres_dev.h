@@ -14,26 +14,31 @@ namespace ofd {
 
 constexpr int kPartStride = kMaxKspace + 1;
 
-// s_out[j] = sum over b < nb of part[b][j], j < m <= 65 (all block threads)
+// s_out[j] = sum over b < nb of part[b][j], j < m <= 65 (all block threads).
+// Thread (g, j) adds blocks g, g+G, ... with G = 256 / jw groups (jw = m
+// rounded up to a power of two), then the G group sums of each j are added
+// by a fixed-shape tree: the result is the same on every run and in every
+// block.
 __device__ __forceinline__ void final_sums(const double *part, int nb, int m,
                                            double *s_out) {
   __shared__ double s_acc[kRedBlock];
   const int t = threadIdx.x;
-  const int G = (m <= 32) ? 8 : ((m <= 64) ? 4 : 2);
-  const int jw = kRedBlock / G;
+  int jw = 1;
+  while (jw < m) jw <<= 1;
+  const int G = kRedBlock / jw;  // 0 if m > 256 (not used: m <= 65)
   const int j = t % jw, g = t / jw;
   double s = 0.0;
   if (j < m) {
-#pragma unroll 8
+#pragma unroll 4
     for (int b = g; b < nb; b += G) s += part[(size_t)b * kPartStride + j];
   }
   s_acc[t] = s;
   __syncthreads();
-  if (t < m) {
-    double r = 0.0;
-    for (int q = 0; q < G; ++q) r += s_acc[q * jw + t];
-    s_out[t] = r;
+  for (int h = G >> 1; h > 0; h >>= 1) {
+    if (g < h) s_acc[t] += s_acc[t + h * jw];
+    __syncthreads();
   }
+  if (t < m) s_out[t] = s_acc[t];
   __syncthreads();
 }
 
@@ -49,7 +54,21 @@ __device__ __forceinline__ bool arnoldi_close(GmresState *__restrict__ g,
                                               double *s_next) {
   __shared__ double s_sum[1];
   __shared__ double s_res[2];
+  __shared__ double s_col[kMaxKspace + 2];
+  __shared__ double s_rc[kMaxKspace], s_rs[kMaxKspace];
   final_sums(part, nb, 1, s_sum);
+  if (k >= 0) {
+    // stage column k and the previous rotations in LDS with all threads
+    const int K1 = g->kspace + 1;
+    for (int q = threadIdx.x; q <= k; q += blockDim.x) {
+      s_col[q] = cgs ? g->h1[q] + g->h2[q] : g->H[q + K1 * k];
+      if (q < k) {
+        s_rc[q] = g->rc[q];
+        s_rs[q] = g->rs[q];
+      }
+    }
+    __syncthreads();
+  }
   if (threadIdx.x == 0) {
     const double nrm = sqrt(s_sum[0]);
     const double sc = (nrm > 0.0) ? 1.0 / nrm : 0.0;
@@ -68,12 +87,10 @@ __device__ __forceinline__ bool arnoldi_close(GmresState *__restrict__ g,
       }
     } else {
       const int K1 = g->kspace + 1;
-      double col[kMaxKspace + 1];
-      for (int q = 0; q <= k; ++q)
-        col[q] = cgs ? g->h1[q] + g->h2[q] : g->H[q + K1 * k];
+      double *col = s_col;
       col[k + 1] = nrm;
       for (int q = 0; q < k; ++q) {
-        const double c = g->rc[q], s = g->rs[q];
+        const double c = s_rc[q], s = s_rs[q];
         const double t = c * col[q] + s * col[q + 1];
         col[q + 1] = -s * col[q] + c * col[q + 1];
         col[q] = t;
@@ -86,10 +103,9 @@ __device__ __forceinline__ bool arnoldi_close(GmresState *__restrict__ g,
       const double ek1 = -rs * e0 + rc * e1;
       const double er = fabs(ek1);
       done = (er <= g->eps && k + 1 >= g->kmin) || !(er == er) || hk1 == 0.0;
+      col[k] = t;
+      col[k + 1] = 0.0;
       if (w) {
-        for (int q = 0; q < k; ++q) g->H[q + K1 * k] = col[q];
-        g->H[k + K1 * k] = t;
-        g->H[k + 1 + K1 * k] = 0.0;
         g->rc[k] = rc;
         g->rs[k] = rs;
         g->e[k] = ek;
@@ -105,6 +121,10 @@ __device__ __forceinline__ bool arnoldi_close(GmresState *__restrict__ g,
     s_res[1] = sc;
   }
   __syncthreads();
+  if (k >= 0 && blockIdx.x == 0) {
+    const int K1 = g->kspace + 1;
+    for (int q = threadIdx.x; q <= k + 1; q += blockDim.x) g->H[q + K1 * k] = s_col[q];
+  }
   *s_next = s_res[1];
   return s_res[0] != 0.0;
 }

@@ -126,7 +126,7 @@ static int ensure_prec(Ctx &c, int precision) {
 // one GMRES cycle on right-hand side b (device [N][4]); x += increment
 static int gmres_cycle(Ctx &c, int precision, int ortho, const double *b, double tol,
                        int K, int kmin, double *x, int *iters, double *resid,
-                       double *rnorm0) {
+                       double *rnorm0, bool sync = true) {
   int rc = ensure_krylov(c, K);
   if (rc) return rc;
   rc = ensure_prec(c, precision);
@@ -159,6 +159,7 @@ static int gmres_cycle(Ctx &c, int precision, int ortho, const double *b, double
   }
   if (klast >= 0) OFD_HIP(launch_gmres_close(c, klast, ortho));
   OFD_HIP(launch_gmres_finish(c, x));
+  if (!sync) return OFD_OK;  // the caller reads the cycle scalars later
   OFD_HIP(hipMemcpyAsync(c.h_gs, c.d_gs, sizeof(GmresState), hipMemcpyDeviceToHost,
                          c.stream));
   OFD_HIP(hipStreamSynchronize(c.stream));
@@ -698,16 +699,22 @@ int ofd_solve(ofd_ctx *x, int precision, int ortho, double rtol, double inner_to
   const double *rhs = c.d_rhs;
   c.solve_hist.clear();
   for (cyc = 1; cyc <= max_cycles; ++cyc) {
-    int it = 0;
-    rc = gmres_cycle(c, precision, ortho, rhs, inner_tol, kspace, 0, c.d_x, &it, nullptr,
-                     nullptr);
-    total += it;
+    rc = gmres_cycle(c, precision, ortho, rhs, inner_tol, kspace, 0, c.d_x, nullptr,
+                     nullptr, nullptr, false);
     if (rc) break;
-    // FP64 outer residual r = b - A x
+    // FP64 outer residual r = b - A x and its norm; one host sync per cycle
     OFD_HIP(launch_spmv(c, OFD_FP64, c.d_x, c.d_r, c.d_rhs, 1));
-    double rn = 0.0;
-    rc = norm2_dev(c, c.d_r, &rn);
-    if (rc) break;
+    OFD_HIP(launch_norm2(c, c.d_r, 0));
+    OFD_HIP(hipMemcpyAsync(c.h_gs, c.d_gs, sizeof(GmresState), hipMemcpyDeviceToHost,
+                           c.stream));
+    OFD_HIP(hipStreamSynchronize(c.stream));
+    total += c.h_gs->iters;
+    const double rn = c.h_gs->scal[0];
+    if (!std::isfinite(c.h_gs->resid) || !std::isfinite(c.h_gs->rnorm0)) {
+      rc = OFD_ERR_NONFINITE;
+      set_error("GMRES: non-finite residual");
+      break;
+    }
     rel = rn / bnorm;
     c.solve_hist.push_back(rel);
     rhs = c.d_r;
