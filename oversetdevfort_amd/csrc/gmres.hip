@@ -67,10 +67,10 @@ __device__ void block_partials(const double (&acc)[NV], int n, double *part, int
 }  // namespace
 
 // W_0 = P^-1 b and partials of |W_0|^2 (SparseGMRES_NS.f90:124-155); block 0
-// also resets the cycle scalars.  The first Arnoldi SpMV closes the norm.
-template <typename T>
+// also resets the cycle scalars.  The first orthogonalisation kernel of
+// iteration 0 closes the norm.
 __global__ void __launch_bounds__(256)
-k_gmres_init(int N, const T *__restrict__ P, const double *__restrict__ b,
+k_gmres_init(int N, const double *__restrict__ P, const double *__restrict__ b,
              double *__restrict__ W0, double *__restrict__ part, GmresState *__restrict__ g,
              double tol, int kspace, int kmin) {
   if (blockIdx.x == 0) {
@@ -89,32 +89,36 @@ k_gmres_init(int N, const T *__restrict__ P, const double *__restrict__ b,
        i += gridDim.x * blockDim.x) {
     double r[4], z[4];
     ld4(b + (size_t)i * 4, r);
-    const T *p = P + (size_t)i * 16;
+    const double *p = P + (size_t)i * 16;
 #pragma unroll
     for (int c = 0; c < 4; ++c)
-      z[c] = (double)p[c * 4 + 0] * r[0] + (double)p[c * 4 + 1] * r[1] +
-             (double)p[c * 4 + 2] * r[2] + (double)p[c * 4 + 3] * r[3];
+      z[c] = p[c * 4 + 0] * r[0] + p[c * 4 + 1] * r[1] + p[c * 4 + 2] * r[2] +
+             p[c * 4 + 3] * r[3];
     st4(W0 + (size_t)i * 4, z);
     acc[0] += z[0] * z[0] + z[1] * z[1] + z[2] * z[2] + z[3] * z[3];
   }
   block_partials<1>(acc, 1, part, kPartStride, 0);
 }
 
-// One MGS sweep j of Arnoldi step k (SparseGMRES_NS.f90:246-287):
-//   j > 0:  beta = <z, v_{j-1}> from the previous sweep's partials (H(j-1,k));
-//           z -= beta v_{j-1}
-//   partials of <z, W_j> (j <= k) or <z, z> (j = k+1) into buffer j & 1
+// One MGS sweep j of Arnoldi step k (SparseGMRES_NS.f90:246-287) on the
+// unscaled product z = (P^-1 A) W_k:
+//   j = 0:  close step k-1 first (partials in `rd`)
+//   j > 0:  beta = <z, v_{j-1}> from the previous sweep's partials (kept
+//           unscaled in H(j-1,k)); z -= beta v_{j-1}
+//   partials of <z, W_j> (j <= k) or <z, z> (j = k+1) into `wr`
 __global__ void __launch_bounds__(256)
-k_mgs_step(int N, double *__restrict__ W, double *__restrict__ part_base,
-           GmresState *__restrict__ g, int k, int j, int nb) {
+k_mgs_step(int N, double *__restrict__ W, const double *__restrict__ rd,
+           double *__restrict__ wr, GmresState *__restrict__ g, int k, int j, int nb,
+           int32_t *hflag) {
   if (g->done) return;
   __shared__ double s_b[1];
   const size_t vs = (size_t)N * 4;
-  const size_t pb = (size_t)kMaxRedBlocks * kPartStride;
   double *z = W + vs * (k + 1);
   double cp = 0.0;
-  if (j > 0) {
-    final_sums(part_base + pb * ((j - 1) & 1), nb, 1, s_b);
+  if (j == 0) {
+    if (arnoldi_close(g, rd, nb, k - 1, 0, hflag)) return;
+  } else {
+    final_sums(rd, nb, 1, s_b);
     const double beta = s_b[0] * g->s[j - 1];
     if (blockIdx.x == 0 && threadIdx.x == 0) g->H[(j - 1) + (g->kspace + 1) * k] = beta;
     cp = beta * g->s[j - 1];
@@ -142,25 +146,28 @@ k_mgs_step(int N, double *__restrict__ W, double *__restrict__ part_base,
     }
     acc[0] += zz[0] * v[0] + zz[1] * v[1] + zz[2] * v[2] + zz[3] * v[3];
   }
-  block_partials<1>(acc, 1, part_base + pb * (j & 1), kPartStride, 0);
+  block_partials<1>(acc, 1, wr, kPartStride, 0);
 }
 
-// CGS2 sweeps of Arnoldi step k (partials: pass 1 -> buffer 0, pass 2 -> 1,
-// pass 3 -> 0):
-//   pass 1: partials of <z, W_j>                        j = 0..k
-//   pass 2: h1 = sums of pass 1; z -= sum h1_j v_j; partials of <z, W_j>
-//   pass 3: h2 = sums of pass 2; z -= sum h2_j v_j; partial of |z|^2
+// CGS2 sweeps of Arnoldi step k on the unscaled product z = (P^-1 A) W_k:
+//   pass 1: close step k-1; partials of <z, W_j>            j = 0..k
+//   pass 2: h1_j = <z, v_j>; z -= sum h1_j v_j; partials of <z, W_j>
+//   pass 3: h2_j = <z, v_j>; z -= sum h2_j v_j; partials of |z|^2
+// Each pass reads the partials of the previous kernel from `rd` and writes
+// its own to `wr`.
 __global__ void __launch_bounds__(256)
-k_cgs_pass(int N, double *__restrict__ W, double *__restrict__ part_base,
-           GmresState *__restrict__ g, int k, int pass, int nb) {
+k_cgs_pass(int N, double *__restrict__ W, const double *__restrict__ rd,
+           double *__restrict__ wr, GmresState *__restrict__ g, int k, int pass, int nb,
+           int32_t *hflag) {
   if (g->done) return;
   const size_t vs = (size_t)N * 4;
-  const size_t pb = (size_t)kMaxRedBlocks * kPartStride;
   double *z = W + vs * (k + 1);
   const int m = k + 1;
   __shared__ double s_coef[kMaxKspace + 1];
-  if (pass >= 2) {
-    final_sums(part_base + pb * (pass == 2 ? 0 : 1), nb, m, s_coef);
+  if (pass == 1) {
+    if (arnoldi_close(g, rd, nb, k - 1, 1, hflag)) return;
+  } else {
+    final_sums(rd, nb, m, s_coef);
     for (int j = threadIdx.x; j < m; j += blockDim.x) {
       const double h = s_coef[j] * g->s[j];
       if (blockIdx.x == 0) {
@@ -206,12 +213,11 @@ k_cgs_pass(int N, double *__restrict__ W, double *__restrict__ part_base,
       acc[0] += zz[0] * zz[0] + zz[1] * zz[1] + zz[2] * zz[2] + zz[3] * zz[3];
     }
     if (pass == 3) {
-      block_partials<1>(acc, 1, part_base, kPartStride, 0);
+      block_partials<1>(acc, 1, wr, kPartStride, 0);
       return;
     }
   }
   // dots <z, W_j> in chunks of kCh; pass 2 reads the z the same thread wrote
-  double *part = part_base + pb * (pass == 2 ? 1 : 0);
   for (int j0 = 0; j0 < m; j0 += kCh) {
     const int n = (m - j0 < kCh) ? (m - j0) : kCh;
     double acc[kCh];
@@ -230,17 +236,16 @@ k_cgs_pass(int N, double *__restrict__ W, double *__restrict__ part_base,
         }
       }
     }
-    block_partials<kCh>(acc, n, part, kPartStride, j0);
+    block_partials<kCh>(acc, n, wr, kPartStride, j0);
   }
 }
 
-// closes the last Arnoldi step of a cycle that ran to kspace (one block)
+// closes the last Arnoldi step of a cycle (one block)
 __global__ void __launch_bounds__(256)
 k_gmres_close(const double *__restrict__ part, int nb, GmresState *__restrict__ g, int k,
               int cgs) {
   if (g->done) return;
-  double sc;
-  arnoldi_close(g, part, nb, k, cgs, nullptr, &sc);
+  arnoldi_close(g, part, nb, k, cgs, nullptr);
 }
 
 // |v|^2 partials of a [N][4] vector
@@ -323,37 +328,40 @@ int nblocks_nodes(const Ctx &c) {
   return nb;
 }
 
-hipError_t launch_gmres_init(Ctx &c, int precision, const double *b, double tol,
-                             int kspace, int kmin) {
+static double *part_buf(Ctx &c, int which) {
+  return c.d_part + (size_t)kMaxRedBlocks * kPartStride * which;
+}
+
+hipError_t launch_gmres_init(Ctx &c, const double *b, double tol, int kspace, int kmin) {
   const int nb = nblocks_nodes(c);
-  if (precision == OFD_FP32)
-    hipLaunchKernelGGL(k_gmres_init<float>, dim3(nb), dim3(kRedBlock), 0, c.stream,
-                       c.nnode, c.d_P32, b, c.d_W, c.d_part, c.d_gs, tol, kspace, kmin);
-  else
-    hipLaunchKernelGGL(k_gmres_init<double>, dim3(nb), dim3(kRedBlock), 0, c.stream,
-                       c.nnode, c.d_P64, b, c.d_W, c.d_part, c.d_gs, tol, kspace, kmin);
+  hipLaunchKernelGGL(k_gmres_init, dim3(nb), dim3(kRedBlock), 0, c.stream, c.nnode,
+                     c.d_P64, b, c.d_W, part_buf(c, 0), c.d_gs, tol, kspace, kmin);
+  c.part_cur = 0;
   return hipGetLastError();
 }
 
 hipError_t launch_mgs_step(Ctx &c, int k, int j) {
   const int nb = nblocks_nodes(c);
   hipLaunchKernelGGL(k_mgs_step, dim3(nb), dim3(kRedBlock), 0, c.stream, c.nnode, c.d_W,
-                     c.d_part, c.d_gs, k, j, nb);
+                     part_buf(c, c.part_cur), part_buf(c, c.part_cur ^ 1), c.d_gs, k, j, nb,
+                     c.h_flags);
+  c.part_cur ^= 1;
   return hipGetLastError();
 }
 
 hipError_t launch_cgs_pass(Ctx &c, int k, int pass) {
   const int nb = nblocks_nodes(c);
   hipLaunchKernelGGL(k_cgs_pass, dim3(nb), dim3(kRedBlock), 0, c.stream, c.nnode, c.d_W,
-                     c.d_part, c.d_gs, k, pass, nb);
+                     part_buf(c, c.part_cur), part_buf(c, c.part_cur ^ 1), c.d_gs, k, pass,
+                     nb, c.h_flags);
+  c.part_cur ^= 1;
   return hipGetLastError();
 }
 
 hipError_t launch_gmres_close(Ctx &c, int k, int ortho) {
-  const double *part =
-      c.d_part + (size_t)kMaxRedBlocks * kPartStride * close_buffer(k, ortho);
-  hipLaunchKernelGGL(k_gmres_close, dim3(1), dim3(kRedBlock), 0, c.stream, part,
-                     nblocks_nodes(c), c.d_gs, k, ortho == OFD_ORTHO_MGS ? 0 : 1);
+  hipLaunchKernelGGL(k_gmres_close, dim3(1), dim3(kRedBlock), 0, c.stream,
+                     part_buf(c, c.part_cur), nblocks_nodes(c), c.d_gs, k,
+                     ortho == OFD_ORTHO_MGS ? 0 : 1);
   return hipGetLastError();
 }
 
@@ -365,12 +373,12 @@ hipError_t launch_gmres_finish(Ctx &c, double *x) {
 
 hipError_t launch_norm2(Ctx &c, const double *v, int slot) {
   const int nb = nblocks_nodes(c);
-  hipLaunchKernelGGL(k_norm2, dim3(nb), dim3(kRedBlock), 0, c.stream, c.nnode, v,
-                     c.d_part);
+  double *pb = part_buf(c, 0);
+  hipLaunchKernelGGL(k_norm2, dim3(nb), dim3(kRedBlock), 0, c.stream, c.nnode, v, pb);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_norm_fin, dim3(1), dim3(kRedBlock), 0, c.stream, c.d_part, nb,
-                     c.d_gs, slot);
+  hipLaunchKernelGGL(k_norm_fin, dim3(1), dim3(kRedBlock), 0, c.stream, pb, nb, c.d_gs,
+                     slot);
   return hipGetLastError();
 }
 

@@ -29,8 +29,16 @@ __device__ __forceinline__ void final_sums(const double *part, int nb, int m,
   const int j = t % jw, g = t / jw;
   double s = 0.0;
   if (j < m) {
-#pragma unroll 4
-    for (int b = g; b < nb; b += G) s += part[(size_t)b * kPartStride + j];
+    // independent loads in batches of 8, added in block order
+    int b = g;
+    for (; b + 7 * G < nb; b += 8 * G) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = part[(size_t)(b + u * G) * kPartStride + j];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    for (; b < nb; b += G) s += part[(size_t)b * kPartStride + j];
   }
   s_acc[t] = s;
   __syncthreads();
@@ -42,16 +50,17 @@ __device__ __forceinline__ void final_sums(const double *part, int nb, int m,
   __syncthreads();
 }
 
-// Close Arnoldi step k (SparseGMRES_NS.f90:289-327) from the partials of
-// |z_{k+1}|^2: H(k+1,k) = |z|, apply the previous rotations to column k,
-// form the new one, rotate e, test |e_{k+1}| <= eps and k+1 >= kmin.
-// k = -1 closes the initial residual instead (e_0 = |P^-1 b|, eps).
-// Column entries 0..k come from h1 + h2 (CGS2) or H (MGS).  Returns the
-// convergence flag; *s_next receives 1/|z| (the scale of basis vector k+1).
+// Close Arnoldi step k (SparseGMRES_NS.f90:289-327).  The product of step k
+// is stored unscaled, z = (P^-1 A) W_k = (P^-1 A v_k) / s_k, so every
+// coefficient found for z is multiplied by s_k here: H(j,k) = s_k h_j for
+// j <= k (h1 + h2 for CGS2, the MGS betas otherwise) and
+// H(k+1,k) = s_k |z_perp| from the partials of |z_perp|^2.  Then the previous
+// rotations are applied to column k, the new one is formed, e is rotated and
+// |e_{k+1}| <= eps and k+1 >= kmin is tested.  k = -1 closes the initial
+// residual instead (e_0 = |P^-1 b|, eps).  Returns the convergence flag.
 __device__ __forceinline__ bool arnoldi_close(GmresState *__restrict__ g,
                                               const double *__restrict__ part, int nb,
-                                              int k, int cgs, int32_t *hflag,
-                                              double *s_next) {
+                                              int k, int cgs, int32_t *hflag) {
   __shared__ double s_sum[1];
   __shared__ double s_res[2];
   __shared__ double s_col[kMaxKspace + 2];
@@ -60,8 +69,9 @@ __device__ __forceinline__ bool arnoldi_close(GmresState *__restrict__ g,
   if (k >= 0) {
     // stage column k and the previous rotations in LDS with all threads
     const int K1 = g->kspace + 1;
+    const double sk = g->s[k];
     for (int q = threadIdx.x; q <= k; q += blockDim.x) {
-      s_col[q] = cgs ? g->h1[q] + g->h2[q] : g->H[q + K1 * k];
+      s_col[q] = sk * (cgs ? g->h1[q] + g->h2[q] : g->H[q + K1 * k]);
       if (q < k) {
         s_rc[q] = g->rc[q];
         s_rs[q] = g->rs[q];
@@ -86,9 +96,8 @@ __device__ __forceinline__ bool arnoldi_close(GmresState *__restrict__ g,
         g->done = done ? 1 : 0;
       }
     } else {
-      const int K1 = g->kspace + 1;
       double *col = s_col;
-      col[k + 1] = nrm;
+      col[k + 1] = g->s[k] * nrm;
       for (int q = 0; q < k; ++q) {
         const double c = s_rc[q], s = s_rs[q];
         const double t = c * col[q] + s * col[q + 1];
@@ -125,7 +134,6 @@ __device__ __forceinline__ bool arnoldi_close(GmresState *__restrict__ g,
     const int K1 = g->kspace + 1;
     for (int q = threadIdx.x; q <= k + 1; q += blockDim.x) g->H[q + K1 * k] = s_col[q];
   }
-  *s_next = s_res[1];
   return s_res[0] != 0.0;
 }
 

@@ -99,27 +99,37 @@ static int ensure_krylov(Ctx &c, int K) {
   return OFD_OK;
 }
 
-// preconditioner (and FP32 copies) of the current matrix
+// FP32 copy of the system matrix (plain SpMV in FP32)
+static int ensure_a32(Ctx &c) {
+  OFD_CHECK(c.mat_valid, OFD_ERR_ARG, "no assembled matrix");
+  if (c.a32_valid) return OFD_OK;
+  if (!c.d_A32) {
+    int rc = dalloc(c, &c.d_A32, (size_t)c.icnt * 16);
+    if (rc) return rc;
+  }
+  OFD_HIP(launch_convert_a32(c));
+  c.a32_valid = true;
+  return OFD_OK;
+}
+
+// block-Jacobi inverse and the left-preconditioned matrix P^-1 A (FP64, and
+// FP32 when the mixed solver asks for it) of the current matrix
 static int ensure_prec(Ctx &c, int precision) {
   OFD_CHECK(c.mat_valid, OFD_ERR_ARG, "no assembled matrix");
-  if (precision == OFD_FP32 && !c.a32_valid) {
-    if (!c.d_A32) {
-      int rc = dalloc(c, &c.d_A32, (size_t)c.icnt * 16);
-      if (rc) return rc;
-    }
-    if (!c.d_P32) {
-      int rc = dalloc(c, &c.d_P32, (size_t)c.nnode * 16);
-      if (rc) return rc;
-    }
-    OFD_HIP(launch_convert_a32(c));
-    OFD_HIP(launch_build_prec(c, true));
-    c.a32_valid = true;
-    c.prec_valid = true;
+  const bool want32 = precision == OFD_FP32;
+  if (c.prec_valid && (!want32 || c.ap32_valid)) return OFD_OK;
+  if (!c.d_Ap64) {
+    int rc = dalloc(c, &c.d_Ap64, (size_t)c.icnt * 16);
+    if (rc) return rc;
   }
-  if (!c.prec_valid) {
-    OFD_HIP(launch_build_prec(c, c.d_P32 != nullptr));
-    c.prec_valid = true;
+  if (want32 && !c.d_Ap32) {
+    int rc = dalloc(c, &c.d_Ap32, (size_t)c.icnt * 16);
+    if (rc) return rc;
   }
+  OFD_HIP(launch_build_prec(c));
+  OFD_HIP(launch_precondition_matrix(c, want32));
+  c.prec_valid = true;
+  c.ap32_valid = want32;
   return OFD_OK;
 }
 
@@ -131,27 +141,30 @@ static int gmres_cycle(Ctx &c, int precision, int ortho, const double *b, double
   if (rc) return rc;
   rc = ensure_prec(c, precision);
   if (rc) return rc;
-  OFD_HIP(launch_gmres_init(c, precision, b, tol, K, kmin));
+  OFD_HIP(launch_gmres_init(c, b, tol, K, kmin));
   // The host enqueues iterations ahead of the device and learns about
-  // convergence through a pinned copy of the device flag made after every
-  // iteration; it waits only on the flag of iteration k - kLag, so the
-  // device queue never drains.  Iterations launched after convergence
-  // return at once (their kernels test the flag).
+  // convergence through a flag the device writes to pinned host memory when
+  // it closes each step; it waits only on the flag of step k - kLag - 1, so
+  // the device queue never drains.  Kernels launched after convergence
+  // return at once.
   constexpr int kLag = 3;
   for (int k = 0; k < K; ++k) c.h_flags[k] = 0;
   int klast = -1;
   for (int k = 0; k < K; ++k) {
-    // the SpMV of iteration j closes step j-1 and writes its flag to h_flags[j-1]
+    // the first orthogonalisation kernel of iteration j closes step j-1
     if (k >= kLag + 1) {
       OFD_HIP(hipEventSynchronize(c.iter_ev[k - kLag]));
       if (c.h_flags[k - kLag - 1]) break;
     }
-    OFD_HIP(launch_spmv_prec(c, precision, k, ortho));
-    OFD_HIP(hipEventRecord(c.iter_ev[k], c.stream));
+    OFD_HIP(launch_spmv_arnoldi(c, precision, k));
     if (ortho == OFD_ORTHO_MGS) {
-      for (int j = 0; j <= k + 1; ++j) OFD_HIP(launch_mgs_step(c, k, j));
+      for (int j = 0; j <= k + 1; ++j) {
+        OFD_HIP(launch_mgs_step(c, k, j));
+        if (j == 0) OFD_HIP(hipEventRecord(c.iter_ev[k], c.stream));
+      }
     } else {
       OFD_HIP(launch_cgs_pass(c, k, 1));
+      OFD_HIP(hipEventRecord(c.iter_ev[k], c.stream));
       OFD_HIP(launch_cgs_pass(c, k, 2));
       OFD_HIP(launch_cgs_pass(c, k, 3));
     }
@@ -255,7 +268,8 @@ int ofd_ctx_destroy(ofd_ctx *x) {
   dfree(c.d_A64);
   dfree(c.d_A32);
   dfree(c.d_P64);
-  dfree(c.d_P32);
+  dfree(c.d_Ap64);
+  dfree(c.d_Ap32);
   dfree(c.d_rhs);
   dfree(c.d_b);
   dfree(c.d_x);
@@ -294,7 +308,7 @@ int ofd_upload_mesh(ofd_ctx *x, int nnode, int nelem, int nshl, const int32_t *i
   c.nelem = nelem;
   c.nshl = nshl;
   c.icnt = 0;
-  c.mat_valid = c.prec_valid = c.a32_valid = false;
+  c.mat_valid = c.prec_valid = c.a32_valid = c.ap32_valid = false;
   int rc = dalloc(c, &c.d_ien, ne);
   if (!rc) rc = dalloc(c, &c.d_elm_id, (size_t)nelem);
   for (int f = 0; f < OFD_NFIELDS && !rc; ++f)
@@ -412,7 +426,8 @@ int ofd_build_pattern(ofd_ctx *x, int64_t *icnt_out) {
   if (!rc) rc = dalloc(c, &c.d_A64, (size_t)c.icnt * 16);
   if (rc) return rc;
   dfree(c.d_A32);
-  dfree(c.d_P32);
+  dfree(c.d_Ap64);
+  dfree(c.d_Ap32);
   rc = h2d(c, c.d_col, c.h_col.data(), (N + 1) * sizeof(int32_t));
   if (!rc) rc = h2d(c, c.d_row, c.h_row.data(), c.icnt * sizeof(int32_t));
   if (!rc) rc = h2d(c, c.d_diag, diag.data(), N * sizeof(int32_t));
@@ -421,7 +436,7 @@ int ofd_build_pattern(ofd_ctx *x, int64_t *icnt_out) {
   if (rc) return rc;
   OFD_HIP(hipMemsetAsync(c.d_A64, 0, (size_t)c.icnt * 16 * sizeof(double), c.stream));
   OFD_HIP(hipStreamSynchronize(c.stream));
-  c.mat_valid = c.prec_valid = c.a32_valid = false;
+  c.mat_valid = c.prec_valid = c.a32_valid = c.ap32_valid = false;
   if (icnt_out) *icnt_out = c.icnt;
   return OFD_OK;
 }
@@ -526,7 +541,7 @@ int ofd_assemble_ns(ofd_ctx *x, int tensor_flag, int subdomain_flag, double *cfl
   if (tensor_flag & OFD_ASSEMBLE_MAT) {
     OFD_HIP(launch_zero(c, c.d_A64, (size_t)c.icnt * 16));
     c.mat_valid = true;
-    c.prec_valid = c.a32_valid = false;
+    c.prec_valid = c.a32_valid = c.ap32_valid = false;
   }
   OFD_HIP(hipMemsetAsync(c.d_negdet, 0, sizeof(int32_t), c.stream));
   for (int col = 0; col + 1 < (int)c.color_off.size(); ++col)
@@ -620,7 +635,7 @@ int ofd_set_lhs(ofd_ctx *x, const double *k11, const double *g, const double *d1
   rc = h2d(c, c.d_A64, A.data(), A.size() * sizeof(double));
   if (rc) return rc;
   c.mat_valid = true;
-  c.prec_valid = c.a32_valid = false;
+  c.prec_valid = c.a32_valid = c.ap32_valid = false;
   return OFD_OK;
 }
 
@@ -634,7 +649,7 @@ int ofd_spmv_up(ofd_ctx *x, int precision, const double *xu, const double *xp,
   OFD_CHECK(c.mat_valid, OFD_ERR_ARG, "no assembled matrix");
   OFD_HIP(hipSetDevice(c.device));
   if (precision == OFD_FP32) {
-    rc = ensure_prec(c, OFD_FP32);
+    rc = ensure_a32(c);
     if (rc) return rc;
   }
   double *su = c.d_field[OFD_SOLU], *sp = c.d_field[OFD_SOLP];
@@ -808,7 +823,7 @@ int ofd_time_spmv(ofd_ctx *x, int precision, int reps, double *ms) {
   OFD_CHECK(c.mat_valid, OFD_ERR_ARG, "no assembled matrix");
   OFD_HIP(hipSetDevice(c.device));
   if (precision == OFD_FP32) {
-    rc = ensure_prec(c, OFD_FP32);
+    rc = ensure_a32(c);
     if (rc) return rc;
   }
   hipEvent_t e0, e1;
@@ -852,7 +867,7 @@ int ofd_time_assemble(ofd_ctx *x, int tensor_flag, int reps, double *ms) {
   (void)hipEventDestroy(e1);
   if (tensor_flag & OFD_ASSEMBLE_MAT) {
     c.mat_valid = true;
-    c.prec_valid = c.a32_valid = false;
+    c.prec_valid = c.a32_valid = c.ap32_valid = false;
   }
   *ms = (double)t / reps;
   return OFD_OK;

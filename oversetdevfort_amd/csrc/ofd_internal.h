@@ -45,8 +45,6 @@ void set_error(const std::string &msg);
 constexpr int kMaxKspace = 64;     // GMRES restart length upper bound
 constexpr int kRedBlock = 256;     // threads per reduction block
 constexpr int kMaxRedBlocks = 2048;
-// partial buffer read by the step that closes Arnoldi step k
-inline int close_buffer(int k, int ortho) { return ortho == OFD_ORTHO_MGS ? ((k + 1) & 1) : 0; }
 
 // per-solve scalars living on the device (one GMRES cycle)
 struct GmresState {
@@ -99,8 +97,9 @@ struct Ctx {
   double *d_A64 = nullptr;            // [icnt][16]
   float *d_A32 = nullptr;
   double *d_P64 = nullptr;            // [N][16]
-  float *d_P32 = nullptr;
-  bool mat_valid = false, prec_valid = false, a32_valid = false;
+  double *d_Ap64 = nullptr;           // [icnt][16] block-Jacobi-preconditioned P^-1 A
+  float *d_Ap32 = nullptr;
+  bool mat_valid = false, prec_valid = false, a32_valid = false, ap32_valid = false;
   double *d_rhs = nullptr;            // [N][4]
   double *d_b = nullptr;              // [N][4] right-hand side of a solve
   double *d_x = nullptr;              // [N][4] solution accumulator
@@ -113,6 +112,7 @@ struct Ctx {
   double *d_part = nullptr;           // 2 x [kMaxRedBlocks][kMaxKspace + 1] (double-buffered)
   GmresState *d_gs = nullptr;
   GmresState *h_gs = nullptr;         // pinned mirror
+  int part_cur = 0;                   // partial buffer holding the latest sums
   int32_t *h_flags = nullptr;         // pinned: per-iteration copy of GmresState::done
   std::vector<hipEvent_t> iter_ev;    // event after each iteration's flag copy
   double *h_pin = nullptr;            // pinned scratch (host transfers)
@@ -129,11 +129,12 @@ hipError_t launch_assemble_ns_hex(Ctx &c, int tensor_flag, int subdomain_flag,
 hipError_t launch_zero(Ctx &c, double *p, size_t n);
 hipError_t launch_max_reduce(Ctx &c, const double *v, int n, double *out_dev);
 // matrix
-hipError_t launch_build_prec(Ctx &c, bool want32);
+hipError_t launch_build_prec(Ctx &c);
+hipError_t launch_precondition_matrix(Ctx &c, bool want32);
 hipError_t launch_convert_a32(Ctx &c);
 hipError_t launch_spmv(Ctx &c, int precision, const double *x, double *y,
                        const double *b, int mode);
-hipError_t launch_spmv_prec(Ctx &c, int precision, int k, int ortho);
+hipError_t launch_spmv_arnoldi(Ctx &c, int precision, int k);
 // vectors
 hipError_t launch_soa_to_node4(Ctx &c, const double *u, const double *p,
                                double *out);
@@ -141,8 +142,7 @@ hipError_t launch_node4_to_soa(Ctx &c, const double *in, double *u, double *p);
 hipError_t launch_axpy_fields(Ctx &c, int which);
 hipError_t launch_apply_dirichlet(Ctx &c);
 // gmres pieces
-hipError_t launch_gmres_init(Ctx &c, int precision, const double *b,
-                             double tol, int kspace, int kmin);
+hipError_t launch_gmres_init(Ctx &c, const double *b, double tol, int kspace, int kmin);
 hipError_t launch_mgs_step(Ctx &c, int k, int j);
 hipError_t launch_cgs_pass(Ctx &c, int k, int pass);
 hipError_t launch_gmres_close(Ctx &c, int k, int ortho);

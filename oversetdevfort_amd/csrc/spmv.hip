@@ -13,9 +13,10 @@
 // at HBM rate; x is gathered from the 27 neighbours and stays in L2.
 //
 // The block-Jacobi inverse (SparseGMRES_NS.f90:53-121) is kept as a 4x4
-// block-diagonal [Binv 0; 0 1/M] so that applying it is a quad-lane 4x4
-// product fused into the SpMV epilogue (Arnoldi step z = P^-1 A v).
-#include "gmres_dev.h"
+// block-diagonal [Binv 0; 0 1/M].  Left preconditioning is applied once per
+// assembled matrix: the Krylov loop runs on the stored product P^-1 A (same
+// pattern, FP64 and FP32 copies), so its SpMV has no epilogue.
+#include "ofd_internal.h"
 
 namespace ofd {
 
@@ -59,21 +60,15 @@ __device__ __forceinline__ void load_x4(const double *x, double &x0, double &x1,
 
 }  // namespace
 
-// mode 0: y = A x; mode 1: y = b - A x; mode 2: Arnoldi product
-// y = s_k P^-1 (A W_k), whose prologue first closes Arnoldi step k-1 from the
-// partials the previous sweep left in `part` (k = 0: the initial residual)
+// mode 0: y = A x; mode 1: y = b - A x; mode 2: Arnoldi product y = A x that
+// returns at once when the cycle has converged
 template <typename T, int MODE>
 __global__ void __launch_bounds__(256)
 k_spmv(int N, const int32_t *__restrict__ col, const int32_t *__restrict__ row,
        const T *__restrict__ A, const double *__restrict__ x,
        double *__restrict__ y, const double *__restrict__ b,
-       const T *__restrict__ Pinv, GmresState *__restrict__ gs, int kidx,
-       const double *__restrict__ part, int nb, int cgs, int32_t *hflag) {
-  double s = 1.0;
-  if (MODE == 2) {
-    if (gs->done) return;
-    if (arnoldi_close(gs, part, nb, kidx - 1, cgs, hflag, &s)) return;
-  }
+       const GmresState *__restrict__ gs) {
+  if (MODE == 2 && gs->done) return;
   const int gt = blockIdx.x * blockDim.x + threadIdx.x;
   const int r = gt >> 2;
   const int c = gt & 3;
@@ -88,27 +83,16 @@ k_spmv(int N, const int32_t *__restrict__ col, const int32_t *__restrict__ row,
     load_x4(x + (size_t)cn * 4, x0, x1, x2, x3);
     acc += a0 * x0 + a1 * x1 + a2 * x2 + a3 * x3;
   }
-  if (MODE == 0) {
-    y[(size_t)r * 4 + c] = acc;
-  } else if (MODE == 1) {
+  if (MODE == 1)
     y[(size_t)r * 4 + c] = b[(size_t)r * 4 + c] - acc;
-  } else {
-    const int base = threadIdx.x & ~3;
-    const double z0 = __shfl(acc, base + 0);
-    const double z1 = __shfl(acc, base + 1);
-    const double z2 = __shfl(acc, base + 2);
-    const double z3 = __shfl(acc, base + 3);
-    double p0, p1, p2, p3;
-    Row4<T>::load(Pinv + (size_t)r * 16 + c * 4, p0, p1, p2, p3);
-    y[(size_t)r * 4 + c] = s * (p0 * z0 + p1 * z1 + p2 * z2 + p3 * z3);
-  }
+  else
+    y[(size_t)r * 4 + c] = acc;
 }
 
 // block-Jacobi inverse: 3x3 cofactor inverse of the K11 diagonal block and
 // 1/M (0 -> 1), SparseGMRES_NS.f90:73-121
 __global__ void k_build_prec(int N, const int32_t *__restrict__ diag,
-                             const double *__restrict__ A, double *__restrict__ P,
-                             float *__restrict__ P32) {
+                             const double *__restrict__ A, double *__restrict__ P) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const double *blk = A + (size_t)diag[i] * 16;
@@ -146,12 +130,31 @@ __global__ void k_build_prec(int N, const int32_t *__restrict__ diag,
   double2 *o = reinterpret_cast<double2 *>(P + (size_t)i * 16);
 #pragma unroll
   for (int t = 0; t < 8; ++t) o[t] = make_double2(out[2 * t], out[2 * t + 1]);
-  if (P32) {
-    float4 *o32 = reinterpret_cast<float4 *>(P32 + (size_t)i * 16);
+}
+
+// Ap[k] = P_i^-1 A[k] for every nonzero k of block row i; thread (i, c)
+// forms row c of the row's blocks
+__global__ void k_precondition_matrix(int N, const int32_t *__restrict__ col,
+                                      const double *__restrict__ P,
+                                      const double *__restrict__ A,
+                                      double *__restrict__ Ap, float *__restrict__ Ap32) {
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = gt >> 2, c = gt & 3;
+  if (i >= N) return;
+  const double *p = P + (size_t)i * 16 + c * 4;
+  const double p0 = p[0], p1 = p[1], p2 = p[2], p3 = p[3];
+  for (int k = col[i]; k < col[i + 1]; ++k) {
+    const double *a = A + (size_t)k * 16;
+    double o[4];
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
-      o32[t] = make_float4((float)out[4 * t], (float)out[4 * t + 1],
-                           (float)out[4 * t + 2], (float)out[4 * t + 3]);
+    for (int q = 0; q < 4; ++q)
+      o[q] = p0 * a[0 + q] + p1 * a[4 + q] + p2 * a[8 + q] + p3 * a[12 + q];
+    double2 *d = reinterpret_cast<double2 *>(Ap + (size_t)k * 16 + c * 4);
+    d[0] = make_double2(o[0], o[1]);
+    d[1] = make_double2(o[2], o[3]);
+    if (Ap32)
+      reinterpret_cast<float4 *>(Ap32 + (size_t)k * 16 + c * 4)[0] =
+          make_float4((float)o[0], (float)o[1], (float)o[2], (float)o[3]);
   }
 }
 
@@ -167,10 +170,17 @@ __global__ void k_convert_a32(size_t n4, const double *__restrict__ A,
   }
 }
 
-hipError_t launch_build_prec(Ctx &c, bool want32) {
+hipError_t launch_build_prec(Ctx &c) {
   const int N = c.nnode;
   hipLaunchKernelGGL(k_build_prec, dim3((N + 255) / 256), dim3(256), 0, c.stream, N,
-                     c.d_diag, c.d_A64, c.d_P64, want32 ? c.d_P32 : nullptr);
+                     c.d_diag, c.d_A64, c.d_P64);
+  return hipGetLastError();
+}
+
+hipError_t launch_precondition_matrix(Ctx &c, bool want32) {
+  const unsigned nb = (unsigned)(((size_t)c.nnode * 4 + 255) / 256);
+  hipLaunchKernelGGL(k_precondition_matrix, dim3(nb), dim3(256), 0, c.stream, c.nnode,
+                     c.d_col, c.d_P64, c.d_A64, c.d_Ap64, want32 ? c.d_Ap32 : nullptr);
   return hipGetLastError();
 }
 
@@ -193,43 +203,33 @@ hipError_t launch_spmv(Ctx &c, int precision, const double *x, double *y,
   if (precision == OFD_FP32) {
     if (mode == 0)
       hipLaunchKernelGGL((k_spmv<float, 0>), dim3(nb), dim3(256), 0, c.stream, c.nnode,
-                         c.d_col, c.d_row, c.d_A32, x, y, b, c.d_P32, c.d_gs, 0,
-                         nullptr, 0, 0, nullptr);
+                         c.d_col, c.d_row, c.d_A32, x, y, b, c.d_gs);
     else
       hipLaunchKernelGGL((k_spmv<float, 1>), dim3(nb), dim3(256), 0, c.stream, c.nnode,
-                         c.d_col, c.d_row, c.d_A32, x, y, b, c.d_P32, c.d_gs, 0,
-                         nullptr, 0, 0, nullptr);
+                         c.d_col, c.d_row, c.d_A32, x, y, b, c.d_gs);
   } else {
     if (mode == 0)
       hipLaunchKernelGGL((k_spmv<double, 0>), dim3(nb), dim3(256), 0, c.stream,
-                         c.nnode, c.d_col, c.d_row, c.d_A64, x, y, b, c.d_P64,
-                         c.d_gs, 0, nullptr, 0, 0, nullptr);
+                         c.nnode, c.d_col, c.d_row, c.d_A64, x, y, b, c.d_gs);
     else
       hipLaunchKernelGGL((k_spmv<double, 1>), dim3(nb), dim3(256), 0, c.stream,
-                         c.nnode, c.d_col, c.d_row, c.d_A64, x, y, b, c.d_P64,
-                         c.d_gs, 0, nullptr, 0, 0, nullptr);
+                         c.nnode, c.d_col, c.d_row, c.d_A64, x, y, b, c.d_gs);
   }
   return hipGetLastError();
 }
 
-// Arnoldi product: W_{k+1} = s_k P^-1 A W_k (closes step k-1 first)
-hipError_t launch_spmv_prec(Ctx &c, int precision, int k, int ortho) {
+// Arnoldi product W_{k+1} = (P^-1 A) W_k, unscaled
+hipError_t launch_spmv_arnoldi(Ctx &c, int precision, int k) {
   const unsigned nb = spmv_blocks(c);
   const size_t vs = (size_t)c.nnode * 4;
   const double *x = c.d_W + vs * k;
   double *y = c.d_W + vs * (k + 1);
-  const double *part =
-      c.d_part + (size_t)kMaxRedBlocks * kPartStride * close_buffer(k - 1, ortho);
-  const int pnb = nblocks_nodes(c);
-  const int cgs = ortho == OFD_ORTHO_MGS ? 0 : 1;
   if (precision == OFD_FP32)
     hipLaunchKernelGGL((k_spmv<float, 2>), dim3(nb), dim3(256), 0, c.stream, c.nnode,
-                       c.d_col, c.d_row, c.d_A32, x, y, nullptr, c.d_P32, c.d_gs, k, part,
-                       pnb, cgs, c.h_flags);
+                       c.d_col, c.d_row, c.d_Ap32, x, y, nullptr, c.d_gs);
   else
     hipLaunchKernelGGL((k_spmv<double, 2>), dim3(nb), dim3(256), 0, c.stream, c.nnode,
-                       c.d_col, c.d_row, c.d_A64, x, y, nullptr, c.d_P64, c.d_gs, k, part,
-                       pnb, cgs, c.h_flags);
+                       c.d_col, c.d_row, c.d_Ap64, x, y, nullptr, c.d_gs);
   return hipGetLastError();
 }
 
