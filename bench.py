@@ -41,7 +41,7 @@ def spmv_bytes(Z, N, s_a, s_x=8, s_y=8, b=4, S=16):
     return Z * (S * s_a + 4) + 4 * (N + 1) + N * b * (s_x + s_y)
 
 
-def run_steps(ctx, S, L, prob, precision, steps, warmup, kspace=30):
+def run_steps(ctx, S, L, prob, precision, steps, warmup, kspace=30, ortho=1):
     """Reset to the initial state, run warmup + timed time steps."""
     ctx.set_state(prob.fields)
     newton_ms, step_ms, iters, cycles, rel, res = [], [], [], [], [], []
@@ -50,7 +50,7 @@ def run_steps(ctx, S, L, prob, precision, steps, warmup, kspace=30):
         S.begin_step(ctx)
         t1 = time.perf_counter()
         info = S.newton_step(ctx, rtol=1e-8, precision=precision,
-                             ortho=L.OFD_ORTHO_CGS2, kspace=kspace, max_cycles=400,
+                             ortho=ortho, kspace=kspace, max_cycles=400,
                              raise_noconv=False)
         if not info["converged"]:
             h = ctx.solve_history()
@@ -146,6 +146,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the host baseline")
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--kspace", type=int, default=30, help="GMRES restart length")
+    ap.add_argument("--ortho", type=str, default="cgs2", choices=["mgs", "cgs2", "cgs1"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs one GPU as a single rank; --gpus {args.gpus} "
@@ -167,11 +168,13 @@ def main():
     log(f"bench: setup {time.perf_counter() - t0:.2f} s, Z = {Z} blocks, "
         f"device bytes {ctx.device_bytes() / 2**20:.1f} MiB")
 
+    ortho = {"mgs": L.OFD_ORTHO_MGS, "cgs2": L.OFD_ORTHO_CGS2,
+             "cgs1": L.OFD_ORTHO_CGS1}[args.ortho]
     res64 = run_steps(ctx, S, L, prob, L.OFD_FP64, args.steps, args.warmup,
-                      kspace=args.kspace)
+                      kspace=args.kspace, ortho=ortho)
     log(f"bench: FP64 newton {res64['newton_ms']:.2f} ms")
     res32 = run_steps(ctx, S, L, prob, L.OFD_FP32, args.steps, args.warmup,
-                      kspace=args.kspace)
+                      kspace=args.kspace, ortho=ortho)
     log(f"bench: mixed newton {res32['newton_ms']:.2f} ms")
 
     # kernel-level figures on the context stream (HIP events)
@@ -234,7 +237,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": (f"channel [0,4]x[0,1]^2, {nx}x{ny}x{nz} hex, Re=1000, "
                                 "dt=0.005, 1 Newton step/time step, rtol 1e-8, "
-                                f"block-Jacobi GMRES({args.kspace})"),
+                                f"block-Jacobi GMRES({args.kspace}), {args.ortho}"),
                    "elements": E, "nodes": N, "dof": 4 * N, "nnz_blocks": Z},
         "newton_ms": {"mixed": res32["newton_ms"], "fp64": res64["newton_ms"]},
         "speedup_mixed_vs_fp64": res64["newton_ms"] / res32["newton_ms"],

@@ -49,7 +49,8 @@ enum { OFD_FP64 = 0, OFD_FP32 = 1 };
 /* Gram-Schmidt variant of the Arnoldi loop */
 enum {
   OFD_ORTHO_MGS = 0,  /* SparseGMRES_NS.f90:245-287 sequence, k+1 sweeps */
-  OFD_ORTHO_CGS2 = 1  /* classical GS with one re-orthogonalisation, 3 fused sweeps */
+  OFD_ORTHO_CGS2 = 1, /* classical GS with one re-orthogonalisation */
+  OFD_ORTHO_CGS1 = 2  /* classical GS, one projection (restart cycles are short) */
 };
 
 /* node fields held by a context, all (NNODE,3) except OFD_PG/OFD_PGOLD (NNODE) */

@@ -20,6 +20,7 @@ OFD_FP64 = 0
 OFD_FP32 = 1
 OFD_ORTHO_MGS = 0
 OFD_ORTHO_CGS2 = 1
+OFD_ORTHO_CGS1 = 2
 
 FIELDS = ["XG", "DG", "DGOLD", "UG", "UGOLD", "UGM", "UGMOLD", "ACG", "ACGOLD",
           "ACGM", "ACGMOLD", "PG", "PGOLD", "SOLU", "SOLP"]

@@ -19,14 +19,6 @@ namespace ofd {
 
 namespace {
 
-constexpr int kCh = 16;  // dot products per reduction chunk
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m);
-  return v;
-}
-
 __device__ __forceinline__ void ld4(const double *p, double v[4]) {
   const double2 a = reinterpret_cast<const double2 *>(p)[0];
   const double2 b = reinterpret_cast<const double2 *>(p)[1];
@@ -40,29 +32,6 @@ __device__ __forceinline__ void st4(double *p, const double v[4]) {
   reinterpret_cast<double2 *>(p)[0] = make_double2(v[0], v[1]);
   reinterpret_cast<double2 *>(p)[1] = make_double2(v[2], v[3]);
 }
-
-// Reduce acc[0..n) over the block into part[blockIdx][off + j].
-template <int NV>
-__device__ void block_partials(const double (&acc)[NV], int n, double *part, int stride,
-                               int off) {
-  __shared__ double s_red[4][kCh];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-  for (int j = 0; j < NV; ++j) {
-    if (j < n) {
-      const double v = wave_sum(acc[j]);
-      if (lane == 0) s_red[w][j] = v;
-    }
-  }
-  __syncthreads();
-  if ((int)threadIdx.x < n) {
-    const int j = threadIdx.x;
-    part[(size_t)blockIdx.x * stride + off + j] =
-        s_red[0][j] + s_red[1][j] + s_red[2][j] + s_red[3][j];
-  }
-  __syncthreads();
-}
-
 
 }  // namespace
 
@@ -97,7 +66,7 @@ k_gmres_init(int N, const double *__restrict__ P, const double *__restrict__ b,
     st4(W0 + (size_t)i * 4, z);
     acc[0] += z[0] * z[0] + z[1] * z[1] + z[2] * z[2] + z[3] * z[3];
   }
-  block_partials<1>(acc, 1, part, kPartStride, 0);
+  block_partials<1>(acc, 1, part, 0);
 }
 
 // One MGS sweep j of Arnoldi step k (SparseGMRES_NS.f90:246-287) on the
@@ -146,7 +115,7 @@ k_mgs_step(int N, double *__restrict__ W, const double *__restrict__ rd,
     }
     acc[0] += zz[0] * v[0] + zz[1] * v[1] + zz[2] * v[2] + zz[3] * v[3];
   }
-  block_partials<1>(acc, 1, wr, kPartStride, 0);
+  block_partials<1>(acc, 1, wr, 0);
 }
 
 // CGS2 sweeps of Arnoldi step k on the unscaled product z = (P^-1 A) W_k:
@@ -213,7 +182,7 @@ k_cgs_pass(int N, double *__restrict__ W, const double *__restrict__ rd,
       acc[0] += zz[0] * zz[0] + zz[1] * zz[1] + zz[2] * zz[2] + zz[3] * zz[3];
     }
     if (pass == 3) {
-      block_partials<1>(acc, 1, wr, kPartStride, 0);
+      block_partials<1>(acc, 1, wr, 0);
       return;
     }
   }
@@ -236,7 +205,106 @@ k_cgs_pass(int N, double *__restrict__ W, const double *__restrict__ rd,
         }
       }
     }
-    block_partials<kCh>(acc, n, wr, kPartStride, j0);
+    block_partials<kCh>(acc, n, wr, j0);
+  }
+}
+
+// Update sweeps of the CGS variants whose first projection (dots of the
+// unscaled product z = (P^-1 A) W_k against W_0..W_k) came from the SpMV
+// epilogue (partials in `rd_dots`):
+//   mode 1 (CGS1):        close step k-1; h1 = <z, v_j>, h2 = 0;
+//                         z -= sum h1_j v_j; partials of |z|^2 -> wr
+//   mode 2 (CGS2 first):  close step k-1; h1 = <z, v_j>; z -= sum h1_j v_j;
+//                         partials of <z, W_j> -> wr
+//   mode 3 (CGS2 second): h2 = <z, v_j>; z -= sum h2_j v_j; partials of |z|^2 -> wr
+__global__ void __launch_bounds__(256)
+k_cgs_update(int N, double *__restrict__ W, const double *__restrict__ rd_norm,
+             const double *__restrict__ rd_dots, double *__restrict__ wr,
+             GmresState *__restrict__ g, int k, int mode, int nb, int32_t *hflag) {
+  if (g->done) return;
+  const size_t vs = (size_t)N * 4;
+  double *z = W + vs * (k + 1);
+  const int m = k + 1;
+  __shared__ double s_coef[kMaxKspace + 1];
+  double sk = g->s[k];
+  if (mode != 3) {
+    if (arnoldi_close(g, rd_norm, nb, k - 1, 1, hflag, &sk)) return;
+  }
+  final_sums(rd_dots, nb, m, s_coef);
+  for (int j = threadIdx.x; j < m; j += blockDim.x) {
+    const double sj = (j == k) ? sk : g->s[j];
+    const double h = s_coef[j] * sj;
+    if (blockIdx.x == 0) {
+      if (mode == 3) {
+        g->h2[j] = h;
+      } else {
+        g->h1[j] = h;
+        if (mode == 1) g->h2[j] = 0.0;
+      }
+    }
+    s_coef[j] = h * sj;
+  }
+  __syncthreads();
+  double acc[1] = {0.0};
+  double dacc[kCh];
+#pragma unroll
+  for (int t = 0; t < kCh; ++t) dacc[t] = 0.0;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N) {
+    double zz[4];
+    ld4(z + (size_t)i * 4, zz);
+    int j = 0;
+    for (; j + 4 <= m; j += 4) {
+      double v0[4], v1[4], v2[4], v3[4];
+      ld4(W + vs * (j + 0) + (size_t)i * 4, v0);
+      ld4(W + vs * (j + 1) + (size_t)i * 4, v1);
+      ld4(W + vs * (j + 2) + (size_t)i * 4, v2);
+      ld4(W + vs * (j + 3) + (size_t)i * 4, v3);
+      const double c0 = s_coef[j], c1 = s_coef[j + 1], c2 = s_coef[j + 2],
+                   c3 = s_coef[j + 3];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        zz[c] -= c0 * v0[c];
+        zz[c] -= c1 * v1[c];
+        zz[c] -= c2 * v2[c];
+        zz[c] -= c3 * v3[c];
+      }
+    }
+    for (; j < m; ++j) {
+      double v[4];
+      ld4(W + vs * j + (size_t)i * 4, v);
+      const double cj = s_coef[j];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) zz[c] -= cj * v[c];
+    }
+    st4(z + (size_t)i * 4, zz);
+    acc[0] = zz[0] * zz[0] + zz[1] * zz[1] + zz[2] * zz[2] + zz[3] * zz[3];
+    if (mode == 2) {
+      // the dots of the second projection reuse zz; chunked below
+      (void)dacc;
+    }
+  }
+  if (mode != 2) {
+    block_partials<1>(acc, 1, wr, 0);
+    return;
+  }
+  for (int j0 = 0; j0 < m; j0 += kCh) {
+    const int n = (m - j0 < kCh) ? (m - j0) : kCh;
+#pragma unroll
+    for (int t = 0; t < kCh; ++t) dacc[t] = 0.0;
+    if (i < N) {
+      double zz[4];
+      ld4(z + (size_t)i * 4, zz);
+#pragma unroll
+      for (int t = 0; t < kCh; ++t) {
+        if (t < n) {
+          double v[4];
+          ld4(W + vs * (j0 + t) + (size_t)i * 4, v);
+          dacc[t] = zz[0] * v[0] + zz[1] * v[1] + zz[2] * v[2] + zz[3] * v[3];
+        }
+      }
+    }
+    block_partials<kCh>(dacc, n, wr, j0);
   }
 }
 
@@ -258,7 +326,7 @@ k_norm2(int N, const double *__restrict__ v, double *__restrict__ part) {
     ld4(v + (size_t)i * 4, z);
     acc[0] += z[0] * z[0] + z[1] * z[1] + z[2] * z[2] + z[3] * z[3];
   }
-  block_partials<1>(acc, 1, part, kPartStride, 0);
+  block_partials<1>(acc, 1, part, 0);
 }
 
 __global__ void __launch_bounds__(256)
@@ -355,6 +423,20 @@ hipError_t launch_cgs_pass(Ctx &c, int k, int pass) {
                      part_buf(c, c.part_cur), part_buf(c, c.part_cur ^ 1), c.d_gs, k, pass,
                      nb, c.h_flags);
   c.part_cur ^= 1;
+  return hipGetLastError();
+}
+
+// partial buffers: 0/1 alternate for the MGS / 3-pass CGS2 chain and for
+// the closing norms of the fused CGS variants (parity of k), 2 holds the
+// SpMV-epilogue dots, 3 the dots of the second CGS2 projection
+hipError_t launch_cgs_update(Ctx &c, int k, int mode) {
+  const int nb = nblocks_nodes(c);
+  const double *rd_norm = part_buf(c, k & 1);
+  const double *rd_dots = part_buf(c, mode == 3 ? 3 : 2);
+  double *wr = (mode == 2) ? part_buf(c, 3) : part_buf(c, (k + 1) & 1);
+  hipLaunchKernelGGL(k_cgs_update, dim3(nb), dim3(kRedBlock), 0, c.stream, c.nnode,
+                     c.d_W, rd_norm, rd_dots, wr, c.d_gs, k, mode, nb, c.h_flags);
+  c.part_cur = (k + 1) & 1;
   return hipGetLastError();
 }
 

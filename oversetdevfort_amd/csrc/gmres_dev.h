@@ -13,6 +13,38 @@
 namespace ofd {
 
 constexpr int kPartStride = kMaxKspace + 1;
+constexpr int kCh = 16;  // dot products per reduction chunk
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m);
+  return v;
+}
+
+// Reduce acc[0..n) over the block (blockDim.x = 64 * nw, nw <= 16) into
+// part[blockIdx][off + j], waves added in a fixed order.
+template <int NV>
+__device__ __forceinline__ void block_partials(const double (&acc)[NV], int n, double *part,
+                                               int off) {
+  __shared__ double s_red[16][kCh > NV ? kCh : NV];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+#pragma unroll
+  for (int j = 0; j < NV; ++j) {
+    if (j < n) {
+      const double v = wave_sum(acc[j]);
+      if (lane == 0) s_red[w][j] = v;
+    }
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < n) {
+    const int j = threadIdx.x;
+    double r = 0.0;
+    for (int q = 0; q < nw; ++q) r += s_red[q][j];
+    part[(size_t)blockIdx.x * kPartStride + off + j] = r;
+  }
+  __syncthreads();
+}
 
 // s_out[j] = sum over b < nb of part[b][j], j < m <= 65 (all block threads).
 // Thread (g, j) adds blocks g, g+G, ... with G = 256 / jw groups (jw = m
@@ -57,10 +89,12 @@ __device__ __forceinline__ void final_sums(const double *part, int nb, int m,
 // H(k+1,k) = s_k |z_perp| from the partials of |z_perp|^2.  Then the previous
 // rotations are applied to column k, the new one is formed, e is rotated and
 // |e_{k+1}| <= eps and k+1 >= kmin is tested.  k = -1 closes the initial
-// residual instead (e_0 = |P^-1 b|, eps).  Returns the convergence flag.
+// residual instead (e_0 = |P^-1 b|, eps).  Returns the convergence flag;
+// *s_next (if given) receives s_{k+1} in every block.
 __device__ __forceinline__ bool arnoldi_close(GmresState *__restrict__ g,
                                               const double *__restrict__ part, int nb,
-                                              int k, int cgs, int32_t *hflag) {
+                                              int k, int cgs, int32_t *hflag,
+                                              double *s_next = nullptr) {
   __shared__ double s_sum[1];
   __shared__ double s_res[2];
   __shared__ double s_col[kMaxKspace + 2];
@@ -134,6 +168,7 @@ __device__ __forceinline__ bool arnoldi_close(GmresState *__restrict__ g,
     const int K1 = g->kspace + 1;
     for (int q = threadIdx.x; q <= k + 1; q += blockDim.x) g->H[q + K1 * k] = s_col[q];
   }
+  if (s_next) *s_next = s_res[1];
   return s_res[0] != 0.0;
 }
 

@@ -156,17 +156,23 @@ static int gmres_cycle(Ctx &c, int precision, int ortho, const double *b, double
       OFD_HIP(hipEventSynchronize(c.iter_ev[k - kLag]));
       if (c.h_flags[k - kLag - 1]) break;
     }
-    OFD_HIP(launch_spmv_arnoldi(c, precision, k));
     if (ortho == OFD_ORTHO_MGS) {
+      OFD_HIP(launch_spmv_arnoldi(c, precision, k));
       for (int j = 0; j <= k + 1; ++j) {
         OFD_HIP(launch_mgs_step(c, k, j));
         if (j == 0) OFD_HIP(hipEventRecord(c.iter_ev[k], c.stream));
       }
     } else {
-      OFD_HIP(launch_cgs_pass(c, k, 1));
-      OFD_HIP(hipEventRecord(c.iter_ev[k], c.stream));
-      OFD_HIP(launch_cgs_pass(c, k, 2));
-      OFD_HIP(launch_cgs_pass(c, k, 3));
+      // product and first projection in one sweep, then the update sweep(s)
+      OFD_HIP(launch_spmv_dots(c, precision, k));
+      if (ortho == OFD_ORTHO_CGS1) {
+        OFD_HIP(launch_cgs_update(c, k, 1));
+        OFD_HIP(hipEventRecord(c.iter_ev[k], c.stream));
+      } else {
+        OFD_HIP(launch_cgs_update(c, k, 2));
+        OFD_HIP(hipEventRecord(c.iter_ev[k], c.stream));
+        OFD_HIP(launch_cgs_update(c, k, 3));
+      }
     }
     klast = k;
   }
@@ -222,7 +228,7 @@ int ofd_ctx_create(int device, ofd_ctx **out) {
     return OFD_ERR_HIP;
   }
   int rc = dalloc(c, &c.d_gs, 1);
-  if (!rc) rc = dalloc(c, &c.d_part, (size_t)2 * kMaxRedBlocks * (kMaxKspace + 1));
+  if (!rc) rc = dalloc(c, &c.d_part, (size_t)4 * kMaxRedBlocks * (kMaxKspace + 1));
   if (!rc) rc = dalloc(c, &c.d_negdet, 1);
   if (rc) {
     ofd_ctx_destroy(x);
@@ -292,6 +298,8 @@ int ofd_upload_mesh(ofd_ctx *x, int nnode, int nelem, int nshl, const int32_t *i
                     const int32_t *elm_id, const double *xg) {
   OFD_CHECK(x && ien && elm_id && xg, OFD_ERR_ARG, "null argument");
   OFD_CHECK(nnode > 0 && nelem > 0, OFD_ERR_ARG, "empty mesh");
+  OFD_CHECK(nnode <= kMaxRedBlocks * kRedBlock, OFD_ERR_UNSUPPORTED,
+            "more than 1,048,576 nodes per context");
   OFD_CHECK(nshl == 8, OFD_ERR_UNSUPPORTED,
             "only trilinear hex elements (NSHL=8) are implemented on the GPU");
   Ctx &c = x->c;
@@ -670,7 +678,8 @@ int ofd_gmres_up(ofd_ctx *x, int precision, int ortho, double tol, int kspace,
                  int kspace_min, double *solu, double *solp, int *iters, double *resid) {
   OFD_CHECK(x, OFD_ERR_ARG, "null context");
   OFD_CHECK(precision == OFD_FP64 || precision == OFD_FP32, OFD_ERR_ARG, "bad precision");
-  OFD_CHECK(ortho == OFD_ORTHO_MGS || ortho == OFD_ORTHO_CGS2, OFD_ERR_ARG, "bad ortho");
+  OFD_CHECK(ortho == OFD_ORTHO_MGS || ortho == OFD_ORTHO_CGS2 || ortho == OFD_ORTHO_CGS1,
+            OFD_ERR_ARG, "bad ortho");
   Ctx &c = x->c;
   int rc = need_pattern(c);
   if (rc) return rc;

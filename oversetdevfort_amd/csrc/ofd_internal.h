@@ -44,7 +44,7 @@ void set_error(const std::string &msg);
 
 constexpr int kMaxKspace = 64;     // GMRES restart length upper bound
 constexpr int kRedBlock = 256;     // threads per reduction block
-constexpr int kMaxRedBlocks = 2048;
+constexpr int kMaxRedBlocks = 4096;  // node blocks of 256: N <= 1,048,576
 
 // per-solve scalars living on the device (one GMRES cycle)
 struct GmresState {
@@ -109,7 +109,7 @@ struct Ctx {
   // GMRES workspace
   int kalloc = 0;
   double *d_W = nullptr;              // [(kalloc+1)][N][4]
-  double *d_part = nullptr;           // 2 x [kMaxRedBlocks][kMaxKspace + 1] (double-buffered)
+  double *d_part = nullptr;           // 4 x [kMaxRedBlocks][kMaxKspace + 1] partial-sum buffers
   GmresState *d_gs = nullptr;
   GmresState *h_gs = nullptr;         // pinned mirror
   int part_cur = 0;                   // partial buffer holding the latest sums
@@ -135,6 +135,8 @@ hipError_t launch_convert_a32(Ctx &c);
 hipError_t launch_spmv(Ctx &c, int precision, const double *x, double *y,
                        const double *b, int mode);
 hipError_t launch_spmv_arnoldi(Ctx &c, int precision, int k);
+hipError_t launch_spmv_dots(Ctx &c, int precision, int k);
+hipError_t launch_cgs_update(Ctx &c, int k, int mode);
 // vectors
 hipError_t launch_soa_to_node4(Ctx &c, const double *u, const double *p,
                                double *out);
