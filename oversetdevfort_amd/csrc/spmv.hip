@@ -96,7 +96,7 @@ k_spmv(int N, const int32_t *__restrict__ col, const int32_t *__restrict__ row,
 template <typename T>
 __global__ void __launch_bounds__(1024)
 k_spmv_dots(int N, const int32_t *__restrict__ col, const int32_t *__restrict__ row,
-            const T *__restrict__ A, const double *__restrict__ W, int k,
+            const T *__restrict__ A, double *__restrict__ W, int k,
             double *__restrict__ part, const GmresState *__restrict__ gs) {
   if (gs->done) return;
   const size_t vs = (size_t)N * 4;
