@@ -118,121 +118,56 @@ k_mgs_step(int N, double *__restrict__ W, const double *__restrict__ rd,
   block_partials<1>(acc, 1, wr, 0);
 }
 
-// CGS2 sweeps of Arnoldi step k on the unscaled product z = (P^-1 A) W_k:
-//   pass 1: close step k-1; partials of <z, W_j>            j = 0..k
-//   pass 2: h1_j = <z, v_j>; z -= sum h1_j v_j; partials of <z, W_j>
-//   pass 3: h2_j = <z, v_j>; z -= sum h2_j v_j; partials of |z|^2
-// Each pass reads the partials of the previous kernel from `rd` and writes
-// its own to `wr`.
+// First projection of the CGS variants: close step k-1 (norm partials in
+// `rd`), then partials of <z, W_j>, j = 0..k, of the unscaled product
+// z = (P^-1 A) W_k into `wr`.
 __global__ void __launch_bounds__(256)
-k_cgs_pass(int N, double *__restrict__ W, const double *__restrict__ rd,
-           double *__restrict__ wr, GmresState *__restrict__ g, int k, int pass, int nb,
+k_cgs_dots(int N, const double *__restrict__ W, const double *__restrict__ rd,
+           double *__restrict__ wr, GmresState *__restrict__ g, int k, int nb,
            int32_t *hflag) {
   if (g->done) return;
+  if (arnoldi_close(g, rd, nb, k - 1, 1, hflag)) return;
   const size_t vs = (size_t)N * 4;
-  double *z = W + vs * (k + 1);
+  const double *z = W + vs * (k + 1);
   const int m = k + 1;
-  __shared__ double s_coef[kMaxKspace + 1];
-  if (pass == 1) {
-    if (arnoldi_close(g, rd, nb, k - 1, 1, hflag)) return;
-  } else {
-    final_sums(rd, nb, m, s_coef);
-    for (int j = threadIdx.x; j < m; j += blockDim.x) {
-      const double h = s_coef[j] * g->s[j];
-      if (blockIdx.x == 0) {
-        if (pass == 2)
-          g->h1[j] = h;
-        else
-          g->h2[j] = h;
-      }
-      s_coef[j] = h * g->s[j];
-    }
-    __syncthreads();
-    double acc[1] = {0.0};
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N;
-         i += gridDim.x * blockDim.x) {
-      double zz[4];
-      ld4(z + (size_t)i * 4, zz);
-      // sum_j c_j v_j in batches of 4 independent loads, added in j order
-      int j = 0;
-      for (; j + 4 <= m; j += 4) {
-        double v0[4], v1[4], v2[4], v3[4];
-        ld4(W + vs * (j + 0) + (size_t)i * 4, v0);
-        ld4(W + vs * (j + 1) + (size_t)i * 4, v1);
-        ld4(W + vs * (j + 2) + (size_t)i * 4, v2);
-        ld4(W + vs * (j + 3) + (size_t)i * 4, v3);
-        const double c0 = s_coef[j], c1 = s_coef[j + 1], c2 = s_coef[j + 2],
-                     c3 = s_coef[j + 3];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          zz[c] -= c0 * v0[c];
-          zz[c] -= c1 * v1[c];
-          zz[c] -= c2 * v2[c];
-          zz[c] -= c3 * v3[c];
-        }
-      }
-      for (; j < m; ++j) {
-        double v[4];
-        ld4(W + vs * j + (size_t)i * 4, v);
-        const double cj = s_coef[j];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) zz[c] -= cj * v[c];
-      }
-      st4(z + (size_t)i * 4, zz);
-      acc[0] += zz[0] * zz[0] + zz[1] * zz[1] + zz[2] * zz[2] + zz[3] * zz[3];
-    }
-    if (pass == 3) {
-      block_partials<1>(acc, 1, wr, 0);
-      return;
-    }
-  }
-  // dots <z, W_j> in chunks of kCh; pass 2 reads the z the same thread wrote
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double zz[4] = {0.0, 0.0, 0.0, 0.0};
+  if (i < N) ld4(z + (size_t)i * 4, zz);
   for (int j0 = 0; j0 < m; j0 += kCh) {
     const int n = (m - j0 < kCh) ? (m - j0) : kCh;
     double acc[kCh];
 #pragma unroll
-    for (int t = 0; t < kCh; ++t) acc[t] = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N;
-         i += gridDim.x * blockDim.x) {
-      double zz[4];
-      ld4(z + (size_t)i * 4, zz);
-#pragma unroll
-      for (int t = 0; t < kCh; ++t) {
-        if (t < n) {
-          double v[4];
-          ld4(W + vs * (j0 + t) + (size_t)i * 4, v);
-          acc[t] += zz[0] * v[0] + zz[1] * v[1] + zz[2] * v[2] + zz[3] * v[3];
-        }
+    for (int t = 0; t < kCh; ++t) {
+      acc[t] = 0.0;
+      if (i < N && t < n) {
+        double v[4];
+        ld4(W + vs * (j0 + t) + (size_t)i * 4, v);
+        acc[t] = zz[0] * v[0] + zz[1] * v[1] + zz[2] * v[2] + zz[3] * v[3];
       }
     }
     block_partials<kCh>(acc, n, wr, j0);
   }
 }
 
-// Update sweeps of the CGS variants whose first projection (dots of the
-// unscaled product z = (P^-1 A) W_k against W_0..W_k) came from the SpMV
-// epilogue (partials in `rd_dots`):
-//   mode 1 (CGS1):        close step k-1; h1 = <z, v_j>, h2 = 0;
-//                         z -= sum h1_j v_j; partials of |z|^2 -> wr
-//   mode 2 (CGS2 first):  close step k-1; h1 = <z, v_j>; z -= sum h1_j v_j;
-//                         partials of <z, W_j> -> wr
+// Update sweeps of the CGS variants after the projection dots of the
+// unscaled product z = (P^-1 A) W_k against W_0..W_k (partials in `rd_dots`):
+//   mode 1 (CGS1):        h1 = <z, v_j>, h2 = 0; z -= sum h1_j v_j;
+//                         partials of |z|^2 -> wr
+//   mode 2 (CGS2 first):  h1 = <z, v_j>; z -= sum h1_j v_j; partials of
+//                         <z, W_j> -> wr
 //   mode 3 (CGS2 second): h2 = <z, v_j>; z -= sum h2_j v_j; partials of |z|^2 -> wr
 __global__ void __launch_bounds__(256)
-k_cgs_update(int N, double *__restrict__ W, const double *__restrict__ rd_norm,
-             const double *__restrict__ rd_dots, double *__restrict__ wr,
-             GmresState *__restrict__ g, int k, int mode, int nb, int32_t *hflag) {
+k_cgs_update(int N, double *__restrict__ W, const double *__restrict__ rd_dots,
+             double *__restrict__ wr, GmresState *__restrict__ g, int k, int mode,
+             int nb) {
   if (g->done) return;
   const size_t vs = (size_t)N * 4;
   double *z = W + vs * (k + 1);
   const int m = k + 1;
   __shared__ double s_coef[kMaxKspace + 1];
-  double sk = g->s[k];
-  if (mode != 3) {
-    if (arnoldi_close(g, rd_norm, nb, k - 1, 1, hflag, &sk)) return;
-  }
   final_sums(rd_dots, nb, m, s_coef);
   for (int j = threadIdx.x; j < m; j += blockDim.x) {
-    const double sj = (j == k) ? sk : g->s[j];
+    const double sj = g->s[j];
     const double h = s_coef[j] * sj;
     if (blockIdx.x == 0) {
       if (mode == 3) {
@@ -417,25 +352,22 @@ hipError_t launch_mgs_step(Ctx &c, int k, int j) {
   return hipGetLastError();
 }
 
-hipError_t launch_cgs_pass(Ctx &c, int k, int pass) {
+// partial buffers: 0/1 alternate for the MGS chain and for the closing
+// norms of the CGS variants (parity of k); 2 holds the projection dots, 3
+// the dots of the second CGS2 projection
+hipError_t launch_cgs_dots(Ctx &c, int k) {
   const int nb = nblocks_nodes(c);
-  hipLaunchKernelGGL(k_cgs_pass, dim3(nb), dim3(kRedBlock), 0, c.stream, c.nnode, c.d_W,
-                     part_buf(c, c.part_cur), part_buf(c, c.part_cur ^ 1), c.d_gs, k, pass,
-                     nb, c.h_flags);
-  c.part_cur ^= 1;
+  hipLaunchKernelGGL(k_cgs_dots, dim3(nb), dim3(kRedBlock), 0, c.stream, c.nnode, c.d_W,
+                     part_buf(c, k & 1), part_buf(c, 2), c.d_gs, k, nb, c.h_flags);
   return hipGetLastError();
 }
 
-// partial buffers: 0/1 alternate for the MGS / 3-pass CGS2 chain and for
-// the closing norms of the fused CGS variants (parity of k), 2 holds the
-// SpMV-epilogue dots, 3 the dots of the second CGS2 projection
 hipError_t launch_cgs_update(Ctx &c, int k, int mode) {
   const int nb = nblocks_nodes(c);
-  const double *rd_norm = part_buf(c, k & 1);
   const double *rd_dots = part_buf(c, mode == 3 ? 3 : 2);
   double *wr = (mode == 2) ? part_buf(c, 3) : part_buf(c, (k + 1) & 1);
   hipLaunchKernelGGL(k_cgs_update, dim3(nb), dim3(kRedBlock), 0, c.stream, c.nnode,
-                     c.d_W, rd_norm, rd_dots, wr, c.d_gs, k, mode, nb, c.h_flags);
+                     c.d_W, rd_dots, wr, c.d_gs, k, mode, nb);
   c.part_cur = (k + 1) & 1;
   return hipGetLastError();
 }

@@ -163,14 +163,14 @@ static int gmres_cycle(Ctx &c, int precision, int ortho, const double *b, double
         if (j == 0) OFD_HIP(hipEventRecord(c.iter_ev[k], c.stream));
       }
     } else {
-      // product and first projection in one sweep, then the update sweep(s)
-      OFD_HIP(launch_spmv_dots(c, precision, k));
+      // product; projection dots (closing step k-1); update sweep(s)
+      OFD_HIP(launch_spmv_arnoldi(c, precision, k));
+      OFD_HIP(launch_cgs_dots(c, k));
+      OFD_HIP(hipEventRecord(c.iter_ev[k], c.stream));
       if (ortho == OFD_ORTHO_CGS1) {
         OFD_HIP(launch_cgs_update(c, k, 1));
-        OFD_HIP(hipEventRecord(c.iter_ev[k], c.stream));
       } else {
         OFD_HIP(launch_cgs_update(c, k, 2));
-        OFD_HIP(hipEventRecord(c.iter_ev[k], c.stream));
         OFD_HIP(launch_cgs_update(c, k, 3));
       }
     }

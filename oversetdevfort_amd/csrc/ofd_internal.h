@@ -135,7 +135,7 @@ hipError_t launch_convert_a32(Ctx &c);
 hipError_t launch_spmv(Ctx &c, int precision, const double *x, double *y,
                        const double *b, int mode);
 hipError_t launch_spmv_arnoldi(Ctx &c, int precision, int k);
-hipError_t launch_spmv_dots(Ctx &c, int precision, int k);
+hipError_t launch_cgs_dots(Ctx &c, int k);
 hipError_t launch_cgs_update(Ctx &c, int k, int mode);
 // vectors
 hipError_t launch_soa_to_node4(Ctx &c, const double *u, const double *p,
@@ -146,7 +146,6 @@ hipError_t launch_apply_dirichlet(Ctx &c);
 // gmres pieces
 hipError_t launch_gmres_init(Ctx &c, const double *b, double tol, int kspace, int kmin);
 hipError_t launch_mgs_step(Ctx &c, int k, int j);
-hipError_t launch_cgs_pass(Ctx &c, int k, int pass);
 hipError_t launch_gmres_close(Ctx &c, int k, int ortho);
 hipError_t launch_gmres_finish(Ctx &c, double *x);
 hipError_t launch_norm2(Ctx &c, const double *v, int slot);

@@ -89,47 +89,6 @@ k_spmv(int N, const int32_t *__restrict__ col, const int32_t *__restrict__ row,
     y[(size_t)r * 4 + c] = acc;
 }
 
-// Arnoldi product with the first projection fused into its epilogue:
-// z = (P^-1 A) W_k (unscaled) written to W_{k+1}, and the per-block partials
-// of <z, W_j>, j = 0..k, written to `part`.  1024-thread blocks (256 block
-// rows) so that the partial count matches the 256-node vector kernels.
-template <typename T>
-__global__ void __launch_bounds__(1024)
-k_spmv_dots(int N, const int32_t *__restrict__ col, const int32_t *__restrict__ row,
-            const T *__restrict__ A, double *__restrict__ W, int k,
-            double *__restrict__ part, const GmresState *__restrict__ gs) {
-  if (gs->done) return;
-  const size_t vs = (size_t)N * 4;
-  const double *__restrict__ x = W + vs * k;
-  double *__restrict__ y = W + vs * (k + 1);
-  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
-  const int r = gt >> 2;
-  const int c = gt & 3;
-  const bool live = r < N;
-  double acc = 0.0;
-  if (live) {
-    const int k0 = col[r], k1 = col[r + 1];
-#pragma unroll 4
-    for (int q = k0; q < k1; ++q) {
-      const int cn = row[q];
-      double a0, a1, a2, a3, x0, x1, x2, x3;
-      Row4<T>::load(A + (size_t)q * 16 + c * 4, a0, a1, a2, a3);
-      load_x4(x + (size_t)cn * 4, x0, x1, x2, x3);
-      acc += a0 * x0 + a1 * x1 + a2 * x2 + a3 * x3;
-    }
-    y[(size_t)r * 4 + c] = acc;
-  }
-  const int m = k + 1;
-  const size_t o = (size_t)r * 4 + c;
-  for (int j0 = 0; j0 < m; j0 += kCh) {
-    const int n = (m - j0 < kCh) ? (m - j0) : kCh;
-    double d[kCh];
-#pragma unroll
-    for (int t = 0; t < kCh; ++t) d[t] = (live && t < n) ? acc * W[vs * (j0 + t) + o] : 0.0;
-    block_partials<kCh>(d, n, part, j0);
-  }
-}
-
 // block-Jacobi inverse: 3x3 cofactor inverse of the K11 diagonal block and
 // 1/M (0 -> 1), SparseGMRES_NS.f90:73-121
 __global__ void k_build_prec(int N, const int32_t *__restrict__ diag,
@@ -271,19 +230,6 @@ hipError_t launch_spmv_arnoldi(Ctx &c, int precision, int k) {
   else
     hipLaunchKernelGGL((k_spmv<double, 2>), dim3(nb), dim3(256), 0, c.stream, c.nnode,
                        c.d_col, c.d_row, c.d_Ap64, x, y, nullptr, c.d_gs);
-  return hipGetLastError();
-}
-
-// Arnoldi product + first-projection partials into partial buffer 2
-hipError_t launch_spmv_dots(Ctx &c, int precision, int k) {
-  const unsigned nb = (unsigned)nblocks_nodes(c);  // 256 rows per 1024-thread block
-  double *part = c.d_part + (size_t)kMaxRedBlocks * kPartStride * 2;
-  if (precision == OFD_FP32)
-    hipLaunchKernelGGL(k_spmv_dots<float>, dim3(nb), dim3(1024), 0, c.stream, c.nnode,
-                       c.d_col, c.d_row, c.d_Ap32, c.d_W, k, part, c.d_gs);
-  else
-    hipLaunchKernelGGL(k_spmv_dots<double>, dim3(nb), dim3(1024), 0, c.stream, c.nnode,
-                       c.d_col, c.d_row, c.d_Ap64, c.d_W, k, part, c.d_gs);
   return hipGetLastError();
 }
 
