@@ -1,16 +1,23 @@
 // Block-Jacobi GMRES building blocks on gfx950.
 //
 // Restates the Arnoldi/Givens loop of SparseGMRES_up
-// (SparseGMRES_NS.f90:123-352) with every scalar of the cycle on the device:
-// each sweep writes per-block partial sums, and a one-block finalize kernel
-// adds them in a fixed order and writes the Hessenberg column, the Givens
-// rotations, the residual estimate and the convergence flag, so a cycle runs
-// without host round trips.  The kernel boundary orders the two (no
-// device-scope fences, which on a multi-XCD part write back a whole L2).
-// Results are bitwise reproducible run to run.
+// (SparseGMRES_NS.f90:123-352) with every scalar of the cycle on the device.
+// Each sweep writes per-block partial sums; the next kernel adds them in a
+// fixed order (every block the same way) before its own sweep, and its
+// block 0 writes the Hessenberg column, the Givens rotation, the residual
+// estimate and the convergence flag.  The kernel boundary orders producer
+// and consumer: no device-scope fences (on a multi-XCD part each one writes
+// back a whole L2).  Results are bitwise reproducible run to run.
 //
-// Basis vectors are stored unscaled: v_j = s_j W_j with s_j = 1/|W_j| kept
-// in GmresState, which saves the normalisation sweep of :293-294.
+// Krylov loop on the stored left-preconditioned matrix P^-1 A:
+//   product   z = (P^-1 A) W_k, stored unscaled as W_{k+1}
+//   CGS       dots <z, W_j> (closing step k-1 first), update z -= sum h_j v_j
+//             and |z|^2 (CGS1), or two projections (CGS2)
+//   MGS       the k+1 sweeps of :246-287, for parity with the reference
+// Basis vectors are stored unscaled: v_j = s_j W_j with s_j = 1/|W_j| kept in
+// GmresState, which saves the normalisation sweep of :293-294.  The basis is
+// FP64, or FP32 in the mixed solver (dot products, Hessenberg, Givens and the
+// solution update always accumulate in FP64).
 //
 // Vectors are node-interleaved [N][4]; one thread per node.
 #include "gmres_dev.h"
@@ -28,9 +35,31 @@ __device__ __forceinline__ void ld4(const double *p, double v[4]) {
   v[3] = b.y;
 }
 
-__device__ __forceinline__ void st4(double *p, const double v[4]) {
+__device__ __forceinline__ void ld4(const float *p, double v[4]) {
+  const float4 a = reinterpret_cast<const float4 *>(p)[0];
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = a.z;
+  v[3] = a.w;
+}
+
+// store, leaving v as the stored (possibly rounded) values
+__device__ __forceinline__ void st4(double *p, double v[4]) {
   reinterpret_cast<double2 *>(p)[0] = make_double2(v[0], v[1]);
   reinterpret_cast<double2 *>(p)[1] = make_double2(v[2], v[3]);
+}
+
+__device__ __forceinline__ void st4(float *p, double v[4]) {
+  const float4 f = make_float4((float)v[0], (float)v[1], (float)v[2], (float)v[3]);
+  reinterpret_cast<float4 *>(p)[0] = f;
+  v[0] = f.x;
+  v[1] = f.y;
+  v[2] = f.z;
+  v[3] = f.w;
+}
+
+__device__ __forceinline__ double dot4(const double a[4], const double b[4]) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3];
 }
 
 }  // namespace
@@ -38,9 +67,10 @@ __device__ __forceinline__ void st4(double *p, const double v[4]) {
 // W_0 = P^-1 b and partials of |W_0|^2 (SparseGMRES_NS.f90:124-155); block 0
 // also resets the cycle scalars.  The first orthogonalisation kernel of
 // iteration 0 closes the norm.
+template <typename V>
 __global__ void __launch_bounds__(256)
 k_gmres_init(int N, const double *__restrict__ P, const double *__restrict__ b,
-             double *__restrict__ W0, double *__restrict__ part, GmresState *__restrict__ g,
+             V *__restrict__ W0, double *__restrict__ part, GmresState *__restrict__ g,
              double tol, int kspace, int kmin) {
   if (blockIdx.x == 0) {
     for (int q = threadIdx.x; q <= kspace; q += blockDim.x) g->e[q] = 0.0;
@@ -64,13 +94,13 @@ k_gmres_init(int N, const double *__restrict__ P, const double *__restrict__ b,
       z[c] = p[c * 4 + 0] * r[0] + p[c * 4 + 1] * r[1] + p[c * 4 + 2] * r[2] +
              p[c * 4 + 3] * r[3];
     st4(W0 + (size_t)i * 4, z);
-    acc[0] += z[0] * z[0] + z[1] * z[1] + z[2] * z[2] + z[3] * z[3];
+    acc[0] += dot4(z, z);
   }
   block_partials<1>(acc, 1, part, 0);
 }
 
 // One MGS sweep j of Arnoldi step k (SparseGMRES_NS.f90:246-287) on the
-// unscaled product z = (P^-1 A) W_k:
+// unscaled product z = (P^-1 A) W_k (FP64 basis):
 //   j = 0:  close step k-1 first (partials in `rd`)
 //   j > 0:  beta = <z, v_{j-1}> from the previous sweep's partials (kept
 //           unscaled in H(j-1,k)); z -= beta v_{j-1}
@@ -113,7 +143,7 @@ k_mgs_step(int N, double *__restrict__ W, const double *__restrict__ rd,
 #pragma unroll
       for (int c = 0; c < 4; ++c) v[c] = zz[c];
     }
-    acc[0] += zz[0] * v[0] + zz[1] * v[1] + zz[2] * v[2] + zz[3] * v[3];
+    acc[0] += dot4(zz, v);
   }
   block_partials<1>(acc, 1, wr, 0);
 }
@@ -121,14 +151,15 @@ k_mgs_step(int N, double *__restrict__ W, const double *__restrict__ rd,
 // First projection of the CGS variants: close step k-1 (norm partials in
 // `rd`), then partials of <z, W_j>, j = 0..k, of the unscaled product
 // z = (P^-1 A) W_k into `wr`.
+template <typename V>
 __global__ void __launch_bounds__(256)
-k_cgs_dots(int N, const double *__restrict__ W, const double *__restrict__ rd,
+k_cgs_dots(int N, const V *__restrict__ W, const double *__restrict__ rd,
            double *__restrict__ wr, GmresState *__restrict__ g, int k, int nb,
            int32_t *hflag) {
   if (g->done) return;
   if (arnoldi_close(g, rd, nb, k - 1, 1, hflag)) return;
   const size_t vs = (size_t)N * 4;
-  const double *z = W + vs * (k + 1);
+  const V *z = W + vs * (k + 1);
   const int m = k + 1;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double zz[4] = {0.0, 0.0, 0.0, 0.0};
@@ -142,7 +173,7 @@ k_cgs_dots(int N, const double *__restrict__ W, const double *__restrict__ rd,
       if (i < N && t < n) {
         double v[4];
         ld4(W + vs * (j0 + t) + (size_t)i * 4, v);
-        acc[t] = zz[0] * v[0] + zz[1] * v[1] + zz[2] * v[2] + zz[3] * v[3];
+        acc[t] = dot4(zz, v);
       }
     }
     block_partials<kCh>(acc, n, wr, j0);
@@ -156,13 +187,14 @@ k_cgs_dots(int N, const double *__restrict__ W, const double *__restrict__ rd,
 //   mode 2 (CGS2 first):  h1 = <z, v_j>; z -= sum h1_j v_j; partials of
 //                         <z, W_j> -> wr
 //   mode 3 (CGS2 second): h2 = <z, v_j>; z -= sum h2_j v_j; partials of |z|^2 -> wr
+template <typename V>
 __global__ void __launch_bounds__(256)
-k_cgs_update(int N, double *__restrict__ W, const double *__restrict__ rd_dots,
+k_cgs_update(int N, V *__restrict__ W, const double *__restrict__ rd_dots,
              double *__restrict__ wr, GmresState *__restrict__ g, int k, int mode,
              int nb) {
   if (g->done) return;
   const size_t vs = (size_t)N * 4;
-  double *z = W + vs * (k + 1);
+  V *z = W + vs * (k + 1);
   const int m = k + 1;
   __shared__ double s_coef[kMaxKspace + 1];
   final_sums(rd_dots, nb, m, s_coef);
@@ -180,14 +212,11 @@ k_cgs_update(int N, double *__restrict__ W, const double *__restrict__ rd_dots,
     s_coef[j] = h * sj;
   }
   __syncthreads();
-  double acc[1] = {0.0};
-  double dacc[kCh];
-#pragma unroll
-  for (int t = 0; t < kCh; ++t) dacc[t] = 0.0;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double zz[4] = {0.0, 0.0, 0.0, 0.0};
   if (i < N) {
-    double zz[4];
     ld4(z + (size_t)i * 4, zz);
+    // sum_j c_j v_j in batches of 4 independent loads, subtracted in j order
     int j = 0;
     for (; j + 4 <= m; j += 4) {
       double v0[4], v1[4], v2[4], v3[4];
@@ -213,33 +242,25 @@ k_cgs_update(int N, double *__restrict__ W, const double *__restrict__ rd_dots,
       for (int c = 0; c < 4; ++c) zz[c] -= cj * v[c];
     }
     st4(z + (size_t)i * 4, zz);
-    acc[0] = zz[0] * zz[0] + zz[1] * zz[1] + zz[2] * zz[2] + zz[3] * zz[3];
-    if (mode == 2) {
-      // the dots of the second projection reuse zz; chunked below
-      (void)dacc;
-    }
   }
   if (mode != 2) {
+    double acc[1] = {dot4(zz, zz)};
     block_partials<1>(acc, 1, wr, 0);
     return;
   }
   for (int j0 = 0; j0 < m; j0 += kCh) {
     const int n = (m - j0 < kCh) ? (m - j0) : kCh;
+    double acc[kCh];
 #pragma unroll
-    for (int t = 0; t < kCh; ++t) dacc[t] = 0.0;
-    if (i < N) {
-      double zz[4];
-      ld4(z + (size_t)i * 4, zz);
-#pragma unroll
-      for (int t = 0; t < kCh; ++t) {
-        if (t < n) {
-          double v[4];
-          ld4(W + vs * (j0 + t) + (size_t)i * 4, v);
-          dacc[t] = zz[0] * v[0] + zz[1] * v[1] + zz[2] * v[2] + zz[3] * v[3];
-        }
+    for (int t = 0; t < kCh; ++t) {
+      acc[t] = 0.0;
+      if (i < N && t < n) {
+        double v[4];
+        ld4(W + vs * (j0 + t) + (size_t)i * 4, v);
+        acc[t] = dot4(zz, v);
       }
     }
-    block_partials<kCh>(dacc, n, wr, j0);
+    block_partials<kCh>(acc, n, wr, j0);
   }
 }
 
@@ -259,7 +280,7 @@ k_norm2(int N, const double *__restrict__ v, double *__restrict__ part) {
        i += gridDim.x * blockDim.x) {
     double z[4];
     ld4(v + (size_t)i * 4, z);
-    acc[0] += z[0] * z[0] + z[1] * z[1] + z[2] * z[2] + z[3] * z[3];
+    acc[0] += dot4(z, z);
   }
   block_partials<1>(acc, 1, part, 0);
 }
@@ -272,8 +293,9 @@ k_norm_fin(const double *__restrict__ part, int nb, GmresState *__restrict__ g, 
 }
 
 // back substitution (SparseGMRES_NS.f90:341-346) and x += sum y_j v_j (:349-352)
+template <typename V>
 __global__ void __launch_bounds__(256)
-k_gmres_finish(int N, const double *__restrict__ W, const GmresState *__restrict__ g,
+k_gmres_finish(int N, const V *__restrict__ W, const GmresState *__restrict__ g,
                double *__restrict__ x) {
   __shared__ double s_y[kMaxKspace];
   __shared__ double s_H[kMaxKspace * kMaxKspace];  // upper triangle, column-major it x it
@@ -331,14 +353,23 @@ int nblocks_nodes(const Ctx &c) {
   return nb;
 }
 
+// partial-sum buffers: 0/1 alternate along the MGS chain and hold the
+// closing norms of the CGS variants (parity of k); 2 holds the projection
+// dots, 3 the dots of the second CGS2 projection
 static double *part_buf(Ctx &c, int which) {
   return c.d_part + (size_t)kMaxRedBlocks * kPartStride * which;
 }
 
 hipError_t launch_gmres_init(Ctx &c, const double *b, double tol, int kspace, int kmin) {
   const int nb = nblocks_nodes(c);
-  hipLaunchKernelGGL(k_gmres_init, dim3(nb), dim3(kRedBlock), 0, c.stream, c.nnode,
-                     c.d_P64, b, c.d_W, part_buf(c, 0), c.d_gs, tol, kspace, kmin);
+  if (c.basis32)
+    hipLaunchKernelGGL(k_gmres_init<float>, dim3(nb), dim3(kRedBlock), 0, c.stream,
+                       c.nnode, c.d_P64, b, c.d_W32, part_buf(c, 0), c.d_gs, tol, kspace,
+                       kmin);
+  else
+    hipLaunchKernelGGL(k_gmres_init<double>, dim3(nb), dim3(kRedBlock), 0, c.stream,
+                       c.nnode, c.d_P64, b, c.d_W, part_buf(c, 0), c.d_gs, tol, kspace,
+                       kmin);
   c.part_cur = 0;
   return hipGetLastError();
 }
@@ -352,13 +383,16 @@ hipError_t launch_mgs_step(Ctx &c, int k, int j) {
   return hipGetLastError();
 }
 
-// partial buffers: 0/1 alternate for the MGS chain and for the closing
-// norms of the CGS variants (parity of k); 2 holds the projection dots, 3
-// the dots of the second CGS2 projection
 hipError_t launch_cgs_dots(Ctx &c, int k) {
   const int nb = nblocks_nodes(c);
-  hipLaunchKernelGGL(k_cgs_dots, dim3(nb), dim3(kRedBlock), 0, c.stream, c.nnode, c.d_W,
-                     part_buf(c, k & 1), part_buf(c, 2), c.d_gs, k, nb, c.h_flags);
+  if (c.basis32)
+    hipLaunchKernelGGL(k_cgs_dots<float>, dim3(nb), dim3(kRedBlock), 0, c.stream, c.nnode,
+                       c.d_W32, part_buf(c, k & 1), part_buf(c, 2), c.d_gs, k, nb,
+                       c.h_flags);
+  else
+    hipLaunchKernelGGL(k_cgs_dots<double>, dim3(nb), dim3(kRedBlock), 0, c.stream,
+                       c.nnode, c.d_W, part_buf(c, k & 1), part_buf(c, 2), c.d_gs, k, nb,
+                       c.h_flags);
   return hipGetLastError();
 }
 
@@ -366,8 +400,12 @@ hipError_t launch_cgs_update(Ctx &c, int k, int mode) {
   const int nb = nblocks_nodes(c);
   const double *rd_dots = part_buf(c, mode == 3 ? 3 : 2);
   double *wr = (mode == 2) ? part_buf(c, 3) : part_buf(c, (k + 1) & 1);
-  hipLaunchKernelGGL(k_cgs_update, dim3(nb), dim3(kRedBlock), 0, c.stream, c.nnode,
-                     c.d_W, rd_dots, wr, c.d_gs, k, mode, nb);
+  if (c.basis32)
+    hipLaunchKernelGGL(k_cgs_update<float>, dim3(nb), dim3(kRedBlock), 0, c.stream,
+                       c.nnode, c.d_W32, rd_dots, wr, c.d_gs, k, mode, nb);
+  else
+    hipLaunchKernelGGL(k_cgs_update<double>, dim3(nb), dim3(kRedBlock), 0, c.stream,
+                       c.nnode, c.d_W, rd_dots, wr, c.d_gs, k, mode, nb);
   c.part_cur = (k + 1) & 1;
   return hipGetLastError();
 }
@@ -380,8 +418,12 @@ hipError_t launch_gmres_close(Ctx &c, int k, int ortho) {
 }
 
 hipError_t launch_gmres_finish(Ctx &c, double *x) {
-  hipLaunchKernelGGL(k_gmres_finish, dim3(nblocks_nodes(c)), dim3(kRedBlock), 0,
-                     c.stream, c.nnode, c.d_W, c.d_gs, x);
+  if (c.basis32)
+    hipLaunchKernelGGL(k_gmres_finish<float>, dim3(nblocks_nodes(c)), dim3(kRedBlock), 0,
+                       c.stream, c.nnode, c.d_W32, c.d_gs, x);
+  else
+    hipLaunchKernelGGL(k_gmres_finish<double>, dim3(nblocks_nodes(c)), dim3(kRedBlock), 0,
+                       c.stream, c.nnode, c.d_W, c.d_gs, x);
   return hipGetLastError();
 }
 

@@ -92,6 +92,13 @@ static int ensure_krylov(Ctx &c, int K) {
     c.iter_ev.resize(kMaxKspace);
     for (auto &ev : c.iter_ev) OFD_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
   }
+  if (c.basis32) {
+    if (c.kalloc32 >= K) return OFD_OK;
+    int rc = dalloc(c, &c.d_W32, (size_t)(K + 1) * c.nnode * 4);
+    if (rc) return rc;
+    c.kalloc32 = K;
+    return OFD_OK;
+  }
   if (c.kalloc >= K) return OFD_OK;
   int rc = dalloc(c, &c.d_W, (size_t)(K + 1) * c.nnode * 4);
   if (rc) return rc;
@@ -137,6 +144,8 @@ static int ensure_prec(Ctx &c, int precision) {
 static int gmres_cycle(Ctx &c, int precision, int ortho, const double *b, double tol,
                        int K, int kmin, double *x, int *iters, double *resid,
                        double *rnorm0, bool sync = true) {
+  // the mixed CGS solver keeps its Krylov basis in FP32
+  c.basis32 = precision == OFD_FP32 && ortho != OFD_ORTHO_MGS;
   int rc = ensure_krylov(c, K);
   if (rc) return rc;
   rc = ensure_prec(c, precision);
@@ -283,6 +292,7 @@ int ofd_ctx_destroy(ofd_ctx *x) {
   dfree(c.d_cfl);
   dfree(c.d_negdet);
   dfree(c.d_W);
+  dfree(c.d_W32);
   dfree(c.d_part);
   dfree(c.d_gs);
   if (c.h_gs) (void)hipHostFree(c.h_gs);

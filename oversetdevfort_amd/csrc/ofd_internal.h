@@ -108,7 +108,10 @@ struct Ctx {
   int32_t *d_negdet = nullptr;        // count of negative Jacobians
   // GMRES workspace
   int kalloc = 0;
-  double *d_W = nullptr;              // [(kalloc+1)][N][4]
+  double *d_W = nullptr;              // [(kalloc+1)][N][4] FP64 Krylov basis
+  float *d_W32 = nullptr;             // [(kalloc32+1)][N][4] FP32 basis (mixed CGS solver)
+  int kalloc32 = 0;
+  bool basis32 = false;               // the running cycle uses d_W32
   double *d_part = nullptr;           // 4 x [kMaxRedBlocks][kMaxKspace + 1] partial-sum buffers
   GmresState *d_gs = nullptr;
   GmresState *h_gs = nullptr;         // pinned mirror

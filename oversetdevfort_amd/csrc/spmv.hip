@@ -58,16 +58,25 @@ __device__ __forceinline__ void load_x4(const double *x, double &x0, double &x1,
   x3 = v.y;
 }
 
+__device__ __forceinline__ void load_x4(const float *x, double &x0, double &x1,
+                                        double &x2, double &x3) {
+  const float4 u = reinterpret_cast<const float4 *>(x)[0];
+  x0 = u.x;
+  x1 = u.y;
+  x2 = u.z;
+  x3 = u.w;
+}
+
 }  // namespace
 
 // mode 0: y = A x; mode 1: y = b - A x; mode 2: Arnoldi product y = A x that
-// returns at once when the cycle has converged
-template <typename T, int MODE>
+// returns at once when the cycle has converged.  T: matrix entries, V:
+// vector entries (accumulation in FP64 either way).
+template <typename T, typename V, int MODE>
 __global__ void __launch_bounds__(256)
 k_spmv(int N, const int32_t *__restrict__ col, const int32_t *__restrict__ row,
-       const T *__restrict__ A, const double *__restrict__ x,
-       double *__restrict__ y, const double *__restrict__ b,
-       const GmresState *__restrict__ gs) {
+       const T *__restrict__ A, const V *__restrict__ x, V *__restrict__ y,
+       const double *__restrict__ b, const GmresState *__restrict__ gs) {
   if (MODE == 2 && gs->done) return;
   const int gt = blockIdx.x * blockDim.x + threadIdx.x;
   const int r = gt >> 2;
@@ -84,9 +93,9 @@ k_spmv(int N, const int32_t *__restrict__ col, const int32_t *__restrict__ row,
     acc += a0 * x0 + a1 * x1 + a2 * x2 + a3 * x3;
   }
   if (MODE == 1)
-    y[(size_t)r * 4 + c] = b[(size_t)r * 4 + c] - acc;
+    y[(size_t)r * 4 + c] = (V)(b[(size_t)r * 4 + c] - acc);
   else
-    y[(size_t)r * 4 + c] = acc;
+    y[(size_t)r * 4 + c] = (V)acc;
 }
 
 // block-Jacobi inverse: 3x3 cofactor inverse of the K11 diagonal block and
@@ -202,34 +211,42 @@ hipError_t launch_spmv(Ctx &c, int precision, const double *x, double *y,
   const unsigned nb = spmv_blocks(c);
   if (precision == OFD_FP32) {
     if (mode == 0)
-      hipLaunchKernelGGL((k_spmv<float, 0>), dim3(nb), dim3(256), 0, c.stream, c.nnode,
+      hipLaunchKernelGGL((k_spmv<float, double, 0>), dim3(nb), dim3(256), 0, c.stream, c.nnode,
                          c.d_col, c.d_row, c.d_A32, x, y, b, c.d_gs);
     else
-      hipLaunchKernelGGL((k_spmv<float, 1>), dim3(nb), dim3(256), 0, c.stream, c.nnode,
+      hipLaunchKernelGGL((k_spmv<float, double, 1>), dim3(nb), dim3(256), 0, c.stream, c.nnode,
                          c.d_col, c.d_row, c.d_A32, x, y, b, c.d_gs);
   } else {
     if (mode == 0)
-      hipLaunchKernelGGL((k_spmv<double, 0>), dim3(nb), dim3(256), 0, c.stream,
+      hipLaunchKernelGGL((k_spmv<double, double, 0>), dim3(nb), dim3(256), 0, c.stream,
                          c.nnode, c.d_col, c.d_row, c.d_A64, x, y, b, c.d_gs);
     else
-      hipLaunchKernelGGL((k_spmv<double, 1>), dim3(nb), dim3(256), 0, c.stream,
+      hipLaunchKernelGGL((k_spmv<double, double, 1>), dim3(nb), dim3(256), 0, c.stream,
                          c.nnode, c.d_col, c.d_row, c.d_A64, x, y, b, c.d_gs);
   }
   return hipGetLastError();
 }
 
-// Arnoldi product W_{k+1} = (P^-1 A) W_k, unscaled
+// Arnoldi product W_{k+1} = (P^-1 A) W_k, unscaled (FP32 basis in the mixed
+// CGS solver)
 hipError_t launch_spmv_arnoldi(Ctx &c, int precision, int k) {
   const unsigned nb = spmv_blocks(c);
   const size_t vs = (size_t)c.nnode * 4;
+  if (c.basis32) {
+    const float *x = c.d_W32 + vs * k;
+    float *y = c.d_W32 + vs * (k + 1);
+    hipLaunchKernelGGL((k_spmv<float, float, 2>), dim3(nb), dim3(256), 0, c.stream,
+                       c.nnode, c.d_col, c.d_row, c.d_Ap32, x, y, nullptr, c.d_gs);
+    return hipGetLastError();
+  }
   const double *x = c.d_W + vs * k;
   double *y = c.d_W + vs * (k + 1);
   if (precision == OFD_FP32)
-    hipLaunchKernelGGL((k_spmv<float, 2>), dim3(nb), dim3(256), 0, c.stream, c.nnode,
-                       c.d_col, c.d_row, c.d_Ap32, x, y, nullptr, c.d_gs);
+    hipLaunchKernelGGL((k_spmv<float, double, 2>), dim3(nb), dim3(256), 0, c.stream,
+                       c.nnode, c.d_col, c.d_row, c.d_Ap32, x, y, nullptr, c.d_gs);
   else
-    hipLaunchKernelGGL((k_spmv<double, 2>), dim3(nb), dim3(256), 0, c.stream, c.nnode,
-                       c.d_col, c.d_row, c.d_Ap64, x, y, nullptr, c.d_gs);
+    hipLaunchKernelGGL((k_spmv<double, double, 2>), dim3(nb), dim3(256), 0, c.stream,
+                       c.nnode, c.d_col, c.d_row, c.d_Ap64, x, y, nullptr, c.d_gs);
   return hipGetLastError();
 }
 

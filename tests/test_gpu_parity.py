@@ -179,14 +179,20 @@ def test_gmres_mgs_matches_oracle(ofd, oracle_mod, tol, kspace, kmin):
     sol = np.concatenate([got["solu"].ravel(), got["solp"]])
     rs = np.concatenate([ref["solu"].ravel(), ref["solp"]])
     assert _rel(sol, rs) < 1e-9
-    got2 = ctx.SparseGMRES_up(tol, kspace, kmin, L.OFD_FP64, L.OFD_ORTHO_CGS2)
-    assert abs(got2["iters"] - ref["iters"]) <= 1
-    assert _rel(np.concatenate([got2["solu"].ravel(), got2["solp"]]), rs) < 1e-7
+    for ortho in (L.OFD_ORTHO_CGS2, L.OFD_ORTHO_CGS1):
+        got2 = ctx.SparseGMRES_up(tol, kspace, kmin, L.OFD_FP64, ortho)
+        assert abs(got2["iters"] - ref["iters"]) <= 1
+        assert _rel(np.concatenate([got2["solu"].ravel(), got2["solp"]]), rs) < 1e-7
     ctx.close()
 
 
-@pytest.mark.parametrize("precision", [L.OFD_FP64, L.OFD_FP32])
-def test_solve_reaches_1e8_true_residual(ofd, oracle_mod, precision):
+@pytest.mark.parametrize("precision,ortho", [(L.OFD_FP64, L.OFD_ORTHO_CGS1),
+                                             (L.OFD_FP32, L.OFD_ORTHO_CGS1),
+                                             (L.OFD_FP64, L.OFD_ORTHO_CGS2),
+                                             (L.OFD_FP32, L.OFD_ORTHO_CGS2),
+                                             (L.OFD_FP64, L.OFD_ORTHO_MGS),
+                                             (L.OFD_FP32, L.OFD_ORTHO_MGS)])
+def test_solve_reaches_1e8_true_residual(ofd, oracle_mod, precision, ortho):
     O = oracle_mod
     prob, st = _well_posed(6, seed=12)
     m = prob.mesh
@@ -194,7 +200,7 @@ def test_solve_reaches_1e8_true_residual(ofd, oracle_mod, precision):
     ctx.assembleNavSto(6, 1)
     A = ctx.get_lhs()
     ru, rp = ctx.get_rhs()
-    info = ctx.solve(rtol=1e-8, precision=precision, kspace=30)
+    info = ctx.solve(rtol=1e-8, precision=precision, ortho=ortho, kspace=30)
     assert info["relres"] <= 1e-8
     xu, xp = ctx.get_field("SOLU"), ctx.get_field("SOLP")
     col, row = O.sparsity(m.nnode, m.ien)
