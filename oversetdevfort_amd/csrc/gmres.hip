@@ -197,9 +197,12 @@ k_cgs_update(int N, V *__restrict__ W, const double *__restrict__ rd_dots,
   V *z = W + vs * (k + 1);
   const int m = k + 1;
   __shared__ double s_coef[kMaxKspace + 1];
+  // the scales are loaded before the partial sums so both round trips overlap
+  const int t0 = threadIdx.x;
+  const double s_own = (t0 < m) ? g->s[t0] : 0.0;
   final_sums(rd_dots, nb, m, s_coef);
   for (int j = threadIdx.x; j < m; j += blockDim.x) {
-    const double sj = g->s[j];
+    const double sj = (j == t0) ? s_own : g->s[j];
     const double h = s_coef[j] * sj;
     if (blockIdx.x == 0) {
       if (mode == 3) {

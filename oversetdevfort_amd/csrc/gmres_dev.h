@@ -61,14 +61,14 @@ __device__ __forceinline__ void final_sums(const double *part, int nb, int m,
   const int j = t % jw, g = t / jw;
   double s = 0.0;
   if (j < m) {
-    // independent loads in batches of 16, added in block order
+    // independent loads in batches of 24, added in block order
     int b = g;
-    for (; b + 15 * G < nb; b += 16 * G) {
-      double v[16];
+    for (; b + 23 * G < nb; b += 24 * G) {
+      double v[24];
 #pragma unroll
-      for (int u = 0; u < 16; ++u) v[u] = part[(size_t)(b + u * G) * kPartStride + j];
+      for (int u = 0; u < 24; ++u) v[u] = part[(size_t)(b + u * G) * kPartStride + j];
 #pragma unroll
-      for (int u = 0; u < 16; ++u) s += v[u];
+      for (int u = 0; u < 24; ++u) s += v[u];
     }
     for (; b < nb; b += G) s += part[(size_t)b * kPartStride + j];
   }
@@ -80,6 +80,22 @@ __device__ __forceinline__ void final_sums(const double *part, int nb, int m,
   }
   if (t < m) s_out[t] = s_acc[t];
   __syncthreads();
+}
+
+// Sum over b < nb of part[b][0], in every block thread: thread t adds
+// blocks t, t+256, ..., then a butterfly over the wave and the 4 wave sums
+// in order -- a fixed shape, so every block gets the same bits.
+__device__ __forceinline__ double sum1(const double *part, int nb) {
+  __shared__ double s_w[kRedBlock / 64];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += kRedBlock) s += part[(size_t)b * kPartStride];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = s;
+  __syncthreads();
+  double r = 0.0;
+#pragma unroll
+  for (int w = 0; w < kRedBlock / 64; ++w) r += s_w[w];
+  return r;
 }
 
 // Close Arnoldi step k (SparseGMRES_NS.f90:289-327).  The product of step k
@@ -95,26 +111,45 @@ __device__ __forceinline__ bool arnoldi_close(GmresState *__restrict__ g,
                                               const double *__restrict__ part, int nb,
                                               int k, int cgs, int32_t *hflag,
                                               double *s_next = nullptr) {
-  __shared__ double s_sum[1];
   __shared__ double s_res[2];
   __shared__ double s_col[kMaxKspace + 2];
   __shared__ double s_rc[kMaxKspace], s_rs[kMaxKspace];
-  final_sums(part, nb, 1, s_sum);
-  if (k >= 0) {
-    // stage column k and the previous rotations in LDS with all threads
-    const int K1 = g->kspace + 1;
-    const double sk = g->s[k];
-    for (int q = threadIdx.x; q <= k; q += blockDim.x) {
-      s_col[q] = sk * (cgs ? g->h1[q] + g->h2[q] : g->H[q + K1 * k]);
-      if (q < k) {
-        s_rc[q] = g->rc[q];
-        s_rs[q] = g->rs[q];
-      }
+  const int t = threadIdx.x;
+  // issue every global load of the close before the first barrier: the
+  // column entries and rotations (threads q <= k), the cycle scalars
+  // (thread 0) and the norm partials (all threads, inside sum1)
+  const int K1 = g->kspace + 1;
+  double colq = 0.0, rcq = 0.0, rsq = 0.0;
+  if (k >= 0 && t <= k) {
+    colq = cgs ? g->h1[t] + g->h2[t] : g->H[t + K1 * k];
+    if (t < k) {
+      rcq = g->rc[t];
+      rsq = g->rs[t];
     }
-    __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    const double nrm = sqrt(s_sum[0]);
+  double sk = 0.0, e0 = 0.0, e1 = 0.0, eps = 0.0, tol = 0.0;
+  int kmin = 0;
+  if (t == 0) {
+    if (k >= 0) {
+      sk = g->s[k];
+      e0 = g->e[k];
+      e1 = g->e[k + 1];
+      eps = g->eps;
+      kmin = g->kmin;
+    } else {
+      tol = g->tol;
+    }
+  }
+  const double nrm = sqrt(sum1(part, nb));
+  if (k >= 0 && t <= k) {
+    s_col[t] = colq;
+    if (t < k) {
+      s_rc[t] = rcq;
+      s_rs[t] = rsq;
+    }
+  }
+  __syncthreads();
+  if (t == 0) {
     const double sc = (nrm > 0.0) ? 1.0 / nrm : 0.0;
     bool done;
     const bool w = blockIdx.x == 0;
@@ -122,7 +157,7 @@ __device__ __forceinline__ bool arnoldi_close(GmresState *__restrict__ g,
       done = !(nrm > 0.0 && nrm == nrm);
       if (w) {
         g->rnorm0 = nrm;
-        g->eps = g->tol * nrm;
+        g->eps = tol * nrm;
         g->e[0] = nrm;
         g->s[0] = sc;
         g->resid = nrm;
@@ -131,22 +166,22 @@ __device__ __forceinline__ bool arnoldi_close(GmresState *__restrict__ g,
       }
     } else {
       double *col = s_col;
-      col[k + 1] = g->s[k] * nrm;
+      for (int q = 0; q <= k; ++q) col[q] *= sk;
+      col[k + 1] = sk * nrm;
       for (int q = 0; q < k; ++q) {
         const double c = s_rc[q], s = s_rs[q];
-        const double t = c * col[q] + s * col[q + 1];
+        const double tq = c * col[q] + s * col[q + 1];
         col[q + 1] = -s * col[q] + c * col[q + 1];
-        col[q] = t;
+        col[q] = tq;
       }
       const double hk = col[k], hk1 = col[k + 1];
-      const double t = sqrt(hk * hk + hk1 * hk1);
-      const double rc = hk / t, rs = hk1 / t;
-      const double e0 = g->e[k], e1 = g->e[k + 1];
+      const double tn = sqrt(hk * hk + hk1 * hk1);
+      const double rc = hk / tn, rs = hk1 / tn;
       const double ek = rc * e0 + rs * e1;
       const double ek1 = -rs * e0 + rc * e1;
       const double er = fabs(ek1);
-      done = (er <= g->eps && k + 1 >= g->kmin) || !(er == er) || hk1 == 0.0;
-      col[k] = t;
+      done = (er <= eps && k + 1 >= kmin) || !(er == er) || hk1 == 0.0;
+      col[k] = tn;
       col[k + 1] = 0.0;
       if (w) {
         g->rc[k] = rc;
@@ -165,8 +200,7 @@ __device__ __forceinline__ bool arnoldi_close(GmresState *__restrict__ g,
   }
   __syncthreads();
   if (k >= 0 && blockIdx.x == 0) {
-    const int K1 = g->kspace + 1;
-    for (int q = threadIdx.x; q <= k + 1; q += blockDim.x) g->H[q + K1 * k] = s_col[q];
+    for (int q = t; q <= k + 1; q += blockDim.x) g->H[q + K1 * k] = s_col[q];
   }
   if (s_next) *s_next = s_res[1];
   return s_res[0] != 0.0;
