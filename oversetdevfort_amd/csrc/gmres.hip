@@ -62,6 +62,50 @@ __device__ __forceinline__ double dot4(const double a[4], const double b[4]) {
   return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3];
 }
 
+// Partials of <z, W_j>, j < m, for the node of this thread, reduced over the
+// block in chunks of kCh.  Every load of a chunk is issued unconditionally
+// (indices past m clamped to m-1, their products dropped) so all of them
+// are in flight together.
+template <typename V>
+__device__ __forceinline__ void dots_chunks(const V *__restrict__ W, size_t vs, int i,
+                                            bool live, const double zz[4], int m,
+                                            double *__restrict__ wr) {
+  for (int j0 = 0; j0 < m; j0 += kCh) {
+    const int n = (m - j0 < kCh) ? (m - j0) : kCh;
+    double v[kCh][4];
+#pragma unroll
+    for (int t = 0; t < kCh; ++t) {
+      const int j = (j0 + t < m) ? j0 + t : m - 1;
+      ld4(W + vs * j + (size_t)i * 4, v[t]);
+    }
+    double acc[kCh];
+#pragma unroll
+    for (int t = 0; t < kCh; ++t) acc[t] = (live && t < n) ? dot4(zz, v[t]) : 0.0;
+    block_partials<kCh>(acc, n, wr, j0);
+  }
+}
+
+// z -= sum_{j<m} coef[j] W_j for the node of this thread, 8 loads in flight
+// per batch (indices past m clamped, their weights zero), subtracted in j order
+template <typename V>
+__device__ __forceinline__ void axpy_batches(const V *__restrict__ W, size_t vs, int i,
+                                             const double *coef, int m, double zz[4]) {
+  for (int j0 = 0; j0 < m; j0 += 8) {
+    double v[8][4];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int j = (j0 + t < m) ? j0 + t : m - 1;
+      ld4(W + vs * j + (size_t)i * 4, v[t]);
+    }
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const double cj = (j0 + t < m) ? coef[j0 + t] : 0.0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) zz[c] -= cj * v[t][c];
+    }
+  }
+}
+
 }  // namespace
 
 // W_0 = P^-1 b and partials of |W_0|^2 (SparseGMRES_NS.f90:124-155); block 0
@@ -164,20 +208,7 @@ k_cgs_dots(int N, const V *__restrict__ W, const double *__restrict__ rd,
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double zz[4] = {0.0, 0.0, 0.0, 0.0};
   if (i < N) ld4(z + (size_t)i * 4, zz);
-  for (int j0 = 0; j0 < m; j0 += kCh) {
-    const int n = (m - j0 < kCh) ? (m - j0) : kCh;
-    double acc[kCh];
-#pragma unroll
-    for (int t = 0; t < kCh; ++t) {
-      acc[t] = 0.0;
-      if (i < N && t < n) {
-        double v[4];
-        ld4(W + vs * (j0 + t) + (size_t)i * 4, v);
-        acc[t] = dot4(zz, v);
-      }
-    }
-    block_partials<kCh>(acc, n, wr, j0);
-  }
+  dots_chunks(W, vs, i < N ? i : 0, i < N, zz, m, wr);
 }
 
 // Update sweeps of the CGS variants after the projection dots of the
@@ -219,31 +250,7 @@ k_cgs_update(int N, V *__restrict__ W, const double *__restrict__ rd_dots,
   double zz[4] = {0.0, 0.0, 0.0, 0.0};
   if (i < N) {
     ld4(z + (size_t)i * 4, zz);
-    // sum_j c_j v_j in batches of 4 independent loads, subtracted in j order
-    int j = 0;
-    for (; j + 4 <= m; j += 4) {
-      double v0[4], v1[4], v2[4], v3[4];
-      ld4(W + vs * (j + 0) + (size_t)i * 4, v0);
-      ld4(W + vs * (j + 1) + (size_t)i * 4, v1);
-      ld4(W + vs * (j + 2) + (size_t)i * 4, v2);
-      ld4(W + vs * (j + 3) + (size_t)i * 4, v3);
-      const double c0 = s_coef[j], c1 = s_coef[j + 1], c2 = s_coef[j + 2],
-                   c3 = s_coef[j + 3];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        zz[c] -= c0 * v0[c];
-        zz[c] -= c1 * v1[c];
-        zz[c] -= c2 * v2[c];
-        zz[c] -= c3 * v3[c];
-      }
-    }
-    for (; j < m; ++j) {
-      double v[4];
-      ld4(W + vs * j + (size_t)i * 4, v);
-      const double cj = s_coef[j];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) zz[c] -= cj * v[c];
-    }
+    axpy_batches(W, vs, i, s_coef, m, zz);
     st4(z + (size_t)i * 4, zz);
   }
   if (mode != 2) {
@@ -251,20 +258,7 @@ k_cgs_update(int N, V *__restrict__ W, const double *__restrict__ rd_dots,
     block_partials<1>(acc, 1, wr, 0);
     return;
   }
-  for (int j0 = 0; j0 < m; j0 += kCh) {
-    const int n = (m - j0 < kCh) ? (m - j0) : kCh;
-    double acc[kCh];
-#pragma unroll
-    for (int t = 0; t < kCh; ++t) {
-      acc[t] = 0.0;
-      if (i < N && t < n) {
-        double v[4];
-        ld4(W + vs * (j0 + t) + (size_t)i * 4, v);
-        acc[t] = dot4(zz, v);
-      }
-    }
-    block_partials<kCh>(acc, n, wr, j0);
-  }
+  dots_chunks(W, vs, i < N ? i : 0, i < N, zz, m, wr);
 }
 
 // closes the last Arnoldi step of a cycle (one block)

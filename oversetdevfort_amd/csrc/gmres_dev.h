@@ -61,14 +61,14 @@ __device__ __forceinline__ void final_sums(const double *part, int nb, int m,
   const int j = t % jw, g = t / jw;
   double s = 0.0;
   if (j < m) {
-    // independent loads in batches of 24, added in block order
+    // independent loads in batches of 16, added in block order
     int b = g;
-    for (; b + 23 * G < nb; b += 24 * G) {
-      double v[24];
+    for (; b + 15 * G < nb; b += 16 * G) {
+      double v[16];
 #pragma unroll
-      for (int u = 0; u < 24; ++u) v[u] = part[(size_t)(b + u * G) * kPartStride + j];
+      for (int u = 0; u < 16; ++u) v[u] = part[(size_t)(b + u * G) * kPartStride + j];
 #pragma unroll
-      for (int u = 0; u < 24; ++u) s += v[u];
+      for (int u = 0; u < 16; ++u) s += v[u];
     }
     for (; b < nb; b += G) s += part[(size_t)b * kPartStride + j];
   }
