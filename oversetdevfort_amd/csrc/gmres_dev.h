@@ -15,33 +15,34 @@ namespace ofd {
 constexpr int kPartStride = kMaxKspace + 1;
 constexpr int kCh = 16;  // dot products per reduction chunk
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m);
-  return v;
-}
-
-// Reduce acc[0..n) over the block (blockDim.x = 64 * nw, nw <= 16) into
-// part[blockIdx][off + j], waves added in a fixed order.
+// Reduce acc[0..n) over a 256-thread block into part[blockIdx][off + j]
+// through LDS: thread t stores its values at s[j][t]; thread (j, q) adds
+// s[j][q + 16 r], r = 0..15 (consecutive q -> consecutive banks); thread j
+// adds its 16 group sums.  A fixed order, no cross-lane shuffles.
 template <int NV>
 __device__ __forceinline__ void block_partials(const double (&acc)[NV], int n, double *part,
                                                int off) {
-  __shared__ double s_red[16][kCh > NV ? kCh : NV];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int nw = blockDim.x >> 6;
+  __shared__ double s_v[NV][kRedBlock];
+  __shared__ double s_g[NV][16];
+  const int t = threadIdx.x;
 #pragma unroll
-  for (int j = 0; j < NV; ++j) {
-    if (j < n) {
-      const double v = wave_sum(acc[j]);
-      if (lane == 0) s_red[w][j] = v;
+  for (int j = 0; j < NV; ++j) s_v[j][t] = acc[j];
+  __syncthreads();
+  {
+    const int j = t >> 4, q = t & 15;
+    if (j < NV && j < n) {
+      double r = 0.0;
+#pragma unroll
+      for (int u = 0; u < 16; ++u) r += s_v[j][q + 16 * u];
+      s_g[j][q] = r;
     }
   }
   __syncthreads();
-  if ((int)threadIdx.x < n) {
-    const int j = threadIdx.x;
+  if (t < n) {
     double r = 0.0;
-    for (int q = 0; q < nw; ++q) r += s_red[q][j];
-    part[(size_t)blockIdx.x * kPartStride + off + j] = r;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) r += s_g[t][q];
+    part[(size_t)blockIdx.x * kPartStride + off + t] = r;
   }
   __syncthreads();
 }
@@ -83,18 +84,26 @@ __device__ __forceinline__ void final_sums(const double *part, int nb, int m,
 }
 
 // Sum over b < nb of part[b][0], in every block thread: thread t adds
-// blocks t, t+256, ..., then a butterfly over the wave and the 4 wave sums
-// in order -- a fixed shape, so every block gets the same bits.
+// blocks t, t+256, ..., then the 256 values are added through LDS in a
+// fixed order -- every block gets the same bits.
 __device__ __forceinline__ double sum1(const double *part, int nb) {
-  __shared__ double s_w[kRedBlock / 64];
+  __shared__ double s_v[kRedBlock];
+  __shared__ double s_g[16];
+  const int t = threadIdx.x;
   double s = 0.0;
-  for (int b = threadIdx.x; b < nb; b += kRedBlock) s += part[(size_t)b * kPartStride];
-  s = wave_sum(s);
-  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = s;
+  for (int b = t; b < nb; b += kRedBlock) s += part[(size_t)b * kPartStride];
+  s_v[t] = s;
+  __syncthreads();
+  if (t < 16) {
+    double r = 0.0;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) r += s_v[t + 16 * u];
+    s_g[t] = r;
+  }
   __syncthreads();
   double r = 0.0;
 #pragma unroll
-  for (int w = 0; w < kRedBlock / 64; ++w) r += s_w[w];
+  for (int q = 0; q < 16; ++q) r += s_g[q];
   return r;
 }
 
