@@ -41,11 +41,18 @@ def spmv_bytes(Z, N, s_a, s_x=8, s_y=8, b=4, S=16):
     return Z * (S * s_a + 4) + 4 * (N + 1) + N * b * (s_x + s_y)
 
 
-def run_steps(ctx, S, L, prob, precision, steps, warmup, kspace=30, ortho=1):
-    """Reset to the initial state, run warmup + timed time steps."""
+def run_steps(ctx, S, L, prob, precision, steps, warmup, kspace=30, ortho=1,
+              probe_every=4):
+    """Reset to the initial state, run warmup + timed time steps.  During the
+    timed steps every ``probe_every``-th Arnoldi product launch is bracketed
+    by HIP timing events on the solver stream; their mean is returned as
+    ``arnoldi_spmv_ms``."""
     ctx.set_state(prob.fields)
     newton_ms, step_ms, iters, cycles, rel, res = [], [], [], [], [], []
     for k in range(warmup + steps):
+        if k == warmup:
+            ctx.probe_set(probe_every)
+            ctx.probe_read()
         t0 = time.perf_counter()
         S.begin_step(ctx)
         t1 = time.perf_counter()
@@ -69,10 +76,56 @@ def run_steps(ctx, S, L, prob, precision, steps, warmup, kspace=30, ortho=1):
             cycles.append(info["cycles"])
             rel.append(info["relres"])
             res.append(info["res"])
-    return dict(newton_ms=sum(newton_ms) / len(newton_ms),
+    nprobe, probe_ms = ctx.probe_read()
+    ctx.probe_set(0)
+    return dict(arnoldi_spmv_ms=probe_ms / max(nprobe, 1), arnoldi_spmv_samples=nprobe,
+                newton_ms=sum(newton_ms) / len(newton_ms),
                 step_ms=sum(step_ms) / len(step_ms),
                 total_ms=sum(step_ms), gmres_iters=iters, cycles=cycles,
                 relres=max(rel), res_u=[r[0] for r in res])
+
+
+ARN32 = "k_spmv<float, float, 2>"    # Arnoldi product, mixed solver
+ARN64 = "k_spmv<double, double, 2>"  # Arnoldi product, FP64 solver
+
+
+def traffic(kernel, workload):
+    """HBM bytes per launch of ``kernel`` from the committed hardware-counter
+    summary (FETCH_SIZE and WRITE_SIZE passes, scaled by a copy kernel of
+    known size), or None when no summary covers this workload."""
+    path = os.path.join(ROOT, "profiles", f"spmv_arnoldi_{workload}_pmc.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        # table rows: | kernel | fetch KiB | write KiB | scaled bytes |
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 4 and cells[0] == f"`{kernel}`":
+            try:
+                return float(cells[3])
+            except ValueError:
+                return None
+    return None
+
+
+def pmc_sample(ctx, S, L, prob, ortho, kspace):
+    """Kernels for one hardware-counter pass: a device copy of 256 MiB
+    (three launches; reads and writes 2^28 bytes each) to calibrate the
+    counters, then one assembly and one full GMRES(kspace) cycle per
+    precision (kspace Arnoldi products each)."""
+    import torch
+
+    src = torch.ones(1 << 26, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    ctx.set_state(prob.fields)
+    S.begin_step(ctx)
+    ctx.assembleNavSto(6, 1)
+    for prec in (L.OFD_FP64, L.OFD_FP32):
+        g = ctx.SparseGMRES_up(1e-30, kspace, kspace_min=kspace, precision=prec,
+                               ortho=ortho)
+        log(f"pmc sample: precision {prec}: {g['iters']} Arnoldi steps")
 
 
 def cpu_baseline(prob, budget_s=15.0):
@@ -146,7 +199,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the host baseline")
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--kspace", type=int, default=30, help="GMRES restart length")
-    ap.add_argument("--ortho", type=str, default="cgs2", choices=["mgs", "cgs2", "cgs1"])
+    ap.add_argument("--ortho", type=str, default="cgs1", choices=["mgs", "cgs2", "cgs1"])
+    ap.add_argument("--probe-every", type=int, default=4,
+                    help="sample every n-th Arnoldi product launch with HIP events")
+    ap.add_argument("--pmc-sample", action="store_true",
+                    help="short run for hardware-counter passes: a 256 MiB copy "
+                         "(calibration) and one full GMRES(30) cycle per precision")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs one GPU as a single rank; --gpus {args.gpus} "
@@ -170,12 +228,18 @@ def main():
 
     ortho = {"mgs": L.OFD_ORTHO_MGS, "cgs2": L.OFD_ORTHO_CGS2,
              "cgs1": L.OFD_ORTHO_CGS1}[args.ortho]
+    if args.pmc_sample:
+        pmc_sample(ctx, S, L, prob, ortho, args.kspace)
+        ctx.close()
+        return
     res64 = run_steps(ctx, S, L, prob, L.OFD_FP64, args.steps, args.warmup,
-                      kspace=args.kspace, ortho=ortho)
-    log(f"bench: FP64 newton {res64['newton_ms']:.2f} ms")
+                      kspace=args.kspace, ortho=ortho, probe_every=args.probe_every)
+    log(f"bench: FP64 newton {res64['newton_ms']:.2f} ms; Arnoldi SpMV "
+        f"{1e3 * res64['arnoldi_spmv_ms']:.1f} us ({res64['arnoldi_spmv_samples']} samples)")
     res32 = run_steps(ctx, S, L, prob, L.OFD_FP32, args.steps, args.warmup,
-                      kspace=args.kspace, ortho=ortho)
-    log(f"bench: mixed newton {res32['newton_ms']:.2f} ms")
+                      kspace=args.kspace, ortho=ortho, probe_every=args.probe_every)
+    log(f"bench: mixed newton {res32['newton_ms']:.2f} ms; Arnoldi SpMV "
+        f"{1e3 * res32['arnoldi_spmv_ms']:.1f} us ({res32['arnoldi_spmv_samples']} samples)")
 
     # kernel-level figures on the context stream (HIP events)
     ctx.set_state(prob.fields)
@@ -187,6 +251,13 @@ def main():
     spmv32_ms = ctx.time_spmv(L.OFD_FP32, reps=args.spmv_reps)
     b64 = spmv_bytes(Z, N, 8)
     b32 = spmv_bytes(Z, N, 4)
+    # Arnoldi product W_{k+1} = (P^-1 A) W_k: FP32 matrix and FP32 basis in
+    # the mixed solver, FP64 both in the FP64 one
+    ba32 = spmv_bytes(Z, N, 4, 4, 4)
+    ba64 = spmv_bytes(Z, N, 8, 8, 8)
+    bwa32 = ba32 / (res32["arnoldi_spmv_ms"] * 1e-3)
+    bwa64 = ba64 / (res64["arnoldi_spmv_ms"] * 1e-3)
+    wl = f"channel_{nx}x{ny}x{nz}"
     bw64 = b64 / (spmv64_ms * 1e-3)
     bw32 = b32 / (spmv32_ms * 1e-3)
     asm_tf = E * F_NS / (asm_ms * 1e-3)
@@ -249,10 +320,18 @@ def main():
         "spmv": {"fp64_us": spmv64_ms * 1e3, "fp64_GBps": bw64 / 1e9,
                  "fp32_us": spmv32_ms * 1e3, "fp32_GBps": bw32 / 1e9,
                  "fp64_bytes": b64, "fp32_bytes": b32},
-        "roofline": {"bound": "hbm", "kernel": "k_spmv<double> (block-CSR SpMV, FP64)",
-                     "achieved": bw64 / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                     "frac": bw64 / PEAK_HBM, "traffic": None,
-                     "algorithmic_bytes": b64},
+        "roofline": {"bound": "hbm", "kernel": ARN32,
+                     "achieved": bwa32 / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                     "frac": bwa32 / PEAK_HBM, "traffic": traffic(ARN32, wl),
+                     "algorithmic_bytes": ba32,
+                     "avg_launch_us": 1e3 * res32["arnoldi_spmv_ms"],
+                     "samples": res32["arnoldi_spmv_samples"]},
+        "roofline_fp64": {"bound": "hbm", "kernel": ARN64,
+                          "achieved": bwa64 / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                          "frac": bwa64 / PEAK_HBM, "traffic": traffic(ARN64, wl),
+                          "algorithmic_bytes": ba64,
+                          "avg_launch_us": 1e3 * res64["arnoldi_spmv_ms"],
+                          "samples": res64["arnoldi_spmv_samples"]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

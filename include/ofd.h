@@ -161,6 +161,15 @@ int ofd_rhs_norms(ofd_ctx *ctx, double *norms2);
 int ofd_time_spmv(ofd_ctx *ctx, int precision, int reps, double *ms);
 int ofd_time_assemble(ofd_ctx *ctx, int tensor_flag, int reps, double *ms);
 
+/* sampled timing of the Arnoldi product kernel (the preconditioned SpMV of
+ * each GMRES iteration) inside ofd_solve / ofd_gmres_up: every `every`-th
+ * iteration's launch is bracketed by HIP timing events on the context
+ * stream (0 switches sampling off).  ofd_probe_read returns the number of
+ * sampled launches and their summed duration since the last read, and
+ * resets both.  Measurement only; no reference counterpart. */
+int ofd_probe_set(ofd_ctx *ctx, int every);
+int ofd_probe_read(ofd_ctx *ctx, int64_t *count, double *total_ms);
+
 /* device memory footprint of the context in bytes */
 int ofd_ctx_bytes(ofd_ctx *ctx, int64_t *bytes);
 

@@ -140,6 +140,19 @@ static int ensure_prec(Ctx &c, int precision) {
   return OFD_OK;
 }
 
+// read back the sampled Arnoldi-product timings recorded since the last
+// host synchronisation (the caller has synchronised the stream)
+static int probe_harvest(Ctx &c) {
+  for (int i = 0; i < c.probe_used; ++i) {
+    float t = 0.f;
+    OFD_HIP(hipEventElapsedTime(&t, c.probe_ev[2 * i], c.probe_ev[2 * i + 1]));
+    c.probe_ms += t;
+    ++c.probe_count;
+  }
+  c.probe_used = 0;
+  return OFD_OK;
+}
+
 // one GMRES cycle on right-hand side b (device [N][4]); x += increment
 static int gmres_cycle(Ctx &c, int precision, int ortho, const double *b, double tol,
                        int K, int kmin, double *x, int *iters, double *resid,
@@ -165,15 +178,18 @@ static int gmres_cycle(Ctx &c, int precision, int ortho, const double *b, double
       OFD_HIP(hipEventSynchronize(c.iter_ev[k - kLag]));
       if (c.h_flags[k - kLag - 1]) break;
     }
+    const bool probe = c.probe_every > 0 && k % c.probe_every == 0 &&
+                       c.probe_used < (int)c.probe_ev.size() / 2;
+    if (probe) OFD_HIP(hipEventRecord(c.probe_ev[2 * c.probe_used], c.stream));
+    OFD_HIP(launch_spmv_arnoldi(c, precision, k));
+    if (probe) OFD_HIP(hipEventRecord(c.probe_ev[2 * c.probe_used++ + 1], c.stream));
     if (ortho == OFD_ORTHO_MGS) {
-      OFD_HIP(launch_spmv_arnoldi(c, precision, k));
       for (int j = 0; j <= k + 1; ++j) {
         OFD_HIP(launch_mgs_step(c, k, j));
         if (j == 0) OFD_HIP(hipEventRecord(c.iter_ev[k], c.stream));
       }
     } else {
-      // product; projection dots (closing step k-1); update sweep(s)
-      OFD_HIP(launch_spmv_arnoldi(c, precision, k));
+      // (product above); projection dots (closing step k-1); update sweep(s)
       OFD_HIP(launch_cgs_dots(c, k));
       OFD_HIP(hipEventRecord(c.iter_ev[k], c.stream));
       if (ortho == OFD_ORTHO_CGS1) {
@@ -191,6 +207,8 @@ static int gmres_cycle(Ctx &c, int precision, int ortho, const double *b, double
   OFD_HIP(hipMemcpyAsync(c.h_gs, c.d_gs, sizeof(GmresState), hipMemcpyDeviceToHost,
                          c.stream));
   OFD_HIP(hipStreamSynchronize(c.stream));
+  rc = probe_harvest(c);
+  if (rc) return rc;
   if (iters) *iters = c.h_gs->iters;
   if (resid) *resid = c.h_gs->resid;
   if (rnorm0) *rnorm0 = c.h_gs->rnorm0;
@@ -298,6 +316,7 @@ int ofd_ctx_destroy(ofd_ctx *x) {
   if (c.h_gs) (void)hipHostFree(c.h_gs);
   if (c.h_flags) (void)hipHostFree(c.h_flags);
   for (auto &ev : c.iter_ev) (void)hipEventDestroy(ev);
+  for (auto &ev : c.probe_ev) (void)hipEventDestroy(ev);
   if (c.h_pin) (void)hipHostFree(c.h_pin);
   if (c.stream) (void)hipStreamDestroy(c.stream);
   delete x;
@@ -742,6 +761,8 @@ int ofd_solve(ofd_ctx *x, int precision, int ortho, double rtol, double inner_to
     OFD_HIP(hipMemcpyAsync(c.h_gs, c.d_gs, sizeof(GmresState), hipMemcpyDeviceToHost,
                            c.stream));
     OFD_HIP(hipStreamSynchronize(c.stream));
+    rc = probe_harvest(c);
+    if (rc) break;
     total += c.h_gs->iters;
     const double rn = c.h_gs->scal[0];
     if (!std::isfinite(c.h_gs->resid) || !std::isfinite(c.h_gs->rnorm0)) {
@@ -889,6 +910,30 @@ int ofd_time_assemble(ofd_ctx *x, int tensor_flag, int reps, double *ms) {
     c.prec_valid = c.a32_valid = c.ap32_valid = false;
   }
   *ms = (double)t / reps;
+  return OFD_OK;
+}
+
+int ofd_probe_set(ofd_ctx *x, int every) {
+  OFD_CHECK(x && every >= 0, OFD_ERR_ARG, "bad argument");
+  Ctx &c = x->c;
+  OFD_HIP(hipSetDevice(c.device));
+  if (every > 0 && c.probe_ev.empty()) {
+    // one cycle holds at most kMaxKspace iterations, read back at its end
+    c.probe_ev.resize(2 * kMaxKspace);
+    for (auto &ev : c.probe_ev) OFD_HIP(hipEventCreate(&ev));
+  }
+  c.probe_every = every;
+  c.probe_used = 0;
+  return OFD_OK;
+}
+
+int ofd_probe_read(ofd_ctx *x, int64_t *count, double *total_ms) {
+  OFD_CHECK(x && count && total_ms, OFD_ERR_ARG, "null argument");
+  Ctx &c = x->c;
+  *count = c.probe_count;
+  *total_ms = c.probe_ms;
+  c.probe_count = 0;
+  c.probe_ms = 0.0;
   return OFD_OK;
 }
 

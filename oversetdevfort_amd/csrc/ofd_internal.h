@@ -122,6 +122,14 @@ struct Ctx {
   size_t h_pin_bytes = 0;
   int64_t dev_bytes = 0;
   std::vector<double> solve_hist;     // relres after each cycle of the last solve
+  // sampled timing of the Arnoldi product kernel inside real solves: every
+  // probe_every-th iteration is bracketed by a pair of timing events, which
+  // are read back at the next host synchronisation of the solve
+  int probe_every = 0;
+  std::vector<hipEvent_t> probe_ev;   // 2 per slot
+  int probe_used = 0;                 // slots recorded since the last read-back
+  int64_t probe_count = 0;
+  double probe_ms = 0.0;
 };
 
 // ---- launchers (defined in the .hip files) ----

@@ -220,6 +220,18 @@ class Context:
                                            ctypes.byref(ms)))
         return ms.value
 
+    def probe_set(self, every):
+        """Sample every ``every``-th Arnoldi product launch of later solves
+        with HIP timing events (0: off)."""
+        L.check(self.lib.ofd_probe_set(self.h, int(every)))
+
+    def probe_read(self):
+        """(launches sampled, summed ms) since the last read; resets both."""
+        n = ctypes.c_int64()
+        ms = ctypes.c_double()
+        L.check(self.lib.ofd_probe_read(self.h, ctypes.byref(n), ctypes.byref(ms)))
+        return n.value, ms.value
+
     def device_bytes(self):
         b = ctypes.c_int64()
         L.check(self.lib.ofd_ctx_bytes(self.h, ctypes.byref(b)))
