@@ -109,16 +109,11 @@ def traffic(kernel, workload):
 
 def pmc_sample(ctx, S, L, prob, ortho, kspace):
     """Kernels for one hardware-counter pass: a device copy of 256 MiB
-    (three launches; reads and writes 2^28 bytes each) to calibrate the
-    counters, then one assembly and one full GMRES(kspace) cycle per
+    (k_copy, four launches; each reads and writes 2^28 bytes) to calibrate
+    the counters, then one assembly and one full GMRES(kspace) cycle per
     precision (kspace Arnoldi products each)."""
-    import torch
-
-    src = torch.ones(1 << 26, dtype=torch.float32, device="cuda:0")
-    dst = torch.empty_like(src)
-    for _ in range(3):
-        dst.copy_(src)
-    torch.cuda.synchronize()
+    ms = ctx.time_copy(1 << 28, reps=3)
+    log(f"pmc sample: copy 256 MiB {1e3 * ms:.1f} us")
     ctx.set_state(prob.fields)
     S.begin_step(ctx)
     ctx.assembleNavSto(6, 1)
@@ -249,6 +244,10 @@ def main():
     ctx.assembleNavSto(6, 1)
     spmv64_ms = ctx.time_spmv(L.OFD_FP64, reps=args.spmv_reps)
     spmv32_ms = ctx.time_spmv(L.OFD_FP32, reps=args.spmv_reps)
+    copy_ms = ctx.time_copy(1 << 30, reps=10)
+    copy_bw = 2 * (1 << 30) / (copy_ms * 1e-3)
+    log(f"bench: device copy 1 GiB: {1e3 * copy_ms:.1f} us, {copy_bw / 1e9:.0f} GB/s "
+        "(read + write)")
     b64 = spmv_bytes(Z, N, 8)
     b32 = spmv_bytes(Z, N, 4)
     # Arnoldi product W_{k+1} = (P^-1 A) W_k: FP32 matrix and FP32 basis in
@@ -320,6 +319,7 @@ def main():
         "spmv": {"fp64_us": spmv64_ms * 1e3, "fp64_GBps": bw64 / 1e9,
                  "fp32_us": spmv32_ms * 1e3, "fp32_GBps": bw32 / 1e9,
                  "fp64_bytes": b64, "fp32_bytes": b32},
+        "hbm_copy_GBps": copy_bw / 1e9,
         "roofline": {"bound": "hbm", "kernel": ARN32,
                      "achieved": bwa32 / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                      "frac": bwa32 / PEAK_HBM, "traffic": traffic(ARN32, wl),

@@ -161,6 +161,12 @@ int ofd_rhs_norms(ofd_ctx *ctx, double *norms2);
 int ofd_time_spmv(ofd_ctx *ctx, int precision, int reps, double *ms);
 int ofd_time_assemble(ofd_ctx *ctx, int tensor_flag, int reps, double *ms);
 
+/* plain device-to-device copy of nbytes (multiple of 16) by a grid-stride
+ * kernel; ms = average of `reps` launches.  The bandwidth reference for the
+ * HBM-bound kernels (and the calibration of hardware byte counters).
+ * Allocates and frees its two buffers.  No reference counterpart. */
+int ofd_time_copy(ofd_ctx *ctx, int64_t nbytes, int reps, double *ms);
+
 /* sampled timing of the Arnoldi product kernel (the preconditioned SpMV of
  * each GMRES iteration) inside ofd_solve / ofd_gmres_up: every `every`-th
  * iteration's launch is bracketed by HIP timing events on the context

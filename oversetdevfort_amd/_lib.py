@@ -68,6 +68,7 @@ SIGNATURES = {
     "ofd_rhs_norms": ([_vp, _dp], _c_int),
     "ofd_time_spmv": ([_vp, _c_int, _c_int, _dp], _c_int),
     "ofd_time_assemble": ([_vp, _c_int, _c_int, _dp], _c_int),
+    "ofd_time_copy": ([_vp, ctypes.c_int64, _c_int, _dp], _c_int),
     "ofd_probe_set": ([_vp, _c_int], _c_int),
     "ofd_probe_read": ([_vp, _i64p, _dp], _c_int),
     "ofd_ctx_bytes": ([_vp, _i64p], _c_int),

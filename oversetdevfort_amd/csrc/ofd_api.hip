@@ -180,9 +180,13 @@ static int gmres_cycle(Ctx &c, int precision, int ortho, const double *b, double
     }
     const bool probe = c.probe_every > 0 && k % c.probe_every == 0 &&
                        c.probe_used < (int)c.probe_ev.size() / 2;
-    if (probe) OFD_HIP(hipEventRecord(c.probe_ev[2 * c.probe_used], c.stream));
-    OFD_HIP(launch_spmv_arnoldi(c, precision, k));
-    if (probe) OFD_HIP(hipEventRecord(c.probe_ev[2 * c.probe_used++ + 1], c.stream));
+    if (probe) {
+      OFD_HIP(launch_spmv_arnoldi(c, precision, k, c.probe_ev[2 * c.probe_used],
+                                  c.probe_ev[2 * c.probe_used + 1]));
+      ++c.probe_used;
+    } else {
+      OFD_HIP(launch_spmv_arnoldi(c, precision, k));
+    }
     if (ortho == OFD_ORTHO_MGS) {
       for (int j = 0; j <= k + 1; ++j) {
         OFD_HIP(launch_mgs_step(c, k, j));
@@ -909,6 +913,39 @@ int ofd_time_assemble(ofd_ctx *x, int tensor_flag, int reps, double *ms) {
     c.mat_valid = true;
     c.prec_valid = c.a32_valid = c.ap32_valid = false;
   }
+  *ms = (double)t / reps;
+  return OFD_OK;
+}
+
+int ofd_time_copy(ofd_ctx *x, int64_t nbytes, int reps, double *ms) {
+  OFD_CHECK(x && ms && reps > 0, OFD_ERR_ARG, "bad argument");
+  OFD_CHECK(nbytes >= 16 && nbytes % 16 == 0 && nbytes <= (int64_t(1) << 34), OFD_ERR_ARG,
+            "nbytes must be a multiple of 16 in [16, 16 GiB]");
+  Ctx &c = x->c;
+  OFD_HIP(hipSetDevice(c.device));
+  void *src = nullptr, *dst = nullptr;
+  OFD_HIP(hipMalloc(&src, (size_t)nbytes));
+  hipError_t e = hipMalloc(&dst, (size_t)nbytes);
+  if (e != hipSuccess) {
+    (void)hipFree(src);
+    OFD_HIP(e);
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  float t = 0.f;
+  e = hipMemsetAsync(src, 0, (size_t)nbytes, c.stream);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = launch_copy(c, src, dst, (size_t)nbytes);  // warm
+  if (e == hipSuccess) e = hipEventRecord(e0, c.stream);
+  for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_copy(c, src, dst, (size_t)nbytes);
+  if (e == hipSuccess) e = hipEventRecord(e1, c.stream);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(src);
+  (void)hipFree(dst);
+  OFD_HIP(e);
   *ms = (double)t / reps;
   return OFD_OK;
 }

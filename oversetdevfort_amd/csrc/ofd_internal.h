@@ -145,7 +145,8 @@ hipError_t launch_precondition_matrix(Ctx &c, bool want32);
 hipError_t launch_convert_a32(Ctx &c);
 hipError_t launch_spmv(Ctx &c, int precision, const double *x, double *y,
                        const double *b, int mode);
-hipError_t launch_spmv_arnoldi(Ctx &c, int precision, int k);
+hipError_t launch_spmv_arnoldi(Ctx &c, int precision, int k, hipEvent_t ev0 = nullptr,
+                               hipEvent_t ev1 = nullptr);
 hipError_t launch_cgs_dots(Ctx &c, int k);
 hipError_t launch_cgs_update(Ctx &c, int k, int mode);
 // vectors
@@ -154,6 +155,7 @@ hipError_t launch_soa_to_node4(Ctx &c, const double *u, const double *p,
 hipError_t launch_node4_to_soa(Ctx &c, const double *in, double *u, double *p);
 hipError_t launch_axpy_fields(Ctx &c, int which);
 hipError_t launch_apply_dirichlet(Ctx &c);
+hipError_t launch_copy(Ctx &c, const void *src, void *dst, size_t nbytes);
 // gmres pieces
 hipError_t launch_gmres_init(Ctx &c, const double *b, double tol, int kspace, int kmin);
 hipError_t launch_mgs_step(Ctx &c, int k, int j);

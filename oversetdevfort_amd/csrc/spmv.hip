@@ -16,6 +16,8 @@
 // block-diagonal [Binv 0; 0 1/M].  Left preconditioning is applied once per
 // assembled matrix: the Krylov loop runs on the stored product P^-1 A (same
 // pattern, FP64 and FP32 copies), so its SpMV has no epilogue.
+#include <hip/hip_ext.h>
+
 #include "gmres_dev.h"
 
 namespace ofd {
@@ -229,24 +231,32 @@ hipError_t launch_spmv(Ctx &c, int precision, const double *x, double *y,
 
 // Arnoldi product W_{k+1} = (P^-1 A) W_k, unscaled (FP32 basis in the mixed
 // CGS solver)
-hipError_t launch_spmv_arnoldi(Ctx &c, int precision, int k) {
+// ev0/ev1 (may be null) receive the kernel's own start and end timestamps
+hipError_t launch_spmv_arnoldi(Ctx &c, int precision, int k, hipEvent_t ev0,
+                               hipEvent_t ev1) {
   const unsigned nb = spmv_blocks(c);
   const size_t vs = (size_t)c.nnode * 4;
   if (c.basis32) {
     const float *x = c.d_W32 + vs * k;
     float *y = c.d_W32 + vs * (k + 1);
-    hipLaunchKernelGGL((k_spmv<float, float, 2>), dim3(nb), dim3(256), 0, c.stream,
-                       c.nnode, c.d_col, c.d_row, c.d_Ap32, x, y, nullptr, c.d_gs);
+    hipExtLaunchKernelGGL((k_spmv<float, float, 2>), dim3(nb), dim3(256), 0, c.stream, ev0,
+                          ev1, 0, c.nnode, (const int32_t *)c.d_col, (const int32_t *)c.d_row,
+                          (const float *)c.d_Ap32, x, y, (const double *)nullptr,
+                          (const GmresState *)c.d_gs);
     return hipGetLastError();
   }
   const double *x = c.d_W + vs * k;
   double *y = c.d_W + vs * (k + 1);
   if (precision == OFD_FP32)
-    hipLaunchKernelGGL((k_spmv<float, double, 2>), dim3(nb), dim3(256), 0, c.stream,
-                       c.nnode, c.d_col, c.d_row, c.d_Ap32, x, y, nullptr, c.d_gs);
+    hipExtLaunchKernelGGL((k_spmv<float, double, 2>), dim3(nb), dim3(256), 0, c.stream, ev0,
+                          ev1, 0, c.nnode, (const int32_t *)c.d_col, (const int32_t *)c.d_row,
+                          (const float *)c.d_Ap32, x, y, (const double *)nullptr,
+                          (const GmresState *)c.d_gs);
   else
-    hipLaunchKernelGGL((k_spmv<double, double, 2>), dim3(nb), dim3(256), 0, c.stream,
-                       c.nnode, c.d_col, c.d_row, c.d_Ap64, x, y, nullptr, c.d_gs);
+    hipExtLaunchKernelGGL((k_spmv<double, double, 2>), dim3(nb), dim3(256), 0, c.stream, ev0,
+                          ev1, 0, c.nnode, (const int32_t *)c.d_col, (const int32_t *)c.d_row,
+                          (const double *)c.d_Ap64, x, y, (const double *)nullptr,
+                          (const GmresState *)c.d_gs);
   return hipGetLastError();
 }
 

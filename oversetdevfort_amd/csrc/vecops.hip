@@ -88,6 +88,22 @@ __global__ void k_apply_dirichlet(int N, const uint8_t *__restrict__ ibc,
   }
 }
 
+// plain device copy in 16-byte units, grid-stride: the bandwidth and
+// hardware-counter reference for the bandwidth-bound kernels
+__global__ void __launch_bounds__(256) k_copy(size_t n16, const double2 *__restrict__ src,
+                                              double2 *__restrict__ dst) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
+    dst[i] = src[i];
+}
+
+hipError_t launch_copy(Ctx &c, const void *src, void *dst, size_t nbytes) {
+  const size_t n16 = nbytes / 16;
+  hipLaunchKernelGGL(k_copy, dim3(8192), dim3(256), 0, c.stream, n16,
+                     static_cast<const double2 *>(src), static_cast<double2 *>(dst));
+  return hipGetLastError();
+}
+
 hipError_t launch_apply_dirichlet(Ctx &c) {
   double **f = c.d_field;
   hipLaunchKernelGGL(k_apply_dirichlet, dim3((c.nnode + 255) / 256), dim3(256), 0,

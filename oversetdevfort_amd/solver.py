@@ -220,6 +220,13 @@ class Context:
                                            ctypes.byref(ms)))
         return ms.value
 
+    def time_copy(self, nbytes=1 << 28, reps=10):
+        """Average ms of one device copy of ``nbytes`` (reads and writes
+        ``nbytes`` each)."""
+        ms = ctypes.c_double()
+        L.check(self.lib.ofd_time_copy(self.h, int(nbytes), int(reps), ctypes.byref(ms)))
+        return ms.value
+
     def probe_set(self, every):
         """Sample every ``every``-th Arnoldi product launch of later solves
         with HIP timing events (0: off)."""
