@@ -62,15 +62,48 @@ __device__ __forceinline__ double dot4(const double a[4], const double b[4]) {
   return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3];
 }
 
-// Partials of <z, W_j>, j < m, for the node of this thread, reduced over the
-// block in chunks of kCh.  Every load of a chunk is issued unconditionally
-// (indices past m clamped to m-1, their products dropped) so all of them
-// are in flight together.
+// four entries of a node as loaded (float4 or two double2), converted to
+// FP64 only when used, so loads can be issued early without doubling the
+// registers an FP32 basis needs
+template <typename V>
+struct Raw4;
+template <>
+struct Raw4<float> {
+  float4 a;
+  __device__ __forceinline__ void load(const float *p) {
+    a = reinterpret_cast<const float4 *>(p)[0];
+  }
+  __device__ __forceinline__ void get(double v[4]) const {
+    v[0] = a.x;
+    v[1] = a.y;
+    v[2] = a.z;
+    v[3] = a.w;
+  }
+};
+template <>
+struct Raw4<double> {
+  double2 a, b;
+  __device__ __forceinline__ void load(const double *p) {
+    a = reinterpret_cast<const double2 *>(p)[0];
+    b = reinterpret_cast<const double2 *>(p)[1];
+  }
+  __device__ __forceinline__ void get(double v[4]) const {
+    v[0] = a.x;
+    v[1] = a.y;
+    v[2] = b.x;
+    v[3] = b.y;
+  }
+};
+
+// Partials of <z, W_j>, j0 <= j < m, for the node of this thread, reduced
+// over the block in chunks of kCh.  Every load of a chunk is issued
+// unconditionally (indices past m clamped to m-1, their products dropped)
+// so all of them are in flight together.
 template <typename V>
 __device__ __forceinline__ void dots_chunks(const V *__restrict__ W, size_t vs, int i,
                                             bool live, const double zz[4], int m,
-                                            double *__restrict__ wr) {
-  for (int j0 = 0; j0 < m; j0 += kCh) {
+                                            double *__restrict__ wr, int jstart = 0) {
+  for (int j0 = jstart; j0 < m; j0 += kCh) {
     const int n = (m - j0 < kCh) ? (m - j0) : kCh;
     double v[kCh][4];
 #pragma unroll
@@ -89,8 +122,9 @@ __device__ __forceinline__ void dots_chunks(const V *__restrict__ W, size_t vs, 
 // per batch (indices past m clamped, their weights zero), subtracted in j order
 template <typename V>
 __device__ __forceinline__ void axpy_batches(const V *__restrict__ W, size_t vs, int i,
-                                             const double *coef, int m, double zz[4]) {
-  for (int j0 = 0; j0 < m; j0 += 8) {
+                                             const double *coef, int m, double zz[4],
+                                             int jstart = 0) {
+  for (int j0 = jstart; j0 < m; j0 += 8) {
     double v[8][4];
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
@@ -201,14 +235,32 @@ k_cgs_dots(int N, const V *__restrict__ W, const double *__restrict__ rd,
            double *__restrict__ wr, GmresState *__restrict__ g, int k, int nb,
            int32_t *hflag) {
   if (g->done) return;
-  if (arnoldi_close(g, rd, nb, k - 1, 1, hflag)) return;
   const size_t vs = (size_t)N * 4;
   const V *z = W + vs * (k + 1);
   const int m = k + 1;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double zz[4] = {0.0, 0.0, 0.0, 0.0};
-  if (i < N) ld4(z + (size_t)i * 4, zz);
-  dots_chunks(W, vs, i < N ? i : 0, i < N, zz, m, wr);
+  const bool live = i < N;
+  const int ii = live ? i : 0;
+  // the first chunk's loads do not depend on the close: in flight during it
+  Raw4<V> zr, vr[kCh];
+  zr.load(z + (size_t)ii * 4);
+#pragma unroll
+  for (int t = 0; t < kCh; ++t) vr[t].load(W + vs * (t < m ? t : m - 1) + (size_t)ii * 4);
+  if (arnoldi_close(g, rd, nb, k - 1, 1, hflag)) return;
+  double zz[4];
+  zr.get(zz);
+  {
+    const int n = m < kCh ? m : kCh;
+    double acc[kCh];
+#pragma unroll
+    for (int t = 0; t < kCh; ++t) {
+      double v[4];
+      vr[t].get(v);
+      acc[t] = (live && t < n) ? dot4(zz, v) : 0.0;
+    }
+    block_partials<kCh>(acc, n, wr, 0);
+  }
+  if (m > kCh) dots_chunks(W, vs, ii, live, zz, m, wr, kCh);
 }
 
 // Update sweeps of the CGS variants after the projection dots of the
@@ -228,7 +280,15 @@ k_cgs_update(int N, V *__restrict__ W, const double *__restrict__ rd_dots,
   V *z = W + vs * (k + 1);
   const int m = k + 1;
   __shared__ double s_coef[kMaxKspace + 1];
-  // the scales are loaded before the partial sums so both round trips overlap
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = i < N;
+  const int ii = live ? i : 0;
+  // z and the first 8 basis vectors do not depend on the coefficients:
+  // their loads, the scales and the partial sums are all in flight together
+  Raw4<V> zr, vr[8];
+  zr.load(z + (size_t)ii * 4);
+#pragma unroll
+  for (int t = 0; t < 8; ++t) vr[t].load(W + vs * (t < m ? t : m - 1) + (size_t)ii * 4);
   const int t0 = threadIdx.x;
   const double s_own = (t0 < m) ? g->s[t0] : 0.0;
   final_sums(rd_dots, nb, m, s_coef);
@@ -246,19 +306,24 @@ k_cgs_update(int N, V *__restrict__ W, const double *__restrict__ rd_dots,
     s_coef[j] = h * sj;
   }
   __syncthreads();
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double zz[4] = {0.0, 0.0, 0.0, 0.0};
-  if (i < N) {
-    ld4(z + (size_t)i * 4, zz);
-    axpy_batches(W, vs, i, s_coef, m, zz);
-    st4(z + (size_t)i * 4, zz);
+  double zz[4];
+  zr.get(zz);
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    double v[4];
+    vr[t].get(v);
+    const double cj = (t < m) ? s_coef[t] : 0.0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) zz[c] -= cj * v[c];
   }
+  axpy_batches(W, vs, ii, s_coef, m, zz, 8);
+  if (live) st4(z + (size_t)i * 4, zz);
   if (mode != 2) {
-    double acc[1] = {dot4(zz, zz)};
+    double acc[1] = {live ? dot4(zz, zz) : 0.0};
     block_partials<1>(acc, 1, wr, 0);
     return;
   }
-  dots_chunks(W, vs, i < N ? i : 0, i < N, zz, m, wr);
+  dots_chunks(W, vs, ii, live, zz, m, wr);
 }
 
 // closes the last Arnoldi step of a cycle (one block)

@@ -121,26 +121,27 @@ __device__ __forceinline__ bool arnoldi_close(GmresState *__restrict__ g,
                                               int k, int cgs, int32_t *hflag,
                                               double *s_next = nullptr) {
   __shared__ double s_res[2];
-  __shared__ double s_col[kMaxKspace + 2];
+  __shared__ double s_h[kMaxKspace + 1];   // s_k h_q, q <= k
+  __shared__ double s_col[kMaxKspace + 2]; // rotated column
   __shared__ double s_rc[kMaxKspace], s_rs[kMaxKspace];
   const int t = threadIdx.x;
   // issue every global load of the close before the first barrier: the
-  // column entries and rotations (threads q <= k), the cycle scalars
+  // column entries, rotations and s_k (threads q <= k), the cycle scalars
   // (thread 0) and the norm partials (all threads, inside sum1)
   const int K1 = g->kspace + 1;
-  double colq = 0.0, rcq = 0.0, rsq = 0.0;
+  double colq = 0.0, rcq = 0.0, rsq = 0.0, sk = 0.0;
   if (k >= 0 && t <= k) {
     colq = cgs ? g->h1[t] + g->h2[t] : g->H[t + K1 * k];
+    sk = g->s[k];
     if (t < k) {
       rcq = g->rc[t];
       rsq = g->rs[t];
     }
   }
-  double sk = 0.0, e0 = 0.0, e1 = 0.0, eps = 0.0, tol = 0.0;
+  double e0 = 0.0, e1 = 0.0, eps = 0.0, tol = 0.0;
   int kmin = 0;
   if (t == 0) {
     if (k >= 0) {
-      sk = g->s[k];
       e0 = g->e[k];
       e1 = g->e[k + 1];
       eps = g->eps;
@@ -151,7 +152,7 @@ __device__ __forceinline__ bool arnoldi_close(GmresState *__restrict__ g,
   }
   const double nrm = sqrt(sum1(part, nb));
   if (k >= 0 && t <= k) {
-    s_col[t] = colq;
+    s_h[t] = colq * sk;
     if (t < k) {
       s_rc[t] = rcq;
       s_rs[t] = rsq;
@@ -174,24 +175,24 @@ __device__ __forceinline__ bool arnoldi_close(GmresState *__restrict__ g,
         g->done = done ? 1 : 0;
       }
     } else {
-      double *col = s_col;
-      for (int q = 0; q <= k; ++q) col[q] *= sk;
-      col[k + 1] = sk * nrm;
+      // previous rotations applied in order; entry q of the running column
+      // is carried in a register, the loads of h, c, s do not depend on it
+      double carry = s_h[0];
+#pragma unroll 8
       for (int q = 0; q < k; ++q) {
-        const double c = s_rc[q], s = s_rs[q];
-        const double tq = c * col[q] + s * col[q + 1];
-        col[q + 1] = -s * col[q] + c * col[q + 1];
-        col[q] = tq;
+        const double c = s_rc[q], s = s_rs[q], h = s_h[q + 1];
+        s_col[q] = c * carry + s * h;
+        carry = -s * carry + c * h;
       }
-      const double hk = col[k], hk1 = col[k + 1];
+      const double hk = carry, hk1 = sk * nrm;
       const double tn = sqrt(hk * hk + hk1 * hk1);
       const double rc = hk / tn, rs = hk1 / tn;
       const double ek = rc * e0 + rs * e1;
       const double ek1 = -rs * e0 + rc * e1;
       const double er = fabs(ek1);
       done = (er <= eps && k + 1 >= kmin) || !(er == er) || hk1 == 0.0;
-      col[k] = tn;
-      col[k + 1] = 0.0;
+      s_col[k] = tn;
+      s_col[k + 1] = 0.0;
       if (w) {
         g->rc[k] = rc;
         g->rs[k] = rs;

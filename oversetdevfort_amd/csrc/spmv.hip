@@ -80,7 +80,13 @@ k_spmv(int N, const int32_t *__restrict__ col, const int32_t *__restrict__ row,
        const T *__restrict__ A, const V *__restrict__ x, V *__restrict__ y,
        const double *__restrict__ b, const GmresState *__restrict__ gs) {
   if (MODE == 2 && gs->done) return;
-  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+  // L2.  The grid is a multiple of 8 and XCD j takes the j-th contiguous
+  // eighth of the rows, so neighbouring rows, which gather the same x
+  // entries, share one L2 instead of every L2 fetching all of x.
+  const int per = gridDim.x >> 3;
+  const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  const int gt = lb * blockDim.x + threadIdx.x;
   const int r = gt >> 2;
   const int c = gt & 3;
   if (r >= N) return;
@@ -204,8 +210,11 @@ hipError_t launch_convert_a32(Ctx &c) {
   return hipGetLastError();
 }
 
+// 4 lanes per row, 256-thread blocks, rounded up to a multiple of 8 (one
+// share per XCD; blocks past the last row return at once)
 static inline unsigned spmv_blocks(const Ctx &c) {
-  return (unsigned)(((size_t)c.nnode * 4 + 255) / 256);
+  const unsigned nb = (unsigned)(((size_t)c.nnode * 4 + 255) / 256);
+  return (nb + 7u) & ~7u;
 }
 
 hipError_t launch_spmv(Ctx &c, int precision, const double *x, double *y,

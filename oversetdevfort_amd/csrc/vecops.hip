@@ -90,16 +90,26 @@ __global__ void k_apply_dirichlet(int N, const uint8_t *__restrict__ ibc,
 
 // plain device copy in 16-byte units, grid-stride: the bandwidth and
 // hardware-counter reference for the bandwidth-bound kernels
+// (4 independent 16-byte loads in flight per thread and pass)
 __global__ void __launch_bounds__(256) k_copy(size_t n16, const double2 *__restrict__ src,
                                               double2 *__restrict__ dst) {
   const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
-    dst[i] = src[i];
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < n16; i += 4 * stride) {
+    double2 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = src[i + u * stride];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) dst[i + u * stride] = v[u];
+  }
+  for (; i < n16; i += stride) dst[i] = src[i];
 }
 
 hipError_t launch_copy(Ctx &c, const void *src, void *dst, size_t nbytes) {
   const size_t n16 = nbytes / 16;
-  hipLaunchKernelGGL(k_copy, dim3(8192), dim3(256), 0, c.stream, n16,
+  const size_t want = (n16 + 4 * 256 - 1) / (4 * 256);
+  const unsigned nb = (unsigned)(want < 16384 ? (want > 0 ? want : 1) : 16384);
+  hipLaunchKernelGGL(k_copy, dim3(nb), dim3(256), 0, c.stream, n16,
                      static_cast<const double2 *>(src), static_cast<double2 *>(dst));
   return hipGetLastError();
 }
