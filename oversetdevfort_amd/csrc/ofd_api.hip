@@ -87,10 +87,9 @@ static int need_pattern(const Ctx &c) {
 static int ensure_krylov(Ctx &c, int K) {
   OFD_CHECK(K >= 1 && K <= kMaxKspace, OFD_ERR_ARG, "kspace out of range [1, 64]");
   if (!c.h_flags) {
+    // coherent: the device's flag stores reach the host while kernels run
     OFD_HIP(hipHostMalloc(reinterpret_cast<void **>(&c.h_flags),
-                          sizeof(int32_t) * kMaxKspace));
-    c.iter_ev.resize(kMaxKspace);
-    for (auto &ev : c.iter_ev) OFD_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+                          sizeof(int32_t) * kMaxKspace, hipHostMallocCoherent));
   }
   if (c.basis32) {
     if (c.kalloc32 >= K) return OFD_OK;
@@ -153,6 +152,30 @@ static int probe_harvest(Ctx &c) {
   return OFD_OK;
 }
 
+// Wait until the device has written the convergence flag of Arnoldi step
+// `idx` (-1 = not yet).  Kernels launched after convergence return at once
+// without writing flags, so a stream that has gone idle with the flag still
+// unwritten means the cycle stopped earlier: reported as converged.
+static int wait_flag(Ctx &c, int idx, bool *done) {
+  volatile int32_t *f = c.h_flags;
+  for (unsigned spin = 1;; ++spin) {
+    const int32_t v = f[idx];
+    if (v >= 0) {
+      *done = v != 0;
+      return OFD_OK;
+    }
+    if ((spin & 255u) == 0u) {
+      const hipError_t e = hipStreamQuery(c.stream);
+      if (e == hipSuccess) {
+        *done = f[idx] != 0;
+        return OFD_OK;
+      }
+      if (e != hipErrorNotReady) OFD_HIP(e);
+    }
+    __builtin_ia32_pause();
+  }
+}
+
 // one GMRES cycle on right-hand side b (device [N][4]); x += increment
 static int gmres_cycle(Ctx &c, int precision, int ortho, const double *b, double tol,
                        int K, int kmin, double *x, int *iters, double *resid,
@@ -165,18 +188,21 @@ static int gmres_cycle(Ctx &c, int precision, int ortho, const double *b, double
   if (rc) return rc;
   OFD_HIP(launch_gmres_init(c, b, tol, K, kmin));
   // The host enqueues iterations ahead of the device and learns about
-  // convergence through a flag the device writes to pinned host memory when
-  // it closes each step; it waits only on the flag of step k - kLag - 1, so
-  // the device queue never drains.  Kernels launched after convergence
-  // return at once.
+  // convergence through a flag the device writes to coherent pinned host
+  // memory when it closes each step; it waits only on the flag of step
+  // k - kLag - 1, so the device queue never drains, and puts no marker
+  // between the kernels of an iteration.  Kernels launched after
+  // convergence return at once.
   constexpr int kLag = 3;
-  for (int k = 0; k < K; ++k) c.h_flags[k] = 0;
+  for (int k = 0; k < K; ++k) c.h_flags[k] = -1;
   int klast = -1;
   for (int k = 0; k < K; ++k) {
     // the first orthogonalisation kernel of iteration j closes step j-1
     if (k >= kLag + 1) {
-      OFD_HIP(hipEventSynchronize(c.iter_ev[k - kLag]));
-      if (c.h_flags[k - kLag - 1]) break;
+      bool stop = false;
+      rc = wait_flag(c, k - kLag - 1, &stop);
+      if (rc) return rc;
+      if (stop) break;
     }
     const bool probe = c.probe_every > 0 && k % c.probe_every == 0 &&
                        c.probe_used < (int)c.probe_ev.size() / 2;
@@ -188,14 +214,10 @@ static int gmres_cycle(Ctx &c, int precision, int ortho, const double *b, double
       OFD_HIP(launch_spmv_arnoldi(c, precision, k));
     }
     if (ortho == OFD_ORTHO_MGS) {
-      for (int j = 0; j <= k + 1; ++j) {
-        OFD_HIP(launch_mgs_step(c, k, j));
-        if (j == 0) OFD_HIP(hipEventRecord(c.iter_ev[k], c.stream));
-      }
+      for (int j = 0; j <= k + 1; ++j) OFD_HIP(launch_mgs_step(c, k, j));
     } else {
       // (product above); projection dots (closing step k-1); update sweep(s)
       OFD_HIP(launch_cgs_dots(c, k));
-      OFD_HIP(hipEventRecord(c.iter_ev[k], c.stream));
       if (ortho == OFD_ORTHO_CGS1) {
         OFD_HIP(launch_cgs_update(c, k, 1));
       } else {
@@ -319,7 +341,6 @@ int ofd_ctx_destroy(ofd_ctx *x) {
   dfree(c.d_gs);
   if (c.h_gs) (void)hipHostFree(c.h_gs);
   if (c.h_flags) (void)hipHostFree(c.h_flags);
-  for (auto &ev : c.iter_ev) (void)hipEventDestroy(ev);
   for (auto &ev : c.probe_ev) (void)hipEventDestroy(ev);
   if (c.h_pin) (void)hipHostFree(c.h_pin);
   if (c.stream) (void)hipStreamDestroy(c.stream);

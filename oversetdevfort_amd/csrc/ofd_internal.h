@@ -116,8 +116,7 @@ struct Ctx {
   GmresState *d_gs = nullptr;
   GmresState *h_gs = nullptr;         // pinned mirror
   int part_cur = 0;                   // partial buffer holding the latest sums
-  int32_t *h_flags = nullptr;         // pinned: per-iteration copy of GmresState::done
-  std::vector<hipEvent_t> iter_ev;    // event after each iteration's flag copy
+  int32_t *h_flags = nullptr;         // coherent pinned: GmresState::done per step (-1 unset)
   double *h_pin = nullptr;            // pinned scratch (host transfers)
   size_t h_pin_bytes = 0;
   int64_t dev_bytes = 0;
