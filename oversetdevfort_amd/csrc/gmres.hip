@@ -150,6 +150,14 @@ __global__ void __launch_bounds__(256)
 k_gmres_init(int N, const double *__restrict__ P, const double *__restrict__ b,
              V *__restrict__ W0, double *__restrict__ part, GmresState *__restrict__ g,
              double tol, int kspace, int kmin) {
+  if (g->outer_done) {
+    // a cycle queued behind the one that met the outer tolerance
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      g->done = 1;
+      g->iters = 0;
+    }
+    return;
+  }
   if (blockIdx.x == 0) {
     for (int q = threadIdx.x; q <= kspace; q += blockDim.x) g->e[q] = 0.0;
     if (threadIdx.x == 0) {
@@ -348,10 +356,24 @@ k_norm2(int N, const double *__restrict__ v, double *__restrict__ part) {
 }
 
 __global__ void __launch_bounds__(256)
-k_norm_fin(const double *__restrict__ part, int nb, GmresState *__restrict__ g, int slot) {
+k_norm_fin(const double *__restrict__ part, int nb, GmresState *__restrict__ g, int slot,
+           int outer) {
   __shared__ double s_sum[1];
   final_sums(part, nb, 1, s_sum);
-  if (threadIdx.x == 0) g->scal[slot] = sqrt(s_sum[0]);
+  if (threadIdx.x == 0) {
+    const double rn = sqrt(s_sum[0]);
+    g->scal[slot] = rn;
+    // outer residual of a restarted solve: at or below the threshold (or
+    // not finite) stops the cycles queued behind this one
+    if (outer) g->outer_done = (rn > g->outer_thr) ? 0 : 1;
+  }
+}
+
+__global__ void k_outer_reset(GmresState *__restrict__ g, double thr) {
+  if (threadIdx.x == 0) {
+    g->outer_thr = thr;
+    g->outer_done = 0;
+  }
 }
 
 // back substitution (SparseGMRES_NS.f90:341-346) and x += sum y_j v_j (:349-352)
@@ -359,6 +381,7 @@ template <typename V>
 __global__ void __launch_bounds__(256)
 k_gmres_finish(int N, const V *__restrict__ W, const GmresState *__restrict__ g,
                double *__restrict__ x) {
+  if (g->outer_done) return;  // queued behind the cycle that met the tolerance
   __shared__ double s_y[kMaxKspace];
   __shared__ double s_H[kMaxKspace * kMaxKspace];  // upper triangle, column-major it x it
   __shared__ double s_e[kMaxKspace + 1];
@@ -369,13 +392,16 @@ k_gmres_finish(int N, const V *__restrict__ W, const GmresState *__restrict__ g,
     s_H[t] = g->H[r + K1 * cidx];
   }
   for (int t = threadIdx.x; t <= it; t += blockDim.x) s_e[t] = g->e[t];
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int j = it - 1; j >= 0; --j) {
-      const double yj = s_e[j] / s_H[j + it * j];
-      s_y[j] = yj * g->s[j];
-      for (int l = 0; l < j; ++l) s_e[l] -= yj * s_H[l + it * j];
-    }
+  // column-oriented back substitution: y_j once e_j is final, then every
+  // thread l < j updates its own e_l (the same subtraction order per entry
+  // as the serial loop)
+  const int t = threadIdx.x;
+  const double s_t = (t < it) ? g->s[t] : 0.0;
+  for (int j = it - 1; j >= 0; --j) {
+    __syncthreads();
+    const double yj = s_e[j] / s_H[j + it * j];
+    if (t < j) s_e[t] -= yj * s_H[t + it * j];
+    if (t == j) s_y[j] = yj * s_t;
   }
   __syncthreads();
   const size_t vs = (size_t)N * 4;
@@ -440,7 +466,7 @@ hipError_t launch_mgs_step(Ctx &c, int k, int j) {
   const int nb = nblocks_nodes(c);
   hipLaunchKernelGGL(k_mgs_step, dim3(nb), dim3(kRedBlock), 0, c.stream, c.nnode, c.d_W,
                      part_buf(c, c.part_cur), part_buf(c, c.part_cur ^ 1), c.d_gs, k, j, nb,
-                     c.h_flags);
+                     c.cur_flags);
   c.part_cur ^= 1;
   return hipGetLastError();
 }
@@ -450,11 +476,11 @@ hipError_t launch_cgs_dots(Ctx &c, int k) {
   if (c.basis32)
     hipLaunchKernelGGL(k_cgs_dots<float>, dim3(nb), dim3(kRedBlock), 0, c.stream, c.nnode,
                        c.d_W32, part_buf(c, k & 1), part_buf(c, 2), c.d_gs, k, nb,
-                       c.h_flags);
+                       c.cur_flags);
   else
     hipLaunchKernelGGL(k_cgs_dots<double>, dim3(nb), dim3(kRedBlock), 0, c.stream,
                        c.nnode, c.d_W, part_buf(c, k & 1), part_buf(c, 2), c.d_gs, k, nb,
-                       c.h_flags);
+                       c.cur_flags);
   return hipGetLastError();
 }
 
@@ -489,14 +515,19 @@ hipError_t launch_gmres_finish(Ctx &c, double *x) {
   return hipGetLastError();
 }
 
-hipError_t launch_norm2(Ctx &c, const double *v, int slot) {
+hipError_t launch_outer_reset(Ctx &c, double thr) {
+  hipLaunchKernelGGL(k_outer_reset, dim3(1), dim3(64), 0, c.stream, c.d_gs, thr);
+  return hipGetLastError();
+}
+
+hipError_t launch_norm2(Ctx &c, const double *v, int slot, int outer) {
   const int nb = nblocks_nodes(c);
   double *pb = part_buf(c, 0);
   hipLaunchKernelGGL(k_norm2, dim3(nb), dim3(kRedBlock), 0, c.stream, c.nnode, v, pb);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_norm_fin, dim3(1), dim3(kRedBlock), 0, c.stream, pb, nb, c.d_gs,
-                     slot);
+                     slot, outer);
   return hipGetLastError();
 }
 

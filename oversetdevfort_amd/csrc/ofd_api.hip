@@ -89,7 +89,9 @@ static int ensure_krylov(Ctx &c, int K) {
   if (!c.h_flags) {
     // coherent: the device's flag stores reach the host while kernels run
     OFD_HIP(hipHostMalloc(reinterpret_cast<void **>(&c.h_flags),
-                          sizeof(int32_t) * kMaxKspace, hipHostMallocCoherent));
+                          sizeof(int32_t) * 2 * kMaxKspace, hipHostMallocCoherent));
+    OFD_HIP(hipHostMalloc(reinterpret_cast<void **>(&c.h_gs_ring), sizeof(GmresState) * 2));
+    for (auto &ev : c.cyc_ev) OFD_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
   }
   if (c.basis32) {
     if (c.kalloc32 >= K) return OFD_OK;
@@ -157,7 +159,7 @@ static int probe_harvest(Ctx &c) {
 // without writing flags, so a stream that has gone idle with the flag still
 // unwritten means the cycle stopped earlier: reported as converged.
 static int wait_flag(Ctx &c, int idx, bool *done) {
-  volatile int32_t *f = c.h_flags;
+  volatile int32_t *f = c.cur_flags;
   for (unsigned spin = 1;; ++spin) {
     const int32_t v = f[idx];
     if (v >= 0) {
@@ -176,10 +178,11 @@ static int wait_flag(Ctx &c, int idx, bool *done) {
   }
 }
 
-// one GMRES cycle on right-hand side b (device [N][4]); x += increment
+// one GMRES cycle on right-hand side b (device [N][4]); x += increment.
+// flag_set selects the step-flag set (cycles queued back to back alternate).
 static int gmres_cycle(Ctx &c, int precision, int ortho, const double *b, double tol,
                        int K, int kmin, double *x, int *iters, double *resid,
-                       double *rnorm0, bool sync = true) {
+                       double *rnorm0, bool sync = true, int flag_set = 0) {
   // the mixed CGS solver keeps its Krylov basis in FP32
   c.basis32 = precision == OFD_FP32 && ortho != OFD_ORTHO_MGS;
   int rc = ensure_krylov(c, K);
@@ -194,7 +197,8 @@ static int gmres_cycle(Ctx &c, int precision, int ortho, const double *b, double
   // between the kernels of an iteration.  Kernels launched after
   // convergence return at once.
   constexpr int kLag = 3;
-  for (int k = 0; k < K; ++k) c.h_flags[k] = -1;
+  c.cur_flags = c.h_flags + (size_t)(flag_set & 1) * kMaxKspace;
+  for (int k = 0; k < K; ++k) c.cur_flags[k] = -1;
   int klast = -1;
   for (int k = 0; k < K; ++k) {
     // the first orthogonalisation kernel of iteration j closes step j-1
@@ -341,6 +345,9 @@ int ofd_ctx_destroy(ofd_ctx *x) {
   dfree(c.d_gs);
   if (c.h_gs) (void)hipHostFree(c.h_gs);
   if (c.h_flags) (void)hipHostFree(c.h_flags);
+  if (c.h_gs_ring) (void)hipHostFree(c.h_gs_ring);
+  for (auto &ev : c.cyc_ev)
+    if (ev) (void)hipEventDestroy(ev);
   for (auto &ev : c.probe_ev) (void)hipEventDestroy(ev);
   if (c.h_pin) (void)hipHostFree(c.h_pin);
   if (c.stream) (void)hipStreamDestroy(c.stream);
@@ -739,6 +746,7 @@ int ofd_gmres_up(ofd_ctx *x, int precision, int ortho, double tol, int kspace,
   if (rc) return rc;
   OFD_HIP(hipSetDevice(c.device));
   OFD_HIP(hipMemsetAsync(c.d_x, 0, (size_t)c.nnode * 4 * sizeof(double), c.stream));
+  OFD_HIP(launch_outer_reset(c, 0.0));
   int it = 0;
   double res = 0.0;
   rc = gmres_cycle(c, precision, ortho, c.d_rhs, tol, kspace, kspace_min, c.d_x, &it,
@@ -776,35 +784,56 @@ int ofd_solve(ofd_ctx *x, int precision, int ortho, double rtol, double inner_to
   }
   const double *rhs = c.d_rhs;
   c.solve_hist.clear();
-  for (cyc = 1; cyc <= max_cycles; ++cyc) {
-    rc = gmres_cycle(c, precision, ortho, rhs, inner_tol, kspace, 0, c.d_x, nullptr,
-                     nullptr, nullptr, false);
-    if (rc) break;
-    // FP64 outer residual r = b - A x and its norm; one host sync per cycle
-    OFD_HIP(launch_spmv(c, OFD_FP64, c.d_x, c.d_r, c.d_rhs, 1));
-    OFD_HIP(launch_norm2(c, c.d_r, 0));
-    OFD_HIP(hipMemcpyAsync(c.h_gs, c.d_gs, sizeof(GmresState), hipMemcpyDeviceToHost,
-                           c.stream));
-    OFD_HIP(hipStreamSynchronize(c.stream));
-    rc = probe_harvest(c);
-    if (rc) break;
-    total += c.h_gs->iters;
-    const double rn = c.h_gs->scal[0];
-    if (!std::isfinite(c.h_gs->resid) || !std::isfinite(c.h_gs->rnorm0)) {
-      rc = OFD_ERR_NONFINITE;
-      set_error("GMRES: non-finite residual");
-      break;
+  // Cycle c+1 is queued before the host reads cycle c's FP64 residual, so
+  // the device never waits for the host between cycles.  The residual norm
+  // kernel sets GmresState::outer_done once |b - A x| <= rtol |b|; every
+  // kernel of a cycle queued behind that one returns at once (x and r stay
+  // as they are).  Each cycle's state is copied to its own pinned slot.
+  OFD_HIP(launch_outer_reset(c, rtol * bnorm));
+  int done_cycles = 0;
+  bool conv = false;
+  for (cyc = 1; cyc <= max_cycles + 1; ++cyc) {
+    if (cyc <= max_cycles) {
+      rc = gmres_cycle(c, precision, ortho, rhs, inner_tol, kspace, 0, c.d_x, nullptr,
+                       nullptr, nullptr, false, cyc & 1);
+      if (rc) break;
+      // FP64 outer residual r = b - A x and its norm
+      OFD_HIP(launch_spmv(c, OFD_FP64, c.d_x, c.d_r, c.d_rhs, 1));
+      OFD_HIP(launch_norm2(c, c.d_r, 0, 1));
+      OFD_HIP(hipMemcpyAsync(&c.h_gs_ring[cyc & 1], c.d_gs, sizeof(GmresState),
+                             hipMemcpyDeviceToHost, c.stream));
+      OFD_HIP(hipEventRecord(c.cyc_ev[cyc & 1], c.stream));
+      rhs = c.d_r;
     }
-    rel = rn / bnorm;
-    c.solve_hist.push_back(rel);
-    rhs = c.d_r;
-    if (!std::isfinite(rel)) {
-      rc = OFD_ERR_NONFINITE;
-      set_error("solve: non-finite residual");
-      break;
+    if (cyc >= 2) {
+      const int p = cyc - 1;  // the cycle whose result is read now
+      OFD_HIP(hipEventSynchronize(c.cyc_ev[p & 1]));
+      const GmresState &st = c.h_gs_ring[p & 1];
+      done_cycles = p;
+      total += st.iters;
+      if (!std::isfinite(st.resid) || !std::isfinite(st.rnorm0)) {
+        rc = OFD_ERR_NONFINITE;
+        set_error("GMRES: non-finite residual");
+        break;
+      }
+      rel = st.scal[0] / bnorm;
+      c.solve_hist.push_back(rel);
+      if (!std::isfinite(rel)) {
+        rc = OFD_ERR_NONFINITE;
+        set_error("solve: non-finite residual");
+        break;
+      }
+      if (rel <= rtol) {
+        conv = true;
+        break;
+      }
     }
-    if (rel <= rtol) break;
   }
+  cyc = done_cycles;
+  // the queued cycle behind the last one read is a no-op or its result is
+  // discarded with the error; drain it before touching x
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  (void)conv;
   if (cyc > max_cycles) cyc = max_cycles;
   if (iters) *iters = total;
   if (cycles) *cycles = cyc;

@@ -60,12 +60,13 @@ struct GmresState {
   double rnorm0;
   double resid;               // |e_{k+1}| at exit
   double scal[4];             // misc reductions (norms)
+  double outer_thr;           // restarted solve: stop once |b - A x| <= outer_thr
   int32_t iters;
   int32_t done;
   int32_t kmin;
   int32_t kspace;
-  uint32_t counter;           // last-block-done ticket
-  uint32_t pad;
+  int32_t outer_done;         // restarted solve reached outer_thr: later cycles are no-ops
+  int32_t pad;
 };
 
 struct Ctx {
@@ -116,7 +117,11 @@ struct Ctx {
   GmresState *d_gs = nullptr;
   GmresState *h_gs = nullptr;         // pinned mirror
   int part_cur = 0;                   // partial buffer holding the latest sums
-  int32_t *h_flags = nullptr;         // coherent pinned: GmresState::done per step (-1 unset)
+  int32_t *h_flags = nullptr;         // coherent pinned: 2 x [kMaxKspace] GmresState::done
+                                      // per step (-1 unset), one set per cycle parity
+  int32_t *cur_flags = nullptr;       // the set of the cycle being enqueued
+  GmresState *h_gs_ring = nullptr;    // pinned: state snapshot per cycle parity
+  hipEvent_t cyc_ev[2] = {nullptr, nullptr};  // after each snapshot copy
   double *h_pin = nullptr;            // pinned scratch (host transfers)
   size_t h_pin_bytes = 0;
   int64_t dev_bytes = 0;
@@ -160,7 +165,8 @@ hipError_t launch_gmres_init(Ctx &c, const double *b, double tol, int kspace, in
 hipError_t launch_mgs_step(Ctx &c, int k, int j);
 hipError_t launch_gmres_close(Ctx &c, int k, int ortho);
 hipError_t launch_gmres_finish(Ctx &c, double *x);
-hipError_t launch_norm2(Ctx &c, const double *v, int slot);
+hipError_t launch_norm2(Ctx &c, const double *v, int slot, int outer = 0);
+hipError_t launch_outer_reset(Ctx &c, double thr);
 
 int nblocks_nodes(const Ctx &c);
 

@@ -80,6 +80,9 @@ k_spmv(int N, const int32_t *__restrict__ col, const int32_t *__restrict__ row,
        const T *__restrict__ A, const V *__restrict__ x, V *__restrict__ y,
        const double *__restrict__ b, const GmresState *__restrict__ gs) {
   if (MODE == 2 && gs->done) return;
+  // restarted solve: the residual of a cycle queued behind the one that met
+  // the tolerance is the one already computed
+  if (MODE == 1 && gs && gs->outer_done) return;
   // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
   // L2.  The grid is a multiple of 8 and XCD j takes the j-th contiguous
   // eighth of the rows, so neighbouring rows, which gather the same x
