@@ -254,6 +254,8 @@ def main():
     # the mixed solver, FP64 both in the FP64 one
     ba32 = spmv_bytes(Z, N, 4, 4, 4)
     ba64 = spmv_bytes(Z, N, 8, 8, 8)
+    if not (res32["arnoldi_spmv_samples"] and res64["arnoldi_spmv_samples"]):
+        raise RuntimeError("no Arnoldi product launches were sampled")
     bwa32 = ba32 / (res32["arnoldi_spmv_ms"] * 1e-3)
     bwa64 = ba64 / (res64["arnoldi_spmv_ms"] * 1e-3)
     wl = f"channel_{nx}x{ny}x{nz}"

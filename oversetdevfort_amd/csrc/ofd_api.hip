@@ -143,6 +143,8 @@ static int ensure_prec(Ctx &c, int precision) {
 
 // read back the sampled Arnoldi-product timings recorded since the last
 // host synchronisation (the caller has synchronised the stream)
+constexpr int kProbeSlots = 4096;
+
 static int probe_harvest(Ctx &c) {
   for (int i = 0; i < c.probe_used; ++i) {
     float t = 0.f;
@@ -834,6 +836,8 @@ int ofd_solve(ofd_ctx *x, int precision, int ortho, double rtol, double inner_to
   // discarded with the error; drain it before touching x
   OFD_HIP(hipStreamSynchronize(c.stream));
   (void)conv;
+  const int prc = probe_harvest(c);
+  if (!rc) rc = prc;
   if (cyc > max_cycles) cyc = max_cycles;
   if (iters) *iters = total;
   if (cycles) *cycles = cyc;
@@ -1005,8 +1009,9 @@ int ofd_probe_set(ofd_ctx *x, int every) {
   Ctx &c = x->c;
   OFD_HIP(hipSetDevice(c.device));
   if (every > 0 && c.probe_ev.empty()) {
-    // one cycle holds at most kMaxKspace iterations, read back at its end
-    c.probe_ev.resize(2 * kMaxKspace);
+    // read back when a solve returns; launches past the capacity of one
+    // solve are not sampled
+    c.probe_ev.resize(2 * kProbeSlots);
     for (auto &ev : c.probe_ev) OFD_HIP(hipEventCreate(&ev));
   }
   c.probe_every = every;
