@@ -42,7 +42,7 @@ def spmv_bytes(Z, N, s_a, s_x=8, s_y=8, b=4, S=16):
 
 
 def run_steps(ctx, S, L, prob, precision, steps, warmup, kspace=30, ortho=1,
-              probe_every=4):
+              probe_every=32):
     """Reset to the initial state, run warmup + timed time steps.  During the
     timed steps every ``probe_every``-th Arnoldi product launch is bracketed
     by HIP timing events on the solver stream; their mean is returned as
@@ -195,7 +195,7 @@ def main():
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--kspace", type=int, default=30, help="GMRES restart length")
     ap.add_argument("--ortho", type=str, default="cgs1", choices=["mgs", "cgs2", "cgs1"])
-    ap.add_argument("--probe-every", type=int, default=4,
+    ap.add_argument("--probe-every", type=int, default=32,
                     help="sample every n-th Arnoldi product launch with HIP events")
     ap.add_argument("--pmc-sample", action="store_true",
                     help="short run for hardware-counter passes: a 256 MiB copy "

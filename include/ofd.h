@@ -169,8 +169,9 @@ int ofd_time_copy(ofd_ctx *ctx, int64_t nbytes, int reps, double *ms);
 
 /* sampled timing of the Arnoldi product kernel (the preconditioned SpMV of
  * each GMRES iteration) inside ofd_solve / ofd_gmres_up: every `every`-th
- * iteration's launch is bracketed by HIP timing events on the context
- * stream (0 switches sampling off).  ofd_probe_read returns the number of
+ * such launch (counted across cycles and solves) records the kernel's own
+ * start and end in HIP timing events on the context stream (0 switches
+ * sampling off).  ofd_probe_read returns the number of
  * sampled launches and their summed duration since the last read, and
  * resets both.  Measurement only; no reference counterpart. */
 int ofd_probe_set(ofd_ctx *ctx, int every);

@@ -210,7 +210,8 @@ static int gmres_cycle(Ctx &c, int precision, int ortho, const double *b, double
       if (rc) return rc;
       if (stop) break;
     }
-    const bool probe = c.probe_every > 0 && k % c.probe_every == 0 &&
+    // sampled by a running launch count, so every step index k is covered
+    const bool probe = c.probe_every > 0 && (c.probe_tick++ % c.probe_every) == 0 &&
                        c.probe_used < (int)c.probe_ev.size() / 2;
     if (probe) {
       OFD_HIP(launch_spmv_arnoldi(c, precision, k, c.probe_ev[2 * c.probe_used],
@@ -1016,6 +1017,7 @@ int ofd_probe_set(ofd_ctx *x, int every) {
   }
   c.probe_every = every;
   c.probe_used = 0;
+  c.probe_tick = 0;
   return OFD_OK;
 }
 

@@ -132,6 +132,7 @@ struct Ctx {
   int probe_every = 0;
   std::vector<hipEvent_t> probe_ev;   // 2 per slot
   int probe_used = 0;                 // slots recorded since the last read-back
+  int64_t probe_tick = 0;             // Arnoldi product launches since ofd_probe_set
   int64_t probe_count = 0;
   double probe_ms = 0.0;
 };
