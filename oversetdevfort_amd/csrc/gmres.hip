@@ -369,6 +369,28 @@ k_norm_fin(const double *__restrict__ part, int nb, GmresState *__restrict__ g, 
   }
 }
 
+// |u|^2 and |p|^2 partials of a [N][4] vector (velocity and pressure parts)
+__global__ void __launch_bounds__(256)
+k_norm2_up(int N, const double *__restrict__ v, double *__restrict__ part) {
+  double acc[2] = {0.0, 0.0};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += gridDim.x * blockDim.x) {
+    double z[4];
+    ld4(v + (size_t)i * 4, z);
+    acc[0] += z[0] * z[0] + z[1] * z[1] + z[2] * z[2];
+    acc[1] += z[3] * z[3];
+  }
+  block_partials<2>(acc, 2, part, 0);
+}
+
+// scal[slot], scal[slot+1] = sqrt of the two summed partial columns
+__global__ void __launch_bounds__(256)
+k_norm_fin2(const double *__restrict__ part, int nb, GmresState *__restrict__ g, int slot) {
+  __shared__ double s_sum[2];
+  final_sums(part, nb, 2, s_sum);
+  if (threadIdx.x < 2) g->scal[slot + threadIdx.x] = sqrt(s_sum[threadIdx.x]);
+}
+
 __global__ void k_outer_reset(GmresState *__restrict__ g, double thr) {
   if (threadIdx.x == 0) {
     g->outer_thr = thr;
@@ -512,6 +534,17 @@ hipError_t launch_gmres_finish(Ctx &c, double *x) {
   else
     hipLaunchKernelGGL(k_gmres_finish<double>, dim3(nblocks_nodes(c)), dim3(kRedBlock), 0,
                        c.stream, c.nnode, c.d_W, c.d_gs, x);
+  return hipGetLastError();
+}
+
+hipError_t launch_norm2_up(Ctx &c, const double *v, int slot) {
+  const int nb = nblocks_nodes(c);
+  double *pb = part_buf(c, 0);
+  hipLaunchKernelGGL(k_norm2_up, dim3(nb), dim3(kRedBlock), 0, c.stream, c.nnode, v, pb);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_norm_fin2, dim3(1), dim3(kRedBlock), 0, c.stream, pb, nb, c.d_gs,
+                     slot);
   return hipGetLastError();
 }
 

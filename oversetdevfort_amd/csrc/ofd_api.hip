@@ -900,17 +900,13 @@ int ofd_rhs_norms(ofd_ctx *x, double *norms2) {
   int rc = need_mesh(c);
   if (rc) return rc;
   OFD_HIP(hipSetDevice(c.device));
-  std::vector<double> r((size_t)c.nnode * 4);
-  rc = d2h(c, r.data(), c.d_rhs, r.size() * sizeof(double));
-  if (rc) return rc;
-  double su = 0.0, sp = 0.0;
-  for (int i = 0; i < c.nnode; ++i) {
-    su += r[(size_t)i * 4] * r[(size_t)i * 4] + r[(size_t)i * 4 + 1] * r[(size_t)i * 4 + 1] +
-          r[(size_t)i * 4 + 2] * r[(size_t)i * 4 + 2];
-    sp += r[(size_t)i * 4 + 3] * r[(size_t)i * 4 + 3];
-  }
-  norms2[0] = std::sqrt(su);
-  norms2[1] = std::sqrt(sp);
+  // two-pass fixed-order reduction on the device into GmresState::scal[1..2]
+  OFD_HIP(launch_norm2_up(c, c.d_rhs, 1));
+  OFD_HIP(hipMemcpyAsync(c.h_gs, c.d_gs, sizeof(GmresState), hipMemcpyDeviceToHost,
+                         c.stream));
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  norms2[0] = c.h_gs->scal[1];
+  norms2[1] = c.h_gs->scal[2];
   return OFD_OK;
 }
 

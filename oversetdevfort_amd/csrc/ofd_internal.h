@@ -168,6 +168,7 @@ hipError_t launch_gmres_close(Ctx &c, int k, int ortho);
 hipError_t launch_gmres_finish(Ctx &c, double *x);
 hipError_t launch_norm2(Ctx &c, const double *v, int slot, int outer = 0);
 hipError_t launch_outer_reset(Ctx &c, double thr);
+hipError_t launch_norm2_up(Ctx &c, const double *v, int slot);
 
 int nblocks_nodes(const Ctx &c);
 

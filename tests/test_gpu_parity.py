@@ -257,4 +257,13 @@ def test_newton_time_step_matches_oracle(ofd, oracle_mod):
     for name in ("ug", "acg", "pg"):
         g = ctx.get_field(name.upper())
         assert _rel(g, st[name]) < 1e-10, (name, _rel(g, st[name]))
+    # residual norms of the Newton loop (calculate_residual_overset,
+    # solveOverset.f90:131-173) against the oracle's re-assembled residual
+    R = O.assemble_ns(m, prob.ibc, _alpha(st, p), prm, col, row, 2, 1)
+    ctx.assembleNavSto(2, 1)
+    nu, npr = ctx.rhs_norms()
+    ru, rp = ctx.get_rhs()
+    assert abs(nu - np.linalg.norm(ru)) <= 1e-13 * np.linalg.norm(ru)
+    assert abs(npr - np.linalg.norm(rp)) <= 1e-13 * np.linalg.norm(rp)
+    assert abs(nu - np.linalg.norm(R["rhsu"])) <= 1e-8 * np.linalg.norm(R["rhsu"]) + 1e-14
     ctx.close()
