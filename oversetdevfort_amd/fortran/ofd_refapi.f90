@@ -112,7 +112,7 @@ subroutine solveNS_resident(rtol, kspace, iters, relres)
   real(c_double) :: rel, itol
   itol = 1.0d-2
   if (ofd_precision == OFD_FP32) itol = 1.0d-4
-  call ofd_check(ofd_solve(ofd_ctx, ofd_precision, OFD_ORTHO_CGS2, rtol, itol, &
+  call ofd_check(ofd_solve(ofd_ctx, ofd_precision, OFD_ORTHO_CGS1, rtol, itol, &
                            int(kspace, c_int), 100_c_int, it, cyc, rel), &
                  'solveNS_resident')
   iters = it

@@ -53,7 +53,7 @@ def _python_path(n, nsteps, nnewton):
         ctx.apply_dirichlet()
         for _ in range(nnewton):
             ctx.assembleNavSto(6, 1)
-            ctx.solve(rtol=1e-10, precision=L.OFD_FP64, ortho=L.OFD_ORTHO_CGS2,
+            ctx.solve(rtol=1e-10, precision=L.OFD_FP64, ortho=L.OFD_ORTHO_CGS1,
                       inner_tol=1e-2, kspace=30, max_cycles=100)
             ctx.newton_update()
             ctx.assembleNavSto(2, 1)
