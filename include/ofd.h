@@ -156,6 +156,28 @@ int ofd_advance(ofd_ctx *ctx);
  * solveOverset.f90:131-173, one subdomain) */
 int ofd_rhs_norms(ofd_ctx *ctx, double *norms2);
 
+/* interpolation_tet (interpolation.f90:5-76) on the device.  Receivers are
+ * the points of a boundary mesh, xr and dr (nrecv,3) column-major (its xg
+ * and dg).  Donors are the context's elements with ELM_ID == src_id in
+ * element index order, in the current configuration xg + dg; the first
+ * element that contains a receiver wins and ug is interpolated there.
+ * Linear tets: the reference's barycentric test (all four >= 0, no
+ * tolerance) with its receiver offset xr - x_1 + dr.  Trilinear hexes (no
+ * reference counterpart; the reference skips them): local coordinates from
+ * 6 Newton steps with the linhexshl shape functions, all in
+ * [-1-1e-10, 1+1e-10], among elements whose bounding box, widened by 1e-8
+ * of its largest extent, holds xr + dr.  donor[j] = 1-based element index
+ * (0: no element holds the receiver); val (nrecv,3) column-major, 0 where
+ * there is no donor. */
+int ofd_fringe_interp(ofd_ctx *ctx, int32_t src_id, int nrecv, const double *xr,
+                      const double *dr, int32_t *donor, double *val);
+
+/* setBCs_overset step 3 (setBCs.f90:158-165): for j < nrecv, node
+ * k = nodes[j] (1-based, distinct) gets IBC(k,1:3) = 1, ug(k,:) = val(j,:)
+ * and acg(k,:) = ((val(j,:) - ugold(k,:))/Delt + (gami-1) acgold(k,:))/gami.
+ * val is (nrecv,3) column-major. */
+int ofd_apply_fringe(ofd_ctx *ctx, int nrecv, const int32_t *nodes, const double *val);
+
 /* kernel timing on the context stream (HIP events around `reps`
  * back-to-back launches); ms = average duration of one launch */
 int ofd_time_spmv(ofd_ctx *ctx, int precision, int reps, double *ms);

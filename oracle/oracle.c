@@ -820,3 +820,102 @@ void orc_interp_tet(int nnode, int nelem, int nshl, const int *ien,
     }
   }
 }
+
+/* ---------------------------------------------------------------------
+ * Hex donor search and trilinear interpolation: the extension of
+ * interpolation_tet (interpolation.f90:5-76) to trilinear hexes, which the
+ * reference skips (interpolation.f90:28, nshl > 4 -> cycle).
+ *
+ * Rule: in element index order, the candidates are the elements with
+ * elm_id == src_id whose bounding box in the current configuration x + d,
+ * widened by 1e-8 of its largest extent, holds the point p = xr + dr.  The
+ * donor is the first candidate whose local coordinates, from
+ * ORC_HEX_NEWTON Newton steps on x(xi) = p from xi = 0 with the linhexshl
+ * shape functions (eval_shape_FEM.f90:148-196) and get_inverse_3x3
+ * (eval_shape.f90:85-112), all lie in [-1 - ORC_HEX_TOL, 1 + ORC_HEX_TOL].
+ * The value is sum_a N_a(xi) ug_a, a = 1..8 in order.
+ * --------------------------------------------------------------------- */
+#define ORC_HEX_NEWTON 6
+#define ORC_HEX_TOL 1e-10
+#define ORC_BOX_MARGIN 1e-8
+
+void orc_hex_bbox(const double X[8][3], double lo[3], double hi[3]) {
+  double ext = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    lo[k] = X[0][k];
+    hi[k] = X[0][k];
+    for (int a = 1; a < 8; ++a) {
+      lo[k] = X[a][k] < lo[k] ? X[a][k] : lo[k];
+      hi[k] = X[a][k] > hi[k] ? X[a][k] : hi[k];
+    }
+    const double w = hi[k] - lo[k];
+    ext = w > ext ? w : ext;
+  }
+  const double m = ORC_BOX_MARGIN * ext;
+  for (int k = 0; k < 3; ++k) {
+    lo[k] = lo[k] - m;
+    hi[k] = hi[k] + m;
+  }
+}
+
+/* local coordinates of p in the hex with corners X (linhexshl order);
+ * returns 1 when they lie in the widened reference cube */
+int orc_hex_local(const double X[8][3], const double p[3], double xi[3]) {
+  double N[8], dN[24], J[9], Ji[9], det, r[3];
+  xi[0] = xi[1] = xi[2] = 0.0;
+  for (int it = 0; it < ORC_HEX_NEWTON; ++it) {
+    orc_linhexshl(xi, N, dN);
+    for (int k = 0; k < 3; ++k) {
+      double x = 0.0;
+      for (int a = 0; a < 8; ++a) x = x + N[a] * X[a][k];
+      r[k] = x - p[k];
+      for (int d = 0; d < 3; ++d) {
+        double j = 0.0;
+        for (int a = 0; a < 8; ++a) j = j + dN[a + 8 * d] * X[a][k];
+        M3(J, k, d) = j;
+      }
+    }
+    orc_inverse3x3(J, Ji, &det);
+    for (int d = 0; d < 3; ++d)
+      xi[d] = xi[d] - (M3(Ji, d, 0) * r[0] + M3(Ji, d, 1) * r[1] + M3(Ji, d, 2) * r[2]);
+  }
+  const double lim = 1.0 + ORC_HEX_TOL;
+  return fabs(xi[0]) <= lim && fabs(xi[1]) <= lim && fabs(xi[2]) <= lim;
+}
+
+void orc_interp_hex(int nnode, int nelem, const int *ien, const int *elm_id,
+                    int src_id, const double *xg, const double *dg,
+                    const double *ug, int nrecv, const double *xr,
+                    const double *dr, int *donor, double *val) {
+  for (int j = 0; j < nrecv; ++j) {
+    donor[j] = -1;
+    val[j] = val[nrecv + j] = val[2 * nrecv + j] = 0.0;
+  }
+  for (int e = 0; e < nelem; ++e) {
+    if (elm_id[e] != src_id) continue;
+    double X[8][3], lo[3], hi[3];
+    for (int a = 0; a < 8; ++a)
+      for (int k = 0; k < 3; ++k) {
+        const int n = ien[a * nelem + e];
+        X[a][k] = xg[k * nnode + n] + dg[k * nnode + n];
+      }
+    orc_hex_bbox(X, lo, hi);
+    for (int j = 0; j < nrecv; ++j) {
+      if (donor[j] >= 0) continue;
+      double p[3], xi[3];
+      for (int k = 0; k < 3; ++k) p[k] = xr[k * nrecv + j] + dr[k * nrecv + j];
+      if (!(p[0] >= lo[0] && p[0] <= hi[0] && p[1] >= lo[1] && p[1] <= hi[1] &&
+            p[2] >= lo[2] && p[2] <= hi[2]))
+        continue;
+      if (!orc_hex_local(X, p, xi)) continue;
+      double N[8], dN[24];
+      orc_linhexshl(xi, N, dN);
+      for (int k = 0; k < 3; ++k) {
+        double s = 0.0;
+        for (int a = 0; a < 8; ++a) s = s + N[a] * ug[k * nnode + ien[a * nelem + e]];
+        val[k * nrecv + j] = s;
+      }
+      donor[j] = e;
+    }
+  }
+}

@@ -109,6 +109,18 @@ void orc_interp_tet(int nnode, int nelem, int nshl, const int *ien,
                     const double *xr, const double *dr, int *donor,
                     double *val);
 
+/*
+ * Extension to trilinear hexes (the reference searches tets only): donor =
+ * first element (index order) with elm_id == src_id whose widened bounding
+ * box holds p = xr + dr and whose 6-step Newton local coordinates lie in
+ * [-1-1e-10, 1+1e-10]^3; val = trilinear interpolation of ug.
+ */
+void orc_interp_hex(int nnode, int nelem, const int *ien, const int *elm_id,
+                    int src_id, const double *xg, const double *dg,
+                    const double *ug, int nrecv, const double *xr,
+                    const double *dr, int *donor, double *val);
+int orc_hex_local(const double X[8][3], const double p[3], double xi[3]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -28,6 +28,10 @@ gmres_up               reference SparseGMRES_up; Arnoldi invariants and a
                        direct solve (tests)
 interp_tet             reference interpolation_tet values + donors parsed
                        from its own "detected element" print; brute force
+interp_hex             (extension, no reference code) closed-form local
+                       coordinates on axis-aligned and affine (rotated)
+                       grids; brute-force first-index donor over all
+                       elements; exact interpolation of linear fields
 =====================  =====================================================
 """
 from __future__ import annotations
@@ -81,6 +85,10 @@ def lib():
         L.orc_gmres_up.restype = ctypes.c_int
         L.orc_interp_tet.argtypes = ([ctypes.c_int] * 3 + [_ip, _ip, ctypes.c_int] +
                                      [_dp] * 3 + [ctypes.c_int, _dp, _dp, _ip, _dp])
+        L.orc_interp_hex.argtypes = ([ctypes.c_int] * 2 + [_ip, _ip, ctypes.c_int] +
+                                     [_dp] * 3 + [ctypes.c_int, _dp, _dp, _ip, _dp])
+        L.orc_hex_local.argtypes = [_dp, _dp, _dp]
+        L.orc_hex_local.restype = ctypes.c_int
         _LIB = L
     return _LIB
 
@@ -212,6 +220,26 @@ def gmres_up(col, row, A, rhsu, rhsp, tol, kspace, kspace_min=0):
                              int(kspace_min), res, r0)
     return dict(solu=solu.reshape(3, N).T.copy(), solp=solp, iters=int(its),
                 resid=float(res[0]), rnorm0=float(r0[0]))
+
+
+def interp_hex(mesh, src_id, dg, ug, xr, dr):
+    """Hex donor search + trilinear interpolation (extension of
+    interpolation_tet to hexes)."""
+    N, E = mesh.nnode, mesh.nelem
+    nr = xr.shape[0]
+    donor = np.zeros(nr, np.int32)
+    val = np.zeros(3 * nr)
+    lib().orc_interp_hex(N, E, _c(mesh.ien.T, np.int32).ravel(),
+                         _c(mesh.elm_id, np.int32), int(src_id), _soa(mesh.xg),
+                         _soa(dg), _soa(ug), nr, _soa(xr), _soa(dr), donor, val)
+    return donor, val.reshape(3, nr).T.copy()
+
+
+def hex_local(X, p):
+    """(inside, xi) of point p in the hex with corners X (8,3)."""
+    xi = np.zeros(3)
+    ok = lib().orc_hex_local(_c(X, np.float64).ravel(), _c(p, np.float64), xi)
+    return bool(ok), xi
 
 
 def interp_tet(mesh, src_id, dg, ug, xr, dr):

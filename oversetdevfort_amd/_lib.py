@@ -66,6 +66,8 @@ SIGNATURES = {
     "ofd_predictor": ([_vp], _c_int),
     "ofd_advance": ([_vp], _c_int),
     "ofd_rhs_norms": ([_vp, _dp], _c_int),
+    "ofd_fringe_interp": ([_vp, ctypes.c_int32, _c_int, _vp, _vp, _vp, _vp], _c_int),
+    "ofd_apply_fringe": ([_vp, _c_int, _vp, _vp], _c_int),
     "ofd_time_spmv": ([_vp, _c_int, _c_int, _dp], _c_int),
     "ofd_time_assemble": ([_vp, _c_int, _c_int, _dp], _c_int),
     "ofd_time_copy": ([_vp, ctypes.c_int64, _c_int, _dp], _c_int),

@@ -346,6 +346,14 @@ int ofd_ctx_destroy(ofd_ctx *x) {
   dfree(c.d_W32);
   dfree(c.d_part);
   dfree(c.d_gs);
+  dfree(c.d_fr_box);
+  dfree(c.d_fr_ok);
+  dfree(c.d_fr_geo);
+  dfree(c.d_fr_xr);
+  dfree(c.d_fr_dr);
+  dfree(c.d_fr_val);
+  dfree(c.d_fr_donor);
+  dfree(c.d_fr_nodes);
   if (c.h_gs) (void)hipHostFree(c.h_gs);
   if (c.h_flags) (void)hipHostFree(c.h_flags);
   if (c.h_gs_ring) (void)hipHostFree(c.h_gs_ring);
@@ -364,8 +372,8 @@ int ofd_upload_mesh(ofd_ctx *x, int nnode, int nelem, int nshl, const int32_t *i
   OFD_CHECK(nnode > 0 && nelem > 0, OFD_ERR_ARG, "empty mesh");
   OFD_CHECK(nnode <= kMaxRedBlocks * kRedBlock, OFD_ERR_UNSUPPORTED,
             "more than 1,048,576 nodes per context");
-  OFD_CHECK(nshl == 8, OFD_ERR_UNSUPPORTED,
-            "only trilinear hex elements (NSHL=8) are implemented on the GPU");
+  OFD_CHECK(nshl == 8 || nshl == 4, OFD_ERR_UNSUPPORTED,
+            "elements must be trilinear hexes (NSHL=8) or linear tets (NSHL=4)");
   Ctx &c = x->c;
   OFD_HIP(hipSetDevice(c.device));
   const size_t ne = (size_t)nelem * nshl;
@@ -605,6 +613,8 @@ int ofd_assemble_ns(ofd_ctx *x, int tensor_flag, int subdomain_flag, double *cfl
   Ctx &c = x->c;
   int rc = need_pattern(c);
   if (rc) return rc;
+  OFD_CHECK(c.nshl == 8, OFD_ERR_UNSUPPORTED,
+            "the GPU element assembly is implemented for trilinear hexes (NSHL=8)");
   OFD_CHECK((tensor_flag & ~(OFD_ASSEMBLE_VEC | OFD_ASSEMBLE_MAT)) == 0, OFD_ERR_ARG,
             "tensor_flag must combine OFD_ASSEMBLE_VEC / OFD_ASSEMBLE_MAT");
   OFD_HIP(hipSetDevice(c.device));
@@ -942,6 +952,8 @@ int ofd_time_assemble(ofd_ctx *x, int tensor_flag, int reps, double *ms) {
   Ctx &c = x->c;
   int rc = need_pattern(c);
   if (rc) return rc;
+  OFD_CHECK(c.nshl == 8, OFD_ERR_UNSUPPORTED,
+            "the GPU element assembly is implemented for trilinear hexes (NSHL=8)");
   OFD_HIP(hipSetDevice(c.device));
   hipEvent_t e0, e1;
   OFD_HIP(hipEventCreate(&e0));
@@ -965,6 +977,81 @@ int ofd_time_assemble(ofd_ctx *x, int tensor_flag, int reps, double *ms) {
     c.prec_valid = c.a32_valid = c.ap32_valid = false;
   }
   *ms = (double)t / reps;
+  return OFD_OK;
+}
+
+static int ensure_fringe(Ctx &c, int nrecv) {
+  int rc = OFD_OK;
+  if (c.fr_alloc_e < c.nelem) {
+    rc = dalloc(c, &c.d_fr_box, (size_t)c.nelem * 6);
+    if (!rc) rc = dalloc(c, &c.d_fr_ok, (size_t)c.nelem);
+    if (!rc) rc = dalloc(c, &c.d_fr_geo, (size_t)c.nelem * 24);
+    if (rc) return rc;
+    c.fr_alloc_e = c.nelem;
+  }
+  if (c.fr_alloc_r < nrecv) {
+    rc = dalloc(c, &c.d_fr_xr, (size_t)nrecv * 3);
+    if (!rc) rc = dalloc(c, &c.d_fr_dr, (size_t)nrecv * 3);
+    if (!rc) rc = dalloc(c, &c.d_fr_val, (size_t)nrecv * 3);
+    if (!rc) rc = dalloc(c, &c.d_fr_donor, (size_t)nrecv);
+    if (!rc) rc = dalloc(c, &c.d_fr_nodes, (size_t)nrecv);
+    if (rc) return rc;
+    c.fr_alloc_r = nrecv;
+  }
+  return OFD_OK;
+}
+
+int ofd_fringe_interp(ofd_ctx *x, int32_t src_id, int nrecv, const double *xr,
+                      const double *dr, int32_t *donor, double *val) {
+  OFD_CHECK(x, OFD_ERR_ARG, "null context");
+  OFD_CHECK(nrecv >= 0, OFD_ERR_ARG, "nrecv < 0");
+  Ctx &c = x->c;
+  int rc = need_mesh(c);
+  if (rc) return rc;
+  if (nrecv == 0) return OFD_OK;
+  OFD_CHECK(xr && dr && donor && val, OFD_ERR_ARG, "null argument");
+  OFD_HIP(hipSetDevice(c.device));
+  rc = ensure_fringe(c, nrecv);
+  if (rc) return rc;
+  const size_t vb = (size_t)nrecv * 3 * sizeof(double);
+  rc = h2d(c, c.d_fr_xr, xr, vb);
+  if (!rc) rc = h2d(c, c.d_fr_dr, dr, vb);
+  if (rc) return rc;
+  OFD_HIP(launch_fringe(c, src_id, nrecv, c.d_fr_xr, c.d_fr_dr, c.d_fr_donor, c.d_fr_val));
+  rc = d2h(c, donor, c.d_fr_donor, (size_t)nrecv * sizeof(int32_t));
+  if (!rc) rc = d2h(c, val, c.d_fr_val, vb);
+  if (rc) return rc;
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  for (int j = 0; j < nrecv; ++j) donor[j] += 1;  // 1-based, 0 = no donor
+  return OFD_OK;
+}
+
+int ofd_apply_fringe(ofd_ctx *x, int nrecv, const int32_t *nodes, const double *val) {
+  OFD_CHECK(x, OFD_ERR_ARG, "null context");
+  OFD_CHECK(nrecv >= 0, OFD_ERR_ARG, "nrecv < 0");
+  Ctx &c = x->c;
+  int rc = need_mesh(c);
+  if (rc) return rc;
+  if (nrecv == 0) return OFD_OK;
+  OFD_CHECK(nodes && val, OFD_ERR_ARG, "null argument");
+  // distinct nodes in range: one thread writes each node
+  std::vector<int32_t> nd(nodes, nodes + nrecv);
+  std::vector<uint8_t> seen((size_t)c.nnode, 0);
+  for (int j = 0; j < nrecv; ++j) {
+    const int k = nd[j] - 1;
+    OFD_CHECK(k >= 0 && k < c.nnode, OFD_ERR_ARG, "fringe node out of range [1, NNODE]");
+    OFD_CHECK(!seen[k], OFD_ERR_ARG, "fringe nodes must be distinct");
+    seen[k] = 1;
+    nd[j] = k;
+  }
+  OFD_HIP(hipSetDevice(c.device));
+  rc = ensure_fringe(c, nrecv);
+  if (rc) return rc;
+  rc = h2d(c, c.d_fr_nodes, nd.data(), (size_t)nrecv * sizeof(int32_t));
+  if (!rc) rc = h2d(c, c.d_fr_val, val, (size_t)nrecv * 3 * sizeof(double));
+  if (rc) return rc;
+  OFD_HIP(launch_fringe_apply(c, nrecv, c.d_fr_nodes, c.d_fr_val));
+  OFD_HIP(hipStreamSynchronize(c.stream));
   return OFD_OK;
 }
 

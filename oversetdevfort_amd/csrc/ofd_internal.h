@@ -126,6 +126,13 @@ struct Ctx {
   size_t h_pin_bytes = 0;
   int64_t dev_bytes = 0;
   std::vector<double> solve_hist;     // relres after each cycle of the last solve
+  // overset fringe: per-element boxes, source flags and geometry; receivers
+  int fr_alloc_e = 0, fr_alloc_r = 0;
+  double *d_fr_box = nullptr;         // [E][6] widened bounding box
+  int8_t *d_fr_ok = nullptr;          // [E] element is a source element
+  double *d_fr_geo = nullptr;         // [E][24] tet inverse edge matrix / hex corners
+  double *d_fr_xr = nullptr, *d_fr_dr = nullptr, *d_fr_val = nullptr;  // (nrecv,3)
+  int32_t *d_fr_donor = nullptr, *d_fr_nodes = nullptr;                // (nrecv)
   // sampled timing of the Arnoldi product kernel inside real solves: every
   // probe_every-th iteration is bracketed by a pair of timing events, which
   // are read back at the next host synchronisation of the solve
@@ -161,6 +168,10 @@ hipError_t launch_node4_to_soa(Ctx &c, const double *in, double *u, double *p);
 hipError_t launch_axpy_fields(Ctx &c, int which);
 hipError_t launch_apply_dirichlet(Ctx &c);
 hipError_t launch_copy(Ctx &c, const void *src, void *dst, size_t nbytes);
+// overset fringe
+hipError_t launch_fringe(Ctx &c, int32_t src_id, int nrecv, const double *xr,
+                         const double *dr, int32_t *donor, double *val);
+hipError_t launch_fringe_apply(Ctx &c, int nrecv, const int32_t *nodes, const double *val);
 // gmres pieces
 hipError_t launch_gmres_init(Ctx &c, const double *b, double tol, int kspace, int kmin);
 hipError_t launch_mgs_step(Ctx &c, int k, int j);

@@ -206,6 +206,28 @@ module ofd_iface
       integer(c_int) :: rc
     end function
 
+    function ofd_fringe_interp(ctx, src_id, nrecv, xr, dr, donor, val) &
+        bind(C, name='ofd_fringe_interp') result(rc)
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: src_id
+      integer(c_int), value :: nrecv
+      real(c_double), intent(in) :: xr(*), dr(*)
+      integer(c_int32_t), intent(out) :: donor(*)
+      real(c_double), intent(out) :: val(*)
+      integer(c_int) :: rc
+    end function
+
+    function ofd_apply_fringe(ctx, nrecv, nodes, val) bind(C, name='ofd_apply_fringe') &
+        result(rc)
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nrecv
+      integer(c_int32_t), intent(in) :: nodes(*)
+      real(c_double), intent(in) :: val(*)
+      integer(c_int) :: rc
+    end function
+
     function ofd_rhs_norms(ctx, norms2) bind(C, name='ofd_rhs_norms') result(rc)
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx

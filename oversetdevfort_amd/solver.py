@@ -208,6 +208,26 @@ class Context:
         L.check(self.lib.ofd_rhs_norms(self.h, out))
         return float(out[0]), float(out[1])
 
+    # ---- overset fringe (interpolation.f90:5-76, setBCs.f90:124-167) ----
+    def interpolation(self, src_id, xr, dr=None):
+        """Donor search and interpolation of ug at receivers xr + dr
+        ((nrecv,3) each).  Returns (donor, val): donor 1-based element
+        index (0: none), val (nrecv,3)."""
+        xr_ = _soa(xr)
+        nr = xr_.shape[1]
+        dr_ = _soa(np.zeros((nr, 3)) if dr is None else dr)
+        donor = np.zeros(nr, np.int32)
+        val = np.zeros((3, nr))
+        L.check(self.lib.ofd_fringe_interp(self.h, int(src_id), nr, _ptr(xr_), _ptr(dr_),
+                                           _ptr(donor), _ptr(val)))
+        return donor, val.T.copy()
+
+    def apply_fringe(self, nodes, val):
+        """setBCs_overset step 3 for 1-based ``nodes`` with values (n,3)."""
+        nodes_ = np.ascontiguousarray(nodes, dtype=np.int32)
+        val_ = _soa(val)
+        L.check(self.lib.ofd_apply_fringe(self.h, int(nodes_.size), _ptr(nodes_), _ptr(val_)))
+
     # ---- measurement ----
     def time_spmv(self, precision=L.OFD_FP64, reps=20):
         ms = ctypes.c_double()
