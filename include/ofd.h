@@ -53,17 +53,34 @@ enum {
   OFD_ORTHO_CGS1 = 2  /* classical GS, one projection (restart cycles are short) */
 };
 
-/* node fields held by a context, all (NNODE,3) except OFD_PG/OFD_PGOLD (NNODE) */
+/* node fields held by a context: (NNODE,3) vectors from OFD_XG to OFD_ACGMOLD
+ * and OFD_SOLU; (NNODE) scalars otherwise.  The quenching fields (VOF phig,
+ * its rate rphig, temperature Tg, its rate rTg and the T increment SOLT)
+ * follow modules.f90:291-300. */
 enum {
   OFD_XG = 0, OFD_DG, OFD_DGOLD, OFD_UG, OFD_UGOLD, OFD_UGM, OFD_UGMOLD,
   OFD_ACG, OFD_ACGOLD, OFD_ACGM, OFD_ACGMOLD, OFD_PG, OFD_PGOLD,
-  OFD_SOLU, OFD_SOLP, OFD_NFIELDS
+  OFD_SOLU, OFD_SOLP,
+  OFD_PHIG, OFD_PHIGOLD, OFD_RPHIG, OFD_RPHIGOLD, OFD_TG, OFD_TGOLD,
+  OFD_RTG, OFD_RTGOLD, OFD_SOLT, OFD_NFIELDS
 };
 
 /* parameter vector of ofd_set_params */
 enum {
   OFD_P_ALMI = 0, OFD_P_ALFI, OFD_P_GAMI, OFD_P_BETI, OFD_P_DELT, OFD_P_RHO,
   OFD_P_MU, OFD_P_KDC, OFD_P_GX, OFD_P_GY, OFD_P_GZ, OFD_NPARAMS
+};
+
+/* material vector of ofd_set_quench_params (param.f90:222-242, 405-406):
+ * air/vapour (a), water (w) and solid (s) properties, saturation
+ * temperature, latent heat, phase-change rate coefficients, the T
+ * discontinuity-capturing coefficient and reference temperature, and the
+ * ELM_ID of fluid elements (102 in the reference, solveQuenching.f90:693) */
+enum {
+  OFD_Q_RHOA = 0, OFD_Q_RHOW, OFD_Q_MUA, OFD_Q_MUW, OFD_Q_CPA, OFD_Q_CPW,
+  OFD_Q_KAPPAA, OFD_Q_KAPPAW, OFD_Q_RHOS, OFD_Q_MUS, OFD_Q_CPS, OFD_Q_KAPPAS,
+  OFD_Q_TS, OFD_Q_LH, OFD_Q_CCOND, OFD_Q_CEVAP, OFD_Q_CDMDOT, OFD_Q_TEMKDC,
+  OFD_Q_TREF, OFD_Q_FLUID_ID, OFD_NQPARAMS
 };
 
 const char *ofd_last_error(void);
@@ -88,7 +105,8 @@ int ofd_get_pattern(ofd_ctx *ctx, int32_t *col, int32_t *row);
 int ofd_set_params(ofd_ctx *ctx, const double *prm);
 /* ELM_ID_SET(1:2) (param.f90:246-247) */
 int ofd_set_subdomains(ofd_ctx *ctx, int32_t id1, int32_t id2);
-/* Dirichlet flags IBC(NNODE,ncol), columns 1..4 = u,v,w,p (BCLhs_3D.f90) */
+/* Dirichlet flags IBC(NNODE,ncol), 4 <= ncol <= 8: columns 1..4 = u,v,w,p
+ * (BCLhs_3D.f90:4-122), column 6 = T (BCLhs_tem, BCLhs_3D.f90:247-274) */
 int ofd_set_ibc(ofd_ctx *ctx, const int32_t *ibc, int ncol);
 
 /* Dirichlet velocity values BCugValu per node, (NNODE,3); used by
@@ -149,12 +167,52 @@ int ofd_solve_history(ofd_ctx *ctx, double *relres, int maxn, int *n);
 /* Newton update (solveOverset.f90:76-78): acg += du; ug += gami*Delt*du;
  * pg += alfi*gami*Delt*dp with du, dp the context solution */
 int ofd_newton_update(ofd_ctx *ctx);
-/* predictor (driver.f90:83-85) and old <- new (driver.f90:98-103) */
+/* predictor (driver.f90:83-85) and old <- new (driver.f90:98-103); once
+ * quench parameters are set they also predict (Tg = Tgold, rTg =
+ * (gami-1)/gami rTgold, likewise phig, rphig) and advance the quench fields */
 int ofd_predictor(ofd_ctx *ctx);
 int ofd_advance(ofd_ctx *ctx);
 /* |RHSGu|_2, |RHSGp|_2 over all nodes (calculate_residual_overset,
  * solveOverset.f90:131-173, one subdomain) */
 int ofd_rhs_norms(ofd_ctx *ctx, double *norms2);
+
+/* ---- temperature equation of the quenching solver ---- */
+
+/* material data, q[OFD_NQPARAMS] in the order of the OFD_Q_* enum */
+int ofd_set_quench_params(ofd_ctx *ctx, const double *q);
+
+/* assembleQuenching(tensor_flag, ASSEMBLE_FIELD_TEM) (solveQuenching.f90:4-
+ * 117) -> IntElmAss_Tem_Quenching_STAB (:579-882) with BCLhs_tem (IBC
+ * column 6, BCLhs_3D.f90:247-274) and FillSparseMat_tem (FillSparseMat.f90:
+ * 175-201), from the alpha stage of the context's fields (:25-37).  Hex
+ * elements; the face term FaceAssembly_ConvectiveH is not part of it (the
+ * reference defines it nowhere). */
+int ofd_assemble_tem(ofd_ctx *ctx, int tensor_flag);
+
+/* LHSTem(icnt) and RHSGTem(NNODE) in the reference layout */
+int ofd_get_tem_system(ofd_ctx *ctx, double *lhs, double *rhs);
+int ofd_set_tem_system(ofd_ctx *ctx, const double *lhs, const double *rhs);
+
+/* SparseProd_LS (SparseProd.f90:128-161): y = LHSTem x */
+int ofd_spmv_tem(ofd_ctx *ctx, int precision, const double *x, double *y);
+
+/* SparseGMRES_tem (SparseGMRES_tem.f90:1-256): one cycle with the diagonal
+ * preconditioner (zero diagonal -> 1) on LHSTem sol = RHSGTem.  sol (may be
+ * NULL) receives the increment, 0 when |RHSGTem| = 0; it also stays in the
+ * context as OFD_SOLT. */
+int ofd_gmres_tem(ofd_ctx *ctx, int precision, int ortho, double tol, int kspace,
+                  int kspace_min, double *sol, int *iters, double *resid);
+
+/* restarted SparseGMRES_tem with an FP64 outer residual, as ofd_solve */
+int ofd_solve_tem(ofd_ctx *ctx, int precision, int ortho, double rtol, double inner_tol,
+                  int kspace, int max_cycles, int *iters, int *cycles, double *relres);
+
+/* temperature Newton update (solveFlow.f90:139-140): rTg += dT,
+ * Tg += gami*Delt*dT with dT = OFD_SOLT */
+int ofd_tem_update(ofd_ctx *ctx);
+
+/* |RHSGTem|_2 */
+int ofd_tem_rhs_norm(ofd_ctx *ctx, double *norm2);
 
 /* interpolation_tet (interpolation.f90:5-76) on the device.  Receivers are
  * the points of a boundary mesh, xr and dr (nrecv,3) column-major (its xg

@@ -919,3 +919,312 @@ void orc_interp_hex(int nnode, int nelem, const int *ien, const int *elm_id,
     }
   }
 }
+
+/* ---------------------------------------------------------------------
+ * Temperature equation of the quenching solver.
+ * qprm = {rhoa, rhow, mua, muw, cpa, cpw, kappaa, kappaw, rhos, mus, cps,
+ *         kappas, Ts, lh, c_cond, c_evap, c_dmdot, tem_kdc, tref, fluid_id}
+ * --------------------------------------------------------------------- */
+
+/* prop_interp (e3int.f90:4-15) */
+static double orc_prop(double a, double b, double phi) {
+  const double pc = fmax(0.0, fmin(1.0, phi));
+  return a * (1.0 - pc) + b * pc;
+}
+
+/* e3STAB_3D_TEM (e3STAB.f90:338-367) */
+static double orc_tau_tem(const double *G, const double *uadv, double dt, double rhoi,
+                          double cpi, double hki) {
+  double t = 0.0;
+  for (int j = 0; j < 3; ++j)
+    for (int i = 0; i < 3; ++i) t = t + uadv[i] * M3(G, i, j) * uadv[j];
+  t = t + 4.0 / (dt * dt);
+  const double k = hki / (rhoi * cpi);
+  double gg = 0.0;
+  for (int t2 = 0; t2 < 9; ++t2) gg = gg + G[t2] * G[t2];
+  t = t + k * k * 12.0 * gg;
+  return 1.0 / (rhoi * cpi * sqrt(t));
+}
+
+/* IntElmAss_Tem_Quenching_STAB element body (solveQuenching.f90:706-848);
+ * te(NSHL,NSHL) column-major te[b*nshl+a], rt(NSHL) accumulate */
+int orc_element_tem(int nshl, int ngauss, const double *xl, const double *dl,
+                    const double *ul, const double *uml, const double *phil,
+                    const double *Tl, const double *rTl, const double *prm,
+                    const double *qp, int is_fluid, int tensor_flag, double *te,
+                    double *rt) {
+  const double almi = prm[0], alfi = prm[1], gami = prm[2], delt = prm[4];
+  const double rhoa = qp[0], rhow = qp[1], cpa = qp[4], cpw = qp[5];
+  const double kappaa = qp[6], kappaw = qp[7], rhos = qp[8], cps = qp[10];
+  const double kappas = qp[11], Ts = qp[12], lh = qp[13], c_cond = qp[14];
+  const double c_evap = qp[15], c_dmdot = qp[16], tem_kdc = qp[17], tref = qp[18];
+  const double fact1 = almi, fact2 = alfi * gami * delt;
+  double gp[8 * 3], gw[8];
+  int nneg = 0;
+  if (orc_gauss_volume(ngauss, gp, gw) != 0) return -1;
+  const int do_mat = (tensor_flag & 4) != 0, do_vec = (tensor_flag & 2) != 0;
+  for (int q = 0; q < ngauss; ++q) {
+    double pt[3] = {gp[0 * ngauss + q], gp[1 * ngauss + q], gp[2 * ngauss + q]};
+    double shlu[8], shg[8 * 3], dxidx[9], Gij[9], Ginv[9], detj;
+    if (orc_eval_shape_fem(nshl, pt, xl, dl, shlu, shg, dxidx, Gij, Ginv, &detj) > 0) ++nneg;
+    const double dw = detj * gw[q];
+    /* e3int_qr, e3int_qr_grad (e3int.f90:18-47) */
+    double phii = 0.0, rTi = 0.0, Ti = 0.0, ui[3], umi[3], dT[3];
+    for (int a = 0; a < nshl; ++a) phii = phii + phil[a] * shlu[a];
+    for (int a = 0; a < nshl; ++a) rTi = rTi + rTl[a] * shlu[a];
+    for (int a = 0; a < nshl; ++a) Ti = Ti + Tl[a] * shlu[a];
+    for (int i = 0; i < 3; ++i) {
+      double su = 0.0, sm = 0.0;
+      for (int a = 0; a < nshl; ++a) su = su + ul[i * nshl + a] * shlu[a];
+      for (int a = 0; a < nshl; ++a) sm = sm + uml[i * nshl + a] * shlu[a];
+      ui[i] = su;
+      umi[i] = sm;
+    }
+    for (int j = 0; j < 3; ++j) {
+      double sd = 0.0;
+      for (int a = 0; a < nshl; ++a) sd = sd + Tl[a] * shg[j * nshl + a];
+      dT[j] = sd;
+    }
+    /* phase change (solveQuenching.f90:742-752) */
+    const double phic = fmax(fmin(phii, 1.0), 0.0);
+    const double T0c = Ti;
+    double mdot, dmdT;
+    if (Ti > Ts) {
+      mdot = c_evap * (1 - phic) * rhow * (T0c - Ts) / Ts;
+      dmdT = c_evap * (1 - phic) * rhow / Ts * c_dmdot;
+    } else {
+      mdot = c_cond * (phic)*rhoa * (T0c - Ts) / Ts;
+      dmdT = c_cond * (phic)*rhoa / Ts * c_dmdot;
+    }
+    /* properties (:756-769) */
+    double rhoi, cpi, hki, rhocpi;
+    if (is_fluid) {
+      rhoi = orc_prop(rhow, rhoa, phic);
+      cpi = orc_prop(cpw, cpa, phic);
+      hki = orc_prop(kappaw, kappaa, phic);
+      rhocpi = orc_prop(rhow * cpw, rhoa * cpa, phic);
+    } else {
+      rhoi = rhos;
+      cpi = cps;
+      hki = kappas;
+      rhocpi = rhos * cps;
+    }
+    double uadv[3];
+    for (int i = 0; i < 3; ++i) uadv[i] = ui[i] - umi[i];
+    const double Se = ((cpw - cpa) * (Ti - Ts) - lh) * mdot;
+    /* the Hessian term of res_tem1 is identically zero for these elements */
+    const double udT = uadv[0] * dT[0] + uadv[1] * dT[1] + uadv[2] * dT[2];
+    const double res = rhocpi * (rTi + udT) - Se - hki * (0.0 + 0.0 + 0.0);
+    const double tau = orc_tau_tem(Gij, uadv, delt, rhoi, cpi, hki);
+    double kdc = 0.0;
+    if (fabs(tem_kdc) > 0.0 && is_fluid) {
+      double gg = 0.0;
+      for (int t2 = 0; t2 < 9; ++t2) gg = gg + Gij[t2] * Gij[t2];
+      kdc = tem_kdc * fabs(res) / sqrt(gg) / tref;
+    }
+    double sc[8];
+    for (int a = 0; a < nshl; ++a)
+      sc[a] = shg[a] * uadv[0] + shg[nshl + a] * uadv[1] + shg[2 * nshl + a] * uadv[2];
+    const double dSedT = (cpw - cpa) * mdot + ((cpw - cpa) * (Ti - Ts) - lh) * dmdT;
+    if (do_mat) {
+      for (int b = 0; b < nshl; ++b)
+        for (int a = 0; a < nshl; ++a) {
+          const double gab = shg[a] * shg[b] + shg[nshl + a] * shg[nshl + b] +
+                             shg[2 * nshl + a] * shg[2 * nshl + b];
+          double *x = te + (size_t)b * nshl + a;
+          if (is_fluid) {
+            const double lb = fact1 * shlu[b] + fact2 * sc[b];
+            *x = *x + shlu[a] * rhocpi * lb * dw - shlu[a] * fact2 * dSedT * shlu[b] * dw +
+                 tau * rhocpi * sc[a] * rhocpi * lb * dw -
+                 tau * rhocpi * sc[a] * fact2 * dSedT * shlu[b] * dw +
+                 fact2 * (hki + kdc) * gab * dw;
+          } else {
+            *x = *x + shlu[a] * rhocpi * fact1 * shlu[b] * dw + fact2 * hki * gab * dw;
+          }
+        }
+    }
+    if (do_vec) {
+      for (int a = 0; a < nshl; ++a) {
+        double r = rt[a];
+        if (is_fluid) {
+          r = r - shlu[a] * rhocpi * (rTi + udT) * dw;
+          r = r + shlu[a] * Se * dw;
+          r = r - rhocpi * tau * sc[a] * res * dw;
+          r = r - (hki + kdc) * shg[a] * dT[0] * dw;
+          r = r - (hki + kdc) * shg[nshl + a] * dT[1] * dw;
+          r = r - (hki + kdc) * shg[2 * nshl + a] * dT[2] * dw;
+        } else {
+          r = r - shlu[a] * rhocpi * rTi * dw;
+          r = r - hki * shg[a] * dT[0] * dw;
+          r = r - hki * shg[nshl + a] * dT[1] * dw;
+          r = r - hki * shg[2 * nshl + a] * dT[2] * dw;
+        }
+        rt[a] = r;
+      }
+    }
+  }
+  return nneg;
+}
+
+/* IntElmAss_Tem_Quenching_STAB over all elements, BCLhs_tem
+ * (BCLhs_3D.f90:247-274; IBC column 6 = T), FillSparseMat_tem
+ * (FillSparseMat.f90:175-201) and the RHS scatter (:841-845). */
+int orc_assemble_tem(int nnode, int nelem, int nshl, int ngauss, const int *ien,
+                     const int *elm_id, int tensor_flag, const double *xg,
+                     const int *ibc_t, const double *dgA, const double *ugA,
+                     const double *ugmA, const double *phigA, const double *TgA,
+                     const double *rTgA, const double *prm, const double *qp,
+                     const int *col, const int *row, double *lhs, double *rhs) {
+  const int fluid_id = (int)qp[19];
+  double xl[24], dl[24], ul[24], uml[24], phil[8], Tl[8], rTl[8], te[64], rt[8];
+  int nneg = 0;
+  for (int e = 0; e < nelem; ++e) {
+    for (int a = 0; a < nshl; ++a) {
+      const int n = ien[a * nelem + e];
+      for (int i = 0; i < 3; ++i) {
+        xl[i * nshl + a] = xg[i * nnode + n];
+        dl[i * nshl + a] = dgA[i * nnode + n];
+        ul[i * nshl + a] = ugA[i * nnode + n];
+        uml[i * nshl + a] = ugmA[i * nnode + n];
+      }
+      phil[a] = phigA[n];
+      Tl[a] = TgA[n];
+      rTl[a] = rTgA[n];
+    }
+    memset(te, 0, sizeof te);
+    memset(rt, 0, sizeof rt);
+    const int r = orc_element_tem(nshl, ngauss, xl, dl, ul, uml, phil, Tl, rTl, prm, qp,
+                                  elm_id[e] == fluid_id, tensor_flag, te, rt);
+    if (r < 0) return r;
+    nneg += r;
+    for (int a = 0; a < nshl; ++a) {
+      if (ibc_t[ien[a * nelem + e]] != 1) continue;
+      for (int b = 0; b < nshl; ++b) {
+        te[(size_t)b * nshl + a] = 0.0;
+        te[(size_t)a * nshl + b] = 0.0;
+      }
+      te[(size_t)a * nshl + a] = 1.0;
+      rt[a] = 0.0;
+    }
+    if (tensor_flag & 4) {
+      for (int a = 0; a < nshl; ++a) {
+        const int i = ien[a * nelem + e];
+        for (int b = 0; b < nshl; ++b) {
+          const int k = orc_sparse_loc(row, col[i], col[i + 1], ien[b * nelem + e]);
+          lhs[k] += te[(size_t)b * nshl + a];
+        }
+      }
+    }
+    if (tensor_flag & 2)
+      for (int a = 0; a < nshl; ++a) rhs[ien[a * nelem + e]] += rt[a];
+  }
+  return nneg;
+}
+
+/* SparseProd_LS (SparseProd.f90:128-161) */
+void orc_spmv_s(int nnode, const int *col, const int *row, const double *A, const double *x,
+                double *y) {
+  for (int i = 0; i < nnode; ++i) {
+    double t = 0.0;
+    for (int k = col[i]; k < col[i + 1]; ++k) t = t + A[k] * x[row[k]];
+    y[i] = 0.0 + t;
+  }
+}
+
+/* SparseGMRES_tem (SparseGMRES_tem.f90:1-256): diagonal (0 -> 1) left
+ * preconditioner, one MGS cycle; sol accumulates the increment, or is set
+ * to 0 when |b| = 0.  Returns the iteration count. */
+int orc_gmres_tem(int nnode, const int *col, const int *row, const double *A,
+                  const double *b, double *sol, double tol, int kspace, int kspace_min,
+                  double *resid) {
+  const int N = nnode, K = kspace;
+  double *dinv = calloc((size_t)N, sizeof(double));
+  double *V = calloc((size_t)N * (K + 1), sizeof(double));
+  double *r = calloc((size_t)N, sizeof(double));
+  double *H = calloc((size_t)(K + 1) * K, sizeof(double));
+  double *rc = calloc((size_t)K, sizeof(double)), *rs = calloc((size_t)K, sizeof(double));
+  double *e = calloc((size_t)K + 1, sizeof(double)), *y = calloc((size_t)K, sizeof(double));
+#define HB(i, j) H[(i) + (K + 1) * (j)]
+  int its = 0;
+  for (int i = 0; i < N; ++i)
+    for (int k = col[i]; k < col[i + 1]; ++k)
+      if (row[k] == i) dinv[i] = A[k];
+  for (int i = 0; i < N; ++i) dinv[i] = dinv[i] != 0.0 ? 1.0 / dinv[i] : 1.0;
+  double rr = 0.0;
+  for (int i = 0; i < N; ++i) rr = rr + b[i] * b[i];
+  if (rr == 0.0) {
+    for (int i = 0; i < N; ++i) sol[i] = 0.0;
+    *resid = 0.0;
+    goto done;
+  }
+  for (int i = 0; i < N; ++i) V[i] = dinv[i] * b[i];
+  rr = 0.0;
+  for (int i = 0; i < N; ++i) rr = rr + V[i] * V[i];
+  double unorm = sqrt(rr);
+  const double eps = tol * unorm;
+  e[0] = unorm;
+  unorm = 1.0 / unorm;
+  for (int i = 0; i < N; ++i) V[i] = V[i] * unorm;
+  double er = 0.0;
+  int iks = 0;
+  for (int ik = 1; ik <= K; ++ik) {
+    iks = ik;
+    double *vn = V + (size_t)N * ik;
+    orc_spmv_s(N, col, row, A, V + (size_t)N * (ik - 1), vn);
+    for (int i = 0; i < N; ++i) vn[i] = dinv[i] * vn[i];
+    double beta = 0.0;
+    for (int jk = 1; jk <= ik + 1; ++jk) {
+      if (jk == 1) {
+        double s = 0.0;
+        for (int i = 0; i < N; ++i) s = s + vn[i] * V[i];
+        beta = s;
+      } else {
+        const double *vp = V + (size_t)N * (jk - 2), *vj = V + (size_t)N * (jk - 1);
+        for (int i = 0; i < N; ++i) vn[i] = vn[i] - beta * vp[i];
+        double s = 0.0;
+        for (int i = 0; i < N; ++i) s = s + vn[i] * vj[i];
+        beta = s;
+      }
+      HB(jk - 1, ik - 1) = beta;
+    }
+    double un = sqrt(beta);
+    HB(ik, ik - 1) = un;
+    un = 1.0 / un;
+    for (int i = 0; i < N; ++i) vn[i] = vn[i] * un;
+    for (int jk = 1; jk <= ik - 1; ++jk) {
+      const double t = rc[jk - 1] * HB(jk - 1, ik - 1) + rs[jk - 1] * HB(jk, ik - 1);
+      HB(jk, ik - 1) = -rs[jk - 1] * HB(jk - 1, ik - 1) + rc[jk - 1] * HB(jk, ik - 1);
+      HB(jk - 1, ik - 1) = t;
+    }
+    const double t = sqrt(HB(ik - 1, ik - 1) * HB(ik - 1, ik - 1) + HB(ik, ik - 1) * HB(ik, ik - 1));
+    const double t1 = 1.0 / t;
+    rc[ik - 1] = HB(ik - 1, ik - 1) * t1;
+    rs[ik - 1] = HB(ik, ik - 1) * t1;
+    HB(ik - 1, ik - 1) = t;
+    HB(ik, ik - 1) = 0.0;
+    const double te2 = rc[ik - 1] * e[ik - 1] + rs[ik - 1] * e[ik];
+    e[ik] = -rs[ik - 1] * e[ik - 1] + rc[ik - 1] * e[ik];
+    e[ik - 1] = te2;
+    er = fabs(e[ik]);
+    if (er <= eps && ik >= kspace_min) break;
+  }
+  its = iks;
+  for (int jk = iks; jk >= 1; --jk) {
+    y[jk - 1] = e[jk - 1] / HB(jk - 1, jk - 1);
+    for (int l = 0; l < jk - 1; ++l) e[l] = e[l] - y[jk - 1] * HB(l, jk - 1);
+  }
+  for (int jk = 1; jk <= iks; ++jk)
+    for (int i = 0; i < N; ++i) sol[i] = sol[i] + y[jk - 1] * V[(size_t)N * (jk - 1) + i];
+  *resid = er;
+done:
+#undef HB
+  free(dinv);
+  free(V);
+  free(r);
+  free(H);
+  free(rc);
+  free(rs);
+  free(e);
+  free(y);
+  return its;
+}

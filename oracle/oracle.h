@@ -121,6 +121,33 @@ void orc_interp_hex(int nnode, int nelem, const int *ien, const int *elm_id,
                     const double *dr, int *donor, double *val);
 int orc_hex_local(const double X[8][3], const double p[3], double xi[3]);
 
+/*
+ * Temperature equation of the quenching solver (solveQuenching.f90:579-882):
+ * IntElmAss_Tem_Quenching_STAB + BCLhs_tem + FillSparseMat_tem.
+ * prm = the NS parameter vector (almi, alfi, gami, beti, delt, ...);
+ * qp = {rhoa, rhow, mua, muw, cpa, cpw, kappaa, kappaw, rhos, mus, cps,
+ *       kappas, Ts, lh, c_cond, c_evap, c_dmdot, tem_kdc, tref, fluid_id};
+ * ibc_t(NNODE) = IBC(:,6).  Returns the number of negative Jacobians.
+ */
+int orc_assemble_tem(int nnode, int nelem, int nshl, int ngauss, const int *ien,
+                     const int *elm_id, int tensor_flag, const double *xg,
+                     const int *ibc_t, const double *dgA, const double *ugA,
+                     const double *ugmA, const double *phigA, const double *TgA,
+                     const double *rTgA, const double *prm, const double *qp,
+                     const int *col, const int *row, double *lhs, double *rhs);
+int orc_element_tem(int nshl, int ngauss, const double *xl, const double *dl,
+                    const double *ul, const double *uml, const double *phil,
+                    const double *Tl, const double *rTl, const double *prm,
+                    const double *qp, int is_fluid, int tensor_flag, double *te,
+                    double *rt);
+/* SparseProd_LS: y = A x, scalar CSR */
+void orc_spmv_s(int nnode, const int *col, const int *row, const double *A, const double *x,
+                double *y);
+/* SparseGMRES_tem: one MGS cycle, diagonal preconditioner */
+int orc_gmres_tem(int nnode, const int *col, const int *row, const double *A,
+                  const double *b, double *sol, double tol, int kspace, int kspace_min,
+                  double *resid);
+
 #ifdef __cplusplus
 }
 #endif

@@ -85,6 +85,13 @@ def lib():
         L.orc_gmres_up.restype = ctypes.c_int
         L.orc_interp_tet.argtypes = ([ctypes.c_int] * 3 + [_ip, _ip, ctypes.c_int] +
                                      [_dp] * 3 + [ctypes.c_int, _dp, _dp, _ip, _dp])
+        L.orc_assemble_tem.argtypes = ([ctypes.c_int] * 4 + [_ip, _ip, ctypes.c_int, _dp,
+                                        _ip] + [_dp] * 8 + [_ip, _ip, _dp, _dp])
+        L.orc_assemble_tem.restype = ctypes.c_int
+        L.orc_spmv_s.argtypes = [ctypes.c_int, _ip, _ip, _dp, _dp, _dp]
+        L.orc_gmres_tem.argtypes = ([ctypes.c_int, _ip, _ip, _dp, _dp, _dp, ctypes.c_double,
+                                     ctypes.c_int, ctypes.c_int, _dp])
+        L.orc_gmres_tem.restype = ctypes.c_int
         L.orc_interp_hex.argtypes = ([ctypes.c_int] * 2 + [_ip, _ip, ctypes.c_int] +
                                      [_dp] * 3 + [ctypes.c_int, _dp, _dp, _ip, _dp])
         L.orc_hex_local.argtypes = [_dp, _dp, _dp]
@@ -220,6 +227,44 @@ def gmres_up(col, row, A, rhsu, rhsp, tol, kspace, kspace_min=0):
                              int(kspace_min), res, r0)
     return dict(solu=solu.reshape(3, N).T.copy(), solp=solp, iters=int(its),
                 resid=float(res[0]), rnorm0=float(r0[0]))
+
+
+def assemble_tem(mesh, ibc_t, st, prm, qprm, col, row, tensor_flag, lhs=None, rhs=None):
+    """IntElmAss_Tem_Quenching_STAB + BCLhs_tem + FillSparseMat_tem.  st holds
+    the alpha-stage fields dg, ug, ugm (N,3) and phig, Tg, rTg (N); prm the
+    NS parameter vector (almi, alfi, gami, [beti], delt, ...) or the 4-vector
+    (almi, alfi, gami, delt); qprm the 19 material values (+ fluid id)."""
+    N, E = mesh.nnode, mesh.nelem
+    p = np.asarray(prm, np.float64)
+    if p.size == 4:
+        p = np.array([p[0], p[1], p[2], 0.0, p[3]])
+    q = np.zeros(20)
+    q[:len(qprm)] = qprm
+    if len(qprm) < 20:
+        q[19] = 102
+    lhs = np.zeros(len(row)) if lhs is None else lhs
+    rhs = np.zeros(N) if rhs is None else rhs
+    nneg = lib().orc_assemble_tem(
+        N, E, mesh.nshl, mesh.nshl, _c(mesh.ien.T, np.int32).ravel(),
+        _c(mesh.elm_id, np.int32), int(tensor_flag), _soa(mesh.xg),
+        _c(ibc_t, np.int32), _soa(st["dg"]), _soa(st["ug"]), _soa(st["ugm"]),
+        _c(st["phig"]), _c(st["Tg"]), _c(st["rTg"]), _c(p), _c(q), _c(col, np.int32),
+        _c(row, np.int32), lhs, rhs)
+    return dict(lhs=lhs, rhs=rhs, nneg=nneg)
+
+
+def spmv_s(col, row, A, x):
+    y = np.zeros(len(col) - 1)
+    lib().orc_spmv_s(len(col) - 1, _c(col, np.int32), _c(row, np.int32), _c(A), _c(x), y)
+    return y
+
+
+def gmres_tem(col, row, A, b, tol, kspace, kspace_min):
+    sol = np.zeros(len(col) - 1)
+    res = np.zeros(1)
+    it = lib().orc_gmres_tem(len(col) - 1, _c(col, np.int32), _c(row, np.int32), _c(A),
+                             _c(b), sol, float(tol), int(kspace), int(kspace_min), res)
+    return dict(sol=sol, iters=it, resid=float(res[0]))
 
 
 def interp_hex(mesh, src_id, dg, ug, xr, dr):

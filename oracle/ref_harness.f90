@@ -18,6 +18,9 @@
 !              SparseProdUP_3D on a given vector, SparseGMRES_up on the
 !              assembled system
 !   3  interp: interpolation_tet for receiver points
+!   4  tem:    genSparStruc, IntElmAss_Tem_Quenching_STAB (with BCLhs_tem and
+!              FillSparseMat_tem), SparseProd_LS on a given vector,
+!              SparseGMRES_tem on the assembled system
 !======================================================================
 program ofd_refharness
   use aAdjKeep
@@ -51,6 +54,8 @@ program ofd_refharness
     call run_ns()
   case (3)
     call run_interp()
+  case (4)
+    call run_tem()
   case default
     write (*, *) 'unknown mode', mode
   end select
@@ -203,5 +208,60 @@ contains
     call interpolation_tet(1)
     write (11) bmesh(1)%ug
   end subroutine run_interp
+
+  subroutine run_tem()
+    integer :: nshl, ngauss, tflag, kspace, kmin
+    real(8) :: prm(4), qprm(19), tol
+    real(8), allocatable :: dA(:, :), uA(:, :), umA(:, :), aA(:, :), &
+                            amA(:, :), pA(:), phiA(:), rphiA(:), TA(:), rTA(:), &
+                            x(:), y(:), sol(:)
+
+    read (10) NNODE, NELEM, nshl, ngauss, tflag
+    maxNSHL = nshl
+    allocate (IEN(NELEM, nshl), ELM_ID(NELEM), ELMNSHL(NELEM), &
+              ELMNGAUSS(NELEM), IBC(NNODE, 8), xg(NNODE, 3), wg(NNODE), &
+              dgold(NNODE, 3), ugold(NNODE, 3))
+    read (10) IEN, ELM_ID, IBC
+    read (10) xg
+    wg = 1.0d0
+    dgold = 0.0d0
+    ugold = 0.0d0
+    ELMNSHL = nshl
+    ELMNGAUSS = ngauss
+    allocate (dA(NNODE, 3), uA(NNODE, 3), umA(NNODE, 3), aA(NNODE, 3), &
+              amA(NNODE, 3), pA(NNODE), phiA(NNODE), rphiA(NNODE), TA(NNODE), &
+              rTA(NNODE))
+    read (10) dA, uA, umA, phiA, TA, rTA, prm, qprm
+    aA = 0.0d0; amA = 0.0d0; pA = 0.0d0; rphiA = 0.0d0
+    almi = prm(1); alfi = prm(2); gami = prm(3); Delt = prm(4)
+    rhoa = qprm(1); rhow = qprm(2); mua = qprm(3); muw = qprm(4)
+    cpa = qprm(5); cpw = qprm(6); kappaa = qprm(7); kappaw = qprm(8)
+    rhos = qprm(9); mus = qprm(10); cps = qprm(11); kappas = qprm(12)
+    Ts = qprm(13); lh = qprm(14); c_cond = qprm(15); c_evap = qprm(16)
+    C_DMDOT = qprm(17); TEM_kdc = qprm(18); Tref = qprm(19)
+    read (10) tol, kspace, kmin
+    allocate (x(NNODE), y(NNODE), sol(NNODE))
+    read (10) x
+
+    call genSparStruc()
+    allocate (LHSTem(icnt), RHSGTem(NNODE))
+    LHSTem = 0.0d0
+    RHSGTem = 0.0d0
+    call IntElmAss_Tem_Quenching_STAB(dA, uA, umA, aA, amA, pA, phiA, rphiA, &
+                                      TA, rTA, tflag)
+    write (11) icnt
+    write (11) col, row
+    write (11) LHSTem, RHSGTem
+
+    call SparseProd_LS(LHSTem, col, row, x, y, NNODE, nshl, icnt, NSD)
+    write (11) y
+
+    sol = 0.0d0
+    if (kspace > 0) then
+      call SparseGMRES_tem(LHSTem, tol, col, row, RHSGTem, sol, kspace, kmin, &
+                           NNODE, nshl, icnt, NSD)
+    end if
+    write (11) sol
+  end subroutine run_tem
 
 end program ofd_refharness

@@ -23,10 +23,17 @@ OFD_ORTHO_CGS2 = 1
 OFD_ORTHO_CGS1 = 2
 
 FIELDS = ["XG", "DG", "DGOLD", "UG", "UGOLD", "UGM", "UGMOLD", "ACG", "ACGOLD",
-          "ACGM", "ACGMOLD", "PG", "PGOLD", "SOLU", "SOLP"]
+          "ACGM", "ACGMOLD", "PG", "PGOLD", "SOLU", "SOLP",
+          "PHIG", "PHIGOLD", "RPHIG", "RPHIGOLD", "TG", "TGOLD", "RTG", "RTGOLD", "SOLT"]
 FIELD = {n: i for i, n in enumerate(FIELDS)}
+SCALAR_FIELDS = {"PG", "PGOLD", "SOLP", "PHIG", "PHIGOLD", "RPHIG", "RPHIGOLD", "TG",
+                 "TGOLD", "RTG", "RTGOLD", "SOLT"}
 PARAMS = ["almi", "alfi", "gami", "beti", "delt", "rho", "mu", "kdc", "gx", "gy",
           "gz"]
+# material vector of ofd_set_quench_params
+QPARAMS = ["rhoa", "rhow", "mua", "muw", "cpa", "cpw", "kappaa", "kappaw", "rhos", "mus",
+           "cps", "kappas", "Ts", "lh", "c_cond", "c_evap", "c_dmdot", "tem_kdc", "tref",
+           "fluid_id"]
 
 _c_int = ctypes.c_int
 _c_dbl = ctypes.c_double
@@ -66,6 +73,16 @@ SIGNATURES = {
     "ofd_predictor": ([_vp], _c_int),
     "ofd_advance": ([_vp], _c_int),
     "ofd_rhs_norms": ([_vp, _dp], _c_int),
+    "ofd_set_quench_params": ([_vp, _vp], _c_int),
+    "ofd_assemble_tem": ([_vp, _c_int], _c_int),
+    "ofd_get_tem_system": ([_vp, _vp, _vp], _c_int),
+    "ofd_set_tem_system": ([_vp, _vp, _vp], _c_int),
+    "ofd_spmv_tem": ([_vp, _c_int, _vp, _vp], _c_int),
+    "ofd_gmres_tem": ([_vp, _c_int, _c_int, _c_dbl, _c_int, _c_int, _vp, _ip, _dp], _c_int),
+    "ofd_solve_tem": ([_vp, _c_int, _c_int, _c_dbl, _c_dbl, _c_int, _c_int, _ip, _ip, _dp],
+                      _c_int),
+    "ofd_tem_update": ([_vp], _c_int),
+    "ofd_tem_rhs_norm": ([_vp, _dp], _c_int),
     "ofd_fringe_interp": ([_vp, ctypes.c_int32, _c_int, _vp, _vp, _vp, _vp], _c_int),
     "ofd_apply_fringe": ([_vp, _c_int, _vp, _vp], _c_int),
     "ofd_time_spmv": ([_vp, _c_int, _c_int, _dp], _c_int),

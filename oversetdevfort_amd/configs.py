@@ -11,7 +11,15 @@ and setBCs_CFD; the GPU path consumes the resulting arrays.
   initial field = Poiseuille profile + a seeded divergence-free sinusoidal
   perturbation of amplitude 0.01; Re = 1000 (rho 1, mu 1e-3); dt = 0.005.
 
-Both use generalized-alpha with rho_inf = 0.5 (param.f90:469-480) and the
+* ``quench`` -- n^3 hex tank on the unit cube: water below z = 0.6 with a
+  seeded vapour sphere of radius 0.1, vapour above; density ratio 1000,
+  viscosity ratio 100; a solid block [0.4,0.6]^3 (ELM_ID 101, fluid 102 as
+  solveQuenching.f90:693) at 900 K in liquid at 300 K; piecewise-constant
+  k and c_p; dt = 1e-3.  The reference's kernels treat the VOF field phi as
+  the gas fraction (prop_interp(rhow, rhoa, phi), evaporation ~ (1 - phi)),
+  so the liquid is phi = 0.
+
+All use generalized-alpha with rho_inf = 0.5 (param.f90:469-480) and the
 reference's hard-coded gravity (0, 0, -9.81) (param.f90:253-257).
 """
 from __future__ import annotations
@@ -132,3 +140,45 @@ def apply_dirichlet(ug, acg, ugold, acgold, prob: Problem):
     acg[m] = (((ug - ugold) * (1.0 / p["delt"]) + (p["gami"] - 1.0) * acgold) /
               p["gami"])[m]
     return ug, acg
+
+
+# water / vapour / solid data for the quenching tank (param.f90:222-242)
+QUENCH_MATERIAL = dict(rhoa=1.0, rhow=1000.0, mua=1.0e-5, muw=1.0e-3, cpa=2000.0,
+                       cpw=4200.0, kappaa=0.025, kappaw=0.6, rhos=7800.0, mus=0.0,
+                       cps=500.0, kappas=30.0, Ts=373.15, lh=2.26e6, c_cond=0.1,
+                       c_evap=0.1, c_dmdot=1.0, tem_kdc=0.0, tref=425.0, fluid_id=102)
+
+
+def quench(n=48, seed=1, rhoinf=0.5, jitter=0.0) -> Problem:
+    mesh = M.structured_hex(n, n, n, jitter=jitter, seed=seed)
+    N, E = mesh.nnode, mesh.nelem
+    x = mesh.xg
+    rng = np.random.default_rng(seed)
+    # element ids: solid block [0.4, 0.6]^3 by element centroid
+    cen = x[mesh.ien].mean(axis=1)
+    solid = np.all((cen > 0.4) & (cen < 0.6), axis=1)
+    mesh.elm_id = np.where(solid, 101, 102).astype(np.int32)
+    # gas fraction: 0 in the liquid below z = 0.6, 1 above and in a bubble
+    c = rng.uniform([0.15, 0.15, 0.15], [0.85, 0.85, 0.45])
+    while np.all(np.abs(c - 0.5) < 0.2):      # keep the bubble off the block
+        c = rng.uniform([0.15, 0.15, 0.15], [0.85, 0.85, 0.45])
+    phi = (x[:, 2] >= 0.6).astype(np.float64)
+    phi[np.linalg.norm(x - c, axis=1) < 0.1] = 1.0
+    # temperature: 900 K on the nodes of solid elements, 300 K elsewhere
+    T = np.full(N, 300.0)
+    T[np.unique(mesh.ien[solid].ravel())] = 900.0
+    faces = M.boundary_masks(mesh, (0, 0, 0), (1, 1, 1))
+    wall = np.zeros(N, bool)
+    for k in ("xlo", "xhi", "ylo", "yhi", "zlo"):
+        wall |= faces[k]
+    ibc = np.zeros((N, 8), np.int32)
+    ibc[wall, :3] = 1                # no-slip tank walls; open top
+    f = _zero_fields(N)
+    f.update(phig=phi.copy(), phigold=phi.copy(), rphig=np.zeros(N), rphigold=np.zeros(N),
+             Tg=T.copy(), Tgold=T.copy(), rTg=np.zeros(N), rTgold=np.zeros(N))
+    rho = QUENCH_MATERIAL["rhow"]
+    p = dict(genalpha(rhoinf), delt=1e-3, rho=rho, mu=QUENCH_MATERIAL["muw"], kdc=0.0,
+             grav=(0.0, 0.0, -9.81))
+    prob = Problem("quench", mesh, ibc, np.zeros((N, 3)), f, p)
+    prob.material = dict(QUENCH_MATERIAL)
+    return prob

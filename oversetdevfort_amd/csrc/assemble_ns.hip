@@ -46,7 +46,8 @@ __device__ __forceinline__ double max8(double v) {
 
 }  // namespace
 
-// alpha stage (assembleNavSto.f90:25-30) packed into one 128-byte record per node
+// alpha stage (assembleNavSto.f90:25-30; solveQuenching.f90:25-37 for the
+// quenching scalars) packed into one 128-byte record per node
 __global__ void k_alpha_stage(int N, const double *__restrict__ xg,
                               const double *__restrict__ dg,
                               const double *__restrict__ dgold,
@@ -56,7 +57,13 @@ __global__ void k_alpha_stage(int N, const double *__restrict__ xg,
                               const double *__restrict__ ugmold,
                               const double *__restrict__ acg,
                               const double *__restrict__ acgold,
-                              const double *__restrict__ pg, double almi,
+                              const double *__restrict__ pg,
+                              const double *__restrict__ phig,
+                              const double *__restrict__ phigold,
+                              const double *__restrict__ tg,
+                              const double *__restrict__ tgold,
+                              const double *__restrict__ rtg,
+                              const double *__restrict__ rtgold, double almi,
                               double alfi, double *__restrict__ rec) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
@@ -71,7 +78,14 @@ __global__ void k_alpha_stage(int N, const double *__restrict__ xg,
     r[9 + c] = ugmold[o] + alfi * (ugm[o] - ugmold[o]);
   }
   r[12] = pg[i];
-  r[13] = r[14] = r[15] = 0.0;
+  // quenching scalars (solveQuenching.f90:33-37): phi, T at alfi, rT at almi
+  if (phig) {
+    r[13] = phigold[i] + alfi * (phig[i] - phigold[i]);
+    r[14] = tgold[i] + alfi * (tg[i] - tgold[i]);
+    r[15] = rtgold[i] + almi * (rtg[i] - rtgold[i]);
+  } else {
+    r[13] = r[14] = r[15] = 0.0;
+  }
   double2 *out = reinterpret_cast<double2 *>(rec + (size_t)i * 16);
 #pragma unroll
   for (int t = 0; t < 8; ++t) out[t] = make_double2(r[2 * t], r[2 * t + 1]);
@@ -397,8 +411,10 @@ hipError_t launch_alpha_stage(Ctx &c) {
                      c.d_field[OFD_XG], c.d_field[OFD_DG], c.d_field[OFD_DGOLD],
                      c.d_field[OFD_UG], c.d_field[OFD_UGOLD], c.d_field[OFD_UGM],
                      c.d_field[OFD_UGMOLD], c.d_field[OFD_ACG],
-                     c.d_field[OFD_ACGOLD], c.d_field[OFD_PG], almi, alfi,
-                     c.d_noderec);
+                     c.d_field[OFD_ACGOLD], c.d_field[OFD_PG],
+                     c.quench ? c.d_field[OFD_PHIG] : nullptr, c.d_field[OFD_PHIGOLD],
+                     c.d_field[OFD_TG], c.d_field[OFD_TGOLD], c.d_field[OFD_RTG],
+                     c.d_field[OFD_RTGOLD], almi, alfi, c.d_noderec);
   return hipGetLastError();
 }
 

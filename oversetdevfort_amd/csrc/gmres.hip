@@ -145,9 +145,11 @@ __device__ __forceinline__ void axpy_batches(const V *__restrict__ W, size_t vs,
 // W_0 = P^-1 b and partials of |W_0|^2 (SparseGMRES_NS.f90:124-155); block 0
 // also resets the cycle scalars.  The first orthogonalisation kernel of
 // iteration 0 closes the norm.
-template <typename V>
+// SCALAR: the T system, P = inverse diagonal of the nsc scalar unknowns,
+// taken 4 at a time (entries past nsc are 0)
+template <typename V, bool SCALAR>
 __global__ void __launch_bounds__(256)
-k_gmres_init(int N, const double *__restrict__ P, const double *__restrict__ b,
+k_gmres_init(int N, int nsc, const double *__restrict__ P, const double *__restrict__ b,
              V *__restrict__ W0, double *__restrict__ part, GmresState *__restrict__ g,
              double tol, int kspace, int kmin) {
   if (g->outer_done) {
@@ -174,11 +176,19 @@ k_gmres_init(int N, const double *__restrict__ P, const double *__restrict__ b,
        i += gridDim.x * blockDim.x) {
     double r[4], z[4];
     ld4(b + (size_t)i * 4, r);
-    const double *p = P + (size_t)i * 16;
+    if (SCALAR) {
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
-      z[c] = p[c * 4 + 0] * r[0] + p[c * 4 + 1] * r[1] + p[c * 4 + 2] * r[2] +
-             p[c * 4 + 3] * r[3];
+      for (int c = 0; c < 4; ++c) {
+        const int t = 4 * i + c;
+        z[c] = (t < nsc) ? P[t] * r[c] : 0.0;
+      }
+    } else {
+      const double *p = P + (size_t)i * 16;
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        z[c] = p[c * 4 + 0] * r[0] + p[c * 4 + 1] * r[1] + p[c * 4 + 2] * r[2] +
+               p[c * 4 + 3] * r[3];
+    }
     st4(W0 + (size_t)i * 4, z);
     acc[0] += dot4(z, z);
   }
@@ -463,6 +473,14 @@ int nblocks_nodes(const Ctx &c) {
   return nb;
 }
 
+// blocks of the GMRES sweeps over the current system's gm_n tuples
+int nblocks_gm(const Ctx &c) {
+  int nb = (c.gm_n + kRedBlock - 1) / kRedBlock;
+  if (nb > kMaxRedBlocks) nb = kMaxRedBlocks;
+  if (nb < 1) nb = 1;
+  return nb;
+}
+
 // partial-sum buffers: 0/1 alternate along the MGS chain and hold the
 // closing norms of the CGS variants (parity of k); 2 holds the projection
 // dots, 3 the dots of the second CGS2 projection
@@ -471,22 +489,34 @@ static double *part_buf(Ctx &c, int which) {
 }
 
 hipError_t launch_gmres_init(Ctx &c, const double *b, double tol, int kspace, int kmin) {
-  const int nb = nblocks_nodes(c);
-  if (c.basis32)
-    hipLaunchKernelGGL(k_gmres_init<float>, dim3(nb), dim3(kRedBlock), 0, c.stream,
-                       c.nnode, c.d_P64, b, c.d_W32, part_buf(c, 0), c.d_gs, tol, kspace,
-                       kmin);
-  else
-    hipLaunchKernelGGL(k_gmres_init<double>, dim3(nb), dim3(kRedBlock), 0, c.stream,
-                       c.nnode, c.d_P64, b, c.d_W, part_buf(c, 0), c.d_gs, tol, kspace,
-                       kmin);
+  const int nb = nblocks_gm(c);
+  const double *P = c.gm_scalar ? c.d_PT : c.d_P64;
+  if (c.basis32) {
+    if (c.gm_scalar)
+      hipLaunchKernelGGL((k_gmres_init<float, true>), dim3(nb), dim3(kRedBlock), 0, c.stream,
+                         c.gm_n, c.nnode, P, b, c.d_W32, part_buf(c, 0), c.d_gs, tol,
+                         kspace, kmin);
+    else
+      hipLaunchKernelGGL((k_gmres_init<float, false>), dim3(nb), dim3(kRedBlock), 0,
+                         c.stream, c.gm_n, c.nnode, P, b, c.d_W32, part_buf(c, 0), c.d_gs,
+                         tol, kspace, kmin);
+  } else {
+    if (c.gm_scalar)
+      hipLaunchKernelGGL((k_gmres_init<double, true>), dim3(nb), dim3(kRedBlock), 0,
+                         c.stream, c.gm_n, c.nnode, P, b, c.d_W, part_buf(c, 0), c.d_gs,
+                         tol, kspace, kmin);
+    else
+      hipLaunchKernelGGL((k_gmres_init<double, false>), dim3(nb), dim3(kRedBlock), 0,
+                         c.stream, c.gm_n, c.nnode, P, b, c.d_W, part_buf(c, 0), c.d_gs,
+                         tol, kspace, kmin);
+  }
   c.part_cur = 0;
   return hipGetLastError();
 }
 
 hipError_t launch_mgs_step(Ctx &c, int k, int j) {
-  const int nb = nblocks_nodes(c);
-  hipLaunchKernelGGL(k_mgs_step, dim3(nb), dim3(kRedBlock), 0, c.stream, c.nnode, c.d_W,
+  const int nb = nblocks_gm(c);
+  hipLaunchKernelGGL(k_mgs_step, dim3(nb), dim3(kRedBlock), 0, c.stream, c.gm_n, c.d_W,
                      part_buf(c, c.part_cur), part_buf(c, c.part_cur ^ 1), c.d_gs, k, j, nb,
                      c.cur_flags);
   c.part_cur ^= 1;
@@ -494,46 +524,46 @@ hipError_t launch_mgs_step(Ctx &c, int k, int j) {
 }
 
 hipError_t launch_cgs_dots(Ctx &c, int k) {
-  const int nb = nblocks_nodes(c);
+  const int nb = nblocks_gm(c);
   if (c.basis32)
-    hipLaunchKernelGGL(k_cgs_dots<float>, dim3(nb), dim3(kRedBlock), 0, c.stream, c.nnode,
+    hipLaunchKernelGGL(k_cgs_dots<float>, dim3(nb), dim3(kRedBlock), 0, c.stream, c.gm_n,
                        c.d_W32, part_buf(c, k & 1), part_buf(c, 2), c.d_gs, k, nb,
                        c.cur_flags);
   else
     hipLaunchKernelGGL(k_cgs_dots<double>, dim3(nb), dim3(kRedBlock), 0, c.stream,
-                       c.nnode, c.d_W, part_buf(c, k & 1), part_buf(c, 2), c.d_gs, k, nb,
+                       c.gm_n, c.d_W, part_buf(c, k & 1), part_buf(c, 2), c.d_gs, k, nb,
                        c.cur_flags);
   return hipGetLastError();
 }
 
 hipError_t launch_cgs_update(Ctx &c, int k, int mode) {
-  const int nb = nblocks_nodes(c);
+  const int nb = nblocks_gm(c);
   const double *rd_dots = part_buf(c, mode == 3 ? 3 : 2);
   double *wr = (mode == 2) ? part_buf(c, 3) : part_buf(c, (k + 1) & 1);
   if (c.basis32)
     hipLaunchKernelGGL(k_cgs_update<float>, dim3(nb), dim3(kRedBlock), 0, c.stream,
-                       c.nnode, c.d_W32, rd_dots, wr, c.d_gs, k, mode, nb);
+                       c.gm_n, c.d_W32, rd_dots, wr, c.d_gs, k, mode, nb);
   else
     hipLaunchKernelGGL(k_cgs_update<double>, dim3(nb), dim3(kRedBlock), 0, c.stream,
-                       c.nnode, c.d_W, rd_dots, wr, c.d_gs, k, mode, nb);
+                       c.gm_n, c.d_W, rd_dots, wr, c.d_gs, k, mode, nb);
   c.part_cur = (k + 1) & 1;
   return hipGetLastError();
 }
 
 hipError_t launch_gmres_close(Ctx &c, int k, int ortho) {
   hipLaunchKernelGGL(k_gmres_close, dim3(1), dim3(kRedBlock), 0, c.stream,
-                     part_buf(c, c.part_cur), nblocks_nodes(c), c.d_gs, k,
+                     part_buf(c, c.part_cur), nblocks_gm(c), c.d_gs, k,
                      ortho == OFD_ORTHO_MGS ? 0 : 1);
   return hipGetLastError();
 }
 
 hipError_t launch_gmres_finish(Ctx &c, double *x) {
   if (c.basis32)
-    hipLaunchKernelGGL(k_gmres_finish<float>, dim3(nblocks_nodes(c)), dim3(kRedBlock), 0,
-                       c.stream, c.nnode, c.d_W32, c.d_gs, x);
+    hipLaunchKernelGGL(k_gmres_finish<float>, dim3(nblocks_gm(c)), dim3(kRedBlock), 0,
+                       c.stream, c.gm_n, c.d_W32, c.d_gs, x);
   else
-    hipLaunchKernelGGL(k_gmres_finish<double>, dim3(nblocks_nodes(c)), dim3(kRedBlock), 0,
-                       c.stream, c.nnode, c.d_W, c.d_gs, x);
+    hipLaunchKernelGGL(k_gmres_finish<double>, dim3(nblocks_gm(c)), dim3(kRedBlock), 0,
+                       c.stream, c.gm_n, c.d_W, c.d_gs, x);
   return hipGetLastError();
 }
 
@@ -554,9 +584,9 @@ hipError_t launch_outer_reset(Ctx &c, double thr) {
 }
 
 hipError_t launch_norm2(Ctx &c, const double *v, int slot, int outer) {
-  const int nb = nblocks_nodes(c);
+  const int nb = nblocks_gm(c);
   double *pb = part_buf(c, 0);
-  hipLaunchKernelGGL(k_norm2, dim3(nb), dim3(kRedBlock), 0, c.stream, c.nnode, v, pb);
+  hipLaunchKernelGGL(k_norm2, dim3(nb), dim3(kRedBlock), 0, c.stream, c.gm_n, v, pb);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_norm_fin, dim3(1), dim3(kRedBlock), 0, c.stream, pb, nb, c.d_gs,

@@ -70,7 +70,8 @@ static int d2h(Ctx &c, void *dst, const void *src, size_t bytes) {
 }
 
 static int field_len(const Ctx &c, int f) {
-  return (f == OFD_PG || f == OFD_PGOLD || f == OFD_SOLP) ? c.nnode : 3 * c.nnode;
+  const bool vec3 = f <= OFD_ACGMOLD || f == OFD_SOLU;
+  return vec3 ? 3 * c.nnode : c.nnode;
 }
 
 static int need_mesh(const Ctx &c) {
@@ -180,6 +181,40 @@ static int wait_flag(Ctx &c, int idx, bool *done) {
   }
 }
 
+// the T system's preconditioner: inverse diagonal and D^-1 LHSTem (FP64,
+// plus FP32 for the mixed solver)
+static int ensure_prec_s(Ctx &c, int precision) {
+  OFD_CHECK(c.matT_valid, OFD_ERR_ARG, "no assembled temperature matrix");
+  const bool want32 = precision == OFD_FP32;
+  if (c.precT_valid && (!want32 || c.apT32_valid)) return OFD_OK;
+  if (!c.d_ApT) {
+    int rc = dalloc(c, &c.d_ApT, (size_t)c.icnt);
+    if (rc) return rc;
+  }
+  if (want32 && !c.d_ApT32) {
+    int rc = dalloc(c, &c.d_ApT32, (size_t)c.icnt);
+    if (rc) return rc;
+  }
+  OFD_HIP(launch_build_prec_s(c, want32));
+  c.precT_valid = true;
+  c.apT32_valid = want32;
+  return OFD_OK;
+}
+
+// Points the GMRES kernels at the T system (ceil(N/4) 4-tuples) for the
+// lifetime of the guard; the NS node vectors otherwise.
+struct TemSystem {
+  Ctx &c;
+  explicit TemSystem(Ctx &cc) : c(cc) {
+    c.gm_scalar = true;
+    c.gm_n = (c.nnode + 3) / 4;
+  }
+  ~TemSystem() {
+    c.gm_scalar = false;
+    c.gm_n = c.nnode;
+  }
+};
+
 // one GMRES cycle on right-hand side b (device [N][4]); x += increment.
 // flag_set selects the step-flag set (cycles queued back to back alternate).
 static int gmres_cycle(Ctx &c, int precision, int ortho, const double *b, double tol,
@@ -189,7 +224,7 @@ static int gmres_cycle(Ctx &c, int precision, int ortho, const double *b, double
   c.basis32 = precision == OFD_FP32 && ortho != OFD_ORTHO_MGS;
   int rc = ensure_krylov(c, K);
   if (rc) return rc;
-  rc = ensure_prec(c, precision);
+  rc = c.gm_scalar ? ensure_prec_s(c, precision) : ensure_prec(c, precision);
   if (rc) return rc;
   OFD_HIP(launch_gmres_init(c, b, tol, K, kmin));
   // The host enqueues iterations ahead of the device and learns about
@@ -346,6 +381,15 @@ int ofd_ctx_destroy(ofd_ctx *x) {
   dfree(c.d_W32);
   dfree(c.d_part);
   dfree(c.d_gs);
+  dfree(c.d_AT);
+  dfree(c.d_AT32);
+  dfree(c.d_PT);
+  dfree(c.d_ApT);
+  dfree(c.d_ApT32);
+  dfree(c.d_rhsT);
+  dfree(c.d_xT);
+  dfree(c.d_rT);
+  dfree(c.d_yT);
   dfree(c.d_fr_box);
   dfree(c.d_fr_ok);
   dfree(c.d_fr_geo);
@@ -387,6 +431,10 @@ int ofd_upload_mesh(ofd_ctx *x, int nnode, int nelem, int nshl, const int32_t *i
   c.nnode = nnode;
   c.nelem = nelem;
   c.nshl = nshl;
+  c.gm_n = nnode;
+  c.gm_scalar = false;
+  c.matT_valid = c.precT_valid = c.aT32_valid = c.apT32_valid = false;
+  c.temT_n4 = 0;
   c.icnt = 0;
   c.mat_valid = c.prec_valid = c.a32_valid = c.ap32_valid = false;
   int rc = dalloc(c, &c.d_ien, ne);
@@ -548,14 +596,14 @@ int ofd_set_subdomains(ofd_ctx *x, int32_t id1, int32_t id2) {
 
 int ofd_set_ibc(ofd_ctx *x, const int32_t *ibc, int ncol) {
   OFD_CHECK(x && ibc, OFD_ERR_ARG, "null argument");
-  OFD_CHECK(ncol >= 4, OFD_ERR_ARG, "IBC needs at least 4 columns (u,v,w,p)");
+  OFD_CHECK(ncol >= 4 && ncol <= 8, OFD_ERR_ARG, "IBC needs 4 to 8 columns (u,v,w,p,..,T)");
   Ctx &c = x->c;
   int rc = need_mesh(c);
   if (rc) return rc;
   OFD_HIP(hipSetDevice(c.device));
   std::vector<uint8_t> m(c.nnode, 0);
   for (int i = 0; i < c.nnode; ++i)
-    for (int col = 0; col < 4; ++col)
+    for (int col = 0; col < ncol; ++col)
       if (ibc[(size_t)col * c.nnode + i] == 1) m[i] |= (uint8_t)(1u << col);
   return h2d(c, c.d_ibc, m.data(), m.size());
 }
@@ -774,18 +822,14 @@ int ofd_gmres_up(ofd_ctx *x, int precision, int ortho, double tol, int kspace,
   return rc;
 }
 
-int ofd_solve(ofd_ctx *x, int precision, int ortho, double rtol, double inner_tol,
-              int kspace, int max_cycles, int *iters, int *cycles, double *relres) {
-  OFD_CHECK(x, OFD_ERR_ARG, "null context");
-  OFD_CHECK(precision == OFD_FP64 || precision == OFD_FP32, OFD_ERR_ARG, "bad precision");
-  OFD_CHECK(max_cycles >= 1, OFD_ERR_ARG, "max_cycles >= 1");
-  Ctx &c = x->c;
-  int rc = need_pattern(c);
-  if (rc) return rc;
-  OFD_HIP(hipSetDevice(c.device));
-  OFD_HIP(hipMemsetAsync(c.d_x, 0, (size_t)c.nnode * 4 * sizeof(double), c.stream));
+// Restarted GMRES with an FP64 outer residual on the current system (NS or,
+// under TemSystem, T): x (gm_n 4-tuples) starts at 0 and solves A x = b.
+static int solve_core(Ctx &c, int precision, int ortho, const double *b, double *x,
+                      double *r, double rtol, double inner_tol, int kspace, int max_cycles,
+                      int *iters, int *cycles, double *relres) {
+  OFD_HIP(hipMemsetAsync(x, 0, (size_t)c.gm_n * 4 * sizeof(double), c.stream));
   double bnorm = 0.0;
-  rc = norm2_dev(c, c.d_rhs, &bnorm);
+  int rc = norm2_dev(c, b, &bnorm);
   if (rc) return rc;
   int total = 0, cyc = 0;
   double rel = 0.0;
@@ -795,7 +839,7 @@ int ofd_solve(ofd_ctx *x, int precision, int ortho, double rtol, double inner_to
     if (relres) *relres = 0.0;
     return std::isfinite(bnorm) ? OFD_OK : OFD_ERR_NONFINITE;
   }
-  const double *rhs = c.d_rhs;
+  const double *rhs = b;
   c.solve_hist.clear();
   // Cycle c+1 is queued before the host reads cycle c's FP64 residual, so
   // the device never waits for the host between cycles.  The residual norm
@@ -804,19 +848,21 @@ int ofd_solve(ofd_ctx *x, int precision, int ortho, double rtol, double inner_to
   // as they are).  Each cycle's state is copied to its own pinned slot.
   OFD_HIP(launch_outer_reset(c, rtol * bnorm));
   int done_cycles = 0;
-  bool conv = false;
   for (cyc = 1; cyc <= max_cycles + 1; ++cyc) {
     if (cyc <= max_cycles) {
-      rc = gmres_cycle(c, precision, ortho, rhs, inner_tol, kspace, 0, c.d_x, nullptr,
-                       nullptr, nullptr, false, cyc & 1);
+      rc = gmres_cycle(c, precision, ortho, rhs, inner_tol, kspace, 0, x, nullptr, nullptr,
+                       nullptr, false, cyc & 1);
       if (rc) break;
       // FP64 outer residual r = b - A x and its norm
-      OFD_HIP(launch_spmv(c, OFD_FP64, c.d_x, c.d_r, c.d_rhs, 1));
-      OFD_HIP(launch_norm2(c, c.d_r, 0, 1));
+      if (c.gm_scalar)
+        OFD_HIP(launch_spmv_s(c, OFD_FP64, x, r, b, 1));
+      else
+        OFD_HIP(launch_spmv(c, OFD_FP64, x, r, b, 1));
+      OFD_HIP(launch_norm2(c, r, 0, 1));
       OFD_HIP(hipMemcpyAsync(&c.h_gs_ring[cyc & 1], c.d_gs, sizeof(GmresState),
                              hipMemcpyDeviceToHost, c.stream));
       OFD_HIP(hipEventRecord(c.cyc_ev[cyc & 1], c.stream));
-      rhs = c.d_r;
+      rhs = r;
     }
     if (cyc >= 2) {
       const int p = cyc - 1;  // the cycle whose result is read now
@@ -836,31 +882,40 @@ int ofd_solve(ofd_ctx *x, int precision, int ortho, double rtol, double inner_to
         set_error("solve: non-finite residual");
         break;
       }
-      if (rel <= rtol) {
-        conv = true;
-        break;
-      }
+      if (rel <= rtol) break;
     }
   }
-  cyc = done_cycles;
   // the queued cycle behind the last one read is a no-op or its result is
   // discarded with the error; drain it before touching x
   OFD_HIP(hipStreamSynchronize(c.stream));
-  (void)conv;
   const int prc = probe_harvest(c);
   if (!rc) rc = prc;
-  if (cyc > max_cycles) cyc = max_cycles;
   if (iters) *iters = total;
-  if (cycles) *cycles = cyc;
+  if (cycles) *cycles = done_cycles;
   if (relres) *relres = rel;
   if (rc) return rc;
-  OFD_HIP(launch_node4_to_soa(c, c.d_x, c.d_field[OFD_SOLU], c.d_field[OFD_SOLP]));
-  OFD_HIP(hipStreamSynchronize(c.stream));
   if (rel > rtol) {
     set_error("solve: tolerance not reached within max_cycles");
     return OFD_ERR_NOCONV;
   }
   return OFD_OK;
+}
+
+int ofd_solve(ofd_ctx *x, int precision, int ortho, double rtol, double inner_tol,
+              int kspace, int max_cycles, int *iters, int *cycles, double *relres) {
+  OFD_CHECK(x, OFD_ERR_ARG, "null context");
+  OFD_CHECK(precision == OFD_FP64 || precision == OFD_FP32, OFD_ERR_ARG, "bad precision");
+  OFD_CHECK(max_cycles >= 1, OFD_ERR_ARG, "max_cycles >= 1");
+  Ctx &c = x->c;
+  int rc = need_pattern(c);
+  if (rc) return rc;
+  OFD_HIP(hipSetDevice(c.device));
+  rc = solve_core(c, precision, ortho, c.d_rhs, c.d_x, c.d_r, rtol, inner_tol, kspace,
+                  max_cycles, iters, cycles, relres);
+  if (rc && rc != OFD_ERR_NOCONV) return rc;
+  OFD_HIP(launch_node4_to_soa(c, c.d_x, c.d_field[OFD_SOLU], c.d_field[OFD_SOLP]));
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  return rc;
 }
 
 int ofd_solve_history(ofd_ctx *x, double *relres, int maxn, int *n) {
@@ -889,6 +944,11 @@ int ofd_predictor(ofd_ctx *x) {
   if (rc) return rc;
   OFD_HIP(hipSetDevice(c.device));
   OFD_HIP(launch_axpy_fields(c, 1));
+  if (c.quench) {
+    OFD_HIP(launch_fields_s(c, 1, nullptr, OFD_TG, OFD_TGOLD, OFD_RTG, OFD_RTGOLD));
+    OFD_HIP(launch_fields_s(c, 1, nullptr, OFD_PHIG, OFD_PHIGOLD, OFD_RPHIG,
+                            OFD_RPHIGOLD));
+  }
   OFD_HIP(hipStreamSynchronize(c.stream));
   return OFD_OK;
 }
@@ -900,6 +960,11 @@ int ofd_advance(ofd_ctx *x) {
   if (rc) return rc;
   OFD_HIP(hipSetDevice(c.device));
   OFD_HIP(launch_axpy_fields(c, 2));
+  if (c.quench) {
+    OFD_HIP(launch_fields_s(c, 2, nullptr, OFD_TG, OFD_TGOLD, OFD_RTG, OFD_RTGOLD));
+    OFD_HIP(launch_fields_s(c, 2, nullptr, OFD_PHIG, OFD_PHIGOLD, OFD_RPHIG,
+                            OFD_RPHIGOLD));
+  }
   OFD_HIP(hipStreamSynchronize(c.stream));
   return OFD_OK;
 }
@@ -978,6 +1043,215 @@ int ofd_time_assemble(ofd_ctx *x, int tensor_flag, int reps, double *ms) {
   }
   *ms = (double)t / reps;
   return OFD_OK;
+}
+
+// ---- temperature system ----
+
+// LHSTem for the current pattern and the zero-padded T vectors
+static int ensure_tem(Ctx &c) {
+  int rc = need_pattern(c);
+  if (rc) return rc;
+  const int n4 = (c.nnode + 3) / 4;
+  if (c.temT_icnt != c.icnt) {
+    rc = dalloc(c, &c.d_AT, (size_t)c.icnt);
+    if (!rc) rc = dalloc(c, &c.d_AT32, (size_t)0);
+    if (!rc) rc = dalloc(c, &c.d_ApT, (size_t)0);
+    if (!rc) rc = dalloc(c, &c.d_ApT32, (size_t)0);
+    if (rc) return rc;
+    c.temT_icnt = c.icnt;
+    c.matT_valid = c.precT_valid = c.aT32_valid = c.apT32_valid = false;
+  }
+  if (c.temT_n4 != n4) {
+    const size_t np = (size_t)n4 * 4;
+    rc = dalloc(c, &c.d_rhsT, np);
+    if (!rc) rc = dalloc(c, &c.d_xT, np);
+    if (!rc) rc = dalloc(c, &c.d_rT, np);
+    if (!rc) rc = dalloc(c, &c.d_yT, np);
+    if (!rc) rc = dalloc(c, &c.d_PT, (size_t)c.nnode);
+    if (rc) return rc;
+    OFD_HIP(hipMemsetAsync(c.d_rhsT, 0, np * sizeof(double), c.stream));
+    OFD_HIP(hipMemsetAsync(c.d_xT, 0, np * sizeof(double), c.stream));
+    OFD_HIP(hipMemsetAsync(c.d_rT, 0, np * sizeof(double), c.stream));
+    OFD_HIP(hipMemsetAsync(c.d_yT, 0, np * sizeof(double), c.stream));
+    c.temT_n4 = n4;
+  }
+  return OFD_OK;
+}
+
+static int ensure_aT32(Ctx &c) {
+  OFD_CHECK(c.matT_valid, OFD_ERR_ARG, "no assembled temperature matrix");
+  if (c.aT32_valid) return OFD_OK;
+  if (!c.d_AT32) {
+    int rc = dalloc(c, &c.d_AT32, (size_t)c.icnt);
+    if (rc) return rc;
+  }
+  OFD_HIP(launch_convert_aT32(c));
+  c.aT32_valid = true;
+  return OFD_OK;
+}
+
+int ofd_set_quench_params(ofd_ctx *x, const double *q) {
+  OFD_CHECK(x && q, OFD_ERR_ARG, "null argument");
+  Ctx &c = x->c;
+  for (int i = 0; i < OFD_NQPARAMS; ++i) c.qprm[i] = q[i];
+  OFD_CHECK(c.qprm[OFD_Q_TS] != 0.0, OFD_ERR_ARG, "T_SAT must be nonzero");
+  c.quench = true;
+  return OFD_OK;
+}
+
+int ofd_assemble_tem(ofd_ctx *x, int tensor_flag) {
+  OFD_CHECK(x, OFD_ERR_ARG, "null context");
+  Ctx &c = x->c;
+  int rc = ensure_tem(c);
+  if (rc) return rc;
+  OFD_CHECK(c.quench, OFD_ERR_ARG, "no quench parameters (ofd_set_quench_params)");
+  OFD_CHECK(c.nshl == 8, OFD_ERR_UNSUPPORTED,
+            "the GPU element assembly is implemented for trilinear hexes (NSHL=8)");
+  OFD_CHECK((tensor_flag & ~(OFD_ASSEMBLE_VEC | OFD_ASSEMBLE_MAT)) == 0, OFD_ERR_ARG,
+            "tensor_flag must combine OFD_ASSEMBLE_VEC / OFD_ASSEMBLE_MAT");
+  OFD_HIP(hipSetDevice(c.device));
+  OFD_HIP(launch_alpha_stage(c));
+  if (tensor_flag & OFD_ASSEMBLE_VEC)
+    OFD_HIP(launch_zero(c, c.d_rhsT, (size_t)c.temT_n4 * 4));
+  if (tensor_flag & OFD_ASSEMBLE_MAT) {
+    OFD_HIP(launch_zero(c, c.d_AT, (size_t)c.icnt));
+    c.matT_valid = true;
+    c.precT_valid = c.aT32_valid = c.apT32_valid = false;
+  }
+  OFD_HIP(hipMemsetAsync(c.d_negdet, 0, sizeof(int32_t), c.stream));
+  for (int col = 0; col + 1 < (int)c.color_off.size(); ++col)
+    OFD_HIP(launch_assemble_tem_hex(c, tensor_flag, col));
+  int32_t neg = 0;
+  OFD_HIP(hipMemcpyAsync(&neg, c.d_negdet, sizeof(int32_t), hipMemcpyDeviceToHost,
+                         c.stream));
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  if (neg > 0) {
+    set_error("negative determinant Jacobian at " + std::to_string(neg) +
+              " Gauss points (eval_shape_fem warning)");
+    return OFD_ERR_NEGDETJ;
+  }
+  return OFD_OK;
+}
+
+int ofd_get_tem_system(ofd_ctx *x, double *lhs, double *rhs) {
+  OFD_CHECK(x, OFD_ERR_ARG, "null context");
+  Ctx &c = x->c;
+  int rc = ensure_tem(c);
+  if (rc) return rc;
+  OFD_HIP(hipSetDevice(c.device));
+  if (lhs) rc = d2h(c, lhs, c.d_AT, (size_t)c.icnt * sizeof(double));
+  if (!rc && rhs) rc = d2h(c, rhs, c.d_rhsT, (size_t)c.nnode * sizeof(double));
+  return rc;
+}
+
+int ofd_set_tem_system(ofd_ctx *x, const double *lhs, const double *rhs) {
+  OFD_CHECK(x, OFD_ERR_ARG, "null context");
+  Ctx &c = x->c;
+  int rc = ensure_tem(c);
+  if (rc) return rc;
+  OFD_HIP(hipSetDevice(c.device));
+  if (lhs) {
+    rc = h2d(c, c.d_AT, lhs, (size_t)c.icnt * sizeof(double));
+    if (rc) return rc;
+    c.matT_valid = true;
+    c.precT_valid = c.aT32_valid = c.apT32_valid = false;
+  }
+  if (rhs) rc = h2d(c, c.d_rhsT, rhs, (size_t)c.nnode * sizeof(double));
+  return rc;
+}
+
+int ofd_spmv_tem(ofd_ctx *x, int precision, const double *xin, double *y) {
+  OFD_CHECK(x && xin && y, OFD_ERR_ARG, "null argument");
+  OFD_CHECK(precision == OFD_FP64 || precision == OFD_FP32, OFD_ERR_ARG, "bad precision");
+  Ctx &c = x->c;
+  int rc = ensure_tem(c);
+  if (rc) return rc;
+  OFD_CHECK(c.matT_valid, OFD_ERR_ARG, "no assembled temperature matrix");
+  OFD_HIP(hipSetDevice(c.device));
+  if (precision == OFD_FP32) {
+    rc = ensure_aT32(c);
+    if (rc) return rc;
+  }
+  TemSystem sys(c);
+  rc = h2d(c, c.d_yT, xin, (size_t)c.nnode * sizeof(double));
+  if (rc) return rc;
+  OFD_HIP(launch_spmv_s(c, precision, c.d_yT, c.d_rT, nullptr, 0));
+  return d2h(c, y, c.d_rT, (size_t)c.nnode * sizeof(double));
+}
+
+int ofd_gmres_tem(ofd_ctx *x, int precision, int ortho, double tol, int kspace,
+                  int kspace_min, double *sol, int *iters, double *resid) {
+  OFD_CHECK(x, OFD_ERR_ARG, "null context");
+  OFD_CHECK(precision == OFD_FP64 || precision == OFD_FP32, OFD_ERR_ARG, "bad precision");
+  OFD_CHECK(ortho == OFD_ORTHO_MGS || ortho == OFD_ORTHO_CGS2 || ortho == OFD_ORTHO_CGS1,
+            OFD_ERR_ARG, "bad ortho");
+  Ctx &c = x->c;
+  int rc = ensure_tem(c);
+  if (rc) return rc;
+  OFD_HIP(hipSetDevice(c.device));
+  TemSystem sys(c);
+  OFD_HIP(hipMemsetAsync(c.d_xT, 0, (size_t)c.gm_n * 4 * sizeof(double), c.stream));
+  // |b| = 0: the reference returns sol = 0 (SparseGMRES_tem.f90:68-71)
+  double bn = 0.0;
+  rc = norm2_dev(c, c.d_rhsT, &bn);
+  if (rc) return rc;
+  int it = 0;
+  double res = 0.0;
+  if (bn > 0.0) {
+    OFD_HIP(launch_outer_reset(c, 0.0));
+    rc = gmres_cycle(c, precision, ortho, c.d_rhsT, tol, kspace, kspace_min, c.d_xT, &it,
+                     &res, nullptr);
+  }
+  if (iters) *iters = it;
+  if (resid) *resid = res;
+  if (rc) return rc;
+  OFD_HIP(hipMemcpyAsync(c.d_field[OFD_SOLT], c.d_xT, (size_t)c.nnode * sizeof(double),
+                         hipMemcpyDeviceToDevice, c.stream));
+  if (sol) rc = d2h(c, sol, c.d_xT, (size_t)c.nnode * sizeof(double));
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  return rc;
+}
+
+int ofd_solve_tem(ofd_ctx *x, int precision, int ortho, double rtol, double inner_tol,
+                  int kspace, int max_cycles, int *iters, int *cycles, double *relres) {
+  OFD_CHECK(x, OFD_ERR_ARG, "null context");
+  OFD_CHECK(precision == OFD_FP64 || precision == OFD_FP32, OFD_ERR_ARG, "bad precision");
+  OFD_CHECK(max_cycles >= 1, OFD_ERR_ARG, "max_cycles >= 1");
+  Ctx &c = x->c;
+  int rc = ensure_tem(c);
+  if (rc) return rc;
+  OFD_CHECK(c.matT_valid, OFD_ERR_ARG, "no assembled temperature matrix");
+  OFD_HIP(hipSetDevice(c.device));
+  TemSystem sys(c);
+  rc = solve_core(c, precision, ortho, c.d_rhsT, c.d_xT, c.d_rT, rtol, inner_tol, kspace,
+                  max_cycles, iters, cycles, relres);
+  if (rc && rc != OFD_ERR_NOCONV) return rc;
+  OFD_HIP(hipMemcpyAsync(c.d_field[OFD_SOLT], c.d_xT, (size_t)c.nnode * sizeof(double),
+                         hipMemcpyDeviceToDevice, c.stream));
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  return rc;
+}
+
+int ofd_tem_update(ofd_ctx *x) {
+  OFD_CHECK(x, OFD_ERR_ARG, "null context");
+  Ctx &c = x->c;
+  int rc = need_mesh(c);
+  if (rc) return rc;
+  OFD_HIP(hipSetDevice(c.device));
+  OFD_HIP(launch_fields_s(c, 0, c.d_field[OFD_SOLT], OFD_TG, OFD_TGOLD, OFD_RTG,
+                          OFD_RTGOLD));
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  return OFD_OK;
+}
+
+int ofd_tem_rhs_norm(ofd_ctx *x, double *norm2) {
+  OFD_CHECK(x && norm2, OFD_ERR_ARG, "null argument");
+  Ctx &c = x->c;
+  int rc = ensure_tem(c);
+  if (rc) return rc;
+  OFD_HIP(hipSetDevice(c.device));
+  TemSystem sys(c);
+  return norm2_dev(c, c.d_rhsT, norm2);
 }
 
 static int ensure_fringe(Ctx &c, int nrecv) {

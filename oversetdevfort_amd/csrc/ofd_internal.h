@@ -126,6 +126,24 @@ struct Ctx {
   size_t h_pin_bytes = 0;
   int64_t dev_bytes = 0;
   std::vector<double> solve_hist;     // relres after each cycle of the last solve
+  // quenching temperature system (LHSTem/RHSGTem): scalar per nonzero / node;
+  // vectors zero-padded to whole 4-tuples for the GMRES kernels
+  double qprm[OFD_NQPARAMS] = {0};
+  bool quench = false;                // ofd_set_quench_params was called
+  double *d_AT = nullptr;             // [icnt] LHSTem
+  float *d_AT32 = nullptr;            // its FP32 copy (plain SpMV)
+  double *d_PT = nullptr;             // [N] inverse diagonal, 0 -> 1
+  double *d_ApT = nullptr;            // [icnt] diag(P)^-1 LHSTem
+  float *d_ApT32 = nullptr;
+  bool matT_valid = false, precT_valid = false, aT32_valid = false, apT32_valid = false;
+  double *d_rhsT = nullptr, *d_xT = nullptr, *d_rT = nullptr;  // [4 * ceil(N/4)]
+  double *d_yT = nullptr;             // [4 * ceil(N/4)] product input scratch
+  int64_t temT_icnt = 0;              // pattern size LHSTem was allocated for
+  int temT_n4 = 0;                    // 4-tuples the T vectors were allocated for
+  // the system the GMRES kernels run on: the NS node vectors [N][4]
+  // (gm_n = N) or the T vector as ceil(N/4) 4-tuples (gm_scalar)
+  int gm_n = 0;
+  bool gm_scalar = false;
   // overset fringe: per-element boxes, source flags and geometry; receivers
   int fr_alloc_e = 0, fr_alloc_r = 0;
   double *d_fr_box = nullptr;         // [E][6] widened bounding box
@@ -166,6 +184,8 @@ hipError_t launch_soa_to_node4(Ctx &c, const double *u, const double *p,
                                double *out);
 hipError_t launch_node4_to_soa(Ctx &c, const double *in, double *u, double *p);
 hipError_t launch_axpy_fields(Ctx &c, int which);
+hipError_t launch_fields_s(Ctx &c, int which, const double *x, int fa, int faold, int fra,
+                           int fraold);
 hipError_t launch_apply_dirichlet(Ctx &c);
 hipError_t launch_copy(Ctx &c, const void *src, void *dst, size_t nbytes);
 // overset fringe
@@ -180,6 +200,13 @@ hipError_t launch_gmres_finish(Ctx &c, double *x);
 hipError_t launch_norm2(Ctx &c, const double *v, int slot, int outer = 0);
 hipError_t launch_outer_reset(Ctx &c, double thr);
 hipError_t launch_norm2_up(Ctx &c, const double *v, int slot);
+int nblocks_gm(const Ctx &c);
+// temperature system
+hipError_t launch_assemble_tem_hex(Ctx &c, int tensor_flag, int color);
+hipError_t launch_spmv_s(Ctx &c, int precision, const double *x, double *y,
+                         const double *b, int mode);
+hipError_t launch_build_prec_s(Ctx &c, bool want32);
+hipError_t launch_convert_aT32(Ctx &c);
 
 int nblocks_nodes(const Ctx &c);
 

@@ -109,6 +109,66 @@ k_spmv(int N, const int32_t *__restrict__ col, const int32_t *__restrict__ row,
     y[(size_t)r * 4 + c] = (V)acc;
 }
 
+// Scalar CSR product of the temperature system (SparseProd_LS,
+// SparseProd.f90:128-161): mode 0 y = A x, 1 y = b - A x, 2 the Arnoldi
+// product (returns at once once the cycle has converged).  4 lanes per row
+// stride the row's nonzeros and are added in a fixed shuffle order; rows
+// n .. npad-1 (the zero padding to whole 4-tuples) are written as 0.  The
+// same XCD-aware block order as the block product.
+template <typename T, typename V, int MODE>
+__global__ void __launch_bounds__(256)
+k_spmv_s(int n, int npad, const int32_t *__restrict__ col, const int32_t *__restrict__ row,
+         const T *__restrict__ A, const V *__restrict__ x, V *__restrict__ y,
+         const double *__restrict__ b, const GmresState *__restrict__ gs) {
+  if (MODE == 2 && gs->done) return;
+  if (MODE == 1 && gs && gs->outer_done) return;
+  const int per = gridDim.x >> 3;
+  const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  const int gt = lb * blockDim.x + threadIdx.x;
+  const int r = gt >> 2;
+  const int c = gt & 3;
+  if (r >= npad) return;  // uniform over each group of 4 lanes
+  double acc = 0.0;
+  if (r < n) {
+    const int k1 = col[r + 1];
+    for (int k = col[r] + c; k < k1; k += 4) acc += (double)A[k] * (double)x[row[k]];
+  }
+  acc += __shfl_xor(acc, 1);
+  acc += __shfl_xor(acc, 2);
+  if (c == 0) {
+    if (r >= n)
+      y[r] = (V)0.0;
+    else if (MODE == 1)
+      y[r] = (V)(b[r] - acc);
+    else
+      y[r] = (V)acc;
+  }
+}
+
+// inverse diagonal of the temperature matrix, 0 -> 1 (SparseGMRES_tem.f90:
+// 38-59), and the preconditioned copy Ap[k] = dinv_i A[k] (FP64, FP32)
+__global__ void k_build_prec_s(int N, const int32_t *__restrict__ col,
+                               const int32_t *__restrict__ diag, const double *__restrict__ A,
+                               double *__restrict__ P, double *__restrict__ Ap,
+                               float *__restrict__ Ap32) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double d = A[diag[i]];
+  const double di = (d != 0.0) ? 1.0 / d : 1.0;
+  P[i] = di;
+  for (int k = col[i]; k < col[i + 1]; ++k) {
+    const double v = di * A[k];
+    Ap[k] = v;
+    if (Ap32) Ap32[k] = (float)v;
+  }
+}
+
+__global__ void k_convert_s32(size_t n, const double *__restrict__ A, float *__restrict__ A32) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) A32[i] = (float)A[i];
+}
+
 // block-Jacobi inverse: 3x3 cofactor inverse of the K11 diagonal block and
 // 1/M (0 -> 1), SparseGMRES_NS.f90:73-121
 __global__ void k_build_prec(int N, const int32_t *__restrict__ diag,
@@ -243,9 +303,79 @@ hipError_t launch_spmv(Ctx &c, int precision, const double *x, double *y,
 
 // Arnoldi product W_{k+1} = (P^-1 A) W_k, unscaled (FP32 basis in the mixed
 // CGS solver)
+// scalar rows, 4 lanes each, over the padded length, rounded to 8 blocks
+static inline unsigned spmv_s_blocks(const Ctx &c) {
+  const unsigned nb = (unsigned)(((size_t)c.gm_n * 16 + 255) / 256);
+  return (nb + 7u) & ~7u;
+}
+
+hipError_t launch_spmv_s(Ctx &c, int precision, const double *x, double *y,
+                         const double *b, int mode) {
+  const unsigned nb = spmv_s_blocks(c);
+  const int n = c.nnode, np = c.gm_n * 4;
+  if (precision == OFD_FP32) {
+    if (mode == 0)
+      hipLaunchKernelGGL((k_spmv_s<float, double, 0>), dim3(nb), dim3(256), 0, c.stream, n,
+                         np, c.d_col, c.d_row, c.d_AT32, x, y, b, c.d_gs);
+    else
+      hipLaunchKernelGGL((k_spmv_s<float, double, 1>), dim3(nb), dim3(256), 0, c.stream, n,
+                         np, c.d_col, c.d_row, c.d_AT32, x, y, b, c.d_gs);
+  } else {
+    if (mode == 0)
+      hipLaunchKernelGGL((k_spmv_s<double, double, 0>), dim3(nb), dim3(256), 0, c.stream, n,
+                         np, c.d_col, c.d_row, c.d_AT, x, y, b, c.d_gs);
+    else
+      hipLaunchKernelGGL((k_spmv_s<double, double, 1>), dim3(nb), dim3(256), 0, c.stream, n,
+                         np, c.d_col, c.d_row, c.d_AT, x, y, b, c.d_gs);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_build_prec_s(Ctx &c, bool want32) {
+  const int N = c.nnode;
+  hipLaunchKernelGGL(k_build_prec_s, dim3((N + 255) / 256), dim3(256), 0, c.stream, N,
+                     c.d_col, c.d_diag, c.d_AT, c.d_PT, c.d_ApT, want32 ? c.d_ApT32 : nullptr);
+  return hipGetLastError();
+}
+
+hipError_t launch_convert_aT32(Ctx &c) {
+  hipLaunchKernelGGL(k_convert_s32, dim3(2048), dim3(256), 0, c.stream, (size_t)c.icnt,
+                     c.d_AT, c.d_AT32);
+  return hipGetLastError();
+}
+
+// Arnoldi product of the temperature system: W_{k+1} = (D^-1 A) W_k
+static hipError_t launch_spmv_arnoldi_s(Ctx &c, int precision, int k, hipEvent_t ev0,
+                                        hipEvent_t ev1) {
+  const unsigned nb = spmv_s_blocks(c);
+  const int n = c.nnode, np = c.gm_n * 4;
+  const size_t vs = (size_t)c.gm_n * 4;
+  if (c.basis32) {
+    hipExtLaunchKernelGGL((k_spmv_s<float, float, 2>), dim3(nb), dim3(256), 0, c.stream, ev0,
+                          ev1, 0, n, np, (const int32_t *)c.d_col, (const int32_t *)c.d_row,
+                          (const float *)c.d_ApT32, (const float *)(c.d_W32 + vs * k),
+                          c.d_W32 + vs * (k + 1), (const double *)nullptr,
+                          (const GmresState *)c.d_gs);
+  } else if (precision == OFD_FP32) {
+    hipExtLaunchKernelGGL((k_spmv_s<float, double, 2>), dim3(nb), dim3(256), 0, c.stream,
+                          ev0, ev1, 0, n, np, (const int32_t *)c.d_col,
+                          (const int32_t *)c.d_row, (const float *)c.d_ApT32,
+                          (const double *)(c.d_W + vs * k), c.d_W + vs * (k + 1),
+                          (const double *)nullptr, (const GmresState *)c.d_gs);
+  } else {
+    hipExtLaunchKernelGGL((k_spmv_s<double, double, 2>), dim3(nb), dim3(256), 0, c.stream,
+                          ev0, ev1, 0, n, np, (const int32_t *)c.d_col,
+                          (const int32_t *)c.d_row, (const double *)c.d_ApT,
+                          (const double *)(c.d_W + vs * k), c.d_W + vs * (k + 1),
+                          (const double *)nullptr, (const GmresState *)c.d_gs);
+  }
+  return hipGetLastError();
+}
+
 // ev0/ev1 (may be null) receive the kernel's own start and end timestamps
 hipError_t launch_spmv_arnoldi(Ctx &c, int precision, int k, hipEvent_t ev0,
                                hipEvent_t ev1) {
+  if (c.gm_scalar) return launch_spmv_arnoldi_s(c, precision, k, ev0, ev1);
   const unsigned nb = spmv_blocks(c);
   const size_t vs = (size_t)c.nnode * 4;
   if (c.basis32) {

@@ -135,6 +135,36 @@ hipError_t launch_node4_to_soa(Ctx &c, const double *in, double *u, double *p) {
   return hipGetLastError();
 }
 
+// scalar field pair (a, ra) = (Tg, rTg) or (phig, rphig): which 0 Newton
+// update ra += x, a += gami*Delt*x (solveFlow.f90:139-140); 1 predictor
+// a = aold, ra = (gami-1)/gami raold; 2 old <- new
+__global__ void k_fields_s(int N, int which, double gami, double delt,
+                           const double *__restrict__ x, double *a, double *aold, double *ra,
+                           double *raold) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  if (which == 0) {
+    const double d = x[i];
+    ra[i] = ra[i] + d;
+    a[i] = a[i] + gami * delt * d;
+  } else if (which == 1) {
+    a[i] = aold[i];
+    ra[i] = (gami - 1.0) / gami * raold[i];
+  } else {
+    aold[i] = a[i];
+    raold[i] = ra[i];
+  }
+}
+
+hipError_t launch_fields_s(Ctx &c, int which, const double *x, int fa, int faold, int fra,
+                           int fraold) {
+  double **f = c.d_field;
+  hipLaunchKernelGGL(k_fields_s, dim3((c.nnode + 255) / 256), dim3(256), 0, c.stream, c.nnode,
+                     which, c.prm[OFD_P_GAMI], c.prm[OFD_P_DELT], x, f[fa], f[faold], f[fra],
+                     f[fraold]);
+  return hipGetLastError();
+}
+
 hipError_t launch_axpy_fields(Ctx &c, int which) {
   double **f = c.d_field;
   hipLaunchKernelGGL(k_fields, dim3((c.nnode + 255) / 256), dim3(256), 0, c.stream,

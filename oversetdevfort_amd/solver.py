@@ -92,9 +92,11 @@ class Context:
         L.check(self.lib.ofd_set_subdomains(self.h, int(id1), int(id2)))
 
     def set_ibc(self, ibc):
-        """IBC (NNODE, >=4) int, columns u,v,w,p."""
-        a = np.ascontiguousarray(np.asarray(ibc, np.int32)[:, :4].T)
-        L.check(self.lib.ofd_set_ibc(self.h, _ptr(a), 4))
+        """IBC (NNODE, 4..8) int: columns u,v,w,p[,.,T,..] (column 6 = T)."""
+        b = np.asarray(ibc, np.int32)
+        nc = min(b.shape[1], 8)
+        a = np.ascontiguousarray(b[:, :nc].T)
+        L.check(self.lib.ofd_set_ibc(self.h, _ptr(a), nc))
 
     def set_bcvals(self, vals):
         a = _soa(vals)
@@ -110,7 +112,7 @@ class Context:
 
     def get_field(self, name):
         name = name.upper()
-        if name in ("PG", "PGOLD", "SOLP"):
+        if name in L.SCALAR_FIELDS:
             out = np.zeros(self.nnode)
             L.check(self.lib.ofd_get_field(self.h, L.FIELD[name], _ptr(out)))
             return out
@@ -207,6 +209,69 @@ class Context:
         out = (ctypes.c_double * 2)()
         L.check(self.lib.ofd_rhs_norms(self.h, out))
         return float(out[0]), float(out[1])
+
+    # ---- temperature equation of the quenching solver (solveQuenching.f90) ----
+    def set_quench_params(self, q):
+        """q: dict with the keys of _lib.QPARAMS (fluid_id defaults to 102)."""
+        d = dict(q)
+        d.setdefault("fluid_id", 102)
+        arr = np.array([float(d[k]) for k in L.QPARAMS], dtype=np.float64)
+        L.check(self.lib.ofd_set_quench_params(self.h, _ptr(arr)))
+
+    def assemble_tem(self, tensor_flag=6):
+        """IntElmAss_Tem_Quenching_STAB + BCLhs_tem + FillSparseMat_tem."""
+        L.check(self.lib.ofd_assemble_tem(self.h, int(tensor_flag)))
+
+    def get_tem_system(self):
+        lhs = np.zeros(self.icnt)
+        rhs = np.zeros(self.nnode)
+        L.check(self.lib.ofd_get_tem_system(self.h, _ptr(lhs), _ptr(rhs)))
+        return lhs, rhs
+
+    def set_tem_system(self, lhs=None, rhs=None):
+        lp = None if lhs is None else _f64(lhs)
+        rp = None if rhs is None else _f64(rhs)
+        L.check(self.lib.ofd_set_tem_system(self.h, None if lp is None else _ptr(lp),
+                                            None if rp is None else _ptr(rp)))
+
+    def SparseProd_LS(self, x, precision=L.OFD_FP64):
+        x_ = _f64(x)
+        y = np.zeros(self.nnode)
+        L.check(self.lib.ofd_spmv_tem(self.h, precision, _ptr(x_), _ptr(y)))
+        return y
+
+    def SparseGMRES_tem(self, tol, kspace, kspace_min=0, precision=L.OFD_FP64,
+                        ortho=L.OFD_ORTHO_MGS):
+        sol = np.zeros(self.nnode)
+        it = ctypes.c_int()
+        res = ctypes.c_double()
+        L.check(self.lib.ofd_gmres_tem(self.h, precision, ortho, float(tol), int(kspace),
+                                       int(kspace_min), _ptr(sol), ctypes.byref(it),
+                                       ctypes.byref(res)))
+        return dict(sol=sol, iters=it.value, resid=res.value)
+
+    def solve_tem(self, rtol=1e-8, precision=L.OFD_FP64, ortho=L.OFD_ORTHO_CGS1,
+                  inner_tol=None, kspace=30, max_cycles=50, raise_noconv=True):
+        if inner_tol is None:
+            inner_tol = 1e-4 if precision == L.OFD_FP32 else 1e-2
+        it = ctypes.c_int()
+        cyc = ctypes.c_int()
+        rel = ctypes.c_double()
+        rc = self.lib.ofd_solve_tem(self.h, precision, ortho, float(rtol), float(inner_tol),
+                                    int(kspace), int(max_cycles), ctypes.byref(it),
+                                    ctypes.byref(cyc), ctypes.byref(rel))
+        if rc != L.OFD_OK and (raise_noconv or rc != 5):
+            L.check(rc)
+        return dict(iters=it.value, cycles=cyc.value, relres=rel.value,
+                    converged=rc == L.OFD_OK)
+
+    def tem_update(self):
+        L.check(self.lib.ofd_tem_update(self.h))
+
+    def tem_rhs_norm(self):
+        out = ctypes.c_double()
+        L.check(self.lib.ofd_tem_rhs_norm(self.h, ctypes.byref(out)))
+        return out.value
 
     # ---- overset fringe (interpolation.f90:5-76, setBCs.f90:124-167) ----
     def interpolation(self, src_id, xr, dr=None):

@@ -176,6 +176,56 @@ def ns_case(name, dims, seed, tensor_flag, subdomain_flag, tol, kspace, kmin,
         **{"st_" + k: v for k, v in st.items()})
 
 
+# quenching material data: air/vapour (a), water (w), solid (s)
+QPRM = np.array([1.0, 1000.0, 1.0e-5, 1.0e-3, 2000.0, 4200.0, 0.025, 0.6, 2700.0, 0.0,
+                 900.0, 200.0, 373.0, 2.26e6, 0.1, 0.1, 1.0, 0.0, 425.0])
+
+
+def tem_case(name, dims, seed, tensor_flag, tol, kspace, kmin, tem_kdc=0.0,
+             frac_ibc=0.2):
+    """IntElmAss_Tem_Quenching_STAB + BCLhs_tem + FillSparseMat_tem on a tet
+    mesh with fluid (ELM_ID 102) and solid (101) elements, VOF values beyond
+    [0, 1] (clipped by the kernel) and temperatures on both sides of T_sat;
+    then SparseProd_LS and SparseGMRES_tem on the assembled system."""
+    rng = np.random.default_rng(seed)
+    hexm = M.structured_hex(*dims, lo=(0, 0, 0), hi=(0.01, 0.012, 0.009), jitter=0.05,
+                            seed=seed)
+    tm = M.hex_to_tets(hexm)
+    N, E, nshl = tm.nnode, tm.nelem, tm.nshl
+    elm_id = np.where(rng.uniform(size=E) < 0.7, 102, 101).astype(np.int32)
+    ibc = np.zeros((N, 8), np.int32)
+    ibc[:, 5] = (rng.uniform(size=N) < frac_ibc).astype(np.int32)
+    st = {"dg": 1e-5 * rng.normal(size=(N, 3)), "ug": 0.3 * rng.normal(size=(N, 3)),
+          "ugm": 0.05 * rng.normal(size=(N, 3)), "phig": rng.uniform(-0.2, 1.2, size=N),
+          "Tg": rng.uniform(300.0, 900.0, size=N), "rTg": 50.0 * rng.normal(size=N)}
+    prm = np.array([0.8333333333333334, 0.6666666666666666, 0.6666666666666667, 1e-3])
+    qprm = QPRM.copy()
+    qprm[17] = tem_kdc
+    x = rng.normal(size=N)
+    payload = (np.int32(4).tobytes() +
+               np.array([N, E, nshl, nshl, tensor_flag], np.int32).tobytes() +
+               _i32(tm.ien + 1) + _i32(elm_id) + _i32(ibc) + _f64(tm.xg) +
+               _f64(st["dg"]) + _f64(st["ug"]) + _f64(st["ugm"]) + _f64(st["phig"]) +
+               _f64(st["Tg"]) + _f64(st["rTg"]) + _f64(prm) + _f64(qprm) +
+               np.float64(tol).tobytes() + np.array([kspace, kmin], np.int32).tobytes() +
+               _f64(x))
+    out, _ = _run(payload)
+    r = _Reader(out)
+    icnt = int(r.i32(1)[0])
+    col = r.i32(N + 1)
+    row = r.i32(icnt)
+    lhs = r.f64(icnt)
+    rhs = r.f64(N)
+    y = r.f64(N)
+    sol = r.f64(N)
+    assert r.off == len(out)
+    np.savez_compressed(
+        os.path.join(HERE, f"{name}.npz"), xg=tm.xg, ien=tm.ien, elm_id=elm_id,
+        ibc_t=ibc[:, 5], prm=prm, qprm=qprm, tensor_flag=tensor_flag, tol=tol,
+        kspace=kspace, kmin=kmin, x=x, col=col - 1, row=row - 1, lhs=lhs, rhs=rhs,
+        y=y, sol=sol, **{"st_" + k: v for k, v in st.items()})
+
+
 def make_interp(seed=21):
     rng = np.random.default_rng(seed)
     hexm = M.structured_hex(3, 3, 2, lo=(0, 0, 0), hi=(1.0, 1.0, 0.7), jitter=0.04,
@@ -227,9 +277,20 @@ def make_interp(seed=21):
                         val=val, donor=donor)
 
 
+def make_tem():
+    # fixed-length GMRES cycle, no discontinuity capturing
+    tem_case("tem_tet_a", (2, 2, 2), seed=31, tensor_flag=6, tol=1e-30, kspace=20, kmin=0)
+    # with discontinuity capturing, tolerance-terminated GMRES
+    tem_case("tem_tet_b", (3, 2, 2), seed=32, tensor_flag=6, tol=1e-6, kspace=40, kmin=2,
+             tem_kdc=0.3, frac_ibc=0.1)
+
+
 def main():
     if not os.path.exists(HARNESS):
         raise SystemExit(f"missing {HARNESS}: run make -C oracle ref")
+    if sys.argv[1:] == ["tem"]:
+        make_tem()
+        return
     make_shape()
     # element kernels + BC + scatter + SpMV + a fixed-length GMRES cycle
     ns_case("ns_tet_a", (2, 2, 3), seed=3, tensor_flag=6, subdomain_flag=1,
@@ -241,6 +302,7 @@ def main():
     ns_case("ns_tet_vec", (2, 3, 2), seed=5, tensor_flag=2, subdomain_flag=3,
             tol=1e-3, kspace=0, kmin=0, frac_ibc=0.0, two_ids=False)
     make_interp()
+    make_tem()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

@@ -92,3 +92,35 @@ def test_interp_tet_donors_and_values(oracle_mod):
     assert np.array_equal(donor, z["donor"])
     assert (donor >= 0).sum() > 30
     assert np.abs(val - z["val"]).max() <= 1e-14 * np.abs(z["val"]).max()
+
+
+@pytest.mark.parametrize("name", ["tem_tet_a", "tem_tet_b"])
+def test_tem_assembly_spmv_gmres(oracle_mod, name):
+    """IntElmAss_Tem_Quenching_STAB + BCLhs_tem + FillSparseMat_tem,
+    SparseProd_LS and SparseGMRES_tem against the reference.  Case a is
+    bit-exact; case b turns on discontinuity capturing, whose norm2(Gij) the
+    reference's compiler evaluates to within one rounding of sqrt(sum)."""
+    O = oracle_mod
+    z = golden(name)
+    m = Mesh(xg=z["xg"], ien=z["ien"], elm_id=z["elm_id"])
+    col, row = O.sparsity(m.nnode, m.ien)
+    assert np.array_equal(col, z["col"]) and np.array_equal(row, z["row"])
+    st = {k[3:]: z[k] for k in z.files if k.startswith("st_")}
+    R = O.assemble_tem(m, z["ibc_t"], st, z["prm"], z["qprm"], col, row,
+                       int(z["tensor_flag"]))
+    if name == "tem_tet_a":
+        assert np.array_equal(R["lhs"], z["lhs"]) and np.array_equal(R["rhs"], z["rhs"])
+    else:
+        for k in ("lhs", "rhs"):
+            assert np.abs(R[k] - z[k]).max() <= 1e-15 * np.abs(z[k]).max(), k
+    # Dirichlet rows of T are identity rows with zero RHS
+    rows = np.repeat(np.arange(m.nnode), np.diff(col))
+    fixed = z["ibc_t"] == 1
+    assert np.all(R["rhs"][fixed] == 0.0)
+    off = fixed[rows] & (row != rows)
+    assert np.all(R["lhs"][off] == 0.0)
+    y = O.spmv_s(col, row, z["lhs"], z["x"])
+    assert np.array_equal(y, z["y"])
+    g = O.gmres_tem(col, row, z["lhs"], z["rhs"], float(z["tol"]), int(z["kspace"]),
+                    int(z["kmin"]))
+    assert np.array_equal(g["sol"], z["sol"])
