@@ -1058,6 +1058,7 @@ static int ensure_tem(Ctx &c) {
     if (!rc) rc = dalloc(c, &c.d_ApT, (size_t)0);
     if (!rc) rc = dalloc(c, &c.d_ApT32, (size_t)0);
     if (rc) return rc;
+    OFD_HIP(hipMemsetAsync(c.d_AT, 0, (size_t)c.icnt * sizeof(double), c.stream));
     c.temT_icnt = c.icnt;
     c.matT_valid = c.precT_valid = c.aT32_valid = c.apT32_valid = false;
   }
