@@ -20,8 +20,11 @@ module ofd_iface
                                OFD_UG = 3, OFD_UGOLD = 4, OFD_UGM = 5, &
                                OFD_UGMOLD = 6, OFD_ACG = 7, OFD_ACGOLD = 8, &
                                OFD_ACGM = 9, OFD_ACGMOLD = 10, OFD_PG = 11, &
-                               OFD_PGOLD = 12, OFD_SOLU = 13, OFD_SOLP = 14
-  integer(c_int), parameter :: OFD_NPARAMS = 11
+                               OFD_PGOLD = 12, OFD_SOLU = 13, OFD_SOLP = 14, &
+                               OFD_PHIG = 15, OFD_PHIGOLD = 16, OFD_RPHIG = 17, &
+                               OFD_RPHIGOLD = 18, OFD_TG = 19, OFD_TGOLD = 20, &
+                               OFD_RTG = 21, OFD_RTGOLD = 22, OFD_SOLT = 23
+  integer(c_int), parameter :: OFD_NPARAMS = 11, OFD_NQPARAMS = 20
 
   interface
     function ofd_last_error() bind(C, name='ofd_last_error') result(p)
@@ -203,6 +206,79 @@ module ofd_iface
     function ofd_advance(ctx) bind(C, name='ofd_advance') result(rc)
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+
+    function ofd_set_quench_params(ctx, q) bind(C, name='ofd_set_quench_params') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: q(*)
+      integer(c_int) :: rc
+    end function
+
+    function ofd_assemble_tem(ctx, tensor_flag) bind(C, name='ofd_assemble_tem') result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: tensor_flag
+      integer(c_int) :: rc
+    end function
+
+    function ofd_get_tem_system(ctx, lhs, rhs) bind(C, name='ofd_get_tem_system') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: lhs(*), rhs(*)
+      integer(c_int) :: rc
+    end function
+
+    function ofd_set_tem_system(ctx, lhs, rhs) bind(C, name='ofd_set_tem_system') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: lhs(*), rhs(*)
+      integer(c_int) :: rc
+    end function
+
+    function ofd_spmv_tem(ctx, precision, x, y) bind(C, name='ofd_spmv_tem') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: precision
+      real(c_double), intent(in) :: x(*)
+      real(c_double), intent(out) :: y(*)
+      integer(c_int) :: rc
+    end function
+
+    function ofd_gmres_tem(ctx, precision, ortho, tol, kspace, kspace_min, sol, iters, &
+                           resid) bind(C, name='ofd_gmres_tem') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: precision, ortho, kspace, kspace_min
+      real(c_double), value :: tol
+      real(c_double), intent(out) :: sol(*)
+      integer(c_int), intent(out) :: iters
+      real(c_double), intent(out) :: resid
+      integer(c_int) :: rc
+    end function
+
+    function ofd_solve_tem(ctx, precision, ortho, rtol, inner_tol, kspace, max_cycles, &
+                           iters, cycles, relres) bind(C, name='ofd_solve_tem') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: precision, ortho, kspace, max_cycles
+      real(c_double), value :: rtol, inner_tol
+      integer(c_int), intent(out) :: iters, cycles
+      real(c_double), intent(out) :: relres
+      integer(c_int) :: rc
+    end function
+
+    function ofd_tem_update(ctx) bind(C, name='ofd_tem_update') result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+
+    function ofd_tem_rhs_norm(ctx, norm2) bind(C, name='ofd_tem_rhs_norm') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: norm2
       integer(c_int) :: rc
     end function
 

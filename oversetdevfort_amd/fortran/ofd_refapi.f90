@@ -85,6 +85,34 @@ subroutine SparseGMRES_up(col, row, IBC, IPER, D_FLAG, P_FLAG, &
 end subroutine SparseGMRES_up
 
 !----------------------------------------------------------------------
+! SparseGMRES_tem.f90:1-256: the increment is added to sol (or sol = 0
+! when |rhsG| = 0), as the reference does
+!----------------------------------------------------------------------
+subroutine SparseGMRES_tem(lhsM, Utol, col, row, rhsG, sol, Kspaceu, Kspaceu_mn, &
+                           NNODZu, NSHLu, icntu, NSD)
+  use ofd_state
+  implicit none
+  integer, intent(in) :: NNODZu, NSHLu, icntu, NSD, Kspaceu, Kspaceu_mn
+  integer, intent(in) :: col(NNODZu + 1), row(icntu)
+  real(8), intent(in) :: lhsM(icntu), rhsG(NNODZu), Utol
+  real(8), intent(inout) :: sol(NNODZu)
+  real(8), allocatable :: dT(:)
+  integer(c_int) :: iters
+  real(c_double) :: resid
+  allocate (dT(NNODZu))
+  call ofd_check(ofd_set_tem_system(ofd_ctx, lhsM, rhsG), 'SparseGMRES_tem')
+  call ofd_check(ofd_gmres_tem(ofd_ctx, ofd_precision, ofd_ortho, Utol, &
+                               int(Kspaceu, c_int), int(Kspaceu_mn, c_int), dT, iters, &
+                               resid), 'SparseGMRES_tem')
+  if (iters == 0) then
+    sol = 0.0d0
+  else
+    sol = sol + dT
+  end if
+  deallocate (dT)
+end subroutine SparseGMRES_tem
+
+!----------------------------------------------------------------------
 ! assembleNavSto.f90:4-72 on the device-resident state
 !----------------------------------------------------------------------
 subroutine assembleNavSto(assemble_tensor_flag, subdomain_flag)
