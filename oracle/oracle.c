@@ -1228,3 +1228,503 @@ done:
   free(y);
   return its;
 }
+
+/* ---------------------------------------------------------------------
+ * Coupled NS + VOF assembly of the quenching solver:
+ * IntElmAss_NSVOF_Quenching_STAB (solveQuenching.f90:122-573) with the
+ * 5-field projection BCLhs_NS_conv (BCLhs_3D.f90:278-428; IBC columns
+ * 1-3 = u, 7 = p, 8 = phi) and scatter FillSparseMat_NS_conv
+ * (FillSparseMat.f90:237-288) / LocaltoGlobalNSVOF_3D (LocaltoGlobal.f90:
+ * 26-51).  The NS discontinuity-capturing coefficients are taken as 0 (the
+ * reference's e3DC_beta2 receives them through an integer dummy).
+ * qp as orc_assemble_tem; ex = {gx, gy, gz, lsc_kdc}.
+ * --------------------------------------------------------------------- */
+int orc_element_nsvof(int nshl, int ngauss, const double *xl, const double *dl,
+                      const double *ul, const double *uml, const double *acl,
+                      const double *pl, const double *phil, const double *rphil,
+                      const double *Tl, const double *prm, const double *qp,
+                      const double *ex, int is_fluid, int tensor_flag, double *ke,
+                      double *ge, double *de, double *me, double *lse, double *lsue,
+                      double *ulse, double *plse, double *ru, double *rp, double *rls,
+                      double *cfl) {
+  const double almi = prm[0], alfi = prm[1], gami = prm[2], delt = prm[4];
+  const double rhoa = qp[0], rhow = qp[1], mua = qp[2], muw = qp[3], rhos = qp[8];
+  const double mus = qp[9], Ts = qp[12], c_cond = qp[14], c_evap = qp[15];
+  const double c_dmdot = qp[16];
+  const double grav[3] = {ex[0], ex[1], ex[2]}, lsc_kdc = ex[3];
+  const double fact1 = almi, fact2 = alfi * delt * gami;
+  const double cr = 1 / rhoa - 1 / rhow;
+  double gp[8 * 3], gw[8];
+  int nneg = 0;
+  if (orc_gauss_volume(ngauss, gp, gw) != 0) return -1;
+  const int do_mat = (tensor_flag & 4) != 0, do_vec = (tensor_flag & 2) != 0;
+  const int n = nshl;
+  for (int q = 0; q < ngauss; ++q) {
+    double pt[3] = {gp[0 * ngauss + q], gp[1 * ngauss + q], gp[2 * ngauss + q]};
+    double shlu[8], shg[8 * 3], dxidx[9], Gij[9], Ginv[9], detj;
+    if (orc_eval_shape_fem(n, pt, xl, dl, shlu, shg, dxidx, Gij, Ginv, &detj) > 0) ++nneg;
+    const double dw = detj * gw[q];
+    double ui[3], aci[3], umi[3], du[9], dp[3], dphi[3];
+    double pri = 0.0, phii = 0.0, rphii = 0.0, Ti = 0.0;
+    for (int i = 0; i < 3; ++i) {
+      double su = 0.0, sa = 0.0, sm = 0.0;
+      for (int a = 0; a < n; ++a) su = su + ul[i * n + a] * shlu[a];
+      for (int a = 0; a < n; ++a) sa = sa + acl[i * n + a] * shlu[a];
+      for (int a = 0; a < n; ++a) sm = sm + uml[i * n + a] * shlu[a];
+      ui[i] = su;
+      aci[i] = sa;
+      umi[i] = sm;
+    }
+    for (int a = 0; a < n; ++a) pri = pri + pl[a] * shlu[a];
+    for (int a = 0; a < n; ++a) phii = phii + phil[a] * shlu[a];
+    for (int a = 0; a < n; ++a) rphii = rphii + rphil[a] * shlu[a];
+    for (int a = 0; a < n; ++a) Ti = Ti + Tl[a] * shlu[a];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        double s = 0.0;
+        for (int a = 0; a < n; ++a) s = s + ul[i * n + a] * shg[j * n + a];
+        M3(du, i, j) = s;
+      }
+    for (int j = 0; j < 3; ++j) {
+      double s = 0.0, t = 0.0;
+      for (int a = 0; a < n; ++a) s = s + pl[a] * shg[j * n + a];
+      for (int a = 0; a < n; ++a) t = t + phil[a] * shg[j * n + a];
+      dp[j] = s;
+      dphi[j] = t;
+    }
+    /* phase change (:343-350) */
+    const double phic = fmax(fmin(phii, 1.0), 0.0);
+    const double Tic = Ti;
+    double mdot, dmdphi;
+    if (Ti > Ts) {
+      mdot = c_evap * (1 - phic) * rhow * (Tic - Ts) / Ts;
+      dmdphi = -c_evap * rhow * (Tic - Ts) / Ts * c_dmdot;
+    } else {
+      mdot = c_cond * (phic)*rhoa * (Tic - Ts) / Ts;
+      dmdphi = c_cond * rhoa * (Tic - Ts) / Ts * c_dmdot;
+    }
+    const double vdot = mdot / rhoa - mdot / rhow;
+    double rhoi, mui;
+    if (is_fluid) {
+      rhoi = orc_prop(rhow, rhoa, phic);
+      mui = orc_prop(muw, mua, phic);
+    } else {
+      rhoi = rhos;
+      mui = mus;
+    }
+    const double lambda = -2.0 / 3.0 * mui;
+    if (phii < 0.0 || phii > 1.0) dmdphi = 0.0;
+    double uadv[3];
+    for (int i = 0; i < 3; ++i) uadv[i] = ui[i] - umi[i];
+    const double divu = M3(du, 0, 0) + M3(du, 1, 1) + M3(du, 2, 2);
+    double fi[3], rLi[3];
+    for (int i = 0; i < 3; ++i) fi[i] = grav[i] * rhoi;
+    /* rLi (:378-382); the second-derivative terms are zero */
+    for (int i = 0; i < 3; ++i)
+      rLi[i] = rhoi * aci[i] +
+               rhoi * (M3(du, i, 0) * uadv[0] + M3(du, i, 1) * uadv[1] +
+                       M3(du, i, 2) * uadv[2]) +
+               dp[i] - fi[i];
+    const double res = rphii + (uadv[0] * dphi[0] + uadv[1] * dphi[1] + uadv[2] * dphi[2]) +
+                       phii * vdot - mdot / rhoa;
+    /* e3STAB_3D_NSVOF (e3STAB.f90:292-333) */
+    double taua = 0.0, gg = 0.0;
+    for (int j = 0; j < 3; ++j)
+      for (int i = 0; i < 3; ++i) taua = taua + uadv[i] * M3(Gij, i, j) * uadv[j];
+    for (int t = 0; t < 9; ++t) gg = gg + Gij[t] * Gij[t];
+    const double nu = mui / rhoi;
+    const double taud = nu * nu * 12.0 * gg;
+    const double taut = 4.0 / (delt * delt);
+    const double tauM = 1.0 / (rhoi * sqrt(taut + taua + taud));
+    const double tauLS = 1.0 / (sqrt(taut + taua));
+    const double tauC = rhoi * sqrt(taua + taud) / (M3(Gij, 0, 0) + M3(Gij, 1, 1) + M3(Gij, 2, 2));
+    double up[3];
+    for (int i = 0; i < 3; ++i) up[i] = -tauM * rLi[i];
+    double sc[8];
+    for (int a = 0; a < n; ++a)
+      sc[a] = uadv[0] * shg[a] + uadv[1] * shg[n + a] + uadv[2] * shg[2 * n + a];
+    const double kdc = 0.0;
+    double kdcphi = 0.0;
+    if (fabs(lsc_kdc) > 0.0) kdcphi = lsc_kdc * fabs(res) / sqrt(gg);
+    /* e3CFL (e3STAB.f90:394-412) */
+    double cf = 0.0;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) cf = cf + uadv[i] * M3(Gij, i, j) * uadv[j];
+    cf = delt * sqrt(cf);
+    if (cf > *cfl) *cfl = cf;
+    if (do_mat && is_fluid) {
+      for (int b = 0; b < n; ++b)
+        for (int a = 0; a < n; ++a) {
+          const size_t ab = (size_t)b * n + a;
+          const double Na = shlu[a], Nb = shlu[b], sa = sc[a], sb = sc[b];
+          const double ga[3] = {shg[a], shg[n + a], shg[2 * n + a]};
+          const double gb[3] = {shg[b], shg[n + b], shg[2 * n + b]};
+          const double gab = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
+          double *K = ke + ab * 9;
+          for (int i = 0; i < 3; ++i) {
+            for (int j = 0; j < 3; ++j)
+              K[i * 3 + j] = K[i * 3 + j] + fact2 * ga[j] * gb[i] * (mui + kdc) * dw +
+                             fact2 * ga[i] * gb[j] * (tauC + lambda) * dw;
+            K[i * 3 + i] = K[i * 3 + i] + fact1 * Na * rhoi * Nb * dw +
+                           fact2 * Na * rhoi * sb * dw + fact2 * (mui + kdc) * gab * dw +
+                           fact1 * rhoi * sa * tauM * rhoi * Nb * dw +
+                           fact2 * rhoi * sa * tauM * rhoi * sb * dw;
+          }
+          for (int i = 0; i < 3; ++i) {
+            ge[ab * 3 + i] = ge[ab * 3 + i] - fact2 * ga[i] * Nb * dw +
+                             fact2 * rhoi * sa * tauM * gb[i] * dw;
+            de[ab * 3 + i] = de[ab * 3 + i] + fact2 * Na * gb[i] * dw +
+                             ga[i] * tauM * rhoi * fact1 * Nb * dw +
+                             ga[i] * tauM * rhoi * fact2 * sb * dw;
+          }
+          me[ab] = me[ab] + fact2 * tauM * gab * dw;
+          const double wa = Na + tauLS * sa;
+          lse[ab] = lse[ab] + wa * fact1 * Nb * dw + wa * fact2 * sb * dw;
+          lse[ab] = lse[ab] + wa * fact2 * Nb * vdot * dw +
+                    wa * fact2 * phii * cr * dmdphi * Nb * dw - wa * fact2 * dmdphi * Nb / rhoa * dw;
+          lse[ab] = lse[ab] + fact2 * kdcphi * gab * dw;
+          lse[ab] = lse[ab] + fact2 * dmdphi * cr * Na * tauC * dmdphi * cr * Nb * dw;
+          for (int i = 0; i < 3; ++i)
+            ulse[ab * 3 + i] = ulse[ab * 3 + i] + fact2 * ga[i] * tauC * dmdphi * Nb * cr * dw;
+          plse[ab] = plse[ab] - fact2 * Na * dmdphi * Nb * cr * dw;
+          for (int i = 0; i < 3; ++i) {
+            lsue[ab * 3 + i] = lsue[ab * 3 + i] + fact2 * wa * Nb * dphi[i] * dw;
+            lsue[ab * 3 + i] = lsue[ab * 3 + i] - dmdphi * cr * fact2 * Na * tauC * gb[i] * dw;
+          }
+        }
+    }
+    if (do_vec && is_fluid) {
+      double t1[3];
+      for (int i = 0; i < 3; ++i)
+        t1[i] = rhoi * aci[i] + rhoi * uadv[0] * M3(du, i, 0) + rhoi * uadv[1] * M3(du, i, 1) +
+                rhoi * uadv[2] * M3(du, i, 2) - fi[i];
+      for (int a = 0; a < n; ++a) {
+        const double Na = shlu[a], sa = sc[a];
+        const double ga[3] = {shg[a], shg[n + a], shg[2 * n + a]};
+        /* the viscous sums are scalars subtracted from every component, as
+         * written in the reference (:500-502) */
+        double S[3];
+        for (int k = 0; k < 3; ++k)
+          S[k] = ga[0] * (M3(du, 0, k) + M3(du, k, 0)) + ga[1] * (M3(du, 1, k) + M3(du, k, 1)) +
+                 ga[2] * (M3(du, 2, k) + M3(du, k, 2));
+        for (int i = 0; i < 3; ++i)
+          ru[a * 3 + i] = ru[a * 3 + i] - Na * t1[i] * dw - ga[i] * (-pri + tauC * divu) * dw +
+                          ga[i] * tauC * vdot * dw - ga[i] * lambda * divu * dw -
+                          (mui + kdc) * S[0] * dw - (mui + kdc) * S[1] * dw -
+                          (mui + kdc) * S[2] * dw + rhoi * sa * up[i] * dw;
+        rp[a] = rp[a] - Na * (divu - vdot) * dw +
+                (ga[0] * up[0] + ga[1] * up[1] + ga[2] * up[2]) * dw;
+        rls[a] = rls[a] - (Na + tauLS * sa) * res * dw -
+                 kdcphi * (ga[0] * dphi[0] + ga[1] * dphi[1] + ga[2] * dphi[2]) * dw +
+                 Na * dmdphi * cr * tauC * (divu - vdot) * dw;
+      }
+    }
+  }
+  return nneg;
+}
+
+/* BCLhs_NS_conv on element locals; ibcl(NSHL,8) column-major */
+void orc_bclhs_ns_conv(int n, const int *ibcl, double *ke, double *ge, double *de,
+                       double *me, double *lse, double *lsue, double *ulse, double *plse,
+                       double *ru, double *rp, double *rls) {
+#define AB(a, b) ((size_t)(b) * n + (a))
+  for (int a = 0; a < n; ++a) {
+    for (int c = 0; c < 3; ++c) {
+      if (ibcl[c * n + a] != 1) continue;
+      for (int b = 0; b < n; ++b) {
+        for (int j = 0; j < 3; ++j) ke[AB(a, b) * 9 + c * 3 + j] = 0.0;
+        ge[AB(a, b) * 3 + c] = 0.0;
+        lsue[AB(a, b) * 3 + c] = 0.0;
+      }
+      for (int b = 0; b < n; ++b) {
+        for (int i = 0; i < 3; ++i) ke[AB(b, a) * 9 + i * 3 + c] = 0.0;
+        de[AB(b, a) * 3 + c] = 0.0;
+        lsue[AB(b, a) * 3 + c] = 0.0;
+      }
+      ke[AB(a, a) * 9 + c * 3 + c] = 1.0;
+      ru[a * 3 + c] = 0.0;
+    }
+    if (ibcl[6 * n + a] == 1) {
+      for (int b = 0; b < n; ++b) {
+        me[AB(a, b)] = 0.0;
+        for (int i = 0; i < 3; ++i) de[AB(a, b) * 3 + i] = 0.0;
+        for (int i = 0; i < 3; ++i) ge[AB(b, a) * 3 + i] = 0.0;
+        plse[AB(a, b)] = 0.0;
+      }
+      me[AB(a, a)] = 1.0;
+      rp[a] = 0.0;
+    }
+    if (ibcl[7 * n + a] == 1) {
+      for (int b = 0; b < n; ++b) {
+        lse[AB(a, b)] = 0.0;
+        lse[AB(b, a)] = 0.0;
+        for (int i = 0; i < 3; ++i) lsue[AB(a, b) * 3 + i] = 0.0;
+        plse[AB(b, a)] = 0.0;
+        for (int i = 0; i < 3; ++i) ulse[AB(b, a) * 3 + i] = 0.0;
+      }
+      lse[AB(a, a)] = 1.0;
+      rls[a] = 0.0;
+    }
+  }
+#undef AB
+}
+
+int orc_assemble_nsvof(int nnode, int nelem, int nshl, int ngauss, const int *ien,
+                       const int *elm_id, int tensor_flag, const double *xg, const int *ibc8,
+                       const double *dgA, const double *ugA, const double *ugmA,
+                       const double *acgA, const double *pgA, const double *phigA,
+                       const double *rphigA, const double *TgA, const double *prm,
+                       const double *qp, const double *ex, const int *col, const int *row,
+                       double *k11, double *g, double *d1, double *m, double *ls, double *lsu,
+                       double *pls, double *uls, double *rhsu, double *rhsp, double *rhsls,
+                       double *cfl_out) {
+  const int fluid_id = (int)qp[19];
+  const int n = nshl, n2 = nshl * nshl;
+  double xl[24], dl[24], ul[24], uml[24], acl[24], pl[8], phil[8], rphil[8], Tl[8];
+  double ke[9 * 64], ge[3 * 64], de[3 * 64], me[64], lse[64], lsue[3 * 64], ulse[3 * 64];
+  double plse[64], ru[24], rp[8], rls[8];
+  int ibcl[64];
+  int nneg = 0;
+  double cfl = 0.0;
+  for (int e = 0; e < nelem; ++e) {
+    for (int a = 0; a < n; ++a) {
+      const int nd = ien[a * nelem + e];
+      for (int i = 0; i < 3; ++i) {
+        xl[i * n + a] = xg[i * nnode + nd];
+        dl[i * n + a] = dgA[i * nnode + nd];
+        ul[i * n + a] = ugA[i * nnode + nd];
+        uml[i * n + a] = ugmA[i * nnode + nd];
+        acl[i * n + a] = acgA[i * nnode + nd];
+      }
+      pl[a] = pgA[nd];
+      phil[a] = phigA[nd];
+      rphil[a] = rphigA[nd];
+      Tl[a] = TgA[nd];
+      for (int c = 0; c < 8; ++c) ibcl[c * n + a] = ibc8[c * nnode + nd];
+    }
+    memset(ke, 0, sizeof(double) * 9 * n2);
+    memset(ge, 0, sizeof(double) * 3 * n2);
+    memset(de, 0, sizeof(double) * 3 * n2);
+    memset(me, 0, sizeof(double) * n2);
+    memset(lse, 0, sizeof(double) * n2);
+    memset(lsue, 0, sizeof(double) * 3 * n2);
+    memset(ulse, 0, sizeof(double) * 3 * n2);
+    memset(plse, 0, sizeof(double) * n2);
+    memset(ru, 0, sizeof(double) * 3 * n);
+    memset(rp, 0, sizeof(double) * n);
+    memset(rls, 0, sizeof(double) * n);
+    const int r = orc_element_nsvof(n, ngauss, xl, dl, ul, uml, acl, pl, phil, rphil, Tl, prm,
+                                    qp, ex, elm_id[e] == fluid_id, tensor_flag, ke, ge, de,
+                                    me, lse, lsue, ulse, plse, ru, rp, rls, &cfl);
+    if (r < 0) return r;
+    nneg += r;
+    orc_bclhs_ns_conv(n, ibcl, ke, ge, de, me, lse, lsue, ulse, plse, ru, rp, rls);
+    if (tensor_flag & 4) {
+      for (int a = 0; a < n; ++a) {
+        const int i = ien[a * nelem + e];
+        for (int b = 0; b < n; ++b) {
+          const int k = orc_sparse_loc(row, col[i], col[i + 1], ien[b * nelem + e]);
+          const size_t ab = (size_t)b * n + a;
+          for (int t = 0; t < 9; ++t) k11[(size_t)k * 9 + t] += ke[ab * 9 + t];
+          for (int t = 0; t < 3; ++t) {
+            g[(size_t)k * 3 + t] += ge[ab * 3 + t];
+            d1[(size_t)k * 3 + t] += de[ab * 3 + t];
+            lsu[(size_t)k * 3 + t] += lsue[ab * 3 + t];
+            uls[(size_t)k * 3 + t] += ulse[ab * 3 + t];
+          }
+          m[k] += me[ab];
+          ls[k] += lse[ab];
+          pls[k] += plse[ab];
+        }
+      }
+    }
+    if (tensor_flag & 2) {
+      for (int b = 0; b < n; ++b) {
+        const int nd = ien[b * nelem + e];
+        for (int i = 0; i < 3; ++i) rhsu[i * nnode + nd] += ru[b * 3 + i];
+        rhsp[nd] += rp[b];
+        rhsls[nd] += rls[b];
+      }
+    }
+  }
+  if (cfl_out) *cfl_out = cfl;
+  return nneg;
+}
+
+/* SparseProd_NS_conv (SparseProd.f90:167-242); x, y as u(N,3), p(N), phi(N) */
+void orc_spmv_ns_conv(int nnode, const int *col, const int *row, const double *k11,
+                      const double *g, const double *d1, const double *m, const double *ls,
+                      const double *lsu, const double *pls, const double *uls,
+                      const double *xu, const double *xp, const double *xl, double *yu,
+                      double *yp, double *yl) {
+  const int N = nnode;
+  for (int a = 0; a < N; ++a) {
+    double t[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int k = col[a]; k < col[a + 1]; ++k) {
+      const int c = row[k];
+      const double u0 = xu[c], u1 = xu[N + c], u2 = xu[2 * N + c], p = xp[c], f = xl[c];
+      for (int i = 0; i < 3; ++i)
+        t[i] = t[i] + k11[(size_t)k * 9 + i * 3 + 0] * u0 + k11[(size_t)k * 9 + i * 3 + 1] * u1 +
+               k11[(size_t)k * 9 + i * 3 + 2] * u2 + uls[(size_t)k * 3 + i] * f +
+               g[(size_t)k * 3 + i] * p;
+      t[3] = t[3] + d1[(size_t)k * 3 + 0] * u0 + d1[(size_t)k * 3 + 1] * u1 +
+             d1[(size_t)k * 3 + 2] * u2 + pls[k] * f + m[k] * p;
+      t[4] = t[4] + lsu[(size_t)k * 3 + 0] * u0 + lsu[(size_t)k * 3 + 1] * u1 +
+             lsu[(size_t)k * 3 + 2] * u2 + ls[k] * f;
+    }
+    for (int i = 0; i < 3; ++i) yu[i * N + a] = 0.0 + t[i];
+    yp[a] = 0.0 + t[3];
+    yl[a] = 0.0 + t[4];
+  }
+}
+
+/* SparseGMRES_ns_conv (SparseGMRES_NS_conv.f:1-406): block-Jacobi left
+ * preconditioner (3x3 cofactor inverse of the K11 diagonal block, 1/M with
+ * 0 -> 1, 1/LS with 0 -> 0), one MGS cycle.  sol accumulates; returns the
+ * iteration count. */
+int orc_gmres_ns_conv(int nnode, const int *col, const int *row, const double *k11,
+                      const double *g, const double *d1, const double *m, const double *ls,
+                      const double *lsu, const double *pls, const double *uls,
+                      const double *bu, const double *bp, const double *bl, double *solu,
+                      double *solp, double *soll, double tol, int kspace, int kspace_min,
+                      double *resid) {
+  const int N = nnode, K = kspace;
+  const size_t vs = (size_t)N * 5;  /* per basis vector: u(N,3), p(N), phi(N) */
+  double *Bi = calloc((size_t)N * 9, sizeof(double));
+  double *Mi = calloc((size_t)N, sizeof(double)), *Li = calloc((size_t)N, sizeof(double));
+  double *V = calloc(vs * (K + 1), sizeof(double));
+  double *tmp5 = calloc(vs, sizeof(double));
+  double *H = calloc((size_t)(K + 1) * K, sizeof(double));
+  double *rc = calloc((size_t)K, sizeof(double)), *rs = calloc((size_t)K, sizeof(double));
+  double *e = calloc((size_t)K + 1, sizeof(double)), *y = calloc((size_t)K, sizeof(double));
+#define HB(i, j) H[(i) + (K + 1) * (j)]
+#define VU(j, c) (V + (size_t)(j) * vs + (size_t)(c) * N)
+#define VP(j) (V + (size_t)(j) * vs + (size_t)3 * N)
+#define VL(j) (V + (size_t)(j) * vs + (size_t)4 * N)
+  for (int i = 0; i < N; ++i) {
+    double Kd[9] = {0}, md = 0.0, ld = 0.0;
+    for (int k = col[i]; k < col[i + 1]; ++k)
+      if (row[k] == i) {
+        for (int t = 0; t < 9; ++t) Kd[t] = k11[(size_t)k * 9 + t];
+        md = m[k];
+        ld = ls[k];
+      }
+    double *B = Bi + (size_t)i * 9;
+    B[0] = Kd[4] * Kd[8] - Kd[7] * Kd[5];
+    B[1] = Kd[7] * Kd[2] - Kd[1] * Kd[8];
+    B[2] = Kd[1] * Kd[5] - Kd[2] * Kd[4];
+    const double t2 = B[0] * Kd[0] + B[1] * Kd[3] + B[2] * Kd[6];
+    const double t = 1.0 / t2;
+    B[0] = B[0] * t;
+    B[1] = B[1] * t;
+    B[2] = B[2] * t;
+    B[3] = (Kd[5] * Kd[6] - Kd[3] * Kd[8]) * t;
+    B[4] = (Kd[0] * Kd[8] - Kd[6] * Kd[2]) * t;
+    B[5] = (Kd[3] * Kd[2] - Kd[0] * Kd[5]) * t;
+    B[6] = (Kd[3] * Kd[7] - Kd[4] * Kd[6]) * t;
+    B[7] = (Kd[6] * Kd[1] - Kd[0] * Kd[7]) * t;
+    B[8] = (Kd[0] * Kd[4] - Kd[1] * Kd[3]) * t;
+    Mi[i] = (md == 0.0) ? 1.0 : 1.0 / md;
+    Li[i] = (ld == 0.0) ? 0.0 : 1.0 / ld;
+  }
+  /* z = P^-1 r for a 5-field vector r (in place into out) */
+#define PREC(ru_, rp_, rl_, j)                                                          \
+  for (int i = 0; i < N; ++i) {                                                      \
+    const double *B = Bi + (size_t)i * 9;                                            \
+    const double r0 = (ru_)[i], r1 = (ru_)[N + i], r2 = (ru_)[2 * N + i];            \
+    VU(j, 0)[i] = B[0] * r0 + B[1] * r1 + B[2] * r2;                                 \
+    VU(j, 1)[i] = B[3] * r0 + B[4] * r1 + B[5] * r2;                                 \
+    VU(j, 2)[i] = B[6] * r0 + B[7] * r1 + B[8] * r2;                                 \
+    VP(j)[i] = (rp_)[i] * Mi[i];                                                     \
+    VL(j)[i] = (rl_)[i] * Li[i];                                                     \
+  }
+  PREC(bu, bp, bl, 0);
+  /* composite dot in the reference's order: p, phi, then u1..u3 */
+#define DOT5(ja, jb, out)                                                               \
+  do {                                                                               \
+    double s_ = 0.0, t_ = 0.0;                                                       \
+    for (int i = 0; i < N; ++i) s_ = s_ + VP(ja)[i] * VP(jb)[i];                     \
+    for (int i = 0; i < N; ++i) t_ = t_ + VL(ja)[i] * VL(jb)[i];                     \
+    double r_ = s_ + t_;                                                             \
+    for (int c = 0; c < 3; ++c) {                                                    \
+      double u_ = 0.0;                                                               \
+      for (int i = 0; i < N; ++i) u_ = u_ + VU(ja, c)[i] * VU(jb, c)[i];             \
+      r_ = r_ + u_;                                                                  \
+    }                                                                                \
+    (out) = r_;                                                                      \
+  } while (0)
+  double rr;
+  DOT5(0, 0, rr);
+  double unorm = sqrt(rr);
+  const double eps = tol * unorm;
+  e[0] = unorm;
+  for (size_t t = 0; t < vs; ++t) V[t] = V[t] / unorm;
+  int iks = 0;
+  double er = 0.0;
+  for (int ik = 1; ik <= K; ++ik) {
+    iks = ik;
+    orc_spmv_ns_conv(N, col, row, k11, g, d1, m, ls, lsu, pls, uls, VU(ik - 1, 0), VP(ik - 1),
+                     VL(ik - 1), tmp5, tmp5 + 3 * (size_t)N, tmp5 + 4 * (size_t)N);
+    PREC(tmp5, tmp5 + 3 * (size_t)N, tmp5 + 4 * (size_t)N, ik);
+    double beta = 0.0;
+    for (int jk = 1; jk <= ik + 1; ++jk) {
+      if (jk > 1) {
+        double *vn = V + (size_t)ik * vs;
+        const double *vp = V + (size_t)(jk - 2) * vs;
+        for (size_t t = 0; t < vs; ++t) vn[t] = vn[t] - beta * vp[t];
+      }
+      DOT5(ik, jk - 1, beta);
+      HB(jk - 1, ik - 1) = beta;
+    }
+    unorm = sqrt(beta);
+    HB(ik, ik - 1) = unorm;
+    {
+      double *vn = V + (size_t)ik * vs;
+      for (size_t t = 0; t < vs; ++t) vn[t] = vn[t] / unorm;
+    }
+    for (int jk = 1; jk <= ik - 1; ++jk) {
+      const double t = rc[jk - 1] * HB(jk - 1, ik - 1) + rs[jk - 1] * HB(jk, ik - 1);
+      HB(jk, ik - 1) = -rs[jk - 1] * HB(jk - 1, ik - 1) + rc[jk - 1] * HB(jk, ik - 1);
+      HB(jk - 1, ik - 1) = t;
+    }
+    const double t = sqrt(HB(ik - 1, ik - 1) * HB(ik - 1, ik - 1) + HB(ik, ik - 1) * HB(ik, ik - 1));
+    rc[ik - 1] = HB(ik - 1, ik - 1) / t;
+    rs[ik - 1] = HB(ik, ik - 1) / t;
+    HB(ik - 1, ik - 1) = t;
+    HB(ik, ik - 1) = 0.0;
+    const double te2 = rc[ik - 1] * e[ik - 1] + rs[ik - 1] * e[ik];
+    e[ik] = -rs[ik - 1] * e[ik - 1] + rc[ik - 1] * e[ik];
+    e[ik - 1] = te2;
+    er = fabs(e[ik]);
+    if (er <= eps && iks >= kspace_min) break;
+  }
+  for (int jk = iks; jk >= 1; --jk) {
+    y[jk - 1] = e[jk - 1] / HB(jk - 1, jk - 1);
+    for (int l = 0; l < jk - 1; ++l) e[l] = e[l] - y[jk - 1] * HB(l, jk - 1);
+  }
+  for (int jk = 1; jk <= iks; ++jk) {
+    for (int i = 0; i < N; ++i) solp[i] = solp[i] + y[jk - 1] * VP(jk - 1)[i];
+    for (int i = 0; i < N; ++i) soll[i] = soll[i] + y[jk - 1] * VL(jk - 1)[i];
+    for (int c = 0; c < 3; ++c)
+      for (int i = 0; i < N; ++i)
+        solu[c * N + i] = solu[c * N + i] + y[jk - 1] * VU(jk - 1, c)[i];
+  }
+  *resid = er;
+#undef PREC
+#undef DOT5
+#undef HB
+#undef VU
+#undef VP
+#undef VL
+  free(Bi);
+  free(Mi);
+  free(Li);
+  free(V);
+  free(tmp5);
+  free(H);
+  free(rc);
+  free(rs);
+  free(e);
+  free(y);
+  return iks;
+}

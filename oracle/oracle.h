@@ -148,6 +148,36 @@ int orc_gmres_tem(int nnode, const int *col, const int *row, const double *A,
                   const double *b, double *sol, double tol, int kspace, int kspace_min,
                   double *resid);
 
+/*
+ * Coupled NS + VOF assembly (solveQuenching.f90:122-573) with the 5-field
+ * projection BCLhs_NS_conv and scatter FillSparseMat_NS_conv;
+ * ibc8(NNODE,8) = IBC; ex = {gx, gy, gz, lsc_kdc}; outputs in the reference
+ * layouts (k11 9, g/d1/lsu/uls 3, m/ls/pls 1 per nonzero; rhsu (N,3),
+ * rhsp, rhsls (N)).  cfl_out may be NULL.
+ */
+int orc_assemble_nsvof(int nnode, int nelem, int nshl, int ngauss, const int *ien,
+                       const int *elm_id, int tensor_flag, const double *xg, const int *ibc8,
+                       const double *dgA, const double *ugA, const double *ugmA,
+                       const double *acgA, const double *pgA, const double *phigA,
+                       const double *rphigA, const double *TgA, const double *prm,
+                       const double *qp, const double *ex, const int *col, const int *row,
+                       double *k11, double *g, double *d1, double *m, double *ls, double *lsu,
+                       double *pls, double *uls, double *rhsu, double *rhsp, double *rhsls,
+                       double *cfl_out);
+/* SparseProd_NS_conv */
+void orc_spmv_ns_conv(int nnode, const int *col, const int *row, const double *k11,
+                      const double *g, const double *d1, const double *m, const double *ls,
+                      const double *lsu, const double *pls, const double *uls,
+                      const double *xu, const double *xp, const double *xl, double *yu,
+                      double *yp, double *yl);
+/* SparseGMRES_ns_conv: one MGS cycle, sol accumulates */
+int orc_gmres_ns_conv(int nnode, const int *col, const int *row, const double *k11,
+                      const double *g, const double *d1, const double *m, const double *ls,
+                      const double *lsu, const double *pls, const double *uls,
+                      const double *bu, const double *bp, const double *bl, double *solu,
+                      double *solp, double *soll, double tol, int kspace, int kspace_min,
+                      double *resid);
+
 #ifdef __cplusplus
 }
 #endif

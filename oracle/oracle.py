@@ -92,6 +92,13 @@ def lib():
         L.orc_gmres_tem.argtypes = ([ctypes.c_int, _ip, _ip, _dp, _dp, _dp, ctypes.c_double,
                                      ctypes.c_int, ctypes.c_int, _dp])
         L.orc_gmres_tem.restype = ctypes.c_int
+        L.orc_assemble_nsvof.argtypes = ([ctypes.c_int] * 4 + [_ip, _ip, ctypes.c_int, _dp,
+                                          _ip] + [_dp] * 11 + [_ip, _ip] + [_dp] * 12)
+        L.orc_assemble_nsvof.restype = ctypes.c_int
+        L.orc_spmv_ns_conv.argtypes = [ctypes.c_int, _ip, _ip] + [_dp] * 14
+        L.orc_gmres_ns_conv.argtypes = ([ctypes.c_int, _ip, _ip] + [_dp] * 14 +
+                                        [ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp])
+        L.orc_gmres_ns_conv.restype = ctypes.c_int
         L.orc_interp_hex.argtypes = ([ctypes.c_int] * 2 + [_ip, _ip, ctypes.c_int] +
                                      [_dp] * 3 + [ctypes.c_int, _dp, _dp, _ip, _dp])
         L.orc_hex_local.argtypes = [_dp, _dp, _dp]
@@ -251,6 +258,69 @@ def assemble_tem(mesh, ibc_t, st, prm, qprm, col, row, tensor_flag, lhs=None, rh
         _c(st["phig"]), _c(st["Tg"]), _c(st["rTg"]), _c(p), _c(q), _c(col, np.int32),
         _c(row, np.int32), lhs, rhs)
     return dict(lhs=lhs, rhs=rhs, nneg=nneg)
+
+
+NSVOF_BLOCKS = ("k11", "g", "d1", "m", "ls", "lsu", "pls", "uls")
+_NSVOF_WIDTH = dict(k11=9, g=3, d1=3, m=1, ls=1, lsu=3, pls=1, uls=3)
+
+
+def assemble_nsvof(mesh, ibc8, st, prm, qprm, ex, col, row, tensor_flag):
+    """IntElmAss_NSVOF_Quenching_STAB + BCLhs_NS_conv + FillSparseMat_NS_conv.
+    st: alpha-stage dg, ug, ugm, acg (N,3) and pg, phig, rphig, Tg (N); prm as
+    assemble_tem; ex = (gx, gy, gz, lsc_kdc); ibc8 (N, 8)."""
+    N, E = mesh.nnode, mesh.nelem
+    p = np.asarray(prm, np.float64)
+    if p.size == 4:
+        p = np.array([p[0], p[1], p[2], 0.0, p[3]])
+    q = np.zeros(20)
+    q[:len(qprm)] = qprm
+    if len(qprm) < 20:
+        q[19] = 102
+    Z = len(row)
+    out = {k: np.zeros(Z * w) for k, w in _NSVOF_WIDTH.items()}
+    rhsu = np.zeros(3 * N)
+    rhsp = np.zeros(N)
+    rhsl = np.zeros(N)
+    cfl = np.zeros(1)
+    nneg = lib().orc_assemble_nsvof(
+        N, E, mesh.nshl, mesh.nshl, _c(mesh.ien.T, np.int32).ravel(),
+        _c(mesh.elm_id, np.int32), int(tensor_flag), _soa(mesh.xg),
+        _c(np.asarray(ibc8, np.int32).T, np.int32).ravel(), _soa(st["dg"]), _soa(st["ug"]),
+        _soa(st["ugm"]), _soa(st["acg"]), _c(st["pg"]), _c(st["phig"]), _c(st["rphig"]),
+        _c(st["Tg"]), _c(p), _c(q), _c(ex), _c(col, np.int32), _c(row, np.int32),
+        *[out[k] for k in NSVOF_BLOCKS], rhsu, rhsp, rhsl, cfl)
+    res = {k: (v.reshape(Z, _NSVOF_WIDTH[k]) if _NSVOF_WIDTH[k] > 1 else v)
+           for k, v in out.items()}
+    res.update(rhsu=rhsu.reshape(3, N).T.copy(), rhsp=rhsp, rhsls=rhsl, nneg=nneg,
+               cfl=float(cfl[0]))
+    return res
+
+
+def _nsvof_mats(A):
+    return [_c(A[k]).ravel() for k in NSVOF_BLOCKS]
+
+
+def spmv_ns_conv(col, row, A, xu, xp, xl):
+    N = len(col) - 1
+    yu = np.zeros(3 * N)
+    yp = np.zeros(N)
+    yl = np.zeros(N)
+    lib().orc_spmv_ns_conv(N, _c(col, np.int32), _c(row, np.int32), *_nsvof_mats(A),
+                           _soa(xu), _c(xp), _c(xl), yu, yp, yl)
+    return yu.reshape(3, N).T.copy(), yp, yl
+
+
+def gmres_ns_conv(col, row, A, bu, bp, bl, tol, kspace, kspace_min):
+    N = len(col) - 1
+    su = np.zeros(3 * N)
+    sp = np.zeros(N)
+    sl = np.zeros(N)
+    res = np.zeros(1)
+    it = lib().orc_gmres_ns_conv(N, _c(col, np.int32), _c(row, np.int32), *_nsvof_mats(A),
+                                 _soa(bu), _c(bp), _c(bl), su, sp, sl, float(tol),
+                                 int(kspace), int(kspace_min), res)
+    return dict(solu=su.reshape(3, N).T.copy(), solp=sp, solls=sl, iters=it,
+                resid=float(res[0]))
 
 
 def spmv_s(col, row, A, x):

@@ -21,6 +21,18 @@
 !   4  tem:    genSparStruc, IntElmAss_Tem_Quenching_STAB (with BCLhs_tem and
 !              FillSparseMat_tem), SparseProd_LS on a given vector,
 !              SparseGMRES_tem on the assembled system
+!   5  nsvof:  genSparStruc, IntElmAss_NSVOF_Quenching_STAB with the
+!              5-field Dirichlet projection and scatter BCLhs_NS_conv /
+!              FillSparseMat_NS_conv, SparseProd_NS_conv on a given vector,
+!              SparseGMRES_ns_conv on the assembled system
+!
+! IntElmAss_NSVOF_Quenching_STAB calls BCLhs_3D and FillSparseMat_3D with
+! the 5-field argument lists, which the reference's routines of those names
+! do not take (solveQuenching.f90:526-534).  oracle/Makefile compiles
+! solveQuenching.f90 with those two call names mapped to the adapters at
+! the end of this file, which reorder the arguments and call the
+! reference's own 5-field routines, BCLhs_NS_conv and FillSparseMat_NS_conv,
+! with a zero second divergence block.
 !======================================================================
 program ofd_refharness
   use aAdjKeep
@@ -56,6 +68,8 @@ program ofd_refharness
     call run_interp()
   case (4)
     call run_tem()
+  case (5)
+    call run_nsvof()
   case default
     write (*, *) 'unknown mode', mode
   end select
@@ -264,4 +278,101 @@ contains
     write (11) sol
   end subroutine run_tem
 
+  subroutine run_nsvof()
+    integer :: nshl, ngauss, tflag, kspace, kmin, i
+    real(8) :: prm(4), qprm(19), extra(6), tol
+    real(8), allocatable :: dA(:, :), uA(:, :), umA(:, :), aA(:, :), &
+                            amA(:, :), pA(:), phiA(:), rphiA(:), TA(:), rTA(:), &
+                            xu(:, :), xp(:), xl(:), yu(:, :), yp(:), yl(:), &
+                            solu(:, :), solp(:), soll(:)
+    integer, allocatable :: iper(:), dflag(:), pflag(:)
+
+    read (10) NNODE, NELEM, nshl, ngauss, tflag
+    maxNSHL = nshl
+    allocate (IEN(NELEM, nshl), ELM_ID(NELEM), ELMNSHL(NELEM), &
+              ELMNGAUSS(NELEM), IBC(NNODE, 8), xg(NNODE, 3), wg(NNODE))
+    read (10) IEN, ELM_ID, IBC
+    read (10) xg
+    wg = 1.0d0
+    ELMNSHL = nshl
+    ELMNGAUSS = ngauss
+    allocate (dA(NNODE, 3), uA(NNODE, 3), umA(NNODE, 3), aA(NNODE, 3), &
+              amA(NNODE, 3), pA(NNODE), phiA(NNODE), rphiA(NNODE), TA(NNODE), &
+              rTA(NNODE))
+    read (10) dA, uA, umA, aA, amA, pA, phiA, rphiA, TA, rTA, prm, qprm, extra
+    almi = prm(1); alfi = prm(2); gami = prm(3); Delt = prm(4)
+    rhoa = qprm(1); rhow = qprm(2); mua = qprm(3); muw = qprm(4)
+    cpa = qprm(5); cpw = qprm(6); kappaa = qprm(7); kappaw = qprm(8)
+    rhos = qprm(9); mus = qprm(10); cps = qprm(11); kappas = qprm(12)
+    Ts = qprm(13); lh = qprm(14); c_cond = qprm(15); c_evap = qprm(16)
+    C_DMDOT = qprm(17); TEM_kdc = qprm(18); Tref = qprm(19)
+    gravvec = extra(1:3); NS_kdc_w = extra(4); NS_kdc_a = extra(5); LSC_kdc = extra(6)
+    read (10) tol, kspace, kmin
+    allocate (xu(NNODE, 3), xp(NNODE), xl(NNODE), yu(NNODE, 3), yp(NNODE), yl(NNODE))
+    read (10) xu, xp, xl
+
+    call genSparStruc()
+    allocate (LHSK11(9, icnt), LHSG(3, icnt), LHSD1(3, icnt), LHSD2(3, icnt), &
+              LHSM(icnt), LHSls(icnt), LHSlsu(3, icnt), LHSPls(icnt), &
+              LHSUls(3, icnt), RHSGu(NNODE, 3), RHSGp(NNODE), RHSGls(NNODE))
+    LHSK11 = 0.0d0; LHSG = 0.0d0; LHSD1 = 0.0d0; LHSD2 = 0.0d0; LHSM = 0.0d0
+    LHSls = 0.0d0; LHSlsu = 0.0d0; LHSPls = 0.0d0; LHSUls = 0.0d0
+    RHSGu = 0.0d0; RHSGp = 0.0d0; RHSGls = 0.0d0
+    call IntElmAss_NSVOF_Quenching_STAB(dA, uA, umA, aA, amA, pA, phiA, rphiA, &
+                                        TA, rTA, tflag)
+    write (11) icnt
+    write (11) col, row
+    write (11) LHSK11, LHSG, LHSD1, LHSM, LHSls, LHSlsu, LHSPls, LHSUls, &
+               RHSGu, RHSGp, RHSGls
+
+    allocate (solu(NNODE, 3), solp(NNODE), soll(NNODE), iper(NNODE), dflag(NNODE), &
+              pflag(NNODE))
+    iper = (/(i, i=1, NNODE)/)
+    dflag = 0; pflag = 0
+    call SparseProd_NS_conv(col, row, LHSK11, LHSG, LHSD1, LHSM, LHSls, LHSlsu, &
+                            LHSPls, LHSUls, xu, xp, xl, yu, yp, yl, dflag, pflag, &
+                            NNODE, nshl, icnt, NSD)
+    write (11) yu, yp, yl
+
+    solu = 0.0d0; solp = 0.0d0; soll = 0.0d0
+    if (kspace > 0) then
+      call SparseGMRES_ns_conv(col, row, IBC, iper, dflag, pflag, RHSGu, RHSGp, &
+                               RHSGls, solu, solp, soll, LHSK11, LHSG, LHSD1, LHSM, &
+                               LHSls, LHSlsu, LHSPls, LHSUls, icnt, tol, kspace, kmin, &
+                               NNODE, nshl, NSD)
+    end if
+    write (11) solu, solp, soll
+  end subroutine run_nsvof
+
 end program ofd_refharness
+
+!----------------------------------------------------------------------
+! Adapters for the two 5-field calls of IntElmAss_NSVOF_Quenching_STAB
+! (see the header): same element arrays, the reference's 5-field routines.
+!----------------------------------------------------------------------
+subroutine ofdh_bclhs_nsvof(nshl, iel, xKebe11, xGebe, xDebe1, xMebe, Rhsu, Rhsp, &
+                            xLSebe, xLSUebe, xULSebe, xPLSebe, Rhsphi)
+  implicit none
+  integer, intent(in) :: nshl, iel
+  real(8) :: xKebe11(9, nshl, nshl), xGebe(3, nshl, nshl), xDebe1(3, nshl, nshl), &
+             xMebe(nshl, nshl), Rhsu(3, nshl), Rhsp(nshl), xLSebe(nshl, nshl), &
+             xLSUebe(3, nshl, nshl), xULSebe(3, nshl, nshl), xPLSebe(nshl, nshl), &
+             Rhsphi(nshl)
+  real(8) :: xDebe2(3, nshl, nshl)
+  xDebe2 = 0.0d0
+  call BCLhs_NS_conv(nshl, iel, xKebe11, xGebe, xDebe1, xDebe2, xMebe, xLSebe, &
+                     xLSUebe, xPLSebe, xULSebe, Rhsu, Rhsp, Rhsphi)
+end subroutine ofdh_bclhs_nsvof
+
+subroutine ofdh_fill_nsvof(nshl, iel, xKebe11, xGebe, xDebe1, xMebe, xLSebe, xLSUebe, &
+                           xULSebe, xPLSebe)
+  implicit none
+  integer, intent(in) :: nshl, iel
+  real(8) :: xKebe11(9, nshl, nshl), xGebe(3, nshl, nshl), xDebe1(3, nshl, nshl), &
+             xMebe(nshl, nshl), xLSebe(nshl, nshl), xLSUebe(3, nshl, nshl), &
+             xULSebe(3, nshl, nshl), xPLSebe(nshl, nshl)
+  real(8) :: xDebe2(3, nshl, nshl)
+  xDebe2 = 0.0d0
+  call FillSparseMat_NS_conv(nshl, iel, xKebe11, xGebe, xDebe1, xDebe2, xMebe, &
+                             xLSebe, xLSUebe, xPLSebe, xULSebe)
+end subroutine ofdh_fill_nsvof

@@ -277,6 +277,83 @@ def make_interp(seed=21):
                         val=val, donor=donor)
 
 
+def nsvof_case(name, dims, seed, tensor_flag, tol, kspace, kmin, lsc_kdc=0.0,
+               frac_ibc=0.15):
+    """IntElmAss_NSVOF_Quenching_STAB with BCLhs_NS_conv / FillSparseMat_NS_conv
+    on a tet mesh of fluid (102) and solid (101) elements; VOF values beyond
+    [0, 1] and temperatures on both sides of T_sat; then SparseProd_NS_conv
+    and SparseGMRES_ns_conv.  Nodes that no fluid element touches are held
+    in u, v, w, p, phi so the system is non-singular."""
+    rng = np.random.default_rng(seed)
+    hexm = M.structured_hex(*dims, lo=(0, 0, 0), hi=(0.01, 0.012, 0.009), jitter=0.05,
+                            seed=seed)
+    tm = M.hex_to_tets(hexm)
+    N, E, nshl = tm.nnode, tm.nelem, tm.nshl
+    elm_id = np.where(rng.uniform(size=E) < 0.8, 102, 101).astype(np.int32)
+    ibc = np.zeros((N, 8), np.int32)
+    for c in (0, 1, 2, 6, 7):
+        ibc[:, c] = (rng.uniform(size=N) < frac_ibc).astype(np.int32)
+    touched = np.zeros(N, bool)
+    touched[tm.ien[elm_id == 102].ravel()] = True
+    ibc[np.ix_(~touched, [0, 1, 2, 6, 7])] = 1
+    st = {"dg": 1e-5 * rng.normal(size=(N, 3)), "ug": 0.3 * rng.normal(size=(N, 3)),
+          "ugm": 0.05 * rng.normal(size=(N, 3)), "acg": 2.0 * rng.normal(size=(N, 3)),
+          "pg": 100.0 * rng.normal(size=N), "phig": rng.uniform(-0.2, 1.2, size=N),
+          "rphig": 5.0 * rng.normal(size=N), "Tg": rng.uniform(300.0, 900.0, size=N)}
+    prm = np.array([0.8333333333333334, 0.6666666666666666, 0.6666666666666667, 1e-3])
+    qprm = QPRM.copy()
+    qprm[9] = 1.0e-3          # solid viscosity (unused by the fluid terms)
+    ex = np.array([0.0, 0.0, -9.81, 0.0, 0.0, lsc_kdc])
+    xu = rng.normal(size=(N, 3))
+    xp = rng.normal(size=N)
+    xl = rng.normal(size=N)
+    payload = (np.int32(5).tobytes() +
+               np.array([N, E, nshl, nshl, tensor_flag], np.int32).tobytes() +
+               _i32(tm.ien + 1) + _i32(elm_id) + _i32(ibc) + _f64(tm.xg) +
+               _f64(st["dg"]) + _f64(st["ug"]) + _f64(st["ugm"]) + _f64(st["acg"]) +
+               _f64(np.zeros((N, 3))) + _f64(st["pg"]) + _f64(st["phig"]) +
+               _f64(st["rphig"]) + _f64(st["Tg"]) + _f64(np.zeros(N)) + _f64(prm) +
+               _f64(qprm) + _f64(ex) + np.float64(tol).tobytes() +
+               np.array([kspace, kmin], np.int32).tobytes() + _f64(xu) + _f64(xp) + _f64(xl))
+    out, _ = _run(payload)
+    r = _Reader(out)
+    icnt = int(r.i32(1)[0])
+    col = r.i32(N + 1)
+    row = r.i32(icnt)
+    k11 = r.f64(9, icnt).T
+    g = r.f64(3, icnt).T
+    d1 = r.f64(3, icnt).T
+    m = r.f64(icnt)
+    ls = r.f64(icnt)
+    lsu = r.f64(3, icnt).T
+    pls = r.f64(icnt)
+    uls = r.f64(3, icnt).T
+    rhsu = r.f64(N, 3)
+    rhsp = r.f64(N)
+    rhsls = r.f64(N)
+    yu = r.f64(N, 3)
+    yp = r.f64(N)
+    yl = r.f64(N)
+    solu = r.f64(N, 3)
+    solp = r.f64(N)
+    solls = r.f64(N)
+    assert r.off == len(out)
+    np.savez_compressed(
+        os.path.join(HERE, f"{name}.npz"), xg=tm.xg, ien=tm.ien, elm_id=elm_id, ibc=ibc,
+        prm=prm, qprm=qprm, ex=ex[[0, 1, 2, 5]], tensor_flag=tensor_flag, tol=tol,
+        kspace=kspace, kmin=kmin, xu=xu, xp=xp, xl=xl, col=col - 1, row=row - 1, k11=k11,
+        g=g, d1=d1, m=m, ls=ls, lsu=lsu, pls=pls, uls=uls, rhsu=rhsu, rhsp=rhsp,
+        rhsls=rhsls, yu=yu, yp=yp, yl=yl, solu=solu, solp=solp, solls=solls,
+        **{"st_" + k: v for k, v in st.items()})
+
+
+def make_nsvof():
+    nsvof_case("nsvof_tet_a", (2, 2, 2), seed=41, tensor_flag=6, tol=1e-30, kspace=20,
+               kmin=0)
+    nsvof_case("nsvof_tet_b", (3, 2, 2), seed=42, tensor_flag=6, tol=1e-5, kspace=40,
+               kmin=2, lsc_kdc=0.4, frac_ibc=0.1)
+
+
 def make_tem():
     # fixed-length GMRES cycle, no discontinuity capturing
     tem_case("tem_tet_a", (2, 2, 2), seed=31, tensor_flag=6, tol=1e-30, kspace=20, kmin=0)
@@ -291,6 +368,9 @@ def main():
     if sys.argv[1:] == ["tem"]:
         make_tem()
         return
+    if sys.argv[1:] == ["nsvof"]:
+        make_nsvof()
+        return
     make_shape()
     # element kernels + BC + scatter + SpMV + a fixed-length GMRES cycle
     ns_case("ns_tet_a", (2, 2, 3), seed=3, tensor_flag=6, subdomain_flag=1,
@@ -303,6 +383,7 @@ def main():
             tol=1e-3, kspace=0, kmin=0, frac_ibc=0.0, two_ids=False)
     make_interp()
     make_tem()
+    make_nsvof()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

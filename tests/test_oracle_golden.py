@@ -124,3 +124,33 @@ def test_tem_assembly_spmv_gmres(oracle_mod, name):
     g = O.gmres_tem(col, row, z["lhs"], z["rhs"], float(z["tol"]), int(z["kspace"]),
                     int(z["kmin"]))
     assert np.array_equal(g["sol"], z["sol"])
+
+
+@pytest.mark.parametrize("name", ["nsvof_tet_a", "nsvof_tet_b"])
+def test_nsvof_assembly_spmv_gmres(oracle_mod, name):
+    """IntElmAss_NSVOF_Quenching_STAB with BCLhs_NS_conv and
+    FillSparseMat_NS_conv, SparseProd_NS_conv and SparseGMRES_ns_conv
+    against the reference.  Case a is bit-exact; case b turns on the VOF
+    discontinuity capturing, whose norm2(Gij) the reference's compiler
+    evaluates to within one rounding of sqrt(sum)."""
+    O = oracle_mod
+    z = golden(name)
+    m = Mesh(xg=z["xg"], ien=z["ien"], elm_id=z["elm_id"])
+    col, row = O.sparsity(m.nnode, m.ien)
+    assert np.array_equal(col, z["col"]) and np.array_equal(row, z["row"])
+    st = {k[3:]: z[k] for k in z.files if k.startswith("st_")}
+    R = O.assemble_nsvof(m, z["ibc"], st, z["prm"], z["qprm"], z["ex"], col, row,
+                         int(z["tensor_flag"]))
+    for k in O.NSVOF_BLOCKS + ("rhsu", "rhsp", "rhsls"):
+        if name == "nsvof_tet_a":
+            assert np.array_equal(R[k], z[k]), k
+        else:
+            assert np.abs(R[k] - z[k]).max() <= 1e-15 * np.abs(z[k]).max(), k
+    A = {k: z[k] for k in O.NSVOF_BLOCKS}
+    yu, yp, yl = O.spmv_ns_conv(col, row, A, z["xu"], z["xp"], z["xl"])
+    assert np.array_equal(yu, z["yu"]) and np.array_equal(yp, z["yp"])
+    assert np.array_equal(yl, z["yl"])
+    g = O.gmres_ns_conv(col, row, A, z["rhsu"], z["rhsp"], z["rhsls"], float(z["tol"]),
+                        int(z["kspace"]), int(z["kmin"]))
+    for k in ("solu", "solp", "solls"):
+        assert np.array_equal(g[k], z[k]), k
