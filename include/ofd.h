@@ -62,7 +62,7 @@ enum {
   OFD_ACG, OFD_ACGOLD, OFD_ACGM, OFD_ACGMOLD, OFD_PG, OFD_PGOLD,
   OFD_SOLU, OFD_SOLP,
   OFD_PHIG, OFD_PHIGOLD, OFD_RPHIG, OFD_RPHIGOLD, OFD_TG, OFD_TGOLD,
-  OFD_RTG, OFD_RTGOLD, OFD_SOLT, OFD_NFIELDS
+  OFD_RTG, OFD_RTGOLD, OFD_SOLT, OFD_SOLPHI, OFD_NFIELDS
 };
 
 /* parameter vector of ofd_set_params */
@@ -74,13 +74,14 @@ enum {
 /* material vector of ofd_set_quench_params (param.f90:222-242, 405-406):
  * air/vapour (a), water (w) and solid (s) properties, saturation
  * temperature, latent heat, phase-change rate coefficients, the T
- * discontinuity-capturing coefficient and reference temperature, and the
- * ELM_ID of fluid elements (102 in the reference, solveQuenching.f90:693) */
+ * discontinuity-capturing coefficient and reference temperature, the
+ * ELM_ID of fluid elements (102 in the reference, solveQuenching.f90:693)
+ * and the VOF discontinuity-capturing coefficient LSC_kdc */
 enum {
   OFD_Q_RHOA = 0, OFD_Q_RHOW, OFD_Q_MUA, OFD_Q_MUW, OFD_Q_CPA, OFD_Q_CPW,
   OFD_Q_KAPPAA, OFD_Q_KAPPAW, OFD_Q_RHOS, OFD_Q_MUS, OFD_Q_CPS, OFD_Q_KAPPAS,
   OFD_Q_TS, OFD_Q_LH, OFD_Q_CCOND, OFD_Q_CEVAP, OFD_Q_CDMDOT, OFD_Q_TEMKDC,
-  OFD_Q_TREF, OFD_Q_FLUID_ID, OFD_NQPARAMS
+  OFD_Q_TREF, OFD_Q_FLUID_ID, OFD_Q_LSCKDC, OFD_NQPARAMS
 };
 
 const char *ofd_last_error(void);
@@ -213,6 +214,53 @@ int ofd_tem_update(ofd_ctx *ctx);
 
 /* |RHSGTem|_2 */
 int ofd_tem_rhs_norm(ofd_ctx *ctx, double *norm2);
+
+/* ---- coupled NS + VOF system of the quenching solver ---- */
+
+/* assembleQuenching(tensor_flag, ASSEMBLE_FIELD_NS + ASSEMBLE_FIELD_VOF)
+ * (solveQuenching.f90:4-117) -> IntElmAss_NSVOF_Quenching_STAB (:122-573)
+ * with the 5-field projection BCLhs_NS_conv (BCLhs_3D.f90:278-428; IBC
+ * columns 1-3 = u, 7 = p, 8 = phi) and scatter FillSparseMat_NS_conv /
+ * LocaltoGlobalNSVOF_3D, from the alpha stage of the context's fields.  Hex
+ * elements; the NS discontinuity-capturing coefficient is 0; the face terms
+ * of assembleQuenching are not part of it.  cfl (may be NULL) gets the
+ * largest Gauss-point CFL number. */
+int ofd_assemble_nsvof(ofd_ctx *ctx, int tensor_flag, double *cfl);
+
+/* the reference arrays LHSK11(9,icnt), LHSG, LHSD1, LHSM, LHSls, LHSlsu(3,
+ * icnt), LHSPls, LHSUls(3,icnt) and RHSGu(NNODE,3), RHSGp, RHSGls (any
+ * pointer may be NULL) */
+int ofd_get_nsvof_system(ofd_ctx *ctx, double *k11, double *g, double *d1, double *m,
+                         double *ls, double *lsu, double *pls, double *uls, double *rhsu,
+                         double *rhsp, double *rhsls);
+int ofd_set_nsvof_system(ofd_ctx *ctx, const double *k11, const double *g, const double *d1,
+                         const double *m, const double *ls, const double *lsu,
+                         const double *pls, const double *uls, const double *rhsu,
+                         const double *rhsp, const double *rhsls);
+
+/* SparseProd_NS_conv (SparseProd.f90:167-242) on host vectors u(NNODE,3),
+ * p(NNODE), phi(NNODE) */
+int ofd_spmv_nsvof(ofd_ctx *ctx, int precision, const double *xu, const double *xp,
+                   const double *xl, double *yu, double *yp, double *yl);
+
+/* SparseGMRES_ns_conv (SparseGMRES_NS_conv.f:1-406): one cycle with the
+ * 5-field block-Jacobi preconditioner (3x3 momentum inverse, 1/M with
+ * 0 -> 1, 1/LS with 0 -> 0).  solu/solp/solls (may be NULL) receive the
+ * increment, also kept as OFD_SOLU/OFD_SOLP/OFD_SOLPHI. */
+int ofd_gmres_nsvof(ofd_ctx *ctx, int precision, int ortho, double tol, int kspace,
+                    int kspace_min, double *solu, double *solp, double *solls, int *iters,
+                    double *resid);
+
+/* restarted SparseGMRES_ns_conv with an FP64 outer residual, as ofd_solve */
+int ofd_solve_nsvof(ofd_ctx *ctx, int precision, int ortho, double rtol, double inner_tol,
+                    int kspace, int max_cycles, int *iters, int *cycles, double *relres);
+
+/* Newton update (solveFlow.f90:99-104): acg += du; ug += gami*Delt*du;
+ * pg += alfi*gami*Delt*dp; rphig += dphi; phig += gami*Delt*dphi */
+int ofd_nsvof_update(ofd_ctx *ctx);
+
+/* |RHSGu|_2, |RHSGp|_2, |RHSGls|_2 */
+int ofd_nsvof_rhs_norms(ofd_ctx *ctx, double *norms3);
 
 /* interpolation_tet (interpolation.f90:5-76) on the device.  Receivers are
  * the points of a boundary mesh, xr and dr (nrecv,3) column-major (its xg

@@ -236,6 +236,18 @@ def gmres_up(col, row, A, rhsu, rhsp, tol, kspace, kspace_min=0):
                 resid=float(res[0]), rnorm0=float(r0[0]))
 
 
+def _qvec(qprm):
+    """The 20 material values of the C restatement (the 19 of the quench
+    material, then the fluid id, 102 when absent); entries beyond are
+    ignored (lsc_kdc goes in through ``ex``)."""
+    qp = np.asarray(qprm, np.float64)[:20]
+    q = np.zeros(20)
+    q[:qp.size] = qp
+    if qp.size < 20:
+        q[19] = 102
+    return q
+
+
 def assemble_tem(mesh, ibc_t, st, prm, qprm, col, row, tensor_flag, lhs=None, rhs=None):
     """IntElmAss_Tem_Quenching_STAB + BCLhs_tem + FillSparseMat_tem.  st holds
     the alpha-stage fields dg, ug, ugm (N,3) and phig, Tg, rTg (N); prm the
@@ -245,10 +257,7 @@ def assemble_tem(mesh, ibc_t, st, prm, qprm, col, row, tensor_flag, lhs=None, rh
     p = np.asarray(prm, np.float64)
     if p.size == 4:
         p = np.array([p[0], p[1], p[2], 0.0, p[3]])
-    q = np.zeros(20)
-    q[:len(qprm)] = qprm
-    if len(qprm) < 20:
-        q[19] = 102
+    q = _qvec(qprm)
     lhs = np.zeros(len(row)) if lhs is None else lhs
     rhs = np.zeros(N) if rhs is None else rhs
     nneg = lib().orc_assemble_tem(
@@ -272,10 +281,7 @@ def assemble_nsvof(mesh, ibc8, st, prm, qprm, ex, col, row, tensor_flag):
     p = np.asarray(prm, np.float64)
     if p.size == 4:
         p = np.array([p[0], p[1], p[2], 0.0, p[3]])
-    q = np.zeros(20)
-    q[:len(qprm)] = qprm
-    if len(qprm) < 20:
-        q[19] = 102
+    q = _qvec(qprm)
     Z = len(row)
     out = {k: np.zeros(Z * w) for k, w in _NSVOF_WIDTH.items()}
     rhsu = np.zeros(3 * N)

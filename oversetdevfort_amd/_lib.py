@@ -24,16 +24,17 @@ OFD_ORTHO_CGS1 = 2
 
 FIELDS = ["XG", "DG", "DGOLD", "UG", "UGOLD", "UGM", "UGMOLD", "ACG", "ACGOLD",
           "ACGM", "ACGMOLD", "PG", "PGOLD", "SOLU", "SOLP",
-          "PHIG", "PHIGOLD", "RPHIG", "RPHIGOLD", "TG", "TGOLD", "RTG", "RTGOLD", "SOLT"]
+          "PHIG", "PHIGOLD", "RPHIG", "RPHIGOLD", "TG", "TGOLD", "RTG", "RTGOLD", "SOLT",
+          "SOLPHI"]
 FIELD = {n: i for i, n in enumerate(FIELDS)}
 SCALAR_FIELDS = {"PG", "PGOLD", "SOLP", "PHIG", "PHIGOLD", "RPHIG", "RPHIGOLD", "TG",
-                 "TGOLD", "RTG", "RTGOLD", "SOLT"}
+                 "TGOLD", "RTG", "RTGOLD", "SOLT", "SOLPHI"}
 PARAMS = ["almi", "alfi", "gami", "beti", "delt", "rho", "mu", "kdc", "gx", "gy",
           "gz"]
 # material vector of ofd_set_quench_params
 QPARAMS = ["rhoa", "rhow", "mua", "muw", "cpa", "cpw", "kappaa", "kappaw", "rhos", "mus",
            "cps", "kappas", "Ts", "lh", "c_cond", "c_evap", "c_dmdot", "tem_kdc", "tref",
-           "fluid_id"]
+           "fluid_id", "lsc_kdc"]
 
 _c_int = ctypes.c_int
 _c_dbl = ctypes.c_double
@@ -83,6 +84,16 @@ SIGNATURES = {
                       _c_int),
     "ofd_tem_update": ([_vp], _c_int),
     "ofd_tem_rhs_norm": ([_vp, _dp], _c_int),
+    "ofd_assemble_nsvof": ([_vp, _c_int, _dp], _c_int),
+    "ofd_get_nsvof_system": ([_vp] + [_vp] * 11, _c_int),
+    "ofd_set_nsvof_system": ([_vp] + [_vp] * 11, _c_int),
+    "ofd_spmv_nsvof": ([_vp, _c_int] + [_vp] * 6, _c_int),
+    "ofd_gmres_nsvof": ([_vp, _c_int, _c_int, _c_dbl, _c_int, _c_int, _vp, _vp, _vp, _ip,
+                         _dp], _c_int),
+    "ofd_solve_nsvof": ([_vp, _c_int, _c_int, _c_dbl, _c_dbl, _c_int, _c_int, _ip, _ip,
+                         _dp], _c_int),
+    "ofd_nsvof_update": ([_vp], _c_int),
+    "ofd_nsvof_rhs_norms": ([_vp, _vp], _c_int),
     "ofd_fringe_interp": ([_vp, ctypes.c_int32, _c_int, _vp, _vp, _vp, _vp], _c_int),
     "ofd_apply_fringe": ([_vp, _c_int, _vp, _vp], _c_int),
     "ofd_time_spmv": ([_vp, _c_int, _c_int, _dp], _c_int),

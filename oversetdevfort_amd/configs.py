@@ -146,7 +146,8 @@ def apply_dirichlet(ug, acg, ugold, acgold, prob: Problem):
 QUENCH_MATERIAL = dict(rhoa=1.0, rhow=1000.0, mua=1.0e-5, muw=1.0e-3, cpa=2000.0,
                        cpw=4200.0, kappaa=0.025, kappaw=0.6, rhos=7800.0, mus=0.0,
                        cps=500.0, kappas=30.0, Ts=373.15, lh=2.26e6, c_cond=0.1,
-                       c_evap=0.1, c_dmdot=1.0, tem_kdc=0.0, tref=425.0, fluid_id=102)
+                       c_evap=0.1, c_dmdot=1.0, tem_kdc=0.0, tref=425.0, fluid_id=102,
+                       lsc_kdc=0.0)
 
 
 def quench(n=48, seed=1, rhoinf=0.5, jitter=0.0) -> Problem:

@@ -78,7 +78,8 @@ __global__ void k_alpha_stage(int N, const double *__restrict__ xg,
     r[9 + c] = ugmold[o] + alfi * (ugm[o] - ugmold[o]);
   }
   r[12] = pg[i];
-  // quenching scalars (solveQuenching.f90:33-37): phi, T at alfi, rT at almi
+  // quenching scalars (solveQuenching.f90:33-37): phi, T at alfi, and rT or
+  // rphi at almi
   if (phig) {
     r[13] = phigold[i] + alfi * (phig[i] - phigold[i]);
     r[14] = tgold[i] + alfi * (tg[i] - tgold[i]);
@@ -404,7 +405,9 @@ __global__ void k_max_reduce(const double *v, int n, double *out) {
   if (threadIdx.x == 0) *out = s[0];
 }
 
-hipError_t launch_alpha_stage(Ctx &c) {
+// the record's last slot holds rT (temperature equation) or, with rphi set,
+// the VOF rate rphi (coupled NS + VOF system)
+hipError_t launch_alpha_stage(Ctx &c, bool rphi) {
   const int N = c.nnode;
   const double almi = c.prm[OFD_P_ALMI], alfi = c.prm[OFD_P_ALFI];
   hipLaunchKernelGGL(k_alpha_stage, dim3((N + 255) / 256), dim3(256), 0, c.stream, N,
@@ -413,8 +416,9 @@ hipError_t launch_alpha_stage(Ctx &c) {
                      c.d_field[OFD_UGMOLD], c.d_field[OFD_ACG],
                      c.d_field[OFD_ACGOLD], c.d_field[OFD_PG],
                      c.quench ? c.d_field[OFD_PHIG] : nullptr, c.d_field[OFD_PHIGOLD],
-                     c.d_field[OFD_TG], c.d_field[OFD_TGOLD], c.d_field[OFD_RTG],
-                     c.d_field[OFD_RTGOLD], almi, alfi, c.d_noderec);
+                     c.d_field[OFD_TG], c.d_field[OFD_TGOLD],
+                     c.d_field[rphi ? OFD_RPHIG : OFD_RTG],
+                     c.d_field[rphi ? OFD_RPHIGOLD : OFD_RTGOLD], almi, alfi, c.d_noderec);
   return hipGetLastError();
 }
 

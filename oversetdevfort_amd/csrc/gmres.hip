@@ -145,9 +145,12 @@ __device__ __forceinline__ void axpy_batches(const V *__restrict__ W, size_t vs,
 // W_0 = P^-1 b and partials of |W_0|^2 (SparseGMRES_NS.f90:124-155); block 0
 // also resets the cycle scalars.  The first orthogonalisation kernel of
 // iteration 0 closes the norm.
-// SCALAR: the T system, P = inverse diagonal of the nsc scalar unknowns,
-// taken 4 at a time (entries past nsc are 0)
-template <typename V, bool SCALAR>
+// KIND 0: NS node vectors, P = [N][16] block-Jacobi inverse;
+// KIND 1: the T system, P = inverse diagonal of the nsc scalar unknowns;
+// KIND 2: the NSVOF system, [N][5] node values, P = [N][12]: 3x3 momentum
+//         inverse, 1/M, 1/LS.
+// Kinds 1 and 2 are taken 4 values at a time; entries past nsc are 0.
+template <typename V, int KIND>
 __global__ void __launch_bounds__(256)
 k_gmres_init(int N, int nsc, const double *__restrict__ P, const double *__restrict__ b,
              V *__restrict__ W0, double *__restrict__ part, GmresState *__restrict__ g,
@@ -176,11 +179,29 @@ k_gmres_init(int N, int nsc, const double *__restrict__ P, const double *__restr
        i += gridDim.x * blockDim.x) {
     double r[4], z[4];
     ld4(b + (size_t)i * 4, r);
-    if (SCALAR) {
+    if (KIND == 1) {
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         const int t = 4 * i + c;
         z[c] = (t < nsc) ? P[t] * r[c] : 0.0;
+      }
+    } else if (KIND == 2) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int t = 4 * i + c;
+        double v = 0.0;
+        if (t < nsc) {
+          const int nd = t / 5, cc = t - 5 * nd;
+          const double *p = P + (size_t)nd * 12;
+          const double *bb = b + (size_t)nd * 5;
+          if (cc < 3)
+            v = p[cc * 3 + 0] * bb[0] + p[cc * 3 + 1] * bb[1] + p[cc * 3 + 2] * bb[2];
+          else if (cc == 3)
+            v = bb[3] * p[9];
+          else
+            v = bb[4] * p[10];
+        }
+        z[c] = v;
       }
     } else {
       const double *p = P + (size_t)i * 16;
@@ -393,12 +414,27 @@ k_norm2_up(int N, const double *__restrict__ v, double *__restrict__ part) {
   block_partials<2>(acc, 2, part, 0);
 }
 
-// scal[slot], scal[slot+1] = sqrt of the two summed partial columns
+// |u|^2, |p|^2, |phi|^2 partials of an [N][5] vector
 __global__ void __launch_bounds__(256)
-k_norm_fin2(const double *__restrict__ part, int nb, GmresState *__restrict__ g, int slot) {
-  __shared__ double s_sum[2];
-  final_sums(part, nb, 2, s_sum);
-  if (threadIdx.x < 2) g->scal[slot + threadIdx.x] = sqrt(s_sum[threadIdx.x]);
+k_norm2_v(int N, const double *__restrict__ v, double *__restrict__ part) {
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += gridDim.x * blockDim.x) {
+    const double *z = v + (size_t)i * 5;
+    acc[0] += z[0] * z[0] + z[1] * z[1] + z[2] * z[2];
+    acc[1] += z[3] * z[3];
+    acc[2] += z[4] * z[4];
+  }
+  block_partials<3>(acc, 3, part, 0);
+}
+
+// scal[slot + j] = sqrt of summed partial column j < m
+__global__ void __launch_bounds__(256)
+k_norm_fin2(const double *__restrict__ part, int nb, GmresState *__restrict__ g, int slot,
+            int m) {
+  __shared__ double s_sum[3];
+  final_sums(part, nb, m, s_sum);
+  if ((int)threadIdx.x < m) g->scal[slot + threadIdx.x] = sqrt(s_sum[threadIdx.x]);
 }
 
 __global__ void k_outer_reset(GmresState *__restrict__ g, double thr) {
@@ -490,26 +526,28 @@ static double *part_buf(Ctx &c, int which) {
 
 hipError_t launch_gmres_init(Ctx &c, const double *b, double tol, int kspace, int kmin) {
   const int nb = nblocks_gm(c);
-  const double *P = c.gm_scalar ? c.d_PT : c.d_P64;
+  const int kind = c.gm_kind;
+  const double *P = kind == 1 ? c.d_PT : (kind == 2 ? c.d_PV : c.d_P64);
+  const int nsc = kind == 2 ? 5 * c.nnode : c.nnode;
+#define OFD_INIT(V, K, W)                                                                   \
+  hipLaunchKernelGGL((k_gmres_init<V, K>), dim3(nb), dim3(kRedBlock), 0, c.stream, c.gm_n, \
+                     nsc, P, b, W, part_buf(c, 0), c.d_gs, tol, kspace, kmin)
   if (c.basis32) {
-    if (c.gm_scalar)
-      hipLaunchKernelGGL((k_gmres_init<float, true>), dim3(nb), dim3(kRedBlock), 0, c.stream,
-                         c.gm_n, c.nnode, P, b, c.d_W32, part_buf(c, 0), c.d_gs, tol,
-                         kspace, kmin);
+    if (kind == 1)
+      OFD_INIT(float, 1, c.d_W32);
+    else if (kind == 2)
+      OFD_INIT(float, 2, c.d_W32);
     else
-      hipLaunchKernelGGL((k_gmres_init<float, false>), dim3(nb), dim3(kRedBlock), 0,
-                         c.stream, c.gm_n, c.nnode, P, b, c.d_W32, part_buf(c, 0), c.d_gs,
-                         tol, kspace, kmin);
+      OFD_INIT(float, 0, c.d_W32);
   } else {
-    if (c.gm_scalar)
-      hipLaunchKernelGGL((k_gmres_init<double, true>), dim3(nb), dim3(kRedBlock), 0,
-                         c.stream, c.gm_n, c.nnode, P, b, c.d_W, part_buf(c, 0), c.d_gs,
-                         tol, kspace, kmin);
+    if (kind == 1)
+      OFD_INIT(double, 1, c.d_W);
+    else if (kind == 2)
+      OFD_INIT(double, 2, c.d_W);
     else
-      hipLaunchKernelGGL((k_gmres_init<double, false>), dim3(nb), dim3(kRedBlock), 0,
-                         c.stream, c.gm_n, c.nnode, P, b, c.d_W, part_buf(c, 0), c.d_gs,
-                         tol, kspace, kmin);
+      OFD_INIT(double, 0, c.d_W);
   }
+#undef OFD_INIT
   c.part_cur = 0;
   return hipGetLastError();
 }
@@ -574,7 +612,19 @@ hipError_t launch_norm2_up(Ctx &c, const double *v, int slot) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_norm_fin2, dim3(1), dim3(kRedBlock), 0, c.stream, pb, nb, c.d_gs,
-                     slot);
+                     slot, 2);
+  return hipGetLastError();
+}
+
+// |u|, |p|, |phi| of an [N][5] vector into GmresState::scal[1..3]
+hipError_t launch_norm2_v(Ctx &c, const double *v) {
+  const int nb = nblocks_nodes(c);
+  double *pb = part_buf(c, 0);
+  hipLaunchKernelGGL(k_norm2_v, dim3(nb), dim3(kRedBlock), 0, c.stream, c.nnode, v, pb);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_norm_fin2, dim3(1), dim3(kRedBlock), 0, c.stream, pb, nb, c.d_gs, 1,
+                     3);
   return hipGetLastError();
 }
 

@@ -94,15 +94,23 @@ static int ensure_krylov(Ctx &c, int K) {
     OFD_HIP(hipHostMalloc(reinterpret_cast<void **>(&c.h_gs_ring), sizeof(GmresState) * 2));
     for (auto &ev : c.cyc_ev) OFD_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
   }
+  // vectors of the largest system seen so far (NS, T or NSVOF tuples)
+  const int nt = c.gm_n > c.kalloc_n ? c.gm_n : c.kalloc_n;
+  if (nt > c.kalloc_n) {
+    dfree(c.d_W);
+    dfree(c.d_W32);
+    c.kalloc = c.kalloc32 = 0;
+    c.kalloc_n = nt;
+  }
   if (c.basis32) {
     if (c.kalloc32 >= K) return OFD_OK;
-    int rc = dalloc(c, &c.d_W32, (size_t)(K + 1) * c.nnode * 4);
+    int rc = dalloc(c, &c.d_W32, (size_t)(K + 1) * c.kalloc_n * 4);
     if (rc) return rc;
     c.kalloc32 = K;
     return OFD_OK;
   }
   if (c.kalloc >= K) return OFD_OK;
-  int rc = dalloc(c, &c.d_W, (size_t)(K + 1) * c.nnode * 4);
+  int rc = dalloc(c, &c.d_W, (size_t)(K + 1) * c.kalloc_n * 4);
   if (rc) return rc;
   c.kalloc = K;
   return OFD_OK;
@@ -206,14 +214,46 @@ static int ensure_prec_s(Ctx &c, int precision) {
 struct TemSystem {
   Ctx &c;
   explicit TemSystem(Ctx &cc) : c(cc) {
-    c.gm_scalar = true;
+    c.gm_kind = 1;
     c.gm_n = (c.nnode + 3) / 4;
   }
   ~TemSystem() {
-    c.gm_scalar = false;
+    c.gm_kind = 0;
     c.gm_n = c.nnode;
   }
 };
+
+// the NSVOF system's [N][5] vectors as ceil(5N/4) 4-tuples
+struct NsvofSystem {
+  Ctx &c;
+  explicit NsvofSystem(Ctx &cc) : c(cc) {
+    c.gm_kind = 2;
+    c.gm_n = (5 * c.nnode + 3) / 4;
+  }
+  ~NsvofSystem() {
+    c.gm_kind = 0;
+    c.gm_n = c.nnode;
+  }
+};
+
+// the NSVOF preconditioner: block inverses and P^-1 A (FP64, FP32)
+static int ensure_prec_v(Ctx &c, int precision) {
+  OFD_CHECK(c.matV_valid, OFD_ERR_ARG, "no assembled NSVOF matrix");
+  const bool want32 = precision == OFD_FP32;
+  if (c.precV_valid && (!want32 || c.apV32_valid)) return OFD_OK;
+  if (!c.d_ApV) {
+    int rc = dalloc(c, &c.d_ApV, (size_t)c.icnt * 24);
+    if (rc) return rc;
+  }
+  if (want32 && !c.d_ApV32) {
+    int rc = dalloc(c, &c.d_ApV32, (size_t)c.icnt * 24);
+    if (rc) return rc;
+  }
+  OFD_HIP(launch_build_prec_v(c, want32));
+  c.precV_valid = true;
+  c.apV32_valid = want32;
+  return OFD_OK;
+}
 
 // one GMRES cycle on right-hand side b (device [N][4]); x += increment.
 // flag_set selects the step-flag set (cycles queued back to back alternate).
@@ -224,7 +264,8 @@ static int gmres_cycle(Ctx &c, int precision, int ortho, const double *b, double
   c.basis32 = precision == OFD_FP32 && ortho != OFD_ORTHO_MGS;
   int rc = ensure_krylov(c, K);
   if (rc) return rc;
-  rc = c.gm_scalar ? ensure_prec_s(c, precision) : ensure_prec(c, precision);
+  rc = c.gm_kind == 1 ? ensure_prec_s(c, precision)
+                      : (c.gm_kind == 2 ? ensure_prec_v(c, precision) : ensure_prec(c, precision));
   if (rc) return rc;
   OFD_HIP(launch_gmres_init(c, b, tol, K, kmin));
   // The host enqueues iterations ahead of the device and learns about
@@ -381,6 +422,15 @@ int ofd_ctx_destroy(ofd_ctx *x) {
   dfree(c.d_W32);
   dfree(c.d_part);
   dfree(c.d_gs);
+  dfree(c.d_AV);
+  dfree(c.d_AV32);
+  dfree(c.d_PV);
+  dfree(c.d_ApV);
+  dfree(c.d_ApV32);
+  dfree(c.d_rhsV);
+  dfree(c.d_xV);
+  dfree(c.d_rV);
+  dfree(c.d_yV);
   dfree(c.d_AT);
   dfree(c.d_AT32);
   dfree(c.d_PT);
@@ -432,7 +482,9 @@ int ofd_upload_mesh(ofd_ctx *x, int nnode, int nelem, int nshl, const int32_t *i
   c.nelem = nelem;
   c.nshl = nshl;
   c.gm_n = nnode;
-  c.gm_scalar = false;
+  c.gm_kind = 0;
+  c.matV_valid = c.precV_valid = c.aV32_valid = c.apV32_valid = false;
+  c.nsvV_n4 = 0;
   c.matT_valid = c.precT_valid = c.aT32_valid = c.apT32_valid = false;
   c.temT_n4 = 0;
   c.icnt = 0;
@@ -628,6 +680,12 @@ int ofd_apply_dirichlet(ofd_ctx *x) {
   return OFD_OK;
 }
 
+// SOLU / SOLP (/ SOLPHI) are views of the last solved system's increment
+static hipError_t sol_view(Ctx &c) {
+  if (c.sol_src == 2 && c.d_xV) return launch_node5_to_soa(c, c.d_xV);
+  return launch_node4_to_soa(c, c.d_x, c.d_field[OFD_SOLU], c.d_field[OFD_SOLP]);
+}
+
 int ofd_set_field(ofd_ctx *x, int field, const double *host) {
   OFD_CHECK(x && host, OFD_ERR_ARG, "null argument");
   OFD_CHECK(field >= 0 && field < OFD_NFIELDS, OFD_ERR_ARG, "unknown field");
@@ -638,6 +696,7 @@ int ofd_set_field(ofd_ctx *x, int field, const double *host) {
   rc = h2d(c, c.d_field[field], host, (size_t)field_len(c, field) * sizeof(double));
   if (rc) return rc;
   if (field == OFD_SOLU || field == OFD_SOLP) {
+    c.sol_src = 0;
     OFD_HIP(launch_soa_to_node4(c, c.d_field[OFD_SOLU], c.d_field[OFD_SOLP], c.d_x));
     OFD_HIP(hipStreamSynchronize(c.stream));
   }
@@ -651,8 +710,7 @@ int ofd_get_field(ofd_ctx *x, int field, double *host) {
   int rc = need_mesh(c);
   if (rc) return rc;
   OFD_HIP(hipSetDevice(c.device));
-  if (field == OFD_SOLU || field == OFD_SOLP)
-    OFD_HIP(launch_node4_to_soa(c, c.d_x, c.d_field[OFD_SOLU], c.d_field[OFD_SOLP]));
+  if (field == OFD_SOLU || field == OFD_SOLP || field == OFD_SOLPHI) OFD_HIP(sol_view(c));
   return d2h(c, host, c.d_field[field], (size_t)field_len(c, field) * sizeof(double));
 }
 
@@ -703,7 +761,7 @@ int ofd_get_rhs(ofd_ctx *x, double *rhsu, double *rhsp) {
   rc = d2h(c, rhsu, c.d_field[OFD_SOLU], (size_t)c.nnode * 3 * sizeof(double));
   if (!rc) rc = d2h(c, rhsp, c.d_field[OFD_SOLP], (size_t)c.nnode * sizeof(double));
   // restore the solution view of the scratch fields
-  OFD_HIP(launch_node4_to_soa(c, c.d_x, c.d_field[OFD_SOLU], c.d_field[OFD_SOLP]));
+  OFD_HIP(sol_view(c));
   OFD_HIP(hipStreamSynchronize(c.stream));
   return rc;
 }
@@ -791,7 +849,7 @@ int ofd_spmv_up(ofd_ctx *x, int precision, const double *xu, const double *xp,
   OFD_HIP(launch_node4_to_soa(c, c.d_r, su, sp));
   rc = d2h(c, yu, su, (size_t)c.nnode * 3 * sizeof(double));
   if (!rc) rc = d2h(c, yp, sp, (size_t)c.nnode * sizeof(double));
-  OFD_HIP(launch_node4_to_soa(c, c.d_x, su, sp));
+  OFD_HIP(sol_view(c));
   OFD_HIP(hipStreamSynchronize(c.stream));
   return rc;
 }
@@ -815,6 +873,7 @@ int ofd_gmres_up(ofd_ctx *x, int precision, int ortho, double tol, int kspace,
   if (iters) *iters = it;
   if (resid) *resid = res;
   if (rc) return rc;
+  c.sol_src = 0;
   OFD_HIP(launch_node4_to_soa(c, c.d_x, c.d_field[OFD_SOLU], c.d_field[OFD_SOLP]));
   if (solu) rc = d2h(c, solu, c.d_field[OFD_SOLU], (size_t)c.nnode * 3 * sizeof(double));
   if (!rc && solp) rc = d2h(c, solp, c.d_field[OFD_SOLP], (size_t)c.nnode * sizeof(double));
@@ -854,8 +913,10 @@ static int solve_core(Ctx &c, int precision, int ortho, const double *b, double 
                        nullptr, false, cyc & 1);
       if (rc) break;
       // FP64 outer residual r = b - A x and its norm
-      if (c.gm_scalar)
+      if (c.gm_kind == 1)
         OFD_HIP(launch_spmv_s(c, OFD_FP64, x, r, b, 1));
+      else if (c.gm_kind == 2)
+        OFD_HIP(launch_spmv_v(c, OFD_FP64, x, r, b, 1));
       else
         OFD_HIP(launch_spmv(c, OFD_FP64, x, r, b, 1));
       OFD_HIP(launch_norm2(c, r, 0, 1));
@@ -913,6 +974,7 @@ int ofd_solve(ofd_ctx *x, int precision, int ortho, double rtol, double inner_to
   rc = solve_core(c, precision, ortho, c.d_rhs, c.d_x, c.d_r, rtol, inner_tol, kspace,
                   max_cycles, iters, cycles, relres);
   if (rc && rc != OFD_ERR_NOCONV) return rc;
+  c.sol_src = 0;
   OFD_HIP(launch_node4_to_soa(c, c.d_x, c.d_field[OFD_SOLU], c.d_field[OFD_SOLP]));
   OFD_HIP(hipStreamSynchronize(c.stream));
   return rc;
@@ -1253,6 +1315,296 @@ int ofd_tem_rhs_norm(ofd_ctx *x, double *norm2) {
   OFD_HIP(hipSetDevice(c.device));
   TemSystem sys(c);
   return norm2_dev(c, c.d_rhsT, norm2);
+}
+
+// ---- coupled NS + VOF system ----
+
+static int ensure_nsvof(Ctx &c) {
+  int rc = need_pattern(c);
+  if (rc) return rc;
+  OFD_CHECK((size_t)5 * c.nnode <= (size_t)4 * kMaxRedBlocks * kRedBlock, OFD_ERR_UNSUPPORTED,
+            "the NSVOF system supports at most 838,860 nodes per context");
+  const int n4 = (5 * c.nnode + 3) / 4;
+  if (c.nsvV_icnt != c.icnt) {
+    rc = dalloc(c, &c.d_AV, (size_t)c.icnt * 24);
+    if (!rc) rc = dalloc(c, &c.d_AV32, (size_t)0);
+    if (!rc) rc = dalloc(c, &c.d_ApV, (size_t)0);
+    if (!rc) rc = dalloc(c, &c.d_ApV32, (size_t)0);
+    if (rc) return rc;
+    OFD_HIP(hipMemsetAsync(c.d_AV, 0, (size_t)c.icnt * 24 * sizeof(double), c.stream));
+    c.nsvV_icnt = c.icnt;
+    c.matV_valid = c.precV_valid = c.aV32_valid = c.apV32_valid = false;
+  }
+  if (c.nsvV_n4 != n4) {
+    const size_t np = (size_t)n4 * 4;
+    rc = dalloc(c, &c.d_rhsV, np);
+    if (!rc) rc = dalloc(c, &c.d_xV, np);
+    if (!rc) rc = dalloc(c, &c.d_rV, np);
+    if (!rc) rc = dalloc(c, &c.d_yV, np);
+    if (!rc) rc = dalloc(c, &c.d_PV, (size_t)c.nnode * 12);
+    if (rc) return rc;
+    OFD_HIP(hipMemsetAsync(c.d_rhsV, 0, np * sizeof(double), c.stream));
+    OFD_HIP(hipMemsetAsync(c.d_xV, 0, np * sizeof(double), c.stream));
+    OFD_HIP(hipMemsetAsync(c.d_rV, 0, np * sizeof(double), c.stream));
+    OFD_HIP(hipMemsetAsync(c.d_yV, 0, np * sizeof(double), c.stream));
+    c.nsvV_n4 = n4;
+  }
+  return OFD_OK;
+}
+
+static int ensure_aV32(Ctx &c) {
+  OFD_CHECK(c.matV_valid, OFD_ERR_ARG, "no assembled NSVOF matrix");
+  if (c.aV32_valid) return OFD_OK;
+  if (!c.d_AV32) {
+    int rc = dalloc(c, &c.d_AV32, (size_t)c.icnt * 24);
+    if (rc) return rc;
+  }
+  OFD_HIP(launch_convert_aV32(c));
+  c.aV32_valid = true;
+  return OFD_OK;
+}
+
+// reference arrays <-> the 24-entry block of a nonzero
+static void nsvof_to_block(int64_t k, const double *k11, const double *g, const double *d1,
+                           const double *m, const double *ls, const double *lsu,
+                           const double *pls, const double *uls, double *b) {
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) b[5 * r + c] = k11[k * 9 + r * 3 + c];
+    b[5 * r + 3] = g[k * 3 + r];
+    b[5 * r + 4] = uls[k * 3 + r];
+  }
+  for (int c = 0; c < 3; ++c) b[15 + c] = d1[k * 3 + c];
+  b[18] = m[k];
+  b[19] = pls[k];
+  for (int c = 0; c < 3; ++c) b[20 + c] = lsu[k * 3 + c];
+  b[23] = ls[k];
+}
+
+int ofd_assemble_nsvof(ofd_ctx *x, int tensor_flag, double *cfl) {
+  OFD_CHECK(x, OFD_ERR_ARG, "null context");
+  Ctx &c = x->c;
+  int rc = ensure_nsvof(c);
+  if (rc) return rc;
+  OFD_CHECK(c.quench, OFD_ERR_ARG, "no quench parameters (ofd_set_quench_params)");
+  OFD_CHECK(c.nshl == 8, OFD_ERR_UNSUPPORTED,
+            "the GPU element assembly is implemented for trilinear hexes (NSHL=8)");
+  OFD_CHECK((tensor_flag & ~(OFD_ASSEMBLE_VEC | OFD_ASSEMBLE_MAT)) == 0, OFD_ERR_ARG,
+            "tensor_flag must combine OFD_ASSEMBLE_VEC / OFD_ASSEMBLE_MAT");
+  OFD_HIP(hipSetDevice(c.device));
+  OFD_HIP(launch_alpha_stage(c, true));
+  if (tensor_flag & OFD_ASSEMBLE_VEC)
+    OFD_HIP(launch_zero(c, c.d_rhsV, (size_t)c.nsvV_n4 * 4));
+  if (tensor_flag & OFD_ASSEMBLE_MAT) {
+    OFD_HIP(launch_zero(c, c.d_AV, (size_t)c.icnt * 24));
+    c.matV_valid = true;
+    c.precV_valid = c.aV32_valid = c.apV32_valid = false;
+  }
+  OFD_HIP(hipMemsetAsync(c.d_negdet, 0, sizeof(int32_t), c.stream));
+  for (int col = 0; col + 1 < (int)c.color_off.size(); ++col)
+    OFD_HIP(launch_assemble_nsvof_hex(c, tensor_flag, col));
+  double *d_out = c.d_part;  // scratch for the CFL maximum
+  OFD_HIP(launch_max_reduce(c, c.d_cfl, c.nelem, d_out));
+  double h[1] = {0.0};
+  int32_t neg = 0;
+  OFD_HIP(hipMemcpyAsync(h, d_out, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  OFD_HIP(hipMemcpyAsync(&neg, c.d_negdet, sizeof(int32_t), hipMemcpyDeviceToHost,
+                         c.stream));
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  if (cfl) *cfl = h[0];
+  if (neg > 0) {
+    set_error("negative determinant Jacobian at " + std::to_string(neg) +
+              " Gauss points (eval_shape_fem warning)");
+    return OFD_ERR_NEGDETJ;
+  }
+  return OFD_OK;
+}
+
+int ofd_get_nsvof_system(ofd_ctx *x, double *k11, double *g, double *d1, double *m,
+                         double *ls, double *lsu, double *pls, double *uls, double *rhsu,
+                         double *rhsp, double *rhsls) {
+  OFD_CHECK(x, OFD_ERR_ARG, "null context");
+  Ctx &c = x->c;
+  int rc = ensure_nsvof(c);
+  if (rc) return rc;
+  OFD_HIP(hipSetDevice(c.device));
+  const int64_t Z = c.icnt;
+  const int N = c.nnode;
+  if (k11 || g || d1 || m || ls || lsu || pls || uls) {
+    std::vector<double> A((size_t)Z * 24);
+    rc = d2h(c, A.data(), c.d_AV, A.size() * sizeof(double));
+    if (rc) return rc;
+    for (int64_t k = 0; k < Z; ++k) {
+      const double *b = A.data() + k * 24;
+      for (int r = 0; r < 3; ++r) {
+        for (int cc = 0; cc < 3; ++cc)
+          if (k11) k11[k * 9 + r * 3 + cc] = b[5 * r + cc];
+        if (g) g[k * 3 + r] = b[5 * r + 3];
+        if (uls) uls[k * 3 + r] = b[5 * r + 4];
+        if (d1) d1[k * 3 + r] = b[15 + r];
+        if (lsu) lsu[k * 3 + r] = b[20 + r];
+      }
+      if (m) m[k] = b[18];
+      if (pls) pls[k] = b[19];
+      if (ls) ls[k] = b[23];
+    }
+  }
+  if (rhsu || rhsp || rhsls) {
+    std::vector<double> r((size_t)N * 5);
+    rc = d2h(c, r.data(), c.d_rhsV, r.size() * sizeof(double));
+    if (rc) return rc;
+    for (int i = 0; i < N; ++i) {
+      if (rhsu)
+        for (int cc = 0; cc < 3; ++cc) rhsu[(size_t)cc * N + i] = r[(size_t)i * 5 + cc];
+      if (rhsp) rhsp[i] = r[(size_t)i * 5 + 3];
+      if (rhsls) rhsls[i] = r[(size_t)i * 5 + 4];
+    }
+  }
+  return OFD_OK;
+}
+
+int ofd_set_nsvof_system(ofd_ctx *x, const double *k11, const double *g, const double *d1,
+                         const double *m, const double *ls, const double *lsu,
+                         const double *pls, const double *uls, const double *rhsu,
+                         const double *rhsp, const double *rhsls) {
+  OFD_CHECK(x, OFD_ERR_ARG, "null context");
+  Ctx &c = x->c;
+  int rc = ensure_nsvof(c);
+  if (rc) return rc;
+  OFD_HIP(hipSetDevice(c.device));
+  const int64_t Z = c.icnt;
+  const int N = c.nnode;
+  if (k11) {
+    OFD_CHECK(g && d1 && m && ls && lsu && pls && uls, OFD_ERR_ARG,
+              "the matrix needs all eight arrays");
+    std::vector<double> A((size_t)Z * 24);
+    for (int64_t k = 0; k < Z; ++k)
+      nsvof_to_block(k, k11, g, d1, m, ls, lsu, pls, uls, A.data() + k * 24);
+    rc = h2d(c, c.d_AV, A.data(), A.size() * sizeof(double));
+    if (rc) return rc;
+    c.matV_valid = true;
+    c.precV_valid = c.aV32_valid = c.apV32_valid = false;
+  }
+  if (rhsu) {
+    OFD_CHECK(rhsp && rhsls, OFD_ERR_ARG, "the right-hand side needs all three arrays");
+    std::vector<double> r((size_t)N * 5);
+    for (int i = 0; i < N; ++i) {
+      for (int cc = 0; cc < 3; ++cc) r[(size_t)i * 5 + cc] = rhsu[(size_t)cc * N + i];
+      r[(size_t)i * 5 + 3] = rhsp[i];
+      r[(size_t)i * 5 + 4] = rhsls[i];
+    }
+    rc = h2d(c, c.d_rhsV, r.data(), r.size() * sizeof(double));
+  }
+  return rc;
+}
+
+int ofd_spmv_nsvof(ofd_ctx *x, int precision, const double *xu, const double *xp,
+                   const double *xl, double *yu, double *yp, double *yl) {
+  OFD_CHECK(x && xu && xp && xl && yu && yp && yl, OFD_ERR_ARG, "null argument");
+  OFD_CHECK(precision == OFD_FP64 || precision == OFD_FP32, OFD_ERR_ARG, "bad precision");
+  Ctx &c = x->c;
+  int rc = ensure_nsvof(c);
+  if (rc) return rc;
+  OFD_CHECK(c.matV_valid, OFD_ERR_ARG, "no assembled NSVOF matrix");
+  OFD_HIP(hipSetDevice(c.device));
+  if (precision == OFD_FP32) {
+    rc = ensure_aV32(c);
+    if (rc) return rc;
+  }
+  const int N = c.nnode;
+  std::vector<double> v((size_t)N * 5);
+  for (int i = 0; i < N; ++i) {
+    for (int cc = 0; cc < 3; ++cc) v[(size_t)i * 5 + cc] = xu[(size_t)cc * N + i];
+    v[(size_t)i * 5 + 3] = xp[i];
+    v[(size_t)i * 5 + 4] = xl[i];
+  }
+  NsvofSystem sys(c);
+  rc = h2d(c, c.d_yV, v.data(), v.size() * sizeof(double));
+  if (rc) return rc;
+  OFD_HIP(launch_spmv_v(c, precision, c.d_yV, c.d_rV, nullptr, 0));
+  rc = d2h(c, v.data(), c.d_rV, v.size() * sizeof(double));
+  if (rc) return rc;
+  for (int i = 0; i < N; ++i) {
+    for (int cc = 0; cc < 3; ++cc) yu[(size_t)cc * N + i] = v[(size_t)i * 5 + cc];
+    yp[i] = v[(size_t)i * 5 + 3];
+    yl[i] = v[(size_t)i * 5 + 4];
+  }
+  return OFD_OK;
+}
+
+int ofd_gmres_nsvof(ofd_ctx *x, int precision, int ortho, double tol, int kspace,
+                    int kspace_min, double *solu, double *solp, double *solls, int *iters,
+                    double *resid) {
+  OFD_CHECK(x, OFD_ERR_ARG, "null context");
+  OFD_CHECK(precision == OFD_FP64 || precision == OFD_FP32, OFD_ERR_ARG, "bad precision");
+  OFD_CHECK(ortho == OFD_ORTHO_MGS || ortho == OFD_ORTHO_CGS2 || ortho == OFD_ORTHO_CGS1,
+            OFD_ERR_ARG, "bad ortho");
+  Ctx &c = x->c;
+  int rc = ensure_nsvof(c);
+  if (rc) return rc;
+  OFD_HIP(hipSetDevice(c.device));
+  NsvofSystem sys(c);
+  OFD_HIP(hipMemsetAsync(c.d_xV, 0, (size_t)c.gm_n * 4 * sizeof(double), c.stream));
+  OFD_HIP(launch_outer_reset(c, 0.0));
+  int it = 0;
+  double res = 0.0;
+  rc = gmres_cycle(c, precision, ortho, c.d_rhsV, tol, kspace, kspace_min, c.d_xV, &it, &res,
+                   nullptr);
+  if (iters) *iters = it;
+  if (resid) *resid = res;
+  if (rc) return rc;
+  c.sol_src = 2;
+  OFD_HIP(launch_node5_to_soa(c, c.d_xV));
+  const size_t nb = (size_t)c.nnode * sizeof(double);
+  if (solu) rc = d2h(c, solu, c.d_field[OFD_SOLU], 3 * nb);
+  if (!rc && solp) rc = d2h(c, solp, c.d_field[OFD_SOLP], nb);
+  if (!rc && solls) rc = d2h(c, solls, c.d_field[OFD_SOLPHI], nb);
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  return rc;
+}
+
+int ofd_solve_nsvof(ofd_ctx *x, int precision, int ortho, double rtol, double inner_tol,
+                    int kspace, int max_cycles, int *iters, int *cycles, double *relres) {
+  OFD_CHECK(x, OFD_ERR_ARG, "null context");
+  OFD_CHECK(precision == OFD_FP64 || precision == OFD_FP32, OFD_ERR_ARG, "bad precision");
+  OFD_CHECK(max_cycles >= 1, OFD_ERR_ARG, "max_cycles >= 1");
+  Ctx &c = x->c;
+  int rc = ensure_nsvof(c);
+  if (rc) return rc;
+  OFD_CHECK(c.matV_valid, OFD_ERR_ARG, "no assembled NSVOF matrix");
+  OFD_HIP(hipSetDevice(c.device));
+  NsvofSystem sys(c);
+  rc = solve_core(c, precision, ortho, c.d_rhsV, c.d_xV, c.d_rV, rtol, inner_tol, kspace,
+                  max_cycles, iters, cycles, relres);
+  if (rc && rc != OFD_ERR_NOCONV) return rc;
+  c.sol_src = 2;
+  OFD_HIP(launch_node5_to_soa(c, c.d_xV));
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  return rc;
+}
+
+int ofd_nsvof_update(ofd_ctx *x) {
+  OFD_CHECK(x, OFD_ERR_ARG, "null context");
+  Ctx &c = x->c;
+  int rc = ensure_nsvof(c);
+  if (rc) return rc;
+  OFD_HIP(hipSetDevice(c.device));
+  OFD_HIP(launch_fields_v(c));
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  return OFD_OK;
+}
+
+int ofd_nsvof_rhs_norms(ofd_ctx *x, double *norms3) {
+  OFD_CHECK(x && norms3, OFD_ERR_ARG, "null argument");
+  Ctx &c = x->c;
+  int rc = ensure_nsvof(c);
+  if (rc) return rc;
+  OFD_HIP(hipSetDevice(c.device));
+  OFD_HIP(launch_norm2_v(c, c.d_rhsV));
+  OFD_HIP(hipMemcpyAsync(c.h_gs, c.d_gs, sizeof(GmresState), hipMemcpyDeviceToHost,
+                         c.stream));
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  for (int j = 0; j < 3; ++j) norms3[j] = c.h_gs->scal[1 + j];
+  return OFD_OK;
 }
 
 static int ensure_fringe(Ctx &c, int nrecv) {

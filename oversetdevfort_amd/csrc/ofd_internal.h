@@ -140,10 +140,25 @@ struct Ctx {
   double *d_yT = nullptr;             // [4 * ceil(N/4)] product input scratch
   int64_t temT_icnt = 0;              // pattern size LHSTem was allocated for
   int temT_n4 = 0;                    // 4-tuples the T vectors were allocated for
-  // the system the GMRES kernels run on: the NS node vectors [N][4]
-  // (gm_n = N) or the T vector as ceil(N/4) 4-tuples (gm_scalar)
+  // NSVOF system: 24 entries per nonzero (rows u0 u1 u2 p phi), node
+  // vectors [N][5] zero-padded to whole 4-tuples
+  double *d_AV = nullptr;             // [icnt][24]
+  float *d_AV32 = nullptr;
+  double *d_PV = nullptr;             // [N][12] momentum 3x3 inverse, 1/M, 1/LS
+  double *d_ApV = nullptr;            // [icnt][24] P^-1 A
+  float *d_ApV32 = nullptr;
+  bool matV_valid = false, precV_valid = false, aV32_valid = false, apV32_valid = false;
+  double *d_rhsV = nullptr, *d_xV = nullptr, *d_rV = nullptr, *d_yV = nullptr;
+  int64_t nsvV_icnt = 0;
+  int nsvV_n4 = 0;
+  // the system the GMRES kernels run on: 0 the NS node vectors [N][4]
+  // (gm_n = N), 1 the T vector as ceil(N/4) 4-tuples, 2 the NSVOF vectors
+  // as ceil(5N/4) 4-tuples
   int gm_n = 0;
-  bool gm_scalar = false;
+  int gm_kind = 0;
+  int kalloc_n = 0;                   // tuples per Krylov vector allocated
+  // which increment SOLU / SOLP (/ SOLPHI) show: 0 the NS d_x, 2 the NSVOF d_xV
+  int sol_src = 0;
   // overset fringe: per-element boxes, source flags and geometry; receivers
   int fr_alloc_e = 0, fr_alloc_r = 0;
   double *d_fr_box = nullptr;         // [E][6] widened bounding box
@@ -164,7 +179,7 @@ struct Ctx {
 
 // ---- launchers (defined in the .hip files) ----
 // assembly
-hipError_t launch_alpha_stage(Ctx &c);
+hipError_t launch_alpha_stage(Ctx &c, bool rphi = false);
 hipError_t launch_assemble_ns_hex(Ctx &c, int tensor_flag, int subdomain_flag,
                                   int color);
 hipError_t launch_zero(Ctx &c, double *p, size_t n);
@@ -186,6 +201,9 @@ hipError_t launch_node4_to_soa(Ctx &c, const double *in, double *u, double *p);
 hipError_t launch_axpy_fields(Ctx &c, int which);
 hipError_t launch_fields_s(Ctx &c, int which, const double *x, int fa, int faold, int fra,
                            int fraold);
+hipError_t launch_fields_v(Ctx &c);
+hipError_t launch_node5_to_soa(Ctx &c, const double *in);
+hipError_t launch_norm2_v(Ctx &c, const double *v);
 hipError_t launch_apply_dirichlet(Ctx &c);
 hipError_t launch_copy(Ctx &c, const void *src, void *dst, size_t nbytes);
 // overset fringe
@@ -207,6 +225,12 @@ hipError_t launch_spmv_s(Ctx &c, int precision, const double *x, double *y,
                          const double *b, int mode);
 hipError_t launch_build_prec_s(Ctx &c, bool want32);
 hipError_t launch_convert_aT32(Ctx &c);
+// NSVOF system
+hipError_t launch_assemble_nsvof_hex(Ctx &c, int tensor_flag, int color);
+hipError_t launch_spmv_v(Ctx &c, int precision, const double *x, double *y,
+                         const double *b, int mode);
+hipError_t launch_build_prec_v(Ctx &c, bool want32);
+hipError_t launch_convert_aV32(Ctx &c);
 
 int nblocks_nodes(const Ctx &c);
 

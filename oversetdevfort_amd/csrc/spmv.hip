@@ -169,6 +169,110 @@ __global__ void k_convert_s32(size_t n, const double *__restrict__ A, float *__r
   for (; i < n; i += stride) A32[i] = (float)A[i];
 }
 
+// 5-field product of the NSVOF system (SparseProd_NS_conv, SparseProd.f90:
+// 167-242): one lane per (row, field); rows u0 u1 u2 p of a block hold 5
+// entries (u0 u1 u2 p phi), the phi row 4 (u0 u1 u2 phi).  Flat entries
+// 5N .. npad-1 (the padding to whole 4-tuples) are written as 0.  mode as
+// k_spmv; XCD-aware block order.
+template <typename T, typename V, int MODE>
+__global__ void __launch_bounds__(256)
+k_spmv_v(int n, int npad, const int32_t *__restrict__ col, const int32_t *__restrict__ row,
+         const T *__restrict__ A, const V *__restrict__ x, V *__restrict__ y,
+         const double *__restrict__ b, const GmresState *__restrict__ gs) {
+  if (MODE == 2 && gs->done) return;
+  if (MODE == 1 && gs && gs->outer_done) return;
+  const int per = gridDim.x >> 3;
+  const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  const int t = lb * blockDim.x + threadIdx.x;
+  if (t >= npad) return;
+  if (t >= 5 * n) {
+    y[t] = (V)0.0;
+    return;
+  }
+  const int r = t / 5, c = t - 5 * r;
+  double acc = 0.0;
+  const int k1 = col[r + 1];
+  if (c < 4) {
+    for (int k = col[r]; k < k1; ++k) {
+      const T *a = A + (size_t)k * 24 + 5 * c;
+      const V *xv = x + (size_t)row[k] * 5;
+      acc += (double)a[0] * (double)xv[0] + (double)a[1] * (double)xv[1] +
+             (double)a[2] * (double)xv[2] + (double)a[3] * (double)xv[3] +
+             (double)a[4] * (double)xv[4];
+    }
+  } else {
+    for (int k = col[r]; k < k1; ++k) {
+      const T *a = A + (size_t)k * 24 + 20;
+      const V *xv = x + (size_t)row[k] * 5;
+      acc += (double)a[0] * (double)xv[0] + (double)a[1] * (double)xv[1] +
+             (double)a[2] * (double)xv[2] + (double)a[3] * (double)xv[4];
+    }
+  }
+  if (MODE == 1)
+    y[t] = (V)(b[t] - acc);
+  else
+    y[t] = (V)acc;
+}
+
+// NSVOF block-Jacobi preconditioner (SparseGMRES_NS_conv.f:53-150): 3x3
+// cofactor inverse of the diagonal K11 block, 1/M (0 -> 1), 1/LS (0 -> 0).
+__global__ void k_build_prec_v(int N, const int32_t *__restrict__ diag,
+                               const double *__restrict__ A, double *__restrict__ P) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double *a = A + (size_t)diag[i] * 24;
+  const double K[9] = {a[0], a[1], a[2], a[5], a[6], a[7], a[10], a[11], a[12]};
+  double B[9];
+  B[0] = K[4] * K[8] - K[7] * K[5];
+  B[1] = K[7] * K[2] - K[1] * K[8];
+  B[2] = K[1] * K[5] - K[2] * K[4];
+  const double t = 1.0 / (B[0] * K[0] + B[1] * K[3] + B[2] * K[6]);
+  B[0] *= t;
+  B[1] *= t;
+  B[2] *= t;
+  B[3] = (K[5] * K[6] - K[3] * K[8]) * t;
+  B[4] = (K[0] * K[8] - K[6] * K[2]) * t;
+  B[5] = (K[3] * K[2] - K[0] * K[5]) * t;
+  B[6] = (K[3] * K[7] - K[4] * K[6]) * t;
+  B[7] = (K[6] * K[1] - K[0] * K[7]) * t;
+  B[8] = (K[0] * K[4] - K[1] * K[3]) * t;
+  double *p = P + (size_t)i * 12;
+  for (int q = 0; q < 9; ++q) p[q] = B[q];
+  const double m = a[18], l = a[23];
+  p[9] = (m == 0.0) ? 1.0 : 1.0 / m;
+  p[10] = (l == 0.0) ? 0.0 : 1.0 / l;
+  p[11] = 0.0;
+}
+
+// Ap = P_i^-1 A for every nonzero of block row i; thread (i, r) forms row r
+__global__ void k_precondition_v(int N, const int32_t *__restrict__ col,
+                                 const double *__restrict__ P, const double *__restrict__ A,
+                                 double *__restrict__ Ap, float *__restrict__ Ap32) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 5 * N) return;
+  const int i = t / 5, r = t - 5 * i;
+  const double *p = P + (size_t)i * 12;
+  for (int k = col[i]; k < col[i + 1]; ++k) {
+    const double *a = A + (size_t)k * 24;
+    double o[5];
+    int nw = 5, off = 5 * r;
+    if (r < 3) {
+      for (int c = 0; c < 5; ++c)
+        o[c] = p[r * 3 + 0] * a[c] + p[r * 3 + 1] * a[5 + c] + p[r * 3 + 2] * a[10 + c];
+    } else if (r == 3) {
+      for (int c = 0; c < 5; ++c) o[c] = a[15 + c] * p[9];
+    } else {
+      nw = 4;
+      off = 20;
+      for (int c = 0; c < 4; ++c) o[c] = a[20 + c] * p[10];
+    }
+    for (int c = 0; c < nw; ++c) {
+      Ap[(size_t)k * 24 + off + c] = o[c];
+      if (Ap32) Ap32[(size_t)k * 24 + off + c] = (float)o[c];
+    }
+  }
+}
+
 // block-Jacobi inverse: 3x3 cofactor inverse of the K11 diagonal block and
 // 1/M (0 -> 1), SparseGMRES_NS.f90:73-121
 __global__ void k_build_prec(int N, const int32_t *__restrict__ diag,
@@ -344,6 +448,79 @@ hipError_t launch_convert_aT32(Ctx &c) {
   return hipGetLastError();
 }
 
+// flat [N][5] entries, one lane each, over the padded length, 8 blocks
+static inline unsigned spmv_v_blocks(const Ctx &c) {
+  const unsigned nb = (unsigned)(((size_t)c.gm_n * 4 + 255) / 256);
+  return (nb + 7u) & ~7u;
+}
+
+hipError_t launch_spmv_v(Ctx &c, int precision, const double *x, double *y,
+                         const double *b, int mode) {
+  const unsigned nb = spmv_v_blocks(c);
+  const int n = c.nnode, np = c.gm_n * 4;
+  if (precision == OFD_FP32) {
+    if (mode == 0)
+      hipLaunchKernelGGL((k_spmv_v<float, double, 0>), dim3(nb), dim3(256), 0, c.stream, n,
+                         np, c.d_col, c.d_row, c.d_AV32, x, y, b, c.d_gs);
+    else
+      hipLaunchKernelGGL((k_spmv_v<float, double, 1>), dim3(nb), dim3(256), 0, c.stream, n,
+                         np, c.d_col, c.d_row, c.d_AV32, x, y, b, c.d_gs);
+  } else {
+    if (mode == 0)
+      hipLaunchKernelGGL((k_spmv_v<double, double, 0>), dim3(nb), dim3(256), 0, c.stream, n,
+                         np, c.d_col, c.d_row, c.d_AV, x, y, b, c.d_gs);
+    else
+      hipLaunchKernelGGL((k_spmv_v<double, double, 1>), dim3(nb), dim3(256), 0, c.stream, n,
+                         np, c.d_col, c.d_row, c.d_AV, x, y, b, c.d_gs);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_build_prec_v(Ctx &c, bool want32) {
+  const int N = c.nnode;
+  hipLaunchKernelGGL(k_build_prec_v, dim3((N + 255) / 256), dim3(256), 0, c.stream, N,
+                     c.d_diag, c.d_AV, c.d_PV);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_precondition_v, dim3((5 * N + 255) / 256), dim3(256), 0, c.stream, N,
+                     c.d_col, c.d_PV, c.d_AV, c.d_ApV, want32 ? c.d_ApV32 : nullptr);
+  return hipGetLastError();
+}
+
+hipError_t launch_convert_aV32(Ctx &c) {
+  hipLaunchKernelGGL(k_convert_s32, dim3(4096), dim3(256), 0, c.stream,
+                     (size_t)c.icnt * 24, c.d_AV, c.d_AV32);
+  return hipGetLastError();
+}
+
+// Arnoldi product of the NSVOF system: W_{k+1} = (P^-1 A) W_k
+static hipError_t launch_spmv_arnoldi_v(Ctx &c, int precision, int k, hipEvent_t ev0,
+                                        hipEvent_t ev1) {
+  const unsigned nb = spmv_v_blocks(c);
+  const int n = c.nnode, np = c.gm_n * 4;
+  const size_t vs = (size_t)c.gm_n * 4;
+  if (c.basis32) {
+    hipExtLaunchKernelGGL((k_spmv_v<float, float, 2>), dim3(nb), dim3(256), 0, c.stream, ev0,
+                          ev1, 0, n, np, (const int32_t *)c.d_col, (const int32_t *)c.d_row,
+                          (const float *)c.d_ApV32, (const float *)(c.d_W32 + vs * k),
+                          c.d_W32 + vs * (k + 1), (const double *)nullptr,
+                          (const GmresState *)c.d_gs);
+  } else if (precision == OFD_FP32) {
+    hipExtLaunchKernelGGL((k_spmv_v<float, double, 2>), dim3(nb), dim3(256), 0, c.stream,
+                          ev0, ev1, 0, n, np, (const int32_t *)c.d_col,
+                          (const int32_t *)c.d_row, (const float *)c.d_ApV32,
+                          (const double *)(c.d_W + vs * k), c.d_W + vs * (k + 1),
+                          (const double *)nullptr, (const GmresState *)c.d_gs);
+  } else {
+    hipExtLaunchKernelGGL((k_spmv_v<double, double, 2>), dim3(nb), dim3(256), 0, c.stream,
+                          ev0, ev1, 0, n, np, (const int32_t *)c.d_col,
+                          (const int32_t *)c.d_row, (const double *)c.d_ApV,
+                          (const double *)(c.d_W + vs * k), c.d_W + vs * (k + 1),
+                          (const double *)nullptr, (const GmresState *)c.d_gs);
+  }
+  return hipGetLastError();
+}
+
 // Arnoldi product of the temperature system: W_{k+1} = (D^-1 A) W_k
 static hipError_t launch_spmv_arnoldi_s(Ctx &c, int precision, int k, hipEvent_t ev0,
                                         hipEvent_t ev1) {
@@ -375,7 +552,8 @@ static hipError_t launch_spmv_arnoldi_s(Ctx &c, int precision, int k, hipEvent_t
 // ev0/ev1 (may be null) receive the kernel's own start and end timestamps
 hipError_t launch_spmv_arnoldi(Ctx &c, int precision, int k, hipEvent_t ev0,
                                hipEvent_t ev1) {
-  if (c.gm_scalar) return launch_spmv_arnoldi_s(c, precision, k, ev0, ev1);
+  if (c.gm_kind == 1) return launch_spmv_arnoldi_s(c, precision, k, ev0, ev1);
+  if (c.gm_kind == 2) return launch_spmv_arnoldi_v(c, precision, k, ev0, ev1);
   const unsigned nb = spmv_blocks(c);
   const size_t vs = (size_t)c.nnode * 4;
   if (c.basis32) {

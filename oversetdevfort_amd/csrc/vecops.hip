@@ -34,6 +34,7 @@ __global__ void k_fields(int N, int which, double gami, double delt, double alfi
                          double *acg, double *acgold, double *pg, double *pgold,
                          double *dg, double *dgold, double *ugm, double *ugmold,
                          double *acgm, double *acgmold) {
+#pragma clang fp contract(off)  // products and sums rounded separately
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   if (which == 0) {
@@ -75,6 +76,7 @@ __global__ void k_apply_dirichlet(int N, const uint8_t *__restrict__ ibc,
                                   const double *__restrict__ val, double dtgl,
                                   double gami, double *ug, const double *ugold,
                                   double *acg, const double *acgold) {
+#pragma clang fp contract(off)  // products and sums rounded separately
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const unsigned m = ibc[i];
@@ -141,6 +143,7 @@ hipError_t launch_node4_to_soa(Ctx &c, const double *in, double *u, double *p) {
 __global__ void k_fields_s(int N, int which, double gami, double delt,
                            const double *__restrict__ x, double *a, double *aold, double *ra,
                            double *raold) {
+#pragma clang fp contract(off)  // products and sums rounded separately
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   if (which == 0) {
@@ -162,6 +165,53 @@ hipError_t launch_fields_s(Ctx &c, int which, const double *x, int fa, int faold
   hipLaunchKernelGGL(k_fields_s, dim3((c.nnode + 255) / 256), dim3(256), 0, c.stream, c.nnode,
                      which, c.prm[OFD_P_GAMI], c.prm[OFD_P_DELT], x, f[fa], f[faold], f[fra],
                      f[fraold]);
+  return hipGetLastError();
+}
+
+// NSVOF Newton update (solveFlow.f90:99-104) from x = [N][5]
+__global__ void k_fields_v(int N, double gami, double delt, double alfi,
+                           const double *__restrict__ x, double *ug, double *acg, double *pg,
+                           double *rphig, double *phig) {
+#pragma clang fp contract(off)  // products and sums rounded separately
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double *d = x + (size_t)i * 5;
+  const double fu = gami * delt;
+  for (int c = 0; c < 3; ++c) {
+    const size_t o = (size_t)c * N + i;
+    acg[o] = acg[o] + d[c];
+    ug[o] = ug[o] + fu * d[c];
+  }
+  pg[i] = pg[i] + alfi * gami * delt * d[3];
+  rphig[i] = rphig[i] + d[4];
+  phig[i] = phig[i] + fu * d[4];
+}
+
+// [N][5] -> u (N,3), p (N), phi (N)
+__global__ void k_node5_to_soa(int N, const double *__restrict__ in, double *u, double *p,
+                               double *phi) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double *d = in + (size_t)i * 5;
+  u[i] = d[0];
+  u[(size_t)N + i] = d[1];
+  u[2 * (size_t)N + i] = d[2];
+  p[i] = d[3];
+  phi[i] = d[4];
+}
+
+hipError_t launch_fields_v(Ctx &c) {
+  double **f = c.d_field;
+  hipLaunchKernelGGL(k_fields_v, dim3((c.nnode + 255) / 256), dim3(256), 0, c.stream, c.nnode,
+                     c.prm[OFD_P_GAMI], c.prm[OFD_P_DELT], c.prm[OFD_P_ALFI], c.d_xV, f[OFD_UG],
+                     f[OFD_ACG], f[OFD_PG], f[OFD_RPHIG], f[OFD_PHIG]);
+  return hipGetLastError();
+}
+
+hipError_t launch_node5_to_soa(Ctx &c, const double *in) {
+  double **f = c.d_field;
+  hipLaunchKernelGGL(k_node5_to_soa, dim3((c.nnode + 255) / 256), dim3(256), 0, c.stream,
+                     c.nnode, in, f[OFD_SOLU], f[OFD_SOLP], f[OFD_SOLPHI]);
   return hipGetLastError();
 }
 

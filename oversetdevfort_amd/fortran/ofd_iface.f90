@@ -23,8 +23,9 @@ module ofd_iface
                                OFD_PGOLD = 12, OFD_SOLU = 13, OFD_SOLP = 14, &
                                OFD_PHIG = 15, OFD_PHIGOLD = 16, OFD_RPHIG = 17, &
                                OFD_RPHIGOLD = 18, OFD_TG = 19, OFD_TGOLD = 20, &
-                               OFD_RTG = 21, OFD_RTGOLD = 22, OFD_SOLT = 23
-  integer(c_int), parameter :: OFD_NPARAMS = 11, OFD_NQPARAMS = 20
+                               OFD_RTG = 21, OFD_RTGOLD = 22, OFD_SOLT = 23, &
+                               OFD_SOLPHI = 24
+  integer(c_int), parameter :: OFD_NPARAMS = 11, OFD_NQPARAMS = 21
 
   interface
     function ofd_last_error() bind(C, name='ofd_last_error') result(p)
@@ -279,6 +280,79 @@ module ofd_iface
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
       real(c_double), intent(out) :: norm2
+      integer(c_int) :: rc
+    end function
+
+    function ofd_assemble_nsvof(ctx, tensor_flag, cfl) bind(C, name='ofd_assemble_nsvof') &
+        result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: tensor_flag
+      real(c_double), intent(out) :: cfl
+      integer(c_int) :: rc
+    end function
+
+    function ofd_get_nsvof_system(ctx, k11, g, d1, m, ls, lsu, pls, uls, rhsu, rhsp, &
+                                  rhsls) bind(C, name='ofd_get_nsvof_system') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: k11(*), g(*), d1(*), m(*), ls(*), lsu(*), pls(*), &
+                                     uls(*), rhsu(*), rhsp(*), rhsls(*)
+      integer(c_int) :: rc
+    end function
+
+    function ofd_set_nsvof_system(ctx, k11, g, d1, m, ls, lsu, pls, uls, rhsu, rhsp, &
+                                  rhsls) bind(C, name='ofd_set_nsvof_system') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: k11(*), g(*), d1(*), m(*), ls(*), lsu(*), pls(*), &
+                                    uls(*), rhsu(*), rhsp(*), rhsls(*)
+      integer(c_int) :: rc
+    end function
+
+    function ofd_spmv_nsvof(ctx, precision, xu, xp, xl, yu, yp, yl) &
+        bind(C, name='ofd_spmv_nsvof') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: precision
+      real(c_double), intent(in) :: xu(*), xp(*), xl(*)
+      real(c_double), intent(out) :: yu(*), yp(*), yl(*)
+      integer(c_int) :: rc
+    end function
+
+    function ofd_gmres_nsvof(ctx, precision, ortho, tol, kspace, kspace_min, solu, solp, &
+                             solls, iters, resid) bind(C, name='ofd_gmres_nsvof') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: precision, ortho, kspace, kspace_min
+      real(c_double), value :: tol
+      real(c_double), intent(out) :: solu(*), solp(*), solls(*)
+      integer(c_int), intent(out) :: iters
+      real(c_double), intent(out) :: resid
+      integer(c_int) :: rc
+    end function
+
+    function ofd_solve_nsvof(ctx, precision, ortho, rtol, inner_tol, kspace, max_cycles, &
+                             iters, cycles, relres) bind(C, name='ofd_solve_nsvof') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: precision, ortho, kspace, max_cycles
+      real(c_double), value :: rtol, inner_tol
+      integer(c_int), intent(out) :: iters, cycles
+      real(c_double), intent(out) :: relres
+      integer(c_int) :: rc
+    end function
+
+    function ofd_nsvof_update(ctx) bind(C, name='ofd_nsvof_update') result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+
+    function ofd_nsvof_rhs_norms(ctx, norms3) bind(C, name='ofd_nsvof_rhs_norms') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: norms3(3)
       integer(c_int) :: rc
     end function
 

@@ -13,6 +13,9 @@ import numpy as np
 
 from . import _lib as L
 
+# nonzero arrays of the coupled NS + VOF system and their entries per nonzero
+NSVOF_BLOCKS = ("k11", "g", "d1", "m", "ls", "lsu", "pls", "uls")
+_NSVOF_WIDTH = dict(k11=9, g=3, d1=3, m=1, ls=1, lsu=3, pls=1, uls=3)
 
 def _ptr(a):
     return ctypes.c_void_p(a.ctypes.data)
@@ -212,9 +215,11 @@ class Context:
 
     # ---- temperature equation of the quenching solver (solveQuenching.f90) ----
     def set_quench_params(self, q):
-        """q: dict with the keys of _lib.QPARAMS (fluid_id defaults to 102)."""
+        """q: dict with the keys of _lib.QPARAMS (fluid_id defaults to 102,
+        lsc_kdc, the VOF discontinuity-capturing constant, to 0)."""
         d = dict(q)
         d.setdefault("fluid_id", 102)
+        d.setdefault("lsc_kdc", 0.0)
         arr = np.array([float(d[k]) for k in L.QPARAMS], dtype=np.float64)
         L.check(self.lib.ofd_set_quench_params(self.h, _ptr(arr)))
 
@@ -272,6 +277,80 @@ class Context:
         out = ctypes.c_double()
         L.check(self.lib.ofd_tem_rhs_norm(self.h, ctypes.byref(out)))
         return out.value
+
+    # ---- coupled NS + VOF system of the quenching solver ----
+    def assemble_nsvof(self, tensor_flag=6):
+        """IntElmAss_NSVOF_Quenching_STAB (solveQuenching.f90) with the
+        BCLhs_NS_conv / FillSparseMat_NS_conv scatter; returns the CFL maximum."""
+        cfl = ctypes.c_double()
+        L.check(self.lib.ofd_assemble_nsvof(self.h, int(tensor_flag), ctypes.byref(cfl)))
+        return cfl.value
+
+    def get_nsvof_system(self):
+        """The eight nonzero arrays in the reference's layout (LHSK11 (icnt,9),
+        LHSG, LHSD1, LHSLSU, LHSULS (icnt,3), LHSM, LHSLS, LHSPLS (icnt)) and
+        RHSGu (N,3), RHSGp, RHSGls (N)."""
+        Z, N = self.icnt, self.nnode
+        A = {k: np.zeros((Z, w) if w > 1 else Z) for k, w in _NSVOF_WIDTH.items()}
+        ru = np.zeros((3, N))
+        rp = np.zeros(N)
+        rl = np.zeros(N)
+        L.check(self.lib.ofd_get_nsvof_system(self.h, *[_ptr(A[k]) for k in NSVOF_BLOCKS],
+                                              _ptr(ru), _ptr(rp), _ptr(rl)))
+        return A, dict(rhsu=ru.T.copy(), rhsp=rp, rhsls=rl)
+
+    def set_nsvof_system(self, A=None, rhs=None):
+        mats = [None] * 8 if A is None else [_f64(A[k]) for k in NSVOF_BLOCKS]
+        rh = [None] * 3 if rhs is None else [_soa(rhs["rhsu"]), _f64(rhs["rhsp"]),
+                                             _f64(rhs["rhsls"])]
+        L.check(self.lib.ofd_set_nsvof_system(
+            self.h, *[None if a is None else _ptr(a) for a in mats + rh]))
+
+    def SparseProd_NS_conv(self, xu, xp, xl, precision=L.OFD_FP64):
+        xu_, xp_, xl_ = _soa(xu), _f64(xp), _f64(xl)
+        N = self.nnode
+        yu = np.zeros((3, N))
+        yp = np.zeros(N)
+        yl = np.zeros(N)
+        L.check(self.lib.ofd_spmv_nsvof(self.h, precision, _ptr(xu_), _ptr(xp_), _ptr(xl_),
+                                        _ptr(yu), _ptr(yp), _ptr(yl)))
+        return yu.T.copy(), yp, yl
+
+    def SparseGMRES_ns_conv(self, tol, kspace, kspace_min=0, precision=L.OFD_FP64,
+                            ortho=L.OFD_ORTHO_MGS):
+        N = self.nnode
+        su = np.zeros((3, N))
+        sp = np.zeros(N)
+        sl = np.zeros(N)
+        it = ctypes.c_int()
+        res = ctypes.c_double()
+        L.check(self.lib.ofd_gmres_nsvof(self.h, precision, ortho, float(tol), int(kspace),
+                                         int(kspace_min), _ptr(su), _ptr(sp), _ptr(sl),
+                                         ctypes.byref(it), ctypes.byref(res)))
+        return dict(solu=su.T.copy(), solp=sp, solls=sl, iters=it.value, resid=res.value)
+
+    def solve_nsvof(self, rtol=1e-8, precision=L.OFD_FP64, ortho=L.OFD_ORTHO_CGS1,
+                    inner_tol=None, kspace=30, max_cycles=50, raise_noconv=True):
+        if inner_tol is None:
+            inner_tol = 1e-4 if precision == L.OFD_FP32 else 1e-2
+        it = ctypes.c_int()
+        cyc = ctypes.c_int()
+        rel = ctypes.c_double()
+        rc = self.lib.ofd_solve_nsvof(self.h, precision, ortho, float(rtol), float(inner_tol),
+                                      int(kspace), int(max_cycles), ctypes.byref(it),
+                                      ctypes.byref(cyc), ctypes.byref(rel))
+        if rc != L.OFD_OK and (raise_noconv or rc != 5):
+            L.check(rc)
+        return dict(iters=it.value, cycles=cyc.value, relres=rel.value,
+                    converged=rc == L.OFD_OK)
+
+    def nsvof_update(self):
+        L.check(self.lib.ofd_nsvof_update(self.h))
+
+    def nsvof_rhs_norms(self):
+        out = (ctypes.c_double * 3)()
+        L.check(self.lib.ofd_nsvof_rhs_norms(self.h, out))
+        return tuple(float(v) for v in out)
 
     # ---- overset fringe (interpolation.f90:5-76, setBCs.f90:124-167) ----
     def interpolation(self, src_id, xr, dr=None):
