@@ -12,6 +12,11 @@ old <- new.  It runs once with the FP64 solver and once with the mixed one
 Gram-Schmidt, Givens and outer residual).  The headline value is the mixed
 Newton step in elements per second; the FP64 figures ride along.
 
+Beside the headline the line carries kernel figures of the quenching solver
+on the 48^3 tank (NSVOF and temperature assembly against their per-element
+op counts, the 5-field and scalar SpMV against HBM) and the overset fringe
+search on the 96x48x48 background + rotated 32^3 component mesh.
+
 Prints progress on stderr and ONE JSON line on stdout.
 """
 from __future__ import annotations
@@ -29,6 +34,10 @@ sys.path.insert(0, ROOT)
 METRIC = ("Newton-step wall time & Melem/s assembly, FP32-mixed vs FP64 at 1e-8 "
           "residual")
 F_NS = 1.07e5          # flop per hex element, one VEC+MAT NS assembly
+# flop per hex element of the quenching assemblies (VEC+MAT, VEC only): the
+# op count of the oracle's restatement, recounted by tests/test_flop_counts.py
+F_NSVOF, F_NSVOF_VEC = 218060.0, 20428.0
+F_TEM, F_TEM_VEC = 27720.0, 9800.0
 PEAK_HBM = 8.0e12      # B/s
 PEAK_FP64 = 78.6e12    # flop/s
 
@@ -121,6 +130,115 @@ def pmc_sample(ctx, S, L, prob, ortho, kspace):
         g = ctx.SparseGMRES_up(1e-30, kspace, kspace_min=kspace, precision=prec,
                                ortho=ortho)
         log(f"pmc sample: precision {prec}: {g['iters']} Arnoldi steps")
+
+
+def quench_kernels(L, C, S, n=48, reps=5, spmv_reps=20):
+    """Kernel figures of the quenching solver on the n^3 tank (ELM_ID 101
+    solid block, water/vapour, 300/900 K): NSVOF and temperature assembly
+    (VEC+MAT, VEC) and the 5-field and scalar SpMV, HIP events on the context
+    stream."""
+    prob = C.quench(n)
+    E, N = prob.mesh.nelem, prob.mesh.nnode
+    ctx = S.setup(prob, device=0)
+    ctx.set_quench_params(prob.material)
+    Z = ctx.icnt
+    cfl = ctx.assemble_nsvof(6)
+    ctx.assemble_tem(6)
+    t = {}
+    t["nsvof_vecmat"] = ctx.time_quench(L.OFD_TIME_ASM_NSVOF, 6, reps)
+    t["nsvof_vec"] = ctx.time_quench(L.OFD_TIME_ASM_NSVOF, 2, reps)
+    t["tem_vecmat"] = ctx.time_quench(L.OFD_TIME_ASM_TEM, 6, reps)
+    t["tem_vec"] = ctx.time_quench(L.OFD_TIME_ASM_TEM, 2, reps)
+    sp = {}
+    for name, what in (("nsvof", L.OFD_TIME_SPMV_NSVOF), ("tem", L.OFD_TIME_SPMV_TEM)):
+        for prec, tag in ((L.OFD_FP64, "fp64"), (L.OFD_FP32, "fp32")):
+            sp[f"{name}_{tag}"] = ctx.time_quench(what, prec, spmv_reps)
+    ctx.close()
+
+    def asm(ms, f):
+        fl = E * f / (ms * 1e-3)
+        return {"ms": ms, "melem_per_s": E / ms / 1e3, "fp64_tflops": fl / 1e12,
+                "frac_fp64_peak": fl / PEAK_FP64, "flop_per_elem": f}
+
+    def prod(ms, nbytes):
+        bw = nbytes / (ms * 1e-3)
+        return {"us": 1e3 * ms, "bytes": nbytes, "GBps": bw / 1e9, "frac_hbm": bw / PEAK_HBM}
+
+    out = {"workload": f"quench tank {n}^3 hex (water/vapour, solid block 101, 300/900 K)",
+           "elements": E, "nodes": N, "nnz_blocks": Z, "cfl": cfl,
+           "nsvof_assembly": {"vecmat": asm(t["nsvof_vecmat"], F_NSVOF),
+                              "vec": asm(t["nsvof_vec"], F_NSVOF_VEC)},
+           "tem_assembly": {"vecmat": asm(t["tem_vecmat"], F_TEM),
+                            "vec": asm(t["tem_vec"], F_TEM_VEC)},
+           "nsvof_spmv": {"fp64": prod(sp["nsvof_fp64"], spmv_bytes(Z, N, 8, b=5, S=24)),
+                          "fp32": prod(sp["nsvof_fp32"], spmv_bytes(Z, N, 4, b=5, S=24))},
+           "tem_spmv": {"fp64": prod(sp["tem_fp64"], spmv_bytes(Z, N, 8, b=1, S=1)),
+                        "fp32": prod(sp["tem_fp32"], spmv_bytes(Z, N, 4, b=1, S=1))}}
+    log(f"bench: quench {n}^3: NSVOF VEC+MAT {t['nsvof_vecmat']:.3f} ms "
+        f"({E / t['nsvof_vecmat'] / 1e3:.1f} Melem/s, "
+        f"{out['nsvof_assembly']['vecmat']['fp64_tflops']:.2f} TFLOP/s); "
+        f"Tem VEC+MAT {t['tem_vecmat']:.3f} ms ({E / t['tem_vecmat'] / 1e3:.1f} Melem/s); "
+        f"NSVOF SpMV FP64 {1e3 * sp['nsvof_fp64']:.1f} us "
+        f"({out['nsvof_spmv']['fp64']['GBps']:.0f} GB/s); T SpMV FP64 "
+        f"{1e3 * sp['tem_fp64']:.1f} us")
+    return out, prob
+
+
+def fringe_search(C, S, reps=5):
+    """Overset fringe of the background + rotated component mesh: donor
+    search and interpolation for the component's boundary nodes (donors in
+    the background) and the background fringe (donors in the component).
+    Host wall time of the entry point, which includes copying the receiver
+    coordinates in and the donors and values out."""
+    import numpy as np
+    prob = C.overset()
+    m = prob.mesh
+    ctx = S.Context(0)
+    ctx.upload_mesh(m)
+    ctx.set_field("DG", np.zeros((m.nnode, 3)))
+    ctx.set_field("UG", prob.fields["ug"])
+    out = {"workload": (f"overset: background 96x48x48 + component 32^3 rotated "
+                        f"{np.rad2deg(prob.theta):.1f} deg about z"),
+           "elements": m.nelem, "nodes": m.nnode}
+    for name, (nodes, src) in prob.fringe.items():
+        xr = m.xg[nodes]
+        donor, _ = ctx.interpolation(src, xr)  # warm-up
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            donor, _ = ctx.interpolation(src, xr)
+            ts.append(time.perf_counter() - t0)
+        ts.sort()
+        out[name] = {"receivers": int(len(nodes)), "src_id": int(src),
+                     "found": int((donor > 0).sum()),
+                     "source_elements": int((m.elm_id == src).sum()),
+                     "ms": 1e3 * ts[len(ts) // 2]}
+        log(f"bench: fringe {name}: {len(nodes)} receivers, "
+            f"{out[name]['found']} donors, {out[name]['ms']:.3f} ms")
+    ctx.close()
+    return out
+
+
+def cpu_quench(prob):
+    """Host (C oracle, one thread) NSVOF and temperature assembly VEC+MAT of
+    the quench tank: one call each."""
+    import numpy as np
+
+    from oracle import oracle as O
+    from oversetdevfort_amd import _lib as L
+    m, p, f = prob.mesh, prob.params, prob.fields
+    col, row = O.sparsity(m.nnode, m.ien)
+    st = {k: f[k] for k in ("dg", "ug", "ugm", "acg", "pg", "phig", "rphig", "Tg", "rTg")}
+    qv = [prob.material[k] for k in L.QPARAMS]
+    prm = O.params_array(p)
+    t0 = time.perf_counter()
+    O.assemble_nsvof(m, prob.ibc, st, prm, qv, np.array([*p["grav"], 0.0]), col, row, 6)
+    t1 = time.perf_counter()
+    O.assemble_tem(m, prob.ibc[:, 5], st, prm, qv, col, row, 6)
+    t2 = time.perf_counter()
+    E = m.nelem
+    return {"nsvof_vecmat_ms": 1e3 * (t1 - t0), "nsvof_melem_per_s": E / (t1 - t0) / 1e6,
+            "tem_vecmat_ms": 1e3 * (t2 - t1), "tem_melem_per_s": E / (t2 - t1) / 1e6}
 
 
 def cpu_baseline(prob, budget_s=15.0):
@@ -268,6 +386,9 @@ def main():
         f"FP32-matrix {spmv32_ms * 1e3:.1f} us {bw32 / 1e9:.0f} GB/s")
     ctx.close()
 
+    quench, qprob = quench_kernels(L, C, S)
+    fringe = fringe_search(C, S)
+
     cpu = None
     if not args.no_cpu:
         log("bench: host baseline (C oracle, 1 thread) ...")
@@ -293,6 +414,11 @@ def main():
                "newton_ms": 1e3 * newton_s, "assembly_ms": 1e3 * c["assemble_s"],
                "assembly_melem_per_s": E / c["assemble_s"] / 1e6,
                "ms_per_gmres_iter": 1e3 * c["s_per_iter"]}
+        log("bench: host quench assembly (C oracle, 1 thread) ...")
+        cq = cpu_quench(qprob)
+        log(f"bench: host quench: NSVOF {cq['nsvof_vecmat_ms']:.0f} ms, "
+            f"Tem {cq['tem_vecmat_ms']:.0f} ms")
+        cpu["quench_assembly"] = cq
 
     out = {
         "metric": METRIC,
@@ -334,6 +460,8 @@ def main():
                           "algorithmic_bytes": ba64,
                           "avg_launch_us": 1e3 * res64["arnoldi_spmv_ms"],
                           "samples": res64["arnoldi_spmv_samples"]},
+        "quench": quench,
+        "fringe": fringe,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

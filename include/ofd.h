@@ -295,6 +295,15 @@ int ofd_time_assemble(ofd_ctx *ctx, int tensor_flag, int reps, double *ms);
  * Allocates and frees its two buffers.  No reference counterpart. */
 int ofd_time_copy(ofd_ctx *ctx, int64_t nbytes, int reps, double *ms);
 
+/* Mean device time (HIP events on the context stream) of `reps` back-to-back
+ * launches of one quenching-solver kernel: what = OFD_TIME_ASM_NSVOF or
+ * OFD_TIME_ASM_TEM (arg = tensor flag; alpha stage, zeroing and the colour
+ * launches), OFD_TIME_SPMV_NSVOF or OFD_TIME_SPMV_TEM (arg = precision; needs
+ * an assembled system).  Measurement only, no reference counterpart. */
+enum { OFD_TIME_ASM_NSVOF = 0, OFD_TIME_ASM_TEM = 1, OFD_TIME_SPMV_NSVOF = 2,
+       OFD_TIME_SPMV_TEM = 3 };
+int ofd_time_quench(ofd_ctx *ctx, int what, int arg, int reps, double *ms);
+
 /* sampled timing of the Arnoldi product kernel (the preconditioned SpMV of
  * each GMRES iteration) inside ofd_solve / ofd_gmres_up: every `every`-th
  * such launch (counted across cycles and solves) records the kernel's own

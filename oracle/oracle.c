@@ -1157,6 +1157,7 @@ int orc_gmres_tem(int nnode, const int *col, const int *row, const double *A,
     *resid = 0.0;
     goto done;
   }
+  {  /* one cycle; a block of its own so the jump above skips it whole */
   for (int i = 0; i < N; ++i) V[i] = dinv[i] * b[i];
   rr = 0.0;
   for (int i = 0; i < N; ++i) rr = rr + V[i] * V[i];
@@ -1216,6 +1217,7 @@ int orc_gmres_tem(int nnode, const int *col, const int *row, const double *A,
   for (int jk = 1; jk <= iks; ++jk)
     for (int i = 0; i < N; ++i) sol[i] = sol[i] + y[jk - 1] * V[(size_t)N * (jk - 1) + i];
   *resid = er;
+  }
 done:
 #undef HB
   free(dinv);

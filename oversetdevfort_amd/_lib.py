@@ -21,6 +21,10 @@ OFD_FP32 = 1
 OFD_ORTHO_MGS = 0
 OFD_ORTHO_CGS2 = 1
 OFD_ORTHO_CGS1 = 2
+OFD_TIME_ASM_NSVOF = 0
+OFD_TIME_ASM_TEM = 1
+OFD_TIME_SPMV_NSVOF = 2
+OFD_TIME_SPMV_TEM = 3
 
 FIELDS = ["XG", "DG", "DGOLD", "UG", "UGOLD", "UGM", "UGMOLD", "ACG", "ACGOLD",
           "ACGM", "ACGMOLD", "PG", "PGOLD", "SOLU", "SOLP",
@@ -99,6 +103,7 @@ SIGNATURES = {
     "ofd_time_spmv": ([_vp, _c_int, _c_int, _dp], _c_int),
     "ofd_time_assemble": ([_vp, _c_int, _c_int, _dp], _c_int),
     "ofd_time_copy": ([_vp, ctypes.c_int64, _c_int, _dp], _c_int),
+    "ofd_time_quench": ([_vp, _c_int, _c_int, _c_int, _dp], _c_int),
     "ofd_probe_set": ([_vp, _c_int], _c_int),
     "ofd_probe_read": ([_vp, _i64p, _dp], _c_int),
     "ofd_ctx_bytes": ([_vp, _i64p], _c_int),

@@ -19,6 +19,18 @@ and setBCs_CFD; the GPU path consumes the resulting arrays.
   the gas fraction (prop_interp(rhow, rhoa, phi), evaporation ~ (1 - phi)),
   so the liquid is phi = 0.
 
+* ``overset`` -- background [0,2]x[0,1]x[0,1] on nb hexes (ELM_ID 1) plus a
+  cube component of side 0.4 on nc^3 hexes (ELM_ID 2), rotated by a seeded
+  angle in [0, 30] degrees about z and centred at (0.9, 0.5, 0.5), in one
+  mesh of two disconnected blocks.  Re = 500 cross-flow (u = (1, 0, 0) at
+  x = 0, slip walls y, z, natural outflow at x = 2; rho 1, mu 8e-4 on the
+  component side 0.4); dt = 0.005.  Fringe receivers: the component's
+  outer-boundary nodes (donors in the background, interpolation.f90:46-60)
+  and, as the background-side fringe the reference leaves to pre-cut
+  meshes, the background nodes inside the component at a depth between two
+  component cells and two component cells plus one background cell (donors
+  in the component).
+
 All use generalized-alpha with rho_inf = 0.5 (param.f90:469-480) and the
 reference's hard-coded gravity (0, 0, -9.81) (param.f90:253-257).
 """
@@ -182,4 +194,61 @@ def quench(n=48, seed=1, rhoinf=0.5, jitter=0.0) -> Problem:
              grav=(0.0, 0.0, -9.81))
     prob = Problem("quench", mesh, ibc, np.zeros((N, 3)), f, p)
     prob.material = dict(QUENCH_MATERIAL)
+    return prob
+
+
+def _rotz(theta):
+    c, s = np.cos(theta), np.sin(theta)
+    return np.array([[c, -s, 0.0], [s, c, 0.0], [0.0, 0.0, 1.0]])
+
+
+def overset(nb=(96, 48, 48), nc=32, seed=1, rhoinf=0.5, theta=None) -> Problem:
+    lo, hi = (0.0, 0.0, 0.0), (2.0, 1.0, 1.0)
+    bg = M.structured_hex(*nb, lo=lo, hi=hi, elm_id=1)
+    half = 0.2
+    comp = M.structured_hex(nc, nc, nc, lo=(-half,) * 3, hi=(half,) * 3, elm_id=2)
+    rng = np.random.default_rng(seed)
+    if theta is None:
+        theta = np.deg2rad(rng.uniform(0.0, 30.0))
+    R = _rotz(theta)
+    centre = np.array([0.9, 0.5, 0.5])
+    xc = comp.xg @ R.T + centre
+    Nb = bg.nnode
+    mesh = M.Mesh(xg=np.concatenate([bg.xg, xc]),
+                  ien=np.concatenate([bg.ien, comp.ien + Nb]).astype(np.int32),
+                  elm_id=np.concatenate([bg.elm_id, comp.elm_id]).astype(np.int32))
+    N = mesh.nnode
+    # component outer boundary (local frame)
+    comp_bnd = np.nonzero(np.any(np.abs(np.abs(comp.xg) - half) < 1e-12, axis=1))[0] + Nb
+    # background fringe: inside the component, between 2 h_c and 2 h_c + h_b deep
+    hc = 2 * half / nc
+    hb = (hi[0] - lo[0]) / nb[0]
+    loc = (bg.xg - centre) @ R          # background nodes in the component frame
+    depth = half - np.abs(loc).max(axis=1)
+    bg_fr = np.nonzero((depth >= 2 * hc) & (depth < 2 * hc + hb))[0]
+    faces = M.boundary_masks(bg, lo, hi)
+    ibc = np.zeros((N, 4), np.int32)
+    inflow = np.zeros(N, bool)
+    inflow[:Nb] = faces["xlo"]
+    ibc[inflow, :3] = 1
+    yw = np.zeros(N, bool)
+    yw[:Nb] = faces["ylo"] | faces["yhi"]
+    zw = np.zeros(N, bool)
+    zw[:Nb] = faces["zlo"] | faces["zhi"]
+    ibc[yw, 1] = 1
+    ibc[zw, 2] = 1
+    bcval = np.zeros((N, 3))
+    bcval[inflow, 0] = 1.0
+    f = _zero_fields(N)
+    u0 = np.zeros((N, 3))
+    u0[:, 0] = 1.0
+    u0[yw, 1] = 0.0
+    u0[zw, 2] = 0.0
+    f["ug"][:] = u0
+    f["ugold"][:] = u0
+    p = dict(genalpha(rhoinf), delt=0.005, rho=1.0, mu=8e-4, kdc=0.0,
+             grav=(0.0, 0.0, -9.81))
+    prob = Problem("overset", mesh, ibc, bcval, f, p)
+    prob.fringe = {"component": (comp_bnd, 1), "background": (bg_fr, 2)}
+    prob.theta = float(theta)
     return prob

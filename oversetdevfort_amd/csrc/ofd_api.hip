@@ -1107,6 +1107,78 @@ int ofd_time_assemble(ofd_ctx *x, int tensor_flag, int reps, double *ms) {
   return OFD_OK;
 }
 
+static int ensure_tem(Ctx &c);
+static int ensure_aT32(Ctx &c);
+static int ensure_nsvof(Ctx &c);
+static int ensure_aV32(Ctx &c);
+
+int ofd_time_quench(ofd_ctx *x, int what, int arg, int reps, double *ms) {
+  OFD_CHECK(x && ms && reps > 0, OFD_ERR_ARG, "bad argument");
+  OFD_CHECK(what >= OFD_TIME_ASM_NSVOF && what <= OFD_TIME_SPMV_TEM, OFD_ERR_ARG,
+            "unknown kernel");
+  Ctx &c = x->c;
+  const bool nsvof = what == OFD_TIME_ASM_NSVOF || what == OFD_TIME_SPMV_NSVOF;
+  const bool spmv = what >= OFD_TIME_SPMV_NSVOF;
+  int rc = nsvof ? ensure_nsvof(c) : ensure_tem(c);
+  if (rc) return rc;
+  OFD_HIP(hipSetDevice(c.device));
+  if (!spmv) {
+    OFD_CHECK(c.quench, OFD_ERR_ARG, "no quench parameters (ofd_set_quench_params)");
+    OFD_CHECK(c.nshl == 8, OFD_ERR_UNSUPPORTED,
+              "the GPU element assembly is implemented for trilinear hexes (NSHL=8)");
+    OFD_CHECK((arg & ~(OFD_ASSEMBLE_VEC | OFD_ASSEMBLE_MAT)) == 0, OFD_ERR_ARG,
+              "arg must combine OFD_ASSEMBLE_VEC / OFD_ASSEMBLE_MAT");
+  } else {
+    OFD_CHECK(arg == OFD_FP64 || arg == OFD_FP32, OFD_ERR_ARG, "bad precision");
+    OFD_CHECK(nsvof ? c.matV_valid : c.matT_valid, OFD_ERR_ARG, "no assembled system");
+    if (arg == OFD_FP32) {
+      rc = nsvof ? ensure_aV32(c) : ensure_aT32(c);
+      if (rc) return rc;
+    }
+  }
+  hipEvent_t e0, e1;
+  OFD_HIP(hipEventCreate(&e0));
+  OFD_HIP(hipEventCreate(&e1));
+  const int ncol = (int)c.color_off.size() - 1;
+  for (int r = -1; r < reps; ++r) {  // r = -1: warm-up launch
+    if (r == 0) OFD_HIP(hipEventRecord(e0, c.stream));
+    if (what == OFD_TIME_ASM_NSVOF) {
+      OFD_HIP(launch_alpha_stage(c, true));
+      if (arg & OFD_ASSEMBLE_VEC) OFD_HIP(launch_zero(c, c.d_rhsV, (size_t)c.nsvV_n4 * 4));
+      if (arg & OFD_ASSEMBLE_MAT) OFD_HIP(launch_zero(c, c.d_AV, (size_t)c.icnt * 24));
+      for (int col = 0; col < ncol; ++col) OFD_HIP(launch_assemble_nsvof_hex(c, arg, col));
+    } else if (what == OFD_TIME_ASM_TEM) {
+      OFD_HIP(launch_alpha_stage(c, false));
+      if (arg & OFD_ASSEMBLE_VEC) OFD_HIP(launch_zero(c, c.d_rhsT, (size_t)c.temT_n4 * 4));
+      if (arg & OFD_ASSEMBLE_MAT) OFD_HIP(launch_zero(c, c.d_AT, (size_t)c.icnt));
+      for (int col = 0; col < ncol; ++col) OFD_HIP(launch_assemble_tem_hex(c, arg, col));
+    } else if (what == OFD_TIME_SPMV_NSVOF) {
+      NsvofSystem sys(c);
+      OFD_HIP(launch_spmv_v(c, arg, c.d_xV, c.d_yV, nullptr, 0));
+    } else {
+      TemSystem sys(c);
+      OFD_HIP(launch_spmv_s(c, arg, c.d_xT, c.d_yT, nullptr, 0));
+    }
+  }
+  OFD_HIP(hipEventRecord(e1, c.stream));
+  OFD_HIP(hipEventSynchronize(e1));
+  float t = 0.f;
+  OFD_HIP(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (!spmv && (arg & OFD_ASSEMBLE_MAT)) {
+    if (nsvof) {
+      c.matV_valid = true;
+      c.precV_valid = c.aV32_valid = c.apV32_valid = false;
+    } else {
+      c.matT_valid = true;
+      c.precT_valid = c.aT32_valid = c.apT32_valid = false;
+    }
+  }
+  *ms = (double)t / reps;
+  return OFD_OK;
+}
+
 // ---- temperature system ----
 
 // LHSTem for the current pattern and the zero-padded T vectors

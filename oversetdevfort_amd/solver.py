@@ -373,6 +373,14 @@ class Context:
         L.check(self.lib.ofd_apply_fringe(self.h, int(nodes_.size), _ptr(nodes_), _ptr(val_)))
 
     # ---- measurement ----
+    def time_quench(self, what, arg, reps=5):
+        """Mean device ms of one quenching-solver kernel (L.OFD_TIME_*; arg is
+        the tensor flag for an assembly, the precision for a product)."""
+        ms = ctypes.c_double()
+        L.check(self.lib.ofd_time_quench(self.h, int(what), int(arg), int(reps),
+                                         ctypes.byref(ms)))
+        return ms.value
+
     def time_spmv(self, precision=L.OFD_FP64, reps=20):
         ms = ctypes.c_double()
         L.check(self.lib.ofd_time_spmv(self.h, precision, int(reps), ctypes.byref(ms)))
