@@ -11,11 +11,12 @@
 // (ELM_ID 102) contribute; solid elements scatter their projected zero
 // element matrices.  The NS discontinuity-capturing coefficient is 0.
 //
-// Block layout of a nonzero (24 doubles, rows u0 u1 u2 p phi):
-//   r = 0..2: K11(r,0) K11(r,1) K11(r,2) G(r) ULS(r)    (offset 5 r)
-//   r = 3:    D1(0) D1(1) D1(2) M PLS                   (offset 15)
-//   r = 4:    LSU(0) LSU(1) LSU(2) LS                   (offset 20)
-// Node vectors are [N][5] (u0 u1 u2 p phi).
+// Storage of the system, 24 contiguous doubles per nonzero k:
+//   A[k][0..15]   the 4x4 block [K11 G; D1 M] row-major, as the NS system
+//   A[k][16..23]  the phi couplings as pairs (col phi of row r, row phi of
+//                 col r): (ULS(0), LSU(0)) (ULS(1), LSU(1)) (ULS(2), LSU(2))
+//                 (PLS, LS)
+// Node vectors are [N][4] (u0 u1 u2 p) followed by [N] phi.
 //
 // One 64-lane wavefront per element, as the NS kernel: lane = (Gauss point,
 // node) for shape functions and Gauss-point data (LDS), lane = (a, b) for
@@ -57,7 +58,7 @@ k_assemble_nsvof_hex(int E, const int32_t *__restrict__ ien, const int32_t *__re
                      const int32_t *__restrict__ elems, int first, int count,
                      const int32_t *__restrict__ emap, const double *__restrict__ rec,
                      const uint8_t *__restrict__ ibc, NsvofParams P, int do_mat, int do_vec,
-                     double *__restrict__ A, double *__restrict__ rhs,
+                     int N, double *__restrict__ A, double *__restrict__ rhs,
                      double *__restrict__ cfl_out, int32_t *__restrict__ negdet) {
   __shared__ double s_sh[kWaves][8][8][5];  // [q][a]: N, dN/dx (3), shconv
   __shared__ double s_q[kWaves][8][kQs];
@@ -342,7 +343,7 @@ k_assemble_nsvof_hex(int E, const int32_t *__restrict__ ien, const int32_t *__re
     const unsigned ma = ibc[s_node[w][pa_]], mb = ibc[s_node[w][pb_]];
     const bool diag = pa_ == pb_;
     if (do_mat) {
-      double v[24];
+      double v[16], w8[8];
 #pragma unroll
       for (int r = 0; r < 3; ++r) {
         const bool rz = (ma >> r) & 1u;
@@ -350,37 +351,45 @@ k_assemble_nsvof_hex(int E, const int32_t *__restrict__ ien, const int32_t *__re
         for (int c = 0; c < 3; ++c) {
           double x = (rz || ((mb >> c) & 1u)) ? 0.0 : K[r * 3 + c];
           if (diag && r == c && rz) x = 1.0;
-          v[5 * r + c] = x;
+          v[4 * r + c] = x;
         }
-        v[5 * r + 3] = (rz || ((mb >> 6) & 1u)) ? 0.0 : Gv[r];
-        v[5 * r + 4] = ((mb >> 7) & 1u) ? 0.0 : ULS[r];
+        v[4 * r + 3] = (rz || ((mb >> 6) & 1u)) ? 0.0 : Gv[r];
+        w8[2 * r] = ((mb >> 7) & 1u) ? 0.0 : ULS[r];
       }
       const bool pz = (ma >> 6) & 1u, fz = (ma >> 7) & 1u;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) v[15 + c] = (pz || ((mb >> c) & 1u)) ? 0.0 : D[c];
-      v[18] = pz ? (diag ? 1.0 : 0.0) : Mv;
-      v[19] = (pz || ((mb >> 7) & 1u)) ? 0.0 : PLS;
+      for (int c = 0; c < 3; ++c) v[12 + c] = (pz || ((mb >> c) & 1u)) ? 0.0 : D[c];
+      v[15] = pz ? (diag ? 1.0 : 0.0) : Mv;
+      w8[6] = (pz || ((mb >> 7) & 1u)) ? 0.0 : PLS;
 #pragma unroll
       for (int c = 0; c < 3; ++c)
-        v[20 + c] = (((ma >> c) & 1u) || ((mb >> c) & 1u) || fz) ? 0.0 : LSU[c];
-      v[23] = (fz || ((mb >> 7) & 1u)) ? ((fz && diag) ? 1.0 : 0.0) : LS;
+        w8[2 * c + 1] = (((ma >> c) & 1u) || ((mb >> c) & 1u) || fz) ? 0.0 : LSU[c];
+      w8[7] = (fz || ((mb >> 7) & 1u)) ? ((fz && diag) ? 1.0 : 0.0) : LS;
       const int k = emap[(size_t)e * 64 + lane];
-      double2 *blk = reinterpret_cast<double2 *>(A + (size_t)k * 24);
+      double2 *b16 = reinterpret_cast<double2 *>(A + (size_t)k * 24);
+      double2 *b8 = reinterpret_cast<double2 *>(A + (size_t)k * 24 + 16);
 #pragma unroll
-      for (int t = 0; t < 12; ++t) {
-        double2 o = blk[t];
+      for (int t = 0; t < 8; ++t) {
+        double2 o = b16[t];
         o.x += v[2 * t];
         o.y += v[2 * t + 1];
-        blk[t] = o;
+        b16[t] = o;
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        double2 o = b8[t];
+        o.x += w8[2 * t];
+        o.y += w8[2 * t + 1];
+        b8[t] = o;
       }
     }
     if (do_vec && lane < 8) {
       const unsigned m0 = ibc[na];
-      double *rr = rhs + (size_t)na * 5;
+      double *rr = rhs + (size_t)na * 4;
 #pragma unroll
       for (int c = 0; c < 3; ++c) rr[c] += ((m0 >> c) & 1u) ? 0.0 : ru[c];
       rr[3] += ((m0 >> 6) & 1u) ? 0.0 : rp;
-      rr[4] += ((m0 >> 7) & 1u) ? 0.0 : rl;
+      rhs[(size_t)4 * N + na] += ((m0 >> 7) & 1u) ? 0.0 : rl;
     }
   }
 }
@@ -414,8 +423,8 @@ hipError_t launch_assemble_nsvof_hex(Ctx &c, int tensor_flag, int color) {
   hipLaunchKernelGGL(k_assemble_nsvof_hex, dim3(blocks), dim3(64 * kWaves), 0, c.stream,
                      c.nelem, c.d_ien, c.d_elm_id, c.d_color_elems, first, count, c.d_emap,
                      c.d_noderec, c.d_ibc, P, (tensor_flag & OFD_ASSEMBLE_MAT) ? 1 : 0,
-                     (tensor_flag & OFD_ASSEMBLE_VEC) ? 1 : 0, c.d_AV, c.d_rhsV, c.d_cfl,
-                     c.d_negdet);
+                     (tensor_flag & OFD_ASSEMBLE_VEC) ? 1 : 0, c.nnode, c.d_AV, c.d_rhsV,
+                     c.d_cfl, c.d_negdet);
   return hipGetLastError();
 }
 

@@ -147,9 +147,11 @@ __device__ __forceinline__ void axpy_batches(const V *__restrict__ W, size_t vs,
 // iteration 0 closes the norm.
 // KIND 0: NS node vectors, P = [N][16] block-Jacobi inverse;
 // KIND 1: the T system, P = inverse diagonal of the nsc scalar unknowns;
-// KIND 2: the NSVOF system, [N][5] node values, P = [N][12]: 3x3 momentum
-//         inverse, 1/M, 1/LS.
-// Kinds 1 and 2 are taken 4 values at a time; entries past nsc are 0.
+// KIND 2: the NSVOF system, [N][4] (u, p) node values then [N] phi (nsc = N);
+//         P = [N][16] block inverses of the 4x4 (u, p) diagonal blocks, then
+//         [N] 1/LS.
+// The scalar values of kind 1 and the phi values of kind 2 are taken 4 at a
+// time; entries past the last node are 0.
 template <typename V, int KIND>
 __global__ void __launch_bounds__(256)
 k_gmres_init(int N, int nsc, const double *__restrict__ P, const double *__restrict__ b,
@@ -185,23 +187,13 @@ k_gmres_init(int N, int nsc, const double *__restrict__ P, const double *__restr
         const int t = 4 * i + c;
         z[c] = (t < nsc) ? P[t] * r[c] : 0.0;
       }
-    } else if (KIND == 2) {
+    } else if (KIND == 2 && i >= nsc) {
+      // phi entries: tuples nsc.. hold phi of nodes 4 (i - nsc) + c, scaled
+      // by 1/LS (P + 16 nsc), zero past the last node
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        const int t = 4 * i + c;
-        double v = 0.0;
-        if (t < nsc) {
-          const int nd = t / 5, cc = t - 5 * nd;
-          const double *p = P + (size_t)nd * 12;
-          const double *bb = b + (size_t)nd * 5;
-          if (cc < 3)
-            v = p[cc * 3 + 0] * bb[0] + p[cc * 3 + 1] * bb[1] + p[cc * 3 + 2] * bb[2];
-          else if (cc == 3)
-            v = bb[3] * p[9];
-          else
-            v = bb[4] * p[10];
-        }
-        z[c] = v;
+        const int t = 4 * (i - nsc) + c;
+        z[c] = (t < nsc) ? P[(size_t)16 * nsc + t] * r[c] : 0.0;
       }
     } else {
       const double *p = P + (size_t)i * 16;
@@ -414,16 +406,17 @@ k_norm2_up(int N, const double *__restrict__ v, double *__restrict__ part) {
   block_partials<2>(acc, 2, part, 0);
 }
 
-// |u|^2, |p|^2, |phi|^2 partials of an [N][5] vector
+// |u|^2, |p|^2, |phi|^2 partials of an NSVOF vector ([N][4] then [N] phi)
 __global__ void __launch_bounds__(256)
 k_norm2_v(int N, const double *__restrict__ v, double *__restrict__ part) {
   double acc[3] = {0.0, 0.0, 0.0};
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N;
        i += gridDim.x * blockDim.x) {
-    const double *z = v + (size_t)i * 5;
+    const double *z = v + (size_t)i * 4;
+    const double f = v[(size_t)4 * N + i];
     acc[0] += z[0] * z[0] + z[1] * z[1] + z[2] * z[2];
     acc[1] += z[3] * z[3];
-    acc[2] += z[4] * z[4];
+    acc[2] += f * f;
   }
   block_partials<3>(acc, 3, part, 0);
 }
@@ -528,7 +521,7 @@ hipError_t launch_gmres_init(Ctx &c, const double *b, double tol, int kspace, in
   const int nb = nblocks_gm(c);
   const int kind = c.gm_kind;
   const double *P = kind == 1 ? c.d_PT : (kind == 2 ? c.d_PV : c.d_P64);
-  const int nsc = kind == 2 ? 5 * c.nnode : c.nnode;
+  const int nsc = c.nnode;
 #define OFD_INIT(V, K, W)                                                                   \
   hipLaunchKernelGGL((k_gmres_init<V, K>), dim3(nb), dim3(kRedBlock), 0, c.stream, c.gm_n, \
                      nsc, P, b, W, part_buf(c, 0), c.d_gs, tol, kspace, kmin)
@@ -616,7 +609,7 @@ hipError_t launch_norm2_up(Ctx &c, const double *v, int slot) {
   return hipGetLastError();
 }
 
-// |u|, |p|, |phi| of an [N][5] vector into GmresState::scal[1..3]
+// |u|, |p|, |phi| of an NSVOF vector into GmresState::scal[1..3]
 hipError_t launch_norm2_v(Ctx &c, const double *v) {
   const int nb = nblocks_nodes(c);
   double *pb = part_buf(c, 0);

@@ -223,7 +223,7 @@ struct TemSystem {
   }
 };
 
-// the NSVOF system's [N][5] vectors as ceil(5N/4) 4-tuples
+// the NSVOF system's vectors ([N][4] then [N] phi) as ceil(5N/4) 4-tuples
 struct NsvofSystem {
   Ctx &c;
   explicit NsvofSystem(Ctx &cc) : c(cc) {
@@ -682,7 +682,7 @@ int ofd_apply_dirichlet(ofd_ctx *x) {
 
 // SOLU / SOLP (/ SOLPHI) are views of the last solved system's increment
 static hipError_t sol_view(Ctx &c) {
-  if (c.sol_src == 2 && c.d_xV) return launch_node5_to_soa(c, c.d_xV);
+  if (c.sol_src == 2 && c.d_xV) return launch_nsvof_to_soa(c, c.d_xV);
   return launch_node4_to_soa(c, c.d_x, c.d_field[OFD_SOLU], c.d_field[OFD_SOLP]);
 }
 
@@ -1413,7 +1413,7 @@ static int ensure_nsvof(Ctx &c) {
     if (!rc) rc = dalloc(c, &c.d_xV, np);
     if (!rc) rc = dalloc(c, &c.d_rV, np);
     if (!rc) rc = dalloc(c, &c.d_yV, np);
-    if (!rc) rc = dalloc(c, &c.d_PV, (size_t)c.nnode * 12);
+    if (!rc) rc = dalloc(c, &c.d_PV, (size_t)c.nnode * 17);
     if (rc) return rc;
     OFD_HIP(hipMemsetAsync(c.d_rhsV, 0, np * sizeof(double), c.stream));
     OFD_HIP(hipMemsetAsync(c.d_xV, 0, np * sizeof(double), c.stream));
@@ -1436,20 +1436,44 @@ static int ensure_aV32(Ctx &c) {
   return OFD_OK;
 }
 
-// reference arrays <-> the 24-entry block of a nonzero
-static void nsvof_to_block(int64_t k, const double *k11, const double *g, const double *d1,
-                           const double *m, const double *ls, const double *lsu,
-                           const double *pls, const double *uls, double *b) {
-  for (int r = 0; r < 3; ++r) {
-    for (int c = 0; c < 3; ++c) b[5 * r + c] = k11[k * 9 + r * 3 + c];
-    b[5 * r + 3] = g[k * 3 + r];
-    b[5 * r + 4] = uls[k * 3 + r];
+// reference arrays (LHSK11 (9,Z), LHSG/LHSD1/LHSLSU/LHSULS (3,Z), LHSM/LHSLS/
+// LHSPLS (Z)) -> the device blocks A[k][24]: [K11 G; D1 M] row-major, then
+// (ULS(r), LSU(r)) r < 3, (PLS, LS)
+static void nsvof_to_device(int64_t Z, const double *k11, const double *g, const double *d1,
+                            const double *m, const double *ls, const double *lsu,
+                            const double *pls, const double *uls, double *A) {
+  for (int64_t k = 0; k < Z; ++k) {
+    double *b = A + k * 24, *e = b + 16;
+    for (int r = 0; r < 3; ++r) {
+      for (int c = 0; c < 3; ++c) b[4 * r + c] = k11[k * 9 + r * 3 + c];
+      b[4 * r + 3] = g[k * 3 + r];
+      b[12 + r] = d1[k * 3 + r];
+      e[2 * r] = uls[k * 3 + r];
+      e[2 * r + 1] = lsu[k * 3 + r];
+    }
+    b[15] = m[k];
+    e[6] = pls[k];
+    e[7] = ls[k];
   }
-  for (int c = 0; c < 3; ++c) b[15 + c] = d1[k * 3 + c];
-  b[18] = m[k];
-  b[19] = pls[k];
-  for (int c = 0; c < 3; ++c) b[20 + c] = lsu[k * 3 + c];
-  b[23] = ls[k];
+}
+
+// node vector: reference (u (N,3) column-major, p, phi) <-> [N][4] then [N]
+static void nsvof_pack(int N, const double *u, const double *p, const double *phi,
+                       double *v) {
+  for (int i = 0; i < N; ++i) {
+    for (int c = 0; c < 3; ++c) v[(size_t)i * 4 + c] = u[(size_t)c * N + i];
+    v[(size_t)i * 4 + 3] = p[i];
+    v[(size_t)4 * N + i] = phi[i];
+  }
+}
+
+static void nsvof_unpack(int N, const double *v, double *u, double *p, double *phi) {
+  for (int i = 0; i < N; ++i) {
+    if (u)
+      for (int c = 0; c < 3; ++c) u[(size_t)c * N + i] = v[(size_t)i * 4 + c];
+    if (p) p[i] = v[(size_t)i * 4 + 3];
+    if (phi) phi[i] = v[(size_t)4 * N + i];
+  }
 }
 
 int ofd_assemble_nsvof(ofd_ctx *x, int tensor_flag, double *cfl) {
@@ -1506,30 +1530,25 @@ int ofd_get_nsvof_system(ofd_ctx *x, double *k11, double *g, double *d1, double 
     rc = d2h(c, A.data(), c.d_AV, A.size() * sizeof(double));
     if (rc) return rc;
     for (int64_t k = 0; k < Z; ++k) {
-      const double *b = A.data() + k * 24;
+      const double *b = A.data() + k * 24, *e = b + 16;
       for (int r = 0; r < 3; ++r) {
         for (int cc = 0; cc < 3; ++cc)
-          if (k11) k11[k * 9 + r * 3 + cc] = b[5 * r + cc];
-        if (g) g[k * 3 + r] = b[5 * r + 3];
-        if (uls) uls[k * 3 + r] = b[5 * r + 4];
-        if (d1) d1[k * 3 + r] = b[15 + r];
-        if (lsu) lsu[k * 3 + r] = b[20 + r];
+          if (k11) k11[k * 9 + r * 3 + cc] = b[4 * r + cc];
+        if (g) g[k * 3 + r] = b[4 * r + 3];
+        if (d1) d1[k * 3 + r] = b[12 + r];
+        if (uls) uls[k * 3 + r] = e[2 * r];
+        if (lsu) lsu[k * 3 + r] = e[2 * r + 1];
       }
-      if (m) m[k] = b[18];
-      if (pls) pls[k] = b[19];
-      if (ls) ls[k] = b[23];
+      if (m) m[k] = b[15];
+      if (pls) pls[k] = e[6];
+      if (ls) ls[k] = e[7];
     }
   }
   if (rhsu || rhsp || rhsls) {
     std::vector<double> r((size_t)N * 5);
     rc = d2h(c, r.data(), c.d_rhsV, r.size() * sizeof(double));
     if (rc) return rc;
-    for (int i = 0; i < N; ++i) {
-      if (rhsu)
-        for (int cc = 0; cc < 3; ++cc) rhsu[(size_t)cc * N + i] = r[(size_t)i * 5 + cc];
-      if (rhsp) rhsp[i] = r[(size_t)i * 5 + 3];
-      if (rhsls) rhsls[i] = r[(size_t)i * 5 + 4];
-    }
+    nsvof_unpack(N, r.data(), rhsu, rhsp, rhsls);
   }
   return OFD_OK;
 }
@@ -1549,8 +1568,7 @@ int ofd_set_nsvof_system(ofd_ctx *x, const double *k11, const double *g, const d
     OFD_CHECK(g && d1 && m && ls && lsu && pls && uls, OFD_ERR_ARG,
               "the matrix needs all eight arrays");
     std::vector<double> A((size_t)Z * 24);
-    for (int64_t k = 0; k < Z; ++k)
-      nsvof_to_block(k, k11, g, d1, m, ls, lsu, pls, uls, A.data() + k * 24);
+    nsvof_to_device(Z, k11, g, d1, m, ls, lsu, pls, uls, A.data());
     rc = h2d(c, c.d_AV, A.data(), A.size() * sizeof(double));
     if (rc) return rc;
     c.matV_valid = true;
@@ -1559,11 +1577,7 @@ int ofd_set_nsvof_system(ofd_ctx *x, const double *k11, const double *g, const d
   if (rhsu) {
     OFD_CHECK(rhsp && rhsls, OFD_ERR_ARG, "the right-hand side needs all three arrays");
     std::vector<double> r((size_t)N * 5);
-    for (int i = 0; i < N; ++i) {
-      for (int cc = 0; cc < 3; ++cc) r[(size_t)i * 5 + cc] = rhsu[(size_t)cc * N + i];
-      r[(size_t)i * 5 + 3] = rhsp[i];
-      r[(size_t)i * 5 + 4] = rhsls[i];
-    }
+    nsvof_pack(N, rhsu, rhsp, rhsls, r.data());
     rc = h2d(c, c.d_rhsV, r.data(), r.size() * sizeof(double));
   }
   return rc;
@@ -1584,22 +1598,14 @@ int ofd_spmv_nsvof(ofd_ctx *x, int precision, const double *xu, const double *xp
   }
   const int N = c.nnode;
   std::vector<double> v((size_t)N * 5);
-  for (int i = 0; i < N; ++i) {
-    for (int cc = 0; cc < 3; ++cc) v[(size_t)i * 5 + cc] = xu[(size_t)cc * N + i];
-    v[(size_t)i * 5 + 3] = xp[i];
-    v[(size_t)i * 5 + 4] = xl[i];
-  }
+  nsvof_pack(N, xu, xp, xl, v.data());
   NsvofSystem sys(c);
   rc = h2d(c, c.d_yV, v.data(), v.size() * sizeof(double));
   if (rc) return rc;
   OFD_HIP(launch_spmv_v(c, precision, c.d_yV, c.d_rV, nullptr, 0));
   rc = d2h(c, v.data(), c.d_rV, v.size() * sizeof(double));
   if (rc) return rc;
-  for (int i = 0; i < N; ++i) {
-    for (int cc = 0; cc < 3; ++cc) yu[(size_t)cc * N + i] = v[(size_t)i * 5 + cc];
-    yp[i] = v[(size_t)i * 5 + 3];
-    yl[i] = v[(size_t)i * 5 + 4];
-  }
+  nsvof_unpack(N, v.data(), yu, yp, yl);
   return OFD_OK;
 }
 
@@ -1625,7 +1631,7 @@ int ofd_gmres_nsvof(ofd_ctx *x, int precision, int ortho, double tol, int kspace
   if (resid) *resid = res;
   if (rc) return rc;
   c.sol_src = 2;
-  OFD_HIP(launch_node5_to_soa(c, c.d_xV));
+  OFD_HIP(launch_nsvof_to_soa(c, c.d_xV));
   const size_t nb = (size_t)c.nnode * sizeof(double);
   if (solu) rc = d2h(c, solu, c.d_field[OFD_SOLU], 3 * nb);
   if (!rc && solp) rc = d2h(c, solp, c.d_field[OFD_SOLP], nb);
@@ -1649,7 +1655,7 @@ int ofd_solve_nsvof(ofd_ctx *x, int precision, int ortho, double rtol, double in
                   max_cycles, iters, cycles, relres);
   if (rc && rc != OFD_ERR_NOCONV) return rc;
   c.sol_src = 2;
-  OFD_HIP(launch_node5_to_soa(c, c.d_xV));
+  OFD_HIP(launch_nsvof_to_soa(c, c.d_xV));
   OFD_HIP(hipStreamSynchronize(c.stream));
   return rc;
 }

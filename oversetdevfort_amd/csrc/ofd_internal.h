@@ -140,12 +140,13 @@ struct Ctx {
   double *d_yT = nullptr;             // [4 * ceil(N/4)] product input scratch
   int64_t temT_icnt = 0;              // pattern size LHSTem was allocated for
   int temT_n4 = 0;                    // 4-tuples the T vectors were allocated for
-  // NSVOF system: 24 entries per nonzero (rows u0 u1 u2 p phi), node
-  // vectors [N][5] zero-padded to whole 4-tuples
-  double *d_AV = nullptr;             // [icnt][24]
+  // NSVOF system: [icnt][24] per nonzero, the 4x4 block [K11 G; D1 M] (the
+  // NS layout) then the phi couplings (ULS(r), LSU(r)) r < 3, (PLS, LS);
+  // node vectors [N][4] (u, p) then [N] phi, zero-padded to whole 4-tuples
+  double *d_AV = nullptr;
   float *d_AV32 = nullptr;
-  double *d_PV = nullptr;             // [N][12] momentum 3x3 inverse, 1/M, 1/LS
-  double *d_ApV = nullptr;            // [icnt][24] P^-1 A
+  double *d_PV = nullptr;             // [N][16] 4x4 block inverses, then [N] 1/LS
+  double *d_ApV = nullptr;            // P^-1 A, same two-array layout
   float *d_ApV32 = nullptr;
   bool matV_valid = false, precV_valid = false, aV32_valid = false, apV32_valid = false;
   double *d_rhsV = nullptr, *d_xV = nullptr, *d_rV = nullptr, *d_yV = nullptr;
@@ -202,7 +203,7 @@ hipError_t launch_axpy_fields(Ctx &c, int which);
 hipError_t launch_fields_s(Ctx &c, int which, const double *x, int fa, int faold, int fra,
                            int fraold);
 hipError_t launch_fields_v(Ctx &c);
-hipError_t launch_node5_to_soa(Ctx &c, const double *in);
+hipError_t launch_nsvof_to_soa(Ctx &c, const double *in);
 hipError_t launch_norm2_v(Ctx &c, const double *v);
 hipError_t launch_apply_dirichlet(Ctx &c);
 hipError_t launch_copy(Ctx &c, const void *src, void *dst, size_t nbytes);

@@ -50,6 +50,25 @@ struct Row4<float> {
   }
 };
 
+template <typename T>
+struct Pair;
+template <>
+struct Pair<double> {
+  static __device__ __forceinline__ void load(const double *p, double &a0, double &a1) {
+    const double2 u = reinterpret_cast<const double2 *>(p)[0];
+    a0 = u.x;
+    a1 = u.y;
+  }
+};
+template <>
+struct Pair<float> {
+  static __device__ __forceinline__ void load(const float *p, double &a0, double &a1) {
+    const float2 u = reinterpret_cast<const float2 *>(p)[0];
+    a0 = u.x;
+    a1 = u.y;
+  }
+};
+
 __device__ __forceinline__ void load_x4(const double *x, double &x0, double &x1,
                                         double &x2, double &x3) {
   const double2 u = reinterpret_cast<const double2 *>(x)[0];
@@ -169,117 +188,105 @@ __global__ void k_convert_s32(size_t n, const double *__restrict__ A, float *__r
   for (; i < n; i += stride) A32[i] = (float)A[i];
 }
 
-// 5-field product of the NSVOF system (SparseProd_NS_conv, SparseProd.f90:
-// 167-242): one lane per (row, field); rows u0 u1 u2 p of a block hold 5
-// entries (u0 u1 u2 p phi), the phi row 4 (u0 u1 u2 phi).  Flat entries
-// 5N .. npad-1 (the padding to whole 4-tuples) are written as 0.  mode as
-// k_spmv; XCD-aware block order.
+// Product of the coupled NS + VOF system (SparseProd_NS_conv, SparseProd.f90:
+// 167-242).  A nonzero k is 24 contiguous entries A[k][24]: the 4x4 block
+// [K11 G; D1 M] row-major (as the NS system), then the phi couplings as
+// pairs (col phi of row r, row phi of col r): (ULS(r), LSU(r)) for r < 3,
+// (PLS, LS).  Vectors are [N][4] (u0 u1 u2 p) followed by [N] phi,
+// zero-padded to whole 4-tuples (npad entries).
+//
+// 4 lanes per block row as in k_spmv: lane c reads row c of the 4x4 block
+// (32 / 16 bytes) and its pair (16 / 8 bytes) and forms output c, so the 4
+// lanes read the nonzero's 192 / 96 contiguous bytes;
+// it also forms the part of the phi row that multiplies x_c (LSU(c) x_uc,
+// or LS x_phi on lane 3); the four parts are added in a fixed shuffle order.
+// The 4 lanes after the last row write the padding zeros.  mode as k_spmv.
 template <typename T, typename V, int MODE>
 __global__ void __launch_bounds__(256)
-k_spmv_v(int n, int npad, const int32_t *__restrict__ col, const int32_t *__restrict__ row,
-         const T *__restrict__ A, const V *__restrict__ x, V *__restrict__ y,
-         const double *__restrict__ b, const GmresState *__restrict__ gs) {
+k_spmv_v(int N, int npad, const int32_t *__restrict__ col, const int32_t *__restrict__ row,
+         const T *__restrict__ A, const V *__restrict__ x,
+         V *__restrict__ y, const double *__restrict__ b, const GmresState *__restrict__ gs) {
   if (MODE == 2 && gs->done) return;
   if (MODE == 1 && gs && gs->outer_done) return;
   const int per = gridDim.x >> 3;
   const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-  const int t = lb * blockDim.x + threadIdx.x;
-  if (t >= npad) return;
-  if (t >= 5 * n) {
-    y[t] = (V)0.0;
+  const int gt = lb * blockDim.x + threadIdx.x;
+  const int r = gt >> 2;
+  const int c = gt & 3;
+  if (r > N) return;  // uniform over each group of 4 lanes
+  if (r == N) {
+    const int t = 5 * N + c;
+    if (t < npad) y[t] = (V)0.0;
     return;
   }
-  const int r = t / 5, c = t - 5 * r;
-  double acc = 0.0;
-  const int k1 = col[r + 1];
-  if (c < 4) {
-    for (int k = col[r]; k < k1; ++k) {
-      const T *a = A + (size_t)k * 24 + 5 * c;
-      const V *xv = x + (size_t)row[k] * 5;
-      acc += (double)a[0] * (double)xv[0] + (double)a[1] * (double)xv[1] +
-             (double)a[2] * (double)xv[2] + (double)a[3] * (double)xv[3] +
-             (double)a[4] * (double)xv[4];
-    }
-  } else {
-    for (int k = col[r]; k < k1; ++k) {
-      const T *a = A + (size_t)k * 24 + 20;
-      const V *xv = x + (size_t)row[k] * 5;
-      acc += (double)a[0] * (double)xv[0] + (double)a[1] * (double)xv[1] +
-             (double)a[2] * (double)xv[2] + (double)a[3] * (double)xv[4];
-    }
+  const V *xphi = x + (size_t)4 * N;
+  const int k0 = col[r], k1 = col[r + 1];
+  double acc = 0.0, accp = 0.0;
+#pragma unroll 4
+  for (int k = k0; k < k1; ++k) {
+    const int cn = row[k];
+    double a0, a1, a2, a3, x0, x1, x2, x3, e0, e1;
+    Row4<T>::load(A + (size_t)k * 24 + c * 4, a0, a1, a2, a3);
+    Pair<T>::load(A + (size_t)k * 24 + 16 + c * 2, e0, e1);
+    load_x4(x + (size_t)cn * 4, x0, x1, x2, x3);
+    const double xp = (double)xphi[cn];
+    acc += a0 * x0 + a1 * x1 + a2 * x2 + a3 * x3 + e0 * xp;
+    const double xc = (c == 0) ? x0 : ((c == 1) ? x1 : ((c == 2) ? x2 : xp));
+    accp += e1 * xc;
   }
-  if (MODE == 1)
-    y[t] = (V)(b[t] - acc);
-  else
-    y[t] = (V)acc;
+  accp += __shfl_xor(accp, 1);
+  accp += __shfl_xor(accp, 2);
+  if (MODE == 1) {
+    y[(size_t)r * 4 + c] = (V)(b[(size_t)r * 4 + c] - acc);
+    if (c == 0) y[(size_t)4 * N + r] = (V)(b[(size_t)4 * N + r] - accp);
+  } else {
+    y[(size_t)r * 4 + c] = (V)acc;
+    if (c == 0) y[(size_t)4 * N + r] = (V)accp;
+  }
 }
 
-// NSVOF block-Jacobi preconditioner (SparseGMRES_NS_conv.f:53-150): 3x3
-// cofactor inverse of the diagonal K11 block, 1/M (0 -> 1), 1/LS (0 -> 0).
-__global__ void k_build_prec_v(int N, const int32_t *__restrict__ diag,
-                               const double *__restrict__ A, double *__restrict__ P) {
+// phi part of the NSVOF block-Jacobi preconditioner (SparseGMRES_NS_conv.f:
+// 53-150): 1/LS with 0 -> 0; the u-p part is the NS block inverse of the 4x4
+// diagonal block (3x3 cofactor inverse of K11, 1/M with 0 -> 1)
+__global__ void k_build_prec_phi(int N, const int32_t *__restrict__ diag,
+                                 const double *__restrict__ A, double *__restrict__ pphi) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
-  const double *a = A + (size_t)diag[i] * 24;
-  const double K[9] = {a[0], a[1], a[2], a[5], a[6], a[7], a[10], a[11], a[12]};
-  double B[9];
-  B[0] = K[4] * K[8] - K[7] * K[5];
-  B[1] = K[7] * K[2] - K[1] * K[8];
-  B[2] = K[1] * K[5] - K[2] * K[4];
-  const double t = 1.0 / (B[0] * K[0] + B[1] * K[3] + B[2] * K[6]);
-  B[0] *= t;
-  B[1] *= t;
-  B[2] *= t;
-  B[3] = (K[5] * K[6] - K[3] * K[8]) * t;
-  B[4] = (K[0] * K[8] - K[6] * K[2]) * t;
-  B[5] = (K[3] * K[2] - K[0] * K[5]) * t;
-  B[6] = (K[3] * K[7] - K[4] * K[6]) * t;
-  B[7] = (K[6] * K[1] - K[0] * K[7]) * t;
-  B[8] = (K[0] * K[4] - K[1] * K[3]) * t;
-  double *p = P + (size_t)i * 12;
-  for (int q = 0; q < 9; ++q) p[q] = B[q];
-  const double m = a[18], l = a[23];
-  p[9] = (m == 0.0) ? 1.0 : 1.0 / m;
-  p[10] = (l == 0.0) ? 0.0 : 1.0 / l;
-  p[11] = 0.0;
+  const double l = A[(size_t)diag[i] * 24 + 23];
+  pphi[i] = (l == 0.0) ? 0.0 : 1.0 / l;
 }
 
-// Ap = P_i^-1 A for every nonzero of block row i; thread (i, r) forms row r
-__global__ void k_precondition_v(int N, const int32_t *__restrict__ col,
-                                 const double *__restrict__ P, const double *__restrict__ A,
-                                 double *__restrict__ Ap, float *__restrict__ Ap32) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 5 * N) return;
-  const int i = t / 5, r = t - 5 * i;
-  const double *p = P + (size_t)i * 12;
+// the couplings of P^-1 A: thread (i, c) forms, for every nonzero k of block
+// row i, entry (row c, col phi) = sum_s P_i(c, s) e[2s] and entry (row phi,
+// col c) = pphi_i e[2c+1], e = A[k][16..23]
+__global__ void k_precondition_phi(int N, const int32_t *__restrict__ col,
+                                   const double *__restrict__ P, const double *__restrict__ pphi,
+                                   const double *__restrict__ A, double *__restrict__ Ap,
+                                   float *__restrict__ Ap32) {
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = gt >> 2, c = gt & 3;
+  if (i >= N) return;
+  const double *p = P + (size_t)i * 16 + c * 4;
+  const double p0 = p[0], p1 = p[1], p2 = p[2], p3 = p[3], pf = pphi[i];
   for (int k = col[i]; k < col[i + 1]; ++k) {
-    const double *a = A + (size_t)k * 24;
-    double o[5];
-    int nw = 5, off = 5 * r;
-    if (r < 3) {
-      for (int c = 0; c < 5; ++c)
-        o[c] = p[r * 3 + 0] * a[c] + p[r * 3 + 1] * a[5 + c] + p[r * 3 + 2] * a[10 + c];
-    } else if (r == 3) {
-      for (int c = 0; c < 5; ++c) o[c] = a[15 + c] * p[9];
-    } else {
-      nw = 4;
-      off = 20;
-      for (int c = 0; c < 4; ++c) o[c] = a[20 + c] * p[10];
-    }
-    for (int c = 0; c < nw; ++c) {
-      Ap[(size_t)k * 24 + off + c] = o[c];
-      if (Ap32) Ap32[(size_t)k * 24 + off + c] = (float)o[c];
-    }
+    const double *a = A + (size_t)k * 24 + 16;
+    const double o0 = p0 * a[0] + p1 * a[2] + p2 * a[4] + p3 * a[6];
+    const double o1 = pf * a[2 * c + 1];
+    reinterpret_cast<double2 *>(Ap + (size_t)k * 24 + 16)[c] = make_double2(o0, o1);
+    if (Ap32)
+      reinterpret_cast<float2 *>(Ap32 + (size_t)k * 24 + 16)[c] =
+          make_float2((float)o0, (float)o1);
   }
 }
 
 // block-Jacobi inverse: 3x3 cofactor inverse of the K11 diagonal block and
 // 1/M (0 -> 1), SparseGMRES_NS.f90:73-121
+// (blocks of `stride` doubles whose first 16 are the 4x4 block)
 __global__ void k_build_prec(int N, const int32_t *__restrict__ diag,
-                             const double *__restrict__ A, double *__restrict__ P) {
+                             const double *__restrict__ A, int stride, double *__restrict__ P) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
-  const double *blk = A + (size_t)diag[i] * 16;
+  const double *blk = A + (size_t)diag[i] * stride;
   double B[9];
 #pragma unroll
   for (int r = 0; r < 3; ++r)
@@ -317,10 +324,10 @@ __global__ void k_build_prec(int N, const int32_t *__restrict__ diag,
 }
 
 // Ap[k] = P_i^-1 A[k] for every nonzero k of block row i; thread (i, c)
-// forms row c of the row's blocks
+// forms row c of the row's 4x4 blocks (the first 16 of `stride` doubles)
 __global__ void k_precondition_matrix(int N, const int32_t *__restrict__ col,
                                       const double *__restrict__ P,
-                                      const double *__restrict__ A,
+                                      const double *__restrict__ A, int stride,
                                       double *__restrict__ Ap, float *__restrict__ Ap32) {
   const int gt = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = gt >> 2, c = gt & 3;
@@ -328,16 +335,16 @@ __global__ void k_precondition_matrix(int N, const int32_t *__restrict__ col,
   const double *p = P + (size_t)i * 16 + c * 4;
   const double p0 = p[0], p1 = p[1], p2 = p[2], p3 = p[3];
   for (int k = col[i]; k < col[i + 1]; ++k) {
-    const double *a = A + (size_t)k * 16;
+    const double *a = A + (size_t)k * stride;
     double o[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q)
       o[q] = p0 * a[0 + q] + p1 * a[4 + q] + p2 * a[8 + q] + p3 * a[12 + q];
-    double2 *d = reinterpret_cast<double2 *>(Ap + (size_t)k * 16 + c * 4);
+    double2 *d = reinterpret_cast<double2 *>(Ap + (size_t)k * stride + c * 4);
     d[0] = make_double2(o[0], o[1]);
     d[1] = make_double2(o[2], o[3]);
     if (Ap32)
-      reinterpret_cast<float4 *>(Ap32 + (size_t)k * 16 + c * 4)[0] =
+      reinterpret_cast<float4 *>(Ap32 + (size_t)k * stride + c * 4)[0] =
           make_float4((float)o[0], (float)o[1], (float)o[2], (float)o[3]);
   }
 }
@@ -357,14 +364,14 @@ __global__ void k_convert_a32(size_t n4, const double *__restrict__ A,
 hipError_t launch_build_prec(Ctx &c) {
   const int N = c.nnode;
   hipLaunchKernelGGL(k_build_prec, dim3((N + 255) / 256), dim3(256), 0, c.stream, N,
-                     c.d_diag, c.d_A64, c.d_P64);
+                     c.d_diag, c.d_A64, 16, c.d_P64);
   return hipGetLastError();
 }
 
 hipError_t launch_precondition_matrix(Ctx &c, bool want32) {
   const unsigned nb = (unsigned)(((size_t)c.nnode * 4 + 255) / 256);
   hipLaunchKernelGGL(k_precondition_matrix, dim3(nb), dim3(256), 0, c.stream, c.nnode,
-                     c.d_col, c.d_P64, c.d_A64, c.d_Ap64, want32 ? c.d_Ap32 : nullptr);
+                     c.d_col, c.d_P64, c.d_A64, 16, c.d_Ap64, want32 ? c.d_Ap32 : nullptr);
   return hipGetLastError();
 }
 
@@ -448,42 +455,50 @@ hipError_t launch_convert_aT32(Ctx &c) {
   return hipGetLastError();
 }
 
-// flat [N][5] entries, one lane each, over the padded length, 8 blocks
+// 4 lanes per node row plus one group for the padding, rounded to 8 blocks
 static inline unsigned spmv_v_blocks(const Ctx &c) {
-  const unsigned nb = (unsigned)(((size_t)c.gm_n * 4 + 255) / 256);
+  const unsigned nb = (unsigned)(((size_t)(c.nnode + 1) * 4 + 255) / 256);
   return (nb + 7u) & ~7u;
+}
+
+template <typename T, typename V, int MODE>
+static void launch_v(Ctx &c, const T *A, const V *x, V *y, const double *b,
+                     hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
+  hipExtLaunchKernelGGL((k_spmv_v<T, V, MODE>), dim3(spmv_v_blocks(c)), dim3(256), 0,
+                        c.stream, ev0, ev1, 0, c.nnode, c.gm_n * 4, (const int32_t *)c.d_col,
+                        (const int32_t *)c.d_row, A, x, y, b, (const GmresState *)c.d_gs);
 }
 
 hipError_t launch_spmv_v(Ctx &c, int precision, const double *x, double *y,
                          const double *b, int mode) {
-  const unsigned nb = spmv_v_blocks(c);
-  const int n = c.nnode, np = c.gm_n * 4;
   if (precision == OFD_FP32) {
     if (mode == 0)
-      hipLaunchKernelGGL((k_spmv_v<float, double, 0>), dim3(nb), dim3(256), 0, c.stream, n,
-                         np, c.d_col, c.d_row, c.d_AV32, x, y, b, c.d_gs);
+      launch_v<float, double, 0>(c, c.d_AV32, x, y, b);
     else
-      hipLaunchKernelGGL((k_spmv_v<float, double, 1>), dim3(nb), dim3(256), 0, c.stream, n,
-                         np, c.d_col, c.d_row, c.d_AV32, x, y, b, c.d_gs);
+      launch_v<float, double, 1>(c, c.d_AV32, x, y, b);
   } else {
     if (mode == 0)
-      hipLaunchKernelGGL((k_spmv_v<double, double, 0>), dim3(nb), dim3(256), 0, c.stream, n,
-                         np, c.d_col, c.d_row, c.d_AV, x, y, b, c.d_gs);
+      launch_v<double, double, 0>(c, c.d_AV, x, y, b);
     else
-      hipLaunchKernelGGL((k_spmv_v<double, double, 1>), dim3(nb), dim3(256), 0, c.stream, n,
-                         np, c.d_col, c.d_row, c.d_AV, x, y, b, c.d_gs);
+      launch_v<double, double, 1>(c, c.d_AV, x, y, b);
   }
   return hipGetLastError();
 }
 
+// P (the 4x4 block inverses, [N][16]) and pphi ([N], after it in d_PV); the
+// preconditioned copy Ap = P^-1 A in the same layout
 hipError_t launch_build_prec_v(Ctx &c, bool want32) {
   const int N = c.nnode;
-  hipLaunchKernelGGL(k_build_prec_v, dim3((N + 255) / 256), dim3(256), 0, c.stream, N,
-                     c.d_diag, c.d_AV, c.d_PV);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_precondition_v, dim3((5 * N + 255) / 256), dim3(256), 0, c.stream, N,
-                     c.d_col, c.d_PV, c.d_AV, c.d_ApV, want32 ? c.d_ApV32 : nullptr);
+  double *P = c.d_PV, *pphi = c.d_PV + (size_t)N * 16;
+  hipLaunchKernelGGL(k_build_prec, dim3((N + 255) / 256), dim3(256), 0, c.stream, N, c.d_diag,
+                     c.d_AV, 24, P);
+  hipLaunchKernelGGL(k_build_prec_phi, dim3((N + 255) / 256), dim3(256), 0, c.stream, N,
+                     c.d_diag, c.d_AV, pphi);
+  const unsigned nb = (unsigned)(((size_t)N * 4 + 255) / 256);
+  hipLaunchKernelGGL(k_precondition_matrix, dim3(nb), dim3(256), 0, c.stream, N, c.d_col, P,
+                     c.d_AV, 24, c.d_ApV, want32 ? c.d_ApV32 : nullptr);
+  hipLaunchKernelGGL(k_precondition_phi, dim3(nb), dim3(256), 0, c.stream, N, c.d_col, P,
+                     pphi, c.d_AV, c.d_ApV, want32 ? c.d_ApV32 : nullptr);
   return hipGetLastError();
 }
 
@@ -496,28 +511,16 @@ hipError_t launch_convert_aV32(Ctx &c) {
 // Arnoldi product of the NSVOF system: W_{k+1} = (P^-1 A) W_k
 static hipError_t launch_spmv_arnoldi_v(Ctx &c, int precision, int k, hipEvent_t ev0,
                                         hipEvent_t ev1) {
-  const unsigned nb = spmv_v_blocks(c);
-  const int n = c.nnode, np = c.gm_n * 4;
   const size_t vs = (size_t)c.gm_n * 4;
-  if (c.basis32) {
-    hipExtLaunchKernelGGL((k_spmv_v<float, float, 2>), dim3(nb), dim3(256), 0, c.stream, ev0,
-                          ev1, 0, n, np, (const int32_t *)c.d_col, (const int32_t *)c.d_row,
-                          (const float *)c.d_ApV32, (const float *)(c.d_W32 + vs * k),
-                          c.d_W32 + vs * (k + 1), (const double *)nullptr,
-                          (const GmresState *)c.d_gs);
-  } else if (precision == OFD_FP32) {
-    hipExtLaunchKernelGGL((k_spmv_v<float, double, 2>), dim3(nb), dim3(256), 0, c.stream,
-                          ev0, ev1, 0, n, np, (const int32_t *)c.d_col,
-                          (const int32_t *)c.d_row, (const float *)c.d_ApV32,
-                          (const double *)(c.d_W + vs * k), c.d_W + vs * (k + 1),
-                          (const double *)nullptr, (const GmresState *)c.d_gs);
-  } else {
-    hipExtLaunchKernelGGL((k_spmv_v<double, double, 2>), dim3(nb), dim3(256), 0, c.stream,
-                          ev0, ev1, 0, n, np, (const int32_t *)c.d_col,
-                          (const int32_t *)c.d_row, (const double *)c.d_ApV,
-                          (const double *)(c.d_W + vs * k), c.d_W + vs * (k + 1),
-                          (const double *)nullptr, (const GmresState *)c.d_gs);
-  }
+  if (c.basis32)
+    launch_v<float, float, 2>(c, c.d_ApV32, c.d_W32 + vs * k, c.d_W32 + vs * (k + 1), nullptr,
+                              ev0, ev1);
+  else if (precision == OFD_FP32)
+    launch_v<float, double, 2>(c, c.d_ApV32, c.d_W + vs * k, c.d_W + vs * (k + 1), nullptr,
+                               ev0, ev1);
+  else
+    launch_v<double, double, 2>(c, c.d_ApV, c.d_W + vs * k, c.d_W + vs * (k + 1), nullptr,
+                                ev0, ev1);
   return hipGetLastError();
 }
 

@@ -168,14 +168,15 @@ hipError_t launch_fields_s(Ctx &c, int which, const double *x, int fa, int faold
   return hipGetLastError();
 }
 
-// NSVOF Newton update (solveFlow.f90:99-104) from x = [N][5]
+// NSVOF Newton update (solveFlow.f90:99-104) from x = [N][4] (u, p), [N] phi
 __global__ void k_fields_v(int N, double gami, double delt, double alfi,
                            const double *__restrict__ x, double *ug, double *acg, double *pg,
                            double *rphig, double *phig) {
 #pragma clang fp contract(off)  // products and sums rounded separately
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
-  const double *d = x + (size_t)i * 5;
+  const double *d = x + (size_t)i * 4;
+  const double dphi = x[(size_t)4 * N + i];
   const double fu = gami * delt;
   for (int c = 0; c < 3; ++c) {
     const size_t o = (size_t)c * N + i;
@@ -183,21 +184,22 @@ __global__ void k_fields_v(int N, double gami, double delt, double alfi,
     ug[o] = ug[o] + fu * d[c];
   }
   pg[i] = pg[i] + alfi * gami * delt * d[3];
-  rphig[i] = rphig[i] + d[4];
-  phig[i] = phig[i] + fu * d[4];
+  rphig[i] = rphig[i] + dphi;
+  phig[i] = phig[i] + fu * dphi;
 }
 
-// [N][5] -> u (N,3), p (N), phi (N)
-__global__ void k_node5_to_soa(int N, const double *__restrict__ in, double *u, double *p,
+// NSVOF vector ([N][4] u, p, then [N] phi) -> u (N,3), p (N), phi (N)
+__global__ void k_nsvof_to_soa(int N, const double *__restrict__ in, double *u, double *p,
                                double *phi) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
-  const double *d = in + (size_t)i * 5;
-  u[i] = d[0];
-  u[(size_t)N + i] = d[1];
-  u[2 * (size_t)N + i] = d[2];
-  p[i] = d[3];
-  phi[i] = d[4];
+  const double2 *s = reinterpret_cast<const double2 *>(in + (size_t)i * 4);
+  const double2 a = s[0], b = s[1];
+  u[i] = a.x;
+  u[(size_t)N + i] = a.y;
+  u[2 * (size_t)N + i] = b.x;
+  p[i] = b.y;
+  phi[i] = in[(size_t)4 * N + i];
 }
 
 hipError_t launch_fields_v(Ctx &c) {
@@ -208,9 +210,9 @@ hipError_t launch_fields_v(Ctx &c) {
   return hipGetLastError();
 }
 
-hipError_t launch_node5_to_soa(Ctx &c, const double *in) {
+hipError_t launch_nsvof_to_soa(Ctx &c, const double *in) {
   double **f = c.d_field;
-  hipLaunchKernelGGL(k_node5_to_soa, dim3((c.nnode + 255) / 256), dim3(256), 0, c.stream,
+  hipLaunchKernelGGL(k_nsvof_to_soa, dim3((c.nnode + 255) / 256), dim3(256), 0, c.stream,
                      c.nnode, in, f[OFD_SOLU], f[OFD_SOLP], f[OFD_SOLPHI]);
   return hipGetLastError();
 }
