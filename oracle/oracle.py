@@ -32,6 +32,20 @@ interp_hex             (extension, no reference code) closed-form local
                        coordinates on axis-aligned and affine (rotated)
                        grids; brute-force first-index donor over all
                        elements; exact interpolation of linear fields
+assemble_tem           reference IntElmAss_Tem_Quenching_STAB + BCLhs_tem +
+                       FillSparseMat_tem on tet meshes of fluid and solid
+                       elements: bit-exact, and within one rounding of
+                       norm2 with discontinuity capturing; Dirichlet rows
+                       are identity rows with zero RHS (tests)
+spmv_s                 reference SparseProd_LS bit-exact
+gmres_tem              reference SparseGMRES_tem bit-exact (solution)
+assemble_nsvof         reference IntElmAss_NSVOF_Quenching_STAB relinked to
+                       BCLhs_NS_conv + FillSparseMat_NS_conv on tet meshes:
+                       all eight LHS arrays and three RHS bit-exact, and
+                       within one rounding of norm2 with VOF discontinuity
+                       capturing
+spmv_ns_conv           reference SparseProd_NS_conv bit-exact
+gmres_ns_conv          reference SparseGMRES_ns_conv bit-exact (solution)
 =====================  =====================================================
 """
 from __future__ import annotations

@@ -227,14 +227,16 @@ module ofd_iface
     function ofd_get_tem_system(ctx, lhs, rhs) bind(C, name='ofd_get_tem_system') result(rc)
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
-      real(c_double), intent(out) :: lhs(*), rhs(*)
+      real(c_double), intent(out), optional :: lhs(*), rhs(*)
       integer(c_int) :: rc
     end function
 
+    ! an absent optional array reaches the library as a null pointer: that
+    ! part of the system is left as it is
     function ofd_set_tem_system(ctx, lhs, rhs) bind(C, name='ofd_set_tem_system') result(rc)
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
-      real(c_double), intent(in) :: lhs(*), rhs(*)
+      real(c_double), intent(in), optional :: lhs(*), rhs(*)
       integer(c_int) :: rc
     end function
 
@@ -296,8 +298,8 @@ module ofd_iface
                                   rhsls) bind(C, name='ofd_get_nsvof_system') result(rc)
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
-      real(c_double), intent(out) :: k11(*), g(*), d1(*), m(*), ls(*), lsu(*), pls(*), &
-                                     uls(*), rhsu(*), rhsp(*), rhsls(*)
+      real(c_double), intent(out), optional :: k11(*), g(*), d1(*), m(*), ls(*), lsu(*), &
+                                               pls(*), uls(*), rhsu(*), rhsp(*), rhsls(*)
       integer(c_int) :: rc
     end function
 
@@ -305,8 +307,8 @@ module ofd_iface
                                   rhsls) bind(C, name='ofd_set_nsvof_system') result(rc)
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
-      real(c_double), intent(in) :: k11(*), g(*), d1(*), m(*), ls(*), lsu(*), pls(*), &
-                                    uls(*), rhsu(*), rhsp(*), rhsls(*)
+      real(c_double), intent(in), optional :: k11(*), g(*), d1(*), m(*), ls(*), lsu(*), &
+                                              pls(*), uls(*), rhsu(*), rhsp(*), rhsls(*)
       integer(c_int) :: rc
     end function
 

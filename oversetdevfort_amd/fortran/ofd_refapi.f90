@@ -8,6 +8,9 @@
 !   * SparseProdUP_3D / SparseGMRES_up take host matrices exactly as the
 !     reference (SparseProd.f90:4-7, SparseGMRES_NS.f90:4-8) and upload them
 !     on every call (drop-in, correct, not the fast path);
+!   * SparseProd_LS, SparseGMRES_tem, SparseProd_NS_conv and
+!     SparseGMRES_ns_conv do the same for the quenching solver's temperature
+!     and coupled NS + VOF systems;
 !   * assembleNavSto(flag, sub) (assembleNavSto.f90:4) assembles from the
 !     state held on the device; solveNS_resident solves that system without
 !     leaving the device -- the fast path a GPU driver uses.
@@ -111,6 +114,76 @@ subroutine SparseGMRES_tem(lhsM, Utol, col, row, rhsG, sol, Kspaceu, Kspaceu_mn,
   end if
   deallocate (dT)
 end subroutine SparseGMRES_tem
+
+!----------------------------------------------------------------------
+! SparseProd.f90:128-161
+!----------------------------------------------------------------------
+subroutine SparseProd_LS(lhsM, col, row, rhstmp, prodtmp, NNODZu, NSHLu, icntu, NSD)
+  use ofd_state
+  implicit none
+  integer, intent(in) :: NNODZu, NSHLu, icntu, NSD
+  integer, intent(in) :: col(NNODZu + 1), row(icntu)
+  real(8), intent(in) :: lhsM(icntu), rhstmp(NNODZu)
+  real(8), intent(out) :: prodtmp(NNODZu)
+  call ofd_check(ofd_set_tem_system(ofd_ctx, lhs=lhsM), 'SparseProd_LS')
+  call ofd_check(ofd_spmv_tem(ofd_ctx, OFD_FP64, rhstmp, prodtmp), 'SparseProd_LS')
+end subroutine SparseProd_LS
+
+!----------------------------------------------------------------------
+! SparseProd.f90:167-242
+!----------------------------------------------------------------------
+subroutine SparseProd_NS_conv(col, row, lhsK11, lhsG, lhsD1, lhsM, lhsLS, lhsLSu, &
+                              LHSPls, LHSUls, rhstmpu, rhstmpp, rhstmpls, &
+                              prodtmpu, prodtmpp, prodtmpls, D_FLAG, P_FLAG, &
+                              NNODZu, NSHLu, icntu, NSD)
+  use ofd_state
+  implicit none
+  integer, intent(in) :: NNODZu, NSHLu, icntu, NSD
+  integer, intent(in) :: col(NNODZu + 1), row(icntu), D_FLAG(NNODZu), P_FLAG(NNODZu)
+  real(8), intent(in) :: lhsK11(NSD*NSD, icntu), lhsG(NSD, icntu), lhsD1(NSD, icntu), &
+                         lhsM(icntu), lhsLS(icntu), lhsLSu(NSD, icntu), LHSPls(icntu), &
+                         LHSUls(NSD, icntu), rhstmpu(NNODZu, NSD), rhstmpp(NNODZu), &
+                         rhstmpls(NNODZu)
+  real(8), intent(out) :: prodtmpu(NNODZu, NSD), prodtmpp(NNODZu), prodtmpls(NNODZu)
+  call ofd_check(ofd_set_nsvof_system(ofd_ctx, lhsK11, lhsG, lhsD1, lhsM, lhsLS, lhsLSu, &
+                                      LHSPls, LHSUls), 'SparseProd_NS_conv')
+  call ofd_check(ofd_spmv_nsvof(ofd_ctx, OFD_FP64, rhstmpu, rhstmpp, rhstmpls, prodtmpu, &
+                                prodtmpp, prodtmpls), 'SparseProd_NS_conv')
+end subroutine SparseProd_NS_conv
+
+!----------------------------------------------------------------------
+! SparseGMRES_NS_conv.f:1-406: the increment is added to solu/solp/solls,
+! as the reference does
+!----------------------------------------------------------------------
+subroutine SparseGMRES_ns_conv(col, row, IBC, IPER, D_FLAG, P_FLAG, rhsGu, rhsGp, rhsGls, &
+                               solu, solp, solls, lhsK11, lhsG, lhsD1, lhsM, lhsls, &
+                               LHSlsu, LHSPls, LHSUls, icntu, Utol, Kspaceu, Kspaceu_mn, &
+                               NNODZu, NSHLu, NSD)
+  use ofd_state
+  implicit none
+  integer, intent(in) :: icntu, NNODZu, NSHLu, NSD, Kspaceu, Kspaceu_mn
+  integer, intent(in) :: col(NNODZu + 1), row(icntu), IBC(NNODZu, 2*NSD + 1), &
+                         IPER(NNODZu), D_FLAG(NNODZu), P_FLAG(NNODZu)
+  real(8), intent(in) :: rhsGu(NNODZu, NSD), rhsGp(NNODZu), rhsGls(NNODZu), &
+                         lhsK11(NSD*NSD, icntu), lhsG(NSD, icntu), lhsD1(NSD, icntu), &
+                         lhsM(icntu), lhsls(icntu), LHSlsu(NSD, icntu), LHSPls(icntu), &
+                         LHSUls(NSD, icntu), Utol
+  real(8), intent(inout) :: solu(NNODZu, NSD), solp(NNODZu), solls(NNODZu)
+  real(8), allocatable :: du(:, :), dp(:), dl(:)
+  integer(c_int) :: iters
+  real(c_double) :: resid
+  allocate (du(NNODZu, NSD), dp(NNODZu), dl(NNODZu))
+  call ofd_check(ofd_set_nsvof_system(ofd_ctx, lhsK11, lhsG, lhsD1, lhsM, lhsls, LHSlsu, &
+                                      LHSPls, LHSUls, rhsGu, rhsGp, rhsGls), &
+                 'SparseGMRES_ns_conv')
+  call ofd_check(ofd_gmres_nsvof(ofd_ctx, ofd_precision, ofd_ortho, Utol, &
+                                 int(Kspaceu, c_int), int(Kspaceu_mn, c_int), du, dp, dl, &
+                                 iters, resid), 'SparseGMRES_ns_conv')
+  solu = solu + du
+  solp = solp + dp
+  solls = solls + dl
+  deallocate (du, dp, dl)
+end subroutine SparseGMRES_ns_conv
 
 !----------------------------------------------------------------------
 ! assembleNavSto.f90:4-72 on the device-resident state

@@ -1,0 +1,136 @@
+!======================================================================
+! Drop-ins that read and write the reference's module state (aAdjKeep,
+! commonvars, commonpars).  They compile only inside the reference's own
+! build, after modules.f90 and next to ofd_iface.f90 / ofd_refapi.f90:
+!
+!   * assembleQuenching(assemble_tensor_flag, assemble_field_flag)
+!     (solveQuenching.f90:4-120): copies the module's node fields, flags and
+!     material to the device, assembles the NS + VOF and/or temperature
+!     systems there (ofd_assemble_nsvof, ofd_assemble_tem) and copies the
+!     assembled parts back into LHSK11 ... LHSTem, RHSGu ... RHSGtem.  The
+!     reference's face terms (FaceAssembly_NS_weak_CF, FaceAssembly_NS_outflow,
+!     FaceAssembly_ConvectiveH) are not part of this drop-in.
+!   * interpolation_tet(mesh_id) (interpolation.f90:5-76): fills
+!     bmesh(mesh_id)%ug from the device donor search over the elements with
+!     ELM_ID == SRC_ID_SET(mesh_id).  The reference's setBCs_overset
+!     (setBCs.f90:124-167) calls it unchanged.
+!
+! The mesh and its pattern are uploaded once by the driver (ofd_upload_mesh
+! with IEN and ELM_ID, ofd_build_pattern), as for the other drop-ins.
+!======================================================================
+module ofd_refmod
+  use ofd_state
+  use aAdjKeep
+  use commonvars
+  use commonpars
+  implicit none
+contains
+  ! module node fields, IBC, the generalized-alpha constants and the quench
+  ! material (param.f90:222-257) -> the device context
+  subroutine ofd_push_quench_state()
+    real(8) :: prm(OFD_NPARAMS), q(OFD_NQPARAMS)
+    prm = (/almi, alfi, gami, beti, Delt, rhow, muw, NS_kdc_w, gravvec(1), gravvec(2), &
+            gravvec(3)/)
+    call ofd_check(ofd_set_params(ofd_ctx, prm), 'ofd_push_quench_state')
+    q = (/rhoa, rhow, mua, muw, cpa, cpw, kappaa, kappaw, rhos, mus, cps, kappas, Ts, lh, &
+          c_cond, c_evap, C_DMDOT, TEM_kdc, Tref, 102.0d0, LSC_kdc/)
+    call ofd_check(ofd_set_quench_params(ofd_ctx, q), 'ofd_push_quench_state')
+    call ofd_check(ofd_set_ibc(ofd_ctx, IBC, int(size(IBC, 2), c_int)), 'ofd_push_quench_state')
+    call push(OFD_XG, xg)
+    call push(OFD_DG, dg)
+    call push(OFD_DGOLD, dgold)
+    call push(OFD_UG, ug)
+    call push(OFD_UGOLD, ugold)
+    call push(OFD_UGM, ugm)
+    call push(OFD_UGMOLD, ugmold)
+    call push(OFD_ACG, acg)
+    call push(OFD_ACGOLD, acgold)
+    call push(OFD_ACGM, acgm)
+    call push(OFD_ACGMOLD, acgmold)
+    call push(OFD_PG, pg)
+    call push(OFD_PGOLD, pgold)
+    call push(OFD_PHIG, phig)
+    call push(OFD_PHIGOLD, phigold)
+    call push(OFD_RPHIG, rphig)
+    call push(OFD_RPHIGOLD, rphigold)
+    call push(OFD_TG, Tg)
+    call push(OFD_TGOLD, Tgold)
+    call push(OFD_RTG, rTg)
+    call push(OFD_RTGOLD, rTgold)
+  end subroutine ofd_push_quench_state
+
+  subroutine push(field, a)
+    integer(c_int), intent(in) :: field
+    real(8), intent(in) :: a(*)
+    call ofd_check(ofd_set_field(ofd_ctx, field, a), 'ofd_push_quench_state')
+  end subroutine push
+end module ofd_refmod
+
+!----------------------------------------------------------------------
+! solveQuenching.f90:4-120 (without the face terms)
+!----------------------------------------------------------------------
+subroutine assembleQuenching(assemble_tensor_flag, assemble_field_flag)
+  use ofd_refmod
+  implicit none
+  integer, intent(in) :: assemble_tensor_flag, assemble_field_flag
+  logical :: vec, mat
+  real(c_double) :: cfl
+  vec = iand(assemble_tensor_flag, ASSEMBLE_TENSOR_VEC) > 0
+  mat = iand(assemble_tensor_flag, ASSEMBLE_TENSOR_MAT) > 0
+  ! the reference zeroes every quench RHS / LHS array the flags name,
+  ! whichever fields it then assembles
+  if (vec) then
+    RHSGu = 0.0d0
+    RHSGp = 0.0d0
+    RHSGls = 0.0d0
+    RHSGtem = 0.0d0
+  end if
+  if (mat) then
+    LHSK11 = 0.0d0
+    LHSG = 0.0d0
+    LHSD1 = 0.0d0
+    LHSM = 0.0d0
+    LHSLS = 0.0d0
+    LHSULS = 0.0d0
+    LHSLSU = 0.0d0
+    LHSPLS = 0.0d0
+    LHSTem = 0.0d0
+  end if
+  call ofd_push_quench_state()
+  if (iand(assemble_field_flag, ASSEMBLE_FIELD_NS + ASSEMBLE_FIELD_VOF) > 0) then
+    call ofd_check(ofd_assemble_nsvof(ofd_ctx, int(assemble_tensor_flag, c_int), cfl), &
+                   'assembleQuenching')
+    if (mat) call ofd_check(ofd_get_nsvof_system(ofd_ctx, k11=LHSK11, g=LHSG, d1=LHSD1, &
+                                                 m=LHSM, ls=LHSLS, lsu=LHSLSU, pls=LHSPLS, &
+                                                 uls=LHSULS), 'assembleQuenching')
+    if (vec) call ofd_check(ofd_get_nsvof_system(ofd_ctx, rhsu=RHSGu, rhsp=RHSGp, &
+                                                 rhsls=RHSGls), 'assembleQuenching')
+  end if
+  if (iand(assemble_field_flag, ASSEMBLE_FIELD_TEM) > 0) then
+    call ofd_check(ofd_assemble_tem(ofd_ctx, int(assemble_tensor_flag, c_int)), &
+                   'assembleQuenching')
+    if (mat) call ofd_check(ofd_get_tem_system(ofd_ctx, lhs=LHSTem), 'assembleQuenching')
+    if (vec) call ofd_check(ofd_get_tem_system(ofd_ctx, rhs=RHSGtem), 'assembleQuenching')
+  end if
+end subroutine assembleQuenching
+
+!----------------------------------------------------------------------
+! interpolation.f90:5-76: receivers are bmesh(mesh_id)%xg + %dg; donors
+! the elements with ELM_ID == SRC_ID_SET(mesh_id), first in index order;
+! receivers without a donor get 0, as the reference leaves them
+!----------------------------------------------------------------------
+subroutine interpolation_tet(mesh_id)
+  use ofd_refmod
+  implicit none
+  integer, intent(in) :: mesh_id
+  integer(c_int32_t), allocatable :: donor(:)
+  integer :: n
+  n = bmesh(mesh_id)%NNODE
+  allocate (donor(n))
+  call ofd_check(ofd_set_field(ofd_ctx, OFD_DG, dg), 'interpolation_tet')
+  call ofd_check(ofd_set_field(ofd_ctx, OFD_UG, ug), 'interpolation_tet')
+  call ofd_check(ofd_fringe_interp(ofd_ctx, int(SRC_ID_SET(mesh_id), c_int32_t), &
+                                   int(n, c_int), bmesh(mesh_id)%xg, bmesh(mesh_id)%dg, &
+                                   donor, bmesh(mesh_id)%ug), 'interpolation_tet')
+  deallocate (donor)
+end subroutine interpolation_tet
