@@ -53,16 +53,20 @@ enum {
   OFD_ORTHO_CGS1 = 2  /* classical GS, one projection (restart cycles are short) */
 };
 
-/* node fields held by a context: (NNODE,3) vectors from OFD_XG to OFD_ACGMOLD
- * and OFD_SOLU; (NNODE) scalars otherwise.  The quenching fields (VOF phig,
- * its rate rphig, temperature Tg, its rate rTg and the T increment SOLT)
- * follow modules.f90:291-300. */
+/* node fields held by a context: (NNODE,3) vectors from OFD_XG to OFD_ACGMOLD,
+ * OFD_SOLU and OFD_LAMU; (NNODE) scalars otherwise.  The quenching fields
+ * (VOF phig, its rate rphig, temperature Tg, its rate rTg and the T
+ * increment SOLT) follow modules.f90:291-300.  OFD_LAMU/OFD_LAMP hold the
+ * last adjoint solution; OFD_MUSCALE, once set, scales the viscosity of the
+ * NS assembly per node (mu at a Gauss point = mu * sum_a N_a s_a), the
+ * parameter of the viscosity gradient (no reference counterpart). */
 enum {
   OFD_XG = 0, OFD_DG, OFD_DGOLD, OFD_UG, OFD_UGOLD, OFD_UGM, OFD_UGMOLD,
   OFD_ACG, OFD_ACGOLD, OFD_ACGM, OFD_ACGMOLD, OFD_PG, OFD_PGOLD,
   OFD_SOLU, OFD_SOLP,
   OFD_PHIG, OFD_PHIGOLD, OFD_RPHIG, OFD_RPHIGOLD, OFD_TG, OFD_TGOLD,
-  OFD_RTG, OFD_RTGOLD, OFD_SOLT, OFD_SOLPHI, OFD_NFIELDS
+  OFD_RTG, OFD_RTGOLD, OFD_SOLT, OFD_SOLPHI, OFD_LAMU, OFD_LAMP, OFD_MUSCALE,
+  OFD_NFIELDS
 };
 
 /* parameter vector of ofd_set_params */
@@ -294,6 +298,74 @@ int ofd_time_assemble(ofd_ctx *ctx, int tensor_flag, int reps, double *ms);
  * HBM-bound kernels (and the calibration of hardware byte counters).
  * Allocates and frees its two buffers.  No reference counterpart. */
 int ofd_time_copy(ofd_ctx *ctx, int64_t nbytes, int reps, double *ms);
+
+/* ---- adjoint of the NS Newton step (no reference code) ----
+ *
+ * The residual is the assembled NS right-hand side as the reference computes
+ * it (IntElmAss_NS_overset, assembleNavSto.f90:76-336: e3Rhs_3D_fluid with
+ * uprime = 0), F(u, ac, p; s) at the alpha stage, before the Dirichlet rows
+ * are cleared.  Its exact Jacobian with respect to the Newton increments
+ * (acg += d, ug += gami*Delt d, pg += alfi*gami*Delt dp) is
+ * J = -dF/d(d, dp); J_bc is J with the Dirichlet rows and columns of IBC
+ * replaced as BCLhs_3D does (identity per element sharing the node).  J
+ * differs from the assembled LHS (SUPG/PSPG blocks, Picard convection). */
+
+/* Transposed product of the assembled LHS: y = LHS^T x, (N,3)+(N) vectors. */
+int ofd_spmv_up_t(ofd_ctx *ctx, int precision, const double *xu, const double *xp,
+                  double *yu, double *yp);
+
+/* Restarted GMRES on LHS^T lambda = g (FP64 outer residual, as ofd_solve);
+ * lambda goes to OFD_LAMU / OFD_LAMP. */
+int ofd_solve_t(ofd_ctx *ctx, int precision, int ortho, double rtol, double inner_tol,
+                int kspace, int max_cycles, const double *gu, const double *gp, int *iters,
+                int *cycles, double *relres);
+
+/* Element VJP of the residual: for lambda (lu (N,3), lp (N)) the gradients
+ * of lambda . F with respect to the alpha-stage u, ac, p and to the
+ * viscosity scale OFD_MUSCALE (constant 1 when unset), before any Dirichlet
+ * projection; elements of the subdomain flag as ofd_assemble_ns.  Any
+ * output may be NULL. */
+int ofd_vjp_ns(ofd_ctx *ctx, int subdomain_flag, const double *lu, const double *lp,
+               double *gu, double *gac, double *gp, double *gmus);
+
+/* y = J_bc^T lambda, matrix-free (element VJP with the Dirichlet projection). */
+int ofd_jt_apply(ofd_ctx *ctx, int subdomain_flag, const double *lu, const double *lp,
+                 double *yu, double *yp);
+
+/* NS residual form for ofd_assemble_ns and the adjoint: 0 (default) the
+ * reference's live call, e3Rhs_3D_fluid with uprime = 0; 1 the residual with
+ * the fine-scale velocity uprime = -tauM rLi (e3RHS.f90:31, tauBar = 0),
+ * whose continuity rows carry the PSPG term.  The LHS is the same for both.
+ * With the reference's form the pressure rows of an enclosed flow may have
+ * no exact root (the equal-order element without PSPG); the fine-scale form
+ * is stabilized. */
+int ofd_set_residual_form(ofd_ctx *ctx, int form);
+
+/* y = J_bc v, v a Newton increment (forward tangent of the residual). */
+int ofd_j_apply(ofd_ctx *ctx, int subdomain_flag, const double *vu, const double *vp,
+                double *yu, double *yp);
+
+/* J_bc^T lambda = g: restarted flexible GMRES (kspace outer steps, at most
+ * max_cycles restarts) on the exact operator, right-preconditioned by LHS^T
+ * solved to inner_tol with the restarted device GMRES in `precision` (FP64
+ * or mixed); the outer residual and basis are FP64.  iters counts the inner
+ * GMRES iterations, cycles the restarts.  lambda goes to OFD_LAMU /
+ * OFD_LAMP.  Needs an assembled LHS.  J_bc has no pressure block, so g must
+ * not load the pressure modes it leaves free (velocity objectives have
+ * g_p = 0). */
+int ofd_solve_adjoint(ofd_ctx *ctx, int subdomain_flag, int precision, int ortho, double rtol,
+                      double inner_tol, int kspace, int max_cycles, const double *gu,
+                      const double *gp, int *iters, int *cycles, double *relres);
+
+/* The consistent Newton correction: J_bc d = RHS (the assembled right-hand
+ * side) by the same flexible GMRES, preconditioned by the assembled LHS;
+ * the increment goes where ofd_solve puts it, for ofd_newton_update.  With
+ * it the Newton iteration converges to the residual's own root, which the
+ * LHS of the reference approaches only linearly.  Needs an assembled LHS
+ * and RHS. */
+int ofd_solve_newton(ofd_ctx *ctx, int subdomain_flag, int precision, int ortho, double rtol,
+                     double inner_tol, int kspace, int max_cycles, int *iters, int *cycles,
+                     double *relres);
 
 /* Mean device time (HIP events on the context stream) of `reps` back-to-back
  * launches of one quenching-solver kernel: what = OFD_TIME_ASM_NSVOF or

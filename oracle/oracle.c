@@ -331,12 +331,12 @@ int orc_sparse_loc(const int *row, int lo, int hi, int goal) {
  */
 int orc_element_ns(int nshl, int ngauss, const double *xl, const double *dl,
                    const double *ul, const double *uml, const double *acl,
-                   const double *acml, const double *pl, const double *prm,
-                   int tensor_flag, double *ke, double *ge, double *de,
-                   double *me, double *ru, double *rp, double *cfl) {
+                   const double *acml, const double *pl, const double *musl,
+                   const double *prm, int vms, int tensor_flag, double *ke, double *ge,
+                   double *de, double *me, double *ru, double *rp, double *cfl) {
   (void)acml;
   const double almi = prm[0], alfi = prm[1], gami = prm[2];
-  const double delt = prm[4], rhoi = prm[5], mui = prm[6], kdc = prm[7];
+  const double delt = prm[4], rhoi = prm[5], mu0 = prm[6], kdc = prm[7];
   const double grav[3] = {prm[8], prm[9], prm[10]};
   double gp[8 * 3], gw[8];
   int nneg = 0;
@@ -364,6 +364,14 @@ int orc_element_ns(int nshl, int ngauss, const double *xl, const double *dl,
       aci[i] = sa;
     }
     for (int a = 0; a < nshl; ++a) pri += pl[a] * shlu[a];
+    /* viscosity at the Gauss point: mu, or mu times the interpolated
+     * per-node scale (an extension for the viscosity gradient) */
+    double mui = mu0;
+    if (musl) {
+      double sc = 0.0;
+      for (int a = 0; a < nshl; ++a) sc += musl[a] * shlu[a];
+      mui = mu0 * sc;
+    }
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j) {
         double s = 0.0;
@@ -390,7 +398,6 @@ int orc_element_ns(int nshl, int ngauss, const double *xl, const double *dl,
     orc_stab_ns(Gij, delt, uadv, rhoi, mui, &tauM, &tauP, &tauC);
     double c = orc_cfl(uadv, Gij, delt);
     if (c > *cfl) *cfl = c;
-    (void)rLi;
 
     if (do_mat) {
       /* e3LHS.f90:34-159 */
@@ -433,13 +440,18 @@ int orc_element_ns(int nshl, int ngauss, const double *xl, const double *dl,
         }
     }
     if (do_vec) {
-      /* e3RHS.f90:26-95 with uprime = 0 */
+      /* e3RHS.f90:26-95 with tauBar = 0 and uprime = 0 as in the live call,
+       * or (vms, an extension) uprime = -tauM rLi (e3RHS.f90:31) */
       const double mupkdc = mui + kdc;
       const double ptot = pri - tauC * divu;
-      double t1[3], t2[9];
+      double up[3] = {0.0, 0.0, 0.0}, adv[3], t1[3], t2[9];
+      for (int i = 0; i < 3; ++i) {
+        if (vms) up[i] = -tauM * rLi[i];
+        adv[i] = uadv[i] + up[i];
+      }
       for (int i = 0; i < 3; ++i)
-        t1[i] = rhoi * (aci[i] + uadv[0] * M3(du, i, 0) + uadv[1] * M3(du, i, 1) +
-                        uadv[2] * M3(du, i, 2)) -
+        t1[i] = rhoi * (aci[i] + adv[0] * M3(du, i, 0) + adv[1] * M3(du, i, 1) +
+                        adv[2] * M3(du, i, 2)) -
                 fi[i];
       for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) {
@@ -447,6 +459,7 @@ int orc_element_ns(int nshl, int ngauss, const double *xl, const double *dl,
             M3(t2, i, j) = -ptot + 2.0 * mupkdc * M3(du, i, i);
           else
             M3(t2, i, j) = mupkdc * (M3(du, i, j) + M3(du, j, i));
+          if (vms) M3(t2, i, j) = M3(t2, i, j) - rhoi * adv[j] * up[i];
         }
       for (int a = 0; a < nshl; ++a) {
         for (int i = 0; i < 3; ++i) {
@@ -455,8 +468,12 @@ int orc_element_ns(int nshl, int ngauss, const double *xl, const double *dl,
           ru[a * 3 + i] = ru[a * 3 + i] - (shlu[a] * t1[i] + s) * detj * gwt;
         }
       }
-      for (int a = 0; a < nshl; ++a)
-        rp[a] = rp[a] - (shlu[a] * divu - 0.0) * detj * gwt;
+      for (int a = 0; a < nshl; ++a) {
+        double s = 0.0;
+        if (vms)
+          for (int j = 0; j < 3; ++j) s += shg[j * nshl + a] * up[j];
+        rp[a] = rp[a] - (shlu[a] * divu - s) * detj * gwt;
+      }
     }
   }
   return nneg;
@@ -510,13 +527,13 @@ int orc_assemble_ns(int nnode, int nelem, int nshl, int ngauss,
                     int subdomain_flag, int tensor_flag, const double *xg,
                     const int *ibc, int ibc_ncol, const double *dgA,
                     const double *ugA, const double *ugmA, const double *acgA,
-                    const double *acgmA, const double *pgA, const double *prm,
-                    const int *col, const int *row, double *k11, double *g,
-                    double *d1, double *m, double *rhsu, double *rhsp,
-                    double *cfl_out) {
+                    const double *acgmA, const double *pgA, const double *mus,
+                    const double *prm, int vms, const int *col, const int *row,
+                    double *k11, double *g, double *d1, double *m, double *rhsu,
+                    double *rhsp, double *cfl_out) {
   (void)ibc_ncol;
   const int n2 = nshl * nshl;
-  double xl[24], dl[24], ul[24], uml[24], acl[24], acml[24], pl[8];
+  double xl[24], dl[24], ul[24], uml[24], acl[24], acml[24], pl[8], sl[8];
   double ke[9 * 64], ge[3 * 64], de[3 * 64], me[64], ru[24], rp[8];
   int ibcl[32];
   int nneg = 0;
@@ -536,6 +553,7 @@ int orc_assemble_ns(int nnode, int nelem, int nshl, int ngauss,
         acml[i * nshl + a] = acgmA[i * nnode + n];
       }
       pl[a] = pgA[n];
+      sl[a] = mus ? mus[n] : 1.0;
       for (int c = 0; c < 4; ++c) ibcl[c * nshl + a] = ibc[c * nnode + n];
     }
     memset(ke, 0, sizeof(double) * 9 * n2);
@@ -545,8 +563,9 @@ int orc_assemble_ns(int nnode, int nelem, int nshl, int ngauss,
     memset(ru, 0, sizeof(double) * 3 * nshl);
     memset(rp, 0, sizeof(double) * nshl);
     if (tgt) {
-      int r = orc_element_ns(nshl, ngauss, xl, dl, ul, uml, acl, acml, pl, prm,
-                             tensor_flag, ke, ge, de, me, ru, rp, &cfl);
+      int r = orc_element_ns(nshl, ngauss, xl, dl, ul, uml, acl, acml, pl,
+                             mus ? sl : NULL, prm, vms, tensor_flag, ke, ge, de, me, ru,
+                             rp, &cfl);
       if (r < 0) return r;
       nneg += r;
     } else {
@@ -559,8 +578,9 @@ int orc_assemble_ns(int nnode, int nelem, int nshl, int ngauss,
       memset(tmpm, 0, sizeof tmpm);
       memset(tmpu, 0, sizeof tmpu);
       memset(tmpp, 0, sizeof tmpp);
-      int r = orc_element_ns(nshl, ngauss, xl, dl, ul, uml, acl, acml, pl, prm,
-                             0, tmpk, tmpg, tmpd, tmpm, tmpu, tmpp, &cfl);
+      int r = orc_element_ns(nshl, ngauss, xl, dl, ul, uml, acl, acml, pl,
+                             mus ? sl : NULL, prm, vms, 0, tmpk, tmpg, tmpd, tmpm, tmpu,
+                             tmpp, &cfl);
       if (r < 0) return r;
       nneg += r;
     }

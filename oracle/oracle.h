@@ -57,7 +57,10 @@ int orc_sparse_loc(const int *row, int lo, int hi, int goal);
 
 /*
  * IntElmAss_NS_overset + BCLhs_3D + FillSparseMat_3D + RHS scatter.
- * prm = {almi, alfi, gami, beti, delt, rho, mu, kdc, gx, gy, gz}
+ * prm = {almi, alfi, gami, beti, delt, rho, mu, kdc, gx, gy, gz}; mus (N) a
+ * per-node viscosity scale (mu at a Gauss point = mu * sum_a N_a mus_a) or
+ * NULL for the reference's constant mu; vms 0 is the live call's residual
+ * (uprime = 0), 1 the residual with uprime = -tauM rLi (an extension)
  * Caller zeroes (or not) the outputs, exactly like assembleNavSto does.
  * cfl_out receives the maximum Gauss-point CFL number.
  */
@@ -66,17 +69,17 @@ int orc_assemble_ns(int nnode, int nelem, int nshl, int ngauss,
                     int subdomain_flag, int tensor_flag, const double *xg,
                     const int *ibc, int ibc_ncol, const double *dgA,
                     const double *ugA, const double *ugmA, const double *acgA,
-                    const double *acgmA, const double *pgA, const double *prm,
-                    const int *col, const int *row, double *k11, double *g,
-                    double *d1, double *m, double *rhsu, double *rhsp,
-                    double *cfl_out);
+                    const double *acgmA, const double *pgA, const double *mus,
+                    const double *prm, int vms, const int *col, const int *row,
+                    double *k11, double *g, double *d1, double *m, double *rhsu,
+                    double *rhsp, double *cfl_out);
 
 /* one element only, before BC (for kernel-level tests) */
 int orc_element_ns(int nshl, int ngauss, const double *xl, const double *dl,
                    const double *ul, const double *uml, const double *acl,
-                   const double *acml, const double *pl, const double *prm,
-                   int tensor_flag, double *ke, double *ge, double *de,
-                   double *me, double *ru, double *rp, double *cfl);
+                   const double *acml, const double *pl, const double *musl,
+                   const double *prm, int vms, int tensor_flag, double *ke, double *ge,
+                   double *de, double *me, double *ru, double *rp, double *cfl);
 
 /* BCLhs_3D on element locals; ibcl(NSHL,4) col-major */
 void orc_bclhs_3d(int nshl, const int *ibcl, double *ke, double *ge, double *de,

@@ -85,11 +85,13 @@ def lib():
         L.orc_sparsity.restype = ctypes.c_long
         L.orc_assemble_ns.argtypes = (
             [ctypes.c_int] * 4 + [_ip, _ip, _ip, ctypes.c_int, ctypes.c_int, _dp,
-                                  _ip, ctypes.c_int] + [_dp] * 7 +
+                                  _ip, ctypes.c_int] + [_dp] * 6 + [ctypes.c_void_p, _dp,
+                                                                   ctypes.c_int] +
             [_ip, _ip] + [_dp] * 6 + [_dp])
         L.orc_assemble_ns.restype = ctypes.c_int
-        L.orc_element_ns.argtypes = ([ctypes.c_int, ctypes.c_int] + [_dp] * 8 +
-                                     [ctypes.c_int] + [_dp] * 7)
+        L.orc_element_ns.argtypes = ([ctypes.c_int, ctypes.c_int] + [_dp] * 7 +
+                                     [ctypes.c_void_p, _dp, ctypes.c_int, ctypes.c_int] +
+                                     [_dp] * 7)
         L.orc_element_ns.restype = ctypes.c_int
         L.orc_bclhs_3d.argtypes = [ctypes.c_int, _ip] + [_dp] * 6
         L.orc_spmv_up.argtypes = [ctypes.c_int, _ip, _ip] + [_dp] * 8
@@ -192,10 +194,13 @@ def params_array(p):
 
 
 def assemble_ns(mesh, ibc, state, prm, col, row, tensor_flag=6,
-                subdomain_flag=1, elm_id_set=(1, 2), ngauss=None):
+                subdomain_flag=1, elm_id_set=(1, 2), ngauss=None, mus=None, vms=0):
     """IntElmAss_NS_overset on zeroed outputs.
 
     state: dict of alpha-stage node fields dg, ug, ugm, acg, acgm (N,3), pg (N,).
+    mus: optional per-node viscosity scale (N,) (an extension; None is the
+    reference's constant viscosity).  vms=1: the residual with the fine-scale
+    velocity uprime = -tauM rLi (an extension; 0 is the live call's).
     ibc: (N, 4) int.  Returns dict with k11 (icnt,9), g (icnt,3), d1 (icnt,3),
     m (icnt,), rhsu (N,3), rhsp (N,), cfl, nneg.
     """
@@ -216,7 +221,8 @@ def assemble_ns(mesh, ibc, state, prm, col, row, tensor_flag=6,
         _c(mesh.elm_id, np.int32), _c(elm_id_set, np.int32), subdomain_flag,
         tensor_flag, _soa(mesh.xg), ibcf, 4, _soa(state["dg"]),
         _soa(state["ug"]), _soa(state["ugm"]), _soa(state["acg"]),
-        _soa(state["acgm"]), _c(state["pg"]), _c(prm), _c(col, np.int32),
+        _soa(state["acgm"]), _c(state["pg"]),
+        None if mus is None else _c(mus).ctypes.data, _c(prm), int(vms), _c(col, np.int32),
         _c(row, np.int32), k11, g, d1, m, rhsu, rhsp, cfl)
     if nneg < 0:
         raise RuntimeError("oracle assembly failed")

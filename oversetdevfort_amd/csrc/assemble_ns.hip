@@ -94,6 +94,7 @@ __global__ void k_alpha_stage(int N, const double *__restrict__ xg,
 
 struct AsmParams {
   double almi, alfi, gami, delt, rho, mu, kdc, gx, gy, gz;
+  int vms;  // 1: the residual with the fine-scale velocity uprime = -tauM rLi
 };
 
 __global__ void __launch_bounds__(256)
@@ -101,13 +102,13 @@ k_assemble_ns_hex(int N, int E, const int32_t *__restrict__ ien,
                   const int32_t *__restrict__ elm_id,
                   const int32_t *__restrict__ elems, int first, int count,
                   const int32_t *__restrict__ emap,
-                  const double *__restrict__ rec,
+                  const double *__restrict__ rec, const double *__restrict__ mus,
                   const uint8_t *__restrict__ ibc, AsmParams P, int id1,
                   int id2, int subdomain_flag, int do_mat, int do_vec,
                   double *__restrict__ A, double *__restrict__ rhs,
                   double *__restrict__ cfl_out, int32_t *__restrict__ negdet) {
   __shared__ double s_sh[kWavesPerBlock][8][8][5];  // [q][a]: N, dN/dx(3), tmp1
-  __shared__ double s_q[kWavesPerBlock][8][4];      // [q]: tauM, tauP, tauC, DetJ*w
+  __shared__ double s_q[kWavesPerBlock][8][5];      // [q]: tauM, tauP, tauC, DetJ*w, mu
   __shared__ int32_t s_node[kWavesPerBlock][8];
 
   const int lane = threadIdx.x & 63;
@@ -213,9 +214,10 @@ k_assemble_ns_hex(int N, int E, const int32_t *__restrict__ ien,
     for (int j = 0; j < 3; ++j) du[i + 3 * j] = red8(ua[i] * ga[j]);
 #pragma unroll
   for (int j = 0; j < 3; ++j) dp[j] = red8(pa * ga[j]);
-  (void)dp;
 
-  const double rho = P.rho, mu = P.mu;
+  const double rho = P.rho;
+  // viscosity at q: mu, or mu times the interpolated per-node scale
+  const double mu = mus ? P.mu * red8(mus[na] * Na) : P.mu;
   double uadv[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) uadv[i] = ui[i] - umi[i];
@@ -256,29 +258,59 @@ k_assemble_ns_hex(int N, int E, const int32_t *__restrict__ ien,
     s_q[w][q][1] = tauP;
     s_q[w][q][2] = tauC;
     s_q[w][q][3] = dw;
+    s_q[w][q][4] = mu;
   }
 
-  // RHS (e3Rhs_3D_fluid with uprime = 0): contribution of Gauss point q to node a
+  // RHS (e3Rhs_3D_fluid; the live call's uprime = 0, or with P.vms the
+  // fine-scale velocity uprime = -tauM rLi, rLi from assembleNavSto.f90:
+  // 258-266 with the zero Hessian, tauBar = 0): contribution of Gauss point
+  // q to node a
   double rcu[3] = {0.0, 0.0, 0.0}, rcp = 0.0;
   if (vec) {
     const double mupkdc = mu + P.kdc;
     const double ptot = pri - tauC * divu;
     const double fi[3] = {P.gx * rho, P.gy * rho, P.gz * rho};
     double t1[3], t2[9];
+    if (P.vms) {
+      double up[3], adv[3];
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
-      t1[i] = rho * (aci[i] + uadv[0] * du[i + 0] + uadv[1] * du[i + 3] +
-                     uadv[2] * du[i + 6]) - fi[i];
+      for (int i = 0; i < 3; ++i) {
+        const double rLi = rho * (aci[i] + uadv[0] * du[i + 0] + uadv[1] * du[i + 3] +
+                                  uadv[2] * du[i + 6]) - fi[i] + dp[i];
+        up[i] = -tauM * rLi;
+        adv[i] = uadv[i] + up[i];
+      }
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+      for (int i = 0; i < 3; ++i)
+        t1[i] = rho * (aci[i] + adv[0] * du[i + 0] + adv[1] * du[i + 3] + adv[2] * du[i + 6]) -
+                fi[i];
 #pragma unroll
-      for (int j = 0; j < 3; ++j)
-        t2[i + 3 * j] = (i == j) ? (-ptot + 2.0 * mupkdc * du[i + 3 * i])
-                                 : mupkdc * (du[i + 3 * j] + du[j + 3 * i]);
+      for (int i = 0; i < 3; ++i)
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
-      rcu[i] = -(Na * t1[i] + (ga[0] * t2[i] + ga[1] * t2[i + 3] + ga[2] * t2[i + 6])) * dw;
-    rcp = -(Na * divu) * dw;
+        for (int j = 0; j < 3; ++j)
+          t2[i + 3 * j] = ((i == j) ? (-ptot + 2.0 * mupkdc * du[i + 3 * i])
+                                    : mupkdc * (du[i + 3 * j] + du[j + 3 * i])) -
+                          rho * adv[j] * up[i];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+        rcu[i] = -(Na * t1[i] + (ga[0] * t2[i] + ga[1] * t2[i + 3] + ga[2] * t2[i + 6])) * dw;
+      rcp = -(Na * divu - (ga[0] * up[0] + ga[1] * up[1] + ga[2] * up[2])) * dw;
+    } else {
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+        t1[i] = rho * (aci[i] + uadv[0] * du[i + 0] + uadv[1] * du[i + 3] +
+                       uadv[2] * du[i + 6]) - fi[i];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          t2[i + 3 * j] = (i == j) ? (-ptot + 2.0 * mupkdc * du[i + 3 * i])
+                                   : mupkdc * (du[i + 3 * j] + du[j + 3 * i]);
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+        rcu[i] = -(Na * t1[i] + (ga[0] * t2[i] + ga[1] * t2[i + 3] + ga[2] * t2[i + 6])) * dw;
+      rcp = -(Na * divu) * dw;
+    }
   }
   // reduce over the 8 Gauss points (lanes a, a+8, ..., a+56)
 #pragma unroll
@@ -302,9 +334,9 @@ k_assemble_ns_hex(int N, int E, const int32_t *__restrict__ ien,
   if (mat) {
     const double fact1 = P.almi;
     const double fact2 = P.alfi * P.gami * P.delt;
-    const double mupkdc = mu + P.kdc;
 #pragma unroll 2
     for (int qq = 0; qq < 8; ++qq) {
+      const double mupkdc = s_q[w][qq][4] + P.kdc;
       const double Nq = s_sh[w][qq][pa_][0];
       const double g0 = s_sh[w][qq][pa_][1], g1 = s_sh[w][qq][pa_][2],
                    g2 = s_sh[w][qq][pa_][3], t1a = s_sh[w][qq][pa_][4];
@@ -438,10 +470,12 @@ hipError_t launch_assemble_ns_hex(Ctx &c, int tensor_flag, int subdomain_flag,
   P.gx = c.prm[OFD_P_GX];
   P.gy = c.prm[OFD_P_GY];
   P.gz = c.prm[OFD_P_GZ];
+  P.vms = c.res_form;
   const int blocks = (count + kWavesPerBlock - 1) / kWavesPerBlock;
   hipLaunchKernelGGL(k_assemble_ns_hex, dim3(blocks), dim3(64 * kWavesPerBlock), 0,
                      c.stream, c.nnode, c.nelem, c.d_ien, c.d_elm_id,
-                     c.d_color_elems, first, count, c.d_emap, c.d_noderec, c.d_ibc,
+                     c.d_color_elems, first, count, c.d_emap, c.d_noderec,
+                     c.mus_on ? c.d_field[OFD_MUSCALE] : nullptr, c.d_ibc,
                      P, c.elm_id_set[0], c.elm_id_set[1], subdomain_flag,
                      (tensor_flag & OFD_ASSEMBLE_MAT) ? 1 : 0,
                      (tensor_flag & OFD_ASSEMBLE_VEC) ? 1 : 0, c.d_A64, c.d_rhs,

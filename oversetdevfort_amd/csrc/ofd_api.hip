@@ -70,7 +70,7 @@ static int d2h(Ctx &c, void *dst, const void *src, size_t bytes) {
 }
 
 static int field_len(const Ctx &c, int f) {
-  const bool vec3 = f <= OFD_ACGMOLD || f == OFD_SOLU;
+  const bool vec3 = f <= OFD_ACGMOLD || f == OFD_SOLU || f == OFD_LAMU;
   return vec3 ? 3 * c.nnode : c.nnode;
 }
 
@@ -422,6 +422,22 @@ int ofd_ctx_destroy(ofd_ctx *x) {
   dfree(c.d_W32);
   dfree(c.d_part);
   dfree(c.d_gs);
+  dfree(c.d_tmap);
+  dfree(c.t_A64);
+  dfree(c.t_A32);
+  dfree(c.t_P64);
+  dfree(c.t_Ap64);
+  dfree(c.t_Ap32);
+  dfree(c.d_jt);
+  dfree(c.d_lam);
+  dfree(c.d_lamr);
+  dfree(c.d_lamb);
+  dfree(c.d_lamd);
+  dfree(c.d_vjp);
+  dfree(c.d_fv);
+  dfree(c.d_fz);
+  dfree(c.d_fh);
+  c.fg_m = 0;
   dfree(c.d_AV);
   dfree(c.d_AV32);
   dfree(c.d_PV);
@@ -489,6 +505,20 @@ int ofd_upload_mesh(ofd_ctx *x, int nnode, int nelem, int nshl, const int32_t *i
   c.temT_n4 = 0;
   c.icnt = 0;
   c.mat_valid = c.prec_valid = c.a32_valid = c.ap32_valid = false;
+  ++c.mat_version;
+  c.mus_on = false;
+  // adjoint buffers are sized by the mesh and the pattern
+  dfree(c.d_jt);
+  dfree(c.d_lam);
+  dfree(c.d_lamr);
+  dfree(c.d_lamb);
+  dfree(c.d_lamd);
+  dfree(c.d_vjp);
+  dfree(c.d_fv);
+  dfree(c.d_fz);
+  dfree(c.d_fh);
+  c.fg_m = 0;
+  dfree(c.t_P64);
   int rc = dalloc(c, &c.d_ien, ne);
   if (!rc) rc = dalloc(c, &c.d_elm_id, (size_t)nelem);
   for (int f = 0; f < OFD_NFIELDS && !rc; ++f)
@@ -565,6 +595,18 @@ int ofd_build_pattern(ofd_ctx *x, int64_t *icnt_out) {
   for (int i = 0; i < N; ++i)
     for (int k = c.h_col[i]; k < c.h_col[i + 1]; ++k)
       if (c.h_row[k] == i) diag[i] = k;
+  // the mirrored nonzero (j, i) of each nonzero (i, j): the pattern is
+  // symmetric, so the transposed matrix keeps it
+  std::vector<int32_t> tmap(c.icnt);
+  for (int i = 0; i < N; ++i)
+    for (int k = c.h_col[i]; k < c.h_col[i + 1]; ++k) {
+      const int j = c.h_row[k];
+      const int32_t *lo = c.h_row.data() + c.h_col[j];
+      const int32_t *hi = c.h_row.data() + c.h_col[j + 1];
+      const int32_t *p = std::lower_bound(lo, hi, i);
+      OFD_CHECK(p != hi && *p == i, OFD_ERR_ARG, "sparsity pattern is not symmetric");
+      tmap[k] = (int32_t)(p - c.h_row.data());
+    }
   // scatter map: SparseMatLoc_3D's search done once per (e, a, b)
   std::vector<int32_t> emap((size_t)E * 64, 0);
   for (int e = 0; e < E; ++e)
@@ -604,19 +646,27 @@ int ofd_build_pattern(ofd_ctx *x, int64_t *icnt_out) {
   if (!rc) rc = dalloc(c, &c.d_emap, (size_t)E * 64);
   if (!rc) rc = dalloc(c, &c.d_color_elems, (size_t)E);
   if (!rc) rc = dalloc(c, &c.d_A64, (size_t)c.icnt * 16);
+  if (!rc) rc = dalloc(c, &c.d_tmap, (size_t)c.icnt);
   if (rc) return rc;
   dfree(c.d_A32);
   dfree(c.d_Ap64);
   dfree(c.d_Ap32);
+  dfree(c.t_A64);
+  dfree(c.t_A32);
+  dfree(c.t_Ap64);
+  dfree(c.t_Ap32);
+  c.t_mat_valid = c.t_prec_valid = c.t_a32_valid = c.t_ap32_valid = false;
   rc = h2d(c, c.d_col, c.h_col.data(), (N + 1) * sizeof(int32_t));
   if (!rc) rc = h2d(c, c.d_row, c.h_row.data(), c.icnt * sizeof(int32_t));
   if (!rc) rc = h2d(c, c.d_diag, diag.data(), N * sizeof(int32_t));
   if (!rc) rc = h2d(c, c.d_emap, emap.data(), emap.size() * sizeof(int32_t));
+  if (!rc) rc = h2d(c, c.d_tmap, tmap.data(), tmap.size() * sizeof(int32_t));
   if (!rc) rc = h2d(c, c.d_color_elems, order.data(), E * sizeof(int32_t));
   if (rc) return rc;
   OFD_HIP(hipMemsetAsync(c.d_A64, 0, (size_t)c.icnt * 16 * sizeof(double), c.stream));
   OFD_HIP(hipStreamSynchronize(c.stream));
   c.mat_valid = c.prec_valid = c.a32_valid = c.ap32_valid = false;
+  ++c.mat_version;
   if (icnt_out) *icnt_out = c.icnt;
   return OFD_OK;
 }
@@ -695,6 +745,7 @@ int ofd_set_field(ofd_ctx *x, int field, const double *host) {
   OFD_HIP(hipSetDevice(c.device));
   rc = h2d(c, c.d_field[field], host, (size_t)field_len(c, field) * sizeof(double));
   if (rc) return rc;
+  if (field == OFD_MUSCALE) c.mus_on = true;
   if (field == OFD_SOLU || field == OFD_SOLP) {
     c.sol_src = 0;
     OFD_HIP(launch_soa_to_node4(c, c.d_field[OFD_SOLU], c.d_field[OFD_SOLP], c.d_x));
@@ -730,6 +781,7 @@ int ofd_assemble_ns(ofd_ctx *x, int tensor_flag, int subdomain_flag, double *cfl
     OFD_HIP(launch_zero(c, c.d_A64, (size_t)c.icnt * 16));
     c.mat_valid = true;
     c.prec_valid = c.a32_valid = c.ap32_valid = false;
+    ++c.mat_version;
   }
   OFD_HIP(hipMemsetAsync(c.d_negdet, 0, sizeof(int32_t), c.stream));
   for (int col = 0; col + 1 < (int)c.color_off.size(); ++col)
@@ -824,6 +876,7 @@ int ofd_set_lhs(ofd_ctx *x, const double *k11, const double *g, const double *d1
   if (rc) return rc;
   c.mat_valid = true;
   c.prec_valid = c.a32_valid = c.ap32_valid = false;
+  ++c.mat_version;
   return OFD_OK;
 }
 
@@ -1102,6 +1155,7 @@ int ofd_time_assemble(ofd_ctx *x, int tensor_flag, int reps, double *ms) {
   if (tensor_flag & OFD_ASSEMBLE_MAT) {
     c.mat_valid = true;
     c.prec_valid = c.a32_valid = c.ap32_valid = false;
+    ++c.mat_version;
   }
   *ms = (double)t / reps;
   return OFD_OK;
@@ -1817,6 +1871,355 @@ int ofd_probe_read(ofd_ctx *x, int64_t *count, double *total_ms) {
   c.probe_count = 0;
   c.probe_ms = 0.0;
   return OFD_OK;
+}
+
+// ---- adjoint of the NS Newton step ----
+
+// adjoint vectors and scratch, allocated on first use
+static int ensure_adjoint(Ctx &c) {
+  int rc = need_pattern(c);
+  if (rc) return rc;
+  const size_t n4 = (size_t)c.nnode * 4;
+  double **bufs[5] = {&c.d_jt, &c.d_lam, &c.d_lamr, &c.d_lamb, &c.d_lamd};
+  for (double **b : bufs)
+    if (!*b) {
+      rc = dalloc(c, b, n4);
+      if (rc) return rc;
+      OFD_HIP(hipMemsetAsync(*b, 0, n4 * sizeof(double), c.stream));
+    }
+  if (!c.d_vjp) {
+    rc = dalloc(c, &c.d_vjp, (size_t)c.nnode * 8);
+    if (rc) return rc;
+  }
+  return OFD_OK;
+}
+
+// t_A64 = the assembled LHS transposed (blocks mirrored through d_tmap)
+static int ensure_transpose(Ctx &c) {
+  OFD_CHECK(c.mat_valid, OFD_ERR_ARG, "no assembled matrix");
+  if (c.t_mat_valid && c.t_version == c.mat_version) return OFD_OK;
+  int rc = OFD_OK;
+  if (!c.t_A64) rc = dalloc(c, &c.t_A64, (size_t)c.icnt * 16);
+  if (!rc && !c.t_P64) rc = dalloc(c, &c.t_P64, (size_t)c.nnode * 16);
+  if (rc) return rc;
+  OFD_HIP(launch_block_transpose(c));
+  c.t_mat_valid = true;
+  c.t_prec_valid = c.t_a32_valid = c.t_ap32_valid = false;
+  c.t_version = c.mat_version;
+  return OFD_OK;
+}
+
+// the stored-matrix kernels see LHS^T (and its own FP32, block-Jacobi and
+// preconditioned copies) while a view is alive
+struct TransposeView {
+  Ctx &c;
+  explicit TransposeView(Ctx &cc) : c(cc) { swap_all(); }
+  ~TransposeView() { swap_all(); }
+  void swap_all() {
+    std::swap(c.d_A64, c.t_A64);
+    std::swap(c.d_A32, c.t_A32);
+    std::swap(c.d_P64, c.t_P64);
+    std::swap(c.d_Ap64, c.t_Ap64);
+    std::swap(c.d_Ap32, c.t_Ap32);
+    std::swap(c.mat_valid, c.t_mat_valid);
+    std::swap(c.prec_valid, c.t_prec_valid);
+    std::swap(c.a32_valid, c.t_a32_valid);
+    std::swap(c.ap32_valid, c.t_ap32_valid);
+  }
+};
+
+static int put_node4(Ctx &c, const double *u, const double *p, double *dst) {
+  double *su = c.d_field[OFD_LAMU], *sp = c.d_field[OFD_LAMP];
+  int rc = h2d(c, su, u, (size_t)c.nnode * 3 * sizeof(double));
+  if (!rc) rc = h2d(c, sp, p, (size_t)c.nnode * sizeof(double));
+  if (rc) return rc;
+  OFD_HIP(launch_soa_to_node4(c, su, sp, dst));
+  return OFD_OK;
+}
+
+static int get_node4(Ctx &c, const double *src, double *u, double *p) {
+  double *su = c.d_field[OFD_LAMU], *sp = c.d_field[OFD_LAMP];
+  OFD_HIP(launch_node4_to_soa(c, src, su, sp));
+  int rc = d2h(c, u, su, (size_t)c.nnode * 3 * sizeof(double));
+  if (!rc) rc = d2h(c, p, sp, (size_t)c.nnode * sizeof(double));
+  return rc;
+}
+
+int ofd_spmv_up_t(ofd_ctx *x, int precision, const double *xu, const double *xp,
+                  double *yu, double *yp) {
+  OFD_CHECK(x && xu && xp && yu && yp, OFD_ERR_ARG, "null argument");
+  Ctx &c = x->c;
+  int rc = need_pattern(c);
+  if (rc) return rc;
+  OFD_HIP(hipSetDevice(c.device));
+  rc = ensure_transpose(c);
+  if (rc) return rc;
+  TransposeView view(c);
+  return ofd_spmv_up(x, precision, xu, xp, yu, yp);
+}
+
+int ofd_solve_t(ofd_ctx *x, int precision, int ortho, double rtol, double inner_tol,
+                int kspace, int max_cycles, const double *gu, const double *gp, int *iters,
+                int *cycles, double *relres) {
+  OFD_CHECK(x && gu && gp, OFD_ERR_ARG, "null argument");
+  OFD_CHECK(precision == OFD_FP64 || precision == OFD_FP32, OFD_ERR_ARG, "bad precision");
+  OFD_CHECK(max_cycles >= 1, OFD_ERR_ARG, "max_cycles >= 1");
+  Ctx &c = x->c;
+  int rc = ensure_adjoint(c);
+  if (rc) return rc;
+  OFD_HIP(hipSetDevice(c.device));
+  rc = ensure_transpose(c);
+  if (rc) return rc;
+  rc = put_node4(c, gu, gp, c.d_lamb);
+  if (rc) return rc;
+  {
+    TransposeView view(c);
+    rc = solve_core(c, precision, ortho, c.d_lamb, c.d_lam, c.d_lamr, rtol, inner_tol, kspace,
+                    max_cycles, iters, cycles, relres);
+  }
+  if (rc && rc != OFD_ERR_NOCONV) return rc;
+  OFD_HIP(launch_node4_to_soa(c, c.d_lam, c.d_field[OFD_LAMU], c.d_field[OFD_LAMP]));
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  return rc;
+}
+
+int ofd_vjp_ns(ofd_ctx *x, int subdomain_flag, const double *lu, const double *lp,
+               double *gu, double *gac, double *gp, double *gmus) {
+  OFD_CHECK(x && lu && lp, OFD_ERR_ARG, "null argument");
+  OFD_CHECK(subdomain_flag >= 1 && subdomain_flag <= 3, OFD_ERR_ARG, "subdomain_flag 1..3");
+  Ctx &c = x->c;
+  int rc = ensure_adjoint(c);
+  if (rc) return rc;
+  OFD_CHECK(c.nshl == 8, OFD_ERR_UNSUPPORTED,
+            "the GPU element kernels are implemented for trilinear hexes (NSHL=8)");
+  OFD_HIP(hipSetDevice(c.device));
+  rc = put_node4(c, lu, lp, c.d_lamb);
+  if (rc) return rc;
+  OFD_HIP(launch_alpha_stage(c));
+  OFD_HIP(launch_zero(c, c.d_vjp, (size_t)c.nnode * 8));
+  OFD_HIP(launch_vjp_ns(c, subdomain_flag, c.d_lamb));
+  const size_t N = c.nnode;
+  if (gu) rc = d2h(c, gu, c.d_vjp, 3 * N * sizeof(double));
+  if (!rc && gac) rc = d2h(c, gac, c.d_vjp + 3 * N, 3 * N * sizeof(double));
+  if (!rc && gp) rc = d2h(c, gp, c.d_vjp + 6 * N, N * sizeof(double));
+  if (!rc && gmus) rc = d2h(c, gmus, c.d_vjp + 7 * N, N * sizeof(double));
+  return rc;
+}
+
+int ofd_jt_apply(ofd_ctx *x, int subdomain_flag, const double *lu, const double *lp,
+                 double *yu, double *yp) {
+  OFD_CHECK(x && lu && lp && yu && yp, OFD_ERR_ARG, "null argument");
+  OFD_CHECK(subdomain_flag >= 1 && subdomain_flag <= 3, OFD_ERR_ARG, "subdomain_flag 1..3");
+  Ctx &c = x->c;
+  int rc = ensure_adjoint(c);
+  if (rc) return rc;
+  OFD_CHECK(c.nshl == 8, OFD_ERR_UNSUPPORTED,
+            "the GPU element kernels are implemented for trilinear hexes (NSHL=8)");
+  OFD_HIP(hipSetDevice(c.device));
+  rc = put_node4(c, lu, lp, c.d_lamb);
+  if (rc) return rc;
+  OFD_HIP(launch_alpha_stage(c));
+  OFD_HIP(launch_jt_ns(c, subdomain_flag, c.d_lamb, c.d_jt));
+  rc = get_node4(c, c.d_jt, yu, yp);
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  return rc;
+}
+
+// Restarted flexible GMRES (right preconditioning) on y = J_bc v (op 0) or
+// y = J_bc^T v (op 1), both applied exactly by the element kernels, with the
+// assembled LHS (op 0) or LHS^T (op 1) as the preconditioner, solved to
+// inner_tol by the restarted device GMRES in `precision`.  The outer basis
+// is FP64 with classical Gram-Schmidt applied twice; x accumulates.
+static int fgmres(Ctx &c, int op, int sub, int precision, int ortho, const double *b,
+                  double *x, double rtol, double inner_tol, int m, int max_cycles,
+                  int *iters, int *cycles, double *relres) {
+  OFD_CHECK(m >= 1 && m <= 100, OFD_ERR_ARG, "kspace out of range [1, 100]");
+  const size_t n = (size_t)c.nnode * 4;
+  if (c.fg_m < m) {
+    int rc = dalloc(c, &c.d_fv, n * (m + 1));
+    if (!rc) rc = dalloc(c, &c.d_fz, n * m);
+    if (!rc) rc = dalloc(c, &c.d_fh, (size_t)4 * (m + 2));
+    if (rc) return rc;
+    c.fg_m = m;
+  }
+  double *V = c.d_fv, *Z = c.d_fz, *w = c.d_jt, *h1 = c.d_fh, *h2 = c.d_fh + (m + 2),
+         *nr = c.d_fh + 2 * (m + 2), *yd = c.d_fh + 3 * (m + 2);
+  auto apply = [&](const double *v, double *y) -> hipError_t {
+    return op == 0 ? launch_jv_ns(c, sub, v, y) : launch_jt_ns(c, sub, v, y);
+  };
+  std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), e(m + 1), y(m), hbuf(2 * m + 3);
+  double bnorm = 0.0;
+  int rc = norm2_dev(c, b, &bnorm);
+  if (rc) return rc;
+  int total = 0, cyc = 0;
+  double rel = bnorm > 0.0 ? 1.0 : 0.0;
+  std::vector<double> hist;
+  while (bnorm > 0.0) {
+    // true residual r = b - Op x in V_0
+    OFD_HIP(apply(x, w));
+    OFD_HIP(launch_axpby(c, b, w, -1.0, V, n));
+    double beta = 0.0;
+    rc = norm2_dev(c, V, &beta);
+    if (rc) return rc;
+    rel = beta / bnorm;
+    hist.push_back(rel);
+    if (!std::isfinite(rel)) {
+      set_error("fgmres: non-finite residual");
+      return OFD_ERR_NONFINITE;
+    }
+    if (rel <= rtol || cyc >= max_cycles) break;
+    ++cyc;
+    OFD_HIP(launch_scale(c, n, V, 1.0 / beta, V));
+    std::fill(e.begin(), e.end(), 0.0);
+    e[0] = beta;
+    int k = 0;
+    for (; k < m; ++k) {
+      double *vk = V + n * k, *zk = Z + n * k;
+      int it = 0, cy = 0;
+      double ir = 0.0;
+      if (op == 1) {
+        TransposeView view(c);
+        rc = solve_core(c, precision, ortho, vk, zk, c.d_lamd, inner_tol,
+                        precision == OFD_FP32 ? 1e-4 : 1e-2, 30, 100, &it, &cy, &ir);
+      } else {
+        rc = solve_core(c, precision, ortho, vk, zk, c.d_lamd, inner_tol,
+                        precision == OFD_FP32 ? 1e-4 : 1e-2, 30, 100, &it, &cy, &ir);
+      }
+      if (rc && rc != OFD_ERR_NOCONV) return rc;
+      total += it;
+      OFD_HIP(apply(zk, w));
+      OFD_HIP(launch_multidot(c, n, w, V, n, k + 1, h1));
+      OFD_HIP(launch_multi_axpy(c, n, w, V, n, h1, k + 1, -1.0));
+      OFD_HIP(launch_multidot(c, n, w, V, n, k + 1, h2));
+      OFD_HIP(launch_multi_axpy(c, n, w, V, n, h2, k + 1, -1.0));
+      OFD_HIP(launch_multidot(c, n, w, w, 0, 1, nr));
+      OFD_HIP(hipMemcpyAsync(hbuf.data(), h1, sizeof(double) * (k + 1),
+                             hipMemcpyDeviceToHost, c.stream));
+      OFD_HIP(hipMemcpyAsync(hbuf.data() + m + 1, h2, sizeof(double) * (k + 1),
+                             hipMemcpyDeviceToHost, c.stream));
+      double hn = 0.0;
+      OFD_HIP(hipMemcpyAsync(&hn, nr, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+      OFD_HIP(hipStreamSynchronize(c.stream));
+      hn = sqrt(hn);
+      double *hc = H.data() + (size_t)(m + 1) * k;  // column k
+      for (int i = 0; i <= k; ++i) hc[i] = hbuf[i] + hbuf[m + 1 + i];
+      hc[k + 1] = hn;
+      if (hn > 0.0) OFD_HIP(launch_scale(c, n, w, 1.0 / hn, V + n * (k + 1)));
+      for (int i = 0; i < k; ++i) {
+        const double t = cs[i] * hc[i] + sn[i] * hc[i + 1];
+        hc[i + 1] = -sn[i] * hc[i] + cs[i] * hc[i + 1];
+        hc[i] = t;
+      }
+      const double r = sqrt(hc[k] * hc[k] + hc[k + 1] * hc[k + 1]);
+      cs[k] = hc[k] / r;
+      sn[k] = hc[k + 1] / r;
+      hc[k] = r;
+      hc[k + 1] = 0.0;
+      e[k + 1] = -sn[k] * e[k];
+      e[k] = cs[k] * e[k];
+      if (fabs(e[k + 1]) <= rtol * bnorm || hn == 0.0) {
+        ++k;
+        break;
+      }
+    }
+    // x += Z y, H y = e (upper triangular, k columns)
+    for (int j = k - 1; j >= 0; --j) {
+      double t = e[j];
+      for (int l = j + 1; l < k; ++l) t -= H[(size_t)(m + 1) * l + j] * y[l];
+      y[j] = t / H[(size_t)(m + 1) * j + j];
+    }
+    OFD_HIP(hipMemcpyAsync(yd, y.data(), sizeof(double) * k, hipMemcpyHostToDevice,
+                           c.stream));
+    OFD_HIP(launch_multi_axpy(c, n, x, Z, n, yd, k, 1.0));
+  }
+  c.solve_hist = hist;
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  if (iters) *iters = total;
+  if (cycles) *cycles = cyc;
+  if (relres) *relres = rel;
+  if (rel > rtol) {
+    set_error("fgmres: tolerance not reached within max_cycles");
+    return OFD_ERR_NOCONV;
+  }
+  return OFD_OK;
+}
+
+int ofd_solve_adjoint(ofd_ctx *x, int subdomain_flag, int precision, int ortho, double rtol,
+                      double inner_tol, int kspace, int max_cycles, const double *gu,
+                      const double *gp, int *iters, int *cycles, double *relres) {
+  OFD_CHECK(x && gu && gp, OFD_ERR_ARG, "null argument");
+  OFD_CHECK(subdomain_flag >= 1 && subdomain_flag <= 3, OFD_ERR_ARG, "subdomain_flag 1..3");
+  OFD_CHECK(precision == OFD_FP64 || precision == OFD_FP32, OFD_ERR_ARG, "bad precision");
+  OFD_CHECK(max_cycles >= 1, OFD_ERR_ARG, "max_cycles >= 1");
+  Ctx &c = x->c;
+  int rc = ensure_adjoint(c);
+  if (rc) return rc;
+  OFD_CHECK(c.nshl == 8, OFD_ERR_UNSUPPORTED,
+            "the GPU element kernels are implemented for trilinear hexes (NSHL=8)");
+  OFD_HIP(hipSetDevice(c.device));
+  rc = ensure_transpose(c);
+  if (rc) return rc;
+  rc = put_node4(c, gu, gp, c.d_lamb);
+  if (rc) return rc;
+  OFD_HIP(launch_alpha_stage(c));
+  OFD_HIP(launch_zero(c, c.d_lam, (size_t)c.nnode * 4));
+  rc = fgmres(c, 1, subdomain_flag, precision, ortho, c.d_lamb, c.d_lam, rtol, inner_tol,
+              kspace, max_cycles, iters, cycles, relres);
+  if (rc && rc != OFD_ERR_NOCONV) return rc;
+  OFD_HIP(launch_node4_to_soa(c, c.d_lam, c.d_field[OFD_LAMU], c.d_field[OFD_LAMP]));
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  return rc;
+}
+
+int ofd_solve_newton(ofd_ctx *x, int subdomain_flag, int precision, int ortho, double rtol,
+                     double inner_tol, int kspace, int max_cycles, int *iters, int *cycles,
+                     double *relres) {
+  OFD_CHECK(x, OFD_ERR_ARG, "null context");
+  OFD_CHECK(subdomain_flag >= 1 && subdomain_flag <= 3, OFD_ERR_ARG, "subdomain_flag 1..3");
+  OFD_CHECK(precision == OFD_FP64 || precision == OFD_FP32, OFD_ERR_ARG, "bad precision");
+  OFD_CHECK(max_cycles >= 1, OFD_ERR_ARG, "max_cycles >= 1");
+  Ctx &c = x->c;
+  int rc = ensure_adjoint(c);
+  if (rc) return rc;
+  OFD_CHECK(c.mat_valid, OFD_ERR_ARG, "no assembled matrix");
+  OFD_CHECK(c.nshl == 8, OFD_ERR_UNSUPPORTED,
+            "the GPU element kernels are implemented for trilinear hexes (NSHL=8)");
+  OFD_HIP(hipSetDevice(c.device));
+  OFD_HIP(launch_alpha_stage(c));
+  OFD_HIP(launch_zero(c, c.d_x, (size_t)c.nnode * 4));
+  rc = fgmres(c, 0, subdomain_flag, precision, ortho, c.d_rhs, c.d_x, rtol, inner_tol, kspace,
+              max_cycles, iters, cycles, relres);
+  if (rc && rc != OFD_ERR_NOCONV) return rc;
+  c.sol_src = 0;
+  OFD_HIP(launch_node4_to_soa(c, c.d_x, c.d_field[OFD_SOLU], c.d_field[OFD_SOLP]));
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  return rc;
+}
+
+int ofd_set_residual_form(ofd_ctx *x, int form) {
+  OFD_CHECK(x, OFD_ERR_ARG, "null context");
+  OFD_CHECK(form == 0 || form == 1, OFD_ERR_ARG, "residual form 0 or 1");
+  x->c.res_form = form;
+  return OFD_OK;
+}
+
+int ofd_j_apply(ofd_ctx *x, int subdomain_flag, const double *vu, const double *vp,
+                double *yu, double *yp) {
+  OFD_CHECK(x && vu && vp && yu && yp, OFD_ERR_ARG, "null argument");
+  OFD_CHECK(subdomain_flag >= 1 && subdomain_flag <= 3, OFD_ERR_ARG, "subdomain_flag 1..3");
+  Ctx &c = x->c;
+  int rc = ensure_adjoint(c);
+  if (rc) return rc;
+  OFD_CHECK(c.nshl == 8, OFD_ERR_UNSUPPORTED,
+            "the GPU element kernels are implemented for trilinear hexes (NSHL=8)");
+  OFD_HIP(hipSetDevice(c.device));
+  rc = put_node4(c, vu, vp, c.d_lamb);
+  if (rc) return rc;
+  OFD_HIP(launch_alpha_stage(c));
+  OFD_HIP(launch_jv_ns(c, subdomain_flag, c.d_lamb, c.d_jt));
+  rc = get_node4(c, c.d_jt, yu, yp);
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  return rc;
 }
 
 int ofd_ctx_bytes(ofd_ctx *x, int64_t *bytes) {

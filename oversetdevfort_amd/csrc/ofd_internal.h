@@ -160,6 +160,27 @@ struct Ctx {
   int kalloc_n = 0;                   // tuples per Krylov vector allocated
   // which increment SOLU / SOLP (/ SOLPHI) show: 0 the NS d_x, 2 the NSVOF d_xV
   int sol_src = 0;
+  // per-node viscosity scale (OFD_MUSCALE) in use by the NS assembly
+  bool mus_on = false;
+  // NS residual form: 0 the reference's (uprime = 0), 1 with the fine-scale
+  // velocity uprime = -tauM rLi
+  int res_form = 0;
+  // adjoint: the mirrored nonzero of each nonzero (the pattern is symmetric),
+  // the transposed LHS and its solver copies (swapped in for the stored-
+  // matrix kernels by a view), bumped / recorded versions of the LHS
+  int32_t *d_tmap = nullptr;          // [icnt]
+  double *t_A64 = nullptr, *t_P64 = nullptr, *t_Ap64 = nullptr;
+  float *t_A32 = nullptr, *t_Ap32 = nullptr;
+  bool t_mat_valid = false, t_prec_valid = false, t_a32_valid = false, t_ap32_valid = false;
+  int64_t mat_version = 0, t_version = -1;
+  double *d_jt = nullptr;             // [N][4] J_bc^T output, inner-solve residual
+  // [N][4] adjoint solution, residual, right-hand side and correction
+  double *d_lam = nullptr, *d_lamr = nullptr, *d_lamb = nullptr, *d_lamd = nullptr;
+  // flexible GMRES on J_bc or J_bc^T: basis V [(m+1)][N][4], preconditioned
+  // directions Z [m][N][4], small device vectors of coefficients
+  int fg_m = 0;
+  double *d_fv = nullptr, *d_fz = nullptr, *d_fh = nullptr;
+  double *d_vjp = nullptr;            // [8N] raw gradients u, ac, p, scale
   // overset fringe: per-element boxes, source flags and geometry; receivers
   int fr_alloc_e = 0, fr_alloc_r = 0;
   double *d_fr_box = nullptr;         // [E][6] widened bounding box
@@ -183,6 +204,24 @@ struct Ctx {
 hipError_t launch_alpha_stage(Ctx &c, bool rphi = false);
 hipError_t launch_assemble_ns_hex(Ctx &c, int tensor_flag, int subdomain_flag,
                                   int color);
+// adjoint (adjoint.hip)
+hipError_t launch_block_transpose(Ctx &c);  // t_A64 = blocks of A64 mirrored and transposed
+// gradients of lambda . F (lam [N][4]) into d_vjp (zeroed by the caller)
+hipError_t launch_vjp_ns(Ctx &c, int subdomain_flag, const double *lam);
+// y = J_bc^T lam
+hipError_t launch_jt_ns(Ctx &c, int subdomain_flag, const double *lam, double *y);
+// y = J_bc v, v a Newton increment
+hipError_t launch_jv_ns(Ctx &c, int subdomain_flag, const double *v, double *y);
+// out = x + a y
+hipError_t launch_axpby(Ctx &c, const double *x, const double *y, double a, double *out,
+                        size_t n);
+// out[j] = <w, V_j>, j < m <= 128 (deterministic; uses d_part)
+hipError_t launch_multidot(Ctx &c, size_t n, const double *w, const double *V, size_t vs,
+                           int m, double *out);
+// w += sign sum_j h_j V_j (h on the device)
+hipError_t launch_multi_axpy(Ctx &c, size_t n, double *w, const double *V, size_t vs,
+                             const double *h, int m, double sign);
+hipError_t launch_scale(Ctx &c, size_t n, const double *x, double a, double *y);
 hipError_t launch_zero(Ctx &c, double *p, size_t n);
 hipError_t launch_max_reduce(Ctx &c, const double *v, int n, double *out_dev);
 // matrix

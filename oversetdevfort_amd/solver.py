@@ -372,6 +372,103 @@ class Context:
         val_ = _soa(val)
         L.check(self.lib.ofd_apply_fringe(self.h, int(nodes_.size), _ptr(nodes_), _ptr(val_)))
 
+    # ---- adjoint of the NS Newton step (no reference code) ----
+    def SparseProdUP_3D_T(self, xu, xp, precision=L.OFD_FP64):
+        """y = LHS^T x for the assembled LHS."""
+        xu_, xp_ = _soa(xu), _f64(xp)
+        yu = np.zeros((3, self.nnode))
+        yp = np.zeros(self.nnode)
+        L.check(self.lib.ofd_spmv_up_t(self.h, precision, _ptr(xu_), _ptr(xp_), _ptr(yu),
+                                       _ptr(yp)))
+        return yu.T.copy(), yp
+
+    def _adj_solve(self, fn, args, gu, gp, raise_noconv):
+        gu_, gp_ = _soa(gu), _f64(gp)
+        it = ctypes.c_int()
+        cyc = ctypes.c_int()
+        rel = ctypes.c_double()
+        rc = fn(self.h, *args, _ptr(gu_), _ptr(gp_), ctypes.byref(it), ctypes.byref(cyc),
+                ctypes.byref(rel))
+        if rc != L.OFD_OK and (raise_noconv or rc != 5):
+            L.check(rc)
+        return dict(lamu=self.get_field("LAMU"), lamp=self.get_field("LAMP"),
+                    iters=it.value, cycles=cyc.value, relres=rel.value,
+                    converged=rc == L.OFD_OK)
+
+    def solve_t(self, gu, gp, rtol=1e-8, precision=L.OFD_FP64, ortho=L.OFD_ORTHO_CGS1,
+                inner_tol=None, kspace=30, max_cycles=100, raise_noconv=True):
+        """LHS^T lambda = g for the assembled LHS."""
+        if inner_tol is None:
+            inner_tol = 1e-4 if precision == L.OFD_FP32 else 1e-2
+        return self._adj_solve(self.lib.ofd_solve_t,
+                               (precision, ortho, float(rtol), float(inner_tol), int(kspace),
+                                int(max_cycles)), gu, gp, raise_noconv)
+
+    def vjp_ns(self, lu, lp, subdomain_flag=1):
+        """Gradients of lambda . F (the NS residual as assembled, before the
+        Dirichlet projection) with respect to the alpha-stage u, ac, p and
+        the per-node viscosity scale."""
+        lu_, lp_ = _soa(lu), _f64(lp)
+        N = self.nnode
+        gu = np.zeros((3, N))
+        gac = np.zeros((3, N))
+        gp = np.zeros(N)
+        gm = np.zeros(N)
+        L.check(self.lib.ofd_vjp_ns(self.h, int(subdomain_flag), _ptr(lu_), _ptr(lp_),
+                                    _ptr(gu), _ptr(gac), _ptr(gp), _ptr(gm)))
+        return dict(u=gu.T.copy(), ac=gac.T.copy(), p=gp, mus=gm)
+
+    def jt_apply(self, lu, lp, subdomain_flag=1):
+        """y = J_bc^T lambda, the exact Jacobian of the residual with respect
+        to the Newton increments with the Dirichlet projection, transposed."""
+        lu_, lp_ = _soa(lu), _f64(lp)
+        yu = np.zeros((3, self.nnode))
+        yp = np.zeros(self.nnode)
+        L.check(self.lib.ofd_jt_apply(self.h, int(subdomain_flag), _ptr(lu_), _ptr(lp_),
+                                      _ptr(yu), _ptr(yp)))
+        return yu.T.copy(), yp
+
+    def set_residual_form(self, form):
+        """0: the reference's NS residual (uprime = 0); 1: with the fine-scale
+        velocity uprime = -tauM rLi."""
+        L.check(self.lib.ofd_set_residual_form(self.h, int(form)))
+
+    def j_apply(self, vu, vp, subdomain_flag=1):
+        """y = J_bc v for a Newton increment v (forward tangent)."""
+        vu_, vp_ = _soa(vu), _f64(vp)
+        yu = np.zeros((3, self.nnode))
+        yp = np.zeros(self.nnode)
+        L.check(self.lib.ofd_j_apply(self.h, int(subdomain_flag), _ptr(vu_), _ptr(vp_),
+                                     _ptr(yu), _ptr(yp)))
+        return yu.T.copy(), yp
+
+    def solve_newton(self, rtol=1e-10, subdomain_flag=1, precision=L.OFD_FP64,
+                     ortho=L.OFD_ORTHO_CGS1, inner_tol=1e-3, kspace=30, max_cycles=20,
+                     raise_noconv=True):
+        """The consistent Newton correction J_bc d = RHS (flexible GMRES
+        preconditioned by the assembled LHS); apply it with newton_update."""
+        it = ctypes.c_int()
+        cyc = ctypes.c_int()
+        rel = ctypes.c_double()
+        rc = self.lib.ofd_solve_newton(self.h, int(subdomain_flag), precision, ortho,
+                                       float(rtol), float(inner_tol), int(kspace),
+                                       int(max_cycles), ctypes.byref(it), ctypes.byref(cyc),
+                                       ctypes.byref(rel))
+        if rc != L.OFD_OK and (raise_noconv or rc != 5):
+            L.check(rc)
+        return dict(iters=it.value, cycles=cyc.value, relres=rel.value,
+                    converged=rc == L.OFD_OK)
+
+    def solve_adjoint(self, gu, gp, rtol=1e-8, subdomain_flag=1, precision=L.OFD_FP64,
+                      ortho=L.OFD_ORTHO_CGS1, inner_tol=1e-3, kspace=30, max_cycles=20,
+                      raise_noconv=True):
+        """J_bc^T lambda = g: flexible GMRES on the exact operator (element
+        VJP), right-preconditioned by LHS^T solves."""
+        return self._adj_solve(self.lib.ofd_solve_adjoint,
+                               (int(subdomain_flag), precision, ortho, float(rtol),
+                                float(inner_tol), int(kspace), int(max_cycles)), gu, gp,
+                               raise_noconv)
+
     # ---- measurement ----
     def time_quench(self, what, arg, reps=5):
         """Mean device ms of one quenching-solver kernel (L.OFD_TIME_*; arg is

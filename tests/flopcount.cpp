@@ -110,8 +110,8 @@ int main() {
   for (int f = 0; f < 2; ++f) {
     char name[64];
     reset();
-    orc_element_ns(8, 8, xl, dl, ul, uml, acl, acml, pl, prm, flags[f], ke, ge, de, me, ru, rp,
-                   &cfl);
+    orc_element_ns(8, 8, xl, dl, ul, uml, acl, acml, pl, nullptr, prm, 0, flags[f], ke, ge, de,
+                   me, ru, rp, &cfl);
     std::snprintf(name, sizeof name, "ns_%s", fname[f]);
     report(name);
     reset();

@@ -1,0 +1,366 @@
+"""Adjoint of the NS Newton step on the GPU (no reference code).
+
+* LHS^T products and solves against the transposed sparse matrix (scipy).
+* The element VJP of the residual as implemented (the assembled RHS of
+  IntElmAss_NS_overset with uprime = 0) by central differences, eps = 1e-6, on
+  16 seeded directions, for the alpha-stage u, ac, p and the per-node
+  viscosity scale.
+* J_bc^T (the exact Jacobian with respect to the Newton increments, Dirichlet
+  projection of BCLhs_3D, transposed) by central differences along Newton
+  increments, and its identity part on the Dirichlet dofs.
+* The matrix-free adjoint solve J_bc^T lambda = g to a 1e-8 true residual, and
+  the "LHS consistency gap" |LHS^T lambda - J_bc^T lambda| / |J_bc^T lambda|,
+  which is expected to be nonzero (the LHS keeps SUPG/PSPG blocks and Picard
+  convection that the residual does not have).
+* J_bc v (the forward tangent) against central differences and against J_bc^T
+  through <lambda, J_bc v> = <J_bc^T lambda, v>.
+* With the fine-scale residual form (uprime = -tauM rLi; the reference's
+  form has no PSPG term and, for enclosed flows, no exact root), the
+  consistent Newton iteration (J_bc d = RHS by flexible GMRES preconditioned
+  with the assembled LHS) converges quadratically to machine precision.
+* The gradient of a Newton-converged objective with respect to the viscosity
+  scale and the lid amplitude against central differences of full re-solves.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oversetdevfort_amd import _lib as L
+from oversetdevfort_amd import configs as C
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def S():
+    from oversetdevfort_amd import solver
+    return solver
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def _perturbed_cavity(S, n=6, seed=1, form=0):
+    rng = np.random.default_rng(seed)
+    prob = C.cavity(n, seed=seed, jitter=0.05)
+    N = prob.mesh.nnode
+    f = prob.fields
+    f["ugold"] = 0.3 * rng.normal(size=(N, 3))
+    f["ug"] = f["ugold"] + 0.05 * rng.normal(size=(N, 3))
+    f["acgold"] = rng.normal(size=(N, 3))
+    f["acg"] = f["acgold"] + 0.2 * rng.normal(size=(N, 3))
+    f["pg"] = rng.normal(size=N)
+    ctx = S.setup(prob)
+    ctx.set_residual_form(form)
+    return ctx, prob, rng
+
+
+def _bsr(O, prob, A):
+    N = prob.mesh.nnode
+    col, row = O.sparsity(N, prob.mesh.ien)
+    Z = len(row)
+    blk = np.zeros((Z, 4, 4))
+    blk[:, :3, :3] = A["k11"].reshape(Z, 3, 3)
+    blk[:, :3, 3] = A["g"]
+    blk[:, 3, :3] = A["d1"]
+    blk[:, 3, 3] = A["m"]
+    return sp.bsr_matrix((blk, row, col), shape=(4 * N, 4 * N)).tocsr()
+
+
+def _vec(u, p):
+    return np.concatenate([u, p[:, None]], axis=1).ravel()
+
+
+def _split(v):
+    v = v.reshape(-1, 4)
+    return v[:, :3].copy(), v[:, 3].copy()
+
+
+def _free_mask(prob):
+    return (prob.ibc[:, :4] == 0).astype(np.float64)
+
+
+def test_transposed_spmv_and_solve(S, oracle_mod):
+    ctx, prob, rng = _perturbed_cavity(S, n=6, seed=2)
+    N = prob.mesh.nnode
+    ctx.assembleNavSto(6, 1)
+    T = _bsr(oracle_mod, prob, ctx.get_lhs()).T.tocsr()
+    xu, xp = rng.normal(size=(N, 3)), rng.normal(size=N)
+    want = T @ _vec(xu, xp)
+    for prec, tol in ((L.OFD_FP64, 1e-13), (L.OFD_FP32, 1e-6)):
+        yu, yp = ctx.SparseProdUP_3D_T(xu, xp, prec)
+        assert _rel(_vec(yu, yp), want) <= tol
+    g = rng.normal(size=(N, 4)) * _free_mask(prob)
+    for prec in (L.OFD_FP64, L.OFD_FP32):
+        res = ctx.solve_t(g[:, :3], g[:, 3], rtol=1e-8, precision=prec, max_cycles=300)
+        lam = _vec(res["lamu"], res["lamp"])
+        print(f"LHS^T solve precision {prec}: {res}", flush=True)
+        assert np.linalg.norm(T @ lam - g.ravel()) <= 1.01e-8 * np.linalg.norm(g)
+    # the forward system is untouched by the transposed solve
+    yu, yp = ctx.SparseProdUP_3D(xu, xp)
+    assert _rel(_vec(yu, yp), T.T @ _vec(xu, xp)) <= 1e-13
+    ctx.close()
+
+
+def test_fine_scale_residual_matches_oracle(S, oracle_mod):
+    """The residual with uprime = -tauM rLi on the GPU against the oracle's
+    restatement (with and without a per-node viscosity scale)."""
+    O = oracle_mod
+    ctx, prob, rng = _perturbed_cavity(S, n=5, seed=8, form=1)
+    m, p, f = prob.mesh, prob.params, prob.fields
+    N = m.nnode
+    col, row = O.sparsity(N, m.ien)
+    st = dict(dg=f["dgold"] + p["alfi"] * (f["dg"] - f["dgold"]),
+              ug=f["ugold"] + p["alfi"] * (f["ug"] - f["ugold"]),
+              ugm=f["ugmold"] + p["alfi"] * (f["ugm"] - f["ugmold"]),
+              acg=f["acgold"] + p["almi"] * (f["acg"] - f["acgold"]),
+              acgm=f["acgmold"] + p["almi"] * (f["acgm"] - f["acgmold"]), pg=f["pg"])
+    for mus in (None, 1.0 + 0.3 * rng.uniform(size=N)):
+        if mus is not None:
+            ctx.set_field("MUSCALE", mus)
+        ctx.assembleNavSto(6, 1)
+        ru, rp = ctx.get_rhs()
+        A = ctx.get_lhs()
+        R = O.assemble_ns(m, prob.ibc, st, O.params_array(p), col, row, 6, 1, mus=mus, vms=1)
+        R0 = O.assemble_ns(m, prob.ibc, st, O.params_array(p), col, row, 2, 1, mus=mus, vms=0)
+        assert _rel(ru, R["rhsu"]) <= 1e-12 and _rel(rp, R["rhsp"]) <= 1e-12
+        assert _rel(A["k11"], R["k11"]) <= 1e-12 and _rel(A["m"], R["m"]) <= 1e-12
+        # the fine-scale terms are present: the two forms differ
+        assert _rel(R["rhsp"], R0["rhsp"]) > 1e-3
+    ctx.close()
+
+
+def _rhs(ctx):
+    ctx.assembleNavSto(2, 1)
+    u, p = ctx.get_rhs()
+    return _vec(u, p)
+
+
+@pytest.mark.parametrize("form", [0, 1])
+def test_vjp_matches_central_differences(S, form):
+    ctx, prob, rng = _perturbed_cavity(S, n=5, seed=3, form=form)
+    N = prob.mesh.nnode
+    p = prob.params
+    f = prob.fields
+    mus = 1.0 + 0.3 * rng.uniform(size=N)
+    ctx.set_field("MUSCALE", mus)
+    lam = rng.normal(size=(N, 4)) * _free_mask(prob)
+    g = ctx.vjp_ns(lam[:, :3], lam[:, 3])
+    eps = 1e-6
+    fields = (("UG", f["ug"], p["alfi"] * g["u"].ravel(), (N, 3)),
+              ("ACG", f["acg"], p["almi"] * g["ac"].ravel(), (N, 3)),
+              ("PG", f["pg"], g["p"], (N,)),
+              ("MUSCALE", mus, g["mus"], (N,)))
+    for name, base, grad, shape in fields:
+        worst = 0.0
+        for d in range(16):
+            v = np.random.default_rng(100 + d).normal(size=shape)
+            ctx.set_field(name, base + eps * v)
+            fp = _rhs(ctx)
+            ctx.set_field(name, base - eps * v)
+            fm = _rhs(ctx)
+            fd = lam.ravel() @ (fp - fm) / (2 * eps)
+            ad = grad @ v.ravel()
+            err = abs(fd - ad) / max(abs(fd), abs(ad), 1e-300)
+            worst = max(worst, err)
+        ctx.set_field(name, base)
+        print(f"form {form}: VJP wrt {name}: worst relative FD error over 16 directions "
+              f"{worst:.2e}",
+              flush=True)
+        assert worst <= 1e-6, name
+    ctx.close()
+
+
+def _increment(ctx, f0, v, s, p):
+    """Newton increment s*v applied as the update does (acg += d, ug +=
+    gami Delt d, pg += alfi gami Delt dp)."""
+    ga, dt, af = p["gami"], p["delt"], p["alfi"]
+    ctx.set_field("ACG", f0["acg"] + s * v[:, :3])
+    ctx.set_field("UG", f0["ug"] + ga * dt * s * v[:, :3])
+    ctx.set_field("PG", f0["pg"] + af * ga * dt * s * v[:, 3])
+
+
+@pytest.mark.parametrize("form", [0, 1])
+def test_jt_apply_matches_central_differences(S, form):
+    ctx, prob, rng = _perturbed_cavity(S, n=5, seed=4, form=form)
+    N = prob.mesh.nnode
+    p = prob.params
+    f0 = {k: prob.fields[k] for k in ("ug", "acg", "pg")}
+    free = _free_mask(prob)
+    lam = rng.normal(size=(N, 4)) * free
+    yu, yp = ctx.jt_apply(lam[:, :3], lam[:, 3])
+    y = np.concatenate([yu, yp[:, None]], axis=1)
+    eps = 1e-6
+    worst = 0.0
+    for d in range(16):
+        v = np.random.default_rng(200 + d).normal(size=(N, 4)) * free
+        _increment(ctx, f0, v, eps, p)
+        fp = _rhs(ctx)
+        _increment(ctx, f0, v, -eps, p)
+        fm = _rhs(ctx)
+        jv = -(fp - fm) / (2 * eps)            # J = -dF/d(increment)
+        fd = lam.ravel() @ jv
+        ad = y.ravel() @ v.ravel()
+        worst = max(worst, abs(fd - ad) / max(abs(fd), abs(ad)))
+    print(f"form {form}: J_bc^T: worst relative FD error over 16 directions {worst:.2e}",
+          flush=True)
+    assert worst <= 1e-6
+    # Dirichlet part: identity once per element sharing the node, nothing
+    # reaches the free dofs
+    lam_d = rng.normal(size=(N, 4)) * (1.0 - free)
+    yu, yp = ctx.jt_apply(lam_d[:, :3], lam_d[:, 3])
+    y = np.concatenate([yu, yp[:, None]], axis=1)
+    count = np.bincount(prob.mesh.ien.ravel(), minlength=N).astype(np.float64)
+    assert np.allclose(y, count[:, None] * lam_d, rtol=1e-15, atol=0)
+    ctx.close()
+
+
+@pytest.mark.parametrize("form", [0, 1])
+def test_j_apply_tangent(S, form):
+    ctx, prob, rng = _perturbed_cavity(S, n=5, seed=7, form=form)
+    N = prob.mesh.nnode
+    p = prob.params
+    f0 = {k: prob.fields[k] for k in ("ug", "acg", "pg")}
+    free = _free_mask(prob)
+    lam = rng.normal(size=(N, 4)) * free
+    tu, tp = ctx.jt_apply(lam[:, :3], lam[:, 3])
+    lt = np.concatenate([tu, tp[:, None]], axis=1)
+    eps = 1e-6
+    worst_fd = worst_dot = 0.0
+    for d in range(16):
+        v = np.random.default_rng(400 + d).normal(size=(N, 4)) * free
+        _increment(ctx, f0, v, 0.0, p)
+        yu, yp = ctx.j_apply(v[:, :3], v[:, 3])
+        jv = np.concatenate([yu, yp[:, None]], axis=1)
+        _increment(ctx, f0, v, eps, p)
+        fp = _rhs(ctx)
+        _increment(ctx, f0, v, -eps, p)
+        fm = _rhs(ctx)
+        fd = -(fp - fm) / (2 * eps)
+        worst_fd = max(worst_fd, _rel(jv.ravel(), fd))
+        a, b = lam.ravel() @ jv.ravel(), lt.ravel() @ v.ravel()
+        worst_dot = max(worst_dot, abs(a - b) / max(abs(a), abs(b)))
+    print(f"form {form}: J_bc v: worst relative error vs central differences {worst_fd:.2e}, "
+          f"dot-product test vs J_bc^T {worst_dot:.2e}", flush=True)
+    assert worst_fd <= 1e-6
+    assert worst_dot <= 1e-12
+    ctx.close()
+
+
+def test_consistent_newton_converges(S):
+    """With the fine-scale residual form the exact Newton iteration reaches a
+    1e-13 relative residual in a few steps."""
+    prob = C.cavity(5, seed=6, jitter=0.05)
+    ctx = S.setup(prob)
+    ctx.set_residual_form(1)
+    ctx.predictor()
+    ctx.apply_dirichlet()
+    ctx.assembleNavSto(2, 1)
+    hist = [ctx.rhs_norms()[0]]
+    for _ in range(5):
+        ctx.assembleNavSto(6, 1)
+        ctx.solve_newton(rtol=1e-12, inner_tol=1e-3)
+        ctx.newton_update()
+        ctx.assembleNavSto(2, 1)
+        hist.append(ctx.rhs_norms()[0])
+    print("consistent Newton |Ru|:", ["%.1e" % h for h in hist], flush=True)
+    # quadratic: the second correction gains far more than the first
+    assert hist[2] <= 1e-6 * hist[1]
+    assert min(hist) <= 1e-12 * hist[0]
+    ctx.close()
+
+
+def test_adjoint_solve_and_consistency_gap(S, oracle_mod):
+    ctx, prob, rng = _perturbed_cavity(S, n=6, seed=5, form=1)
+    N = prob.mesh.nnode
+    ctx.assembleNavSto(6, 1)
+    # a velocity objective: g has no pressure part (J_bc has no pressure
+    # block, so g must be orthogonal to the pressure modes it leaves free)
+    g = rng.normal(size=(N, 4)) * _free_mask(prob)
+    g[:, 3] = 0.0
+    for prec in (L.OFD_FP64, L.OFD_FP32):
+        res = ctx.solve_adjoint(g[:, :3], g[:, 3], rtol=1e-8, precision=prec)
+        print(f"adjoint solve precision {prec}: {res['cycles']} restarts, "
+              f"{res['iters']} inner iterations, relres {res['relres']:.2e}", flush=True)
+        yu, yp = ctx.jt_apply(res["lamu"], res["lamp"])
+        r = _vec(yu, yp) - g.ravel()
+        assert np.linalg.norm(r) <= 1.01e-8 * np.linalg.norm(g)
+    # LHS consistency gap on the adjoint solution
+    lu, lp = ctx.SparseProdUP_3D_T(res["lamu"], res["lamp"])
+    gap = _rel(_vec(lu, lp), _vec(yu, yp))
+    print(f"LHS consistency gap |LHS^T lam - J_bc^T lam| / |J_bc^T lam| = {gap:.3e}",
+          flush=True)
+    assert gap > 1e-3
+    ctx.close()
+
+
+def _converge(ctx, prob, mus, amp, iters=12):
+    """One time step from the problem's old state: predictor, Dirichlet
+    values (lid scaled by amp), consistent Newton to a 1e-12 relative
+    residual."""
+    ctx.set_state(prob.fields)
+    ctx.set_field("MUSCALE", mus)
+    ctx.set_bcvals(amp * prob.bcval)
+    ctx.predictor()
+    ctx.apply_dirichlet()
+    ctx.assembleNavSto(2, 1)
+    r0 = ctx.rhs_norms()[0]
+    for _ in range(iters):
+        ctx.assembleNavSto(6, 1)
+        ctx.solve_newton(rtol=1e-12, inner_tol=1e-3)
+        ctx.newton_update()
+        ctx.assembleNavSto(2, 1)
+        ru, rp = ctx.rhs_norms()
+        if ru <= 1e-12 * r0:
+            break
+    assert ru <= 1e-12 * r0
+    return ctx.get_field("UG")
+
+
+def test_newton_objective_gradient(S):
+    """j = 1/2 sum over free velocity dofs of ug^2 after one converged time
+    step; dj/d(viscosity scale) and dj/d(lid amplitude) from one adjoint solve
+    against central differences of re-solved steps."""
+    prob = C.cavity(5, seed=6, jitter=0.05)
+    N = prob.mesh.nnode
+    p = prob.params
+    rng = np.random.default_rng(6)
+    prob.fields["ugold"] = 0.2 * rng.normal(size=(N, 3)) * (prob.ibc[:, :3] == 0)
+    ctx = S.setup(prob)
+    ctx.set_residual_form(1)
+    mus = 1.0 + 0.2 * rng.uniform(size=N)
+    free_u = (prob.ibc[:, :3] == 0).astype(np.float64)
+
+    def j_of(ug):
+        return 0.5 * np.sum((ug * free_u) ** 2)
+
+    ug = _converge(ctx, prob, mus, 1.0)
+    ctx.assembleNavSto(6, 1)              # the LHS at the converged state
+    gd = np.zeros((N, 4))
+    gd[:, :3] = p["gami"] * p["delt"] * ug * free_u   # dj/d(increment)
+    res = ctx.solve_adjoint(gd[:, :3], gd[:, 3], rtol=1e-12)
+    print(f"adjoint: {res['cycles']} restarts, relres {res['relres']:.2e}", flush=True)
+    g = ctx.vjp_ns(res["lamu"], res["lamp"])
+    d_mus = g["mus"]
+    prof = prob.bcval * (prob.ibc[:, :3] == 1)
+    d_amp = np.sum((p["alfi"] * g["u"] + p["almi"] / (p["gami"] * p["delt"]) * g["ac"]) *
+                   prof)
+    eps = 1e-6
+    worst = 0.0
+    for d in range(16):
+        v = np.random.default_rng(300 + d).normal(size=N)
+        jp = j_of(_converge(ctx, prob, mus + eps * v, 1.0))
+        jm = j_of(_converge(ctx, prob, mus - eps * v, 1.0))
+        fd = (jp - jm) / (2 * eps)
+        ad = d_mus @ v
+        worst = max(worst, abs(fd - ad) / max(abs(fd), abs(ad)))
+    print(f"dj/d(viscosity scale): worst relative FD error over 16 directions {worst:.2e}",
+          flush=True)
+    assert worst <= 1e-5
+    jp = j_of(_converge(ctx, prob, mus, 1.0 + eps))
+    jm = j_of(_converge(ctx, prob, mus, 1.0 - eps))
+    fd = (jp - jm) / (2 * eps)
+    print(f"dj/d(lid amplitude): adjoint {d_amp:.10e} FD {fd:.10e}", flush=True)
+    assert abs(fd - d_amp) <= 1e-5 * abs(fd)
+    ctx.close()
