@@ -59,13 +59,17 @@ enum {
  * increment SOLT) follow modules.f90:291-300.  OFD_LAMU/OFD_LAMP hold the
  * last adjoint solution; OFD_MUSCALE, once set, scales the viscosity of the
  * NS assembly per node (mu at a Gauss point = mu * sum_a N_a s_a), the
- * parameter of the viscosity gradient (no reference counterpart). */
+ * parameter of the viscosity gradient (no reference counterpart).
+ * OFD_KSCALE does the same for the conductivity of the temperature
+ * assembly (fluid and solid elements) and OFD_LAMT holds the last
+ * temperature adjoint solution. */
 enum {
   OFD_XG = 0, OFD_DG, OFD_DGOLD, OFD_UG, OFD_UGOLD, OFD_UGM, OFD_UGMOLD,
   OFD_ACG, OFD_ACGOLD, OFD_ACGM, OFD_ACGMOLD, OFD_PG, OFD_PGOLD,
   OFD_SOLU, OFD_SOLP,
   OFD_PHIG, OFD_PHIGOLD, OFD_RPHIG, OFD_RPHIGOLD, OFD_TG, OFD_TGOLD,
   OFD_RTG, OFD_RTGOLD, OFD_SOLT, OFD_SOLPHI, OFD_LAMU, OFD_LAMP, OFD_MUSCALE,
+  OFD_KSCALE, OFD_LAMT,
   OFD_NFIELDS
 };
 
@@ -350,9 +354,9 @@ int ofd_j_apply(ofd_ctx *ctx, int subdomain_flag, const double *vu, const double
  * solved to inner_tol with the restarted device GMRES in `precision` (FP64
  * or mixed); the outer residual and basis are FP64.  iters counts the inner
  * GMRES iterations, cycles the restarts.  lambda goes to OFD_LAMU /
- * OFD_LAMP.  Needs an assembled LHS.  J_bc has no pressure block, so g must
- * not load the pressure modes it leaves free (velocity objectives have
- * g_p = 0). */
+ * OFD_LAMP.  Needs an assembled LHS.  With the reference's residual form
+ * J_bc has no pressure-pressure block, so g must not load the pressure modes
+ * it leaves free (velocity objectives have g_p = 0). */
 int ofd_solve_adjoint(ofd_ctx *ctx, int subdomain_flag, int precision, int ortho, double rtol,
                       double inner_tol, int kspace, int max_cycles, const double *gu,
                       const double *gp, int *iters, int *cycles, double *relres);
@@ -366,6 +370,44 @@ int ofd_solve_adjoint(ofd_ctx *ctx, int subdomain_flag, int precision, int ortho
 int ofd_solve_newton(ofd_ctx *ctx, int subdomain_flag, int precision, int ortho, double rtol,
                      double inner_tol, int kspace, int max_cycles, int *iters, int *cycles,
                      double *relres);
+
+/* ---- adjoint of the quenching temperature equation (no reference code) ----
+ *
+ * The residual is the assembled RHSGTem of IntElmAss_Tem_Quenching_STAB
+ * (solveQuenching.f90:579-882), R(T, rT, u; s) at the alpha stage, before
+ * the Dirichlet rows of IBC column 6 are cleared; s is OFD_KSCALE (constant
+ * 1 when unset) and the VOF field and mesh velocity are held.  The Newton
+ * increments (rTg += d, Tg += gami*Delt*d, solveFlow.f90:139-140) give
+ * J_T = -dR/dd; J_T,bc carries the BCLhs_tem projection (BCLhs_3D.f90:
+ * 247-274).  J_T equals the assembled LHSTem when TEM_KDC = 0 and
+ * C_DMDOT = 1: the LHS leaves out the derivative of the discontinuity-
+ * capturing conductivity and scales that of the phase-change source by
+ * C_DMDOT.  Vectors are (NNODE). */
+
+/* y = LHSTem^T x. */
+int ofd_spmv_tem_t(ofd_ctx *ctx, int precision, const double *x, double *y);
+
+/* Restarted GMRES on LHSTem^T lambda = g (FP64 outer residual, as
+ * ofd_solve_tem); lambda goes to OFD_LAMT. */
+int ofd_solve_tem_t(ofd_ctx *ctx, int precision, int ortho, double rtol, double inner_tol,
+                    int kspace, int max_cycles, const double *g, int *iters, int *cycles,
+                    double *relres);
+
+/* Element VJP of the temperature residual: gradients of lambda . R with
+ * respect to the alpha-stage T, rT and u (N,3) and to OFD_KSCALE, before any
+ * Dirichlet projection.  Any output may be NULL. */
+int ofd_vjp_tem(ofd_ctx *ctx, const double *lam, double *gT, double *grT, double *gu,
+                double *gks);
+
+/* y = J_T,bc^T lambda, matrix-free. */
+int ofd_jt_apply_tem(ofd_ctx *ctx, const double *lam, double *y);
+
+/* J_T,bc^T lambda = g: restarted flexible GMRES on the exact operator,
+ * right-preconditioned by LHSTem^T solved to inner_tol; arguments as
+ * ofd_solve_adjoint.  lambda goes to OFD_LAMT.  Needs an assembled LHSTem. */
+int ofd_solve_tem_adjoint(ofd_ctx *ctx, int precision, int ortho, double rtol,
+                          double inner_tol, int kspace, int max_cycles, const double *g,
+                          int *iters, int *cycles, double *relres);
 
 /* Mean device time (HIP events on the context stream) of `reps` back-to-back
  * launches of one quenching-solver kernel: what = OFD_TIME_ASM_NSVOF or

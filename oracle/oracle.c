@@ -970,9 +970,9 @@ static double orc_tau_tem(const double *G, const double *uadv, double dt, double
  * te(NSHL,NSHL) column-major te[b*nshl+a], rt(NSHL) accumulate */
 int orc_element_tem(int nshl, int ngauss, const double *xl, const double *dl,
                     const double *ul, const double *uml, const double *phil,
-                    const double *Tl, const double *rTl, const double *prm,
-                    const double *qp, int is_fluid, int tensor_flag, double *te,
-                    double *rt) {
+                    const double *Tl, const double *rTl, const double *ksl,
+                    const double *prm, const double *qp, int is_fluid, int tensor_flag,
+                    double *te, double *rt) {
   const double almi = prm[0], alfi = prm[1], gami = prm[2], delt = prm[4];
   const double rhoa = qp[0], rhow = qp[1], cpa = qp[4], cpw = qp[5];
   const double kappaa = qp[6], kappaw = qp[7], rhos = qp[8], cps = qp[10];
@@ -1028,6 +1028,11 @@ int orc_element_tem(int nshl, int ngauss, const double *xl, const double *dl,
       cpi = cps;
       hki = kappas;
       rhocpi = rhos * cps;
+    }
+    if (ksl) { /* per-node conductivity scale (an extension; NULL = 1) */
+      double s = 0.0;
+      for (int a = 0; a < nshl; ++a) s = s + ksl[a] * shlu[a];
+      hki = hki * s;
     }
     double uadv[3];
     for (int i = 0; i < 3; ++i) uadv[i] = ui[i] - umi[i];
@@ -1093,10 +1098,11 @@ int orc_assemble_tem(int nnode, int nelem, int nshl, int ngauss, const int *ien,
                      const int *elm_id, int tensor_flag, const double *xg,
                      const int *ibc_t, const double *dgA, const double *ugA,
                      const double *ugmA, const double *phigA, const double *TgA,
-                     const double *rTgA, const double *prm, const double *qp,
-                     const int *col, const int *row, double *lhs, double *rhs) {
+                     const double *rTgA, const double *ksA, const double *prm,
+                     const double *qp, const int *col, const int *row, double *lhs,
+                     double *rhs) {
   const int fluid_id = (int)qp[19];
-  double xl[24], dl[24], ul[24], uml[24], phil[8], Tl[8], rTl[8], te[64], rt[8];
+  double xl[24], dl[24], ul[24], uml[24], phil[8], Tl[8], rTl[8], ksl[8], te[64], rt[8];
   int nneg = 0;
   for (int e = 0; e < nelem; ++e) {
     for (int a = 0; a < nshl; ++a) {
@@ -1110,10 +1116,12 @@ int orc_assemble_tem(int nnode, int nelem, int nshl, int ngauss, const int *ien,
       phil[a] = phigA[n];
       Tl[a] = TgA[n];
       rTl[a] = rTgA[n];
+      ksl[a] = ksA ? ksA[n] : 1.0;
     }
     memset(te, 0, sizeof te);
     memset(rt, 0, sizeof rt);
-    const int r = orc_element_tem(nshl, ngauss, xl, dl, ul, uml, phil, Tl, rTl, prm, qp,
+    const int r = orc_element_tem(nshl, ngauss, xl, dl, ul, uml, phil, Tl, rTl,
+                                  ksA ? ksl : NULL, prm, qp,
                                   elm_id[e] == fluid_id, tensor_flag, te, rt);
     if (r < 0) return r;
     nneg += r;

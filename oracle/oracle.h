@@ -130,19 +130,23 @@ int orc_hex_local(const double X[8][3], const double p[3], double xi[3]);
  * prm = the NS parameter vector (almi, alfi, gami, beti, delt, ...);
  * qp = {rhoa, rhow, mua, muw, cpa, cpw, kappaa, kappaw, rhos, mus, cps,
  *       kappas, Ts, lh, c_cond, c_evap, c_dmdot, tem_kdc, tref, fluid_id};
- * ibc_t(NNODE) = IBC(:,6).  Returns the number of negative Jacobians.
+ * ibc_t(NNODE) = IBC(:,6).  ksA (NULL: none) scales the conductivity per
+ * node, kappa at a Gauss point times sum_a N_a ks_a (an extension of the
+ * reference, for the conductivity gradient).  Returns the number of
+ * negative Jacobians.
  */
 int orc_assemble_tem(int nnode, int nelem, int nshl, int ngauss, const int *ien,
                      const int *elm_id, int tensor_flag, const double *xg,
                      const int *ibc_t, const double *dgA, const double *ugA,
                      const double *ugmA, const double *phigA, const double *TgA,
-                     const double *rTgA, const double *prm, const double *qp,
-                     const int *col, const int *row, double *lhs, double *rhs);
+                     const double *rTgA, const double *ksA, const double *prm,
+                     const double *qp, const int *col, const int *row, double *lhs,
+                     double *rhs);
 int orc_element_tem(int nshl, int ngauss, const double *xl, const double *dl,
                     const double *ul, const double *uml, const double *phil,
-                    const double *Tl, const double *rTl, const double *prm,
-                    const double *qp, int is_fluid, int tensor_flag, double *te,
-                    double *rt);
+                    const double *Tl, const double *rTl, const double *ksl,
+                    const double *prm, const double *qp, int is_fluid, int tensor_flag,
+                    double *te, double *rt);
 /* SparseProd_LS: y = A x, scalar CSR */
 void orc_spmv_s(int nnode, const int *col, const int *row, const double *A, const double *x,
                 double *y);

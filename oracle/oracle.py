@@ -102,7 +102,8 @@ def lib():
         L.orc_interp_tet.argtypes = ([ctypes.c_int] * 3 + [_ip, _ip, ctypes.c_int] +
                                      [_dp] * 3 + [ctypes.c_int, _dp, _dp, _ip, _dp])
         L.orc_assemble_tem.argtypes = ([ctypes.c_int] * 4 + [_ip, _ip, ctypes.c_int, _dp,
-                                        _ip] + [_dp] * 8 + [_ip, _ip, _dp, _dp])
+                                        _ip] + [_dp] * 6 + [ctypes.c_void_p] + [_dp] * 2 +
+                                       [_ip, _ip, _dp, _dp])
         L.orc_assemble_tem.restype = ctypes.c_int
         L.orc_spmv_s.argtypes = [ctypes.c_int, _ip, _ip, _dp, _dp, _dp]
         L.orc_gmres_tem.argtypes = ([ctypes.c_int, _ip, _ip, _dp, _dp, _dp, ctypes.c_double,
@@ -216,13 +217,14 @@ def assemble_ns(mesh, ibc, state, prm, col, row, tensor_flag=6,
     rhsp = np.zeros(N)
     cfl = np.zeros(1)
     ibcf = _c(np.asarray(ibc)[:, :4].T, np.int32).ravel()
+    mus_ = None if mus is None else _c(mus)    # kept alive across the call
     nneg = lib().orc_assemble_ns(
         N, E, nshl, ngauss, _c(mesh.ien.T, np.int32).ravel(),
         _c(mesh.elm_id, np.int32), _c(elm_id_set, np.int32), subdomain_flag,
         tensor_flag, _soa(mesh.xg), ibcf, 4, _soa(state["dg"]),
         _soa(state["ug"]), _soa(state["ugm"]), _soa(state["acg"]),
         _soa(state["acgm"]), _c(state["pg"]),
-        None if mus is None else _c(mus).ctypes.data, _c(prm), int(vms), _c(col, np.int32),
+        None if mus_ is None else mus_.ctypes.data, _c(prm), int(vms), _c(col, np.int32),
         _c(row, np.int32), k11, g, d1, m, rhsu, rhsp, cfl)
     if nneg < 0:
         raise RuntimeError("oracle assembly failed")
@@ -268,11 +270,13 @@ def _qvec(qprm):
     return q
 
 
-def assemble_tem(mesh, ibc_t, st, prm, qprm, col, row, tensor_flag, lhs=None, rhs=None):
+def assemble_tem(mesh, ibc_t, st, prm, qprm, col, row, tensor_flag, lhs=None, rhs=None,
+                 ks=None):
     """IntElmAss_Tem_Quenching_STAB + BCLhs_tem + FillSparseMat_tem.  st holds
     the alpha-stage fields dg, ug, ugm (N,3) and phig, Tg, rTg (N); prm the
     NS parameter vector (almi, alfi, gami, [beti], delt, ...) or the 4-vector
-    (almi, alfi, gami, delt); qprm the 19 material values (+ fluid id)."""
+    (almi, alfi, gami, delt); qprm the 19 material values (+ fluid id); ks
+    an optional per-node conductivity scale (an extension)."""
     N, E = mesh.nnode, mesh.nelem
     p = np.asarray(prm, np.float64)
     if p.size == 4:
@@ -280,12 +284,14 @@ def assemble_tem(mesh, ibc_t, st, prm, qprm, col, row, tensor_flag, lhs=None, rh
     q = _qvec(qprm)
     lhs = np.zeros(len(row)) if lhs is None else lhs
     rhs = np.zeros(N) if rhs is None else rhs
+    ks_ = None if ks is None else _c(ks)
     nneg = lib().orc_assemble_tem(
         N, E, mesh.nshl, mesh.nshl, _c(mesh.ien.T, np.int32).ravel(),
         _c(mesh.elm_id, np.int32), int(tensor_flag), _soa(mesh.xg),
         _c(ibc_t, np.int32), _soa(st["dg"]), _soa(st["ug"]), _soa(st["ugm"]),
-        _c(st["phig"]), _c(st["Tg"]), _c(st["rTg"]), _c(p), _c(q), _c(col, np.int32),
-        _c(row, np.int32), lhs, rhs)
+        _c(st["phig"]), _c(st["Tg"]), _c(st["rTg"]),
+        None if ks_ is None else ks_.ctypes.data_as(ctypes.c_void_p), _c(p), _c(q),
+        _c(col, np.int32), _c(row, np.int32), lhs, rhs)
     return dict(lhs=lhs, rhs=rhs, nneg=nneg)
 
 

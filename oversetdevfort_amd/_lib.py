@@ -29,11 +29,11 @@ OFD_TIME_SPMV_TEM = 3
 FIELDS = ["XG", "DG", "DGOLD", "UG", "UGOLD", "UGM", "UGMOLD", "ACG", "ACGOLD",
           "ACGM", "ACGMOLD", "PG", "PGOLD", "SOLU", "SOLP",
           "PHIG", "PHIGOLD", "RPHIG", "RPHIGOLD", "TG", "TGOLD", "RTG", "RTGOLD", "SOLT",
-          "SOLPHI", "LAMU", "LAMP", "MUSCALE"]
+          "SOLPHI", "LAMU", "LAMP", "MUSCALE", "KSCALE", "LAMT"]
 FIELD = {n: i for i, n in enumerate(FIELDS)}
 SCALAR_FIELDS = {"PG", "PGOLD", "SOLP", "PHIG", "PHIGOLD", "RPHIG", "RPHIGOLD", "TG",
                  "TGOLD", "RTG", "RTGOLD", "SOLT", "SOLPHI", "LAMP",
-                 "MUSCALE"}
+                 "MUSCALE", "KSCALE", "LAMT"}
 PARAMS = ["almi", "alfi", "gami", "beti", "delt", "rho", "mu", "kdc", "gx", "gy",
           "gz"]
 # material vector of ofd_set_quench_params
@@ -110,6 +110,13 @@ SIGNATURES = {
                           _ip, _dp], _c_int),
     "ofd_solve_adjoint": ([_vp, _c_int, _c_int, _c_int, _c_dbl, _c_dbl, _c_int, _c_int, _vp,
                            _vp, _ip, _ip, _dp], _c_int),
+    "ofd_spmv_tem_t": ([_vp, _c_int, _vp, _vp], _c_int),
+    "ofd_solve_tem_t": ([_vp, _c_int, _c_int, _c_dbl, _c_dbl, _c_int, _c_int, _vp, _ip, _ip,
+                         _dp], _c_int),
+    "ofd_vjp_tem": ([_vp] + [_vp] * 5, _c_int),
+    "ofd_jt_apply_tem": ([_vp, _vp, _vp], _c_int),
+    "ofd_solve_tem_adjoint": ([_vp, _c_int, _c_int, _c_dbl, _c_dbl, _c_int, _c_int, _vp, _ip,
+                               _ip, _dp], _c_int),
     "ofd_fringe_interp": ([_vp, ctypes.c_int32, _c_int, _vp, _vp, _vp, _vp], _c_int),
     "ofd_apply_fringe": ([_vp, _c_int, _vp, _vp], _c_int),
     "ofd_time_spmv": ([_vp, _c_int, _c_int, _dp], _c_int),

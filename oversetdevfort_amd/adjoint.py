@@ -1,7 +1,7 @@
 """torch.autograd bindings of the hand-written adjoint kernels (the reference
 has no adjoint).
 
-Two functions sit behind ``torch.autograd.Function``:
+Three functions sit behind ``torch.autograd.Function``:
 
 * ``NSResidual``: the Navier-Stokes residual R(ug, acg, pg, muscale) as
   assembled by IntElmAss_NS_overset (assembleNavSto.f90:76-336) at the
@@ -15,11 +15,19 @@ Two functions sit behind ``torch.autograd.Function``:
   one adjoint solve J^T lam = s on the converged state and one element VJP,
   by the implicit function theorem; the step's explicit dependence on the old
   state follows the predictor/update formulas (Newmark in the increments).
+* ``TemStep``: one time step of the quenching temperature equation with the
+  flow held, (Tgold, rTgold, kscale, u) -> (Tg, rTg): predictor, then Newton
+  iterations on IntElmAss_Tem_Quenching_STAB (solveQuenching.f90:579-882)
+  with the assembled LHSTem (solveFlow.f90:139-140 updates) until the
+  residual has dropped by ``rtol``.  Backward: one adjoint solve on the exact
+  Jacobian J_T^T (flexible GMRES preconditioned by LHSTem^T) and one element
+  VJP.
 
-``run_steps`` chains ``NavStoStep`` over time and, with ``checkpoint_every``
-= k, keeps only every k-th state for the reverse sweep: each segment of k
-steps is re-solved during backward (torch.utils.checkpoint).  The kernels
-are deterministic, so a re-solved segment repeats the forward bit for bit.
+``run_steps`` / ``run_tem_steps`` chain the steps over time and, with
+``checkpoint_every`` = k, keep only every k-th state for the reverse sweep:
+each segment of k steps is re-solved during backward
+(torch.utils.checkpoint).  The kernels are deterministic, so a re-solved
+segment repeats the forward bit for bit.
 
 Every array crosses the C ABI through host memory; tensors may live on any
 device and gradients come back on the device of the input.  The solver
@@ -35,7 +43,7 @@ from torch.utils.checkpoint import checkpoint
 from . import _lib as L
 
 __all__ = ["NSStepper", "NSResidual", "NavStoStep", "ns_residual", "nav_sto_step",
-           "run_steps"]
+           "run_steps", "TemStepper", "TemStep", "tem_step", "run_tem_steps"]
 
 
 def _np(t):
@@ -231,5 +239,141 @@ def run_steps(st, ugold, acgold, pgold, mus, bcval, nsteps, checkpoint_every=0):
             state = checkpoint(_segment, st, n, *state, mus, bcval, use_reentrant=False)
         else:
             state = _segment(st, n, *state, mus, bcval)
+        done += n
+    return state
+
+
+class TemStepper:
+    """Solver settings of the quenching temperature equation on a
+    ``solver.Context`` holding the mesh, pattern, parameters and quench
+    material.  The flow is held: the VOF field ``phi`` (N,) is the same at the
+    old and new level, the velocity is an input of each step.  ``ibc`` (N, 6+)
+    Dirichlet flags, column 6 the temperature (the node keeps its old value).
+    With TEM_KDC = 0 and C_DMDOT = 1 the assembled LHSTem is the exact
+    Jacobian and the Newton iteration converges quadratically."""
+
+    def __init__(self, ctx, ibc, params, phi, rtol=1e-12, max_newton=30, lin_rtol=1e-13,
+                 adj_rtol=1e-12, precision=L.OFD_FP64, inner_tol=1e-3, kspace=30,
+                 max_cycles=20):
+        self.ctx = ctx
+        self.free = (np.asarray(ibc)[:, 5] != 1).astype(np.float64)
+        self.p = dict(params)
+        self.phi = np.asarray(phi, np.float64)
+        self.rtol, self.max_newton, self.lin_rtol = rtol, max_newton, lin_rtol
+        self.adj_rtol, self.precision = adj_rtol, precision
+        self.inner_tol, self.kspace, self.max_cycles = inner_tol, kspace, max_cycles
+        self.newton_history = []
+
+    def _flow(self, u, ks):
+        c = self.ctx
+        c.set_field("UG", u)
+        c.set_field("UGOLD", u)
+        c.set_field("PHIG", self.phi)
+        c.set_field("PHIGOLD", self.phi)
+        c.set_field("KSCALE", ks)
+
+    def solve(self, Told, rTold, ks, u):
+        """One converged step from the old temperature; returns (Tg, rTg)."""
+        c = self.ctx
+        self._flow(u, ks)
+        c.set_field("TGOLD", Told)
+        c.set_field("RTGOLD", rTold)
+        c.predictor()
+        c.assemble_tem(2)
+        r0 = r = c.tem_rhs_norm()
+        hist = [r0]
+        it = 0
+        while r > self.rtol * r0:
+            if it == self.max_newton:
+                raise RuntimeError(f"temperature Newton did not converge: residual {hist}")
+            c.assemble_tem(6)
+            c.solve_tem(rtol=self.lin_rtol, precision=self.precision, kspace=self.kspace)
+            c.tem_update()
+            c.assemble_tem(2)
+            r = c.tem_rhs_norm()
+            hist.append(r)
+            it += 1
+        self.newton_history.append(hist)
+        return c.get_field("TG"), c.get_field("RTG")
+
+    def load(self, Told, rTold, ks, u, T, rT):
+        c = self.ctx
+        self._flow(u, ks)
+        for name, v in (("TGOLD", Told), ("RTGOLD", rTold), ("TG", T), ("RTG", rT)):
+            c.set_field(name, v)
+
+    def step_vjp(self, gT, grT):
+        """Reverse of ``solve`` on the loaded converged state: gradients with
+        respect to (Tgold, rTgold, kscale, u) of <gT, Tg> + <grT, rTg>."""
+        c, p = self.ctx, self.p
+        ga, dt, af, am = p["gami"], p["delt"], p["alfi"], p["almi"]
+        fr = self.free
+        di = 1.0 - fr
+        # seeds on the Newton increments (rTg += d, Tg += gami Delt d)
+        s = (grT + ga * dt * gT) * fr
+        if np.any(s):
+            c.assemble_tem(6)                 # LHSTem preconditions J_T^T
+            res = c.solve_tem_adjoint(s, rtol=self.adj_rtol, precision=self.precision,
+                                      inner_tol=self.inner_tol, kspace=self.kspace,
+                                      max_cycles=self.max_cycles)
+            lam = res["lam"] * fr
+        else:
+            lam = np.zeros_like(s)
+        g = c.vjp_tem(lam)
+        GT, GrT = g["T"], g["rT"]
+        # alpha stage: free  T_a = Told + alfi Delt ((1-gami) rTold + gami rTg),
+        #                    rT_a = (1-almi) rTold + almi rTg;
+        #              Dirichlet  T = Told, rTg = (gami-1)/gami rTold
+        d_Told = gT + GT
+        d_rTold = (fr * ((1.0 - ga) * dt * gT + af * (1.0 - ga) * dt * GT + (1.0 - am) * GrT)
+                   + di * ((ga - 1.0) / ga * grT + (1.0 - am / ga) * GrT))
+        return d_Told, d_rTold, g["ks"], g["u"]
+
+
+class TemStep(torch.autograd.Function):
+    """(Tgold, rTgold, kscale, u) -> (Tg, rTg) of one converged temperature
+    step with the flow held."""
+
+    @staticmethod
+    def forward(fc, st, Told, rTold, ks, u):
+        T, rT = st.solve(_np(Told), _np(rTold), _np(ks), _np(u))
+        T, rT = _t(T, Told), _t(rT, Told)
+        fc.st = st
+        fc.save_for_backward(Told, rTold, ks, u, T, rT)
+        return T, rT
+
+    @staticmethod
+    def backward(fc, gT, grT):
+        st = fc.st
+        saved = fc.saved_tensors
+        st.load(*(_np(v) for v in saved))
+        z = np.zeros(st.free.shape)
+        grads = st.step_vjp(_np(gT) if gT is not None else z,
+                            _np(grT) if grT is not None else z)
+        like = saved[0]
+        return (None,) + tuple(_t(g, like) for g in grads)
+
+
+def tem_step(st, Told, rTold, ks, u):
+    return TemStep.apply(st, Told, rTold, ks, u)
+
+
+def _tem_segment(st, n, T, rT, ks, u):
+    for _ in range(n):
+        T, rT = TemStep.apply(st, T, rT, ks, u)
+    return T, rT
+
+
+def run_tem_steps(st, Told, rTold, ks, u, nsteps, checkpoint_every=0):
+    """``nsteps`` converged temperature steps, checkpointed as ``run_steps``."""
+    state = (Told, rTold)
+    done = 0
+    k = int(checkpoint_every)
+    while done < nsteps:
+        n = min(k, nsteps - done) if k > 0 else nsteps - done
+        if k > 0 and torch.is_grad_enabled():
+            state = checkpoint(_tem_segment, st, n, *state, ks, u, use_reentrant=False)
+        else:
+            state = _tem_segment(st, n, *state, ks, u)
         done += n
     return state

@@ -54,7 +54,8 @@ __global__ void __launch_bounds__(256)
 k_assemble_tem_hex(int E, const int32_t *__restrict__ ien, const int32_t *__restrict__ elm_id,
                    const int32_t *__restrict__ elems, int first, int count,
                    const int32_t *__restrict__ emap, const double *__restrict__ rec,
-                   const uint8_t *__restrict__ ibc, TemParams P, int do_mat, int do_vec,
+                   const double *__restrict__ ks, const uint8_t *__restrict__ ibc,
+                   TemParams P, int do_mat, int do_vec,
                    double *__restrict__ A, double *__restrict__ rhs,
                    int32_t *__restrict__ negdet) {
   __shared__ double s_sh[kWaves][8][8][5];  // [q][a]: N, dN/dx (3), shconv
@@ -163,6 +164,8 @@ k_assemble_tem_hex(int E, const int32_t *__restrict__ ien, const int32_t *__rest
     hki = P.kappas;
     rhocpi = P.rhos * P.cps;
   }
+  // per-node conductivity scale (an extension; none: 1)
+  if (ks) hki = hki * sum8(ks[na] * Na);
   const double Se = ((P.cpw - P.cpa) * (Ti - P.Ts) - P.lh) * mdot;
   const double udT = uadv[0] * dT[0] + uadv[1] * dT[1] + uadv[2] * dT[2];
   // the second-derivative term of the residual is zero for these elements
@@ -291,7 +294,8 @@ hipError_t launch_assemble_tem_hex(Ctx &c, int tensor_flag, int color) {
   const int blocks = (count + kWaves - 1) / kWaves;
   hipLaunchKernelGGL(k_assemble_tem_hex, dim3(blocks), dim3(64 * kWaves), 0, c.stream,
                      c.nelem, c.d_ien, c.d_elm_id, c.d_color_elems, first, count, c.d_emap,
-                     c.d_noderec, c.d_ibc, P, (tensor_flag & OFD_ASSEMBLE_MAT) ? 1 : 0,
+                     c.d_noderec, c.ks_on ? c.d_field[OFD_KSCALE] : nullptr, c.d_ibc, P,
+                     (tensor_flag & OFD_ASSEMBLE_MAT) ? 1 : 0,
                      (tensor_flag & OFD_ASSEMBLE_VEC) ? 1 : 0, c.d_AT, c.d_rhsT, c.d_negdet);
   return hipGetLastError();
 }

@@ -84,6 +84,26 @@ static int need_pattern(const Ctx &c) {
   return OFD_OK;
 }
 
+// LHSTem transposed and its solver copies (sized by the pattern)
+static void free_tem_transpose(Ctx &c) {
+  dfree(c.tT_A);
+  dfree(c.tT_A32);
+  dfree(c.tT_P);
+  dfree(c.tT_Ap);
+  dfree(c.tT_Ap32);
+  c.tT_mat_valid = c.tT_prec_valid = c.tT_a32_valid = c.tT_ap32_valid = false;
+}
+
+// temperature adjoint vectors (sized by the mesh)
+static void free_tem_adjoint(Ctx &c) {
+  dfree(c.d_lamT);
+  dfree(c.d_lamTb);
+  dfree(c.d_lamTd);
+  dfree(c.d_jtT);
+  dfree(c.d_vjpT);
+  c.tadj_n4 = 0;
+}
+
 // Krylov workspace for restart length K
 static int ensure_krylov(Ctx &c, int K) {
   OFD_CHECK(K >= 1 && K <= kMaxKspace, OFD_ERR_ARG, "kspace out of range [1, 64]");
@@ -438,6 +458,9 @@ int ofd_ctx_destroy(ofd_ctx *x) {
   dfree(c.d_fz);
   dfree(c.d_fh);
   c.fg_m = 0;
+  c.fg_n = 0;
+  free_tem_transpose(c);
+  free_tem_adjoint(c);
   dfree(c.d_AV);
   dfree(c.d_AV32);
   dfree(c.d_PV);
@@ -506,7 +529,11 @@ int ofd_upload_mesh(ofd_ctx *x, int nnode, int nelem, int nshl, const int32_t *i
   c.icnt = 0;
   c.mat_valid = c.prec_valid = c.a32_valid = c.ap32_valid = false;
   ++c.mat_version;
+  ++c.matT_version;
   c.mus_on = false;
+  c.ks_on = false;
+  free_tem_transpose(c);
+  free_tem_adjoint(c);
   // adjoint buffers are sized by the mesh and the pattern
   dfree(c.d_jt);
   dfree(c.d_lam);
@@ -518,6 +545,7 @@ int ofd_upload_mesh(ofd_ctx *x, int nnode, int nelem, int nshl, const int32_t *i
   dfree(c.d_fz);
   dfree(c.d_fh);
   c.fg_m = 0;
+  c.fg_n = 0;
   dfree(c.t_P64);
   int rc = dalloc(c, &c.d_ien, ne);
   if (!rc) rc = dalloc(c, &c.d_elm_id, (size_t)nelem);
@@ -656,6 +684,7 @@ int ofd_build_pattern(ofd_ctx *x, int64_t *icnt_out) {
   dfree(c.t_Ap64);
   dfree(c.t_Ap32);
   c.t_mat_valid = c.t_prec_valid = c.t_a32_valid = c.t_ap32_valid = false;
+  free_tem_transpose(c);
   rc = h2d(c, c.d_col, c.h_col.data(), (N + 1) * sizeof(int32_t));
   if (!rc) rc = h2d(c, c.d_row, c.h_row.data(), c.icnt * sizeof(int32_t));
   if (!rc) rc = h2d(c, c.d_diag, diag.data(), N * sizeof(int32_t));
@@ -746,6 +775,7 @@ int ofd_set_field(ofd_ctx *x, int field, const double *host) {
   rc = h2d(c, c.d_field[field], host, (size_t)field_len(c, field) * sizeof(double));
   if (rc) return rc;
   if (field == OFD_MUSCALE) c.mus_on = true;
+  if (field == OFD_KSCALE) c.ks_on = true;
   if (field == OFD_SOLU || field == OFD_SOLP) {
     c.sol_src = 0;
     OFD_HIP(launch_soa_to_node4(c, c.d_field[OFD_SOLU], c.d_field[OFD_SOLP], c.d_x));
@@ -1227,6 +1257,7 @@ int ofd_time_quench(ofd_ctx *x, int what, int arg, int reps, double *ms) {
     } else {
       c.matT_valid = true;
       c.precT_valid = c.aT32_valid = c.apT32_valid = false;
+      ++c.matT_version;
     }
   }
   *ms = (double)t / reps;
@@ -1249,6 +1280,8 @@ static int ensure_tem(Ctx &c) {
     OFD_HIP(hipMemsetAsync(c.d_AT, 0, (size_t)c.icnt * sizeof(double), c.stream));
     c.temT_icnt = c.icnt;
     c.matT_valid = c.precT_valid = c.aT32_valid = c.apT32_valid = false;
+    free_tem_transpose(c);
+    ++c.matT_version;
   }
   if (c.temT_n4 != n4) {
     const size_t np = (size_t)n4 * 4;
@@ -1306,6 +1339,7 @@ int ofd_assemble_tem(ofd_ctx *x, int tensor_flag) {
     OFD_HIP(launch_zero(c, c.d_AT, (size_t)c.icnt));
     c.matT_valid = true;
     c.precT_valid = c.aT32_valid = c.apT32_valid = false;
+    ++c.matT_version;
   }
   OFD_HIP(hipMemsetAsync(c.d_negdet, 0, sizeof(int32_t), c.stream));
   for (int col = 0; col + 1 < (int)c.color_off.size(); ++col)
@@ -1344,6 +1378,7 @@ int ofd_set_tem_system(ofd_ctx *x, const double *lhs, const double *rhs) {
     if (rc) return rc;
     c.matT_valid = true;
     c.precT_valid = c.aT32_valid = c.apT32_valid = false;
+    ++c.matT_version;
   }
   if (rhs) rc = h2d(c, c.d_rhsT, rhs, (size_t)c.nnode * sizeof(double));
   return rc;
@@ -2025,28 +2060,29 @@ int ofd_jt_apply(ofd_ctx *x, int subdomain_flag, const double *lu, const double 
   return rc;
 }
 
-// Restarted flexible GMRES (right preconditioning) on y = J_bc v (op 0) or
-// y = J_bc^T v (op 1), both applied exactly by the element kernels, with the
-// assembled LHS (op 0) or LHS^T (op 1) as the preconditioner, solved to
-// inner_tol by the restarted device GMRES in `precision`.  The outer basis
-// is FP64 with classical Gram-Schmidt applied twice; x accumulates.
-static int fgmres(Ctx &c, int op, int sub, int precision, int ortho, const double *b,
-                  double *x, double rtol, double inner_tol, int m, int max_cycles,
-                  int *iters, int *cycles, double *relres) {
+// Restarted flexible GMRES (right preconditioning) on vectors of n doubles
+// (the current system's gm_n 4-tuples): apply(v, y) is the operator,
+// prec(v, z, &inner_iters) an inner solve z ~ M^-1 v.  The outer basis is
+// FP64 with classical Gram-Schmidt applied twice; x accumulates; w is an
+// n-vector of scratch.
+extern "C++" {
+template <class Apply, class Prec>
+static int fgmres(Ctx &c, size_t n, Apply &&apply, Prec &&prec, const double *b, double *x,
+                  double *w, double rtol, int m, int max_cycles, int *iters, int *cycles,
+                  double *relres) {
   OFD_CHECK(m >= 1 && m <= 100, OFD_ERR_ARG, "kspace out of range [1, 100]");
-  const size_t n = (size_t)c.nnode * 4;
-  if (c.fg_m < m) {
-    int rc = dalloc(c, &c.d_fv, n * (m + 1));
-    if (!rc) rc = dalloc(c, &c.d_fz, n * m);
-    if (!rc) rc = dalloc(c, &c.d_fh, (size_t)4 * (m + 2));
+  if (c.fg_m < m || c.fg_n < n) {
+    const int mm = std::max(c.fg_m, m);
+    const size_t nn = std::max(c.fg_n, n);
+    int rc = dalloc(c, &c.d_fv, nn * (mm + 1));
+    if (!rc) rc = dalloc(c, &c.d_fz, nn * mm);
+    if (!rc) rc = dalloc(c, &c.d_fh, (size_t)4 * (mm + 2));
     if (rc) return rc;
-    c.fg_m = m;
+    c.fg_m = mm;
+    c.fg_n = nn;
   }
-  double *V = c.d_fv, *Z = c.d_fz, *w = c.d_jt, *h1 = c.d_fh, *h2 = c.d_fh + (m + 2),
+  double *V = c.d_fv, *Z = c.d_fz, *h1 = c.d_fh, *h2 = c.d_fh + (m + 2),
          *nr = c.d_fh + 2 * (m + 2), *yd = c.d_fh + 3 * (m + 2);
-  auto apply = [&](const double *v, double *y) -> hipError_t {
-    return op == 0 ? launch_jv_ns(c, sub, v, y) : launch_jt_ns(c, sub, v, y);
-  };
   std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), e(m + 1), y(m), hbuf(2 * m + 3);
   double bnorm = 0.0;
   int rc = norm2_dev(c, b, &bnorm);
@@ -2075,16 +2111,8 @@ static int fgmres(Ctx &c, int op, int sub, int precision, int ortho, const doubl
     int k = 0;
     for (; k < m; ++k) {
       double *vk = V + n * k, *zk = Z + n * k;
-      int it = 0, cy = 0;
-      double ir = 0.0;
-      if (op == 1) {
-        TransposeView view(c);
-        rc = solve_core(c, precision, ortho, vk, zk, c.d_lamd, inner_tol,
-                        precision == OFD_FP32 ? 1e-4 : 1e-2, 30, 100, &it, &cy, &ir);
-      } else {
-        rc = solve_core(c, precision, ortho, vk, zk, c.d_lamd, inner_tol,
-                        precision == OFD_FP32 ? 1e-4 : 1e-2, 30, 100, &it, &cy, &ir);
-      }
+      int it = 0;
+      rc = prec(vk, zk, &it);
       if (rc && rc != OFD_ERR_NOCONV) return rc;
       total += it;
       OFD_HIP(apply(zk, w));
@@ -2143,6 +2171,32 @@ static int fgmres(Ctx &c, int op, int sub, int precision, int ortho, const doubl
   }
   return OFD_OK;
 }
+}  // extern "C++"
+
+// flexible GMRES on y = J_bc v (op 0) or y = J_bc^T v (op 1), both applied
+// exactly by the element kernels, preconditioned by the assembled LHS (op 0)
+// or LHS^T (op 1) solved to inner_tol by the restarted device GMRES
+static int fgmres_ns(Ctx &c, int op, int sub, int precision, int ortho, const double *b,
+                     double *x, double rtol, double inner_tol, int m, int max_cycles,
+                     int *iters, int *cycles, double *relres) {
+  auto apply = [&](const double *v, double *y) -> hipError_t {
+    return op == 0 ? launch_jv_ns(c, sub, v, y) : launch_jt_ns(c, sub, v, y);
+  };
+  auto prec = [&](const double *v, double *z, int *it) -> int {
+    int cy = 0;
+    double ir = 0.0;
+    const double ctol = precision == OFD_FP32 ? 1e-4 : 1e-2;
+    if (op == 1) {
+      TransposeView view(c);
+      return solve_core(c, precision, ortho, v, z, c.d_lamd, inner_tol, ctol, 30, 100, it,
+                        &cy, &ir);
+    }
+    return solve_core(c, precision, ortho, v, z, c.d_lamd, inner_tol, ctol, 30, 100, it, &cy,
+                      &ir);
+  };
+  return fgmres(c, (size_t)c.nnode * 4, apply, prec, b, x, c.d_jt, rtol, m, max_cycles, iters,
+                cycles, relres);
+}
 
 int ofd_solve_adjoint(ofd_ctx *x, int subdomain_flag, int precision, int ortho, double rtol,
                       double inner_tol, int kspace, int max_cycles, const double *gu,
@@ -2163,7 +2217,7 @@ int ofd_solve_adjoint(ofd_ctx *x, int subdomain_flag, int precision, int ortho, 
   if (rc) return rc;
   OFD_HIP(launch_alpha_stage(c));
   OFD_HIP(launch_zero(c, c.d_lam, (size_t)c.nnode * 4));
-  rc = fgmres(c, 1, subdomain_flag, precision, ortho, c.d_lamb, c.d_lam, rtol, inner_tol,
+  rc = fgmres_ns(c, 1, subdomain_flag, precision, ortho, c.d_lamb, c.d_lam, rtol, inner_tol,
               kspace, max_cycles, iters, cycles, relres);
   if (rc && rc != OFD_ERR_NOCONV) return rc;
   OFD_HIP(launch_node4_to_soa(c, c.d_lam, c.d_field[OFD_LAMU], c.d_field[OFD_LAMP]));
@@ -2187,11 +2241,186 @@ int ofd_solve_newton(ofd_ctx *x, int subdomain_flag, int precision, int ortho, d
   OFD_HIP(hipSetDevice(c.device));
   OFD_HIP(launch_alpha_stage(c));
   OFD_HIP(launch_zero(c, c.d_x, (size_t)c.nnode * 4));
-  rc = fgmres(c, 0, subdomain_flag, precision, ortho, c.d_rhs, c.d_x, rtol, inner_tol, kspace,
+  rc = fgmres_ns(c, 0, subdomain_flag, precision, ortho, c.d_rhs, c.d_x, rtol, inner_tol, kspace,
               max_cycles, iters, cycles, relres);
   if (rc && rc != OFD_ERR_NOCONV) return rc;
   c.sol_src = 0;
   OFD_HIP(launch_node4_to_soa(c, c.d_x, c.d_field[OFD_SOLU], c.d_field[OFD_SOLP]));
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  return rc;
+}
+
+// ---- temperature adjoint ----
+
+// the adjoint vectors of the T system (zero padded to whole 4-tuples)
+static int ensure_tem_adjoint(Ctx &c) {
+  int rc = ensure_tem(c);
+  if (rc) return rc;
+  if (c.tadj_n4 == c.temT_n4 && c.d_vjpT) return OFD_OK;
+  free_tem_adjoint(c);
+  const size_t np = (size_t)c.temT_n4 * 4;
+  double **bufs[4] = {&c.d_lamT, &c.d_lamTb, &c.d_lamTd, &c.d_jtT};
+  for (double **b : bufs) {
+    rc = dalloc(c, b, np);
+    if (rc) return rc;
+    OFD_HIP(hipMemsetAsync(*b, 0, np * sizeof(double), c.stream));
+  }
+  rc = dalloc(c, &c.d_vjpT, (size_t)c.nnode * 6);
+  if (rc) return rc;
+  c.tadj_n4 = c.temT_n4;
+  return OFD_OK;
+}
+
+// tT_A = LHSTem transposed (mirrored through d_tmap)
+static int ensure_tem_transpose(Ctx &c) {
+  OFD_CHECK(c.matT_valid, OFD_ERR_ARG, "no assembled temperature matrix");
+  if (c.tT_mat_valid && c.tT_version == c.matT_version) return OFD_OK;
+  int rc = OFD_OK;
+  if (!c.tT_A) rc = dalloc(c, &c.tT_A, (size_t)c.icnt);
+  if (!rc && !c.tT_P) rc = dalloc(c, &c.tT_P, (size_t)c.nnode);
+  if (rc) return rc;
+  OFD_HIP(launch_scalar_transpose(c));
+  c.tT_mat_valid = true;
+  c.tT_prec_valid = c.tT_a32_valid = c.tT_ap32_valid = false;
+  c.tT_version = c.matT_version;
+  return OFD_OK;
+}
+
+// the T-system kernels see LHSTem^T (and its own copies) while a view lives
+struct TemTransposeView {
+  Ctx &c;
+  explicit TemTransposeView(Ctx &cc) : c(cc) { swap_all(); }
+  ~TemTransposeView() { swap_all(); }
+  void swap_all() {
+    std::swap(c.d_AT, c.tT_A);
+    std::swap(c.d_AT32, c.tT_A32);
+    std::swap(c.d_PT, c.tT_P);
+    std::swap(c.d_ApT, c.tT_Ap);
+    std::swap(c.d_ApT32, c.tT_Ap32);
+    std::swap(c.matT_valid, c.tT_mat_valid);
+    std::swap(c.precT_valid, c.tT_prec_valid);
+    std::swap(c.aT32_valid, c.tT_a32_valid);
+    std::swap(c.apT32_valid, c.tT_ap32_valid);
+  }
+};
+
+static int tem_adjoint_ready(Ctx &c) {
+  OFD_HIP(hipSetDevice(c.device));
+  int rc = ensure_tem_adjoint(c);
+  if (rc) return rc;
+  OFD_CHECK(c.quench, OFD_ERR_ARG, "no quench parameters (ofd_set_quench_params)");
+  OFD_CHECK(c.nshl == 8, OFD_ERR_UNSUPPORTED,
+            "the GPU element kernels are implemented for trilinear hexes (NSHL=8)");
+  return OFD_OK;
+}
+
+int ofd_spmv_tem_t(ofd_ctx *x, int precision, const double *xin, double *y) {
+  OFD_CHECK(x && xin && y, OFD_ERR_ARG, "null argument");
+  Ctx &c = x->c;
+  int rc = ensure_tem(c);
+  if (rc) return rc;
+  OFD_HIP(hipSetDevice(c.device));
+  rc = ensure_tem_transpose(c);
+  if (rc) return rc;
+  TemTransposeView view(c);
+  return ofd_spmv_tem(x, precision, xin, y);
+}
+
+int ofd_solve_tem_t(ofd_ctx *x, int precision, int ortho, double rtol, double inner_tol,
+                    int kspace, int max_cycles, const double *g, int *iters, int *cycles,
+                    double *relres) {
+  OFD_CHECK(x && g, OFD_ERR_ARG, "null argument");
+  OFD_CHECK(precision == OFD_FP64 || precision == OFD_FP32, OFD_ERR_ARG, "bad precision");
+  OFD_CHECK(max_cycles >= 1, OFD_ERR_ARG, "max_cycles >= 1");
+  Ctx &c = x->c;
+  OFD_HIP(hipSetDevice(c.device));
+  int rc = ensure_tem_adjoint(c);
+  if (rc) return rc;
+  rc = ensure_tem_transpose(c);
+  if (rc) return rc;
+  rc = h2d(c, c.d_lamTb, g, (size_t)c.nnode * sizeof(double));
+  if (rc) return rc;
+  {
+    TemSystem sys(c);
+    TemTransposeView view(c);
+    rc = solve_core(c, precision, ortho, c.d_lamTb, c.d_lamT, c.d_lamTd, rtol, inner_tol,
+                    kspace, max_cycles, iters, cycles, relres);
+  }
+  if (rc && rc != OFD_ERR_NOCONV) return rc;
+  OFD_HIP(hipMemcpyAsync(c.d_field[OFD_LAMT], c.d_lamT, (size_t)c.nnode * sizeof(double),
+                         hipMemcpyDeviceToDevice, c.stream));
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  return rc;
+}
+
+int ofd_vjp_tem(ofd_ctx *x, const double *lam, double *gT, double *grT, double *gu,
+                double *gks) {
+  OFD_CHECK(x && lam, OFD_ERR_ARG, "null argument");
+  Ctx &c = x->c;
+  int rc = tem_adjoint_ready(c);
+  if (rc) return rc;
+  const size_t N = (size_t)c.nnode;
+  rc = h2d(c, c.d_lamTb, lam, N * sizeof(double));
+  if (rc) return rc;
+  OFD_HIP(launch_alpha_stage(c));
+  OFD_HIP(launch_zero(c, c.d_vjpT, N * 6));
+  OFD_HIP(launch_vjp_tem(c, c.d_lamTb, c.d_vjpT));
+  if (gT) rc = d2h(c, gT, c.d_vjpT, N * sizeof(double));
+  if (!rc && grT) rc = d2h(c, grT, c.d_vjpT + N, N * sizeof(double));
+  if (!rc && gu) rc = d2h(c, gu, c.d_vjpT + 2 * N, 3 * N * sizeof(double));
+  if (!rc && gks) rc = d2h(c, gks, c.d_vjpT + 5 * N, N * sizeof(double));
+  if (rc) return rc;
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  return OFD_OK;
+}
+
+int ofd_jt_apply_tem(ofd_ctx *x, const double *lam, double *y) {
+  OFD_CHECK(x && lam && y, OFD_ERR_ARG, "null argument");
+  Ctx &c = x->c;
+  int rc = tem_adjoint_ready(c);
+  if (rc) return rc;
+  const size_t N = (size_t)c.nnode;
+  rc = h2d(c, c.d_lamTb, lam, N * sizeof(double));
+  if (rc) return rc;
+  OFD_HIP(launch_alpha_stage(c));
+  OFD_HIP(launch_jt_tem(c, c.d_lamTb, c.d_jtT));
+  return d2h(c, y, c.d_jtT, N * sizeof(double));
+}
+
+int ofd_solve_tem_adjoint(ofd_ctx *x, int precision, int ortho, double rtol,
+                          double inner_tol, int kspace, int max_cycles, const double *g,
+                          int *iters, int *cycles, double *relres) {
+  OFD_CHECK(x && g, OFD_ERR_ARG, "null argument");
+  OFD_CHECK(precision == OFD_FP64 || precision == OFD_FP32, OFD_ERR_ARG, "bad precision");
+  OFD_CHECK(max_cycles >= 1, OFD_ERR_ARG, "max_cycles >= 1");
+  Ctx &c = x->c;
+  int rc = tem_adjoint_ready(c);
+  if (rc) return rc;
+  rc = ensure_tem_transpose(c);
+  if (rc) return rc;
+  const size_t N = (size_t)c.nnode;
+  rc = h2d(c, c.d_lamTb, g, N * sizeof(double));
+  if (rc) return rc;
+  OFD_HIP(launch_alpha_stage(c));
+  OFD_HIP(launch_zero(c, c.d_lamT, (size_t)c.temT_n4 * 4));
+  {
+    TemSystem sys(c);
+    auto apply = [&](const double *v, double *y) -> hipError_t {
+      return launch_jt_tem(c, v, y);
+    };
+    auto prec = [&](const double *v, double *z, int *it) -> int {
+      TemTransposeView view(c);
+      int cy = 0;
+      double ir = 0.0;
+      return solve_core(c, precision, ortho, v, z, c.d_lamTd, inner_tol,
+                        precision == OFD_FP32 ? 1e-4 : 1e-2, 30, 100, it, &cy, &ir);
+    };
+    rc = fgmres(c, (size_t)c.temT_n4 * 4, apply, prec, c.d_lamTb, c.d_lamT, c.d_jtT, rtol,
+                kspace, max_cycles, iters, cycles, relres);
+  }
+  if (rc && rc != OFD_ERR_NOCONV) return rc;
+  OFD_HIP(hipMemcpyAsync(c.d_field[OFD_LAMT], c.d_lamT, N * sizeof(double),
+                         hipMemcpyDeviceToDevice, c.stream));
   OFD_HIP(hipStreamSynchronize(c.stream));
   return rc;
 }

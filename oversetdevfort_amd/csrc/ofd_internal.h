@@ -179,8 +179,23 @@ struct Ctx {
   // flexible GMRES on J_bc or J_bc^T: basis V [(m+1)][N][4], preconditioned
   // directions Z [m][N][4], small device vectors of coefficients
   int fg_m = 0;
+  size_t fg_n = 0;
   double *d_fv = nullptr, *d_fz = nullptr, *d_fh = nullptr;
   double *d_vjp = nullptr;            // [8N] raw gradients u, ac, p, scale
+  // temperature adjoint: per-node conductivity scale (OFD_KSCALE) in use,
+  // LHSTem transposed and its solver copies (swapped in by a view), the
+  // bumped / recorded versions of LHSTem, and the adjoint vectors
+  // (4 * ceil(N/4) each, zero padded): lambda, right-hand side, inner-solve
+  // residual, operator output; raw gradients [6N] T, rT, u, scale
+  bool ks_on = false;
+  double *tT_A = nullptr, *tT_P = nullptr, *tT_Ap = nullptr;
+  float *tT_A32 = nullptr, *tT_Ap32 = nullptr;
+  bool tT_mat_valid = false, tT_prec_valid = false, tT_a32_valid = false,
+       tT_ap32_valid = false;
+  int64_t matT_version = 0, tT_version = -1;
+  double *d_lamT = nullptr, *d_lamTb = nullptr, *d_lamTd = nullptr, *d_jtT = nullptr;
+  double *d_vjpT = nullptr;
+  int tadj_n4 = 0;
   // overset fringe: per-element boxes, source flags and geometry; receivers
   int fr_alloc_e = 0, fr_alloc_r = 0;
   double *d_fr_box = nullptr;         // [E][6] widened bounding box
@@ -212,6 +227,12 @@ hipError_t launch_vjp_ns(Ctx &c, int subdomain_flag, const double *lam);
 hipError_t launch_jt_ns(Ctx &c, int subdomain_flag, const double *lam, double *y);
 // y = J_bc v, v a Newton increment
 hipError_t launch_jv_ns(Ctx &c, int subdomain_flag, const double *v, double *y);
+// temperature adjoint (adjoint_tem.hip): tT_A = LHSTem mirrored through d_tmap
+hipError_t launch_scalar_transpose(Ctx &c);
+// gradients of lambda . R_T (lam [N]) into out [6N] (zeroed by the caller)
+hipError_t launch_vjp_tem(Ctx &c, const double *lam, double *out);
+// y = J_T,bc^T lam (y: 4 ceil(N/4), zeroed here)
+hipError_t launch_jt_tem(Ctx &c, const double *lam, double *y);
 // out = x + a y
 hipError_t launch_axpby(Ctx &c, const double *x, const double *y, double a, double *out,
                         size_t n);

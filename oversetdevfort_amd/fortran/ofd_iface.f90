@@ -25,7 +25,7 @@ module ofd_iface
                                OFD_RPHIGOLD = 18, OFD_TG = 19, OFD_TGOLD = 20, &
                                OFD_RTG = 21, OFD_RTGOLD = 22, OFD_SOLT = 23, &
                                OFD_SOLPHI = 24, OFD_LAMU = 25, OFD_LAMP = 26, &
-                               OFD_MUSCALE = 27
+                               OFD_MUSCALE = 27, OFD_KSCALE = 28, OFD_LAMT = 29
   integer(c_int), parameter :: OFD_NPARAMS = 11, OFD_NQPARAMS = 21
 
   interface

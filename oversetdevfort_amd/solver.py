@@ -469,6 +469,63 @@ class Context:
                                 float(inner_tol), int(kspace), int(max_cycles)), gu, gp,
                                raise_noconv)
 
+    # ---- adjoint of the quenching temperature equation (no reference code) ----
+    def SparseProd_LS_T(self, x, precision=L.OFD_FP64):
+        """y = LHSTem^T x."""
+        x_ = _f64(x)
+        y = np.zeros(self.nnode)
+        L.check(self.lib.ofd_spmv_tem_t(self.h, precision, _ptr(x_), _ptr(y)))
+        return y
+
+    def _tem_adj_solve(self, fn, args, g, raise_noconv):
+        g_ = _f64(g)
+        it = ctypes.c_int()
+        cyc = ctypes.c_int()
+        rel = ctypes.c_double()
+        rc = fn(self.h, *args, _ptr(g_), ctypes.byref(it), ctypes.byref(cyc),
+                ctypes.byref(rel))
+        if rc != L.OFD_OK and (raise_noconv or rc != 5):
+            L.check(rc)
+        return dict(lam=self.get_field("LAMT"), iters=it.value, cycles=cyc.value,
+                    relres=rel.value, converged=rc == L.OFD_OK)
+
+    def solve_tem_t(self, g, rtol=1e-8, precision=L.OFD_FP64, ortho=L.OFD_ORTHO_CGS1,
+                    inner_tol=None, kspace=30, max_cycles=100, raise_noconv=True):
+        """LHSTem^T lambda = g for the assembled LHSTem."""
+        if inner_tol is None:
+            inner_tol = 1e-4 if precision == L.OFD_FP32 else 1e-2
+        return self._tem_adj_solve(self.lib.ofd_solve_tem_t,
+                                   (precision, ortho, float(rtol), float(inner_tol),
+                                    int(kspace), int(max_cycles)), g, raise_noconv)
+
+    def vjp_tem(self, lam):
+        """Gradients of lambda . R_T (the temperature residual as assembled,
+        before the Dirichlet projection) with respect to the alpha-stage T,
+        rT, u and the per-node conductivity scale."""
+        lam_ = _f64(lam)
+        N = self.nnode
+        gT, grT, gks = np.zeros(N), np.zeros(N), np.zeros(N)
+        gu = np.zeros((3, N))
+        L.check(self.lib.ofd_vjp_tem(self.h, _ptr(lam_), _ptr(gT), _ptr(grT), _ptr(gu),
+                                     _ptr(gks)))
+        return dict(T=gT, rT=grT, u=gu.T.copy(), ks=gks)
+
+    def jt_apply_tem(self, lam):
+        """y = J_T,bc^T lambda (exact Jacobian of the temperature residual with
+        respect to the Newton increments, with the Dirichlet projection)."""
+        lam_ = _f64(lam)
+        y = np.zeros(self.nnode)
+        L.check(self.lib.ofd_jt_apply_tem(self.h, _ptr(lam_), _ptr(y)))
+        return y
+
+    def solve_tem_adjoint(self, g, rtol=1e-8, precision=L.OFD_FP64, ortho=L.OFD_ORTHO_CGS1,
+                          inner_tol=1e-3, kspace=30, max_cycles=20, raise_noconv=True):
+        """J_T,bc^T lambda = g: flexible GMRES on the exact operator,
+        right-preconditioned by LHSTem^T solves."""
+        return self._tem_adj_solve(self.lib.ofd_solve_tem_adjoint,
+                                   (precision, ortho, float(rtol), float(inner_tol),
+                                    int(kspace), int(max_cycles)), g, raise_noconv)
+
     # ---- measurement ----
     def time_quench(self, what, arg, reps=5):
         """Mean device ms of one quenching-solver kernel (L.OFD_TIME_*; arg is

@@ -120,7 +120,8 @@ int main() {
     std::snprintf(name, sizeof name, "nsvof_%s", fname[f]);
     report(name);
     reset();
-    orc_element_tem(8, 8, xl, dl, ul, uml, phil, Tl, rTl, prm, qp, 1, flags[f], te, rt);
+    orc_element_tem(8, 8, xl, dl, ul, uml, phil, Tl, rTl, nullptr, prm, qp, 1, flags[f], te,
+                    rt);
     std::snprintf(name, sizeof name, "tem_%s", fname[f]);
     report(name);
   }
