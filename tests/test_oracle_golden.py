@@ -59,20 +59,30 @@ def _mesh(z):
     return Mesh(xg=z["xg"], ien=z["ien"], elm_id=z["elm_id"])
 
 
-@pytest.mark.parametrize("case", ["ns_tet_a", "ns_tet_b", "ns_tet_vec"])
+@pytest.mark.parametrize("case", ["ns_tet_a", "ns_tet_b", "ns_tet_vec", "ns_tet_vms"])
 def test_ns_assembly_spmv_gmres_vs_reference(oracle_mod, case):
+    """ns_tet_vms: the reference's IntElmAss_NS_overset relinked so that its
+    e3Rhs_3D_fluid receives uprime = -tauM rLi (the line e3RHS.f90:31 keeps
+    commented out), against the oracle's fine-scale residual form."""
     O = oracle_mod
     z = golden(case)
     m = _mesh(z)
     col, row = O.sparsity(m.nnode, m.ien)
     assert np.array_equal(col, z["col"]) and np.array_equal(row, z["row"])
     st = {k[3:]: z[k] for k in z.files if k.startswith("st_")}
+    vms = int(z["vms"]) if "vms" in z.files else 0
     A = O.assemble_ns(m, z["ibc"], st, z["prm"], col, row, int(z["tensor_flag"]),
-                      int(z["subdomain_flag"]))
+                      int(z["subdomain_flag"]), vms=vms)
     for k in ("k11", "g", "d1", "m", "rhsu", "rhsp"):
         ref = z[k]
         scale = max(np.abs(ref).max(), 1e-300)
         assert np.abs(A[k] - ref).max() <= 1e-13 * scale, k
+    if vms:
+        for k in ("k11", "g", "d1", "m", "rhsu", "rhsp"):
+            assert np.array_equal(A[k], z[k]), k
+        # the fine-scale terms are present: the reference's own form differs
+        A0 = O.assemble_ns(m, z["ibc"], st, z["prm"], col, row, 2, int(z["subdomain_flag"]))
+        assert np.abs(A0["rhsp"] - z["rhsp"]).max() > 1e-3 * np.abs(z["rhsp"]).max()
     yu, yp = O.spmv_up(col, row, A, z["xu"], z["xp"])
     sc = max(np.abs(z["yu"]).max(), np.abs(z["yp"]).max())
     assert np.abs(yu - z["yu"]).max() <= 1e-13 * sc
