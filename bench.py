@@ -14,8 +14,10 @@ Newton step in elements per second; the FP64 figures ride along.
 
 Beside the headline the line carries kernel figures of the quenching solver
 on the 48^3 tank (NSVOF and temperature assembly against their per-element
-op counts, the 5-field and scalar SpMV against HBM) and the overset fringe
-search on the 96x48x48 background + rotated 32^3 component mesh.
+op counts, the 5-field and scalar SpMV against HBM), the overset fringe
+search on the 96x48x48 background + rotated 32^3 component mesh, and the
+adjoint (element VJP and J^T kernels, one adjoint solve) of the channel's NS
+step and of the tank's temperature equation.
 
 Prints progress on stderr and ONE JSON line on stdout.
 """
@@ -153,6 +155,7 @@ def quench_kernels(L, C, S, n=48, reps=5, spmv_reps=20):
     for name, what in (("nsvof", L.OFD_TIME_SPMV_NSVOF), ("tem", L.OFD_TIME_SPMV_TEM)):
         for prec, tag in ((L.OFD_FP64, "fp64"), (L.OFD_FP32, "fp32")):
             sp[f"{name}_{tag}"] = ctx.time_quench(what, prec, spmv_reps)
+    adj = adjoint_tem(ctx, L, prob, t["tem_vec"], reps)
     ctx.close()
 
     def asm(ms, f):
@@ -173,7 +176,8 @@ def quench_kernels(L, C, S, n=48, reps=5, spmv_reps=20):
            "nsvof_spmv": {"fp64": prod(sp["nsvof_fp64"], spmv_bytes(Z, N, 8, b=5, S=24)),
                           "fp32": prod(sp["nsvof_fp32"], spmv_bytes(Z, N, 4, b=5, S=24))},
            "tem_spmv": {"fp64": prod(sp["tem_fp64"], spmv_bytes(Z, N, 8, b=1, S=1)),
-                        "fp32": prod(sp["tem_fp32"], spmv_bytes(Z, N, 4, b=1, S=1))}}
+                        "fp32": prod(sp["tem_fp32"], spmv_bytes(Z, N, 4, b=1, S=1))},
+           "tem_adjoint": adj}
     log(f"bench: quench {n}^3: NSVOF VEC+MAT {t['nsvof_vecmat']:.3f} ms "
         f"({E / t['nsvof_vecmat'] / 1e3:.1f} Melem/s, "
         f"{out['nsvof_assembly']['vecmat']['fp64_tflops']:.2f} TFLOP/s); "
@@ -182,6 +186,70 @@ def quench_kernels(L, C, S, n=48, reps=5, spmv_reps=20):
         f"({out['nsvof_spmv']['fp64']['GBps']:.0f} GB/s); T SpMV FP64 "
         f"{1e3 * sp['tem_fp64']:.1f} us")
     return out, prob
+
+
+def adjoint_ns(ctx, L, prob, asm_vec_ms, reps=5):
+    """Adjoint of the channel's NS step on the state of the kernel figures:
+    the element VJP (raw gradients) and J_bc^T kernels against the forward
+    VEC assembly, and one adjoint solve J_bc^T lam = g of the fine-scale
+    residual form (flexible GMRES, FP64 outer basis, preconditioned by LHS^T
+    solves in FP64 or mixed precision) to a 1e-8 true residual; g is a
+    seeded velocity load on the free dofs."""
+    import numpy as np
+    N = prob.mesh.nnode
+    free = (prob.ibc[:, :4] != 1).astype(np.float64)
+    lam = np.random.default_rng(7).normal(size=(N, 4)) * free
+    ctx.vjp_ns(lam[:, :3], lam[:, 3])        # uploads lambda
+    vjp_ms = ctx.time_adjoint(L.OFD_TIME_VJP_NS, reps)
+    jt_ms = ctx.time_adjoint(L.OFD_TIME_JT_NS, reps)
+    ctx.set_residual_form(1)
+    ctx.assembleNavSto(6, 1)
+    g = lam.copy()
+    g[:, 3] = 0.0
+    solves = {}
+    for prec, tag in ((L.OFD_FP64, "fp64"), (L.OFD_FP32, "mixed")):
+        t0 = time.perf_counter()
+        r = ctx.solve_adjoint(g[:, :3], g[:, 3], rtol=1e-8, precision=prec, max_cycles=10,
+                              raise_noconv=False)
+        ms = 1e3 * (time.perf_counter() - t0)
+        solves[tag] = {"ms": ms, "restarts": r["cycles"], "inner_gmres_iters": r["iters"],
+                       "relres": r["relres"], "converged": r["converged"]}
+        log(f"bench: channel adjoint solve ({tag} inner): {ms:.1f} ms, {r['cycles']} "
+            f"restarts, {r['iters']} inner iterations, relres {r['relres']:.1e}")
+    ctx.set_residual_form(0)
+    E = prob.mesh.nelem
+    log(f"bench: channel adjoint kernels: VJP {vjp_ms:.3f} ms, J^T {jt_ms:.3f} ms "
+        f"(forward VEC assembly {asm_vec_ms:.3f} ms)")
+    return {"vjp_ms": vjp_ms, "jt_ms": jt_ms, "vjp_melem_per_s": E / vjp_ms / 1e3,
+            "vjp_over_vec_assembly": vjp_ms / asm_vec_ms, "solve": solves,
+            "residual_form": "fine-scale (uprime = -tauM rLi)"}
+
+
+def adjoint_tem(ctx, L, prob, tem_vec_ms, reps=5):
+    """Temperature adjoint on the quench tank: VJP and J_T^T kernels against
+    the forward VEC assembly, one J_T^T solve (flexible GMRES preconditioned
+    by LHSTem^T) to a 1e-8 true residual."""
+    import numpy as np
+    N = prob.mesh.nnode
+    free = (prob.ibc[:, 5] != 1).astype(np.float64)
+    lam = np.random.default_rng(8).normal(size=N) * free
+    ctx.vjp_tem(lam)                          # uploads lambda
+    vjp_ms = ctx.time_adjoint(L.OFD_TIME_VJP_TEM, reps)
+    jt_ms = ctx.time_adjoint(L.OFD_TIME_JT_TEM, reps)
+    ctx.assemble_tem(6)
+    solves = {}
+    for prec, tag in ((L.OFD_FP64, "fp64"), (L.OFD_FP32, "mixed")):
+        t0 = time.perf_counter()
+        r = ctx.solve_tem_adjoint(lam, rtol=1e-8, precision=prec, max_cycles=10,
+                                  raise_noconv=False)
+        ms = 1e3 * (time.perf_counter() - t0)
+        solves[tag] = {"ms": ms, "restarts": r["cycles"], "inner_gmres_iters": r["iters"],
+                       "relres": r["relres"], "converged": r["converged"]}
+    E = prob.mesh.nelem
+    log(f"bench: tank temperature adjoint: VJP {vjp_ms:.3f} ms, J^T {jt_ms:.3f} ms "
+        f"(forward VEC {tem_vec_ms:.3f} ms); solve FP64 {solves['fp64']['ms']:.1f} ms")
+    return {"vjp_ms": vjp_ms, "jt_ms": jt_ms, "vjp_melem_per_s": E / vjp_ms / 1e3,
+            "vjp_over_vec_assembly": vjp_ms / tem_vec_ms, "solve": solves}
 
 
 def fringe_search(C, S, reps=5):
@@ -384,6 +452,8 @@ def main():
         f"{asm_tf / 1e12:.2f} TFLOP/s); VEC {asm_vec_ms:.3f} ms")
     log(f"bench: SpMV FP64 {spmv64_ms * 1e3:.1f} us {bw64 / 1e9:.0f} GB/s; "
         f"FP32-matrix {spmv32_ms * 1e3:.1f} us {bw32 / 1e9:.0f} GB/s")
+    adjoint = adjoint_ns(ctx, L, prob, asm_vec_ms)
+    adjoint["newton_step_fp64_ms"] = res64["newton_ms"]
     ctx.close()
 
     quench, qprob = quench_kernels(L, C, S)
@@ -460,6 +530,7 @@ def main():
                           "algorithmic_bytes": ba64,
                           "avg_launch_us": 1e3 * res64["arnoldi_spmv_ms"],
                           "samples": res64["arnoldi_spmv_samples"]},
+        "adjoint": adjoint,
         "quench": quench,
         "fringe": fringe,
         "cpu_baseline": cpu,

@@ -410,6 +410,15 @@ int ofd_solve_tem_adjoint(ofd_ctx *ctx, int precision, int ortho, double rtol,
                           int *iters, int *cycles, double *relres);
 
 /* Mean device time (HIP events on the context stream) of `reps` back-to-back
+ * launches of one adjoint kernel on the current state, all colour launches
+ * and the zeroing of its output included: what = OFD_TIME_VJP_NS (raw
+ * gradients), OFD_TIME_JT_NS (J_bc^T lambda), OFD_TIME_VJP_TEM,
+ * OFD_TIME_JT_TEM.  lambda is the one the last ofd_vjp_ns / ofd_jt_apply
+ * (ofd_vjp_tem / ofd_jt_apply_tem) call uploaded.  Measurement only. */
+enum { OFD_TIME_VJP_NS = 0, OFD_TIME_JT_NS = 1, OFD_TIME_VJP_TEM = 2, OFD_TIME_JT_TEM = 3 };
+int ofd_time_adjoint(ofd_ctx *ctx, int what, int reps, double *ms);
+
+/* Mean device time (HIP events on the context stream) of `reps` back-to-back
  * launches of one quenching-solver kernel: what = OFD_TIME_ASM_NSVOF or
  * OFD_TIME_ASM_TEM (arg = tensor flag; alpha stage, zeroing and the colour
  * launches), OFD_TIME_SPMV_NSVOF or OFD_TIME_SPMV_TEM (arg = precision; needs

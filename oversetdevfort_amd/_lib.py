@@ -25,6 +25,10 @@ OFD_TIME_ASM_NSVOF = 0
 OFD_TIME_ASM_TEM = 1
 OFD_TIME_SPMV_NSVOF = 2
 OFD_TIME_SPMV_TEM = 3
+OFD_TIME_VJP_NS = 0
+OFD_TIME_JT_NS = 1
+OFD_TIME_VJP_TEM = 2
+OFD_TIME_JT_TEM = 3
 
 FIELDS = ["XG", "DG", "DGOLD", "UG", "UGOLD", "UGM", "UGMOLD", "ACG", "ACGOLD",
           "ACGM", "ACGMOLD", "PG", "PGOLD", "SOLU", "SOLP",
@@ -123,6 +127,7 @@ SIGNATURES = {
     "ofd_time_assemble": ([_vp, _c_int, _c_int, _dp], _c_int),
     "ofd_time_copy": ([_vp, ctypes.c_int64, _c_int, _dp], _c_int),
     "ofd_time_quench": ([_vp, _c_int, _c_int, _c_int, _dp], _c_int),
+    "ofd_time_adjoint": ([_vp, _c_int, _c_int, _dp], _c_int),
     "ofd_probe_set": ([_vp, _c_int], _c_int),
     "ofd_probe_read": ([_vp, _i64p, _dp], _c_int),
     "ofd_ctx_bytes": ([_vp, _i64p], _c_int),

@@ -2425,6 +2425,46 @@ int ofd_solve_tem_adjoint(ofd_ctx *x, int precision, int ortho, double rtol,
   return rc;
 }
 
+int ofd_time_adjoint(ofd_ctx *x, int what, int reps, double *ms) {
+  OFD_CHECK(x && ms && reps > 0, OFD_ERR_ARG, "bad argument");
+  OFD_CHECK(what >= OFD_TIME_VJP_NS && what <= OFD_TIME_JT_TEM, OFD_ERR_ARG,
+            "unknown kernel");
+  Ctx &c = x->c;
+  const bool tem = what >= OFD_TIME_VJP_TEM;
+  int rc = tem ? tem_adjoint_ready(c) : ensure_adjoint(c);
+  if (rc) return rc;
+  OFD_CHECK(c.nshl == 8, OFD_ERR_UNSUPPORTED,
+            "the GPU element kernels are implemented for trilinear hexes (NSHL=8)");
+  OFD_HIP(hipSetDevice(c.device));
+  const size_t N = (size_t)c.nnode;
+  OFD_HIP(launch_alpha_stage(c));
+  hipEvent_t e0, e1;
+  OFD_HIP(hipEventCreate(&e0));
+  OFD_HIP(hipEventCreate(&e1));
+  for (int r = -1; r < reps; ++r) {  // r = -1: warm-up launch
+    if (r == 0) OFD_HIP(hipEventRecord(e0, c.stream));
+    if (what == OFD_TIME_VJP_NS) {
+      OFD_HIP(launch_zero(c, c.d_vjp, N * 8));
+      OFD_HIP(launch_vjp_ns(c, 1, c.d_lamb));
+    } else if (what == OFD_TIME_JT_NS) {
+      OFD_HIP(launch_jt_ns(c, 1, c.d_lamb, c.d_jt));
+    } else if (what == OFD_TIME_VJP_TEM) {
+      OFD_HIP(launch_zero(c, c.d_vjpT, N * 6));
+      OFD_HIP(launch_vjp_tem(c, c.d_lamTb, c.d_vjpT));
+    } else {
+      OFD_HIP(launch_jt_tem(c, c.d_lamTb, c.d_jtT));
+    }
+  }
+  OFD_HIP(hipEventRecord(e1, c.stream));
+  OFD_HIP(hipEventSynchronize(e1));
+  float t = 0.f;
+  OFD_HIP(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = (double)t / reps;
+  return OFD_OK;
+}
+
 int ofd_set_residual_form(ofd_ctx *x, int form) {
   OFD_CHECK(x, OFD_ERR_ARG, "null context");
   OFD_CHECK(form == 0 || form == 1, OFD_ERR_ARG, "residual form 0 or 1");

@@ -387,6 +387,142 @@ module ofd_iface
       real(c_double), intent(out) :: norms2(2)
       integer(c_int) :: rc
     end function
+
+    ! ---- adjoint (no reference counterpart); (NNODE,3) + (NNODE) vectors ----
+    function ofd_spmv_up_t(ctx, precision, xu, xp, yu, yp) bind(C, name='ofd_spmv_up_t') &
+        result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: precision
+      real(c_double), intent(in) :: xu(*), xp(*)
+      real(c_double), intent(out) :: yu(*), yp(*)
+      integer(c_int) :: rc
+    end function
+
+    function ofd_solve_t(ctx, precision, ortho, rtol, inner_tol, kspace, max_cycles, gu, gp, &
+                         iters, cycles, relres) bind(C, name='ofd_solve_t') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: precision, ortho, kspace, max_cycles
+      real(c_double), value :: rtol, inner_tol
+      real(c_double), intent(in) :: gu(*), gp(*)
+      integer(c_int), intent(out) :: iters, cycles
+      real(c_double), intent(out) :: relres
+      integer(c_int) :: rc
+    end function
+
+    function ofd_vjp_ns(ctx, subdomain_flag, lu, lp, gu, gac, gp, gmus) &
+        bind(C, name='ofd_vjp_ns') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: subdomain_flag
+      real(c_double), intent(in) :: lu(*), lp(*)
+      real(c_double), intent(out) :: gu(*), gac(*), gp(*), gmus(*)
+      integer(c_int) :: rc
+    end function
+
+    function ofd_jt_apply(ctx, subdomain_flag, lu, lp, yu, yp) bind(C, name='ofd_jt_apply') &
+        result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: subdomain_flag
+      real(c_double), intent(in) :: lu(*), lp(*)
+      real(c_double), intent(out) :: yu(*), yp(*)
+      integer(c_int) :: rc
+    end function
+
+    function ofd_j_apply(ctx, subdomain_flag, vu, vp, yu, yp) bind(C, name='ofd_j_apply') &
+        result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: subdomain_flag
+      real(c_double), intent(in) :: vu(*), vp(*)
+      real(c_double), intent(out) :: yu(*), yp(*)
+      integer(c_int) :: rc
+    end function
+
+    function ofd_set_residual_form(ctx, form) bind(C, name='ofd_set_residual_form') &
+        result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: form
+      integer(c_int) :: rc
+    end function
+
+    function ofd_solve_adjoint(ctx, subdomain_flag, precision, ortho, rtol, inner_tol, &
+                               kspace, max_cycles, gu, gp, iters, cycles, relres) &
+        bind(C, name='ofd_solve_adjoint') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: subdomain_flag, precision, ortho, kspace, max_cycles
+      real(c_double), value :: rtol, inner_tol
+      real(c_double), intent(in) :: gu(*), gp(*)
+      integer(c_int), intent(out) :: iters, cycles
+      real(c_double), intent(out) :: relres
+      integer(c_int) :: rc
+    end function
+
+    function ofd_solve_newton(ctx, subdomain_flag, precision, ortho, rtol, inner_tol, &
+                              kspace, max_cycles, iters, cycles, relres) &
+        bind(C, name='ofd_solve_newton') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: subdomain_flag, precision, ortho, kspace, max_cycles
+      real(c_double), value :: rtol, inner_tol
+      integer(c_int), intent(out) :: iters, cycles
+      real(c_double), intent(out) :: relres
+      integer(c_int) :: rc
+    end function
+
+    function ofd_spmv_tem_t(ctx, precision, x, y) bind(C, name='ofd_spmv_tem_t') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: precision
+      real(c_double), intent(in) :: x(*)
+      real(c_double), intent(out) :: y(*)
+      integer(c_int) :: rc
+    end function
+
+    function ofd_solve_tem_t(ctx, precision, ortho, rtol, inner_tol, kspace, max_cycles, g, &
+                             iters, cycles, relres) bind(C, name='ofd_solve_tem_t') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: precision, ortho, kspace, max_cycles
+      real(c_double), value :: rtol, inner_tol
+      real(c_double), intent(in) :: g(*)
+      integer(c_int), intent(out) :: iters, cycles
+      real(c_double), intent(out) :: relres
+      integer(c_int) :: rc
+    end function
+
+    function ofd_vjp_tem(ctx, lam, gT, grT, gu, gks) bind(C, name='ofd_vjp_tem') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: lam(*)
+      real(c_double), intent(out) :: gT(*), grT(*), gu(*), gks(*)
+      integer(c_int) :: rc
+    end function
+
+    function ofd_jt_apply_tem(ctx, lam, y) bind(C, name='ofd_jt_apply_tem') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: lam(*)
+      real(c_double), intent(out) :: y(*)
+      integer(c_int) :: rc
+    end function
+
+    function ofd_solve_tem_adjoint(ctx, precision, ortho, rtol, inner_tol, kspace, &
+                                   max_cycles, g, iters, cycles, relres) &
+        bind(C, name='ofd_solve_tem_adjoint') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: precision, ortho, kspace, max_cycles
+      real(c_double), value :: rtol, inner_tol
+      real(c_double), intent(in) :: g(*)
+      integer(c_int), intent(out) :: iters, cycles
+      real(c_double), intent(out) :: relres
+      integer(c_int) :: rc
+    end function
   end interface
 
 contains

@@ -535,6 +535,14 @@ class Context:
                                          ctypes.byref(ms)))
         return ms.value
 
+    def time_adjoint(self, what, reps=5):
+        """Mean device ms of one adjoint kernel (L.OFD_TIME_VJP_NS, ..._JT_NS,
+        ..._VJP_TEM, ..._JT_TEM) on the current state and the last uploaded
+        lambda."""
+        ms = ctypes.c_double()
+        L.check(self.lib.ofd_time_adjoint(self.h, int(what), int(reps), ctypes.byref(ms)))
+        return ms.value
+
     def time_spmv(self, precision=L.OFD_FP64, reps=20):
         ms = ctypes.c_double()
         L.check(self.lib.ofd_time_spmv(self.h, precision, int(reps), ctypes.byref(ms)))
