@@ -43,7 +43,8 @@ from torch.utils.checkpoint import checkpoint
 from . import _lib as L
 
 __all__ = ["NSStepper", "NSResidual", "NavStoStep", "ns_residual", "nav_sto_step",
-           "run_steps", "TemStepper", "TemStep", "tem_step", "run_tem_steps"]
+           "run_steps", "TemStepper", "TemStep", "tem_step", "run_tem_steps",
+           "solid_heat_loss"]
 
 
 def _np(t):
@@ -253,14 +254,16 @@ class TemStepper:
     Jacobian and the Newton iteration converges quadratically."""
 
     def __init__(self, ctx, ibc, params, phi, rtol=1e-12, max_newton=30, lin_rtol=1e-13,
-                 adj_rtol=1e-12, precision=L.OFD_FP64, inner_tol=1e-3, kspace=30,
-                 max_cycles=20):
+                 adj_rtol=1e-12, precision=L.OFD_FP64, adj_precision=None, inner_tol=1e-3,
+                 kspace=30, max_cycles=20):
         self.ctx = ctx
         self.free = (np.asarray(ibc)[:, 5] != 1).astype(np.float64)
         self.p = dict(params)
         self.phi = np.asarray(phi, np.float64)
         self.rtol, self.max_newton, self.lin_rtol = rtol, max_newton, lin_rtol
         self.adj_rtol, self.precision = adj_rtol, precision
+        # inner solves of the adjoint: FP64 or mixed (FP32-stored LHSTem^T)
+        self.adj_precision = precision if adj_precision is None else adj_precision
         self.inner_tol, self.kspace, self.max_cycles = inner_tol, kspace, max_cycles
         self.newton_history = []
 
@@ -313,7 +316,7 @@ class TemStepper:
         s = (grT + ga * dt * gT) * fr
         if np.any(s):
             c.assemble_tem(6)                 # LHSTem preconditions J_T^T
-            res = c.solve_tem_adjoint(s, rtol=self.adj_rtol, precision=self.precision,
+            res = c.solve_tem_adjoint(s, rtol=self.adj_rtol, precision=self.adj_precision,
                                       inner_tol=self.inner_tol, kspace=self.kspace,
                                       max_cycles=self.max_cycles)
             lam = res["lam"] * fr
@@ -377,3 +380,12 @@ def run_tem_steps(st, Told, rTold, ks, u, nsteps, checkpoint_every=0):
             state = _tem_segment(st, n, *state, ks, u)
         done += n
     return state
+
+
+def solid_heat_loss(st, Told, rTold, ks, u, rho_cp, solid_volume, nsteps, checkpoint_every=0):
+    """Heat the solid gives off over ``nsteps`` temperature steps,
+    sum_i V_i rho_s c_s (T0_i - T_i), V_i the solid's lumped nodal volumes:
+    by the solid's energy balance, the time-integrated heat flux across its
+    surface."""
+    T, _ = run_tem_steps(st, Told, rTold, ks, u, nsteps, checkpoint_every)
+    return torch.sum(rho_cp * solid_volume * (Told - T))

@@ -162,8 +162,10 @@ QUENCH_MATERIAL = dict(rhoa=1.0, rhow=1000.0, mua=1.0e-5, muw=1.0e-3, cpa=2000.0
                        lsc_kdc=0.0)
 
 
-def quench(n=48, seed=1, rhoinf=0.5, jitter=0.0) -> Problem:
-    mesh = M.structured_hex(n, n, n, jitter=jitter, seed=seed)
+def quench(n=48, seed=1, rhoinf=0.5, jitter=0.0, dims=None) -> Problem:
+    """The quench tank on the unit cube, n^3 hexes (or ``dims``)."""
+    nx, ny, nz = dims if dims is not None else (n, n, n)
+    mesh = M.structured_hex(nx, ny, nz, jitter=jitter, seed=seed)
     N, E = mesh.nnode, mesh.nelem
     x = mesh.xg
     rng = np.random.default_rng(seed)
@@ -194,6 +196,54 @@ def quench(n=48, seed=1, rhoinf=0.5, jitter=0.0) -> Problem:
              grav=(0.0, 0.0, -9.81))
     prob = Problem("quench", mesh, ibc, np.zeros((N, 3)), f, p)
     prob.material = dict(QUENCH_MATERIAL)
+    return prob
+
+
+# linhexshl corner signs (xi, eta, zeta) in the reference node order
+_HEX_SGN = np.array([[-1, 1, -1], [1, 1, -1], [1, 1, 1], [-1, 1, 1],
+                     [-1, -1, -1], [1, -1, -1], [1, -1, 1], [-1, -1, 1]], np.float64)
+
+
+def lumped_volumes(mesh, elements=None):
+    """Row sums of the hex mass matrix, sum_e int N_a dV (2x2x2 Gauss rule),
+    over all elements or the boolean selection ``elements``."""
+    g = 1.0 / np.sqrt(3.0)
+    ien = mesh.ien if elements is None else mesh.ien[elements]
+    X = mesh.xg[ien]                                   # (E, 8, 3)
+    vol = np.zeros(mesh.nnode)
+    for q in range(8):
+        xi = np.array([g if q & 1 else -g, g if q & 2 else -g, g if q & 4 else -g])
+        f = 1.0 + _HEX_SGN * xi                        # (8, 3)
+        N = 0.125 * f.prod(axis=1)
+        dN = np.stack([0.125 * _HEX_SGN[:, 0] * f[:, 1] * f[:, 2],
+                       0.125 * _HEX_SGN[:, 1] * f[:, 0] * f[:, 2],
+                       0.125 * _HEX_SGN[:, 2] * f[:, 0] * f[:, 1]], axis=1)   # (8, 3)
+        J = np.einsum("eai,aj->eij", X, dN)
+        dv = np.linalg.det(J)
+        np.add.at(vol, ien.ravel(), (dv[:, None] * N[None, :]).ravel())
+    return vol
+
+
+def circulation(x):
+    """A divergence-free swirl about the tank's vertical axis, tangential at
+    the walls, unit amplitude: u = curl(0, 0, psi),
+    psi = sin(pi x) sin(pi y) sin(pi z) / pi."""
+    sx, sy, sz = (np.sin(np.pi * x[:, k]) for k in range(3))
+    cx, cy = np.cos(np.pi * x[:, 0]), np.cos(np.pi * x[:, 1])
+    return np.stack([sx * cy * sz, -cx * sy * sz, np.zeros(len(x))], axis=1)
+
+
+def adjoint_quench(dims=(128, 64, 64), seed=5, speed=0.05):
+    """The adjoint configuration: the quench tank on ``dims`` hexes with the
+    flow held (VOF field as set up, the velocity ``speed`` x circulation),
+    a per-node conductivity scale exp(0.1 z), z seeded standard normal, and
+    the solid block's lumped nodal volumes for the heat it gives off."""
+    prob = quench(dims=dims, seed=seed)
+    rng = np.random.default_rng(seed)
+    N = prob.mesh.nnode
+    prob.kscale = np.exp(0.1 * rng.standard_normal(N))
+    prob.circulation = speed * circulation(prob.mesh.xg)
+    prob.solid_volume = lumped_volumes(prob.mesh, prob.mesh.elm_id == 101)
     return prob
 
 
