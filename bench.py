@@ -270,7 +270,9 @@ def adjoint_c5(L, C, S, nsteps=20, every=5):
     ctx.set_quench_params(prob.material)
     log(f"bench: adjoint tank 128x64x64: {E} elements, {N} nodes, setup "
         f"{time.perf_counter() - t0:.1f} s")
-    dev = torch.device("cuda:0") if torch.cuda.is_available() else torch.device("cpu")
+    # the tensors stay in host memory: the binding moves every array through
+    # host memory, so device tensors would only add copies
+    dev = torch.device("cpu")
 
     def t(a):
         return torch.tensor(np.asarray(a, np.float64), device=dev)
@@ -282,6 +284,11 @@ def adjoint_c5(L, C, S, nsteps=20, every=5):
     out = {"workload": "quench tank 128x64x64 hex, temperature with the flow held, "
                        f"{nsteps} steps, dt 1e-3, checkpoint every {every}",
            "elements": E, "nodes": N}
+    # one untimed step through autograd: first-use allocations on the device
+    # and in torch's autograd and checkpoint machinery
+    st = A.TemStepper(ctx, prob.ibc, prob.params, prob.fields["phig"])
+    ks = t(prob.kscale).requires_grad_()
+    A.solid_heat_loss(st, T0, rT0, ks, circ, rc, vol, 1, 1).backward()
     grads = {}
     for tag, aprec in (("fp64", L.OFD_FP64), ("fp32_matvec", L.OFD_FP32)):
         st = A.TemStepper(ctx, prob.ibc, prob.params, prob.fields["phig"],
