@@ -130,10 +130,13 @@ k_spmv(int N, const int32_t *__restrict__ col, const int32_t *__restrict__ row,
 
 // Scalar CSR product of the temperature system (SparseProd_LS,
 // SparseProd.f90:128-161): mode 0 y = A x, 1 y = b - A x, 2 the Arnoldi
-// product (returns at once once the cycle has converged).  4 lanes per row
-// stride the row's nonzeros and are added in a fixed shuffle order; rows
-// n .. npad-1 (the zero padding to whole 4-tuples) are written as 0.  The
-// same XCD-aware block order as the block product.
+// product (returns at once once the cycle has converged).  8 lanes per row:
+// lane c takes nonzeros c, c+8, ... of the row, the first 32 as four
+// independent (value, column) loads issued before any x is gathered, so a
+// wave keeps its 8 rows' contiguous nonzeros in flight; the lanes are added
+// in a fixed shuffle order.  Rows n .. npad-1 (the zero padding to whole
+// 4-tuples) are written as 0.  The same XCD-aware block order as the block
+// product.
 template <typename T, typename V, int MODE>
 __global__ void __launch_bounds__(256)
 k_spmv_s(int n, int npad, const int32_t *__restrict__ col, const int32_t *__restrict__ row,
@@ -144,16 +147,29 @@ k_spmv_s(int n, int npad, const int32_t *__restrict__ col, const int32_t *__rest
   const int per = gridDim.x >> 3;
   const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
   const int gt = lb * blockDim.x + threadIdx.x;
-  const int r = gt >> 2;
-  const int c = gt & 3;
-  if (r >= npad) return;  // uniform over each group of 4 lanes
+  const int r = gt >> 3;
+  const int c = gt & 7;
+  if (r >= npad) return;  // uniform over each group of 8 lanes
   double acc = 0.0;
   if (r < n) {
-    const int k1 = col[r + 1];
-    for (int k = col[r] + c; k < k1; k += 4) acc += (double)A[k] * (double)x[row[k]];
+    const int k0 = col[r] + c, k1 = col[r + 1];
+    T av[4];
+    int jv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = k0 + 8 * u;
+      const bool in = k < k1;
+      av[u] = in ? A[k] : (T)0;
+      jv[u] = in ? row[k] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (jv[u] >= 0) acc += (double)av[u] * (double)x[jv[u]];
+    for (int k = k0 + 32; k < k1; k += 8) acc += (double)A[k] * (double)x[row[k]];
   }
   acc += __shfl_xor(acc, 1);
   acc += __shfl_xor(acc, 2);
+  acc += __shfl_xor(acc, 4);
   if (c == 0) {
     if (r >= n)
       y[r] = (V)0.0;
@@ -165,17 +181,19 @@ k_spmv_s(int n, int npad, const int32_t *__restrict__ col, const int32_t *__rest
 }
 
 // inverse diagonal of the temperature matrix, 0 -> 1 (SparseGMRES_tem.f90:
-// 38-59), and the preconditioned copy Ap[k] = dinv_i A[k] (FP64, FP32)
-__global__ void k_build_prec_s(int N, const int32_t *__restrict__ col,
-                               const int32_t *__restrict__ diag, const double *__restrict__ A,
-                               double *__restrict__ P, double *__restrict__ Ap,
-                               float *__restrict__ Ap32) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// 38-59), and the preconditioned copy Ap[k] = dinv_i A[k] (FP64, FP32); 8
+// lanes per row stream the row's contiguous nonzeros
+__global__ void __launch_bounds__(256)
+k_build_prec_s(int N, const int32_t *__restrict__ col, const int32_t *__restrict__ diag,
+               const double *__restrict__ A, double *__restrict__ P, double *__restrict__ Ap,
+               float *__restrict__ Ap32) {
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = gt >> 3, c = gt & 7;
   if (i >= N) return;
   const double d = A[diag[i]];
   const double di = (d != 0.0) ? 1.0 / d : 1.0;
-  P[i] = di;
-  for (int k = col[i]; k < col[i + 1]; ++k) {
+  if (c == 0) P[i] = di;
+  for (int k = col[i] + c; k < col[i + 1]; k += 8) {
     const double v = di * A[k];
     Ap[k] = v;
     if (Ap32) Ap32[k] = (float)v;
@@ -414,9 +432,9 @@ hipError_t launch_spmv(Ctx &c, int precision, const double *x, double *y,
 
 // Arnoldi product W_{k+1} = (P^-1 A) W_k, unscaled (FP32 basis in the mixed
 // CGS solver)
-// scalar rows, 4 lanes each, over the padded length, rounded to 8 blocks
+// scalar rows, 8 lanes each, over the padded length, rounded to 8 blocks
 static inline unsigned spmv_s_blocks(const Ctx &c) {
-  const unsigned nb = (unsigned)(((size_t)c.gm_n * 16 + 255) / 256);
+  const unsigned nb = (unsigned)(((size_t)c.gm_n * 32 + 255) / 256);
   return (nb + 7u) & ~7u;
 }
 
@@ -444,7 +462,8 @@ hipError_t launch_spmv_s(Ctx &c, int precision, const double *x, double *y,
 
 hipError_t launch_build_prec_s(Ctx &c, bool want32) {
   const int N = c.nnode;
-  hipLaunchKernelGGL(k_build_prec_s, dim3((N + 255) / 256), dim3(256), 0, c.stream, N,
+  hipLaunchKernelGGL(k_build_prec_s, dim3((unsigned)(((size_t)N * 8 + 255) / 256)), dim3(256),
+                     0, c.stream, N,
                      c.d_col, c.d_diag, c.d_AT, c.d_PT, c.d_ApT, want32 ? c.d_ApT32 : nullptr);
   return hipGetLastError();
 }
