@@ -90,7 +90,9 @@ def run_steps(ctx, S, L, prob, precision, steps, warmup, kspace=30, ortho=1,
             res.append(info["res"])
     nprobe, probe_ms = ctx.probe_read()
     ctx.probe_set(0)
+    fields = {"ug": ctx.get_field("UG"), "pg": ctx.get_field("PG"), "acg": ctx.get_field("ACG")}
     return dict(arnoldi_spmv_ms=probe_ms / max(nprobe, 1), arnoldi_spmv_samples=nprobe,
+                fields=fields,
                 newton_ms=sum(newton_ms) / len(newton_ms),
                 step_ms=sum(step_ms) / len(step_ms),
                 total_ms=sum(step_ms), gmres_iters=iters, cycles=cycles,
@@ -486,6 +488,19 @@ def main():
     log(f"bench: mixed newton {res32['newton_ms']:.2f} ms; Arnoldi SpMV "
         f"{1e3 * res32['arnoldi_spmv_ms']:.1f} us ({res32['arnoldi_spmv_samples']} samples)")
 
+    # each field after the same time steps: mixed solver against FP64
+    import numpy as np
+    f64, f32 = res64.pop("fields"), res32.pop("fields")
+    field_err = {}
+    for name, a, b in (("u", f32["ug"][:, 0], f64["ug"][:, 0]),
+                       ("v", f32["ug"][:, 1], f64["ug"][:, 1]),
+                       ("w", f32["ug"][:, 2], f64["ug"][:, 2]), ("p", f32["pg"], f64["pg"]),
+                       ("acg", f32["acg"], f64["acg"])):
+        field_err[name] = {"rel_l2": float(np.linalg.norm(a - b) / np.linalg.norm(b)),
+                           "max_abs": float(np.abs(a - b).max())}
+    log("bench: mixed vs FP64 after the same steps, relative L2: " +
+        ", ".join(f"{k} {v['rel_l2']:.1e}" for k, v in field_err.items()))
+
     # kernel-level figures on the context stream (HIP events)
     ctx.set_state(prob.fields)
     S.begin_step(ctx)
@@ -576,6 +591,7 @@ def main():
         "speedup_mixed_vs_fp64": res64["newton_ms"] / res32["newton_ms"],
         "gmres_iters": {"mixed": res32["gmres_iters"], "fp64": res64["gmres_iters"]},
         "relres": {"mixed": res32["relres"], "fp64": res64["relres"]},
+        "field_error_mixed_vs_fp64": field_err,
         "assembly": {"vecmat_ms": asm_ms, "vec_ms": asm_vec_ms,
                      "melem_per_s": E / asm_ms / 1e3,
                      "fp64_tflops": asm_tf / 1e12, "frac_fp64_peak": asm_tf / PEAK_FP64},

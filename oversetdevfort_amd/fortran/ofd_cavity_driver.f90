@@ -9,7 +9,11 @@
 !
 ! usage: ofd_cavity_driver [n=8] [nsteps=2] [nnewton=2] [precision=0|1]
 ! Prints "NEWTON istep inewt |Ru| |Rp| iters relres", the drop-in checks
-! "SPMVNORM" / "GMRESNORM" and finally "FIELDNORMS |ug| |acg| |pg|".
+! "SPMVNORM" / "GMRESNORM", "FIELDNORMS |ug| |acg| |pg|", and then the
+! adjoint entries on the final state (no reference counterpart):
+! "ADJOINT |lam_u| |lam_p| |dF/du| |dF/dac| |dF/dp| restarts" for the
+! velocity load on the free dofs with the fine-scale residual form, and
+! "SPMVTNORM" for the transposed product of the assembled LHS.
 !======================================================================
 program ofd_cavity_driver
   use ofd_state
@@ -21,7 +25,8 @@ program ofd_cavity_driver
                           dflag(:), pflag(:)
   real(8), allocatable :: xg(:, :), zero3(:, :), zero1(:), bcval(:, :), ug(:, :), &
                           acg(:, :), pg(:), k11(:, :), g(:, :), d1(:, :), m(:), &
-                          ru(:, :), rp(:), yu(:, :), yp(:), su(:, :), sp(:)
+                          ru(:, :), rp(:), yu(:, :), yp(:), su(:, :), sp(:), &
+                          lu(:, :), lp(:), gu(:, :), gac(:, :), gp(:), gm(:)
   real(8) :: prm(OFD_NPARAMS), rhoinf, almi, alfi, gami, beti, relres, res(2), h
   character(len=32) :: arg
   integer :: di(8), dj(8), dk(8)
@@ -149,6 +154,23 @@ program ofd_cavity_driver
   call ofd_check(ofd_get_field(ofd_ctx, OFD_PG, pg), 'get pg')
   write (*, '(a,3es25.16)') 'FIELDNORMS ', sqrt(sum(ug*ug)), sqrt(sum(acg*acg)), &
     sqrt(sum(pg*pg))
+
+  ! ---- adjoint on the final state (no reference counterpart) ----
+  allocate (lu(NNODE, 3), lp(NNODE), gu(NNODE, 3), gac(NNODE, 3), gp(NNODE), gm(NNODE))
+  call ofd_check(ofd_set_residual_form(ofd_ctx, 1), 'residual form')
+  call assembleNavSto(OFD_ASSEMBLE_VEC + OFD_ASSEMBLE_MAT, 1)
+  su = ug
+  where (IBC(:, 1:3) == 1) su = 0.0d0
+  sp = 0.0d0
+  call ofd_check(ofd_solve_adjoint(ofd_ctx, 1, OFD_FP64, OFD_ORTHO_CGS1, 1.0d-10, 1.0d-3, &
+                                   30, 20, su, sp, iters, kk, relres), 'adjoint')
+  call ofd_check(ofd_get_field(ofd_ctx, OFD_LAMU, lu), 'lambda')
+  call ofd_check(ofd_get_field(ofd_ctx, OFD_LAMP, lp), 'lambda')
+  call ofd_check(ofd_vjp_ns(ofd_ctx, 1, lu, lp, gu, gac, gp, gm), 'VJP')
+  write (*, '(a,5es25.16,i4)') 'ADJOINT ', sqrt(sum(lu*lu)), sqrt(sum(lp*lp)), &
+    sqrt(sum(gu*gu)), sqrt(sum(gac*gac)), sqrt(sum(gp*gp)), kk
+  call ofd_check(ofd_spmv_up_t(ofd_ctx, OFD_FP64, su, lp, yu, yp), 'transposed product')
+  write (*, '(a,2es25.16)') 'SPMVTNORM ', sqrt(sum(yu*yu)), sqrt(sum(yp*yp))
   kk = 0
   call ofd_state_final()
 end program ofd_cavity_driver

@@ -1,7 +1,9 @@
 """The Fortran driver (ISO_C_BINDING -> libofd.so) against the Python
 binding of the same C ABI: the drop-in SparseProdUP_3D / SparseGMRES_up
-wrappers and two generalized-alpha steps of the cavity must give the same
-numbers (both run the same deterministic kernels in the same order)."""
+wrappers, two generalized-alpha steps of the cavity, and the adjoint
+entries (flexible-GMRES adjoint solve, element VJP, transposed product) on
+the final state must give the same numbers (both run the same
+deterministic kernels in the same order)."""
 import os
 import subprocess
 
@@ -24,8 +26,10 @@ def _parse(out):
             continue
         if f[0] == "NEWTON":
             rec["NEWTON"].append([float(v) for v in f[3:5]])
-        elif f[0] in ("SPMVNORM", "GMRESNORM", "FIELDNORMS"):
+        elif f[0] in ("SPMVNORM", "GMRESNORM", "FIELDNORMS", "SPMVTNORM"):
             rec[f[0]] = [float(v) for v in f[1:]]
+        elif f[0] == "ADJOINT":
+            rec[f[0]] = [float(v) for v in f[1:6]]
     return rec
 
 
@@ -59,9 +63,22 @@ def _python_path(n, nsteps, nnewton):
             ctx.assembleNavSto(2, 1)
             rec["NEWTON"].append(list(ctx.rhs_norms()))
         ctx.advance()
-    rec["FIELDNORMS"] = [np.linalg.norm(ctx.get_field("UG")),
-                         np.linalg.norm(ctx.get_field("ACG")),
+    ug = ctx.get_field("UG")
+    rec["FIELDNORMS"] = [np.linalg.norm(ug), np.linalg.norm(ctx.get_field("ACG")),
                          np.linalg.norm(ctx.get_field("PG"))]
+    # the driver's adjoint section: velocity load on the free dofs, fine-scale
+    # residual form, flexible GMRES, element VJP, transposed product
+    ctx.set_residual_form(1)
+    ctx.assembleNavSto(6, 1)
+    su = ug * (prob.ibc[:, :3] == 0)
+    sp = np.zeros(prob.mesh.nnode)
+    res = ctx.solve_adjoint(su, sp, rtol=1e-10, precision=L.OFD_FP64, ortho=L.OFD_ORTHO_CGS1,
+                            inner_tol=1e-3, kspace=30, max_cycles=20)
+    g = ctx.vjp_ns(res["lamu"], res["lamp"])
+    rec["ADJOINT"] = [np.linalg.norm(res["lamu"]), np.linalg.norm(res["lamp"]),
+                      np.linalg.norm(g["u"]), np.linalg.norm(g["ac"]), np.linalg.norm(g["p"])]
+    yu, yp = ctx.SparseProdUP_3D_T(su, res["lamp"])
+    rec["SPMVTNORM"] = [np.linalg.norm(yu), np.linalg.norm(yp)]
     ctx.close()
     return rec
 
@@ -76,7 +93,7 @@ def test_fortran_driver_matches_python_binding():
     assert p.returncode == 0, p.stdout + p.stderr
     f = _parse(p.stdout)
     py = _python_path(n, nsteps, nnewton)
-    for key in ("SPMVNORM", "GMRESNORM", "FIELDNORMS"):
+    for key in ("SPMVNORM", "GMRESNORM", "FIELDNORMS", "ADJOINT", "SPMVTNORM"):
         np.testing.assert_allclose(f[key], py[key], rtol=1e-13, atol=0, err_msg=key)
     assert len(f["NEWTON"]) == nsteps * nnewton
     np.testing.assert_allclose(np.array(f["NEWTON"]), np.array(py["NEWTON"]), rtol=1e-12,
