@@ -188,6 +188,12 @@ def quench(n=48, seed=1, rhoinf=0.5, jitter=0.0, dims=None) -> Problem:
         wall |= faces[k]
     ibc = np.zeros((N, 8), np.int32)
     ibc[wall, :3] = 1                # no-slip tank walls; open top
+    # nodes inside the solid block: the NS + VOF element skips solid
+    # elements, so a node no fluid element touches is held in u, p and phi
+    # (BCLhs_NS_conv columns 1-3, 7, 8)
+    wet = np.zeros(N, bool)
+    wet[mesh.ien[~solid].ravel()] = True
+    ibc[np.ix_(~wet, [0, 1, 2, 6, 7])] = 1
     f = _zero_fields(N)
     f.update(phig=phi.copy(), phigold=phi.copy(), rphig=np.zeros(N), rphigold=np.zeros(N),
              Tg=T.copy(), Tgold=T.copy(), rTg=np.zeros(N), rTgold=np.zeros(N))

@@ -54,6 +54,20 @@ __device__ __forceinline__ double red8t(double v) {
   return v;
 }
 
+// Gauss-point value summed over the element's 8 nodes in node order, as the
+// forward kernel forms it (the order of the reference's e3int_qr).  Used
+// for the values compared against a threshold (the VOF field clamped to
+// [0, 1], T against T_sat): inside one phase every nodal value is exactly 0
+// or 1, and a tree-ordered sum can land one rounding on the other side of
+// the clamp and flip the branch.
+__device__ __forceinline__ double seq8t(double v) {
+  const int b = (int)(threadIdx.x & 63) & ~7;
+  double s = 0.0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) s = s + __shfl(v, b + j);
+  return s;
+}
+
 __device__ __forceinline__ double interp(double a, double b, double phi) {
   const double pc = fmax(0.0, fmin(1.0, phi));
   return a * (1.0 - pc) + b * pc;
@@ -156,8 +170,8 @@ k_vjp_tem_hex(int N, int E, const int32_t *__restrict__ ien, const int32_t *__re
                      Ji[2 + 3 * i] * Ji[2 + 3 * j];
 
   // Gauss-point values
-  const double phii = red8t(phia * Na);
-  const double Ti = red8t(Ta * Na);
+  const double phii = seq8t(phia * Na);
+  const double Ti = seq8t(Ta * Na);
   const double rTi = red8t(rTa * Na);
   double uadv[3], dT[3];
 #pragma unroll

@@ -40,6 +40,19 @@ __device__ __forceinline__ double s8(double v) {
   return v;
 }
 
+// Gauss-point value summed over the element's 8 nodes in node order, the
+// order of the reference's e3int_qr.  Used for the values compared against a
+// threshold (the VOF field clamped to [0, 1], T against T_sat): inside one
+// phase every nodal value is exactly 0 or 1, and a tree-ordered sum can land
+// one rounding on the other side of the clamp and flip the branch.
+__device__ __forceinline__ double seq8(double v) {
+  const int b = (int)(threadIdx.x & 63) & ~7;
+  double s = 0.0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) s = s + __shfl(v, b + j);
+  return s;
+}
+
 __device__ __forceinline__ double pinterp(double a, double b, double phi) {
   const double pc = fmax(0.0, fmin(1.0, phi));
   return a * (1.0 - pc) + b * pc;
@@ -151,7 +164,7 @@ k_assemble_nsvof_hex(int E, const int32_t *__restrict__ ien, const int32_t *__re
     dp[j] = s8(pa * ga[j]);
     dphi[j] = s8(phia * ga[j]);
   }
-  const double pri = s8(pa * Na), phii = s8(phia * Na), Ti = s8(Ta * Na);
+  const double pri = s8(pa * Na), phii = seq8(phia * Na), Ti = seq8(Ta * Na);
   const double rphii = s8(rphia * Na);
 
   // phase change and properties (:343-366)

@@ -36,6 +36,19 @@ __device__ __forceinline__ double sum8(double v) {
   return v;
 }
 
+// Gauss-point value summed over the element's 8 nodes in node order, the
+// order of the reference's e3int_qr.  Used for the values compared against a
+// threshold (the VOF field clamped to [0, 1], T against T_sat): inside one
+// phase every nodal value is exactly 0 or 1, and a tree-ordered sum can land
+// one rounding on the other side of the clamp and flip the branch.
+__device__ __forceinline__ double seq8(double v) {
+  const int b = (int)(threadIdx.x & 63) & ~7;
+  double s = 0.0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) s = s + __shfl(v, b + j);
+  return s;
+}
+
 __device__ __forceinline__ double prop_interp(double a, double b, double phi) {
   const double pc = fmax(0.0, fmin(1.0, phi));
   return a * (1.0 - pc) + b * pc;
@@ -132,8 +145,8 @@ k_assemble_tem_hex(int E, const int32_t *__restrict__ ien, const int32_t *__rest
                      Ji[2 + 3 * i] * Ji[2 + 3 * j];
 
   // Gauss-point values (e3int_qr, e3int_qr_grad)
-  const double phii = sum8(phia * Na);
-  const double Ti = sum8(Ta * Na);
+  const double phii = seq8(phia * Na);
+  const double Ti = seq8(Ta * Na);
   const double rTi = sum8(rTa * Na);
   double uadv[3], dT[3];
 #pragma unroll
