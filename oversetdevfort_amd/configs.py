@@ -293,6 +293,11 @@ def overset(nb=(96, 48, 48), nc=32, seed=1, rhoinf=0.5, theta=None) -> Problem:
     zw[:Nb] = faces["zlo"] | faces["zhi"]
     ibc[yw, 1] = 1
     ibc[zw, 2] = 1
+    # the fringe encloses the component with velocity data only
+    # (setBCs.f90:158-165), which leaves its pressure level free: it is held
+    # at the node nearest the component's centre, as the cavity's is
+    pin = Nb + int(np.argmin(np.linalg.norm(comp.xg, axis=1)))
+    ibc[pin, 3] = 1
     bcval = np.zeros((N, 3))
     bcval[inflow, 0] = 1.0
     f = _zero_fields(N)
@@ -306,5 +311,6 @@ def overset(nb=(96, 48, 48), nc=32, seed=1, rhoinf=0.5, theta=None) -> Problem:
              grav=(0.0, 0.0, -9.81))
     prob = Problem("overset", mesh, ibc, bcval, f, p)
     prob.fringe = {"component": (comp_bnd, 1), "background": (bg_fr, 2)}
+    prob.pressure_pin = pin
     prob.theta = float(theta)
     return prob

@@ -599,15 +599,52 @@ def begin_step(ctx: Context):
 
 
 def newton_step(ctx: Context, rtol=1e-8, precision=L.OFD_FP64, ortho=L.OFD_ORTHO_CGS2,
-                kspace=30, inner_tol=None, max_cycles=50, raise_noconv=True):
-    """One Newton iteration of solveOverset_stag for one block
-    (solveOverset.f90:55-121): assemble VEC+MAT, solve to rtol, update,
-    re-assemble VEC and return the residual norms."""
-    ctx.assembleNavSto(6, 1)
+                kspace=30, inner_tol=None, max_cycles=50, raise_noconv=True,
+                subdomain_flag=1):
+    """One Newton iteration of solveOverset_stag (solveOverset.f90:55-121) on
+    the elements of ``subdomain_flag``: assemble VEC+MAT, solve to rtol,
+    update, re-assemble VEC and return the residual norms."""
+    ctx.assembleNavSto(6, subdomain_flag)
     info = ctx.solve(rtol=rtol, precision=precision, ortho=ortho, kspace=kspace,
                      inner_tol=inner_tol, max_cycles=max_cycles,
                      raise_noconv=raise_noconv)
     ctx.newton_update()
-    ctx.assembleNavSto(2, 1)
+    ctx.assembleNavSto(2, subdomain_flag)
     info["res"] = ctx.rhs_norms()
     return info
+
+
+def fill_fringe(ctx: Context, fringe, xg):
+    """setBCs_overset's fringe step (setBCs.f90:124-167) for every block:
+    donor search and interpolation of ug in the source block
+    (interpolation_tet, interpolation.f90:5-76, or the hex extension), then
+    IBC(k,1:3) = 1, ug = the interpolated value and the consistent acg at
+    the receivers.  ``fringe``: {name: (0-based receiver nodes, source
+    ELM_ID)}.  Returns the donors per block."""
+    donors = {}
+    for name, (nodes, src) in fringe.items():
+        donor, val = ctx.interpolation(src, xg[nodes])
+        if np.any(donor <= 0):
+            raise RuntimeError(f"fringe {name}: {int(np.sum(donor <= 0))} receivers have no "
+                               "donor")
+        ctx.apply_fringe(np.asarray(nodes, np.int32) + 1, val)
+        donors[name] = donor
+    return donors
+
+
+def overset_step(ctx: Context, fringe, xg, nnewton=3, rtol=1e-8, precision=L.OFD_FP64,
+                 ortho=L.OFD_ORTHO_CGS1, kspace=30, inner_tol=None, max_cycles=400,
+                 raise_noconv=True):
+    """One time step of an overset pair coupled in one system: predictor
+    and Dirichlet values, then per Newton iteration the fringe of every
+    block from the other's latest iterate, one assembly launch over both
+    blocks (subdomain flags 1 | 2) and one solve of the coupled system.
+    Returns the per-iteration solver information."""
+    begin_step(ctx)
+    out = []
+    for _ in range(nnewton):
+        fill_fringe(ctx, fringe, xg)
+        out.append(newton_step(ctx, rtol=rtol, precision=precision, ortho=ortho,
+                               kspace=kspace, inner_tol=inner_tol, max_cycles=max_cycles,
+                               raise_noconv=raise_noconv, subdomain_flag=3))
+    return out
