@@ -319,33 +319,39 @@ def adjoint_c5(L, C, S, nsteps=20, every=5):
     return out
 
 
-def overset_steps(L, C, S, nsteps=10, nnewton=3):
+def overset_steps(L, C, S, nsteps=10, nnewton=3, kspace=64):
     """The overset configuration end to end: the 96x48x48 background and the
     rotated 32^3 component coupled in one system, Re = 500 cross-flow,
     dt = 0.005; each Newton iteration fills both fringes (donor search and
     interpolation), assembles both blocks in one launch and solves the
-    coupled system with the mixed solver to a 1e-8 true FP64 residual."""
+    coupled system with the mixed solver to a 1e-8 true FP64 residual.
+    Restarted GMRES(30) stalls near 1e-7 on the later Newton iterations of
+    the coupled system (the fringe refill leaves a residual that block
+    Jacobi barely touches); GMRES(64) reaches 1e-8 in every solve."""
     prob = C.overset()
     E, N = prob.mesh.nelem, prob.mesh.nnode
     ctx = S.setup(prob, device=0)
     S.overset_step(ctx, prob.fringe, prob.mesh.xg, nnewton=1, precision=L.OFD_FP32,
-                   raise_noconv=False)                   # warm-up, then restart
+                   kspace=kspace, raise_noconv=False)    # warm-up, then restart
     ctx.set_state(prob.fields)
-    step_ms, iters, rel, res = [], [], [], []
+    step_ms, iters, cycles, rel, res = [], [], [], [], []
     for _ in range(nsteps):
         t0 = time.perf_counter()
         info = S.overset_step(ctx, prob.fringe, prob.mesh.xg, nnewton=nnewton,
-                              precision=L.OFD_FP32, raise_noconv=False)
+                              precision=L.OFD_FP32, kspace=kspace, raise_noconv=False)
         ctx.advance()
         step_ms.append(1e3 * (time.perf_counter() - t0))
         iters.append([i["iters"] for i in info])
+        cycles.append([i["cycles"] for i in info])
         rel.append(max(i["relres"] for i in info))
         res.append(info[-1]["res"][0])
     ctx.close()
     out = {"workload": "overset 96x48x48 background + rotated 32^3 component, coupled, "
-                       f"{nsteps} steps x {nnewton} Newton iterations, mixed, rtol 1e-8",
+                       f"{nsteps} steps x {nnewton} Newton iterations, mixed, "
+                       f"GMRES({kspace}) to rtol 1e-8",
            "elements": E, "nodes": N, "ms_per_step": sum(step_ms) / nsteps,
-           "step_ms": step_ms, "gmres_iters": iters, "max_relres": max(rel),
+           "step_ms": step_ms, "gmres_iters": iters, "gmres_cycles": cycles,
+           "max_relres": max(rel), "all_converged": bool(max(rel) <= 1e-8),
            "final_res_u": res}
     log(f"bench: overset {nsteps} steps: {out['ms_per_step']:.0f} ms per step, max relres "
         f"{max(rel):.1e}, |Ru| after the last step {res[-1]:.2e}")

@@ -100,6 +100,16 @@ class Context:
         nc = min(b.shape[1], 8)
         a = np.ascontiguousarray(b[:, :nc].T)
         L.check(self.lib.ofd_set_ibc(self.h, _ptr(a), nc))
+        self._ibc_bc = (a, nc)
+
+    def reset_ibc(self):
+        """IBC back to the flags last given to set_ibc, dropping the fringe
+        receivers flagged since: IBC(:,:) = 0 then setBCs_CFD, as each
+        setBCs_overset call starts (solveOverset.f90:33-34, setBCs.f90:134)."""
+        if getattr(self, "_ibc_bc", None) is None:
+            raise RuntimeError("reset_ibc before set_ibc")
+        a, nc = self._ibc_bc
+        L.check(self.lib.ofd_set_ibc(self.h, _ptr(a), nc))
 
     def set_bcvals(self, vals):
         a = _soa(vals)
@@ -639,7 +649,11 @@ def overset_step(ctx: Context, fringe, xg, nnewton=3, rtol=1e-8, precision=L.OFD
     and Dirichlet values, then per Newton iteration the fringe of every
     block from the other's latest iterate, one assembly launch over both
     blocks (subdomain flags 1 | 2) and one solve of the coupled system.
+    The physical Dirichlet values go only on the physical flags: the
+    receivers flagged by the previous step's fringe are cleared first, so
+    they keep their interpolated velocity into the next donor search.
     Returns the per-iteration solver information."""
+    ctx.reset_ibc()
     begin_step(ctx)
     out = []
     for _ in range(nnewton):
