@@ -163,8 +163,10 @@ int ofd_gmres_up(ofd_ctx *ctx, int precision, int ortho, double tol,
 
 /* Restarted GMRES with an FP64 outer residual (iterative refinement):
  * repeat GMRES(kspace) cycles in `precision` on r = b - A64 x until
- * |b - A64 x|_2 <= rtol*|b|_2 or max_cycles.  x stays in the context
- * (OFD_SOLU/OFD_SOLP).  iters = total Arnoldi iterations. */
+ * |b - A64 x|_2 <= rtol*|b|_2 or max_cycles.  A cycle stops early once its
+ * preconditioned residual has fallen by inner_tol; a loose inner_tol (1e-2)
+ * restarts often and can need several times the cycles of 1e-4.  x stays in
+ * the context (OFD_SOLU/OFD_SOLP).  iters = total Arnoldi iterations. */
 int ofd_solve(ofd_ctx *ctx, int precision, int ortho, double rtol,
               double inner_tol, int kspace, int max_cycles, int *iters,
               int *cycles, double *relres);

@@ -212,7 +212,7 @@ k_gmres_init(int N, int nsc, const double *__restrict__ P, const double *__restr
 // unscaled product z = (P^-1 A) W_k (FP64 basis):
 //   j = 0:  close step k-1 first (partials in `rd`)
 //   j > 0:  beta = <z, v_{j-1}> from the previous sweep's partials (kept
-//           unscaled in H(j-1,k)); z -= beta v_{j-1}
+//           in h1(j-1), h2(j-1) = 0); z -= beta v_{j-1}
 //   partials of <z, W_j> (j <= k) or <z, z> (j = k+1) into `wr`
 __global__ void __launch_bounds__(256)
 k_mgs_step(int N, double *__restrict__ W, const double *__restrict__ rd,
@@ -224,11 +224,14 @@ k_mgs_step(int N, double *__restrict__ W, const double *__restrict__ rd,
   double *z = W + vs * (k + 1);
   double cp = 0.0;
   if (j == 0) {
-    if (arnoldi_close(g, rd, nb, k - 1, 0, hflag)) return;
+    if (arnoldi_close(g, rd, nb, k - 1, hflag)) return;
   } else {
     final_sums(rd, nb, 1, s_b);
     const double beta = s_b[0] * g->s[j - 1];
-    if (blockIdx.x == 0 && threadIdx.x == 0) g->H[(j - 1) + (g->kspace + 1) * k] = beta;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      g->h1[j - 1] = beta;
+      g->h2[j - 1] = 0.0;
+    }
     cp = beta * g->s[j - 1];
   }
   const double *vp = W + vs * (j > 0 ? j - 1 : 0);
@@ -277,7 +280,7 @@ k_cgs_dots(int N, const V *__restrict__ W, const double *__restrict__ rd,
   zr.load(z + (size_t)ii * 4);
 #pragma unroll
   for (int t = 0; t < kCh; ++t) vr[t].load(W + vs * (t < m ? t : m - 1) + (size_t)ii * 4);
-  if (arnoldi_close(g, rd, nb, k - 1, 1, hflag)) return;
+  if (arnoldi_close(g, rd, nb, k - 1, hflag)) return;
   double zz[4];
   zr.get(zz);
   {
@@ -359,10 +362,9 @@ k_cgs_update(int N, V *__restrict__ W, const double *__restrict__ rd_dots,
 
 // closes the last Arnoldi step of a cycle (one block)
 __global__ void __launch_bounds__(256)
-k_gmres_close(const double *__restrict__ part, int nb, GmresState *__restrict__ g, int k,
-              int cgs) {
+k_gmres_close(const double *__restrict__ part, int nb, GmresState *__restrict__ g, int k) {
   if (g->done) return;
-  arnoldi_close(g, part, nb, k, cgs, nullptr);
+  arnoldi_close(g, part, nb, k, nullptr);
 }
 
 // |v|^2 partials of a [N][4] vector
@@ -582,9 +584,9 @@ hipError_t launch_cgs_update(Ctx &c, int k, int mode) {
 }
 
 hipError_t launch_gmres_close(Ctx &c, int k, int ortho) {
+  (void)ortho;
   hipLaunchKernelGGL(k_gmres_close, dim3(1), dim3(kRedBlock), 0, c.stream,
-                     part_buf(c, c.part_cur), nblocks_gm(c), c.d_gs, k,
-                     ortho == OFD_ORTHO_MGS ? 0 : 1);
+                     part_buf(c, c.part_cur), nblocks_gm(c), c.d_gs, k);
   return hipGetLastError();
 }
 

@@ -109,16 +109,23 @@ __device__ __forceinline__ double sum1(const double *part, int nb) {
 
 // Close Arnoldi step k (SparseGMRES_NS.f90:289-327).  The product of step k
 // is stored unscaled, z = (P^-1 A) W_k = (P^-1 A v_k) / s_k, so every
-// coefficient found for z is multiplied by s_k here: H(j,k) = s_k h_j for
-// j <= k (h1 + h2 for CGS2, the MGS betas otherwise) and
+// coefficient found for z is multiplied by s_k here: H(j,k) = s_k (h1_j + h2_j)
+// for j <= k (the CGS projections, or the MGS betas in h1 with h2 = 0) and
 // H(k+1,k) = s_k |z_perp| from the partials of |z_perp|^2.  Then the previous
 // rotations are applied to column k, the new one is formed, e is rotated and
 // |e_{k+1}| <= eps and k+1 >= kmin is tested.  k = -1 closes the initial
 // residual instead (e_0 = |P^-1 b|, eps).  Returns the convergence flag;
 // *s_next (if given) receives s_{k+1} in every block.
+//
+// Every block reads only what earlier kernels wrote: h1, h2, s_k, the
+// rotations q < k and ec[k] (e_k before this rotation; e_{k+1} is still the
+// zero the cycle started with).  Block 0 writes H(:,k), rc/rs[k], e[k],
+// e[k+1], ec[k+1] and s[k+1], none of which another block of the kernel
+// reads, so a block that starts after block 0 has finished (a grid larger
+// than one round of resident blocks) takes the same decision.
 __device__ __forceinline__ bool arnoldi_close(GmresState *__restrict__ g,
                                               const double *__restrict__ part, int nb,
-                                              int k, int cgs, int32_t *hflag,
+                                              int k, int32_t *hflag,
                                               double *s_next = nullptr) {
   __shared__ double s_res[2];
   __shared__ double s_h[kMaxKspace + 1];   // s_k h_q, q <= k
@@ -131,7 +138,7 @@ __device__ __forceinline__ bool arnoldi_close(GmresState *__restrict__ g,
   const int K1 = g->kspace + 1;
   double colq = 0.0, rcq = 0.0, rsq = 0.0, sk = 0.0;
   if (k >= 0 && t <= k) {
-    colq = cgs ? g->h1[t] + g->h2[t] : g->H[t + K1 * k];
+    colq = g->h1[t] + g->h2[t];
     sk = g->s[k];
     if (t < k) {
       rcq = g->rc[t];
@@ -142,8 +149,7 @@ __device__ __forceinline__ bool arnoldi_close(GmresState *__restrict__ g,
   int kmin = 0;
   if (t == 0) {
     if (k >= 0) {
-      e0 = g->e[k];
-      e1 = g->e[k + 1];
+      e0 = g->ec[k];
       eps = g->eps;
       kmin = g->kmin;
     } else {
@@ -169,6 +175,7 @@ __device__ __forceinline__ bool arnoldi_close(GmresState *__restrict__ g,
         g->rnorm0 = nrm;
         g->eps = tol * nrm;
         g->e[0] = nrm;
+        g->ec[0] = nrm;
         g->s[0] = sc;
         g->resid = nrm;
         g->iters = 0;
@@ -198,6 +205,7 @@ __device__ __forceinline__ bool arnoldi_close(GmresState *__restrict__ g,
         g->rs[k] = rs;
         g->e[k] = ek;
         g->e[k + 1] = ek1;
+        g->ec[k + 1] = ek1;
         g->s[k + 1] = sc;
         g->resid = er;
         g->iters = k + 1;

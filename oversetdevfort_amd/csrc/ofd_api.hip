@@ -377,6 +377,10 @@ int ofd_ctx_create(int device, ofd_ctx **out) {
   OFD_CHECK(x, OFD_ERR_ALLOC, "out of host memory");
   Ctx &c = x->c;
   c.device = device;
+  if (hipDeviceGetAttribute(&c.n_cu, hipDeviceAttributeMultiprocessorCount, device) !=
+          hipSuccess ||
+      c.n_cu < 1)
+    c.n_cu = 256;
   hipError_t e = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete x;

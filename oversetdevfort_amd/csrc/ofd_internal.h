@@ -50,6 +50,8 @@ constexpr int kMaxRedBlocks = 4096;  // node blocks of 256: N <= 1,048,576
 struct GmresState {
   double H[(kMaxKspace + 1) * kMaxKspace];  // column-major (K+1) x K
   double e[kMaxKspace + 1];
+  double ec[kMaxKspace + 1];  // e_k as the close of step k-1 left it: the close of
+                             // step k reads it here, never in e (which it rotates)
   double rc[kMaxKspace], rs[kMaxKspace];
   double s[kMaxKspace + 1];   // scale of stored basis vector j (v_j = s_j W_j)
   double h1[kMaxKspace + 1], h2[kMaxKspace + 1];
@@ -71,6 +73,7 @@ struct GmresState {
 
 struct Ctx {
   int device = 0;
+  int n_cu = 256;                     // compute units of the device
   hipStream_t stream = nullptr;
   // mesh
   int nnode = 0, nelem = 0, nshl = 0;

@@ -108,10 +108,12 @@ k_spmv(int N, const int32_t *__restrict__ col, const int32_t *__restrict__ row,
   // entries, share one L2 instead of every L2 fetching all of x.
   const int per = gridDim.x >> 3;
   const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-  const int gt = lb * blockDim.x + threadIdx.x;
-  const int r = gt >> 2;
-  const int c = gt & 3;
-  if (r >= N) return;
+  // rows are spread evenly over the grid: rpb <= 64 per 256-thread block
+  const int rpb = (N + (int)gridDim.x - 1) / (int)gridDim.x;
+  const int rb = threadIdx.x >> 2;
+  const int r = lb * rpb + rb;
+  const int c = threadIdx.x & 3;
+  if (rb >= rpb || r >= N) return;
   const int k0 = col[r], k1 = col[r + 1];
   double acc = 0.0;
 #pragma unroll 4
@@ -402,10 +404,17 @@ hipError_t launch_convert_a32(Ctx &c) {
   return hipGetLastError();
 }
 
-// 4 lanes per row, 256-thread blocks, rounded up to a multiple of 8 (one
-// share per XCD; blocks past the last row return at once)
+// 4 lanes per row, 256-thread blocks: at least ceil(N/64) blocks, rounded up
+// to whole rounds over the compute units, so every CU holds the same number
+// of blocks and the rows are spread evenly over them (k_spmv takes
+// ceil(N/grid) <= 64 rows per block).  ceil(N/64) blocks as such leave some
+// CUs one block more than the others, and the product takes as long as
+// those do: 46.2 -> 43.3 us for the FP64 Arnoldi product of the 64x32x32
+// channel.  A multiple of 8 (one share per XCD).
 static inline unsigned spmv_blocks(const Ctx &c) {
-  const unsigned nb = (unsigned)(((size_t)c.nnode * 4 + 255) / 256);
+  const unsigned need = (unsigned)(((size_t)c.nnode * 4 + 255) / 256);
+  const unsigned ncu = (unsigned)c.n_cu;
+  const unsigned nb = ((need + ncu - 1) / ncu) * ncu;
   return (nb + 7u) & ~7u;
 }
 

@@ -211,8 +211,9 @@ subroutine solveNS_resident(rtol, kspace, iters, relres)
   real(8), intent(out) :: relres
   integer(c_int) :: it, cyc
   real(c_double) :: rel, itol
-  itol = 1.0d-2
-  if (ofd_precision == OFD_FP32) itol = 1.0d-4
+  ! a cycle stops once its preconditioned residual has fallen by itol;
+  ! looser values restart early and cost cycles (both precisions)
+  itol = 1.0d-4
   call ofd_check(ofd_solve(ofd_ctx, ofd_precision, OFD_ORTHO_CGS1, rtol, itol, &
                            int(kspace, c_int), 100_c_int, it, cyc, rel), &
                  'solveNS_resident')

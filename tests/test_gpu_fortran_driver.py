@@ -58,7 +58,7 @@ def _python_path(n, nsteps, nnewton):
         for _ in range(nnewton):
             ctx.assembleNavSto(6, 1)
             ctx.solve(rtol=1e-10, precision=L.OFD_FP64, ortho=L.OFD_ORTHO_CGS1,
-                      inner_tol=1e-2, kspace=30, max_cycles=100)
+                      inner_tol=1e-4, kspace=30, max_cycles=100)
             ctx.newton_update()
             ctx.assembleNavSto(2, 1)
             rec["NEWTON"].append(list(ctx.rhs_norms()))
