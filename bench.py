@@ -489,6 +489,10 @@ def main():
     ap.add_argument("--ortho", type=str, default="cgs1", choices=["mgs", "cgs2", "cgs1"])
     ap.add_argument("--probe-every", type=int, default=32,
                     help="sample every n-th Arnoldi product launch with HIP events")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the channel Newton steps and their kernel figures "
+                         "(for profiled runs); no quench, fringe, overset, adjoint "
+                         "or host sections")
     ap.add_argument("--pmc-sample", action="store_true",
                     help="short run for hardware-counter passes: a 256 MiB copy "
                          "(calibration) and one full GMRES(30) cycle per precision")
@@ -571,17 +575,20 @@ def main():
         f"{asm_tf / 1e12:.2f} TFLOP/s); VEC {asm_vec_ms:.3f} ms")
     log(f"bench: SpMV FP64 {spmv64_ms * 1e3:.1f} us {bw64 / 1e9:.0f} GB/s; "
         f"FP32-matrix {spmv32_ms * 1e3:.1f} us {bw32 / 1e9:.0f} GB/s")
-    adjoint = adjoint_ns(ctx, L, prob, asm_vec_ms)
-    adjoint["newton_step_fp64_ms"] = res64["newton_ms"]
+    adjoint = quench = fringe = None
+    if not args.headline_only:
+        adjoint = adjoint_ns(ctx, L, prob, asm_vec_ms)
+        adjoint["newton_step_fp64_ms"] = res64["newton_ms"]
     ctx.close()
 
-    quench, qprob = quench_kernels(L, C, S)
-    fringe = fringe_search(C, S)
-    fringe["coupled_steps"] = overset_steps(L, C, S)
-    adjoint["quench_transient"] = adjoint_c5(L, C, S)
+    if not args.headline_only:
+        quench, qprob = quench_kernels(L, C, S)
+        fringe = fringe_search(C, S)
+        fringe["coupled_steps"] = overset_steps(L, C, S)
+        adjoint["quench_transient"] = adjoint_c5(L, C, S)
 
     cpu = None
-    if not args.no_cpu:
+    if not (args.no_cpu or args.headline_only):
         log("bench: host baseline (C oracle, 1 thread) ...")
         c = cpu_baseline(prob)
         # the full solve takes as many iterations as the same FP64 algorithm
