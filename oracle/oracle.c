@@ -1758,3 +1758,291 @@ int orc_gmres_ns_conv(int nnode, const int *col, const int *row, const double *k
   free(y);
   return iks;
 }
+
+/* ====================================================================
+ * Boundary faces: FaceAssembly_NS_weak (FaceAssembly_weak.f90:4-193) and
+ * FaceAssembly_NS_outflow (FaceAssembly_weak.f90:197-409), both called by
+ * assembleNavSto (assembleNavSto.f90:54-60) after the volume loop.
+ * ==================================================================== */
+
+/* Face quadrature.  Tets: genGPandGWtri (genGP_GW.f90:436-473) gives the
+ * weights; genGPtritet (genGP_GW.f90:478-572) the images of the points in
+ * the tet for face orientation `fo` (1..4), ngp 1 or 3.  Hexes (no reference
+ * code; an extension): face fo (1..6) holds xi_c = -1 (fo odd) or +1 (fo
+ * even), c = (fo-1)/2; its 4 points take the two other coordinates
+ * (p, q) = ((c+1)%3, (c+2)%3) at (-a,-a), (a,-a), (-a,a), (a,a) with the
+ * volume rule's a = 0.57735026918963 and unit weights.  mgp[q*3+i]. */
+static int orc_face_gauss(int nshl, int fo, int ngp, double *mgp, double *gw) {
+  if (nshl == 4) {
+    const double t = 0.3333333333333333, s = 0.166666666666666666667,
+                 b = 0.666666666666666666667;
+    if (ngp == 1) {
+      static const int zero[5] = {-1, -1, 0, 1, 2};
+      if (fo < 1 || fo > 4) return -1;
+      for (int i = 0; i < 3; ++i) mgp[i] = (i == zero[fo]) ? 0.0 : t;
+      gw[0] = 0.5;
+      return 0;
+    }
+    if (ngp != 3) return -1;
+    /* rows of genGPtritet, face by face */
+    const double tab[4][3][3] = {{{s, b, s}, {s, s, b}, {b, s, s}},
+                                 {{0.0, s, s}, {0.0, s, b}, {0.0, b, s}},
+                                 {{b, 0.0, s}, {s, 0.0, b}, {s, 0.0, s}},
+                                 {{s, b, 0.0}, {b, s, 0.0}, {s, s, 0.0}}};
+    if (fo < 1 || fo > 4) return -1;
+    for (int q = 0; q < 3; ++q) {
+      for (int i = 0; i < 3; ++i) mgp[q * 3 + i] = tab[fo - 1][q][i];
+      gw[q] = 0.166666666666666666667;
+    }
+    return 0;
+  }
+  if (nshl == 8) {
+    if (fo < 1 || fo > 6 || ngp != 4) return -1;
+    const double a = 0.57735026918963;
+    const int c = (fo - 1) / 2, p = (c + 1) % 3, qq = (c + 2) % 3;
+    const double sc = (fo % 2) ? -1.0 : 1.0;
+    for (int q = 0; q < 4; ++q) {
+      mgp[q * 3 + c] = sc;
+      mgp[q * 3 + p] = (q & 1) ? a : -a;
+      mgp[q * 3 + qq] = (q & 2) ? a : -a;
+      gw[q] = 1.0;
+    }
+    return 0;
+  }
+  return -1;
+}
+
+/* Unit outward normal and area Jacobian DetJb of the face at one point.
+ * Tets: eval_faceshape_tet (eval_shape_FEM.f90:66-143): the face's three
+ * displaced nodes in the order of fo, lintrishl gradients
+ * (eval_shape_FEM.f90:261-282), lnor = dx/dxi1 x dx/dxi2.  Hexes (an
+ * extension): n = sc (dx/dxi_p x dx/dxi_q) for the face's fixed coordinate
+ * c at sc = -1/+1 and (p, q) = ((c+1)%3, (c+2)%3), which points out of an
+ * element with positive Jacobian. */
+static void orc_face_normal(int nshl, int fo, const double *pt, const double *xl,
+                            const double *dl, double nor[3], double *detjb) {
+  double t1[3], t2[3];
+  if (nshl == 4) {
+    static const int nodes[5][3] = {{0, 0, 0}, {1, 2, 0}, {3, 2, 1}, {0, 2, 3}, {1, 0, 3}};
+    const double gr[3][2] = {{1.0, 0.0}, {0.0, 1.0}, {-1.0, -1.0}};
+    double dlx[3][2] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+    for (int i = 0; i < 3; ++i) {
+      const int a = nodes[fo][i];
+      for (int dd = 0; dd < 3; ++dd) {
+        const double bx = xl[dd * nshl + a] + dl[dd * nshl + a];
+        dlx[dd][0] = dlx[dd][0] + bx * gr[i][0];
+        dlx[dd][1] = dlx[dd][1] + bx * gr[i][1];
+      }
+    }
+    for (int dd = 0; dd < 3; ++dd) {
+      t1[dd] = dlx[dd][0];
+      t2[dd] = dlx[dd][1];
+    }
+  } else {
+    double shl[8], shg[24];
+    orc_linhexshl(pt, shl, shg);
+    const int c = (fo - 1) / 2, p = (c + 1) % 3, q = (c + 2) % 3;
+    const double sc = (fo % 2) ? -1.0 : 1.0;
+    for (int dd = 0; dd < 3; ++dd) {
+      double s1 = 0.0, s2 = 0.0;
+      for (int a = 0; a < 8; ++a) {
+        const double x = xl[dd * 8 + a] + dl[dd * 8 + a];
+        s1 += x * shg[p * 8 + a];
+        s2 += x * shg[q * 8 + a];
+      }
+      t1[dd] = sc * s1;
+      t2[dd] = s2;
+    }
+  }
+  nor[0] = t1[1] * t2[2] - t1[2] * t2[1];
+  nor[1] = t1[2] * t2[0] - t1[0] * t2[2];
+  nor[2] = t1[0] * t2[1] - t1[1] * t2[0];
+  *detjb = sqrt(nor[0] * nor[0] + nor[1] * nor[1] + nor[2] * nor[2]);
+  for (int i = 0; i < 3; ++i) nor[i] = nor[i] / *detjb;
+}
+
+/* e3bSTAB_weak (e3STAB.f90:256-287): tauB = 4 mu / h_n, h_n = 2 / |dxidx n|
+ * (the reference's formula for every element type), tauNor = tauB */
+static double orc_tau_weak(const double *dxidx, const double nor[3], double mui) {
+  double tt = 0.0;
+  for (int i = 0; i < 3; ++i) {
+    double s = 0.0;
+    for (int j = 0; j < 3; ++j) s += M3(dxidx, i, j) * nor[j];
+    tt += s * s;
+  }
+  const double hn = 2.0 / sqrt(tt);
+  return 4.0 * mui / hn;
+}
+
+int orc_face_ns(int nnode, int nelem, int nshl, const int *ien, const int *elm_id,
+                const int *elm_id_set, int subdomain_flag, int tensor_flag,
+                const double *xg, const int *ibc, const double *dgA, const double *ugA,
+                const double *ugmA, const double *pgA, const double *prm, int nface,
+                const int *f2e, const int *face_or, int ngb, const int *col,
+                const int *row, double *k11, double *g, double *d1, double *m,
+                double *rhsu, double *rhsp) {
+  const double rhoi = prm[5], mui = prm[6];
+  const double fact2 = prm[1] * prm[2] * prm[4];
+  const int n2 = nshl * nshl;
+  double xl[24], dl[24], ul[24], uml[24], pl[8];
+  double ke[9 * 64], ge[3 * 64], de[3 * 64], me[64], ru[24], rp[8];
+  double mgp[12], gw[4];
+  int ibcl[32];
+  int nneg = 0;
+  if (nshl != 4 && nshl != 8) return -1;
+  /* pass 0: FaceAssembly_NS_weak; pass 1: FaceAssembly_NS_outflow, which the
+   * reference runs on the same faces (BCugType == 2) and without the
+   * subdomain test */
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int f = 0; f < nface; ++f) {
+      const int e = f2e[f];
+      if (e < 0 || e >= nelem) return -1;
+      int tgt = 0;
+      if (subdomain_flag & 1) tgt = tgt || elm_id[e] == elm_id_set[0];
+      if (subdomain_flag & 2) tgt = tgt || elm_id[e] == elm_id_set[1];
+      for (int a = 0; a < nshl; ++a) {
+        const int n = ien[a * nelem + e];
+        for (int i = 0; i < 3; ++i) {
+          xl[i * nshl + a] = xg[i * nnode + n];
+          dl[i * nshl + a] = dgA[i * nnode + n];
+          ul[i * nshl + a] = ugA[i * nnode + n];
+          uml[i * nshl + a] = ugmA[i * nnode + n];
+        }
+        pl[a] = pgA[n];
+        for (int c = 0; c < 4; ++c) ibcl[c * nshl + a] = ibc[c * nnode + n];
+      }
+      memset(ke, 0, sizeof(double) * 9 * n2);
+      memset(ge, 0, sizeof(double) * 3 * n2);
+      memset(de, 0, sizeof(double) * 3 * n2);
+      memset(me, 0, sizeof(double) * n2);
+      memset(ru, 0, sizeof(double) * 3 * nshl);
+      memset(rp, 0, sizeof(double) * nshl);
+      if (orc_face_gauss(nshl, face_or[f], ngb, mgp, gw) != 0) return -1;
+      for (int q = 0; q < ngb; ++q) {
+        const double *pt = mgp + q * 3;
+        double shlu[8], shg[24], dxidx[9], Gij[9], Ginv[9], detj, nor[3], detjb;
+        orc_face_normal(nshl, face_or[f], pt, xl, dl, nor, &detjb);
+        if (orc_eval_shape_fem(nshl, pt, xl, dl, shlu, shg, dxidx, Gij, Ginv, &detj) > 0)
+          ++nneg;
+        /* e3int_qr / e3int_qr_grad (e3int.f90:18-47) */
+        double ui[3], umi[3], pri = 0.0, du[9];
+        for (int i = 0; i < 3; ++i) {
+          double su = 0.0, sm = 0.0;
+          for (int a = 0; a < nshl; ++a) {
+            su += ul[i * nshl + a] * shlu[a];
+            sm += uml[i * nshl + a] * shlu[a];
+          }
+          ui[i] = su;
+          umi[i] = sm;
+        }
+        for (int a = 0; a < nshl; ++a) pri += pl[a] * shlu[a];
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j) {
+            double s = 0.0;
+            for (int a = 0; a < nshl; ++a) s += ul[i * nshl + a] * shg[j * nshl + a];
+            M3(du, i, j) = s;
+          }
+        const double *gi = umi; /* the wall moves with the mesh */
+        double unor_rel = 0.0, unor = 0.0, gnor = 0.0;
+        for (int i = 0; i < 3; ++i) unor_rel += (ui[i] - umi[i]) * nor[i];
+        const double uneg = 0.5 * (unor_rel - fabs(unor_rel));
+        for (int i = 0; i < 3; ++i) gnor += gi[i] * nor[i];
+        for (int i = 0; i < 3; ++i) unor += ui[i] * nor[i];
+        if (pass == 1) {
+          /* FaceAssembly_weak.f90:344-381 */
+          const double dgw = detjb * gw[q];
+          if (tensor_flag & 2)
+            for (int a = 0; a < nshl; ++a)
+              for (int i = 0; i < 3; ++i)
+                ru[a * 3 + i] = ru[a * 3 + i] + shlu[a] * rhoi * uneg * (ui[i] - gi[i]) * dgw;
+          if (tensor_flag & 4)
+            for (int b = 0; b < nshl; ++b)
+              for (int a = 0; a < nshl; ++a) {
+                const double t2 = -(shlu[a] * uneg * rhoi * shlu[b]);
+                for (int i = 0; i < 3; ++i)
+                  ke[(b * nshl + a) * 9 + 4 * i] = ke[(b * nshl + a) * 9 + 4 * i] + fact2 * t2 * dgw;
+              }
+          continue;
+        }
+        if (!tgt) continue;
+        /* e3bSTAB_weak; the reference forms tauB in the VEC branch only and
+         * a MAT-only call reuses the last one formed -- here it is formed at
+         * every point */
+        const double tauB = orc_tau_weak(dxidx, nor, mui), tauNor = tauB;
+        if (tensor_flag & 2) {
+          /* e3bRHS_weak (e3RHS.f90:105-170), gmul = 1 */
+          double ti[3], t1[3], t2[9];
+          for (int k = 0; k < 3; ++k)
+            ti[k] = -pri * nor[k] +
+                    mui * (M3(du, k, 0) * nor[0] + M3(du, k, 1) * nor[1] + M3(du, k, 2) * nor[2] +
+                           M3(du, 0, k) * nor[0] + M3(du, 1, k) * nor[1] + M3(du, 2, k) * nor[2]);
+          for (int k = 0; k < 3; ++k)
+            t1[k] = -ti[k] + tauB * (ui[k] - gi[k]) - uneg * rhoi * (ui[k] - gi[k]) +
+                    (tauNor - tauB) * unor * nor[k];
+          for (int aa = 0; aa < 3; ++aa)
+            for (int bb = 0; bb < 3; ++bb)
+              M3(t2, aa, bb) = -(1.0 * mui * ((ui[aa] - gi[aa]) * nor[bb] + (ui[bb] - gi[bb]) * nor[aa]));
+          for (int a = 0; a < nshl; ++a) {
+            for (int i = 0; i < 3; ++i) {
+              double s = 0.0;
+              for (int j = 0; j < 3; ++j) s += shg[j * nshl + a] * M3(t2, i, j);
+              ru[a * 3 + i] = ru[a * 3 + i] - (shlu[a] * t1[i] + s) * detjb * gw[q];
+            }
+            rp[a] = rp[a] + shlu[a] * (unor - gnor) * detjb * gw[q];
+          }
+        }
+        if (tensor_flag & 4) {
+          /* e3bLHS_weak (e3LHS.f90:167-294), gmul = 1 */
+          double tn[8];
+          for (int a = 0; a < nshl; ++a)
+            tn[a] = shg[0 * nshl + a] * nor[0] + shg[1 * nshl + a] * nor[1] + shg[2 * nshl + a] * nor[2];
+          const double munor = tauNor - tauB;
+          const double w = detjb * gw[q];
+          for (int b = 0; b < nshl; ++b)
+            for (int a = 0; a < nshl; ++a) {
+              const double t2 = -shlu[a] * mui * tn[b] - 1.0 * tn[a] * mui * shlu[b] +
+                                shlu[a] * tauB * shlu[b] - shlu[a] * uneg * rhoi * shlu[b];
+              double *kk = ke + (size_t)(b * nshl + a) * 9;
+              for (int j = 0; j < 3; ++j)
+                for (int i = 0; i < 3; ++i) {
+                  const double A = shlu[a] * mui * shg[j * nshl + b] * nor[i];
+                  const double B = 1.0 * shg[i * nshl + a] * nor[j] * mui * shlu[b];
+                  const double C = shlu[a] * nor[j] * munor * nor[i] * shlu[b];
+                  const double v = (i == j) ? (t2 - A - B + C) : (-A - B + C);
+                  kk[i + 3 * j] = kk[i + 3 * j] + fact2 * v * detjb * gw[q];
+                }
+              for (int d = 0; d < 3; ++d) {
+                const double v = fact2 * shlu[a] * shlu[b] * nor[d] * detjb * gw[q];
+                de[(b * nshl + a) * 3 + d] = de[(b * nshl + a) * 3 + d] - v;
+                ge[(b * nshl + a) * 3 + d] = ge[(b * nshl + a) * 3 + d] + v;
+              }
+              (void)w;
+            }
+        }
+      }
+      orc_bclhs_3d(nshl, ibcl, ke, ge, de, me, ru, rp);
+      if (tensor_flag & 4) {
+        for (int a = 0; a < nshl; ++a) {
+          const int i = ien[a * nelem + e];
+          const int c0 = col[i], c1 = col[i + 1];
+          for (int b = 0; b < nshl; ++b) {
+            const int k = orc_sparse_loc(row, c0, c1, ien[b * nelem + e]);
+            const size_t ab = (size_t)b * nshl + a;
+            for (int q = 0; q < 9; ++q) k11[(size_t)k * 9 + q] += ke[ab * 9 + q];
+            for (int q = 0; q < 3; ++q) g[(size_t)k * 3 + q] += ge[ab * 3 + q];
+            for (int q = 0; q < 3; ++q) d1[(size_t)k * 3 + q] += de[ab * 3 + q];
+            m[k] += me[ab];
+          }
+        }
+      }
+      if (tensor_flag & 2) {
+        for (int b = 0; b < nshl; ++b) {
+          const int n = ien[b * nelem + e];
+          for (int i = 0; i < 3; ++i) rhsu[i * nnode + n] += ru[b * 3 + i];
+          rhsp[n] += rp[b];
+        }
+      }
+    }
+  }
+  return nneg;
+}

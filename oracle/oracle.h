@@ -74,6 +74,22 @@ int orc_assemble_ns(int nnode, int nelem, int nshl, int ngauss,
                     double *k11, double *g, double *d1, double *m, double *rhsu,
                     double *rhsp, double *cfl_out);
 
+/*
+ * FaceAssembly_NS_weak then FaceAssembly_NS_outflow (FaceAssembly_weak.f90)
+ * over the nface faces (f2e 0-based elements, face_or their orientations,
+ * ngb points per face), each with BCLhs_3D and the scatter, accumulated into
+ * the outputs (the reference adds them after IntElmAss_NS_overset).  Tets:
+ * the reference's face rule (ngb 1 or 3, face_or 1..4); hexes (an
+ * extension): face_or 1..6 = xi_1 -1/+1, xi_2 -1/+1, xi_3 -1/+1, ngb 4.
+ */
+int orc_face_ns(int nnode, int nelem, int nshl, const int *ien, const int *elm_id,
+                const int *elm_id_set, int subdomain_flag, int tensor_flag,
+                const double *xg, const int *ibc, const double *dgA, const double *ugA,
+                const double *ugmA, const double *pgA, const double *prm, int nface,
+                const int *f2e, const int *face_or, int ngb, const int *col,
+                const int *row, double *k11, double *g, double *d1, double *m,
+                double *rhsu, double *rhsp);
+
 /* one element only, before BC (for kernel-level tests) */
 int orc_element_ns(int nshl, int ngauss, const double *xl, const double *dl,
                    const double *ul, const double *uml, const double *acl,

@@ -89,6 +89,10 @@ def lib():
                                                                    ctypes.c_int] +
             [_ip, _ip] + [_dp] * 6 + [_dp])
         L.orc_assemble_ns.restype = ctypes.c_int
+        L.orc_face_ns.argtypes = ([ctypes.c_int] * 3 + [_ip] * 3 + [ctypes.c_int] * 2 +
+                                  [_dp, _ip] + [_dp] * 5 + [ctypes.c_int, _ip, _ip,
+                                                            ctypes.c_int, _ip, _ip] +
+                                  [_dp] * 6)
         L.orc_element_ns.argtypes = ([ctypes.c_int, ctypes.c_int] + [_dp] * 7 +
                                      [ctypes.c_void_p, _dp, ctypes.c_int, ctypes.c_int] +
                                      [_dp] * 7)
@@ -231,6 +235,38 @@ def assemble_ns(mesh, ibc, state, prm, col, row, tensor_flag=6,
     return dict(k11=k11.reshape(icnt, 9), g=g.reshape(icnt, 3),
                 d1=d1.reshape(icnt, 3), m=m, rhsu=rhsu.reshape(3, N).T.copy(),
                 rhsp=rhsp, cfl=float(cfl[0]), nneg=int(nneg))
+
+
+def face_ns(mesh, ibc, state, prm, col, row, faces, tensor_flag=6, subdomain_flag=1,
+            elm_id_set=(1, 2), ngb=None, out=None):
+    """FaceAssembly_NS_weak + FaceAssembly_NS_outflow over `faces` = (f2e
+    0-based elements, face_or), added into `out` (a dict as assemble_ns
+    returns; zeros if None).  ngb: points per face (hex 4, tet 1 or 3)."""
+    N, E, nshl = mesh.nnode, mesh.nelem, mesh.nshl
+    if ngb is None:
+        ngb = 4 if nshl == 8 else 3
+    icnt = row.size
+    if out is None:
+        out = dict(k11=np.zeros((icnt, 9)), g=np.zeros((icnt, 3)), d1=np.zeros((icnt, 3)),
+                   m=np.zeros(icnt), rhsu=np.zeros((N, 3)), rhsp=np.zeros(N))
+    k11 = _c(out["k11"]).ravel().copy()
+    g = _c(out["g"]).ravel().copy()
+    d1 = _c(out["d1"]).ravel().copy()
+    m = _c(out["m"]).copy()
+    rhsu = _c(np.asarray(out["rhsu"]).T).ravel().copy()
+    rhsp = _c(out["rhsp"]).copy()
+    f2e, fo = (np.ascontiguousarray(a, np.int32) for a in faces)
+    ibcf = _c(np.asarray(ibc)[:, :4].T, np.int32).ravel()
+    r = lib().orc_face_ns(
+        N, E, nshl, _c(mesh.ien.T, np.int32).ravel(), _c(mesh.elm_id, np.int32),
+        _c(elm_id_set, np.int32), subdomain_flag, tensor_flag, _soa(mesh.xg), ibcf,
+        _soa(state["dg"]), _soa(state["ug"]), _soa(state["ugm"]), _c(state["pg"]), _c(prm),
+        len(f2e), f2e, fo, int(ngb), _c(col, np.int32), _c(row, np.int32),
+        k11, g, d1, m, rhsu, rhsp)
+    if r < 0:
+        raise RuntimeError("oracle face assembly failed")
+    return dict(k11=k11.reshape(icnt, 9), g=g.reshape(icnt, 3), d1=d1.reshape(icnt, 3), m=m,
+                rhsu=rhsu.reshape(3, N).T.copy(), rhsp=rhsp)
 
 
 def spmv_up(col, row, A, xu, xp):

@@ -21,6 +21,9 @@
 !   4  tem:    genSparStruc, IntElmAss_Tem_Quenching_STAB (with BCLhs_tem and
 !              FillSparseMat_tem), SparseProd_LS on a given vector,
 !              SparseGMRES_tem on the assembled system
+!   6  face:   genSparStruc, then FaceAssembly_NS_weak and
+!              FaceAssembly_NS_outflow (as assembleNavSto calls them) over
+!              one weak boundary (BCugType 2) of given tet faces
 !   5  nsvof:  genSparStruc, IntElmAss_NSVOF_Quenching_STAB with the
 !              5-field Dirichlet projection and scatter BCLhs_NS_conv /
 !              FillSparseMat_NS_conv, SparseProd_NS_conv on a given vector,
@@ -70,6 +73,8 @@ program ofd_refharness
     call run_tem()
   case (5)
     call run_nsvof()
+  case (6)
+    call run_face()
   case default
     write (*, *) 'unknown mode', mode
   end select
@@ -343,6 +348,49 @@ contains
     end if
     write (11) solu, solp, soll
   end subroutine run_nsvof
+
+  subroutine run_face()
+    integer :: nshl, ngauss, tflag, sflag, nface, ngb
+    real(8) :: prm(11)
+    real(8), allocatable :: dA(:, :), uA(:, :), umA(:, :), aA(:, :), &
+                            amA(:, :), pA(:)
+
+    read (10) NNODE, NELEM, nshl, ngauss, tflag, sflag, ELM_ID_SET(1:2), nface, ngb
+    maxNSHL = nshl
+    NSHLBmax = 3
+    allocate (IEN(NELEM, nshl), ELM_ID(NELEM), ELMNSHL(NELEM), &
+              ELMNGAUSS(NELEM), IBC(NNODE, 8), xg(NNODE, 3), wg(NNODE))
+    read (10) IEN, ELM_ID, IBC
+    read (10) xg
+    wg = 1.0d0
+    ELMNSHL = nshl
+    ELMNGAUSS = ngauss
+    allocate (dA(NNODE, 3), uA(NNODE, 3), umA(NNODE, 3), aA(NNODE, 3), &
+              amA(NNODE, 3), pA(NNODE))
+    read (10) dA, uA, umA, aA, amA, pA, prm
+    almi = prm(1); alfi = prm(2); gami = prm(3); beti = prm(4)
+    Delt = prm(5); rhoa = prm(6); mua = prm(7); NS_kdc_a = prm(8)
+    gravvec = prm(9:11)
+    NBOUND = 1
+    allocate (bound(1), BCugType(1, 3))
+    BCugType = 2
+    bound(1)%FACE_ID = 1
+    bound(1)%NFACE = nface
+    allocate (bound(1)%F2E(nface), bound(1)%FACE_OR(nface), bound(1)%NGAUSSB(nface))
+    read (10) bound(1)%F2E, bound(1)%FACE_OR
+    bound(1)%NGAUSSB = ngb
+
+    call genSparStruc()
+    allocate (LHSK11(9, icnt), LHSG(3, icnt), LHSD1(3, icnt), LHSM(icnt), &
+              RHSGu(NNODE, 3), RHSGp(NNODE))
+    LHSK11 = 0.0d0; LHSG = 0.0d0; LHSD1 = 0.0d0; LHSM = 0.0d0
+    RHSGu = 0.0d0; RHSGp = 0.0d0
+    call FaceAssembly_NS_weak(dA, uA, umA, aA, amA, pA, tflag, sflag)
+    call FaceAssembly_NS_outflow(dA, uA, umA, aA, amA, pA, tflag, sflag)
+    write (11) icnt
+    write (11) col, row
+    write (11) LHSK11, LHSG, LHSD1, LHSM, RHSGu, RHSGp
+  end subroutine run_face
 
 end program ofd_refharness
 

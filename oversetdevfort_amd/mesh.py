@@ -128,6 +128,44 @@ def hex_to_tets(mesh: Mesh) -> Mesh:
                 dims=mesh.dims)
 
 
+# Local nodes of each hex face, FACE_OR 1..6 = xi -1, xi +1, eta -1, eta +1,
+# zeta -1, zeta +1 (the reference has no hex face map; this numbering is the
+# one of the face kernels and the oracle).
+HEX_FACE_NODES = tuple(
+    tuple(int(a) for a in np.nonzero(HEX_CORNERS[:, c] == side)[0])
+    for c in range(3) for side in (0, 1))
+
+# FACE_OR of a tet face by the local node it leaves out (genGPtritet and
+# eval_faceshape_tet, genGP_GW.f90:478-572, eval_shape_FEM.f90:84-99: face 1
+# holds nodes 2,3,1; 2: 4,3,2; 3: 1,3,4; 4: 2,1,4)
+TET_FACE_BY_MISSING = {3: 1, 0: 2, 1: 3, 2: 4}
+
+
+def boundary_faces(mesh: Mesh, node_mask=None):
+    """(f2e, face_or): 0-based element and FACE_OR of every face that no
+    other element shares, sorted by (element, face); with ``node_mask`` only
+    the faces whose nodes all lie in it."""
+    if mesh.nshl == 8:
+        local = [(fo + 1, HEX_FACE_NODES[fo]) for fo in range(6)]
+    elif mesh.nshl == 4:
+        local = [(TET_FACE_BY_MISSING[miss], tuple(a for a in range(4) if a != miss))
+                 for miss in range(4)]
+    else:
+        raise ValueError("boundary faces of hex and tet meshes only")
+    seen = {}
+    for e, nodes in enumerate(np.asarray(mesh.ien)):
+        for fo, loc in local:
+            key = tuple(sorted(int(nodes[a]) for a in loc))
+            seen.setdefault(key, []).append((e, fo))
+    faces = []
+    for key, v in seen.items():
+        if len(v) == 1 and (node_mask is None or all(node_mask[n] for n in key)):
+            faces.append(v[0])
+    faces.sort()
+    return (np.array([f[0] for f in faces], np.int32),
+            np.array([f[1] for f in faces], np.int32))
+
+
 def boundary_masks(mesh: Mesh, lo, hi, tol=1e-12):
     """Boolean node masks of the six faces of a box mesh: xlo,xhi,ylo,yhi,zlo,zhi."""
     x = mesh.xg

@@ -372,6 +372,59 @@ def make_vms():
             kspace=0, kmin=0, frac_ibc=0.15, vms=1)
 
 
+def face_case(name, dims, seed, tensor_flag, subdomain_flag, ngb, frac_ibc=0.15):
+    """FaceAssembly_NS_weak + FaceAssembly_NS_outflow over the boundary faces
+    of a jittered tet mesh: two element ids, some constraints, a random state
+    with a moving wall (ugm), every face on one weak boundary."""
+    rng = np.random.default_rng(seed)
+    hexm = M.structured_hex(*dims, lo=(0, 0, 0), hi=(1.0, 0.8, 1.2), jitter=0.05,
+                            seed=seed)
+    tm = M.hex_to_tets(hexm)
+    N, E, nshl = tm.nnode, tm.nelem, tm.nshl
+    elm_id = rng.integers(1, 3, size=E).astype(np.int32)
+    ibc = (rng.uniform(size=(N, 8)) < frac_ibc).astype(np.int32)
+    ibc[:, 4:] = 0
+    st = {k: rng.normal(size=(N, 3)) for k in ("ug", "ugm", "acg", "acgm")}
+    st["dg"] = 0.01 * rng.normal(size=(N, 3))
+    st["pg"] = rng.normal(size=N)
+    st["ugm"] *= 0.2
+    prm = np.array([0.8333333333333334, 0.6666666666666666, 0.6666666666666667,
+                    0.34722222222222227, 0.01, 1.3, 0.02, 0.0, 0.1, -0.2, -9.81])
+    f2e, fo = M.boundary_faces(tm)
+    payload = (np.int32(6).tobytes() +
+               np.array([N, E, nshl, nshl, tensor_flag, subdomain_flag, 1, 2, len(f2e), ngb],
+                        np.int32).tobytes() +
+               _i32(tm.ien + 1) + _i32(elm_id) + _i32(ibc) + _f64(tm.xg) +
+               _f64(st["dg"]) + _f64(st["ug"]) + _f64(st["ugm"]) +
+               _f64(st["acg"]) + _f64(st["acgm"]) + _f64(st["pg"]) + _f64(prm) +
+               _i32(f2e + 1) + _i32(fo))
+    out, _ = _run(payload)
+    r = _Reader(out)
+    icnt = int(r.i32(1)[0])
+    col = r.i32(N + 1)
+    row = r.i32(icnt)
+    k11 = r.f64(9, icnt).T
+    g = r.f64(3, icnt).T
+    d1 = r.f64(3, icnt).T
+    m = r.f64(icnt)
+    rhsu = r.f64(N, 3)
+    rhsp = r.f64(N)
+    assert r.off == len(out)
+    np.savez_compressed(
+        os.path.join(HERE, f"{name}.npz"), xg=tm.xg, ien=tm.ien, elm_id=elm_id,
+        ibc=ibc[:, :4], prm=prm, tensor_flag=tensor_flag, subdomain_flag=subdomain_flag,
+        f2e=f2e, face_or=fo, ngb=ngb, col=col - 1, row=row - 1, k11=k11, g=g, d1=d1, m=m,
+        rhsu=rhsu, rhsp=rhsp, **{"st_" + k: v for k, v in st.items()})
+
+
+def make_face():
+    """Weak-Dirichlet and outflow faces: 3-point faces with both subdomains,
+    1-point faces with one subdomain, and the vector-only call."""
+    face_case("face_tet_a", (2, 2, 2), seed=21, tensor_flag=6, subdomain_flag=3, ngb=3)
+    face_case("face_tet_b", (3, 2, 2), seed=22, tensor_flag=6, subdomain_flag=1, ngb=1)
+    face_case("face_tet_vec", (2, 3, 2), seed=23, tensor_flag=2, subdomain_flag=2, ngb=3)
+
+
 def main():
     if not os.path.exists(HARNESS):
         raise SystemExit(f"missing {HARNESS}: run make -C oracle ref")
@@ -383,6 +436,9 @@ def main():
         return
     if sys.argv[1:] == ["vms"]:
         make_vms()
+        return
+    if sys.argv[1:] == ["face"]:
+        make_face()
         return
     make_shape()
     # element kernels + BC + scatter + SpMV + a fixed-length GMRES cycle
@@ -398,6 +454,7 @@ def main():
     make_tem()
     make_nsvof()
     make_vms()
+    make_face()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))
