@@ -125,6 +125,16 @@ int ofd_set_bcvals(ofd_ctx *ctx, const double *vals);
  * acg = ((ug - ugold)/Delt + (gami-1) acgold)/gami on Dirichlet dofs */
 int ofd_apply_dirichlet(ofd_ctx *ctx);
 
+/* The faces of the weak-Dirichlet boundaries (BCugType = 2): bound(b)%F2E
+ * and bound(b)%FACE_OR of input.f90:80-92, nface faces, f2e 1-based
+ * elements.  The reference has no hex face map; here FACE_OR 1..6 of a hex
+ * is the face at xi = -1, xi = +1, eta = -1, eta = +1, zeta = -1, zeta = +1
+ * in linhexshl's node order, 2x2 Gauss points.  The weak velocity is the mesh
+ * velocity (gi = umi, FaceAssembly_weak.f90:147).  Replaces the faces given
+ * before; nface = 0 clears them.  The adjoint entries refuse a context with
+ * faces (they differentiate the volume residual only). */
+int ofd_set_weak_faces(ofd_ctx *ctx, int nface, const int32_t *f2e, const int32_t *face_or);
+
 /* copy a node field between host and the context */
 int ofd_set_field(ofd_ctx *ctx, int field, const double *host);
 int ofd_get_field(ofd_ctx *ctx, int field, double *host);
@@ -132,7 +142,10 @@ int ofd_get_field(ofd_ctx *ctx, int field, double *host);
 /* assembleNavSto(assemble_tensor_flag, subdomain_flag)
  * (assembleNavSto.f90:4-72 -> IntElmAss_NS_overset :76-336, BCLhs_3D,
  * FillSparseMat_3D): alpha stage from the context fields, zeroing, element
- * loop, Dirichlet projection, scatter.  LHS/RHS stay on the device.
+ * loop, Dirichlet projection, scatter; then, over the faces registered with
+ * ofd_set_weak_faces, FaceAssembly_NS_weak and FaceAssembly_NS_outflow
+ * (FaceAssembly_weak.f90:4-409) with their own projection and scatter, as
+ * assembleNavSto calls them (:54-60).  LHS/RHS stay on the device.
  * cfl (may be NULL) receives the maximum Gauss-point CFL number. */
 int ofd_assemble_ns(ofd_ctx *ctx, int tensor_flag, int subdomain_flag,
                     double *cfl);

@@ -95,6 +95,26 @@ def cavity(n=16, seed=1, rhoinf=0.5, jitter=0.05) -> Problem:
     return Problem("cavity", mesh, ibc, bcval, f, p)
 
 
+def weak_cavity(n=16, seed=1, rhoinf=0.5, jitter=0.05) -> Problem:
+    """The lid-driven cavity with the no-slip walls imposed weakly: the lid
+    keeps its Dirichlet velocity; the other five walls carry no velocity IBC
+    and are weak-Dirichlet boundaries (BCugType 2, FaceAssembly_NS_weak)
+    whose wall velocity is the (zero) mesh velocity.  Pressure held at node 0."""
+    prob = cavity(n, seed, rhoinf, jitter)
+    mesh = prob.mesh
+    faces = M.boundary_masks(mesh, (0, 0, 0), (1, 1, 1))
+    prob.ibc[:, :3] = 0
+    prob.ibc[faces["zhi"], :3] = 1
+    parts = [M.boundary_faces(mesh, node_mask=faces[w])
+             for w in ("xlo", "xhi", "ylo", "yhi", "zlo")]
+    f2e = np.concatenate([f[0] for f in parts])
+    fo = np.concatenate([f[1] for f in parts])
+    order = np.lexsort((fo, f2e))
+    prob.weak_faces = (f2e[order], fo[order])
+    prob.name = "weak_cavity"
+    return prob
+
+
 def poiseuille(y, z):
     return 16.0 * y * (1.0 - y) * z * (1.0 - z)
 

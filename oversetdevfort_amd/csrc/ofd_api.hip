@@ -426,6 +426,8 @@ int ofd_ctx_destroy(ofd_ctx *x) {
   dfree(c.d_row);
   dfree(c.d_emap);
   dfree(c.d_color_elems);
+  dfree(c.d_fe_elem);
+  dfree(c.d_fe_mask);
   dfree(c.d_diag);
   for (auto &f : c.d_field) dfree(f);
   dfree(c.d_ibc);
@@ -666,6 +668,8 @@ int ofd_build_pattern(ofd_ctx *x, int64_t *icnt_out) {
   }
   c.color_off.assign(ncolor + 1, 0);
   for (int e = 0; e < E; ++e) c.color_off[ecol[e] + 1]++;
+  c.h_elem_color = ecol;
+  c.fe_dirty = true;
   for (int k = 0; k < ncolor; ++k) c.color_off[k + 1] += c.color_off[k];
   std::vector<int32_t> order(E);
   {
@@ -799,6 +803,68 @@ int ofd_get_field(ofd_ctx *x, int field, double *host) {
   return d2h(c, host, c.d_field[field], (size_t)field_len(c, field) * sizeof(double));
 }
 
+// the elements holding registered weak faces, grouped by element colour,
+// with their faces as a bitmask (bit fo-1), on the device
+static int ensure_faces(Ctx &c) {
+  if (!c.fe_dirty) return OFD_OK;
+  c.fe_color_off.clear();
+  if (c.h_face_e.empty()) {
+    c.fe_dirty = false;
+    return OFD_OK;
+  }
+  const int E = c.nelem, ncolor = (int)c.color_off.size() - 1;
+  OFD_CHECK((int)c.h_elem_color.size() == E, OFD_ERR_ARG, "no element colouring");
+  std::vector<int32_t> mask(E, 0);
+  for (size_t f = 0; f < c.h_face_e.size(); ++f)
+    mask[c.h_face_e[f]] |= 1 << (c.h_face_or[f] - 1);
+  c.fe_color_off.assign(ncolor + 1, 0);
+  for (int e = 0; e < E; ++e)
+    if (mask[e]) c.fe_color_off[c.h_elem_color[e] + 1]++;
+  for (int k = 0; k < ncolor; ++k) c.fe_color_off[k + 1] += c.fe_color_off[k];
+  const int nfe = c.fe_color_off[ncolor];
+  std::vector<int32_t> el(nfe), mk(nfe);
+  std::vector<int32_t> pos(c.fe_color_off.begin(), c.fe_color_off.end() - 1);
+  for (int e = 0; e < E; ++e)
+    if (mask[e]) {
+      const int p = pos[c.h_elem_color[e]]++;
+      el[p] = e;
+      mk[p] = mask[e];
+    }
+  int rc = dalloc(c, &c.d_fe_elem, (size_t)nfe);
+  if (!rc) rc = dalloc(c, &c.d_fe_mask, (size_t)nfe);
+  if (!rc) rc = h2d(c, c.d_fe_elem, el.data(), nfe * sizeof(int32_t));
+  if (!rc) rc = h2d(c, c.d_fe_mask, mk.data(), nfe * sizeof(int32_t));
+  if (rc) return rc;
+  c.fe_dirty = false;
+  return OFD_OK;
+}
+
+int ofd_set_weak_faces(ofd_ctx *x, int nface, const int32_t *f2e, const int32_t *face_or) {
+  OFD_CHECK(x, OFD_ERR_ARG, "null context");
+  OFD_CHECK(nface >= 0, OFD_ERR_ARG, "nface < 0");
+  Ctx &c = x->c;
+  int rc = need_mesh(c);
+  if (rc) return rc;
+  OFD_CHECK(nface == 0 || (f2e && face_or), OFD_ERR_ARG, "null argument");
+  OFD_CHECK(nface == 0 || c.nshl == 8, OFD_ERR_UNSUPPORTED,
+            "boundary faces are implemented for trilinear hexes (NSHL=8)");
+  std::vector<int32_t> fe(nface), fo(nface);
+  std::vector<uint8_t> seen((size_t)c.nelem, 0);
+  for (int f = 0; f < nface; ++f) {
+    const int e = f2e[f] - 1, o = face_or[f];
+    OFD_CHECK(e >= 0 && e < c.nelem, OFD_ERR_ARG, "F2E out of range [1, NELEM]");
+    OFD_CHECK(o >= 1 && o <= 6, OFD_ERR_ARG, "FACE_OR out of range [1, 6]");
+    OFD_CHECK(!((seen[e] >> (o - 1)) & 1), OFD_ERR_ARG, "face listed twice");
+    seen[e] |= (uint8_t)(1u << (o - 1));
+    fe[f] = e;
+    fo[f] = o;
+  }
+  c.h_face_e.swap(fe);
+  c.h_face_or.swap(fo);
+  c.fe_dirty = true;
+  return OFD_OK;
+}
+
 int ofd_assemble_ns(ofd_ctx *x, int tensor_flag, int subdomain_flag, double *cfl) {
   OFD_CHECK(x, OFD_ERR_ARG, "null context");
   Ctx &c = x->c;
@@ -820,6 +886,11 @@ int ofd_assemble_ns(ofd_ctx *x, int tensor_flag, int subdomain_flag, double *cfl
   OFD_HIP(hipMemsetAsync(c.d_negdet, 0, sizeof(int32_t), c.stream));
   for (int col = 0; col + 1 < (int)c.color_off.size(); ++col)
     OFD_HIP(launch_assemble_ns_hex(c, tensor_flag, subdomain_flag, col));
+  // FaceAssembly_NS_weak and FaceAssembly_NS_outflow (assembleNavSto.f90:54-60)
+  rc = ensure_faces(c);
+  if (rc) return rc;
+  for (int col = 0; col + 1 < (int)c.fe_color_off.size(); ++col)
+    OFD_HIP(launch_face_ns_hex(c, tensor_flag, subdomain_flag, col));
   double *d_out = c.d_part;  // scratch for the CFL maximum
   OFD_HIP(launch_max_reduce(c, c.d_cfl, c.nelem, d_out));
   double h[1] = {0.0};
@@ -1918,6 +1989,9 @@ int ofd_probe_read(ofd_ctx *x, int64_t *count, double *total_ms) {
 static int ensure_adjoint(Ctx &c) {
   int rc = need_pattern(c);
   if (rc) return rc;
+  OFD_CHECK(c.h_face_e.empty(), OFD_ERR_UNSUPPORTED,
+            "the adjoint differentiates the volume residual; this context has weak "
+            "boundary faces (ofd_set_weak_faces)");
   const size_t n4 = (size_t)c.nnode * 4;
   double **bufs[5] = {&c.d_jt, &c.d_lam, &c.d_lamr, &c.d_lamb, &c.d_lamd};
   for (double **b : bufs)

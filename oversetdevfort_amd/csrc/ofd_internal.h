@@ -88,6 +88,13 @@ struct Ctx {
   int32_t *d_emap = nullptr;          // [E][64]
   int32_t *d_color_elems = nullptr;   // elements grouped by colour
   std::vector<int32_t> color_off;     // ncolor+1
+  std::vector<int32_t> h_elem_color;  // colour of each element
+  // weak boundary faces (BCugType 2): registered list, and the elements
+  // that hold them grouped by colour with their faces as a bitmask
+  std::vector<int32_t> h_face_e, h_face_or;  // 0-based element, FACE_OR 1..6
+  int32_t *d_fe_elem = nullptr, *d_fe_mask = nullptr;
+  std::vector<int32_t> fe_color_off;  // ncolor+1 (empty: no faces)
+  bool fe_dirty = false;
   int32_t *d_diag = nullptr;          // nonzero index of the diagonal block per row
   // parameters
   double prm[OFD_NPARAMS] = {0};
@@ -257,6 +264,7 @@ hipError_t launch_spmv(Ctx &c, int precision, const double *x, double *y,
 hipError_t launch_spmv_arnoldi(Ctx &c, int precision, int k, hipEvent_t ev0 = nullptr,
                                hipEvent_t ev1 = nullptr);
 hipError_t launch_cgs_dots(Ctx &c, int k);
+hipError_t launch_face_ns_hex(Ctx &c, int tensor_flag, int subdomain_flag, int color);
 hipError_t launch_cgs_update(Ctx &c, int k, int mode);
 // vectors
 hipError_t launch_soa_to_node4(Ctx &c, const double *u, const double *p,

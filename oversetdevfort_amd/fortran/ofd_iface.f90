@@ -371,6 +371,15 @@ module ofd_iface
       integer(c_int) :: rc
     end function
 
+    ! bound(b)%F2E / %FACE_OR of the weak-Dirichlet (BCugType 2) boundaries
+    function ofd_set_weak_faces(ctx, nface, f2e, face_or) &
+        bind(C, name='ofd_set_weak_faces') result(rc)
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nface
+      integer(c_int32_t), intent(in) :: f2e(*), face_or(*)
+      integer(c_int) :: rc
+    end function
     function ofd_apply_fringe(ctx, nrecv, nodes, val) bind(C, name='ofd_apply_fringe') &
         result(rc)
       import :: c_int, c_ptr, c_int32_t, c_double

@@ -119,6 +119,17 @@ class Context:
         a, nc = self._ibc_bc
         L.check(self.lib.ofd_set_ibc(self.h, _ptr(a), nc))
 
+    def set_weak_faces(self, f2e, face_or):
+        """Faces of the weak-Dirichlet boundaries, bound(b)%F2E and
+        bound(b)%FACE_OR (input.f90:80-92), here with 0-based elements; hex
+        FACE_OR 1..6 = xi -1/+1, eta -1/+1, zeta -1/+1.  assembleNavSto then
+        adds FaceAssembly_NS_weak and FaceAssembly_NS_outflow over them."""
+        e = np.ascontiguousarray(np.asarray(f2e, np.int32) + 1)
+        o = np.ascontiguousarray(face_or, dtype=np.int32)
+        if e.shape != o.shape:
+            raise ValueError("f2e and face_or differ in length")
+        L.check(self.lib.ofd_set_weak_faces(self.h, int(e.size), _ptr(e), _ptr(o)))
+
     def set_bcvals(self, vals):
         a = _soa(vals)
         L.check(self.lib.ofd_set_bcvals(self.h, _ptr(a)))
@@ -607,6 +618,9 @@ def setup(prob, device=0) -> Context:
     ctx.set_ibc(prob.ibc)
     ctx.set_bcvals(prob.bcval)
     ctx.set_state(prob.fields)
+    faces = getattr(prob, "weak_faces", None)
+    if faces is not None:
+        ctx.set_weak_faces(*faces)
     return ctx
 
 
