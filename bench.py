@@ -415,6 +415,61 @@ def cpu_quench(prob):
             "tem_vecmat_ms": 1e3 * (t2 - t1), "tem_melem_per_s": E / (t2 - t1) / 1e6}
 
 
+def weak_faces(L, C, S, reps=20):
+    """The 64x32x32 channel with its four walls weak-Dirichlet (BCugType 2)
+    instead of strong: FaceAssembly_NS_weak + FaceAssembly_NS_outflow over
+    the 8,192 wall faces, timed alone (ofd_time_faces, HIP events on the
+    context stream) and with the volume loop (ofd_time_assemble, both with
+    and without the faces)."""
+    prob = C.weak_channel()
+    nf = len(prob.weak_faces[0])
+    ctx = S.setup(prob, device=0)
+    S.begin_step(ctx)
+    ctx.set_weak_faces([], [])
+    vol_ms = ctx.time_assemble(6, reps=5)
+    ctx.set_weak_faces(*prob.weak_faces)
+    all_ms = ctx.time_assemble(6, reps=5)
+    face_ms = ctx.time_faces(6, reps=reps)
+    face_vec_ms = ctx.time_faces(2, reps=reps)
+    ctx.close()
+    out = {"workload": "channel 64x32x32, four walls weak-Dirichlet, "
+                       f"{nf} hex faces x 4 points", "faces": nf,
+           "face_vecmat_ms": face_ms, "face_vec_ms": face_vec_ms,
+           "faces_per_s": nf / (face_ms * 1e-3),
+           "assembly_vecmat_ms_with_faces": all_ms, "assembly_vecmat_ms_without": vol_ms}
+    log(f"bench: weak faces: {nf} faces, face kernels VEC+MAT {1e3 * face_ms:.1f} us "
+        f"({out['faces_per_s'] / 1e6:.1f} Mfaces/s), VEC {1e3 * face_vec_ms:.1f} us; "
+        f"assembly {vol_ms:.3f} ms -> {all_ms:.3f} ms with the faces")
+    return out, prob
+
+
+def cpu_faces(prob):
+    """Host (C oracle, one thread) face assembly VEC+MAT of the weak-wall
+    channel's 8,192 wall faces: one call."""
+    import numpy as np
+
+    from oracle import oracle as O
+    m, f = prob.mesh, prob.fields
+    N, E = m.nnode, m.nelem
+    col, row = O.sparsity(N, m.ien)
+    Z = row.size
+    c = np.ascontiguousarray
+    soa = {k: c(np.asarray(f[k], np.float64).T).ravel() for k in ("dg", "ug", "ugm")}
+    f2e, fo = (c(a, np.int32) for a in prob.weak_faces)
+    args = [N, E, 8, c(m.ien.T, np.int32).ravel(), c(m.elm_id, np.int32),
+            c([1, 2], np.int32), 1, 6, c(m.xg.T).ravel(),
+            c(np.asarray(prob.ibc)[:, :4].T, np.int32).ravel(), soa["dg"], soa["ug"],
+            soa["ugm"], c(f["pg"], np.float64), O.params_array(prob.params), len(f2e), f2e,
+            fo, 4, c(col, np.int32), c(row, np.int32), np.zeros(Z * 9), np.zeros(Z * 3),
+            np.zeros(Z * 3), np.zeros(Z), np.zeros(3 * N), np.zeros(N)]
+    lib = O.lib()
+    t0 = time.perf_counter()
+    lib.orc_face_ns(*args)              # the C routine alone, arrays prepared
+    t1 = time.perf_counter()
+    return {"face_vecmat_ms": 1e3 * (t1 - t0),
+            "faces_per_s": len(f2e) / (t1 - t0)}
+
+
 def cpu_baseline(prob, budget_s=15.0):
     """FP64 Newton step of the same workload on the host, single thread,
     through the C oracle (loop-for-loop restatement of the reference):
@@ -575,7 +630,7 @@ def main():
         f"{asm_tf / 1e12:.2f} TFLOP/s); VEC {asm_vec_ms:.3f} ms")
     log(f"bench: SpMV FP64 {spmv64_ms * 1e3:.1f} us {bw64 / 1e9:.0f} GB/s; "
         f"FP32-matrix {spmv32_ms * 1e3:.1f} us {bw32 / 1e9:.0f} GB/s")
-    adjoint = quench = fringe = None
+    adjoint = quench = fringe = faces = None
     if not args.headline_only:
         adjoint = adjoint_ns(ctx, L, prob, asm_vec_ms)
         adjoint["newton_step_fp64_ms"] = res64["newton_ms"]
@@ -586,6 +641,7 @@ def main():
         fringe = fringe_search(C, S)
         fringe["coupled_steps"] = overset_steps(L, C, S)
         adjoint["quench_transient"] = adjoint_c5(L, C, S)
+        faces, fprob = weak_faces(L, C, S)
 
     cpu = None
     if not (args.no_cpu or args.headline_only):
@@ -617,6 +673,9 @@ def main():
         log(f"bench: host quench: NSVOF {cq['nsvof_vecmat_ms']:.0f} ms, "
             f"Tem {cq['tem_vecmat_ms']:.0f} ms")
         cpu["quench_assembly"] = cq
+        cf = cpu_faces(fprob)
+        log(f"bench: host weak faces: {cf['face_vecmat_ms']:.0f} ms")
+        cpu["weak_faces"] = cf
 
     out = {
         "metric": METRIC,
@@ -662,6 +721,7 @@ def main():
         "adjoint": adjoint,
         "quench": quench,
         "fringe": fringe,
+        "weak_faces": faces,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -311,6 +311,10 @@ int ofd_apply_fringe(ofd_ctx *ctx, int nrecv, const int32_t *nodes, const double
  * back-to-back launches); ms = average duration of one launch */
 int ofd_time_spmv(ofd_ctx *ctx, int precision, int reps, double *ms);
 int ofd_time_assemble(ofd_ctx *ctx, int tensor_flag, int reps, double *ms);
+/* the weak-face launches of ofd_assemble_ns alone (k_face_ns_hex, every
+ * colour), `reps` times into the assembled arrays: the context holds no
+ * valid system afterwards.  Measurement only; no reference counterpart. */
+int ofd_time_faces(ofd_ctx *ctx, int tensor_flag, int reps, double *ms);
 
 /* plain device-to-device copy of nbytes (multiple of 16) by a grid-stride
  * kernel; ms = average of `reps` launches.  The bandwidth reference for the

@@ -129,6 +129,7 @@ SIGNATURES = {
     "ofd_time_quench": ([_vp, _c_int, _c_int, _c_int, _dp], _c_int),
     "ofd_time_adjoint": ([_vp, _c_int, _c_int, _dp], _c_int),
     "ofd_set_weak_faces": ([_vp, _c_int, _vp, _vp], _c_int),
+    "ofd_time_faces": ([_vp, _c_int, _c_int, _dp], _c_int),
     "ofd_probe_set": ([_vp, _c_int], _c_int),
     "ofd_probe_read": ([_vp, _i64p, _dp], _c_int),
     "ofd_ctx_bytes": ([_vp, _i64p], _c_int),

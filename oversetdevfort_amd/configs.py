@@ -105,13 +105,22 @@ def weak_cavity(n=16, seed=1, rhoinf=0.5, jitter=0.05) -> Problem:
     faces = M.boundary_masks(mesh, (0, 0, 0), (1, 1, 1))
     prob.ibc[:, :3] = 0
     prob.ibc[faces["zhi"], :3] = 1
-    parts = [M.boundary_faces(mesh, node_mask=faces[w])
-             for w in ("xlo", "xhi", "ylo", "yhi", "zlo")]
-    f2e = np.concatenate([f[0] for f in parts])
-    fo = np.concatenate([f[1] for f in parts])
-    order = np.lexsort((fo, f2e))
-    prob.weak_faces = (f2e[order], fo[order])
+    prob.weak_faces = M.box_faces(mesh, ("xlo", "xhi", "ylo", "yhi", "zlo"))
     prob.name = "weak_cavity"
+    return prob
+
+
+def weak_channel(nx=64, ny=32, nz=32, seed=1, rhoinf=0.5) -> Problem:
+    """The channel with its four walls weak-Dirichlet (BCugType 2,
+    FaceAssembly_NS_weak + FaceAssembly_NS_outflow over their faces) instead
+    of strong; the inflow keeps its Dirichlet profile."""
+    prob = channel(nx, ny, nz, seed, rhoinf)
+    mesh = prob.mesh
+    b = M.boundary_masks(mesh, (0.0, 0.0, 0.0), (4.0, 1.0, 1.0))
+    wall = b["ylo"] | b["yhi"] | b["zlo"] | b["zhi"]
+    prob.ibc[wall & ~b["xlo"], :3] = 0
+    prob.weak_faces = M.box_faces(mesh, ("ylo", "yhi", "zlo", "zhi"))
+    prob.name = "weak_channel"
     return prob
 
 

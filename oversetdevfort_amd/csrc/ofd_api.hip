@@ -1232,10 +1232,43 @@ int ofd_time_spmv(ofd_ctx *x, int precision, int reps, double *ms) {
   return OFD_OK;
 }
 
+int ofd_time_faces(ofd_ctx *x, int tensor_flag, int reps, double *ms) {
+  OFD_CHECK(x && ms && reps > 0, OFD_ERR_ARG, "bad argument");
+  Ctx &c = x->c;
+  int rc = need_pattern(c);
+  if (rc) return rc;
+  rc = ensure_faces(c);
+  if (rc) return rc;
+  OFD_CHECK(!c.fe_color_off.empty(), OFD_ERR_ARG, "no weak faces (ofd_set_weak_faces)");
+  OFD_HIP(hipSetDevice(c.device));
+  OFD_HIP(launch_alpha_stage(c));
+  hipEvent_t e0, e1;
+  OFD_HIP(hipEventCreate(&e0));
+  OFD_HIP(hipEventCreate(&e1));
+  OFD_HIP(hipEventRecord(e0, c.stream));
+  for (int r = 0; r < reps; ++r)
+    for (int col = 0; col + 1 < (int)c.fe_color_off.size(); ++col)
+      OFD_HIP(launch_face_ns_hex(c, tensor_flag, 3, col));
+  OFD_HIP(hipEventRecord(e1, c.stream));
+  OFD_HIP(hipEventSynchronize(e1));
+  float t = 0.f;
+  OFD_HIP(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  // the launches added into the system: it no longer holds an assembly
+  c.mat_valid = false;
+  c.prec_valid = c.a32_valid = c.ap32_valid = false;
+  ++c.mat_version;
+  *ms = (double)t / reps;
+  return OFD_OK;
+}
+
 int ofd_time_assemble(ofd_ctx *x, int tensor_flag, int reps, double *ms) {
   OFD_CHECK(x && ms && reps > 0, OFD_ERR_ARG, "bad argument");
   Ctx &c = x->c;
   int rc = need_pattern(c);
+  if (rc) return rc;
+  rc = ensure_faces(c);
   if (rc) return rc;
   OFD_CHECK(c.nshl == 8, OFD_ERR_UNSUPPORTED,
             "the GPU element assembly is implemented for trilinear hexes (NSHL=8)");
@@ -1250,6 +1283,8 @@ int ofd_time_assemble(ofd_ctx *x, int tensor_flag, int reps, double *ms) {
     if (tensor_flag & OFD_ASSEMBLE_MAT) OFD_HIP(launch_zero(c, c.d_A64, (size_t)c.icnt * 16));
     for (int col = 0; col + 1 < (int)c.color_off.size(); ++col)
       OFD_HIP(launch_assemble_ns_hex(c, tensor_flag, 3, col));
+    for (int col = 0; col + 1 < (int)c.fe_color_off.size(); ++col)
+      OFD_HIP(launch_face_ns_hex(c, tensor_flag, 3, col));
   }
   OFD_HIP(hipEventRecord(e1, c.stream));
   OFD_HIP(hipEventSynchronize(e1));

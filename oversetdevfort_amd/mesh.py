@@ -166,6 +166,21 @@ def boundary_faces(mesh: Mesh, node_mask=None):
             np.array([f[1] for f in faces], np.int32))
 
 
+def box_faces(mesh: Mesh, sides):
+    """(f2e, face_or) of the faces of a structured box mesh on the given
+    sides ('xlo', 'xhi', 'ylo', 'yhi', 'zlo', 'zhi'), sorted by (element,
+    face); the same faces as boundary_faces with the side's node mask."""
+    nx, ny, nz = mesh.dims
+    e = np.arange(nx * ny * nz)
+    ci, cj, ck = e % nx, (e // nx) % ny, e // (nx * ny)
+    pick = {"xlo": (ci == 0, 1), "xhi": (ci == nx - 1, 2), "ylo": (cj == 0, 3),
+            "yhi": (cj == ny - 1, 4), "zlo": (ck == 0, 5), "zhi": (ck == nz - 1, 6)}
+    f2e = np.concatenate([e[pick[s][0]] for s in sides]).astype(np.int32)
+    fo = np.concatenate([np.full(int(pick[s][0].sum()), pick[s][1]) for s in sides])
+    order = np.lexsort((fo, f2e))
+    return f2e[order], fo[order].astype(np.int32)
+
+
 def boundary_masks(mesh: Mesh, lo, hi, tol=1e-12):
     """Boolean node masks of the six faces of a box mesh: xlo,xhi,ylo,yhi,zlo,zhi."""
     x = mesh.xg

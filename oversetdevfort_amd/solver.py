@@ -590,6 +590,13 @@ class Context:
         L.check(self.lib.ofd_time_copy(self.h, int(nbytes), int(reps), ctypes.byref(ms)))
         return ms.value
 
+    def time_faces(self, tensor_flag=6, reps=20):
+        """Mean ms of the weak-face launches of one assembly (HIP events);
+        the system must be assembled again afterwards."""
+        ms = ctypes.c_double()
+        L.check(self.lib.ofd_time_faces(self.h, int(tensor_flag), int(reps), ctypes.byref(ms)))
+        return ms.value
+
     def probe_set(self, every):
         """Sample every ``every``-th Arnoldi product launch of later solves
         with HIP timing events (0: off)."""

@@ -94,3 +94,15 @@ def test_hex_face_residual_vanishes_at_the_wall_velocity(oracle_mod):
     R = O.face_ns(m, np.zeros((m.nnode, 4), np.int32), _state(m, wall, ugm=wall), PRM, col,
                   row, faces, tensor_flag=2)
     assert np.abs(R["rhsu"]).max() <= 1e-15 and np.abs(R["rhsp"]).max() <= 1e-15
+
+
+def test_box_faces_are_the_boundary_faces():
+    m = M.structured_hex(4, 3, 2, jitter=0.1, seed=3)
+    b = M.boundary_masks(m, (0, 0, 0), (1, 1, 1))
+    for side in b:
+        got = M.box_faces(m, (side,))
+        want = M.boundary_faces(m, node_mask=b[side])
+        assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1]), side
+    allf = M.box_faces(m, tuple(b))
+    want = M.boundary_faces(m)
+    assert np.array_equal(allf[0], want[0]) and np.array_equal(allf[1], want[1])
