@@ -443,6 +443,36 @@ def weak_faces(L, C, S, reps=20):
     return out, prob
 
 
+def pattern_build(S, reps=3):
+    """genSparStruc of the 128x64x64 hex mesh (545,025 nodes) on the device:
+    host wall time of ofd_build_pattern (incidence, rows, diagonal, transpose
+    and scatter maps on the device; the element colouring on the host)."""
+    from oversetdevfort_amd import mesh as M
+    mesh = M.structured_hex(128, 64, 64)
+    ctx = S.Context(0)
+    ctx.upload_mesh(mesh)
+    icnt = ctx.build_pattern()                       # warm-up
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        ctx.build_pattern()
+        ts.append(1e3 * (time.perf_counter() - t0))
+    ctx.close()
+    out = {"workload": "genSparStruc, 128x64x64 hex mesh", "nodes": mesh.nnode,
+           "nonzeros": icnt, "ms": min(ts)}
+    log(f"bench: pattern build 128x64x64 ({mesh.nnode} nodes, {icnt} blocks): "
+        f"{min(ts):.1f} ms")
+    return out, mesh
+
+
+def cpu_pattern(mesh):
+    """Host (C oracle, one thread) genSparStruc of the same mesh."""
+    from oracle import oracle as O
+    t0 = time.perf_counter()
+    O.sparsity(mesh.nnode, mesh.ien)
+    return {"ms": 1e3 * (time.perf_counter() - t0)}
+
+
 def cpu_faces(prob):
     """Host (C oracle, one thread) face assembly VEC+MAT of the weak-wall
     channel's 8,192 wall faces: one call."""
@@ -630,7 +660,7 @@ def main():
         f"{asm_tf / 1e12:.2f} TFLOP/s); VEC {asm_vec_ms:.3f} ms")
     log(f"bench: SpMV FP64 {spmv64_ms * 1e3:.1f} us {bw64 / 1e9:.0f} GB/s; "
         f"FP32-matrix {spmv32_ms * 1e3:.1f} us {bw32 / 1e9:.0f} GB/s")
-    adjoint = quench = fringe = faces = None
+    adjoint = quench = fringe = faces = pattern = None
     if not args.headline_only:
         adjoint = adjoint_ns(ctx, L, prob, asm_vec_ms)
         adjoint["newton_step_fp64_ms"] = res64["newton_ms"]
@@ -642,6 +672,7 @@ def main():
         fringe["coupled_steps"] = overset_steps(L, C, S)
         adjoint["quench_transient"] = adjoint_c5(L, C, S)
         faces, fprob = weak_faces(L, C, S)
+        pattern, pmesh = pattern_build(S)
 
     cpu = None
     if not (args.no_cpu or args.headline_only):
@@ -676,6 +707,9 @@ def main():
         cf = cpu_faces(fprob)
         log(f"bench: host weak faces: {cf['face_vecmat_ms']:.0f} ms")
         cpu["weak_faces"] = cf
+        cp = cpu_pattern(pmesh)
+        log(f"bench: host pattern build: {cp['ms']:.0f} ms")
+        cpu["pattern"] = cp
 
     out = {
         "metric": METRIC,
@@ -722,6 +756,7 @@ def main():
         "quench": quench,
         "fringe": fringe,
         "weak_faces": faces,
+        "pattern": pattern,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -589,69 +589,62 @@ int ofd_build_pattern(ofd_ctx *x, int64_t *icnt_out) {
   if (rc) return rc;
   OFD_HIP(hipSetDevice(c.device));
   const int N = c.nnode, E = c.nelem, S = c.nshl;
-  const int cap = 27 * S;
-  // Asadj (genSparStruc.f90:52-105): per-node neighbour lists in element order
-  std::vector<int32_t> list((size_t)N * cap);
-  std::vector<int32_t> cnt(N, 0);
-  for (int e = 0; e < E; ++e) {
-    for (int j = 0; j < S; ++j) {
-      const int jn = c.h_ien[(size_t)j * E + e];
-      int32_t *L = &list[(size_t)jn * cap];
-      int len = cnt[jn];
-      for (int k = 0; k < S; ++k) {
-        const int kn = c.h_ien[(size_t)k * E + e];
-        bool found = false;
-        for (int l = 0; l < len; ++l)
-          if (L[l] == kn) {
-            found = true;
-            break;
-          }
-        if (!found) {
-          OFD_CHECK(len < cap, OFD_ERR_ARG, "node adjacency exceeds 27*NSHL");
-          L[len++] = kn;
-        }
-      }
-      cnt[jn] = len;
+  // genSparStruc on the device (pattern.hip): incidence, rows, diagonal,
+  // tmap and emap; the temporaries are freed on every path
+  struct Tmp {
+    std::vector<void *> p;
+    ~Tmp() {
+      for (void *q : p) (void)hipFree(q);
     }
-  }
-  // col / row with each row sorted (genSparStruc.f90:27-45)
+    hipError_t get(void **out, size_t bytes) {
+      hipError_t e = hipMalloc(out, bytes ? bytes : 1);
+      if (e == hipSuccess) p.push_back(*out);
+      return e;
+    }
+  } tmp;
+  int32_t *d_cnt, *d_off, *d_pos, *d_inc, *d_err;
+  OFD_HIP(tmp.get((void **)&d_cnt, sizeof(int32_t) * N));
+  OFD_HIP(tmp.get((void **)&d_off, sizeof(int32_t) * ((size_t)N + 1)));
+  OFD_HIP(tmp.get((void **)&d_pos, sizeof(int32_t) * N));
+  OFD_HIP(tmp.get((void **)&d_inc, sizeof(int32_t) * (size_t)E * S));
+  OFD_HIP(tmp.get((void **)&d_err, sizeof(int32_t)));
+  OFD_HIP(hipMemsetAsync(d_cnt, 0, sizeof(int32_t) * N, c.stream));
+  OFD_HIP(hipMemsetAsync(d_pos, 0, sizeof(int32_t) * N, c.stream));
+  OFD_HIP(hipMemsetAsync(d_err, 0, sizeof(int32_t), c.stream));
+  size_t scan_bytes = 0;
+  OFD_HIP(scan_offsets(c, d_cnt, d_off, N, nullptr, scan_bytes));
+  void *d_scan;
+  OFD_HIP(tmp.get(&d_scan, scan_bytes));
+  OFD_HIP(launch_node_elem_count(c, d_cnt));
+  OFD_HIP(scan_offsets(c, d_cnt, d_off, N, d_scan, scan_bytes));
+  OFD_HIP(launch_node_elem_fill(c, d_off, d_pos, d_inc));
+  // row lengths, then col (genSparStruc.f90:27-45)
+  OFD_HIP(launch_adjacency(c, d_off, d_inc, nullptr, d_cnt, nullptr, d_err));
+  rc = dalloc(c, &c.d_col, (size_t)N + 1);
+  if (rc) return rc;
+  OFD_HIP(scan_offsets(c, d_cnt, c.d_col, N, d_scan, scan_bytes));
+  int32_t err = 0;
+  rc = d2h(c, &err, d_err, sizeof(int32_t));
+  if (rc) return rc;
+  OFD_CHECK(!(err & 1), OFD_ERR_ARG, "node adjacency exceeds 27*NSHL");
   c.h_col.assign(N + 1, 0);
-  for (int i = 0; i < N; ++i) c.h_col[i + 1] = c.h_col[i] + cnt[i];
+  rc = d2h(c, c.h_col.data(), c.d_col, (N + 1) * sizeof(int32_t));
+  if (rc) return rc;
   c.icnt = c.h_col[N];
+  rc = dalloc(c, &c.d_row, (size_t)c.icnt);
+  if (!rc) rc = dalloc(c, &c.d_diag, (size_t)N);
+  if (!rc) rc = dalloc(c, &c.d_emap, (size_t)E * 64);
+  if (!rc) rc = dalloc(c, &c.d_tmap, (size_t)c.icnt);
+  if (rc) return rc;
+  OFD_HIP(hipMemsetAsync(c.d_diag, 0xff, sizeof(int32_t) * N, c.stream));  // -1: none
+  OFD_HIP(launch_adjacency(c, d_off, d_inc, c.d_col, c.d_row, c.d_diag, d_err));
+  OFD_HIP(launch_tmap_emap(c, d_err));
+  rc = d2h(c, &err, d_err, sizeof(int32_t));
+  if (rc) return rc;
+  OFD_CHECK(!(err & 2), OFD_ERR_ARG, "sparsity pattern is not symmetric");
   c.h_row.resize(c.icnt);
-  for (int i = 0; i < N; ++i) {
-    int32_t *L = &list[(size_t)i * cap];
-    std::sort(L, L + cnt[i]);
-    std::copy(L, L + cnt[i], c.h_row.begin() + c.h_col[i]);
-  }
-  // diagonal block of each row (SparseGMRES_NS.f90:53-61)
-  std::vector<int32_t> diag(N, -1);
-  for (int i = 0; i < N; ++i)
-    for (int k = c.h_col[i]; k < c.h_col[i + 1]; ++k)
-      if (c.h_row[k] == i) diag[i] = k;
-  // the mirrored nonzero (j, i) of each nonzero (i, j): the pattern is
-  // symmetric, so the transposed matrix keeps it
-  std::vector<int32_t> tmap(c.icnt);
-  for (int i = 0; i < N; ++i)
-    for (int k = c.h_col[i]; k < c.h_col[i + 1]; ++k) {
-      const int j = c.h_row[k];
-      const int32_t *lo = c.h_row.data() + c.h_col[j];
-      const int32_t *hi = c.h_row.data() + c.h_col[j + 1];
-      const int32_t *p = std::lower_bound(lo, hi, i);
-      OFD_CHECK(p != hi && *p == i, OFD_ERR_ARG, "sparsity pattern is not symmetric");
-      tmap[k] = (int32_t)(p - c.h_row.data());
-    }
-  // scatter map: SparseMatLoc_3D's search done once per (e, a, b)
-  std::vector<int32_t> emap((size_t)E * 64, 0);
-  for (int e = 0; e < E; ++e)
-    for (int b = 0; b < S; ++b)
-      for (int a = 0; a < S; ++a) {
-        const int ia = c.h_ien[(size_t)a * E + e], ib = c.h_ien[(size_t)b * E + e];
-        const int32_t *lo = c.h_row.data() + c.h_col[ia];
-        const int32_t *hi = c.h_row.data() + c.h_col[ia + 1];
-        const int32_t *p = std::lower_bound(lo, hi, ib);
-        emap[(size_t)e * 64 + a + 8 * b] = (int32_t)(p - c.h_row.data());
-      }
+  rc = d2h(c, c.h_row.data(), c.d_row, c.icnt * sizeof(int32_t));
+  if (rc) return rc;
   // greedy colouring: elements of one colour share no node
   std::vector<uint32_t> nodemask(N, 0u);
   std::vector<int32_t> ecol(E);
@@ -676,13 +669,8 @@ int ofd_build_pattern(ofd_ctx *x, int64_t *icnt_out) {
     std::vector<int32_t> pos(c.color_off.begin(), c.color_off.end() - 1);
     for (int e = 0; e < E; ++e) order[pos[ecol[e]]++] = e;
   }
-  rc = dalloc(c, &c.d_col, (size_t)N + 1);
-  if (!rc) rc = dalloc(c, &c.d_row, (size_t)c.icnt);
-  if (!rc) rc = dalloc(c, &c.d_diag, (size_t)N);
-  if (!rc) rc = dalloc(c, &c.d_emap, (size_t)E * 64);
-  if (!rc) rc = dalloc(c, &c.d_color_elems, (size_t)E);
+  rc = dalloc(c, &c.d_color_elems, (size_t)E);
   if (!rc) rc = dalloc(c, &c.d_A64, (size_t)c.icnt * 16);
-  if (!rc) rc = dalloc(c, &c.d_tmap, (size_t)c.icnt);
   if (rc) return rc;
   dfree(c.d_A32);
   dfree(c.d_Ap64);
@@ -693,12 +681,7 @@ int ofd_build_pattern(ofd_ctx *x, int64_t *icnt_out) {
   dfree(c.t_Ap32);
   c.t_mat_valid = c.t_prec_valid = c.t_a32_valid = c.t_ap32_valid = false;
   free_tem_transpose(c);
-  rc = h2d(c, c.d_col, c.h_col.data(), (N + 1) * sizeof(int32_t));
-  if (!rc) rc = h2d(c, c.d_row, c.h_row.data(), c.icnt * sizeof(int32_t));
-  if (!rc) rc = h2d(c, c.d_diag, diag.data(), N * sizeof(int32_t));
-  if (!rc) rc = h2d(c, c.d_emap, emap.data(), emap.size() * sizeof(int32_t));
-  if (!rc) rc = h2d(c, c.d_tmap, tmap.data(), tmap.size() * sizeof(int32_t));
-  if (!rc) rc = h2d(c, c.d_color_elems, order.data(), E * sizeof(int32_t));
+  rc = h2d(c, c.d_color_elems, order.data(), E * sizeof(int32_t));
   if (rc) return rc;
   OFD_HIP(hipMemsetAsync(c.d_A64, 0, (size_t)c.icnt * 16 * sizeof(double), c.stream));
   OFD_HIP(hipStreamSynchronize(c.stream));

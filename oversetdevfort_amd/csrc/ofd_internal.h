@@ -265,6 +265,14 @@ hipError_t launch_spmv_arnoldi(Ctx &c, int precision, int k, hipEvent_t ev0 = nu
                                hipEvent_t ev1 = nullptr);
 hipError_t launch_cgs_dots(Ctx &c, int k);
 hipError_t launch_face_ns_hex(Ctx &c, int tensor_flag, int subdomain_flag, int color);
+// genSparStruc on the device (pattern.hip)
+hipError_t launch_node_elem_count(Ctx &c, int32_t *cnt);
+hipError_t launch_node_elem_fill(Ctx &c, const int32_t *off, int32_t *pos, int32_t *inc);
+hipError_t launch_adjacency(Ctx &c, const int32_t *off, const int32_t *inc, const int32_t *col,
+                            int32_t *out, int32_t *diag, int32_t *err);
+hipError_t launch_tmap_emap(Ctx &c, int32_t *err);
+hipError_t scan_offsets(Ctx &c, const int32_t *in, int32_t *out, int n, void *tmp,
+                        size_t &tmp_bytes);
 hipError_t launch_cgs_update(Ctx &c, int k, int mode);
 // vectors
 hipError_t launch_soa_to_node4(Ctx &c, const double *u, const double *p,
