@@ -21,6 +21,9 @@
 !   4  tem:    genSparStruc, IntElmAss_Tem_Quenching_STAB (with BCLhs_tem and
 !              FillSparseMat_tem), SparseProd_LS on a given vector,
 !              SparseGMRES_tem on the assembled system
+!   7  io:     input_mesh (part.<id>.dat) and readSol (restart.<r>.1) in
+!              the working directory, everything they read written out;
+!              then writeSol (restart.<w>.1) from given fields
 !   6  face:   genSparStruc, then FaceAssembly_NS_weak and
 !              FaceAssembly_NS_outflow (as assembleNavSto calls them) over
 !              one weak boundary (BCugType 2) of given tet faces
@@ -59,7 +62,8 @@ program ofd_refharness
   NBOUND = 0
 
   open (10, file=trim(fin), access='stream', form='unformatted', status='old')
-  open (11, file=trim(fout), access='stream', form='unformatted', &
+  ! unit 82: input_mesh opens unit 11 for the mesh file
+  open (82, file=trim(fout), access='stream', form='unformatted', &
         status='replace')
   read (10) mode
   select case (mode)
@@ -75,11 +79,13 @@ program ofd_refharness
     call run_nsvof()
   case (6)
     call run_face()
+  case (7)
+    call run_io()
   case default
     write (*, *) 'unknown mode', mode
   end select
   close (10)
-  close (11)
+  close (82)
   call MPI_FINALIZE(ierr)
 
 contains
@@ -102,29 +108,29 @@ contains
     ! reference-element shape functions at each point
     do ip = 1, npt
       call linhexshl(pts(ip, :), 8, 3, shl, sg)
-      write (11) shl(1:8), sg(1:8, :)
+      write (82) shl(1:8), sg(1:8, :)
     end do
     do ip = 1, npt
       call lintetshl(pts(ip, :), 4, 3, shlu4, shgradgu4)
-      write (11) shlu4, shgradgu4
+      write (82) shlu4, shgradgu4
     end do
     do ip = 1, npt
       call linprishl(pts(ip, :), 6, 3, shlu, shgradgu)
-      write (11) shlu, shgradgu
+      write (82) shlu, shgradgu
     end do
     ! quadrature rules
     call genGPandGWtetpri(gp1, gw1, 1)
     call genGPandGWtetpri(gp4, gw4, 4)
     call genGPandGWtetpri(gp6, gw6, 6)
     call genGPandGWtetpri(gp8, gw8, 8)
-    write (11) gp1, gw1, gp4, gw4, gp6, gw6, gp8, gw8
+    write (82) gp1, gw1, gp4, gw4, gp6, gw6, gp8, gw8
     ! 3x3 inverse of each point-derived matrix
     do ip = 1, npt
       A(:, 1) = pts(ip, :) + (/2.0d0, 0.1d0, -0.3d0/)
       A(:, 2) = (/0.2d0, 1.5d0, 0.4d0/) - pts(ip, :)
       A(:, 3) = (/-0.1d0, 0.3d0, 1.7d0/) + 0.5d0*pts(ip, (/2, 3, 1/))
       call get_inverse_3x3(A, Ai, dj)
-      write (11) Ai, dj
+      write (82) Ai, dj
     end do
     ! eval_shape_fem on given tets, then given prisms, at each quadrature pt
     read (10) ntet
@@ -136,7 +142,7 @@ contains
       do q = 1, 4
         call eval_shape_fem(4, gp4(q, :), xl4, dl4, shlu4, shgradgu4, dxidx, &
                             Gij, Ginv)
-        write (11) shlu4, shgradgu4, dxidx, Gij, Ginv, DetJ
+        write (82) shlu4, shgradgu4, dxidx, Gij, Ginv, DetJ
       end do
     end do
     read (10) npri
@@ -148,7 +154,7 @@ contains
       do q = 1, 6
         call eval_shape_fem(6, gp6(q, :), xl, dl, shlu, shgradgu, dxidx, &
                             Gij, Ginv)
-        write (11) shlu, shgradgu, dxidx, Gij, Ginv, DetJ
+        write (82) shlu, shgradgu, dxidx, Gij, Ginv, DetJ
       end do
     end do
   end subroutine run_shape
@@ -187,9 +193,9 @@ contains
     RHSGu = 0.0d0; RHSGp = 0.0d0
     call IntElmAss_NS_overset(dA, uA, umA, aA, amA, pA, tflag, sflag)
 
-    write (11) icnt
-    write (11) col, row
-    write (11) LHSK11, LHSG, LHSD1, LHSM, RHSGu, RHSGp
+    write (82) icnt
+    write (82) col, row
+    write (82) LHSK11, LHSG, LHSD1, LHSM, RHSGu, RHSGp
 
     allocate (solu(NNODE, 3), solp(NNODE), iper(NNODE), dflag(NNODE), &
               pflag(NNODE))
@@ -198,7 +204,7 @@ contains
 
     call SparseProdUP_3D(col, row, LHSK11, LHSG, LHSD1, LHSM, xu, xp, yu, yp, &
                          dflag, pflag, NNODE, nshl, icnt, NSD)
-    write (11) yu, yp
+    write (82) yu, yp
 
     solu = 0.0d0; solp = 0.0d0
     if (kspace > 0) then
@@ -206,7 +212,7 @@ contains
                           solu, solp, LHSK11, LHSG, LHSD1, LHSM, icnt, tol, &
                           kspace, kmin, NNODE, nshl, NSD)
     end if
-    write (11) solu, solp
+    write (82) solu, solp
   end subroutine run_ns
 
   subroutine run_interp()
@@ -225,7 +231,7 @@ contains
               bmesh(1)%ug(nrecv, 3))
     read (10) bmesh(1)%xg, bmesh(1)%dg
     call interpolation_tet(1)
-    write (11) bmesh(1)%ug
+    write (82) bmesh(1)%ug
   end subroutine run_interp
 
   subroutine run_tem()
@@ -268,19 +274,19 @@ contains
     RHSGTem = 0.0d0
     call IntElmAss_Tem_Quenching_STAB(dA, uA, umA, aA, amA, pA, phiA, rphiA, &
                                       TA, rTA, tflag)
-    write (11) icnt
-    write (11) col, row
-    write (11) LHSTem, RHSGTem
+    write (82) icnt
+    write (82) col, row
+    write (82) LHSTem, RHSGTem
 
     call SparseProd_LS(LHSTem, col, row, x, y, NNODE, nshl, icnt, NSD)
-    write (11) y
+    write (82) y
 
     sol = 0.0d0
     if (kspace > 0) then
       call SparseGMRES_tem(LHSTem, tol, col, row, RHSGTem, sol, kspace, kmin, &
                            NNODE, nshl, icnt, NSD)
     end if
-    write (11) sol
+    write (82) sol
   end subroutine run_tem
 
   subroutine run_nsvof()
@@ -325,9 +331,9 @@ contains
     RHSGu = 0.0d0; RHSGp = 0.0d0; RHSGls = 0.0d0
     call IntElmAss_NSVOF_Quenching_STAB(dA, uA, umA, aA, amA, pA, phiA, rphiA, &
                                         TA, rTA, tflag)
-    write (11) icnt
-    write (11) col, row
-    write (11) LHSK11, LHSG, LHSD1, LHSM, LHSls, LHSlsu, LHSPls, LHSUls, &
+    write (82) icnt
+    write (82) col, row
+    write (82) LHSK11, LHSG, LHSD1, LHSM, LHSls, LHSlsu, LHSPls, LHSUls, &
                RHSGu, RHSGp, RHSGls
 
     allocate (solu(NNODE, 3), solp(NNODE), soll(NNODE), iper(NNODE), dflag(NNODE), &
@@ -337,7 +343,7 @@ contains
     call SparseProd_NS_conv(col, row, LHSK11, LHSG, LHSD1, LHSM, LHSls, LHSlsu, &
                             LHSPls, LHSUls, xu, xp, xl, yu, yp, yl, dflag, pflag, &
                             NNODE, nshl, icnt, NSD)
-    write (11) yu, yp, yl
+    write (82) yu, yp, yl
 
     solu = 0.0d0; solp = 0.0d0; soll = 0.0d0
     if (kspace > 0) then
@@ -346,7 +352,7 @@ contains
                                LHSls, LHSlsu, LHSPls, LHSUls, icnt, tol, kspace, kmin, &
                                NNODE, nshl, NSD)
     end if
-    write (11) solu, solp, soll
+    write (82) solu, solp, soll
   end subroutine run_nsvof
 
   subroutine run_face()
@@ -387,10 +393,41 @@ contains
     RHSGu = 0.0d0; RHSGp = 0.0d0
     call FaceAssembly_NS_weak(dA, uA, umA, aA, amA, pA, tflag, sflag)
     call FaceAssembly_NS_outflow(dA, uA, umA, aA, amA, pA, tflag, sflag)
-    write (11) icnt
-    write (11) col, row
-    write (11) LHSK11, LHSG, LHSD1, LHSM, RHSGu, RHSGp
+    write (82) icnt
+    write (82) col, row
+    write (82) LHSK11, LHSG, LHSD1, LHSM, RHSGu, RHSGp
   end subroutine run_face
+
+  subroutine run_io()
+    integer :: id, rstep, wstep, b, nsh
+
+    read (10) id, fem_flag, rstep, wstep, bmesh_ID_set(1:2)
+    call input_mesh(id)
+    nsh = size(IEN, 2)
+    write (82) NSD, nsh, NSHLBmax, NNODE, NELEM, NBOUND, NPATCH
+    write (82) xg, NodeID, ELMNSHL, IEN, ELM_ID
+    do b = 1, NBOUND
+      write (82) bound(b)%FACE_ID, bound(b)%NFACE, bound(b)%NNODE
+      write (82) bound(b)%FACE_IEN, bound(b)%NSHLB, bound(b)%F2E, bound(b)%FACE_OR, &
+                 bound(b)%BNODES, bound(b)%L2SNODE, bound(b)%L2SELEM
+    end do
+    call input_bmesh()
+    do b = 1, 2
+      write (82) bmesh(b)%NNODE
+      write (82) bmesh(b)%XG, bmesh(b)%owned
+    end do
+    allocate (dgold(NNODE, 3), ugold(NNODE, 3), ugmold(NNODE, 3), acgold(NNODE, 3), &
+              acgmold(NNODE, 3), phigold(NNODE), rphigold(NNODE), pgold(NNODE), &
+              Tgold(NNODE), rTgold(NNODE))
+    call readSol(rstep)
+    write (82) rstep, time
+    write (82) dgold, ugold, ugmold, acgold, acgmold, phigold, rphigold, pgold, Tgold, rTgold
+    allocate (dg(NNODE, 3), ug(NNODE, 3), ugm(NNODE, 3), acg(NNODE, 3), acgm(NNODE, 3), &
+              phig(NNODE), rphig(NNODE), pg(NNODE))
+    read (10) time
+    read (10) dg, ug, ugm, acg, acgm, phig, rphig, pg, Tgold, rTgold
+    call writeSol(wstep)
+  end subroutine run_io
 
 end program ofd_refharness
 

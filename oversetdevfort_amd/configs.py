@@ -110,6 +110,27 @@ def weak_cavity(n=16, seed=1, rhoinf=0.5, jitter=0.05) -> Problem:
     return prob
 
 
+def from_part_file(path, params, fem_flag=1, weak_face_ids=()) -> Problem:
+    """A problem on the mesh of a reference part file (input_mesh,
+    input.f90:15-135): zero fields, no Dirichlet flags (setBCs sets them from
+    the run's parameters, which the file does not hold), and the faces of the
+    boundaries listed in ``weak_face_ids`` registered as weak-Dirichlet
+    (BCugType 2)."""
+    from . import refio
+    r = refio.read_part(path, fem_flag)
+    mesh = r["mesh"]
+    N = mesh.nnode
+    prob = Problem("part", mesh, np.zeros((N, 4), np.int32), np.zeros((N, 3)),
+                   _zero_fields(N), dict(params))
+    weak = [b for b in r["bounds"] if b["face_id"] in set(weak_face_ids)]
+    if weak:
+        prob.weak_faces = (np.concatenate([b["f2e"] for b in weak]),
+                           np.concatenate([b["face_or"] for b in weak]))
+    prob.node_id = r["node_id"]
+    prob.bounds = r["bounds"]
+    return prob
+
+
 def weak_channel(nx=64, ny=32, nz=32, seed=1, rhoinf=0.5) -> Problem:
     """The channel with its four walls weak-Dirichlet (BCugType 2,
     FaceAssembly_NS_weak + FaceAssembly_NS_outflow over their faces) instead

@@ -691,3 +691,30 @@ def overset_step(ctx: Context, fringe, xg, nnewton=3, rtol=1e-8, precision=L.OFD
                                kspace=kspace, inner_tol=inner_tol, max_cycles=max_cycles,
                                raise_noconv=raise_noconv, subdomain_flag=3))
     return out
+
+
+# writeSol's fields (iosol.f90:400-429): the current values, except the
+# temperature and its rate, which writeSol takes from the old arrays
+_WRITESOL = (("dg", "DG"), ("ug", "UG"), ("ugm", "UGM"), ("acg", "ACG"), ("acgm", "ACGM"),
+             ("phig", "PHIG"), ("rphig", "RPHIG"), ("pg", "PG"), ("Tg", "TGOLD"),
+             ("rTg", "RTGOLD"))
+# readSol's targets (iosol.f90:146-175): every field into the old arrays
+_READSOL = (("dg", "DGOLD"), ("ug", "UGOLD"), ("ugm", "UGMOLD"), ("acg", "ACGOLD"),
+            ("acgm", "ACGMOLD"), ("phig", "PHIGOLD"), ("rphig", "RPHIGOLD"), ("pg", "PGOLD"),
+            ("Tg", "TGOLD"), ("rTg", "RTGOLD"))
+
+
+def write_restart(ctx: Context, path, step, time):
+    """writeSol (iosol.f90:377-454) from the context's fields."""
+    from . import refio
+    refio.write_restart(str(path), step, time, {k: ctx.get_field(f) for k, f in _WRITESOL})
+
+
+def read_restart(ctx: Context, path):
+    """readSol (iosol.f90:122-194) into the context's old fields; returns
+    (step, time)."""
+    from . import refio
+    r = refio.read_restart(str(path), ctx.nnode)
+    for k, f in _READSOL:
+        ctx.set_field(f, r[k])
+    return r["step"], r["time"]

@@ -417,6 +417,86 @@ def face_case(name, dims, seed, tensor_flag, subdomain_flag, ngb, frac_ibc=0.15)
         rhsu=rhsu, rhsp=rhsp, **{"st_" + k: v for k, v in st.items()})
 
 
+def make_io():
+    """The mesh and restart files both ways: this build's write_part and
+    write_restart read by the reference's input_mesh and readSol (what they
+    read is stored), and a restart written by the reference's writeSol (its
+    text is stored) for this build's read_restart."""
+    from oversetdevfort_amd import refio as R
+    rng = np.random.default_rng(31)
+    m = M.structured_hex(3, 2, 2, jitter=0.1, seed=31)
+    N, E = m.nnode, m.nelem
+    m.elm_id[:] = rng.integers(1, 3, size=E)
+    node_id = rng.integers(1, 3, size=N).astype(np.int32)
+    bounds = [R.bound_from_faces(m, 11, *M.box_faces(m, ("xlo", "zhi"))),
+              R.bound_from_faces(m, 12, *M.box_faces(m, ("ylo",)))]
+
+    def field(*shape):
+        return rng.normal(size=shape) * 10.0 ** rng.integers(-12, 12, size=shape)
+
+    fields = {k: field(N, 3) if k in R._VECTOR else field(N) for k in R.RESTART_FIELDS}
+    wfields = {k: field(N, 3) if k in R._VECTOR else field(N) for k in R.RESTART_FIELDS}
+    t_read, t_write = 0.12345678901234568, 7.0e-3 / 3.0
+    with tempfile.TemporaryDirectory() as td:
+        R.write_part(os.path.join(td, R.part_name(1)), m, node_id, bounds)
+        R.write_l2b(os.path.join(td, R.l2b_name(1)), bounds)
+        for i, b in enumerate(bounds):
+            R.write_bmesh(os.path.join(td, R.bmesh_name(b["face_id"])), m, b, i + 1)
+        R.write_restart(os.path.join(td, R.restart_name(7)), 7, t_read, fields)
+        texts = {}
+        for key, name in (("part_text", R.part_name(1)), ("l2b_text", R.l2b_name(1)),
+                          ("bmesh0_text", R.bmesh_name(11)), ("bmesh1_text", R.bmesh_name(12)),
+                          ("restart_text", R.restart_name(7))):
+            with open(os.path.join(td, name)) as f:
+                texts[key] = np.array(f.read())
+        payload = (np.int32(7).tobytes() + np.array([1, 1, 7, 9, 11, 12], np.int32).tobytes() +
+                   np.float64(t_write).tobytes() +
+                   b"".join(_f64(wfields[k]) for k in R.RESTART_FIELDS))
+        fin = os.path.join(td, "in.bin")
+        fout = os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(payload)
+        subprocess.run([HARNESS, fin, fout], capture_output=True, text=True, check=True,
+                       cwd=td)
+        with open(fout, "rb") as f:
+            out = f.read()
+        with open(os.path.join(td, R.restart_name(9))) as f:
+            ref_restart_text = f.read()
+    r = _Reader(out)
+    nsd, nsh, nshlbmax, nnode, nelem, nbound, npatch = (int(v) for v in r.i32(7))
+    assert (nnode, nelem, nbound) == (N, E, 2)
+    got = dict(xg=r.f64(N, 3), node_id=r.i32(N), elmnshl=r.i32(E), ien=r.i32(E, nsh) - 1,
+               elm_id=r.i32(E))
+    for b in range(nbound):
+        face_id, nface, bnn = (int(v) for v in r.i32(3))
+        got[f"b{b}_face_id"] = face_id
+        got[f"b{b}_face_ien"] = r.i32(nface, nshlbmax) - 1
+        got[f"b{b}_nshlb"] = r.i32(nface)
+        got[f"b{b}_f2e"] = r.i32(nface) - 1
+        got[f"b{b}_face_or"] = r.i32(nface)
+        got[f"b{b}_bnodes"] = r.i32(bnn) - 1
+        got[f"b{b}_l2snode"] = r.i32(bnn) - 1
+        got[f"b{b}_l2selem"] = r.i32(nface) - 1
+    for b in range(2):
+        nn = int(r.i32(1)[0])
+        got[f"bmesh{b}_xg"] = r.f64(nn, 3)
+        got[f"bmesh{b}_owned"] = r.i32(nn)
+    got["rstep"] = int(r.i32(1)[0])
+    got["time"] = float(r.f64(1)[0])
+    for k, old in zip(R.RESTART_FIELDS, ("dgold", "ugold", "ugmold", "acgold", "acgmold",
+                                         "phigold", "rphigold", "pgold", "Tgold", "rTgold")):
+        got["read_" + k] = r.f64(N, 3) if k in R._VECTOR else r.f64(N)
+    assert r.off == len(out)
+    np.savez_compressed(
+        os.path.join(HERE, "io_files.npz"), xg=m.xg, ien=m.ien, elm_id=m.elm_id,
+        node_id=node_id, ref_restart_text=np.array(ref_restart_text), t_read=t_read,
+        t_write=t_write, **texts,
+        **{"bound_" + k + str(i): np.asarray(v) for i, b in enumerate(bounds)
+           for k, v in b.items()},
+        **{"in_" + k: v for k, v in fields.items()}, **{"w_" + k: v for k, v in wfields.items()},
+        **{"ref_" + k: np.asarray(v) for k, v in got.items()})
+
+
 def make_face():
     """Weak-Dirichlet and outflow faces: 3-point faces with both subdomains,
     1-point faces with one subdomain, and the vector-only call."""
@@ -440,6 +520,9 @@ def main():
     if sys.argv[1:] == ["face"]:
         make_face()
         return
+    if sys.argv[1:] == ["io"]:
+        make_io()
+        return
     make_shape()
     # element kernels + BC + scatter + SpMV + a fixed-length GMRES cycle
     ns_case("ns_tet_a", (2, 2, 3), seed=3, tensor_flag=6, subdomain_flag=1,
@@ -455,6 +538,7 @@ def main():
     make_nsvof()
     make_vms()
     make_face()
+    make_io()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))
