@@ -578,6 +578,9 @@ def main():
                     help="only the channel Newton steps and their kernel figures "
                          "(for profiled runs); no quench, fringe, overset, adjoint "
                          "or host sections")
+    ap.add_argument("--faces-pattern-only", action="store_true",
+                    help="only the weak-face and device pattern sections (for profiled "
+                         "runs of those kernels); no JSON line")
     ap.add_argument("--pmc-sample", action="store_true",
                     help="short run for hardware-counter passes: a 256 MiB copy "
                          "(calibration) and one full GMRES(30) cycle per precision")
@@ -591,6 +594,11 @@ def main():
     from oversetdevfort_amd import _lib as L
     from oversetdevfort_amd import configs as C
     from oversetdevfort_amd import solver as S
+
+    if args.faces_pattern_only:
+        weak_faces(L, C, S)
+        pattern_build(S)
+        return
 
     nx, ny, nz = (int(v) for v in args.dims.split(","))
     prob = C.channel(nx, ny, nz)
