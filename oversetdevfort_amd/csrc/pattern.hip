@@ -13,7 +13,8 @@
 //                               col, a second pass writes row and the
 //                               diagonal position (SparseGMRES_NS.f90:53-61)
 //   tmap                        per nonzero (i, j), the position of (j, i)
-//                               (the pattern is symmetric)
+//                               (the pattern is symmetric), one thread per
+//                               nonzero
 //   emap                        per element and node pair (a, b), the nonzero
 //                               of (ien[a], ien[b]): SparseMatLoc_3D's search
 //                               done once, at e*64 + a + 8 b
@@ -102,19 +103,26 @@ __device__ __forceinline__ int lower_bound(const int32_t *__restrict__ row, int 
   return lo;
 }
 
-__global__ void k_tmap(int N, const int32_t *__restrict__ col, const int32_t *__restrict__ row,
-                       int32_t *__restrict__ tmap, int32_t *__restrict__ err) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  for (int k = col[i]; k < col[i + 1]; ++k) {
-    const int j = row[k];
-    const int p = lower_bound(row, col[j], col[j + 1], i);
-    if (p == col[j + 1] || row[p] != i) {
-      atomicOr(err, 2);
-      tmap[k] = 0;
-    } else {
-      tmap[k] = p;
-    }
+// one thread per nonzero k: its row i is the last with col[i] <= k
+__global__ void k_tmap(int N, size_t icnt, const int32_t *__restrict__ col,
+                       const int32_t *__restrict__ row, int32_t *__restrict__ tmap,
+                       int32_t *__restrict__ err) {
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= icnt) return;
+  int lo = 0, hi = N;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if ((size_t)col[mid] <= k) lo = mid + 1;
+    else hi = mid;
+  }
+  const int i = lo - 1;
+  const int j = row[k];
+  const int p = lower_bound(row, col[j], col[j + 1], i);
+  if (p == col[j + 1] || row[p] != i) {
+    atomicOr(err, 2);
+    tmap[k] = 0;
+  } else {
+    tmap[k] = p;
   }
 }
 
@@ -154,8 +162,9 @@ hipError_t launch_adjacency(Ctx &c, const int32_t *off, const int32_t *inc, cons
 }
 
 hipError_t launch_tmap_emap(Ctx &c, int32_t *err) {
-  hipLaunchKernelGGL(k_tmap, dim3((c.nnode + 255) / 256), dim3(256), 0, c.stream, c.nnode,
-                     c.d_col, c.d_row, c.d_tmap, err);
+  const size_t nz = (size_t)c.icnt;
+  hipLaunchKernelGGL(k_tmap, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, c.stream,
+                     c.nnode, nz, c.d_col, c.d_row, c.d_tmap, err);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const size_t nt = (size_t)c.nelem * 64;
