@@ -97,8 +97,15 @@ struct AsmParams {
   int vms;  // 1: the residual with the fine-scale velocity uprime = -tauM rLi
 };
 
+// S = 8: the trilinear hex, 8 Gauss points (the layout above).  S = 4: the
+// linear tet of the reference's live dispatch (lintetshl, eval_shape_FEM.f90:
+// 198-226; the 4-point rule of genGPandGWtetpri), padded to the same 8x8
+// lanes: node lanes a >= 4 carry zero shape values and gradients (the
+// butterfly sums are unchanged), point lanes q >= 4 repeat points 0..3 with
+// zero weight, and the padded (a, b) pairs are not scattered.
+template <int S>
 __global__ void __launch_bounds__(256)
-k_assemble_ns_hex(int N, int E, const int32_t *__restrict__ ien,
+k_assemble_ns_el(int N, int E, const int32_t *__restrict__ ien,
                   const int32_t *__restrict__ elm_id,
                   const int32_t *__restrict__ elems, int first, int count,
                   const int32_t *__restrict__ emap,
@@ -129,7 +136,7 @@ k_assemble_ns_hex(int N, int E, const int32_t *__restrict__ ien,
   // ---------------- phase A: lane = (q, a) ----------------
   const int q = lane >> 3;
   const int a = lane & 7;
-  const int na = valid ? ien[(size_t)a * E + e] : 0;
+  const int na = valid ? ien[(size_t)(a < S ? a : 0) * E + e] : 0;
   if (lane < 8) s_node[w][lane] = na;
 
   const double2 *r2 = reinterpret_cast<const double2 *>(rec + (size_t)na * 16);
@@ -147,19 +154,37 @@ k_assemble_ns_hex(int N, int E, const int32_t *__restrict__ ien,
   const double *uma = xr + 9;  // um
   const double pa = xr[12];
 
-  // reference shape functions (linhexshl) at Gauss point q
-  const double gpt = 0.57735026918963;
-  const double xi = (q & 1) ? gpt : -gpt;
-  const double eta = (q & 2) ? gpt : -gpt;
-  const double zeta = (q & 4) ? gpt : -gpt;
-  const double sx = c_sign[a][0], sy = c_sign[a][1], sz = c_sign[a][2];
-  const double fx = (sx > 0) ? (xi + 1.0) : (1.0 - xi);
-  const double fy = (sy > 0) ? (eta + 1.0) : (1.0 - eta);
-  const double fz = (sz > 0) ? (zeta + 1.0) : (1.0 - zeta);
-  const double Na = fx * fy * fz * 0.125;
-  const double dl0 = sx * (fy * fz * 0.125);
-  const double dl1 = sy * (fx * fz * 0.125);
-  const double dl2 = sz * (fx * fy * 0.125);
+  // reference shape functions at Gauss point q
+  double Na, dl0, dl1, dl2, gwq;
+  if constexpr (S == 8) {
+    // linhexshl, 8-point rule with unit weights
+    const double gpt = 0.57735026918963;
+    const double xi = (q & 1) ? gpt : -gpt;
+    const double eta = (q & 2) ? gpt : -gpt;
+    const double zeta = (q & 4) ? gpt : -gpt;
+    const double sx = c_sign[a][0], sy = c_sign[a][1], sz = c_sign[a][2];
+    const double fx = (sx > 0) ? (xi + 1.0) : (1.0 - xi);
+    const double fy = (sy > 0) ? (eta + 1.0) : (1.0 - eta);
+    const double fz = (sz > 0) ? (zeta + 1.0) : (1.0 - zeta);
+    Na = fx * fy * fz * 0.125;
+    dl0 = sx * (fy * fz * 0.125);
+    dl1 = sy * (fx * fz * 0.125);
+    dl2 = sz * (fx * fy * 0.125);
+    gwq = 1.0;
+  } else {
+    // lintetshl at the 4-point rule (a, b, b), (b, a, b), (b, b, a), (b, b, b)
+    const double ga_ = 0.5854101966249685, gb_ = 0.1381966011250105;
+    const int q4 = q & 3;
+    const double xi = (q4 == 0) ? ga_ : gb_;
+    const double eta = (q4 == 1) ? ga_ : gb_;
+    const double zeta = (q4 == 2) ? ga_ : gb_;
+    Na = (a == 0) ? xi : (a == 1) ? eta : (a == 2) ? zeta
+                                  : (a == 3) ? 1.0 - xi - eta - zeta : 0.0;
+    dl0 = (a == 0) ? 1.0 : (a == 3) ? -1.0 : 0.0;
+    dl1 = (a == 1) ? 1.0 : (a == 3) ? -1.0 : 0.0;
+    dl2 = (a == 2) ? 1.0 : (a == 3) ? -1.0 : 0.0;
+    gwq = (q < 4) ? 0.2500000000000000 / 6.0 : 0.0;
+  }
 
   // dx/dxi (i,j) = sum_a (x+d)_a,i dN_a/dxi_j
   double J[9];
@@ -244,8 +269,8 @@ k_assemble_ns_hex(int N, int E, const int32_t *__restrict__ ien,
 #pragma unroll
     for (int j = 0; j < 3; ++j) cflq += uadv[i] * G[i + 3 * j] * uadv[j];
   cflq = P.delt * sqrt(cflq);
-  const double dw = detj;  // 8-point rule weights are 1
-  if (valid && a == 0 && detj < 0.0) atomicAdd(negdet, 1);
+  const double dw = detj * gwq;  // hex weights are 1; padded tet points 0
+  if (valid && a == 0 && (S == 8 || q < 4) && detj < 0.0) atomicAdd(negdet, 1);
 
   const double tmp1a = rho * (uadv[0] * ga[0] + uadv[1] * ga[1] + uadv[2] * ga[2]);
   s_sh[w][q][a][0] = Na;
@@ -335,7 +360,7 @@ k_assemble_ns_hex(int N, int E, const int32_t *__restrict__ ien,
     const double fact1 = P.almi;
     const double fact2 = P.alfi * P.gami * P.delt;
 #pragma unroll 2
-    for (int qq = 0; qq < 8; ++qq) {
+    for (int qq = 0; qq < (S == 8 ? 8 : 4); ++qq) {
       const double mupkdc = s_q[w][qq][4] + P.kdc;
       const double Nq = s_sh[w][qq][pa_][0];
       const double g0 = s_sh[w][qq][pa_][1], g1 = s_sh[w][qq][pa_][2],
@@ -372,7 +397,7 @@ k_assemble_ns_hex(int N, int E, const int32_t *__restrict__ ien,
   }
 
   // ---------------- phase C: Dirichlet projection + scatter ----------------
-  if (valid) {
+  if (valid && pa_ < S && pb_ < S) {
     const int nra = s_node[w][pa_];
     const int nrb = s_node[w][pb_];
     const unsigned ma = ibc[nra];
@@ -399,8 +424,8 @@ k_assemble_ns_hex(int N, int E, const int32_t *__restrict__ ien,
         blk[t] = old;
       }
     }
-    if (do_vec && lane < 8) {
-      // lane < 8 holds the q-reduced RHS of node a = lane
+    if (do_vec && lane < S) {
+      // lane < S holds the q-reduced RHS of node a = lane
       const unsigned m0 = ibc[na];
       double o[4] = {rcu[0], rcu[1], rcu[2], rcp};
 #pragma unroll
@@ -472,7 +497,8 @@ hipError_t launch_assemble_ns_hex(Ctx &c, int tensor_flag, int subdomain_flag,
   P.gz = c.prm[OFD_P_GZ];
   P.vms = c.res_form;
   const int blocks = (count + kWavesPerBlock - 1) / kWavesPerBlock;
-  hipLaunchKernelGGL(k_assemble_ns_hex, dim3(blocks), dim3(64 * kWavesPerBlock), 0,
+  hipLaunchKernelGGL((c.nshl == 4 ? k_assemble_ns_el<4> : k_assemble_ns_el<8>), dim3(blocks),
+                     dim3(64 * kWavesPerBlock), 0,
                      c.stream, c.nnode, c.nelem, c.d_ien, c.d_elm_id,
                      c.d_color_elems, first, count, c.d_emap, c.d_noderec,
                      c.mus_on ? c.d_field[OFD_MUSCALE] : nullptr, c.d_ibc,

@@ -853,8 +853,8 @@ int ofd_assemble_ns(ofd_ctx *x, int tensor_flag, int subdomain_flag, double *cfl
   Ctx &c = x->c;
   int rc = need_pattern(c);
   if (rc) return rc;
-  OFD_CHECK(c.nshl == 8, OFD_ERR_UNSUPPORTED,
-            "the GPU element assembly is implemented for trilinear hexes (NSHL=8)");
+  OFD_CHECK(c.nshl == 8 || c.nshl == 4, OFD_ERR_UNSUPPORTED,
+            "the GPU NS assembly is implemented for trilinear hexes and linear tets");
   OFD_CHECK((tensor_flag & ~(OFD_ASSEMBLE_VEC | OFD_ASSEMBLE_MAT)) == 0, OFD_ERR_ARG,
             "tensor_flag must combine OFD_ASSEMBLE_VEC / OFD_ASSEMBLE_MAT");
   OFD_HIP(hipSetDevice(c.device));
@@ -1253,8 +1253,8 @@ int ofd_time_assemble(ofd_ctx *x, int tensor_flag, int reps, double *ms) {
   if (rc) return rc;
   rc = ensure_faces(c);
   if (rc) return rc;
-  OFD_CHECK(c.nshl == 8, OFD_ERR_UNSUPPORTED,
-            "the GPU element assembly is implemented for trilinear hexes (NSHL=8)");
+  OFD_CHECK(c.nshl == 8 || c.nshl == 4, OFD_ERR_UNSUPPORTED,
+            "the GPU NS assembly is implemented for trilinear hexes and linear tets");
   OFD_HIP(hipSetDevice(c.device));
   hipEvent_t e0, e1;
   OFD_HIP(hipEventCreate(&e0));
