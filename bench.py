@@ -443,6 +443,45 @@ def weak_faces(L, C, S, reps=20):
     return out, prob
 
 
+def tet_assembly(C, S, reps=5):
+    """The 64x32x32 channel's hexes each split into 6 Kuhn tets (393,216
+    linear tets on the same 70,785 nodes, 4 Gauss points each):
+    IntElmAss_NS_overset + BCLhs_3D + FillSparseMat_3D on the GPU
+    (ofd_time_assemble, HIP events on the context stream), VEC+MAT and VEC."""
+    from oversetdevfort_amd import mesh as M
+    prob = C.channel(64, 32, 32)
+    prob.mesh = M.hex_to_tets(prob.mesh)
+    E = prob.mesh.nelem
+    ctx = S.setup(prob, device=0)
+    S.begin_step(ctx)
+    vm_ms = ctx.time_assemble(6, reps=reps)
+    v_ms = ctx.time_assemble(2, reps=reps)
+    ctx.close()
+    out = {"workload": "channel 64x32x32 split into 6 tets per hex, "
+                       f"{E} linear tets x 4 Gauss points",
+           "elements": E, "vecmat_ms": vm_ms, "vec_ms": v_ms,
+           "vecmat_melem_per_s": E / (vm_ms * 1e-3) / 1e6}
+    log(f"bench: tet assembly: {E} tets, VEC+MAT {vm_ms:.3f} ms "
+        f"({out['vecmat_melem_per_s']:.0f} Melem/s), VEC {v_ms:.3f} ms")
+    return out, prob
+
+
+def cpu_tet_assembly(prob):
+    """Host (C oracle, one thread) VEC+MAT assembly of the same tet mesh:
+    one call of the oracle's assembly (array marshalling included)."""
+    import numpy as np
+
+    from oracle import oracle as O
+    m = prob.mesh
+    col, row = O.sparsity(m.nnode, m.ien)
+    st = {k: np.asarray(prob.fields[k], np.float64) for k in ("dg", "ug", "ugm", "acg", "acgm", "pg")}
+    prm = O.params_array(prob.params)
+    t0 = time.perf_counter()
+    O.assemble_ns(m, prob.ibc, st, prm, col, row, 6, 1)
+    t1 = time.perf_counter()
+    return {"vecmat_ms": 1e3 * (t1 - t0), "vecmat_melem_per_s": m.nelem / (t1 - t0) / 1e6}
+
+
 def pattern_build(S, reps=3):
     """genSparStruc of the 128x64x64 hex mesh (545,025 nodes) on the device:
     host wall time of ofd_build_pattern (incidence, rows, diagonal, transpose
@@ -597,6 +636,7 @@ def main():
 
     if args.faces_pattern_only:
         weak_faces(L, C, S)
+        tet_assembly(C, S)
         pattern_build(S)
         return
 
@@ -668,7 +708,7 @@ def main():
         f"{asm_tf / 1e12:.2f} TFLOP/s); VEC {asm_vec_ms:.3f} ms")
     log(f"bench: SpMV FP64 {spmv64_ms * 1e3:.1f} us {bw64 / 1e9:.0f} GB/s; "
         f"FP32-matrix {spmv32_ms * 1e3:.1f} us {bw32 / 1e9:.0f} GB/s")
-    adjoint = quench = fringe = faces = pattern = None
+    adjoint = quench = fringe = faces = tets = pattern = None
     if not args.headline_only:
         adjoint = adjoint_ns(ctx, L, prob, asm_vec_ms)
         adjoint["newton_step_fp64_ms"] = res64["newton_ms"]
@@ -680,6 +720,7 @@ def main():
         fringe["coupled_steps"] = overset_steps(L, C, S)
         adjoint["quench_transient"] = adjoint_c5(L, C, S)
         faces, fprob = weak_faces(L, C, S)
+        tets, tprob = tet_assembly(C, S)
         pattern, pmesh = pattern_build(S)
 
     cpu = None
@@ -715,6 +756,9 @@ def main():
         cf = cpu_faces(fprob)
         log(f"bench: host weak faces: {cf['face_vecmat_ms']:.0f} ms")
         cpu["weak_faces"] = cf
+        ct = cpu_tet_assembly(tprob)
+        log(f"bench: host tet assembly: {ct['vecmat_ms']:.0f} ms")
+        cpu["tet_assembly"] = ct
         cp = cpu_pattern(pmesh)
         log(f"bench: host pattern build: {cp['ms']:.0f} ms")
         cpu["pattern"] = cp
@@ -764,6 +808,7 @@ def main():
         "quench": quench,
         "fringe": fringe,
         "weak_faces": faces,
+        "tet_assembly": tets,
         "pattern": pattern,
         "cpu_baseline": cpu,
     }
