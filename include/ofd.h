@@ -108,6 +108,16 @@ int ofd_upload_mesh(ofd_ctx *ctx, int nnode, int nelem, int nshl,
 int ofd_build_pattern(ofd_ctx *ctx, int64_t *icnt);
 /* the pattern in the reference's 1-based form */
 int ofd_get_pattern(ofd_ctx *ctx, int32_t *col, int32_t *row);
+/* Checks a caller's pattern against the context's: the col(NNODZu+1),
+ * row(icntu) arguments of SparseProdUP_3D (SparseProd.f90:4-7),
+ * SparseGMRES_up (SparseGMRES_NS.f90:4-8), SparseProd_LS (SparseProd.f90:
+ * 128-132), SparseGMRES_tem (SparseGMRES_tem.f90:1-3), SparseProd_NS_conv
+ * (SparseProd.f90:167-172) and SparseGMRES_ns_conv (SparseGMRES_NS_conv.f:
+ * 1-6), which index the caller's matrices with them.  OFD_ERR_ARG unless
+ * nnode, icnt and every entry of col and row (1-based) equal the context's;
+ * the drop-ins call it before they read a matrix of icnt blocks. */
+int ofd_check_pattern(ofd_ctx *ctx, int nnode, int64_t icnt, const int32_t *col,
+                      const int32_t *row);
 
 /* physics and time-integration parameters (getparam, param.f90:205-490):
  * prm[OFD_NPARAMS] = almi alfi gami beti Delt rho mu NS_kdc gravvec(3) */

@@ -61,6 +61,7 @@ SIGNATURES = {
     "ofd_upload_mesh": ([_vp, _c_int, _c_int, _c_int, _vp, _vp, _vp], _c_int),
     "ofd_build_pattern": ([_vp, _i64p], _c_int),
     "ofd_get_pattern": ([_vp, _vp, _vp], _c_int),
+    "ofd_check_pattern": ([_vp, _c_int, ctypes.c_int64, _vp, _vp], _c_int),
     "ofd_set_params": ([_vp, _vp], _c_int),
     "ofd_set_subdomains": ([_vp, _c_int, _c_int], _c_int),
     "ofd_set_ibc": ([_vp, _vp, _c_int], _c_int),

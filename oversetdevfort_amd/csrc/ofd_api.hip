@@ -538,6 +538,13 @@ int ofd_upload_mesh(ofd_ctx *x, int nnode, int nelem, int nshl, const int32_t *i
   ++c.matT_version;
   c.mus_on = false;
   c.ks_on = false;
+  // the weak faces name elements of the old mesh
+  c.h_face_e.clear();
+  c.h_face_or.clear();
+  c.fe_color_off.clear();
+  c.fe_dirty = false;
+  dfree(c.d_fe_elem);
+  dfree(c.d_fe_mask);
   free_tem_transpose(c);
   free_tem_adjoint(c);
   // adjoint buffers are sized by the mesh and the pattern
@@ -645,19 +652,35 @@ int ofd_build_pattern(ofd_ctx *x, int64_t *icnt_out) {
   c.h_row.resize(c.icnt);
   rc = d2h(c, c.h_row.data(), c.d_row, c.icnt * sizeof(int32_t));
   if (rc) return rc;
-  // greedy colouring: elements of one colour share no node
-  std::vector<uint32_t> nodemask(N, 0u);
+  // greedy colouring: elements of one colour share no node.  Each node keeps
+  // the colours of its elements as a bitset of nw 64-bit words, widened when
+  // an element needs a colour past 64 nw (first fit needs at least as many
+  // colours as the most elements at one node: 8 on a structured hex mesh,
+  // tens on an unstructured tet mesh)
+  int nw = 1;
+  std::vector<uint64_t> nodemask((size_t)N, 0ull);
   std::vector<int32_t> ecol(E);
+  std::vector<uint64_t> used;
   int ncolor = 0;
   for (int e = 0; e < E; ++e) {
-    uint32_t used = 0;
-    for (int a = 0; a < S; ++a) used |= nodemask[c.h_ien[(size_t)a * E + e]];
-    OFD_CHECK(used != 0xffffffffu, OFD_ERR_UNSUPPORTED, "element colouring needs > 32 colours");
+    used.assign(nw, 0ull);
+    for (int a = 0; a < S; ++a) {
+      const uint64_t *m = &nodemask[(size_t)c.h_ien[(size_t)a * E + e] * nw];
+      for (int w = 0; w < nw; ++w) used[w] |= m[w];
+    }
     int col = 0;
-    while (used & (1u << col)) ++col;
+    while (col < 64 * nw && ((used[col >> 6] >> (col & 63)) & 1ull)) ++col;
+    if (col == 64 * nw) {
+      std::vector<uint64_t> wider((size_t)N * (nw + 1), 0ull);
+      for (int i = 0; i < N; ++i)
+        for (int w = 0; w < nw; ++w) wider[(size_t)i * (nw + 1) + w] = nodemask[(size_t)i * nw + w];
+      nodemask.swap(wider);
+      ++nw;
+    }
     ecol[e] = col;
     ncolor = std::max(ncolor, col + 1);
-    for (int a = 0; a < S; ++a) nodemask[c.h_ien[(size_t)a * E + e]] |= (1u << col);
+    for (int a = 0; a < S; ++a)
+      nodemask[(size_t)c.h_ien[(size_t)a * E + e] * nw + (col >> 6)] |= 1ull << (col & 63);
   }
   c.color_off.assign(ncolor + 1, 0);
   for (int e = 0; e < E; ++e) c.color_off[ecol[e] + 1]++;
@@ -698,6 +721,27 @@ int ofd_get_pattern(ofd_ctx *x, int32_t *col, int32_t *row) {
   if (rc) return rc;
   for (int i = 0; i <= c.nnode; ++i) col[i] = c.h_col[i] + 1;
   for (int64_t k = 0; k < c.icnt; ++k) row[k] = c.h_row[k] + 1;
+  return OFD_OK;
+}
+
+int ofd_check_pattern(ofd_ctx *x, int nnode, int64_t icnt, const int32_t *col,
+                      const int32_t *row) {
+  OFD_CHECK(x && col && row, OFD_ERR_ARG, "null argument");
+  Ctx &c = x->c;
+  int rc = need_pattern(c);
+  if (rc) return rc;
+  OFD_CHECK(nnode == c.nnode, OFD_ERR_ARG,
+            "pattern check: NNODZu " + std::to_string(nnode) + " differs from the context's " +
+                std::to_string(c.nnode));
+  OFD_CHECK(icnt == c.icnt, OFD_ERR_ARG,
+            "pattern check: icntu " + std::to_string(icnt) + " differs from the context's " +
+                std::to_string(c.icnt));
+  for (int i = 0; i <= nnode; ++i)
+    OFD_CHECK(col[i] - 1 == c.h_col[i], OFD_ERR_ARG,
+              "pattern check: col(" + std::to_string(i + 1) + ") differs from the context's");
+  for (int64_t k = 0; k < icnt; ++k)
+    OFD_CHECK(row[k] - 1 == c.h_row[k], OFD_ERR_ARG,
+              "pattern check: row(" + std::to_string(k + 1) + ") differs from the context's");
   return OFD_OK;
 }
 
@@ -789,6 +833,8 @@ int ofd_get_field(ofd_ctx *x, int field, double *host) {
 // the elements holding registered weak faces, grouped by element colour,
 // with their faces as a bitmask (bit fo-1), on the device
 static int ensure_faces(Ctx &c) {
+  OFD_CHECK(c.h_face_e.empty() || c.nshl == 8, OFD_ERR_UNSUPPORTED,
+            "boundary faces are implemented for trilinear hexes (NSHL=8)");
   if (!c.fe_dirty) return OFD_OK;
   c.fe_color_off.clear();
   if (c.h_face_e.empty()) {
@@ -1006,6 +1052,10 @@ int ofd_gmres_up(ofd_ctx *x, int precision, int ortho, double tol, int kspace,
   if (rc) return rc;
   OFD_HIP(hipSetDevice(c.device));
   OFD_HIP(hipMemsetAsync(c.d_x, 0, (size_t)c.nnode * 4 * sizeof(double), c.stream));
+  double bn = 0.0;
+  rc = norm2_dev(c, c.d_rhs, &bn);
+  if (rc) return rc;
+  OFD_CHECK(std::isfinite(bn), OFD_ERR_NONFINITE, "SparseGMRES_up: non-finite right-hand side");
   OFD_HIP(launch_outer_reset(c, 0.0));
   int it = 0;
   double res = 0.0;
@@ -1512,6 +1562,7 @@ int ofd_gmres_tem(ofd_ctx *x, int precision, int ortho, double tol, int kspace,
   double bn = 0.0;
   rc = norm2_dev(c, c.d_rhsT, &bn);
   if (rc) return rc;
+  OFD_CHECK(std::isfinite(bn), OFD_ERR_NONFINITE, "SparseGMRES_tem: non-finite right-hand side");
   int it = 0;
   double res = 0.0;
   if (bn > 0.0) {
@@ -1804,6 +1855,11 @@ int ofd_gmres_nsvof(ofd_ctx *x, int precision, int ortho, double tol, int kspace
   OFD_HIP(hipSetDevice(c.device));
   NsvofSystem sys(c);
   OFD_HIP(hipMemsetAsync(c.d_xV, 0, (size_t)c.gm_n * 4 * sizeof(double), c.stream));
+  double bn = 0.0;
+  rc = norm2_dev(c, c.d_rhsV, &bn);
+  if (rc) return rc;
+  OFD_CHECK(std::isfinite(bn), OFD_ERR_NONFINITE,
+            "SparseGMRES_ns_conv: non-finite right-hand side");
   OFD_HIP(launch_outer_reset(c, 0.0));
   int it = 0;
   double res = 0.0;
@@ -2183,6 +2239,7 @@ static int fgmres(Ctx &c, size_t n, Apply &&apply, Prec &&prec, const double *b,
   double bnorm = 0.0;
   int rc = norm2_dev(c, b, &bnorm);
   if (rc) return rc;
+  OFD_CHECK(std::isfinite(bnorm), OFD_ERR_NONFINITE, "fgmres: non-finite right-hand side");
   int total = 0, cyc = 0;
   double rel = bnorm > 0.0 ? 1.0 : 0.0;
   std::vector<double> hist;

@@ -71,6 +71,16 @@ module ofd_iface
       integer(c_int) :: rc
     end function
 
+    function ofd_check_pattern(ctx, nnode, icnt, col, row) &
+        bind(C, name='ofd_check_pattern') result(rc)
+      import :: c_int, c_ptr, c_int32_t, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nnode
+      integer(c_int64_t), value :: icnt
+      integer(c_int32_t), intent(in) :: col(*), row(*)
+      integer(c_int) :: rc
+    end function
+
     function ofd_set_params(ctx, prm) bind(C, name='ofd_set_params') result(rc)
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx

@@ -50,6 +50,8 @@ subroutine SparseProdUP_3D(col, row, lhsK11, lhsG, lhsD1, lhsM, &
                          lhsD1(NSD, icntu), lhsM(icntu), &
                          rhstmpu(NNODZu, NSD), rhstmpp(NNODZu)
   real(8), intent(out) :: prodtmpu(NNODZu, NSD), prodtmpp(NNODZu)
+  call ofd_check(ofd_check_pattern(ofd_ctx, int(NNODZu, c_int), int(icntu, c_int64_t), &
+                                   col, row), 'SparseProdUP_3D')
   call ofd_check(ofd_set_lhs(ofd_ctx, lhsK11, lhsG, lhsD1, lhsM), 'SparseProdUP_3D')
   call ofd_check(ofd_spmv_up(ofd_ctx, OFD_FP64, rhstmpu, rhstmpp, prodtmpu, &
                              prodtmpp), 'SparseProdUP_3D')
@@ -76,6 +78,8 @@ subroutine SparseGMRES_up(col, row, IBC, IPER, D_FLAG, P_FLAG, &
   real(8), allocatable :: du(:, :), dp(:)
   integer(c_int) :: iters
   real(c_double) :: resid
+  call ofd_check(ofd_check_pattern(ofd_ctx, int(NNODZu, c_int), int(icntu, c_int64_t), &
+                                   col, row), 'SparseGMRES_up')
   allocate (du(NNODZu, NSD), dp(NNODZu))
   call ofd_check(ofd_set_lhs(ofd_ctx, lhsK11, lhsG, lhsD1, lhsM), 'SparseGMRES_up')
   call ofd_check(ofd_set_rhs(ofd_ctx, rhsGu, rhsGp), 'SparseGMRES_up')
@@ -102,6 +106,8 @@ subroutine SparseGMRES_tem(lhsM, Utol, col, row, rhsG, sol, Kspaceu, Kspaceu_mn,
   real(8), allocatable :: dT(:)
   integer(c_int) :: iters
   real(c_double) :: resid
+  call ofd_check(ofd_check_pattern(ofd_ctx, int(NNODZu, c_int), int(icntu, c_int64_t), &
+                                   col, row), 'SparseGMRES_tem')
   allocate (dT(NNODZu))
   call ofd_check(ofd_set_tem_system(ofd_ctx, lhsM, rhsG), 'SparseGMRES_tem')
   call ofd_check(ofd_gmres_tem(ofd_ctx, ofd_precision, ofd_ortho, Utol, &
@@ -125,6 +131,8 @@ subroutine SparseProd_LS(lhsM, col, row, rhstmp, prodtmp, NNODZu, NSHLu, icntu, 
   integer, intent(in) :: col(NNODZu + 1), row(icntu)
   real(8), intent(in) :: lhsM(icntu), rhstmp(NNODZu)
   real(8), intent(out) :: prodtmp(NNODZu)
+  call ofd_check(ofd_check_pattern(ofd_ctx, int(NNODZu, c_int), int(icntu, c_int64_t), &
+                                   col, row), 'SparseProd_LS')
   call ofd_check(ofd_set_tem_system(ofd_ctx, lhs=lhsM), 'SparseProd_LS')
   call ofd_check(ofd_spmv_tem(ofd_ctx, OFD_FP64, rhstmp, prodtmp), 'SparseProd_LS')
 end subroutine SparseProd_LS
@@ -145,6 +153,8 @@ subroutine SparseProd_NS_conv(col, row, lhsK11, lhsG, lhsD1, lhsM, lhsLS, lhsLSu
                          LHSUls(NSD, icntu), rhstmpu(NNODZu, NSD), rhstmpp(NNODZu), &
                          rhstmpls(NNODZu)
   real(8), intent(out) :: prodtmpu(NNODZu, NSD), prodtmpp(NNODZu), prodtmpls(NNODZu)
+  call ofd_check(ofd_check_pattern(ofd_ctx, int(NNODZu, c_int), int(icntu, c_int64_t), &
+                                   col, row), 'SparseProd_NS_conv')
   call ofd_check(ofd_set_nsvof_system(ofd_ctx, lhsK11, lhsG, lhsD1, lhsM, lhsLS, lhsLSu, &
                                       LHSPls, LHSUls), 'SparseProd_NS_conv')
   call ofd_check(ofd_spmv_nsvof(ofd_ctx, OFD_FP64, rhstmpu, rhstmpp, rhstmpls, prodtmpu, &
@@ -172,6 +182,8 @@ subroutine SparseGMRES_ns_conv(col, row, IBC, IPER, D_FLAG, P_FLAG, rhsGu, rhsGp
   real(8), allocatable :: du(:, :), dp(:), dl(:)
   integer(c_int) :: iters
   real(c_double) :: resid
+  call ofd_check(ofd_check_pattern(ofd_ctx, int(NNODZu, c_int), int(icntu, c_int64_t), &
+                                   col, row), 'SparseGMRES_ns_conv')
   allocate (du(NNODZu, NSD), dp(NNODZu), dl(NNODZu))
   call ofd_check(ofd_set_nsvof_system(ofd_ctx, lhsK11, lhsG, lhsD1, lhsM, lhsls, LHSlsu, &
                                       LHSPls, LHSUls, rhsGu, rhsGp, rhsGls), &

@@ -85,6 +85,14 @@ class Context:
         L.check(self.lib.ofd_get_pattern(self.h, _ptr(col), _ptr(row)))
         return col - 1, row - 1
 
+    def check_pattern(self, col, row):
+        """ofd_check_pattern with a 0-based (col, row): raises OfdError
+        (OFD_ERR_ARG) unless it is the context's pattern."""
+        c = np.ascontiguousarray(np.asarray(col, np.int32) + 1)
+        r = np.ascontiguousarray(np.asarray(row, np.int32) + 1)
+        L.check(self.lib.ofd_check_pattern(self.h, int(c.size) - 1, int(r.size), _ptr(c),
+                                           _ptr(r)))
+
     def build_pattern(self):
         n = ctypes.c_int64()
         L.check(self.lib.ofd_build_pattern(self.h, ctypes.byref(n)))
