@@ -138,6 +138,27 @@ def pmc_sample(ctx, S, L, prob, ortho, kspace):
         log(f"pmc sample: precision {prec}: {g['iters']} Arnoldi steps")
 
 
+def assembly_sample(L, C, S, reps=3):
+    """Assembly launches alone, for hardware-counter and kernel-trace runs."""
+    nx, ny, nz = 64, 32, 32
+    prob = C.channel(nx, ny, nz)
+    ctx = S.setup(prob, device=0)
+    ms = ctx.time_copy(1 << 28, reps=3)
+    log(f"assembly sample: copy 256 MiB {1e3 * ms:.1f} us")
+    S.begin_step(ctx)
+    for flag in (6, 2):
+        ms = ctx.time_assemble(flag, reps=reps)
+        log(f"assembly sample: channel {nx}x{ny}x{nz} NS flag {flag}: {ms:.3f} ms")
+    ctx.close()
+    prob = C.quench(48)
+    ctx = S.setup(prob, device=0)
+    ctx.set_quench_params(prob.material)
+    for what, name in ((L.OFD_TIME_ASM_TEM, "Tem"), (L.OFD_TIME_ASM_NSVOF, "NSVOF")):
+        ms = ctx.time_quench(what, 6, reps)
+        log(f"assembly sample: quench 48^3 {name} VEC+MAT: {ms:.3f} ms")
+    ctx.close()
+
+
 def quench_kernels(L, C, S, n=48, reps=5, spmv_reps=20):
     """Kernel figures of the quenching solver on the n^3 tank (ELM_ID 101
     solid block, water/vapour, 300/900 K): NSVOF and temperature assembly
@@ -356,6 +377,93 @@ def overset_steps(L, C, S, nsteps=10, nnewton=3, kspace=64):
     log(f"bench: overset {nsteps} steps: {out['ms_per_step']:.0f} ms per step, max relres "
         f"{max(rel):.1e}, |Ru| after the last step {res[-1]:.2e}")
     return out
+
+
+def spmv_large(L, C, S, dims=(128, 64, 64), reps=20):
+    """The NS block product beyond the caches: the 128x64x64 channel's
+    pattern (14,340,865 blocks; 2.86 GB per FP64 product, 1.46 GB with the
+    FP32 matrix, several times the 256 MB Infinity Cache), beside a
+    read-only stream of 4 GiB and a 4 GiB copy on the same device."""
+    prob = C.channel(*dims)
+    E, N = prob.mesh.nelem, prob.mesh.nnode
+    ctx = S.setup(prob, device=0)
+    Z = ctx.icnt
+    S.begin_step(ctx)
+    ctx.assembleNavSto(6, 1)
+    t64 = ctx.time_spmv(L.OFD_FP64, reps=reps)
+    t32 = ctx.time_spmv(L.OFD_FP32, reps=reps)
+    read_ms = ctx.time_read(1 << 32, reps=5)
+    copy_ms = ctx.time_copy(1 << 32, reps=5)
+    ctx.close()
+    b64, b32 = spmv_bytes(Z, N, 8), spmv_bytes(Z, N, 4)
+    read_bw = (1 << 32) / (read_ms * 1e-3)
+    copy_bw = 2 * (1 << 32) / (copy_ms * 1e-3)
+
+    def prod(ms, nbytes):
+        bw = nbytes / (ms * 1e-3)
+        return {"us": 1e3 * ms, "bytes": nbytes, "GBps": bw / 1e9, "frac_hbm": bw / PEAK_HBM,
+                "frac_read_stream": bw / read_bw}
+
+    out = {"workload": f"channel {dims[0]}x{dims[1]}x{dims[2]} hex, NS block product",
+           "elements": E, "nodes": N, "nnz_blocks": Z,
+           "fp64": prod(t64, b64), "fp32_matrix": prod(t32, b32),
+           "read_stream_4GiB_GBps": read_bw / 1e9, "copy_4GiB_GBps": copy_bw / 1e9}
+    log(f"bench: SpMV {dims[0]}x{dims[1]}x{dims[2]} ({Z} blocks): FP64 {1e3 * t64:.0f} us "
+        f"{out['fp64']['GBps']:.0f} GB/s ({100 * out['fp64']['frac_hbm']:.0f}% of 8 TB/s), "
+        f"FP32-matrix {1e3 * t32:.0f} us {out['fp32_matrix']['GBps']:.0f} GB/s; read stream "
+        f"{read_bw / 1e9:.0f} GB/s, copy {copy_bw / 1e9:.0f} GB/s")
+    return out
+
+
+def quench_steps(L, C, S, n=48, nsteps=20, nnewton=3):
+    """The quench configuration end to end: the 48^3 tank (water/vapour,
+    density ratio 1000, solid block at 900 K in liquid at 300 K), dt = 1e-3,
+    ``nsteps`` time steps of ``nnewton`` Newton iterations, each the NS + VOF
+    system then the temperature system (solveFlow.f90:4-164), every linear
+    solve restarted GMRES(30) to a 1e-8 true FP64 residual.  Run once with
+    the FP64 solver and once with the mixed one from the same initial state;
+    each field of the mixed run is reported against the FP64 run."""
+    import numpy as np
+    prob = C.quench(n)
+    E, N = prob.mesh.nelem, prob.mesh.nnode
+    ctx = S.setup(prob, device=0)
+    ctx.set_quench_params(prob.material)
+    S.quench_step(ctx, nnewton=1, precision=L.OFD_FP32, raise_noconv=False)  # warm-up
+    runs, fields = {}, {}
+    for prec, tag in ((L.OFD_FP64, "fp64"), (L.OFD_FP32, "mixed")):
+        ctx.set_state(prob.fields)
+        step_ms, it_v, it_t, rel = [], [], [], []
+        for _ in range(nsteps):
+            t0 = time.perf_counter()
+            info = S.quench_step(ctx, nnewton=nnewton, rtol=1e-8, precision=prec,
+                                 raise_noconv=False)
+            ctx.advance()
+            step_ms.append(1e3 * (time.perf_counter() - t0))
+            it_v.append([i["nsvof"]["iters"] for i in info])
+            it_t.append([i["tem"]["iters"] for i in info])
+            rel.append(max(max(i["nsvof"]["relres"], i["tem"]["relres"]) for i in info))
+        runs[tag] = {"ms_per_step": sum(step_ms) / nsteps, "step_ms": step_ms,
+                     "nsvof_gmres_iters": it_v, "tem_gmres_iters": it_t,
+                     "max_relres": max(rel), "all_converged": bool(max(rel) <= 1e-8)}
+        ug = ctx.get_field("UG")
+        fields[tag] = {"u": ug[:, 0], "v": ug[:, 1], "w": ug[:, 2], "p": ctx.get_field("PG"),
+                       "phi": ctx.get_field("PHIG"), "T": ctx.get_field("TG")}
+        log(f"bench: quench {n}^3 {nsteps} steps ({tag}): {runs[tag]['ms_per_step']:.1f} ms "
+            f"per step, max relres {max(rel):.1e}")
+    ctx.close()
+    err = {}
+    for k in fields["fp64"]:
+        a, b = fields["mixed"][k], fields["fp64"][k]
+        err[k] = {"rel_l2": float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)),
+                  "max_abs": float(np.abs(a - b).max())}
+    log("bench: quench mixed vs FP64 after the same steps, relative L2: " +
+        ", ".join(f"{k} {v['rel_l2']:.1e}" for k, v in err.items()))
+    return {"workload": f"quench tank {n}^3 hex, {nsteps} steps x {nnewton} Newton iterations "
+                        "(NS + VOF then temperature), dt 1e-3, GMRES(30) to rtol 1e-8",
+            "elements": E, "nodes": N, "dof": 6 * N, "fp64": runs["fp64"],
+            "mixed": runs["mixed"],
+            "speedup_mixed_vs_fp64": runs["fp64"]["ms_per_step"] / runs["mixed"]["ms_per_step"],
+            "field_error_mixed_vs_fp64": err}
 
 
 def fringe_search(C, S, reps=5):
@@ -620,6 +728,14 @@ def main():
     ap.add_argument("--faces-pattern-only", action="store_true",
                     help="only the weak-face and device pattern sections (for profiled "
                          "runs of those kernels); no JSON line")
+    ap.add_argument("--assembly-only", action="store_true",
+                    help="only element-assembly launches (for profiled runs): a 256 MiB "
+                         "copy (counter calibration), then the channel NS assembly "
+                         "VEC+MAT and VEC and the 48^3 tank's temperature and NS + VOF "
+                         "assemblies, 3 times each; no JSON line")
+    ap.add_argument("--quench-steps-only", action="store_true",
+                    help="only the quench configuration's time steps (FP64 and mixed); "
+                         "no JSON line")
     ap.add_argument("--pmc-sample", action="store_true",
                     help="short run for hardware-counter passes: a 256 MiB copy "
                          "(calibration) and one full GMRES(30) cycle per precision")
@@ -638,6 +754,12 @@ def main():
         weak_faces(L, C, S)
         tet_assembly(C, S)
         pattern_build(S)
+        return
+    if args.assembly_only:
+        assembly_sample(L, C, S)
+        return
+    if args.quench_steps_only:
+        quench_steps(L, C, S)
         return
 
     nx, ny, nz = (int(v) for v in args.dims.split(","))
@@ -708,7 +830,7 @@ def main():
         f"{asm_tf / 1e12:.2f} TFLOP/s); VEC {asm_vec_ms:.3f} ms")
     log(f"bench: SpMV FP64 {spmv64_ms * 1e3:.1f} us {bw64 / 1e9:.0f} GB/s; "
         f"FP32-matrix {spmv32_ms * 1e3:.1f} us {bw32 / 1e9:.0f} GB/s")
-    adjoint = quench = fringe = faces = tets = pattern = None
+    adjoint = quench = fringe = faces = tets = pattern = large = None
     if not args.headline_only:
         adjoint = adjoint_ns(ctx, L, prob, asm_vec_ms)
         adjoint["newton_step_fp64_ms"] = res64["newton_ms"]
@@ -716,6 +838,8 @@ def main():
 
     if not args.headline_only:
         quench, qprob = quench_kernels(L, C, S)
+        quench["time_steps"] = quench_steps(L, C, S)
+        large = spmv_large(L, C, S)
         fringe = fringe_search(C, S)
         fringe["coupled_steps"] = overset_steps(L, C, S)
         adjoint["quench_transient"] = adjoint_c5(L, C, S)
@@ -792,6 +916,7 @@ def main():
                  "fp32_us": spmv32_ms * 1e3, "fp32_GBps": bw32 / 1e9,
                  "fp64_bytes": b64, "fp32_bytes": b32},
         "hbm_copy_GBps": copy_bw / 1e9,
+        "spmv_large": large,
         "roofline": {"bound": "hbm", "kernel": ARN32,
                      "achieved": bwa32 / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                      "frac": bwa32 / PEAK_HBM, "traffic": traffic(ARN32, wl),

@@ -318,7 +318,11 @@ int ofd_fringe_interp(ofd_ctx *ctx, int32_t src_id, int nrecv, const double *xr,
 int ofd_apply_fringe(ofd_ctx *ctx, int nrecv, const int32_t *nodes, const double *val);
 
 /* kernel timing on the context stream (HIP events around `reps`
- * back-to-back launches); ms = average duration of one launch */
+ * back-to-back launches); ms = average duration of one launch.
+ * ofd_time_assemble times the device work of one ofd_assemble_ns (elements
+ * of subdomain flags 1 | 2): the alpha stage, the RHS zeroing, every colour
+ * launch of the element kernel and of the weak faces, and the CFL maximum;
+ * it leaves out only the entry's host synchronisation and read-back. */
 int ofd_time_spmv(ofd_ctx *ctx, int precision, int reps, double *ms);
 int ofd_time_assemble(ofd_ctx *ctx, int tensor_flag, int reps, double *ms);
 /* the weak-face launches of ofd_assemble_ns alone (k_face_ns_hex, every
@@ -331,6 +335,12 @@ int ofd_time_faces(ofd_ctx *ctx, int tensor_flag, int reps, double *ms);
  * HBM-bound kernels (and the calibration of hardware byte counters).
  * Allocates and frees its two buffers.  No reference counterpart. */
 int ofd_time_copy(ofd_ctx *ctx, int64_t nbytes, int reps, double *ms);
+/* read-only stream of nbytes (multiple of 16, at most 32 GiB) by a
+ * grid-stride kernel with 8 loads in flight per thread, one sum stored per
+ * block; ms = average of `reps` launches.  The achievable-read reference
+ * for the SpMV, whose matrix stream is read-only.  Allocates and frees its
+ * buffers.  No reference counterpart. */
+int ofd_time_read(ofd_ctx *ctx, int64_t nbytes, int reps, double *ms);
 
 /* ---- adjoint of the NS Newton step (no reference code) ----
  *

@@ -14,6 +14,13 @@ LIB_PATH = os.path.join(HERE, "libofd.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "ofd.h")
 
 OFD_OK = 0
+OFD_ERR_ARG = 1
+OFD_ERR_HIP = 2
+OFD_ERR_NEGDETJ = 3
+OFD_ERR_NONFINITE = 4
+OFD_ERR_NOCONV = 5
+OFD_ERR_ALLOC = 6
+OFD_ERR_UNSUPPORTED = 7
 OFD_ASSEMBLE_VEC = 2
 OFD_ASSEMBLE_MAT = 4
 OFD_FP64 = 0
@@ -127,6 +134,7 @@ SIGNATURES = {
     "ofd_time_spmv": ([_vp, _c_int, _c_int, _dp], _c_int),
     "ofd_time_assemble": ([_vp, _c_int, _c_int, _dp], _c_int),
     "ofd_time_copy": ([_vp, ctypes.c_int64, _c_int, _dp], _c_int),
+    "ofd_time_read": ([_vp, ctypes.c_int64, _c_int, _dp], _c_int),
     "ofd_time_quench": ([_vp, _c_int, _c_int, _c_int, _dp], _c_int),
     "ofd_time_adjoint": ([_vp, _c_int, _c_int, _dp], _c_int),
     "ofd_set_weak_faces": ([_vp, _c_int, _vp, _vp], _c_int),

@@ -38,6 +38,7 @@
 // point q, node a), 8-lane butterflies for the Gauss-point sums, a butterfly
 // over q for the node sums, and a coloured read-modify-write scatter.
 #include "ofd_internal.h"
+#include "wave_dpp.h"
 
 namespace ofd {
 
@@ -47,12 +48,7 @@ constexpr int kWaves = 4;
 __constant__ int8_t c_sgn_adj[8][3] = {{-1, 1, -1}, {1, 1, -1}, {1, 1, 1},  {-1, 1, 1},
                                        {-1, -1, -1}, {1, -1, -1}, {1, -1, 1}, {-1, -1, 1}};
 
-__device__ __forceinline__ double red8(double v) {
-  v += __shfl_xor(v, 1);
-  v += __shfl_xor(v, 2);
-  v += __shfl_xor(v, 4);
-  return v;
-}
+__device__ __forceinline__ double red8(double v) { return wsum8(v); }
 
 struct AdjParams {
   double rho, mu, kdc, delt, cu, ca, cp, gx, gy, gz;

@@ -38,6 +38,7 @@
 // (Gauss point q, node a), 8-lane butterflies for the Gauss-point sums, a
 // butterfly over q for the node sums, a coloured read-modify-write scatter.
 #include "ofd_internal.h"
+#include "wave_dpp.h"
 
 namespace ofd {
 
@@ -47,12 +48,7 @@ constexpr int kWaves = 4;
 __constant__ int8_t c_sgn_tadj[8][3] = {{-1, 1, -1}, {1, 1, -1}, {1, 1, 1},  {-1, 1, 1},
                                         {-1, -1, -1}, {1, -1, -1}, {1, -1, 1}, {-1, -1, 1}};
 
-__device__ __forceinline__ double red8t(double v) {
-  v += __shfl_xor(v, 1);
-  v += __shfl_xor(v, 2);
-  v += __shfl_xor(v, 4);
-  return v;
-}
+__device__ __forceinline__ double red8t(double v) { return wsum8(v); }
 
 // Gauss-point value summed over the element's 8 nodes in node order, as the
 // forward kernel forms it (the order of the reference's e3int_qr).  Used

@@ -29,6 +29,7 @@
 //            share a node).
 // One wave per block: the face loop's trip count differs between elements.
 #include "ofd_internal.h"
+#include "wave_dpp.h"
 
 namespace ofd {
 
@@ -37,12 +38,7 @@ namespace {
 __constant__ int8_t c_fsign[8][3] = {{-1, 1, -1}, {1, 1, -1}, {1, 1, 1},  {-1, 1, 1},
                                      {-1, -1, -1}, {1, -1, -1}, {1, -1, 1}, {-1, -1, 1}};
 
-__device__ __forceinline__ double fred8(double v) {
-  v += __shfl_xor(v, 1);
-  v += __shfl_xor(v, 2);
-  v += __shfl_xor(v, 4);
-  return v;
-}
+__device__ __forceinline__ double fred8(double v) { return wsum8(v); }
 
 }  // namespace
 

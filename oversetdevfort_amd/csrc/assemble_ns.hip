@@ -20,6 +20,7 @@
 //            launched one colour at a time and no two elements of a colour
 //            share a node, so no atomics and a fixed summation order.
 #include "ofd_internal.h"
+#include "wave_dpp.h"
 
 namespace ofd {
 
@@ -30,19 +31,8 @@ constexpr int kWavesPerBlock = 4;
 __constant__ int8_t c_sign[8][3] = {{-1, 1, -1}, {1, 1, -1}, {1, 1, 1},  {-1, 1, 1},
                                     {-1, -1, -1}, {1, -1, -1}, {1, -1, 1}, {-1, -1, 1}};
 
-__device__ __forceinline__ double red8(double v) {
-  v += __shfl_xor(v, 1);
-  v += __shfl_xor(v, 2);
-  v += __shfl_xor(v, 4);
-  return v;
-}
-
-__device__ __forceinline__ double max8(double v) {
-  v = fmax(v, __shfl_xor(v, 1));
-  v = fmax(v, __shfl_xor(v, 2));
-  v = fmax(v, __shfl_xor(v, 4));
-  return v;
-}
+__device__ __forceinline__ double red8(double v) { return wsum8(v); }
+__device__ __forceinline__ double max8(double v) { return wmax8(v); }
 
 }  // namespace
 
@@ -108,14 +98,18 @@ __global__ void __launch_bounds__(256)
 k_assemble_ns_el(int N, int E, const int32_t *__restrict__ ien,
                   const int32_t *__restrict__ elm_id,
                   const int32_t *__restrict__ elems, int first, int count,
-                  const int32_t *__restrict__ emap,
+                  const int32_t *__restrict__ emap, const uint64_t *__restrict__ firstmask,
                   const double *__restrict__ rec, const double *__restrict__ mus,
                   const uint8_t *__restrict__ ibc, AsmParams P, int id1,
                   int id2, int subdomain_flag, int do_mat, int do_vec,
                   double *__restrict__ A, double *__restrict__ rhs,
                   double *__restrict__ cfl_out, int32_t *__restrict__ negdet) {
-  __shared__ double s_sh[kWavesPerBlock][8][8][5];  // [q][a]: N, dN/dx(3), tmp1
-  __shared__ double s_q[kWavesPerBlock][8][5];      // [q]: tauM, tauP, tauC, DetJ*w, mu
+  // [q][a]: N, dN/dx (3), tmp1 = rho uadv . dN/dx, pad (16-byte rows)
+  __shared__ double2 s_sh[kWavesPerBlock][8][8][3];
+  // [q]: the Gauss point's factors of e3LHS, DetJ*w folded in:
+  // cf = fact2 dw, cB = fact1 rho dw, cm = fact2 (mu + kdc) dw,
+  // ct = fact2 tauC dw, tauM (= tauP), pad
+  __shared__ double2 s_q[kWavesPerBlock][8][3];
   __shared__ int32_t s_node[kWavesPerBlock][8];
 
   const int lane = threadIdx.x & 63;
@@ -273,18 +267,17 @@ k_assemble_ns_el(int N, int E, const int32_t *__restrict__ ien,
   if (valid && a == 0 && (S == 8 || q < 4) && detj < 0.0) atomicAdd(negdet, 1);
 
   const double tmp1a = rho * (uadv[0] * ga[0] + uadv[1] * ga[1] + uadv[2] * ga[2]);
-  s_sh[w][q][a][0] = Na;
-  s_sh[w][q][a][1] = ga[0];
-  s_sh[w][q][a][2] = ga[1];
-  s_sh[w][q][a][3] = ga[2];
-  s_sh[w][q][a][4] = tmp1a;
+  s_sh[w][q][a][0] = make_double2(Na, ga[0]);
+  s_sh[w][q][a][1] = make_double2(ga[1], ga[2]);
+  s_sh[w][q][a][2] = make_double2(tmp1a, 0.0);
   if (a == 0) {
-    s_q[w][q][0] = tauM;
-    s_q[w][q][1] = tauP;
-    s_q[w][q][2] = tauC;
-    s_q[w][q][3] = dw;
-    s_q[w][q][4] = mu;
+    const double fact2 = P.alfi * P.gami * P.delt;
+    const double cf = fact2 * dw;
+    s_q[w][q][0] = make_double2(cf, P.almi * rho * dw);
+    s_q[w][q][1] = make_double2(cf * (mu + P.kdc), cf * tauC);
+    s_q[w][q][2] = make_double2(tauM, 0.0);
   }
+  (void)tauP;
 
   // RHS (e3Rhs_3D_fluid; the live call's uprime = 0, or with P.vms the
   // fine-scale velocity uprime = -tauM rLi, rLi from assembleNavSto.f90:
@@ -357,42 +350,50 @@ k_assemble_ns_el(int N, int E, const int32_t *__restrict__ ien,
 #pragma unroll
   for (int t = 0; t < 16; ++t) acc[t] = 0.0;
   if (mat) {
-    const double fact1 = P.almi;
-    const double fact2 = P.alfi * P.gami * P.delt;
+    // e3LHS_3D_fluid_Old (e3LHS.f90:56-159) with tauBar = 0, regrouped so
+    // that DetJ*w and fact1 = almi, fact2 = alfi*gami*Delt sit in the Gauss
+    // point's factors (tauP = tauM):
+    //   tmp4 dw = (N_a + tauM t1_a)(cB N_b + cf t1_b) + cm g_a.g_b
+    //   K11_rc += delta_rc tmp4 dw + cm g_a,c g_b,r + ct g_a,r g_b,c
+    //   G_r    += -g_a,r cf N_b + tauM t1_a cf g_b,r
+    //   D1_c   += N_a cf g_b,c + g_a,c tauM (cB N_b + cf t1_b)
+    //   M      += tauM cf g_a.g_b
 #pragma unroll 2
     for (int qq = 0; qq < (S == 8 ? 8 : 4); ++qq) {
-      const double mupkdc = s_q[w][qq][4] + P.kdc;
-      const double Nq = s_sh[w][qq][pa_][0];
-      const double g0 = s_sh[w][qq][pa_][1], g1 = s_sh[w][qq][pa_][2],
-                   g2 = s_sh[w][qq][pa_][3], t1a = s_sh[w][qq][pa_][4];
-      const double Mb = s_sh[w][qq][pb_][0];
-      const double h0 = s_sh[w][qq][pb_][1], h1 = s_sh[w][qq][pb_][2],
-                   h2 = s_sh[w][qq][pb_][3], t1b = s_sh[w][qq][pb_][4];
-      const double tM = s_q[w][qq][0], tP = s_q[w][qq][1], tC = s_q[w][qq][2],
-                   dj = s_q[w][qq][3];
-      const double gab = g0 * h0 + g1 * h1 + g2 * h2;
-      const double tmp4 = fact1 * (Nq * rho * Mb + t1a * tM * rho * Mb) +
-                          fact2 * (Nq * t1b + t1a * tM * t1b + mupkdc * gab);
-      // K11 (row-major r*3+c in block slots r*4+c)
-      acc[0] += (tmp4 + fact2 * (g0 * mupkdc * h0 + g0 * tC * h0)) * dj;
-      acc[5] += (tmp4 + fact2 * (g1 * mupkdc * h1 + g1 * tC * h1)) * dj;
-      acc[10] += (tmp4 + fact2 * (g2 * mupkdc * h2 + g2 * tC * h2)) * dj;
-      acc[1] += fact2 * (g1 * mupkdc * h0 + g0 * tC * h1) * dj;
-      acc[4] += fact2 * (g0 * mupkdc * h1 + g1 * tC * h0) * dj;
-      acc[2] += fact2 * (g2 * mupkdc * h0 + g0 * tC * h2) * dj;
-      acc[8] += fact2 * (g0 * mupkdc * h2 + g2 * tC * h0) * dj;
-      acc[6] += fact2 * (g2 * mupkdc * h1 + g1 * tC * h2) * dj;
-      acc[9] += fact2 * (g1 * mupkdc * h2 + g2 * tC * h1) * dj;
-      // G (column p)
-      acc[3] += fact2 * (-g0 * Mb + t1a * tM * h0) * dj;
-      acc[7] += fact2 * (-g1 * Mb + t1a * tM * h1) * dj;
-      acc[11] += fact2 * (-g2 * Mb + t1a * tM * h2) * dj;
-      // D1 (row p)
-      acc[12] += (fact2 * (Nq * h0 + g0 * tP * t1b) + fact1 * (g0 * tP * rho * Mb)) * dj;
-      acc[13] += (fact2 * (Nq * h1 + g1 * tP * t1b) + fact1 * (g1 * tP * rho * Mb)) * dj;
-      acc[14] += (fact2 * (Nq * h2 + g2 * tP * t1b) + fact1 * (g2 * tP * rho * Mb)) * dj;
-      // M
-      acc[15] += fact2 * tP * gab * dj;
+      const double2 r0 = s_sh[w][qq][pa_][0], r1 = s_sh[w][qq][pa_][1],
+                    r2 = s_sh[w][qq][pa_][2];
+      const double2 c0 = s_sh[w][qq][pb_][0], c1 = s_sh[w][qq][pb_][1],
+                    c2 = s_sh[w][qq][pb_][2];
+      const double2 f0 = s_q[w][qq][0], f1 = s_q[w][qq][1], f2 = s_q[w][qq][2];
+      const double Nq = r0.x, g0 = r0.y, g1 = r1.x, g2 = r1.y, t1a = r2.x;
+      const double Mb = c0.x, h0 = c0.y, h1 = c1.x, h2 = c1.y, t1b = c2.x;
+      const double cf = f0.x, cB = f0.y, cm = f1.x, ct = f1.y, tM = f2.x;
+      // column factors
+      const double Bb = cB * Mb + cf * t1b;
+      const double hf0 = cf * h0, hf1 = cf * h1, hf2 = cf * h2;
+      const double hm0 = cm * h0, hm1 = cm * h1, hm2 = cm * h2;
+      const double ht0 = ct * h0, ht1 = ct * h1, ht2 = ct * h2;
+      const double Mf = cf * Mb, PB = tM * Bb;
+      // row factors
+      const double Ta = tM * t1a;
+      const double Aa = Nq + Ta;
+      const double t4 = Aa * Bb + (g0 * hm0 + g1 * hm1 + g2 * hm2);
+      acc[0] += t4 + g0 * (hm0 + ht0);
+      acc[5] += t4 + g1 * (hm1 + ht1);
+      acc[10] += t4 + g2 * (hm2 + ht2);
+      acc[1] += g1 * hm0 + g0 * ht1;
+      acc[4] += g0 * hm1 + g1 * ht0;
+      acc[2] += g2 * hm0 + g0 * ht2;
+      acc[8] += g0 * hm2 + g2 * ht0;
+      acc[6] += g2 * hm1 + g1 * ht2;
+      acc[9] += g1 * hm2 + g2 * ht1;
+      acc[3] += Ta * hf0 - g0 * Mf;
+      acc[7] += Ta * hf1 - g1 * Mf;
+      acc[11] += Ta * hf2 - g2 * Mf;
+      acc[12] += Nq * hf0 + g0 * PB;
+      acc[13] += Nq * hf1 + g1 * PB;
+      acc[14] += Nq * hf2 + g2 * PB;
+      acc[15] += tM * (g0 * hf0 + g1 * hf1 + g2 * hf2);
     }
   }
 
@@ -404,9 +405,13 @@ k_assemble_ns_el(int N, int E, const int32_t *__restrict__ ien,
     const unsigned mb = ibc[nrb];
     if (do_mat) {
       const int k = emap[(size_t)e * 64 + lane];
+      // the first element (in colour order) to reach this nonzero stores,
+      // the later ones add: no zeroing pass over the matrix
+      const bool first = (firstmask[e] >> lane) & 1ull;
       double2 *blk = reinterpret_cast<double2 *>(A + (size_t)k * 16);
 #pragma unroll
       for (int t = 0; t < 8; ++t) {
+        double2 old = first ? make_double2(0.0, 0.0) : blk[t];
         double v[2] = {acc[2 * t], acc[2 * t + 1]};
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
@@ -418,7 +423,6 @@ k_assemble_ns_el(int N, int E, const int32_t *__restrict__ ien,
           if (((ma >> r) & 1u) || colz) v[h] = 0.0;
           if (pa_ == pb_ && r == cc && ((ma >> r) & 1u)) v[h] = 1.0;
         }
-        double2 old = blk[t];
         old.x += v[0];
         old.y += v[1];
         blk[t] = old;
@@ -449,7 +453,27 @@ __global__ void k_zero(double *p, size_t n) {
   for (; i < n; i += stride) p[i] = 0.0;
 }
 
-__global__ void k_max_reduce(const double *v, int n, double *out) {
+// Maximum of v[0..n) (n >= 0; 0 for n = 0) in two launches: each block of
+// the first writes the maximum of its grid-stride share to part[block], the
+// single block of the second reduces the parts.  The maximum does not depend
+// on the order, so the result is the same bits on every run.
+__global__ void __launch_bounds__(256) k_max_part(const double *__restrict__ v, int n,
+                                                  double *__restrict__ part) {
+  __shared__ double s[256];
+  double m = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    m = fmax(m, v[i]);
+  s[threadIdx.x] = m;
+  __syncthreads();
+  for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + h]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+__global__ void __launch_bounds__(256) k_max_reduce(const double *__restrict__ v, int n,
+                                                    double *__restrict__ out) {
   __shared__ double s[256];
   double m = 0.0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, v[i]);
@@ -500,7 +524,7 @@ hipError_t launch_assemble_ns_hex(Ctx &c, int tensor_flag, int subdomain_flag,
   hipLaunchKernelGGL((c.nshl == 4 ? k_assemble_ns_el<4> : k_assemble_ns_el<8>), dim3(blocks),
                      dim3(64 * kWavesPerBlock), 0,
                      c.stream, c.nnode, c.nelem, c.d_ien, c.d_elm_id,
-                     c.d_color_elems, first, count, c.d_emap, c.d_noderec,
+                     c.d_color_elems, first, count, c.d_emap, c.d_first, c.d_noderec,
                      c.mus_on ? c.d_field[OFD_MUSCALE] : nullptr, c.d_ibc,
                      P, c.elm_id_set[0], c.elm_id_set[1], subdomain_flag,
                      (tensor_flag & OFD_ASSEMBLE_MAT) ? 1 : 0,
@@ -517,8 +541,15 @@ hipError_t launch_zero(Ctx &c, double *p, size_t n) {
   return hipGetLastError();
 }
 
+// out_dev[0] = max(v[0..n)); out_dev[1 .. 1 + 512) holds the per-block parts
 hipError_t launch_max_reduce(Ctx &c, const double *v, int n, double *out_dev) {
-  hipLaunchKernelGGL(k_max_reduce, dim3(1), dim3(256), 0, c.stream, v, n, out_dev);
+  int blocks = (n + 2047) / 2048;  // at least 8 values per thread
+  blocks = blocks < 1 ? 1 : (blocks > 512 ? 512 : blocks);
+  hipLaunchKernelGGL(k_max_part, dim3(blocks), dim3(256), 0, c.stream, v, n, out_dev + 1);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_max_reduce, dim3(1), dim3(256), 0, c.stream, out_dev + 1, blocks,
+                     out_dev);
   return hipGetLastError();
 }
 

@@ -23,6 +23,7 @@
 // the 24 entries of block (a, b), masks for BCLhs_NS_conv, and a coloured
 // read-modify-write scatter.
 #include "ofd_internal.h"
+#include "wave_dpp.h"
 
 namespace ofd {
 
@@ -33,12 +34,7 @@ constexpr int kQs = 14;  // per-Gauss-point scalars staged in LDS
 __constant__ int8_t c_sgn5[8][3] = {{-1, 1, -1}, {1, 1, -1}, {1, 1, 1},  {-1, 1, 1},
                                     {-1, -1, -1}, {1, -1, -1}, {1, -1, 1}, {-1, -1, 1}};
 
-__device__ __forceinline__ double s8(double v) {
-  v += __shfl_xor(v, 1);
-  v += __shfl_xor(v, 2);
-  v += __shfl_xor(v, 4);
-  return v;
-}
+__device__ __forceinline__ double s8(double v) { return wsum8(v); }
 
 // Gauss-point value summed over the element's 8 nodes in node order, the
 // order of the reference's e3int_qr.  Used for the values compared against a
@@ -69,7 +65,8 @@ struct NsvofParams {
 __global__ void __launch_bounds__(256)
 k_assemble_nsvof_hex(int E, const int32_t *__restrict__ ien, const int32_t *__restrict__ elm_id,
                      const int32_t *__restrict__ elems, int first, int count,
-                     const int32_t *__restrict__ emap, const double *__restrict__ rec,
+                     const int32_t *__restrict__ emap,
+                     const uint64_t *__restrict__ firstmask, const double *__restrict__ rec,
                      const uint8_t *__restrict__ ibc, NsvofParams P, int do_mat, int do_vec,
                      int N, double *__restrict__ A, double *__restrict__ rhs,
                      double *__restrict__ cfl_out, int32_t *__restrict__ negdet) {
@@ -286,9 +283,7 @@ k_assemble_nsvof_hex(int E, const int32_t *__restrict__ ien, const int32_t *__re
     rl += __shfl_xor(rl, m);
     cf = fmax(cf, __shfl_xor(cf, m));
   }
-  cf = fmax(cf, __shfl_xor(cf, 1));
-  cf = fmax(cf, __shfl_xor(cf, 2));
-  cf = fmax(cf, __shfl_xor(cf, 4));
+  cf = wmax8(cf);
   if (valid && lane == 0) cfl_out[e] = cf;
 
   __syncthreads();
@@ -379,18 +374,20 @@ k_assemble_nsvof_hex(int E, const int32_t *__restrict__ ien, const int32_t *__re
         w8[2 * c + 1] = (((ma >> c) & 1u) || ((mb >> c) & 1u) || fz) ? 0.0 : LSU[c];
       w8[7] = (fz || ((mb >> 7) & 1u)) ? ((fz && diag) ? 1.0 : 0.0) : LS;
       const int k = emap[(size_t)e * 64 + lane];
+      // the first element (in colour order) to reach the nonzero stores
+      const bool firstk = (firstmask[e] >> lane) & 1ull;
       double2 *b16 = reinterpret_cast<double2 *>(A + (size_t)k * 24);
       double2 *b8 = reinterpret_cast<double2 *>(A + (size_t)k * 24 + 16);
 #pragma unroll
       for (int t = 0; t < 8; ++t) {
-        double2 o = b16[t];
+        double2 o = firstk ? make_double2(0.0, 0.0) : b16[t];
         o.x += v[2 * t];
         o.y += v[2 * t + 1];
         b16[t] = o;
       }
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        double2 o = b8[t];
+        double2 o = firstk ? make_double2(0.0, 0.0) : b8[t];
         o.x += w8[2 * t];
         o.y += w8[2 * t + 1];
         b8[t] = o;
@@ -435,7 +432,7 @@ hipError_t launch_assemble_nsvof_hex(Ctx &c, int tensor_flag, int color) {
   const int blocks = (count + kWaves - 1) / kWaves;
   hipLaunchKernelGGL(k_assemble_nsvof_hex, dim3(blocks), dim3(64 * kWaves), 0, c.stream,
                      c.nelem, c.d_ien, c.d_elm_id, c.d_color_elems, first, count, c.d_emap,
-                     c.d_noderec, c.d_ibc, P, (tensor_flag & OFD_ASSEMBLE_MAT) ? 1 : 0,
+                     c.d_first, c.d_noderec, c.d_ibc, P, (tensor_flag & OFD_ASSEMBLE_MAT) ? 1 : 0,
                      (tensor_flag & OFD_ASSEMBLE_VEC) ? 1 : 0, c.nnode, c.d_AV, c.d_rhsV,
                      c.d_cfl, c.d_negdet);
   return hipGetLastError();

@@ -19,6 +19,7 @@
 //            LHSTem at emap[e][a + 8 b]; one colour per launch, so no two
 //            elements in flight share a node.
 #include "ofd_internal.h"
+#include "wave_dpp.h"
 
 namespace ofd {
 
@@ -29,12 +30,7 @@ constexpr int kWaves = 4;
 __constant__ int8_t c_sgn[8][3] = {{-1, 1, -1}, {1, 1, -1}, {1, 1, 1},  {-1, 1, 1},
                                    {-1, -1, -1}, {1, -1, -1}, {1, -1, 1}, {-1, -1, 1}};
 
-__device__ __forceinline__ double sum8(double v) {
-  v += __shfl_xor(v, 1);
-  v += __shfl_xor(v, 2);
-  v += __shfl_xor(v, 4);
-  return v;
-}
+__device__ __forceinline__ double sum8(double v) { return wsum8(v); }
 
 // Gauss-point value summed over the element's 8 nodes in node order, the
 // order of the reference's e3int_qr.  Used for the values compared against a
@@ -66,7 +62,8 @@ struct TemParams {
 __global__ void __launch_bounds__(256)
 k_assemble_tem_hex(int E, const int32_t *__restrict__ ien, const int32_t *__restrict__ elm_id,
                    const int32_t *__restrict__ elems, int first, int count,
-                   const int32_t *__restrict__ emap, const double *__restrict__ rec,
+                   const int32_t *__restrict__ emap, const uint64_t *__restrict__ firstmask,
+                   const double *__restrict__ rec,
                    const double *__restrict__ ks, const uint8_t *__restrict__ ibc,
                    TemParams P, int do_mat, int do_vec,
                    double *__restrict__ A, double *__restrict__ rhs,
@@ -271,7 +268,8 @@ k_assemble_tem_hex(int E, const int32_t *__restrict__ ien, const int32_t *__rest
       double v = (ca || cb) ? 0.0 : acc;
       if (pa == pb && ca) v = 1.0;
       const int k = emap[(size_t)e * 64 + lane];
-      A[k] += v;
+      // the first element (in colour order) to reach the nonzero stores
+      A[k] = ((firstmask[e] >> lane) & 1ull) ? v : A[k] + v;
     }
     if (do_vec && lane < 8) rhs[na] += ca ? 0.0 : rt;
   }
@@ -307,7 +305,8 @@ hipError_t launch_assemble_tem_hex(Ctx &c, int tensor_flag, int color) {
   const int blocks = (count + kWaves - 1) / kWaves;
   hipLaunchKernelGGL(k_assemble_tem_hex, dim3(blocks), dim3(64 * kWaves), 0, c.stream,
                      c.nelem, c.d_ien, c.d_elm_id, c.d_color_elems, first, count, c.d_emap,
-                     c.d_noderec, c.ks_on ? c.d_field[OFD_KSCALE] : nullptr, c.d_ibc, P,
+                     c.d_first, c.d_noderec, c.ks_on ? c.d_field[OFD_KSCALE] : nullptr,
+                     c.d_ibc, P,
                      (tensor_flag & OFD_ASSEMBLE_MAT) ? 1 : 0,
                      (tensor_flag & OFD_ASSEMBLE_VEC) ? 1 : 0, c.d_AT, c.d_rhsT, c.d_negdet);
   return hipGetLastError();

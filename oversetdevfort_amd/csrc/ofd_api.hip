@@ -426,6 +426,7 @@ int ofd_ctx_destroy(ofd_ctx *x) {
   dfree(c.d_row);
   dfree(c.d_emap);
   dfree(c.d_color_elems);
+  dfree(c.d_first);
   dfree(c.d_fe_elem);
   dfree(c.d_fe_mask);
   dfree(c.d_diag);
@@ -706,6 +707,16 @@ int ofd_build_pattern(ofd_ctx *x, int64_t *icnt_out) {
   free_tem_transpose(c);
   rc = h2d(c, c.d_color_elems, order.data(), E * sizeof(int32_t));
   if (rc) return rc;
+  // which element writes each nonzero first in the colour order: it stores,
+  // the later ones add (the assemblies need no zeroing pass of the matrix)
+  rc = dalloc(c, &c.d_first, (size_t)E);
+  if (rc) return rc;
+  {
+    uint8_t *d_touched;
+    OFD_HIP(tmp.get((void **)&d_touched, (size_t)c.icnt));
+    OFD_HIP(hipMemsetAsync(d_touched, 0, (size_t)c.icnt, c.stream));
+    OFD_HIP(launch_first_touch(c, d_touched));
+  }
   OFD_HIP(hipMemsetAsync(c.d_A64, 0, (size_t)c.icnt * 16 * sizeof(double), c.stream));
   OFD_HIP(hipStreamSynchronize(c.stream));
   c.mat_valid = c.prec_valid = c.a32_valid = c.ap32_valid = false;
@@ -907,7 +918,7 @@ int ofd_assemble_ns(ofd_ctx *x, int tensor_flag, int subdomain_flag, double *cfl
   OFD_HIP(launch_alpha_stage(c));
   if (tensor_flag & OFD_ASSEMBLE_VEC) OFD_HIP(launch_zero(c, c.d_rhs, (size_t)c.nnode * 4));
   if (tensor_flag & OFD_ASSEMBLE_MAT) {
-    OFD_HIP(launch_zero(c, c.d_A64, (size_t)c.icnt * 16));
+    // no zeroing: the first element to reach a nonzero stores into it
     c.mat_valid = true;
     c.prec_valid = c.a32_valid = c.ap32_valid = false;
     ++c.mat_version;
@@ -1313,11 +1324,12 @@ int ofd_time_assemble(ofd_ctx *x, int tensor_flag, int reps, double *ms) {
   for (int r = 0; r < reps; ++r) {
     OFD_HIP(launch_alpha_stage(c));
     if (tensor_flag & OFD_ASSEMBLE_VEC) OFD_HIP(launch_zero(c, c.d_rhs, (size_t)c.nnode * 4));
-    if (tensor_flag & OFD_ASSEMBLE_MAT) OFD_HIP(launch_zero(c, c.d_A64, (size_t)c.icnt * 16));
+    OFD_HIP(hipMemsetAsync(c.d_negdet, 0, sizeof(int32_t), c.stream));
     for (int col = 0; col + 1 < (int)c.color_off.size(); ++col)
       OFD_HIP(launch_assemble_ns_hex(c, tensor_flag, 3, col));
     for (int col = 0; col + 1 < (int)c.fe_color_off.size(); ++col)
       OFD_HIP(launch_face_ns_hex(c, tensor_flag, 3, col));
+    OFD_HIP(launch_max_reduce(c, c.d_cfl, c.nelem, c.d_part));
   }
   OFD_HIP(hipEventRecord(e1, c.stream));
   OFD_HIP(hipEventSynchronize(e1));
@@ -1372,12 +1384,10 @@ int ofd_time_quench(ofd_ctx *x, int what, int arg, int reps, double *ms) {
     if (what == OFD_TIME_ASM_NSVOF) {
       OFD_HIP(launch_alpha_stage(c, true));
       if (arg & OFD_ASSEMBLE_VEC) OFD_HIP(launch_zero(c, c.d_rhsV, (size_t)c.nsvV_n4 * 4));
-      if (arg & OFD_ASSEMBLE_MAT) OFD_HIP(launch_zero(c, c.d_AV, (size_t)c.icnt * 24));
       for (int col = 0; col < ncol; ++col) OFD_HIP(launch_assemble_nsvof_hex(c, arg, col));
     } else if (what == OFD_TIME_ASM_TEM) {
       OFD_HIP(launch_alpha_stage(c, false));
       if (arg & OFD_ASSEMBLE_VEC) OFD_HIP(launch_zero(c, c.d_rhsT, (size_t)c.temT_n4 * 4));
-      if (arg & OFD_ASSEMBLE_MAT) OFD_HIP(launch_zero(c, c.d_AT, (size_t)c.icnt));
       for (int col = 0; col < ncol; ++col) OFD_HIP(launch_assemble_tem_hex(c, arg, col));
     } else if (what == OFD_TIME_SPMV_NSVOF) {
       NsvofSystem sys(c);
@@ -1479,7 +1489,7 @@ int ofd_assemble_tem(ofd_ctx *x, int tensor_flag) {
   if (tensor_flag & OFD_ASSEMBLE_VEC)
     OFD_HIP(launch_zero(c, c.d_rhsT, (size_t)c.temT_n4 * 4));
   if (tensor_flag & OFD_ASSEMBLE_MAT) {
-    OFD_HIP(launch_zero(c, c.d_AT, (size_t)c.icnt));
+    // no zeroing: the first element to reach a nonzero stores into it
     c.matT_valid = true;
     c.precT_valid = c.aT32_valid = c.apT32_valid = false;
     ++c.matT_version;
@@ -1724,7 +1734,7 @@ int ofd_assemble_nsvof(ofd_ctx *x, int tensor_flag, double *cfl) {
   if (tensor_flag & OFD_ASSEMBLE_VEC)
     OFD_HIP(launch_zero(c, c.d_rhsV, (size_t)c.nsvV_n4 * 4));
   if (tensor_flag & OFD_ASSEMBLE_MAT) {
-    OFD_HIP(launch_zero(c, c.d_AV, (size_t)c.icnt * 24));
+    // no zeroing: the first element to reach a nonzero stores into it
     c.matV_valid = true;
     c.precV_valid = c.aV32_valid = c.apV32_valid = false;
   }
@@ -2026,6 +2036,40 @@ int ofd_time_copy(ofd_ctx *x, int64_t nbytes, int reps, double *ms) {
   if (e1) (void)hipEventDestroy(e1);
   (void)hipFree(src);
   (void)hipFree(dst);
+  OFD_HIP(e);
+  *ms = (double)t / reps;
+  return OFD_OK;
+}
+
+int ofd_time_read(ofd_ctx *x, int64_t nbytes, int reps, double *ms) {
+  OFD_CHECK(x && ms && reps > 0, OFD_ERR_ARG, "bad argument");
+  OFD_CHECK(nbytes >= 16 && nbytes % 16 == 0 && nbytes <= (int64_t(1) << 35), OFD_ERR_ARG,
+            "nbytes must be a multiple of 16 in [16, 32 GiB]");
+  Ctx &c = x->c;
+  OFD_HIP(hipSetDevice(c.device));
+  void *src = nullptr;
+  double *out = nullptr;
+  OFD_HIP(hipMalloc(&src, (size_t)nbytes));
+  hipError_t e = hipMalloc(reinterpret_cast<void **>(&out), sizeof(double) * kReadBlocks);
+  if (e != hipSuccess) {
+    (void)hipFree(src);
+    OFD_HIP(e);
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  float t = 0.f;
+  e = hipMemsetAsync(src, 0, (size_t)nbytes, c.stream);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = launch_read(c, src, (size_t)nbytes, out);  // warm
+  if (e == hipSuccess) e = hipEventRecord(e0, c.stream);
+  for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_read(c, src, (size_t)nbytes, out);
+  if (e == hipSuccess) e = hipEventRecord(e1, c.stream);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(src);
+  (void)hipFree(out);
   OFD_HIP(e);
   *ms = (double)t / reps;
   return OFD_OK;

@@ -87,6 +87,8 @@ struct Ctx {
   int32_t *d_col = nullptr, *d_row = nullptr;
   int32_t *d_emap = nullptr;          // [E][64]
   int32_t *d_color_elems = nullptr;   // elements grouped by colour
+  uint64_t *d_first = nullptr;        // [E] bit a + 8 b: the element is the first (in
+                                      // colour order) to add into pair (a, b)'s nonzero
   std::vector<int32_t> color_off;     // ncolor+1
   std::vector<int32_t> h_elem_color;  // colour of each element
   // weak boundary faces (BCugType 2): registered list, and the elements
@@ -271,6 +273,8 @@ hipError_t launch_node_elem_fill(Ctx &c, const int32_t *off, int32_t *pos, int32
 hipError_t launch_adjacency(Ctx &c, const int32_t *off, const int32_t *inc, const int32_t *col,
                             int32_t *out, int32_t *diag, int32_t *err);
 hipError_t launch_tmap_emap(Ctx &c, int32_t *err);
+// d_first from d_emap and the colouring (touched: [icnt] zeroed bytes)
+hipError_t launch_first_touch(Ctx &c, uint8_t *touched);
 hipError_t scan_offsets(Ctx &c, const int32_t *in, int32_t *out, int n, void *tmp,
                         size_t &tmp_bytes);
 hipError_t launch_cgs_update(Ctx &c, int k, int mode);
@@ -286,6 +290,8 @@ hipError_t launch_nsvof_to_soa(Ctx &c, const double *in);
 hipError_t launch_norm2_v(Ctx &c, const double *v);
 hipError_t launch_apply_dirichlet(Ctx &c);
 hipError_t launch_copy(Ctx &c, const void *src, void *dst, size_t nbytes);
+constexpr int kReadBlocks = 8192;  // blocks of the read-only stream kernel
+hipError_t launch_read(Ctx &c, const void *src, size_t nbytes, double *out);
 // overset fringe
 hipError_t launch_fringe(Ctx &c, int32_t src_id, int nrecv, const double *xr,
                          const double *dr, int32_t *donor, double *val);

@@ -140,7 +140,43 @@ __global__ void k_emap(int E, int S, const int32_t *__restrict__ ien,
   emap[t] = lower_bound(row, col[ia], col[ia + 1], ib);
 }
 
+// First touch of each nonzero in the assembly's launch order (colour by
+// colour, elements of a colour share no node, hence no nonzero): bit
+// a + 8 b of first[e] is set when element e is the first to add into the
+// nonzero of its pair (a, b).  One wavefront per element of one colour;
+// launched colour by colour, so the result does not depend on timing.
+__global__ void __launch_bounds__(256)
+k_first_touch(int S, const int32_t *__restrict__ elems, int first_idx, int count,
+              const int32_t *__restrict__ emap, uint8_t *__restrict__ touched,
+              uint64_t *__restrict__ first) {
+  const int lane = threadIdx.x & 63;
+  const int idx = first_idx + blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (idx >= first_idx + count) return;  // uniform over the wave
+  const int e = elems[idx];
+  const int a = lane & 7, b = lane >> 3;
+  bool mine = false;
+  if (a < S && b < S) {
+    const int k = emap[(size_t)e * 64 + lane];
+    mine = touched[k] == 0;
+    touched[k] = 1;
+  }
+  const unsigned long long m = __ballot(mine);
+  if (lane == 0) first[e] = m;
+}
+
 }  // namespace
+
+hipError_t launch_first_touch(Ctx &c, uint8_t *touched) {
+  for (int col = 0; col + 1 < (int)c.color_off.size(); ++col) {
+    const int f = c.color_off[col], n = c.color_off[col + 1] - f;
+    if (n <= 0) continue;
+    hipLaunchKernelGGL(k_first_touch, dim3((n + 3) / 4), dim3(256), 0, c.stream, c.nshl,
+                       c.d_color_elems, f, n, c.d_emap, touched, c.d_first);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
 
 hipError_t launch_node_elem_count(Ctx &c, int32_t *cnt) {
   hipLaunchKernelGGL(k_node_elem_count, dim3((c.nelem + 255) / 256), dim3(256), 0, c.stream,

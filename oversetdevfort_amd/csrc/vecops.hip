@@ -107,6 +107,47 @@ __global__ void __launch_bounds__(256) k_copy(size_t n16, const double2 *__restr
   for (; i < n16; i += stride) dst[i] = src[i];
 }
 
+// read-only stream: every thread sums its grid-stride share of 16-byte
+// loads (8 in flight per pass) and each block stores one value, so no load
+// can be dropped; the achievable-read reference for the SpMV, whose matrix
+// stream is read-only
+__global__ void __launch_bounds__(256) k_read(size_t n16, const double2 *__restrict__ src,
+                                              double *__restrict__ out) {
+  __shared__ double s[256];
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (; i + 7 * stride < n16; i += 8 * stride) {
+    double2 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = src[i + u * stride];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc[u] += v[u].x + v[u].y;
+  }
+  for (; i < n16; i += stride) acc[0] += src[i].x + src[i].y;
+  double t = 0.0;
+#pragma unroll
+  for (int u = 0; u < 8; ++u) t += acc[u];
+  s[threadIdx.x] = t;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) s[threadIdx.x] += s[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = s[0];
+}
+
+// out holds kReadBlocks doubles
+hipError_t launch_read(Ctx &c, const void *src, size_t nbytes, double *out) {
+  const size_t n16 = nbytes / 16;
+  size_t want = (n16 + 8 * 256 - 1) / (8 * 256);
+  if (want < 1) want = 1;
+  const unsigned nb = (unsigned)(want < (size_t)kReadBlocks ? want : (size_t)kReadBlocks);
+  hipLaunchKernelGGL(k_read, dim3(nb), dim3(256), 0, c.stream, n16,
+                     static_cast<const double2 *>(src), out);
+  return hipGetLastError();
+}
+
 hipError_t launch_copy(Ctx &c, const void *src, void *dst, size_t nbytes) {
   const size_t n16 = nbytes / 16;
   const size_t want = (n16 + 4 * 256 - 1) / (4 * 256);

@@ -591,6 +591,12 @@ class Context:
                                            ctypes.byref(ms)))
         return ms.value
 
+    def time_read(self, nbytes=1 << 30, reps=10):
+        """Average ms of one read-only stream of ``nbytes``."""
+        ms = ctypes.c_double()
+        L.check(self.lib.ofd_time_read(self.h, int(nbytes), int(reps), ctypes.byref(ms)))
+        return ms.value
+
     def time_copy(self, nbytes=1 << 28, reps=10):
         """Average ms of one device copy of ``nbytes`` (reads and writes
         ``nbytes`` each)."""
@@ -659,6 +665,45 @@ def newton_step(ctx: Context, rtol=1e-8, precision=L.OFD_FP64, ortho=L.OFD_ORTHO
     ctx.assembleNavSto(2, subdomain_flag)
     info["res"] = ctx.rhs_norms()
     return info
+
+
+def quench_step(ctx: Context, nnewton=3, rtol=1e-8, precision=L.OFD_FP64,
+                 ortho=L.OFD_ORTHO_CGS1, kspace=30, max_cycles=200, raise_noconv=True,
+                 ntol=None):
+    """One time step of the quenching solver, solmultiphasethermofluid_stag
+    (solveFlow.f90:4-164) after the predictor (driver.f90:83-85): per
+    Newton iteration the NS + VOF system (assembleQuenching VEC+MAT with the
+    NS and VOF fields, restarted SparseGMRES_ns_conv to rtol, the update of
+    solveFlow.f90:99-104), then the temperature system (VEC+MAT with the
+    temperature field, restarted SparseGMRES_tem, the update of :139-140),
+    then the residuals of both (:145-155).  With ``ntol`` = (u, p, phi, T)
+    relative tolerances the loop stops once every residual has fallen by its
+    tolerance from the first (check_convergence, solveFlow.f90:178-194);
+    without it, ``nnewton`` iterations run.  Returns the per-iteration solver
+    information."""
+    ctx.predictor()
+    ctx.apply_dirichlet()
+    ctx.assemble_nsvof(2)
+    ctx.assemble_tem(2)
+    r0 = (*ctx.nsvof_rhs_norms(), ctx.tem_rhs_norm())
+    out = []
+    for _ in range(nnewton):
+        ctx.assemble_nsvof(6)
+        iv = ctx.solve_nsvof(rtol=rtol, precision=precision, ortho=ortho, kspace=kspace,
+                             max_cycles=max_cycles, raise_noconv=raise_noconv)
+        ctx.nsvof_update()
+        ctx.assemble_tem(6)
+        it = ctx.solve_tem(rtol=rtol, precision=precision, ortho=ortho, kspace=kspace,
+                           max_cycles=max_cycles, raise_noconv=raise_noconv)
+        ctx.tem_update()
+        ctx.assemble_nsvof(2)
+        ctx.assemble_tem(2)
+        res = (*ctx.nsvof_rhs_norms(), ctx.tem_rhs_norm())
+        out.append(dict(nsvof=iv, tem=it, res=res))
+        if ntol is not None and all(r <= t * max(1e-15, r0_) for r, t, r0_ in
+                                    zip(res, ntol, r0)):
+            break
+    return out
 
 
 def fill_fringe(ctx: Context, fringe, xg):

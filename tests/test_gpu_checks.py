@@ -82,8 +82,8 @@ def test_unstructured_tet_mesh_beyond_32_colours(oracle_mod):
     ibc[rng.uniform(size=N) < 0.1, :3] = 1
     ibc[rng.uniform(size=N) < 0.05, 3] = 1
     ctx.set_ibc(ibc)
-    st = {k: rng.normal(size=(N, 3)) for k in ("dg", "ug", "ugm", "acg", "acgm")}
-    st["dg"] *= 1e-3
+    st = {k: rng.normal(size=(N, 3)) for k in ("ug", "ugm", "acg", "acgm")}
+    st["dg"] = np.zeros((N, 3))    # the tets keep the orientation they were given
     st["pg"] = rng.normal(size=N)
     for k, v in st.items():
         ctx.set_field(k.upper(), v)
