@@ -311,6 +311,21 @@ int ofd_nsvof_rhs_norms(ofd_ctx *ctx, double *norms3);
 int ofd_fringe_interp(ofd_ctx *ctx, int32_t src_id, int nrecv, const double *xr,
                       const double *dr, int32_t *donor, double *val);
 
+/* ofd_fringe_interp without allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * results and errors as ofd_fringe_interp; it calls no hipMalloc, hipFree,
+ * hipHostMalloc or allocating library routine.  Its scratch is a static
+ * device array of 64 MiB per GPU, held until the process ends: per element
+ * 48 bytes of box, 192 (hex) or 120 (tet) of geometry and 1 of flag, per
+ * receiver 76 bytes, each region rounded up to 256 bytes.  A search that
+ * needs more (a hex mesh of more than about 270,000 elements) is refused
+ * with OFD_ERR_ARG.  All contexts on one GPU share the array: calls that
+ * overlap in time on one GPU race; the array keeps the last call's
+ * contents. */
+int ofd_fringe_interp_noalloc(ofd_ctx *ctx, int32_t src_id, int nrecv, const double *xr,
+                              const double *dr, int32_t *donor, double *val);
+
 /* setBCs_overset step 3 (setBCs.f90:158-165): for j < nrecv, node
  * k = nodes[j] (1-based, distinct) gets IBC(k,1:3) = 1, ug(k,:) = val(j,:)
  * and acg(k,:) = ((val(j,:) - ugold(k,:))/Delt + (gami-1) acgold(k,:))/gami.

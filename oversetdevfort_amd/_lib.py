@@ -130,6 +130,7 @@ SIGNATURES = {
     "ofd_solve_tem_adjoint": ([_vp, _c_int, _c_int, _c_dbl, _c_dbl, _c_int, _c_int, _vp, _ip,
                                _ip, _dp], _c_int),
     "ofd_fringe_interp": ([_vp, ctypes.c_int32, _c_int, _vp, _vp, _vp, _vp], _c_int),
+    "ofd_fringe_interp_noalloc": ([_vp, ctypes.c_int32, _c_int, _vp, _vp, _vp, _vp], _c_int),
     "ofd_apply_fringe": ([_vp, _c_int, _vp, _vp], _c_int),
     "ofd_time_spmv": ([_vp, _c_int, _c_int, _dp], _c_int),
     "ofd_time_assemble": ([_vp, _c_int, _c_int, _dp], _c_int),

@@ -22,6 +22,8 @@
 // bits as the CPU restatement.
 #include "ofd_internal.h"
 
+#include <mutex>
+
 namespace ofd {
 
 namespace {
@@ -270,25 +272,123 @@ __global__ void k_fringe_apply(int nrecv, int N, const int32_t *__restrict__ nod
 
 #undef OFD_M3
 
+// ---- the donor search without allocation (ofd_fringe_interp_noalloc) ----
+//
+// The search's scratch -- per element the widened box, the source flag and
+// the geometry, per receiver the staged coordinates, donors and values --
+// lives in one file-scope device array of kFringeScratchBytes, one copy per
+// GPU for the life of the process.  Its device address is looked up once per
+// device at context creation (hipGetSymbolAddress) and used from then on;
+// host code never passes the array's own name to a kernel.  Every region is
+// written in full by the kernels before it is read (prep writes the flag of
+// every element, the search writes every receiver's donor and value), so
+// nothing is cleared between calls.  Calls on one GPU share the array: calls
+// that overlap in time on one device (two contexts on two streams) race.
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+__device__ __align__(256) unsigned char scratch[kFringeScratchBytes];
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxScratchDevices] = {nullptr};
+
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+}  // namespace
+
+int fringe_scratch_setup(int device) {
+  OFD_CHECK(device >= 0 && device < kMaxScratchDevices, OFD_ERR_UNSUPPORTED,
+            "device ordinal beyond the fringe scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return OFD_OK;
+  void *ptr = nullptr;
+  OFD_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  OFD_CHECK(ptr != nullptr, OFD_ERR_HIP, "hipGetSymbolAddress gave no address for the scratch");
+  g_scratch_ptr[device] = ptr;
+  return OFD_OK;
+}
+
+size_t fringe_scratch_need(int nshl, int nelem, int nrecv) {
+  const int geo = nshl == 4 ? kTetGeo : kHexGeo;
+  return align256((size_t)nelem * 6 * sizeof(double)) +
+         align256((size_t)nelem * geo * sizeof(double)) + align256((size_t)nelem) +
+         3 * align256((size_t)nrecv * 3 * sizeof(double)) +
+         align256((size_t)nrecv * sizeof(int32_t));
+}
+
+}  // namespace ofd
+
+using namespace ofd;
+
+extern "C" int ofd_fringe_interp_noalloc(ofd_ctx *x, int32_t src_id, int nrecv,
+                                         const double *xr, const double *dr, int32_t *donor,
+                                         double *val) {
+  OFD_CHECK(x, OFD_ERR_ARG, "null context");
+  OFD_CHECK(nrecv >= 0, OFD_ERR_ARG, "nrecv < 0");
+  Ctx &c = x->c;
+  OFD_CHECK(c.nnode > 0 && c.d_ien, OFD_ERR_ARG, "no mesh uploaded");
+  if (nrecv == 0) return OFD_OK;
+  OFD_CHECK(xr && dr && donor && val, OFD_ERR_ARG, "null argument");
+  const size_t need = fringe_scratch_need(c.nshl, c.nelem, nrecv);
+  OFD_CHECK(need <= kFringeScratchBytes, OFD_ERR_ARG,
+            "fringe search needs " + std::to_string(need) + " bytes of scratch for " +
+                std::to_string(c.nelem) + " elements and " + std::to_string(nrecv) +
+                " receivers; the static scratch holds " + std::to_string(kFringeScratchBytes));
+  void *base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (c.device >= 0 && c.device < kMaxScratchDevices) base = g_scratch_ptr[c.device];
+  }
+  OFD_CHECK(base, OFD_ERR_ARG, "fringe scratch not set up for this device");
+  OFD_HIP(hipSetDevice(c.device));
+  const int geo_w = c.nshl == 4 ? kTetGeo : kHexGeo;
+  unsigned char *p = static_cast<unsigned char *>(base);
+  double *box = reinterpret_cast<double *>(p);
+  p += align256((size_t)c.nelem * 6 * sizeof(double));
+  double *geo = reinterpret_cast<double *>(p);
+  p += align256((size_t)c.nelem * geo_w * sizeof(double));
+  int8_t *ok = reinterpret_cast<int8_t *>(p);
+  p += align256((size_t)c.nelem);
+  const size_t vb = (size_t)nrecv * 3 * sizeof(double);
+  double *sxr = reinterpret_cast<double *>(p);
+  p += align256(vb);
+  double *sdr = reinterpret_cast<double *>(p);
+  p += align256(vb);
+  double *sval = reinterpret_cast<double *>(p);
+  p += align256(vb);
+  int32_t *sdonor = reinterpret_cast<int32_t *>(p);
+  // pageable host memory on both sides: no staging buffer of the library's
+  OFD_HIP(hipMemcpyAsync(sxr, xr, vb, hipMemcpyHostToDevice, c.stream));
+  OFD_HIP(hipMemcpyAsync(sdr, dr, vb, hipMemcpyHostToDevice, c.stream));
+  OFD_HIP(launch_fringe(c, src_id, nrecv, sxr, sdr, sdonor, sval, box, ok, geo));
+  OFD_HIP(hipMemcpyAsync(donor, sdonor, (size_t)nrecv * sizeof(int32_t), hipMemcpyDeviceToHost,
+                         c.stream));
+  OFD_HIP(hipMemcpyAsync(val, sval, vb, hipMemcpyDeviceToHost, c.stream));
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  for (int j = 0; j < nrecv; ++j) donor[j] += 1;  // 1-based, 0 = no donor
+  return OFD_OK;
+}
+
+namespace ofd {
+
 hipError_t launch_fringe(Ctx &c, int32_t src_id, int nrecv, const double *xr,
-                         const double *dr, int32_t *donor, double *val) {
+                         const double *dr, int32_t *donor, double *val, double *box,
+                         int8_t *ok, double *geo) {
   const int E = c.nelem, N = c.nnode;
   const dim3 gp((E + 255) / 256), gs((nrecv + 3) / 4);
   const double *xg = c.d_field[OFD_XG], *dg = c.d_field[OFD_DG], *ug = c.d_field[OFD_UG];
   if (c.nshl == 4) {
     hipLaunchKernelGGL(k_fringe_prep<4>, gp, dim3(256), 0, c.stream, E, N, c.d_ien,
-                       c.d_elm_id, src_id, xg, dg, c.d_fr_box, c.d_fr_ok, c.d_fr_geo);
+                       c.d_elm_id, src_id, xg, dg, box, ok, geo);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_fringe_search<4>, gs, dim3(256), 0, c.stream, E, N, nrecv, xr, dr,
-                       c.d_fr_box, c.d_fr_ok, c.d_fr_geo, c.d_ien, ug, donor, val);
+                       box, ok, geo, c.d_ien, ug, donor, val);
   } else {
     hipLaunchKernelGGL(k_fringe_prep<8>, gp, dim3(256), 0, c.stream, E, N, c.d_ien,
-                       c.d_elm_id, src_id, xg, dg, c.d_fr_box, c.d_fr_ok, c.d_fr_geo);
+                       c.d_elm_id, src_id, xg, dg, box, ok, geo);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_fringe_search<8>, gs, dim3(256), 0, c.stream, E, N, nrecv, xr, dr,
-                       c.d_fr_box, c.d_fr_ok, c.d_fr_geo, c.d_ien, ug, donor, val);
+                       box, ok, geo, c.d_ien, ug, donor, val);
   }
   return hipGetLastError();
 }

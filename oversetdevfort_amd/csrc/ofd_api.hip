@@ -387,7 +387,13 @@ int ofd_ctx_create(int device, ofd_ctx **out) {
     set_error(std::string("hipStreamCreate: ") + hipGetErrorString(e));
     return OFD_ERR_HIP;
   }
-  int rc = dalloc(c, &c.d_gs, 1);
+  // the device address of the no-allocation fringe search's static scratch
+  int rc = fringe_scratch_setup(device);
+  if (rc) {
+    ofd_ctx_destroy(x);
+    return rc;
+  }
+  rc = dalloc(c, &c.d_gs, 1);
   if (!rc) rc = dalloc(c, &c.d_part, (size_t)4 * kMaxRedBlocks * (kMaxKspace + 1));
   if (!rc) rc = dalloc(c, &c.d_negdet, 1);
   if (rc) {
@@ -1970,7 +1976,8 @@ int ofd_fringe_interp(ofd_ctx *x, int32_t src_id, int nrecv, const double *xr,
   rc = h2d(c, c.d_fr_xr, xr, vb);
   if (!rc) rc = h2d(c, c.d_fr_dr, dr, vb);
   if (rc) return rc;
-  OFD_HIP(launch_fringe(c, src_id, nrecv, c.d_fr_xr, c.d_fr_dr, c.d_fr_donor, c.d_fr_val));
+  OFD_HIP(launch_fringe(c, src_id, nrecv, c.d_fr_xr, c.d_fr_dr, c.d_fr_donor, c.d_fr_val,
+                        c.d_fr_box, c.d_fr_ok, c.d_fr_geo));
   rc = d2h(c, donor, c.d_fr_donor, (size_t)nrecv * sizeof(int32_t));
   if (!rc) rc = d2h(c, val, c.d_fr_val, vb);
   if (rc) return rc;

@@ -294,7 +294,14 @@ constexpr int kReadBlocks = 8192;  // blocks of the read-only stream kernel
 hipError_t launch_read(Ctx &c, const void *src, size_t nbytes, double *out);
 // overset fringe
 hipError_t launch_fringe(Ctx &c, int32_t src_id, int nrecv, const double *xr,
-                         const double *dr, int32_t *donor, double *val);
+                         const double *dr, int32_t *donor, double *val, double *box,
+                         int8_t *ok, double *geo);
+// static scratch of ofd_fringe_interp_noalloc (fringe.hip): its size, the
+// once-per-device lookup of its address (context creation) and the bytes a
+// search over nelem elements and nrecv receivers needs
+constexpr size_t kFringeScratchBytes = size_t(64) << 20;
+int fringe_scratch_setup(int device);
+size_t fringe_scratch_need(int nshl, int nelem, int nrecv);
 hipError_t launch_fringe_apply(Ctx &c, int nrecv, const int32_t *nodes, const double *val);
 // gmres pieces
 hipError_t launch_gmres_init(Ctx &c, const double *b, double tol, int kspace, int kmin);
