@@ -415,14 +415,19 @@ def spmv_large(L, C, S, dims=(128, 64, 64), reps=20):
     return out
 
 
-def quench_steps(L, C, S, n=48, nsteps=20, nnewton=3):
+def quench_steps(L, C, S, n=48, nsteps=20, nnewton=3, max_cycles=40):
     """The quench configuration end to end: the 48^3 tank (water/vapour,
     density ratio 1000, solid block at 900 K in liquid at 300 K), dt = 1e-3,
     ``nsteps`` time steps of ``nnewton`` Newton iterations, each the NS + VOF
     system then the temperature system (solveFlow.f90:4-164), every linear
-    solve restarted GMRES(30) to a 1e-8 true FP64 residual.  Run once with
-    the FP64 solver and once with the mixed one from the same initial state;
-    each field of the mixed run is reported against the FP64 run."""
+    solve restarted GMRES(30) towards a 1e-8 true FP64 residual, at most
+    ``max_cycles`` restarts.  The NS + VOF systems of this tank stagnate
+    under the block-Jacobi preconditioner (the reference's own) well above
+    1e-8 -- GMRES(64) and 100 restarts, or a hydrostatic initial pressure,
+    change little -- so the relative residual each solve reached is
+    reported with the times.  Run once with the FP64 solver and once with
+    the mixed one from the same initial state; each field of the mixed run is
+    reported against the FP64 run."""
     import numpy as np
     prob = C.quench(n)
     E, N = prob.mesh.nelem, prob.mesh.nnode
@@ -436,15 +441,18 @@ def quench_steps(L, C, S, n=48, nsteps=20, nnewton=3):
         for _ in range(nsteps):
             t0 = time.perf_counter()
             info = S.quench_step(ctx, nnewton=nnewton, rtol=1e-8, precision=prec,
-                                 raise_noconv=False)
+                                 max_cycles=max_cycles, raise_noconv=False)
             ctx.advance()
             step_ms.append(1e3 * (time.perf_counter() - t0))
             it_v.append([i["nsvof"]["iters"] for i in info])
             it_t.append([i["tem"]["iters"] for i in info])
             rel.append(max(max(i["nsvof"]["relres"], i["tem"]["relres"]) for i in info))
+            log(f"  quench {tag} step {len(step_ms)}: {step_ms[-1]:.1f} ms, NSVOF its "
+                f"{it_v[-1]}, T its {it_t[-1]}, max relres {rel[-1]:.1e}")
         runs[tag] = {"ms_per_step": sum(step_ms) / nsteps, "step_ms": step_ms,
                      "nsvof_gmres_iters": it_v, "tem_gmres_iters": it_t,
-                     "max_relres": max(rel), "all_converged": bool(max(rel) <= 1e-8)}
+                     "max_relres_per_step": rel, "max_relres": max(rel),
+                     "all_converged": bool(max(rel) <= 1e-8)}
         ug = ctx.get_field("UG")
         fields[tag] = {"u": ug[:, 0], "v": ug[:, 1], "w": ug[:, 2], "p": ctx.get_field("PG"),
                        "phi": ctx.get_field("PHIG"), "T": ctx.get_field("TG")}
@@ -459,7 +467,8 @@ def quench_steps(L, C, S, n=48, nsteps=20, nnewton=3):
     log("bench: quench mixed vs FP64 after the same steps, relative L2: " +
         ", ".join(f"{k} {v['rel_l2']:.1e}" for k, v in err.items()))
     return {"workload": f"quench tank {n}^3 hex, {nsteps} steps x {nnewton} Newton iterations "
-                        "(NS + VOF then temperature), dt 1e-3, GMRES(30) to rtol 1e-8",
+                        "(NS + VOF then temperature), dt 1e-3, GMRES(30) towards rtol 1e-8, "
+                        f"at most {max_cycles} restarts per solve",
             "elements": E, "nodes": N, "dof": 6 * N, "fp64": runs["fp64"],
             "mixed": runs["mixed"],
             "speedup_mixed_vs_fp64": runs["fp64"]["ms_per_step"] / runs["mixed"]["ms_per_step"],
@@ -647,16 +656,47 @@ def cpu_faces(prob):
             "faces_per_s": len(f2e) / (t1 - t0)}
 
 
-def cpu_baseline(prob, budget_s=15.0):
-    """FP64 Newton step of the same workload on the host, single thread,
-    through the C oracle (loop-for-loop restatement of the reference):
-    assembly VEC+MAT, restarted MGS GMRES(30) cycles to |b-Ax| <= 1e-8 |b|,
-    update, VEC re-assembly and norm.  The solve is bounded to ``budget_s``;
-    the per-iteration cost of the cycles run is returned so the caller can
-    scale to the iteration count the full solve takes."""
+def _host_cpu():
+    """(model name, cores this process may use, logical CPUs of the host)."""
+    model = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return model, usable, os.cpu_count() or usable
+
+
+def _native_oracle():
+    """The oracle restatement built for this host (make -C oracle native:
+    -O3 -march=native with OpenMP), in a temporary directory."""
+    import subprocess
+    import tempfile
+    out = os.path.join(tempfile.gettempdir(), f"ofd_native_oracle_{os.getuid()}")
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "native",
+                    f"NATIVE_OUT={out}"], check=True, stdout=subprocess.DEVNULL)
+    return os.path.join(out, "liboracle_native.so")
+
+
+def cpu_newton(O, prob, threads, budget_s):
+    """One FP64 Newton step of the workload on the host through the oracle
+    (loop-for-loop restatement of the reference): VEC+MAT assembly (element
+    colours in parallel with OpenMP), restarted MGS GMRES(30) cycles to
+    |b - A x| <= 1e-8 |b| (the vector loops in parallel), update, VEC
+    re-assembly and norm.  The solve stops early after ``budget_s``; the
+    per-iteration cost is returned for the caller to scale."""
     import numpy as np
 
-    from oracle import oracle as O
+    from oversetdevfort_amd import configs as C
+    from oversetdevfort_amd import mesh as M
+    used = O.lib().orc_set_threads(threads)
     m, p = prob.mesh, prob.params
     N = m.nnode
     st = {k: v.copy() for k, v in prob.fields.items()}
@@ -664,10 +704,10 @@ def cpu_baseline(prob, budget_s=15.0):
     st["ug"] = st["ugold"].copy()
     st["acg"] = (p["gami"] - 1.0) / p["gami"] * st["acgold"]
     st["pg"] = st["pgold"].copy()
-    from oversetdevfort_amd import configs as C
     st["ug"], st["acg"] = C.apply_dirichlet(st["ug"], st["acg"], st["ugold"],
                                             st["acgold"], prob)
     col, row = O.sparsity(N, m.ien)
+    colors = M.hex_colors(m)
     prm = O.params_array(p)
 
     def alpha(s):
@@ -678,7 +718,7 @@ def cpu_baseline(prob, budget_s=15.0):
                     dg=s["dgold"] + p["alfi"] * (s["dg"] - s["dgold"]), pg=s["pg"])
 
     t0 = time.perf_counter()
-    R = O.assemble_ns(m, prob.ibc, alpha(st), prm, col, row, 6, 1)
+    R = O.assemble_ns_col(m, prob.ibc, alpha(st), prm, col, row, colors, 6, 1)
     t1 = time.perf_counter()
     bu, bp = R["rhsu"], R["rhsp"]
     bn = np.sqrt((bu ** 2).sum() + (bp ** 2).sum())
@@ -687,7 +727,7 @@ def cpu_baseline(prob, budget_s=15.0):
     ru, rp = bu, bp
     its = 0
     rel = 1.0
-    for cyc in range(60):
+    for _ in range(400):
         g = O.gmres_up(col, row, R, ru, rp, 1e-2, 30, 0)
         its += g["iters"]
         xu += g["solu"]
@@ -701,12 +741,26 @@ def cpu_baseline(prob, budget_s=15.0):
     st["acg"] = st["acg"] + xu
     st["ug"] = st["ug"] + p["gami"] * p["delt"] * xu
     st["pg"] = st["pg"] + p["alfi"] * p["gami"] * p["delt"] * xp
-    R2 = O.assemble_ns(m, prob.ibc, alpha(st), prm, col, row, 2, 1)
+    R2 = O.assemble_ns_col(m, prob.ibc, alpha(st), prm, col, row, colors, 2, 1)
     np.sqrt((R2["rhsu"] ** 2).sum())
     t3 = time.perf_counter()
-    return dict(newton_s=t3 - t0, assemble_s=t1 - t0, solve_s=t2 - t1,
+    return dict(threads=used, newton_s=t3 - t0, assemble_s=t1 - t0, solve_s=t2 - t1,
                 vec_s=t3 - t2, gmres_iters=its, s_per_iter=(t2 - t1) / max(its, 1),
                 relres=float(rel), converged=bool(rel <= 1e-8))
+
+
+def cpu_baseline(prob, one_budget_s=8.0, all_budget_s=30.0):
+    """The host baseline on this machine's cores: the oracle built with
+    -O3 -march=native, one thread (the reference's own model: one MPI rank,
+    one core) and every usable core (OpenMP).  Returns both and the host."""
+    from oracle import oracle as O
+    model, usable, logical = _host_cpu()
+    path = _native_oracle()
+    with O.using(path):
+        one = cpu_newton(O, prob, 1, one_budget_s)
+        allc = cpu_newton(O, prob, usable, all_budget_s)
+    return dict(one=one, all=allc, model=model, usable=usable, logical=logical,
+                build="gcc -O3 -march=native -fopenmp (make -C oracle native)")
 
 
 def main():
@@ -733,6 +787,9 @@ def main():
                          "copy (counter calibration), then the channel NS assembly "
                          "VEC+MAT and VEC and the 48^3 tank's temperature and NS + VOF "
                          "assemblies, 3 times each; no JSON line")
+    ap.add_argument("--spmv-large-only", action="store_true",
+                    help="only the 128x64x64 NS block product beside the read and copy "
+                         "streams (for profiled runs); no JSON line")
     ap.add_argument("--quench-steps-only", action="store_true",
                     help="only the quench configuration's time steps (FP64 and mixed); "
                          "no JSON line")
@@ -760,6 +817,9 @@ def main():
         return
     if args.quench_steps_only:
         quench_steps(L, C, S)
+        return
+    if args.spmv_large_only:
+        spmv_large(L, C, S, reps=5)
         return
 
     nx, ny, nz = (int(v) for v in args.dims.split(","))
@@ -849,29 +909,46 @@ def main():
 
     cpu = None
     if not (args.no_cpu or args.headline_only):
-        log("bench: host baseline (C oracle, 1 thread) ...")
+        log("bench: host baseline (oracle restatement, -O3 -march=native) ...")
         c = cpu_baseline(prob)
-        # the full solve takes as many iterations as the same FP64 algorithm
-        # on the GPU (mean over the timed steps); scale the sampled cycles
+        # a solve stopped by its time budget is scaled to the iteration count
+        # the same FP64 algorithm needs on the GPU (mean over the timed steps)
         need = sum(res64["gmres_iters"]) / len(res64["gmres_iters"])
-        solve_s = c["solve_s"] if c["converged"] else c["s_per_iter"] * need
-        newton_s = c["assemble_s"] + solve_s + c["vec_s"]
-        log(f"bench: host sample: assembly {c['assemble_s']:.2f} s, {c['gmres_iters']} "
-            f"GMRES its in {c['solve_s']:.2f} s ({1e3 * c['s_per_iter']:.1f} ms/it), "
-            f"VEC {c['vec_s']:.2f} s -> Newton step {newton_s:.2f} s")
-        cpu = {"value": E / newton_s / 1e6, "unit": "Melem/s (FP64 Newton step)",
-               "cores": 1, "kind": "port",
-               "sample": (f"{nx}x{ny}x{nz} channel, host, 1 thread, C oracle "
-                          f"(restatement of IntElmAss_NS_overset + BCLhs_3D + "
-                          f"FillSparseMat_3D, SparseProdUP_3D, SparseGMRES_up): full "
-                          f"VEC+MAT assembly {c['assemble_s']:.2f} s, "
-                          f"{c['gmres_iters']} restarted-GMRES(30) iterations in "
-                          f"{c['solve_s']:.2f} s (relres {c['relres']:.1e}) scaled to "
-                          f"{need:.0f} iterations, VEC re-assembly {c['vec_s']:.2f} s; "
-                          f"host {platform.processor() or platform.machine()}"),
-               "newton_ms": 1e3 * newton_s, "assembly_ms": 1e3 * c["assemble_s"],
-               "assembly_melem_per_s": E / c["assemble_s"] / 1e6,
-               "ms_per_gmres_iter": 1e3 * c["s_per_iter"]}
+
+        def newton_s(r):
+            solve = r["solve_s"] if r["converged"] else r["s_per_iter"] * need
+            return r["assemble_s"] + solve + r["vec_s"]
+
+        for tag in ("one", "all"):
+            r = c[tag]
+            log(f"bench: host {r['threads']} thread(s): assembly {r['assemble_s']:.3f} s, "
+                f"{r['gmres_iters']} GMRES its in {r['solve_s']:.2f} s "
+                f"({1e3 * r['s_per_iter']:.2f} ms/it, relres {r['relres']:.1e}), VEC "
+                f"{r['vec_s']:.3f} s -> Newton step {newton_s(r):.2f} s")
+        a, o = c["all"], c["one"]
+        cpu = {"value": E / newton_s(a) / 1e6, "unit": "Melem/s (FP64 Newton step)",
+               "cores": a["threads"], "kind": "port",
+               "sample": (f"{nx}x{ny}x{nz} channel, one FP64 Newton step on the host: the "
+                          f"oracle restatement (IntElmAss_NS_overset + BCLhs_3D + "
+                          f"FillSparseMat_3D by element colours, SparseProdUP_3D, "
+                          f"SparseGMRES_up) built {c['build']}, {a['threads']} OpenMP "
+                          f"threads; VEC+MAT assembly {a['assemble_s']:.2f} s, "
+                          f"{a['gmres_iters']} restarted-GMRES(30) iterations in "
+                          f"{a['solve_s']:.2f} s to relres {a['relres']:.1e}"
+                          + ("" if a["converged"] else
+                             f" (stopped by its budget, scaled to {need:.0f} iterations)")
+                          + f", VEC re-assembly {a['vec_s']:.2f} s; host {c['model']}"),
+               "cpu_model": c["model"], "usable_cores": c["usable"],
+               "logical_cpus": c["logical"], "build": c["build"],
+               "newton_ms": 1e3 * newton_s(a), "assembly_ms": 1e3 * a["assemble_s"],
+               "assembly_melem_per_s": E / a["assemble_s"] / 1e6,
+               "ms_per_gmres_iter": 1e3 * a["s_per_iter"], "converged": a["converged"],
+               "single_thread": {"newton_ms": 1e3 * newton_s(o),
+                                 "assembly_ms": 1e3 * o["assemble_s"],
+                                 "ms_per_gmres_iter": 1e3 * o["s_per_iter"],
+                                 "gmres_iters_run": o["gmres_iters"],
+                                 "converged": o["converged"],
+                                 "value": E / newton_s(o) / 1e6}}
         log("bench: host quench assembly (C oracle, 1 thread) ...")
         cq = cpu_quench(qprob)
         log(f"bench: host quench: NSVOF {cq['nsvof_vecmat_ms']:.0f} ms, "

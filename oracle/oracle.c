@@ -17,6 +17,27 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* The host baseline build (make native: -O3 -march=native -fopenmp) runs the
+ * element loop colour by colour with the elements of a colour in parallel,
+ * and the GMRES vector loops in parallel; the parity build has no OpenMP and
+ * keeps every loop sequential, in the reference's order. */
+#ifdef _OPENMP
+#include <omp.h>
+#define ORC_OMP(x) _Pragma(#x)
+#else
+#define ORC_OMP(x)
+#endif
+
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}
+
 #define NSD 3
 
 /* param.f90:469-480 */
@@ -520,6 +541,92 @@ void orc_bclhs_3d(int nshl, const int *ibcl, double *ke, double *ge, double *de,
 #undef ME
 }
 
+/* one element of IntElmAss_NS_overset (assembleNavSto.f90:76-336): gather,
+ * element routine, BCLhs_3D, FillSparseMat_3D (FillSparseMat.f90:33-67) and
+ * the RHS scatter of :300-305; returns the count of negative Jacobians or a
+ * negative error */
+static int orc_ns_one_element(int e, int nnode, int nelem, int nshl, int ngauss,
+                              const int *ien, const int *elm_id, const int *elm_id_set,
+                              int subdomain_flag, int tensor_flag, const double *xg,
+                              const int *ibc, const double *dgA, const double *ugA,
+                              const double *ugmA, const double *acgA, const double *acgmA,
+                              const double *pgA, const double *mus, const double *prm,
+                              int vms, const int *col, const int *row, double *k11,
+                              double *g, double *d1, double *m, double *rhsu, double *rhsp,
+                              double *cfl) {
+  const int n2 = nshl * nshl;
+  double xl[24], dl[24], ul[24], uml[24], acl[24], acml[24], pl[8], sl[8];
+  double ke[9 * 64], ge[3 * 64], de[3 * 64], me[64], ru[24], rp[8];
+  int ibcl[32];
+  int nneg = 0;
+  int tgt = 0;
+  if (subdomain_flag & 1) tgt = tgt || elm_id[e] == elm_id_set[0];
+  if (subdomain_flag & 2) tgt = tgt || elm_id[e] == elm_id_set[1];
+  for (int a = 0; a < nshl; ++a) {
+    const int n = ien[a * nelem + e];
+    for (int i = 0; i < 3; ++i) {
+      xl[i * nshl + a] = xg[i * nnode + n];
+      dl[i * nshl + a] = dgA[i * nnode + n];
+      ul[i * nshl + a] = ugA[i * nnode + n];
+      acl[i * nshl + a] = acgA[i * nnode + n];
+      uml[i * nshl + a] = ugmA[i * nnode + n];
+      acml[i * nshl + a] = acgmA[i * nnode + n];
+    }
+    pl[a] = pgA[n];
+    sl[a] = mus ? mus[n] : 1.0;
+    for (int c = 0; c < 4; ++c) ibcl[c * nshl + a] = ibc[c * nnode + n];
+  }
+  memset(ke, 0, sizeof(double) * 9 * n2);
+  memset(ge, 0, sizeof(double) * 3 * n2);
+  memset(de, 0, sizeof(double) * 3 * n2);
+  memset(me, 0, sizeof(double) * n2);
+  memset(ru, 0, sizeof(double) * 3 * nshl);
+  memset(rp, 0, sizeof(double) * nshl);
+  if (tgt) {
+    int r = orc_element_ns(nshl, ngauss, xl, dl, ul, uml, acl, acml, pl, mus ? sl : NULL, prm,
+                           vms, tensor_flag, ke, ge, de, me, ru, rp, cfl);
+    if (r < 0) return r;
+    nneg += r;
+  } else {
+    /* non-target elements still evaluate shape/CFL in the reference but
+     * contribute nothing before the BC projection */
+    double tmpk[9 * 64], tmpg[3 * 64], tmpd[3 * 64], tmpm[64], tmpu[24], tmpp[8];
+    memset(tmpk, 0, sizeof tmpk);
+    memset(tmpg, 0, sizeof tmpg);
+    memset(tmpd, 0, sizeof tmpd);
+    memset(tmpm, 0, sizeof tmpm);
+    memset(tmpu, 0, sizeof tmpu);
+    memset(tmpp, 0, sizeof tmpp);
+    int r = orc_element_ns(nshl, ngauss, xl, dl, ul, uml, acl, acml, pl, mus ? sl : NULL, prm,
+                           vms, 0, tmpk, tmpg, tmpd, tmpm, tmpu, tmpp, cfl);
+    if (r < 0) return r;
+    nneg += r;
+  }
+  orc_bclhs_3d(nshl, ibcl, ke, ge, de, me, ru, rp);
+  if (tensor_flag & 4) {
+    for (int a = 0; a < nshl; ++a) {
+      const int i = ien[a * nelem + e];
+      const int c0 = col[i], c1 = col[i + 1];
+      for (int b = 0; b < nshl; ++b) {
+        const int k = orc_sparse_loc(row, c0, c1, ien[b * nelem + e]);
+        const size_t ab = (size_t)b * nshl + a;
+        for (int q = 0; q < 9; ++q) k11[(size_t)k * 9 + q] += ke[ab * 9 + q];
+        for (int q = 0; q < 3; ++q) g[(size_t)k * 3 + q] += ge[ab * 3 + q];
+        for (int q = 0; q < 3; ++q) d1[(size_t)k * 3 + q] += de[ab * 3 + q];
+        m[k] += me[ab];
+      }
+    }
+  }
+  if (tensor_flag & 2) {
+    for (int b = 0; b < nshl; ++b) {
+      const int n = ien[b * nelem + e];
+      for (int i = 0; i < 3; ++i) rhsu[i * nnode + n] += ru[b * 3 + i];
+      rhsp[n] += rp[b];
+    }
+  }
+  return nneg;
+}
+
 /* assembleNavSto.f90:76-336 (IntElmAss_NS_overset) with BCLhs_3D,
  * FillSparseMat_3D (FillSparseMat.f90:33-67) and the RHS scatter of :300-305 */
 int orc_assemble_ns(int nnode, int nelem, int nshl, int ngauss,
@@ -532,81 +639,49 @@ int orc_assemble_ns(int nnode, int nelem, int nshl, int ngauss,
                     double *k11, double *g, double *d1, double *m, double *rhsu,
                     double *rhsp, double *cfl_out) {
   (void)ibc_ncol;
-  const int n2 = nshl * nshl;
-  double xl[24], dl[24], ul[24], uml[24], acl[24], acml[24], pl[8], sl[8];
-  double ke[9 * 64], ge[3 * 64], de[3 * 64], me[64], ru[24], rp[8];
-  int ibcl[32];
   int nneg = 0;
   double cfl = 0.0;
   for (int e = 0; e < nelem; ++e) {
-    int tgt = 0;
-    if (subdomain_flag & 1) tgt = tgt || elm_id[e] == elm_id_set[0];
-    if (subdomain_flag & 2) tgt = tgt || elm_id[e] == elm_id_set[1];
-    for (int a = 0; a < nshl; ++a) {
-      const int n = ien[a * nelem + e];
-      for (int i = 0; i < 3; ++i) {
-        xl[i * nshl + a] = xg[i * nnode + n];
-        dl[i * nshl + a] = dgA[i * nnode + n];
-        ul[i * nshl + a] = ugA[i * nnode + n];
-        acl[i * nshl + a] = acgA[i * nnode + n];
-        uml[i * nshl + a] = ugmA[i * nnode + n];
-        acml[i * nshl + a] = acgmA[i * nnode + n];
-      }
-      pl[a] = pgA[n];
-      sl[a] = mus ? mus[n] : 1.0;
-      for (int c = 0; c < 4; ++c) ibcl[c * nshl + a] = ibc[c * nnode + n];
-    }
-    memset(ke, 0, sizeof(double) * 9 * n2);
-    memset(ge, 0, sizeof(double) * 3 * n2);
-    memset(de, 0, sizeof(double) * 3 * n2);
-    memset(me, 0, sizeof(double) * n2);
-    memset(ru, 0, sizeof(double) * 3 * nshl);
-    memset(rp, 0, sizeof(double) * nshl);
-    if (tgt) {
-      int r = orc_element_ns(nshl, ngauss, xl, dl, ul, uml, acl, acml, pl,
-                             mus ? sl : NULL, prm, vms, tensor_flag, ke, ge, de, me, ru,
-                             rp, &cfl);
-      if (r < 0) return r;
-      nneg += r;
-    } else {
-      /* non-target elements still evaluate shape/CFL in the reference but
-       * contribute nothing before the BC projection */
-      double tmpk[9 * 64], tmpg[3 * 64], tmpd[3 * 64], tmpm[64], tmpu[24], tmpp[8];
-      memset(tmpk, 0, sizeof tmpk);
-      memset(tmpg, 0, sizeof tmpg);
-      memset(tmpd, 0, sizeof tmpd);
-      memset(tmpm, 0, sizeof tmpm);
-      memset(tmpu, 0, sizeof tmpu);
-      memset(tmpp, 0, sizeof tmpp);
-      int r = orc_element_ns(nshl, ngauss, xl, dl, ul, uml, acl, acml, pl,
-                             mus ? sl : NULL, prm, vms, 0, tmpk, tmpg, tmpd, tmpm, tmpu,
-                             tmpp, &cfl);
-      if (r < 0) return r;
-      nneg += r;
-    }
-    orc_bclhs_3d(nshl, ibcl, ke, ge, de, me, ru, rp);
-    if (tensor_flag & 4) {
-      for (int a = 0; a < nshl; ++a) {
-        const int i = ien[a * nelem + e];
-        const int c0 = col[i], c1 = col[i + 1];
-        for (int b = 0; b < nshl; ++b) {
-          const int k = orc_sparse_loc(row, c0, c1, ien[b * nelem + e]);
-          const size_t ab = (size_t)b * nshl + a;
-          for (int q = 0; q < 9; ++q) k11[(size_t)k * 9 + q] += ke[ab * 9 + q];
-          for (int q = 0; q < 3; ++q) g[(size_t)k * 3 + q] += ge[ab * 3 + q];
-          for (int q = 0; q < 3; ++q) d1[(size_t)k * 3 + q] += de[ab * 3 + q];
-          m[k] += me[ab];
-        }
-      }
-    }
-    if (tensor_flag & 2) {
-      for (int b = 0; b < nshl; ++b) {
-        const int n = ien[b * nelem + e];
-        for (int i = 0; i < 3; ++i) rhsu[i * nnode + n] += ru[b * 3 + i];
-        rhsp[n] += rp[b];
-      }
+    const int r = orc_ns_one_element(e, nnode, nelem, nshl, ngauss, ien, elm_id, elm_id_set,
+                                     subdomain_flag, tensor_flag, xg, ibc, dgA, ugA, ugmA,
+                                     acgA, acgmA, pgA, mus, prm, vms, col, row, k11, g, d1, m,
+                                     rhsu, rhsp, &cfl);
+    if (r < 0) return r;
+    nneg += r;
+  }
+  if (cfl_out) *cfl_out = cfl;
+  return nneg;
+}
+
+/* The same element loop, colour by colour (elements of one colour share no
+ * node, so their scatters touch disjoint entries): with OpenMP the elements
+ * of a colour run in parallel.  The host baseline's all-core form; the
+ * summation order per entry is the colour order, not the element order. */
+int orc_assemble_ns_col(int nnode, int nelem, int nshl, int ngauss, const int *ien,
+                        const int *elm_id, const int *elm_id_set, int subdomain_flag,
+                        int tensor_flag, const double *xg, const int *ibc, const double *dgA,
+                        const double *ugA, const double *ugmA, const double *acgA,
+                        const double *acgmA, const double *pgA, const double *prm,
+                        const int *col, const int *row, int ncolor, const int *color_off,
+                        const int *color_elems, double *k11, double *g, double *d1,
+                        double *m, double *rhsu, double *rhsp, double *cfl_out) {
+  int nneg = 0, err = 0;
+  double cfl = 0.0;
+  for (int c = 0; c < ncolor; ++c) {
+    ORC_OMP(omp parallel for schedule(static) reduction(+ : nneg) reduction(min : err) \
+                reduction(max : cfl))
+    for (int t = color_off[c]; t < color_off[c + 1]; ++t) {
+      const int r = orc_ns_one_element(color_elems[t], nnode, nelem, nshl, ngauss, ien, elm_id,
+                                       elm_id_set, subdomain_flag, tensor_flag, xg, ibc, dgA,
+                                       ugA, ugmA, acgA, acgmA, pgA, NULL, prm, 0, col, row, k11,
+                                       g, d1, m, rhsu, rhsp, &cfl);
+      if (r < 0)
+        err = r < err ? r : err;
+      else
+        nneg += r;
     }
   }
+  if (err < 0) return err;
   if (cfl_out) *cfl_out = cfl;
   return nneg;
 }
@@ -615,6 +690,7 @@ int orc_assemble_ns(int nnode, int nelem, int nshl, int ngauss,
 void orc_spmv_up(int nnode, const int *col, const int *row, const double *k11,
                  const double *g, const double *d1, const double *m,
                  const double *xu, const double *xp, double *yu, double *yp) {
+  ORC_OMP(omp parallel for schedule(static))
   for (int a = 0; a < nnode; ++a) {
     double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
     for (int b = col[a]; b < col[a + 1]; ++b) {
@@ -656,6 +732,7 @@ int orc_gmres_up(int nnode, const int *col, const int *row, const double *k11,
 #define VU(i, c, k) Vu[(i) + N * (c) + N * 3 * (k)]
 #define VP(i, k) Vp[(i) + N * (k)]
   /* :53-61 diagonal blocks */
+  ORC_OMP(omp parallel for schedule(static))
   for (size_t i = 0; i < N; ++i)
     for (int j = col[i]; j < col[i + 1]; ++j)
       if ((size_t)row[j] == i) {
@@ -663,6 +740,7 @@ int orc_gmres_up(int nnode, const int *col, const int *row, const double *k11,
         Md[i] = m[j];
       }
   /* :73-121 invert */
+  ORC_OMP(omp parallel for schedule(static))
   for (size_t i = 0; i < N; ++i) {
     double *B = Bd + i * 9, Bi[9], tmp, tmp2;
     Bi[0] = B[4] * B[8] - B[7] * B[5];
@@ -683,6 +761,7 @@ int orc_gmres_up(int nnode, const int *col, const int *row, const double *k11,
     Md[i] = (Md[i] == 0.0) ? 1.0 : 1.0 / Md[i];
   }
   /* :124-139 precondition residual */
+  ORC_OMP(omp parallel for schedule(static))
   for (size_t i = 0; i < N; ++i) {
     const double *B = Bd + i * 9;
     const double r0 = rhsu[i], r1 = rhsu[N + i], r2 = rhsu[2 * N + i];
@@ -693,9 +772,11 @@ int orc_gmres_up(int nnode, const int *col, const int *row, const double *k11,
   }
   /* :143-155 norm */
   double rr = 0.0;
+  ORC_OMP(omp parallel for schedule(static) reduction(+ : rr))
   for (size_t i = 0; i < N; ++i) rr += VP(i, 0) * VP(i, 0);
   for (int c = 0; c < 3; ++c) {
     double s = 0.0;
+    ORC_OMP(omp parallel for schedule(static) reduction(+ : s))
     for (size_t i = 0; i < N; ++i) s += VU(i, c, 0) * VU(i, c, 0);
     rr = rr + s;
   }
@@ -703,6 +784,7 @@ int orc_gmres_up(int nnode, const int *col, const int *row, const double *k11,
   if (rnorm0) *rnorm0 = unorm;
   const double epsnrm = tol * unorm;
   e[0] = unorm;
+  ORC_OMP(omp parallel for schedule(static))
   for (size_t i = 0; i < N; ++i) {
     for (int c = 0; c < 3; ++c) VU(i, c, 0) = VU(i, c, 0) / unorm;
     VP(i, 0) = VP(i, 0) / unorm;
@@ -715,6 +797,7 @@ int orc_gmres_up(int nnode, const int *col, const int *row, const double *k11,
     /* :200-203 product into v_{k+1} */
     orc_spmv_up(nnode, col, row, k11, g, d1, m, &VU(0, 0, kc), &VP(0, kc), tu, tp);
     /* :231-243 precondition product */
+    ORC_OMP(omp parallel for schedule(static))
     for (size_t i = 0; i < N; ++i) {
       const double *B = Bd + i * 9;
       const double r0 = tu[i], r1 = tu[N + i], r2 = tu[2 * N + i];
@@ -727,6 +810,7 @@ int orc_gmres_up(int nnode, const int *col, const int *row, const double *k11,
     for (int jK = 1; jK <= iKs + 1; ++jK) {
       const int jc = jK - 1;
       if (jK > 1) {
+        ORC_OMP(omp parallel for schedule(static))
         for (size_t i = 0; i < N; ++i) {
           for (int c = 0; c < 3; ++c)
             VU(i, c, kc + 1) = VU(i, c, kc + 1) - beta * VU(i, c, jc - 1);
@@ -734,9 +818,11 @@ int orc_gmres_up(int nnode, const int *col, const int *row, const double *k11,
         }
       }
       double r = 0.0;
+      ORC_OMP(omp parallel for schedule(static) reduction(+ : r))
       for (size_t i = 0; i < N; ++i) r += VP(i, kc + 1) * VP(i, jc);
       for (int c = 0; c < 3; ++c) {
         double s = 0.0;
+        ORC_OMP(omp parallel for schedule(static) reduction(+ : s))
         for (size_t i = 0; i < N; ++i) s += VU(i, c, kc + 1) * VU(i, c, jc);
         r = r + s;
       }
@@ -745,6 +831,7 @@ int orc_gmres_up(int nnode, const int *col, const int *row, const double *k11,
     }
     unorm = sqrt(beta);
     HH(kc + 1, kc) = unorm;
+    ORC_OMP(omp parallel for schedule(static))
     for (size_t i = 0; i < N; ++i) {
       for (int c = 0; c < 3; ++c) VU(i, c, kc + 1) = VU(i, c, kc + 1) / unorm;
       VP(i, kc + 1) = VP(i, kc + 1) / unorm;
@@ -771,11 +858,13 @@ int orc_gmres_up(int nnode, const int *col, const int *row, const double *k11,
     y[jK] = e[jK] / HH(jK, jK);
     for (int l = 0; l < jK; ++l) e[l] = e[l] - y[jK] * HH(l, jK);
   }
-  for (int jK = 0; jK < iKs; ++jK)
+  for (int jK = 0; jK < iKs; ++jK) {
+    ORC_OMP(omp parallel for schedule(static))
     for (size_t i = 0; i < N; ++i) {
       for (int c = 0; c < 3; ++c) solu[c * N + i] = solu[c * N + i] + y[jK] * VU(i, c, jK);
       solp[i] = solp[i] + y[jK] * VP(i, jK);
     }
+  }
   if (resid) *resid = ercheck;
 #undef HH
 #undef VU

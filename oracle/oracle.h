@@ -201,6 +201,20 @@ int orc_gmres_ns_conv(int nnode, const int *col, const int *row, const double *k
                       double *solp, double *soll, double tol, int kspace, int kspace_min,
                       double *resid);
 
+/* host baseline: threads of the OpenMP build (n > 0 sets them; returns the
+ * count in use, 1 for the parity build) */
+int orc_set_threads(int n);
+/* the NS element loop colour by colour, the elements of a colour in parallel
+ * in the OpenMP build (constant viscosity, the reference's residual form) */
+int orc_assemble_ns_col(int nnode, int nelem, int nshl, int ngauss, const int *ien,
+                        const int *elm_id, const int *elm_id_set, int subdomain_flag,
+                        int tensor_flag, const double *xg, const int *ibc, const double *dgA,
+                        const double *ugA, const double *ugmA, const double *acgA,
+                        const double *acgmA, const double *pgA, const double *prm,
+                        const int *col, const int *row, int ncolor, const int *color_off,
+                        const int *color_elems, double *k11, double *g, double *d1,
+                        double *m, double *rhsu, double *rhsp, double *cfl_out);
+
 #ifdef __cplusplus
 }
 #endif

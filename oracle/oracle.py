@@ -46,7 +46,13 @@ assemble_nsvof         reference IntElmAss_NSVOF_Quenching_STAB relinked to
                        capturing
 spmv_ns_conv           reference SparseProd_NS_conv bit-exact
 gmres_ns_conv          reference SparseGMRES_ns_conv bit-exact (solution)
+assemble_ns_col        the same per-element routine as assemble_ns in colour
+                       order; equal to assemble_ns to rounding (tests)
 =====================  =====================================================
+
+``using(path)`` swaps in another build of the same source for the host
+baseline (``make -C oracle native``: -O3 -march=native with OpenMP), whose
+rounding differs from the parity build's.
 """
 from __future__ import annotations
 
@@ -68,64 +74,92 @@ def lib():
         path = os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             raise RuntimeError(f"oracle not built: {path} (run make -C oracle)")
-        L = ctypes.CDLL(path)
-        L.orc_genalpha.argtypes = [ctypes.c_double, _dp]
-        L.orc_gauss_volume.argtypes = [ctypes.c_int, _dp, _dp]
-        L.orc_gauss_volume.restype = ctypes.c_int
-        for f in (L.orc_linhexshl, L.orc_lintetshl, L.orc_linprishl):
-            f.argtypes = [_dp, _dp, _dp]
-        L.orc_inverse3x3.argtypes = [_dp, _dp, _dp]
-        L.orc_eval_shape_fem.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp,
-                                         _dp, _dp, _dp, _dp]
-        L.orc_eval_shape_fem.restype = ctypes.c_int
-        L.orc_stab_ns.argtypes = [_dp, ctypes.c_double, _dp, ctypes.c_double,
-                                  ctypes.c_double, _dp, _dp, _dp]
-        L.orc_sparsity.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                   _ip, ctypes.c_void_p, ctypes.c_void_p]
-        L.orc_sparsity.restype = ctypes.c_long
-        L.orc_assemble_ns.argtypes = (
-            [ctypes.c_int] * 4 + [_ip, _ip, _ip, ctypes.c_int, ctypes.c_int, _dp,
-                                  _ip, ctypes.c_int] + [_dp] * 6 + [ctypes.c_void_p, _dp,
-                                                                   ctypes.c_int] +
-            [_ip, _ip] + [_dp] * 6 + [_dp])
-        L.orc_assemble_ns.restype = ctypes.c_int
-        L.orc_face_ns.argtypes = ([ctypes.c_int] * 3 + [_ip] * 3 + [ctypes.c_int] * 2 +
-                                  [_dp, _ip] + [_dp] * 5 + [ctypes.c_int, _ip, _ip,
-                                                            ctypes.c_int, _ip, _ip] +
-                                  [_dp] * 6)
-        L.orc_element_ns.argtypes = ([ctypes.c_int, ctypes.c_int] + [_dp] * 7 +
-                                     [ctypes.c_void_p, _dp, ctypes.c_int, ctypes.c_int] +
-                                     [_dp] * 7)
-        L.orc_element_ns.restype = ctypes.c_int
-        L.orc_bclhs_3d.argtypes = [ctypes.c_int, _ip] + [_dp] * 6
-        L.orc_spmv_up.argtypes = [ctypes.c_int, _ip, _ip] + [_dp] * 8
-        L.orc_gmres_up.argtypes = ([ctypes.c_int, _ip, _ip] + [_dp] * 8 +
-                                   [ctypes.c_double, ctypes.c_int, ctypes.c_int,
-                                    _dp, _dp])
-        L.orc_gmres_up.restype = ctypes.c_int
-        L.orc_interp_tet.argtypes = ([ctypes.c_int] * 3 + [_ip, _ip, ctypes.c_int] +
-                                     [_dp] * 3 + [ctypes.c_int, _dp, _dp, _ip, _dp])
-        L.orc_assemble_tem.argtypes = ([ctypes.c_int] * 4 + [_ip, _ip, ctypes.c_int, _dp,
-                                        _ip] + [_dp] * 6 + [ctypes.c_void_p] + [_dp] * 2 +
-                                       [_ip, _ip, _dp, _dp])
-        L.orc_assemble_tem.restype = ctypes.c_int
-        L.orc_spmv_s.argtypes = [ctypes.c_int, _ip, _ip, _dp, _dp, _dp]
-        L.orc_gmres_tem.argtypes = ([ctypes.c_int, _ip, _ip, _dp, _dp, _dp, ctypes.c_double,
-                                     ctypes.c_int, ctypes.c_int, _dp])
-        L.orc_gmres_tem.restype = ctypes.c_int
-        L.orc_assemble_nsvof.argtypes = ([ctypes.c_int] * 4 + [_ip, _ip, ctypes.c_int, _dp,
-                                          _ip] + [_dp] * 11 + [_ip, _ip] + [_dp] * 12)
-        L.orc_assemble_nsvof.restype = ctypes.c_int
-        L.orc_spmv_ns_conv.argtypes = [ctypes.c_int, _ip, _ip] + [_dp] * 14
-        L.orc_gmres_ns_conv.argtypes = ([ctypes.c_int, _ip, _ip] + [_dp] * 14 +
-                                        [ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp])
-        L.orc_gmres_ns_conv.restype = ctypes.c_int
-        L.orc_interp_hex.argtypes = ([ctypes.c_int] * 2 + [_ip, _ip, ctypes.c_int] +
-                                     [_dp] * 3 + [ctypes.c_int, _dp, _dp, _ip, _dp])
-        L.orc_hex_local.argtypes = [_dp, _dp, _dp]
-        L.orc_hex_local.restype = ctypes.c_int
-        _LIB = L
+        _LIB = _load(path)
     return _LIB
+
+
+class using:
+    """``with using(path):`` every wrapper calls the build at ``path``."""
+
+    def __init__(self, path):
+        self.path = path
+
+    def __enter__(self):
+        global _LIB
+        self.saved = _LIB
+        _LIB = _load(self.path)
+        return _LIB
+
+    def __exit__(self, *a):
+        global _LIB
+        _LIB = self.saved
+
+
+def _load(path):
+    """The oracle library at ``path`` with every signature set."""
+    L = ctypes.CDLL(path)
+    L.orc_set_threads.argtypes = [ctypes.c_int]
+    L.orc_set_threads.restype = ctypes.c_int
+    L.orc_assemble_ns_col.argtypes = (
+        [ctypes.c_int] * 4 + [_ip, _ip, _ip, ctypes.c_int, ctypes.c_int, _dp, _ip] +
+        [_dp] * 7 + [_ip, _ip, ctypes.c_int, _ip, _ip] + [_dp] * 7)
+    L.orc_assemble_ns_col.restype = ctypes.c_int
+    L.orc_genalpha.argtypes = [ctypes.c_double, _dp]
+    L.orc_gauss_volume.argtypes = [ctypes.c_int, _dp, _dp]
+    L.orc_gauss_volume.restype = ctypes.c_int
+    for f in (L.orc_linhexshl, L.orc_lintetshl, L.orc_linprishl):
+        f.argtypes = [_dp, _dp, _dp]
+    L.orc_inverse3x3.argtypes = [_dp, _dp, _dp]
+    L.orc_eval_shape_fem.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp,
+                                     _dp, _dp, _dp, _dp]
+    L.orc_eval_shape_fem.restype = ctypes.c_int
+    L.orc_stab_ns.argtypes = [_dp, ctypes.c_double, _dp, ctypes.c_double,
+                              ctypes.c_double, _dp, _dp, _dp]
+    L.orc_sparsity.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                               _ip, ctypes.c_void_p, ctypes.c_void_p]
+    L.orc_sparsity.restype = ctypes.c_long
+    L.orc_assemble_ns.argtypes = (
+        [ctypes.c_int] * 4 + [_ip, _ip, _ip, ctypes.c_int, ctypes.c_int, _dp,
+                              _ip, ctypes.c_int] + [_dp] * 6 + [ctypes.c_void_p, _dp,
+                                                               ctypes.c_int] +
+        [_ip, _ip] + [_dp] * 6 + [_dp])
+    L.orc_assemble_ns.restype = ctypes.c_int
+    L.orc_face_ns.argtypes = ([ctypes.c_int] * 3 + [_ip] * 3 + [ctypes.c_int] * 2 +
+                              [_dp, _ip] + [_dp] * 5 + [ctypes.c_int, _ip, _ip,
+                                                        ctypes.c_int, _ip, _ip] +
+                              [_dp] * 6)
+    L.orc_element_ns.argtypes = ([ctypes.c_int, ctypes.c_int] + [_dp] * 7 +
+                                 [ctypes.c_void_p, _dp, ctypes.c_int, ctypes.c_int] +
+                                 [_dp] * 7)
+    L.orc_element_ns.restype = ctypes.c_int
+    L.orc_bclhs_3d.argtypes = [ctypes.c_int, _ip] + [_dp] * 6
+    L.orc_spmv_up.argtypes = [ctypes.c_int, _ip, _ip] + [_dp] * 8
+    L.orc_gmres_up.argtypes = ([ctypes.c_int, _ip, _ip] + [_dp] * 8 +
+                               [ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                _dp, _dp])
+    L.orc_gmres_up.restype = ctypes.c_int
+    L.orc_interp_tet.argtypes = ([ctypes.c_int] * 3 + [_ip, _ip, ctypes.c_int] +
+                                 [_dp] * 3 + [ctypes.c_int, _dp, _dp, _ip, _dp])
+    L.orc_assemble_tem.argtypes = ([ctypes.c_int] * 4 + [_ip, _ip, ctypes.c_int, _dp,
+                                    _ip] + [_dp] * 6 + [ctypes.c_void_p] + [_dp] * 2 +
+                                   [_ip, _ip, _dp, _dp])
+    L.orc_assemble_tem.restype = ctypes.c_int
+    L.orc_spmv_s.argtypes = [ctypes.c_int, _ip, _ip, _dp, _dp, _dp]
+    L.orc_gmres_tem.argtypes = ([ctypes.c_int, _ip, _ip, _dp, _dp, _dp, ctypes.c_double,
+                                 ctypes.c_int, ctypes.c_int, _dp])
+    L.orc_gmres_tem.restype = ctypes.c_int
+    L.orc_assemble_nsvof.argtypes = ([ctypes.c_int] * 4 + [_ip, _ip, ctypes.c_int, _dp,
+                                      _ip] + [_dp] * 11 + [_ip, _ip] + [_dp] * 12)
+    L.orc_assemble_nsvof.restype = ctypes.c_int
+    L.orc_spmv_ns_conv.argtypes = [ctypes.c_int, _ip, _ip] + [_dp] * 14
+    L.orc_gmres_ns_conv.argtypes = ([ctypes.c_int, _ip, _ip] + [_dp] * 14 +
+                                    [ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp])
+    L.orc_gmres_ns_conv.restype = ctypes.c_int
+    L.orc_interp_hex.argtypes = ([ctypes.c_int] * 2 + [_ip, _ip, ctypes.c_int] +
+                                 [_dp] * 3 + [ctypes.c_int, _dp, _dp, _ip, _dp])
+    L.orc_hex_local.argtypes = [_dp, _dp, _dp]
+    L.orc_hex_local.restype = ctypes.c_int
+    return L
 
 
 def _c(a, dt=np.float64):
@@ -235,6 +269,36 @@ def assemble_ns(mesh, ibc, state, prm, col, row, tensor_flag=6,
     return dict(k11=k11.reshape(icnt, 9), g=g.reshape(icnt, 3),
                 d1=d1.reshape(icnt, 3), m=m, rhsu=rhsu.reshape(3, N).T.copy(),
                 rhsp=rhsp, cfl=float(cfl[0]), nneg=int(nneg))
+
+
+def assemble_ns_col(mesh, ibc, state, prm, col, row, colors, tensor_flag=6,
+                    subdomain_flag=1, elm_id_set=(1, 2)):
+    """IntElmAss_NS_overset colour by colour (``colors``: colour of each
+    element); the elements of a colour in parallel in an OpenMP build."""
+    N, E, nshl = mesh.nnode, mesh.nelem, mesh.nshl
+    colors = np.asarray(colors)
+    order = np.argsort(colors, kind="stable").astype(np.int32)
+    ncol = int(colors.max()) + 1
+    off = np.zeros(ncol + 1, np.int32)
+    off[1:] = np.cumsum(np.bincount(colors, minlength=ncol))
+    icnt = row.size
+    out = dict(k11=np.zeros(icnt * 9), g=np.zeros(icnt * 3), d1=np.zeros(icnt * 3),
+               m=np.zeros(icnt), rhsu=np.zeros(3 * N), rhsp=np.zeros(N))
+    cfl = np.zeros(1)
+    ibcf = _c(np.asarray(ibc)[:, :4].T, np.int32).ravel()
+    nneg = lib().orc_assemble_ns_col(
+        N, E, nshl, nshl, _c(mesh.ien.T, np.int32).ravel(), _c(mesh.elm_id, np.int32),
+        _c(elm_id_set, np.int32), subdomain_flag, tensor_flag, _soa(mesh.xg), ibcf,
+        _soa(state["dg"]), _soa(state["ug"]), _soa(state["ugm"]), _soa(state["acg"]),
+        _soa(state["acgm"]), _c(state["pg"]), _c(prm), _c(col, np.int32), _c(row, np.int32),
+        ncol, off, order, out["k11"], out["g"], out["d1"], out["m"], out["rhsu"],
+        out["rhsp"], cfl)
+    if nneg < 0:
+        raise RuntimeError("oracle assembly failed")
+    return dict(k11=out["k11"].reshape(icnt, 9), g=out["g"].reshape(icnt, 3),
+                d1=out["d1"].reshape(icnt, 3), m=out["m"],
+                rhsu=out["rhsu"].reshape(3, N).T.copy(), rhsp=out["rhsp"], cfl=float(cfl[0]),
+                nneg=int(nneg))
 
 
 def face_ns(mesh, ibc, state, prm, col, row, faces, tensor_flag=6, subdomain_flag=1,

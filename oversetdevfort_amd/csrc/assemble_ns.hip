@@ -15,21 +15,21 @@
 //            accumulated over the 8 Gauss points in registers.
 //   phase C  Dirichlet projection as row/column masks (BCLhs_3D zeroes rows and
 //            columns of constrained dofs and sets their diagonal to 1, in an
-//            order that makes the masks exact), then a deterministic
-//            read-modify-write of the block at emap[e][lane]: elements are
-//            launched one colour at a time and no two elements of a colour
+//            order that makes the masks exact), then the scatter of the 64
+//            blocks to their nonzeros emap[e][a + 8 b], 8 lanes per 128-byte
+//            block (a transpose through LDS): the first element (in colour
+//            order) to reach a nonzero stores, the later ones add.  Elements
+//            are launched one colour at a time and no two elements of a colour
 //            share a node, so no atomics and a fixed summation order.
 #include "ofd_internal.h"
 #include "wave_dpp.h"
+#include "element_shape.h"
 
 namespace ofd {
 
 namespace {
 
 constexpr int kWavesPerBlock = 4;
-// linhexshl corner signs (sx, sy, sz) in the reference node order
-__constant__ int8_t c_sign[8][3] = {{-1, 1, -1}, {1, 1, -1}, {1, 1, 1},  {-1, 1, 1},
-                                    {-1, -1, -1}, {1, -1, -1}, {1, -1, 1}, {-1, -1, 1}};
 
 __device__ __forceinline__ double red8(double v) { return wsum8(v); }
 __device__ __forceinline__ double max8(double v) { return wmax8(v); }
@@ -87,12 +87,13 @@ struct AsmParams {
   int vms;  // 1: the residual with the fine-scale velocity uprime = -tauM rLi
 };
 
-// S = 8: the trilinear hex, 8 Gauss points (the layout above).  S = 4: the
-// linear tet of the reference's live dispatch (lintetshl, eval_shape_FEM.f90:
-// 198-226; the 4-point rule of genGPandGWtetpri), padded to the same 8x8
-// lanes: node lanes a >= 4 carry zero shape values and gradients (the
-// butterfly sums are unchanged), point lanes q >= 4 repeat points 0..3 with
-// zero weight, and the padded (a, b) pairs are not scattered.
+// S = 8: the trilinear hex, 8 Gauss points (the layout above).  S = 4 and
+// S = 6: the linear tet and prism of the reference's live dispatch
+// (lintetshl, linprishl; eval_shape_FEM.f90:18-26), at the 4- and 6-point
+// rules of genGPandGWtetpri, padded to the same 8x8 lanes: node lanes a >= S
+// carry zero shape values and gradients (the butterfly sums are unchanged),
+// point lanes q >= S repeat the first points with zero weight, and the padded
+// (a, b) pairs are not scattered.
 template <int S>
 __global__ void __launch_bounds__(256)
 k_assemble_ns_el(int N, int E, const int32_t *__restrict__ ien,
@@ -111,6 +112,9 @@ k_assemble_ns_el(int N, int E, const int32_t *__restrict__ ien,
   // ct = fact2 tauC dw, tauM (= tauP), pad
   __shared__ double2 s_q[kWavesPerBlock][8][3];
   __shared__ int32_t s_node[kWavesPerBlock][8];
+  // [lane][16 entries + 2 spare]: the spare doubles put the 64 lanes' 16-byte
+  // pieces in distinct bank groups on the way in and on the way out
+  __shared__ __align__(16) double s_blk[kWavesPerBlock][64][18];
 
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
@@ -148,37 +152,10 @@ k_assemble_ns_el(int N, int E, const int32_t *__restrict__ ien,
   const double *uma = xr + 9;  // um
   const double pa = xr[12];
 
-  // reference shape functions at Gauss point q
+  // reference shape functions at Gauss point q (linhexshl, lintetshl or
+  // linprishl at the rule of genGPandGWtetpri)
   double Na, dl0, dl1, dl2, gwq;
-  if constexpr (S == 8) {
-    // linhexshl, 8-point rule with unit weights
-    const double gpt = 0.57735026918963;
-    const double xi = (q & 1) ? gpt : -gpt;
-    const double eta = (q & 2) ? gpt : -gpt;
-    const double zeta = (q & 4) ? gpt : -gpt;
-    const double sx = c_sign[a][0], sy = c_sign[a][1], sz = c_sign[a][2];
-    const double fx = (sx > 0) ? (xi + 1.0) : (1.0 - xi);
-    const double fy = (sy > 0) ? (eta + 1.0) : (1.0 - eta);
-    const double fz = (sz > 0) ? (zeta + 1.0) : (1.0 - zeta);
-    Na = fx * fy * fz * 0.125;
-    dl0 = sx * (fy * fz * 0.125);
-    dl1 = sy * (fx * fz * 0.125);
-    dl2 = sz * (fx * fy * 0.125);
-    gwq = 1.0;
-  } else {
-    // lintetshl at the 4-point rule (a, b, b), (b, a, b), (b, b, a), (b, b, b)
-    const double ga_ = 0.5854101966249685, gb_ = 0.1381966011250105;
-    const int q4 = q & 3;
-    const double xi = (q4 == 0) ? ga_ : gb_;
-    const double eta = (q4 == 1) ? ga_ : gb_;
-    const double zeta = (q4 == 2) ? ga_ : gb_;
-    Na = (a == 0) ? xi : (a == 1) ? eta : (a == 2) ? zeta
-                                  : (a == 3) ? 1.0 - xi - eta - zeta : 0.0;
-    dl0 = (a == 0) ? 1.0 : (a == 3) ? -1.0 : 0.0;
-    dl1 = (a == 1) ? 1.0 : (a == 3) ? -1.0 : 0.0;
-    dl2 = (a == 2) ? 1.0 : (a == 3) ? -1.0 : 0.0;
-    gwq = (q < 4) ? 0.2500000000000000 / 6.0 : 0.0;
-  }
+  elem_shape<S>(q, a, Na, dl0, dl1, dl2, gwq);
 
   // dx/dxi (i,j) = sum_a (x+d)_a,i dN_a/dxi_j
   double J[9];
@@ -263,8 +240,8 @@ k_assemble_ns_el(int N, int E, const int32_t *__restrict__ ien,
 #pragma unroll
     for (int j = 0; j < 3; ++j) cflq += uadv[i] * G[i + 3 * j] * uadv[j];
   cflq = P.delt * sqrt(cflq);
-  const double dw = detj * gwq;  // hex weights are 1; padded tet points 0
-  if (valid && a == 0 && (S == 8 || q < 4) && detj < 0.0) atomicAdd(negdet, 1);
+  const double dw = detj * gwq;  // padded points have weight 0
+  if (valid && a == 0 && q < S && detj < 0.0) atomicAdd(negdet, 1);
 
   const double tmp1a = rho * (uadv[0] * ga[0] + uadv[1] * ga[1] + uadv[2] * ga[2]);
   s_sh[w][q][a][0] = make_double2(Na, ga[0]);
@@ -359,7 +336,7 @@ k_assemble_ns_el(int N, int E, const int32_t *__restrict__ ien,
     //   D1_c   += N_a cf g_b,c + g_a,c tauM (cB N_b + cf t1_b)
     //   M      += tauM cf g_a.g_b
 #pragma unroll 2
-    for (int qq = 0; qq < (S == 8 ? 8 : 4); ++qq) {
+    for (int qq = 0; qq < S; ++qq) {
       const double2 r0 = s_sh[w][qq][pa_][0], r1 = s_sh[w][qq][pa_][1],
                     r2 = s_sh[w][qq][pa_][2];
       const double2 c0 = s_sh[w][qq][pb_][0], c1 = s_sh[w][qq][pb_][1],
@@ -404,30 +381,50 @@ k_assemble_ns_el(int N, int E, const int32_t *__restrict__ ien,
     const unsigned ma = ibc[nra];
     const unsigned mb = ibc[nrb];
     if (do_mat) {
-      const int k = emap[(size_t)e * 64 + lane];
-      // the first element (in colour order) to reach this nonzero stores,
-      // the later ones add: no zeroing pass over the matrix
-      const bool first = (firstmask[e] >> lane) & 1ull;
-      double2 *blk = reinterpret_cast<double2 *>(A + (size_t)k * 16);
 #pragma unroll
-      for (int t = 0; t < 8; ++t) {
-        double2 old = first ? make_double2(0.0, 0.0) : blk[t];
-        double v[2] = {acc[2 * t], acc[2 * t + 1]};
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const int r = (2 * t + h) >> 2, cc = (2 * t + h) & 3;
-          // row r of node a and column cc of node b are projected away, except
-          // that a pressure constraint clears the G column but not the M
-          // column (BCLhs_3D.f90:105-118)
-          const bool colz = ((mb >> cc) & 1u) && !(r == 3 && cc == 3);
-          if (((ma >> r) & 1u) || colz) v[h] = 0.0;
-          if (pa_ == pb_ && r == cc && ((ma >> r) & 1u)) v[h] = 1.0;
-        }
-        old.x += v[0];
-        old.y += v[1];
-        blk[t] = old;
+      for (int t = 0; t < 16; ++t) {
+        const int r = t >> 2, cc = t & 3;
+        // row r of node a and column cc of node b are projected away, except
+        // that a pressure constraint clears the G column but not the M
+        // column (BCLhs_3D.f90:105-118)
+        const bool colz = ((mb >> cc) & 1u) && !(r == 3 && cc == 3);
+        if (((ma >> r) & 1u) || colz) acc[t] = 0.0;
+        if (pa_ == pb_ && r == cc && ((ma >> r) & 1u)) acc[t] = 1.0;
       }
     }
+  }
+  if (valid && do_mat) {
+    // the block of lane (a, b) goes through LDS so that 8 lanes move each
+    // 128-byte block: lane = (block in a group of 8, 16-byte piece)
+    double2 *mine = reinterpret_cast<double2 *>(s_blk[w][lane]);
+#pragma unroll
+    for (int t = 0; t < 8; ++t) mine[t] = make_double2(acc[2 * t], acc[2 * t + 1]);
+    // the lanes of this wavefront read what the others wrote
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    // the first element (in colour order) to reach a nonzero stores, the
+    // later ones add: no zeroing pass over the matrix
+    const uint64_t fm = firstmask[e];
+    const int piece = lane & 7;
+#pragma unroll
+    for (int rnd = 0; rnd < 8; ++rnd) {
+      const int j = rnd * 8 + (lane >> 3);  // block pair j = a + 8 b
+      if ((j & 7) < S && (j >> 3) < S) {
+        const int k = emap[(size_t)e * 64 + j];
+        const double2 v = reinterpret_cast<const double2 *>(s_blk[w][j])[piece];
+        double2 *dst = reinterpret_cast<double2 *>(A + (size_t)k * 16) + piece;
+        if ((fm >> j) & 1ull) {
+          *dst = v;
+        } else {
+          double2 o = *dst;
+          o.x += v.x;
+          o.y += v.y;
+          *dst = o;
+        }
+      }
+    }
+  }
+  if (valid && pa_ < S && pb_ < S) {
     if (do_vec && lane < S) {
       // lane < S holds the q-reduced RHS of node a = lane
       const unsigned m0 = ibc[na];
@@ -521,8 +518,9 @@ hipError_t launch_assemble_ns_hex(Ctx &c, int tensor_flag, int subdomain_flag,
   P.gz = c.prm[OFD_P_GZ];
   P.vms = c.res_form;
   const int blocks = (count + kWavesPerBlock - 1) / kWavesPerBlock;
-  hipLaunchKernelGGL((c.nshl == 4 ? k_assemble_ns_el<4> : k_assemble_ns_el<8>), dim3(blocks),
-                     dim3(64 * kWavesPerBlock), 0,
+  hipLaunchKernelGGL((c.nshl == 4 ? k_assemble_ns_el<4>
+                      : c.nshl == 6 ? k_assemble_ns_el<6> : k_assemble_ns_el<8>),
+                     dim3(blocks), dim3(64 * kWavesPerBlock), 0,
                      c.stream, c.nnode, c.nelem, c.d_ien, c.d_elm_id,
                      c.d_color_elems, first, count, c.d_emap, c.d_first, c.d_noderec,
                      c.mus_on ? c.d_field[OFD_MUSCALE] : nullptr, c.d_ibc,

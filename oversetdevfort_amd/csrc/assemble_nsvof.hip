@@ -24,6 +24,7 @@
 // read-modify-write scatter.
 #include "ofd_internal.h"
 #include "wave_dpp.h"
+#include "element_shape.h"
 
 namespace ofd {
 
@@ -31,8 +32,6 @@ namespace {
 
 constexpr int kWaves = 4;
 constexpr int kQs = 14;  // per-Gauss-point scalars staged in LDS
-__constant__ int8_t c_sgn5[8][3] = {{-1, 1, -1}, {1, 1, -1}, {1, 1, 1},  {-1, 1, 1},
-                                    {-1, -1, -1}, {1, -1, -1}, {1, -1, 1}, {-1, -1, 1}};
 
 __device__ __forceinline__ double s8(double v) { return wsum8(v); }
 
@@ -62,8 +61,9 @@ struct NsvofParams {
   int fluid_id;
 };
 
+template <int NSHL>
 __global__ void __launch_bounds__(256)
-k_assemble_nsvof_hex(int E, const int32_t *__restrict__ ien, const int32_t *__restrict__ elm_id,
+k_assemble_nsvof_el(int E, const int32_t *__restrict__ ien, const int32_t *__restrict__ elm_id,
                      const int32_t *__restrict__ elems, int first, int count,
                      const int32_t *__restrict__ emap,
                      const uint64_t *__restrict__ firstmask, const double *__restrict__ rec,
@@ -84,7 +84,7 @@ k_assemble_nsvof_hex(int E, const int32_t *__restrict__ ien, const int32_t *__re
   // ---------------- phase A: lane = (q, a) ----------------
   const int q = lane >> 3;
   const int a = lane & 7;
-  const int na = valid ? ien[(size_t)a * E + e] : 0;
+  const int na = valid ? ien[(size_t)(a < NSHL ? a : 0) * E + e] : 0;
   if (lane < 8) s_node[w][lane] = na;
   // record: x+d (0-2), u (3-5), ac (6-8), um (9-11), p, phi, T, rphi (12-15)
   const double2 *r2 = reinterpret_cast<const double2 *>(rec + (size_t)na * 16);
@@ -98,18 +98,8 @@ k_assemble_nsvof_hex(int E, const int32_t *__restrict__ ien, const int32_t *__re
   const double *xd = rv, *ua = rv + 3, *aca = rv + 6, *uma = rv + 9;
   const double pa = rv[12], phia = rv[13], Ta = rv[14], rphia = rv[15];
 
-  const double gpt = 0.57735026918963;
-  const double xi = (q & 1) ? gpt : -gpt;
-  const double eta = (q & 2) ? gpt : -gpt;
-  const double zeta = (q & 4) ? gpt : -gpt;
-  const double sx = c_sgn5[a][0], sy = c_sgn5[a][1], sz = c_sgn5[a][2];
-  const double fx = (sx > 0) ? (xi + 1.0) : (1.0 - xi);
-  const double fy = (sy > 0) ? (eta + 1.0) : (1.0 - eta);
-  const double fz = (sz > 0) ? (zeta + 1.0) : (1.0 - zeta);
-  const double Na = fx * fy * fz * 0.125;
-  const double dl0 = sx * (fy * fz * 0.125);
-  const double dl1 = sy * (fx * fz * 0.125);
-  const double dl2 = sz * (fx * fy * 0.125);
+  double Na, dl0, dl1, dl2, gwq;
+  elem_shape<NSHL>(q, a, Na, dl0, dl1, dl2, gwq);
   double J[9];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
@@ -222,8 +212,8 @@ k_assemble_nsvof_hex(int E, const int32_t *__restrict__ ien, const int32_t *__re
     for (int j = 0; j < 3; ++j) cf = cf + uadv[i] * G[i + 3 * j] * uadv[j];
   cf = P.delt * sqrt(cf);
   const double sc = uadv[0] * ga[0] + uadv[1] * ga[1] + uadv[2] * ga[2];
-  const double dw = detj;
-  if (valid && a == 0 && detj < 0.0) atomicAdd(negdet, 1);
+  const double dw = detj * gwq;  // hex weights are 1; padded tet points 0
+  if (valid && a == 0 && (NSHL == 8 || q < 4) && detj < 0.0) atomicAdd(negdet, 1);
 
   s_sh[w][q][a][0] = Na;
   s_sh[w][q][a][1] = ga[0];
@@ -299,7 +289,7 @@ k_assemble_nsvof_hex(int E, const int32_t *__restrict__ ien, const int32_t *__re
   if (do_mat && fluid) {
     const double f1 = P.almi, f2 = P.alfi * P.delt * P.gami;
     const double cr = 1 / P.rhoa - 1 / P.rhow;
-    for (int qq = 0; qq < 8; ++qq) {
+    for (int qq = 0; qq < (NSHL == 8 ? 8 : 4); ++qq) {
       const double Nq = s_sh[w][qq][pa_][0];
       const double ga0 = s_sh[w][qq][pa_][1], ga1 = s_sh[w][qq][pa_][2],
                    ga2 = s_sh[w][qq][pa_][3], sa = s_sh[w][qq][pa_][4];
@@ -347,7 +337,7 @@ k_assemble_nsvof_hex(int E, const int32_t *__restrict__ ien, const int32_t *__re
   }
 
   // ---------------- phase C: BCLhs_NS_conv + scatter ----------------
-  if (valid) {
+  if (valid && pa_ < NSHL && pb_ < NSHL) {
     const unsigned ma = ibc[s_node[w][pa_]], mb = ibc[s_node[w][pb_]];
     const bool diag = pa_ == pb_;
     if (do_mat) {
@@ -393,7 +383,7 @@ k_assemble_nsvof_hex(int E, const int32_t *__restrict__ ien, const int32_t *__re
         b8[t] = o;
       }
     }
-    if (do_vec && lane < 8) {
+    if (do_vec && lane < NSHL) {
       const unsigned m0 = ibc[na];
       double *rr = rhs + (size_t)na * 4;
 #pragma unroll
@@ -430,7 +420,8 @@ hipError_t launch_assemble_nsvof_hex(Ctx &c, int tensor_flag, int color) {
   P.lsc_kdc = q[OFD_Q_LSCKDC];
   P.fluid_id = (int)q[OFD_Q_FLUID_ID];
   const int blocks = (count + kWaves - 1) / kWaves;
-  hipLaunchKernelGGL(k_assemble_nsvof_hex, dim3(blocks), dim3(64 * kWaves), 0, c.stream,
+  hipLaunchKernelGGL((c.nshl == 4 ? k_assemble_nsvof_el<4> : k_assemble_nsvof_el<8>),
+                     dim3(blocks), dim3(64 * kWaves), 0, c.stream,
                      c.nelem, c.d_ien, c.d_elm_id, c.d_color_elems, first, count, c.d_emap,
                      c.d_first, c.d_noderec, c.d_ibc, P, (tensor_flag & OFD_ASSEMBLE_MAT) ? 1 : 0,
                      (tensor_flag & OFD_ASSEMBLE_VEC) ? 1 : 0, c.nnode, c.d_AV, c.d_rhsV,

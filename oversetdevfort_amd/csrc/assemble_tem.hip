@@ -20,16 +20,13 @@
 //            elements in flight share a node.
 #include "ofd_internal.h"
 #include "wave_dpp.h"
+#include "element_shape.h"
 
 namespace ofd {
 
 namespace {
 
 constexpr int kWaves = 4;
-// linhexshl corner signs (xi, eta, zeta) in the reference node order
-__constant__ int8_t c_sgn[8][3] = {{-1, 1, -1}, {1, 1, -1}, {1, 1, 1},  {-1, 1, 1},
-                                   {-1, -1, -1}, {1, -1, -1}, {1, -1, 1}, {-1, -1, 1}};
-
 __device__ __forceinline__ double sum8(double v) { return wsum8(v); }
 
 // Gauss-point value summed over the element's 8 nodes in node order, the
@@ -59,8 +56,9 @@ struct TemParams {
   int fluid_id;
 };
 
+template <int S>
 __global__ void __launch_bounds__(256)
-k_assemble_tem_hex(int E, const int32_t *__restrict__ ien, const int32_t *__restrict__ elm_id,
+k_assemble_tem_el(int E, const int32_t *__restrict__ ien, const int32_t *__restrict__ elm_id,
                    const int32_t *__restrict__ elems, int first, int count,
                    const int32_t *__restrict__ emap, const uint64_t *__restrict__ firstmask,
                    const double *__restrict__ rec,
@@ -82,7 +80,7 @@ k_assemble_tem_hex(int E, const int32_t *__restrict__ ien, const int32_t *__rest
   // ---------------- phase A: lane = (q, a) ----------------
   const int q = lane >> 3;
   const int a = lane & 7;
-  const int na = valid ? ien[(size_t)a * E + e] : 0;
+  const int na = valid ? ien[(size_t)(a < S ? a : 0) * E + e] : 0;
   if (lane < 8) s_node[w][lane] = na;
   // node record: x+d (0-2), u (3-5), ac (6-8), um (9-11), p, phi, T, rT (12-15)
   const double2 *r2 = reinterpret_cast<const double2 *>(rec + (size_t)na * 16);
@@ -93,18 +91,8 @@ k_assemble_tem_hex(int E, const int32_t *__restrict__ ien, const int32_t *__rest
   const double uma[3] = {v4.y, v5.x, v5.y};
   const double phia = v6.y, Ta = v7.x, rTa = v7.y;
 
-  const double gpt = 0.57735026918963;
-  const double xi = (q & 1) ? gpt : -gpt;
-  const double eta = (q & 2) ? gpt : -gpt;
-  const double zeta = (q & 4) ? gpt : -gpt;
-  const double sx = c_sgn[a][0], sy = c_sgn[a][1], sz = c_sgn[a][2];
-  const double fx = (sx > 0) ? (xi + 1.0) : (1.0 - xi);
-  const double fy = (sy > 0) ? (eta + 1.0) : (1.0 - eta);
-  const double fz = (sz > 0) ? (zeta + 1.0) : (1.0 - zeta);
-  const double Na = fx * fy * fz * 0.125;
-  const double dl0 = sx * (fy * fz * 0.125);
-  const double dl1 = sy * (fx * fz * 0.125);
-  const double dl2 = sz * (fx * fy * 0.125);
+  double Na, dl0, dl1, dl2, gwq;
+  elem_shape<S>(q, a, Na, dl0, dl1, dl2, gwq);
 
   double J[9];
 #pragma unroll
@@ -197,8 +185,8 @@ k_assemble_tem_hex(int E, const int32_t *__restrict__ ien, const int32_t *__rest
   const double dSedT = (P.cpw - P.cpa) * mdot + ((P.cpw - P.cpa) * (Ti - P.Ts) - P.lh) * dmdT;
   const double sc = ga[0] * uadv[0] + ga[1] * uadv[1] + ga[2] * uadv[2];
   const double hk = fluid ? hki + kdc : hki;
-  const double dw = detj;  // 8-point rule weights are 1
-  if (valid && a == 0 && detj < 0.0) atomicAdd(negdet, 1);
+  const double dw = detj * gwq;  // hex weights are 1; padded tet points 0
+  if (valid && a == 0 && (S == 8 || q < 4) && detj < 0.0) atomicAdd(negdet, 1);
 
   s_sh[w][q][a][0] = Na;
   s_sh[w][q][a][1] = ga[0];
@@ -240,7 +228,7 @@ k_assemble_tem_hex(int E, const int32_t *__restrict__ ien, const int32_t *__rest
   if (do_mat) {
     const double f1 = P.almi, f2 = P.alfi * P.gami * P.delt;
 #pragma unroll 2
-    for (int qq = 0; qq < 8; ++qq) {
+    for (int qq = 0; qq < (S == 8 ? 8 : 4); ++qq) {
       const double Nq = s_sh[w][qq][pa][0], g0 = s_sh[w][qq][pa][1],
                    g1 = s_sh[w][qq][pa][2], g2 = s_sh[w][qq][pa][3],
                    sca = s_sh[w][qq][pa][4];
@@ -261,7 +249,7 @@ k_assemble_tem_hex(int E, const int32_t *__restrict__ ien, const int32_t *__rest
   }
 
   // ---------------- phase C: BCLhs_tem + scatter ----------------
-  if (valid) {
+  if (valid && pa < S && pb < S) {
     const unsigned ca = (ibc[s_node[w][pa]] >> 5) & 1u;
     const unsigned cb = (ibc[s_node[w][pb]] >> 5) & 1u;
     if (do_mat) {
@@ -271,7 +259,7 @@ k_assemble_tem_hex(int E, const int32_t *__restrict__ ien, const int32_t *__rest
       // the first element (in colour order) to reach the nonzero stores
       A[k] = ((firstmask[e] >> lane) & 1ull) ? v : A[k] + v;
     }
-    if (do_vec && lane < 8) rhs[na] += ca ? 0.0 : rt;
+    if (do_vec && lane < S) rhs[na] += ca ? 0.0 : rt;
   }
 }
 
@@ -303,7 +291,8 @@ hipError_t launch_assemble_tem_hex(Ctx &c, int tensor_flag, int color) {
   P.tref = q[OFD_Q_TREF];
   P.fluid_id = (int)q[OFD_Q_FLUID_ID];
   const int blocks = (count + kWaves - 1) / kWaves;
-  hipLaunchKernelGGL(k_assemble_tem_hex, dim3(blocks), dim3(64 * kWaves), 0, c.stream,
+  hipLaunchKernelGGL((c.nshl == 4 ? k_assemble_tem_el<4> : k_assemble_tem_el<8>), dim3(blocks),
+                     dim3(64 * kWaves), 0, c.stream,
                      c.nelem, c.d_ien, c.d_elm_id, c.d_color_elems, first, count, c.d_emap,
                      c.d_first, c.d_noderec, c.ks_on ? c.d_field[OFD_KSCALE] : nullptr,
                      c.d_ibc, P,

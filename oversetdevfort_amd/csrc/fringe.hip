@@ -327,6 +327,8 @@ extern "C" int ofd_fringe_interp_noalloc(ofd_ctx *x, int32_t src_id, int nrecv,
   OFD_CHECK(c.nnode > 0 && c.d_ien, OFD_ERR_ARG, "no mesh uploaded");
   if (nrecv == 0) return OFD_OK;
   OFD_CHECK(xr && dr && donor && val, OFD_ERR_ARG, "null argument");
+  OFD_CHECK(c.nshl == 4 || c.nshl == 8, OFD_ERR_UNSUPPORTED,
+            "the donor search is implemented for linear tets and trilinear hexes");
   const size_t need = fringe_scratch_need(c.nshl, c.nelem, nrecv);
   OFD_CHECK(need <= kFringeScratchBytes, OFD_ERR_ARG,
             "fringe search needs " + std::to_string(need) + " bytes of scratch for " +

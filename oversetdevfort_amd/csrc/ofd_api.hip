@@ -518,8 +518,9 @@ int ofd_upload_mesh(ofd_ctx *x, int nnode, int nelem, int nshl, const int32_t *i
   OFD_CHECK(nnode > 0 && nelem > 0, OFD_ERR_ARG, "empty mesh");
   OFD_CHECK(nnode <= kMaxRedBlocks * kRedBlock, OFD_ERR_UNSUPPORTED,
             "more than 1,048,576 nodes per context");
-  OFD_CHECK(nshl == 8 || nshl == 4, OFD_ERR_UNSUPPORTED,
-            "elements must be trilinear hexes (NSHL=8) or linear tets (NSHL=4)");
+  OFD_CHECK(nshl == 8 || nshl == 4 || nshl == 6, OFD_ERR_UNSUPPORTED,
+            "elements must be trilinear hexes (NSHL=8), linear tets (NSHL=4) or linear "
+            "prisms (NSHL=6)");
   Ctx &c = x->c;
   OFD_HIP(hipSetDevice(c.device));
   const size_t ne = (size_t)nelem * nshl;
@@ -916,8 +917,8 @@ int ofd_assemble_ns(ofd_ctx *x, int tensor_flag, int subdomain_flag, double *cfl
   Ctx &c = x->c;
   int rc = need_pattern(c);
   if (rc) return rc;
-  OFD_CHECK(c.nshl == 8 || c.nshl == 4, OFD_ERR_UNSUPPORTED,
-            "the GPU NS assembly is implemented for trilinear hexes and linear tets");
+  OFD_CHECK(c.nshl == 8 || c.nshl == 4 || c.nshl == 6, OFD_ERR_UNSUPPORTED,
+            "the GPU NS assembly is implemented for trilinear hexes, linear tets and prisms");
   OFD_CHECK((tensor_flag & ~(OFD_ASSEMBLE_VEC | OFD_ASSEMBLE_MAT)) == 0, OFD_ERR_ARG,
             "tensor_flag must combine OFD_ASSEMBLE_VEC / OFD_ASSEMBLE_MAT");
   OFD_HIP(hipSetDevice(c.device));
@@ -1320,8 +1321,8 @@ int ofd_time_assemble(ofd_ctx *x, int tensor_flag, int reps, double *ms) {
   if (rc) return rc;
   rc = ensure_faces(c);
   if (rc) return rc;
-  OFD_CHECK(c.nshl == 8 || c.nshl == 4, OFD_ERR_UNSUPPORTED,
-            "the GPU NS assembly is implemented for trilinear hexes and linear tets");
+  OFD_CHECK(c.nshl == 8 || c.nshl == 4 || c.nshl == 6, OFD_ERR_UNSUPPORTED,
+            "the GPU NS assembly is implemented for trilinear hexes, linear tets and prisms");
   OFD_HIP(hipSetDevice(c.device));
   hipEvent_t e0, e1;
   OFD_HIP(hipEventCreate(&e0));
@@ -1369,8 +1370,8 @@ int ofd_time_quench(ofd_ctx *x, int what, int arg, int reps, double *ms) {
   OFD_HIP(hipSetDevice(c.device));
   if (!spmv) {
     OFD_CHECK(c.quench, OFD_ERR_ARG, "no quench parameters (ofd_set_quench_params)");
-    OFD_CHECK(c.nshl == 8, OFD_ERR_UNSUPPORTED,
-              "the GPU element assembly is implemented for trilinear hexes (NSHL=8)");
+    OFD_CHECK(c.nshl == 8 || c.nshl == 4, OFD_ERR_UNSUPPORTED,
+              "the GPU element assembly is implemented for trilinear hexes and linear tets");
     OFD_CHECK((arg & ~(OFD_ASSEMBLE_VEC | OFD_ASSEMBLE_MAT)) == 0, OFD_ERR_ARG,
               "arg must combine OFD_ASSEMBLE_VEC / OFD_ASSEMBLE_MAT");
   } else {
@@ -1486,8 +1487,8 @@ int ofd_assemble_tem(ofd_ctx *x, int tensor_flag) {
   int rc = ensure_tem(c);
   if (rc) return rc;
   OFD_CHECK(c.quench, OFD_ERR_ARG, "no quench parameters (ofd_set_quench_params)");
-  OFD_CHECK(c.nshl == 8, OFD_ERR_UNSUPPORTED,
-            "the GPU element assembly is implemented for trilinear hexes (NSHL=8)");
+  OFD_CHECK(c.nshl == 8 || c.nshl == 4, OFD_ERR_UNSUPPORTED,
+            "the GPU element assembly is implemented for trilinear hexes and linear tets");
   OFD_CHECK((tensor_flag & ~(OFD_ASSEMBLE_VEC | OFD_ASSEMBLE_MAT)) == 0, OFD_ERR_ARG,
             "tensor_flag must combine OFD_ASSEMBLE_VEC / OFD_ASSEMBLE_MAT");
   OFD_HIP(hipSetDevice(c.device));
@@ -1731,8 +1732,8 @@ int ofd_assemble_nsvof(ofd_ctx *x, int tensor_flag, double *cfl) {
   int rc = ensure_nsvof(c);
   if (rc) return rc;
   OFD_CHECK(c.quench, OFD_ERR_ARG, "no quench parameters (ofd_set_quench_params)");
-  OFD_CHECK(c.nshl == 8, OFD_ERR_UNSUPPORTED,
-            "the GPU element assembly is implemented for trilinear hexes (NSHL=8)");
+  OFD_CHECK(c.nshl == 8 || c.nshl == 4, OFD_ERR_UNSUPPORTED,
+            "the GPU element assembly is implemented for trilinear hexes and linear tets");
   OFD_CHECK((tensor_flag & ~(OFD_ASSEMBLE_VEC | OFD_ASSEMBLE_MAT)) == 0, OFD_ERR_ARG,
             "tensor_flag must combine OFD_ASSEMBLE_VEC / OFD_ASSEMBLE_MAT");
   OFD_HIP(hipSetDevice(c.device));
@@ -1969,6 +1970,8 @@ int ofd_fringe_interp(ofd_ctx *x, int32_t src_id, int nrecv, const double *xr,
   if (rc) return rc;
   if (nrecv == 0) return OFD_OK;
   OFD_CHECK(xr && dr && donor && val, OFD_ERR_ARG, "null argument");
+  OFD_CHECK(c.nshl == 4 || c.nshl == 8, OFD_ERR_UNSUPPORTED,
+            "the donor search is implemented for linear tets and trilinear hexes");
   OFD_HIP(hipSetDevice(c.device));
   rc = ensure_fringe(c, nrecv);
   if (rc) return rc;

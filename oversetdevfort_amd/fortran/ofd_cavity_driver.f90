@@ -3,7 +3,7 @@
 !
 ! Mirrors driver.f90:67-110 / solveOverset_stag (solveOverset.f90:4-126)
 ! for one block: predictor, Dirichlet values (setBCs_CFD), Newton loop of
-! assembleNavSto(VEC+MAT) -> linear solve -> update -> assembleNavSto(VEC)
+! assembleNavSto_resident(VEC+MAT) -> linear solve -> update -> assembleNavSto(VEC)
 ! -> residual norms, then old <- new.  Everything runs through the
 ! reference-named wrappers and ISO_C_BINDING on libofd.so.
 !
@@ -115,7 +115,7 @@ program ofd_cavity_driver
   ! ---- drop-in wrappers on the first assembled system ----
   call ofd_check(ofd_predictor(ofd_ctx), 'predictor')
   call ofd_check(ofd_apply_dirichlet(ofd_ctx), 'setBCs_CFD')
-  call assembleNavSto(OFD_ASSEMBLE_VEC + OFD_ASSEMBLE_MAT, 1)
+  call assembleNavSto_resident(OFD_ASSEMBLE_VEC + OFD_ASSEMBLE_MAT, 1)
   allocate (k11(9, icnt), g(3, icnt), d1(3, icnt), m(icnt), ru(NNODE, 3), rp(NNODE), &
             yu(NNODE, 3), yp(NNODE), su(NNODE, 3), sp(NNODE), iper(NNODE), &
             dflag(NNODE), pflag(NNODE))
@@ -139,10 +139,10 @@ program ofd_cavity_driver
     call ofd_check(ofd_predictor(ofd_ctx), 'predictor')
     call ofd_check(ofd_apply_dirichlet(ofd_ctx), 'setBCs_CFD')
     do inewt = 1, nnewton
-      call assembleNavSto(OFD_ASSEMBLE_VEC + OFD_ASSEMBLE_MAT, 1)
+      call assembleNavSto_resident(OFD_ASSEMBLE_VEC + OFD_ASSEMBLE_MAT, 1)
       call solveNS_resident(1.0d-10, 30, iters, relres)
       call ofd_check(ofd_newton_update(ofd_ctx), 'update')
-      call assembleNavSto(OFD_ASSEMBLE_VEC, 1)
+      call assembleNavSto_resident(OFD_ASSEMBLE_VEC, 1)
       call ofd_check(ofd_rhs_norms(ofd_ctx, res), 'residual')
       write (*, '(a,2i4,2es25.16,i6,es12.4)') 'NEWTON ', istep, inewt, res(1), res(2), &
         iters, relres
@@ -158,7 +158,7 @@ program ofd_cavity_driver
   ! ---- adjoint on the final state (no reference counterpart) ----
   allocate (lu(NNODE, 3), lp(NNODE), gu(NNODE, 3), gac(NNODE, 3), gp(NNODE), gm(NNODE))
   call ofd_check(ofd_set_residual_form(ofd_ctx, 1), 'residual form')
-  call assembleNavSto(OFD_ASSEMBLE_VEC + OFD_ASSEMBLE_MAT, 1)
+  call assembleNavSto_resident(OFD_ASSEMBLE_VEC + OFD_ASSEMBLE_MAT, 1)
   su = ug
   where (IBC(:, 1:3) == 1) su = 0.0d0
   sp = 0.0d0

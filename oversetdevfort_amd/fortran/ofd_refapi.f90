@@ -11,9 +11,9 @@
 !   * SparseProd_LS, SparseGMRES_tem, SparseProd_NS_conv and
 !     SparseGMRES_ns_conv do the same for the quenching solver's temperature
 !     and coupled NS + VOF systems;
-!   * assembleNavSto(flag, sub) (assembleNavSto.f90:4) assembles from the
-!     state held on the device; solveNS_resident solves that system without
-!     leaving the device -- the fast path a GPU driver uses.
+!   * assembleNavSto_resident(flag, sub) (assembleNavSto.f90:4) assembles
+!     from the state held on the device; solveNS_resident solves that system
+!     without leaving the device -- the fast path a GPU driver uses.
 !======================================================================
 module ofd_state
   use ofd_iface
@@ -198,16 +198,18 @@ subroutine SparseGMRES_ns_conv(col, row, IBC, IPER, D_FLAG, P_FLAG, rhsGu, rhsGp
 end subroutine SparseGMRES_ns_conv
 
 !----------------------------------------------------------------------
-! assembleNavSto.f90:4-72 on the device-resident state
+! assembleNavSto.f90:4-72 on the device-resident state: the system stays on
+! the device for solveNS_resident (the module-state drop-in with the
+! reference's name, which fills LHSK11 ... RHSGp, is in ofd_refmod.f90)
 !----------------------------------------------------------------------
-subroutine assembleNavSto(assemble_tensor_flag, subdomain_flag)
+subroutine assembleNavSto_resident(assemble_tensor_flag, subdomain_flag)
   use ofd_state
   implicit none
   integer, intent(in) :: assemble_tensor_flag, subdomain_flag
   real(c_double) :: cfl
   call ofd_check(ofd_assemble_ns(ofd_ctx, int(assemble_tensor_flag, c_int), &
                                  int(subdomain_flag, c_int), cfl), 'assembleNavSto')
-end subroutine assembleNavSto
+end subroutine assembleNavSto_resident
 
 !----------------------------------------------------------------------
 ! device-resident linear solve of the assembled system to rtol (restarted

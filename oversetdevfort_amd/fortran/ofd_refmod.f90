@@ -13,10 +13,22 @@
 !   * interpolation_tet(mesh_id) (interpolation.f90:5-76): fills
 !     bmesh(mesh_id)%ug from the device donor search over the elements with
 !     ELM_ID == SRC_ID_SET(mesh_id).  The reference's setBCs_overset
-!     (setBCs.f90:124-167) calls it unchanged.
+!     (setBCs.f90:124-167) calls it unchanged, so the overset boundary
+!     condition reaches the device with no drop-in of its own.
+!   * genSparStruc() (genSparStruc.f90:4-47): uploads the module mesh (IEN,
+!     ELM_ID, xg) and builds the pattern on the device, then allocates and
+!     fills the module's col, row and icnt.
+!   * assembleNavSto(assemble_tensor_flag, subdomain_flag)
+!     (assembleNavSto.f90:4-72): copies the module node fields, IBC,
+!     ELM_ID_SET and the parameters to the device, assembles there
+!     (ofd_assemble_ns) and fills the module's RHSGu/RHSGp and
+!     LHSK11/LHSG/LHSD1/LHSM the flags name, zeroed first as the reference
+!     does.  The weak faces are the ones registered with ofd_set_weak_faces.
 !
-! The mesh and its pattern are uploaded once by the driver (ofd_upload_mesh
-! with IEN and ELM_ID, ofd_build_pattern), as for the other drop-ins.
+! With these, solveOverset_stag (solveOverset.f90:4-121) runs unchanged over
+! the module state: genSparStruc once, then per Newton iteration
+! setBCs_overset, assembleNavSto, SparseGMRES_up (ofd_refapi.f90, on the
+! module arrays) and the update.
 !======================================================================
 module ofd_refmod
   use ofd_state
@@ -58,6 +70,31 @@ contains
     call push(OFD_RTG, rTg)
     call push(OFD_RTGOLD, rTgold)
   end subroutine ofd_push_quench_state
+
+  ! module node fields, IBC, ELM_ID_SET and the NS parameters of
+  ! IntElmAss_NS_overset (the air-phase rhoa, mua, NS_kdc_a of
+  ! assembleNavSto.f90:252-254, gravvec) -> the device context
+  subroutine ofd_push_ns_state()
+    real(8) :: prm(OFD_NPARAMS)
+    prm = (/almi, alfi, gami, beti, Delt, rhoa, mua, NS_kdc_a, gravvec(1), gravvec(2), &
+            gravvec(3)/)
+    call ofd_check(ofd_set_params(ofd_ctx, prm), 'ofd_push_ns_state')
+    call ofd_check(ofd_set_subdomains(ofd_ctx, int(ELM_ID_SET(1), c_int32_t), &
+                                      int(ELM_ID_SET(2), c_int32_t)), 'ofd_push_ns_state')
+    call ofd_check(ofd_set_ibc(ofd_ctx, IBC, int(min(size(IBC, 2), 8), c_int)), &
+                   'ofd_push_ns_state')
+    call push(OFD_DG, dg)
+    call push(OFD_DGOLD, dgold)
+    call push(OFD_UG, ug)
+    call push(OFD_UGOLD, ugold)
+    call push(OFD_UGM, ugm)
+    call push(OFD_UGMOLD, ugmold)
+    call push(OFD_ACG, acg)
+    call push(OFD_ACGOLD, acgold)
+    call push(OFD_ACGM, acgm)
+    call push(OFD_ACGMOLD, acgmold)
+    call push(OFD_PG, pg)
+  end subroutine ofd_push_ns_state
 
   subroutine push(field, a)
     integer(c_int), intent(in) :: field
@@ -134,3 +171,40 @@ subroutine interpolation_tet(mesh_id)
                                    donor, bmesh(mesh_id)%ug), 'interpolation_tet')
   deallocate (donor)
 end subroutine interpolation_tet
+
+!----------------------------------------------------------------------
+! genSparStruc.f90:4-47 (with Asadj :53-105) on the device: col(NNODE+1),
+! row(icnt) and icnt of the module, 1-based, the reference's order
+!----------------------------------------------------------------------
+subroutine genSparStruc()
+  use ofd_refmod
+  implicit none
+  integer(c_int64_t) :: n
+  call ofd_check(ofd_upload_mesh(ofd_ctx, int(NNODE, c_int), int(NELEM, c_int), &
+                                 int(maxNSHL, c_int), IEN, ELM_ID, xg), 'genSparStruc')
+  call ofd_check(ofd_build_pattern(ofd_ctx, n), 'genSparStruc')
+  icnt = int(n)
+  if (allocated(col)) deallocate (col)
+  if (allocated(row)) deallocate (row)
+  allocate (col(NNODE + 1), row(icnt))
+  call ofd_check(ofd_get_pattern(ofd_ctx, col, row), 'genSparStruc')
+end subroutine genSparStruc
+
+!----------------------------------------------------------------------
+! assembleNavSto.f90:4-72 over the module state: alpha stage, zeroing,
+! IntElmAss_NS_overset, the weak and outflow faces, on the device; the
+! assembled RHSGu/RHSGp and LHSK11/LHSG/LHSD1/LHSM come back into the module
+!----------------------------------------------------------------------
+subroutine assembleNavSto(assemble_tensor_flag, subdomain_flag)
+  use ofd_refmod
+  implicit none
+  integer, intent(in) :: assemble_tensor_flag, subdomain_flag
+  real(c_double) :: cfl
+  call ofd_push_ns_state()
+  call ofd_check(ofd_assemble_ns(ofd_ctx, int(assemble_tensor_flag, c_int), &
+                                 int(subdomain_flag, c_int), cfl), 'assembleNavSto')
+  if (iand(assemble_tensor_flag, ASSEMBLE_TENSOR_VEC) > 0) &
+    call ofd_check(ofd_get_rhs(ofd_ctx, RHSGu, RHSGp), 'assembleNavSto')
+  if (iand(assemble_tensor_flag, ASSEMBLE_TENSOR_MAT) > 0) &
+    call ofd_check(ofd_get_lhs(ofd_ctx, LHSK11, LHSG, LHSD1, LHSM), 'assembleNavSto')
+end subroutine assembleNavSto

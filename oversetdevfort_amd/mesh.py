@@ -128,6 +128,34 @@ def hex_to_tets(mesh: Mesh) -> Mesh:
                 dims=mesh.dims)
 
 
+def hex_to_prisms(mesh: Mesh) -> Mesh:
+    """Split each hex into 2 linear prisms (wedges) along the diagonal of its
+    z faces.  Prism node order follows ``linprishl`` (eval_shape_FEM.f90:
+    438-477): nodes 1-3 the bottom triangle (zeta = -1) with node 1 at xi = 1,
+    node 2 at eta = 1, node 3 at the origin of (xi, eta); nodes 4-6 the same
+    corners on the top.  Each prism is made positively oriented."""
+    slot = {tuple(c): a for a, c in enumerate(HEX_CORNERS.tolist())}
+    tris = [((1, 0), (0, 1), (0, 0)), ((0, 1), (1, 0), (1, 1))]
+    out = []
+    for t in tris:
+        idx = [slot[(i, j, 0)] for i, j in t] + [slot[(i, j, 1)] for i, j in t]
+        out.append(mesh.ien[:, idx])
+    ien = np.stack(out, axis=1).reshape(-1, 6).astype(np.int32)
+    # orientation: dx/dxi of linprishl at the centroid; flip nodes 1 <-> 2 and
+    # 4 <-> 5 where its determinant is negative
+    x = mesh.xg
+    c = x[ien]
+    e1 = 0.5 * ((c[:, 0] - c[:, 2]) + (c[:, 3] - c[:, 5]))
+    e2 = 0.5 * ((c[:, 1] - c[:, 2]) + (c[:, 4] - c[:, 5]))
+    e3 = (c[:, 3:].mean(axis=1) - c[:, :3].mean(axis=1))
+    d = np.linalg.det(np.stack([e1, e2, e3], axis=2))
+    neg = d < 0
+    ien[neg, 0], ien[neg, 1] = ien[neg, 1].copy(), ien[neg, 0].copy()
+    ien[neg, 3], ien[neg, 4] = ien[neg, 4].copy(), ien[neg, 3].copy()
+    elm = np.repeat(mesh.elm_id, 2)
+    return Mesh(xg=mesh.xg.copy(), ien=ien, elm_id=elm.astype(np.int32), dims=mesh.dims)
+
+
 # Local nodes of each hex face, FACE_OR 1..6 = xi -1, xi +1, eta -1, eta +1,
 # zeta -1, zeta +1 (the reference has no hex face map; this numbering is the
 # one of the face kernels and the oracle).

@@ -1,5 +1,5 @@
 """The drop-ins that use the reference's module state (ofd_refmod.f90:
-assembleQuenching, interpolation_tet) compile against the reference's own
+assembleQuenching, interpolation_tet, genSparStruc, assembleNavSto) compile against the reference's own
 modules (aAdjKeep, commonvars, commonpars, as built under oracle/_ref) and
 export the link names the reference's callers resolve to.  Compile-only:
 running them needs the reference's program and a GPU."""
@@ -26,9 +26,11 @@ def test_refmod_compiles_against_reference_modules(tmp_path):
                        + extra, check=True, cwd=tmp_path)
     syms = subprocess.run(["nm", str(tmp_path / "ofd_refmod.o")], check=True,
                           capture_output=True, text=True).stdout.split()
-    ref = subprocess.run(["nm", os.path.join(REFMOD, "solveQuenching.o"),
-                          os.path.join(REFMOD, "interpolation.o")], check=True,
+    ref = subprocess.run(["nm"] + [os.path.join(REFMOD, f) for f in
+                                   ("solveQuenching.o", "interpolation.o", "genSparStruc.o",
+                                    "assembleNavSto.o")], check=True,
                          capture_output=True, text=True).stdout.split()
-    for name in ("assemblequenching_", "interpolation_tet_"):
+    for name in ("assemblequenching_", "interpolation_tet_", "gensparstruc_",
+                 "assemblenavsto_"):
         assert name in syms, name
         assert name in ref, name

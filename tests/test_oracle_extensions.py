@@ -91,3 +91,39 @@ def test_scales_are_local(oracle_mod):
     assert not np.array_equal(A["rhsu"][near], B["rhsu"][near])
     rows = np.repeat(np.arange(N), np.diff(col))
     assert np.array_equal(A["k11"][far[rows]], B["k11"][far[rows]])
+
+
+def test_colour_ordered_assembly(oracle_mod):
+    """The colour-by-colour assembly of the host baseline: with one element
+    per colour in index order it is the element-order assembly bit for bit;
+    with the 8-colouring of the hex grid only the summation order per entry
+    changes, so it agrees to rounding (1e-14 normwise) with the same
+    Dirichlet rows, CFL and negative-Jacobian count."""
+    from oversetdevfort_amd import mesh as M
+    O = oracle_mod
+    prob = C.cavity(6, seed=4, jitter=0.05)
+    m = prob.mesh
+    N = m.nnode
+    col, row = O.sparsity(N, m.ien)
+    rng = np.random.default_rng(4)
+    st = dict(dg=0.01 * rng.normal(size=(N, 3)), ug=0.3 * rng.normal(size=(N, 3)),
+              ugm=np.zeros((N, 3)), acg=rng.normal(size=(N, 3)), acgm=np.zeros((N, 3)),
+              pg=rng.normal(size=N))
+    prm = O.params_array(prob.params)
+    ref = O.assemble_ns(m, prob.ibc, st, prm, col, row, 6, 1)
+    keys = ("k11", "g", "d1", "m", "rhsu", "rhsp")
+    one = O.assemble_ns_col(m, prob.ibc, st, prm, col, row, np.arange(m.nelem), 6, 1)
+    for k in keys:
+        assert np.array_equal(one[k], ref[k]), k
+    assert one["cfl"] == ref["cfl"] and one["nneg"] == ref["nneg"]
+    colors = M.hex_colors(m)
+    assert len(np.unique(colors)) == 8
+    col8 = O.assemble_ns_col(m, prob.ibc, st, prm, col, row, colors, 6, 1)
+    for k in keys:
+        a, b = col8[k].ravel(), ref[k].ravel()
+        assert np.linalg.norm(a - b) <= 1e-14 * np.linalg.norm(b), k
+    assert not all(np.array_equal(col8[k], ref[k]) for k in keys)
+    assert col8["cfl"] == ref["cfl"] and col8["nneg"] == ref["nneg"]
+    vec = O.assemble_ns_col(m, prob.ibc, st, prm, col, row, colors, 2, 1)
+    assert np.array_equal(vec["rhsu"], col8["rhsu"])
+    assert not vec["k11"].any()
