@@ -218,7 +218,7 @@ __global__ void __launch_bounds__(256)
 k_mgs_step(int N, double *__restrict__ W, const double *__restrict__ rd,
            double *__restrict__ wr, GmresState *__restrict__ g, int k, int j, int nb,
            int32_t *hflag) {
-  if (g->done) return;
+  if (cycle_stopped(g, k, j == 0 ? hflag : nullptr)) return;
   __shared__ double s_b[1];
   const size_t vs = (size_t)N * 4;
   double *z = W + vs * (k + 1);
@@ -268,7 +268,7 @@ __global__ void __launch_bounds__(256)
 k_cgs_dots(int N, const V *__restrict__ W, const double *__restrict__ rd,
            double *__restrict__ wr, GmresState *__restrict__ g, int k, int nb,
            int32_t *hflag) {
-  if (g->done) return;
+  if (cycle_stopped(g, k, hflag)) return;
   const size_t vs = (size_t)N * 4;
   const V *z = W + vs * (k + 1);
   const int m = k + 1;
@@ -380,9 +380,11 @@ k_norm2(int N, const double *__restrict__ v, double *__restrict__ part) {
   block_partials<1>(acc, 1, part, 0);
 }
 
+// rec (may be null): the cycle's record for the host, `seq` stored last
+// with a system-scope release so the fields are visible before it
 __global__ void __launch_bounds__(256)
 k_norm_fin(const double *__restrict__ part, int nb, GmresState *__restrict__ g, int slot,
-           int outer) {
+           int outer, CycleRecord *rec, int32_t seq) {
   __shared__ double s_sum[1];
   final_sums(part, nb, 1, s_sum);
   if (threadIdx.x == 0) {
@@ -391,6 +393,13 @@ k_norm_fin(const double *__restrict__ part, int nb, GmresState *__restrict__ g, 
     // outer residual of a restarted solve: at or below the threshold (or
     // not finite) stops the cycles queued behind this one
     if (outer) g->outer_done = (rn > g->outer_thr) ? 0 : 1;
+    if (rec) {
+      rec->resid = g->resid;
+      rec->rnorm0 = g->rnorm0;
+      rec->rnorm = rn;
+      rec->iters = g->iters;
+      __hip_atomic_store(&rec->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
   }
 }
 
@@ -628,14 +637,15 @@ hipError_t launch_outer_reset(Ctx &c, double thr) {
   return hipGetLastError();
 }
 
-hipError_t launch_norm2(Ctx &c, const double *v, int slot, int outer) {
+hipError_t launch_norm2(Ctx &c, const double *v, int slot, int outer, CycleRecord *rec,
+                        int32_t seq) {
   const int nb = nblocks_gm(c);
   double *pb = part_buf(c, 0);
   hipLaunchKernelGGL(k_norm2, dim3(nb), dim3(kRedBlock), 0, c.stream, c.gm_n, v, pb);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_norm_fin, dim3(1), dim3(kRedBlock), 0, c.stream, pb, nb, c.d_gs,
-                     slot, outer);
+                     slot, outer, rec, seq);
   return hipGetLastError();
 }
 

@@ -107,6 +107,18 @@ __device__ __forceinline__ double sum1(const double *part, int nb) {
   return r;
 }
 
+// Entry test of a step kernel: true if the cycle has already stopped (an
+// earlier close converged, or the cycle was queued behind one that met the
+// outer tolerance).  The host waits on the flag of step k - 1 from the
+// kernel that closes it, so block 0 of a kernel that returns here reports
+// the stop in that flag: every flag the host waits for gets written.
+__device__ __forceinline__ bool cycle_stopped(const GmresState *__restrict__ g, int k,
+                                              int32_t *hflag) {
+  if (!g->done) return false;
+  if (hflag && k >= 1 && blockIdx.x == 0 && threadIdx.x == 0) hflag[k - 1] = 1;
+  return true;
+}
+
 // Close Arnoldi step k (SparseGMRES_NS.f90:289-327).  The product of step k
 // is stored unscaled, z = (P^-1 A) W_k = (P^-1 A v_k) / s_k, so every
 // coefficient found for z is multiplied by s_k here: H(j,k) = s_k (h1_j + h2_j)

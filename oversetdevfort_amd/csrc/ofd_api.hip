@@ -5,6 +5,7 @@
 // SparseMatLoc_3D (FillSparseMat.f90:4-28), and the GMRES / Newton drivers
 // that sequence the kernels.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -111,8 +112,9 @@ static int ensure_krylov(Ctx &c, int K) {
     // coherent: the device's flag stores reach the host while kernels run
     OFD_HIP(hipHostMalloc(reinterpret_cast<void **>(&c.h_flags),
                           sizeof(int32_t) * 2 * kMaxKspace, hipHostMallocCoherent));
-    OFD_HIP(hipHostMalloc(reinterpret_cast<void **>(&c.h_gs_ring), sizeof(GmresState) * 2));
-    for (auto &ev : c.cyc_ev) OFD_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    OFD_HIP(hipHostMalloc(reinterpret_cast<void **>(&c.h_rec), sizeof(CycleRecord) * 2,
+                          hipHostMallocCoherent));
+    std::memset(c.h_rec, 0, sizeof(CycleRecord) * 2);
   }
   // vectors of the largest system seen so far (NS, T or NSVOF tuples)
   const int nt = c.gm_n > c.kalloc_n ? c.gm_n : c.kalloc_n;
@@ -186,9 +188,13 @@ static int probe_harvest(Ctx &c) {
 }
 
 // Wait until the device has written the convergence flag of Arnoldi step
-// `idx` (-1 = not yet).  Kernels launched after convergence return at once
-// without writing flags, so a stream that has gone idle with the flag still
-// unwritten means the cycle stopped earlier: reported as converged.
+// `idx` (-1 = not yet).  The kernel that closes step idx writes it, also
+// when it returns at once behind a stopped cycle.  The stream is queried
+// only every 2^20 spins (tens of ms), as a guard: a query enqueues a marker
+// whose cache flush stalls the queue for several microseconds, and polling
+// it every few hundred spins put such a bubble into about half of the
+// Arnoldi steps.  An idle stream with the flag unwritten is reported as
+// converged.
 static int wait_flag(Ctx &c, int idx, bool *done) {
   volatile int32_t *f = c.cur_flags;
   for (unsigned spin = 1;; ++spin) {
@@ -197,13 +203,42 @@ static int wait_flag(Ctx &c, int idx, bool *done) {
       *done = v != 0;
       return OFD_OK;
     }
-    if ((spin & 255u) == 0u) {
+    if ((spin & ((1u << 20) - 1u)) == 0u) {
       const hipError_t e = hipStreamQuery(c.stream);
       if (e == hipSuccess) {
         *done = f[idx] != 0;
         return OFD_OK;
       }
       if (e != hipErrorNotReady) OFD_HIP(e);
+    }
+    __builtin_ia32_pause();
+  }
+}
+
+// Wait for the record of the cycle numbered `seq` in pinned slot `slot`
+// (written by the outer residual's norm kernel) and copy it out.  No event
+// and no copy on the stream: a device-to-host copy between two cycles held
+// the queue for about 20 us.  The stream is queried only as a guard
+// against a record that can no longer come.
+static int wait_record(Ctx &c, int slot, int32_t seq, CycleRecord *out) {
+  volatile CycleRecord *r = c.h_rec + slot;
+  for (unsigned spin = 1;; ++spin) {
+    if (r->seq == seq) {
+      std::atomic_thread_fence(std::memory_order_acquire);
+      out->resid = r->resid;
+      out->rnorm0 = r->rnorm0;
+      out->rnorm = r->rnorm;
+      out->iters = r->iters;
+      out->seq = seq;
+      return OFD_OK;
+    }
+    if ((spin & ((1u << 20) - 1u)) == 0u) {
+      const hipError_t e = hipStreamQuery(c.stream);
+      if (e == hipSuccess && r->seq != seq) {
+        set_error("solve: cycle record not written");
+        return OFD_ERR_HIP;
+      }
+      if (e != hipSuccess && e != hipErrorNotReady) OFD_HIP(e);
     }
     __builtin_ia32_pause();
   }
@@ -502,9 +537,7 @@ int ofd_ctx_destroy(ofd_ctx *x) {
   dfree(c.d_fr_nodes);
   if (c.h_gs) (void)hipHostFree(c.h_gs);
   if (c.h_flags) (void)hipHostFree(c.h_flags);
-  if (c.h_gs_ring) (void)hipHostFree(c.h_gs_ring);
-  for (auto &ev : c.cyc_ev)
-    if (ev) (void)hipEventDestroy(ev);
+  if (c.h_rec) (void)hipHostFree(c.h_rec);
   for (auto &ev : c.probe_ev) (void)hipEventDestroy(ev);
   if (c.h_pin) (void)hipHostFree(c.h_pin);
   if (c.stream) (void)hipStreamDestroy(c.stream);
@@ -1116,6 +1149,7 @@ static int solve_core(Ctx &c, int precision, int ortho, const double *b, double 
   // as they are).  Each cycle's state is copied to its own pinned slot.
   OFD_HIP(launch_outer_reset(c, rtol * bnorm));
   int done_cycles = 0;
+  int32_t seq[2] = {0, 0};
   for (cyc = 1; cyc <= max_cycles + 1; ++cyc) {
     if (cyc <= max_cycles) {
       rc = gmres_cycle(c, precision, ortho, rhs, inner_tol, kspace, 0, x, nullptr, nullptr,
@@ -1128,16 +1162,15 @@ static int solve_core(Ctx &c, int precision, int ortho, const double *b, double 
         OFD_HIP(launch_spmv_v(c, OFD_FP64, x, r, b, 1));
       else
         OFD_HIP(launch_spmv(c, OFD_FP64, x, r, b, 1));
-      OFD_HIP(launch_norm2(c, r, 0, 1));
-      OFD_HIP(hipMemcpyAsync(&c.h_gs_ring[cyc & 1], c.d_gs, sizeof(GmresState),
-                             hipMemcpyDeviceToHost, c.stream));
-      OFD_HIP(hipEventRecord(c.cyc_ev[cyc & 1], c.stream));
+      seq[cyc & 1] = ++c.rec_seq;
+      OFD_HIP(launch_norm2(c, r, 0, 1, &c.h_rec[cyc & 1], seq[cyc & 1]));
       rhs = r;
     }
     if (cyc >= 2) {
       const int p = cyc - 1;  // the cycle whose result is read now
-      OFD_HIP(hipEventSynchronize(c.cyc_ev[p & 1]));
-      const GmresState &st = c.h_gs_ring[p & 1];
+      CycleRecord st;
+      rc = wait_record(c, p & 1, seq[p & 1], &st);
+      if (rc) break;
       done_cycles = p;
       total += st.iters;
       if (!std::isfinite(st.resid) || !std::isfinite(st.rnorm0)) {
@@ -1145,7 +1178,7 @@ static int solve_core(Ctx &c, int precision, int ortho, const double *b, double 
         set_error("GMRES: non-finite residual");
         break;
       }
-      rel = st.scal[0] / bnorm;
+      rel = st.rnorm / bnorm;
       c.solve_hist.push_back(rel);
       if (!std::isfinite(rel)) {
         rc = OFD_ERR_NONFINITE;

@@ -71,6 +71,17 @@ struct GmresState {
   int32_t pad;
 };
 
+// What the host reads after each cycle of a restarted solve, written by the
+// outer residual's norm kernel into coherent pinned memory: the fields
+// first, then `seq` (the cycle's number) with a system-scope release.
+struct CycleRecord {
+  double resid;   // GmresState::resid
+  double rnorm0;  // GmresState::rnorm0
+  double rnorm;   // |b - A x| after the cycle
+  int32_t iters;  // GmresState::iters
+  int32_t seq;
+};
+
 struct Ctx {
   int device = 0;
   int n_cu = 256;                     // compute units of the device
@@ -132,8 +143,8 @@ struct Ctx {
   int32_t *h_flags = nullptr;         // coherent pinned: 2 x [kMaxKspace] GmresState::done
                                       // per step (-1 unset), one set per cycle parity
   int32_t *cur_flags = nullptr;       // the set of the cycle being enqueued
-  GmresState *h_gs_ring = nullptr;    // pinned: state snapshot per cycle parity
-  hipEvent_t cyc_ev[2] = {nullptr, nullptr};  // after each snapshot copy
+  CycleRecord *h_rec = nullptr;       // coherent pinned: one record per cycle parity
+  int32_t rec_seq = 0;                // number of the last cycle record asked for
   double *h_pin = nullptr;            // pinned scratch (host transfers)
   size_t h_pin_bytes = 0;
   int64_t dev_bytes = 0;
@@ -308,7 +319,8 @@ hipError_t launch_gmres_init(Ctx &c, const double *b, double tol, int kspace, in
 hipError_t launch_mgs_step(Ctx &c, int k, int j);
 hipError_t launch_gmres_close(Ctx &c, int k, int ortho);
 hipError_t launch_gmres_finish(Ctx &c, double *x);
-hipError_t launch_norm2(Ctx &c, const double *v, int slot, int outer = 0);
+hipError_t launch_norm2(Ctx &c, const double *v, int slot, int outer = 0,
+                        CycleRecord *rec = nullptr, int32_t seq = 0);
 hipError_t launch_outer_reset(Ctx &c, double thr);
 hipError_t launch_norm2_up(Ctx &c, const double *v, int slot);
 int nblocks_gm(const Ctx &c);
