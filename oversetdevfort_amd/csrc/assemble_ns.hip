@@ -444,6 +444,485 @@ k_assemble_ns_el(int N, int E, const int32_t *__restrict__ ien,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Trilinear hexes in three kernels.
+//
+// The kernel above gives each element a whole wavefront with lane = (q, a):
+// everything that belongs to a Gauss point alone (J^-1, G, the tau's, the
+// RHS terms before the node split) runs in 8 copies, and every Gauss-point
+// interpolation is an 8-lane reduction.  For hexes the volume loop is split:
+//
+//   k_ns_points_hex  one launch over all elements, eight per wavefront,
+//                    lane = (element, q): the Gauss-point work runs once per
+//                    lane and the sums over the 8 nodes are register loops,
+//                    added in the association ((p0+p1)+(p2+p3))+((p4+p5)+
+//                    (p6+p7)) with every product rounded first.  It writes
+//                    each element's point data at its position in colour
+//                    order: J^-1 and the advective velocity, the point
+//                    factors, the RHS contributions per node (summed over q
+//                    in the same association, e3Rhs_3D_fluid), the node
+//                    numbers; and its CFL number.
+//   k_ns_pairs_hex   one launch per colour, one element per wavefront: lane =
+//                    (q, a) forms dN_a/dx and tmp1_a from the point data (the
+//                    same expressions, so the same bits), then lane = (a, b)
+//                    the 16 entries of block (a, b) over the 8 points
+//                    (e3LHS_3D_fluid_Old), the Dirichlet masks and the
+//                    coalesced first-touch scatter of the kernel above.
+//   k_ns_rhs_gather  one launch, thread per node: the node's contributions in
+//                    colour order (the order of the coloured scatter, so the
+//                    same bits), masked by its Dirichlet flags.
+//
+// Per element the point data take 1536 bytes (kPtsStride doubles).
+// ---------------------------------------------------------------------------
+namespace {
+
+// the sum of eight products in the lane-butterfly association; the adds
+// carry no contraction, so no product is fused into them
+__device__ __forceinline__ double tree8(const double (&p)[8]) {
+#pragma clang fp contract(off)
+  return ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7]));
+}
+
+// linhexshl value of node a at Gauss point q (elem_shape<8>'s Na)
+__device__ __forceinline__ double hex_N(int q, int a) {
+  double Na, d0, d1, d2, w;
+  elem_shape<8>(q, a, Na, d0, d1, d2, w);
+  return Na;
+}
+
+constexpr int kRecStride = 14;  // doubles per staged node record
+
+}  // namespace
+
+__global__ void __launch_bounds__(64)
+k_ns_points_hex(int E, const int32_t *__restrict__ ien, const int32_t *__restrict__ elm_id,
+                const int32_t *__restrict__ cpos, const double *__restrict__ rec,
+                const double *__restrict__ mus, AsmParams P, int id1, int id2,
+                int subdomain_flag, int do_mat, int do_vec, double *__restrict__ pts,
+                double *__restrict__ cfl_out, int32_t *__restrict__ negdet) {
+  // the 8 node records of each element: [el][a][14] (x + d, u, ac, um, p,
+  // viscosity scale)
+  __shared__ __align__(16) double s_rec[8 * 8 * kRecStride];
+  const int lane = threadIdx.x;
+  const int el = lane >> 3;
+  const int q = lane & 7;  // Gauss point, and node a = q for the loads
+  const int e = blockIdx.x * 8 + el;
+  const bool valid = e < E;
+  bool tgt = false;
+  if (valid) {
+    const int id = elm_id[e];
+    if (subdomain_flag & 1) tgt = tgt || id == id1;
+    if (subdomain_flag & 2) tgt = tgt || id == id2;
+  }
+  const bool vec = do_vec && tgt;
+  // this element's point data go to its position in colour order
+  double *po = pts + (size_t)(valid ? cpos[e] : 0) * kPtsStride;
+  {
+    const int n = valid ? ien[(size_t)q * E + e] : 0;
+    const double2 *r2 = reinterpret_cast<const double2 *>(rec + (size_t)n * 16);
+    double2 *sr = reinterpret_cast<double2 *>(s_rec + (el * 8 + q) * kRecStride);
+#pragma unroll
+    for (int t = 0; t < 6; ++t) sr[t] = r2[t];
+    sr[6] = make_double2(r2[6].x, mus ? mus[n] : 1.0);
+    if (valid && do_mat) {
+      int32_t *pn = reinterpret_cast<int32_t *>(po + kPtsN);
+      pn[q] = n;
+      if (q == 0) pn[8] = tgt ? 1 : 0;
+    }
+  }
+  __syncthreads();
+  const double *re = s_rec + el * 8 * kRecStride;
+
+  double Nq[8];
+#pragma unroll
+  for (int a = 0; a < 8; ++a) Nq[a] = hex_N(q, a);
+
+  // dx/dxi (i,j) = sum_a (x+d)_a,i dN_a/dxi_j
+  double J[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    double p0[8], p1[8], p2[8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      double Na, d0, d1, d2, gw;
+      elem_shape<8>(q, a, Na, d0, d1, d2, gw);
+      const double x = re[a * kRecStride + i];
+      p0[a] = x * d0;
+      p1[a] = x * d1;
+      p2[a] = x * d2;
+    }
+    J[i + 0] = tree8(p0);
+    J[i + 3] = tree8(p1);
+    J[i + 6] = tree8(p2);
+  }
+  // get_inverse_3x3 (eval_shape.f90:85-112); column-major Ji(i + 3 j)
+  double Ji[9];
+  Ji[0] = J[4] * J[8] - J[5] * J[7];
+  Ji[3] = J[5] * J[6] - J[3] * J[8];
+  Ji[6] = J[3] * J[7] - J[6] * J[4];
+  const double tmp = 1.0 / (Ji[0] * J[0] + Ji[3] * J[1] + Ji[6] * J[2]);
+  Ji[0] *= tmp;
+  Ji[3] *= tmp;
+  Ji[6] *= tmp;
+  Ji[1] = (J[7] * J[2] - J[1] * J[8]) * tmp;
+  Ji[4] = (J[0] * J[8] - J[2] * J[6]) * tmp;
+  Ji[7] = (J[1] * J[6] - J[0] * J[7]) * tmp;
+  Ji[2] = (J[1] * J[5] - J[4] * J[2]) * tmp;
+  Ji[5] = (J[2] * J[3] - J[0] * J[5]) * tmp;
+  Ji[8] = (J[0] * J[4] - J[3] * J[1]) * tmp;
+  const double detj = 1.0 / tmp;
+
+  // Gauss-point values (e3int_qr)
+  double ui[3], umi[3], aci[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    double pu[8], pm[8], pc[8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      pu[a] = re[a * kRecStride + 3 + i] * Nq[a];
+      pc[a] = re[a * kRecStride + 6 + i] * Nq[a];
+      pm[a] = re[a * kRecStride + 9 + i] * Nq[a];
+    }
+    ui[i] = tree8(pu);
+    umi[i] = tree8(pm);
+    aci[i] = tree8(pc);
+  }
+  double pri;
+  {
+    double pp[8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) pp[a] = re[a * kRecStride + 12] * Nq[a];
+    pri = tree8(pp);
+  }
+  const double rho = P.rho;
+  double mu = P.mu;
+  if (mus) {
+    double pm[8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) pm[a] = re[a * kRecStride + 13] * Nq[a];
+    mu = P.mu * tree8(pm);
+  }
+  double uadv[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) uadv[i] = ui[i] - umi[i];
+
+  // global gradients of the 8 nodes and the gradients (e3int_qr_grad)
+  double ga[8][3];
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    double Na, d0, d1, d2, gw;
+    elem_shape<8>(q, a, Na, d0, d1, d2, gw);
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      ga[a][j] = d0 * Ji[0 + 3 * j] + d1 * Ji[1 + 3 * j] + d2 * Ji[2 + 3 * j];
+  }
+  if (valid && do_mat) {
+    // J^-1 and uadv: the pair kernel forms dN_a/dx and tmp1_a from them with
+    // the expressions above, so the same bits
+    double2 *pj = reinterpret_cast<double2 *>(po + q * 12);
+    pj[0] = make_double2(Ji[0], Ji[1]);
+    pj[1] = make_double2(Ji[2], Ji[3]);
+    pj[2] = make_double2(Ji[4], Ji[5]);
+    pj[3] = make_double2(Ji[6], Ji[7]);
+    pj[4] = make_double2(Ji[8], uadv[0]);
+    pj[5] = make_double2(uadv[1], uadv[2]);
+  }
+  double du[9], dp[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double pd[8];
+#pragma unroll
+      for (int a = 0; a < 8; ++a) pd[a] = re[a * kRecStride + 3 + i] * ga[a][j];
+      du[i + 3 * j] = tree8(pd);
+    }
+    double pd[8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) pd[a] = re[a * kRecStride + 12] * ga[a][j];
+    dp[j] = tree8(pd);
+  }
+  double G[9];
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      G[i + 3 * j] = Ji[0 + 3 * i] * Ji[0 + 3 * j] + Ji[1 + 3 * i] * Ji[1 + 3 * j] +
+                     Ji[2 + 3 * i] * Ji[2 + 3 * j];
+  const double divu = du[0] + du[4] + du[8];
+
+  // e3STAB_3D
+  double taua = 0.0, gg = 0.0;
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) taua += uadv[i] * G[i + 3 * j] * uadv[j];
+#pragma unroll
+  for (int t = 0; t < 9; ++t) gg += G[t] * G[t];
+  const double nu = mu / rho;
+  const double taud = nu * nu * 12.0 * gg;
+  const double taut = 4.0 / (P.delt * P.delt);
+  const double tauM = 1.0 / (rho * sqrt(taut + taua + taud));
+  const double tauC = rho * sqrt(taua + taud) / (G[0] + G[4] + G[8]);
+  // e3CFL
+  double cflq = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) cflq += uadv[i] * G[i + 3 * j] * uadv[j];
+  cflq = P.delt * sqrt(cflq);
+  const double dw = detj;  // the truncated 8-point rule has unit weights
+  if (valid && detj < 0.0) atomicAdd(negdet, 1);
+  if (valid && do_mat) {
+    const double fact2 = P.alfi * P.gami * P.delt;
+    const double cf = fact2 * dw;
+    double2 *pf = reinterpret_cast<double2 *>(po + kPtsF + q * 6);
+    pf[0] = make_double2(cf, P.almi * rho * dw);
+    pf[1] = make_double2(cf * (mu + P.kdc), cf * tauC);
+    pf[2] = make_double2(tauM, 0.0);
+  }
+
+  // RHS (e3Rhs_3D_fluid, as in the kernel above): the point's terms t1, t2,
+  // split over the 8 nodes and summed over the 8 points (the lanes of the
+  // element) in the butterfly association; lane q keeps node a = q
+  if (do_vec) {
+    double own[4] = {0.0, 0.0, 0.0, 0.0};
+    if (vec) {
+      const double mupkdc = mu + P.kdc;
+      const double ptot = pri - tauC * divu;
+      const double fi[3] = {P.gx * rho, P.gy * rho, P.gz * rho};
+      double t1[3], t2[9], up[3] = {0.0, 0.0, 0.0};
+      if (P.vms) {
+        double adv[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          const double rLi = rho * (aci[i] + uadv[0] * du[i + 0] + uadv[1] * du[i + 3] +
+                                    uadv[2] * du[i + 6]) - fi[i] + dp[i];
+          up[i] = -tauM * rLi;
+          adv[i] = uadv[i] + up[i];
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+          t1[i] = rho * (aci[i] + adv[0] * du[i + 0] + adv[1] * du[i + 3] +
+                         adv[2] * du[i + 6]) - fi[i];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j)
+            t2[i + 3 * j] = ((i == j) ? (-ptot + 2.0 * mupkdc * du[i + 3 * i])
+                                      : mupkdc * (du[i + 3 * j] + du[j + 3 * i])) -
+                            rho * adv[j] * up[i];
+      } else {
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+          t1[i] = rho * (aci[i] + uadv[0] * du[i + 0] + uadv[1] * du[i + 3] +
+                         uadv[2] * du[i + 6]) - fi[i];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j)
+            t2[i + 3 * j] = (i == j) ? (-ptot + 2.0 * mupkdc * du[i + 3 * i])
+                                     : mupkdc * (du[i + 3 * j] + du[j + 3 * i]);
+      }
+#pragma unroll
+      for (int a = 0; a < 8; ++a) {
+        double rc[4];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+          rc[i] = -(Nq[a] * t1[i] + (ga[a][0] * t2[i] + ga[a][1] * t2[i + 3] +
+                                     ga[a][2] * t2[i + 6])) * dw;
+        rc[3] = P.vms ? -(Nq[a] * divu - (ga[a][0] * up[0] + ga[a][1] * up[1] +
+                                          ga[a][2] * up[2])) * dw
+                      : -(Nq[a] * divu) * dw;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const double sm = wsum8(rc[c]);
+          if (q == a) own[c] = sm;
+        }
+      }
+    }
+    if (valid) {
+      double2 *pr = reinterpret_cast<double2 *>(po + kPtsR + q * 4);
+      pr[0] = make_double2(own[0], own[1]);
+      pr[1] = make_double2(own[2], own[3]);
+    }
+  }
+  cflq = wmax8(cflq);
+  if (valid && q == 0) cfl_out[e] = cflq;
+}
+
+// One element per wavefront, the elements at positions first .. first +
+// count - 1 of the colour order (one colour).  Everything it reads is
+// indexed by that position: the point data, the scatter map and the
+// first-touch masks.  Every global load is issued before the accumulation
+// (the old values of the nonzeros it adds into included), so only the
+// point data's latency is in front of it.
+__global__ void __launch_bounds__(256)
+k_ns_pairs_hex(int first, int count, const int32_t *__restrict__ emap_c,
+               const uint64_t *__restrict__ first_c, const double *__restrict__ pts,
+               const uint8_t *__restrict__ ibc, double rho, double *__restrict__ A) {
+  // per wavefront: [q][a] (dN_a/dx, dN_a/dy), (dN_a/dz, tmp1_a) 256 doubles,
+  // [q] factors 48, [q][a] N_a 64 during the accumulation; then the block
+  // transpose [64][18]
+  __shared__ __align__(16) double s_w[kWavesPerBlock][64 * 18];
+  __shared__ uint32_t s_m[kWavesPerBlock][8];  // Dirichlet flags of the 8 nodes
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int t = first + blockIdx.x * kWavesPerBlock + w;
+  if (t >= first + count) return;  // whole wavefronts; no block barrier below
+  const double *pt = pts + (size_t)t * kPtsStride;
+  const int piece = lane & 7;
+  const uint64_t fm = first_c[t];
+  int kk[8];
+#pragma unroll
+  for (int rnd = 0; rnd < 8; ++rnd) kk[rnd] = emap_c[(size_t)t * 64 + rnd * 8 + (lane >> 3)];
+  const int32_t *pn = reinterpret_cast<const int32_t *>(pt + kPtsN);
+  const int tgt = pn[8];
+  if (lane < 8) s_m[w][lane] = ibc[pn[lane]];
+  double *sw = s_w[w];
+  // lane = (q, a): dN_a/dx and tmp1_a at point q from J^-1 and uadv
+  {
+    const int q = lane >> 3, a = lane & 7;
+    const double2 *pj = reinterpret_cast<const double2 *>(pt + q * 12);
+    const double2 j0 = pj[0], j1 = pj[1], j2 = pj[2], j3 = pj[3], j4 = pj[4], j5 = pj[5];
+    const double Ji[9] = {j0.x, j0.y, j1.x, j1.y, j2.x, j2.y, j3.x, j3.y, j4.x};
+    const double uadv[3] = {j4.y, j5.x, j5.y};
+    if (lane < 24)
+      reinterpret_cast<double2 *>(sw + 256)[lane] =
+          reinterpret_cast<const double2 *>(pt + kPtsF)[lane];
+    double Na, d0, d1, d2, gw;
+    elem_shape<8>(q, a, Na, d0, d1, d2, gw);
+    double g[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      g[j] = d0 * Ji[0 + 3 * j] + d1 * Ji[1 + 3 * j] + d2 * Ji[2 + 3 * j];
+    const double tmp1a = rho * (uadv[0] * g[0] + uadv[1] * g[1] + uadv[2] * g[2]);
+    double2 *sg = reinterpret_cast<double2 *>(sw) + lane * 2;
+    sg[0] = make_double2(g[0], g[1]);
+    sg[1] = make_double2(g[2], tmp1a);
+    sw[304 + lane] = Na;
+  }
+  // the old values of the nonzeros this lane adds into (not first touches)
+  double2 old[8];
+#pragma unroll
+  for (int rnd = 0; rnd < 8; ++rnd) {
+    const int j = rnd * 8 + (lane >> 3);
+    old[rnd] = ((fm >> j) & 1ull)
+                   ? make_double2(0.0, 0.0)
+                   : reinterpret_cast<const double2 *>(A + (size_t)kk[rnd] * 16)[piece];
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+
+  const int pa_ = lane & 7;   // row node a
+  const int pb_ = lane >> 3;  // column node b
+  double acc[16];
+#pragma unroll
+  for (int u = 0; u < 16; ++u) acc[u] = 0.0;
+  if (tgt) {
+    const double2 *sg = reinterpret_cast<const double2 *>(sw);        // [q][a][2]
+    const double2 *sf = reinterpret_cast<const double2 *>(sw + 256);  // [q][3]
+    const double *sn = sw + 304;                                      // [q][a]
+    // the accumulation of the kernel above, term for term
+#pragma unroll 2
+    for (int qq = 0; qq < 8; ++qq) {
+      const double2 r0 = sg[(qq * 8 + pa_) * 2], r1 = sg[(qq * 8 + pa_) * 2 + 1];
+      const double2 c0 = sg[(qq * 8 + pb_) * 2], c1 = sg[(qq * 8 + pb_) * 2 + 1];
+      const double2 f0 = sf[qq * 3], f1 = sf[qq * 3 + 1], f2 = sf[qq * 3 + 2];
+      const double Nq_ = sn[qq * 8 + pa_], g0 = r0.x, g1 = r0.y, g2 = r1.x, t1a = r1.y;
+      const double Mb = sn[qq * 8 + pb_], h0 = c0.x, h1 = c0.y, h2 = c1.x, t1b = c1.y;
+      const double cf = f0.x, cB = f0.y, cm = f1.x, ct = f1.y, tM = f2.x;
+      const double Bb = cB * Mb + cf * t1b;
+      const double hf0 = cf * h0, hf1 = cf * h1, hf2 = cf * h2;
+      const double hm0 = cm * h0, hm1 = cm * h1, hm2 = cm * h2;
+      const double ht0 = ct * h0, ht1 = ct * h1, ht2 = ct * h2;
+      const double Mf = cf * Mb, PB = tM * Bb;
+      const double Ta = tM * t1a;
+      const double Aa = Nq_ + Ta;
+      const double t4 = Aa * Bb + (g0 * hm0 + g1 * hm1 + g2 * hm2);
+      acc[0] += t4 + g0 * (hm0 + ht0);
+      acc[5] += t4 + g1 * (hm1 + ht1);
+      acc[10] += t4 + g2 * (hm2 + ht2);
+      acc[1] += g1 * hm0 + g0 * ht1;
+      acc[4] += g0 * hm1 + g1 * ht0;
+      acc[2] += g2 * hm0 + g0 * ht2;
+      acc[8] += g0 * hm2 + g2 * ht0;
+      acc[6] += g2 * hm1 + g1 * ht2;
+      acc[9] += g1 * hm2 + g2 * ht1;
+      acc[3] += Ta * hf0 - g0 * Mf;
+      acc[7] += Ta * hf1 - g1 * Mf;
+      acc[11] += Ta * hf2 - g2 * Mf;
+      acc[12] += Nq_ * hf0 + g0 * PB;
+      acc[13] += Nq_ * hf1 + g1 * PB;
+      acc[14] += Nq_ * hf2 + g2 * PB;
+      acc[15] += tM * (g0 * hf0 + g1 * hf1 + g2 * hf2);
+    }
+  }
+  // Dirichlet projection (BCLhs_3D.f90:105-118), as above
+  {
+    const unsigned ma = s_m[w][pa_];
+    const unsigned mb = s_m[w][pb_];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int r = u >> 2, cc = u & 3;
+      const bool colz = ((mb >> cc) & 1u) && !(r == 3 && cc == 3);
+      if (((ma >> r) & 1u) || colz) acc[u] = 0.0;
+      if (pa_ == pb_ && r == cc && ((ma >> r) & 1u)) acc[u] = 1.0;
+    }
+  }
+  // every lane has read the point data: the transpose may overwrite them
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  double2 *mine = reinterpret_cast<double2 *>(sw + lane * 18);
+#pragma unroll
+  for (int u = 0; u < 8; ++u) mine[u] = make_double2(acc[2 * u], acc[2 * u + 1]);
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  // coalesced scatter: 8 lanes per 128-byte block; the first element in
+  // colour order to reach a nonzero stores, the later ones add to the old
+  // value read above
+#pragma unroll
+  for (int rnd = 0; rnd < 8; ++rnd) {
+    const int j = rnd * 8 + (lane >> 3);
+    const double2 v = reinterpret_cast<const double2 *>(sw + j * 18)[piece];
+    double2 *dst = reinterpret_cast<double2 *>(A + (size_t)kk[rnd] * 16) + piece;
+    if ((fm >> j) & 1ull) {
+      *dst = v;
+    } else {
+      double2 o = old[rnd];
+      o.x += v.x;
+      o.y += v.y;
+      *dst = o;
+    }
+  }
+}
+
+// rhs[n] = sum of the node's element contributions in colour order
+// (inc_off/inc: entries t * 8 + a, t the element's position in colour
+// order), with its Dirichlet components zeroed
+__global__ void __launch_bounds__(256)
+k_ns_rhs_gather(int N, const int32_t *__restrict__ inc_off, const int32_t *__restrict__ inc,
+                const double *__restrict__ pts, const uint8_t *__restrict__ ibc,
+                double *__restrict__ rhs) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  const unsigned m0 = ibc[n];
+  for (int t = inc_off[n]; t < inc_off[n + 1]; ++t) {
+    const int ea = inc[t];
+    const double2 *pr =
+        reinterpret_cast<const double2 *>(pts + (size_t)(ea >> 3) * kPtsStride + kPtsR +
+                                          (ea & 7) * 4);
+    const double2 r0 = pr[0], r1 = pr[1];
+    const double v[4] = {r0.x, r0.y, r1.x, r1.y};
+#pragma unroll
+    for (int c = 0; c < 4; ++c) s[c] += ((m0 >> c) & 1u) ? 0.0 : v[c];
+  }
+  double2 *rr = reinterpret_cast<double2 *>(rhs + (size_t)n * 4);
+  rr[0] = make_double2(s[0], s[1]);
+  rr[1] = make_double2(s[2], s[3]);
+}
+
 __global__ void k_zero(double *p, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -500,11 +979,7 @@ hipError_t launch_alpha_stage(Ctx &c, bool rphi) {
   return hipGetLastError();
 }
 
-hipError_t launch_assemble_ns_hex(Ctx &c, int tensor_flag, int subdomain_flag,
-                                  int color) {
-  const int first = c.color_off[color];
-  const int count = c.color_off[color + 1] - first;
-  if (count <= 0) return hipSuccess;
+static AsmParams asm_params(const Ctx &c) {
   AsmParams P;
   P.almi = c.prm[OFD_P_ALMI];
   P.alfi = c.prm[OFD_P_ALFI];
@@ -517,18 +992,57 @@ hipError_t launch_assemble_ns_hex(Ctx &c, int tensor_flag, int subdomain_flag,
   P.gy = c.prm[OFD_P_GY];
   P.gz = c.prm[OFD_P_GZ];
   P.vms = c.res_form;
-  const int blocks = (count + kWavesPerBlock - 1) / kWavesPerBlock;
-  hipLaunchKernelGGL((c.nshl == 4 ? k_assemble_ns_el<4>
-                      : c.nshl == 6 ? k_assemble_ns_el<6> : k_assemble_ns_el<8>),
-                     dim3(blocks), dim3(64 * kWavesPerBlock), 0,
-                     c.stream, c.nnode, c.nelem, c.d_ien, c.d_elm_id,
-                     c.d_color_elems, first, count, c.d_emap, c.d_first, c.d_noderec,
-                     c.mus_on ? c.d_field[OFD_MUSCALE] : nullptr, c.d_ibc,
-                     P, c.elm_id_set[0], c.elm_id_set[1], subdomain_flag,
-                     (tensor_flag & OFD_ASSEMBLE_MAT) ? 1 : 0,
-                     (tensor_flag & OFD_ASSEMBLE_VEC) ? 1 : 0, c.d_A64, c.d_rhs,
-                     c.d_cfl, c.d_negdet);
-  return hipGetLastError();
+  return P;
+}
+
+// The volume loop of assembleNavSto (IntElmAss_NS_overset): hexes through
+// the point, pair and gather kernels (c.d_pts sized by the caller), tets and
+// prisms by the one-wavefront-per-element kernel over zeroed RHS.
+hipError_t launch_assemble_ns(Ctx &c, int tensor_flag, int subdomain_flag) {
+  const AsmParams P = asm_params(c);
+  const int do_mat = (tensor_flag & OFD_ASSEMBLE_MAT) ? 1 : 0;
+  const int do_vec = (tensor_flag & OFD_ASSEMBLE_VEC) ? 1 : 0;
+  const double *mus = c.mus_on ? c.d_field[OFD_MUSCALE] : nullptr;
+  hipError_t e;
+  if (c.nshl == 8) {
+    hipLaunchKernelGGL(k_ns_points_hex, dim3((c.nelem + 7) / 8), dim3(64), 0, c.stream,
+                       c.nelem, c.d_ien, c.d_elm_id, c.d_cpos, c.d_noderec, mus, P,
+                       c.elm_id_set[0], c.elm_id_set[1], subdomain_flag, do_mat, do_vec,
+                       c.d_pts, c.d_cfl, c.d_negdet);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (do_mat) {
+      for (int col = 0; col + 1 < (int)c.color_off.size(); ++col) {
+        const int first = c.color_off[col];
+        const int count = c.color_off[col + 1] - first;
+        if (count <= 0) continue;
+        hipLaunchKernelGGL(k_ns_pairs_hex, dim3((count + kWavesPerBlock - 1) / kWavesPerBlock),
+                           dim3(64 * kWavesPerBlock), 0, c.stream, first, count, c.d_emap_c,
+                           c.d_first_c, c.d_pts, c.d_ibc, P.rho, c.d_A64);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+      }
+    }
+    if (do_vec) {
+      hipLaunchKernelGGL(k_ns_rhs_gather, dim3((c.nnode + 255) / 256), dim3(256), 0, c.stream,
+                         c.nnode, c.d_inc_off, c.d_inc, c.d_pts, c.d_ibc, c.d_rhs);
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    return hipSuccess;
+  }
+  if (do_vec && (e = launch_zero(c, c.d_rhs, (size_t)c.nnode * 4)) != hipSuccess) return e;
+  for (int col = 0; col + 1 < (int)c.color_off.size(); ++col) {
+    const int first = c.color_off[col];
+    const int count = c.color_off[col + 1] - first;
+    if (count <= 0) continue;
+    const int blocks = (count + kWavesPerBlock - 1) / kWavesPerBlock;
+    hipLaunchKernelGGL((c.nshl == 4 ? k_assemble_ns_el<4> : k_assemble_ns_el<6>),
+                       dim3(blocks), dim3(64 * kWavesPerBlock), 0, c.stream, c.nnode,
+                       c.nelem, c.d_ien, c.d_elm_id, c.d_color_elems, first, count, c.d_emap,
+                       c.d_first, c.d_noderec, mus, c.d_ibc, P, c.elm_id_set[0],
+                       c.elm_id_set[1], subdomain_flag, do_mat, do_vec, c.d_A64, c.d_rhs,
+                       c.d_cfl, c.d_negdet);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 hipError_t launch_zero(Ctx &c, double *p, size_t n) {

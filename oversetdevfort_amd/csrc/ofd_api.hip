@@ -105,6 +105,15 @@ static void free_tem_adjoint(Ctx &c) {
   c.tadj_n4 = 0;
 }
 
+// the hex NS assembly's per-element point data (1536 bytes per element),
+// allocated at the first assembly of a hex mesh
+static int ensure_pts(Ctx &c) {
+  if (c.nshl != 8 || (c.d_pts && c.pts_elems == c.nelem)) return OFD_OK;
+  int rc = dalloc(c, &c.d_pts, (size_t)c.nelem * kPtsStride);
+  c.pts_elems = rc ? 0 : c.nelem;
+  return rc;
+}
+
 // Krylov workspace for restart length K
 static int ensure_krylov(Ctx &c, int K) {
   OFD_CHECK(K >= 1 && K <= kMaxKspace, OFD_ERR_ARG, "kspace out of range [1, 64]");
@@ -468,6 +477,12 @@ int ofd_ctx_destroy(ofd_ctx *x) {
   dfree(c.d_emap);
   dfree(c.d_color_elems);
   dfree(c.d_first);
+  dfree(c.d_inc_off);
+  dfree(c.d_inc);
+  dfree(c.d_pts);
+  dfree(c.d_emap_c);
+  dfree(c.d_first_c);
+  dfree(c.d_cpos);
   dfree(c.d_fe_elem);
   dfree(c.d_fe_mask);
   dfree(c.d_diag);
@@ -747,6 +762,25 @@ int ofd_build_pattern(ofd_ctx *x, int64_t *icnt_out) {
   free_tem_transpose(c);
   rc = h2d(c, c.d_color_elems, order.data(), E * sizeof(int32_t));
   if (rc) return rc;
+  {
+    // node -> (position in colour order, node) entries in colour order, the
+    // order in which a coloured scatter adds the elements' contributions
+    const int nshl = c.nshl;
+    std::vector<int32_t> ioff((size_t)N + 1, 0), inc((size_t)E * nshl);
+    for (int e = 0; e < E; ++e)
+      for (int a = 0; a < nshl; ++a) ioff[(size_t)c.h_ien[(size_t)a * E + e] + 1]++;
+    for (int i = 0; i < N; ++i) ioff[i + 1] += ioff[i];
+    std::vector<int32_t> pos(ioff.begin(), ioff.end() - 1);
+    for (int t = 0; t < E; ++t) {
+      const int e = order[t];
+      for (int a = 0; a < nshl; ++a) inc[pos[c.h_ien[(size_t)a * E + e]]++] = t * 8 + a;
+    }
+    rc = dalloc(c, &c.d_inc_off, (size_t)N + 1);
+    if (!rc) rc = dalloc(c, &c.d_inc, inc.size());
+    if (!rc) rc = h2d(c, c.d_inc_off, ioff.data(), ioff.size() * sizeof(int32_t));
+    if (!rc) rc = h2d(c, c.d_inc, inc.data(), inc.size() * sizeof(int32_t));
+    if (rc) return rc;
+  }
   // which element writes each nonzero first in the colour order: it stores,
   // the later ones add (the assemblies need no zeroing pass of the matrix)
   rc = dalloc(c, &c.d_first, (size_t)E);
@@ -756,6 +790,13 @@ int ofd_build_pattern(ofd_ctx *x, int64_t *icnt_out) {
     OFD_HIP(tmp.get((void **)&d_touched, (size_t)c.icnt));
     OFD_HIP(hipMemsetAsync(d_touched, 0, (size_t)c.icnt, c.stream));
     OFD_HIP(launch_first_touch(c, d_touched));
+  }
+  if (c.nshl == 8) {
+    rc = dalloc(c, &c.d_emap_c, (size_t)E * 64);
+    if (!rc) rc = dalloc(c, &c.d_first_c, (size_t)E);
+    if (!rc) rc = dalloc(c, &c.d_cpos, (size_t)E);
+    if (rc) return rc;
+    OFD_HIP(launch_colour_maps(c));
   }
   OFD_HIP(hipMemsetAsync(c.d_A64, 0, (size_t)c.icnt * 16 * sizeof(double), c.stream));
   OFD_HIP(hipStreamSynchronize(c.stream));
@@ -955,8 +996,9 @@ int ofd_assemble_ns(ofd_ctx *x, int tensor_flag, int subdomain_flag, double *cfl
   OFD_CHECK((tensor_flag & ~(OFD_ASSEMBLE_VEC | OFD_ASSEMBLE_MAT)) == 0, OFD_ERR_ARG,
             "tensor_flag must combine OFD_ASSEMBLE_VEC / OFD_ASSEMBLE_MAT");
   OFD_HIP(hipSetDevice(c.device));
+  rc = ensure_pts(c);
+  if (rc) return rc;
   OFD_HIP(launch_alpha_stage(c));
-  if (tensor_flag & OFD_ASSEMBLE_VEC) OFD_HIP(launch_zero(c, c.d_rhs, (size_t)c.nnode * 4));
   if (tensor_flag & OFD_ASSEMBLE_MAT) {
     // no zeroing: the first element to reach a nonzero stores into it
     c.mat_valid = true;
@@ -964,8 +1006,7 @@ int ofd_assemble_ns(ofd_ctx *x, int tensor_flag, int subdomain_flag, double *cfl
     ++c.mat_version;
   }
   OFD_HIP(hipMemsetAsync(c.d_negdet, 0, sizeof(int32_t), c.stream));
-  for (int col = 0; col + 1 < (int)c.color_off.size(); ++col)
-    OFD_HIP(launch_assemble_ns_hex(c, tensor_flag, subdomain_flag, col));
+  OFD_HIP(launch_assemble_ns(c, tensor_flag, subdomain_flag));
   // FaceAssembly_NS_weak and FaceAssembly_NS_outflow (assembleNavSto.f90:54-60)
   rc = ensure_faces(c);
   if (rc) return rc;
@@ -1357,16 +1398,16 @@ int ofd_time_assemble(ofd_ctx *x, int tensor_flag, int reps, double *ms) {
   OFD_CHECK(c.nshl == 8 || c.nshl == 4 || c.nshl == 6, OFD_ERR_UNSUPPORTED,
             "the GPU NS assembly is implemented for trilinear hexes, linear tets and prisms");
   OFD_HIP(hipSetDevice(c.device));
+  rc = ensure_pts(c);
+  if (rc) return rc;
   hipEvent_t e0, e1;
   OFD_HIP(hipEventCreate(&e0));
   OFD_HIP(hipEventCreate(&e1));
   OFD_HIP(hipEventRecord(e0, c.stream));
   for (int r = 0; r < reps; ++r) {
     OFD_HIP(launch_alpha_stage(c));
-    if (tensor_flag & OFD_ASSEMBLE_VEC) OFD_HIP(launch_zero(c, c.d_rhs, (size_t)c.nnode * 4));
     OFD_HIP(hipMemsetAsync(c.d_negdet, 0, sizeof(int32_t), c.stream));
-    for (int col = 0; col + 1 < (int)c.color_off.size(); ++col)
-      OFD_HIP(launch_assemble_ns_hex(c, tensor_flag, 3, col));
+    OFD_HIP(launch_assemble_ns(c, tensor_flag, 3));
     for (int col = 0; col + 1 < (int)c.fe_color_off.size(); ++col)
       OFD_HIP(launch_face_ns_hex(c, tensor_flag, 3, col));
     OFD_HIP(launch_max_reduce(c, c.d_cfl, c.nelem, c.d_part));

@@ -98,6 +98,13 @@ struct Ctx {
   int32_t *d_col = nullptr, *d_row = nullptr;
   int32_t *d_emap = nullptr;          // [E][64]
   int32_t *d_color_elems = nullptr;   // elements grouped by colour
+  int32_t *d_inc_off = nullptr;       // [N+1] node -> its (element, node) entries
+  int32_t *d_inc = nullptr;           // t * 8 + a: position t in colour order, node a
+  double *d_pts = nullptr;            // hex NS assembly: [pts_elems][kPtsStride] point data,
+  int pts_elems = 0;                  // by position in colour order
+  int32_t *d_emap_c = nullptr;        // hexes: emap and first-touch masks by colour
+  uint64_t *d_first_c = nullptr;      // position, and each element's position
+  int32_t *d_cpos = nullptr;
   uint64_t *d_first = nullptr;        // [E] bit a + 8 b: the element is the first (in
                                       // colour order) to add into pair (a, b)'s nonzero
   std::vector<int32_t> color_off;     // ncolor+1
@@ -240,8 +247,7 @@ struct Ctx {
 // ---- launchers (defined in the .hip files) ----
 // assembly
 hipError_t launch_alpha_stage(Ctx &c, bool rphi = false);
-hipError_t launch_assemble_ns_hex(Ctx &c, int tensor_flag, int subdomain_flag,
-                                  int color);
+hipError_t launch_assemble_ns(Ctx &c, int tensor_flag, int subdomain_flag);
 // adjoint (adjoint.hip)
 hipError_t launch_block_transpose(Ctx &c);  // t_A64 = blocks of A64 mirrored and transposed
 // gradients of lambda . F (lam [N][4]) into d_vjp (zeroed by the caller)
@@ -286,6 +292,7 @@ hipError_t launch_adjacency(Ctx &c, const int32_t *off, const int32_t *inc, cons
 hipError_t launch_tmap_emap(Ctx &c, int32_t *err);
 // d_first from d_emap and the colouring (touched: [icnt] zeroed bytes)
 hipError_t launch_first_touch(Ctx &c, uint8_t *touched);
+hipError_t launch_colour_maps(Ctx &c);
 hipError_t scan_offsets(Ctx &c, const int32_t *in, int32_t *out, int n, void *tmp,
                         size_t &tmp_bytes);
 hipError_t launch_cgs_update(Ctx &c, int k, int mode);
@@ -302,6 +309,11 @@ hipError_t launch_norm2_v(Ctx &c, const double *v);
 hipError_t launch_apply_dirichlet(Ctx &c);
 hipError_t launch_copy(Ctx &c, const void *src, void *dst, size_t nbytes);
 constexpr int kReadBlocks = 8192;  // blocks of the read-only stream kernel
+// doubles of hex NS point data per element: [q] (J^-1 column-major, uadv)
+// 96, [q] (cf, cB, cm, ct, tauM, 0) 48, [a] RHS contributions 32, then the
+// 8 node numbers and the subdomain flag as int32 (5 doubles); 1536 bytes
+constexpr int kPtsStride = 192;
+constexpr int kPtsF = 96, kPtsR = 144, kPtsN = 176;
 hipError_t launch_read(Ctx &c, const void *src, size_t nbytes, double *out);
 // overset fringe
 hipError_t launch_fringe(Ctx &c, int32_t src_id, int nrecv, const double *xr,

@@ -164,7 +164,33 @@ k_first_touch(int S, const int32_t *__restrict__ elems, int first_idx, int count
   if (lane == 0) first[e] = m;
 }
 
+// The scatter map and first-touch masks in colour order (position t of
+// the colour-sorted element list), and each element's position: the hex NS
+// pair kernel reads them by position, with no element index in between.
+__global__ void __launch_bounds__(256)
+k_colour_maps(int E, const int32_t *__restrict__ elems, const int32_t *__restrict__ emap,
+              const uint64_t *__restrict__ first, int32_t *__restrict__ emap_c,
+              uint64_t *__restrict__ first_c, int32_t *__restrict__ cpos) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)E * 64) return;
+  const int t = (int)(i >> 6), j = (int)(i & 63);
+  const int e = elems[t];
+  emap_c[i] = emap[(size_t)e * 64 + j];
+  if (j == 0) {
+    first_c[t] = first[e];
+    cpos[e] = t;
+  }
+}
+
 }  // namespace
+
+hipError_t launch_colour_maps(Ctx &c) {
+  const size_t n = (size_t)c.nelem * 64;
+  hipLaunchKernelGGL(k_colour_maps, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream,
+                     c.nelem, c.d_color_elems, c.d_emap, c.d_first, c.d_emap_c, c.d_first_c,
+                     c.d_cpos);
+  return hipGetLastError();
+}
 
 hipError_t launch_first_touch(Ctx &c, uint8_t *touched) {
   for (int col = 0; col + 1 < (int)c.color_off.size(); ++col) {

