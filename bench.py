@@ -381,7 +381,7 @@ def overset_steps(L, C, S, nsteps=10, nnewton=3, kspace=64):
 
 def spmv_large(L, C, S, dims=(128, 64, 64), reps=20):
     """The NS block product beyond the caches: the 128x64x64 channel's
-    pattern (14,340,865 blocks; 2.86 GB per FP64 product, 1.46 GB with the
+    pattern (14,340,865 blocks; 1.93 GB per FP64 product, 1.01 GB with the
     FP32 matrix, several times the 256 MB Infinity Cache), beside a
     read-only stream of 4 GiB and a 4 GiB copy on the same device."""
     prob = C.channel(*dims)
@@ -975,7 +975,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32-matrix/f64-vector (mixed); f64 reported alongside",
+        "dtype": "f32 matrix and Krylov basis, f64 accumulation, dots, Givens, solution and outer residual (mixed); f64 reported alongside",
         "data": "synthetic",
         "config": {"workload": (f"channel [0,4]x[0,1]^2, {nx}x{ny}x{nz} hex, Re=1000, "
                                 "dt=0.005, 1 Newton step/time step, rtol 1e-8, "
