@@ -87,13 +87,12 @@ struct AsmParams {
   int vms;  // 1: the residual with the fine-scale velocity uprime = -tauM rLi
 };
 
-// S = 8: the trilinear hex, 8 Gauss points (the layout above).  S = 4 and
-// S = 6: the linear tet and prism of the reference's live dispatch
-// (lintetshl, linprishl; eval_shape_FEM.f90:18-26), at the 4- and 6-point
-// rules of genGPandGWtetpri, padded to the same 8x8 lanes: node lanes a >= S
-// carry zero shape values and gradients (the butterfly sums are unchanged),
-// point lanes q >= S repeat the first points with zero weight, and the padded
-// (a, b) pairs are not scattered.
+// Launched for the linear prism (S = 6, linprishl at the 6-point rule of
+// genGPandGWtetpri; eval_shape_FEM.f90:18-26); hexes and tets take the split
+// kernels below.  An element of S < 8 nodes is padded to the 8x8 lanes: node
+// lanes a >= S carry zero shape values and gradients (the butterfly sums are
+// unchanged), point lanes q >= S repeat the first points with zero weight,
+// and the padded (a, b) pairs are not scattered.
 template <int S>
 __global__ void __launch_bounds__(256)
 k_assemble_ns_el(int N, int E, const int32_t *__restrict__ ien,
@@ -445,48 +444,66 @@ k_assemble_ns_el(int N, int E, const int32_t *__restrict__ ien,
 }
 
 // ---------------------------------------------------------------------------
-// Trilinear hexes in three kernels.
+// Hexes and tets in three kernels (S = 8 or 4 nodes and Gauss points).
 //
 // The kernel above gives each element a whole wavefront with lane = (q, a):
 // everything that belongs to a Gauss point alone (J^-1, G, the tau's, the
-// RHS terms before the node split) runs in 8 copies, and every Gauss-point
-// interpolation is an 8-lane reduction.  For hexes the volume loop is split:
+// RHS terms before the node split) runs in S copies, every Gauss-point
+// interpolation is a lane reduction, and a tet leaves three quarters of the
+// lanes idle.  For hexes and tets the volume loop is split:
 //
-//   k_ns_points_hex  one launch over all elements, eight per wavefront,
-//                    lane = (element, q): the Gauss-point work runs once per
-//                    lane and the sums over the 8 nodes are register loops,
-//                    added in the association ((p0+p1)+(p2+p3))+((p4+p5)+
-//                    (p6+p7)) with every product rounded first.  It writes
-//                    each element's point data at its position in colour
-//                    order: J^-1 and the advective velocity, the point
-//                    factors, the RHS contributions per node (summed over q
-//                    in the same association, e3Rhs_3D_fluid), the node
-//                    numbers; and its CFL number.
-//   k_ns_pairs_hex   one launch per colour, one element per wavefront: lane =
-//                    (q, a) forms dN_a/dx and tmp1_a from the point data (the
-//                    same expressions, so the same bits), then lane = (a, b)
-//                    the 16 entries of block (a, b) over the 8 points
-//                    (e3LHS_3D_fluid_Old), the Dirichlet masks and the
-//                    coalesced first-touch scatter of the kernel above.
-//   k_ns_rhs_gather  one launch, thread per node: the node's contributions in
-//                    colour order (the order of the coloured scatter, so the
-//                    same bits), masked by its Dirichlet flags.
-//
-// Per element the point data take 1536 bytes (kPtsStride doubles).
+//   k_ns_points<S>  one launch over all elements, 64 / S per wavefront,
+//                   lane = (element, q): the Gauss-point work runs once per
+//                   lane and the sums over the S nodes are register loops,
+//                   added in the association of the lane butterfly
+//                   (((p0+p1)+(p2+p3))+((p4+p5)+(p6+p7)) for hexes) with
+//                   every product rounded first.  It writes each element's
+//                   point data at its position in colour order (PtsLayout):
+//                   J^-1 and the advective velocity, the point factors, the
+//                   RHS contributions per node (summed over q in the same
+//                   association, e3Rhs_3D_fluid), the node numbers; and its
+//                   CFL number.
+//   k_ns_pairs<S>   one launch per colour, 64 / S^2 elements per wavefront:
+//                   lane = (element, q, a) forms dN_a/dx and tmp1_a from the
+//                   point data (the same expressions, so the same bits), then
+//                   lane = (element, a, b) the 16 entries of block (a, b) over
+//                   the S points (e3LHS_3D_fluid_Old), the Dirichlet masks and
+//                   the coalesced first-touch scatter of the kernel above.
+//   k_ns_rhs_gather one launch, thread per node: the node's contributions in
+//                   colour order (the order of the coloured scatter, so the
+//                   same bits), masked by its Dirichlet flags.
 // ---------------------------------------------------------------------------
 namespace {
 
-// the sum of eight products in the lane-butterfly association; the adds
+// the sum of S products in the lane-butterfly association (S = 8:
+// ((p0+p1)+(p2+p3))+((p4+p5)+(p6+p7)); S = 4: (p0+p1)+(p2+p3)); the adds
 // carry no contraction, so no product is fused into them
-__device__ __forceinline__ double tree8(const double (&p)[8]) {
+template <int S>
+__device__ __forceinline__ double tree_sum(const double (&p)[S]) {
 #pragma clang fp contract(off)
-  return ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7]));
+  if constexpr (S == 8)
+    return ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7]));
+  else
+    return (p[0] + p[1]) + (p[2] + p[3]);
 }
 
-// linhexshl value of node a at Gauss point q (elem_shape<8>'s Na)
-__device__ __forceinline__ double hex_N(int q, int a) {
+// sum and maximum over the S lanes of an element's Gauss points
+template <int S>
+__device__ __forceinline__ double point_sum(double v) {
+  if constexpr (S == 8) return wsum8(v);
+  else return wsum4(v);
+}
+template <int S>
+__device__ __forceinline__ double point_max(double v) {
+  if constexpr (S == 8) return wmax8(v);
+  else return wmax4(v);
+}
+
+// the shape value of node a at Gauss point q (elem_shape<S>'s Na)
+template <int S>
+__device__ __forceinline__ double shape_N(int q, int a) {
   double Na, d0, d1, d2, w;
-  elem_shape<8>(q, a, Na, d0, d1, d2, w);
+  elem_shape<S>(q, a, Na, d0, d1, d2, w);
   return Na;
 }
 
@@ -494,19 +511,21 @@ constexpr int kRecStride = 14;  // doubles per staged node record
 
 }  // namespace
 
+template <int S>
 __global__ void __launch_bounds__(64)
-k_ns_points_hex(int E, const int32_t *__restrict__ ien, const int32_t *__restrict__ elm_id,
+k_ns_points(int E, const int32_t *__restrict__ ien, const int32_t *__restrict__ elm_id,
                 const int32_t *__restrict__ cpos, const double *__restrict__ rec,
                 const double *__restrict__ mus, AsmParams P, int id1, int id2,
                 int subdomain_flag, int do_mat, int do_vec, double *__restrict__ pts,
                 double *__restrict__ cfl_out, int32_t *__restrict__ negdet) {
-  // the 8 node records of each element: [el][a][14] (x + d, u, ac, um, p,
+  using L = PtsLayout<S>;
+  // the S node records of each element: [el][a][14] (x + d, u, ac, um, p,
   // viscosity scale)
-  __shared__ __align__(16) double s_rec[8 * 8 * kRecStride];
+  __shared__ __align__(16) double s_rec[64 * kRecStride];
   const int lane = threadIdx.x;
-  const int el = lane >> 3;
-  const int q = lane & 7;  // Gauss point, and node a = q for the loads
-  const int e = blockIdx.x * 8 + el;
+  const int el = lane / S;  // 64 / S elements per wavefront
+  const int q = lane % S;   // Gauss point, and node a = q for the loads
+  const int e = blockIdx.x * (64 / S) + el;
   const bool valid = e < E;
   bool tgt = false;
   if (valid) {
@@ -516,44 +535,44 @@ k_ns_points_hex(int E, const int32_t *__restrict__ ien, const int32_t *__restric
   }
   const bool vec = do_vec && tgt;
   // this element's point data go to its position in colour order
-  double *po = pts + (size_t)(valid ? cpos[e] : 0) * kPtsStride;
+  double *po = pts + (size_t)(valid ? cpos[e] : 0) * L::stride;
   {
     const int n = valid ? ien[(size_t)q * E + e] : 0;
     const double2 *r2 = reinterpret_cast<const double2 *>(rec + (size_t)n * 16);
-    double2 *sr = reinterpret_cast<double2 *>(s_rec + (el * 8 + q) * kRecStride);
+    double2 *sr = reinterpret_cast<double2 *>(s_rec + (el * S + q) * kRecStride);
 #pragma unroll
     for (int t = 0; t < 6; ++t) sr[t] = r2[t];
     sr[6] = make_double2(r2[6].x, mus ? mus[n] : 1.0);
     if (valid && do_mat) {
-      int32_t *pn = reinterpret_cast<int32_t *>(po + kPtsN);
+      int32_t *pn = reinterpret_cast<int32_t *>(po + L::N);
       pn[q] = n;
-      if (q == 0) pn[8] = tgt ? 1 : 0;
+      if (q == 0) pn[S] = tgt ? 1 : 0;
     }
   }
   __syncthreads();
-  const double *re = s_rec + el * 8 * kRecStride;
+  const double *re = s_rec + el * S * kRecStride;
 
-  double Nq[8];
+  double Nq[S];
 #pragma unroll
-  for (int a = 0; a < 8; ++a) Nq[a] = hex_N(q, a);
+  for (int a = 0; a < S; ++a) Nq[a] = shape_N<S>(q, a);
 
   // dx/dxi (i,j) = sum_a (x+d)_a,i dN_a/dxi_j
   double J[9];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    double p0[8], p1[8], p2[8];
+    double p0[S], p1[S], p2[S];
 #pragma unroll
-    for (int a = 0; a < 8; ++a) {
+    for (int a = 0; a < S; ++a) {
       double Na, d0, d1, d2, gw;
-      elem_shape<8>(q, a, Na, d0, d1, d2, gw);
+      elem_shape<S>(q, a, Na, d0, d1, d2, gw);
       const double x = re[a * kRecStride + i];
       p0[a] = x * d0;
       p1[a] = x * d1;
       p2[a] = x * d2;
     }
-    J[i + 0] = tree8(p0);
-    J[i + 3] = tree8(p1);
-    J[i + 6] = tree8(p2);
+    J[i + 0] = tree_sum<S>(p0);
+    J[i + 3] = tree_sum<S>(p1);
+    J[i + 6] = tree_sum<S>(p2);
   }
   // get_inverse_3x3 (eval_shape.f90:85-112); column-major Ji(i + 3 j)
   double Ji[9];
@@ -576,42 +595,42 @@ k_ns_points_hex(int E, const int32_t *__restrict__ ien, const int32_t *__restric
   double ui[3], umi[3], aci[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    double pu[8], pm[8], pc[8];
+    double pu[S], pm[S], pc[S];
 #pragma unroll
-    for (int a = 0; a < 8; ++a) {
+    for (int a = 0; a < S; ++a) {
       pu[a] = re[a * kRecStride + 3 + i] * Nq[a];
       pc[a] = re[a * kRecStride + 6 + i] * Nq[a];
       pm[a] = re[a * kRecStride + 9 + i] * Nq[a];
     }
-    ui[i] = tree8(pu);
-    umi[i] = tree8(pm);
-    aci[i] = tree8(pc);
+    ui[i] = tree_sum<S>(pu);
+    umi[i] = tree_sum<S>(pm);
+    aci[i] = tree_sum<S>(pc);
   }
   double pri;
   {
-    double pp[8];
+    double pp[S];
 #pragma unroll
-    for (int a = 0; a < 8; ++a) pp[a] = re[a * kRecStride + 12] * Nq[a];
-    pri = tree8(pp);
+    for (int a = 0; a < S; ++a) pp[a] = re[a * kRecStride + 12] * Nq[a];
+    pri = tree_sum<S>(pp);
   }
   const double rho = P.rho;
   double mu = P.mu;
   if (mus) {
-    double pm[8];
+    double pm[S];
 #pragma unroll
-    for (int a = 0; a < 8; ++a) pm[a] = re[a * kRecStride + 13] * Nq[a];
-    mu = P.mu * tree8(pm);
+    for (int a = 0; a < S; ++a) pm[a] = re[a * kRecStride + 13] * Nq[a];
+    mu = P.mu * tree_sum<S>(pm);
   }
   double uadv[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) uadv[i] = ui[i] - umi[i];
 
-  // global gradients of the 8 nodes and the gradients (e3int_qr_grad)
-  double ga[8][3];
+  // global gradients of the S nodes and the gradients (e3int_qr_grad)
+  double ga[S][3], gwq = 1.0;
 #pragma unroll
-  for (int a = 0; a < 8; ++a) {
-    double Na, d0, d1, d2, gw;
-    elem_shape<8>(q, a, Na, d0, d1, d2, gw);
+  for (int a = 0; a < S; ++a) {
+    double Na, d0, d1, d2;
+    elem_shape<S>(q, a, Na, d0, d1, d2, gwq);
 #pragma unroll
     for (int j = 0; j < 3; ++j)
       ga[a][j] = d0 * Ji[0 + 3 * j] + d1 * Ji[1 + 3 * j] + d2 * Ji[2 + 3 * j];
@@ -632,15 +651,15 @@ k_ns_points_hex(int E, const int32_t *__restrict__ ien, const int32_t *__restric
   for (int j = 0; j < 3; ++j) {
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      double pd[8];
+      double pd[S];
 #pragma unroll
-      for (int a = 0; a < 8; ++a) pd[a] = re[a * kRecStride + 3 + i] * ga[a][j];
-      du[i + 3 * j] = tree8(pd);
+      for (int a = 0; a < S; ++a) pd[a] = re[a * kRecStride + 3 + i] * ga[a][j];
+      du[i + 3 * j] = tree_sum<S>(pd);
     }
-    double pd[8];
+    double pd[S];
 #pragma unroll
-    for (int a = 0; a < 8; ++a) pd[a] = re[a * kRecStride + 12] * ga[a][j];
-    dp[j] = tree8(pd);
+    for (int a = 0; a < S; ++a) pd[a] = re[a * kRecStride + 12] * ga[a][j];
+    dp[j] = tree_sum<S>(pd);
   }
   double G[9];
 #pragma unroll
@@ -671,19 +690,19 @@ k_ns_points_hex(int E, const int32_t *__restrict__ ien, const int32_t *__restric
 #pragma unroll
     for (int j = 0; j < 3; ++j) cflq += uadv[i] * G[i + 3 * j] * uadv[j];
   cflq = P.delt * sqrt(cflq);
-  const double dw = detj;  // the truncated 8-point rule has unit weights
+  const double dw = detj * gwq;  // hexes: unit weights, so dw = detj
   if (valid && detj < 0.0) atomicAdd(negdet, 1);
   if (valid && do_mat) {
     const double fact2 = P.alfi * P.gami * P.delt;
     const double cf = fact2 * dw;
-    double2 *pf = reinterpret_cast<double2 *>(po + kPtsF + q * 6);
+    double2 *pf = reinterpret_cast<double2 *>(po + L::F + q * 6);
     pf[0] = make_double2(cf, P.almi * rho * dw);
     pf[1] = make_double2(cf * (mu + P.kdc), cf * tauC);
     pf[2] = make_double2(tauM, 0.0);
   }
 
   // RHS (e3Rhs_3D_fluid, as in the kernel above): the point's terms t1, t2,
-  // split over the 8 nodes and summed over the 8 points (the lanes of the
+  // split over the S nodes and summed over the S points (the lanes of the
   // element) in the butterfly association; lane q keeps node a = q
   if (do_vec) {
     double own[4] = {0.0, 0.0, 0.0, 0.0};
@@ -725,7 +744,7 @@ k_ns_points_hex(int E, const int32_t *__restrict__ ien, const int32_t *__restric
                                      : mupkdc * (du[i + 3 * j] + du[j + 3 * i]);
       }
 #pragma unroll
-      for (int a = 0; a < 8; ++a) {
+      for (int a = 0; a < S; ++a) {
         double rc[4];
 #pragma unroll
         for (int i = 0; i < 3; ++i)
@@ -736,101 +755,131 @@ k_ns_points_hex(int E, const int32_t *__restrict__ ien, const int32_t *__restric
                       : -(Nq[a] * divu) * dw;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-          const double sm = wsum8(rc[c]);
+          const double sm = point_sum<S>(rc[c]);
           if (q == a) own[c] = sm;
         }
       }
     }
     if (valid) {
-      double2 *pr = reinterpret_cast<double2 *>(po + kPtsR + q * 4);
+      double2 *pr = reinterpret_cast<double2 *>(po + L::R + q * 4);
       pr[0] = make_double2(own[0], own[1]);
       pr[1] = make_double2(own[2], own[3]);
     }
   }
-  cflq = wmax8(cflq);
+  cflq = point_max<S>(cflq);
   if (valid && q == 0) cfl_out[e] = cflq;
 }
 
-// One element per wavefront, the elements at positions first .. first +
-// count - 1 of the colour order (one colour).  Everything it reads is
-// indexed by that position: the point data, the scatter map and the
+// One colour's elements at positions first .. first + count - 1 of the
+// colour order, 64 / S^2 per wavefront (one hex, four tets).  Everything it
+// reads is indexed by position: the point data, the scatter map and the
 // first-touch masks.  Every global load is issued before the accumulation
 // (the old values of the nonzeros it adds into included), so only the
 // point data's latency is in front of it.
+template <int S>
 __global__ void __launch_bounds__(256)
-k_ns_pairs_hex(int first, int count, const int32_t *__restrict__ emap_c,
-               const uint64_t *__restrict__ first_c, const double *__restrict__ pts,
-               const uint8_t *__restrict__ ibc, double rho, double *__restrict__ A) {
-  // per wavefront: [q][a] (dN_a/dx, dN_a/dy), (dN_a/dz, tmp1_a) 256 doubles,
-  // [q] factors 48, [q][a] N_a 64 during the accumulation; then the block
+k_ns_pairs(int first, int count, const int32_t *__restrict__ emap_c,
+           const uint64_t *__restrict__ first_c, const double *__restrict__ pts,
+           const uint8_t *__restrict__ ibc, double rho, double *__restrict__ A) {
+  using L = PtsLayout<S>;
+  constexpr int SS = S * S;
+  constexpr int EW = 64 / SS;       // elements per wavefront
+  constexpr int FO = 256;           // factors [slot][q] (6 doubles) after the gradients
+  constexpr int NO = FO + EW * S * 6;  // N_a at point q: [q][a]
+  // per wavefront: [slot][q][a] (dN_a/dx, dN_a/dy), (dN_a/dz, tmp1_a), the
+  // factors and the N table during the accumulation; then the block
   // transpose [64][18]
   __shared__ __align__(16) double s_w[kWavesPerBlock][64 * 18];
-  __shared__ uint32_t s_m[kWavesPerBlock][8];  // Dirichlet flags of the 8 nodes
+  __shared__ uint32_t s_m[kWavesPerBlock][16];  // [slot][a] Dirichlet flags
+  __shared__ int32_t s_tg[kWavesPerBlock][4];   // [slot] in the selected subdomains
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
-  const int t = first + blockIdx.x * kWavesPerBlock + w;
-  if (t >= first + count) return;  // whole wavefronts; no block barrier below
-  const double *pt = pts + (size_t)t * kPtsStride;
+  const int t0 = first + (blockIdx.x * kWavesPerBlock + w) * EW;
+  if (t0 >= first + count) return;  // whole wavefronts; no block barrier below
+  const int nv = (first + count - t0 < EW) ? first + count - t0 : EW;  // live slots
   const int piece = lane & 7;
-  const uint64_t fm = first_c[t];
+  // the scatter plan: round rnd moves block jj = rnd * 8 + lane / 8, i.e.
+  // pair (aj, bj) of slot jj / S^2
   int kk[8];
+  bool fst[8];
 #pragma unroll
-  for (int rnd = 0; rnd < 8; ++rnd) kk[rnd] = emap_c[(size_t)t * 64 + rnd * 8 + (lane >> 3)];
-  const int32_t *pn = reinterpret_cast<const int32_t *>(pt + kPtsN);
-  const int tgt = pn[8];
-  if (lane < 8) s_m[w][lane] = ibc[pn[lane]];
+  for (int rnd = 0; rnd < 8; ++rnd) {
+    const int jj = rnd * 8 + (lane >> 3);
+    const int js = jj / SS, rj = jj % SS;
+    const int aj = rj % S, bj = rj / S;
+    const int tj = t0 + (js < nv ? js : 0);
+    kk[rnd] = emap_c[(size_t)tj * 64 + aj + 8 * bj];
+    fst[rnd] = (first_c[tj] >> (aj + 8 * bj)) & 1ull;
+  }
   double *sw = s_w[w];
-  // lane = (q, a): dN_a/dx and tmp1_a at point q from J^-1 and uadv
+  if (lane < EW * S) {
+    const int sl = lane / S, a = lane % S;
+    if (sl < nv) {
+      const int32_t *pn = reinterpret_cast<const int32_t *>(
+          pts + (size_t)(t0 + sl) * L::stride + L::N);
+      s_m[w][lane] = ibc[pn[a]];
+      if (a == 0) s_tg[w][sl] = pn[S];
+    }
+  }
+  if (lane < EW * S * 3) {
+    const int sl = lane / (S * 3), pc = lane % (S * 3);
+    if (sl < nv)
+      reinterpret_cast<double2 *>(sw + FO + sl * S * 6)[pc] =
+          reinterpret_cast<const double2 *>(pts + (size_t)(t0 + sl) * L::stride + L::F)[pc];
+  }
+  if (lane < SS) sw[NO + lane] = shape_N<S>(lane / S, lane % S);
+  // lane = (slot, q, a): dN_a/dx and tmp1_a at point q from J^-1 and uadv
   {
-    const int q = lane >> 3, a = lane & 7;
-    const double2 *pj = reinterpret_cast<const double2 *>(pt + q * 12);
-    const double2 j0 = pj[0], j1 = pj[1], j2 = pj[2], j3 = pj[3], j4 = pj[4], j5 = pj[5];
-    const double Ji[9] = {j0.x, j0.y, j1.x, j1.y, j2.x, j2.y, j3.x, j3.y, j4.x};
-    const double uadv[3] = {j4.y, j5.x, j5.y};
-    if (lane < 24)
-      reinterpret_cast<double2 *>(sw + 256)[lane] =
-          reinterpret_cast<const double2 *>(pt + kPtsF)[lane];
-    double Na, d0, d1, d2, gw;
-    elem_shape<8>(q, a, Na, d0, d1, d2, gw);
-    double g[3];
+    const int sl = lane / SS, r = lane % SS, q = r / S, a = r % S;
+    if (sl < nv) {
+      const double2 *pj =
+          reinterpret_cast<const double2 *>(pts + (size_t)(t0 + sl) * L::stride + q * 12);
+      const double2 j0 = pj[0], j1 = pj[1], j2 = pj[2], j3 = pj[3], j4 = pj[4], j5 = pj[5];
+      const double Ji[9] = {j0.x, j0.y, j1.x, j1.y, j2.x, j2.y, j3.x, j3.y, j4.x};
+      const double uadv[3] = {j4.y, j5.x, j5.y};
+      double Na, d0, d1, d2, gw;
+      elem_shape<S>(q, a, Na, d0, d1, d2, gw);
+      double g[3];
 #pragma unroll
-    for (int j = 0; j < 3; ++j)
-      g[j] = d0 * Ji[0 + 3 * j] + d1 * Ji[1 + 3 * j] + d2 * Ji[2 + 3 * j];
-    const double tmp1a = rho * (uadv[0] * g[0] + uadv[1] * g[1] + uadv[2] * g[2]);
-    double2 *sg = reinterpret_cast<double2 *>(sw) + lane * 2;
-    sg[0] = make_double2(g[0], g[1]);
-    sg[1] = make_double2(g[2], tmp1a);
-    sw[304 + lane] = Na;
+      for (int j = 0; j < 3; ++j)
+        g[j] = d0 * Ji[0 + 3 * j] + d1 * Ji[1 + 3 * j] + d2 * Ji[2 + 3 * j];
+      const double tmp1a = rho * (uadv[0] * g[0] + uadv[1] * g[1] + uadv[2] * g[2]);
+      double2 *sg = reinterpret_cast<double2 *>(sw) + lane * 2;
+      sg[0] = make_double2(g[0], g[1]);
+      sg[1] = make_double2(g[2], tmp1a);
+    }
   }
   // the old values of the nonzeros this lane adds into (not first touches)
   double2 old[8];
 #pragma unroll
   for (int rnd = 0; rnd < 8; ++rnd) {
-    const int j = rnd * 8 + (lane >> 3);
-    old[rnd] = ((fm >> j) & 1ull)
+    const int js = (rnd * 8 + (lane >> 3)) / SS;
+    old[rnd] = (fst[rnd] || js >= nv)
                    ? make_double2(0.0, 0.0)
                    : reinterpret_cast<const double2 *>(A + (size_t)kk[rnd] * 16)[piece];
   }
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   __builtin_amdgcn_wave_barrier();
 
-  const int pa_ = lane & 7;   // row node a
-  const int pb_ = lane >> 3;  // column node b
+  // lane = (slot, a, b)
+  const int sl = lane / SS, rr = lane % SS;
+  const int pa_ = rr % S;  // row node a
+  const int pb_ = rr / S;  // column node b
   double acc[16];
 #pragma unroll
   for (int u = 0; u < 16; ++u) acc[u] = 0.0;
-  if (tgt) {
-    const double2 *sg = reinterpret_cast<const double2 *>(sw);        // [q][a][2]
-    const double2 *sf = reinterpret_cast<const double2 *>(sw + 256);  // [q][3]
-    const double *sn = sw + 304;                                      // [q][a]
+  if (sl < nv && s_tg[w][sl]) {
+    const double2 *sg = reinterpret_cast<const double2 *>(sw) + sl * SS * 2;  // [q][a][2]
+    const double2 *sf = reinterpret_cast<const double2 *>(sw + FO + sl * S * 6);  // [q][3]
+    const double *sn = sw + NO;                                                   // [q][a]
     // the accumulation of the kernel above, term for term
 #pragma unroll 2
-    for (int qq = 0; qq < 8; ++qq) {
-      const double2 r0 = sg[(qq * 8 + pa_) * 2], r1 = sg[(qq * 8 + pa_) * 2 + 1];
-      const double2 c0 = sg[(qq * 8 + pb_) * 2], c1 = sg[(qq * 8 + pb_) * 2 + 1];
+    for (int qq = 0; qq < S; ++qq) {
+      const double2 r0 = sg[(qq * S + pa_) * 2], r1 = sg[(qq * S + pa_) * 2 + 1];
+      const double2 c0 = sg[(qq * S + pb_) * 2], c1 = sg[(qq * S + pb_) * 2 + 1];
       const double2 f0 = sf[qq * 3], f1 = sf[qq * 3 + 1], f2 = sf[qq * 3 + 2];
-      const double Nq_ = sn[qq * 8 + pa_], g0 = r0.x, g1 = r0.y, g2 = r1.x, t1a = r1.y;
-      const double Mb = sn[qq * 8 + pb_], h0 = c0.x, h1 = c0.y, h2 = c1.x, t1b = c1.y;
+      const double Nq_ = sn[qq * S + pa_], g0 = r0.x, g1 = r0.y, g2 = r1.x, t1a = r1.y;
+      const double Mb = sn[qq * S + pb_], h0 = c0.x, h1 = c0.y, h2 = c1.x, t1b = c1.y;
       const double cf = f0.x, cB = f0.y, cm = f1.x, ct = f1.y, tM = f2.x;
       const double Bb = cB * Mb + cf * t1b;
       const double hf0 = cf * h0, hf1 = cf * h1, hf2 = cf * h2;
@@ -859,9 +908,9 @@ k_ns_pairs_hex(int first, int count, const int32_t *__restrict__ emap_c,
     }
   }
   // Dirichlet projection (BCLhs_3D.f90:105-118), as above
-  {
-    const unsigned ma = s_m[w][pa_];
-    const unsigned mb = s_m[w][pb_];
+  if (sl < nv) {
+    const unsigned ma = s_m[w][sl * S + pa_];
+    const unsigned mb = s_m[w][sl * S + pb_];
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
       const int r = u >> 2, cc = u & 3;
@@ -883,10 +932,11 @@ k_ns_pairs_hex(int first, int count, const int32_t *__restrict__ emap_c,
   // value read above
 #pragma unroll
   for (int rnd = 0; rnd < 8; ++rnd) {
-    const int j = rnd * 8 + (lane >> 3);
-    const double2 v = reinterpret_cast<const double2 *>(sw + j * 18)[piece];
+    const int jj = rnd * 8 + (lane >> 3);
+    if (jj / SS >= nv) continue;
+    const double2 v = reinterpret_cast<const double2 *>(sw + jj * 18)[piece];
     double2 *dst = reinterpret_cast<double2 *>(A + (size_t)kk[rnd] * 16) + piece;
-    if ((fm >> j) & 1ull) {
+    if (fst[rnd]) {
       *dst = v;
     } else {
       double2 o = old[rnd];
@@ -902,8 +952,8 @@ k_ns_pairs_hex(int first, int count, const int32_t *__restrict__ emap_c,
 // order), with its Dirichlet components zeroed
 __global__ void __launch_bounds__(256)
 k_ns_rhs_gather(int N, const int32_t *__restrict__ inc_off, const int32_t *__restrict__ inc,
-                const double *__restrict__ pts, const uint8_t *__restrict__ ibc,
-                double *__restrict__ rhs) {
+                const double *__restrict__ pts, int stride, int roff,
+                const uint8_t *__restrict__ ibc, double *__restrict__ rhs) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= N) return;
   double s[4] = {0.0, 0.0, 0.0, 0.0};
@@ -911,7 +961,7 @@ k_ns_rhs_gather(int N, const int32_t *__restrict__ inc_off, const int32_t *__res
   for (int t = inc_off[n]; t < inc_off[n + 1]; ++t) {
     const int ea = inc[t];
     const double2 *pr =
-        reinterpret_cast<const double2 *>(pts + (size_t)(ea >> 3) * kPtsStride + kPtsR +
+        reinterpret_cast<const double2 *>(pts + (size_t)(ea >> 3) * stride + roff +
                                           (ea & 7) * 4);
     const double2 r0 = pr[0], r1 = pr[1];
     const double v[4] = {r0.x, r0.y, r1.x, r1.y};
@@ -1004,26 +1054,32 @@ hipError_t launch_assemble_ns(Ctx &c, int tensor_flag, int subdomain_flag) {
   const int do_vec = (tensor_flag & OFD_ASSEMBLE_VEC) ? 1 : 0;
   const double *mus = c.mus_on ? c.d_field[OFD_MUSCALE] : nullptr;
   hipError_t e;
-  if (c.nshl == 8) {
-    hipLaunchKernelGGL(k_ns_points_hex, dim3((c.nelem + 7) / 8), dim3(64), 0, c.stream,
-                       c.nelem, c.d_ien, c.d_elm_id, c.d_cpos, c.d_noderec, mus, P,
-                       c.elm_id_set[0], c.elm_id_set[1], subdomain_flag, do_mat, do_vec,
-                       c.d_pts, c.d_cfl, c.d_negdet);
+  if (c.nshl == 8 || c.nshl == 4) {
+    const bool hex = c.nshl == 8;
+    const int epw = 64 / c.nshl;           // point kernel: elements per wavefront
+    const int epb = 64 / (c.nshl * c.nshl) * kWavesPerBlock;  // pair kernel: per block
+    hipLaunchKernelGGL((hex ? k_ns_points<8> : k_ns_points<4>),
+                       dim3((c.nelem + epw - 1) / epw), dim3(64), 0, c.stream, c.nelem,
+                       c.d_ien, c.d_elm_id, c.d_cpos, c.d_noderec, mus, P, c.elm_id_set[0],
+                       c.elm_id_set[1], subdomain_flag, do_mat, do_vec, c.d_pts, c.d_cfl,
+                       c.d_negdet);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (do_mat) {
       for (int col = 0; col + 1 < (int)c.color_off.size(); ++col) {
         const int first = c.color_off[col];
         const int count = c.color_off[col + 1] - first;
         if (count <= 0) continue;
-        hipLaunchKernelGGL(k_ns_pairs_hex, dim3((count + kWavesPerBlock - 1) / kWavesPerBlock),
-                           dim3(64 * kWavesPerBlock), 0, c.stream, first, count, c.d_emap_c,
-                           c.d_first_c, c.d_pts, c.d_ibc, P.rho, c.d_A64);
+        hipLaunchKernelGGL((hex ? k_ns_pairs<8> : k_ns_pairs<4>),
+                           dim3((count + epb - 1) / epb), dim3(64 * kWavesPerBlock), 0,
+                           c.stream, first, count, c.d_emap_c, c.d_first_c, c.d_pts, c.d_ibc,
+                           P.rho, c.d_A64);
         if ((e = hipGetLastError()) != hipSuccess) return e;
       }
     }
     if (do_vec) {
       hipLaunchKernelGGL(k_ns_rhs_gather, dim3((c.nnode + 255) / 256), dim3(256), 0, c.stream,
-                         c.nnode, c.d_inc_off, c.d_inc, c.d_pts, c.d_ibc, c.d_rhs);
+                         c.nnode, c.d_inc_off, c.d_inc, c.d_pts, pts_stride(c.nshl),
+                         hex ? PtsLayout<8>::R : PtsLayout<4>::R, c.d_ibc, c.d_rhs);
       if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     return hipSuccess;
@@ -1034,7 +1090,7 @@ hipError_t launch_assemble_ns(Ctx &c, int tensor_flag, int subdomain_flag) {
     const int count = c.color_off[col + 1] - first;
     if (count <= 0) continue;
     const int blocks = (count + kWavesPerBlock - 1) / kWavesPerBlock;
-    hipLaunchKernelGGL((c.nshl == 4 ? k_assemble_ns_el<4> : k_assemble_ns_el<6>),
+    hipLaunchKernelGGL(k_assemble_ns_el<6>,
                        dim3(blocks), dim3(64 * kWavesPerBlock), 0, c.stream, c.nnode,
                        c.nelem, c.d_ien, c.d_elm_id, c.d_color_elems, first, count, c.d_emap,
                        c.d_first, c.d_noderec, mus, c.d_ibc, P, c.elm_id_set[0],

@@ -105,11 +105,11 @@ static void free_tem_adjoint(Ctx &c) {
   c.tadj_n4 = 0;
 }
 
-// the hex NS assembly's per-element point data (1536 bytes per element),
-// allocated at the first assembly of a hex mesh
+// the NS assembly's per-element point data (1536 bytes per hex, 768 per
+// tet), allocated at the first assembly of a hex or tet mesh
 static int ensure_pts(Ctx &c) {
-  if (c.nshl != 8 || (c.d_pts && c.pts_elems == c.nelem)) return OFD_OK;
-  int rc = dalloc(c, &c.d_pts, (size_t)c.nelem * kPtsStride);
+  if ((c.nshl != 8 && c.nshl != 4) || (c.d_pts && c.pts_elems == c.nelem)) return OFD_OK;
+  int rc = dalloc(c, &c.d_pts, (size_t)c.nelem * pts_stride(c.nshl));
   c.pts_elems = rc ? 0 : c.nelem;
   return rc;
 }
@@ -791,7 +791,7 @@ int ofd_build_pattern(ofd_ctx *x, int64_t *icnt_out) {
     OFD_HIP(hipMemsetAsync(d_touched, 0, (size_t)c.icnt, c.stream));
     OFD_HIP(launch_first_touch(c, d_touched));
   }
-  if (c.nshl == 8) {
+  if (c.nshl == 8 || c.nshl == 4) {
     rc = dalloc(c, &c.d_emap_c, (size_t)E * 64);
     if (!rc) rc = dalloc(c, &c.d_first_c, (size_t)E);
     if (!rc) rc = dalloc(c, &c.d_cpos, (size_t)E);

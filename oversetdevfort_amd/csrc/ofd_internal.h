@@ -100,9 +100,9 @@ struct Ctx {
   int32_t *d_color_elems = nullptr;   // elements grouped by colour
   int32_t *d_inc_off = nullptr;       // [N+1] node -> its (element, node) entries
   int32_t *d_inc = nullptr;           // t * 8 + a: position t in colour order, node a
-  double *d_pts = nullptr;            // hex NS assembly: [pts_elems][kPtsStride] point data,
-  int pts_elems = 0;                  // by position in colour order
-  int32_t *d_emap_c = nullptr;        // hexes: emap and first-touch masks by colour
+  double *d_pts = nullptr;            // hex/tet NS assembly: point data per element,
+  int pts_elems = 0;                  // by position in colour order (PtsLayout)
+  int32_t *d_emap_c = nullptr;        // hexes, tets: emap and first-touch masks by colour
   uint64_t *d_first_c = nullptr;      // position, and each element's position
   int32_t *d_cpos = nullptr;
   uint64_t *d_first = nullptr;        // [E] bit a + 8 b: the element is the first (in
@@ -309,11 +309,16 @@ hipError_t launch_norm2_v(Ctx &c, const double *v);
 hipError_t launch_apply_dirichlet(Ctx &c);
 hipError_t launch_copy(Ctx &c, const void *src, void *dst, size_t nbytes);
 constexpr int kReadBlocks = 8192;  // blocks of the read-only stream kernel
-// doubles of hex NS point data per element: [q] (J^-1 column-major, uadv)
-// 96, [q] (cf, cB, cm, ct, tauM, 0) 48, [a] RHS contributions 32, then the
-// 8 node numbers and the subdomain flag as int32 (5 doubles); 1536 bytes
-constexpr int kPtsStride = 192;
-constexpr int kPtsF = 96, kPtsR = 144, kPtsN = 176;
+// NS point data per element of S nodes and S Gauss points (hexes, tets), in
+// doubles: [q] (J^-1 column-major, uadv) 12 S, [q] (cf, cB, cm, ct, tauM, 0)
+// 6 S, [a] RHS contributions 4 S, then the S node numbers and the subdomain
+// flag as int32; 1536 bytes per hex, 768 per tet
+template <int S>
+struct PtsLayout {
+  static constexpr int F = 12 * S, R = 18 * S, N = 22 * S;
+  static constexpr int stride = S == 8 ? 192 : 96;
+};
+constexpr int pts_stride(int nshl) { return nshl == 8 ? 192 : 96; }
 hipError_t launch_read(Ctx &c, const void *src, size_t nbytes, double *out);
 // overset fringe
 hipError_t launch_fringe(Ctx &c, int32_t src_id, int nrecv, const double *xr,

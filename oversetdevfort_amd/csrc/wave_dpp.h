@@ -40,6 +40,20 @@ __device__ __forceinline__ double wsum8(double v) {
   return v;
 }
 
+// the same over the 4 lanes 4k..4k+3: (v0+v1)+(v2+v3)
+__device__ __forceinline__ double wsum4(double v) {
+#pragma clang fp contract(off)
+  v += dpp_f64<0xB1>(v);
+  v += dpp_f64<0x4E>(v);
+  return v;
+}
+
+__device__ __forceinline__ double wmax4(double v) {
+  v = fmax(v, dpp_f64<0xB1>(v));
+  v = fmax(v, dpp_f64<0x4E>(v));
+  return v;
+}
+
 __device__ __forceinline__ double wmax8(double v) {
   v = fmax(v, dpp_f64<0xB1>(v));
   v = fmax(v, dpp_f64<0x4E>(v));
