@@ -108,9 +108,11 @@ static void free_tem_adjoint(Ctx &c) {
 // the NS assembly's per-element point data (1536 bytes per hex, 768 per
 // tet), allocated at the first assembly of a hex or tet mesh
 static int ensure_pts(Ctx &c) {
-  if ((c.nshl != 8 && c.nshl != 4) || (c.d_pts && c.pts_elems == c.nelem)) return OFD_OK;
-  int rc = dalloc(c, &c.d_pts, (size_t)c.nelem * pts_stride(c.nshl));
-  c.pts_elems = rc ? 0 : c.nelem;
+  if (c.nshl != 8 && c.nshl != 4) return OFD_OK;
+  const size_t len = (size_t)c.nelem * pts_stride(c.nshl);
+  if (c.d_pts && c.pts_len == len) return OFD_OK;
+  int rc = dalloc(c, &c.d_pts, len);
+  c.pts_len = rc ? 0 : len;
   return rc;
 }
 

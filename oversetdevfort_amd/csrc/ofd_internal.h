@@ -101,7 +101,7 @@ struct Ctx {
   int32_t *d_inc_off = nullptr;       // [N+1] node -> its (element, node) entries
   int32_t *d_inc = nullptr;           // t * 8 + a: position t in colour order, node a
   double *d_pts = nullptr;            // hex/tet NS assembly: point data per element,
-  int pts_elems = 0;                  // by position in colour order (PtsLayout)
+  size_t pts_len = 0;                 // by position in colour order (PtsLayout); doubles
   int32_t *d_emap_c = nullptr;        // hexes, tets: emap and first-touch masks by colour
   uint64_t *d_first_c = nullptr;      // position, and each element's position
   int32_t *d_cpos = nullptr;
