@@ -360,6 +360,138 @@ k_cgs_update(int N, V *__restrict__ W, const double *__restrict__ rd_dots,
   dots_chunks(W, vs, ii, live, zz, m, wr);
 }
 
+// k_cgs_dots and k_cgs_update (mode 1) for a step whose basis count
+// m = k + 1 = M is a compile-time constant: the basis loads of the sweep are
+// issued unconditionally and with no clamped repeats, all of them at kernel
+// entry up to kFlight<V> (32 FP32 nodes of 4 VGPRs; 16 FP64 nodes of 8,
+// beyond which the registers run out), the rest in exact batches after.
+// The block partials and the subtraction order are those of the runtime-m
+// kernels; with the FP32 basis the results are the same bits.
+template <typename V>
+constexpr int kFlight = sizeof(V) == 4 ? 32 : 16;
+
+// partials of <z, W_j>, J0 <= j < M, chunks of kCh, loads of W_j issued per
+// chunk (exact counts)
+template <typename V, int M, int J0>
+__device__ __forceinline__ void dots_tail(const V *__restrict__ W, size_t vs, int ii,
+                                          bool live, const double zz[4],
+                                          double *__restrict__ wr) {
+#pragma unroll
+  for (int j0 = J0; j0 < M; j0 += kCh) {
+    const int n = (M - j0 < kCh) ? (M - j0) : kCh;
+    Raw4<V> vr[kCh];
+#pragma unroll
+    for (int t = 0; t < kCh; ++t)
+      if (t < n) vr[t].load(W + vs * (j0 + t) + (size_t)ii * 4);
+    double acc[kCh];
+#pragma unroll
+    for (int t = 0; t < kCh; ++t) {
+      double v[4] = {0.0, 0.0, 0.0, 0.0};
+      if (t < n) vr[t].get(v);
+      acc[t] = (live && t < n) ? dot4(zz, v) : 0.0;
+    }
+    block_partials<kCh>(acc, n, wr, j0);
+  }
+}
+
+template <typename V, int M>
+__global__ void __launch_bounds__(256)
+k_cgs_dots_m(int N, const V *__restrict__ W, const double *__restrict__ rd,
+             double *__restrict__ wr, GmresState *__restrict__ g, int nb,
+             int32_t *hflag) {
+  constexpr int k = M - 1;
+  constexpr int M1 = M < kFlight<V> ? M : kFlight<V>;
+  if (cycle_stopped(g, k, hflag)) return;
+  const size_t vs = (size_t)N * 4;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = i < N;
+  const int ii = live ? i : 0;
+  Raw4<V> zr, vr[M1];
+  zr.load(W + vs * M + (size_t)ii * 4);
+#pragma unroll
+  for (int t = 0; t < M1; ++t) vr[t].load(W + vs * t + (size_t)ii * 4);
+  if (arnoldi_close(g, rd, nb, k - 1, hflag)) return;
+  double zz[4];
+  zr.get(zz);
+#pragma unroll
+  for (int j0 = 0; j0 < M1; j0 += kCh) {
+    const int n = (M1 - j0 < kCh) ? (M1 - j0) : kCh;
+    double acc[kCh];
+#pragma unroll
+    for (int t = 0; t < kCh; ++t) {
+      const int j = (j0 + t < M1) ? j0 + t : M1 - 1;
+      double v[4];
+      vr[j].get(v);
+      acc[t] = (live && t < n) ? dot4(zz, v) : 0.0;
+    }
+    block_partials<kCh>(acc, n, wr, j0);
+  }
+  if constexpr (M > M1) dots_tail<V, M, M1>(W, vs, ii, live, zz, wr);
+}
+
+template <typename V, int M>
+__global__ void __launch_bounds__(256)
+k_cgs_update1_m(int N, V *__restrict__ W, const double *__restrict__ rd_dots,
+                double *__restrict__ wr, GmresState *__restrict__ g, int nb) {
+  constexpr int M1 = M < kFlight<V> ? M : kFlight<V>;
+  if (g->done) return;
+  const size_t vs = (size_t)N * 4;
+  V *z = W + vs * M;
+  __shared__ double s_coef[M];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = i < N;
+  const int ii = live ? i : 0;
+  Raw4<V> zr, vr[M1];
+  zr.load(z + (size_t)ii * 4);
+#pragma unroll
+  for (int t = 0; t < M1; ++t) vr[t].load(W + vs * t + (size_t)ii * 4);
+  const int t0 = threadIdx.x;
+  const double s_own = (t0 < M) ? g->s[t0] : 0.0;
+  final_sums(rd_dots, nb, M, s_coef);
+  if (t0 < M) {
+    const double h = s_coef[t0] * s_own;
+    if (blockIdx.x == 0) {
+      g->h1[t0] = h;
+      g->h2[t0] = 0.0;
+    }
+    s_coef[t0] = h * s_own;
+  }
+  __syncthreads();
+  double zz[4];
+  zr.get(zz);
+#pragma unroll
+  for (int t = 0; t < M1; ++t) {
+    double v[4];
+    vr[t].get(v);
+    const double cj = s_coef[t];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) zz[c] -= cj * v[c];
+  }
+  if constexpr (M > M1) {
+#pragma unroll
+    for (int j0 = M1; j0 < M; j0 += kCh) {
+      const int n = (M - j0 < kCh) ? (M - j0) : kCh;
+      Raw4<V> vb[kCh];
+#pragma unroll
+      for (int t = 0; t < kCh; ++t)
+        if (t < n) vb[t].load(W + vs * (j0 + t) + (size_t)ii * 4);
+#pragma unroll
+      for (int t = 0; t < kCh; ++t) {
+        if (t < n) {
+          double v[4];
+          vb[t].get(v);
+          const double cj = s_coef[j0 + t];
+#pragma unroll
+          for (int c = 0; c < 4; ++c) zz[c] -= cj * v[c];
+        }
+      }
+    }
+  }
+  if (live) st4(z + (size_t)i * 4, zz);
+  double acc[1] = {live ? dot4(zz, zz) : 0.0};
+  block_partials<1>(acc, 1, wr, 0);
+}
+
 // closes the last Arnoldi step of a cycle (one block)
 __global__ void __launch_bounds__(256)
 k_gmres_close(const double *__restrict__ part, int nb, GmresState *__restrict__ g, int k) {
@@ -565,8 +697,47 @@ hipError_t launch_mgs_step(Ctx &c, int k, int j) {
   return hipGetLastError();
 }
 
+// basis counts m = k + 1 up to kExactM take the exact-count kernels
+constexpr int kExactM = 32;
+
+#define OFD_M8(X, a) X(a + 1) X(a + 2) X(a + 3) X(a + 4) X(a + 5) X(a + 6) X(a + 7) X(a + 8)
+#define OFD_M32(X) OFD_M8(X, 0) OFD_M8(X, 8) OFD_M8(X, 16) OFD_M8(X, 24)
+
+template <typename V>
+static void launch_dots_m(Ctx &c, int m, int nb, const V *W, const double *rd, double *wr) {
+  switch (m) {
+#define OFD_DM(M)                                                                            \
+  case M:                                                                                  \
+    hipLaunchKernelGGL((k_cgs_dots_m<V, M>), dim3(nb), dim3(kRedBlock), 0, c.stream, c.gm_n, \
+                       W, rd, wr, c.d_gs, nb, c.cur_flags);                                  \
+    break;
+    OFD_M32(OFD_DM)
+#undef OFD_DM
+  }
+}
+
+template <typename V>
+static void launch_update1_m(Ctx &c, int m, int nb, V *W, const double *rd, double *wr) {
+  switch (m) {
+#define OFD_UM(M)                                                                           \
+  case M:                                                                                 \
+    hipLaunchKernelGGL((k_cgs_update1_m<V, M>), dim3(nb), dim3(kRedBlock), 0, c.stream,     \
+                       c.gm_n, W, rd, wr, c.d_gs, nb);                                      \
+    break;
+    OFD_M32(OFD_UM)
+#undef OFD_UM
+  }
+}
+
 hipError_t launch_cgs_dots(Ctx &c, int k) {
   const int nb = nblocks_gm(c);
+  if (k + 1 <= kExactM) {
+    if (c.basis32)
+      launch_dots_m<float>(c, k + 1, nb, c.d_W32, part_buf(c, k & 1), part_buf(c, 2));
+    else
+      launch_dots_m<double>(c, k + 1, nb, c.d_W, part_buf(c, k & 1), part_buf(c, 2));
+    return hipGetLastError();
+  }
   if (c.basis32)
     hipLaunchKernelGGL(k_cgs_dots<float>, dim3(nb), dim3(kRedBlock), 0, c.stream, c.gm_n,
                        c.d_W32, part_buf(c, k & 1), part_buf(c, 2), c.d_gs, k, nb,
@@ -582,7 +753,12 @@ hipError_t launch_cgs_update(Ctx &c, int k, int mode) {
   const int nb = nblocks_gm(c);
   const double *rd_dots = part_buf(c, mode == 3 ? 3 : 2);
   double *wr = (mode == 2) ? part_buf(c, 3) : part_buf(c, (k + 1) & 1);
-  if (c.basis32)
+  if (mode == 1 && k + 1 <= kExactM) {
+    if (c.basis32)
+      launch_update1_m<float>(c, k + 1, nb, c.d_W32, rd_dots, wr);
+    else
+      launch_update1_m<double>(c, k + 1, nb, c.d_W, rd_dots, wr);
+  } else if (c.basis32)
     hipLaunchKernelGGL(k_cgs_update<float>, dim3(nb), dim3(kRedBlock), 0, c.stream,
                        c.gm_n, c.d_W32, rd_dots, wr, c.d_gs, k, mode, nb);
   else
