@@ -53,6 +53,14 @@ enum {
   OFD_ORTHO_CGS1 = 2  /* classical GS, one projection (restart cycles are short) */
 };
 
+/* OR'ed into the ortho argument of ofd_solve_nsvof with OFD_ORTHO_CGS1 or
+ * OFD_ORTHO_CGS2: the same block-Jacobi preconditioner
+ * applied on the right, GMRES on A P^-1 (x = P^-1 y), so each cycle
+ * minimises the true residual.  No counterpart in the reference, which
+ * applies it on the left (SparseGMRES_NS_conv.f:53-150); on the quench tank
+ * from 32^3 nodes up the left form stagnates well above 1e-8. */
+enum { OFD_PREC_RIGHT = 16 };
+
 /* node fields held by a context: (NNODE,3) vectors from OFD_XG to OFD_ACGMOLD,
  * OFD_SOLU and OFD_LAMU; (NNODE) scalars otherwise.  The quenching fields
  * (VOF phig, its rate rphig, temperature Tg, its rate rTg and the T
@@ -284,7 +292,8 @@ int ofd_gmres_nsvof(ofd_ctx *ctx, int precision, int ortho, double tol, int kspa
                     int kspace_min, double *solu, double *solp, double *solls, int *iters,
                     double *resid);
 
-/* restarted SparseGMRES_ns_conv with an FP64 outer residual, as ofd_solve */
+/* restarted SparseGMRES_ns_conv with an FP64 outer residual, as ofd_solve;
+ * ortho may carry OFD_PREC_RIGHT */
 int ofd_solve_nsvof(ofd_ctx *ctx, int precision, int ortho, double rtol, double inner_tol,
                     int kspace, int max_cycles, int *iters, int *cycles, double *relres);
 

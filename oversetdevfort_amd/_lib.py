@@ -28,6 +28,8 @@ OFD_FP32 = 1
 OFD_ORTHO_MGS = 0
 OFD_ORTHO_CGS2 = 1
 OFD_ORTHO_CGS1 = 2
+# OR'ed into ortho of ofd_solve_nsvof: block Jacobi on the right
+OFD_PREC_RIGHT = 16
 OFD_TIME_ASM_NSVOF = 0
 OFD_TIME_ASM_TEM = 1
 OFD_TIME_SPMV_NSVOF = 2

@@ -142,9 +142,7 @@ __device__ __forceinline__ void axpy_batches(const V *__restrict__ W, size_t vs,
 
 }  // namespace
 
-// W_0 = P^-1 b and partials of |W_0|^2 (SparseGMRES_NS.f90:124-155); block 0
-// also resets the cycle scalars.  The first orthogonalisation kernel of
-// iteration 0 closes the norm.
+// z = P^-1 r for 4-tuple i of a system vector.
 // KIND 0: NS node vectors, P = [N][16] block-Jacobi inverse;
 // KIND 1: the T system, P = inverse diagonal of the nsc scalar unknowns;
 // KIND 2: the NSVOF system, [N][4] (u, p) node values then [N] phi (nsc = N);
@@ -152,11 +150,52 @@ __device__ __forceinline__ void axpy_batches(const V *__restrict__ W, size_t vs,
 //         [N] 1/LS.
 // The scalar values of kind 1 and the phi values of kind 2 are taken 4 at a
 // time; entries past the last node are 0.
+template <int KIND>
+__device__ __forceinline__ void apply_prec(int i, int nsc, const double *__restrict__ P,
+                                           const double r[4], double z[4]) {
+  if (KIND == 1) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int t = 4 * i + c;
+      z[c] = (t < nsc) ? P[t] * r[c] : 0.0;
+    }
+  } else if (KIND == 2 && i >= nsc) {
+    // phi entries: tuples nsc.. hold phi of nodes 4 (i - nsc) + c, scaled
+    // by 1/LS (P + 16 nsc), zero past the last node
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int t = 4 * (i - nsc) + c;
+      z[c] = (t < nsc) ? P[(size_t)16 * nsc + t] * r[c] : 0.0;
+    }
+  } else {
+    const double *p = P + (size_t)i * 16;
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      z[c] = p[c * 4 + 0] * r[0] + p[c * 4 + 1] * r[1] + p[c * 4 + 2] * r[2] +
+             p[c * 4 + 3] * r[3];
+  }
+}
+
+// right form: T = P^-1 z for tuple i of the NSVOF system, z as stored in
+// the basis (the input of the next Arnoldi product)
+template <typename V>
+__device__ __forceinline__ void store_right(int i, int nsc, const double *__restrict__ P,
+                                            const double z[4], V *__restrict__ T) {
+  double t[4];
+  apply_prec<2>(i, nsc, P, z, t);
+  st4(T + (size_t)i * 4, t);
+}
+
+// Left form: W_0 = P^-1 b and partials of |W_0|^2 (SparseGMRES_NS.f90:
+// 124-155).  Right form (T given, NSVOF only): W_0 = b, T = P^-1 b, the
+// input of the first Arnoldi product.  Block 0 also resets the cycle
+// scalars.  The first orthogonalisation kernel of iteration 0 closes the
+// norm.
 template <typename V, int KIND>
 __global__ void __launch_bounds__(256)
 k_gmres_init(int N, int nsc, const double *__restrict__ P, const double *__restrict__ b,
-             V *__restrict__ W0, double *__restrict__ part, GmresState *__restrict__ g,
-             double tol, int kspace, int kmin) {
+             V *__restrict__ W0, V *__restrict__ T, double *__restrict__ part,
+             GmresState *__restrict__ g, double tol, int kspace, int kmin) {
   if (g->outer_done) {
     // a cycle queued behind the one that met the outer tolerance
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -181,29 +220,16 @@ k_gmres_init(int N, int nsc, const double *__restrict__ P, const double *__restr
        i += gridDim.x * blockDim.x) {
     double r[4], z[4];
     ld4(b + (size_t)i * 4, r);
-    if (KIND == 1) {
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const int t = 4 * i + c;
-        z[c] = (t < nsc) ? P[t] * r[c] : 0.0;
-      }
-    } else if (KIND == 2 && i >= nsc) {
-      // phi entries: tuples nsc.. hold phi of nodes 4 (i - nsc) + c, scaled
-      // by 1/LS (P + 16 nsc), zero past the last node
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const int t = 4 * (i - nsc) + c;
-        z[c] = (t < nsc) ? P[(size_t)16 * nsc + t] * r[c] : 0.0;
-      }
+    if (T) {
+      st4(W0 + (size_t)i * 4, r);
+      apply_prec<KIND>(i, nsc, P, r, z);
+      st4(T + (size_t)i * 4, z);
+      acc[0] += dot4(r, r);
     } else {
-      const double *p = P + (size_t)i * 16;
-#pragma unroll
-      for (int c = 0; c < 4; ++c)
-        z[c] = p[c * 4 + 0] * r[0] + p[c * 4 + 1] * r[1] + p[c * 4 + 2] * r[2] +
-               p[c * 4 + 3] * r[3];
+      apply_prec<KIND>(i, nsc, P, r, z);
+      st4(W0 + (size_t)i * 4, z);
+      acc[0] += dot4(z, z);
     }
-    st4(W0 + (size_t)i * 4, z);
-    acc[0] += dot4(z, z);
   }
   block_partials<1>(acc, 1, part, 0);
 }
@@ -308,7 +334,7 @@ template <typename V>
 __global__ void __launch_bounds__(256)
 k_cgs_update(int N, V *__restrict__ W, const double *__restrict__ rd_dots,
              double *__restrict__ wr, GmresState *__restrict__ g, int k, int mode,
-             int nb) {
+             int nb, const double *__restrict__ P, int nsc, V *__restrict__ T) {
   if (g->done) return;
   const size_t vs = (size_t)N * 4;
   V *z = W + vs * (k + 1);
@@ -353,6 +379,7 @@ k_cgs_update(int N, V *__restrict__ W, const double *__restrict__ rd_dots,
   axpy_batches(W, vs, ii, s_coef, m, zz, 8);
   if (live) st4(z + (size_t)i * 4, zz);
   if (mode != 2) {
+    if (T && live) store_right(i, nsc, P, zz, T);
     double acc[1] = {live ? dot4(zz, zz) : 0.0};
     block_partials<1>(acc, 1, wr, 0);
     return;
@@ -432,7 +459,8 @@ k_cgs_dots_m(int N, const V *__restrict__ W, const double *__restrict__ rd,
 template <typename V, int M>
 __global__ void __launch_bounds__(256)
 k_cgs_update1_m(int N, V *__restrict__ W, const double *__restrict__ rd_dots,
-                double *__restrict__ wr, GmresState *__restrict__ g, int nb) {
+                double *__restrict__ wr, GmresState *__restrict__ g, int nb,
+                const double *__restrict__ P, int nsc, V *__restrict__ T) {
   constexpr int M1 = M < kFlight<V> ? M : kFlight<V>;
   if (g->done) return;
   const size_t vs = (size_t)N * 4;
@@ -488,6 +516,7 @@ k_cgs_update1_m(int N, V *__restrict__ W, const double *__restrict__ rd_dots,
     }
   }
   if (live) st4(z + (size_t)i * 4, zz);
+  if (T && live) store_right(i, nsc, P, zz, T);
   double acc[1] = {live ? dot4(zz, zz) : 0.0};
   block_partials<1>(acc, 1, wr, 0);
 }
@@ -581,10 +610,11 @@ __global__ void k_outer_reset(GmresState *__restrict__ g, double thr) {
 }
 
 // back substitution (SparseGMRES_NS.f90:341-346) and x += sum y_j v_j (:349-352)
+// Right form (P given, NSVOF only): x += P^-1 (sum y_j v_j).
 template <typename V>
 __global__ void __launch_bounds__(256)
 k_gmres_finish(int N, const V *__restrict__ W, const GmresState *__restrict__ g,
-               double *__restrict__ x) {
+               double *__restrict__ x, const double *__restrict__ P, int nsc) {
   if (g->outer_done) return;  // queued behind the cycle that met the tolerance
   __shared__ double s_y[kMaxKspace];
   __shared__ double s_H[kMaxKspace * kMaxKspace];  // upper triangle, column-major it x it
@@ -611,8 +641,12 @@ k_gmres_finish(int N, const V *__restrict__ W, const GmresState *__restrict__ g,
   const size_t vs = (size_t)N * 4;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N;
        i += gridDim.x * blockDim.x) {
-    double xx[4];
-    ld4(x + (size_t)i * 4, xx);
+    double xx[4], x0[4];
+    ld4(x + (size_t)i * 4, x0);
+    // the left form adds the terms to x itself; the right form sums them
+    // from zero and adds P^-1 of the sum
+#pragma unroll
+    for (int c = 0; c < 4; ++c) xx[c] = P ? 0.0 : x0[c];
     int j = 0;
     for (; j + 4 <= it; j += 4) {
       double v0[4], v1[4], v2[4], v3[4];
@@ -633,6 +667,12 @@ k_gmres_finish(int N, const V *__restrict__ W, const GmresState *__restrict__ g,
       ld4(W + vs * j + (size_t)i * 4, v);
 #pragma unroll
       for (int c = 0; c < 4; ++c) xx[c] += s_y[j] * v[c];
+    }
+    if (P) {
+      double d[4];
+      apply_prec<2>(i, nsc, P, xx, d);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) xx[c] = x0[c] + d[c];
     }
     st4(x + (size_t)i * 4, xx);
   }
@@ -665,23 +705,23 @@ hipError_t launch_gmres_init(Ctx &c, const double *b, double tol, int kspace, in
   const int kind = c.gm_kind;
   const double *P = kind == 1 ? c.d_PT : (kind == 2 ? c.d_PV : c.d_P64);
   const int nsc = c.nnode;
-#define OFD_INIT(V, K, W)                                                                   \
+#define OFD_INIT(V, K, W, T)                                                                \
   hipLaunchKernelGGL((k_gmres_init<V, K>), dim3(nb), dim3(kRedBlock), 0, c.stream, c.gm_n, \
-                     nsc, P, b, W, part_buf(c, 0), c.d_gs, tol, kspace, kmin)
+                     nsc, P, b, W, T, part_buf(c, 0), c.d_gs, tol, kspace, kmin)
   if (c.basis32) {
     if (kind == 1)
-      OFD_INIT(float, 1, c.d_W32);
+      OFD_INIT(float, 1, c.d_W32, (float *)nullptr);
     else if (kind == 2)
-      OFD_INIT(float, 2, c.d_W32);
+      OFD_INIT(float, 2, c.d_W32, c.right ? c.d_T32 : nullptr);
     else
-      OFD_INIT(float, 0, c.d_W32);
+      OFD_INIT(float, 0, c.d_W32, (float *)nullptr);
   } else {
     if (kind == 1)
-      OFD_INIT(double, 1, c.d_W);
+      OFD_INIT(double, 1, c.d_W, (double *)nullptr);
     else if (kind == 2)
-      OFD_INIT(double, 2, c.d_W);
+      OFD_INIT(double, 2, c.d_W, c.right ? c.d_T : nullptr);
     else
-      OFD_INIT(double, 0, c.d_W);
+      OFD_INIT(double, 0, c.d_W, (double *)nullptr);
   }
 #undef OFD_INIT
   c.part_cur = 0;
@@ -717,12 +757,14 @@ static void launch_dots_m(Ctx &c, int m, int nb, const V *W, const double *rd, d
 }
 
 template <typename V>
-static void launch_update1_m(Ctx &c, int m, int nb, V *W, const double *rd, double *wr) {
+static void launch_update1_m(Ctx &c, int m, int nb, V *W, const double *rd, double *wr,
+                             V *T) {
+  const double *P = T ? c.d_PV : nullptr;
   switch (m) {
 #define OFD_UM(M)                                                                           \
   case M:                                                                                 \
     hipLaunchKernelGGL((k_cgs_update1_m<V, M>), dim3(nb), dim3(kRedBlock), 0, c.stream,     \
-                       c.gm_n, W, rd, wr, c.d_gs, nb);                                      \
+                       c.gm_n, W, rd, wr, c.d_gs, nb, P, c.nnode, T);                       \
     break;
     OFD_M32(OFD_UM)
 #undef OFD_UM
@@ -753,17 +795,21 @@ hipError_t launch_cgs_update(Ctx &c, int k, int mode) {
   const int nb = nblocks_gm(c);
   const double *rd_dots = part_buf(c, mode == 3 ? 3 : 2);
   double *wr = (mode == 2) ? part_buf(c, 3) : part_buf(c, (k + 1) & 1);
+  // right form: the sweep that leaves W_{k+1} also writes T = P^-1 W_{k+1}
+  const double *P = c.right ? c.d_PV : nullptr;
+  float *T32 = c.right ? c.d_T32 : nullptr;
+  double *T64 = c.right ? c.d_T : nullptr;
   if (mode == 1 && k + 1 <= kExactM) {
     if (c.basis32)
-      launch_update1_m<float>(c, k + 1, nb, c.d_W32, rd_dots, wr);
+      launch_update1_m<float>(c, k + 1, nb, c.d_W32, rd_dots, wr, T32);
     else
-      launch_update1_m<double>(c, k + 1, nb, c.d_W, rd_dots, wr);
+      launch_update1_m<double>(c, k + 1, nb, c.d_W, rd_dots, wr, T64);
   } else if (c.basis32)
     hipLaunchKernelGGL(k_cgs_update<float>, dim3(nb), dim3(kRedBlock), 0, c.stream,
-                       c.gm_n, c.d_W32, rd_dots, wr, c.d_gs, k, mode, nb);
+                       c.gm_n, c.d_W32, rd_dots, wr, c.d_gs, k, mode, nb, P, c.nnode, T32);
   else
     hipLaunchKernelGGL(k_cgs_update<double>, dim3(nb), dim3(kRedBlock), 0, c.stream,
-                       c.gm_n, c.d_W, rd_dots, wr, c.d_gs, k, mode, nb);
+                       c.gm_n, c.d_W, rd_dots, wr, c.d_gs, k, mode, nb, P, c.nnode, T64);
   c.part_cur = (k + 1) & 1;
   return hipGetLastError();
 }
@@ -776,12 +822,13 @@ hipError_t launch_gmres_close(Ctx &c, int k, int ortho) {
 }
 
 hipError_t launch_gmres_finish(Ctx &c, double *x) {
+  const double *P = c.right ? c.d_PV : nullptr;
   if (c.basis32)
     hipLaunchKernelGGL(k_gmres_finish<float>, dim3(nblocks_gm(c)), dim3(kRedBlock), 0,
-                       c.stream, c.gm_n, c.d_W32, c.d_gs, x);
+                       c.stream, c.gm_n, c.d_W32, c.d_gs, x, P, c.nnode);
   else
     hipLaunchKernelGGL(k_gmres_finish<double>, dim3(nblocks_gm(c)), dim3(kRedBlock), 0,
-                       c.stream, c.gm_n, c.d_W, c.d_gs, x);
+                       c.stream, c.gm_n, c.d_W, c.d_gs, x, P, c.nnode);
   return hipGetLastError();
 }
 

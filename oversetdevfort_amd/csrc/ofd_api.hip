@@ -302,10 +302,32 @@ struct NsvofSystem {
   }
 };
 
-// the NSVOF preconditioner: block inverses and P^-1 A (FP64, FP32)
-static int ensure_prec_v(Ctx &c, int precision) {
+// FP32 copy of the NSVOF matrix
+static int ensure_aV32(Ctx &c) {
+  OFD_CHECK(c.matV_valid, OFD_ERR_ARG, "no assembled NSVOF matrix");
+  if (c.aV32_valid) return OFD_OK;
+  if (!c.d_AV32) {
+    int rc = dalloc(c, &c.d_AV32, (size_t)c.icnt * 24);
+    if (rc) return rc;
+  }
+  OFD_HIP(launch_convert_aV32(c));
+  c.aV32_valid = true;
+  return OFD_OK;
+}
+
+// the NSVOF preconditioner: block inverses and P^-1 A (FP64, FP32); for the
+// right form the block inverses and the plain matrix (FP32 copy for the
+// mixed solver)
+static int ensure_prec_v(Ctx &c, int precision, bool right) {
   OFD_CHECK(c.matV_valid, OFD_ERR_ARG, "no assembled NSVOF matrix");
   const bool want32 = precision == OFD_FP32;
+  if (right) {
+    if (!c.pV_valid) {
+      OFD_HIP(launch_build_prec_v(c, false, false));
+      c.pV_valid = true;
+    }
+    return want32 ? ensure_aV32(c) : OFD_OK;
+  }
   if (c.precV_valid && (!want32 || c.apV32_valid)) return OFD_OK;
   if (!c.d_ApV) {
     int rc = dalloc(c, &c.d_ApV, (size_t)c.icnt * 24);
@@ -315,8 +337,8 @@ static int ensure_prec_v(Ctx &c, int precision) {
     int rc = dalloc(c, &c.d_ApV32, (size_t)c.icnt * 24);
     if (rc) return rc;
   }
-  OFD_HIP(launch_build_prec_v(c, want32));
-  c.precV_valid = true;
+  OFD_HIP(launch_build_prec_v(c, want32, true));
+  c.precV_valid = c.pV_valid = true;
   c.apV32_valid = want32;
   return OFD_OK;
 }
@@ -326,12 +348,20 @@ static int ensure_prec_v(Ctx &c, int precision) {
 static int gmres_cycle(Ctx &c, int precision, int ortho, const double *b, double tol,
                        int K, int kmin, double *x, int *iters, double *resid,
                        double *rnorm0, bool sync = true, int flag_set = 0) {
+  const bool right = (ortho & OFD_PREC_RIGHT) != 0;
+  ortho &= ~OFD_PREC_RIGHT;
+  OFD_CHECK(ortho == OFD_ORTHO_MGS || ortho == OFD_ORTHO_CGS2 || ortho == OFD_ORTHO_CGS1,
+            OFD_ERR_ARG, "bad ortho");
+  OFD_CHECK(!right || (c.gm_kind == 2 && ortho != OFD_ORTHO_MGS), OFD_ERR_ARG,
+            "OFD_PREC_RIGHT applies to the NSVOF system with OFD_ORTHO_CGS1 or CGS2");
+  c.right = right;
   // the mixed CGS solver keeps its Krylov basis in FP32
   c.basis32 = precision == OFD_FP32 && ortho != OFD_ORTHO_MGS;
   int rc = ensure_krylov(c, K);
   if (rc) return rc;
   rc = c.gm_kind == 1 ? ensure_prec_s(c, precision)
-                      : (c.gm_kind == 2 ? ensure_prec_v(c, precision) : ensure_prec(c, precision));
+                      : (c.gm_kind == 2 ? ensure_prec_v(c, precision, right)
+                                        : ensure_prec(c, precision));
   if (rc) return rc;
   OFD_HIP(launch_gmres_init(c, b, tol, K, kmin));
   // The host enqueues iterations ahead of the device and learns about
@@ -531,6 +561,8 @@ int ofd_ctx_destroy(ofd_ctx *x) {
   dfree(c.d_PV);
   dfree(c.d_ApV);
   dfree(c.d_ApV32);
+  dfree(c.d_T);
+  dfree(c.d_T32);
   dfree(c.d_rhsV);
   dfree(c.d_xV);
   dfree(c.d_rV);
@@ -586,7 +618,7 @@ int ofd_upload_mesh(ofd_ctx *x, int nnode, int nelem, int nshl, const int32_t *i
   c.nshl = nshl;
   c.gm_n = nnode;
   c.gm_kind = 0;
-  c.matV_valid = c.precV_valid = c.aV32_valid = c.apV32_valid = false;
+  c.matV_valid = c.precV_valid = c.pV_valid = c.aV32_valid = c.apV32_valid = false;
   c.nsvV_n4 = 0;
   c.matT_valid = c.precT_valid = c.aT32_valid = c.apT32_valid = false;
   c.temT_n4 = 0;
@@ -1432,7 +1464,6 @@ int ofd_time_assemble(ofd_ctx *x, int tensor_flag, int reps, double *ms) {
 static int ensure_tem(Ctx &c);
 static int ensure_aT32(Ctx &c);
 static int ensure_nsvof(Ctx &c);
-static int ensure_aV32(Ctx &c);
 
 int ofd_time_quench(ofd_ctx *x, int what, int arg, int reps, double *ms) {
   OFD_CHECK(x && ms && reps > 0, OFD_ERR_ARG, "bad argument");
@@ -1489,7 +1520,7 @@ int ofd_time_quench(ofd_ctx *x, int what, int arg, int reps, double *ms) {
   if (!spmv && (arg & OFD_ASSEMBLE_MAT)) {
     if (nsvof) {
       c.matV_valid = true;
-      c.precV_valid = c.aV32_valid = c.apV32_valid = false;
+      c.precV_valid = c.pV_valid = c.aV32_valid = c.apV32_valid = false;
     } else {
       c.matT_valid = true;
       c.precT_valid = c.aT32_valid = c.apT32_valid = false;
@@ -1731,7 +1762,7 @@ static int ensure_nsvof(Ctx &c) {
     if (rc) return rc;
     OFD_HIP(hipMemsetAsync(c.d_AV, 0, (size_t)c.icnt * 24 * sizeof(double), c.stream));
     c.nsvV_icnt = c.icnt;
-    c.matV_valid = c.precV_valid = c.aV32_valid = c.apV32_valid = false;
+    c.matV_valid = c.precV_valid = c.pV_valid = c.aV32_valid = c.apV32_valid = false;
   }
   if (c.nsvV_n4 != n4) {
     const size_t np = (size_t)n4 * 4;
@@ -1740,6 +1771,8 @@ static int ensure_nsvof(Ctx &c) {
     if (!rc) rc = dalloc(c, &c.d_rV, np);
     if (!rc) rc = dalloc(c, &c.d_yV, np);
     if (!rc) rc = dalloc(c, &c.d_PV, (size_t)c.nnode * 17);
+    if (!rc) rc = dalloc(c, &c.d_T, np);
+    if (!rc) rc = dalloc(c, &c.d_T32, np);
     if (rc) return rc;
     OFD_HIP(hipMemsetAsync(c.d_rhsV, 0, np * sizeof(double), c.stream));
     OFD_HIP(hipMemsetAsync(c.d_xV, 0, np * sizeof(double), c.stream));
@@ -1747,18 +1780,6 @@ static int ensure_nsvof(Ctx &c) {
     OFD_HIP(hipMemsetAsync(c.d_yV, 0, np * sizeof(double), c.stream));
     c.nsvV_n4 = n4;
   }
-  return OFD_OK;
-}
-
-static int ensure_aV32(Ctx &c) {
-  OFD_CHECK(c.matV_valid, OFD_ERR_ARG, "no assembled NSVOF matrix");
-  if (c.aV32_valid) return OFD_OK;
-  if (!c.d_AV32) {
-    int rc = dalloc(c, &c.d_AV32, (size_t)c.icnt * 24);
-    if (rc) return rc;
-  }
-  OFD_HIP(launch_convert_aV32(c));
-  c.aV32_valid = true;
   return OFD_OK;
 }
 
@@ -1819,7 +1840,7 @@ int ofd_assemble_nsvof(ofd_ctx *x, int tensor_flag, double *cfl) {
   if (tensor_flag & OFD_ASSEMBLE_MAT) {
     // no zeroing: the first element to reach a nonzero stores into it
     c.matV_valid = true;
-    c.precV_valid = c.aV32_valid = c.apV32_valid = false;
+    c.precV_valid = c.pV_valid = c.aV32_valid = c.apV32_valid = false;
   }
   OFD_HIP(hipMemsetAsync(c.d_negdet, 0, sizeof(int32_t), c.stream));
   for (int col = 0; col + 1 < (int)c.color_off.size(); ++col)
@@ -1898,7 +1919,7 @@ int ofd_set_nsvof_system(ofd_ctx *x, const double *k11, const double *g, const d
     rc = h2d(c, c.d_AV, A.data(), A.size() * sizeof(double));
     if (rc) return rc;
     c.matV_valid = true;
-    c.precV_valid = c.aV32_valid = c.apV32_valid = false;
+    c.precV_valid = c.pV_valid = c.aV32_valid = c.apV32_valid = false;
   }
   if (rhsu) {
     OFD_CHECK(rhsp && rhsls, OFD_ERR_ARG, "the right-hand side needs all three arrays");

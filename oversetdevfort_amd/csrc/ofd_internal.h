@@ -179,6 +179,12 @@ struct Ctx {
   double *d_ApV = nullptr;            // P^-1 A, same two-array layout
   float *d_ApV32 = nullptr;
   bool matV_valid = false, precV_valid = false, aV32_valid = false, apV32_valid = false;
+  bool pV_valid = false;              // the NSVOF block inverses alone (right form)
+  // right-preconditioned NSVOF cycle: the Arnoldi product reads
+  // T = P^-1 W_k (d_T, or d_T32 with the FP32 basis) and the plain matrix
+  bool right = false;
+  double *d_T = nullptr;
+  float *d_T32 = nullptr;
   double *d_rhsV = nullptr, *d_xV = nullptr, *d_rV = nullptr, *d_yV = nullptr;
   int64_t nsvV_icnt = 0;
   int nsvV_n4 = 0;
@@ -351,7 +357,7 @@ hipError_t launch_convert_aT32(Ctx &c);
 hipError_t launch_assemble_nsvof_hex(Ctx &c, int tensor_flag, int color);
 hipError_t launch_spmv_v(Ctx &c, int precision, const double *x, double *y,
                          const double *b, int mode);
-hipError_t launch_build_prec_v(Ctx &c, bool want32);
+hipError_t launch_build_prec_v(Ctx &c, bool want32, bool ap);
 hipError_t launch_convert_aV32(Ctx &c);
 
 int nblocks_nodes(const Ctx &c);

@@ -513,15 +513,16 @@ hipError_t launch_spmv_v(Ctx &c, int precision, const double *x, double *y,
   return hipGetLastError();
 }
 
-// P (the 4x4 block inverses, [N][16]) and pphi ([N], after it in d_PV); the
-// preconditioned copy Ap = P^-1 A in the same layout
-hipError_t launch_build_prec_v(Ctx &c, bool want32) {
+// P (the 4x4 block inverses, [N][16]) and pphi ([N], after it in d_PV); with
+// `ap` the preconditioned copy Ap = P^-1 A in the same layout
+hipError_t launch_build_prec_v(Ctx &c, bool want32, bool ap) {
   const int N = c.nnode;
   double *P = c.d_PV, *pphi = c.d_PV + (size_t)N * 16;
   hipLaunchKernelGGL(k_build_prec, dim3((N + 255) / 256), dim3(256), 0, c.stream, N, c.d_diag,
                      c.d_AV, 24, P);
   hipLaunchKernelGGL(k_build_prec_phi, dim3((N + 255) / 256), dim3(256), 0, c.stream, N,
                      c.d_diag, c.d_AV, pphi);
+  if (!ap) return hipGetLastError();
   const unsigned nb = (unsigned)(((size_t)N * 4 + 255) / 256);
   hipLaunchKernelGGL(k_precondition_matrix, dim3(nb), dim3(256), 0, c.stream, N, c.d_col, P,
                      c.d_AV, 24, c.d_ApV, want32 ? c.d_ApV32 : nullptr);
@@ -536,10 +537,19 @@ hipError_t launch_convert_aV32(Ctx &c) {
   return hipGetLastError();
 }
 
-// Arnoldi product of the NSVOF system: W_{k+1} = (P^-1 A) W_k
+// Arnoldi product of the NSVOF system: W_{k+1} = (P^-1 A) W_k, or A P^-1 W_k
 static hipError_t launch_spmv_arnoldi_v(Ctx &c, int precision, int k, hipEvent_t ev0,
                                         hipEvent_t ev1) {
   const size_t vs = (size_t)c.gm_n * 4;
+  if (c.right) {
+    // right form: W_{k+1} = A T with T = P^-1 W_k from the previous sweep
+    if (c.basis32)
+      launch_v<float, float, 2>(c, c.d_AV32, c.d_T32, c.d_W32 + vs * (k + 1), nullptr, ev0,
+                                ev1);
+    else
+      launch_v<double, double, 2>(c, c.d_AV, c.d_T, c.d_W + vs * (k + 1), nullptr, ev0, ev1);
+    return hipGetLastError();
+  }
   if (c.basis32)
     launch_v<float, float, 2>(c, c.d_ApV32, c.d_W32 + vs * k, c.d_W32 + vs * (k + 1), nullptr,
                               ev0, ev1);

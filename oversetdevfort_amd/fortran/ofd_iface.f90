@@ -16,6 +16,7 @@ module ofd_iface
   integer(c_int), parameter :: OFD_ASSEMBLE_VEC = 2, OFD_ASSEMBLE_MAT = 4
   integer(c_int), parameter :: OFD_FP64 = 0, OFD_FP32 = 1
   integer(c_int), parameter :: OFD_ORTHO_MGS = 0, OFD_ORTHO_CGS2 = 1, OFD_ORTHO_CGS1 = 2
+  integer(c_int), parameter :: OFD_PREC_RIGHT = 16  ! ior() into ortho of ofd_solve_nsvof
   integer(c_int), parameter :: OFD_XG = 0, OFD_DG = 1, OFD_DGOLD = 2, &
                                OFD_UG = 3, OFD_UGOLD = 4, OFD_UGM = 5, &
                                OFD_UGMOLD = 6, OFD_ACG = 7, OFD_ACGOLD = 8, &

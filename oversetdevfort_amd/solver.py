@@ -368,6 +368,10 @@ class Context:
 
     def solve_nsvof(self, rtol=1e-8, precision=L.OFD_FP64, ortho=L.OFD_ORTHO_CGS1,
                     inner_tol=None, kspace=30, max_cycles=50, raise_noconv=True):
+        """Restarted SparseGMRES_ns_conv with an FP64 outer residual.  ``ortho``
+        may carry L.OFD_PREC_RIGHT (with CGS1 or CGS2): the block-Jacobi
+        preconditioner on the right, so each cycle minimises the true
+        residual."""
         if inner_tol is None:
             inner_tol = INNER_TOL
         it = ctypes.c_int()
@@ -669,7 +673,7 @@ def newton_step(ctx: Context, rtol=1e-8, precision=L.OFD_FP64, ortho=L.OFD_ORTHO
 
 def quench_step(ctx: Context, nnewton=3, rtol=1e-8, precision=L.OFD_FP64,
                  ortho=L.OFD_ORTHO_CGS1, kspace=30, max_cycles=200, raise_noconv=True,
-                 ntol=None):
+                 ntol=None, nsvof_right=False):
     """One time step of the quenching solver, solmultiphasethermofluid_stag
     (solveFlow.f90:4-164) after the predictor (driver.f90:83-85): per
     Newton iteration the NS + VOF system (assembleQuenching VEC+MAT with the
@@ -679,8 +683,9 @@ def quench_step(ctx: Context, nnewton=3, rtol=1e-8, precision=L.OFD_FP64,
     then the residuals of both (:145-155).  With ``ntol`` = (u, p, phi, T)
     relative tolerances the loop stops once every residual has fallen by its
     tolerance from the first (check_convergence, solveFlow.f90:178-194);
-    without it, ``nnewton`` iterations run.  Returns the per-iteration solver
-    information."""
+    without it, ``nnewton`` iterations run.  ``nsvof_right`` applies the
+    NS + VOF system's block-Jacobi preconditioner on the right
+    (L.OFD_PREC_RIGHT).  Returns the per-iteration solver information."""
     ctx.predictor()
     ctx.apply_dirichlet()
     ctx.assemble_nsvof(2)
@@ -689,8 +694,9 @@ def quench_step(ctx: Context, nnewton=3, rtol=1e-8, precision=L.OFD_FP64,
     out = []
     for _ in range(nnewton):
         ctx.assemble_nsvof(6)
-        iv = ctx.solve_nsvof(rtol=rtol, precision=precision, ortho=ortho, kspace=kspace,
-                             max_cycles=max_cycles, raise_noconv=raise_noconv)
+        iv = ctx.solve_nsvof(rtol=rtol, precision=precision,
+                             ortho=ortho | (L.OFD_PREC_RIGHT if nsvof_right else 0),
+                             kspace=kspace, max_cycles=max_cycles, raise_noconv=raise_noconv)
         ctx.nsvof_update()
         ctx.assemble_tem(6)
         it = ctx.solve_tem(rtol=rtol, precision=precision, ortho=ortho, kspace=kspace,
