@@ -53,12 +53,14 @@ enum {
   OFD_ORTHO_CGS1 = 2  /* classical GS, one projection (restart cycles are short) */
 };
 
-/* OR'ed into the ortho argument of ofd_solve_nsvof with OFD_ORTHO_CGS1 or
- * OFD_ORTHO_CGS2: the same block-Jacobi preconditioner
+/* OR'ed into the ortho argument of ofd_solve_nsvof or ofd_solve with
+ * OFD_ORTHO_CGS1 or OFD_ORTHO_CGS2: the same block-Jacobi preconditioner
  * applied on the right, GMRES on A P^-1 (x = P^-1 y), so each cycle
  * minimises the true residual.  No counterpart in the reference, which
- * applies it on the left (SparseGMRES_NS_conv.f:53-150); on the quench tank
- * from 32^3 nodes up the left form stagnates well above 1e-8. */
+ * applies it on the left (SparseGMRES_NS.f90:53-121,
+ * SparseGMRES_NS_conv.f:53-150); on the quench tank from 32^3 nodes up the
+ * left form of the NS + VOF solve stagnates well above 1e-8.  The other
+ * solve entry points reject it. */
 enum { OFD_PREC_RIGHT = 16 };
 
 /* node fields held by a context: (NNODE,3) vectors from OFD_XG to OFD_ACGMOLD,

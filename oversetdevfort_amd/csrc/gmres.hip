@@ -714,14 +714,14 @@ hipError_t launch_gmres_init(Ctx &c, const double *b, double tol, int kspace, in
     else if (kind == 2)
       OFD_INIT(float, 2, c.d_W32, c.right ? c.d_T32 : nullptr);
     else
-      OFD_INIT(float, 0, c.d_W32, (float *)nullptr);
+      OFD_INIT(float, 0, c.d_W32, c.right ? c.d_T32 : nullptr);
   } else {
     if (kind == 1)
       OFD_INIT(double, 1, c.d_W, (double *)nullptr);
     else if (kind == 2)
       OFD_INIT(double, 2, c.d_W, c.right ? c.d_T : nullptr);
     else
-      OFD_INIT(double, 0, c.d_W, (double *)nullptr);
+      OFD_INIT(double, 0, c.d_W, c.right ? c.d_T : nullptr);
   }
 #undef OFD_INIT
   c.part_cur = 0;
@@ -759,7 +759,7 @@ static void launch_dots_m(Ctx &c, int m, int nb, const V *W, const double *rd, d
 template <typename V>
 static void launch_update1_m(Ctx &c, int m, int nb, V *W, const double *rd, double *wr,
                              V *T) {
-  const double *P = T ? c.d_PV : nullptr;
+  const double *P = T ? (c.gm_kind == 2 ? c.d_PV : c.d_P64) : nullptr;
   switch (m) {
 #define OFD_UM(M)                                                                           \
   case M:                                                                                 \
@@ -796,7 +796,7 @@ hipError_t launch_cgs_update(Ctx &c, int k, int mode) {
   const double *rd_dots = part_buf(c, mode == 3 ? 3 : 2);
   double *wr = (mode == 2) ? part_buf(c, 3) : part_buf(c, (k + 1) & 1);
   // right form: the sweep that leaves W_{k+1} also writes T = P^-1 W_{k+1}
-  const double *P = c.right ? c.d_PV : nullptr;
+  const double *P = c.right ? (c.gm_kind == 2 ? c.d_PV : c.d_P64) : nullptr;
   float *T32 = c.right ? c.d_T32 : nullptr;
   double *T64 = c.right ? c.d_T : nullptr;
   if (mode == 1 && k + 1 <= kExactM) {
@@ -822,7 +822,7 @@ hipError_t launch_gmres_close(Ctx &c, int k, int ortho) {
 }
 
 hipError_t launch_gmres_finish(Ctx &c, double *x) {
-  const double *P = c.right ? c.d_PV : nullptr;
+  const double *P = c.right ? (c.gm_kind == 2 ? c.d_PV : c.d_P64) : nullptr;
   if (c.basis32)
     hipLaunchKernelGGL(k_gmres_finish<float>, dim3(nblocks_gm(c)), dim3(kRedBlock), 0,
                        c.stream, c.gm_n, c.d_W32, c.d_gs, x, P, c.nnode);

@@ -163,10 +163,17 @@ static int ensure_a32(Ctx &c) {
 }
 
 // block-Jacobi inverse and the left-preconditioned matrix P^-1 A (FP64, and
-// FP32 when the mixed solver asks for it) of the current matrix
-static int ensure_prec(Ctx &c, int precision) {
+// FP32 when the mixed solver asks for it) of the current matrix; for the
+// right form the inverse and the plain matrix (FP32 copy for the mixed
+// solver)
+static int ensure_prec(Ctx &c, int precision, bool right = false) {
   OFD_CHECK(c.mat_valid, OFD_ERR_ARG, "no assembled matrix");
   const bool want32 = precision == OFD_FP32;
+  if (right) {
+    int rc = ensure_prec(c, OFD_FP64);
+    if (rc) return rc;
+    return want32 ? ensure_a32(c) : OFD_OK;
+  }
   if (c.prec_valid && (!want32 || c.ap32_valid)) return OFD_OK;
   if (!c.d_Ap64) {
     int rc = dalloc(c, &c.d_Ap64, (size_t)c.icnt * 16);
@@ -352,16 +359,24 @@ static int gmres_cycle(Ctx &c, int precision, int ortho, const double *b, double
   ortho &= ~OFD_PREC_RIGHT;
   OFD_CHECK(ortho == OFD_ORTHO_MGS || ortho == OFD_ORTHO_CGS2 || ortho == OFD_ORTHO_CGS1,
             OFD_ERR_ARG, "bad ortho");
-  OFD_CHECK(!right || (c.gm_kind == 2 && ortho != OFD_ORTHO_MGS), OFD_ERR_ARG,
-            "OFD_PREC_RIGHT applies to the NSVOF system with OFD_ORTHO_CGS1 or CGS2");
+  OFD_CHECK(!right || (c.gm_kind != 1 && ortho != OFD_ORTHO_MGS), OFD_ERR_ARG,
+            "OFD_PREC_RIGHT applies to the NS and NSVOF systems with OFD_ORTHO_CGS1 or CGS2");
   c.right = right;
+  if (right && c.t_n < c.gm_n) {
+    // T = P^-1 W_k, the right form's Arnoldi product input
+    const size_t np = (size_t)c.gm_n * 4;
+    int rc = dalloc(c, &c.d_T, np);
+    if (!rc) rc = dalloc(c, &c.d_T32, np);
+    c.t_n = rc ? 0 : c.gm_n;
+    if (rc) return rc;
+  }
   // the mixed CGS solver keeps its Krylov basis in FP32
   c.basis32 = precision == OFD_FP32 && ortho != OFD_ORTHO_MGS;
   int rc = ensure_krylov(c, K);
   if (rc) return rc;
   rc = c.gm_kind == 1 ? ensure_prec_s(c, precision)
                       : (c.gm_kind == 2 ? ensure_prec_v(c, precision, right)
-                                        : ensure_prec(c, precision));
+                                        : ensure_prec(c, precision, right));
   if (rc) return rc;
   OFD_HIP(launch_gmres_init(c, b, tol, K, kmin));
   // The host enqueues iterations ahead of the device and learns about
@@ -563,6 +578,7 @@ int ofd_ctx_destroy(ofd_ctx *x) {
   dfree(c.d_ApV32);
   dfree(c.d_T);
   dfree(c.d_T32);
+  c.t_n = 0;
   dfree(c.d_rhsV);
   dfree(c.d_xV);
   dfree(c.d_rV);
@@ -1771,8 +1787,6 @@ static int ensure_nsvof(Ctx &c) {
     if (!rc) rc = dalloc(c, &c.d_rV, np);
     if (!rc) rc = dalloc(c, &c.d_yV, np);
     if (!rc) rc = dalloc(c, &c.d_PV, (size_t)c.nnode * 17);
-    if (!rc) rc = dalloc(c, &c.d_T, np);
-    if (!rc) rc = dalloc(c, &c.d_T32, np);
     if (rc) return rc;
     OFD_HIP(hipMemsetAsync(c.d_rhsV, 0, np * sizeof(double), c.stream));
     OFD_HIP(hipMemsetAsync(c.d_xV, 0, np * sizeof(double), c.stream));
@@ -2301,6 +2315,8 @@ int ofd_solve_t(ofd_ctx *x, int precision, int ortho, double rtol, double inner_
                 int *cycles, double *relres) {
   OFD_CHECK(x && gu && gp, OFD_ERR_ARG, "null argument");
   OFD_CHECK(precision == OFD_FP64 || precision == OFD_FP32, OFD_ERR_ARG, "bad precision");
+  OFD_CHECK(!(ortho & OFD_PREC_RIGHT), OFD_ERR_ARG,
+            "OFD_PREC_RIGHT: ofd_solve and ofd_solve_nsvof only");
   OFD_CHECK(max_cycles >= 1, OFD_ERR_ARG, "max_cycles >= 1");
   Ctx &c = x->c;
   int rc = ensure_adjoint(c);
@@ -2507,6 +2523,8 @@ int ofd_solve_adjoint(ofd_ctx *x, int subdomain_flag, int precision, int ortho, 
                       const double *gp, int *iters, int *cycles, double *relres) {
   OFD_CHECK(x && gu && gp, OFD_ERR_ARG, "null argument");
   OFD_CHECK(subdomain_flag >= 1 && subdomain_flag <= 3, OFD_ERR_ARG, "subdomain_flag 1..3");
+  OFD_CHECK(!(ortho & OFD_PREC_RIGHT), OFD_ERR_ARG,
+            "OFD_PREC_RIGHT: ofd_solve and ofd_solve_nsvof only");
   OFD_CHECK(precision == OFD_FP64 || precision == OFD_FP32, OFD_ERR_ARG, "bad precision");
   OFD_CHECK(max_cycles >= 1, OFD_ERR_ARG, "max_cycles >= 1");
   Ctx &c = x->c;
@@ -2534,6 +2552,8 @@ int ofd_solve_newton(ofd_ctx *x, int subdomain_flag, int precision, int ortho, d
                      double *relres) {
   OFD_CHECK(x, OFD_ERR_ARG, "null context");
   OFD_CHECK(subdomain_flag >= 1 && subdomain_flag <= 3, OFD_ERR_ARG, "subdomain_flag 1..3");
+  OFD_CHECK(!(ortho & OFD_PREC_RIGHT), OFD_ERR_ARG,
+            "OFD_PREC_RIGHT: ofd_solve and ofd_solve_nsvof only");
   OFD_CHECK(precision == OFD_FP64 || precision == OFD_FP32, OFD_ERR_ARG, "bad precision");
   OFD_CHECK(max_cycles >= 1, OFD_ERR_ARG, "max_cycles >= 1");
   Ctx &c = x->c;

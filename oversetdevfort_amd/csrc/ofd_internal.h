@@ -180,11 +180,12 @@ struct Ctx {
   float *d_ApV32 = nullptr;
   bool matV_valid = false, precV_valid = false, aV32_valid = false, apV32_valid = false;
   bool pV_valid = false;              // the NSVOF block inverses alone (right form)
-  // right-preconditioned NSVOF cycle: the Arnoldi product reads
+  // right-preconditioned cycle (NS or NSVOF): the Arnoldi product reads
   // T = P^-1 W_k (d_T, or d_T32 with the FP32 basis) and the plain matrix
   bool right = false;
   double *d_T = nullptr;
   float *d_T32 = nullptr;
+  int t_n = 0;                        // 4-tuples d_T / d_T32 hold
   double *d_rhsV = nullptr, *d_xV = nullptr, *d_rV = nullptr, *d_yV = nullptr;
   int64_t nsvV_icnt = 0;
   int nsvV_n4 = 0;

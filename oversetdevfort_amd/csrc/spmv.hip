@@ -597,6 +597,22 @@ hipError_t launch_spmv_arnoldi(Ctx &c, int precision, int k, hipEvent_t ev0,
   if (c.gm_kind == 2) return launch_spmv_arnoldi_v(c, precision, k, ev0, ev1);
   const unsigned nb = spmv_blocks(c);
   const size_t vs = (size_t)c.nnode * 4;
+  if (c.right) {
+    // right form: W_{k+1} = A T with T = P^-1 W_k from the previous sweep
+    if (c.basis32)
+      hipExtLaunchKernelGGL((k_spmv<float, float, 2>), dim3(nb), dim3(256), 0, c.stream, ev0,
+                            ev1, 0, c.nnode, (const int32_t *)c.d_col,
+                            (const int32_t *)c.d_row, (const float *)c.d_A32,
+                            (const float *)c.d_T32, c.d_W32 + vs * (k + 1),
+                            (const double *)nullptr, (const GmresState *)c.d_gs);
+    else
+      hipExtLaunchKernelGGL((k_spmv<double, double, 2>), dim3(nb), dim3(256), 0, c.stream,
+                            ev0, ev1, 0, c.nnode, (const int32_t *)c.d_col,
+                            (const int32_t *)c.d_row, (const double *)c.d_A64,
+                            (const double *)c.d_T, c.d_W + vs * (k + 1),
+                            (const double *)nullptr, (const GmresState *)c.d_gs);
+    return hipGetLastError();
+  }
   if (c.basis32) {
     const float *x = c.d_W32 + vs * k;
     float *y = c.d_W32 + vs * (k + 1);

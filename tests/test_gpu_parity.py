@@ -192,8 +192,13 @@ def test_gmres_mgs_matches_oracle(ofd, oracle_mod, tol, kspace, kmin):
                                              (L.OFD_FP64, L.OFD_ORTHO_CGS2),
                                              (L.OFD_FP32, L.OFD_ORTHO_CGS2),
                                              (L.OFD_FP64, L.OFD_ORTHO_MGS),
-                                             (L.OFD_FP32, L.OFD_ORTHO_MGS)])
+                                             (L.OFD_FP32, L.OFD_ORTHO_MGS),
+                                             (L.OFD_FP64, L.OFD_ORTHO_CGS1 | L.OFD_PREC_RIGHT),
+                                             (L.OFD_FP32, L.OFD_ORTHO_CGS1 | L.OFD_PREC_RIGHT)])
 def test_solve_reaches_1e8_true_residual(ofd, oracle_mod, precision, ortho):
+    """Every precision / orthogonalisation variant, with the block-Jacobi
+    preconditioner on the left and (OFD_PREC_RIGHT) on the right, reaches a
+    1e-8 true residual formed on the host by the oracle's SparseProdUP_3D."""
     O = oracle_mod
     prob, st = _well_posed(6, seed=12)
     m = prob.mesh
