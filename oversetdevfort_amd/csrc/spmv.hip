@@ -215,12 +215,20 @@ __global__ void k_convert_s32(size_t n, const double *__restrict__ A, float *__r
 // (PLS, LS).  Vectors are [N][4] (u0 u1 u2 p) followed by [N] phi,
 // zero-padded to whole 4-tuples (npad entries).
 //
-// 4 lanes per block row as in k_spmv: lane c reads row c of the 4x4 block
-// (32 / 16 bytes) and its pair (16 / 8 bytes) and forms output c, so the 4
-// lanes read the nonzero's 192 / 96 contiguous bytes;
-// it also forms the part of the phi row that multiplies x_c (LSU(c) x_uc,
-// or LS x_phi on lane 3); the four parts are added in a fixed shuffle order.
-// The 4 lanes after the last row write the padding zeros.  mode as k_spmv.
+// 8 lanes per block row (kVStreams = 2 streams of 4): lanes h*4 + c,
+// h = 0, 1, take the row's nonzeros k0 + h, k0 + h + 2, ...; lane c reads
+// row c of the 4x4 block (32 / 16 bytes) and its pair (16 / 8 bytes) and
+// forms output c, so a nonzero's 192 / 96 contiguous bytes are read by 4
+// lanes; it also forms the part of the phi row that multiplies x_c
+// (LSU(c) x_uc, or LS x_phi on lane 3).  The two streams are added, then the
+// four phi parts, in a fixed shuffle order.
+// Two streams per row double the loads in flight per row and spread the
+// 48^3 tank's 117,649 rows over about three rounds of resident waves
+// instead of one and a half: the 48^3 tank's FP64 product went from 135.6 to
+// 120.7 us, the FP32-matrix one from 96.5 to 74.9 us (four streams: 117.3
+// and 73.5 us, no gain in the solver).  The 8 lanes after the last row
+// write the padding zeros.  mode as k_spmv.
+constexpr int kVStreams = 2;  // nonzero streams per row (4 lanes each)
 template <typename T, typename V, int MODE>
 __global__ void __launch_bounds__(256)
 k_spmv_v(int N, int npad, const int32_t *__restrict__ col, const int32_t *__restrict__ row,
@@ -231,19 +239,20 @@ k_spmv_v(int N, int npad, const int32_t *__restrict__ col, const int32_t *__rest
   const int per = gridDim.x >> 3;
   const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
   const int gt = lb * blockDim.x + threadIdx.x;
-  const int r = gt >> 2;
+  const int r = gt / (4 * kVStreams);
+  const int h = (gt >> 2) & (kVStreams - 1);
   const int c = gt & 3;
-  if (r > N) return;  // uniform over each group of 4 lanes
+  if (r > N) return;  // uniform over each group of 4 kVStreams lanes
   if (r == N) {
     const int t = 5 * N + c;
-    if (t < npad) y[t] = (V)0.0;
+    if (h == 0 && t < npad) y[t] = (V)0.0;
     return;
   }
   const V *xphi = x + (size_t)4 * N;
-  const int k0 = col[r], k1 = col[r + 1];
+  const int k0 = col[r] + h, k1 = col[r + 1];
   double acc = 0.0, accp = 0.0;
 #pragma unroll 4
-  for (int k = k0; k < k1; ++k) {
+  for (int k = k0; k < k1; k += kVStreams) {
     const int cn = row[k];
     double a0, a1, a2, a3, x0, x1, x2, x3, e0, e1;
     Row4<T>::load(A + (size_t)k * 24 + c * 4, a0, a1, a2, a3);
@@ -254,8 +263,14 @@ k_spmv_v(int N, int npad, const int32_t *__restrict__ col, const int32_t *__rest
     const double xc = (c == 0) ? x0 : ((c == 1) ? x1 : ((c == 2) ? x2 : xp));
     accp += e1 * xc;
   }
+#pragma unroll
+  for (int m = 4; m < 4 * kVStreams; m <<= 1) {
+    acc += __shfl_xor(acc, m);
+    accp += __shfl_xor(accp, m);
+  }
   accp += __shfl_xor(accp, 1);
   accp += __shfl_xor(accp, 2);
+  if (h) return;
   if (MODE == 1) {
     y[(size_t)r * 4 + c] = (V)(b[(size_t)r * 4 + c] - acc);
     if (c == 0) y[(size_t)4 * N + r] = (V)(b[(size_t)4 * N + r] - accp);
@@ -483,9 +498,10 @@ hipError_t launch_convert_aT32(Ctx &c) {
   return hipGetLastError();
 }
 
-// 4 lanes per node row plus one group for the padding, rounded to 8 blocks
+// 4 kVStreams lanes per node row plus one group for the padding, rounded to
+// 8 blocks
 static inline unsigned spmv_v_blocks(const Ctx &c) {
-  const unsigned nb = (unsigned)(((size_t)(c.nnode + 1) * 4 + 255) / 256);
+  const unsigned nb = (unsigned)(((size_t)(c.nnode + 1) * 4 * kVStreams + 255) / 256);
   return (nb + 7u) & ~7u;
 }
 
