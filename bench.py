@@ -415,7 +415,7 @@ def spmv_large(L, C, S, dims=(128, 64, 64), reps=20):
     return out
 
 
-def quench_steps(L, C, S, n=48, nsteps=20, nnewton=3, max_cycles=300):
+def quench_steps(L, C, S, n=48, nsteps=20, nnewton=3, max_cycles=600):
     """The quench configuration end to end: the 48^3 tank (water/vapour,
     density ratio 1000, solid block at 900 K in liquid at 300 K), dt = 1e-3,
     ``nsteps`` time steps of ``nnewton`` Newton iterations, each the NS + VOF

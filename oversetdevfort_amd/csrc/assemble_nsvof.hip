@@ -11,11 +11,14 @@
 // (ELM_ID 102) contribute; solid elements scatter their projected zero
 // element matrices.  The NS discontinuity-capturing coefficient is 0.
 //
-// Storage of the system, 24 contiguous doubles per nonzero k:
-//   A[k][0..15]   the 4x4 block [K11 G; D1 M] row-major, as the NS system
-//   A[k][16..23]  the phi couplings as pairs (col phi of row r, row phi of
-//                 col r): (ULS(0), LSU(0)) (ULS(1), LSU(1)) (ULS(2), LSU(2))
-//                 (PLS, LS)
+// Storage of the system, 24 doubles per nonzero k in two arrays (one
+// allocation, the pairs after the Z blocks):
+//   A[k][0..15]     the 4x4 block [K11 G; D1 M] row-major, as the NS system
+//   Apair[k][0..7]  the phi couplings as pairs (col phi of row r, row phi of
+//                   col r): (ULS(0), LSU(0)) (ULS(1), LSU(1)) (ULS(2), LSU(2))
+//                   (PLS, LS)
+// so a block is 128 and a pair group 64 aligned bytes (FP32 copies: 64 and
+// 32), which the products read whole.
 // Node vectors are [N][4] (u0 u1 u2 p) followed by [N] phi.
 //
 // One 64-lane wavefront per element, as the NS kernel: lane = (Gauss point,
@@ -68,7 +71,8 @@ k_assemble_nsvof_el(int E, const int32_t *__restrict__ ien, const int32_t *__res
                      const int32_t *__restrict__ emap,
                      const uint64_t *__restrict__ firstmask, const double *__restrict__ rec,
                      const uint8_t *__restrict__ ibc, NsvofParams P, int do_mat, int do_vec,
-                     int N, double *__restrict__ A, double *__restrict__ rhs,
+                     int N, double *__restrict__ A, double *__restrict__ Apair,
+                     double *__restrict__ rhs,
                      double *__restrict__ cfl_out, int32_t *__restrict__ negdet) {
   __shared__ double s_sh[kWaves][8][8][5];  // [q][a]: N, dN/dx (3), shconv
   __shared__ double s_q[kWaves][8][kQs];
@@ -366,8 +370,8 @@ k_assemble_nsvof_el(int E, const int32_t *__restrict__ ien, const int32_t *__res
       const int k = emap[(size_t)e * 64 + lane];
       // the first element (in colour order) to reach the nonzero stores
       const bool firstk = (firstmask[e] >> lane) & 1ull;
-      double2 *b16 = reinterpret_cast<double2 *>(A + (size_t)k * 24);
-      double2 *b8 = reinterpret_cast<double2 *>(A + (size_t)k * 24 + 16);
+      double2 *b16 = reinterpret_cast<double2 *>(A + (size_t)k * 16);
+      double2 *b8 = reinterpret_cast<double2 *>(Apair + (size_t)k * 8);
 #pragma unroll
       for (int t = 0; t < 8; ++t) {
         double2 o = firstk ? make_double2(0.0, 0.0) : b16[t];
@@ -424,8 +428,8 @@ hipError_t launch_assemble_nsvof_hex(Ctx &c, int tensor_flag, int color) {
                      dim3(blocks), dim3(64 * kWaves), 0, c.stream,
                      c.nelem, c.d_ien, c.d_elm_id, c.d_color_elems, first, count, c.d_emap,
                      c.d_first, c.d_noderec, c.d_ibc, P, (tensor_flag & OFD_ASSEMBLE_MAT) ? 1 : 0,
-                     (tensor_flag & OFD_ASSEMBLE_VEC) ? 1 : 0, c.nnode, c.d_AV, c.d_rhsV,
-                     c.d_cfl, c.d_negdet);
+                     (tensor_flag & OFD_ASSEMBLE_VEC) ? 1 : 0, c.nnode, c.d_AV,
+                     c.d_AV + (size_t)16 * c.icnt, c.d_rhsV, c.d_cfl, c.d_negdet);
   return hipGetLastError();
 }
 

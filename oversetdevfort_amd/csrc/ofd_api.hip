@@ -1798,13 +1798,13 @@ static int ensure_nsvof(Ctx &c) {
 }
 
 // reference arrays (LHSK11 (9,Z), LHSG/LHSD1/LHSLSU/LHSULS (3,Z), LHSM/LHSLS/
-// LHSPLS (Z)) -> the device blocks A[k][24]: [K11 G; D1 M] row-major, then
-// (ULS(r), LSU(r)) r < 3, (PLS, LS)
+// LHSPLS (Z)) -> the device layout: Z blocks [K11 G; D1 M] of 16 row-major,
+// then Z pair groups of 8, (ULS(r), LSU(r)) r < 3, (PLS, LS)
 static void nsvof_to_device(int64_t Z, const double *k11, const double *g, const double *d1,
                             const double *m, const double *ls, const double *lsu,
                             const double *pls, const double *uls, double *A) {
   for (int64_t k = 0; k < Z; ++k) {
-    double *b = A + k * 24, *e = b + 16;
+    double *b = A + k * 16, *e = A + 16 * Z + k * 8;
     for (int r = 0; r < 3; ++r) {
       for (int c = 0; c < 3; ++c) b[4 * r + c] = k11[k * 9 + r * 3 + c];
       b[4 * r + 3] = g[k * 3 + r];
@@ -1891,7 +1891,7 @@ int ofd_get_nsvof_system(ofd_ctx *x, double *k11, double *g, double *d1, double 
     rc = d2h(c, A.data(), c.d_AV, A.size() * sizeof(double));
     if (rc) return rc;
     for (int64_t k = 0; k < Z; ++k) {
-      const double *b = A.data() + k * 24, *e = b + 16;
+      const double *b = A.data() + k * 16, *e = A.data() + 16 * Z + k * 8;
       for (int r = 0; r < 3; ++r) {
         for (int cc = 0; cc < 3; ++cc)
           if (k11) k11[k * 9 + r * 3 + cc] = b[4 * r + cc];

@@ -209,17 +209,19 @@ __global__ void k_convert_s32(size_t n, const double *__restrict__ A, float *__r
 }
 
 // Product of the coupled NS + VOF system (SparseProd_NS_conv, SparseProd.f90:
-// 167-242).  A nonzero k is 24 contiguous entries A[k][24]: the 4x4 block
-// [K11 G; D1 M] row-major (as the NS system), then the phi couplings as
-// pairs (col phi of row r, row phi of col r): (ULS(r), LSU(r)) for r < 3,
-// (PLS, LS).  Vectors are [N][4] (u0 u1 u2 p) followed by [N] phi,
-// zero-padded to whole 4-tuples (npad entries).
+// 167-242).  A nonzero k is the 4x4 block A[k][16] [K11 G; D1 M] row-major
+// (as the NS system) and the phi couplings Ap[k][8] as pairs (col phi of row
+// r, row phi of col r): (ULS(r), LSU(r)) for r < 3, (PLS, LS); the pair
+// array follows the Z blocks in the same allocation.  Vectors are [N][4]
+// (u0 u1 u2 p) followed by [N] phi, zero-padded to whole 4-tuples (npad
+// entries).
 //
 // 8 lanes per block row (kVStreams = 2 streams of 4): lanes h*4 + c,
 // h = 0, 1, take the row's nonzeros k0 + h, k0 + h + 2, ...; lane c reads
 // row c of the 4x4 block (32 / 16 bytes) and its pair (16 / 8 bytes) and
-// forms output c, so a nonzero's 192 / 96 contiguous bytes are read by 4
-// lanes; it also forms the part of the phi row that multiplies x_c
+// forms output c, so the 4 lanes read a nonzero's aligned 128 / 64-byte
+// block and 64 / 32-byte pair group; it also forms the part of the phi row
+// that multiplies x_c
 // (LSU(c) x_uc, or LS x_phi on lane 3).  The two streams are added, then the
 // four phi parts, in a fixed shuffle order.
 // Two streams per row double the loads in flight per row and spread the
@@ -228,11 +230,15 @@ __global__ void k_convert_s32(size_t n, const double *__restrict__ A, float *__r
 // 120.7 us, the FP32-matrix one from 96.5 to 74.9 us (four streams: 117.3
 // and 73.5 us, no gain in the solver).  The 8 lanes after the last row
 // write the padding zeros.  mode as k_spmv.
+// Records of 24 contiguous entries (192 / 96 bytes) straddle cache lines;
+// with the blocks and pair groups in separate aligned arrays the FP32
+// Arnoldi product of the tank went from 81.8 to 77.5 us (FP64: 126 us
+// either way).
 constexpr int kVStreams = 2;  // nonzero streams per row (4 lanes each)
 template <typename T, typename V, int MODE>
 __global__ void __launch_bounds__(256)
 k_spmv_v(int N, int npad, const int32_t *__restrict__ col, const int32_t *__restrict__ row,
-         const T *__restrict__ A, const V *__restrict__ x,
+         const T *__restrict__ A, const T *__restrict__ Ap, const V *__restrict__ x,
          V *__restrict__ y, const double *__restrict__ b, const GmresState *__restrict__ gs) {
   if (MODE == 2 && gs->done) return;
   if (MODE == 1 && gs && gs->outer_done) return;
@@ -255,8 +261,8 @@ k_spmv_v(int N, int npad, const int32_t *__restrict__ col, const int32_t *__rest
   for (int k = k0; k < k1; k += kVStreams) {
     const int cn = row[k];
     double a0, a1, a2, a3, x0, x1, x2, x3, e0, e1;
-    Row4<T>::load(A + (size_t)k * 24 + c * 4, a0, a1, a2, a3);
-    Pair<T>::load(A + (size_t)k * 24 + 16 + c * 2, e0, e1);
+    Row4<T>::load(A + (size_t)k * 16 + c * 4, a0, a1, a2, a3);
+    Pair<T>::load(Ap + (size_t)k * 8 + c * 2, e0, e1);
     load_x4(x + (size_t)cn * 4, x0, x1, x2, x3);
     const double xp = (double)xphi[cn];
     acc += a0 * x0 + a1 * x1 + a2 * x2 + a3 * x3 + e0 * xp;
@@ -284,19 +290,20 @@ k_spmv_v(int N, int npad, const int32_t *__restrict__ col, const int32_t *__rest
 // 53-150): 1/LS with 0 -> 0; the u-p part is the NS block inverse of the 4x4
 // diagonal block (3x3 cofactor inverse of K11, 1/M with 0 -> 1)
 __global__ void k_build_prec_phi(int N, const int32_t *__restrict__ diag,
-                                 const double *__restrict__ A, double *__restrict__ pphi) {
+                                 const double *__restrict__ Ap, double *__restrict__ pphi) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
-  const double l = A[(size_t)diag[i] * 24 + 23];
+  const double l = Ap[(size_t)diag[i] * 8 + 7];
   pphi[i] = (l == 0.0) ? 0.0 : 1.0 / l;
 }
 
 // the couplings of P^-1 A: thread (i, c) forms, for every nonzero k of block
 // row i, entry (row c, col phi) = sum_s P_i(c, s) e[2s] and entry (row phi,
-// col c) = pphi_i e[2c+1], e = A[k][16..23]
+// col c) = pphi_i e[2c+1], e = the pair group Ae[k][8]; into the pair groups
+// Ap (and Ap32) of the preconditioned copy
 __global__ void k_precondition_phi(int N, const int32_t *__restrict__ col,
                                    const double *__restrict__ P, const double *__restrict__ pphi,
-                                   const double *__restrict__ A, double *__restrict__ Ap,
+                                   const double *__restrict__ Ae, double *__restrict__ Ap,
                                    float *__restrict__ Ap32) {
   const int gt = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = gt >> 2, c = gt & 3;
@@ -304,13 +311,12 @@ __global__ void k_precondition_phi(int N, const int32_t *__restrict__ col,
   const double *p = P + (size_t)i * 16 + c * 4;
   const double p0 = p[0], p1 = p[1], p2 = p[2], p3 = p[3], pf = pphi[i];
   for (int k = col[i]; k < col[i + 1]; ++k) {
-    const double *a = A + (size_t)k * 24 + 16;
+    const double *a = Ae + (size_t)k * 8;
     const double o0 = p0 * a[0] + p1 * a[2] + p2 * a[4] + p3 * a[6];
     const double o1 = pf * a[2 * c + 1];
-    reinterpret_cast<double2 *>(Ap + (size_t)k * 24 + 16)[c] = make_double2(o0, o1);
+    reinterpret_cast<double2 *>(Ap + (size_t)k * 8)[c] = make_double2(o0, o1);
     if (Ap32)
-      reinterpret_cast<float2 *>(Ap32 + (size_t)k * 24 + 16)[c] =
-          make_float2((float)o0, (float)o1);
+      reinterpret_cast<float2 *>(Ap32 + (size_t)k * 8)[c] = make_float2((float)o0, (float)o1);
   }
 }
 
@@ -505,12 +511,14 @@ static inline unsigned spmv_v_blocks(const Ctx &c) {
   return (nb + 7u) & ~7u;
 }
 
+// A: an NSVOF matrix allocation (Z blocks of 16, then Z pair groups of 8)
 template <typename T, typename V, int MODE>
 static void launch_v(Ctx &c, const T *A, const V *x, V *y, const double *b,
                      hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
+  const T *Ap = A + (size_t)16 * c.icnt;
   hipExtLaunchKernelGGL((k_spmv_v<T, V, MODE>), dim3(spmv_v_blocks(c)), dim3(256), 0,
                         c.stream, ev0, ev1, 0, c.nnode, c.gm_n * 4, (const int32_t *)c.d_col,
-                        (const int32_t *)c.d_row, A, x, y, b, (const GmresState *)c.d_gs);
+                        (const int32_t *)c.d_row, A, Ap, x, y, b, (const GmresState *)c.d_gs);
 }
 
 hipError_t launch_spmv_v(Ctx &c, int precision, const double *x, double *y,
@@ -534,16 +542,17 @@ hipError_t launch_spmv_v(Ctx &c, int precision, const double *x, double *y,
 hipError_t launch_build_prec_v(Ctx &c, bool want32, bool ap) {
   const int N = c.nnode;
   double *P = c.d_PV, *pphi = c.d_PV + (size_t)N * 16;
+  const size_t zp = (size_t)16 * c.icnt;  // the pair groups follow the blocks
   hipLaunchKernelGGL(k_build_prec, dim3((N + 255) / 256), dim3(256), 0, c.stream, N, c.d_diag,
-                     c.d_AV, 24, P);
+                     c.d_AV, 16, P);
   hipLaunchKernelGGL(k_build_prec_phi, dim3((N + 255) / 256), dim3(256), 0, c.stream, N,
-                     c.d_diag, c.d_AV, pphi);
+                     c.d_diag, c.d_AV + zp, pphi);
   if (!ap) return hipGetLastError();
   const unsigned nb = (unsigned)(((size_t)N * 4 + 255) / 256);
   hipLaunchKernelGGL(k_precondition_matrix, dim3(nb), dim3(256), 0, c.stream, N, c.d_col, P,
-                     c.d_AV, 24, c.d_ApV, want32 ? c.d_ApV32 : nullptr);
+                     c.d_AV, 16, c.d_ApV, want32 ? c.d_ApV32 : nullptr);
   hipLaunchKernelGGL(k_precondition_phi, dim3(nb), dim3(256), 0, c.stream, N, c.d_col, P,
-                     pphi, c.d_AV, c.d_ApV, want32 ? c.d_ApV32 : nullptr);
+                     pphi, c.d_AV + zp, c.d_ApV + zp, want32 ? c.d_ApV32 + zp : nullptr);
   return hipGetLastError();
 }
 
