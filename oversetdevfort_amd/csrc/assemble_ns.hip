@@ -475,38 +475,6 @@ k_assemble_ns_el(int N, int E, const int32_t *__restrict__ ien,
 // ---------------------------------------------------------------------------
 namespace {
 
-// the sum of S products in the lane-butterfly association (S = 8:
-// ((p0+p1)+(p2+p3))+((p4+p5)+(p6+p7)); S = 4: (p0+p1)+(p2+p3)); the adds
-// carry no contraction, so no product is fused into them
-template <int S>
-__device__ __forceinline__ double tree_sum(const double (&p)[S]) {
-#pragma clang fp contract(off)
-  if constexpr (S == 8)
-    return ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7]));
-  else
-    return (p[0] + p[1]) + (p[2] + p[3]);
-}
-
-// sum and maximum over the S lanes of an element's Gauss points
-template <int S>
-__device__ __forceinline__ double point_sum(double v) {
-  if constexpr (S == 8) return wsum8(v);
-  else return wsum4(v);
-}
-template <int S>
-__device__ __forceinline__ double point_max(double v) {
-  if constexpr (S == 8) return wmax8(v);
-  else return wmax4(v);
-}
-
-// the shape value of node a at Gauss point q (elem_shape<S>'s Na)
-template <int S>
-__device__ __forceinline__ double shape_N(int q, int a) {
-  double Na, d0, d1, d2, w;
-  elem_shape<S>(q, a, Na, d0, d1, d2, w);
-  return Na;
-}
-
 constexpr int kRecStride = 14;  // doubles per staged node record
 
 }  // namespace

@@ -1,4 +1,4 @@
-// Temperature equation of the quenching solver on gfx950, hex elements.
+// Temperature equation of the quenching solver on gfx950, hexes and tets.
 //
 // Restates IntElmAss_Tem_Quenching_STAB (solveQuenching.f90:579-882): the
 // SUPG-stabilised energy equation with the evaporation / condensation
@@ -8,16 +8,27 @@
 // e3STAB_3D_TEM (e3STAB.f90:338-367), then BCLhs_tem (BCLhs_3D.f90:247-274;
 // IBC column 6) and FillSparseMat_tem (FillSparseMat.f90:175-201).
 //
-// The layout is the NS kernel's: one 64-lane wavefront per element.
-//   phase A  lane = (Gauss point q, node a): shape functions, Jacobian,
-//            Gauss-point values (8-lane butterflies), properties, phase
-//            change, tau; per-(q,a) data staged in LDS; the RHS of node a
-//            reduced over q.
-//   phase B  lane = (a, b): the element matrix entry accumulated over the 8
-//            Gauss points in the reference's term order.
-//   phase C  Dirichlet rows/columns of T, then the read-modify-write of
-//            LHSTem at emap[e][a + 8 b]; one colour per launch, so no two
-//            elements in flight share a node.
+// Three kernels (S = 8 or 4 nodes and Gauss points), as the NS assembly:
+//   k_tem_points<S>  one launch over all elements, 64 / S per wavefront,
+//                    lane = (element, q): Jacobian, Gauss-point values,
+//                    properties, phase change, tau once per point; the sums
+//                    over the S nodes are register sums in the association
+//                    of the lane butterfly, the VOF value and T (compared
+//                    against thresholds) in node order.  It writes each
+//                    element's point data at its position in colour order
+//                    (J^-1, the advective velocity and five point factors per
+//                    point, the RHS per node summed over the points, the node
+//                    numbers).
+//   k_tem_pairs<S>   one launch per colour, 64 / S^2 elements per wavefront:
+//                    lane = (element, q, a) forms dN_a/dx and the streamline
+//                    derivative from the point data, then lane = (element, a,
+//                    b) the entry (a, b) over the S points, BCLhs_tem and the
+//                    first-touch read-modify-write of LHSTem.
+//   k_tem_rhs_gather one launch, thread per node: the node's contributions in
+//                    colour order, the order of a coloured scatter.
+// The one-wavefront-per-element kernel they replace (lane = (q, a), every
+// Gauss-point quantity in S copies, the interpolations as lane reductions)
+// took 0.304 / 0.207 ms (VEC+MAT / VEC) on the 48^3 tank.
 #include "ofd_internal.h"
 #include "wave_dpp.h"
 #include "element_shape.h"
@@ -27,25 +38,23 @@ namespace ofd {
 namespace {
 
 constexpr int kWaves = 4;
-__device__ __forceinline__ double sum8(double v) { return wsum8(v); }
-
-// Gauss-point value summed over the element's 8 nodes in node order, the
-// order of the reference's e3int_qr.  Used for the values compared against a
-// threshold (the VOF field clamped to [0, 1], T against T_sat): inside one
-// phase every nodal value is exactly 0 or 1, and a tree-ordered sum can land
-// one rounding on the other side of the clamp and flip the branch.
-__device__ __forceinline__ double seq8(double v) {
-  const int b = (int)(threadIdx.x & 63) & ~7;
-  double s = 0.0;
-#pragma unroll
-  for (int j = 0; j < 8; ++j) s = s + __shfl(v, b + j);
-  return s;
-}
+constexpr int kTRec = 13;  // staged node record: x + d, u, um, phi, T, rT, ks
 
 __device__ __forceinline__ double prop_interp(double a, double b, double phi) {
   const double pc = fmax(0.0, fmin(1.0, phi));
   return a * (1.0 - pc) + b * pc;
 }
+
+// point data of one element, in doubles: per point [18] (J^-1 column-major,
+// uadv, rho cp, tau, dSe/dT, k (+ k_dc), DetJ w, 0); the RHS per node; the
+// node numbers and the fluid flag as int32
+template <int S>
+struct TemPts {
+  static constexpr int Q = 18, R = 18 * S, N = 18 * S + S;
+  static constexpr int stride = S == 8 ? 160 : 80;
+};
+static_assert(TemPts<8>::N + 5 <= TemPts<8>::stride && TemPts<8>::stride <= 192, "hex layout");
+static_assert(TemPts<4>::N + 3 <= TemPts<4>::stride && TemPts<4>::stride <= 96, "tet layout");
 
 }  // namespace
 
@@ -57,49 +66,68 @@ struct TemParams {
 };
 
 template <int S>
-__global__ void __launch_bounds__(256)
-k_assemble_tem_el(int E, const int32_t *__restrict__ ien, const int32_t *__restrict__ elm_id,
-                   const int32_t *__restrict__ elems, int first, int count,
-                   const int32_t *__restrict__ emap, const uint64_t *__restrict__ firstmask,
-                   const double *__restrict__ rec,
-                   const double *__restrict__ ks, const uint8_t *__restrict__ ibc,
-                   TemParams P, int do_mat, int do_vec,
-                   double *__restrict__ A, double *__restrict__ rhs,
-                   int32_t *__restrict__ negdet) {
-  __shared__ double s_sh[kWaves][8][8][5];  // [q][a]: N, dN/dx (3), shconv
-  __shared__ double s_q[kWaves][8][5];      // [q]: rho cp, tau, dSe/dT, k (+k_dc), DetJ*w
-  __shared__ int32_t s_node[kWaves][8];
-
-  const int lane = threadIdx.x & 63;
-  const int w = threadIdx.x >> 6;
-  const int idx = first + blockIdx.x * kWaves + w;
-  const bool valid = idx < first + count;
-  const int e = valid ? elems[idx] : 0;
+__global__ void __launch_bounds__(64)
+k_tem_points(int E, const int32_t *__restrict__ ien, const int32_t *__restrict__ elm_id,
+             const int32_t *__restrict__ cpos, const double *__restrict__ rec,
+             const double *__restrict__ ks, TemParams P, int do_mat, int do_vec,
+             double *__restrict__ pts, int32_t *__restrict__ negdet) {
+  using L = TemPts<S>;
+  __shared__ __align__(16) double s_rec[64 * kTRec];
+  const int lane = threadIdx.x;
+  const int el = lane / S;  // 64 / S elements per wavefront
+  const int q = lane % S;   // Gauss point, and node a = q for the loads
+  const int e = blockIdx.x * (64 / S) + el;
+  const bool valid = e < E;
   const bool fluid = valid && elm_id[e] == P.fluid_id;
+  double *po = pts + (size_t)(valid ? cpos[e] : 0) * L::stride;
+  {
+    // node record: x+d (0-2), u (3-5), ac (6-8), um (9-11), p, phi, T, rT (12-15)
+    const int n = valid ? ien[(size_t)q * E + e] : 0;
+    const double2 *r2 = reinterpret_cast<const double2 *>(rec + (size_t)n * 16);
+    const double2 v0 = r2[0], v1 = r2[1], v2 = r2[2], v4 = r2[4], v5 = r2[5], v6 = r2[6],
+                  v7 = r2[7];
+    double *sr = s_rec + (el * S + q) * kTRec;
+    sr[0] = v0.x;
+    sr[1] = v0.y;
+    sr[2] = v1.x;
+    sr[3] = v1.y;
+    sr[4] = v2.x;
+    sr[5] = v2.y;
+    sr[6] = v4.y;
+    sr[7] = v5.x;
+    sr[8] = v5.y;
+    sr[9] = v6.y;
+    sr[10] = v7.x;
+    sr[11] = v7.y;
+    sr[12] = ks ? ks[n] : 1.0;
+    if (valid && do_mat) {
+      int32_t *pn = reinterpret_cast<int32_t *>(po + L::N);
+      pn[q] = n;
+      if (q == 0) pn[S] = fluid ? 1 : 0;
+    }
+  }
+  __syncthreads();
+  const double *re = s_rec + el * S * kTRec;
 
-  // ---------------- phase A: lane = (q, a) ----------------
-  const int q = lane >> 3;
-  const int a = lane & 7;
-  const int na = valid ? ien[(size_t)(a < S ? a : 0) * E + e] : 0;
-  if (lane < 8) s_node[w][lane] = na;
-  // node record: x+d (0-2), u (3-5), ac (6-8), um (9-11), p, phi, T, rT (12-15)
-  const double2 *r2 = reinterpret_cast<const double2 *>(rec + (size_t)na * 16);
-  const double2 v0 = r2[0], v1 = r2[1], v2 = r2[2], v4 = r2[4], v5 = r2[5], v6 = r2[6],
-                v7 = r2[7];
-  const double xd[3] = {v0.x, v0.y, v1.x};
-  const double ua[3] = {v1.y, v2.x, v2.y};
-  const double uma[3] = {v4.y, v5.x, v5.y};
-  const double phia = v6.y, Ta = v7.x, rTa = v7.y;
-
-  double Na, dl0, dl1, dl2, gwq;
-  elem_shape<S>(q, a, Na, dl0, dl1, dl2, gwq);
-
+  double Nq[S];
+#pragma unroll
+  for (int a = 0; a < S; ++a) Nq[a] = shape_N<S>(q, a);
   double J[9];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    J[i + 0] = sum8(xd[i] * dl0);
-    J[i + 3] = sum8(xd[i] * dl1);
-    J[i + 6] = sum8(xd[i] * dl2);
+    double p0[S], p1[S], p2[S];
+#pragma unroll
+    for (int a = 0; a < S; ++a) {
+      double Na, d0, d1, d2, gw;
+      elem_shape<S>(q, a, Na, d0, d1, d2, gw);
+      const double x = re[a * kTRec + i];
+      p0[a] = x * d0;
+      p1[a] = x * d1;
+      p2[a] = x * d2;
+    }
+    J[i + 0] = tree_sum<S>(p0);
+    J[i + 3] = tree_sum<S>(p1);
+    J[i + 6] = tree_sum<S>(p2);
   }
   // get_inverse_3x3 (eval_shape.f90:85-112), column-major
   double Ji[9];
@@ -117,10 +145,19 @@ k_assemble_tem_el(int E, const int32_t *__restrict__ ien, const int32_t *__restr
   Ji[5] = (J[2] * J[3] - J[0] * J[5]) * tmp;
   Ji[8] = (J[0] * J[4] - J[3] * J[1]) * tmp;
   const double detj = 1.0 / tmp;
-  double ga[3];
+  // dN_a/dx at this point, formed where it is used (the same expression each
+  // time, so the same bits) rather than kept for all S nodes
+  auto grad = [&](int a, double g[3]) {
+    double Na, d0, d1, d2, gw;
+    elem_shape<S>(q, a, Na, d0, d1, d2, gw);
 #pragma unroll
-  for (int j = 0; j < 3; ++j)
-    ga[j] = dl0 * Ji[0 + 3 * j] + dl1 * Ji[1 + 3 * j] + dl2 * Ji[2 + 3 * j];
+    for (int j = 0; j < 3; ++j) g[j] = d0 * Ji[0 + 3 * j] + d1 * Ji[1 + 3 * j] + d2 * Ji[2 + 3 * j];
+  };
+  double gwq;
+  {
+    double Na, d0, d1, d2;
+    elem_shape<S>(q, 0, Na, d0, d1, d2, gwq);
+  }
   double G[9];
 #pragma unroll
   for (int j = 0; j < 3; ++j)
@@ -129,15 +166,45 @@ k_assemble_tem_el(int E, const int32_t *__restrict__ ien, const int32_t *__restr
       G[i + 3 * j] = Ji[0 + 3 * i] * Ji[0 + 3 * j] + Ji[1 + 3 * i] * Ji[1 + 3 * j] +
                      Ji[2 + 3 * i] * Ji[2 + 3 * j];
 
-  // Gauss-point values (e3int_qr, e3int_qr_grad)
-  const double phii = seq8(phia * Na);
-  const double Ti = seq8(Ta * Na);
-  const double rTi = sum8(rTa * Na);
-  double uadv[3], dT[3];
+  // Gauss-point values (e3int_qr, e3int_qr_grad); the VOF value and T in
+  // node order, since they are compared against thresholds
+  double phii, Ti, rTi, ksi, uadv[3], dT[3];
+  {
+    double pf[S], pt[S], pr[S], pk[S];
 #pragma unroll
-  for (int i = 0; i < 3; ++i) uadv[i] = sum8(ua[i] * Na) - sum8(uma[i] * Na);
+    for (int a = 0; a < S; ++a) {
+      pf[a] = re[a * kTRec + 9] * Nq[a];
+      pt[a] = re[a * kTRec + 10] * Nq[a];
+      pr[a] = re[a * kTRec + 11] * Nq[a];
+      pk[a] = re[a * kTRec + 12] * Nq[a];
+    }
+    phii = seq_sum<S>(pf);
+    Ti = seq_sum<S>(pt);
+    rTi = tree_sum<S>(pr);
+    ksi = tree_sum<S>(pk);
+  }
 #pragma unroll
-  for (int j = 0; j < 3; ++j) dT[j] = sum8(Ta * ga[j]);
+  for (int i = 0; i < 3; ++i) {
+    double pu[S], pm[S];
+#pragma unroll
+    for (int a = 0; a < S; ++a) {
+      pu[a] = re[a * kTRec + 3 + i] * Nq[a];
+      pm[a] = re[a * kTRec + 6 + i] * Nq[a];
+    }
+    uadv[i] = tree_sum<S>(pu) - tree_sum<S>(pm);
+  }
+  {
+    double pd[3][S];
+#pragma unroll
+    for (int a = 0; a < S; ++a) {
+      double g[3];
+      grad(a, g);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) pd[j][a] = re[a * kTRec + 10] * g[j];
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) dT[j] = tree_sum<S>(pd[j]);
+  }
 
   // phase change (solveQuenching.f90:742-752)
   const double phic = fmax(fmin(phii, 1.0), 0.0);
@@ -163,7 +230,7 @@ k_assemble_tem_el(int E, const int32_t *__restrict__ ien, const int32_t *__restr
     rhocpi = P.rhos * P.cps;
   }
   // per-node conductivity scale (an extension; none: 1)
-  if (ks) hki = hki * sum8(ks[na] * Na);
+  if (ks) hki = hki * ksi;
   const double Se = ((P.cpw - P.cpa) * (Ti - P.Ts) - P.lh) * mdot;
   const double udT = uadv[0] * dT[0] + uadv[1] * dT[1] + uadv[2] * dT[2];
   // the second-derivative term of the residual is zero for these elements
@@ -183,90 +250,172 @@ k_assemble_tem_el(int E, const int32_t *__restrict__ ien, const int32_t *__restr
   double kdc = 0.0;
   if (fabs(P.tem_kdc) > 0.0 && fluid) kdc = P.tem_kdc * fabs(res) / sqrt(gg) / P.tref;
   const double dSedT = (P.cpw - P.cpa) * mdot + ((P.cpw - P.cpa) * (Ti - P.Ts) - P.lh) * dmdT;
-  const double sc = ga[0] * uadv[0] + ga[1] * uadv[1] + ga[2] * uadv[2];
   const double hk = fluid ? hki + kdc : hki;
-  const double dw = detj * gwq;  // hex weights are 1; padded tet points 0
-  if (valid && a == 0 && (S == 8 || q < 4) && detj < 0.0) atomicAdd(negdet, 1);
-
-  s_sh[w][q][a][0] = Na;
-  s_sh[w][q][a][1] = ga[0];
-  s_sh[w][q][a][2] = ga[1];
-  s_sh[w][q][a][3] = ga[2];
-  s_sh[w][q][a][4] = sc;
-  if (a == 0) {
-    s_q[w][q][0] = rhocpi;
-    s_q[w][q][1] = tau;
-    s_q[w][q][2] = dSedT;
-    s_q[w][q][3] = hk;
-    s_q[w][q][4] = dw;
+  const double dw = detj * gwq;  // hexes: unit weights
+  if (valid && detj < 0.0) atomicAdd(negdet, 1);
+  if (valid && do_mat) {
+    double2 *pq = reinterpret_cast<double2 *>(po + q * L::Q);
+    pq[0] = make_double2(Ji[0], Ji[1]);
+    pq[1] = make_double2(Ji[2], Ji[3]);
+    pq[2] = make_double2(Ji[4], Ji[5]);
+    pq[3] = make_double2(Ji[6], Ji[7]);
+    pq[4] = make_double2(Ji[8], uadv[0]);
+    pq[5] = make_double2(uadv[1], uadv[2]);
+    pq[6] = make_double2(rhocpi, tau);
+    pq[7] = make_double2(dSedT, hk);
+    pq[8] = make_double2(dw, 0.0);
   }
 
-  // RHS of node a at Gauss point q (:841-855), then summed over q
-  double rt = 0.0;
+  // RHS of node a at this point (:841-855), summed over the S points in the
+  // butterfly association; lane q keeps node a = q
   if (do_vec) {
-    if (fluid) {
-      rt = rt - Na * rhocpi * (rTi + udT) * dw;
-      rt = rt + Na * Se * dw;
-      rt = rt - rhocpi * tau * sc * res * dw;
-    } else {
-      rt = rt - Na * rhocpi * rTi * dw;
-    }
-    rt = rt - hk * ga[0] * dT[0] * dw;
-    rt = rt - hk * ga[1] * dT[1] * dw;
-    rt = rt - hk * ga[2] * dT[2] * dw;
-  }
-  rt += __shfl_xor(rt, 8);
-  rt += __shfl_xor(rt, 16);
-  rt += __shfl_xor(rt, 32);
-
-  __syncthreads();
-
-  // ---------------- phase B: lane = (a, b) ----------------
-  const int pa = lane & 7;
-  const int pb = lane >> 3;
-  double acc = 0.0;
-  if (do_mat) {
-    const double f1 = P.almi, f2 = P.alfi * P.gami * P.delt;
-#pragma unroll 2
-    for (int qq = 0; qq < (S == 8 ? 8 : 4); ++qq) {
-      const double Nq = s_sh[w][qq][pa][0], g0 = s_sh[w][qq][pa][1],
-                   g1 = s_sh[w][qq][pa][2], g2 = s_sh[w][qq][pa][3],
-                   sca = s_sh[w][qq][pa][4];
-      const double Mb = s_sh[w][qq][pb][0], h0 = s_sh[w][qq][pb][1],
-                   h1 = s_sh[w][qq][pb][2], h2 = s_sh[w][qq][pb][3],
-                   scb = s_sh[w][qq][pb][4];
-      const double rc = s_q[w][qq][0], tq = s_q[w][qq][1], dS = s_q[w][qq][2],
-                   kq = s_q[w][qq][3], dj = s_q[w][qq][4];
-      const double gab = g0 * h0 + g1 * h1 + g2 * h2;
+    double own = 0.0;
+#pragma unroll
+    for (int a = 0; a < S; ++a) {
+      const double Na = Nq[a];
+      double ga[3];
+      grad(a, ga);
+      double rt = 0.0;
       if (fluid) {
-        const double lb = f1 * Mb + f2 * scb;
-        acc = acc + Nq * rc * lb * dj - Nq * f2 * dS * Mb * dj + tq * rc * sca * rc * lb * dj -
-              tq * rc * sca * f2 * dS * Mb * dj + f2 * kq * gab * dj;
+        const double sc = ga[0] * uadv[0] + ga[1] * uadv[1] + ga[2] * uadv[2];
+        rt = rt - Na * rhocpi * (rTi + udT) * dw;
+        rt = rt + Na * Se * dw;
+        rt = rt - rhocpi * tau * sc * res * dw;
       } else {
-        acc = acc + Nq * rc * f1 * Mb * dj + f2 * kq * gab * dj;
+        rt = rt - Na * rhocpi * rTi * dw;
       }
+      rt = rt - hk * ga[0] * dT[0] * dw;
+      rt = rt - hk * ga[1] * dT[1] * dw;
+      rt = rt - hk * ga[2] * dT[2] * dw;
+      const double sm = point_sum<S>(rt);
+      if (q == a) own = sm;
     }
-  }
-
-  // ---------------- phase C: BCLhs_tem + scatter ----------------
-  if (valid && pa < S && pb < S) {
-    const unsigned ca = (ibc[s_node[w][pa]] >> 5) & 1u;
-    const unsigned cb = (ibc[s_node[w][pb]] >> 5) & 1u;
-    if (do_mat) {
-      double v = (ca || cb) ? 0.0 : acc;
-      if (pa == pb && ca) v = 1.0;
-      const int k = emap[(size_t)e * 64 + lane];
-      // the first element (in colour order) to reach the nonzero stores
-      A[k] = ((firstmask[e] >> lane) & 1ull) ? v : A[k] + v;
-    }
-    if (do_vec && lane < S) rhs[na] += ca ? 0.0 : rt;
+    if (valid) po[L::R + q] = own;
   }
 }
 
-hipError_t launch_assemble_tem_hex(Ctx &c, int tensor_flag, int color) {
-  const int first = c.color_off[color];
-  const int count = c.color_off[color + 1] - first;
-  if (count <= 0) return hipSuccess;
+// One colour's elements at positions first .. first + count - 1 of the
+// colour order, 64 / S^2 per wavefront (one hex, four tets); the point
+// data, the scatter map and the first-touch masks are indexed by position.
+template <int S>
+__global__ void __launch_bounds__(256)
+k_tem_pairs(int first, int count, const int32_t *__restrict__ emap_c,
+            const uint64_t *__restrict__ first_c, const double *__restrict__ pts,
+            const uint8_t *__restrict__ ibc, double f1, double f2, double *__restrict__ A) {
+  using L = TemPts<S>;
+  constexpr int SS = S * S;
+  constexpr int EW = 64 / SS;  // elements per wavefront
+  __shared__ __align__(16) double s_g[kWaves][64 * 4];      // [slot][q][a]: dN/dx, streamline
+  __shared__ __align__(16) double s_f[kWaves][EW * S * 6];  // [slot][q]: 5 point factors
+  __shared__ double s_n[kWaves][SS];                         // [q][a]: N_a at point q
+  __shared__ uint32_t s_m[kWaves][16];                       // [slot][a]: T Dirichlet flag
+  __shared__ int32_t s_fl[kWaves][4];                        // [slot]: fluid element
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int t0 = first + (blockIdx.x * kWaves + w) * EW;
+  if (t0 >= first + count) return;  // whole wavefronts; no block barrier below
+  const int nv = (first + count - t0 < EW) ? first + count - t0 : EW;  // live slots
+  // lane = (slot, a, b): its nonzero, first-touch flag and old value
+  const int sl = lane / SS, rr = lane % SS;
+  const int pa = rr % S, pb = rr / S;
+  const bool live = sl < nv;
+  const int tl = t0 + (live ? sl : 0);
+  const int k = emap_c[(size_t)tl * 64 + pa + 8 * pb];
+  const bool fst = (first_c[tl] >> (pa + 8 * pb)) & 1ull;
+  const double old = (fst || !live) ? 0.0 : A[k];
+  if (lane < EW * S) {
+    const int s2 = lane / S, x2 = lane % S;
+    if (s2 < nv) {
+      const double *pe = pts + (size_t)(t0 + s2) * L::stride;
+      const int32_t *pn = reinterpret_cast<const int32_t *>(pe + L::N);
+      s_m[w][lane] = (ibc[pn[x2]] >> 5) & 1u;
+      if (x2 == 0) s_fl[w][s2] = pn[S];
+      const double *pf = pe + x2 * L::Q + 12;
+      double *sf = s_f[w] + lane * 6;
+#pragma unroll
+      for (int u = 0; u < 5; ++u) sf[u] = pf[u];
+    }
+  }
+  if (lane < SS) s_n[w][lane] = shape_N<S>(lane / S, lane % S);
+  // lane = (slot, q, a): dN_a/dx and uadv . dN_a/dx at point q
+  {
+    const int s3 = lane / SS, r3 = lane % SS, q3 = r3 / S, a3 = r3 % S;
+    if (s3 < nv) {
+      const double2 *pj =
+          reinterpret_cast<const double2 *>(pts + (size_t)(t0 + s3) * L::stride + q3 * L::Q);
+      const double2 j0 = pj[0], j1 = pj[1], j2 = pj[2], j3 = pj[3], j4 = pj[4], j5 = pj[5];
+      const double Ji[9] = {j0.x, j0.y, j1.x, j1.y, j2.x, j2.y, j3.x, j3.y, j4.x};
+      const double uadv[3] = {j4.y, j5.x, j5.y};
+      double Na, d0, d1, d2, gw;
+      elem_shape<S>(q3, a3, Na, d0, d1, d2, gw);
+      double g[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        g[j] = d0 * Ji[0 + 3 * j] + d1 * Ji[1 + 3 * j] + d2 * Ji[2 + 3 * j];
+      const double sc = g[0] * uadv[0] + g[1] * uadv[1] + g[2] * uadv[2];
+      double2 *sg = reinterpret_cast<double2 *>(s_g[w]) + lane * 2;
+      sg[0] = make_double2(g[0], g[1]);
+      sg[1] = make_double2(g[2], sc);
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+
+  if (!live) return;
+  // entry (a, b) over the points, the reference's term order (:788-838)
+  double acc = 0.0;
+  const bool fluid = s_fl[w][sl] != 0;
+  const double *sg = s_g[w] + sl * SS * 4;  // [q][a][4]
+  const double *sf = s_f[w] + sl * S * 6;   // [q][6]
+  const double *sn = s_n[w];                // [q][a]
+#pragma unroll 2
+  for (int qq = 0; qq < S; ++qq) {
+    const double *ra = sg + (qq * S + pa) * 4, *rb = sg + (qq * S + pb) * 4;
+    const double g0 = ra[0], g1 = ra[1], g2 = ra[2], sca = ra[3];
+    const double h0 = rb[0], h1 = rb[1], h2 = rb[2], scb = rb[3];
+    const double Nq = sn[qq * S + pa], Mb = sn[qq * S + pb];
+    const double rc = sf[qq * 6 + 0], tq = sf[qq * 6 + 1], dS = sf[qq * 6 + 2],
+                 kq = sf[qq * 6 + 3], dj = sf[qq * 6 + 4];
+    const double gab = g0 * h0 + g1 * h1 + g2 * h2;
+    if (fluid) {
+      const double lb = f1 * Mb + f2 * scb;
+      acc = acc + Nq * rc * lb * dj - Nq * f2 * dS * Mb * dj + tq * rc * sca * rc * lb * dj -
+            tq * rc * sca * f2 * dS * Mb * dj + f2 * kq * gab * dj;
+    } else {
+      acc = acc + Nq * rc * f1 * Mb * dj + f2 * kq * gab * dj;
+    }
+  }
+  // BCLhs_tem: rows and columns of fixed-T nodes cleared, 1 on the diagonal
+  const unsigned ca = s_m[w][sl * S + pa], cb = s_m[w][sl * S + pb];
+  double v = (ca || cb) ? 0.0 : acc;
+  if (pa == pb && ca) v = 1.0;
+  // the first element (in colour order) to reach the nonzero stores
+  A[k] = fst ? v : old + v;
+}
+
+// rhs[n] = sum of the node's element contributions in colour order
+// (inc_off/inc: entries t * 8 + a), 0 for a fixed-T node
+__global__ void __launch_bounds__(256)
+k_tem_rhs_gather(int N, const int32_t *__restrict__ inc_off, const int32_t *__restrict__ inc,
+                 const double *__restrict__ pts, int stride, int roff,
+                 const uint8_t *__restrict__ ibc, double *__restrict__ rhs) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  const bool fixed = (ibc[n] >> 5) & 1u;
+  double s = 0.0;
+  for (int t = inc_off[n]; t < inc_off[n + 1]; ++t) {
+    const int ea = inc[t];
+    const double v = pts[(size_t)(ea >> 3) * stride + roff + (ea & 7)];
+    s += fixed ? 0.0 : v;
+  }
+  rhs[n] = s;
+}
+
+// The volume loop of assembleQuenching's temperature field: the point
+// kernel, the pair kernel per colour (MAT), the RHS gather (VEC; the caller
+// zeroes the padded RHS).  c.d_pts holds at least the hex / tet stride of
+// the NS assembly's point data per element.
+hipError_t launch_assemble_tem(Ctx &c, int tensor_flag) {
   const double *q = c.qprm;
   TemParams P;
   P.almi = c.prm[OFD_P_ALMI];
@@ -290,15 +439,38 @@ hipError_t launch_assemble_tem_hex(Ctx &c, int tensor_flag, int color) {
   P.tem_kdc = q[OFD_Q_TEMKDC];
   P.tref = q[OFD_Q_TREF];
   P.fluid_id = (int)q[OFD_Q_FLUID_ID];
-  const int blocks = (count + kWaves - 1) / kWaves;
-  hipLaunchKernelGGL((c.nshl == 4 ? k_assemble_tem_el<4> : k_assemble_tem_el<8>), dim3(blocks),
-                     dim3(64 * kWaves), 0, c.stream,
-                     c.nelem, c.d_ien, c.d_elm_id, c.d_color_elems, first, count, c.d_emap,
-                     c.d_first, c.d_noderec, c.ks_on ? c.d_field[OFD_KSCALE] : nullptr,
-                     c.d_ibc, P,
-                     (tensor_flag & OFD_ASSEMBLE_MAT) ? 1 : 0,
-                     (tensor_flag & OFD_ASSEMBLE_VEC) ? 1 : 0, c.d_AT, c.d_rhsT, c.d_negdet);
-  return hipGetLastError();
+  const int do_mat = (tensor_flag & OFD_ASSEMBLE_MAT) ? 1 : 0;
+  const int do_vec = (tensor_flag & OFD_ASSEMBLE_VEC) ? 1 : 0;
+  const bool hex = c.nshl == 8;
+  const int epw = 64 / c.nshl;                            // point kernel: per wavefront
+  const int epb = 64 / (c.nshl * c.nshl) * kWaves;        // pair kernel: per block
+  const double *ks = c.ks_on ? c.d_field[OFD_KSCALE] : nullptr;
+  hipLaunchKernelGGL((hex ? k_tem_points<8> : k_tem_points<4>),
+                     dim3((c.nelem + epw - 1) / epw), dim3(64), 0, c.stream, c.nelem, c.d_ien,
+                     c.d_elm_id, c.d_cpos, c.d_noderec, ks, P, do_mat, do_vec, c.d_pts,
+                     c.d_negdet);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (do_mat) {
+    const double f1 = P.almi, f2 = P.alfi * P.gami * P.delt;
+    for (int col = 0; col + 1 < (int)c.color_off.size(); ++col) {
+      const int first = c.color_off[col];
+      const int count = c.color_off[col + 1] - first;
+      if (count <= 0) continue;
+      hipLaunchKernelGGL((hex ? k_tem_pairs<8> : k_tem_pairs<4>),
+                         dim3((count + epb - 1) / epb), dim3(64 * kWaves), 0, c.stream, first,
+                         count, c.d_emap_c, c.d_first_c, c.d_pts, c.d_ibc, f1, f2, c.d_AT);
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+  }
+  if (do_vec) {
+    hipLaunchKernelGGL(k_tem_rhs_gather, dim3((c.nnode + 255) / 256), dim3(256), 0, c.stream,
+                       c.nnode, c.d_inc_off, c.d_inc, c.d_pts,
+                       hex ? TemPts<8>::stride : TemPts<4>::stride,
+                       hex ? TemPts<8>::R : TemPts<4>::R, c.d_ibc, c.d_rhsT);
+    e = hipGetLastError();
+  }
+  return e;
 }
 
 }  // namespace ofd

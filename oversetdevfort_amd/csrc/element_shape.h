@@ -15,6 +15,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "wave_dpp.h"
+
 namespace ofd {
 
 template <int S>
@@ -69,6 +71,52 @@ __device__ __forceinline__ void elem_shape(int q, int a, double &Na, double &dl0
     dl2 = (a == 2) ? 1.0 : (a == 3) ? -1.0 : 0.0;
     gwq = (q < 4) ? 0.2500000000000000 / 6.0 : 0.0;
   }
+}
+
+// ---- sums over an element's S nodes or S Gauss points, for the kernels
+// whose lane is (element, Gauss point) ----
+
+// the sum of S products in the lane-butterfly association (S = 8:
+// ((p0+p1)+(p2+p3))+((p4+p5)+(p6+p7)); S = 4: (p0+p1)+(p2+p3)); the adds
+// carry no contraction, so no product is fused into them
+template <int S>
+__device__ __forceinline__ double tree_sum(const double (&p)[S]) {
+#pragma clang fp contract(off)
+  if constexpr (S == 8)
+    return ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7]));
+  else
+    return (p[0] + p[1]) + (p[2] + p[3]);
+}
+
+// sum and maximum over the S lanes of an element's Gauss points
+template <int S>
+__device__ __forceinline__ double point_sum(double v) {
+  if constexpr (S == 8) return wsum8(v);
+  else return wsum4(v);
+}
+template <int S>
+__device__ __forceinline__ double point_max(double v) {
+  if constexpr (S == 8) return wmax8(v);
+  else return wmax4(v);
+}
+
+// the shape value of node a at Gauss point q (elem_shape<S>'s Na)
+template <int S>
+__device__ __forceinline__ double shape_N(int q, int a) {
+  double Na, d0, d1, d2, w;
+  elem_shape<S>(q, a, Na, d0, d1, d2, w);
+  return Na;
+}
+
+// the sum of S products in node order, (((p0 + p1) + p2) + ...), the order of
+// the reference's e3int_qr; no product is fused into the adds
+template <int S>
+__device__ __forceinline__ double seq_sum(const double (&p)[S]) {
+#pragma clang fp contract(off)
+  double s = p[0];
+#pragma unroll
+  for (int a = 1; a < S; ++a) s = s + p[a];
+  return s;
 }
 
 }  // namespace ofd

@@ -1497,6 +1497,8 @@ int ofd_time_quench(ofd_ctx *x, int what, int arg, int reps, double *ms) {
               "the GPU element assembly is implemented for trilinear hexes and linear tets");
     OFD_CHECK((arg & ~(OFD_ASSEMBLE_VEC | OFD_ASSEMBLE_MAT)) == 0, OFD_ERR_ARG,
               "arg must combine OFD_ASSEMBLE_VEC / OFD_ASSEMBLE_MAT");
+    rc = ensure_pts(c);
+    if (rc) return rc;
   } else {
     OFD_CHECK(arg == OFD_FP64 || arg == OFD_FP32, OFD_ERR_ARG, "bad precision");
     OFD_CHECK(nsvof ? c.matV_valid : c.matT_valid, OFD_ERR_ARG, "no assembled system");
@@ -1518,7 +1520,7 @@ int ofd_time_quench(ofd_ctx *x, int what, int arg, int reps, double *ms) {
     } else if (what == OFD_TIME_ASM_TEM) {
       OFD_HIP(launch_alpha_stage(c, false));
       if (arg & OFD_ASSEMBLE_VEC) OFD_HIP(launch_zero(c, c.d_rhsT, (size_t)c.temT_n4 * 4));
-      for (int col = 0; col < ncol; ++col) OFD_HIP(launch_assemble_tem_hex(c, arg, col));
+      OFD_HIP(launch_assemble_tem(c, arg));
     } else if (what == OFD_TIME_SPMV_NSVOF) {
       NsvofSystem sys(c);
       OFD_HIP(launch_spmv_v(c, arg, c.d_xV, c.d_yV, nullptr, 0));
@@ -1615,6 +1617,8 @@ int ofd_assemble_tem(ofd_ctx *x, int tensor_flag) {
   OFD_CHECK((tensor_flag & ~(OFD_ASSEMBLE_VEC | OFD_ASSEMBLE_MAT)) == 0, OFD_ERR_ARG,
             "tensor_flag must combine OFD_ASSEMBLE_VEC / OFD_ASSEMBLE_MAT");
   OFD_HIP(hipSetDevice(c.device));
+  rc = ensure_pts(c);
+  if (rc) return rc;
   OFD_HIP(launch_alpha_stage(c));
   if (tensor_flag & OFD_ASSEMBLE_VEC)
     OFD_HIP(launch_zero(c, c.d_rhsT, (size_t)c.temT_n4 * 4));
@@ -1625,8 +1629,7 @@ int ofd_assemble_tem(ofd_ctx *x, int tensor_flag) {
     ++c.matT_version;
   }
   OFD_HIP(hipMemsetAsync(c.d_negdet, 0, sizeof(int32_t), c.stream));
-  for (int col = 0; col + 1 < (int)c.color_off.size(); ++col)
-    OFD_HIP(launch_assemble_tem_hex(c, tensor_flag, col));
+  OFD_HIP(launch_assemble_tem(c, tensor_flag));
   int32_t neg = 0;
   OFD_HIP(hipMemcpyAsync(&neg, c.d_negdet, sizeof(int32_t), hipMemcpyDeviceToHost,
                          c.stream));

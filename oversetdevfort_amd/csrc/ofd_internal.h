@@ -349,7 +349,7 @@ hipError_t launch_outer_reset(Ctx &c, double thr);
 hipError_t launch_norm2_up(Ctx &c, const double *v, int slot);
 int nblocks_gm(const Ctx &c);
 // temperature system
-hipError_t launch_assemble_tem_hex(Ctx &c, int tensor_flag, int color);
+hipError_t launch_assemble_tem(Ctx &c, int tensor_flag);
 hipError_t launch_spmv_s(Ctx &c, int precision, const double *x, double *y,
                          const double *b, int mode);
 hipError_t launch_build_prec_s(Ctx &c, bool want32);
