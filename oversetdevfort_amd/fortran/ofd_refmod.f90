@@ -12,9 +12,11 @@
 !     FaceAssembly_ConvectiveH) are not part of this drop-in.
 !   * interpolation_tet(mesh_id) (interpolation.f90:5-76): fills
 !     bmesh(mesh_id)%ug from the device donor search over the elements with
-!     ELM_ID == SRC_ID_SET(mesh_id).  The reference's setBCs_overset
-!     (setBCs.f90:124-167) calls it unchanged, so the overset boundary
-!     condition reaches the device with no drop-in of its own.
+!     ELM_ID == SRC_ID_SET(mesh_id).  The reference's own setBCs_overset
+!     runs unchanged over it as well.
+!   * setBCs_overset(subdomain_flag) (setBCs.f90:124-167): setBCs_CFD as
+!     the reference, the other subdomain's nodes constrained, the donor
+!     search and the receivers' ug, acg and IBC written on the device.
 !   * genSparStruc() (genSparStruc.f90:4-47): uploads the module mesh (IEN,
 !     ELM_ID, xg) and builds the pattern on the device, then allocates and
 !     fills the module's col, row and icnt.
@@ -28,7 +30,8 @@
 ! With these, solveOverset_stag (solveOverset.f90:4-121) runs unchanged over
 ! the module state: genSparStruc once, then per Newton iteration
 ! setBCs_overset, assembleNavSto, SparseGMRES_up (ofd_refapi.f90, on the
-! module arrays) and the update.
+! module arrays) and the update.  setBCs_overset replaces the reference's
+! when setBCs.f90 is compiled with -DsetBCs_overset=setBCs_overset_cpu.
 !======================================================================
 module ofd_refmod
   use ofd_state
@@ -208,3 +211,59 @@ subroutine assembleNavSto(assemble_tensor_flag, subdomain_flag)
   if (iand(assemble_tensor_flag, ASSEMBLE_TENSOR_MAT) > 0) &
     call ofd_check(ofd_get_lhs(ofd_ctx, LHSK11, LHSG, LHSD1, LHSM), 'assembleNavSto')
 end subroutine assembleNavSto
+
+!----------------------------------------------------------------------
+! setBCs_overset(subdomain_flag) (setBCs.f90:124-167) with the fringe on
+! the device.  setBCs_CFD runs as in the reference (it stays in setBCs.f90,
+! compiled with setBCs_overset renamed).  For every subdomain isub the flag
+! selects: the other subdomain's nodes (NodeID == ELM_ID_SET(3-isub)) get
+! IBC(:,1:4) = 1; the face whose FACE_ID is BMESH_ID_SET(isub) names the
+! receivers; the donor search (interpolation_tet) fills bmesh(isub)%ug; and
+! ofd_apply_fringe sets IBC(k,1:3) = 1, ug and the Dirichlet-consistent
+! acg = ((ug - ugold) Dtgl + (gami - 1) acgold) / gami of each receiver k
+! on the device, from which ug and acg return into the module.  A selected
+! subdomain without its face stops with a message (the reference would
+! index bound(0)).
+!----------------------------------------------------------------------
+subroutine setBCs_overset(subdomain_flag)
+  use ofd_refmod
+  implicit none
+  integer, intent(in) :: subdomain_flag
+  integer :: isub, b, ifac, i, n
+  integer(c_int32_t), allocatable :: nodes(:)
+  real(8), allocatable :: val(:, :)
+  call setBCs_CFD()
+  if (iand(subdomain_flag, 3) == 0) return
+  call ofd_push_ns_state()
+  do isub = 1, NRES
+    if (iand(subdomain_flag, ishft(1, isub - 1)) == 0) cycle
+    do i = 1, NNODE
+      if (NodeID(i) == ELM_ID_SET(3 - isub)) IBC(i, 1:4) = 1
+    end do
+    ifac = 0
+    do b = 1, NBOUND
+      if (bound(b)%FACE_ID == BMESH_ID_SET(isub)) then
+        ifac = b
+        exit
+      end if
+    end do
+    if (ifac == 0) then
+      write (*, *) 'setBCs_overset: no boundary with FACE_ID = BMESH_ID_SET(', isub, ')'
+      error stop 911
+    end if
+    call interpolation_tet(isub)
+    n = bound(ifac)%NNODE
+    allocate (nodes(n), val(n, NSD))
+    do i = 1, n
+      nodes(i) = int(bound(ifac)%BNODES(i), c_int32_t)
+      val(i, :) = bmesh(isub)%ug(bound(ifac)%L2SNODE(i), :)
+      IBC(bound(ifac)%BNODES(i), 1:3) = 1
+    end do
+    call ofd_check(ofd_apply_fringe(ofd_ctx, int(n, c_int), nodes, val), 'setBCs_overset')
+    deallocate (nodes, val)
+    ! the next subdomain's donors see these receivers' new values, as in the
+    ! reference's loop
+    call ofd_check(ofd_get_field(ofd_ctx, OFD_UG, ug), 'setBCs_overset')
+    call ofd_check(ofd_get_field(ofd_ctx, OFD_ACG, acg), 'setBCs_overset')
+  end do
+end subroutine setBCs_overset

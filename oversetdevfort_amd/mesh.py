@@ -96,9 +96,10 @@ def hex_to_tets(mesh: Mesh) -> Mesh:
 
     Tet node order follows ``lintetshl`` (N1=xi, N2=eta, N3=zeta,
     N4=1-xi-eta-zeta; eval_shape_FEM.f90:201-225): node 4 is the vertex the
-    local frame starts from.  Each tet is made positively oriented.
+    local frame starts from.  Each tet is made positively oriented.  The
+    hexes need not come from one structured block (the overset pair's two
+    blocks split the same way).
     """
-    nx, ny, nz = mesh.dims
     # local corner id -> linhexshl slot
     slot = {tuple(c): a for a, c in enumerate(HEX_CORNERS.tolist())}
     perms = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]

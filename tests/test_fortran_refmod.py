@@ -1,5 +1,6 @@
 """The drop-ins that use the reference's module state (ofd_refmod.f90:
-assembleQuenching, interpolation_tet, genSparStruc, assembleNavSto) compile against the reference's own
+assembleQuenching, interpolation_tet, genSparStruc, assembleNavSto,
+setBCs_overset) compile against the reference's own
 modules (aAdjKeep, commonvars, commonpars, as built under oracle/_ref) and
 export the link names the reference's callers resolve to.  Compile-only:
 running them needs the reference's program and a GPU."""
@@ -28,9 +29,9 @@ def test_refmod_compiles_against_reference_modules(tmp_path):
                           capture_output=True, text=True).stdout.split()
     ref = subprocess.run(["nm"] + [os.path.join(REFMOD, f) for f in
                                    ("solveQuenching.o", "interpolation.o", "genSparStruc.o",
-                                    "assembleNavSto.o")], check=True,
+                                    "assembleNavSto.o", "setBCs.o")], check=True,
                          capture_output=True, text=True).stdout.split()
     for name in ("assemblequenching_", "interpolation_tet_", "gensparstruc_",
-                 "assemblenavsto_"):
+                 "assemblenavsto_", "setbcs_overset_"):
         assert name in syms, name
         assert name in ref, name
