@@ -474,6 +474,50 @@ int ofd_solve_tem_adjoint(ofd_ctx *ctx, int precision, int ortho, double rtol,
                           double inner_tol, int kspace, int max_cycles, const double *g,
                           int *iters, int *cycles, double *relres);
 
+/* ---- exact NS + VOF Jacobian and the coupled quench adjoint (no reference
+ * code) ----
+ *
+ * The residual is the assembled RHSGu/RHSGp/RHSGls of
+ * IntElmAss_NSVOF_Quenching_STAB (solveQuenching.f90:122-573), R(u, ac, p,
+ * phi, rphi, T) at the alpha stage, fluid elements only, before the
+ * Dirichlet rows of IBC columns 1-3, 7, 8 are cleared.  The Newton
+ * increments (acg += d, ug += gami*Delt*d, pg += alfi*gami*Delt*dp, rphig +=
+ * dphi, phig += gami*Delt*dphi; solveFlow.f90:99-104) give J = -dR/d(d, dp,
+ * dphi).  Derivatives come from forward-mode dual numbers through the whole
+ * element residual (no hand differentiation); hexes only. */
+
+/* Replace the stored NS + VOF matrix (the layout ofd_get_nsvof_system
+ * reads) by J_bc: J with the rows and columns of constrained dofs cleared
+ * and 1 on their diagonal once per element.  The (phi row, p column) entry,
+ * which the layout has no slot for, is zero in J.  ofd_solve_nsvof then
+ * solves the exact Newton correction. */
+int ofd_assemble_nsvof_jac(ofd_ctx *ctx);
+
+/* Gradients of lambda . R for lambda = (lu (N,3), lp, lphi) with respect to
+ * the alpha-stage fields: g is [10][N], rows u (3), ac (3), p, phi, rphi, T;
+ * no Dirichlet projection. */
+int ofd_vjp_nsvof(ofd_ctx *ctx, const double *lu, const double *lp, const double *lphi,
+                  double *g);
+
+/* Gradients of lambda . R_T (the temperature residual of ofd_vjp_tem) by
+ * dual numbers, including the VOF field: g is [7][N], rows T, rT, u (3),
+ * phi, OFD_KSCALE. */
+int ofd_vjp_tem_dual(ofd_ctx *ctx, const double *lam, double *g);
+
+/* y = A^T x for the stored NS + VOF matrix (FP64; (N,3), (N), (N)). */
+int ofd_spmv_nsvof_t(ofd_ctx *ctx, const double *xu, const double *xp, const double *xl,
+                     double *yu, double *yp, double *yl);
+
+/* A^T lambda = g for the stored NS + VOF matrix: restarted flexible GMRES
+ * (FP64 basis) on the exact transposed product, right-preconditioned by the
+ * restarted device GMRES on the stored transpose without its (phi row, p
+ * column) coupling, solved to inner_tol in `precision` (ortho as
+ * ofd_solve_nsvof, OFD_PREC_RIGHT allowed).  lambda goes to lu, lp, ll. */
+int ofd_solve_nsvof_t(ofd_ctx *ctx, int precision, int ortho, double rtol, double inner_tol,
+                      int kspace, int max_cycles, const double *gu, const double *gp,
+                      const double *gl, double *lu, double *lp, double *ll, int *iters,
+                      int *cycles, double *relres);
+
 /* Mean device time (HIP events on the context stream) of `reps` back-to-back
  * launches of one adjoint kernel on the current state, all colour launches
  * and the zeroing of its output included: what = OFD_TIME_VJP_NS (raw

@@ -131,6 +131,12 @@ SIGNATURES = {
     "ofd_jt_apply_tem": ([_vp, _vp, _vp], _c_int),
     "ofd_solve_tem_adjoint": ([_vp, _c_int, _c_int, _c_dbl, _c_dbl, _c_int, _c_int, _vp, _ip,
                                _ip, _dp], _c_int),
+    "ofd_assemble_nsvof_jac": ([_vp], _c_int),
+    "ofd_vjp_nsvof": ([_vp, _vp, _vp, _vp, _vp], _c_int),
+    "ofd_vjp_tem_dual": ([_vp, _vp, _vp], _c_int),
+    "ofd_spmv_nsvof_t": ([_vp] + [_vp] * 6, _c_int),
+    "ofd_solve_nsvof_t": ([_vp, _c_int, _c_int, _c_dbl, _c_dbl, _c_int, _c_int] + [_vp] * 6 +
+                          [_ip, _ip, _dp], _c_int),
     "ofd_fringe_interp": ([_vp, ctypes.c_int32, _c_int, _vp, _vp, _vp, _vp], _c_int),
     "ofd_fringe_interp_noalloc": ([_vp, ctypes.c_int32, _c_int, _vp, _vp, _vp, _vp], _c_int),
     "ofd_apply_fringe": ([_vp, _c_int, _vp, _vp], _c_int),

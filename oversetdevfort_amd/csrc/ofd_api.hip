@@ -95,6 +95,23 @@ static void free_tem_transpose(Ctx &c) {
   c.tT_mat_valid = c.tT_prec_valid = c.tT_a32_valid = c.tT_ap32_valid = false;
 }
 
+// NS + VOF adjoint: the transposed matrix and its solver copies, vectors
+static void free_nsvof_adjoint(Ctx &c) {
+  dfree(c.t_AV);
+  dfree(c.t_AV32);
+  dfree(c.t_PV);
+  dfree(c.t_ApV);
+  dfree(c.t_ApV32);
+  c.t_matV_valid = c.t_precV_valid = c.t_pV_valid = c.t_aV32_valid = c.t_apV32_valid = false;
+  c.t_nsv_icnt = 0;
+  dfree(c.d_lamV);
+  dfree(c.d_lamVb);
+  dfree(c.d_lamVd);
+  dfree(c.d_jtV);
+  dfree(c.d_vjpV);
+  c.vadj_n4 = 0;
+}
+
 // temperature adjoint vectors (sized by the mesh)
 static void free_tem_adjoint(Ctx &c) {
   dfree(c.d_lamT);
@@ -571,6 +588,7 @@ int ofd_ctx_destroy(ofd_ctx *x) {
   c.fg_n = 0;
   free_tem_transpose(c);
   free_tem_adjoint(c);
+  free_nsvof_adjoint(c);
   dfree(c.d_AV);
   dfree(c.d_AV32);
   dfree(c.d_PV);
@@ -653,6 +671,7 @@ int ofd_upload_mesh(ofd_ctx *x, int nnode, int nelem, int nshl, const int32_t *i
   dfree(c.d_fe_mask);
   free_tem_transpose(c);
   free_tem_adjoint(c);
+  free_nsvof_adjoint(c);
   // adjoint buffers are sized by the mesh and the pattern
   dfree(c.d_jt);
   dfree(c.d_lam);
@@ -1539,6 +1558,7 @@ int ofd_time_quench(ofd_ctx *x, int what, int arg, int reps, double *ms) {
     if (nsvof) {
       c.matV_valid = true;
       c.precV_valid = c.pV_valid = c.aV32_valid = c.apV32_valid = false;
+      c.t_matV_valid = false;
     } else {
       c.matT_valid = true;
       c.precT_valid = c.aT32_valid = c.apT32_valid = false;
@@ -1858,6 +1878,7 @@ int ofd_assemble_nsvof(ofd_ctx *x, int tensor_flag, double *cfl) {
     // no zeroing: the first element to reach a nonzero stores into it
     c.matV_valid = true;
     c.precV_valid = c.pV_valid = c.aV32_valid = c.apV32_valid = false;
+    c.t_matV_valid = false;
   }
   OFD_HIP(hipMemsetAsync(c.d_negdet, 0, sizeof(int32_t), c.stream));
   for (int col = 0; col + 1 < (int)c.color_off.size(); ++col)
@@ -1937,6 +1958,7 @@ int ofd_set_nsvof_system(ofd_ctx *x, const double *k11, const double *g, const d
     if (rc) return rc;
     c.matV_valid = true;
     c.precV_valid = c.pV_valid = c.aV32_valid = c.apV32_valid = false;
+    c.t_matV_valid = false;
   }
   if (rhsu) {
     OFD_CHECK(rhsp && rhsls, OFD_ERR_ARG, "the right-hand side needs all three arrays");
@@ -2815,6 +2837,177 @@ int ofd_j_apply(ofd_ctx *x, int subdomain_flag, const double *vu, const double *
   OFD_HIP(launch_jv_ns(c, subdomain_flag, c.d_lamb, c.d_jt));
   rc = get_node4(c, c.d_jt, yu, yp);
   OFD_HIP(hipStreamSynchronize(c.stream));
+  return rc;
+}
+
+// ---- exact NS + VOF Jacobian and the coupled quench adjoint ----
+
+static int nsvof_adjoint_ready(Ctx &c) {
+  int rc = ensure_nsvof(c);
+  if (rc) return rc;
+  OFD_CHECK(c.quench, OFD_ERR_ARG, "no quench parameters (ofd_set_quench_params)");
+  OFD_CHECK(c.nshl == 8, OFD_ERR_UNSUPPORTED,
+            "the dual-number kernels are implemented for trilinear hexes (NSHL=8)");
+  OFD_HIP(hipSetDevice(c.device));
+  if (c.vadj_n4 != c.nsvV_n4 || !c.d_vjpV) {
+    const size_t np = (size_t)c.nsvV_n4 * 4;
+    double **bufs[4] = {&c.d_lamV, &c.d_lamVb, &c.d_lamVd, &c.d_jtV};
+    for (double **b : bufs) {
+      rc = dalloc(c, b, np);
+      if (rc) return rc;
+      OFD_HIP(hipMemsetAsync(*b, 0, np * sizeof(double), c.stream));
+    }
+    rc = dalloc(c, &c.d_vjpV, (size_t)c.nnode * 10);
+    if (rc) return rc;
+    c.vadj_n4 = c.nsvV_n4;
+  }
+  if (c.t_nsv_icnt != c.icnt) {
+    rc = dalloc(c, &c.t_AV, (size_t)c.icnt * 24);
+    if (!rc) rc = dalloc(c, &c.t_PV, (size_t)c.nnode * 17);
+    if (!rc) rc = dalloc(c, &c.t_AV32, (size_t)0);
+    if (!rc) rc = dalloc(c, &c.t_ApV, (size_t)0);
+    if (!rc) rc = dalloc(c, &c.t_ApV32, (size_t)0);
+    if (rc) return rc;
+    c.t_nsv_icnt = c.icnt;
+    c.t_matV_valid = false;
+  }
+  return OFD_OK;
+}
+
+// the NS + VOF solver state of the stored matrix swapped with that of its
+// transpose, for the inner solves of the adjoint
+struct NsvofTransposeView {
+  Ctx &c;
+  explicit NsvofTransposeView(Ctx &cc) : c(cc) { swap_all(); }
+  ~NsvofTransposeView() { swap_all(); }
+  void swap_all() {
+    std::swap(c.d_AV, c.t_AV);
+    std::swap(c.d_AV32, c.t_AV32);
+    std::swap(c.d_PV, c.t_PV);
+    std::swap(c.d_ApV, c.t_ApV);
+    std::swap(c.d_ApV32, c.t_ApV32);
+    std::swap(c.matV_valid, c.t_matV_valid);
+    std::swap(c.precV_valid, c.t_precV_valid);
+    std::swap(c.pV_valid, c.t_pV_valid);
+    std::swap(c.aV32_valid, c.t_aV32_valid);
+    std::swap(c.apV32_valid, c.t_apV32_valid);
+  }
+};
+
+int ofd_assemble_nsvof_jac(ofd_ctx *x) {
+  OFD_CHECK(x, OFD_ERR_ARG, "null context");
+  Ctx &c = x->c;
+  int rc = nsvof_adjoint_ready(c);
+  if (rc) return rc;
+  OFD_HIP(launch_alpha_stage(c, true));
+  for (int col = 0; col + 1 < (int)c.color_off.size(); ++col)
+    OFD_HIP(launch_nsvof_jac(c, col));
+  c.matV_valid = true;
+  c.precV_valid = c.pV_valid = c.aV32_valid = c.apV32_valid = false;
+  c.t_matV_valid = false;
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  return OFD_OK;
+}
+
+int ofd_vjp_nsvof(ofd_ctx *x, const double *lu, const double *lp, const double *lphi,
+                  double *g) {
+  OFD_CHECK(x && lu && lp && lphi && g, OFD_ERR_ARG, "null argument");
+  Ctx &c = x->c;
+  int rc = nsvof_adjoint_ready(c);
+  if (rc) return rc;
+  const int N = c.nnode;
+  std::vector<double> v((size_t)c.nsvV_n4 * 4, 0.0);
+  nsvof_pack(N, lu, lp, lphi, v.data());
+  rc = h2d(c, c.d_lamVb, v.data(), v.size() * sizeof(double));
+  if (rc) return rc;
+  OFD_HIP(launch_alpha_stage(c, true));
+  OFD_HIP(launch_zero(c, c.d_vjpV, (size_t)N * 10));
+  for (int col = 0; col + 1 < (int)c.color_off.size(); ++col)
+    OFD_HIP(launch_nsvof_vjp(c, col, c.d_lamVb, c.d_lamVb + (size_t)4 * N, c.d_vjpV));
+  rc = d2h(c, g, c.d_vjpV, (size_t)N * 10 * sizeof(double));
+  if (rc) return rc;
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  return OFD_OK;
+}
+
+int ofd_vjp_tem_dual(ofd_ctx *x, const double *lam, double *g) {
+  OFD_CHECK(x && lam && g, OFD_ERR_ARG, "null argument");
+  Ctx &c = x->c;
+  int rc = nsvof_adjoint_ready(c);
+  if (rc) return rc;
+  const size_t N = (size_t)c.nnode;
+  rc = h2d(c, c.d_lamVb, lam, N * sizeof(double));
+  if (rc) return rc;
+  OFD_HIP(launch_alpha_stage(c));
+  OFD_HIP(launch_zero(c, c.d_vjpV, N * 7));
+  const double *ks = c.ks_on ? c.d_field[OFD_KSCALE] : nullptr;
+  for (int col = 0; col + 1 < (int)c.color_off.size(); ++col)
+    OFD_HIP(launch_tem_dual_vjp(c, col, ks, c.d_lamVb, c.d_vjpV));
+  rc = d2h(c, g, c.d_vjpV, N * 7 * sizeof(double));
+  if (rc) return rc;
+  OFD_HIP(hipStreamSynchronize(c.stream));
+  return OFD_OK;
+}
+
+int ofd_spmv_nsvof_t(ofd_ctx *x, const double *xu, const double *xp, const double *xl,
+                     double *yu, double *yp, double *yl) {
+  OFD_CHECK(x && xu && xp && xl && yu && yp && yl, OFD_ERR_ARG, "null argument");
+  Ctx &c = x->c;
+  int rc = nsvof_adjoint_ready(c);
+  if (rc) return rc;
+  OFD_CHECK(c.matV_valid, OFD_ERR_ARG, "no assembled NSVOF matrix");
+  const int N = c.nnode;
+  std::vector<double> v((size_t)c.nsvV_n4 * 4, 0.0);
+  nsvof_pack(N, xu, xp, xl, v.data());
+  rc = h2d(c, c.d_lamVb, v.data(), v.size() * sizeof(double));
+  if (rc) return rc;
+  OFD_HIP(launch_spmv_v_t(c, c.d_lamVb, c.d_jtV));
+  rc = d2h(c, v.data(), c.d_jtV, v.size() * sizeof(double));
+  if (rc) return rc;
+  nsvof_unpack(N, v.data(), yu, yp, yl);
+  return OFD_OK;
+}
+
+int ofd_solve_nsvof_t(ofd_ctx *x, int precision, int ortho, double rtol, double inner_tol,
+                      int kspace, int max_cycles, const double *gu, const double *gp,
+                      const double *gl, double *lu, double *lp, double *ll, int *iters,
+                      int *cycles, double *relres) {
+  OFD_CHECK(x && gu && gp && gl && lu && lp && ll, OFD_ERR_ARG, "null argument");
+  OFD_CHECK(precision == OFD_FP64 || precision == OFD_FP32, OFD_ERR_ARG, "bad precision");
+  OFD_CHECK(max_cycles >= 1, OFD_ERR_ARG, "max_cycles >= 1");
+  Ctx &c = x->c;
+  int rc = nsvof_adjoint_ready(c);
+  if (rc) return rc;
+  OFD_CHECK(c.matV_valid, OFD_ERR_ARG, "no assembled NSVOF matrix");
+  const int N = c.nnode;
+  if (!c.t_matV_valid) {
+    OFD_HIP(launch_nsvof_transpose(c, c.t_AV));
+    c.t_matV_valid = true;
+    c.t_precV_valid = c.t_pV_valid = c.t_aV32_valid = c.t_apV32_valid = false;
+  }
+  std::vector<double> v((size_t)c.nsvV_n4 * 4, 0.0);
+  nsvof_pack(N, gu, gp, gl, v.data());
+  rc = h2d(c, c.d_lamVb, v.data(), v.size() * sizeof(double));
+  if (rc) return rc;
+  OFD_HIP(launch_zero(c, c.d_lamV, v.size()));
+  NsvofSystem sys(c);
+  auto apply = [&](const double *vin, double *y) -> hipError_t {
+    return launch_spmv_v_t(c, vin, y);
+  };
+  auto prec = [&](const double *vin, double *z, int *it) -> int {
+    int cy = 0;
+    double ir = 0.0;
+    const double ctol = precision == OFD_FP32 ? 1e-4 : 1e-2;
+    NsvofTransposeView view(c);
+    return solve_core(c, precision, ortho, vin, z, c.d_lamVd, inner_tol, ctol, 30, 100, it,
+                      &cy, &ir);
+  };
+  rc = fgmres(c, v.size(), apply, prec, c.d_lamVb, c.d_lamV, c.d_jtV, rtol, kspace, max_cycles,
+              iters, cycles, relres);
+  if (rc && rc != OFD_ERR_NOCONV) return rc;
+  const int rc2 = d2h(c, v.data(), c.d_lamV, v.size() * sizeof(double));
+  if (rc2) return rc2;
+  nsvof_unpack(N, v.data(), lu, lp, ll);
   return rc;
 }
 

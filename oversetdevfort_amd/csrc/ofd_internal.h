@@ -233,6 +233,18 @@ struct Ctx {
   double *d_lamT = nullptr, *d_lamTb = nullptr, *d_lamTd = nullptr, *d_jtT = nullptr;
   double *d_vjpT = nullptr;
   int tadj_n4 = 0;
+  // NS + VOF adjoint: the stored matrix transposed (tmap mirror; its (phi
+  // row, p column) entries dropped) with its solver copies, swapped in by a
+  // view for the inner solves; adjoint vectors (4 nsvV_n4 each) lambda,
+  // right-hand side, inner residual, operator output; raw gradients [10N]
+  double *t_AV = nullptr, *t_PV = nullptr, *t_ApV = nullptr;
+  float *t_AV32 = nullptr, *t_ApV32 = nullptr;
+  bool t_matV_valid = false, t_precV_valid = false, t_pV_valid = false,
+       t_aV32_valid = false, t_apV32_valid = false;
+  int64_t t_nsv_icnt = 0;
+  double *d_lamV = nullptr, *d_lamVb = nullptr, *d_lamVd = nullptr, *d_jtV = nullptr;
+  double *d_vjpV = nullptr;
+  int vadj_n4 = 0;
   // overset fringe: per-element boxes, source flags and geometry; receivers
   int fr_alloc_e = 0, fr_alloc_r = 0;
   double *d_fr_box = nullptr;         // [E][6] widened bounding box
@@ -265,6 +277,20 @@ hipError_t launch_jt_ns(Ctx &c, int subdomain_flag, const double *lam, double *y
 hipError_t launch_jv_ns(Ctx &c, int subdomain_flag, const double *v, double *y);
 // temperature adjoint (adjoint_tem.hip): tT_A = LHSTem mirrored through d_tmap
 hipError_t launch_scalar_transpose(Ctx &c);
+// exact derivatives by dual numbers (quench_dual.hip), hexes, one colour per
+// launch: the projected NS + VOF Jacobian into d_AV; the VJP of the NS + VOF
+// residual (lam4 [N][4], lamphi [N]) into out [10][N]; the VJP of the
+// temperature residual (lam [N]) into out [7][N] (outputs zeroed by the
+// caller; alpha stage with rphi for NS + VOF, with rT for T)
+hipError_t launch_nsvof_jac(Ctx &c, int color);
+hipError_t launch_nsvof_vjp(Ctx &c, int color, const double *lam4, const double *lamphi,
+                            double *out);
+hipError_t launch_tem_dual_vjp(Ctx &c, int color, const double *ks, const double *lam,
+                               double *out);
+// At = the stored NS + VOF matrix transposed without its (phi, p) slot;
+// y = A^T x exactly (FP64)
+hipError_t launch_nsvof_transpose(Ctx &c, double *At);
+hipError_t launch_spmv_v_t(Ctx &c, const double *x, double *y);
 // gradients of lambda . R_T (lam [N]) into out [6N] (zeroed by the caller)
 hipError_t launch_vjp_tem(Ctx &c, const double *lam, double *out);
 // y = J_T,bc^T lam (y: 4 ceil(N/4), zeroed here)

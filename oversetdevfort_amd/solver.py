@@ -567,6 +567,67 @@ class Context:
                                    (precision, ortho, float(rtol), float(inner_tol),
                                     int(kspace), int(max_cycles)), g, raise_noconv)
 
+    # ---- exact NS + VOF Jacobian and the coupled quench adjoint (no reference code) ----
+    def assemble_nsvof_jac(self):
+        """Replace the stored NS + VOF matrix by J_bc, the exact Jacobian of
+        the assembled RHS with respect to the Newton increments (dual-number
+        element kernels), rows and columns of the constrained dofs cleared;
+        solve_nsvof then takes exact Newton steps."""
+        L.check(self.lib.ofd_assemble_nsvof_jac(self.h))
+
+    def vjp_nsvof(self, lu, lp, lphi):
+        """Gradients of lambda . R (the NS + VOF residual as assembled, before
+        the Dirichlet projection) with respect to the alpha-stage u, ac, p,
+        phi, rphi and T."""
+        N = self.nnode
+        lu_, lp_, ll_ = _soa(lu), _f64(lp), _f64(lphi)
+        g = np.zeros(10 * N)
+        L.check(self.lib.ofd_vjp_nsvof(self.h, _ptr(lu_), _ptr(lp_), _ptr(ll_), _ptr(g)))
+        g = g.reshape(10, N)
+        return dict(u=g[0:3].T.copy(), ac=g[3:6].T.copy(), p=g[6].copy(), phi=g[7].copy(),
+                    rphi=g[8].copy(), T=g[9].copy())
+
+    def vjp_tem_dual(self, lam):
+        """Gradients of lambda . R_T with respect to the alpha-stage T, rT, u,
+        phi and the conductivity scale (dual-number element kernel)."""
+        N = self.nnode
+        lam_ = _f64(lam)
+        g = np.zeros(7 * N)
+        L.check(self.lib.ofd_vjp_tem_dual(self.h, _ptr(lam_), _ptr(g)))
+        g = g.reshape(7, N)
+        return dict(T=g[0].copy(), rT=g[1].copy(), u=g[2:5].T.copy(), phi=g[5].copy(),
+                    ks=g[6].copy())
+
+    def SparseProd_NS_conv_T(self, xu, xp, xl):
+        """y = A^T x for the stored NS + VOF matrix (FP64)."""
+        N = self.nnode
+        xu_, xp_, xl_ = _soa(xu), _f64(xp), _f64(xl)
+        yu, yp, yl = np.zeros(3 * N), np.zeros(N), np.zeros(N)
+        L.check(self.lib.ofd_spmv_nsvof_t(self.h, _ptr(xu_), _ptr(xp_), _ptr(xl_), _ptr(yu),
+                                          _ptr(yp), _ptr(yl)))
+        return yu.reshape(3, N).T.copy(), yp, yl
+
+    def solve_nsvof_t(self, gu, gp, gl, rtol=1e-10, precision=L.OFD_FP64,
+                      ortho=L.OFD_ORTHO_CGS1 | L.OFD_PREC_RIGHT, inner_tol=1e-3, kspace=30,
+                      max_cycles=20, raise_noconv=True):
+        """A^T lambda = g for the stored NS + VOF matrix: flexible GMRES on the
+        exact transposed product, right-preconditioned by restarted GMRES on
+        the stored transpose (its (phi, p) coupling dropped) to inner_tol."""
+        N = self.nnode
+        gu_, gp_, gl_ = _soa(gu), _f64(gp), _f64(gl)
+        lu, lp, ll = np.zeros(3 * N), np.zeros(N), np.zeros(N)
+        it = ctypes.c_int()
+        cyc = ctypes.c_int()
+        rel = ctypes.c_double()
+        rc = self.lib.ofd_solve_nsvof_t(self.h, precision, ortho, float(rtol), float(inner_tol),
+                                        int(kspace), int(max_cycles), _ptr(gu_), _ptr(gp_),
+                                        _ptr(gl_), _ptr(lu), _ptr(lp), _ptr(ll),
+                                        ctypes.byref(it), ctypes.byref(cyc), ctypes.byref(rel))
+        if rc != L.OFD_OK and (raise_noconv or rc != L.OFD_ERR_NOCONV):
+            L.check(rc)
+        return dict(lamu=lu.reshape(3, N).T.copy(), lamp=lp, lamphi=ll, iters=it.value,
+                    cycles=cyc.value, relres=rel.value, converged=rc == L.OFD_OK)
+
     # ---- measurement ----
     def time_quench(self, what, arg, reps=5):
         """Mean device ms of one quenching-solver kernel (L.OFD_TIME_*; arg is
