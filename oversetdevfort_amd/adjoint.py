@@ -23,7 +23,14 @@ Three functions sit behind ``torch.autograd.Function``:
   Jacobian J_T^T (flexible GMRES preconditioned by LHSTem^T) and one element
   VJP.
 
-``run_steps`` / ``run_tem_steps`` chain the steps over time and, with
+* ``QuenchStep``: one coupled step of the quenching solver, (old u, ac, p,
+  phi, rphi, T, rT, kscale, inflow amplitude) -> the new fields: the
+  staggered iteration of solveFlow.f90 with exact Newton steps on both
+  systems (the NS + VOF Jacobian by dual numbers).  Backward: the coupled
+  transposed system by block Gauss-Seidel (one temperature adjoint solve
+  and one NS + VOF transposed solve per sweep), then the residuals' VJPs.
+
+``run_steps`` / ``run_tem_steps`` / ``run_quench_steps`` chain the steps over time and, with
 ``checkpoint_every`` = k, keep only every k-th state for the reverse sweep:
 each segment of k steps is re-solved during backward
 (torch.utils.checkpoint).  The kernels are deterministic, so a re-solved
@@ -44,7 +51,8 @@ from . import _lib as L
 
 __all__ = ["NSStepper", "NSResidual", "NavStoStep", "ns_residual", "nav_sto_step",
            "run_steps", "TemStepper", "TemStep", "tem_step", "run_tem_steps",
-           "solid_heat_loss"]
+           "solid_heat_loss", "QuenchStepper", "QuenchStep", "quench_step",
+           "run_quench_steps", "quench_heat_loss"]
 
 
 def _np(t):
@@ -389,3 +397,273 @@ def solid_heat_loss(st, Told, rTold, ks, u, rho_cp, solid_volume, nsteps, checkp
     surface."""
     T, _ = run_tem_steps(st, Told, rTold, ks, u, nsteps, checkpoint_every)
     return torch.sum(rho_cp * solid_volume * (Told - T))
+
+
+class QuenchStepper:
+    """One time step of the quenching solver with the flow, the VOF field and
+    the temperature coupled, differentiable end to end.
+
+    Forward: predictor (driver.f90:83-85) and the Dirichlet values of the
+    velocity, amp x ``profile`` on the inlet and 0 on the walls (setBCs_CFD,
+    setBCs.f90:84-91), then the reference's staggered iteration
+    (solmultiphasethermofluid_stag, solveFlow.f90:57-161) with exact Newton
+    steps: per iteration the NS + VOF system with its exact Jacobian
+    (ofd_assemble_nsvof_jac) and the temperature system with LHSTem (exact
+    for TEM_KDC = 0 and C_DMDOT = 1), until every residual (u, p, phi, T) has
+    dropped by ``rtol`` from the predictor's or sits at its rounding floor.
+
+    Backward, at the converged state, by the implicit function theorem: the
+    seeds on the new fields become seeds on the Newton increments, and the
+    transposed coupled system
+        [J_FF^T  J_TF^T] [lam_F]   [s_F]
+        [J_FT^T  J_TT^T] [lam_T] = [s_T]
+    is solved by block Gauss-Seidel, each sweep one temperature adjoint
+    solve (ofd_solve_tem_adjoint) and one NS + VOF transposed solve
+    (ofd_solve_nsvof_t on the exact Jacobian), the couplings
+    J_FT^T lam_F = -alfi gami Delt dR_F/dT . lam_F and
+    J_TF^T lam_T = -alfi gami Delt dR_T/d(u, phi) . lam_T from the dual-number
+    VJPs.  Then one VJP of each residual gives the gradients with respect to
+    the old state, the conductivity scale and the inflow amplitude.
+
+    ``ibc`` (N, 8): columns 1-3 u, 6 T, 7 p, 8 phi (1 = fixed)."""
+
+    NEW = ("UG", "ACG", "PG", "PHIG", "RPHIG", "TG", "RTG")
+    OLD = ("UGOLD", "ACGOLD", "PGOLD", "PHIGOLD", "RPHIGOLD", "TGOLD", "RTGOLD")
+
+    def __init__(self, ctx, ibc, params, profile, rtol=1e-12, max_newton=12, lin_rtol=1e-12,
+                 adj_rtol=1e-11, precision=L.OFD_FP64, adj_precision=None, inner_tol=1e-3,
+                 kspace=30, max_cycles=200, gs_rtol=1e-11, max_gs=40):
+        b = np.asarray(ibc)
+        self.ctx = ctx
+        self.fu = (b[:, :3] != 1).astype(np.float64)
+        self.fp = (b[:, 6] != 1).astype(np.float64)
+        self.fphi = (b[:, 7] != 1).astype(np.float64)
+        self.fT = (b[:, 5] != 1).astype(np.float64)
+        self.p = dict(params)
+        self.profile = np.asarray(profile, np.float64)
+        self.rtol, self.max_newton, self.lin_rtol = rtol, max_newton, lin_rtol
+        self.adj_rtol, self.precision = adj_rtol, precision
+        self.adj_precision = precision if adj_precision is None else adj_precision
+        self.inner_tol, self.kspace, self.max_cycles = inner_tol, kspace, max_cycles
+        self.gs_rtol, self.max_gs = gs_rtol, max_gs
+        self.newton_history = []
+        self.gs_history = []
+
+    def _load_old(self, old, ks, amp):
+        c = self.ctx
+        for name, v in zip(self.OLD, old):
+            c.set_field(name, v)
+        c.set_field("KSCALE", ks)
+        c.set_bcvals(float(amp) * self.profile)
+
+    def _norms(self):
+        c = self.ctx
+        c.assemble_nsvof(2)
+        r = c.nsvof_rhs_norms()
+        c.assemble_tem(2)
+        return (*r, c.tem_rhs_norm())
+
+    def _converged(self, hist):
+        r0, prev, r = hist[0], hist[-2], hist[-1]
+        # a component is done once it has dropped by rtol, or once it has
+        # dropped by 1e-9 and stopped falling (its rounding floor)
+        return all(ri <= self.rtol * r0i or (ri <= 1e-9 * r0i and ri > 0.2 * pi) or ri == 0.0
+                   for ri, r0i, pi in zip(r, r0, prev))
+
+    # ---- forward ----
+    def solve(self, old, ks, amp):
+        """One converged time step from the old state; returns the new
+        (ug, acg, pg, phig, rphig, Tg, rTg)."""
+        c = self.ctx
+        self._load_old(old, ks, amp)
+        c.predictor()
+        c.apply_dirichlet()
+        hist = [self._norms()]
+        it = 0
+        while it == 0 or not self._converged(hist):
+            if it == self.max_newton:
+                raise RuntimeError(f"coupled Newton did not converge: residuals {hist}")
+            # NS + VOF: the residual assembled by _norms, the exact Jacobian
+            # (a solve that stops at its rounding floor short of lin_rtol
+            # still gives a Newton step; the residual test decides)
+            c.assemble_nsvof_jac()
+            c.solve_nsvof(rtol=self.lin_rtol, precision=self.precision,
+                          ortho=L.OFD_ORTHO_CGS1 | L.OFD_PREC_RIGHT, kspace=self.kspace,
+                          max_cycles=self.max_cycles, raise_noconv=False)
+            c.nsvof_update()
+            # temperature with the new flow
+            c.assemble_tem(6)
+            c.solve_tem(rtol=self.lin_rtol, precision=self.precision, kspace=self.kspace,
+                        max_cycles=self.max_cycles, raise_noconv=False)
+            c.tem_update()
+            hist.append(self._norms())
+            it += 1
+        self.newton_history.append(hist)
+        return tuple(c.get_field(n) for n in self.NEW)
+
+    # ---- reverse ----
+    @staticmethod
+    def _checked(res, floor=1e-8):
+        # an adjoint solve may stop at its rounding floor short of adj_rtol;
+        # one that stays above `floor` is an error
+        if not res["converged"] and not res["relres"] <= floor:
+            raise RuntimeError(f"adjoint solve stopped at relres {res['relres']:.3e}")
+        return res
+
+    def load(self, old, ks, amp, new):
+        self._load_old(old, ks, amp)
+        for name, v in zip(self.NEW, new):
+            self.ctx.set_field(name, v)
+
+    def step_vjp(self, g):
+        """Reverse of ``solve`` on the loaded converged state.  ``g``: seeds
+        on the new fields (ug (N,3), acg (N,3), pg, phig, rphig, Tg, rTg).
+        Returns the gradients with respect to the old fields (same order),
+        the conductivity scale and the inflow amplitude."""
+        c, p = self.ctx, self.p
+        ga, dt, af, am = p["gami"], p["delt"], p["alfi"], p["almi"]
+        gdt = ga * dt
+        fu, fp, fl, fT = self.fu, self.fp, self.fphi, self.fT
+        gug, gacg, gpg, gphig, grphig, gTg, grTg = g
+        # seeds on the Newton increments of the free dofs (acg += d, ug +=
+        # gami Delt d; pg += alfi gami Delt dp; rphig += dphi, phig += gami
+        # Delt dphi; rTg += dT, Tg += gami Delt dT)
+        su = (gacg + gdt * gug) * fu
+        sp = af * gdt * gpg * fp
+        sl = (grphig + gdt * gphig) * fl
+        sT = (grTg + gdt * gTg) * fT
+        N = len(sp)
+        lu, lp, ll = np.zeros((N, 3)), np.zeros(N), np.zeros(N)
+        lT = np.zeros(N)
+        sweeps = []
+        if np.any(su) or np.any(sp) or np.any(sl) or np.any(sT):
+            c.assemble_tem(6)                  # LHSTem preconditions J_TT^T
+            c.assemble_nsvof(2)
+            c.assemble_nsvof_jac()             # J_FF, transposed in the solve
+            for k in range(self.max_gs):
+                gF = c.vjp_nsvof(lu, lp, ll)
+                rT = (sT + af * gdt * gF["T"]) * fT
+                res = self._checked(c.solve_tem_adjoint(
+                    rT, rtol=self.adj_rtol, precision=self.adj_precision,
+                    inner_tol=self.inner_tol, kspace=self.kspace, max_cycles=self.max_cycles,
+                    raise_noconv=False))
+                lT = res["lam"] * fT
+                gT = c.vjp_tem_dual(lT)
+                ru = (su + af * gdt * gT["u"]) * fu
+                rl = (sl + af * gdt * gT["phi"]) * fl
+                # the temperature kernels changed the alpha stage: the NS + VOF
+                # transposed solve reads only the stored Jacobian
+                r = self._checked(c.solve_nsvof_t(
+                    ru, sp, rl, rtol=self.adj_rtol, precision=self.adj_precision,
+                    inner_tol=self.inner_tol, kspace=self.kspace, max_cycles=self.max_cycles,
+                    raise_noconv=False))
+                nu, npp, nl = r["lamu"] * fu, r["lamp"] * fp, r["lamphi"] * fl
+                dn = np.sqrt(np.sum((nu - lu) ** 2) + np.sum((npp - lp) ** 2) +
+                             np.sum((nl - ll) ** 2))
+                nn = np.sqrt(np.sum(nu ** 2) + np.sum(npp ** 2) + np.sum(nl ** 2))
+                lu, lp, ll = nu, npp, nl
+                sweeps.append(dn / max(nn, 1e-300))
+                if sweeps[-1] <= self.gs_rtol:
+                    break
+            else:
+                raise RuntimeError(f"coupled adjoint sweeps did not converge: {sweeps}")
+            # the last lam_T belongs to the lam_F before the last sweep: redo it
+            gF = c.vjp_nsvof(lu, lp, ll)
+            rT = (sT + af * gdt * gF["T"]) * fT
+            res = self._checked(c.solve_tem_adjoint(
+                rT, rtol=self.adj_rtol, precision=self.adj_precision, inner_tol=self.inner_tol,
+                kspace=self.kspace, max_cycles=self.max_cycles, raise_noconv=False))
+            lT = res["lam"] * fT
+        self.gs_history.append(sweeps)
+        gF = c.vjp_nsvof(lu, lp, ll)
+        gT = c.vjp_tem_dual(lT)
+        Gu = gF["u"] + gT["u"]
+        Gac, Gp, Grphi = gF["ac"], gF["p"], gF["rphi"]
+        Gphi = gF["phi"] + gT["phi"]
+        GT = gF["T"] + gT["T"]
+        GrT, Gks = gT["rT"], gT["ks"]
+        # explicit dependence at fixed increments (from the predictor):
+        # free u: ug = ugold + gami Delt d, acg = (gami-1)/gami acgold + d;
+        # Dirichlet u: ug = g, acg = (g - ugold)/(gami Delt) + (gami-1)/gami acgold;
+        # p: pg = pgold + alfi gami Delt dp; phi, T: a = aold + gami Delt d,
+        # ra = (gami-1)/gami raold + d (d = 0 where held)
+        du = 1.0 - fu
+        cg = (ga - 1.0) / ga
+        ca = 1.0 - am / ga
+        d_ugold = fu * (gug + Gu) + du * ((1.0 - af) * Gu - am / gdt * Gac - gacg / gdt)
+        d_acgold = cg * gacg + ca * Gac
+        d_pgold = gpg + Gp
+        d_phigold = gphig + Gphi
+        d_rphigold = cg * grphig + ca * Grphi
+        d_Tgold = gTg + GT
+        d_rTgold = cg * grTg + ca * GrT
+        d_bc = du * (gug + gacg / gdt + af * Gu + am / gdt * Gac)
+        d_amp = float(np.sum(d_bc * self.profile))
+        return (d_ugold, d_acgold, d_pgold, d_phigold, d_rphigold, d_Tgold, d_rTgold, Gks,
+                d_amp)
+
+
+class QuenchStep(torch.autograd.Function):
+    """(ugold, acgold, pgold, phigold, rphigold, Tgold, rTgold, kscale, amp)
+    -> (ug, acg, pg, phig, rphig, Tg, rTg) of one converged coupled step."""
+
+    @staticmethod
+    def forward(fc, st, ugold, acgold, pgold, phigold, rphigold, Tgold, rTgold, ks, amp):
+        old = [_np(v) for v in (ugold, acgold, pgold, phigold, rphigold, Tgold, rTgold)]
+        new = st.solve(old, _np(ks), float(amp))
+        out = tuple(_t(v, ugold) for v in new)
+        fc.st = st
+        fc.save_for_backward(ugold, acgold, pgold, phigold, rphigold, Tgold, rTgold, ks, amp,
+                             *out)
+        return out
+
+    @staticmethod
+    def backward(fc, *gouts):
+        st = fc.st
+        saved = fc.saved_tensors
+        old = [_np(v) for v in saved[:7]]
+        ks, amp = _np(saved[7]), float(saved[8])
+        new = [_np(v) for v in saved[9:]]
+        st.load(old, ks, amp, new)
+        seeds = [(_np(gv) if gv is not None else np.zeros_like(nv))
+                 for gv, nv in zip(gouts, new)]
+        grads = st.step_vjp(seeds)
+        like = saved[0]
+        out = [_t(gv, like) for gv in grads[:8]]
+        g_amp = torch.as_tensor(grads[8], dtype=saved[8].dtype, device=saved[8].device)
+        return (None, *out, g_amp)
+
+
+def quench_step(st, state, ks, amp):
+    """``state`` = (ug, acg, pg, phig, rphig, Tg, rTg) of the old level."""
+    return QuenchStep.apply(st, *state, ks, amp)
+
+
+def _quench_segment(st, n, ks, amp, *state):
+    for _ in range(n):
+        state = QuenchStep.apply(st, *state, ks, amp)
+    return state
+
+
+def run_quench_steps(st, state, ks, amp, nsteps, checkpoint_every=0):
+    """``nsteps`` converged coupled steps from ``state``, checkpointed as
+    ``run_steps``."""
+    state = tuple(state)
+    done = 0
+    k = int(checkpoint_every)
+    while done < nsteps:
+        n = min(k, nsteps - done) if k > 0 else nsteps - done
+        if k > 0 and torch.is_grad_enabled():
+            state = checkpoint(_quench_segment, st, n, ks, amp, *state, use_reentrant=False)
+        else:
+            state = _quench_segment(st, n, ks, amp, *state)
+        done += n
+    return state
+
+
+def quench_heat_loss(st, state, ks, amp, rho_cp, solid_volume, nsteps, checkpoint_every=0):
+    """Heat the solid gives off over ``nsteps`` coupled steps,
+    sum_i V_i rho_s c_s (T0_i - T_i)."""
+    T0 = state[5]
+    out = run_quench_steps(st, state, ks, amp, nsteps, checkpoint_every)
+    return torch.sum(rho_cp * solid_volume * (T0 - out[5]))

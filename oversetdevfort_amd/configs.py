@@ -19,6 +19,12 @@ and setBCs_CFD; the GPU path consumes the resulting arrays.
   the gas fraction (prop_interp(rhow, rhoa, phi), evaporation ~ (1 - phi)),
   so the liquid is phi = 0.
 
+* ``quench_inflow`` -- the quench tank with an inlet patch on the wall x = 0
+  (below the liquid level) whose normal velocity is an amplitude times a
+  sine bump; the fluid leaves through the open top.  The coupled quench
+  adjoint's configuration (gradients with respect to the inflow amplitude
+  and a per-node conductivity scale).
+
 * ``overset`` -- background [0,2]x[0,1]x[0,1] on nb hexes (ELM_ID 1) plus a
   cube component of side 0.4 on nc^3 hexes (ELM_ID 2), rotated by a seeded
   angle in [0, 30] degrees about z and centred at (0.9, 0.5, 0.5), in one
@@ -300,6 +306,53 @@ def adjoint_quench(dims=(128, 64, 64), seed=5, speed=0.05):
     prob.kscale = np.exp(0.1 * rng.standard_normal(N))
     prob.circulation = speed * circulation(prob.mesh.xg)
     prob.solid_volume = lumped_volumes(prob.mesh, prob.mesh.elm_id == 101)
+    return prob
+
+
+def inflow_bump(x, lo=(0.3, 0.1), hi=(0.7, 0.4)):
+    """Profile of the tank's inlet: on the face x = 0, the patch y in
+    [lo0, hi0], z in [lo1, hi1] (below the liquid level), the normal velocity
+    sin(pi (y - lo0) / (hi0 - lo0)) sin(pi (z - lo1) / (hi1 - lo1)), zero on
+    the patch's edges and outside it; (N, 3) with the x component only."""
+    y, z = x[:, 1], x[:, 2]
+    on = (np.abs(x[:, 0]) < 1e-12) & (y >= lo[0]) & (y <= hi[0]) & (z >= lo[1]) & (z <= hi[1])
+    prof = np.zeros((len(x), 3))
+    prof[on, 0] = (np.sin(np.pi * (y[on] - lo[0]) / (hi[0] - lo[0])) *
+                   np.sin(np.pi * (z[on] - lo[1]) / (hi[1] - lo[1])))
+    return prof, on
+
+
+def quench_inflow(dims=(16, 16, 16), seed=5, amp=0.05, diffuse=0.0, rate=None):
+    """The quench tank with an inlet: cold liquid (phi = 0, 300 K) enters
+    through a patch of the wall x = 0 with the normal velocity amp x
+    ``inflow_bump`` and leaves through the open top; the inlet nodes hold
+    their VOF value and temperature (IBC columns 8 and 6).  A per-node
+    conductivity scale exp(0.1 z), z seeded standard normal, and the solid
+    block's lumped nodal volumes (for the heat it gives off) ride along.
+
+    ``diffuse`` > 0 replaces the sharp VOF field by the diffuse interface
+    0.5 + 0.45 tanh((z - 0.6) / diffuse) (no bubble; every value inside (0, 1)
+    away from the clamps); ``rate`` sets both phase-change coefficients
+    (c_evap, c_cond; 0.1 in the tank's material)."""
+    prob = quench(dims=dims, seed=seed)
+    m = prob.mesh
+    N = m.nnode
+    if diffuse > 0.0:
+        phi = 0.5 + 0.45 * np.tanh((m.xg[:, 2] - 0.6) / diffuse)
+        for k in ("phig", "phigold"):
+            prob.fields[k][:] = phi
+    if rate is not None:
+        prob.material.update(c_evap=float(rate), c_cond=float(rate))
+    prof, on = inflow_bump(m.xg)
+    prob.ibc[on, 5] = 1
+    prob.ibc[on, 7] = 1
+    prob.inflow_profile = prof
+    prob.inflow_nodes = on
+    prob.amp = float(amp)
+    prob.bcval = amp * prof
+    rng = np.random.default_rng(seed)
+    prob.kscale = np.exp(0.1 * rng.standard_normal(N))
+    prob.solid_volume = lumped_volumes(m, m.elm_id == 101)
     return prob
 
 
