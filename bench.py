@@ -340,6 +340,83 @@ def adjoint_c5(L, C, S, nsteps=20, every=5):
     return out
 
 
+def adjoint_coupled(L, C, S, dims=(64, 32, 32), nsteps=4, every=2):
+    """The coupled quench adjoint: the tank with an inlet (amplitude 0.05 m/s
+    times a sine bump on a patch of the wall x = 0), a diffuse interface and
+    phase-change coefficients 1e-3, ``nsteps`` converged coupled steps (NS +
+    VOF and temperature, exact Newton), a checkpoint every ``every``; the
+    heat the solid gives off and its gradients with respect to the inflow
+    amplitude and the per-node conductivity scale, with FP64 and with
+    FP32-matrix inner solves in the adjoint; the amplitude gradient against
+    a central difference (step 1e-6) of two re-solved transients."""
+    import numpy as np
+    import torch
+    from oversetdevfort_amd import adjoint as A
+    prob = C.quench_inflow(dims, seed=5, amp=0.05, diffuse=0.08, rate=1e-3)
+    E, N = prob.mesh.nelem, prob.mesh.nnode
+    ctx = S.setup(prob, device=0)
+    ctx.set_quench_params(prob.material)
+    dev = torch.device("cpu")
+    f = prob.fields
+    state = tuple(torch.tensor(f[k], device=dev) for k in
+                  ("ugold", "acgold", "pgold", "phigold", "rphigold", "Tgold", "rTgold"))
+    m = prob.material
+    rc = m["rhos"] * m["cps"]
+    vol = torch.tensor(prob.solid_volume, device=dev)
+    out = {"workload": f"quench tank {dims[0]}x{dims[1]}x{dims[2]} hex with an inlet, flow + "
+                       f"VOF + temperature coupled, {nsteps} steps, dt 1e-3, checkpoint every "
+                       f"{every}, diffuse interface, phase-change coefficients 1e-3",
+           "elements": E, "nodes": N}
+    # the dual-number kernels at this size (host wall time of one call)
+    ctx.assemble_nsvof(2)
+    t0 = time.perf_counter()
+    ctx.assemble_nsvof_jac()
+    out["jacobian_ms"] = 1e3 * (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    ctx.vjp_nsvof(np.ones((N, 3)), np.ones(N), np.ones(N))
+    out["vjp_nsvof_ms"] = 1e3 * (time.perf_counter() - t0)
+    grads = {}
+    for tag, aprec in (("fp64", L.OFD_FP64), ("fp32_matvec", L.OFD_FP32)):
+        st = A.QuenchStepper(ctx, prob.ibc, prob.params, prob.inflow_profile,
+                             adj_precision=aprec)
+        ks = torch.tensor(prob.kscale, device=dev).requires_grad_()
+        amp = torch.tensor(prob.amp, dtype=torch.float64, device=dev, requires_grad=True)
+        t0 = time.perf_counter()
+        Q = A.quench_heat_loss(st, state, ks, amp, rc, vol, nsteps, every)
+        t1 = time.perf_counter()
+        Q.backward()
+        t2 = time.perf_counter()
+        grads[tag] = (ks.grad.numpy(), float(amp.grad))
+        newton = [len(h) - 1 for h in st.newton_history]
+        sweeps = [len(g) for g in st.gs_history]
+        out[tag] = {"heat_J": float(Q.detach()), "forward_ms": 1e3 * (t1 - t0),
+                    "backward_ms": 1e3 * (t2 - t1), "dQ_damp": float(amp.grad),
+                    "grad_kscale_norm": float(np.linalg.norm(grads[tag][0])),
+                    "steps_solved": len(st.newton_history),
+                    "newton_iters_per_step": sum(newton) / len(newton),
+                    "gauss_seidel_sweeps_per_step": sum(sweeps) / max(len(sweeps), 1)}
+        log(f"bench: coupled quench adjoint {dims} ({tag}): Q {float(Q.detach()):.6e} J, "
+            f"forward {1e3 * (t1 - t0):.0f} ms, backward {1e3 * (t2 - t1):.0f} ms, dQ/damp "
+            f"{float(amp.grad):.6e}, Newton {newton}, sweeps {sweeps}")
+    st = A.QuenchStepper(ctx, prob.ibc, prob.params, prob.inflow_profile)
+    a0 = torch.tensor(prob.amp, dtype=torch.float64)
+    ks0 = torch.tensor(prob.kscale, device=dev)
+    eps = 1e-6
+    with torch.no_grad():
+        qp = float(A.quench_heat_loss(st, state, ks0, a0 + eps, rc, vol, nsteps))
+        qm = float(A.quench_heat_loss(st, state, ks0, a0 - eps, rc, vol, nsteps))
+    fd = (qp - qm) / (2 * eps)
+    out["dQ_damp_central_difference"] = fd
+    out["dQ_damp_rel_err"] = abs(fd - grads["fp64"][1]) / abs(fd)
+    out["fp32_vs_fp64_grad_rel"] = float(
+        np.linalg.norm(grads["fp32_matvec"][0] - grads["fp64"][0]) /
+        np.linalg.norm(grads["fp64"][0]))
+    log(f"bench: coupled quench adjoint: dQ/damp central difference {fd:.10e} "
+        f"(adjoint {grads['fp64'][1]:.10e}, rel {out['dQ_damp_rel_err']:.2e})")
+    ctx.close()
+    return out
+
+
 def overset_steps(L, C, S, nsteps=10, nnewton=3, kspace=64):
     """The overset configuration end to end: the 96x48x48 background and the
     rotated 32^3 component coupled in one system, Re = 500 cross-flow,
@@ -795,6 +872,8 @@ def main():
     ap.add_argument("--quench-steps-only", action="store_true",
                     help="only the quench configuration's time steps (FP64 and mixed); "
                          "no JSON line")
+    ap.add_argument("--coupled-adjoint-only", action="store_true",
+                    help="only the coupled quench adjoint section; no JSON line")
     ap.add_argument("--pmc-sample", action="store_true",
                     help="short run for hardware-counter passes: a 256 MiB copy "
                          "(calibration) and one full GMRES(30) cycle per precision")
@@ -822,6 +901,9 @@ def main():
         return
     if args.spmv_large_only:
         spmv_large(L, C, S, reps=5)
+        return
+    if args.coupled_adjoint_only:
+        print(json.dumps(adjoint_coupled(L, C, S)), file=sys.stderr)
         return
 
     nx, ny, nz = (int(v) for v in args.dims.split(","))
@@ -905,6 +987,7 @@ def main():
         fringe = fringe_search(C, S)
         fringe["coupled_steps"] = overset_steps(L, C, S)
         adjoint["quench_transient"] = adjoint_c5(L, C, S)
+        adjoint["quench_coupled"] = adjoint_coupled(L, C, S)
         faces, fprob = weak_faces(L, C, S)
         tets, tprob = tet_assembly(C, S)
         pattern, pmesh = pattern_build(S)
