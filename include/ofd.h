@@ -147,9 +147,11 @@ int ofd_apply_dirichlet(ofd_ctx *ctx);
 
 /* The faces of the weak-Dirichlet boundaries (BCugType = 2): bound(b)%F2E
  * and bound(b)%FACE_OR of input.f90:80-92, nface faces, f2e 1-based
- * elements.  The reference has no hex face map; here FACE_OR 1..6 of a hex
- * is the face at xi = -1, xi = +1, eta = -1, eta = +1, zeta = -1, zeta = +1
- * in linhexshl's node order, 2x2 Gauss points.  The weak velocity is the mesh
+ * elements.  Linear tets: the reference's face map (eval_faceshape_tet,
+ * FACE_OR 1..4, genGPtritet's 3 points per face).  The reference has no hex
+ * face map; here FACE_OR 1..6 of a hex is the face at xi = -1, xi = +1,
+ * eta = -1, eta = +1, zeta = -1, zeta = +1 in linhexshl's node order, 2x2
+ * Gauss points.  Prisms: OFD_ERR_UNSUPPORTED.  The weak velocity is the mesh
  * velocity (gi = umi, FaceAssembly_weak.f90:147).  Replaces the faces given
  * before; nface = 0 clears them.  The adjoint entries refuse a context with
  * faces (they differentiate the volume residual only). */

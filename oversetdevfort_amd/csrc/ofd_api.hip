@@ -994,8 +994,8 @@ int ofd_get_field(ofd_ctx *x, int field, double *host) {
 // the elements holding registered weak faces, grouped by element colour,
 // with their faces as a bitmask (bit fo-1), on the device
 static int ensure_faces(Ctx &c) {
-  OFD_CHECK(c.h_face_e.empty() || c.nshl == 8, OFD_ERR_UNSUPPORTED,
-            "boundary faces are implemented for trilinear hexes (NSHL=8)");
+  OFD_CHECK(c.h_face_e.empty() || c.nshl == 8 || c.nshl == 4, OFD_ERR_UNSUPPORTED,
+            "boundary faces are implemented for trilinear hexes and linear tets");
   if (!c.fe_dirty) return OFD_OK;
   c.fe_color_off.clear();
   if (c.h_face_e.empty()) {
@@ -1036,14 +1036,16 @@ int ofd_set_weak_faces(ofd_ctx *x, int nface, const int32_t *f2e, const int32_t 
   int rc = need_mesh(c);
   if (rc) return rc;
   OFD_CHECK(nface == 0 || (f2e && face_or), OFD_ERR_ARG, "null argument");
-  OFD_CHECK(nface == 0 || c.nshl == 8, OFD_ERR_UNSUPPORTED,
-            "boundary faces are implemented for trilinear hexes (NSHL=8)");
+  OFD_CHECK(nface == 0 || c.nshl == 8 || c.nshl == 4, OFD_ERR_UNSUPPORTED,
+            "boundary faces are implemented for trilinear hexes and linear tets");
+  const int nfo = c.nshl == 8 ? 6 : 4;
   std::vector<int32_t> fe(nface), fo(nface);
   std::vector<uint8_t> seen((size_t)c.nelem, 0);
   for (int f = 0; f < nface; ++f) {
     const int e = f2e[f] - 1, o = face_or[f];
     OFD_CHECK(e >= 0 && e < c.nelem, OFD_ERR_ARG, "F2E out of range [1, NELEM]");
-    OFD_CHECK(o >= 1 && o <= 6, OFD_ERR_ARG, "FACE_OR out of range [1, 6]");
+    OFD_CHECK(o >= 1 && o <= nfo, OFD_ERR_ARG,
+              nfo == 6 ? "FACE_OR out of range [1, 6]" : "FACE_OR out of range [1, 4]");
     OFD_CHECK(!((seen[e] >> (o - 1)) & 1), OFD_ERR_ARG, "face listed twice");
     seen[e] |= (uint8_t)(1u << (o - 1));
     fe[f] = e;
@@ -1080,7 +1082,7 @@ int ofd_assemble_ns(ofd_ctx *x, int tensor_flag, int subdomain_flag, double *cfl
   rc = ensure_faces(c);
   if (rc) return rc;
   for (int col = 0; col + 1 < (int)c.fe_color_off.size(); ++col)
-    OFD_HIP(launch_face_ns_hex(c, tensor_flag, subdomain_flag, col));
+    OFD_HIP(launch_face_ns(c, tensor_flag, subdomain_flag, col));
   double *d_out = c.d_part;  // scratch for the CFL maximum
   OFD_HIP(launch_max_reduce(c, c.d_cfl, c.nelem, d_out));
   double h[1] = {0.0};
@@ -1442,7 +1444,7 @@ int ofd_time_faces(ofd_ctx *x, int tensor_flag, int reps, double *ms) {
   OFD_HIP(hipEventRecord(e0, c.stream));
   for (int r = 0; r < reps; ++r)
     for (int col = 0; col + 1 < (int)c.fe_color_off.size(); ++col)
-      OFD_HIP(launch_face_ns_hex(c, tensor_flag, 3, col));
+      OFD_HIP(launch_face_ns(c, tensor_flag, 3, col));
   OFD_HIP(hipEventRecord(e1, c.stream));
   OFD_HIP(hipEventSynchronize(e1));
   float t = 0.f;
@@ -1478,7 +1480,7 @@ int ofd_time_assemble(ofd_ctx *x, int tensor_flag, int reps, double *ms) {
     OFD_HIP(hipMemsetAsync(c.d_negdet, 0, sizeof(int32_t), c.stream));
     OFD_HIP(launch_assemble_ns(c, tensor_flag, 3));
     for (int col = 0; col + 1 < (int)c.fe_color_off.size(); ++col)
-      OFD_HIP(launch_face_ns_hex(c, tensor_flag, 3, col));
+      OFD_HIP(launch_face_ns(c, tensor_flag, 3, col));
     OFD_HIP(launch_max_reduce(c, c.d_cfl, c.nelem, c.d_part));
   }
   OFD_HIP(hipEventRecord(e1, c.stream));

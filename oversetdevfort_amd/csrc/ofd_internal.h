@@ -316,7 +316,7 @@ hipError_t launch_spmv(Ctx &c, int precision, const double *x, double *y,
 hipError_t launch_spmv_arnoldi(Ctx &c, int precision, int k, hipEvent_t ev0 = nullptr,
                                hipEvent_t ev1 = nullptr);
 hipError_t launch_cgs_dots(Ctx &c, int k);
-hipError_t launch_face_ns_hex(Ctx &c, int tensor_flag, int subdomain_flag, int color);
+hipError_t launch_face_ns(Ctx &c, int tensor_flag, int subdomain_flag, int color);
 // genSparStruc on the device (pattern.hip)
 hipError_t launch_node_elem_count(Ctx &c, int32_t *cnt);
 hipError_t launch_node_elem_fill(Ctx &c, const int32_t *off, int32_t *pos, int32_t *inc);

@@ -5,7 +5,8 @@ unstructured tet mesh.
   needs more than 32 colours: the pattern bit for bit against the oracle's
   genSparStruc restatement and the NS assembly (IntElmAss_NS_overset +
   BCLhs_3D + FillSparseMat_3D) to 1e-12 against the oracle.
-* A mesh upload drops the weak faces registered for the previous mesh.
+* A mesh upload drops the weak faces registered for the previous mesh; tet
+  faces take FACE_OR 1..4, prism faces are refused.
 * A right-hand side with a NaN is reported as OFD_ERR_NONFINITE by every
   solver entry (SparseGMRES_up, SparseGMRES_tem, SparseGMRES_ns_conv, the
   restarted solve and the adjoint's flexible GMRES), never as a zero
@@ -100,7 +101,7 @@ def test_unstructured_tet_mesh_beyond_32_colours(oracle_mod):
 
 def test_upload_drops_weak_faces():
     from oversetdevfort_amd import solver as S
-    from oversetdevfort_amd.mesh import box_faces, hex_to_tets
+    from oversetdevfort_amd.mesh import box_faces, hex_to_prisms, hex_to_tets
     hexm = structured_hex(3, 3, 3, jitter=0.05, seed=2)
     ctx = S.Context(0)
     ctx.upload_mesh(hexm)
@@ -112,6 +113,13 @@ def test_upload_drops_weak_faces():
     ctx.upload_mesh(tets)          # the faces named hexes of the old mesh
     ctx.build_pattern()
     ctx.assembleNavSto(6, 1)       # no faces left, so no face launch
+    # tets take FACE_OR 1..4 (the reference's face map); prisms have no faces
+    with pytest.raises(L.OfdError) as e:
+        ctx.set_weak_faces(np.array([0], np.int32), np.array([5], np.int32))
+    assert e.value.code == L.OFD_ERR_ARG
+    ctx.set_weak_faces(np.array([0], np.int32), np.array([1], np.int32))
+    ctx.upload_mesh(hex_to_prisms(hexm))
+    ctx.build_pattern()
     with pytest.raises(L.OfdError) as e:
         ctx.set_weak_faces(np.array([0], np.int32), np.array([1], np.int32))
     assert e.value.code == L.OFD_ERR_UNSUPPORTED

@@ -16,6 +16,10 @@ is pinned by closed forms.  Here:
 * a time step of the weak-wall cavity (lid strong, five walls weak), two
   Newton iterations, against the oracle's assembly and a direct solve --
   fields to 1e-10;
+* linear tets (the reference's own face map, 3-point faces): the same
+  comparison on a jittered, Kuhn-split 3^3 box with every boundary face weak,
+  and the reference's own face output (fixtures face_tet_a, face_tet_vec)
+  added to the oracle's volume assembly, to 1e-12;
 * bad face lists are refused.
 """
 import numpy as np
@@ -130,6 +134,93 @@ def test_weak_cavity_time_step(S, oracle_mod):
     # the weak walls hold the velocity near zero without IBC
     wall = M.boundary_masks(m, (0, 0, 0), (1, 1, 1))["zlo"]
     assert np.abs(ctx.get_field("UG")[wall]).max() < 0.1 * np.abs(st["ug"]).max()
+    ctx.close()
+
+
+def _tet_case(seed=13):
+    rng = np.random.default_rng(seed)
+    hexm = M.structured_hex(3, 3, 3, jitter=0.08, seed=seed)
+    m = M.hex_to_tets(hexm)
+    N = m.nnode
+    m.elm_id[:] = rng.integers(1, 3, size=m.nelem)
+    ibc = (rng.uniform(size=(N, 4)) < 0.1).astype(np.int32)
+    st = dict(dg=0.005 * rng.normal(size=(N, 3)), ug=rng.normal(size=(N, 3)),
+              ugm=0.3 * rng.normal(size=(N, 3)), acg=rng.normal(size=(N, 3)),
+              acgm=np.zeros((N, 3)), pg=rng.normal(size=N))
+    return m, ibc, st, M.boundary_faces(m)
+
+
+def _tet_ctx(S, m, ibc, st, prm_dict):
+    ctx = S.Context(0)
+    ctx.upload_mesh(m)
+    ctx.build_pattern()
+    ctx.set_params(prm_dict)
+    ctx.set_ibc(ibc)
+    for k in ("dg", "ug", "ugm", "acg", "acgm", "pg"):
+        ctx.set_field(k.upper(), st[k])
+        ctx.set_field(k.upper() + "OLD", st[k])      # alpha stage = the state
+    return ctx
+
+
+@pytest.mark.parametrize("tflag,sflag", [(6, 1), (6, 3), (2, 2), (4, 3)])
+def test_tet_face_assembly_matches_oracle(S, oracle_mod, tflag, sflag):
+    O = oracle_mod
+    m, ibc, st, faces = _tet_case()
+    p = C.cavity(2).params
+    col, row = O.sparsity(m.nnode, m.ien)
+    prm = O.params_array(p)
+    ctx = _tet_ctx(S, m, ibc, st, p)
+    ctx.set_weak_faces(*faces)
+    ctx.assembleNavSto(tflag, sflag)
+    A, (ru, rp) = ctx.get_lhs(), ctx.get_rhs()
+    ref = O.assemble_ns(m, ibc, st, prm, col, row, tflag, sflag)
+    ref = O.face_ns(m, ibc, st, prm, col, row, faces, tflag, sflag, ngb=3, out=ref)
+    bare = O.assemble_ns(m, ibc, st, prm, col, row, tflag, sflag)
+    errs = {}
+    if tflag & 4:
+        for k in ("k11", "g", "d1", "m"):
+            errs[k] = _rel(A[k], ref[k])
+        assert _rel(bare["g"], ref["g"]) > 1e-6          # the faces add to G
+    if tflag & 2:
+        errs["rhsu"], errs["rhsp"] = _rel(ru, ref["rhsu"]), _rel(rp, ref["rhsp"])
+        assert _rel(bare["rhsu"], ref["rhsu"]) > 1e-6
+    print(f"tet faces ({len(faces[0])}), flags {tflag}/{sflag}: " +
+          ", ".join(f"{k} {v:.1e}" for k, v in errs.items()), flush=True)
+    assert max(errs.values()) <= 1e-12, errs
+    ctx.close()
+
+
+@pytest.mark.parametrize("name", ["face_tet_a", "face_tet_vec"])
+def test_tet_faces_replay_reference_fixtures(S, oracle_mod, name):
+    """The reference's own face routines' output (3-point faces; the 1-point
+    fixture is a face rule the reference's input never selects, NGAUSSB =
+    NSHLB = 3) added to the oracle's volume assembly equals the GPU's volume +
+    face assembly."""
+    from conftest import golden
+    O = oracle_mod
+    z = golden(name)
+    m = M.Mesh(xg=z["xg"], ien=z["ien"], elm_id=z["elm_id"])
+    st = {k[3:]: z[k] for k in z.files if k.startswith("st_")}
+    prm = z["prm"]
+    p = dict(almi=prm[0], alfi=prm[1], gami=prm[2], beti=prm[3], delt=prm[4], rho=prm[5],
+             mu=prm[6], kdc=prm[7], grav=tuple(prm[8:11]))
+    tflag, sflag = int(z["tensor_flag"]), int(z["subdomain_flag"])
+    col, row = O.sparsity(m.nnode, m.ien)
+    ctx = _tet_ctx(S, m, z["ibc"], st, p)
+    ctx.set_subdomains(1, 2)
+    ctx.set_weak_faces(z["f2e"], z["face_or"])
+    ctx.assembleNavSto(tflag, sflag)
+    vol = O.assemble_ns(m, z["ibc"], st, prm, col, row, tflag, sflag)
+    errs = {}
+    if tflag & 4:
+        A = ctx.get_lhs()
+        for k in ("k11", "g", "d1", "m"):
+            errs[k] = _rel(A[k], vol[k] + z[k])
+    ru, rp = ctx.get_rhs()
+    errs["rhsu"] = _rel(ru, vol["rhsu"] + z["rhsu"])
+    errs["rhsp"] = _rel(rp, vol["rhsp"] + z["rhsp"])
+    print(f"{name}: " + ", ".join(f"{k} {v:.1e}" for k, v in errs.items()), flush=True)
+    assert max(errs.values()) <= 1e-12, errs
     ctx.close()
 
 
