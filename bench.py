@@ -514,7 +514,7 @@ def quench_steps(L, C, S, n=48, nsteps=20, nnewton=3, max_cycles=600):
     runs, fields = {}, {}
     for prec, tag in ((L.OFD_FP64, "fp64"), (L.OFD_FP32, "mixed")):
         ctx.set_state(prob.fields)
-        step_ms, it_v, it_t, rel = [], [], [], []
+        step_ms, it_v, it_t, rel, drop = [], [], [], [], []
         for _ in range(nsteps):
             t0 = time.perf_counter()
             info = S.quench_step(ctx, nnewton=nnewton, rtol=1e-8, precision=prec,
@@ -525,17 +525,24 @@ def quench_steps(L, C, S, n=48, nsteps=20, nnewton=3, max_cycles=600):
             it_v.append([i["nsvof"]["iters"] for i in info])
             it_t.append([i["tem"]["iters"] for i in info])
             rel.append(max(max(i["nsvof"]["relres"], i["tem"]["relres"]) for i in info))
+            # the step's nonlinear residual after its last iteration against
+            # the predictor's, per field (u, p, phi, T)
+            drop.append([r / max(r0, 1e-300) for r, r0 in zip(info[-1]["res"], info[-1]["res0"])])
             log(f"  quench {tag} step {len(step_ms)}: {step_ms[-1]:.1f} ms, NSVOF its "
                 f"{it_v[-1]}, T its {it_t[-1]}, max relres {rel[-1]:.1e}")
         runs[tag] = {"ms_per_step": sum(step_ms) / nsteps, "step_ms": step_ms,
                      "nsvof_gmres_iters": it_v, "tem_gmres_iters": it_t,
                      "max_relres_per_step": rel, "max_relres": max(rel),
+                     "newton_residual_drop_per_step": drop,
+                     "newton_residual_drop_max": [max(d[j] for d in drop) for j in range(4)],
                      "all_converged": bool(max(rel) <= 1e-8)}
         ug = ctx.get_field("UG")
         fields[tag] = {"u": ug[:, 0], "v": ug[:, 1], "w": ug[:, 2], "p": ctx.get_field("PG"),
                        "phi": ctx.get_field("PHIG"), "T": ctx.get_field("TG")}
         log(f"bench: quench {n}^3 {nsteps} steps ({tag}): {runs[tag]['ms_per_step']:.1f} ms "
-            f"per step, max relres {max(rel):.1e}")
+            f"per step, max relres {max(rel):.1e}, nonlinear residual after {nnewton} "
+            f"iterations / after the predictor, worst step (u, p, phi, T): " +
+            ", ".join(f"{v:.1e}" for v in runs[tag]["newton_residual_drop_max"]))
     ctx.close()
     err = {}
     for k in fields["fp64"]:

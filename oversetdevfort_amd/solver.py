@@ -746,7 +746,9 @@ def quench_step(ctx: Context, nnewton=3, rtol=1e-8, precision=L.OFD_FP64,
     tolerance from the first (check_convergence, solveFlow.f90:178-194);
     without it, ``nnewton`` iterations run.  ``nsvof_right`` applies the
     NS + VOF system's block-Jacobi preconditioner on the right
-    (L.OFD_PREC_RIGHT).  Returns the per-iteration solver information."""
+    (L.OFD_PREC_RIGHT).  Returns the per-iteration solver information, with
+    the residual norms (u, p, phi, T) after each iteration (``res``) and
+    after the predictor (``res0``)."""
     ctx.predictor()
     ctx.apply_dirichlet()
     ctx.assemble_nsvof(2)
@@ -766,7 +768,7 @@ def quench_step(ctx: Context, nnewton=3, rtol=1e-8, precision=L.OFD_FP64,
         ctx.assemble_nsvof(2)
         ctx.assemble_tem(2)
         res = (*ctx.nsvof_rhs_norms(), ctx.tem_rhs_norm())
-        out.append(dict(nsvof=iv, tem=it, res=res))
+        out.append(dict(nsvof=iv, tem=it, res=res, res0=r0))
         if ntol is not None and all(r <= t * max(1e-15, r0_) for r, t, r0_ in
                                     zip(res, ntol, r0)):
             break
