@@ -659,13 +659,23 @@ def tet_assembly(C, S, reps=5):
     S.begin_step(ctx)
     vm_ms = ctx.time_assemble(6, reps=reps)
     v_ms = ctx.time_assemble(2, reps=reps)
+    # the four walls weak-Dirichlet: the tet face kernel (3 points per face)
+    f2e, fo = M.boundary_faces(prob.mesh, M.boundary_masks(
+        prob.mesh, (0.0, 0.0, 0.0), (4.0, 1.0, 1.0))["ylo"] | M.boundary_masks(
+        prob.mesh, (0.0, 0.0, 0.0), (4.0, 1.0, 1.0))["yhi"] | M.boundary_masks(
+        prob.mesh, (0.0, 0.0, 0.0), (4.0, 1.0, 1.0))["zlo"] | M.boundary_masks(
+        prob.mesh, (0.0, 0.0, 0.0), (4.0, 1.0, 1.0))["zhi"])
+    ctx.set_weak_faces(f2e, fo)
+    face_ms = ctx.time_faces(6, reps=20)
     ctx.close()
     out = {"workload": "channel 64x32x32 split into 6 tets per hex, "
                        f"{E} linear tets x 4 Gauss points",
            "elements": E, "vecmat_ms": vm_ms, "vec_ms": v_ms,
-           "vecmat_melem_per_s": E / (vm_ms * 1e-3) / 1e6}
+           "vecmat_melem_per_s": E / (vm_ms * 1e-3) / 1e6,
+           "weak_wall_faces": int(len(f2e)), "face_vecmat_ms": face_ms}
     log(f"bench: tet assembly: {E} tets, VEC+MAT {vm_ms:.3f} ms "
-        f"({out['vecmat_melem_per_s']:.0f} Melem/s), VEC {v_ms:.3f} ms")
+        f"({out['vecmat_melem_per_s']:.0f} Melem/s), VEC {v_ms:.3f} ms; "
+        f"{len(f2e)} weak wall faces (3 points each) {1e3 * face_ms:.1f} us")
     return out, prob
 
 
