@@ -449,7 +449,9 @@ int ofd_solve_newton(ofd_ctx *ctx, int subdomain_flag, int precision, int ortho,
  * 247-274).  J_T equals the assembled LHSTem when TEM_KDC = 0 and
  * C_DMDOT = 1: the LHS leaves out the derivative of the discontinuity-
  * capturing conductivity and scales that of the phase-change source by
- * C_DMDOT.  Vectors are (NNODE). */
+ * C_DMDOT.  Vectors are (NNODE).  Trilinear hexes take the hand-written
+ * kernel, linear tets the dual-number kernel of ofd_vjp_tem_dual (the same
+ * residual differentiated by forward mode). */
 
 /* y = LHSTem^T x. */
 int ofd_spmv_tem_t(ofd_ctx *ctx, int precision, const double *x, double *y);
@@ -486,7 +488,8 @@ int ofd_solve_tem_adjoint(ofd_ctx *ctx, int precision, int ortho, double rtol,
  * increments (acg += d, ug += gami*Delt*d, pg += alfi*gami*Delt*dp, rphig +=
  * dphi, phig += gami*Delt*dphi; solveFlow.f90:99-104) give J = -dR/d(d, dp,
  * dphi).  Derivatives come from forward-mode dual numbers through the whole
- * element residual (no hand differentiation); hexes only. */
+ * element residual (no hand differentiation); trilinear hexes and linear
+ * tets. */
 
 /* Replace the stored NS + VOF matrix (the layout ofd_get_nsvof_system
  * reads) by J_bc: J with the rows and columns of constrained dofs cleared

@@ -267,12 +267,17 @@ _HEX_SGN = np.array([[-1, 1, -1], [1, 1, -1], [1, 1, 1], [-1, 1, 1],
 
 
 def lumped_volumes(mesh, elements=None):
-    """Row sums of the hex mass matrix, sum_e int N_a dV (2x2x2 Gauss rule),
-    over all elements or the boolean selection ``elements``."""
+    """Row sums of the mass matrix, sum_e int N_a dV, over all elements or
+    the boolean selection ``elements``: hexes by the 2x2x2 Gauss rule, linear
+    tets exactly (a quarter of the element's volume at each node)."""
     g = 1.0 / np.sqrt(3.0)
     ien = mesh.ien if elements is None else mesh.ien[elements]
-    X = mesh.xg[ien]                                   # (E, 8, 3)
+    X = mesh.xg[ien]                                   # (E, 8, 3) or (E, 4, 3)
     vol = np.zeros(mesh.nnode)
+    if ien.shape[1] == 4:
+        d = np.abs(np.linalg.det(X[:, :3] - X[:, 3:4])) / 6.0
+        np.add.at(vol, ien.ravel(), np.repeat(0.25 * d, 4))
+        return vol
     for q in range(8):
         xi = np.array([g if q & 1 else -g, g if q & 2 else -g, g if q & 4 else -g])
         f = 1.0 + _HEX_SGN * xi                        # (8, 3)
@@ -322,7 +327,7 @@ def inflow_bump(x, lo=(0.3, 0.1), hi=(0.7, 0.4)):
     return prof, on
 
 
-def quench_inflow(dims=(16, 16, 16), seed=5, amp=0.05, diffuse=0.0, rate=None):
+def quench_inflow(dims=(16, 16, 16), seed=5, amp=0.05, diffuse=0.0, rate=None, tets=False):
     """The quench tank with an inlet: cold liquid (phi = 0, 300 K) enters
     through a patch of the wall x = 0 with the normal velocity amp x
     ``inflow_bump`` and leaves through the open top; the inlet nodes hold
@@ -333,8 +338,11 @@ def quench_inflow(dims=(16, 16, 16), seed=5, amp=0.05, diffuse=0.0, rate=None):
     ``diffuse`` > 0 replaces the sharp VOF field by the diffuse interface
     0.5 + 0.45 tanh((z - 0.6) / diffuse) (no bubble; every value inside (0, 1)
     away from the clamps); ``rate`` sets both phase-change coefficients
-    (c_evap, c_cond; 0.1 in the tank's material)."""
+    (c_evap, c_cond; 0.1 in the tank's material); ``tets`` splits each hex
+    into 6 linear tets (the node fields and conditions are unchanged)."""
     prob = quench(dims=dims, seed=seed)
+    if tets:
+        prob.mesh = M.hex_to_tets(prob.mesh)
     m = prob.mesh
     N = m.nnode
     if diffuse > 0.0:

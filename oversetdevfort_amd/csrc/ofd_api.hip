@@ -2616,7 +2616,7 @@ static int ensure_tem_adjoint(Ctx &c) {
     if (rc) return rc;
     OFD_HIP(hipMemsetAsync(*b, 0, np * sizeof(double), c.stream));
   }
-  rc = dalloc(c, &c.d_vjpT, (size_t)c.nnode * 6);
+  rc = dalloc(c, &c.d_vjpT, (size_t)c.nnode * 7);
   if (rc) return rc;
   c.tadj_n4 = c.temT_n4;
   return OFD_OK;
@@ -2660,8 +2660,8 @@ static int tem_adjoint_ready(Ctx &c) {
   int rc = ensure_tem_adjoint(c);
   if (rc) return rc;
   OFD_CHECK(c.quench, OFD_ERR_ARG, "no quench parameters (ofd_set_quench_params)");
-  OFD_CHECK(c.nshl == 8, OFD_ERR_UNSUPPORTED,
-            "the GPU element kernels are implemented for trilinear hexes (NSHL=8)");
+  OFD_CHECK(c.nshl == 8 || c.nshl == 4, OFD_ERR_UNSUPPORTED,
+            "the temperature adjoint is implemented for trilinear hexes and linear tets");
   return OFD_OK;
 }
 
@@ -2714,12 +2714,19 @@ int ofd_vjp_tem(ofd_ctx *x, const double *lam, double *gT, double *grT, double *
   rc = h2d(c, c.d_lamTb, lam, N * sizeof(double));
   if (rc) return rc;
   OFD_HIP(launch_alpha_stage(c));
-  OFD_HIP(launch_zero(c, c.d_vjpT, N * 6));
-  OFD_HIP(launch_vjp_tem(c, c.d_lamTb, c.d_vjpT));
+  // rows T, rT, u (3), then the scale (hexes) or phi, scale (tets, dual numbers)
+  size_t ks_row = 5;
+  if (c.nshl == 4) {
+    OFD_HIP(launch_vjp_tem_dual(c, c.d_lamTb, c.d_vjpT));
+    ks_row = 6;
+  } else {
+    OFD_HIP(launch_zero(c, c.d_vjpT, N * 6));
+    OFD_HIP(launch_vjp_tem(c, c.d_lamTb, c.d_vjpT));
+  }
   if (gT) rc = d2h(c, gT, c.d_vjpT, N * sizeof(double));
   if (!rc && grT) rc = d2h(c, grT, c.d_vjpT + N, N * sizeof(double));
   if (!rc && gu) rc = d2h(c, gu, c.d_vjpT + 2 * N, 3 * N * sizeof(double));
-  if (!rc && gks) rc = d2h(c, gks, c.d_vjpT + 5 * N, N * sizeof(double));
+  if (!rc && gks) rc = d2h(c, gks, c.d_vjpT + ks_row * N, N * sizeof(double));
   if (rc) return rc;
   OFD_HIP(hipStreamSynchronize(c.stream));
   return OFD_OK;
@@ -2848,8 +2855,8 @@ static int nsvof_adjoint_ready(Ctx &c) {
   int rc = ensure_nsvof(c);
   if (rc) return rc;
   OFD_CHECK(c.quench, OFD_ERR_ARG, "no quench parameters (ofd_set_quench_params)");
-  OFD_CHECK(c.nshl == 8, OFD_ERR_UNSUPPORTED,
-            "the dual-number kernels are implemented for trilinear hexes (NSHL=8)");
+  OFD_CHECK(c.nshl == 8 || c.nshl == 4, OFD_ERR_UNSUPPORTED,
+            "the dual-number kernels are implemented for trilinear hexes and linear tets");
   OFD_HIP(hipSetDevice(c.device));
   if (c.vadj_n4 != c.nsvV_n4 || !c.d_vjpV) {
     const size_t np = (size_t)c.nsvV_n4 * 4;
@@ -2944,7 +2951,7 @@ int ofd_vjp_tem_dual(ofd_ctx *x, const double *lam, double *g) {
   OFD_HIP(launch_zero(c, c.d_vjpV, N * 7));
   const double *ks = c.ks_on ? c.d_field[OFD_KSCALE] : nullptr;
   for (int col = 0; col + 1 < (int)c.color_off.size(); ++col)
-    OFD_HIP(launch_tem_dual_vjp(c, col, ks, c.d_lamVb, c.d_vjpV));
+    OFD_HIP(launch_tem_dual_vjp(c, col, ks, c.d_lamVb, c.d_vjpV, nullptr));
   rc = d2h(c, g, c.d_vjpV, N * 7 * sizeof(double));
   if (rc) return rc;
   OFD_HIP(hipStreamSynchronize(c.stream));

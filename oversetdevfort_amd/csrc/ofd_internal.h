@@ -286,7 +286,11 @@ hipError_t launch_nsvof_jac(Ctx &c, int color);
 hipError_t launch_nsvof_vjp(Ctx &c, int color, const double *lam4, const double *lamphi,
                             double *out);
 hipError_t launch_tem_dual_vjp(Ctx &c, int color, const double *ks, const double *lam,
-                               double *out);
+                               double *out, const uint8_t *tmask);
+// the temperature adjoint's operators by dual numbers: raw gradients [7][N]
+// (T, rT, u (3), phi, scale; zeroed here), y = J_T,bc^T lambda (scratch [7N])
+hipError_t launch_vjp_tem_dual(Ctx &c, const double *lam, double *out);
+hipError_t launch_jt_tem_dual(Ctx &c, const double *lam, double *y, double *scratch);
 // At = the stored NS + VOF matrix transposed without its (phi, p) slot;
 // y = A^T x exactly (FP64)
 hipError_t launch_nsvof_transpose(Ctx &c, double *At);

@@ -1,5 +1,7 @@
 // Exact derivatives of the quenching solver's element residuals by forward-
-// mode dual numbers on gfx950 (no reference counterpart), hex elements.
+// mode dual numbers on gfx950 (no reference counterpart), trilinear hexes
+// and linear tets (S = 8 or 4 nodes and Gauss points, the forward kernels'
+// rules).
 //
 // Residuals differentiated, as the forward kernels assemble them:
 //   NS + VOF  the RHSGu / RHSGp / RHSGls of IntElmAss_NSVOF_Quenching_STAB
@@ -15,7 +17,7 @@
 // Jacobian, exact to rounding, branches (phase change side, VOF clamp) taken
 // where the values put them.  Nothing is hand-differentiated.
 //
-//   k_nsvof_dual<JAC>  lane j < 40: the Newton increment (d_u (3), d_p,
+//   k_nsvof_dual<JAC>  lane j < 5 S: the Newton increment (d_u (3), d_p,
 //                      d_phi) of node j / 5, i.e. the alpha-stage tangents
 //                      ac = almi, u = alfi gami Delt for d_u; p = alfi gami
 //                      Delt for d_p; rphi = almi, phi = alfi gami Delt for
@@ -26,15 +28,17 @@
 //                      layout (the 4x4 block [K11 G; D1 M] and the phi pair
 //                      group of each nonzero) by the first-touch read-modify-
 //                      write of the colour launches.
-//   k_nsvof_dual<VJP>  lanes j and j + 64 (j + 64 < 80): the alpha-stage
+//   k_nsvof_dual<VJP>  lanes j (and j + 64 < 10 S): the alpha-stage
 //                      input j % 10 (u (3), ac (3), p, phi, rphi, T) of node
 //                      j / 10; the lane's tangent contracted with lambda
 //                      gives d(lambda . R)/d(input), added per node into
 //                      out[input][N] (colours: no two elements of a launch
 //                      share a node).
-//   k_tem_dual         lane j < 56: input j % 7 (T, rT, u (3), phi, the
+//   k_tem_dual         lane j < 7 S: input j % 7 (T, rT, u (3), phi, the
 //                      conductivity scale) of node j / 7, contracted with
-//                      lambda as the VJP mode.
+//                      lambda as the VJP mode; with the held-T mask it gives,
+//                      through k_tem_jt_combine, J_T,bc^T lambda (the
+//                      temperature adjoint's operator on tets).
 // One wavefront per element: lanes (q, a) first stage the shape functions,
 // dN/dx, the metric G and DetJ w of the 8 Gauss points in LDS (the forward
 // kernel's expressions, so the same bits), and the element's node records.
@@ -98,9 +102,10 @@ struct ShapeLds {
   double dw[8];        // [q] DetJ * weight
 };
 
+template <int S>
 __device__ __forceinline__ void stage_shape(ShapeLds &s, const double *xd, int q, int a) {
   double Na, dl0, dl1, dl2, gwq;
-  elem_shape<8>(q, a, Na, dl0, dl1, dl2, gwq);
+  elem_shape<S>(q, a, Na, dl0, dl1, dl2, gwq);
   double J[9];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
@@ -150,7 +155,7 @@ enum { kDualJac = 0, kDualVjp = 1 };
 
 // node record (alpha stage): x+d (0-2), u (3-5), ac (6-8), um (9-11), p (12),
 // phi (13), T (14), rphi (15)
-template <int MODE>
+template <int S, int MODE>
 __global__ void __launch_bounds__(64 * kDW)
 k_nsvof_dual(int E, const int32_t *__restrict__ ien, const int32_t *__restrict__ elm_id,
              const int32_t *__restrict__ elems, int first, int count,
@@ -172,10 +177,11 @@ k_nsvof_dual(int E, const int32_t *__restrict__ ien, const int32_t *__restrict__
   ShapeLds &sh = s_shape[w];
   {
     const int q = lane >> 3, a = lane & 7;
-    const int na = valid ? ien[(size_t)a * E + e] : 0;
+    // node lanes a >= S (tets) repeat node 0 with zero shape functions
+    const int na = valid ? ien[(size_t)(a < S ? a : 0) * E + e] : 0;
     const double *r = rec + (size_t)na * 16;
     double xd[3] = {r[0], r[1], r[2]};
-    stage_shape(sh, xd, q, a);
+    stage_shape<S>(sh, xd, q, a);
     if (q == 0) {
       s_node[w][a] = na;
 #pragma unroll
@@ -191,11 +197,12 @@ k_nsvof_dual(int E, const int32_t *__restrict__ ien, const int32_t *__restrict__
 
   // the lane's directions and their seeds on its node's alpha-stage inputs:
   // tangents of u (3), ac (3), p, phi, rphi, T
-  constexpr int kPasses = MODE == kDualJac ? 1 : 2;
+  constexpr int kNdir = MODE == kDualJac ? 5 * S : 10 * S;
+  constexpr int kPasses = (kNdir + 63) / 64;
   const double f2 = P.alfi * P.gami * P.delt;
   for (int pass = 0; pass < kPasses; ++pass) {
     const int dir = lane + 64 * pass;
-    const int ndir = MODE == kDualJac ? 40 : 80;
+    const int ndir = kNdir;
     const bool act = dir < ndir;
     const int nb = MODE == kDualJac ? dir / 5 : dir / 10;
     const int cb = MODE == kDualJac ? dir % 5 : dir % 10;
@@ -223,23 +230,23 @@ k_nsvof_dual(int E, const int32_t *__restrict__ ien, const int32_t *__restrict__
       }
     }
     // JAC: tangents of the 8 x 5 residual entries; VJP: lambda . tangent
-    double acc[8][5];
+    double acc[S][5];
 #pragma unroll
-    for (int a = 0; a < 8; ++a)
+    for (int a = 0; a < S; ++a)
 #pragma unroll
       for (int c = 0; c < 5; ++c) acc[a][c] = 0.0;
     double vj = 0.0;
 
     if (fluid) {
       const double cr = 1 / P.rhoa - 1 / P.rhow;
-      for (int q = 0; q < 8; ++q) {
+      for (int q = 0; q < S; ++q) {
         const double Nb = sh.N[q][b];
         const double gb[3] = {sh.g[q][b][0], sh.g[q][b][1], sh.g[q][b][2]};
         // Gauss-point values in node order, their tangents from node b
         double ui[3] = {0, 0, 0}, aci[3] = {0, 0, 0}, umi[3] = {0, 0, 0};
         double du[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, dpv[3] = {0, 0, 0}, dphv[3] = {0, 0, 0};
         double pri = 0.0, phii = 0.0, Tiv = 0.0, rphii = 0.0;
-        for (int a = 0; a < 8; ++a) {
+        for (int a = 0; a < S; ++a) {
           const double Na = sh.N[q][a];
           const double *r = s_rec[w][a];
 #pragma unroll
@@ -323,7 +330,7 @@ k_nsvof_dual(int E, const int32_t *__restrict__ ien, const int32_t *__restrict__
         const Dd tauC = rhoi * dsqrt(taua + taud) / (G[0] + G[4] + G[8]);
         Dd kdcphi = dd(0.0);
         if (fabs(P.lsc_kdc) > 0.0) kdcphi = P.lsc_kdc * dabs(res) / sqrt(gg);
-        Dd up[3], t1[3], S[3];
+        Dd up[3], t1[3], Sv[3];
 #pragma unroll
         for (int i = 0; i < 3; ++i) up[i] = -(tauM * rLi[i]);
 #pragma unroll
@@ -331,17 +338,17 @@ k_nsvof_dual(int E, const int32_t *__restrict__ ien, const int32_t *__restrict__
           t1[i] = rhoi * ac[i] + rhoi * uadv[0] * D[i + 0] + rhoi * uadv[1] * D[i + 3] +
                   rhoi * uadv[2] * D[i + 6] - fi[i];
         const Dd dv = divu - vdot;
-        // the residual rows of the 8 nodes at this point (:496-521)
+        // the residual rows of the S nodes at this point (:496-521)
 #pragma unroll
-        for (int a = 0; a < 8; ++a) {
+        for (int a = 0; a < S; ++a) {
           const double Na = sh.N[q][a];
           const double ga[3] = {sh.g[q][a][0], sh.g[q][a][1], sh.g[q][a][2]};
           const Dd sc = uadv[0] * ga[0] + uadv[1] * ga[1] + uadv[2] * ga[2];
 #pragma unroll
           for (int k = 0; k < 3; ++k)
-            S[k] = ga[0] * (D[0 + 3 * k] + D[k + 0]) + ga[1] * (D[1 + 3 * k] + D[k + 3]) +
+            Sv[k] = ga[0] * (D[0 + 3 * k] + D[k + 0]) + ga[1] * (D[1 + 3 * k] + D[k + 3]) +
                    ga[2] * (D[2 + 3 * k] + D[k + 6]);
-          const Dd visc = mui * S[0] * dw + mui * S[1] * dw + mui * S[2] * dw;
+          const Dd visc = mui * Sv[0] * dw + mui * Sv[1] * dw + mui * Sv[2] * dw;
           Dd r5[5];
 #pragma unroll
           for (int i = 0; i < 3; ++i)
@@ -371,7 +378,7 @@ k_nsvof_dual(int E, const int32_t *__restrict__ ien, const int32_t *__restrict__
         const int colbit = cb < 3 ? cb : (cb == 3 ? 6 : 7);
         const bool colz = (mb >> colbit) & 1u;
 #pragma unroll
-        for (int a = 0; a < 8; ++a) {
+        for (int a = 0; a < S; ++a) {
           const unsigned ma = ibc[s_node[w][a]];
           const int pl = a + 8 * b;
           const int k = emap[(size_t)e * 64 + pl];
@@ -411,15 +418,17 @@ struct TemDualParams {
   int fluid_id;
 };
 
-// VJP of the temperature residual: lane j < 56 = (node j / 7, input j % 7:
+// VJP of the temperature residual: lane j < 7 S = (node j / 7, input j % 7:
 // T, rT, u (3), phi, conductivity scale); out[input][N] += d(lambda . R)/d.
 // Node record as above with rT in slot 15 (the alpha stage of the
 // temperature system); the scale from ks (none: 1).
+template <int S>
 __global__ void __launch_bounds__(64 * kDW)
 k_tem_dual(int E, const int32_t *__restrict__ ien, const int32_t *__restrict__ elm_id,
            const int32_t *__restrict__ elems, int first, int count,
            const double *__restrict__ rec, const double *__restrict__ ks, TemDualParams P, int N,
-           const double *__restrict__ lam, double *__restrict__ out) {
+           const double *__restrict__ lam, const uint8_t *__restrict__ tmask,
+           double *__restrict__ out) {
   __shared__ ShapeLds s_shape[kDW];
   __shared__ double s_rec[kDW][8][12];  // x+d, u, um, phi, T, rT, ks
   __shared__ int32_t s_node[kDW][8];
@@ -433,10 +442,10 @@ k_tem_dual(int E, const int32_t *__restrict__ ien, const int32_t *__restrict__ e
   ShapeLds &sh = s_shape[w];
   {
     const int q = lane >> 3, a = lane & 7;
-    const int na = valid ? ien[(size_t)a * E + e] : 0;
+    const int na = valid ? ien[(size_t)(a < S ? a : 0) * E + e] : 0;
     const double *r = rec + (size_t)na * 16;
     double xd[3] = {r[0], r[1], r[2]};
-    stage_shape(sh, xd, q, a);
+    stage_shape<S>(sh, xd, q, a);
     if (q == 0) {
       s_node[w][a] = na;
       double *sr = s_rec[w][a];
@@ -450,12 +459,14 @@ k_tem_dual(int E, const int32_t *__restrict__ ien, const int32_t *__restrict__ e
       sr[7] = r[14];
       sr[8] = r[15];
       sr[9] = ks ? ks[na] : 1.0;
-      s_lam[w][a] = lam[na];
+      // tmask: lambda of the nodes whose temperature is held (IBC column 6)
+      // taken as 0, the projection J_T,bc^T applies before the element
+      s_lam[w][a] = (tmask && ((tmask[na] >> 5) & 1u)) ? 0.0 : lam[na];
     }
   }
   __syncthreads();
   const int dir = lane;
-  const bool act = dir < 56;
+  const bool act = dir < 7 * S;
   const int b = act ? dir / 7 : 0;
   const int cb = dir % 7;
   double tT = 0.0, trT = 0.0, tu[3] = {0.0, 0.0, 0.0}, tphi = 0.0, tks = 0.0;
@@ -468,12 +479,12 @@ k_tem_dual(int E, const int32_t *__restrict__ ien, const int32_t *__restrict__ e
   }
   double vj = 0.0;
   if (valid) {
-    for (int q = 0; q < 8; ++q) {
+    for (int q = 0; q < S; ++q) {
       const double Nb = sh.N[q][b];
       const double gb[3] = {sh.g[q][b][0], sh.g[q][b][1], sh.g[q][b][2]};
       double ui[3] = {0, 0, 0}, umi[3] = {0, 0, 0}, dTv[3] = {0, 0, 0};
       double phii = 0.0, Tiv = 0.0, rTi = 0.0, ksi = 0.0;
-      for (int a = 0; a < 8; ++a) {
+      for (int a = 0; a < S; ++a) {
         const double Na = sh.N[q][a];
         const double *r = s_rec[w][a];
 #pragma unroll
@@ -537,7 +548,7 @@ k_tem_dual(int E, const int32_t *__restrict__ ien, const int32_t *__restrict__ e
       const Dd hk = fluid ? hki + kdc : hki;
       // the residual of node a at this point (:841-855)
 #pragma unroll
-      for (int a = 0; a < 8; ++a) {
+      for (int a = 0; a < S; ++a) {
         const double Na = sh.N[q][a];
         const double ga[3] = {sh.g[q][a][0], sh.g[q][a][1], sh.g[q][a][2]};
         Dd rt = dd(0.0);
@@ -593,7 +604,8 @@ hipError_t launch_nsvof_jac(Ctx &c, int color) {
   const int first = c.color_off[color];
   const int count = c.color_off[color + 1] - first;
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_nsvof_dual<kDualJac>, dim3((count + kDW - 1) / kDW), dim3(64 * kDW), 0,
+  hipLaunchKernelGGL((c.nshl == 4 ? k_nsvof_dual<4, kDualJac> : k_nsvof_dual<8, kDualJac>),
+                     dim3((count + kDW - 1) / kDW), dim3(64 * kDW), 0,
                      c.stream, c.nelem, c.d_ien, c.d_elm_id, c.d_color_elems, first, count,
                      c.d_emap, c.d_first, c.d_noderec, c.d_ibc, nsvof_dual_params(c), c.nnode,
                      c.d_AV, c.d_AV + (size_t)16 * c.icnt, nullptr, nullptr, nullptr);
@@ -605,7 +617,8 @@ hipError_t launch_nsvof_vjp(Ctx &c, int color, const double *lam4, const double 
   const int first = c.color_off[color];
   const int count = c.color_off[color + 1] - first;
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_nsvof_dual<kDualVjp>, dim3((count + kDW - 1) / kDW), dim3(64 * kDW), 0,
+  hipLaunchKernelGGL((c.nshl == 4 ? k_nsvof_dual<4, kDualVjp> : k_nsvof_dual<8, kDualVjp>),
+                     dim3((count + kDW - 1) / kDW), dim3(64 * kDW), 0,
                      c.stream, c.nelem, c.d_ien, c.d_elm_id, c.d_color_elems, first, count,
                      c.d_emap, c.d_first, c.d_noderec, c.d_ibc, nsvof_dual_params(c), c.nnode,
                      nullptr, nullptr, lam4, lamphi, out);
@@ -613,7 +626,7 @@ hipError_t launch_nsvof_vjp(Ctx &c, int color, const double *lam4, const double 
 }
 
 hipError_t launch_tem_dual_vjp(Ctx &c, int color, const double *ks, const double *lam,
-                               double *out) {
+                               double *out, const uint8_t *tmask) {
   const int first = c.color_off[color];
   const int count = c.color_off[color + 1] - first;
   if (count <= 0) return hipSuccess;
@@ -637,9 +650,60 @@ hipError_t launch_tem_dual_vjp(Ctx &c, int color, const double *ks, const double
   P.tem_kdc = q[OFD_Q_TEMKDC];
   P.tref = q[OFD_Q_TREF];
   P.fluid_id = (int)q[OFD_Q_FLUID_ID];
-  hipLaunchKernelGGL(k_tem_dual, dim3((count + kDW - 1) / kDW), dim3(64 * kDW), 0, c.stream,
+  hipLaunchKernelGGL((c.nshl == 4 ? k_tem_dual<4> : k_tem_dual<8>),
+                     dim3((count + kDW - 1) / kDW), dim3(64 * kDW), 0, c.stream,
                      c.nelem, c.d_ien, c.d_elm_id, c.d_color_elems, first, count, c.d_noderec,
-                     ks, P, c.nnode, lam, out);
+                     ks, P, c.nnode, lam, tmask, out);
+  return hipGetLastError();
+}
+
+// y = J_T,bc^T lambda from the raw VJP g [7][N] of the masked lambda: free
+// rows -(alfi gami Delt g_T + almi g_rT) (the increments rTg += d, Tg +=
+// gami Delt d), held rows lambda times the number of elements at the node
+// (BCLhs_tem puts 1 on the diagonal once per element); padding rows 0
+__global__ void k_tem_jt_combine(int N, int npad, const double *__restrict__ g,
+                                 const double *__restrict__ lam,
+                                 const uint8_t *__restrict__ ibc,
+                                 const int32_t *__restrict__ inc_off, double cT, double almi,
+                                 double *__restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npad) return;
+  if (i >= N) {
+    y[i] = 0.0;
+    return;
+  }
+  const bool held = (ibc[i] >> 5) & 1u;
+  y[i] = held ? (double)(inc_off[i + 1] - inc_off[i]) * lam[i]
+              : -(cT * g[i] + almi * g[(size_t)N + i]);
+}
+
+// the temperature adjoint's operators by dual numbers (any element the
+// dual kernels take): raw gradients into out [7][N] (zeroed here), or
+// y = J_T,bc^T lambda with out as scratch
+hipError_t launch_vjp_tem_dual(Ctx &c, const double *lam, double *out) {
+  hipError_t e = launch_zero(c, out, (size_t)c.nnode * 7);
+  if (e != hipSuccess) return e;
+  const double *ks = c.ks_on ? c.d_field[OFD_KSCALE] : nullptr;
+  for (int col = 0; col + 1 < (int)c.color_off.size(); ++col) {
+    e = launch_tem_dual_vjp(c, col, ks, lam, out, nullptr);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t launch_jt_tem_dual(Ctx &c, const double *lam, double *y, double *scratch) {
+  hipError_t e = launch_zero(c, scratch, (size_t)c.nnode * 7);
+  if (e != hipSuccess) return e;
+  const double *ks = c.ks_on ? c.d_field[OFD_KSCALE] : nullptr;
+  for (int col = 0; col + 1 < (int)c.color_off.size(); ++col) {
+    e = launch_tem_dual_vjp(c, col, ks, lam, scratch, c.d_ibc);
+    if (e != hipSuccess) return e;
+  }
+  const int npad = c.temT_n4 * 4;
+  hipLaunchKernelGGL(k_tem_jt_combine, dim3((npad + 255) / 256), dim3(256), 0, c.stream,
+                     c.nnode, npad, scratch, lam, c.d_ibc, c.d_inc_off,
+                     c.prm[OFD_P_ALFI] * c.prm[OFD_P_GAMI] * c.prm[OFD_P_DELT],
+                     c.prm[OFD_P_ALMI], y);
   return hipGetLastError();
 }
 

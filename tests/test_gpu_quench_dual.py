@@ -15,7 +15,8 @@ forward assembly (the restatement of solveQuenching.f90:122-573 and
 * the temperature VJP by dual numbers against the hand-written one
   (ofd_vjp_tem) and, for the VOF field, against central differences.
 
-States: a 5^3 tank with perturbed fields, the VOF field strictly inside
+States: a 5^3 tank (trilinear hexes, and the same tank with each hex split
+into 6 linear tets) with perturbed fields, the VOF field strictly inside
 (0, 1) (the clamps' kinks are away from the differences) and temperatures on
 both sides of T_sat, random Dirichlet nodes.  Differences with steps of 1e-6
 of each field's scale agree to about 1e-7; the tests hold 1e-5.
@@ -25,6 +26,7 @@ import pytest
 
 from oversetdevfort_amd import _lib as L
 from oversetdevfort_amd import configs as C
+from oversetdevfort_amd import mesh as M
 
 pytestmark = pytest.mark.gpu
 
@@ -35,11 +37,13 @@ def _rel(a, b):
     return np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-300)
 
 
-@pytest.fixture(scope="module")
-def case():
+@pytest.fixture(scope="module", params=["hex", "tet"])
+def case(request):
     from oversetdevfort_amd import solver as S
     rng = np.random.default_rng(7)
     prob = C.quench(5, seed=3, jitter=0.05)
+    if request.param == "tet":
+        prob.mesh = M.hex_to_tets(prob.mesh)
     m = prob.mesh
     N = m.nnode
     f = prob.fields

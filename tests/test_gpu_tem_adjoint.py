@@ -20,6 +20,9 @@ code).
   then the exact Jacobian), nonzero with it.
 * The adjoint solve J_T,bc^T lam = g (flexible GMRES preconditioned by
   LHSTem^T) to a 1e-8 true residual, FP64 and mixed.
+* The VJP, J_T,bc^T and the adjoint solve again on linear tets (each hex
+  of the tank split into 6), where both operators come from the
+  dual-number kernel.
 """
 import numpy as np
 import pytest
@@ -27,6 +30,7 @@ import scipy.sparse as sp
 
 from oversetdevfort_amd import _lib as L
 from oversetdevfort_amd import configs as C
+from oversetdevfort_amd import mesh as M
 
 pytestmark = pytest.mark.gpu
 
@@ -48,13 +52,16 @@ UNIT_MATERIAL = dict(rhoa=0.5, rhow=1.0, cpa=0.8, cpw=1.0, kappaa=0.05, kappaw=0
                      cps=0.5, kappas=1.0, lh=20.0, Ts=1.0, tref=1.0)
 
 
-def _case(S, n=5, seed=1, tem_kdc=0.0, frac_ibc=0.15, kscale=True, unit=False):
+def _case(S, n=5, seed=1, tem_kdc=0.0, frac_ibc=0.15, kscale=True, unit=False, tet=False):
     """A small quench tank with a perturbed state (temperatures on both sides
     of T_sat, VOF values in the interface range, random T Dirichlet nodes) and
     a per-node conductivity scale; ``unit``: the order-one material and a
-    10 ms step."""
+    10 ms step; ``tet``: each hex split into 6 linear tets (the operators then
+    come from the dual-number kernel)."""
     rng = np.random.default_rng(seed)
     prob = C.quench(n, seed=seed, jitter=0.05)
+    if tet:
+        prob.mesh = M.hex_to_tets(prob.mesh)
     if unit:
         prob.material.update(UNIT_MATERIAL)
         prob.params["delt"] = 1e-2
@@ -142,11 +149,13 @@ def test_tem_transposed_spmv_and_solve(S, oracle_mod):
     ctx.close()
 
 
-@pytest.mark.parametrize("kdc,unit", [(0.0, True), (0.3, True), (0.0, False), (0.3, False)])
-def test_tem_vjp_matches_central_differences(S, kdc, unit):
-    ctx, prob, rng, mat, ks = _case(S, seed=23, tem_kdc=kdc, unit=unit)
+@pytest.mark.parametrize("kdc,unit,tet", [(0.0, True, False), (0.3, True, False),
+                                          (0.0, False, False), (0.3, False, False),
+                                          (0.3, True, True), (0.0, False, True)])
+def test_tem_vjp_matches_central_differences(S, kdc, unit, tet):
+    ctx, prob, rng, mat, ks = _case(S, seed=23, tem_kdc=kdc, unit=unit, tet=tet)
     tol = 1e-6 if unit else 5e-5
-    what = "order-one material" if unit else "quench material"
+    what = ("order-one material" if unit else "quench material") + (", tets" if tet else "")
     N = prob.mesh.nnode
     p, f = prob.params, prob.fields
     free = (prob.ibc[:, 5] == 0).astype(np.float64)
@@ -182,11 +191,13 @@ def _increment(ctx, f, v, s, p):
     ctx.set_field("TG", f["Tg"] + p["gami"] * p["delt"] * s * v)
 
 
-@pytest.mark.parametrize("kdc,unit", [(0.0, True), (0.3, True), (0.0, False), (0.3, False)])
-def test_tem_jt_apply_and_consistency_gap(S, kdc, unit):
-    ctx, prob, rng, mat, ks = _case(S, seed=24, tem_kdc=kdc, unit=unit)
+@pytest.mark.parametrize("kdc,unit,tet", [(0.0, True, False), (0.3, True, False),
+                                          (0.0, False, False), (0.3, False, False),
+                                          (0.0, True, True), (0.3, False, True)])
+def test_tem_jt_apply_and_consistency_gap(S, kdc, unit, tet):
+    ctx, prob, rng, mat, ks = _case(S, seed=24, tem_kdc=kdc, unit=unit, tet=tet)
     tol = 1e-6 if unit else 5e-5
-    what = "order-one material" if unit else "quench material"
+    what = ("order-one material" if unit else "quench material") + (", tets" if tet else "")
     N = prob.mesh.nnode
     p, f = prob.params, prob.fields
     free = (prob.ibc[:, 5] == 0).astype(np.float64)
@@ -223,8 +234,9 @@ def test_tem_jt_apply_and_consistency_gap(S, kdc, unit):
     ctx.close()
 
 
-def test_tem_adjoint_solve(S):
-    ctx, prob, rng, mat, ks = _case(S, n=6, seed=25, tem_kdc=0.3)
+@pytest.mark.parametrize("tet", [False, True])
+def test_tem_adjoint_solve(S, tet):
+    ctx, prob, rng, mat, ks = _case(S, n=6, seed=25, tem_kdc=0.3, tet=tet)
     N = prob.mesh.nnode
     ctx.assemble_tem(6)
     g = rng.normal(size=N) * (prob.ibc[:, 5] == 0)
