@@ -379,7 +379,8 @@ int ofd_time_read(ofd_ctx *ctx, int64_t nbytes, int reps, double *ms);
  * (acg += d, ug += gami*Delt d, pg += alfi*gami*Delt dp) is
  * J = -dF/d(d, dp); J_bc is J with the Dirichlet rows and columns of IBC
  * replaced as BCLhs_3D does (identity per element sharing the node).  J
- * differs from the assembled LHS (SUPG/PSPG blocks, Picard convection). */
+ * differs from the assembled LHS (SUPG/PSPG blocks, Picard convection).
+ * Trilinear hexes and linear tets. */
 
 /* Transposed product of the assembled LHS: y = LHS^T x, (N,3)+(N) vectors. */
 int ofd_spmv_up_t(ofd_ctx *ctx, int precision, const double *xu, const double *xp,

@@ -34,9 +34,12 @@
 // on the diagonal per element) transposes to: lambda masked before, the
 // output masked after, plus lambda at each constrained dof once per element.
 //
-// Layout as the assembly: one 64-lane wavefront per element, lane = (Gauss
-// point q, node a), 8-lane butterflies for the Gauss-point sums, a butterfly
-// over q for the node sums, and a coloured read-modify-write scatter.
+// Layout: one 64-lane wavefront per element, lane = (Gauss point q, node
+// a), 8-lane butterflies for the Gauss-point sums, a butterfly over q for
+// the node sums, and a coloured read-modify-write scatter.  Trilinear hexes
+// fill the 8 x 8 lanes; on linear tets the lanes a >= 4 carry zero shape
+// functions and the lanes q >= 4 zero weight, so they add nothing.
+#include "element_shape.h"
 #include "ofd_internal.h"
 #include "wave_dpp.h"
 
@@ -45,8 +48,6 @@ namespace ofd {
 namespace {
 
 constexpr int kWaves = 4;
-__constant__ int8_t c_sgn_adj[8][3] = {{-1, 1, -1}, {1, 1, -1}, {1, 1, 1},  {-1, 1, 1},
-                                       {-1, -1, -1}, {1, -1, -1}, {1, -1, 1}, {-1, -1, 1}};
 
 __device__ __forceinline__ double red8(double v) { return wsum8(v); }
 
@@ -72,9 +73,9 @@ __global__ void k_block_transpose(size_t n16, const int32_t *__restrict__ tmap,
 // MODE 0: out = [gu (3,N) | gac (3,N) | gp (N) | gmus (N)] += raw gradients
 // MODE 1: out [N][4] += J_bc^T lam
 // MODE 2: out [N][4] += J_bc lam, lam a Newton increment (forward tangent)
-template <int MODE>
+template <int S, int MODE>
 __global__ void __launch_bounds__(256)
-k_vjp_ns_hex(int N, int E, const int32_t *__restrict__ ien, const int32_t *__restrict__ elm_id,
+k_vjp_ns(int N, int E, const int32_t *__restrict__ ien, const int32_t *__restrict__ elm_id,
              const int32_t *__restrict__ elems, int first, int count,
              const double *__restrict__ rec, const double *__restrict__ mus,
              const uint8_t *__restrict__ ibc, AdjParams P, int id1, int id2, int subdomain_flag,
@@ -92,7 +93,8 @@ k_vjp_ns_hex(int N, int E, const int32_t *__restrict__ ien, const int32_t *__res
   }
   const int q = lane >> 3;
   const int a = lane & 7;
-  const int na = valid ? ien[(size_t)a * E + e] : 0;
+  // node lanes a >= S (tets) repeat node 0 with zero shape functions
+  const int na = valid ? ien[(size_t)(a < S ? a : 0) * E + e] : 0;
   const unsigned m = valid ? ibc[na] : 0u;
 
   // alpha-stage record: x + d (0-2), u (3-5), ac (6-8), um (9-11), p (12)
@@ -122,19 +124,10 @@ k_vjp_ns_hex(int N, int E, const int32_t *__restrict__ ien, const int32_t *__res
 #pragma unroll
   for (int c = 0; c < 4; ++c) la[c] = (MODE != 0 && ((m >> c) & 1u)) ? 0.0 : lr[c];
 
-  // linhexshl shape functions at Gauss point q (the truncated abscissa)
-  const double gpt = 0.57735026918963;
-  const double xi = (q & 1) ? gpt : -gpt;
-  const double eta = (q & 2) ? gpt : -gpt;
-  const double zeta = (q & 4) ? gpt : -gpt;
-  const double sx = c_sgn_adj[a][0], sy = c_sgn_adj[a][1], sz = c_sgn_adj[a][2];
-  const double fx = (sx > 0) ? (xi + 1.0) : (1.0 - xi);
-  const double fy = (sy > 0) ? (eta + 1.0) : (1.0 - eta);
-  const double fz = (sz > 0) ? (zeta + 1.0) : (1.0 - zeta);
-  const double Na = fx * fy * fz * 0.125;
-  const double dl0 = sx * (fy * fz * 0.125);
-  const double dl1 = sy * (fx * fz * 0.125);
-  const double dl2 = sz * (fx * fy * 0.125);
+  // shape functions at Gauss point q (linhexshl with the truncated abscissa,
+  // lintetshl with the 4-point rule)
+  double Na, dl0, dl1, dl2, gwq;
+  elem_shape<S>(q, a, Na, dl0, dl1, dl2, gwq);
   double J[9];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
@@ -156,7 +149,7 @@ k_vjp_ns_hex(int N, int E, const int32_t *__restrict__ ien, const int32_t *__res
   Ji[2] = (J[1] * J[5] - J[4] * J[2]) * tmp;
   Ji[5] = (J[2] * J[3] - J[0] * J[5]) * tmp;
   Ji[8] = (J[0] * J[4] - J[3] * J[1]) * tmp;
-  const double dw = 1.0 / tmp;
+  const double dw = gwq * (1.0 / tmp);
   double ga[3];
 #pragma unroll
   for (int j = 0; j < 3; ++j)
@@ -363,7 +356,7 @@ k_vjp_ns_hex(int N, int E, const int32_t *__restrict__ ien, const int32_t *__res
     gp += __shfl_xor(gp, s);
     gmu += __shfl_xor(gmu, s);
   }
-  if (!valid || lane >= 8) return;
+  if (!valid || lane >= S) return;
   if (MODE == 0) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -483,7 +476,8 @@ static hipError_t vjp_colours(Ctx &c, int subdomain_flag, const double *lam, dou
     const int first = c.color_off[col];
     const int count = c.color_off[col + 1] - first;
     if (count <= 0) continue;
-    hipLaunchKernelGGL((k_vjp_ns_hex<MODE>), dim3((count + kWaves - 1) / kWaves),
+    hipLaunchKernelGGL((c.nshl == 4 ? k_vjp_ns<4, MODE> : k_vjp_ns<8, MODE>),
+                       dim3((count + kWaves - 1) / kWaves),
                        dim3(64 * kWaves), 0, c.stream, c.nnode, c.nelem, c.d_ien, c.d_elm_id,
                        c.d_color_elems, first, count, c.d_noderec, mus, c.d_ibc, P,
                        c.elm_id_set[0], c.elm_id_set[1], subdomain_flag, lam, out);

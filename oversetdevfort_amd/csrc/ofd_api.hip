@@ -2371,8 +2371,8 @@ int ofd_vjp_ns(ofd_ctx *x, int subdomain_flag, const double *lu, const double *l
   Ctx &c = x->c;
   int rc = ensure_adjoint(c);
   if (rc) return rc;
-  OFD_CHECK(c.nshl == 8, OFD_ERR_UNSUPPORTED,
-            "the GPU element kernels are implemented for trilinear hexes (NSHL=8)");
+  OFD_CHECK(c.nshl == 8 || c.nshl == 4, OFD_ERR_UNSUPPORTED,
+            "the NS adjoint kernels are implemented for trilinear hexes and linear tets");
   OFD_HIP(hipSetDevice(c.device));
   rc = put_node4(c, lu, lp, c.d_lamb);
   if (rc) return rc;
@@ -2394,8 +2394,8 @@ int ofd_jt_apply(ofd_ctx *x, int subdomain_flag, const double *lu, const double 
   Ctx &c = x->c;
   int rc = ensure_adjoint(c);
   if (rc) return rc;
-  OFD_CHECK(c.nshl == 8, OFD_ERR_UNSUPPORTED,
-            "the GPU element kernels are implemented for trilinear hexes (NSHL=8)");
+  OFD_CHECK(c.nshl == 8 || c.nshl == 4, OFD_ERR_UNSUPPORTED,
+            "the NS adjoint kernels are implemented for trilinear hexes and linear tets");
   OFD_HIP(hipSetDevice(c.device));
   rc = put_node4(c, lu, lp, c.d_lamb);
   if (rc) return rc;
@@ -2557,8 +2557,8 @@ int ofd_solve_adjoint(ofd_ctx *x, int subdomain_flag, int precision, int ortho, 
   Ctx &c = x->c;
   int rc = ensure_adjoint(c);
   if (rc) return rc;
-  OFD_CHECK(c.nshl == 8, OFD_ERR_UNSUPPORTED,
-            "the GPU element kernels are implemented for trilinear hexes (NSHL=8)");
+  OFD_CHECK(c.nshl == 8 || c.nshl == 4, OFD_ERR_UNSUPPORTED,
+            "the NS adjoint kernels are implemented for trilinear hexes and linear tets");
   OFD_HIP(hipSetDevice(c.device));
   rc = ensure_transpose(c);
   if (rc) return rc;
@@ -2587,8 +2587,8 @@ int ofd_solve_newton(ofd_ctx *x, int subdomain_flag, int precision, int ortho, d
   int rc = ensure_adjoint(c);
   if (rc) return rc;
   OFD_CHECK(c.mat_valid, OFD_ERR_ARG, "no assembled matrix");
-  OFD_CHECK(c.nshl == 8, OFD_ERR_UNSUPPORTED,
-            "the GPU element kernels are implemented for trilinear hexes (NSHL=8)");
+  OFD_CHECK(c.nshl == 8 || c.nshl == 4, OFD_ERR_UNSUPPORTED,
+            "the NS adjoint kernels are implemented for trilinear hexes and linear tets");
   OFD_HIP(hipSetDevice(c.device));
   OFD_HIP(launch_alpha_stage(c));
   OFD_HIP(launch_zero(c, c.d_x, (size_t)c.nnode * 4));
@@ -2791,8 +2791,8 @@ int ofd_time_adjoint(ofd_ctx *x, int what, int reps, double *ms) {
   const bool tem = what >= OFD_TIME_VJP_TEM;
   int rc = tem ? tem_adjoint_ready(c) : ensure_adjoint(c);
   if (rc) return rc;
-  OFD_CHECK(c.nshl == 8, OFD_ERR_UNSUPPORTED,
-            "the GPU element kernels are implemented for trilinear hexes (NSHL=8)");
+  OFD_CHECK(c.nshl == 8 || c.nshl == 4, OFD_ERR_UNSUPPORTED,
+            "the NS adjoint kernels are implemented for trilinear hexes and linear tets");
   OFD_HIP(hipSetDevice(c.device));
   const size_t N = (size_t)c.nnode;
   OFD_HIP(launch_alpha_stage(c));
@@ -2806,6 +2806,8 @@ int ofd_time_adjoint(ofd_ctx *x, int what, int reps, double *ms) {
       OFD_HIP(launch_vjp_ns(c, 1, c.d_lamb));
     } else if (what == OFD_TIME_JT_NS) {
       OFD_HIP(launch_jt_ns(c, 1, c.d_lamb, c.d_jt));
+    } else if (what == OFD_TIME_VJP_TEM && c.nshl == 4) {
+      OFD_HIP(launch_vjp_tem_dual(c, c.d_lamTb, c.d_vjpT));
     } else if (what == OFD_TIME_VJP_TEM) {
       OFD_HIP(launch_zero(c, c.d_vjpT, N * 6));
       OFD_HIP(launch_vjp_tem(c, c.d_lamTb, c.d_vjpT));
@@ -2837,8 +2839,8 @@ int ofd_j_apply(ofd_ctx *x, int subdomain_flag, const double *vu, const double *
   Ctx &c = x->c;
   int rc = ensure_adjoint(c);
   if (rc) return rc;
-  OFD_CHECK(c.nshl == 8, OFD_ERR_UNSUPPORTED,
-            "the GPU element kernels are implemented for trilinear hexes (NSHL=8)");
+  OFD_CHECK(c.nshl == 8 || c.nshl == 4, OFD_ERR_UNSUPPORTED,
+            "the NS adjoint kernels are implemented for trilinear hexes and linear tets");
   OFD_HIP(hipSetDevice(c.device));
   rc = put_node4(c, vu, vp, c.d_lamb);
   if (rc) return rc;
