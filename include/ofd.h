@@ -380,7 +380,7 @@ int ofd_time_read(ofd_ctx *ctx, int64_t nbytes, int reps, double *ms);
  * J = -dF/d(d, dp); J_bc is J with the Dirichlet rows and columns of IBC
  * replaced as BCLhs_3D does (identity per element sharing the node).  J
  * differs from the assembled LHS (SUPG/PSPG blocks, Picard convection).
- * Trilinear hexes and linear tets. */
+ * Trilinear hexes, linear tets and linear prisms. */
 
 /* Transposed product of the assembled LHS: y = LHS^T x, (N,3)+(N) vectors. */
 int ofd_spmv_up_t(ofd_ctx *ctx, int precision, const double *xu, const double *xp,
@@ -451,8 +451,8 @@ int ofd_solve_newton(ofd_ctx *ctx, int subdomain_flag, int precision, int ortho,
  * C_DMDOT = 1: the LHS leaves out the derivative of the discontinuity-
  * capturing conductivity and scales that of the phase-change source by
  * C_DMDOT.  Vectors are (NNODE).  Trilinear hexes take the hand-written
- * kernel, linear tets the dual-number kernel of ofd_vjp_tem_dual (the same
- * residual differentiated by forward mode). */
+ * kernel, linear tets and prisms the dual-number kernel of ofd_vjp_tem_dual
+ * (the same residual differentiated by forward mode). */
 
 /* y = LHSTem^T x. */
 int ofd_spmv_tem_t(ofd_ctx *ctx, int precision, const double *x, double *y);
@@ -489,8 +489,8 @@ int ofd_solve_tem_adjoint(ofd_ctx *ctx, int precision, int ortho, double rtol,
  * increments (acg += d, ug += gami*Delt*d, pg += alfi*gami*Delt*dp, rphig +=
  * dphi, phig += gami*Delt*dphi; solveFlow.f90:99-104) give J = -dR/d(d, dp,
  * dphi).  Derivatives come from forward-mode dual numbers through the whole
- * element residual (no hand differentiation); trilinear hexes and linear
- * tets. */
+ * element residual (no hand differentiation); trilinear hexes, linear
+ * tets and linear prisms. */
 
 /* Replace the stored NS + VOF matrix (the layout ofd_get_nsvof_system
  * reads) by J_bc: J with the rows and columns of constrained dofs cleared

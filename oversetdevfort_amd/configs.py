@@ -269,14 +269,30 @@ _HEX_SGN = np.array([[-1, 1, -1], [1, 1, -1], [1, 1, 1], [-1, 1, 1],
 def lumped_volumes(mesh, elements=None):
     """Row sums of the mass matrix, sum_e int N_a dV, over all elements or
     the boolean selection ``elements``: hexes by the 2x2x2 Gauss rule, linear
-    tets exactly (a quarter of the element's volume at each node)."""
+    prisms by the 6-point rule (linprishl's shape functions), linear tets
+    exactly (a quarter of the element's volume at each node)."""
     g = 1.0 / np.sqrt(3.0)
     ien = mesh.ien if elements is None else mesh.ien[elements]
-    X = mesh.xg[ien]                                   # (E, 8, 3) or (E, 4, 3)
+    X = mesh.xg[ien]                                   # (E, nshl, 3)
     vol = np.zeros(mesh.nnode)
     if ien.shape[1] == 4:
         d = np.abs(np.linalg.det(X[:, :3] - X[:, 3:4])) / 6.0
         np.add.at(vol, ien.ravel(), np.repeat(0.25 * d, 4))
+        return vol
+    if ien.shape[1] == 6:
+        for q in range(6):
+            zi = 2.0 / 3.0 if q % 3 == 0 else 1.0 / 6.0
+            eta = 2.0 / 3.0 if q % 3 == 1 else 1.0 / 6.0
+            zeta = -g if q < 3 else g
+            tri = np.array([zi, eta, 1.0 - zi - eta])
+            dtri = np.array([[1.0, 0.0], [0.0, 1.0], [-1.0, -1.0]])
+            lo, hi = 0.5 * (1.0 - zeta), 0.5 * (1.0 + zeta)
+            N = np.concatenate([tri * lo, tri * hi])
+            dN = np.concatenate([np.column_stack([dtri * lo, -0.5 * tri]),
+                                 np.column_stack([dtri * hi, 0.5 * tri])])   # (6, 3)
+            J = np.einsum("eai,aj->eij", X, dN)
+            dv = np.abs(np.linalg.det(J)) / 6.0
+            np.add.at(vol, ien.ravel(), (dv[:, None] * N[None, :]).ravel())
         return vol
     for q in range(8):
         xi = np.array([g if q & 1 else -g, g if q & 2 else -g, g if q & 4 else -g])
@@ -327,7 +343,7 @@ def inflow_bump(x, lo=(0.3, 0.1), hi=(0.7, 0.4)):
     return prof, on
 
 
-def quench_inflow(dims=(16, 16, 16), seed=5, amp=0.05, diffuse=0.0, rate=None, tets=False):
+def quench_inflow(dims=(16, 16, 16), seed=5, amp=0.05, diffuse=0.0, rate=None, elem="hex"):
     """The quench tank with an inlet: cold liquid (phi = 0, 300 K) enters
     through a patch of the wall x = 0 with the normal velocity amp x
     ``inflow_bump`` and leaves through the open top; the inlet nodes hold
@@ -338,11 +354,14 @@ def quench_inflow(dims=(16, 16, 16), seed=5, amp=0.05, diffuse=0.0, rate=None, t
     ``diffuse`` > 0 replaces the sharp VOF field by the diffuse interface
     0.5 + 0.45 tanh((z - 0.6) / diffuse) (no bubble; every value inside (0, 1)
     away from the clamps); ``rate`` sets both phase-change coefficients
-    (c_evap, c_cond; 0.1 in the tank's material); ``tets`` splits each hex
-    into 6 linear tets (the node fields and conditions are unchanged)."""
+    (c_evap, c_cond; 0.1 in the tank's material); ``elem`` "tet" splits each
+    hex into 6 linear tets, "prism" into 2 linear prisms (the node fields and
+    conditions are unchanged)."""
     prob = quench(dims=dims, seed=seed)
-    if tets:
+    if elem == "tet":
         prob.mesh = M.hex_to_tets(prob.mesh)
+    elif elem == "prism":
+        prob.mesh = M.hex_to_prisms(prob.mesh)
     m = prob.mesh
     N = m.nnode
     if diffuse > 0.0:

@@ -37,8 +37,9 @@
 // Layout: one 64-lane wavefront per element, lane = (Gauss point q, node
 // a), 8-lane butterflies for the Gauss-point sums, a butterfly over q for
 // the node sums, and a coloured read-modify-write scatter.  Trilinear hexes
-// fill the 8 x 8 lanes; on linear tets the lanes a >= 4 carry zero shape
-// functions and the lanes q >= 4 zero weight, so they add nothing.
+// fill the 8 x 8 lanes; on linear tets (prisms) the lanes a >= 4 (6) carry
+// zero shape functions and the lanes q >= 4 (6) zero weight, so they add
+// nothing.
 #include "element_shape.h"
 #include "ofd_internal.h"
 #include "wave_dpp.h"
@@ -476,7 +477,9 @@ static hipError_t vjp_colours(Ctx &c, int subdomain_flag, const double *lam, dou
     const int first = c.color_off[col];
     const int count = c.color_off[col + 1] - first;
     if (count <= 0) continue;
-    hipLaunchKernelGGL((c.nshl == 4 ? k_vjp_ns<4, MODE> : k_vjp_ns<8, MODE>),
+    hipLaunchKernelGGL((c.nshl == 4   ? k_vjp_ns<4, MODE>
+                        : c.nshl == 6 ? k_vjp_ns<6, MODE>
+                                      : k_vjp_ns<8, MODE>),
                        dim3((count + kWaves - 1) / kWaves),
                        dim3(64 * kWaves), 0, c.stream, c.nnode, c.nelem, c.d_ien, c.d_elm_id,
                        c.d_color_elems, first, count, c.d_noderec, mus, c.d_ibc, P,

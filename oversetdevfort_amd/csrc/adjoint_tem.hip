@@ -334,9 +334,9 @@ hipError_t launch_vjp_tem(Ctx &c, const double *lam, double *out) {
   return vjp_tem_colours<0>(c, lam, out);
 }
 
-// tets: the dual-number operator (d_vjpT, [7N], as its scratch)
+// tets and prisms: the dual-number operator (d_vjpT, [7N], as its scratch)
 hipError_t launch_jt_tem(Ctx &c, const double *lam, double *y) {
-  if (c.nshl == 4) return launch_jt_tem_dual(c, lam, y, c.d_vjpT);
+  if (c.nshl != 8) return launch_jt_tem_dual(c, lam, y, c.d_vjpT);
   const hipError_t e = launch_zero(c, y, (size_t)c.temT_n4 * 4);
   if (e != hipSuccess) return e;
   return vjp_tem_colours<1>(c, lam, y);

@@ -216,8 +216,8 @@ k_assemble_nsvof_el(int E, const int32_t *__restrict__ ien, const int32_t *__res
     for (int j = 0; j < 3; ++j) cf = cf + uadv[i] * G[i + 3 * j] * uadv[j];
   cf = P.delt * sqrt(cf);
   const double sc = uadv[0] * ga[0] + uadv[1] * ga[1] + uadv[2] * ga[2];
-  const double dw = detj * gwq;  // hex weights are 1; padded tet points 0
-  if (valid && a == 0 && (NSHL == 8 || q < 4) && detj < 0.0) atomicAdd(negdet, 1);
+  const double dw = detj * gwq;  // hex weights are 1; padded tet and prism points 0
+  if (valid && a == 0 && q < NSHL && detj < 0.0) atomicAdd(negdet, 1);
 
   s_sh[w][q][a][0] = Na;
   s_sh[w][q][a][1] = ga[0];
@@ -293,7 +293,7 @@ k_assemble_nsvof_el(int E, const int32_t *__restrict__ ien, const int32_t *__res
   if (do_mat && fluid) {
     const double f1 = P.almi, f2 = P.alfi * P.delt * P.gami;
     const double cr = 1 / P.rhoa - 1 / P.rhow;
-    for (int qq = 0; qq < (NSHL == 8 ? 8 : 4); ++qq) {
+    for (int qq = 0; qq < NSHL; ++qq) {
       const double Nq = s_sh[w][qq][pa_][0];
       const double ga0 = s_sh[w][qq][pa_][1], ga1 = s_sh[w][qq][pa_][2],
                    ga2 = s_sh[w][qq][pa_][3], sa = s_sh[w][qq][pa_][4];
@@ -424,7 +424,9 @@ hipError_t launch_assemble_nsvof_hex(Ctx &c, int tensor_flag, int color) {
   P.lsc_kdc = q[OFD_Q_LSCKDC];
   P.fluid_id = (int)q[OFD_Q_FLUID_ID];
   const int blocks = (count + kWaves - 1) / kWaves;
-  hipLaunchKernelGGL((c.nshl == 4 ? k_assemble_nsvof_el<4> : k_assemble_nsvof_el<8>),
+  hipLaunchKernelGGL((c.nshl == 4   ? k_assemble_nsvof_el<4>
+                      : c.nshl == 6 ? k_assemble_nsvof_el<6>
+                                    : k_assemble_nsvof_el<8>),
                      dim3(blocks), dim3(64 * kWaves), 0, c.stream,
                      c.nelem, c.d_ien, c.d_elm_id, c.d_color_elems, first, count, c.d_emap,
                      c.d_first, c.d_noderec, c.d_ibc, P, (tensor_flag & OFD_ASSEMBLE_MAT) ? 1 : 0,

@@ -51,9 +51,11 @@ __device__ __forceinline__ double prop_interp(double a, double b, double phi) {
 template <int S>
 struct TemPts {
   static constexpr int Q = 18, R = 18 * S, N = 18 * S + S;
-  static constexpr int stride = S == 8 ? 160 : 80;
+  static constexpr int stride = S == 8 ? 160 : S == 6 ? 120 : 80;
 };
 static_assert(TemPts<8>::N + 5 <= TemPts<8>::stride && TemPts<8>::stride <= 192, "hex layout");
+static_assert(TemPts<6>::N + 4 <= TemPts<6>::stride && TemPts<6>::stride <= 128,
+              "prism layout");
 static_assert(TemPts<4>::N + 3 <= TemPts<4>::stride && TemPts<4>::stride <= 96, "tet layout");
 
 }  // namespace
@@ -72,15 +74,16 @@ k_tem_points(int E, const int32_t *__restrict__ ien, const int32_t *__restrict__
              const double *__restrict__ ks, TemParams P, int do_mat, int do_vec,
              double *__restrict__ pts, int32_t *__restrict__ negdet) {
   using L = TemPts<S>;
+  constexpr int PL = point_lanes<S>();  // lanes per element (prisms: 6 points of 8)
   __shared__ __align__(16) double s_rec[64 * kTRec];
   const int lane = threadIdx.x;
-  const int el = lane / S;  // 64 / S elements per wavefront
-  const int q = lane % S;   // Gauss point, and node a = q for the loads
-  const int e = blockIdx.x * (64 / S) + el;
+  const int el = lane / PL;  // 64 / PL elements per wavefront
+  const int q = lane % PL;   // Gauss point, and node a = q for the loads
+  const int e = blockIdx.x * (64 / PL) + el;
   const bool valid = e < E;
   const bool fluid = valid && elm_id[e] == P.fluid_id;
   double *po = pts + (size_t)(valid ? cpos[e] : 0) * L::stride;
-  {
+  if (q < S) {
     // node record: x+d (0-2), u (3-5), ac (6-8), um (9-11), p, phi, T, rT (12-15)
     const int n = valid ? ien[(size_t)q * E + e] : 0;
     const double2 *r2 = reinterpret_cast<const double2 *>(rec + (size_t)n * 16);
@@ -252,8 +255,8 @@ k_tem_points(int E, const int32_t *__restrict__ ien, const int32_t *__restrict__
   const double dSedT = (P.cpw - P.cpa) * mdot + ((P.cpw - P.cpa) * (Ti - P.Ts) - P.lh) * dmdT;
   const double hk = fluid ? hki + kdc : hki;
   const double dw = detj * gwq;  // hexes: unit weights
-  if (valid && detj < 0.0) atomicAdd(negdet, 1);
-  if (valid && do_mat) {
+  if (valid && q < S && detj < 0.0) atomicAdd(negdet, 1);
+  if (valid && do_mat && q < S) {
     double2 *pq = reinterpret_cast<double2 *>(po + q * L::Q);
     pq[0] = make_double2(Ji[0], Ji[1]);
     pq[1] = make_double2(Ji[2], Ji[3]);
@@ -266,8 +269,8 @@ k_tem_points(int E, const int32_t *__restrict__ ien, const int32_t *__restrict__
     pq[8] = make_double2(dw, 0.0);
   }
 
-  // RHS of node a at this point (:841-855), summed over the S points in the
-  // butterfly association; lane q keeps node a = q
+  // RHS of node a at this point (:841-855), summed over the element's point
+  // lanes in the butterfly association; lane q keeps node a = q
   if (do_vec) {
     double own = 0.0;
 #pragma unroll
@@ -290,13 +293,14 @@ k_tem_points(int E, const int32_t *__restrict__ ien, const int32_t *__restrict__
       const double sm = point_sum<S>(rt);
       if (q == a) own = sm;
     }
-    if (valid) po[L::R + q] = own;
+    if (valid && q < S) po[L::R + q] = own;
   }
 }
 
 // One colour's elements at positions first .. first + count - 1 of the
-// colour order, 64 / S^2 per wavefront (one hex, four tets); the point
-// data, the scatter map and the first-touch masks are indexed by position.
+// colour order, 64 / S^2 per wavefront (one hex or prism, four tets); the
+// point data, the scatter map and the first-touch masks are indexed by
+// position.
 template <int S>
 __global__ void __launch_bounds__(256)
 k_tem_pairs(int first, int count, const int32_t *__restrict__ emap_c,
@@ -413,8 +417,8 @@ k_tem_rhs_gather(int N, const int32_t *__restrict__ inc_off, const int32_t *__re
 
 // The volume loop of assembleQuenching's temperature field: the point
 // kernel, the pair kernel per colour (MAT), the RHS gather (VEC; the caller
-// zeroes the padded RHS).  c.d_pts holds at least the hex / tet stride of
-// the NS assembly's point data per element.
+// zeroes the padded RHS).  c.d_pts holds pts_stride(nshl) doubles per
+// element, at least TemPts' stride.
 hipError_t launch_assemble_tem(Ctx &c, int tensor_flag) {
   const double *q = c.qprm;
   TemParams P;
@@ -441,11 +445,11 @@ hipError_t launch_assemble_tem(Ctx &c, int tensor_flag) {
   P.fluid_id = (int)q[OFD_Q_FLUID_ID];
   const int do_mat = (tensor_flag & OFD_ASSEMBLE_MAT) ? 1 : 0;
   const int do_vec = (tensor_flag & OFD_ASSEMBLE_VEC) ? 1 : 0;
-  const bool hex = c.nshl == 8;
-  const int epw = 64 / c.nshl;                            // point kernel: per wavefront
-  const int epb = 64 / (c.nshl * c.nshl) * kWaves;        // pair kernel: per block
+  const int S = c.nshl;
+  const int epw = 64 / (S == 4 ? 4 : 8);                  // point kernel: per wavefront
+  const int epb = 64 / (S * S) * kWaves;                  // pair kernel: per block
   const double *ks = c.ks_on ? c.d_field[OFD_KSCALE] : nullptr;
-  hipLaunchKernelGGL((hex ? k_tem_points<8> : k_tem_points<4>),
+  hipLaunchKernelGGL((S == 8 ? k_tem_points<8> : S == 6 ? k_tem_points<6> : k_tem_points<4>),
                      dim3((c.nelem + epw - 1) / epw), dim3(64), 0, c.stream, c.nelem, c.d_ien,
                      c.d_elm_id, c.d_cpos, c.d_noderec, ks, P, do_mat, do_vec, c.d_pts,
                      c.d_negdet);
@@ -457,7 +461,7 @@ hipError_t launch_assemble_tem(Ctx &c, int tensor_flag) {
       const int first = c.color_off[col];
       const int count = c.color_off[col + 1] - first;
       if (count <= 0) continue;
-      hipLaunchKernelGGL((hex ? k_tem_pairs<8> : k_tem_pairs<4>),
+      hipLaunchKernelGGL((S == 8 ? k_tem_pairs<8> : S == 6 ? k_tem_pairs<6> : k_tem_pairs<4>),
                          dim3((count + epb - 1) / epb), dim3(64 * kWaves), 0, c.stream, first,
                          count, c.d_emap_c, c.d_first_c, c.d_pts, c.d_ibc, f1, f2, c.d_AT);
       if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -466,8 +470,10 @@ hipError_t launch_assemble_tem(Ctx &c, int tensor_flag) {
   if (do_vec) {
     hipLaunchKernelGGL(k_tem_rhs_gather, dim3((c.nnode + 255) / 256), dim3(256), 0, c.stream,
                        c.nnode, c.d_inc_off, c.d_inc, c.d_pts,
-                       hex ? TemPts<8>::stride : TemPts<4>::stride,
-                       hex ? TemPts<8>::R : TemPts<4>::R, c.d_ibc, c.d_rhsT);
+                       S == 8 ? TemPts<8>::stride : S == 6 ? TemPts<6>::stride
+                                                         : TemPts<4>::stride,
+                       S == 8 ? TemPts<8>::R : S == 6 ? TemPts<6>::R : TemPts<4>::R, c.d_ibc,
+                       c.d_rhsT);
     e = hipGetLastError();
   }
   return e;

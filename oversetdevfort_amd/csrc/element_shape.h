@@ -77,27 +77,35 @@ __device__ __forceinline__ void elem_shape(int q, int a, double &Na, double &dl0
 // whose lane is (element, Gauss point) ----
 
 // the sum of S products in the lane-butterfly association (S = 8:
-// ((p0+p1)+(p2+p3))+((p4+p5)+(p6+p7)); S = 4: (p0+p1)+(p2+p3)); the adds
-// carry no contraction, so no product is fused into them
+// ((p0+p1)+(p2+p3))+((p4+p5)+(p6+p7)); S = 6: ((p0+p1)+(p2+p3))+(p4+p5);
+// S = 4: (p0+p1)+(p2+p3)); the adds carry no contraction, so no product is
+// fused into them
 template <int S>
 __device__ __forceinline__ double tree_sum(const double (&p)[S]) {
 #pragma clang fp contract(off)
   if constexpr (S == 8)
     return ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7]));
+  else if constexpr (S == 6)
+    return ((p[0] + p[1]) + (p[2] + p[3])) + (p[4] + p[5]);
   else
     return (p[0] + p[1]) + (p[2] + p[3]);
 }
 
-// sum and maximum over the S lanes of an element's Gauss points
+// lanes per element in the kernels whose lane is (element, Gauss point):
+// the S points, prisms padded to 8 (their lanes 6, 7 carry zero weight)
+template <int S>
+constexpr int point_lanes() { return S == 4 ? 4 : 8; }
+
+// sum and maximum over the lanes of an element's Gauss points
 template <int S>
 __device__ __forceinline__ double point_sum(double v) {
-  if constexpr (S == 8) return wsum8(v);
-  else return wsum4(v);
+  if constexpr (S == 4) return wsum4(v);
+  else return wsum8(v);
 }
 template <int S>
 __device__ __forceinline__ double point_max(double v) {
-  if constexpr (S == 8) return wmax8(v);
-  else return wmax4(v);
+  if constexpr (S == 4) return wmax4(v);
+  else return wmax8(v);
 }
 
 // the shape value of node a at Gauss point q (elem_shape<S>'s Na)

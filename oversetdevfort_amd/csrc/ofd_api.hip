@@ -122,10 +122,10 @@ static void free_tem_adjoint(Ctx &c) {
   c.tadj_n4 = 0;
 }
 
-// the NS assembly's per-element point data (1536 bytes per hex, 768 per
-// tet), allocated at the first assembly of a hex or tet mesh
+// the element assemblies' per-element point data (1536 bytes per hex, 1024
+// per prism, 768 per tet), allocated at the first assembly
 static int ensure_pts(Ctx &c) {
-  if (c.nshl != 8 && c.nshl != 4) return OFD_OK;
+  if (c.nshl != 8 && c.nshl != 4 && c.nshl != 6) return OFD_OK;
   const size_t len = (size_t)c.nelem * pts_stride(c.nshl);
   if (c.d_pts && c.pts_len == len) return OFD_OK;
   int rc = dalloc(c, &c.d_pts, len);
@@ -860,7 +860,7 @@ int ofd_build_pattern(ofd_ctx *x, int64_t *icnt_out) {
     OFD_HIP(hipMemsetAsync(d_touched, 0, (size_t)c.icnt, c.stream));
     OFD_HIP(launch_first_touch(c, d_touched));
   }
-  if (c.nshl == 8 || c.nshl == 4) {
+  if (c.nshl == 8 || c.nshl == 4 || c.nshl == 6) {
     rc = dalloc(c, &c.d_emap_c, (size_t)E * 64);
     if (!rc) rc = dalloc(c, &c.d_first_c, (size_t)E);
     if (!rc) rc = dalloc(c, &c.d_cpos, (size_t)E);
@@ -1514,8 +1514,9 @@ int ofd_time_quench(ofd_ctx *x, int what, int arg, int reps, double *ms) {
   OFD_HIP(hipSetDevice(c.device));
   if (!spmv) {
     OFD_CHECK(c.quench, OFD_ERR_ARG, "no quench parameters (ofd_set_quench_params)");
-    OFD_CHECK(c.nshl == 8 || c.nshl == 4, OFD_ERR_UNSUPPORTED,
-              "the GPU element assembly is implemented for trilinear hexes and linear tets");
+    OFD_CHECK(c.nshl == 8 || c.nshl == 4 || c.nshl == 6, OFD_ERR_UNSUPPORTED,
+              "the GPU element assembly is implemented for trilinear hexes, linear tets and "
+              "prisms");
     OFD_CHECK((arg & ~(OFD_ASSEMBLE_VEC | OFD_ASSEMBLE_MAT)) == 0, OFD_ERR_ARG,
               "arg must combine OFD_ASSEMBLE_VEC / OFD_ASSEMBLE_MAT");
     rc = ensure_pts(c);
@@ -1634,8 +1635,9 @@ int ofd_assemble_tem(ofd_ctx *x, int tensor_flag) {
   int rc = ensure_tem(c);
   if (rc) return rc;
   OFD_CHECK(c.quench, OFD_ERR_ARG, "no quench parameters (ofd_set_quench_params)");
-  OFD_CHECK(c.nshl == 8 || c.nshl == 4, OFD_ERR_UNSUPPORTED,
-            "the GPU element assembly is implemented for trilinear hexes and linear tets");
+  OFD_CHECK(c.nshl == 8 || c.nshl == 4 || c.nshl == 6, OFD_ERR_UNSUPPORTED,
+            "the GPU element assembly is implemented for trilinear hexes, linear tets and "
+            "prisms");
   OFD_CHECK((tensor_flag & ~(OFD_ASSEMBLE_VEC | OFD_ASSEMBLE_MAT)) == 0, OFD_ERR_ARG,
             "tensor_flag must combine OFD_ASSEMBLE_VEC / OFD_ASSEMBLE_MAT");
   OFD_HIP(hipSetDevice(c.device));
@@ -1868,8 +1870,9 @@ int ofd_assemble_nsvof(ofd_ctx *x, int tensor_flag, double *cfl) {
   int rc = ensure_nsvof(c);
   if (rc) return rc;
   OFD_CHECK(c.quench, OFD_ERR_ARG, "no quench parameters (ofd_set_quench_params)");
-  OFD_CHECK(c.nshl == 8 || c.nshl == 4, OFD_ERR_UNSUPPORTED,
-            "the GPU element assembly is implemented for trilinear hexes and linear tets");
+  OFD_CHECK(c.nshl == 8 || c.nshl == 4 || c.nshl == 6, OFD_ERR_UNSUPPORTED,
+            "the GPU element assembly is implemented for trilinear hexes, linear tets and "
+            "prisms");
   OFD_CHECK((tensor_flag & ~(OFD_ASSEMBLE_VEC | OFD_ASSEMBLE_MAT)) == 0, OFD_ERR_ARG,
             "tensor_flag must combine OFD_ASSEMBLE_VEC / OFD_ASSEMBLE_MAT");
   OFD_HIP(hipSetDevice(c.device));
@@ -2371,8 +2374,8 @@ int ofd_vjp_ns(ofd_ctx *x, int subdomain_flag, const double *lu, const double *l
   Ctx &c = x->c;
   int rc = ensure_adjoint(c);
   if (rc) return rc;
-  OFD_CHECK(c.nshl == 8 || c.nshl == 4, OFD_ERR_UNSUPPORTED,
-            "the NS adjoint kernels are implemented for trilinear hexes and linear tets");
+  OFD_CHECK(c.nshl == 8 || c.nshl == 4 || c.nshl == 6, OFD_ERR_UNSUPPORTED,
+            "the NS adjoint kernels are implemented for trilinear hexes, linear tets and prisms");
   OFD_HIP(hipSetDevice(c.device));
   rc = put_node4(c, lu, lp, c.d_lamb);
   if (rc) return rc;
@@ -2394,8 +2397,8 @@ int ofd_jt_apply(ofd_ctx *x, int subdomain_flag, const double *lu, const double 
   Ctx &c = x->c;
   int rc = ensure_adjoint(c);
   if (rc) return rc;
-  OFD_CHECK(c.nshl == 8 || c.nshl == 4, OFD_ERR_UNSUPPORTED,
-            "the NS adjoint kernels are implemented for trilinear hexes and linear tets");
+  OFD_CHECK(c.nshl == 8 || c.nshl == 4 || c.nshl == 6, OFD_ERR_UNSUPPORTED,
+            "the NS adjoint kernels are implemented for trilinear hexes, linear tets and prisms");
   OFD_HIP(hipSetDevice(c.device));
   rc = put_node4(c, lu, lp, c.d_lamb);
   if (rc) return rc;
@@ -2557,8 +2560,8 @@ int ofd_solve_adjoint(ofd_ctx *x, int subdomain_flag, int precision, int ortho, 
   Ctx &c = x->c;
   int rc = ensure_adjoint(c);
   if (rc) return rc;
-  OFD_CHECK(c.nshl == 8 || c.nshl == 4, OFD_ERR_UNSUPPORTED,
-            "the NS adjoint kernels are implemented for trilinear hexes and linear tets");
+  OFD_CHECK(c.nshl == 8 || c.nshl == 4 || c.nshl == 6, OFD_ERR_UNSUPPORTED,
+            "the NS adjoint kernels are implemented for trilinear hexes, linear tets and prisms");
   OFD_HIP(hipSetDevice(c.device));
   rc = ensure_transpose(c);
   if (rc) return rc;
@@ -2587,8 +2590,8 @@ int ofd_solve_newton(ofd_ctx *x, int subdomain_flag, int precision, int ortho, d
   int rc = ensure_adjoint(c);
   if (rc) return rc;
   OFD_CHECK(c.mat_valid, OFD_ERR_ARG, "no assembled matrix");
-  OFD_CHECK(c.nshl == 8 || c.nshl == 4, OFD_ERR_UNSUPPORTED,
-            "the NS adjoint kernels are implemented for trilinear hexes and linear tets");
+  OFD_CHECK(c.nshl == 8 || c.nshl == 4 || c.nshl == 6, OFD_ERR_UNSUPPORTED,
+            "the NS adjoint kernels are implemented for trilinear hexes, linear tets and prisms");
   OFD_HIP(hipSetDevice(c.device));
   OFD_HIP(launch_alpha_stage(c));
   OFD_HIP(launch_zero(c, c.d_x, (size_t)c.nnode * 4));
@@ -2660,8 +2663,8 @@ static int tem_adjoint_ready(Ctx &c) {
   int rc = ensure_tem_adjoint(c);
   if (rc) return rc;
   OFD_CHECK(c.quench, OFD_ERR_ARG, "no quench parameters (ofd_set_quench_params)");
-  OFD_CHECK(c.nshl == 8 || c.nshl == 4, OFD_ERR_UNSUPPORTED,
-            "the temperature adjoint is implemented for trilinear hexes and linear tets");
+  OFD_CHECK(c.nshl == 8 || c.nshl == 4 || c.nshl == 6, OFD_ERR_UNSUPPORTED,
+            "the temperature adjoint is implemented for trilinear hexes, linear tets and prisms");
   return OFD_OK;
 }
 
@@ -2714,9 +2717,10 @@ int ofd_vjp_tem(ofd_ctx *x, const double *lam, double *gT, double *grT, double *
   rc = h2d(c, c.d_lamTb, lam, N * sizeof(double));
   if (rc) return rc;
   OFD_HIP(launch_alpha_stage(c));
-  // rows T, rT, u (3), then the scale (hexes) or phi, scale (tets, dual numbers)
+  // rows T, rT, u (3), then the scale (hexes) or phi, scale (tets and prisms,
+  // dual numbers)
   size_t ks_row = 5;
-  if (c.nshl == 4) {
+  if (c.nshl != 8) {
     OFD_HIP(launch_vjp_tem_dual(c, c.d_lamTb, c.d_vjpT));
     ks_row = 6;
   } else {
@@ -2791,8 +2795,8 @@ int ofd_time_adjoint(ofd_ctx *x, int what, int reps, double *ms) {
   const bool tem = what >= OFD_TIME_VJP_TEM;
   int rc = tem ? tem_adjoint_ready(c) : ensure_adjoint(c);
   if (rc) return rc;
-  OFD_CHECK(c.nshl == 8 || c.nshl == 4, OFD_ERR_UNSUPPORTED,
-            "the NS adjoint kernels are implemented for trilinear hexes and linear tets");
+  OFD_CHECK(c.nshl == 8 || c.nshl == 4 || c.nshl == 6, OFD_ERR_UNSUPPORTED,
+            "the NS adjoint kernels are implemented for trilinear hexes, linear tets and prisms");
   OFD_HIP(hipSetDevice(c.device));
   const size_t N = (size_t)c.nnode;
   OFD_HIP(launch_alpha_stage(c));
@@ -2806,7 +2810,7 @@ int ofd_time_adjoint(ofd_ctx *x, int what, int reps, double *ms) {
       OFD_HIP(launch_vjp_ns(c, 1, c.d_lamb));
     } else if (what == OFD_TIME_JT_NS) {
       OFD_HIP(launch_jt_ns(c, 1, c.d_lamb, c.d_jt));
-    } else if (what == OFD_TIME_VJP_TEM && c.nshl == 4) {
+    } else if (what == OFD_TIME_VJP_TEM && c.nshl != 8) {
       OFD_HIP(launch_vjp_tem_dual(c, c.d_lamTb, c.d_vjpT));
     } else if (what == OFD_TIME_VJP_TEM) {
       OFD_HIP(launch_zero(c, c.d_vjpT, N * 6));
@@ -2839,8 +2843,8 @@ int ofd_j_apply(ofd_ctx *x, int subdomain_flag, const double *vu, const double *
   Ctx &c = x->c;
   int rc = ensure_adjoint(c);
   if (rc) return rc;
-  OFD_CHECK(c.nshl == 8 || c.nshl == 4, OFD_ERR_UNSUPPORTED,
-            "the NS adjoint kernels are implemented for trilinear hexes and linear tets");
+  OFD_CHECK(c.nshl == 8 || c.nshl == 4 || c.nshl == 6, OFD_ERR_UNSUPPORTED,
+            "the NS adjoint kernels are implemented for trilinear hexes, linear tets and prisms");
   OFD_HIP(hipSetDevice(c.device));
   rc = put_node4(c, vu, vp, c.d_lamb);
   if (rc) return rc;
@@ -2857,8 +2861,8 @@ static int nsvof_adjoint_ready(Ctx &c) {
   int rc = ensure_nsvof(c);
   if (rc) return rc;
   OFD_CHECK(c.quench, OFD_ERR_ARG, "no quench parameters (ofd_set_quench_params)");
-  OFD_CHECK(c.nshl == 8 || c.nshl == 4, OFD_ERR_UNSUPPORTED,
-            "the dual-number kernels are implemented for trilinear hexes and linear tets");
+  OFD_CHECK(c.nshl == 8 || c.nshl == 4 || c.nshl == 6, OFD_ERR_UNSUPPORTED,
+            "the dual-number kernels are implemented for trilinear hexes, linear tets and prisms");
   OFD_HIP(hipSetDevice(c.device));
   if (c.vadj_n4 != c.nsvV_n4 || !c.d_vjpV) {
     const size_t np = (size_t)c.nsvV_n4 * 4;

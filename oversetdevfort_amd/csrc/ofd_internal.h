@@ -355,7 +355,7 @@ struct PtsLayout {
   static constexpr int F = 12 * S, R = 18 * S, N = 22 * S;
   static constexpr int stride = S == 8 ? 192 : 96;
 };
-constexpr int pts_stride(int nshl) { return nshl == 8 ? 192 : 96; }
+constexpr int pts_stride(int nshl) { return nshl == 8 ? 192 : nshl == 6 ? 128 : 96; }
 hipError_t launch_read(Ctx &c, const void *src, size_t nbytes, double *out);
 // overset fringe
 hipError_t launch_fringe(Ctx &c, int32_t src_id, int nrecv, const double *xr,

@@ -1,7 +1,7 @@
 // Exact derivatives of the quenching solver's element residuals by forward-
-// mode dual numbers on gfx950 (no reference counterpart), trilinear hexes
-// and linear tets (S = 8 or 4 nodes and Gauss points, the forward kernels'
-// rules).
+// mode dual numbers on gfx950 (no reference counterpart), trilinear hexes,
+// linear prisms and linear tets (S = 8, 6 or 4 nodes and Gauss points, the
+// forward kernels' rules).
 //
 // Residuals differentiated, as the forward kernels assemble them:
 //   NS + VOF  the RHSGu / RHSGp / RHSGls of IntElmAss_NSVOF_Quenching_STAB
@@ -604,7 +604,9 @@ hipError_t launch_nsvof_jac(Ctx &c, int color) {
   const int first = c.color_off[color];
   const int count = c.color_off[color + 1] - first;
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL((c.nshl == 4 ? k_nsvof_dual<4, kDualJac> : k_nsvof_dual<8, kDualJac>),
+  hipLaunchKernelGGL((c.nshl == 4   ? k_nsvof_dual<4, kDualJac>
+                      : c.nshl == 6 ? k_nsvof_dual<6, kDualJac>
+                                    : k_nsvof_dual<8, kDualJac>),
                      dim3((count + kDW - 1) / kDW), dim3(64 * kDW), 0,
                      c.stream, c.nelem, c.d_ien, c.d_elm_id, c.d_color_elems, first, count,
                      c.d_emap, c.d_first, c.d_noderec, c.d_ibc, nsvof_dual_params(c), c.nnode,
@@ -617,7 +619,9 @@ hipError_t launch_nsvof_vjp(Ctx &c, int color, const double *lam4, const double 
   const int first = c.color_off[color];
   const int count = c.color_off[color + 1] - first;
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL((c.nshl == 4 ? k_nsvof_dual<4, kDualVjp> : k_nsvof_dual<8, kDualVjp>),
+  hipLaunchKernelGGL((c.nshl == 4   ? k_nsvof_dual<4, kDualVjp>
+                      : c.nshl == 6 ? k_nsvof_dual<6, kDualVjp>
+                                    : k_nsvof_dual<8, kDualVjp>),
                      dim3((count + kDW - 1) / kDW), dim3(64 * kDW), 0,
                      c.stream, c.nelem, c.d_ien, c.d_elm_id, c.d_color_elems, first, count,
                      c.d_emap, c.d_first, c.d_noderec, c.d_ibc, nsvof_dual_params(c), c.nnode,
@@ -650,7 +654,7 @@ hipError_t launch_tem_dual_vjp(Ctx &c, int color, const double *ks, const double
   P.tem_kdc = q[OFD_Q_TEMKDC];
   P.tref = q[OFD_Q_TREF];
   P.fluid_id = (int)q[OFD_Q_FLUID_ID];
-  hipLaunchKernelGGL((c.nshl == 4 ? k_tem_dual<4> : k_tem_dual<8>),
+  hipLaunchKernelGGL((c.nshl == 4 ? k_tem_dual<4> : c.nshl == 6 ? k_tem_dual<6> : k_tem_dual<8>),
                      dim3((count + kDW - 1) / kDW), dim3(64 * kDW), 0, c.stream,
                      c.nelem, c.d_ien, c.d_elm_id, c.d_color_elems, first, count, c.d_noderec,
                      ks, P, c.nnode, lam, tmask, out);

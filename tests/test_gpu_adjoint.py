@@ -21,7 +21,9 @@
 * The gradient of a Newton-converged objective with respect to the viscosity
   scale and the lid amplitude against central differences of full re-solves.
 * The VJP, J_bc^T, J_bc v, the consistent Newton iteration and the adjoint
-  solve again on linear tets (each hex of the cavity split into 6).
+  solve again on linear tets (each hex of the cavity split into 6), and the
+  fine-scale form's operators and adjoint solve on linear prisms (2 per
+  hex).
 """
 import numpy as np
 import pytest
@@ -45,9 +47,13 @@ def _rel(a, b):
 
 
 def _perturbed_cavity(S, n=6, seed=1, form=0, tet=False):
+    """The cavity with a perturbed state; ``tet`` "tet" or "prism" (or
+    True: tets) splits each hex into 6 linear tets or 2 linear prisms."""
     rng = np.random.default_rng(seed)
     prob = C.cavity(n, seed=seed, jitter=0.05)
-    if tet:
+    if tet == "prism":
+        prob.mesh = M.hex_to_prisms(prob.mesh)
+    elif tet:
         prob.mesh = M.hex_to_tets(prob.mesh)
     N = prob.mesh.nnode
     f = prob.fields
@@ -142,7 +148,7 @@ def _rhs(ctx):
     return _vec(u, p)
 
 
-FORMS = [(0, False), (1, False), (0, True), (1, True)]
+FORMS = [(0, False), (1, False), (0, True), (1, True), (1, "prism")]
 
 
 @pytest.mark.parametrize("form,tet", FORMS)
@@ -173,7 +179,7 @@ def test_vjp_matches_central_differences(S, form, tet):
             err = abs(fd - ad) / max(abs(fd), abs(ad), 1e-300)
             worst = max(worst, err)
         ctx.set_field(name, base)
-        print(f"form {form}{', tets' if tet else ''}: VJP wrt {name}: worst relative FD "
+        print(f"form {form}{'' if not tet else ', prisms' if tet == 'prism' else ', tets'}: VJP wrt {name}: worst relative FD "
               f"error over 16 directions "
               f"{worst:.2e}",
               flush=True)
@@ -212,7 +218,7 @@ def test_jt_apply_matches_central_differences(S, form, tet):
         fd = lam.ravel() @ jv
         ad = y.ravel() @ v.ravel()
         worst = max(worst, abs(fd - ad) / max(abs(fd), abs(ad)))
-    print(f"form {form}{', tets' if tet else ''}: J_bc^T: worst relative FD error over 16 "
+    print(f"form {form}{'' if not tet else ', prisms' if tet == 'prism' else ', tets'}: J_bc^T: worst relative FD error over 16 "
           f"directions {worst:.2e}",
           flush=True)
     assert worst <= 1e-6
@@ -251,7 +257,7 @@ def test_j_apply_tangent(S, form, tet):
         worst_fd = max(worst_fd, _rel(jv.ravel(), fd))
         a, b = lam.ravel() @ jv.ravel(), lt.ravel() @ v.ravel()
         worst_dot = max(worst_dot, abs(a - b) / max(abs(a), abs(b)))
-    print(f"form {form}{', tets' if tet else ''}: J_bc v: worst relative error vs central "
+    print(f"form {form}{'' if not tet else ', prisms' if tet == 'prism' else ', tets'}: J_bc v: worst relative error vs central "
           f"differences {worst_fd:.2e}, "
           f"dot-product test vs J_bc^T {worst_dot:.2e}", flush=True)
     assert worst_fd <= 1e-6
@@ -286,7 +292,7 @@ def test_consistent_newton_converges(S, tet):
     ctx.close()
 
 
-@pytest.mark.parametrize("tet", [False, True])
+@pytest.mark.parametrize("tet", [False, True, "prism"])
 def test_adjoint_solve_and_consistency_gap(S, oracle_mod, tet):
     ctx, prob, rng = _perturbed_cavity(S, n=6, seed=5, form=1, tet=tet)
     N = prob.mesh.nnode

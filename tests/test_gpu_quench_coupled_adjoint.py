@@ -8,7 +8,7 @@ block at 900 K), a diffuse interface and phase-change coefficients 1e-3
 root, which the implicit-function-theorem adjoint needs, is not found).
 
 The tank of 8^3 trilinear hexes and the same tank with each hex split into
-6 linear tets.
+6 linear tets or 2 linear prisms.
 
 Checks, against central differences of re-solved steps (the forward is
 converged to its rounding floor, about 1e-12 of the predictor's residual;
@@ -35,11 +35,11 @@ pytestmark = pytest.mark.gpu
 NAMES = ("ug", "acg", "pg", "phig", "rphig", "Tg", "rTg")
 
 
-@pytest.fixture(scope="module", params=["hex", "tet"])
+@pytest.fixture(scope="module", params=["hex", "tet", "prism"])
 def setup(request):
     from oversetdevfort_amd import solver as S
     prob = C.quench_inflow((8, 8, 8), seed=5, amp=0.05, diffuse=0.08, rate=1e-3,
-                           tets=request.param == "tet")
+                           elem=request.param)
     ctx = S.setup(prob)
     ctx.set_quench_params(prob.material)
     st = A.QuenchStepper(ctx, prob.ibc, prob.params, prob.inflow_profile)

@@ -16,7 +16,7 @@ forward assembly (the restatement of solveQuenching.f90:122-573 and
   (ofd_vjp_tem) and, for the VOF field, against central differences.
 
 States: a 5^3 tank (trilinear hexes, and the same tank with each hex split
-into 6 linear tets) with perturbed fields, the VOF field strictly inside
+into 6 linear tets or 2 linear prisms) with perturbed fields, the VOF field strictly inside
 (0, 1) (the clamps' kinks are away from the differences) and temperatures on
 both sides of T_sat, random Dirichlet nodes.  Differences with steps of 1e-6
 of each field's scale agree to about 1e-7; the tests hold 1e-5.
@@ -37,13 +37,15 @@ def _rel(a, b):
     return np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-300)
 
 
-@pytest.fixture(scope="module", params=["hex", "tet"])
+@pytest.fixture(scope="module", params=["hex", "tet", "prism"])
 def case(request):
     from oversetdevfort_amd import solver as S
     rng = np.random.default_rng(7)
     prob = C.quench(5, seed=3, jitter=0.05)
     if request.param == "tet":
         prob.mesh = M.hex_to_tets(prob.mesh)
+    elif request.param == "prism":
+        prob.mesh = M.hex_to_prisms(prob.mesh)
     m = prob.mesh
     N = m.nnode
     f = prob.fields

@@ -20,9 +20,9 @@ code).
   then the exact Jacobian), nonzero with it.
 * The adjoint solve J_T,bc^T lam = g (flexible GMRES preconditioned by
   LHSTem^T) to a 1e-8 true residual, FP64 and mixed.
-* The VJP, J_T,bc^T and the adjoint solve again on linear tets (each hex
-  of the tank split into 6), where both operators come from the
-  dual-number kernel.
+* The VJP, J_T,bc^T and the adjoint solve again on linear tets and linear
+  prisms (each hex of the tank split into 6 or 2), where both operators
+  come from the dual-number kernel.
 """
 import numpy as np
 import pytest
@@ -52,16 +52,19 @@ UNIT_MATERIAL = dict(rhoa=0.5, rhow=1.0, cpa=0.8, cpw=1.0, kappaa=0.05, kappaw=0
                      cps=0.5, kappas=1.0, lh=20.0, Ts=1.0, tref=1.0)
 
 
-def _case(S, n=5, seed=1, tem_kdc=0.0, frac_ibc=0.15, kscale=True, unit=False, tet=False):
+def _case(S, n=5, seed=1, tem_kdc=0.0, frac_ibc=0.15, kscale=True, unit=False, tet=False,
+          prism=False):
     """A small quench tank with a perturbed state (temperatures on both sides
     of T_sat, VOF values in the interface range, random T Dirichlet nodes) and
     a per-node conductivity scale; ``unit``: the order-one material and a
-    10 ms step; ``tet``: each hex split into 6 linear tets (the operators then
-    come from the dual-number kernel)."""
+    10 ms step; ``tet``, ``prism``: each hex split into 6 linear tets or 2
+    linear prisms (the operators then come from the dual-number kernel)."""
     rng = np.random.default_rng(seed)
     prob = C.quench(n, seed=seed, jitter=0.05)
     if tet:
         prob.mesh = M.hex_to_tets(prob.mesh)
+    if prism:
+        prob.mesh = M.hex_to_prisms(prob.mesh)
     if unit:
         prob.material.update(UNIT_MATERIAL)
         prob.params["delt"] = 1e-2
@@ -149,13 +152,15 @@ def test_tem_transposed_spmv_and_solve(S, oracle_mod):
     ctx.close()
 
 
-@pytest.mark.parametrize("kdc,unit,tet", [(0.0, True, False), (0.3, True, False),
-                                          (0.0, False, False), (0.3, False, False),
-                                          (0.3, True, True), (0.0, False, True)])
-def test_tem_vjp_matches_central_differences(S, kdc, unit, tet):
-    ctx, prob, rng, mat, ks = _case(S, seed=23, tem_kdc=kdc, unit=unit, tet=tet)
+@pytest.mark.parametrize("kdc,unit,elem", [(0.0, True, "hex"), (0.3, True, "hex"),
+                                           (0.0, False, "hex"), (0.3, False, "hex"),
+                                           (0.3, True, "tet"), (0.0, False, "tet"),
+                                           (0.0, True, "prism"), (0.3, False, "prism")])
+def test_tem_vjp_matches_central_differences(S, kdc, unit, elem):
+    ctx, prob, rng, mat, ks = _case(S, seed=23, tem_kdc=kdc, unit=unit, tet=elem == "tet",
+                                    prism=elem == "prism")
     tol = 1e-6 if unit else 5e-5
-    what = ("order-one material" if unit else "quench material") + (", tets" if tet else "")
+    what = ("order-one material" if unit else "quench material") + f", {elem}"
     N = prob.mesh.nnode
     p, f = prob.params, prob.fields
     free = (prob.ibc[:, 5] == 0).astype(np.float64)
@@ -191,13 +196,15 @@ def _increment(ctx, f, v, s, p):
     ctx.set_field("TG", f["Tg"] + p["gami"] * p["delt"] * s * v)
 
 
-@pytest.mark.parametrize("kdc,unit,tet", [(0.0, True, False), (0.3, True, False),
-                                          (0.0, False, False), (0.3, False, False),
-                                          (0.0, True, True), (0.3, False, True)])
-def test_tem_jt_apply_and_consistency_gap(S, kdc, unit, tet):
-    ctx, prob, rng, mat, ks = _case(S, seed=24, tem_kdc=kdc, unit=unit, tet=tet)
+@pytest.mark.parametrize("kdc,unit,elem", [(0.0, True, "hex"), (0.3, True, "hex"),
+                                           (0.0, False, "hex"), (0.3, False, "hex"),
+                                           (0.0, True, "tet"), (0.3, False, "tet"),
+                                           (0.0, True, "prism")])
+def test_tem_jt_apply_and_consistency_gap(S, kdc, unit, elem):
+    ctx, prob, rng, mat, ks = _case(S, seed=24, tem_kdc=kdc, unit=unit, tet=elem == "tet",
+                                    prism=elem == "prism")
     tol = 1e-6 if unit else 5e-5
-    what = ("order-one material" if unit else "quench material") + (", tets" if tet else "")
+    what = ("order-one material" if unit else "quench material") + f", {elem}"
     N = prob.mesh.nnode
     p, f = prob.params, prob.fields
     free = (prob.ibc[:, 5] == 0).astype(np.float64)
@@ -234,9 +241,10 @@ def test_tem_jt_apply_and_consistency_gap(S, kdc, unit, tet):
     ctx.close()
 
 
-@pytest.mark.parametrize("tet", [False, True])
-def test_tem_adjoint_solve(S, tet):
-    ctx, prob, rng, mat, ks = _case(S, n=6, seed=25, tem_kdc=0.3, tet=tet)
+@pytest.mark.parametrize("elem", ["hex", "tet", "prism"])
+def test_tem_adjoint_solve(S, elem):
+    ctx, prob, rng, mat, ks = _case(S, n=6, seed=25, tem_kdc=0.3, tet=elem == "tet",
+                                    prism=elem == "prism")
     N = prob.mesh.nnode
     ctx.assemble_tem(6)
     g = rng.normal(size=N) * (prob.ibc[:, 5] == 0)
