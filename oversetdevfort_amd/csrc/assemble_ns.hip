@@ -469,6 +469,11 @@ k_assemble_ns_el(int N, int E, const int32_t *__restrict__ ien,
 //                   lane = (element, a, b) the 16 entries of block (a, b) over
 //                   the S points (e3LHS_3D_fluid_Old), the Dirichlet masks and
 //                   the coalesced first-touch scatter of the kernel above.
+//   k_ns_pairs_cl   hexes: one launch per cluster colour, one workgroup per
+//                   cluster of up to 8 elements around a node, one
+//                   wavefront per element forming its blocks as
+//                   k_ns_pairs<8> does; the blocks are summed in LDS, one
+//                   slot per distinct nonzero, and each slot written once.
 //   k_ns_rhs_gather one launch, thread per node: the node's contributions in
 //                   colour order (the order of the coloured scatter, so the
 //                   same bits), masked by its Dirichlet flags.
@@ -738,55 +743,34 @@ k_ns_points(int E, const int32_t *__restrict__ ien, const int32_t *__restrict__ 
   if (valid && q == 0) cfl_out[e] = cflq;
 }
 
-// One colour's elements at positions first .. first + count - 1 of the
-// colour order, 64 / S^2 per wavefront (one hex, four tets).  Everything it
-// reads is indexed by position: the point data, the scatter map and the
-// first-touch masks.  Every global load is issued before the accumulation
-// (the old values of the nonzeros it adds into included), so only the
-// point data's latency is in front of it.
+// The per-element part of the pair kernels.  ns_pair_stage<S> puts a
+// wavefront's elements' Dirichlet flags, subdomain flags, point factors, the
+// N table and (lane = (slot, q, a)) dN_a/dx and tmp1_a into the wavefront's
+// LDS area sw: [slot][q][a] (dN_a/dx, dN_a/dy), (dN_a/dz, tmp1_a), the
+// factors from FO, the N table from NO.  ns_pair_acc<S> then forms, for
+// lane = (slot, a, b), the 16 entries of block (a, b) over the S points
+// (e3LHS_3D_fluid_Old) with the Dirichlet projection.
 template <int S>
-__global__ void __launch_bounds__(256)
-k_ns_pairs(int first, int count, const int32_t *__restrict__ emap_c,
-           const uint64_t *__restrict__ first_c, const double *__restrict__ pts,
-           const uint8_t *__restrict__ ibc, double rho, double *__restrict__ A) {
+struct PairLds {
+  static constexpr int SS = S * S, EW = 64 / SS;
+  static constexpr int FO = 256, NO = FO + EW * S * 6, size = NO + SS;
+};
+
+template <int S>
+__device__ __forceinline__ void ns_pair_stage(double *sw, uint32_t *sm, int32_t *stg, int t0,
+                                              int nv, const double *__restrict__ pts,
+                                              const uint8_t *__restrict__ ibc, double rho,
+                                              int lane) {
   using L = PtsLayout<S>;
-  constexpr int SS = S * S;
-  constexpr int EW = 64 / SS;       // elements per wavefront
-  constexpr int FO = 256;           // factors [slot][q] (6 doubles) after the gradients
-  constexpr int NO = FO + EW * S * 6;  // N_a at point q: [q][a]
-  // per wavefront: [slot][q][a] (dN_a/dx, dN_a/dy), (dN_a/dz, tmp1_a), the
-  // factors and the N table during the accumulation; then the block
-  // transpose [64][18]
-  __shared__ __align__(16) double s_w[kWavesPerBlock][64 * 18];
-  __shared__ uint32_t s_m[kWavesPerBlock][16];  // [slot][a] Dirichlet flags
-  __shared__ int32_t s_tg[kWavesPerBlock][4];   // [slot] in the selected subdomains
-  const int lane = threadIdx.x & 63;
-  const int w = threadIdx.x >> 6;
-  const int t0 = first + (blockIdx.x * kWavesPerBlock + w) * EW;
-  if (t0 >= first + count) return;  // whole wavefronts; no block barrier below
-  const int nv = (first + count - t0 < EW) ? first + count - t0 : EW;  // live slots
-  const int piece = lane & 7;
-  // the scatter plan: round rnd moves block jj = rnd * 8 + lane / 8, i.e.
-  // pair (aj, bj) of slot jj / S^2
-  int kk[8];
-  bool fst[8];
-#pragma unroll
-  for (int rnd = 0; rnd < 8; ++rnd) {
-    const int jj = rnd * 8 + (lane >> 3);
-    const int js = jj / SS, rj = jj % SS;
-    const int aj = rj % S, bj = rj / S;
-    const int tj = t0 + (js < nv ? js : 0);
-    kk[rnd] = emap_c[(size_t)tj * 64 + aj + 8 * bj];
-    fst[rnd] = (first_c[tj] >> (aj + 8 * bj)) & 1ull;
-  }
-  double *sw = s_w[w];
+  using W = PairLds<S>;
+  constexpr int SS = W::SS, EW = W::EW, FO = W::FO, NO = W::NO;
   if (lane < EW * S) {
     const int sl = lane / S, a = lane % S;
     if (sl < nv) {
       const int32_t *pn = reinterpret_cast<const int32_t *>(
           pts + (size_t)(t0 + sl) * L::stride + L::N);
-      s_m[w][lane] = ibc[pn[a]];
-      if (a == 0) s_tg[w][sl] = pn[S];
+      sm[lane] = ibc[pn[a]];
+      if (a == 0) stg[sl] = pn[S];
     }
   }
   if (lane < EW * S * 3) {
@@ -817,26 +801,20 @@ k_ns_pairs(int first, int count, const int32_t *__restrict__ emap_c,
       sg[1] = make_double2(g[2], tmp1a);
     }
   }
-  // the old values of the nonzeros this lane adds into (not first touches)
-  double2 old[8];
-#pragma unroll
-  for (int rnd = 0; rnd < 8; ++rnd) {
-    const int js = (rnd * 8 + (lane >> 3)) / SS;
-    old[rnd] = (fst[rnd] || js >= nv)
-                   ? make_double2(0.0, 0.0)
-                   : reinterpret_cast<const double2 *>(A + (size_t)kk[rnd] * 16)[piece];
-  }
-  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-  __builtin_amdgcn_wave_barrier();
+}
 
-  // lane = (slot, a, b)
+template <int S>
+__device__ __forceinline__ void ns_pair_acc(const double *sw, const uint32_t *sm,
+                                            const int32_t *stg, int nv, int lane,
+                                            double acc[16]) {
+  using W = PairLds<S>;
+  constexpr int SS = W::SS, FO = W::FO, NO = W::NO;
   const int sl = lane / SS, rr = lane % SS;
   const int pa_ = rr % S;  // row node a
   const int pb_ = rr / S;  // column node b
-  double acc[16];
 #pragma unroll
   for (int u = 0; u < 16; ++u) acc[u] = 0.0;
-  if (sl < nv && s_tg[w][sl]) {
+  if (sl < nv && stg[sl]) {
     const double2 *sg = reinterpret_cast<const double2 *>(sw) + sl * SS * 2;  // [q][a][2]
     const double2 *sf = reinterpret_cast<const double2 *>(sw + FO + sl * S * 6);  // [q][3]
     const double *sn = sw + NO;                                                   // [q][a]
@@ -877,8 +855,8 @@ k_ns_pairs(int first, int count, const int32_t *__restrict__ emap_c,
   }
   // Dirichlet projection (BCLhs_3D.f90:105-118), as above
   if (sl < nv) {
-    const unsigned ma = s_m[w][sl * S + pa_];
-    const unsigned mb = s_m[w][sl * S + pb_];
+    const unsigned ma = sm[sl * S + pa_];
+    const unsigned mb = sm[sl * S + pb_];
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
       const int r = u >> 2, cc = u & 3;
@@ -887,6 +865,61 @@ k_ns_pairs(int first, int count, const int32_t *__restrict__ emap_c,
       if (pa_ == pb_ && r == cc && ((ma >> r) & 1u)) acc[u] = 1.0;
     }
   }
+}
+
+// One colour's elements at positions first .. first + count - 1 of the
+// colour order, 64 / S^2 per wavefront (one hex, four tets).  Everything it
+// reads is indexed by position: the point data, the scatter map and the
+// first-touch masks.  Every global load is issued before the accumulation
+// (the old values of the nonzeros it adds into included), so only the
+// point data's latency is in front of it.
+template <int S>
+__global__ void __launch_bounds__(256)
+k_ns_pairs(int first, int count, const int32_t *__restrict__ emap_c,
+           const uint64_t *__restrict__ first_c, const double *__restrict__ pts,
+           const uint8_t *__restrict__ ibc, double rho, double *__restrict__ A) {
+  constexpr int SS = S * S;
+  constexpr int EW = 64 / SS;       // elements per wavefront
+  // per wavefront: the staged point data (PairLds), then the block
+  // transpose [64][18]
+  __shared__ __align__(16) double s_w[kWavesPerBlock][64 * 18];
+  __shared__ uint32_t s_m[kWavesPerBlock][16];  // [slot][a] Dirichlet flags
+  __shared__ int32_t s_tg[kWavesPerBlock][4];   // [slot] in the selected subdomains
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int t0 = first + (blockIdx.x * kWavesPerBlock + w) * EW;
+  if (t0 >= first + count) return;  // whole wavefronts; no block barrier below
+  const int nv = (first + count - t0 < EW) ? first + count - t0 : EW;  // live slots
+  const int piece = lane & 7;
+  // the scatter plan: round rnd moves block jj = rnd * 8 + lane / 8, i.e.
+  // pair (aj, bj) of slot jj / S^2
+  int kk[8];
+  bool fst[8];
+#pragma unroll
+  for (int rnd = 0; rnd < 8; ++rnd) {
+    const int jj = rnd * 8 + (lane >> 3);
+    const int js = jj / SS, rj = jj % SS;
+    const int aj = rj % S, bj = rj / S;
+    const int tj = t0 + (js < nv ? js : 0);
+    kk[rnd] = emap_c[(size_t)tj * 64 + aj + 8 * bj];
+    fst[rnd] = (first_c[tj] >> (aj + 8 * bj)) & 1ull;
+  }
+  double *sw = s_w[w];
+  ns_pair_stage<S>(sw, s_m[w], s_tg[w], t0, nv, pts, ibc, rho, lane);
+  // the old values of the nonzeros this lane adds into (not first touches)
+  double2 old[8];
+#pragma unroll
+  for (int rnd = 0; rnd < 8; ++rnd) {
+    const int js = (rnd * 8 + (lane >> 3)) / SS;
+    old[rnd] = (fst[rnd] || js >= nv)
+                   ? make_double2(0.0, 0.0)
+                   : reinterpret_cast<const double2 *>(A + (size_t)kk[rnd] * 16)[piece];
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+
+  double acc[16];
+  ns_pair_acc<S>(sw, s_m[w], s_tg[w], nv, lane, acc);
   // every lane has read the point data: the transpose may overwrite them
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -908,6 +941,95 @@ k_ns_pairs(int first, int count, const int32_t *__restrict__ emap_c,
       *dst = v;
     } else {
       double2 o = old[rnd];
+      o.x += v.x;
+      o.y += v.y;
+      *dst = o;
+    }
+  }
+}
+
+// Hexes by clusters: one workgroup per cluster of up to kNsClElems elements
+// that share a node (one wavefront per element).  Each wavefront forms its
+// element's 64 blocks as k_ns_pairs<8> does; the wavefronts then add them,
+// in member order, into LDS slots, one per distinct nonzero of the cluster
+// (343 for 8 hexes around a node), and the slots go to the matrix once: the
+// first cluster in cluster-colour order to reach a nonzero stores, later
+// ones add to its old value.  Clusters of one colour share no node, so no
+// nonzero.  Per cluster ci: cl_elem[ci][m] the members' positions in the
+// element colour order (-1 past the last), cl_map[ci][m][a + 8 b] the slot
+// of block (a, b) of member m, cl_kf[ci][s] the nonzero of slot s with bit
+// 31 set on a first touch, cl_nslot[ci] the number of slots.
+__global__ void __launch_bounds__(64 * kNsClElems) __attribute__((amdgpu_waves_per_eu(4)))
+k_ns_pairs_cl(int first, const int32_t *__restrict__ cl_elem,
+              const uint16_t *__restrict__ cl_map, const uint32_t *__restrict__ cl_kf,
+              const int32_t *__restrict__ cl_nslot, const double *__restrict__ pts,
+              const uint8_t *__restrict__ ibc, double rho, double *__restrict__ A) {
+  constexpr int kStride = 18;  // doubles per LDS slot: 16 entries, 16-byte aligned rows
+  constexpr int kThreads = 64 * kNsClElems;
+  constexpr int kRounds = (kNsClSlots * 8 + kThreads - 1) / kThreads;
+  __shared__ __align__(16) double s_slot[kNsClSlots * kStride];
+  __shared__ __align__(16) double s_w[kNsClElems][PairLds<8>::size];
+  __shared__ uint32_t s_m[kNsClElems][8];
+  __shared__ int32_t s_tg[kNsClElems][1];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int w = tid >> 6;
+  const size_t ci = (size_t)first + blockIdx.x;
+  const int nslot = cl_nslot[ci];
+  const int t = cl_elem[ci * kNsClElems + w];
+  const int nv = t >= 0 ? 1 : 0;
+  // the write plan: piece p = tid + kThreads * r of slot p / 8
+  uint32_t kf[kRounds];
+#pragma unroll
+  for (int r = 0; r < kRounds; ++r) {
+    const int sl = (tid + kThreads * r) >> 3;
+    kf[r] = sl < nslot ? cl_kf[ci * kNsClSlots + sl] : 0x80000000u;
+  }
+  const int my_slot = nv ? cl_map[(ci * kNsClElems + w) * 64 + lane] : 0;
+  double *sw = s_w[w];
+  if (nv) ns_pair_stage<8>(sw, s_m[w], s_tg[w], t, 1, pts, ibc, rho, lane);
+  for (int i = tid; i < nslot * (kStride / 2); i += kThreads)
+    reinterpret_cast<double2 *>(s_slot)[i] = make_double2(0.0, 0.0);
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  double acc[16];
+  ns_pair_acc<8>(sw, s_m[w], s_tg[w], nv, lane, acc);
+  // the old values of the nonzeros that are not first touches, in flight
+  // over the ordered adds
+  double2 old[kRounds];
+#pragma unroll
+  for (int r = 0; r < kRounds; ++r)
+    old[r] = (kf[r] >> 31) ? make_double2(0.0, 0.0)
+                           : reinterpret_cast<const double2 *>(
+                                 A + (size_t)(kf[r] & 0x7fffffffu) * 16)[(tid + kThreads * r) & 7];
+  __syncthreads();  // the slots are zero
+  // the members' blocks into the slots, in member order (the 64 blocks of
+  // one element are 64 distinct nonzeros)
+  for (int m = 0; m < kNsClElems; ++m) {
+    if (w == m && nv) {
+      double2 *d = reinterpret_cast<double2 *>(s_slot + my_slot * kStride);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        double2 o = d[u];
+        o.x += acc[2 * u];
+        o.y += acc[2 * u + 1];
+        d[u] = o;
+      }
+    }
+    __syncthreads();
+  }
+  // one write per nonzero, 8 threads per 128-byte block
+#pragma unroll
+  for (int r = 0; r < kRounds; ++r) {
+    const int p = tid + kThreads * r;
+    const int sl = p >> 3;
+    if (sl >= nslot) continue;
+    const double2 v = reinterpret_cast<const double2 *>(s_slot + sl * kStride)[p & 7];
+    double2 *dst = reinterpret_cast<double2 *>(A + (size_t)(kf[r] & 0x7fffffffu) * 16) + (p & 7);
+    if (kf[r] >> 31) {
+      *dst = v;
+    } else {
+      double2 o = old[r];
       o.x += v.x;
       o.y += v.y;
       *dst = o;
@@ -1032,7 +1154,17 @@ hipError_t launch_assemble_ns(Ctx &c, int tensor_flag, int subdomain_flag) {
                        c.elm_id_set[1], subdomain_flag, do_mat, do_vec, c.d_pts, c.d_cfl,
                        c.d_negdet);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    if (do_mat) {
+    if (do_mat && hex && !c.cl_off.empty()) {
+      for (int col = 0; col + 1 < (int)c.cl_off.size(); ++col) {
+        const int first = c.cl_off[col];
+        const int count = c.cl_off[col + 1] - first;
+        if (count <= 0) continue;
+        hipLaunchKernelGGL(k_ns_pairs_cl, dim3(count), dim3(64 * kNsClElems), 0, c.stream,
+                           first, c.d_cl_elem, c.d_cl_map, c.d_cl_kf, c.d_cl_nslot, c.d_pts,
+                           c.d_ibc, P.rho, c.d_A64);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+      }
+    } else if (do_mat) {
       for (int col = 0; col + 1 < (int)c.color_off.size(); ++col) {
         const int first = c.color_off[col];
         const int count = c.color_off[col + 1] - first;

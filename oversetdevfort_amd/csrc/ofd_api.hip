@@ -547,6 +547,10 @@ int ofd_ctx_destroy(ofd_ctx *x) {
   dfree(c.d_emap_c);
   dfree(c.d_first_c);
   dfree(c.d_cpos);
+  dfree(c.d_cl_elem);
+  dfree(c.d_cl_map);
+  dfree(c.d_cl_kf);
+  dfree(c.d_cl_nslot);
   dfree(c.d_fe_elem);
   dfree(c.d_fe_mask);
   dfree(c.d_diag);
@@ -714,6 +718,153 @@ int ofd_upload_mesh(ofd_ctx *x, int nnode, int nelem, int nshl, const int32_t *i
   return OFD_OK;
 }
 
+// Clusters of the hex pair kernel.  Greedy over the nodes, for a threshold
+// from kNsClElems down to 1: a node with at least that many unclustered
+// elements takes up to kNsClElems of them as a cluster (on a structured hex
+// mesh the first pass tiles it with the 2 x 2 x 2 stars of every other
+// node), each element while the cluster's distinct nonzeros stay within
+// kNsClSlots.  Clusters are coloured first-fit so that clusters of one
+// colour share no node; the first touch of each nonzero follows that order.
+// More than 64 colours leaves the mesh to the per-element kernel.
+static int build_ns_clusters(Ctx &c, const std::vector<int32_t> &order) {
+  const int E = c.nelem, N = c.nnode, S = 8;
+  OFD_CHECK(c.icnt < (int64_t(1) << 31), OFD_ERR_UNSUPPORTED,
+            "more than 2^31 nonzero blocks");
+  std::vector<int32_t> tpos(E);
+  for (int t = 0; t < E; ++t) tpos[order[t]] = t;
+  std::vector<int32_t> off((size_t)N + 1, 0), inc((size_t)E * S);
+  for (int e = 0; e < E; ++e)
+    for (int a = 0; a < S; ++a) off[(size_t)c.h_ien[(size_t)a * E + e] + 1]++;
+  for (int i = 0; i < N; ++i) off[i + 1] += off[i];
+  {
+    std::vector<int32_t> pos(off.begin(), off.end() - 1);
+    for (int e = 0; e < E; ++e)
+      for (int a = 0; a < S; ++a) inc[pos[c.h_ien[(size_t)a * E + e]]++] = e;
+  }
+  auto node = [&](int e, int a) { return c.h_ien[(size_t)a * E + e]; };
+  std::vector<uint8_t> done(E, 0);
+  std::vector<int32_t> cl_elem;                  // [C][8] element ids, -1 none
+  std::vector<uint16_t> cl_map;                  // [C][8][64]
+  std::vector<int32_t> cl_nslot;
+  std::vector<std::pair<int32_t, int32_t>> cl_pair;  // [C][kNsClSlots] (row, col) nodes
+  int ln[kNsClElems * 8];
+  uint16_t tab[kNsClElems * 8][kNsClElems * 8];
+  std::vector<int32_t> cand;
+  for (int thr = kNsClElems; thr >= 1; --thr) {
+    for (int n = 0; n < N; ++n) {
+      cand.clear();
+      for (int k = off[n]; k < off[n + 1]; ++k)
+        if (!done[inc[k]]) cand.push_back(inc[k]);
+      if ((int)cand.size() < thr) continue;
+      int nln = 0, nslot = 0, nm = 0;
+      const size_t base = cl_elem.size();
+      cl_elem.resize(base + kNsClElems, -1);
+      cl_map.resize((base + kNsClElems) * 64, 0);
+      cl_pair.resize((base / kNsClElems + 1) * kNsClSlots, {-1, -1});
+      for (int e : cand) {
+        if (nm == kNsClElems) break;
+        // local node numbers (new ones appended) and the pairs it adds
+        int li[8], nl = nln;
+        for (int a = 0; a < S; ++a) {
+          const int g = node(e, a);
+          int j = 0;
+          while (j < nl && ln[j] != g) ++j;
+          if (j == nl) ln[nl++] = g;
+          li[a] = j;
+        }
+        int add = 0;
+        for (int b = 0; b < S; ++b)
+          for (int a = 0; a < S; ++a)
+            if (li[a] >= nln || li[b] >= nln || tab[li[a]][li[b]] == 0xffff) ++add;
+        if (nslot + add > kNsClSlots) break;
+        for (int j = nln; j < nl; ++j)
+          for (int k = 0; k < nl; ++k) tab[j][k] = tab[k][j] = 0xffff;
+        nln = nl;
+        for (int b = 0; b < S; ++b)
+          for (int a = 0; a < S; ++a) {
+            uint16_t &sl = tab[li[a]][li[b]];
+            if (sl == 0xffff) {
+              sl = (uint16_t)nslot;
+              cl_pair[base / kNsClElems * kNsClSlots + nslot] = {node(e, a), node(e, b)};
+              ++nslot;
+            }
+            cl_map[(base + nm) * 64 + a + 8 * b] = sl;
+          }
+        cl_elem[base + nm] = e;
+        done[e] = 1;
+        ++nm;
+      }
+      cl_nslot.push_back(nslot);
+    }
+  }
+  const int C = (int)cl_nslot.size();
+  // colours: first fit over one 64-bit word per node
+  std::vector<uint64_t> nmask((size_t)N, 0ull);
+  std::vector<int32_t> ccol(C);
+  int ncol = 0;
+  for (int ci = 0; ci < C; ++ci) {
+    uint64_t used = 0;
+    for (int m = 0; m < kNsClElems; ++m) {
+      const int e = cl_elem[(size_t)ci * kNsClElems + m];
+      if (e >= 0)
+        for (int a = 0; a < S; ++a) used |= nmask[node(e, a)];
+    }
+    if (used == ~0ull) return OFD_OK;  // past 64 colours: no clusters
+    const int col = __builtin_ctzll(~used);
+    ccol[ci] = col;
+    ncol = std::max(ncol, col + 1);
+    for (int m = 0; m < kNsClElems; ++m) {
+      const int e = cl_elem[(size_t)ci * kNsClElems + m];
+      if (e >= 0)
+        for (int a = 0; a < S; ++a) nmask[node(e, a)] |= 1ull << col;
+    }
+  }
+  std::vector<int32_t> coff(ncol + 1, 0), corder(C);
+  for (int ci = 0; ci < C; ++ci) coff[ccol[ci] + 1]++;
+  for (int k = 0; k < ncol; ++k) coff[k + 1] += coff[k];
+  {
+    std::vector<int32_t> pos(coff.begin(), coff.end() - 1);
+    for (int ci = 0; ci < C; ++ci) corder[pos[ccol[ci]]++] = ci;
+  }
+  // the device tables in cluster-colour order; each slot's nonzero from the
+  // pattern (row segments of h_col / h_row) and its first touch
+  std::vector<int32_t> d_elem((size_t)C * kNsClElems), d_nslot(C);
+  std::vector<uint16_t> d_map((size_t)C * kNsClElems * 64);
+  std::vector<uint32_t> d_kf((size_t)C * kNsClSlots, 0x80000000u);
+  std::vector<uint8_t> touched((size_t)c.icnt, 0);
+  for (int r = 0; r < C; ++r) {
+    const int ci = corder[r];
+    for (int m = 0; m < kNsClElems; ++m) {
+      const int e = cl_elem[(size_t)ci * kNsClElems + m];
+      d_elem[(size_t)r * kNsClElems + m] = e >= 0 ? tpos[e] : -1;
+    }
+    std::copy(cl_map.begin() + (size_t)ci * kNsClElems * 64,
+              cl_map.begin() + (size_t)(ci + 1) * kNsClElems * 64,
+              d_map.begin() + (size_t)r * kNsClElems * 64);
+    d_nslot[r] = cl_nslot[ci];
+    for (int sl = 0; sl < cl_nslot[ci]; ++sl) {
+      const auto pr = cl_pair[(size_t)ci * kNsClSlots + sl];
+      int64_t k = c.h_col[pr.first];
+      while (k < c.h_col[pr.first + 1] && c.h_row[k] != pr.second) ++k;
+      OFD_CHECK(k < c.h_col[pr.first + 1], OFD_ERR_ARG,
+                "cluster pair outside the sparsity pattern");
+      d_kf[(size_t)r * kNsClSlots + sl] = (uint32_t)k | (touched[k] ? 0u : 0x80000000u);
+      touched[k] = 1;
+    }
+  }
+  int rc = dalloc(c, &c.d_cl_elem, d_elem.size());
+  if (!rc) rc = dalloc(c, &c.d_cl_map, d_map.size());
+  if (!rc) rc = dalloc(c, &c.d_cl_kf, d_kf.size());
+  if (!rc) rc = dalloc(c, &c.d_cl_nslot, d_nslot.size());
+  if (!rc) rc = h2d(c, c.d_cl_elem, d_elem.data(), d_elem.size() * sizeof(int32_t));
+  if (!rc) rc = h2d(c, c.d_cl_map, d_map.data(), d_map.size() * sizeof(uint16_t));
+  if (!rc) rc = h2d(c, c.d_cl_kf, d_kf.data(), d_kf.size() * sizeof(uint32_t));
+  if (!rc) rc = h2d(c, c.d_cl_nslot, d_nslot.data(), d_nslot.size() * sizeof(int32_t));
+  if (rc) return rc;
+  c.cl_off = coff;
+  return OFD_OK;
+}
+
 int ofd_build_pattern(ofd_ctx *x, int64_t *icnt_out) {
   OFD_CHECK(x, OFD_ERR_ARG, "null context");
   Ctx &c = x->c;
@@ -866,6 +1017,11 @@ int ofd_build_pattern(ofd_ctx *x, int64_t *icnt_out) {
     if (!rc) rc = dalloc(c, &c.d_cpos, (size_t)E);
     if (rc) return rc;
     OFD_HIP(launch_colour_maps(c));
+  }
+  c.cl_off.clear();
+  if (c.nshl == 8) {
+    rc = build_ns_clusters(c, order);
+    if (rc) return rc;
   }
   OFD_HIP(hipMemsetAsync(c.d_A64, 0, (size_t)c.icnt * 16 * sizeof(double), c.stream));
   OFD_HIP(hipStreamSynchronize(c.stream));

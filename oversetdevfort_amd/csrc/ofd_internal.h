@@ -43,6 +43,8 @@ void set_error(const std::string &msg);
   } while (0)
 
 constexpr int kMaxKspace = 64;     // GMRES restart length upper bound
+constexpr int kNsClElems = 8;      // hex pair kernel: elements per cluster (one node's)
+constexpr int kNsClSlots = 352;    // distinct nonzeros per cluster (343 for 8 hexes)
 constexpr int kRedBlock = 256;     // threads per reduction block
 constexpr int kMaxRedBlocks = 4096;  // node blocks of 256: N <= 1,048,576
 
@@ -108,6 +110,13 @@ struct Ctx {
   uint64_t *d_first = nullptr;        // [E] bit a + 8 b: the element is the first (in
                                       // colour order) to add into pair (a, b)'s nonzero
   std::vector<int32_t> color_off;     // ncolor+1
+  // hexes: clusters of the pair kernel (k_ns_pairs_cl), by cluster colour;
+  // empty when the mesh has none
+  std::vector<int32_t> cl_off;        // ncolour+1 over the clusters
+  int32_t *d_cl_elem = nullptr;       // [C][kNsClElems] positions in colour order, -1 none
+  uint16_t *d_cl_map = nullptr;       // [C][kNsClElems][64] slot of block (a, b)
+  uint32_t *d_cl_kf = nullptr;        // [C][kNsClSlots] nonzero | first-touch bit 31
+  int32_t *d_cl_nslot = nullptr;      // [C]
   std::vector<int32_t> h_elem_color;  // colour of each element
   // weak boundary faces (BCugType 2): registered list, and the elements
   // that hold them grouped by colour with their faces as a bitmask
