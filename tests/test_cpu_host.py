@@ -184,3 +184,35 @@ def test_problem_setups():
     inflow = np.abs(q.mesh.xg[:, 0]) < 1e-12
     assert np.all(q.ibc[inflow, :3] == 1)
     assert abs(q.fields["ug"][:, 0].max() - 1.0) < 0.05
+
+
+def _oracle_lumped(O, mesh):
+    """Row sums of the mass matrix from the oracle's shape functions and
+    rule (eval_shape_fem, genGPandGW): sum_e sum_q N_a(q) DetJ(q) w_q."""
+    nshl = mesh.ien.shape[1]
+    gp, gw = O.gauss_volume(nshl)
+    vol = np.zeros(mesh.nnode)
+    for e in range(mesh.nelem):
+        xl = mesh.xg[mesh.ien[e]]
+        for q in range(len(gw)):
+            shl, _, _, _, _, dj = O.eval_shape_fem(nshl, gp[q], xl, np.zeros_like(xl))
+            vol[mesh.ien[e]] += shl * dj * gw[q]
+    return vol
+
+
+@pytest.mark.parametrize("split", ["hex", "tet", "prism"])
+def test_lumped_volumes(oracle_mod, split):
+    """The nodal volumes the heat-loss objective uses: on every element type
+    they sum to the volume of the (jittered) box, and they equal the
+    oracle's quadrature of the shape functions node by node."""
+    m = M.structured_hex(4, 3, 3, hi=(1.0, 0.9, 1.1), jitter=0.08, seed=5)
+    m = {"hex": m, "tet": M.hex_to_tets(m), "prism": M.hex_to_prisms(m)}[split]
+    v = C.lumped_volumes(m)
+    assert abs(v.sum() - 0.99) <= 1e-13
+    w = _oracle_lumped(oracle_mod, m)
+    assert np.abs(v - w).max() <= 1e-14 * np.abs(w).max()
+    # a subset of the elements: the same quadrature over those alone
+    sel = np.arange(m.nelem) % 3 == 0
+    sub = M.Mesh(xg=m.xg, ien=m.ien[sel], elm_id=m.elm_id[sel])
+    assert np.abs(C.lumped_volumes(m, sel) - _oracle_lumped(oracle_mod, sub)).max() <= \
+        1e-14 * np.abs(w).max()
