@@ -31,9 +31,6 @@ namespace {
 
 constexpr int kWavesPerBlock = 4;
 
-__device__ __forceinline__ double red8(double v) { return wsum8(v); }
-__device__ __forceinline__ double max8(double v) { return wmax8(v); }
-
 }  // namespace
 
 // alpha stage (assembleNavSto.f90:25-30; solveQuenching.f90:25-37 for the
@@ -87,364 +84,9 @@ struct AsmParams {
   int vms;  // 1: the residual with the fine-scale velocity uprime = -tauM rLi
 };
 
-// Launched for the linear prism (S = 6, linprishl at the 6-point rule of
-// genGPandGWtetpri; eval_shape_FEM.f90:18-26); hexes and tets take the split
-// kernels below.  An element of S < 8 nodes is padded to the 8x8 lanes: node
-// lanes a >= S carry zero shape values and gradients (the butterfly sums are
-// unchanged), point lanes q >= S repeat the first points with zero weight,
-// and the padded (a, b) pairs are not scattered.
-template <int S>
-__global__ void __launch_bounds__(256)
-k_assemble_ns_el(int N, int E, const int32_t *__restrict__ ien,
-                  const int32_t *__restrict__ elm_id,
-                  const int32_t *__restrict__ elems, int first, int count,
-                  const int32_t *__restrict__ emap, const uint64_t *__restrict__ firstmask,
-                  const double *__restrict__ rec, const double *__restrict__ mus,
-                  const uint8_t *__restrict__ ibc, AsmParams P, int id1,
-                  int id2, int subdomain_flag, int do_mat, int do_vec,
-                  double *__restrict__ A, double *__restrict__ rhs,
-                  double *__restrict__ cfl_out, int32_t *__restrict__ negdet) {
-  // [q][a]: N, dN/dx (3), tmp1 = rho uadv . dN/dx, pad (16-byte rows)
-  __shared__ double2 s_sh[kWavesPerBlock][8][8][3];
-  // [q]: the Gauss point's factors of e3LHS, DetJ*w folded in:
-  // cf = fact2 dw, cB = fact1 rho dw, cm = fact2 (mu + kdc) dw,
-  // ct = fact2 tauC dw, tauM (= tauP), pad
-  __shared__ double2 s_q[kWavesPerBlock][8][3];
-  __shared__ int32_t s_node[kWavesPerBlock][8];
-  // [lane][16 entries + 2 spare]: the spare doubles put the 64 lanes' 16-byte
-  // pieces in distinct bank groups on the way in and on the way out
-  __shared__ __align__(16) double s_blk[kWavesPerBlock][64][18];
-
-  const int lane = threadIdx.x & 63;
-  const int w = threadIdx.x >> 6;
-  const int idx = first + blockIdx.x * kWavesPerBlock + w;
-  const bool valid = idx < first + count;
-  const int e = valid ? elems[idx] : 0;
-
-  bool tgt = false;
-  if (valid) {
-    const int id = elm_id[e];
-    if (subdomain_flag & 1) tgt = tgt || id == id1;
-    if (subdomain_flag & 2) tgt = tgt || id == id2;
-  }
-  const bool mat = do_mat && tgt;
-  const bool vec = do_vec && tgt;
-
-  // ---------------- phase A: lane = (q, a) ----------------
-  const int q = lane >> 3;
-  const int a = lane & 7;
-  const int na = valid ? ien[(size_t)(a < S ? a : 0) * E + e] : 0;
-  if (lane < 8) s_node[w][lane] = na;
-
-  const double2 *r2 = reinterpret_cast<const double2 *>(rec + (size_t)na * 16);
-  double xr[13];
-#pragma unroll
-  for (int t = 0; t < 6; ++t) {
-    const double2 v = r2[t];
-    xr[2 * t] = v.x;
-    xr[2 * t + 1] = v.y;
-  }
-  xr[12] = r2[6].x;
-  const double *xd = xr;       // x + d
-  const double *ua = xr + 3;   // u
-  const double *aca = xr + 6;  // ac
-  const double *uma = xr + 9;  // um
-  const double pa = xr[12];
-
-  // reference shape functions at Gauss point q (linhexshl, lintetshl or
-  // linprishl at the rule of genGPandGWtetpri)
-  double Na, dl0, dl1, dl2, gwq;
-  elem_shape<S>(q, a, Na, dl0, dl1, dl2, gwq);
-
-  // dx/dxi (i,j) = sum_a (x+d)_a,i dN_a/dxi_j
-  double J[9];
-#pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    J[i + 0] = red8(xd[i] * dl0);
-    J[i + 3] = red8(xd[i] * dl1);
-    J[i + 6] = red8(xd[i] * dl2);
-  }
-  // get_inverse_3x3 (eval_shape.f90:85-112); column-major Ji(i + 3 j)
-  double Ji[9];
-  Ji[0] = J[4] * J[8] - J[5] * J[7];
-  Ji[3] = J[5] * J[6] - J[3] * J[8];
-  Ji[6] = J[3] * J[7] - J[6] * J[4];
-  const double tmp = 1.0 / (Ji[0] * J[0] + Ji[3] * J[1] + Ji[6] * J[2]);
-  Ji[0] *= tmp;
-  Ji[3] *= tmp;
-  Ji[6] *= tmp;
-  Ji[1] = (J[7] * J[2] - J[1] * J[8]) * tmp;
-  Ji[4] = (J[0] * J[8] - J[2] * J[6]) * tmp;
-  Ji[7] = (J[1] * J[6] - J[0] * J[7]) * tmp;
-  Ji[2] = (J[1] * J[5] - J[4] * J[2]) * tmp;
-  Ji[5] = (J[2] * J[3] - J[0] * J[5]) * tmp;
-  Ji[8] = (J[0] * J[4] - J[3] * J[1]) * tmp;
-  const double detj = 1.0 / tmp;
-
-  // global gradients and metric Gij = sum_k dxi_k/dx_i dxi_k/dx_j
-  double ga[3];
-#pragma unroll
-  for (int j = 0; j < 3; ++j)
-    ga[j] = dl0 * Ji[0 + 3 * j] + dl1 * Ji[1 + 3 * j] + dl2 * Ji[2 + 3 * j];
-  double G[9];
-#pragma unroll
-  for (int j = 0; j < 3; ++j)
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-      G[i + 3 * j] = Ji[0 + 3 * i] * Ji[0 + 3 * j] + Ji[1 + 3 * i] * Ji[1 + 3 * j] +
-                     Ji[2 + 3 * i] * Ji[2 + 3 * j];
-
-  // Gauss-point interpolation (e3int_qr, e3int_qr_grad)
-  double ui[3], umi[3], aci[3], du[9], dp[3];
-#pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    ui[i] = red8(ua[i] * Na);
-    umi[i] = red8(uma[i] * Na);
-    aci[i] = red8(aca[i] * Na);
-  }
-  const double pri = red8(pa * Na);
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) du[i + 3 * j] = red8(ua[i] * ga[j]);
-#pragma unroll
-  for (int j = 0; j < 3; ++j) dp[j] = red8(pa * ga[j]);
-
-  const double rho = P.rho;
-  // viscosity at q: mu, or mu times the interpolated per-node scale
-  const double mu = mus ? P.mu * red8(mus[na] * Na) : P.mu;
-  double uadv[3];
-#pragma unroll
-  for (int i = 0; i < 3; ++i) uadv[i] = ui[i] - umi[i];
-  const double divu = du[0] + du[4] + du[8];
-
-  // e3STAB_3D
-  double taua = 0.0, gg = 0.0;
-#pragma unroll
-  for (int j = 0; j < 3; ++j)
-#pragma unroll
-    for (int i = 0; i < 3; ++i) taua += uadv[i] * G[i + 3 * j] * uadv[j];
-#pragma unroll
-  for (int t = 0; t < 9; ++t) gg += G[t] * G[t];
-  const double nu = mu / rho;
-  const double taud = nu * nu * 12.0 * gg;
-  const double taut = 4.0 / (P.delt * P.delt);
-  const double tauM = 1.0 / (rho * sqrt(taut + taua + taud));
-  const double tauP = tauM;
-  const double tauC = rho * sqrt(taua + taud) / (G[0] + G[4] + G[8]);
-  // e3CFL
-  double cflq = 0.0;
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) cflq += uadv[i] * G[i + 3 * j] * uadv[j];
-  cflq = P.delt * sqrt(cflq);
-  const double dw = detj * gwq;  // padded points have weight 0
-  if (valid && a == 0 && q < S && detj < 0.0) atomicAdd(negdet, 1);
-
-  const double tmp1a = rho * (uadv[0] * ga[0] + uadv[1] * ga[1] + uadv[2] * ga[2]);
-  s_sh[w][q][a][0] = make_double2(Na, ga[0]);
-  s_sh[w][q][a][1] = make_double2(ga[1], ga[2]);
-  s_sh[w][q][a][2] = make_double2(tmp1a, 0.0);
-  if (a == 0) {
-    const double fact2 = P.alfi * P.gami * P.delt;
-    const double cf = fact2 * dw;
-    s_q[w][q][0] = make_double2(cf, P.almi * rho * dw);
-    s_q[w][q][1] = make_double2(cf * (mu + P.kdc), cf * tauC);
-    s_q[w][q][2] = make_double2(tauM, 0.0);
-  }
-  (void)tauP;
-
-  // RHS (e3Rhs_3D_fluid; the live call's uprime = 0, or with P.vms the
-  // fine-scale velocity uprime = -tauM rLi, rLi from assembleNavSto.f90:
-  // 258-266 with the zero Hessian, tauBar = 0): contribution of Gauss point
-  // q to node a
-  double rcu[3] = {0.0, 0.0, 0.0}, rcp = 0.0;
-  if (vec) {
-    const double mupkdc = mu + P.kdc;
-    const double ptot = pri - tauC * divu;
-    const double fi[3] = {P.gx * rho, P.gy * rho, P.gz * rho};
-    double t1[3], t2[9];
-    if (P.vms) {
-      double up[3], adv[3];
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        const double rLi = rho * (aci[i] + uadv[0] * du[i + 0] + uadv[1] * du[i + 3] +
-                                  uadv[2] * du[i + 6]) - fi[i] + dp[i];
-        up[i] = -tauM * rLi;
-        adv[i] = uadv[i] + up[i];
-      }
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-        t1[i] = rho * (aci[i] + adv[0] * du[i + 0] + adv[1] * du[i + 3] + adv[2] * du[i + 6]) -
-                fi[i];
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-          t2[i + 3 * j] = ((i == j) ? (-ptot + 2.0 * mupkdc * du[i + 3 * i])
-                                    : mupkdc * (du[i + 3 * j] + du[j + 3 * i])) -
-                          rho * adv[j] * up[i];
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-        rcu[i] = -(Na * t1[i] + (ga[0] * t2[i] + ga[1] * t2[i + 3] + ga[2] * t2[i + 6])) * dw;
-      rcp = -(Na * divu - (ga[0] * up[0] + ga[1] * up[1] + ga[2] * up[2])) * dw;
-    } else {
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-        t1[i] = rho * (aci[i] + uadv[0] * du[i + 0] + uadv[1] * du[i + 3] +
-                       uadv[2] * du[i + 6]) - fi[i];
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-          t2[i + 3 * j] = (i == j) ? (-ptot + 2.0 * mupkdc * du[i + 3 * i])
-                                   : mupkdc * (du[i + 3 * j] + du[j + 3 * i]);
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-        rcu[i] = -(Na * t1[i] + (ga[0] * t2[i] + ga[1] * t2[i + 3] + ga[2] * t2[i + 6])) * dw;
-      rcp = -(Na * divu) * dw;
-    }
-  }
-  // reduce over the 8 Gauss points (lanes a, a+8, ..., a+56)
-#pragma unroll
-  for (int m = 8; m < 64; m <<= 1) {
-#pragma unroll
-    for (int i = 0; i < 3; ++i) rcu[i] += __shfl_xor(rcu[i], m);
-    rcp += __shfl_xor(rcp, m);
-    cflq = fmax(cflq, __shfl_xor(cflq, m));
-  }
-  cflq = max8(cflq);
-  if (valid && lane == 0) cfl_out[e] = cflq;
-
-  __syncthreads();
-
-  // ---------------- phase B: lane = (a, b) ----------------
-  const int pa_ = lane & 7;   // row node a
-  const int pb_ = lane >> 3;  // column node b
-  double acc[16];
-#pragma unroll
-  for (int t = 0; t < 16; ++t) acc[t] = 0.0;
-  if (mat) {
-    // e3LHS_3D_fluid_Old (e3LHS.f90:56-159) with tauBar = 0, regrouped so
-    // that DetJ*w and fact1 = almi, fact2 = alfi*gami*Delt sit in the Gauss
-    // point's factors (tauP = tauM):
-    //   tmp4 dw = (N_a + tauM t1_a)(cB N_b + cf t1_b) + cm g_a.g_b
-    //   K11_rc += delta_rc tmp4 dw + cm g_a,c g_b,r + ct g_a,r g_b,c
-    //   G_r    += -g_a,r cf N_b + tauM t1_a cf g_b,r
-    //   D1_c   += N_a cf g_b,c + g_a,c tauM (cB N_b + cf t1_b)
-    //   M      += tauM cf g_a.g_b
-#pragma unroll 2
-    for (int qq = 0; qq < S; ++qq) {
-      const double2 r0 = s_sh[w][qq][pa_][0], r1 = s_sh[w][qq][pa_][1],
-                    r2 = s_sh[w][qq][pa_][2];
-      const double2 c0 = s_sh[w][qq][pb_][0], c1 = s_sh[w][qq][pb_][1],
-                    c2 = s_sh[w][qq][pb_][2];
-      const double2 f0 = s_q[w][qq][0], f1 = s_q[w][qq][1], f2 = s_q[w][qq][2];
-      const double Nq = r0.x, g0 = r0.y, g1 = r1.x, g2 = r1.y, t1a = r2.x;
-      const double Mb = c0.x, h0 = c0.y, h1 = c1.x, h2 = c1.y, t1b = c2.x;
-      const double cf = f0.x, cB = f0.y, cm = f1.x, ct = f1.y, tM = f2.x;
-      // column factors
-      const double Bb = cB * Mb + cf * t1b;
-      const double hf0 = cf * h0, hf1 = cf * h1, hf2 = cf * h2;
-      const double hm0 = cm * h0, hm1 = cm * h1, hm2 = cm * h2;
-      const double ht0 = ct * h0, ht1 = ct * h1, ht2 = ct * h2;
-      const double Mf = cf * Mb, PB = tM * Bb;
-      // row factors
-      const double Ta = tM * t1a;
-      const double Aa = Nq + Ta;
-      const double t4 = Aa * Bb + (g0 * hm0 + g1 * hm1 + g2 * hm2);
-      acc[0] += t4 + g0 * (hm0 + ht0);
-      acc[5] += t4 + g1 * (hm1 + ht1);
-      acc[10] += t4 + g2 * (hm2 + ht2);
-      acc[1] += g1 * hm0 + g0 * ht1;
-      acc[4] += g0 * hm1 + g1 * ht0;
-      acc[2] += g2 * hm0 + g0 * ht2;
-      acc[8] += g0 * hm2 + g2 * ht0;
-      acc[6] += g2 * hm1 + g1 * ht2;
-      acc[9] += g1 * hm2 + g2 * ht1;
-      acc[3] += Ta * hf0 - g0 * Mf;
-      acc[7] += Ta * hf1 - g1 * Mf;
-      acc[11] += Ta * hf2 - g2 * Mf;
-      acc[12] += Nq * hf0 + g0 * PB;
-      acc[13] += Nq * hf1 + g1 * PB;
-      acc[14] += Nq * hf2 + g2 * PB;
-      acc[15] += tM * (g0 * hf0 + g1 * hf1 + g2 * hf2);
-    }
-  }
-
-  // ---------------- phase C: Dirichlet projection + scatter ----------------
-  if (valid && pa_ < S && pb_ < S) {
-    const int nra = s_node[w][pa_];
-    const int nrb = s_node[w][pb_];
-    const unsigned ma = ibc[nra];
-    const unsigned mb = ibc[nrb];
-    if (do_mat) {
-#pragma unroll
-      for (int t = 0; t < 16; ++t) {
-        const int r = t >> 2, cc = t & 3;
-        // row r of node a and column cc of node b are projected away, except
-        // that a pressure constraint clears the G column but not the M
-        // column (BCLhs_3D.f90:105-118)
-        const bool colz = ((mb >> cc) & 1u) && !(r == 3 && cc == 3);
-        if (((ma >> r) & 1u) || colz) acc[t] = 0.0;
-        if (pa_ == pb_ && r == cc && ((ma >> r) & 1u)) acc[t] = 1.0;
-      }
-    }
-  }
-  if (valid && do_mat) {
-    // the block of lane (a, b) goes through LDS so that 8 lanes move each
-    // 128-byte block: lane = (block in a group of 8, 16-byte piece)
-    double2 *mine = reinterpret_cast<double2 *>(s_blk[w][lane]);
-#pragma unroll
-    for (int t = 0; t < 8; ++t) mine[t] = make_double2(acc[2 * t], acc[2 * t + 1]);
-    // the lanes of this wavefront read what the others wrote
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    // the first element (in colour order) to reach a nonzero stores, the
-    // later ones add: no zeroing pass over the matrix
-    const uint64_t fm = firstmask[e];
-    const int piece = lane & 7;
-#pragma unroll
-    for (int rnd = 0; rnd < 8; ++rnd) {
-      const int j = rnd * 8 + (lane >> 3);  // block pair j = a + 8 b
-      if ((j & 7) < S && (j >> 3) < S) {
-        const int k = emap[(size_t)e * 64 + j];
-        const double2 v = reinterpret_cast<const double2 *>(s_blk[w][j])[piece];
-        double2 *dst = reinterpret_cast<double2 *>(A + (size_t)k * 16) + piece;
-        if ((fm >> j) & 1ull) {
-          *dst = v;
-        } else {
-          double2 o = *dst;
-          o.x += v.x;
-          o.y += v.y;
-          *dst = o;
-        }
-      }
-    }
-  }
-  if (valid && pa_ < S && pb_ < S) {
-    if (do_vec && lane < S) {
-      // lane < S holds the q-reduced RHS of node a = lane
-      const unsigned m0 = ibc[na];
-      double o[4] = {rcu[0], rcu[1], rcu[2], rcp};
-#pragma unroll
-      for (int cc = 0; cc < 4; ++cc)
-        if ((m0 >> cc) & 1u) o[cc] = 0.0;
-      double2 *rr = reinterpret_cast<double2 *>(rhs + (size_t)na * 4);
-      double2 r0 = rr[0], r1 = rr[1];
-      r0.x += o[0];
-      r0.y += o[1];
-      r1.x += o[2];
-      r1.y += o[3];
-      rr[0] = r0;
-      rr[1] = r1;
-    }
-  }
-}
-
 // ---------------------------------------------------------------------------
-// Hexes and tets in three kernels (S = 8 or 4 nodes and Gauss points).
+// Hexes, prisms and tets in three kernels (S = 8, 6 or 4 nodes and Gauss
+// points; a prism's 6 points take a group of 8 lanes in the point kernel).
 //
 // The kernel above gives each element a whole wavefront with lane = (q, a):
 // everything that belongs to a Gauss point alone (J^-1, G, the tau's, the
@@ -495,10 +137,11 @@ k_ns_points(int E, const int32_t *__restrict__ ien, const int32_t *__restrict__ 
   // the S node records of each element: [el][a][14] (x + d, u, ac, um, p,
   // viscosity scale)
   __shared__ __align__(16) double s_rec[64 * kRecStride];
+  constexpr int PL = point_lanes<S>();  // lanes per element (prisms: 6 points of 8)
   const int lane = threadIdx.x;
-  const int el = lane / S;  // 64 / S elements per wavefront
-  const int q = lane % S;   // Gauss point, and node a = q for the loads
-  const int e = blockIdx.x * (64 / S) + el;
+  const int el = lane / PL;  // 64 / PL elements per wavefront
+  const int q = lane % PL;   // Gauss point, and node a = q for the loads
+  const int e = blockIdx.x * (64 / PL) + el;
   const bool valid = e < E;
   bool tgt = false;
   if (valid) {
@@ -509,7 +152,7 @@ k_ns_points(int E, const int32_t *__restrict__ ien, const int32_t *__restrict__ 
   const bool vec = do_vec && tgt;
   // this element's point data go to its position in colour order
   double *po = pts + (size_t)(valid ? cpos[e] : 0) * L::stride;
-  {
+  if (q < S) {
     const int n = valid ? ien[(size_t)q * E + e] : 0;
     const double2 *r2 = reinterpret_cast<const double2 *>(rec + (size_t)n * 16);
     double2 *sr = reinterpret_cast<double2 *>(s_rec + (el * S + q) * kRecStride);
@@ -608,7 +251,7 @@ k_ns_points(int E, const int32_t *__restrict__ ien, const int32_t *__restrict__ 
     for (int j = 0; j < 3; ++j)
       ga[a][j] = d0 * Ji[0 + 3 * j] + d1 * Ji[1 + 3 * j] + d2 * Ji[2 + 3 * j];
   }
-  if (valid && do_mat) {
+  if (valid && do_mat && q < S) {
     // J^-1 and uadv: the pair kernel forms dN_a/dx and tmp1_a from them with
     // the expressions above, so the same bits
     double2 *pj = reinterpret_cast<double2 *>(po + q * 12);
@@ -663,9 +306,9 @@ k_ns_points(int E, const int32_t *__restrict__ ien, const int32_t *__restrict__ 
 #pragma unroll
     for (int j = 0; j < 3; ++j) cflq += uadv[i] * G[i + 3 * j] * uadv[j];
   cflq = P.delt * sqrt(cflq);
-  const double dw = detj * gwq;  // hexes: unit weights, so dw = detj
-  if (valid && detj < 0.0) atomicAdd(negdet, 1);
-  if (valid && do_mat) {
+  const double dw = detj * gwq;  // hexes: unit weights; padded prism points 0
+  if (valid && q < S && detj < 0.0) atomicAdd(negdet, 1);
+  if (valid && do_mat && q < S) {
     const double fact2 = P.alfi * P.gami * P.delt;
     const double cf = fact2 * dw;
     double2 *pf = reinterpret_cast<double2 *>(po + L::F + q * 6);
@@ -733,7 +376,7 @@ k_ns_points(int E, const int32_t *__restrict__ ien, const int32_t *__restrict__ 
         }
       }
     }
-    if (valid) {
+    if (valid && q < S) {
       double2 *pr = reinterpret_cast<double2 *>(po + L::R + q * 4);
       pr[0] = make_double2(own[0], own[1]);
       pr[1] = make_double2(own[2], own[3]);
@@ -1144,11 +787,12 @@ hipError_t launch_assemble_ns(Ctx &c, int tensor_flag, int subdomain_flag) {
   const int do_vec = (tensor_flag & OFD_ASSEMBLE_VEC) ? 1 : 0;
   const double *mus = c.mus_on ? c.d_field[OFD_MUSCALE] : nullptr;
   hipError_t e;
-  if (c.nshl == 8 || c.nshl == 4) {
+  {
     const bool hex = c.nshl == 8;
-    const int epw = 64 / c.nshl;           // point kernel: elements per wavefront
-    const int epb = 64 / (c.nshl * c.nshl) * kWavesPerBlock;  // pair kernel: per block
-    hipLaunchKernelGGL((hex ? k_ns_points<8> : k_ns_points<4>),
+    const int S = c.nshl;
+    const int epw = 64 / (S == 4 ? 4 : 8);  // point kernel: elements per wavefront
+    const int epb = 64 / (S * S) * kWavesPerBlock;  // pair kernel: per block
+    hipLaunchKernelGGL((S == 8 ? k_ns_points<8> : S == 6 ? k_ns_points<6> : k_ns_points<4>),
                        dim3((c.nelem + epw - 1) / epw), dim3(64), 0, c.stream, c.nelem,
                        c.d_ien, c.d_elm_id, c.d_cpos, c.d_noderec, mus, P, c.elm_id_set[0],
                        c.elm_id_set[1], subdomain_flag, do_mat, do_vec, c.d_pts, c.d_cfl,
@@ -1169,7 +813,7 @@ hipError_t launch_assemble_ns(Ctx &c, int tensor_flag, int subdomain_flag) {
         const int first = c.color_off[col];
         const int count = c.color_off[col + 1] - first;
         if (count <= 0) continue;
-        hipLaunchKernelGGL((hex ? k_ns_pairs<8> : k_ns_pairs<4>),
+        hipLaunchKernelGGL((S == 8 ? k_ns_pairs<8> : S == 6 ? k_ns_pairs<6> : k_ns_pairs<4>),
                            dim3((count + epb - 1) / epb), dim3(64 * kWavesPerBlock), 0,
                            c.stream, first, count, c.d_emap_c, c.d_first_c, c.d_pts, c.d_ibc,
                            P.rho, c.d_A64);
@@ -1179,26 +823,12 @@ hipError_t launch_assemble_ns(Ctx &c, int tensor_flag, int subdomain_flag) {
     if (do_vec) {
       hipLaunchKernelGGL(k_ns_rhs_gather, dim3((c.nnode + 255) / 256), dim3(256), 0, c.stream,
                          c.nnode, c.d_inc_off, c.d_inc, c.d_pts, pts_stride(c.nshl),
-                         hex ? PtsLayout<8>::R : PtsLayout<4>::R, c.d_ibc, c.d_rhs);
+                         S == 8 ? PtsLayout<8>::R : S == 6 ? PtsLayout<6>::R : PtsLayout<4>::R,
+                         c.d_ibc, c.d_rhs);
       if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     return hipSuccess;
   }
-  if (do_vec && (e = launch_zero(c, c.d_rhs, (size_t)c.nnode * 4)) != hipSuccess) return e;
-  for (int col = 0; col + 1 < (int)c.color_off.size(); ++col) {
-    const int first = c.color_off[col];
-    const int count = c.color_off[col + 1] - first;
-    if (count <= 0) continue;
-    const int blocks = (count + kWavesPerBlock - 1) / kWavesPerBlock;
-    hipLaunchKernelGGL(k_assemble_ns_el<6>,
-                       dim3(blocks), dim3(64 * kWavesPerBlock), 0, c.stream, c.nnode,
-                       c.nelem, c.d_ien, c.d_elm_id, c.d_color_elems, first, count, c.d_emap,
-                       c.d_first, c.d_noderec, mus, c.d_ibc, P, c.elm_id_set[0],
-                       c.elm_id_set[1], subdomain_flag, do_mat, do_vec, c.d_A64, c.d_rhs,
-                       c.d_cfl, c.d_negdet);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-  }
-  return hipSuccess;
 }
 
 hipError_t launch_zero(Ctx &c, double *p, size_t n) {

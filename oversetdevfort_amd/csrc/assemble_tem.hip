@@ -54,7 +54,7 @@ struct TemPts {
   static constexpr int stride = S == 8 ? 160 : S == 6 ? 120 : 80;
 };
 static_assert(TemPts<8>::N + 5 <= TemPts<8>::stride && TemPts<8>::stride <= 192, "hex layout");
-static_assert(TemPts<6>::N + 4 <= TemPts<6>::stride && TemPts<6>::stride <= 128,
+static_assert(TemPts<6>::N + 4 <= TemPts<6>::stride && TemPts<6>::stride <= 136,
               "prism layout");
 static_assert(TemPts<4>::N + 3 <= TemPts<4>::stride && TemPts<4>::stride <= 96, "tet layout");
 

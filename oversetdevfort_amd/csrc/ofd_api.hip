@@ -122,7 +122,7 @@ static void free_tem_adjoint(Ctx &c) {
   c.tadj_n4 = 0;
 }
 
-// the element assemblies' per-element point data (1536 bytes per hex, 1024
+// the element assemblies' per-element point data (1536 bytes per hex, 1088
 // per prism, 768 per tet), allocated at the first assembly
 static int ensure_pts(Ctx &c) {
   if (c.nshl != 8 && c.nshl != 4 && c.nshl != 6) return OFD_OK;

@@ -358,13 +358,15 @@ constexpr int kReadBlocks = 8192;  // blocks of the read-only stream kernel
 // NS point data per element of S nodes and S Gauss points (hexes, tets), in
 // doubles: [q] (J^-1 column-major, uadv) 12 S, [q] (cf, cB, cm, ct, tauM, 0)
 // 6 S, [a] RHS contributions 4 S, then the S node numbers and the subdomain
-// flag as int32; 1536 bytes per hex, 768 per tet
+// flag as int32; 1536 bytes per hex, 1088 per prism, 768 per tet
 template <int S>
 struct PtsLayout {
   static constexpr int F = 12 * S, R = 18 * S, N = 22 * S;
-  static constexpr int stride = S == 8 ? 192 : 96;
+  static constexpr int stride = S == 8 ? 192 : S == 6 ? 136 : 96;
+  // the S + 1 int32 after N fit, and rows stay 16-byte aligned
+  static_assert(2 * N + S + 1 <= 2 * stride && stride % 2 == 0, "point data layout");
 };
-constexpr int pts_stride(int nshl) { return nshl == 8 ? 192 : nshl == 6 ? 128 : 96; }
+constexpr int pts_stride(int nshl) { return nshl == 8 ? 192 : nshl == 6 ? 136 : 96; }
 hipError_t launch_read(Ctx &c, const void *src, size_t nbytes, double *out);
 // overset fringe
 hipError_t launch_fringe(Ctx &c, int32_t src_id, int nrecv, const double *xr,
